@@ -1,0 +1,647 @@
+#!/usr/bin/env python3
+"""CellML 1.0 -> C++ cardiac cell-model generator.
+
+The reference builds its cell classes at configure time with the external
+``chaste_codegen`` package (pinned ``>=0.10.0,<0.11.0`` in
+``chaste_codegen.txt:1``; driven from ``cmake/Modules/ChasteMacros.cmake:65-99``)
+from ``heart/src/odes/cellml/*.cellml``.  That package is not available here, so
+this module restates the part of its job the monodomain path needs: it reads a
+CellML 1.0 model, resolves the component connections, orders the algebraic
+equations and prints one self-contained C++ header per model.
+
+The header is precision-generic (``template <class R>``) and compiles both as
+HIP device code (``__host__ __device__``) and as plain host C++ (the FP64 CPU
+oracle includes the same header).  It provides
+
+* ``evaluate<R, RL>(y, V, i_stim_area, p, cm, dy, g_inf, g_k)``: the
+  right-hand side with V clamped (``dV/dt`` is not written; the tissue path
+  clamps V exactly as ``AbstractCardiacCell::ComputeExceptVoltage`` does,
+  ``heart/src/odes/AbstractCardiacCell.cpp:90-105``).  With ``RL = true`` the
+  gating variables get the Rush-Larsen coefficients instead of a derivative:
+  ``y_new = g_inf + (y - g_inf) * exp(g_k * dt)``
+  (``heart/src/odes/AbstractRushLarsenCardiacCell.cpp:106-128``).
+* ``dvdt<R>(...)``: ``dV/dt`` including the stimulus (single-cell runs, where
+  V is integrated by the cell itself, ``AbstractCardiacCell::Compute``).
+* ``iionic<R>(y, V, p, cm)``: total ionic current in uA/cm^2, the quantity
+  ``AbstractCardiacCellInterface::GetIIonic`` returns
+  (``heart/src/odes/AbstractCardiacCellInterface.hpp:227-256``).
+
+Unit conventions follow the reference's tissue coupling: the stimulus passed
+in is an *area* stimulus in uA/cm^2 (``GetIntracellularAreaStimulus``,
+``heart/src/odes/AbstractCardiacCellInterface.cpp:106-119``).  Models whose
+membrane currents are per capacitance (pA/pF, e.g. ten Tusscher 2006) divide it
+by the tissue capacitance ``cm`` (uF/cm^2) and multiply the summed ionic current
+by ``cm`` to return uA/cm^2.
+
+A state variable is integrated by Rush-Larsen when its derivative is affine in
+itself, ``dy/dt = a + b*y``, with ``a`` and ``b`` depending on no state other
+than the membrane voltage (the classic Hodgkin-Huxley gate of Rush & Larsen,
+IEEE TBME 25:389, 1978); every other state uses forward Euler.
+
+Usage::
+
+    python -m chaste_amd.codegen.cellml2cpp <model.cellml> <out.h> <namespace> \
+        [--param comp.var] [--scale comp.var] ...
+
+``--param`` turns a constant into a run-time parameter (default: its CellML
+value); ``--scale`` multiplies a variable's defining value by a run-time factor
+(default 1).  Parameters are laid out in the order given.
+"""
+import argparse
+import os
+import sys
+import xml.etree.ElementTree as ET
+
+CML = '{http://www.cellml.org/cellml/1.0#}'
+MML = '{http://www.w3.org/1998/Math/MathML}'
+CMETA = '{http://www.cellml.org/metadata/1.0#}'
+CML11 = '{http://www.cellml.org/cellml/1.1#}'
+
+
+def _tag(e):
+    return e.tag.split('}')[1]
+
+
+class CellMLModel:
+    def __init__(self, path):
+        root = ET.parse(path).getroot()
+        self.name = root.get('name')
+        self.vars = {}
+        parent = {}
+
+        def find(k):
+            while parent[k] != k:
+                parent[k] = parent[parent[k]]
+                k = parent[k]
+            return k
+
+        self.components = []
+        for comp in root.findall(CML + 'component'):
+            cname = comp.get('name')
+            self.components.append(cname)
+            for v in comp.findall(CML + 'variable'):
+                key = (cname, v.get('name'))
+                self.vars[key] = dict(units=v.get('units'),
+                                      init=v.get('initial_value'),
+                                      cmeta=v.get(CMETA + 'id'))
+                parent[key] = key
+        for con in root.findall(CML + 'connection'):
+            mc = con.find(CML + 'map_components')
+            c1, c2 = mc.get('component_1'), mc.get('component_2')
+            for mv in con.findall(CML + 'map_variables'):
+                a = find((c1, mv.get('variable_1')))
+                b = find((c2, mv.get('variable_2')))
+                if a != b:
+                    parent[a] = b
+
+        self.defs = {}      # source var -> (component, rhs element)
+        self.odes = {}      # state var -> (component, rhs element)
+        self.time = None
+        raw_defs = []
+        for comp in root.findall(CML + 'component'):
+            cname = comp.get('name')
+            for math in comp.findall(MML + 'math'):
+                for ap in math.findall(MML + 'apply'):
+                    ch = list(ap)
+                    assert _tag(ch[0]) == 'eq', 'only assignment equations supported'
+                    lhs, rhs = ch[1], ch[2]
+                    if _tag(lhs) == 'ci':
+                        raw_defs.append(('def', (cname, lhs.text.strip()), cname, rhs))
+                    else:
+                        assert _tag(lhs) == 'apply' and _tag(list(lhs)[0]) == 'diff'
+                        bvar = lhs.find(MML + 'bvar').find(MML + 'ci').text.strip()
+                        ci = [c for c in lhs if _tag(c) == 'ci'][0].text.strip()
+                        raw_defs.append(('ode', (cname, ci), cname, rhs))
+                        self.time = (cname, bvar)
+        # Every connected set has exactly one source: the member carrying an
+        # initial value or an equation.  All members map onto that source.
+        sources = {}
+        for key, info in self.vars.items():
+            if info['init'] is not None:
+                sources.setdefault(find(key), key)
+        for kind, key, cname, rhs in raw_defs:
+            sources[find(key)] = key
+        self.canon_map = {}
+        for key in self.vars:
+            r = find(key)
+            self.canon_map[key] = sources.get(r, key)
+        for kind, key, cname, rhs in raw_defs:
+            src = self.canon_map[key]
+            if kind == 'def':
+                self.defs[src] = (cname, rhs)
+            else:
+                self.odes[src] = (cname, rhs)
+        self.time = self.canon_map[self.time]
+        # metadata
+        self.meta = {}
+        for key, info in self.vars.items():
+            if info['cmeta']:
+                self.meta[info['cmeta']] = self.canon_map[key]
+        self.V = self.meta['membrane_voltage']
+        self.stim = self.meta['membrane_stimulus_current']
+        # state order: V first, then document order (component, variable)
+        states = [k for k in self.vars if self.canon_map[k] == k and k in self.odes]
+        states.remove(self.V)
+        self.states = [self.V] + states
+
+    def canon(self, cname, name):
+        return self.canon_map[(cname, name)]
+
+    def init(self, key):
+        return float(self.vars[key]['init'])
+
+
+def cident(key):
+    return '%s__%s' % key
+
+
+class Printer:
+    """Translates MathML expression trees to C++ source text."""
+
+    def __init__(self, model, names):
+        self.m = model
+        self.names = names      # canonical var -> C expression
+
+    def ref(self, cname, el):
+        key = self.m.canon(cname, el.text.strip())
+        return key
+
+    def expr(self, el, cname):
+        t = _tag(el)
+        if t == 'ci':
+            key = self.ref(cname, el)
+            return self.names[key]
+        if t == 'cn':
+            return self.number(el)
+        if t == 'pi':
+            return 'R(3.14159265358979323846)'
+        if t == 'exponentiale':
+            return 'R(2.71828182845904523536)'
+        if t == 'true':
+            return 'true'
+        if t == 'false':
+            return 'false'
+        if t == 'piecewise':
+            pieces = []
+            other = 'R(NAN)'
+            for p in el:
+                if _tag(p) == 'piece':
+                    val, cond = list(p)
+                    pieces.append((self.expr(cond, cname), self.expr(val, cname)))
+                elif _tag(p) == 'otherwise':
+                    other = self.expr(list(p)[0], cname)
+            s = other
+            for cond, val in reversed(pieces):
+                s = '((%s) ? (%s) : (%s))' % (cond, val, s)
+            return s
+        if t == 'apply':
+            ch = list(el)
+            op = _tag(ch[0])
+            args = [a for a in ch[1:] if _tag(a) not in ('degree', 'logbase', 'bvar')]
+            a = [self.expr(x, cname) for x in args]
+            if op == 'plus':
+                return '(' + ' + '.join(a) + ')'
+            if op == 'minus':
+                if len(a) == 1:
+                    return '(-%s)' % a[0]
+                return '(%s - %s)' % (a[0], a[1])
+            if op == 'times':
+                return '(' + ' * '.join(a) + ')'
+            if op == 'divide':
+                return '(%s / %s)' % (a[0], a[1])
+            if op == 'power':
+                ex = args[1]
+                if _tag(ex) == 'cn':
+                    v = float(self.number_value(ex))
+                    if v == int(v) and 1 <= v <= 4:
+                        return 'cmx_ipow<%d>(%s)' % (int(v), a[0])
+                return 'pow(%s, %s)' % (a[0], a[1])
+            if op == 'root':
+                deg = el.find(MML + 'degree')
+                if deg is None:
+                    return 'sqrt(%s)' % a[0]
+                return 'pow(%s, R(1) / %s)' % (a[0], self.expr(list(deg)[0], cname))
+            if op == 'exp':
+                return 'exp(%s)' % a[0]
+            if op == 'ln':
+                return 'log(%s)' % a[0]
+            if op == 'log':
+                return 'log10(%s)' % a[0]
+            if op == 'abs':
+                return 'fabs(%s)' % a[0]
+            if op == 'floor':
+                return 'floor(%s)' % a[0]
+            if op == 'ceiling':
+                return 'ceil(%s)' % a[0]
+            rel = {'lt': '<', 'gt': '>', 'leq': '<=', 'geq': '>=', 'eq': '==', 'neq': '!='}
+            if op in rel:
+                return '(%s %s %s)' % (a[0], rel[op], a[1])
+            if op == 'and':
+                return '(' + ' && '.join(a) + ')'
+            if op == 'or':
+                return '(' + ' || '.join(a) + ')'
+            if op == 'not':
+                return '(!%s)' % a[0]
+            raise NotImplementedError(op)
+        raise NotImplementedError(t)
+
+    @staticmethod
+    def number_value(el):
+        if el.get('type') == 'e-notation':
+            parts = [x.strip() for x in el.itertext()]
+            parts = [x for x in parts if x]
+            return '%se%s' % (parts[0], parts[1])
+        return el.text.strip()
+
+    def number(self, el):
+        v = self.number_value(el)
+        f = float(v)
+        return 'R(%s)' % repr(f)
+
+
+def deps_of(el, cname, model, acc):
+    """Canonical variables referenced by an expression."""
+    if _tag(el) == 'ci':
+        acc.add(model.canon(cname, el.text.strip()))
+    for c in el:
+        deps_of(c, cname, model, acc)
+    return acc
+
+
+class Generator:
+    def __init__(self, model, params, scales):
+        self.m = model
+        self.params = params    # list of canonical keys (constants -> parameter)
+        self.scales = scales    # list of canonical keys (defined value x factor)
+        self.param_index = {}
+        for i, k in enumerate(params + scales):
+            self.param_index[k] = i
+        self.deps = {}
+        for k, (c, rhs) in model.defs.items():
+            self.deps[k] = deps_of(rhs, c, model, set())
+        self._state_deps = {}
+
+    def state_deps(self, key, stack=()):
+        """States a variable depends on, transitively (through algebraic defs)."""
+        m = self.m
+        if key in self._state_deps:
+            return self._state_deps[key]
+        if key in m.odes:
+            r = {key}
+        elif key == m.stim:
+            r = set()
+        elif key in m.defs:
+            r = set()
+            for d in self.deps[key]:
+                r |= self.state_deps(d)
+        else:
+            r = set()
+        self._state_deps[key] = r
+        return r
+
+    def closure(self, roots):
+        """Algebraic variables needed by roots, in evaluation order."""
+        m = self.m
+        order, seen = [], set()
+
+        def visit(k):
+            if k in seen:
+                return
+            seen.add(k)
+            if k in m.defs and k != m.stim:
+                for d in sorted(self.deps[k]):
+                    visit(d)
+                order.append(k)
+        for r in roots:
+            visit(r)
+        return order
+
+    # --- affine decomposition dy/dt = a + b*y for Rush-Larsen ---------------
+    def affine(self, el, cname, y, printer):
+        m = self.m
+        t = _tag(el)
+        if t == 'ci':
+            k = m.canon(cname, el.text.strip())
+            if k == y:
+                return (None, 'R(1)')
+            if y in self.state_deps(k) and k in m.defs:
+                c2, rhs = m.defs[k]
+                return self.affine(rhs, c2, y, printer)
+            return (printer.expr(el, cname), None)
+        if not self._depends(el, cname, y):
+            return (printer.expr(el, cname), None)
+        if t != 'apply':
+            return None
+        ch = list(el)
+        op = _tag(ch[0])
+        args = ch[1:]
+        parts = [self.affine(x, cname, y, printer) for x in args]
+        if any(p is None for p in parts):
+            return None
+        if op == 'plus':
+            a, b = None, None
+            for pa, pb in parts:
+                a, b = _add(a, pa), _add(b, pb)
+            return (a, b)
+        if op == 'minus':
+            if len(parts) == 1:
+                return (_neg(parts[0][0]), _neg(parts[0][1]))
+            return (_sub(parts[0][0], parts[1][0]), _sub(parts[0][1], parts[1][1]))
+        if op == 'times':
+            a, b = parts[0]
+            for pa, pb in parts[1:]:
+                if b is not None and pb is not None:
+                    return None
+                na = _mul(a, pa)
+                nb = _add(_mul(a, pb), _mul(b, pa))
+                a, b = na, nb
+            return (a, b)
+        if op == 'divide':
+            (na, nb), (da, db) = parts
+            if db is not None:
+                return None
+            return (_div(na, da), _div(nb, da))
+        return None
+
+    def _depends(self, el, cname, y):
+        for k in deps_of(el, cname, self.m, set()):
+            if k == y or y in self.state_deps(k):
+                return True
+        return False
+
+    def generate(self, ns, source_name):
+        m = self.m
+        names = {}
+        for k, info in m.vars.items():
+            if m.canon_map[k] != k:
+                continue
+            names[k] = cident(k)
+        for i, k in enumerate(m.states):
+            names[k] = 'y[%d]' % i
+        names[m.V] = 'V'
+        names[m.stim] = 'i_stim'
+        names[m.time] = 't'
+        for k in self.params:
+            names[k] = 'p[%d]' % self.param_index[k]
+        pr = Printer(m, names)
+
+        constants = [k for k, info in m.vars.items()
+                     if m.canon_map[k] == k and info['init'] is not None
+                     and k not in m.odes and k not in self.params]
+        state_ix = {k: i for i, k in enumerate(m.states)}
+
+        # Per-capacitance current units (pA/pF == uA/uF) need the tissue cm.
+        v_units = None
+        stim_units = m.vars[m.stim]['units']
+        per_cap = 'per_picoF' in stim_units or 'A_per_F' in stim_units or 'uF' in stim_units and 'cm' not in stim_units
+
+        def body_lines(roots, indent='    '):
+            lines = []
+            for k in self.closure(roots):
+                c, rhs = m.defs[k]
+                e = pr.expr(rhs, c)
+                if k in self.scales:
+                    e = '(%s) * p[%d]' % (e, self.param_index[k])
+                lines.append('%sconst R %s = %s;' % (indent, cident(k), e))
+            return lines
+
+        out = []
+        w = out.append
+        w('// Generated by chaste_amd/codegen/cellml2cpp.py from %s -- do not edit.' % source_name)
+        w('// Model: %s' % m.name)
+        w('#pragma once')
+        w('#include "model_common.h"')
+        w('')
+        w('namespace cmx { namespace %s {' % ns)
+        w('')
+        w('constexpr int kNumStates = %d;' % len(m.states))
+        w('constexpr int kVIndex = 0;')
+        w('constexpr int kNumParams = %d;' % (len(self.params) + len(self.scales)))
+        w('constexpr bool kPerCapacitanceUnits = %s;' % ('true' if per_cap else 'false'))
+        w('')
+        w('// State variables (CellML component__variable), V first then document order.')
+        w('static const char* const kStateNames[kNumStates] = {')
+        for k in m.states:
+            w('    "%s",' % cident(k))
+        w('};')
+        w('static constexpr double kInitialState[kNumStates] = {')
+        for k in m.states:
+            w('    %r,  // %s' % (m.init(k), cident(k)))
+        w('};')
+        npar = len(self.params) + len(self.scales)
+        w('static const char* const kParamNames[%d] = {' % max(npar, 1))
+        for k in self.params:
+            w('    "%s",' % cident(k))
+        for k in self.scales:
+            w('    "scale:%s",' % cident(k))
+        if npar == 0:
+            w('    "",')
+        w('};')
+        w('static constexpr double kParamDefaults[%d] = {' % max(npar, 1))
+        for k in self.params:
+            w('    %r,' % m.init(k))
+        for k in self.scales:
+            w('    1.0,')
+        if npar == 0:
+            w('    0.0,')
+        w('};')
+
+        # Rush-Larsen classification
+        rl = {}
+        for k in m.states[1:]:
+            c, rhs = m.odes[k]
+            aff = self.affine(rhs, c, k, pr)
+            if aff is None or aff[1] is None:
+                continue
+            # coefficient state dependencies: V only
+            ok = True
+            for el_str in aff:
+                pass
+            dep_states = set()
+            for d in self._affine_deps(rhs, c, k):
+                dep_states |= ({d} if d in m.odes else self.state_deps(d))
+            dep_states.discard(k)
+            if dep_states - {m.V}:
+                ok = False
+            if ok:
+                rl[k] = aff
+        w('// 1 = integrated by Rush-Larsen (affine gate, V-dependent coefficients only)')
+        w('static constexpr int kIsGate[kNumStates] = {%s};' %
+          ', '.join('1' if k in rl else '0' for k in m.states))
+        w('')
+
+        # evaluate
+        w('// Right-hand side with V clamped.  Non-gates (and, with RL=false, every')
+        w('// state) get dy[i]; with RL=true gate i gets g_inf[i], g_k[i] such that')
+        w('// y_i(t+dt) = g_inf[i] + (y_i - g_inf[i]) * exp(g_k[i] * dt).')
+        w('// i_stim_area is in uA/cm^2; cm is the tissue capacitance in uF/cm^2.')
+        w('template <class R, bool RL>')
+        w('CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,')
+        w('                            R* dy, R* g_inf, R* g_k)')
+        w('{')
+        w('    CMX_MATH_USING')
+        w('    (void)p; (void)cm; (void)g_inf; (void)g_k; (void)i_stim_area;')
+        roots = set()
+        for k in m.states[1:]:
+            roots |= self.deps_rhs(k)
+        for k in self.used_constants(roots, constants):
+            w('    const R %s = R(%r);' % (cident(k), m.init(k)))
+        w('    const R i_stim = %s;' % ('i_stim_area / cm' if per_cap else 'i_stim_area'))
+        w('    (void)i_stim;')
+        roots = set()
+        for k in m.states[1:]:
+            roots |= self.deps_rhs(k)
+        for line in body_lines(sorted(roots)):
+            w(line)
+        for k in m.states[1:]:
+            i = state_ix[k]
+            c, rhs = m.odes[k]
+            if k in rl:
+                a, b = rl[k]
+                w('    if (RL) {')
+                w('        const R gk = %s;' % b)
+                w('        g_k[%d] = gk;' % i)
+                w('        g_inf[%d] = -(%s) / gk;' % (i, a if a is not None else 'R(0)'))
+                w('    } else {')
+                w('        dy[%d] = %s;' % (i, pr.expr(rhs, c)))
+                w('    }')
+            else:
+                w('    dy[%d] = %s;' % (i, pr.expr(rhs, c)))
+        w('}')
+        w('')
+        # dV/dt
+        w('// dV/dt including the stimulus (single-cell integration).')
+        w('template <class R>')
+        w('CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)')
+        w('{')
+        w('    CMX_MATH_USING')
+        w('    (void)p; (void)cm;')
+        for k in self.used_constants(self.deps_rhs(m.V), constants):
+            w('    const R %s = R(%r);' % (cident(k), m.init(k)))
+        w('    const R i_stim = %s;' % ('i_stim_area / cm' if per_cap else 'i_stim_area'))
+        w('    (void)i_stim;')
+        for line in body_lines(sorted(self.deps_rhs(m.V))):
+            w(line)
+        c, rhs = m.odes[m.V]
+        w('    return %s;' % pr.expr(rhs, c))
+        w('}')
+        w('')
+        # iionic: -dV/dt with zero stimulus, in uA/cm^2
+        cap_key = m.meta.get('Cm')
+        w('// Total ionic current in uA/cm^2 (GetIIonic): minus dV/dt at zero stimulus,')
+        w('// times the membrane capacitance that converts it to a current density.')
+        w('template <class R>')
+        w('CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)')
+        w('{')
+        w('    CMX_MATH_USING')
+        w('    (void)p; (void)cm;')
+        used = self.used_constants(self.deps_rhs(m.V), constants)
+        capname = None if per_cap else self._find_model_capacitance()
+        if capname is not None and capname not in used:
+            used.append(capname)
+        for k in used:
+            w('    const R %s = R(%r);' % (cident(k), m.init(k)))
+        w('    const R i_stim = R(0);')
+        w('    (void)i_stim;')
+        for line in body_lines(sorted(self.deps_rhs(m.V))):
+            w(line)
+        c, rhs = m.odes[m.V]
+        w('    const R dv = %s;' % pr.expr(rhs, c))
+        if per_cap:
+            w('    return -dv * cm;')
+        else:
+            # dV/dt = -(sum I)/C  ->  sum I = -C * dV/dt, C from the model
+            w('    return -dv * %s;' % (cident(capname) if capname else 'R(1)'))
+        w('}')
+        w('')
+        w('}}  // namespace cmx::%s' % ns)
+        return '\n'.join(out) + '\n', rl
+
+    def _find_model_capacitance(self):
+        m = self.m
+        c, rhs = m.odes[m.V]
+        for k in deps_of(rhs, c, m, set()):
+            if m.vars[k]['units'] and 'F_per_cm2' in m.vars[k]['units']:
+                return k
+        return None
+
+    def used_constants(self, roots, constants):
+        need = set(roots)
+        for k in self.closure(sorted(roots)):
+            need |= self.deps[k]
+        return [k for k in constants if k in need]
+
+    def deps_rhs(self, k):
+        c, rhs = self.m.odes[k]
+        return deps_of(rhs, c, self.m, set())
+
+    def _affine_deps(self, el, cname, y):
+        """Non-y variables the affine coefficients reference (inlining defs of y)."""
+        m = self.m
+        acc = set()
+        for k in deps_of(el, cname, m, set()):
+            if k == y:
+                continue
+            if k in m.defs and y in self.state_deps(k):
+                c2, rhs = m.defs[k]
+                acc |= self._affine_deps(rhs, c2, y)
+            else:
+                acc.add(k)
+        return acc
+
+
+def _add(x, y):
+    if x is None:
+        return y
+    if y is None:
+        return x
+    return '(%s + %s)' % (x, y)
+
+
+def _neg(x):
+    return None if x is None else '(-%s)' % x
+
+
+def _sub(x, y):
+    if y is None:
+        return x
+    if x is None:
+        return '(-%s)' % y
+    return '(%s - %s)' % (x, y)
+
+
+def _mul(x, y):
+    if x is None or y is None:
+        return None
+    return '(%s * %s)' % (x, y)
+
+
+def _div(x, d):
+    if x is None:
+        return None
+    return '(%s / %s)' % (x, d)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('cellml')
+    ap.add_argument('out')
+    ap.add_argument('namespace')
+    ap.add_argument('--param', action='append', default=[])
+    ap.add_argument('--scale', action='append', default=[])
+    args = ap.parse_args(argv)
+    model = CellMLModel(args.cellml)
+
+    def key(s):
+        c, v = s.split('.')
+        return model.canon(c, v)
+    gen = Generator(model, [key(s) for s in args.param], [key(s) for s in args.scale])
+    src, rl = gen.generate(args.namespace, os.path.basename(args.cellml))
+    with open(args.out, 'w') as f:
+        f.write(src)
+    print('%s: %d states, %d Rush-Larsen gates: %s' % (
+        args.out, len(model.states), len(rl), ', '.join(cident(k) for k in rl)))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,610 @@
+// FP64 CPU oracle -- test infrastructure only (see oracle.h for the reference
+// files each part restates).  Built by oracle/Makefile into oracle/liboracle.so.
+#include "oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../chaste_amd/csrc/models/lr91.h"
+#include "../chaste_amd/csrc/models/tt06_epi.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// LR91, restated from heart/dynamic/DynamicallyLoadableLr91.cpp:45-280.  The
+// reference keeps the state as (h, j, m, Cai, V, d, f, X); this restatement
+// works on the CellML/codegen order (V, m, h, j, d, f, X, Cai) used everywhere
+// else here, so only the indexing differs.
+// ---------------------------------------------------------------------------
+struct Lr91Hand {
+    static constexpr double kC = 1.0, kF = 96484.6, kR = 8314.0, kT = 310.0;
+    static constexpr double kEb = -59.87, kGb = 0.03921, kGNa = 23.0;
+    static constexpr double kKi = 145.0, kKo = 5.4, kNai = 18.0, kNao = 140.0;
+    static constexpr double kGKp = 0.0183, kPRNaK = 0.01833;
+
+    // Ionic currents at state y (V = y[0]); returns the total, fills Cai' and
+    // the gate rates when rates != nullptr.
+    static double currents(const double* y, double* dy)
+    {
+        const double V = y[0], m = y[1], h = y[2], j = y[3], d = y[4], f = y[5], X = y[6], Cai = y[7];
+        const double RTonF = kR * kT / kF;
+        const double ENa = RTonF * std::log(kNao / kNai);
+        const double iNa = kGNa * std::pow(m, 3.0) * h * j * (V - ENa);
+        const double Esi = 7.7 - 13.0287 * std::log(Cai);
+        const double isi = 0.09 * d * f * (V - Esi);
+        const double gK = 0.282 * std::sqrt(kKo / 5.4);
+        const double Xi = (V > -100.0)
+            ? 2.837 * (std::exp(0.04 * (V + 77.0)) - 1.0) / ((V + 77.0) * std::exp(0.04 * (V + 35.0)))
+            : 1.0;
+        const double EK = RTonF * std::log((kKo + kPRNaK * kNao) / (kKi + kPRNaK * kNai));
+        const double iK = gK * X * Xi * (V - EK);
+        const double gK1 = 0.6047 * std::sqrt(kKo / 5.4);
+        const double EK1 = RTonF * std::log(kKo / kKi);
+        const double aK1 = 1.02 / (1.0 + std::exp(0.2385 * (V - EK1 - 59.215)));
+        const double bK1 = (0.49124 * std::exp(0.08032 * (V + 5.476 - EK1))
+                            + std::exp(0.06175 * (V - (EK1 + 594.31))))
+                           / (1.0 + std::exp(-0.5143 * (V - EK1 + 4.753)));
+        const double iK1 = gK1 * (aK1 / (aK1 + bK1)) * (V - EK1);
+        const double Kp = 1.0 / (1.0 + std::exp((7.488 - V) / 5.98));
+        const double iKp = kGKp * Kp * (V - EK1);
+        const double ib = kGb * (V - kEb);
+        if (dy) {
+            const bool low = V < -40.0;
+            const double ah = low ? 0.135 * std::exp((80.0 + V) / -6.8) : 0.0;
+            const double bh = low ? 3.56 * std::exp(0.079 * V) + 3.1e5 * std::exp(0.35 * V)
+                                  : 1.0 / (0.13 * (1.0 + std::exp((V + 10.66) / -11.1)));
+            const double aj = low ? (-1.2714e5 * std::exp(0.2444 * V) - 3.474e-5 * std::exp(-0.04391 * V))
+                                        * (V + 37.78) / (1.0 + std::exp(0.311 * (V + 79.23)))
+                                  : 0.0;
+            const double bj = low ? 0.1212 * std::exp(-0.01052 * V) / (1.0 + std::exp(-0.1378 * (V + 40.14)))
+                                  : 0.3 * std::exp(-2.535e-7 * V) / (1.0 + std::exp(-0.1 * (V + 32.0)));
+            const double am = 0.32 * (V + 47.13) / (1.0 - std::exp(-0.1 * (V + 47.13)));
+            const double bm = 0.08 * std::exp(-V / 11.0);
+            const double ad = 0.095 * std::exp(-0.01 * (V - 5.0)) / (1.0 + std::exp(-0.072 * (V - 5.0)));
+            const double bd = 0.07 * std::exp(-0.017 * (V + 44.0)) / (1.0 + std::exp(0.05 * (V + 44.0)));
+            const double af = 0.012 * std::exp(-0.008 * (V + 28.0)) / (1.0 + std::exp(0.15 * (V + 28.0)));
+            const double bf = 0.0065 * std::exp(-0.02 * (V + 30.0)) / (1.0 + std::exp(-0.2 * (V + 30.0)));
+            const double aX = 0.0005 * std::exp(0.083 * (V + 50.0)) / (1.0 + std::exp(0.057 * (V + 50.0)));
+            const double bX = 0.0013 * std::exp(-0.06 * (V + 20.0)) / (1.0 + std::exp(-0.04 * (V + 20.0)));
+            dy[1] = am * (1.0 - m) - bm * m;
+            dy[2] = ah * (1.0 - h) - bh * h;
+            dy[3] = aj * (1.0 - j) - bj * j;
+            dy[4] = ad * (1.0 - d) - bd * d;
+            dy[5] = af * (1.0 - f) - bf * f;
+            dy[6] = aX * (1.0 - X) - bX * X;
+            dy[7] = -1e-4 * isi + 0.07 * (1e-4 - Cai);
+        }
+        return iNa + isi + iK + iK1 + iKp + ib;
+    }
+};
+
+// Stimulus functions (heart/src/stimulus/SimpleStimulus.cpp:44-76, RegularStimulus.cpp:58-80).
+struct Stimulus {
+    double mag = 0, dur = 0, start = 0, period = -1, stop = DBL_MAX;
+    Stimulus() = default;
+    Stimulus(double m, double d, double s, double p) : mag(m), dur(d), start(s), period(p)
+    {
+        // SimpleStimulus pads the duration by (duration + start) * eps;
+        // RegularStimulus by period * eps.
+        if (period > 0) dur += period * DBL_EPSILON;
+        else dur += (dur + start) * DBL_EPSILON;
+    }
+    double at(double t) const
+    {
+        if (mag == 0.0) return 0.0;
+        if (period > 0) {
+            if (t < start) return 0.0;
+            const double beat = std::fmod(t - start, period);
+            return (beat >= 0 && beat <= dur && t <= stop) ? mag : 0.0;
+        }
+        return (start <= t && t <= dur + start) ? mag : 0.0;
+    }
+};
+
+int num_states(int model)
+{
+    switch (model) {
+    case ORC_LR91_HAND:
+    case ORC_LR91_CELLML: return cmx::lr91::kNumStates;
+    case ORC_TT06_EPI: return cmx::tt06_epi::kNumStates;
+    default: return -1;
+    }
+}
+
+const double* initial_state(int model)
+{
+    return model == ORC_TT06_EPI ? cmx::tt06_epi::kInitialState : cmx::lr91::kInitialState;
+}
+
+// One cell substep over [t, t+dt] with V clamped unless update_v.  stim_area
+// is the area stimulus (uA/cm^2) at time t.  Returns dV/dt when update_v.
+double cell_substep(int model, bool rl, const double* p, double cm, double* y, double t, double dt,
+                    double stim_area, bool update_v)
+{
+    (void)t;
+    double dy[64], ginf[64], gk[64];
+    const int S = num_states(model);
+    double dv = 0.0;
+    if (model == ORC_LR91_HAND) {
+        const double iion = Lr91Hand::currents(y, dy);
+        dv = -(iion + stim_area) / Lr91Hand::kC;
+        // The hand restatement only has the forward-Euler form (the
+        // reference class is an AbstractCardiacCell).
+        if (update_v) y[0] += dt * dv;
+        for (int i = 1; i < S; ++i) y[i] += dt * dy[i];
+        return dv;
+    }
+    if (model == ORC_LR91_CELLML) {
+        if (update_v) dv = cmx::lr91::dvdt<double>(y, y[0], stim_area, p, cm);
+        if (rl) cmx::lr91::evaluate<double, true>(y, y[0], stim_area, p, cm, dy, ginf, gk);
+        else cmx::lr91::evaluate<double, false>(y, y[0], stim_area, p, cm, dy, ginf, gk);
+        for (int i = 1; i < S; ++i) {
+            if (rl && cmx::lr91::kIsGate[i]) y[i] = ginf[i] + (y[i] - ginf[i]) * std::exp(gk[i] * dt);
+            else y[i] += dt * dy[i];
+        }
+    } else {
+        if (update_v) dv = cmx::tt06_epi::dvdt<double>(y, y[0], stim_area, p, cm);
+        if (rl) cmx::tt06_epi::evaluate<double, true>(y, y[0], stim_area, p, cm, dy, ginf, gk);
+        else cmx::tt06_epi::evaluate<double, false>(y, y[0], stim_area, p, cm, dy, ginf, gk);
+        for (int i = 1; i < S; ++i) {
+            if (rl && cmx::tt06_epi::kIsGate[i]) y[i] = ginf[i] + (y[i] - ginf[i]) * std::exp(gk[i] * dt);
+            else y[i] += dt * dy[i];
+        }
+    }
+    if (update_v) y[0] += dt * dv;
+    return dv;
+}
+
+double cell_iionic(int model, const double* p, double cm, const double* y)
+{
+    switch (model) {
+    case ORC_LR91_HAND: return Lr91Hand::currents(y, nullptr);
+    case ORC_LR91_CELLML: return cmx::lr91::iionic<double>(y, y[0], p, cm);
+    default: return cmx::tt06_epi::iionic<double>(y, y[0], p, cm);
+    }
+}
+
+const double* default_params(int model)
+{
+    return model == ORC_TT06_EPI ? cmx::tt06_epi::kParamDefaults : cmx::lr91::kParamDefaults;
+}
+
+int num_params(int model)
+{
+    return model == ORC_TT06_EPI ? cmx::tt06_epi::kNumParams : cmx::lr91::kNumParams;
+}
+
+// TimeStepper (global/src/TimeStepper.cpp:43-200): t_k = start + k*dt, the last
+// step snapped to the end when within eps.
+struct Stepper {
+    double start, end, dt, eps, time, next;
+    unsigned long long k = 0;
+    Stepper(double s, double e, double d) : start(s), end(e), dt(d), time(s)
+    {
+        eps = end > 1.0 ? DBL_EPSILON * end : DBL_EPSILON;
+        next = calc();
+    }
+    double calc() const
+    {
+        double n = start + (k + 1) * dt;
+        if (end - n <= eps) n = end;
+        return n;
+    }
+    bool at_end() const { return time >= end; }
+    double step_size() const { return next >= end ? end - time : dt; }
+    void advance() { ++k; time = next; next = calc(); }
+};
+
+double seconds_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_model_num_states(int model) { return num_states(model); }
+
+int orc_cell_compute(int model, int rush_larsen, const double* params, double* y,
+                     double t0, double t_end, double dt,
+                     double stim_mag, double stim_dur, double stim_start, double stim_period,
+                     int steps_per_sample, double* samples, int max_samples)
+{
+    const int S = num_states(model);
+    if (S < 0 || dt <= 0 || t_end <= t0 || steps_per_sample <= 0) return -1;
+    if (model == ORC_LR91_HAND && rush_larsen) return -1;
+    const double* p = params ? params : default_params(model);
+    Stimulus stim(stim_mag, stim_dur, stim_start, stim_period);
+    int ns = 0;
+    auto record = [&]() {
+        if (samples && ns < max_samples) std::memcpy(samples + (size_t)ns * S, y, sizeof(double) * S);
+        ++ns;
+    };
+    record();
+    // AbstractCardiacCell::Compute with tSamp = dt: an outer TimeStepper at dt,
+    // each sample interval integrated by one inner step.  Rush-Larsen cells
+    // use curr_time = tStart + i*dt (AbstractRushLarsenCardiacCell.cpp:89-103).
+    Stepper st(t0, t_end, dt);
+    unsigned long long count = 0;
+    while (!st.at_end()) {
+        const double t = rush_larsen ? t0 + st.k * dt : st.time;
+        const double h = rush_larsen ? dt : st.step_size();
+        cell_substep(model, rush_larsen != 0, p, 1.0, y, t, h, stim.at(t), true);
+        st.advance();
+        ++count;
+        if (count % steps_per_sample == 0) record();
+    }
+    return ns;
+}
+
+double orc_cell_iionic(int model, const double* params, const double* y)
+{
+    return cell_iionic(model, params ? params : default_params(model), 1.0, y);
+}
+
+int orc_slab_sizes(int dim, int ex, int ey, int ez, int* n_nodes, int* n_elems)
+{
+    if (dim == 1) { *n_nodes = ex + 1; *n_elems = ex; return 0; }
+    if (dim == 2) { *n_nodes = (ex + 1) * (ey + 1); *n_elems = 2 * ex * ey; return 0; }
+    if (dim == 3) { *n_nodes = (ex + 1) * (ey + 1) * (ez + 1); *n_elems = 6 * ex * ey * ez; return 0; }
+    return -1;
+}
+
+// ConstructLinearMesh / ConstructRectangularMesh (stagger) / ConstructCuboid,
+// then Scale(h): mesh/src/common/AbstractTetrahedralMesh.cpp:277-614.
+int orc_slab_fill(int dim, int ex, int ey, int ez, double h, double* coords, int* elems)
+{
+    if (dim == 1) {
+        for (int i = 0; i <= ex; ++i) coords[i] = i * h;
+        for (int e = 0; e < ex; ++e) { elems[2 * e] = e; elems[2 * e + 1] = e + 1; }
+        return 0;
+    }
+    if (dim == 2) {
+        const int W = ex, H = ey;
+        for (int j = 0; j <= H; ++j)
+            for (int i = 0; i <= W; ++i) {
+                const int n = j * (W + 1) + i;
+                coords[2 * n] = i * h;
+                coords[2 * n + 1] = j * h;
+            }
+        int e = 0;
+        for (int j = 0; j < H; ++j)
+            for (int i = 0; i < W; ++i) {
+                const int parity = (i + (H - j)) % 2;
+                const int nw = (j + 1) * (W + 1) + i, sw = j * (W + 1) + i;
+                int* up = elems + 3 * e++;
+                up[0] = nw; up[1] = nw + 1; up[2] = parity == 1 ? sw + 1 : sw;
+                int* lo = elems + 3 * e++;
+                lo[0] = sw + 1; lo[1] = sw; lo[2] = parity == 1 ? nw : nw + 1;
+            }
+        return 0;
+    }
+    if (dim == 3) {
+        const int W = ex, H = ey, D = ez;
+        for (int k = 0; k <= D; ++k)
+            for (int j = 0; j <= H; ++j)
+                for (int i = 0; i <= W; ++i) {
+                    const long n = ((long)k * (H + 1) + j) * (W + 1) + i;
+                    coords[3 * n] = i * h; coords[3 * n + 1] = j * h; coords[3 * n + 2] = k * h;
+                }
+        static const int tets[6][4] = {{0, 1, 5, 7}, {0, 1, 3, 7}, {0, 2, 3, 7},
+                                       {0, 2, 6, 7}, {0, 4, 6, 7}, {0, 4, 5, 7}};
+        long e = 0;
+        for (int k = 0; k < D; ++k)
+            for (int j = 0; j < H; ++j)
+                for (int i = 0; i < W; ++i) {
+                    int corner[8], c = 0;
+                    for (int z = 0; z < 2; ++z)
+                        for (int y = 0; y < 2; ++y)
+                            for (int x = 0; x < 2; ++x)
+                                corner[c++] = i + x + (W + 1) * (j + y + (H + 1) * (k + z));
+                    for (int m = 0; m < 6; ++m, ++e)
+                        for (int q = 0; q < 4; ++q) elems[4 * e + q] = corner[tets[m][q]];
+                }
+        return 0;
+    }
+    return -1;
+}
+
+// Element integrals with the reference's 2nd-order Gauss rules
+// (pde/src/solver/GaussianQuadratureRule.cpp): 1D 2 points, triangles 3
+// points, tetrahedra 4 points; P1 basis phi_0 = 1 - sum(xi), phi_k = xi_k.
+int orc_assemble(int dim, int n, const double* coords, int ne, const int* elems,
+                 const double* sigma, int lumped, int* nnz_out,
+                 int* row_ptr, int* col, double* mass, double* stiff)
+{
+    if (dim < 1 || dim > 3) return -1;
+    const int nb = dim + 1;
+    // Pattern: node adjacency through elements.
+    std::vector<std::vector<int>> adj(n);
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < nb; ++a)
+            for (int b = 0; b < nb; ++b) adj[elems[e * nb + a]].push_back(elems[e * nb + b]);
+    long nnz = 0;
+    for (auto& r : adj) {
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        nnz += r.size();
+    }
+    *nnz_out = (int)nnz;
+    if (!col) return 0;
+    row_ptr[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        row_ptr[i + 1] = row_ptr[i] + (int)adj[i].size();
+        std::copy(adj[i].begin(), adj[i].end(), col + row_ptr[i]);
+    }
+    std::fill(mass, mass + nnz, 0.0);
+    std::fill(stiff, stiff + nnz, 0.0);
+
+    std::vector<std::vector<double>> qp;
+    std::vector<double> qw;
+    if (dim == 1) {
+        const double s = std::sqrt(1.0 / 3.0);
+        qw = {0.5, 0.5};
+        qp = {{(-s + 1.0) / 2.0}, {(s + 1.0) / 2.0}};
+    } else if (dim == 2) {
+        qw = {1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0};
+        qp = {{2.0 / 3.0, 1.0 / 6.0}, {1.0 / 6.0, 1.0 / 6.0}, {1.0 / 6.0, 2.0 / 3.0}};
+    } else {
+        const double s = std::sqrt(0.2);
+        const double a = (1.0 + 3.0 * s) / 4.0, b = (1.0 - s) / 4.0;
+        qw = {1.0 / 24, 1.0 / 24, 1.0 / 24, 1.0 / 24};
+        qp = {{a, b, b}, {b, a, b}, {b, b, a}, {b, b, b}};
+    }
+
+    auto find = [&](int r, int c) {
+        const int* b = col + row_ptr[r];
+        const int* e = col + row_ptr[r + 1];
+        return (int)(std::lower_bound(b, e, c) - col);
+    };
+
+    for (int e = 0; e < ne; ++e) {
+        const int* nd = elems + e * nb;
+        // Jacobian J[r][c] = d x_r / d xi_c = x_{c+1,r} - x_{0,r}
+        double J[3][3] = {}, Ji[3][3] = {};
+        for (int r = 0; r < dim; ++r)
+            for (int c = 0; c < dim; ++c) J[r][c] = coords[nd[c + 1] * dim + r] - coords[nd[0] * dim + r];
+        double det;
+        if (dim == 1) { det = J[0][0]; Ji[0][0] = 1.0 / det; }
+        else if (dim == 2) {
+            det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+            Ji[0][0] = J[1][1] / det; Ji[0][1] = -J[0][1] / det;
+            Ji[1][0] = -J[1][0] / det; Ji[1][1] = J[0][0] / det;
+        } else {
+            det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1])
+                - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0])
+                + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+            Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+            Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+            Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+            Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+            Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+            Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+            Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+            Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+            Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+        }
+        const double adet = std::fabs(det);
+        // Physical gradients: grad phi_k = Ji^T grad_ref phi_k (constant per element).
+        double gref[4][3] = {};
+        for (int c = 0; c < dim; ++c) { gref[0][c] = -1.0; gref[c + 1][c] = 1.0; }
+        double g[4][3] = {};
+        for (int k = 0; k < nb; ++k)
+            for (int r = 0; r < dim; ++r) {
+                double s = 0;
+                for (int c = 0; c < dim; ++c) s += Ji[c][r] * gref[k][c];
+                g[k][r] = s;
+            }
+        double Me[4][4] = {}, Ke[4][4] = {};
+        for (size_t q = 0; q < qw.size(); ++q) {
+            double phi[4];
+            double sum = 0;
+            for (int c = 0; c < dim; ++c) { phi[c + 1] = qp[q][c]; sum += qp[q][c]; }
+            phi[0] = 1.0 - sum;
+            const double wJ = qw[q] * adet;
+            for (int a = 0; a < nb; ++a)
+                for (int b = 0; b < nb; ++b) {
+                    Me[a][b] += wJ * phi[a] * phi[b];
+                    double s = 0;
+                    for (int r = 0; r < dim; ++r)
+                        for (int c = 0; c < dim; ++c) s += g[a][r] * sigma[r * dim + c] * g[b][c];
+                    Ke[a][b] += wJ * s;
+                }
+        }
+        if (lumped) {
+            // MassMatrixAssembler.hpp:88-101: row sums onto the diagonal.
+            for (int a = 0; a < nb; ++a) {
+                double s = 0;
+                for (int b = 0; b < nb; ++b) { s += Me[a][b]; Me[a][b] = 0; }
+                Me[a][a] = s;
+            }
+        }
+        for (int a = 0; a < nb; ++a)
+            for (int b = 0; b < nb; ++b) {
+                const int p = find(nd[a], nd[b]);
+                mass[p] += Me[a][b];
+                stiff[p] += Ke[a][b];
+            }
+    }
+    return 0;
+}
+
+int orc_pcg(int n, const int* row_ptr, const int* col, const double* val,
+            const double* b, double* x, double rtol, double atol, int max_its)
+{
+    std::vector<double> r(n), z(n), p(n), q(n), dinv(n);
+    for (int i = 0; i < n; ++i) {
+        double d = 0;
+        for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
+            if (col[k] == i) d = val[k];
+        dinv[i] = 1.0 / d;
+    }
+    auto spmv = [&](const double* v, double* out) {
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            double s = 0;
+            for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) s += val[k] * v[col[k]];
+            out[i] = s;
+        }
+    };
+    double bnorm2 = 0;
+#pragma omp parallel for reduction(+ : bnorm2) schedule(static)
+    for (int i = 0; i < n; ++i) bnorm2 += (dinv[i] * b[i]) * (dinv[i] * b[i]);
+    const double tol = std::max(rtol * std::sqrt(bnorm2), atol);
+    spmv(x, q.data());
+    double rz = 0, zz = 0;
+#pragma omp parallel for reduction(+ : rz, zz) schedule(static)
+    for (int i = 0; i < n; ++i) {
+        r[i] = b[i] - q[i];
+        z[i] = dinv[i] * r[i];
+        p[i] = z[i];
+        rz += r[i] * z[i];
+        zz += z[i] * z[i];
+    }
+    if (std::sqrt(zz) <= tol) return 0;
+    for (int it = 1; it <= max_its; ++it) {
+        spmv(p.data(), q.data());
+        double pq = 0;
+#pragma omp parallel for reduction(+ : pq) schedule(static)
+        for (int i = 0; i < n; ++i) pq += p[i] * q[i];
+        const double alpha = rz / pq;
+        double rz_new = 0;
+        zz = 0;
+#pragma omp parallel for reduction(+ : rz_new, zz) schedule(static)
+        for (int i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+            z[i] = dinv[i] * r[i];
+            rz_new += r[i] * z[i];
+            zz += z[i] * z[i];
+        }
+        if (std::sqrt(zz) <= tol) return it;
+        const double beta = rz_new / rz;
+        rz = rz_new;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    return -1;
+}
+
+int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
+                 double* act_out, double act_threshold,
+                 int n_trace, const int* trace_nodes, double* v_trace,
+                 orc_mono_stats* stats)
+{
+    const auto t_start = std::chrono::steady_clock::now();
+#ifdef _OPENMP
+    if (cfg->threads > 0) omp_set_num_threads(cfg->threads);
+#endif
+    const int dim = cfg->dim, model = cfg->model;
+    const int S = num_states(model);
+    if (S < 0 || (model == ORC_LR91_HAND && cfg->rush_larsen)) return -1;
+    int n, ne;
+    if (orc_slab_sizes(dim, cfg->ex, cfg->ey, cfg->ez, &n, &ne)) return -1;
+    std::vector<double> coords((size_t)n * dim);
+    std::vector<int> elems((size_t)ne * (dim + 1));
+    orc_slab_fill(dim, cfg->ex, cfg->ey, cfg->ez, cfg->h, coords.data(), elems.data());
+    double sigma[9] = {};
+    for (int r = 0; r < dim; ++r) sigma[r * dim + r] = cfg->sigma[r];
+    int nnz;
+    orc_assemble(dim, n, coords.data(), ne, elems.data(), sigma, 0, &nnz, nullptr, nullptr, nullptr, nullptr);
+    std::vector<int> rp(n + 1), col(nnz);
+    std::vector<double> M(nnz), K(nnz), A(nnz);
+    orc_assemble(dim, n, coords.data(), ne, elems.data(), sigma, 0, &nnz, rp.data(), col.data(), M.data(), K.data());
+    // A = (Am Cm / dt) M + K  (MonodomainAssembler.cpp:42-66)
+    const double am = cfg->am, cm = cfg->cm, dt = cfg->dt_pde;
+    for (int k = 0; k < nnz; ++k) A[k] = (1.0 / dt) * (am * cm * M[k]) + K[k];
+
+    // Cells: state-major planes y[s*n + i]; V lives in its own vector.
+    std::vector<double> Y((size_t)S * n);
+    const double* y0 = initial_state(model);
+    for (int s = 0; s < S; ++s)
+        for (int i = 0; i < n; ++i) Y[(size_t)s * n + i] = y0[s];
+    std::vector<double> V(n), Vprev(n), iion(n), istim(n), z(n), b(n);
+    for (int i = 0; i < n; ++i) V[i] = y0[0];
+    std::vector<char> stimulated(n);
+    for (int i = 0; i < n; ++i) stimulated[i] = coords[(size_t)i * dim] < cfg->stim_xlim;
+    Stimulus stim(cfg->stim_mag, cfg->stim_dur, cfg->stim_start, -1.0);
+    const int P = num_params(model);
+    const double* pdef = default_params(model);
+    if (act_out) std::fill(act_out, act_out + n, -1.0);
+
+    orc_mono_stats st{};
+    st.n_nodes = n;
+    st.nnz = nnz;
+    Stepper pde(0.0, cfg->t_end, dt);
+    int step = 0;
+    while (!pde.at_end()) {
+        const double t = pde.time, t_next = pde.next;
+        // SolveCellSystems(V, t, t_next): every node integrated with V clamped
+        // over [t, t_next] in ODE substeps, then Iion and Istim(t_next) cached.
+        auto c0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            double y[64], p[64];
+            for (int s = 0; s < S; ++s) y[s] = Y[(size_t)s * n + i];
+            for (int q = 0; q < P; ++q)
+                p[q] = (cfg->params && q < cfg->n_param) ? cfg->params[(size_t)q * n + i] : pdef[q];
+            y[0] = V[i];
+            Stepper ode(t, t_next, cfg->dt_ode);
+            while (!ode.at_end()) {
+                const double ts = ode.time;
+                const double sa = stimulated[i] ? stim.at(ts) / am : 0.0;
+                cell_substep(model, cfg->rush_larsen != 0, p, cm, y, ts, ode.step_size(), sa, false);
+                ode.advance();
+            }
+            y[0] = V[i];
+            for (int s = 1; s < S; ++s) Y[(size_t)s * n + i] = y[s];
+            iion[i] = cell_iionic(model, p, cm, y);
+            istim[i] = stimulated[i] ? stim.at(t_next) : 0.0;
+        }
+        st.t_ode += seconds_since(c0);
+        // z = Am Cm V / dt - Am Iion - Istim ; b = M z
+        c0 = std::chrono::steady_clock::now();
+        const double inv_dt = 1.0 / (t_next - t);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) z[i] = am * cm * V[i] * inv_dt + (-am * iion[i] - istim[i]);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            double s = 0;
+            for (int k = rp[i]; k < rp[i + 1]; ++k) s += M[k] * z[col[k]];
+            b[i] = s;
+        }
+        st.t_rhs += seconds_since(c0);
+        c0 = std::chrono::steady_clock::now();
+        Vprev = V;
+        const int its = orc_pcg(n, rp.data(), col.data(), A.data(), b.data(), V.data(),
+                                cfg->ksp_rtol, cfg->ksp_atol, cfg->max_its);
+        st.t_ksp += seconds_since(c0);
+        if (its < 0) return -2;
+        st.cg_iterations += its;
+        pde.advance();
+        if (act_out) {
+            for (int i = 0; i < n; ++i)
+                if (act_out[i] < 0 && Vprev[i] < act_threshold && V[i] >= act_threshold)
+                    act_out[i] = t + (t_next - t) * (act_threshold - Vprev[i]) / (V[i] - Vprev[i]);
+        }
+        if (v_trace)
+            for (int q = 0; q < n_trace; ++q) v_trace[(size_t)step * n_trace + q] = V[trace_nodes[q]];
+        ++step;
+    }
+    st.steps = step;
+    st.t_total = seconds_since(t_start);
+    if (V_out) std::copy(V.begin(), V.end(), V_out);
+    if (state_out) {
+        std::copy(Y.begin(), Y.end(), state_out);
+        std::copy(V.begin(), V.end(), state_out);
+    }
+    if (stats) *stats = st;
+    return 0;
+}
+
+}  // extern "C"
