@@ -1,0 +1,40 @@
+"""Build recipe for libcmx.so (gfx950) and the C++ host layer."""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, 'chaste_amd')
+CSRC = os.path.join(PKG, 'csrc')
+LIBDIR = os.path.join(PKG, 'lib')
+
+HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+# Packed FP32 instructions stay off: hipcc -O3 emits v_pk_* from scalar code.
+HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
+             '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+             '-I' + os.path.join(ROOT, 'include'), '-I' + CSRC]
+
+SOURCES = ['cmx_device.hip', 'cmx_host.cpp']
+
+
+def library_path():
+    return os.path.join(LIBDIR, 'libcmx.so')
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError('build failed: %s\n%s' % (' '.join(cmd), r.stdout))
+    return r.stdout
+
+
+def build_library(verbose=False):
+    """Compile every HIP/C++ source into chaste_amd/lib/libcmx.so."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = library_path()
+    tmp = out + '.tmp'
+    cmd = [HIPCC] + HIP_FLAGS + ['-shared', '-o', tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+    log = _run(cmd)
+    os.replace(tmp, out)
+    if verbose:
+        print(log)
+    return out

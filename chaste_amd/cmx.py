@@ -1,0 +1,236 @@
+"""ctypes binding of the C ABI in include/cmx.h (test and benchmark harness).
+
+Every call goes to libcmx.so; there is no fallback: a missing or unloadable
+library raises immediately.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .build import library_path
+
+FP64, MIXED = 0, 1
+LR91, TT06_EPI = 0, 1
+FORWARD_EULER, RUSH_LARSEN = 0, 1
+FMT_AUTO, FMT_STENCIL, FMT_ELL = 0, 1, 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class CmxMesh(ctypes.Structure):
+    _fields_ = [('dim', ctypes.c_int32), ('n_nodes', ctypes.c_int32), ('coords', _dp),
+                ('n_elems', ctypes.c_int32), ('elems', _ip), ('sigma_elem', _dp),
+                ('sigma', ctypes.c_double * 9)]
+
+
+class CmxStimulus(ctypes.Structure):
+    _fields_ = [('magnitude', ctypes.c_double), ('start', ctypes.c_double),
+                ('duration', ctypes.c_double), ('period', ctypes.c_double), ('stop', ctypes.c_double)]
+
+
+class CmxTissue(ctypes.Structure):
+    _fields_ = [('model', ctypes.c_int32), ('integrator', ctypes.c_int32),
+                ('n_stimuli', ctypes.c_int32), ('stimuli', ctypes.POINTER(CmxStimulus)),
+                ('stim_index', _ip), ('n_params', ctypes.c_int32), ('params', _dp)]
+
+
+class CmxOpts(ctypes.Structure):
+    _fields_ = [('precision', ctypes.c_int32), ('format', ctypes.c_int32),
+                ('dt_pde', ctypes.c_double), ('dt_ode', ctypes.c_double),
+                ('am', ctypes.c_double), ('cm', ctypes.c_double),
+                ('ksp_rtol', ctypes.c_double), ('ksp_atol', ctypes.c_double),
+                ('ksp_max_its', ctypes.c_int32), ('refine_sweeps', ctypes.c_int32),
+                ('use_graph', ctypes.c_int32), ('mass_lumping', ctypes.c_int32)]
+
+
+class CmxStats(ctypes.Structure):
+    _fields_ = [('steps', ctypes.c_int64), ('cg_iterations', ctypes.c_int64),
+                ('refine_sweeps', ctypes.c_int64), ('last_iterations', ctypes.c_int32),
+                ('format', ctypes.c_int32), ('n_nodes', ctypes.c_int32), ('n_states', ctypes.c_int32),
+                ('row_width', ctypes.c_int32), ('n_patterns', ctypes.c_int32), ('nnz', ctypes.c_int64),
+                ('last_residual', ctypes.c_double), ('device_bytes', ctypes.c_double)]
+
+
+# Every symbol include/cmx.h declares, with its ctypes signature.
+SIGNATURES = {
+    'cmx_create': (ctypes.c_int, [ctypes.POINTER(CmxMesh), ctypes.POINTER(CmxTissue),
+                                  ctypes.POINTER(CmxOpts), ctypes.POINTER(ctypes.c_void_p)]),
+    'cmx_destroy': (ctypes.c_int, [ctypes.c_void_p]),
+    'cmx_last_error': (ctypes.c_char_p, [ctypes.c_void_p]),
+    'cmx_step': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
+    'cmx_get_voltage': (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    'cmx_set_voltage': (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    'cmx_get_state': (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    'cmx_set_state': (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    'cmx_get_iionic': (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    'cmx_get_stats': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CmxStats)]),
+    'cmx_model_info': (ctypes.c_int, [ctypes.c_int32, _ip, _ip]),
+    'cmx_model_state_name': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p)]),
+    'cmx_model_initial_state': (ctypes.c_int, [ctypes.c_int32, _dp]),
+    'cmx_slab_mesh': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_double, _ip, _ip, _dp, _ip]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load libcmx.so (raises if it is missing: there is no CPU fallback)."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError('libcmx.so not built (%s): run __graft_entry__.build()' % path)
+        L = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class CmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__('cmx error %d: %s' % (code, msg))
+        self.code = code
+
+
+def _check(rc, ctx=None):
+    if rc != 0:
+        raise CmxError(rc, lib().cmx_last_error(ctx).decode())
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def model_info(model):
+    s, p = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().cmx_model_info(model, ctypes.byref(s), ctypes.byref(p)))
+    return s.value, p.value
+
+
+def state_names(model):
+    S, _ = model_info(model)
+    out = []
+    for i in range(S):
+        c = ctypes.c_char_p()
+        _check(lib().cmx_model_state_name(model, i, ctypes.byref(c)))
+        out.append(c.value.decode())
+    return out
+
+
+def initial_state(model):
+    S, _ = model_info(model)
+    y = np.zeros(S)
+    _check(lib().cmx_model_initial_state(model, _p(y)))
+    return y
+
+
+def slab_mesh(dim, ex, ey=0, ez=0, h=0.01):
+    """ConstructRegularSlabMesh-numbered slab: (coords[n, dim], elems[ne, dim+1])."""
+    n, ne = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().cmx_slab_mesh(dim, ex, ey, ez, h, ctypes.byref(n), ctypes.byref(ne), None, None))
+    coords = np.zeros((n.value, dim))
+    elems = np.zeros((ne.value, dim + 1), dtype=np.int32)
+    _check(lib().cmx_slab_mesh(dim, ex, ey, ez, h, None, None, _p(coords), _p(elems, _ip)))
+    return coords, elems
+
+
+class Monodomain:
+    """One cmx context: a tissue on a mesh with its options."""
+
+    def __init__(self, coords, elems, sigma, model, integrator=FORWARD_EULER,
+                 stimuli=(), stim_index=None, params=None, precision=FP64,
+                 dt_pde=0.01, dt_ode=0.01, am=1400.0, cm=1.0, ksp_rtol=1e-8, ksp_atol=0.0,
+                 ksp_max_its=1000, refine_sweeps=0, fmt=FMT_AUTO, mass_lumping=False,
+                 sigma_elem=None, use_graph=False):
+        L = lib()
+        self._keep = []
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        elems = np.ascontiguousarray(elems, dtype=np.int32)
+        n, dim = coords.shape
+        self.n, self.dim, self.model = n, dim, model
+        self.n_states, self.n_params = model_info(model)
+        m = CmxMesh()
+        m.dim, m.n_nodes, m.coords = dim, n, _p(coords)
+        m.n_elems, m.elems = elems.shape[0], _p(elems, _ip)
+        sig = np.zeros(9)
+        s = np.asarray(sigma, dtype=np.float64)
+        if s.ndim == 1:
+            s = np.diag(s[:dim])
+        sig[:dim * dim] = s.reshape(-1)
+        m.sigma = (ctypes.c_double * 9)(*sig)
+        if sigma_elem is not None:
+            se = np.ascontiguousarray(sigma_elem, dtype=np.float64)
+            self._keep.append(se)
+            m.sigma_elem = _p(se)
+        t = CmxTissue()
+        t.model, t.integrator = model, integrator
+        st = (CmxStimulus * max(len(stimuli), 1))()
+        for k, row in enumerate(stimuli):
+            mag, start, dur = row[0], row[1], row[2]
+            period = row[3] if len(row) > 3 else -1.0
+            stop = row[4] if len(row) > 4 else 0.0
+            st[k] = CmxStimulus(mag, start, dur, period, stop)
+        t.n_stimuli, t.stimuli = len(stimuli), st
+        if stim_index is not None:
+            si = np.ascontiguousarray(stim_index, dtype=np.int32)
+            self._keep.append(si)
+            t.stim_index = _p(si, _ip)
+        if params is not None:
+            pa = np.ascontiguousarray(params, dtype=np.float64)
+            self._keep.append(pa)
+            t.n_params, t.params = pa.shape[0], _p(pa)
+        o = CmxOpts(precision, fmt, dt_pde, dt_ode, am, cm, ksp_rtol, ksp_atol, ksp_max_its,
+                    refine_sweeps, 1 if use_graph else 0, 1 if mass_lumping else 0)
+        self.dt = dt_pde
+        ctx = ctypes.c_void_p()
+        rc = L.cmx_create(ctypes.byref(m), ctypes.byref(t), ctypes.byref(o), ctypes.byref(ctx))
+        _check(rc, None)
+        self.ctx = ctx
+
+    def close(self):
+        if getattr(self, 'ctx', None):
+            lib().cmx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, t0, n_steps):
+        _check(lib().cmx_step(self.ctx, t0, n_steps), self.ctx)
+
+    def voltage(self):
+        V = np.zeros(self.n)
+        _check(lib().cmx_get_voltage(self.ctx, _p(V)), self.ctx)
+        return V
+
+    def set_voltage(self, V):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        _check(lib().cmx_set_voltage(self.ctx, _p(V)), self.ctx)
+
+    def state(self):
+        S = np.zeros((self.n_states, self.n))
+        _check(lib().cmx_get_state(self.ctx, _p(S)), self.ctx)
+        return S
+
+    def set_state(self, S):
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        _check(lib().cmx_set_state(self.ctx, _p(S)), self.ctx)
+
+    def iionic(self):
+        out = np.zeros(self.n)
+        _check(lib().cmx_get_iionic(self.ctx, _p(out)), self.ctx)
+        return out
+
+    def stats(self):
+        s = CmxStats()
+        _check(lib().cmx_get_stats(self.ctx, ctypes.byref(s)), self.ctx)
+        return {k: getattr(s, k) for k, _ in CmxStats._fields_}

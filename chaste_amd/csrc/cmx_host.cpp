@@ -1,0 +1,282 @@
+// Host-side setup of the monodomain operators: slab meshes, P1 assembly and
+// the sparse formats the device kernels read.  Runs once per context (the
+// reference assembles once too: MonodomainSolver has mMatrixIsConstant,
+// heart/src/solver/electrics/monodomain/MonodomainSolver.cpp:190).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+#include "cmx_internal.h"
+
+namespace cmx {
+
+namespace {
+
+// Inverse-transpose gradients of the P1 basis on one simplex and its measure.
+// grads[a][r] = d phi_a / d x_r.  Returns false for a degenerate element.
+bool simplex_geometry(int dim, const double* X /*[(dim+1)][dim]*/, double grads[4][3], double& measure)
+{
+    double J[3][3] = {};
+    for (int r = 0; r < dim; ++r)
+        for (int c = 0; c < dim; ++c) J[r][c] = X[(c + 1) * dim + r] - X[r];
+    double inv[3][3] = {};
+    double det;
+    if (dim == 1) {
+        det = J[0][0];
+        if (det == 0.0) return false;
+        inv[0][0] = 1.0 / det;
+    } else if (dim == 2) {
+        det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        if (det == 0.0) return false;
+        inv[0][0] = J[1][1] / det;  inv[0][1] = -J[0][1] / det;
+        inv[1][0] = -J[1][0] / det; inv[1][1] = J[0][0] / det;
+    } else {
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        if (det == 0.0) return false;
+        inv[0][0] = c00 / det;
+        inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+        inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+        inv[1][0] = c01 / det;
+        inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+        inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+        inv[2][0] = c02 / det;
+        inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+        inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    }
+    // reference gradients: phi_0 = 1 - sum xi, phi_k = xi_k
+    for (int r = 0; r < dim; ++r) {
+        double s = 0;
+        for (int c = 0; c < dim; ++c) s += inv[c][r];
+        grads[0][r] = -s;
+        for (int a = 1; a <= dim; ++a) grads[a][r] = inv[a - 1][r];
+    }
+    double fact = 1.0;
+    for (int d = 2; d <= dim; ++d) fact *= d;
+    measure = std::fabs(det) / fact;
+    return true;
+}
+
+}  // namespace
+
+bool assemble_p1(const cmx_mesh& mesh, bool lumped, HostCsr& out, std::string& err)
+{
+    const int dim = mesh.dim, n = mesh.n_nodes, ne = mesh.n_elems, nb = dim + 1;
+    if (dim < 1 || dim > 3 || n <= 0 || ne <= 0 || !mesh.coords || !mesh.elems) {
+        err = "mesh: bad dimension, sizes or null arrays";
+        return false;
+    }
+    for (long e = 0; e < (long)ne * nb; ++e)
+        if (mesh.elems[e] < 0 || mesh.elems[e] >= n) {
+            err = "mesh: element references node " + std::to_string(mesh.elems[e]) + " out of range";
+            return false;
+        }
+    // Sparsity: nodes sharing an element.
+    std::vector<int32_t> deg(n, 0);
+    std::vector<std::vector<int32_t>> adj(n);
+    for (int e = 0; e < ne; ++e) {
+        const int32_t* nd = mesh.elems + (size_t)e * nb;
+        for (int a = 0; a < nb; ++a)
+            for (int b = 0; b < nb; ++b) adj[nd[a]].push_back(nd[b]);
+    }
+    out.n = n;
+    out.row_ptr.assign(n + 1, 0);
+    out.max_row = 0;
+    for (int i = 0; i < n; ++i) {
+        auto& r = adj[i];
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        out.row_ptr[i + 1] = out.row_ptr[i] + (int32_t)r.size();
+        out.max_row = std::max<int>(out.max_row, (int)r.size());
+    }
+    const size_t nnz = out.row_ptr[n];
+    out.col.resize(nnz);
+    for (int i = 0; i < n; ++i) std::copy(adj[i].begin(), adj[i].end(), out.col.begin() + out.row_ptr[i]);
+    adj.clear();
+    adj.shrink_to_fit();
+    out.mass.assign(nnz, 0.0);
+    out.stiff.assign(nnz, 0.0);
+
+    const double mass_scale = 1.0 / ((dim + 1) * (dim + 2));
+    for (int e = 0; e < ne; ++e) {
+        const int32_t* nd = mesh.elems + (size_t)e * nb;
+        double X[4 * 3];
+        for (int a = 0; a < nb; ++a)
+            for (int r = 0; r < dim; ++r) X[a * dim + r] = mesh.coords[(size_t)nd[a] * dim + r];
+        double g[4][3], vol;
+        if (!simplex_geometry(dim, X, g, vol)) {
+            err = "mesh: element " + std::to_string(e) + " is degenerate";
+            return false;
+        }
+        const double* sig = mesh.sigma_elem ? mesh.sigma_elem + (size_t)e * dim * dim : mesh.sigma;
+        // (sigma G^T) columns
+        double sg[4][3];
+        for (int b = 0; b < nb; ++b)
+            for (int r = 0; r < dim; ++r) {
+                double s = 0;
+                for (int c = 0; c < dim; ++c) s += sig[r * dim + c] * g[b][c];
+                sg[b][r] = s;
+            }
+        for (int a = 0; a < nb; ++a) {
+            const int i = nd[a];
+            const int32_t* cb = out.col.data() + out.row_ptr[i];
+            const int32_t* ce = out.col.data() + out.row_ptr[i + 1];
+            for (int b = 0; b < nb; ++b) {
+                const size_t p = std::lower_bound(cb, ce, nd[b]) - out.col.data();
+                double kab = 0;
+                for (int r = 0; r < dim; ++r) kab += g[a][r] * sg[b][r];
+                out.stiff[p] += vol * kab;
+                if (lumped) {
+                    // row sum of the consistent element mass, onto the diagonal
+                    // (pde/src/solver/MassMatrixAssembler.hpp:88-101)
+                    if (a == b) out.mass[p] += vol * mass_scale * (nb + 1);
+                } else {
+                    out.mass[p] += vol * mass_scale * (a == b ? 2.0 : 1.0);
+                }
+            }
+        }
+    }
+    return true;
+}
+
+bool build_stencil(const HostCsr& A, int max_patterns, StencilTable& out)
+{
+    const int n = A.n, W = A.max_row;
+    out.width = W;
+    out.pid.assign(n, 0);
+    out.off.clear();
+    out.m.clear();
+    out.k.clear();
+    // Patterns keyed by their relative offsets; candidates compared by value.
+    std::map<std::vector<int32_t>, std::vector<int>> by_offsets;
+    int count = 0;
+    std::vector<int32_t> offs;
+    for (int i = 0; i < n; ++i) {
+        const int b = A.row_ptr[i], e = A.row_ptr[i + 1];
+        offs.assign(e - b, 0);
+        double scale_m = 0, scale_k = 0;
+        for (int q = b; q < e; ++q) {
+            offs[q - b] = A.col[q] - i;
+            scale_m = std::max(scale_m, std::fabs(A.mass[q]));
+            scale_k = std::max(scale_k, std::fabs(A.stiff[q]));
+        }
+        const double tm = 1e-12 * scale_m, tk = 1e-12 * scale_k;
+        auto& cands = by_offsets[offs];
+        int found = -1;
+        for (int c : cands) {
+            bool same = true;
+            for (int q = b; q < e && same; ++q) {
+                const size_t s = (size_t)c * W + (q - b);
+                same = std::fabs(out.m[s] - A.mass[q]) <= tm && std::fabs(out.k[s] - A.stiff[q]) <= tk;
+            }
+            if (same) { found = c; break; }
+        }
+        if (found < 0) {
+            if (count >= max_patterns) return false;
+            found = count++;
+            cands.push_back(found);
+            out.off.resize((size_t)count * W, 0);
+            out.m.resize((size_t)count * W, 0.0);
+            out.k.resize((size_t)count * W, 0.0);
+            for (int q = b; q < e; ++q) {
+                const size_t s = (size_t)found * W + (q - b);
+                out.off[s] = offs[q - b];
+                out.m[s] = A.mass[q];
+                out.k[s] = A.stiff[q];
+            }
+        }
+        out.pid[i] = (uint16_t)found;
+    }
+    out.n_patterns = count;
+    return true;
+}
+
+void build_ell(const HostCsr& A, EllMatrix& out)
+{
+    const int n = A.n, W = A.max_row;
+    out.width = W;
+    out.col.assign((size_t)W * n, 0);
+    out.m.assign((size_t)W * n, 0.0);
+    out.k.assign((size_t)W * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        const int b = A.row_ptr[i], e = A.row_ptr[i + 1];
+        for (int s = 0; s < W; ++s) {
+            const size_t d = (size_t)s * n + i;
+            if (b + s < e) {
+                out.col[d] = A.col[b + s];
+                out.m[d] = A.mass[b + s];
+                out.k[d] = A.stiff[b + s];
+            } else {
+                out.col[d] = i;   // padding: self, zero value
+            }
+        }
+    }
+}
+
+int slab_mesh(int dim, int ex, int ey, int ez, double h, int* n_nodes, int* n_elems,
+              double* coords, int32_t* elems)
+{
+    if (h <= 0 || ex <= 0 || (dim >= 2 && ey <= 0) || (dim >= 3 && ez <= 0)) return CMX_E_ARG;
+    long nn, ne;
+    if (dim == 1) { nn = ex + 1; ne = ex; }
+    else if (dim == 2) { nn = (long)(ex + 1) * (ey + 1); ne = 2L * ex * ey; }
+    else if (dim == 3) { nn = (long)(ex + 1) * (ey + 1) * (ez + 1); ne = 6L * ex * ey * ez; }
+    else return CMX_E_ARG;
+    if (nn > INT32_MAX || ne > INT32_MAX) return CMX_E_ARG;
+    if (n_nodes) *n_nodes = (int)nn;
+    if (n_elems) *n_elems = (int)ne;
+    if (!coords || !elems) return CMX_OK;
+    if (dim == 1) {
+        for (int i = 0; i <= ex; ++i) coords[i] = i * h;
+        for (int e = 0; e < ex; ++e) { elems[2 * e] = e; elems[2 * e + 1] = e + 1; }
+    } else if (dim == 2) {
+        // Staggered: the diagonal of square (i, j) follows the parity of
+        // i + (ey - j) (AbstractTetrahedralMesh.cpp:349-376).
+        for (int j = 0; j <= ey; ++j)
+            for (int i = 0; i <= ex; ++i) {
+                const long q = (long)j * (ex + 1) + i;
+                coords[2 * q] = i * h;
+                coords[2 * q + 1] = j * h;
+            }
+        long e = 0;
+        for (int j = 0; j < ey; ++j)
+            for (int i = 0; i < ex; ++i) {
+                const bool odd = ((i + (ey - j)) % 2) == 1;
+                const int32_t nw = (j + 1) * (ex + 1) + i, sw = j * (ex + 1) + i;
+                int32_t* t = elems + 3 * e++;
+                t[0] = nw; t[1] = nw + 1; t[2] = odd ? sw + 1 : sw;
+                t = elems + 3 * e++;
+                t[0] = sw + 1; t[1] = sw; t[2] = odd ? nw : nw + 1;
+            }
+    } else {
+        // Six tetrahedra around the cube diagonal from local corner 0 to 7,
+        // corner c = x + 2y + 4z (AbstractTetrahedralMesh.cpp:404-440).
+        static const int kTets[6][4] = {{0, 1, 5, 7}, {0, 1, 3, 7}, {0, 2, 3, 7},
+                                        {0, 2, 6, 7}, {0, 4, 6, 7}, {0, 4, 5, 7}};
+        for (int k = 0; k <= ez; ++k)
+            for (int j = 0; j <= ey; ++j)
+                for (int i = 0; i <= ex; ++i) {
+                    const long q = ((long)k * (ey + 1) + j) * (ex + 1) + i;
+                    coords[3 * q] = i * h;
+                    coords[3 * q + 1] = j * h;
+                    coords[3 * q + 2] = k * h;
+                }
+        long e = 0;
+        for (int k = 0; k < ez; ++k)
+            for (int j = 0; j < ey; ++j)
+                for (int i = 0; i < ex; ++i) {
+                    int32_t corner[8];
+                    for (int c = 0; c < 8; ++c)
+                        corner[c] = (i + (c & 1)) + (ex + 1) * ((j + ((c >> 1) & 1)) + (ey + 1) * (k + (c >> 2)));
+                    for (int t = 0; t < 6; ++t, ++e)
+                        for (int v = 0; v < 4; ++v) elems[4 * e + v] = corner[kTets[t][v]];
+                }
+    }
+    return CMX_OK;
+}
+
+}  // namespace cmx

@@ -1,0 +1,153 @@
+/*
+ * cmx -- MI355X monodomain tissue time-step, C ABI.
+ *
+ * The library replaces the reference's monodomain hot path behind one opaque
+ * context: the cell-ODE sweep (AbstractCardiacTissue::SolveCellSystems), the
+ * ICI right-hand side b = M z (MonodomainSolver::SetupLinearSystem) and the
+ * CG diffusion solve (LinearSystem::Solve -> KSPSolve).  Plain pointers and
+ * sizes only; every call returns 0 on success or a negative CMX_E* code, with
+ * the message available from cmx_last_error().  A context is used by one host
+ * thread at a time.  Host arrays are caller-owned and copied; device buffers
+ * are owned by the context.
+ */
+#ifndef CMX_H_
+#define CMX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CMX_ABI_VERSION 1
+
+/* Error codes (reference: EXCEPTION / KSPEXCEPT, global/src/Exception.hpp,
+ * linalg/src/LinearSystem.cpp:1118-1129). */
+enum {
+    CMX_OK = 0,
+    CMX_E_ARG = -1,          /* invalid argument (size, null, unknown model) */
+    CMX_E_HIP = -2,          /* HIP runtime failure */
+    CMX_E_NOMEM = -3,        /* device allocation failed */
+    CMX_E_DIVERGED = -4,     /* CG hit max iterations (KSP_DIVERGED_ITS) */
+    CMX_E_NONFINITE = -5,    /* NaN/Inf in the solution (cell ODE blew up) */
+    CMX_E_STATE = -6         /* call out of order */
+};
+
+/* Cell models (CellML sources heart/src/odes/cellml/LuoRudy1991.cellml,
+ * TenTusscher2006Epi.cellml). */
+enum { CMX_MODEL_LR91 = 0, CMX_MODEL_TT06_EPI = 1 };
+
+/* Cell integrators: EulerIvpOdeSolver (ode/src/solver/EulerIvpOdeSolver.cpp:38-57)
+ * and AbstractRushLarsenCardiacCell (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:57-153). */
+enum { CMX_FORWARD_EULER = 0, CMX_RUSH_LARSEN = 1 };
+
+/* Precision: plain FP64, or FP32 cells + FP32 CG on the correction with FP64
+ * voltage, FP64 dot reductions and FP64 residual refinement. */
+enum { CMX_FP64 = 0, CMX_MIXED = 1 };
+
+/* Sparse format for M, K, A (chosen at create time; CMX_FMT_AUTO picks the
+ * stencil table when the mesh has few distinct row patterns). */
+enum { CMX_FMT_AUTO = 0, CMX_FMT_STENCIL = 1, CMX_FMT_ELL = 2 };
+
+/* Mesh: P1 simplices (replaces AbstractTetrahedralMesh + the FE assemblers,
+ * pde/src/solver/AbstractFeVolumeIntegralAssembler.hpp:257-459). */
+typedef struct {
+    int32_t dim;               /* 1, 2 or 3 */
+    int32_t n_nodes;
+    const double* coords;      /* [n_nodes][dim], cm */
+    int32_t n_elems;
+    const int32_t* elems;      /* [n_elems][dim+1] node indices */
+    const double* sigma_elem;  /* [n_elems][dim][dim] intracellular conductivity, or NULL */
+    double sigma[9];           /* [dim][dim] used when sigma_elem == NULL (mS/cm) */
+} cmx_mesh;
+
+/* Stimulus protocol, one row of a table (SimpleStimulus when period <= 0,
+ * heart/src/stimulus/SimpleStimulus.cpp:44-76; RegularStimulus otherwise,
+ * heart/src/stimulus/RegularStimulus.cpp:58-80).  magnitude in uA/cm^3. */
+typedef struct {
+    double magnitude, start, duration, period, stop;
+} cmx_stimulus;
+
+/* Tissue: one cell model for every node (AbstractCardiacCellFactory). */
+typedef struct {
+    int32_t model;             /* CMX_MODEL_* */
+    int32_t integrator;        /* CMX_FORWARD_EULER / CMX_RUSH_LARSEN */
+    int32_t n_stimuli;         /* rows in stimuli (<= 255) */
+    const cmx_stimulus* stimuli;
+    const int32_t* stim_index; /* [n_nodes]: -1 = no stimulus, else row; NULL = none */
+    int32_t n_params;          /* per-node parameter planes given (<= model's count) */
+    const double* params;      /* [n_params][n_nodes] or NULL (model defaults) */
+} cmx_tissue;
+
+/* Options (HeartConfig subset, heart/src/problem/HeartConfigDefaults.hpp:52-152). */
+typedef struct {
+    int32_t precision;         /* CMX_FP64 / CMX_MIXED */
+    int32_t format;            /* CMX_FMT_* */
+    double dt_pde, dt_ode;     /* ms; dt_ode must divide dt_pde */
+    double am, cm;             /* surface-area-to-volume (1/cm), capacitance (uF/cm^2) */
+    double ksp_rtol, ksp_atol; /* stop when ||D^-1 r|| <= max(rtol ||D^-1 b||, atol) */
+    int32_t ksp_max_its;
+    int32_t refine_sweeps;     /* mixed: FP64 true-residual checks per step (0 = none) */
+    int32_t use_graph;         /* capture each step as a hipGraph */
+    int32_t mass_lumping;      /* HeartConfig::SetUseMassLumping */
+} cmx_opts;
+
+typedef struct {
+    int64_t steps;             /* PDE steps taken */
+    int64_t cg_iterations;     /* total CG iterations */
+    int64_t refine_sweeps;     /* mixed: FP64 residual sweeps */
+    int32_t last_iterations;   /* CG iterations of the last step */
+    int32_t format;            /* CMX_FMT_STENCIL or CMX_FMT_ELL in use */
+    int32_t n_nodes, n_states, row_width, n_patterns;
+    int64_t nnz;
+    double last_residual;      /* ||D^-1 r|| at the end of the last solve */
+    double device_bytes;       /* resident device memory (bytes) */
+} cmx_stats;
+
+typedef struct cmx_ctx cmx_ctx;
+
+/* Replaces MonodomainProblem::Initialise -> CreateCardiacTissue + the
+ * one-time matrix assembly in MonodomainSolver::SetupLinearSystem
+ * (heart/src/problem/AbstractCardiacProblem.cpp:117-214,
+ * heart/src/solver/electrics/monodomain/MonodomainSolver.cpp:51-76).
+ * Cells start from the model's initial state, V from its initial voltage. */
+int cmx_create(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* opts,
+               cmx_ctx** out);
+int cmx_destroy(cmx_ctx* ctx);
+/* Message for the last failing call on ctx (or of the last failed create when ctx is NULL). */
+const char* cmx_last_error(const cmx_ctx* ctx);
+
+/* Advance n_steps PDE steps from time t0: t_k = t0 + k*dt_pde
+ * (AbstractDynamicLinearPdeSolver::Solve, pde/src/solver/AbstractDynamicLinearPdeSolver.hpp:328-497). */
+int cmx_step(cmx_ctx* ctx, double t0, int32_t n_steps);
+
+/* Transmembrane potential (mV): AbstractCardiacProblem::GetSolution /
+ * SetInitialCondition (heart/src/problem/AbstractCardiacProblem.cpp:265-292). */
+int cmx_get_voltage(cmx_ctx* ctx, double* V);
+int cmx_set_voltage(cmx_ctx* ctx, const double* V);
+/* Cell states as [n_states][n_nodes] planes in the model's state order; the
+ * V plane (index 0) is read from / written to the voltage vector
+ * (AbstractCardiacCellInterface::GetStdVecStateVariables / SetStateVariables). */
+int cmx_get_state(cmx_ctx* ctx, double* states);
+int cmx_set_state(cmx_ctx* ctx, const double* states);
+/* Ionic current cache (uA/cm^2) of the last step
+ * (AbstractCardiacTissue::rGetIionicCacheReplicated, heart/src/tissue/AbstractCardiacTissue.cpp:730-734). */
+int cmx_get_iionic(cmx_ctx* ctx, double* iion);
+int cmx_get_stats(cmx_ctx* ctx, cmx_stats* stats);
+
+/* Model metadata (CellML order, V first). */
+int cmx_model_info(int32_t model, int32_t* n_states, int32_t* n_params);
+int cmx_model_state_name(int32_t model, int32_t index, const char** name);
+int cmx_model_initial_state(int32_t model, double* y);
+
+/* Builds a regular slab mesh exactly as ConstructRegularSlabMesh does
+ * (mesh/src/common/AbstractTetrahedralMesh.cpp:277-614): nodes
+ * (k*(ey+1)+j)*(ex+1)+i, 2-D staggered triangles, 3-D six tetrahedra per cube.
+ * With coords == NULL only the sizes are returned. */
+int cmx_slab_mesh(int32_t dim, int32_t ex, int32_t ey, int32_t ez, double h,
+                  int32_t* n_nodes, int32_t* n_elems, double* coords, int32_t* elems);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CMX_H_ */

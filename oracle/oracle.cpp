@@ -558,7 +558,9 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
             while (!ode.at_end()) {
                 const double ts = ode.time;
                 const double sa = stimulated[i] ? stim.at(ts) / am : 0.0;
-                cell_substep(model, cfg->rush_larsen != 0, p, cm, y, ts, ode.step_size(), sa, false);
+                // Rush-Larsen cells step by dt_ode (AbstractRushLarsenCardiacCell.cpp:106-128),
+                // forward Euler by the stepper's next step (AbstractOneStepIvpOdeSolver.cpp:115-153).
+                cell_substep(model, cfg->rush_larsen != 0, p, cm, y, ts, cfg->rush_larsen ? cfg->dt_ode : ode.step_size(), sa, false);
                 ode.advance();
             }
             y[0] = V[i];

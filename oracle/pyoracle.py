@@ -1,0 +1,169 @@
+"""ctypes wrapper of the FP64 CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+always as the checker or the timed CPU baseline, never by chaste_amd.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'liboracle.so')
+
+LR91_HAND, LR91_CELLML, TT06_EPI = 0, 1, 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class MonoCfg(ctypes.Structure):
+    _fields_ = [('dim', ctypes.c_int), ('ex', ctypes.c_int), ('ey', ctypes.c_int), ('ez', ctypes.c_int),
+                ('h', ctypes.c_double), ('model', ctypes.c_int), ('rush_larsen', ctypes.c_int),
+                ('sigma', ctypes.c_double * 3), ('am', ctypes.c_double), ('cm', ctypes.c_double),
+                ('dt_pde', ctypes.c_double), ('dt_ode', ctypes.c_double), ('stim_mag', ctypes.c_double),
+                ('stim_dur', ctypes.c_double), ('stim_start', ctypes.c_double), ('stim_xlim', ctypes.c_double),
+                ('t_end', ctypes.c_double), ('ksp_rtol', ctypes.c_double), ('ksp_atol', ctypes.c_double),
+                ('max_its', ctypes.c_int), ('threads', ctypes.c_int), ('n_param', ctypes.c_int), ('params', _dp)]
+
+
+class MonoStats(ctypes.Structure):
+    _fields_ = [('t_ode', ctypes.c_double), ('t_rhs', ctypes.c_double), ('t_ksp', ctypes.c_double),
+                ('t_total', ctypes.c_double), ('cg_iterations', ctypes.c_longlong), ('steps', ctypes.c_int),
+                ('n_nodes', ctypes.c_int), ('nnz', ctypes.c_int)]
+
+
+_L = None
+
+
+def build():
+    subprocess.run(['make', '-s', '-C', HERE], check=True)
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_model_num_states.argtypes = [ctypes.c_int]
+        L.orc_cell_compute.restype = ctypes.c_int
+        L.orc_cell_compute.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
+        L.orc_cell_iionic.restype = ctypes.c_double
+        L.orc_cell_iionic.argtypes = [ctypes.c_int, _dp, _dp]
+        L.orc_slab_sizes.argtypes = [ctypes.c_int] * 4 + [_ip, _ip]
+        L.orc_slab_fill.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double, _dp, _ip]
+        L.orc_assemble.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int, _ip,
+                                   _ip, _ip, _dp, _dp]
+        L.orc_pcg.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_mono_run.argtypes = [ctypes.POINTER(MonoCfg), _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp,
+                                   ctypes.POINTER(MonoStats)]
+        _L = L
+    return _L
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def num_states(model):
+    return lib().orc_model_num_states(model)
+
+
+def cell_compute(model, y, t0, t_end, dt, stim=(0.0, 0.0, 0.0, -1.0), rush_larsen=False,
+                 steps_per_sample=1, params=None):
+    """AbstractCardiacCell::Compute: returns samples[k, S]; y updated in place."""
+    S = num_states(model)
+    nsteps = int(round((t_end - t0) / dt))
+    cap = nsteps // steps_per_sample + 2
+    out = np.zeros((cap, S))
+    pa = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    ns = lib().orc_cell_compute(model, int(rush_larsen), _p(pa), _p(y), t0, t_end, dt, stim[0], stim[1], stim[2],
+                                stim[3], steps_per_sample, _p(out), cap)
+    if ns < 0:
+        raise RuntimeError('orc_cell_compute failed')
+    return out[:ns]
+
+
+def cell_iionic(model, y, params=None):
+    pa = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    return lib().orc_cell_iionic(model, _p(pa), _p(np.ascontiguousarray(y, dtype=np.float64)))
+
+
+def slab(dim, ex, ey=0, ez=0, h=0.01):
+    n, ne = ctypes.c_int(), ctypes.c_int()
+    lib().orc_slab_sizes(dim, ex, ey, ez, ctypes.byref(n), ctypes.byref(ne))
+    coords = np.zeros((n.value, dim))
+    elems = np.zeros((ne.value, dim + 1), dtype=np.int32)
+    lib().orc_slab_fill(dim, ex, ey, ez, h, _p(coords), _p(elems, _ip))
+    return coords, elems
+
+
+def assemble(coords, elems, sigma, lumped=False):
+    """(row_ptr, col, M, K) with the reference's quadrature."""
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    n, dim = coords.shape
+    sig = np.ascontiguousarray(np.asarray(sigma, dtype=np.float64).reshape(dim, dim))
+    nnz = ctypes.c_int()
+    lib().orc_assemble(dim, n, _p(coords), elems.shape[0], _p(elems, _ip), _p(sig), int(lumped), ctypes.byref(nnz),
+                       None, None, None, None)
+    rp = np.zeros(n + 1, dtype=np.int32)
+    col = np.zeros(nnz.value, dtype=np.int32)
+    M = np.zeros(nnz.value)
+    K = np.zeros(nnz.value)
+    lib().orc_assemble(dim, n, _p(coords), elems.shape[0], _p(elems, _ip), _p(sig), int(lumped), ctypes.byref(nnz),
+                       _p(rp, _ip), _p(col, _ip), _p(M), _p(K))
+    return rp, col, M, K
+
+
+def pcg(rp, col, val, b, x0, rtol=1e-12, atol=0.0, max_its=10000):
+    x = np.array(x0, dtype=np.float64)
+    its = lib().orc_pcg(len(b), _p(rp, _ip), _p(col, _ip), _p(val), _p(np.ascontiguousarray(b)), _p(x), rtol, atol,
+                        max_its)
+    return x, its
+
+
+def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, sigma=(1.75, 1.75, 1.75),
+             am=1400.0, cm=1.0, dt_pde=0.01, dt_ode=0.01, stim_mag=-3e6, stim_dur=0.5, stim_start=0.0,
+             stim_xlim=1e-5, t_end=1.0, ksp_rtol=1e-12, ksp_atol=0.0, max_its=10000, threads=0,
+             params=None, trace_nodes=None, act_threshold=None, want_state=False):
+    """Full FP64 monodomain run (AbstractCardiacProblem::Solve equivalent)."""
+    n = [ex + 1, (ex + 1) * (ey + 1), (ex + 1) * (ey + 1) * (ez + 1)][dim - 1]
+    pa = None
+    npar = 0
+    if params is not None:
+        pa = np.ascontiguousarray(params, dtype=np.float64)
+        npar = pa.shape[0]
+    cfg = MonoCfg(dim, ex, ey, ez, h, model, int(rush_larsen), (ctypes.c_double * 3)(*sigma), am, cm, dt_pde, dt_ode,
+                  stim_mag, stim_dur, stim_start, stim_xlim, t_end, ksp_rtol, ksp_atol, max_its, threads, npar,
+                  _p(pa))
+    V = np.zeros(n)
+    S = num_states(model)
+    state = np.zeros((S, n)) if want_state else None
+    act = np.zeros(n) if act_threshold is not None else None
+    tn = None
+    trace = None
+    ntr = 0
+    if trace_nodes is not None:
+        tn = np.ascontiguousarray(trace_nodes, dtype=np.int32)
+        ntr = len(tn)
+        nsteps = int(round(t_end / dt_pde))
+        trace = np.zeros((nsteps + 1, ntr))
+    st = MonoStats()
+    rc = lib().orc_mono_run(ctypes.byref(cfg), _p(V), _p(state), _p(act),
+                            0.0 if act_threshold is None else act_threshold, ntr, _p(tn, _ip), _p(trace),
+                            ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError('orc_mono_run failed: %d' % rc)
+    out = {'V': V, 'stats': {k: getattr(st, k) for k, _ in MonoStats._fields_}}
+    if want_state:
+        out['state'] = state
+    if act is not None:
+        out['act'] = act
+    if trace is not None:
+        out['trace'] = trace[:st.steps]
+    return out

@@ -1,0 +1,49 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol that
+include/cmx.h declares, and its host-only helpers (slab mesh, model metadata)
+agree with the oracle.  No GPU compute is called here."""
+import os
+import re
+
+import numpy as np
+
+from conftest import ROOT
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, 'include', 'cmx.h')).read()
+    return sorted(set(re.findall(r'\b(cmx_[a-z_]+)\s*\(', src)))
+
+
+def test_exports_every_declared_symbol():
+    from chaste_amd import cmx
+    L = cmx.lib()
+    names = declared_symbols()
+    assert len(names) >= 14
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(cmx.SIGNATURES)
+
+
+def test_library_is_gfx950_code_object():
+    from chaste_amd.build import library_path
+    data = open(library_path(), 'rb').read()
+    assert b'gfx950' in data
+
+
+def test_slab_mesh_matches_oracle(oracle):
+    from chaste_amd import cmx
+    for args in [(1, 7, 0, 0), (2, 5, 4, 0), (3, 4, 3, 2)]:
+        c1, e1 = cmx.slab_mesh(*args, h=0.01)
+        c2, e2 = oracle.slab(*args, h=0.01)
+        assert np.array_equal(c1, c2)
+        assert np.array_equal(e1, e2)
+
+
+def test_model_metadata():
+    from chaste_amd import cmx
+    assert cmx.model_info(cmx.LR91) == (8, 0)
+    assert cmx.model_info(cmx.TT06_EPI)[0] == 19
+    names = cmx.state_names(cmx.LR91)
+    assert names[0] == 'membrane__V' and names[-1] == 'intracellular_calcium_concentration__Cai'
+    y = cmx.initial_state(cmx.LR91)
+    assert y[0] == -83.853 and y[7] == 0.0002

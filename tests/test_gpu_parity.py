@@ -1,0 +1,156 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 CPU oracle.
+
+FP64 mode: max |dV| / max |V| <= 1e-9 with both solvers at rtol 1e-12.
+Mixed mode (FP32 cells + FP32 CG, FP64 voltage/dots/residual): max |dV| <=
+0.5 mV and activation-time error <= 0.1 ms against the oracle's FP64 run.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+FP64_REL = 1e-9
+MIXED_MV = 0.5
+MIXED_ACT_MS = 0.1
+
+
+def make(dim, ex, ey, ez, model, integrator, stim_mag, stim_dur, stim_xlim, precision=0, sigma=(1.75, 1.75, 1.75),
+         rtol=1e-12, atol=0.0, fmt=0, refine=0, dt=0.01, dt_ode=None):
+    from chaste_amd import cmx
+    coords, elems = cmx.slab_mesh(dim, ex, ey, ez, 0.01)
+    stim_index = np.where(coords[:, 0] < stim_xlim, 0, -1).astype(np.int32)
+    return cmx.Monodomain(coords, elems, np.array(sigma[:dim]), model, integrator,
+                          stimuli=[(stim_mag, 0.0, stim_dur)], stim_index=stim_index, precision=precision,
+                          ksp_rtol=rtol, ksp_atol=atol, fmt=fmt, refine_sweeps=refine, dt_pde=dt, dt_ode=dt if dt_ode is None else dt_ode)
+
+
+def first_crossing(trace, dt, thr=0.0):
+    """First upward crossing of thr per column, linearly interpolated (ms)."""
+    act = np.full(trace.shape[1], -1.0)
+    prev = trace[0]
+    for k in range(1, trace.shape[0]):
+        cur = trace[k]
+        hit = (act < 0) & (prev < thr) & (cur >= thr)
+        act[hit] = ((k - 1) + (thr - prev[hit]) / (cur[hit] - prev[hit])) * dt
+        prev = cur
+    return act
+
+
+def test_cfg1_lr91_2d_fp64(oracle):
+    """BASELINE config 1 in full: 101x101 LR91 FE, 10 ms, stimulus x < 0.02."""
+    from chaste_amd import cmx
+    print('case cfg1 lr91 2d fp64', flush=True)
+    m = make(2, 100, 100, 0, cmx.LR91, cmx.FORWARD_EULER, -25000.0, 0.5, 0.02)
+    m.step(0.0, 1000)
+    V = m.voltage()
+    ref = oracle.mono_run(2, 100, 100, 0, model=oracle.LR91_HAND, stim_mag=-25000.0, stim_dur=0.5, stim_xlim=0.02,
+                          t_end=10.0)
+    err = np.abs(V - ref['V']).max() / np.abs(ref['V']).max()
+    print('cfg1 rel err', err, 'its/step', m.stats()['cg_iterations'] / 1000.0)
+    assert err <= FP64_REL
+
+
+@pytest.mark.parametrize('integrator', [0, 1], ids=['fe', 'rl'])
+def test_tt06_3d_fp64(oracle, integrator):
+    from chaste_amd import cmx
+    print('case tt06 3d fp64', integrator, flush=True)
+    # forward Euler on TT06 needs dt_ode <= 0.001 (TestIonicModels.hpp:811): 10 substeps per PDE step
+    dt_ode = 0.01 if integrator else 0.001
+    m = make(3, 20, 7, 4, cmx.TT06_EPI, integrator, -50000.0, 1.0, 0.03, sigma=(1.75, 0.19, 0.19), dt_ode=dt_ode)
+    m.step(0.0, 400)
+    ref = oracle.mono_run(3, 20, 7, 4, model=oracle.TT06_EPI, rush_larsen=bool(integrator), sigma=(1.75, 0.19, 0.19),
+                          stim_mag=-50000.0, stim_dur=1.0, stim_xlim=0.03, t_end=4.0, dt_ode=dt_ode, want_state=True)
+    V = m.voltage()
+    err = np.abs(V - ref['V']).max() / np.abs(ref['V']).max()
+    print('tt06 rel err', err)
+    assert err <= FP64_REL
+    S = m.state()
+    serr = np.abs(S - ref['state']).max(axis=1) / np.maximum(np.abs(ref['state']).max(axis=1), 1e-30)
+    assert serr.max() <= 1e-8
+
+
+def test_ell_matches_stencil():
+    from chaste_amd import cmx
+    print('case ell vs stencil', flush=True)
+    a = make(3, 12, 6, 5, cmx.TT06_EPI, 1, -50000.0, 1.0, 0.03, fmt=cmx.FMT_STENCIL)
+    b = make(3, 12, 6, 5, cmx.TT06_EPI, 1, -50000.0, 1.0, 0.03, fmt=cmx.FMT_ELL)
+    assert a.stats()['format'] == cmx.FMT_STENCIL and b.stats()['format'] == cmx.FMT_ELL
+    a.step(0.0, 200)
+    b.step(0.0, 200)
+    Va, Vb = a.voltage(), b.voltage()
+    assert np.abs(Va - Vb).max() / np.abs(Va).max() <= 1e-11
+
+
+def test_tt06_mixed_precision(oracle):
+    """FP32-bulk path vs the FP64 oracle: 0.5 mV / 0.1 ms."""
+    from chaste_amd import cmx
+    print('case tt06 mixed', flush=True)
+    dims = (3, 30, 6, 4)
+    m = make(*dims, cmx.TT06_EPI, 1, -50000.0, 1.0, 0.03, precision=cmx.MIXED, sigma=(1.75, 0.19, 0.19),
+             rtol=1e-8, refine=1)
+    n = m.n
+    steps = 800
+    trace = np.zeros((steps + 1, n))
+    trace[0] = m.voltage()
+    for k in range(steps):
+        m.step(k * 0.01, 1)
+        trace[k + 1] = m.voltage()
+    ref = oracle.mono_run(*dims, model=oracle.TT06_EPI, rush_larsen=True, sigma=(1.75, 0.19, 0.19),
+                          stim_mag=-50000.0, stim_dur=1.0, stim_xlim=0.03, t_end=8.0, ksp_rtol=1e-12,
+                          trace_nodes=np.arange(n))
+    rtrace = np.vstack([np.full(n, trace[0][0]), ref['trace']])
+    dv = np.abs(trace - rtrace)
+    print('mixed max dV', dv.max(), 'rms', np.sqrt((dv ** 2).mean()))
+    assert dv.max() <= MIXED_MV
+    a1, a2 = first_crossing(trace, 0.01), first_crossing(rtrace, 0.01)
+    assert np.array_equal(a1 < 0, a2 < 0)
+    act = a1 >= 0
+    assert act.sum() > 0
+    print('activation err', np.abs(a1[act] - a2[act]).max())
+    assert np.abs(a1[act] - a2[act]).max() <= MIXED_ACT_MS
+
+
+@pytest.mark.parametrize('case', json.load(open(os.path.join(GOLDEN, 'tissue_goldens.json')))['cases'],
+                         ids=lambda c: c['name'])
+def test_monodomain_problem_goldens_gpu(case):
+    """TestMonodomainProblem.hpp:784-880 goldens on the GPU path (KSP atol 2e-4 as the reference)."""
+    from chaste_amd import cmx
+    print('case golden', case['name'], flush=True)
+    m = make(case['dim'], case['ex'], case['ey'], case['ez'], cmx.LR91, 0, -3e6, 0.5, 1e-5, rtol=0.0, atol=2e-4)
+    m.step(0.0, int(round(case['t_end'] / 0.01)))
+    V = m.voltage()
+    assert np.abs(V[case['nodes']] - np.array(case['V'])).max() < 5e-3
+
+
+def test_state_roundtrip_and_iionic(oracle):
+    from chaste_amd import cmx
+    print('case state roundtrip', flush=True)
+    m = make(2, 10, 10, 0, cmx.LR91, 0, -25000.0, 0.5, 0.02)
+    S0 = m.state()
+    y0 = cmx.initial_state(cmx.LR91)
+    assert np.allclose(S0, y0[:, None])
+    ii = m.iionic()
+    assert np.allclose(ii, oracle.cell_iionic(1, y0), rtol=1e-12)
+    m.step(0.0, 10)
+    S = m.state()
+    m.set_state(S)
+    assert np.array_equal(m.state(), S)
+    ii2 = m.iionic()
+    assert np.all(np.isfinite(ii2))
+
+
+def test_divergence_reported():
+    """KSP_DIVERGED_ITS -> CMX_E_DIVERGED (linalg/src/LinearSystem.cpp:1118-1129)."""
+    from chaste_amd import cmx
+    print('case divergence', flush=True)
+    coords, elems = cmx.slab_mesh(2, 10, 10, 0, 0.01)
+    m = cmx.Monodomain(coords, elems, [1.75, 1.75], cmx.LR91, stimuli=[(-25000.0, 0.0, 0.5)],
+                       stim_index=np.zeros(len(coords), np.int32), ksp_rtol=1e-30, ksp_max_its=1)
+    with pytest.raises(cmx.CmxError) as e:
+        m.step(0.0, 1)
+    assert e.value.code == -4

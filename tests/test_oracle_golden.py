@@ -1,0 +1,191 @@
+"""Pins the FP64 CPU oracle against the reference's own golden data.
+
+Cells: heart/test/ionicmodels/TestIonicModels.hpp:406-500 (LR91, tolerance
+1e-3 mV via heart/src/fortests/RunAndCheckIonicModels.hpp CheckCellModelResults)
+and :793-835 (TT06, all 19 states compared here, GetIIonic -0.1843 and 0.0012).
+Tissue: heart/test/monodomain/TestMonodomainProblem.hpp:784-880 (1-D/2-D/3-D
+voltages at nodes 1, 3, 5, 7, atol 5e-3 mV).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_dat
+
+LR91_INIT = [-83.853, 0.00187018, 0.9804713, 0.98767124, 0.00316354, 0.99427859, 0.16647703, 0.0002]
+
+# TenTusscherValidData.dat column -> CellML state name (the file's column order
+# is the alphabetical component order of the older generator).
+TT06_COLUMNS = {
+    'd_gate_L()': 'L_type_Ca_current_d_gate__d', 'f2_gate_L()': 'L_type_Ca_current_f2_gate__f2',
+    'fca_gate_L()': 'L_type_Ca_current_fCass_gate__fCass', 'f_gate_L()': 'L_type_Ca_current_f_gate__f',
+    'CaSR(mM)': 'calcium_dynamics__Ca_SR', 'Cai(mM)': 'calcium_dynamics__Ca_i',
+    'CaSS(mM)': 'calcium_dynamics__Ca_ss', 'R_gate()': 'calcium_dynamics__R_prime',
+    'h_gate_Na()': 'fast_sodium_current_h_gate__h', 'j_gate_Na()': 'fast_sodium_current_j_gate__j',
+    'm_gate_Na()': 'fast_sodium_current_m_gate__m', 'V(mV)': 'membrane__V', 'Ki(mM)': 'potassium_dynamics__K_i',
+    'xr1_gate()': 'rapid_time_dependent_potassium_current_Xr1_gate__Xr1',
+    'xr2_gate()': 'rapid_time_dependent_potassium_current_Xr2_gate__Xr2',
+    'xS_gate()': 'slow_time_dependent_potassium_current_Xs_gate__Xs', 'Nai(mM)': 'sodium_dynamics__Na_i',
+    'r_gate_Ito()': 'transient_outward_current_r_gate__r', 's_gate_Ito()': 'transient_outward_current_s_gate__s',
+}
+
+
+def tt06_state_names():
+    import re
+    path = os.path.join(os.path.dirname(GOLDEN), '..', 'chaste_amd', 'csrc', 'models', 'tt06_epi.h')
+    src = open(path).read()
+    block = src.split('kStateNames[kNumStates] = {')[1].split('};')[0]
+    return re.findall(r'"([A-Za-z_0-9]+)"', block)
+
+
+def tt06_initial():
+    import re
+    path = os.path.join(os.path.dirname(GOLDEN), '..', 'chaste_amd', 'csrc', 'models', 'tt06_epi.h')
+    src = open(path).read()
+    block = src.split('kInitialState[kNumStates] = {')[1].split('};')[0]
+    return np.array([float(x) for x in re.findall(r'([-0-9.e+]+),', block)])
+
+
+@pytest.mark.parametrize('model', [0, 1])
+def test_lr91_delayed_stimulus(oracle, model):
+    _, ref = load_dat('Lr91DelayedStimValidData.dat')
+    y = np.array(LR91_INIT)
+    out = oracle.cell_compute(model, y, 0.0, 1000.0, 0.01, stim=(-25.5, 2.0, 50.0, -1.0), steps_per_sample=100)
+    assert out.shape[0] == ref.shape[0]
+    assert np.abs(out[:, 0] - ref[:, 1]).max() < 1e-3
+    # GetIIonic after a further 60 ms from that state (TestIonicModels.hpp:451-456)
+    oracle.cell_compute(model, y, 0.0, 60.0, 0.01, stim=(-25.5, 2.0, 50.0, -1.0), steps_per_sample=6000)
+    assert abs(oracle.cell_iionic(model, y) - 1.9411) < 1e-3
+
+
+def test_lr91_regular_stimulus(oracle):
+    hdr, ref = load_dat('Lr91RegularStimValidData.dat')
+    vcol = [i for i, h in enumerate(hdr) if h.startswith('V')][0]
+    y = np.array(LR91_INIT)
+    out = oracle.cell_compute(0, y, 0.0, 1000.0, 0.01, stim=(-25.5, 2.0, 50.0, 500.0), steps_per_sample=100)
+    assert np.abs(out[:, 0] - ref[:, vcol]).max() < 1e-3
+
+
+def test_lr91_cellml_matches_hand_restatement(oracle):
+    """The generated CellML LR91 and the hand restatement of
+    DynamicallyLoadableLr91.cpp agree to round-off over a full beat."""
+    a = np.array(LR91_INIT)
+    b = np.array(LR91_INIT)
+    sa = oracle.cell_compute(0, a, 0.0, 400.0, 0.01, stim=(-25.5, 2.0, 50.0, -1.0), steps_per_sample=10)
+    sb = oracle.cell_compute(1, b, 0.0, 400.0, 0.01, stim=(-25.5, 2.0, 50.0, -1.0), steps_per_sample=10)
+    assert np.abs(sa - sb).max() < 1e-8
+
+
+def test_tt06_all_states(oracle):
+    hdr, ref = load_dat('TenTusscherValidData.dat')
+    names = tt06_state_names()
+    y = tt06_initial()
+    assert abs(oracle.cell_iionic(2, y) - 0.0012) < 1e-3
+    out = oracle.cell_compute(2, y, 0.0, 40.0, 0.001, stim=(-38.0, 1.0, 5.0, -1.0), steps_per_sample=1000)
+    assert out.shape[0] == ref.shape[0] == 41
+    for j, h in enumerate(hdr[1:]):
+        i = names.index(TT06_COLUMNS[h])
+        # the file holds 5 significant figures
+        rel = np.abs(out[:, i] - ref[:, j + 1]) / np.maximum(np.abs(ref[:, j + 1]), 1e-30)
+        assert rel.max() < 6e-5, h
+    assert abs(oracle.cell_iionic(2, y) - (-0.1843)) < 1e-3
+
+
+def test_rush_larsen_first_order(oracle):
+    """RL error halves with dt (TestRushLarsen.hpp:179-196 checks ratio 2 +- 0.1)."""
+    y0 = tt06_initial()
+
+    def v_at(dt):
+        y = y0.copy()
+        oracle.cell_compute(2, y, 0.0, 20.0, dt, stim=(-38.0, 1.0, 5.0, -1.0), rush_larsen=True,
+                            steps_per_sample=int(round(20.0 / dt)))
+        return y[0]
+
+    fine = v_at(0.0005 / 4)
+    e1 = abs(v_at(0.004) - fine)
+    e2 = abs(v_at(0.002) - fine)
+    assert 1.8 < e1 / e2 < 2.2
+
+
+@pytest.mark.parametrize('case', json.load(open(os.path.join(GOLDEN, 'tissue_goldens.json')))['cases'],
+                         ids=lambda c: c['name'])
+def test_monodomain_problem_goldens(oracle, case):
+    out = oracle.mono_run(case['dim'], case['ex'], case['ey'], case['ez'], t_end=case['t_end'],
+                          ksp_rtol=0.0, ksp_atol=2e-4, stim_mag=-3e6, stim_dur=0.5, stim_xlim=1e-5)
+    got = out['V'][case['nodes']]
+    assert np.abs(got - np.array(case['V'])).max() < 5e-3
+
+
+def test_element_matrices_closed_form(oracle):
+    """Quadrature assembly equals the P1 closed forms on one element:
+    M = |T|(1 + delta)/((d+1)(d+2)), K = |T| G sigma G^T; rows of K sum to 0."""
+    rng = np.random.default_rng(3)
+    for dim in (1, 2, 3):
+        X = rng.normal(size=(dim + 1, dim))
+        el = np.arange(dim + 1, dtype=np.int32)[None, :]
+        sig = np.eye(dim) * 1.3 + 0.1 * np.ones((dim, dim))
+        rp, col, M, K = oracle.assemble(X, el, sig)
+        Md = np.zeros((dim + 1, dim + 1))
+        Kd = np.zeros((dim + 1, dim + 1))
+        for i in range(dim + 1):
+            for k in range(rp[i], rp[i + 1]):
+                Md[i, col[k]] = M[k]
+                Kd[i, col[k]] = K[k]
+        J = (X[1:] - X[0]).T
+        vol = abs(np.linalg.det(J)) / np.math.factorial(dim) if hasattr(np, 'math') else None
+        import math
+        vol = abs(np.linalg.det(J)) / math.factorial(dim)
+        Mref = vol * (np.ones((dim + 1, dim + 1)) + np.eye(dim + 1)) / ((dim + 1) * (dim + 2))
+        Gref = np.vstack([-np.ones(dim), np.eye(dim)])
+        G = Gref @ np.linalg.inv(J)
+        Kref = vol * G @ sig @ G.T
+        assert np.allclose(Md, Mref, rtol=1e-13, atol=1e-15)
+        assert np.allclose(Kd, Kref, rtol=1e-12, atol=1e-13)
+        assert np.abs(Kd.sum(axis=1)).max() < 1e-12 * np.abs(Kd).max()
+
+
+def test_stiffness_1d_pattern(oracle):
+    """1-D K = sigma/h [-1 2 -1] (TestMonodomainStiffnessMatrixAssembler.hpp)."""
+    coords, elems = oracle.slab(1, 10, h=0.1)
+    rp, col, M, K = oracle.assemble(coords, elems, [[2.0]])
+    i = 5
+    row = {col[k]: K[k] for k in range(rp[i], rp[i + 1])}
+    assert abs(row[4] + 20.0) < 1e-12 and abs(row[5] - 40.0) < 1e-12 and abs(row[6] + 20.0) < 1e-12
+    mrow = {col[k]: M[k] for k in range(rp[i], rp[i + 1])}
+    assert abs(mrow[5] - 2 * 0.1 / 3) < 1e-15 and abs(mrow[4] - 0.1 / 6) < 1e-15
+
+
+def test_pcg_against_dense_solve(oracle):
+    coords, elems = oracle.slab(2, 8, 8, h=0.01)
+    rp, col, M, K = oracle.assemble(coords, elems, np.eye(2) * 1.75)
+    A = 1400.0 / 0.01 * M + K
+    n = len(rp) - 1
+    dense = np.zeros((n, n))
+    for i in range(n):
+        dense[i, col[rp[i]:rp[i + 1]]] = A[rp[i]:rp[i + 1]]
+    rng = np.random.default_rng(0)
+    b = rng.normal(size=n)
+    x, its = oracle.pcg(rp, col, A, b, np.zeros(n), rtol=1e-14)
+    assert its > 0
+    assert np.allclose(x, np.linalg.solve(dense, b), rtol=1e-11, atol=1e-14)
+
+
+def test_slab_numbering(oracle):
+    """Node (k(H+1)+j)(W+1)+i at (i, j, k) h; 6 tets per cube; staggered 2-D diagonals."""
+    coords, elems = oracle.slab(3, 3, 2, 2, h=0.5)
+    W, H = 3, 2
+    for (i, j, k) in [(0, 0, 0), (3, 2, 2), (1, 2, 1)]:
+        assert np.allclose(coords[(k * (H + 1) + j) * (W + 1) + i], [i * 0.5, j * 0.5, k * 0.5])
+    assert elems.shape == (6 * 3 * 2 * 2, 4)
+    vols = []
+    for e in elems:
+        J = (coords[e[1:]] - coords[e[0]]).T
+        vols.append(abs(np.linalg.det(J)) / 6)
+    assert np.allclose(sum(vols), 1.5 * 1.0 * 1.0)
+    c2, e2 = oracle.slab(2, 2, 2, h=1.0)
+    # square (0,0): parity (0 + (2-0)) % 2 = 0 -> upper triangle uses sw
+    assert list(e2[0]) == [3, 4, 0] and list(e2[1]) == [1, 0, 4]
+    # square (1,0): parity 1 -> upper triangle uses sw+1
+    assert list(e2[2]) == [4, 5, 2] and list(e2[3]) == [2, 1, 4]
