@@ -41,14 +41,19 @@ def first_crossing(trace, dt, thr=0.0):
     return act
 
 
-def test_cfg1_lr91_2d_fp64(oracle):
-    """BASELINE config 1 in full: 101x101 LR91 FE, 10 ms, stimulus x < 0.02."""
+# The configured tissue stimuli (-25000 uA/cm^3 for 0.5 ms on x < 0.02 with
+# LR91, -50000 for 1 ms on x < 0.03 with TT06) stay below threshold; the
+# supra-threshold variants below (-400000 and -100000) launch a propagating
+# wave so the comparison covers upstrokes and activation times.
+@pytest.mark.parametrize('stim', [-25000.0, -400000.0], ids=['configured', 'suprathreshold'])
+def test_cfg1_lr91_2d_fp64(oracle, stim):
+    """BASELINE config 1 geometry in full: 101x101 LR91 FE, 10 ms, stimulus x < 0.02."""
     from chaste_amd import cmx
-    print('case cfg1 lr91 2d fp64', flush=True)
-    m = make(2, 100, 100, 0, cmx.LR91, cmx.FORWARD_EULER, -25000.0, 0.5, 0.02)
+    print('case cfg1 lr91 2d fp64', stim, flush=True)
+    m = make(2, 100, 100, 0, cmx.LR91, cmx.FORWARD_EULER, stim, 0.5, 0.02)
     m.step(0.0, 1000)
     V = m.voltage()
-    ref = oracle.mono_run(2, 100, 100, 0, model=oracle.LR91_HAND, stim_mag=-25000.0, stim_dur=0.5, stim_xlim=0.02,
+    ref = oracle.mono_run(2, 100, 100, 0, model=oracle.LR91_HAND, stim_mag=stim, stim_dur=0.5, stim_xlim=0.02,
                           t_end=10.0)
     err = np.abs(V - ref['V']).max() / np.abs(ref['V']).max()
     print('cfg1 rel err', err, 'its/step', m.stats()['cg_iterations'] / 1000.0)
@@ -61,10 +66,10 @@ def test_tt06_3d_fp64(oracle, integrator):
     print('case tt06 3d fp64', integrator, flush=True)
     # forward Euler on TT06 needs dt_ode <= 0.001 (TestIonicModels.hpp:811): 10 substeps per PDE step
     dt_ode = 0.01 if integrator else 0.001
-    m = make(3, 20, 7, 4, cmx.TT06_EPI, integrator, -50000.0, 1.0, 0.03, sigma=(1.75, 0.19, 0.19), dt_ode=dt_ode)
+    m = make(3, 20, 7, 4, cmx.TT06_EPI, integrator, -100000.0, 1.0, 0.03, sigma=(1.75, 0.19, 0.19), dt_ode=dt_ode)
     m.step(0.0, 400)
     ref = oracle.mono_run(3, 20, 7, 4, model=oracle.TT06_EPI, rush_larsen=bool(integrator), sigma=(1.75, 0.19, 0.19),
-                          stim_mag=-50000.0, stim_dur=1.0, stim_xlim=0.03, t_end=4.0, dt_ode=dt_ode, want_state=True)
+                          stim_mag=-100000.0, stim_dur=1.0, stim_xlim=0.03, t_end=4.0, dt_ode=dt_ode, want_state=True)
     V = m.voltage()
     err = np.abs(V - ref['V']).max() / np.abs(ref['V']).max()
     print('tt06 rel err', err)
@@ -91,7 +96,7 @@ def test_tt06_mixed_precision(oracle):
     from chaste_amd import cmx
     print('case tt06 mixed', flush=True)
     dims = (3, 30, 6, 4)
-    m = make(*dims, cmx.TT06_EPI, 1, -50000.0, 1.0, 0.03, precision=cmx.MIXED, sigma=(1.75, 0.19, 0.19),
+    m = make(*dims, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, precision=cmx.MIXED, sigma=(1.75, 0.19, 0.19),
              rtol=1e-8, refine=1)
     n = m.n
     steps = 800
@@ -101,7 +106,7 @@ def test_tt06_mixed_precision(oracle):
         m.step(k * 0.01, 1)
         trace[k + 1] = m.voltage()
     ref = oracle.mono_run(*dims, model=oracle.TT06_EPI, rush_larsen=True, sigma=(1.75, 0.19, 0.19),
-                          stim_mag=-50000.0, stim_dur=1.0, stim_xlim=0.03, t_end=8.0, ksp_rtol=1e-12,
+                          stim_mag=-100000.0, stim_dur=1.0, stim_xlim=0.03, t_end=8.0, ksp_rtol=1e-12,
                           trace_nodes=np.arange(n))
     rtrace = np.vstack([np.full(n, trace[0][0]), ref['trace']])
     dv = np.abs(trace - rtrace)
