@@ -14,6 +14,7 @@ FP64, MIXED = 0, 1
 LR91, TT06_EPI = 0, 1
 FORWARD_EULER, RUSH_LARSEN = 0, 1
 FMT_AUTO, FMT_STENCIL, FMT_ELL = 0, 1, 2
+KERNEL_KINDS = ['cell', 'residual', 'spmv', 'update', 'pupdate', 'finish']
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -66,6 +67,8 @@ SIGNATURES = {
     'cmx_set_state': (ctypes.c_int, [ctypes.c_void_p, _dp]),
     'cmx_get_iionic': (ctypes.c_int, [ctypes.c_void_p, _dp]),
     'cmx_get_stats': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CmxStats)]),
+    'cmx_set_profiling': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    'cmx_get_kernel_times': (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
     'cmx_model_info': (ctypes.c_int, [ctypes.c_int32, _ip, _ip]),
     'cmx_model_state_name': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p)]),
     'cmx_model_initial_state': (ctypes.c_int, [ctypes.c_int32, _dp]),
@@ -229,6 +232,19 @@ class Monodomain:
         out = np.zeros(self.n)
         _check(lib().cmx_get_iionic(self.ctx, _p(out)), self.ctx)
         return out
+
+    def set_profiling(self, kinds):
+        mask = 0
+        for k in kinds:
+            mask |= 1 << KERNEL_KINDS.index(k)
+        _check(lib().cmx_set_profiling(self.ctx, mask), self.ctx)
+
+    def kernel_times(self, reset=True):
+        ms = np.zeros(len(KERNEL_KINDS))
+        cnt = np.zeros(len(KERNEL_KINDS), dtype=np.int64)
+        _check(lib().cmx_get_kernel_times(self.ctx, _p(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                          1 if reset else 0), self.ctx)
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_KINDS)}
 
     def stats(self):
         s = CmxStats()

@@ -375,6 +375,13 @@ struct cmx_ctx {
     double last_residual = 0;
     double device_bytes = 0;
     std::vector<void*> allocs;
+    // kernel timing (cmx_set_profiling)
+    int prof_mask = 0;
+    struct Pending { int kind; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    double k_ms[CMX_K_COUNT] = {};
+    long long k_launches[CMX_K_COUNT] = {};
 };
 
 namespace {
@@ -431,6 +438,54 @@ const double* model_pdef(int model) { return model == CMX_MODEL_LR91 ? lr91::kPa
 
 int grid_of(int n) { return (n + kBlock - 1) / kBlock; }
 
+hipEvent_t take_event(cmx_ctx* c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// Brackets one launch of the given kind with events when profiling it.
+struct KernelTimer {
+    cmx_ctx* c;
+    int kind;
+    hipEvent_t a = nullptr;
+    KernelTimer(cmx_ctx* ctx, int k) : c(ctx), kind(k)
+    {
+        if (c->prof_mask & (1 << kind)) {
+            a = take_event(c);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~KernelTimer()
+    {
+        if (!a) return;
+        hipEvent_t b = take_event(c);
+        (void)hipEventRecord(b, c->stream);
+        c->pending.push_back({kind, a, b});
+    }
+};
+
+// Folds completed event pairs into the totals (call after a stream sync).
+void collect_times(cmx_ctx* c)
+{
+    for (auto& p : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            c->k_ms[p.kind] += ms;
+            c->k_launches[p.kind] += 1;
+        }
+        c->event_pool.push_back(p.a);
+        c->event_pool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
 // ----- typed launches -------------------------------------------------------
 template <class RS, class RV>
 int launch_cells(cmx_ctx* c, double t, double t_next)
@@ -453,6 +508,7 @@ int launch_cells(cmx_ctx* c, double t, double t_next)
     a.F = (RV*)c->F;
     const int G = grid_of(c->n);
     const bool rl = c->integrator == CMX_RUSH_LARSEN;
+    KernelTimer timer(c, CMX_K_CELL);
     if (c->model == CMX_MODEL_LR91) {
         if (rl) cell_step_kernel<ModelLr91, RS, RV, true><<<G, kBlock, 0, c->stream>>>(a);
         else cell_step_kernel<ModelLr91, RS, RV, false><<<G, kBlock, 0, c->stream>>>(a);
@@ -493,6 +549,7 @@ int launch_residual(cmx_ctx* c, const Rows& R, bool first, bool use_dacc)
     a.d = (RV*)c->d;
     a.partials = c->partials;
     a.sc = c->sc;
+    KernelTimer timer(c, CMX_K_RESIDUAL);
     residual_kernel<Rows, RV><<<grid_of(c->n), kBlock, 0, c->stream>>>(R, mat_vals(c), a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
@@ -504,10 +561,19 @@ int launch_cg_iters(cmx_ctx* c, const Rows& R, int iters)
     const int G = grid_of(c->n);
     const MatVals mv = mat_vals(c);
     for (int k = 0; k < iters; ++k) {
-        cg_spmv_kernel<Rows, RV><<<G, kBlock, 0, c->stream>>>(R, mv, c->n, (const RV*)c->p, (RV*)c->q, c->partials, c->sc);
-        cg_update_kernel<Rows, RV><<<G, kBlock, 0, c->stream>>>(R, mv, c->n, (RV*)c->d, (RV*)c->r, (const RV*)c->p,
-                                                               (const RV*)c->q, c->partials, c->sc);
-        cg_pupdate_kernel<Rows, RV><<<G, kBlock, 0, c->stream>>>(R, mv, c->n, (const RV*)c->r, (RV*)c->p, c->sc);
+        {
+            KernelTimer timer(c, CMX_K_SPMV);
+            cg_spmv_kernel<Rows, RV><<<G, kBlock, 0, c->stream>>>(R, mv, c->n, (const RV*)c->p, (RV*)c->q, c->partials, c->sc);
+        }
+        {
+            KernelTimer timer(c, CMX_K_UPDATE);
+            cg_update_kernel<Rows, RV><<<G, kBlock, 0, c->stream>>>(R, mv, c->n, (RV*)c->d, (RV*)c->r, (const RV*)c->p,
+                                                                   (const RV*)c->q, c->partials, c->sc);
+        }
+        {
+            KernelTimer timer(c, CMX_K_PUPDATE);
+            cg_pupdate_kernel<Rows, RV><<<G, kBlock, 0, c->stream>>>(R, mv, c->n, (const RV*)c->r, (RV*)c->p, c->sc);
+        }
     }
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
@@ -517,6 +583,7 @@ int read_scalars(cmx_ctx* c)
 {
     CMX_HIP(c, hipMemcpyAsync(c->h_sc, c->sc, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
     CMX_HIP(c, hipStreamSynchronize(c->stream));
+    if (!c->pending.empty()) collect_times(c);
     return CMX_OK;
 }
 
@@ -558,6 +625,7 @@ int step_once(cmx_ctx* c, const Rows& R, double t, double t_next)
     if (rc) return rc;
     int its = c->h_sc->it;
     if (sweeps == 0) {
+        KernelTimer timer(c, CMX_K_FINISH);
         accumulate_kernel<RV><<<G, kBlock, 0, c->stream>>>(c->n, c->V, (const RV*)c->d, c->sc);
     } else {
         // FP64 refinement: true residual of V^n + Dacc against the step's b.
@@ -802,6 +870,9 @@ void release(cmx_ctx* c)
 {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    collect_times(c);
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    c->event_pool.clear();
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     if (c->h_sc) (void)hipHostFree(c->h_sc);
@@ -951,6 +1022,26 @@ int cmx_get_stats(cmx_ctx* c, cmx_stats* s)
     s->nnz = c->nnz;
     s->last_residual = c->last_residual;
     s->device_bytes = c->device_bytes;
+    return CMX_OK;
+}
+
+int cmx_set_profiling(cmx_ctx* c, int32_t mask)
+{
+    if (!c) return fail(nullptr, CMX_E_ARG, "null context");
+    c->prof_mask = mask;
+    return CMX_OK;
+}
+
+int cmx_get_kernel_times(cmx_ctx* c, double* total_ms, int64_t* launches, int32_t reset)
+{
+    if (!c) return fail(nullptr, CMX_E_ARG, "null context");
+    CMX_HIP(c, hipStreamSynchronize(c->stream));
+    collect_times(c);
+    for (int k = 0; k < CMX_K_COUNT; ++k) {
+        if (total_ms) total_ms[k] = c->k_ms[k];
+        if (launches) launches[k] = c->k_launches[k];
+        if (reset) { c->k_ms[k] = 0; c->k_launches[k] = 0; }
+    }
     return CMX_OK;
 }
 

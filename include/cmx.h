@@ -135,6 +135,16 @@ int cmx_set_state(cmx_ctx* ctx, const double* states);
 int cmx_get_iionic(cmx_ctx* ctx, double* iion);
 int cmx_get_stats(cmx_ctx* ctx, cmx_stats* stats);
 
+/* Per-kernel timing with HIP events recorded on the context's stream around
+ * every launch of the kinds selected by mask (bit k = kind k), in the spirit
+ * of the reference's HeartEventHandler events Ode / AssRhs / Ksp
+ * (global/src/timing/HeartEventHandler.hpp).  Times accumulate until reset. */
+enum { CMX_K_CELL = 0, CMX_K_RESIDUAL = 1, CMX_K_SPMV = 2, CMX_K_UPDATE = 3, CMX_K_PUPDATE = 4,
+       CMX_K_FINISH = 5, CMX_K_COUNT = 6 };
+int cmx_set_profiling(cmx_ctx* ctx, int32_t mask);
+/* total_ms[CMX_K_COUNT], launches[CMX_K_COUNT]; reset != 0 clears them afterwards. */
+int cmx_get_kernel_times(cmx_ctx* ctx, double* total_ms, int64_t* launches, int32_t reset);
+
 /* Model metadata (CellML order, V first). */
 int cmx_model_info(int32_t model, int32_t* n_states, int32_t* n_params);
 int cmx_model_state_name(int32_t model, int32_t index, const char** name);
