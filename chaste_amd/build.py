@@ -16,6 +16,13 @@ HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
 SOURCES = ['cmx_device.hip', 'cmx_host.cpp']
 
 
+CPP_TESTS = ['TestMonodomainProblem']
+
+
+def cpp_test_path(name):
+    return os.path.join(ROOT, 'tests', 'cpp', 'bin', name)
+
+
 def library_path():
     return os.path.join(LIBDIR, 'libcmx.so')
 
@@ -38,3 +45,13 @@ def build_library(verbose=False):
     if verbose:
         print(log)
     return out
+
+
+def build_cpp_tests():
+    """C++ programs written against include/chaste_amd/Monodomain.hpp (host
+    code only, linked to libcmx.so)."""
+    os.makedirs(os.path.join(ROOT, 'tests', 'cpp', 'bin'), exist_ok=True)
+    for name in CPP_TESTS:
+        src = os.path.join(ROOT, 'tests', 'cpp', name + '.cpp')
+        _run(['g++', '-O2', '-std=c++17', '-Wall', '-I' + os.path.join(ROOT, 'include'), src,
+              '-L' + LIBDIR, '-lcmx', '-Wl,-rpath,$ORIGIN/../../../chaste_amd/lib', '-o', cpp_test_path(name)])

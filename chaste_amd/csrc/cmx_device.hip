@@ -1,23 +1,34 @@
 // MI355X (gfx950) monodomain time-step: device kernels and the C ABI.
 //
-// Per PDE step (AbstractDynamicLinearPdeSolver::Solve body,
-// pde/src/solver/AbstractDynamicLinearPdeSolver.hpp:372-469):
-//   1. cell_step   SolveCellSystems(V, t, t+dt) with V clamped, fused with the
-//                  ICI forcing F_i = -Am Iion_i - Istim_i(t+dt)
-//                  (heart/src/tissue/AbstractCardiacTissue.cpp:464-734,
-//                   heart/src/solver/electrics/monodomain/MonodomainSolver.cpp:92-101)
-//   2. residual    r = b - A V with b = M (Am Cm V/dt + F), A = (Am Cm/dt) M + K,
-//                  evaluated as r = M F - K V (the Am Cm V/dt terms cancel
-//                  exactly), with ||D^-1 b|| for the stopping test
-//                  (MonodomainSolver.cpp:107 MatMult + LinearSystem::Solve)
-//   3. CG          Jacobi-preconditioned CG on A d = r from d = 0 (equivalent
-//                  to the reference's warm start from V^n, KSPSetInitialGuessNonzero,
-//                  linalg/src/LinearSystem.cpp:833-837), stop when
-//                  ||D^-1 r|| <= max(rtol ||D^-1 b||, atol)
-//   4. finish      V^{n+1} = V^n + d in FP64.
-// In CMX_MIXED the cell states, F and the CG vectors are FP32; V, the
-// residual of step 2 (and every refinement sweep), all dot products and the
-// voltage update stay FP64.
+// One PDE step (AbstractDynamicLinearPdeSolver::Solve body,
+// pde/src/solver/AbstractDynamicLinearPdeSolver.hpp:372-469) is the launch
+// sequence
+//   cell_step    SolveCellSystems(V, t, t+dt) with V clamped, fused with the
+//                ICI forcing F_i = -Am Iion_i - Istim_i(t+dt)
+//                (heart/src/tissue/AbstractCardiacTissue.cpp:464-734,
+//                 heart/src/solver/electrics/monodomain/MonodomainSolver.cpp:92-101)
+//   residual     r0 = b - A V^n with b = M (Am Cm V^n/dt + F) and
+//                A = (Am Cm/dt) M + K, evaluated as r0 = M F - K V^n (the
+//                Am Cm V/dt terms cancel exactly), plus ||D^-1 b|| for the test
+//                (MonodomainSolver.cpp:107 MatMult, LinearSystem::Solve)
+//   cg_init      w0 = A D^-1 r0 and the first dot products
+//   cg_iter x B  Jacobi-preconditioned CG on A d = r0 from d = 0, which is the
+//                reference's warm start from V^n (KSPSetInitialGuessNonzero,
+//                linalg/src/LinearSystem.cpp:833-837).  Chronopoulos-Gear
+//                form: one kernel and one reduction per iteration; stops when
+//                ||D^-1 r|| <= max(rtol ||D^-1 b||, atol) (PETSc's default
+//                preconditioned-norm test); iterations past convergence are
+//                no-ops
+//   finish       V^{n+1} = V^n + d in FP64, advance the device step clock.
+// CMX_MIXED keeps cell states, F and the CG vectors in FP32 and adds an FP64
+// refinement sweep (true residual of V^n + d, second CG solve); V, residuals,
+// dot products and the voltage update are FP64 throughout.
+//
+// Steps are enqueued in batches without host synchronisation.  If a step's
+// final solve is still running when its iteration budget B is used up, the
+// finish kernel poisons the clock, every later kernel returns at once, and
+// the host resumes that solve where it stopped (all its buffers are intact),
+// then re-enqueues the rest of the batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,8 +79,9 @@ template <class RS, class RV>
 struct CellArgs {
     int n;
     int n_sub;                 // ODE substeps per PDE step
-    double t, t_next, dt_ode;  // PDE interval and ODE step (ms)
+    double dt_ode;             // ms
     double am, cm;
+    const StepClock* clock;
     const double* __restrict__ V;
     RS* __restrict__ state;    // [NS-1][n]: plane s-1 holds state s (V excluded)
     const RS* __restrict__ params;  // [NP][n] or null
@@ -80,16 +92,18 @@ struct CellArgs {
     RV* __restrict__ F;        // out: -Am Iion - Istim(t_next)
 };
 
-// One thread per node.  The state lives in registers for all substeps.
+// One thread per node; the state lives in registers for all substeps.
 template <class Model, class RS, class RV, bool RL>
 __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
 {
+    const StepClock clk = *a.clock;
+    if (clk.poison) return;
     const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= a.n) return;
+    const double t = step_time(clk, clk.k), t_next = step_next_time(clk, clk.k);
     constexpr int NS = Model::NS;
     constexpr int NP = Model::NP > 0 ? Model::NP : 1;
-    const double Vd = a.V[i];
-    const RS V = (RS)Vd;
+    const RS V = (RS)a.V[i];
     RS y[NS];
     y[0] = V;
 #pragma unroll
@@ -105,8 +119,8 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
     // interval on the last substep, Rush-Larsen always dt_ode
     // (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:106-128).
     for (int k = 0; k < a.n_sub; ++k) {
-        const double ts = a.t + k * a.dt_ode;
-        const double h = (!RL && k + 1 == a.n_sub) ? a.t_next - ts : a.dt_ode;
+        const double ts = t + k * a.dt_ode;
+        const double h = (!RL && k + 1 == a.n_sub) ? t_next - ts : a.dt_ode;
         const RS st = sid ? (RS)(stimulus_at(a.stim[sid - 1], ts) / a.am) : RS(0);
         RS dy[NS], gi[NS], gk[NS];
         Model::template eval<RS, RL>(y, V, st, p, cm, dy, gi, gk);
@@ -118,14 +132,13 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         }
     }
     const RS iion = Model::template iion<RS>(y, V, p, cm);
-    const double istim = sid ? stimulus_at(a.stim[sid - 1], a.t_next) : 0.0;
+    const double istim = sid ? stimulus_at(a.stim[sid - 1], t_next) : 0.0;
     a.F[i] = (RV)(-a.am * (double)iion - istim);
 #pragma unroll
     for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.n + i] = y[s];
 }
 
-// Iion from stored state (cmx_get_iionic after set_state; also used for
-// the cache when no step has run).
+// Iion at the stored state (cmx_get_iionic before any step).
 template <class Model, class RS>
 __global__ __launch_bounds__(kBlock) void iionic_kernel(int n, const double* __restrict__ V, const RS* __restrict__ state,
                                                         const RS* __restrict__ params, const RS* __restrict__ pdef,
@@ -158,32 +171,59 @@ struct MatVals {
     const float* __restrict__ dinv32;
 };
 
+template <class RV> __device__ __forceinline__ const RV* a_vals(const MatVals& m);
+template <> __device__ __forceinline__ const double* a_vals<double>(const MatVals& m) { return m.a64; }
+template <> __device__ __forceinline__ const float* a_vals<float>(const MatVals& m) { return m.a32; }
+template <class RV> __device__ __forceinline__ const RV* dinv_vals(const MatVals& m);
+template <> __device__ __forceinline__ const double* dinv_vals<double>(const MatVals& m) { return m.dinv64; }
+template <> __device__ __forceinline__ const float* dinv_vals<float>(const MatVals& m) { return m.dinv32; }
+
+// Rows of one block: a contiguous chunk [lo, hi) walked with stride kBlock.
+struct RowRange {
+    int lo, hi;
+    __device__ __forceinline__ RowRange(int n, int blk, int G)
+    {
+        const int per = ((n + G - 1) / G + kBlock - 1) / kBlock * kBlock;
+        lo = blk * per;
+        hi = min(n, lo + per);
+    }
+};
+
+// CG vectors (RV precision).  r, w, s are gathered by the stencil and
+// therefore double-buffered by iteration parity; p and x are row-private.
+template <class RV>
+struct CgVecs {
+    RV* r[2];
+    RV* w[2];
+    RV* s[2];
+    RV* p;
+    RV* x;
+};
+
 template <class RV>
 struct ResidualArgs {
     int n;
     double c;                          // Am Cm / dt
-    double rtol2, atol2;
-    int first;                         // first sweep of the step: set tolerance
+    const StepClock* clock;
     const double* __restrict__ V;
-    const double* __restrict__ Dacc;   // accumulated correction or null
+    const double* __restrict__ Dacc;   // accumulated correction (refinement) or null
     const RV* __restrict__ F;
-    RV* __restrict__ r;
-    RV* __restrict__ p;
-    RV* __restrict__ d;
-    double* __restrict__ partials;
-    CgScalars* sc;
+    RV* __restrict__ r0;
+    double* __restrict__ partials;     // [G]: ||D^-1 b||^2
+    SolveCtl* ctl;
 };
 
-// r = M F - K (V + D) - c M D ; b = c M V + M F.  The stiffness term uses
-// differences (V_j - V_i): K's rows sum to zero, and the differences keep the
+// r0 = M F - K (V + D) - c M D ; b = c M V + M F.  The stiffness term uses
+// differences (x_j - x_i): K's rows sum to zero, and the differences keep the
 // diffusion flux free of cancellation against the ~85 mV resting level.
 template <class Rows, class RV>
 __global__ __launch_bounds__(kBlock) void residual_kernel(Rows R, MatVals mv, ResidualArgs<RV> a)
 {
+    if (a.clock->poison) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int i = blk * kBlock + threadIdx.x;
-    double part[3] = {0.0, 0.0, 0.0};
-    if (i < a.n) {
+    const RowRange rr(a.n, blk, gridDim.x);
+    double part[1] = {0.0};
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kBlock) {
         const int b = R.base(i);
         const double xi = a.V[i] + (a.Dacc ? a.Dacc[i] : 0.0);
         double mf = 0, mvv = 0, kx = 0, md = 0;
@@ -198,134 +238,200 @@ __global__ __launch_bounds__(kBlock) void residual_kernel(Rows R, MatVals mv, Re
             md += m * dj;
             kx += kk * ((vj + dj) - xi);
         }
-        const double bi = a.c * mvv + mf;
-        const double ri = mf - kx - a.c * md;
         const double di = mv.dinv64[R.dslot(i, b)];
-        const RV rr = (RV)ri;
-        const RV zz = (RV)(di * (double)rr);
-        a.r[i] = rr;
-        a.p[i] = zz;
-        a.d[i] = RV(0);
-        part[0] = (di * bi) * (di * bi);
-        part[1] = (double)rr * (double)zz;
-        part[2] = (double)zz * (double)zz;
+        const double bi = di * (a.c * mvv + mf);
+        a.r0[i] = (RV)(mf - kx - a.c * md);
+        part[0] += bi * bi;
     }
-    double tot[3];
-    if (reduce_last_block<3>(part, a.partials, &a.sc->ticket, blk, gridDim.x, tot) && threadIdx.x == 0) {
-        CgScalars* sc = a.sc;
-        if (a.first) {
-            sc->bb = tot[0];
-            sc->tol2 = fmax(a.rtol2 * tot[0], a.atol2);
-        }
-        sc->rz = tot[1];
-        sc->zz = tot[2];
-        sc->it = 0;
-        sc->done = (tot[2] <= sc->tol2) ? 1 : 0;
-        if (!(tot[0] == tot[0]) || !(tot[2] == tot[2])) { sc->nonfinite = 1; sc->done = 2; }
+    write_partials<1>(part, a.partials, blk);
+    if (blk == 0 && threadIdx.x == 0) {
+        a.ctl->done = 0;
+        a.ctl->it = 0;
     }
 }
 
-template <class RV> __device__ __forceinline__ const RV* a_vals(const MatVals& m);
-template <> __device__ __forceinline__ const double* a_vals<double>(const MatVals& m) { return m.a64; }
-template <> __device__ __forceinline__ const float* a_vals<float>(const MatVals& m) { return m.a32; }
-template <class RV> __device__ __forceinline__ const RV* dinv_vals(const MatVals& m);
-template <> __device__ __forceinline__ const double* dinv_vals<double>(const MatVals& m) { return m.dinv64; }
-template <> __device__ __forceinline__ const float* dinv_vals<float>(const MatVals& m) { return m.dinv32; }
-
-// q = A p and (p, q); the last block forms alpha = (r, z) / (p, q).
-template <class Rows, class RV>
-__global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Rows R, MatVals mv, int n, const RV* __restrict__ p,
-                                                         RV* __restrict__ q, double* __restrict__ partials, CgScalars* sc)
-{
-    if (sc->done) return;
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int i = blk * kBlock + threadIdx.x;
-    double part[1] = {0.0};
-    if (i < n) {
-        const RV* __restrict__ A = a_vals<RV>(mv);
-        const int b = R.base(i);
-        RV s = 0;
-        for (int k = 0; k < R.width; ++k) s += A[R.slot(i, b, k)] * p[R.col(i, b, k)];
-        q[i] = s;
-        part[0] = (double)p[i] * (double)s;
-    }
-    double tot[1];
-    if (reduce_last_block<1>(part, partials, &sc->ticket, blk, gridDim.x, tot) && threadIdx.x == 0)
-        sc->alpha = sc->rz / tot[0];
-}
-
-// d += alpha p, r -= alpha q, z = D^-1 r; (r, z) and (z, z); the last block
-// tests convergence and forms beta.
-template <class Rows, class RV>
-__global__ __launch_bounds__(kBlock) void cg_update_kernel(Rows R, MatVals mv, int n, RV* __restrict__ d, RV* __restrict__ r,
-                                                           const RV* __restrict__ p, const RV* __restrict__ q,
-                                                           double* __restrict__ partials, CgScalars* sc)
-{
-    if (sc->done) return;
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int i = blk * kBlock + threadIdx.x;
-    double part[2] = {0.0, 0.0};
-    if (i < n) {
-        const RV alpha = (RV)sc->alpha;
-        const RV di = dinv_vals<RV>(mv)[R.dslot(i, R.base(i))];
-        d[i] += alpha * p[i];
-        const RV ri = r[i] - alpha * q[i];
-        r[i] = ri;
-        const RV zi = di * ri;
-        part[0] = (double)ri * (double)zi;
-        part[1] = (double)zi * (double)zi;
-    }
-    double tot[2];
-    if (reduce_last_block<2>(part, partials, &sc->ticket, blk, gridDim.x, tot) && threadIdx.x == 0) {
-        sc->it += 1;
-        sc->beta = tot[0] / sc->rz;
-        sc->rz = tot[0];
-        sc->zz = tot[1];
-        if (!(tot[1] == tot[1])) { sc->nonfinite = 1; sc->done = 2; }
-        else if (tot[1] <= sc->tol2) sc->done = 1;
-        else if (sc->it >= sc->max_its) sc->done = 2;
-    }
-}
-
-// p = D^-1 r + beta p
-template <class Rows, class RV>
-__global__ __launch_bounds__(kBlock) void cg_pupdate_kernel(Rows R, MatVals mv, int n, const RV* __restrict__ r,
-                                                            RV* __restrict__ p, const CgScalars* sc)
-{
-    if (sc->done) return;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const RV di = dinv_vals<RV>(mv)[R.dslot(i, R.base(i))];
-    p[i] = di * r[i] + (RV)sc->beta * p[i];
-}
-
-// V += d (no refinement) or Dacc += d.
 template <class RV>
-__global__ __launch_bounds__(kBlock) void accumulate_kernel(int n, double* __restrict__ X, const RV* __restrict__ d,
-                                                            CgScalars* sc)
+struct CgArgs {
+    int n;
+    int first;                         // first solve of the step: set the tolerance
+    double rtol2, atol2;
+    const StepClock* clock;
+    SolveCtl* ctl;
+    double2* hist;                     // [max_its + 1]: (gamma_i, alpha_i)
+    const double* __restrict__ pin;    // partials read
+    double* __restrict__ pout;         // partials written
+    CgVecs<RV> v;
+};
+
+// z0 = D^-1 r0 (gathered), w0 = A z0, p = s_{-1} = x = 0; (r0,z0), (w0,z0), (z0,z0).
+template <class Rows, class RV>
+__global__ __launch_bounds__(kBlock) void cg_init_kernel(Rows R, MatVals mv, CgArgs<RV> a)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const double x = X[i] + (double)d[i];
-    X[i] = x;
-    if (!isfinite(x)) sc->nonfinite = 1;
+    if (a.clock->poison) return;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    if (a.first) {
+        double bb[1];
+        read_partials<1>(a.pin, gridDim.x, bb);
+        if (blk == 0 && threadIdx.x == 0) {
+            a.ctl->bb = bb[0];
+            a.ctl->tol2 = fmax(a.rtol2 * bb[0], a.atol2);
+        }
+    }
+    const RV* __restrict__ A = a_vals<RV>(mv);
+    const RV* __restrict__ D = dinv_vals<RV>(mv);
+    const RV* __restrict__ r0 = a.v.r[0];
+    const RowRange rr(a.n, blk, gridDim.x);
+    double part[3] = {0.0, 0.0, 0.0};
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kBlock) {
+        const int b = R.base(i);
+        RV w = 0;
+        for (int k = 0; k < R.width; ++k) {
+            const int j = R.col(i, b, k);
+            w += A[R.slot(i, b, k)] * (D[R.dslot(j, R.base(j))] * r0[j]);
+        }
+        const RV ri = r0[i];
+        const RV zi = D[R.dslot(i, b)] * ri;
+        a.v.w[0][i] = w;
+        a.v.s[1][i] = RV(0);
+        a.v.p[i] = RV(0);
+        a.v.x[i] = RV(0);
+        part[0] += (double)ri * (double)zi;
+        part[1] += (double)w * (double)zi;
+        part[2] += (double)zi * (double)zi;
+    }
+    write_partials<3>(part, a.pout, blk);
 }
 
-__global__ __launch_bounds__(kBlock) void add_kernel(int n, double* __restrict__ V, double* __restrict__ D, CgScalars* sc)
+// z_{i+1} at a row from iteration-i data: D^-1 (r_i - alpha (w_i + beta s_{i-1})).
+template <class RV>
+__device__ __forceinline__ RV next_z(RV r, RV w, RV s_old, RV dinv, RV alpha, RV beta)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const double x = V[i] + D[i];
-    V[i] = x;
-    D[i] = 0.0;
-    if (!isfinite(x)) sc->nonfinite = 1;
+    const RV s = w + beta * s_old;
+    return dinv * (r - alpha * s);
 }
 
-template <class T>
-__global__ void fill_kernel(T* p, T v, size_t n)
+// Iteration i (Chronopoulos & Gear, J. Comput. Appl. Math. 25:153, 1989):
+//   p_i = z_i + beta_i p_{i-1};  s_i = w_i + beta_i s_{i-1}   (s_i = A p_i)
+//   x += alpha_i p_i;  r_{i+1} = r_i - alpha_i s_i;  z_{i+1} = D^-1 r_{i+1}
+//   w_{i+1} = A z_{i+1}  (z_{i+1} of each neighbour recomputed from its
+//   iteration-i values, so one kernel and one reduction per iteration)
+//   gamma = (r,z), delta = (w,z), zeta = (z,z) of iteration i+1
+//   beta_{i+1} = gamma_{i+1}/gamma_i,
+//   alpha_{i+1} = gamma_{i+1} / (delta_{i+1} - beta_{i+1} gamma_{i+1}/alpha_i).
+template <class Rows, class RV>
+__global__ __launch_bounds__(kBlock) void cg_iter_kernel(Rows R, MatVals mv, CgArgs<RV> a, int it)
 {
-    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) p[i] = v;
+    if (a.clock->poison) return;
+    SolveCtl* ctl = a.ctl;
+    if (ctl->done) return;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    double g[3];
+    read_partials<3>(a.pin, gridDim.x, g);
+    const double gamma = g[0], delta = g[1], zeta = g[2];
+    const bool lead = blk == 0 && threadIdx.x == 0;
+    if (!(zeta == zeta) || !(gamma == gamma) || !(delta == delta) || fabs(zeta) > DBL_MAX) {
+        if (lead) { ctl->done = 3; ctl->it = it; ctl->zz = zeta; }
+        return;
+    }
+    if (zeta <= ctl->tol2) {
+        if (lead) { ctl->done = 1; ctl->it = it; ctl->zz = zeta; }
+        return;
+    }
+    if (it >= ctl->max_its) {
+        if (lead) { ctl->done = 2; ctl->it = it; ctl->zz = zeta; }
+        return;
+    }
+    double alpha, beta;
+    if (it == 0) {
+        beta = 0.0;
+        alpha = gamma / delta;
+    } else {
+        const double2 h = a.hist[it - 1];
+        beta = gamma / h.x;
+        alpha = gamma / (delta - beta * gamma / h.y);
+    }
+    if (lead) {
+        a.hist[it] = make_double2(gamma, alpha);
+        ctl->it = it + 1;
+        ctl->zz = zeta;
+    }
+    const RV al = (RV)alpha, be = (RV)beta;
+    const RV* __restrict__ A = a_vals<RV>(mv);
+    const RV* __restrict__ D = dinv_vals<RV>(mv);
+    const int cur = it & 1, nxt = cur ^ 1;
+    const RV* __restrict__ r = a.v.r[cur];
+    const RV* __restrict__ w = a.v.w[cur];
+    const RV* __restrict__ s_old = a.v.s[nxt];   // s_{i-1}
+    RV* __restrict__ r_out = a.v.r[nxt];
+    RV* __restrict__ w_out = a.v.w[nxt];
+    RV* __restrict__ s_out = a.v.s[cur];
+    const RowRange rr(a.n, blk, gridDim.x);
+    double part[3] = {0.0, 0.0, 0.0};
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kBlock) {
+        const int b = R.base(i);
+        const RV di = D[R.dslot(i, b)];
+        const RV ri = r[i], wi = w[i], so = s_old[i];
+        const RV pi = di * ri + be * a.v.p[i];
+        const RV si = wi + be * so;
+        a.v.x[i] += al * pi;
+        a.v.p[i] = pi;
+        s_out[i] = si;
+        const RV rn = ri - al * si;
+        r_out[i] = rn;
+        RV wn = 0;
+        for (int k = 0; k < R.width; ++k) {
+            const int j = R.col(i, b, k);
+            const RV zj = next_z<RV>(r[j], w[j], s_old[j], D[R.dslot(j, R.base(j))], al, be);
+            wn += A[R.slot(i, b, k)] * zj;
+        }
+        w_out[i] = wn;
+        const RV zn = di * rn;
+        part[0] += (double)rn * (double)zn;
+        part[1] += (double)wn * (double)zn;
+        part[2] += (double)zn * (double)zn;
+    }
+    write_partials<3>(part, a.pout, blk);
+}
+
+// Dacc = x (first solve of a refined step).
+template <class RV>
+__global__ __launch_bounds__(kBlock) void accumulate_kernel(int n, const StepClock* clock, double* __restrict__ Dacc,
+                                                            const RV* __restrict__ x)
+{
+    if (clock->poison) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) Dacc[i] = (double)x[i];
+}
+
+// V += (Dacc) + x when the step's final solve converged; otherwise poison the
+// clock so the rest of the batch is skipped until the host resumes the solve.
+template <class RV>
+__global__ __launch_bounds__(kBlock) void finish_kernel(int n, StepClock* clock, const SolveCtl* ctl,
+                                                        const SolveCtl* ctl_first, int32_t* its_log, int log_len,
+                                                        double* __restrict__ V, const double* __restrict__ Dacc,
+                                                        const RV* __restrict__ x)
+{
+    if (clock->poison) return;
+    const int k = clock->k;
+    if (ctl->done != 1) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            clock->poison = 1;
+            clock->failed_step = k;
+        }
+        return;
+    }
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const double v = V[i] + (Dacc ? Dacc[i] : 0.0) + (double)x[i];
+        V[i] = v;
+        if (!isfinite(v)) clock->nonfinite = 1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int slot = 2 * (k % log_len);
+        its_log[slot] = ctl_first ? ctl_first->it : ctl->it;
+        its_log[slot + 1] = ctl_first ? ctl->it : 0;
+        clock->k = k + 1;
+    }
 }
 
 }  // namespace cmx
@@ -342,6 +448,7 @@ struct cmx_ctx {
     double dt = 0, dt_ode = 0, am = 0, cm = 0, rtol = 0, atol = 0;
     int max_its = 0, refine = 0, n_sub = 1, use_graph = 0;
     int width = 0, n_patterns = 0;
+    int G = 1;                   // blocks of the reduction kernels
     long long nnz = 0;
     hipStream_t stream = nullptr;
     // device state
@@ -352,7 +459,7 @@ struct cmx_ctx {
     uint8_t* stim_id = nullptr;
     cmx_stimulus* stim = nullptr;
     void* F = nullptr;
-    void *r = nullptr, *p = nullptr, *q = nullptr, *d = nullptr;
+    void* vec[8] = {};           // r0 r1 w0 w1 s0 s1 p x
     double* Dacc = nullptr;
     double* iion = nullptr;      // scratch for cmx_get_iionic
     // matrix
@@ -361,16 +468,25 @@ struct cmx_ctx {
     int32_t* ell_col = nullptr;
     double *m64 = nullptr, *k64 = nullptr, *a64 = nullptr, *dinv64 = nullptr;
     float *a32 = nullptr, *dinv32 = nullptr;
-    double* partials = nullptr;
-    CgScalars* sc = nullptr;
-    CgScalars* h_sc = nullptr;   // pinned
+    double* partials = nullptr;  // 3 buffers of [kMaxBlocks * 3]
+    StepClock* clock = nullptr;
+    SolveCtl* ctl = nullptr;     // [2]: first solve, refinement solve
+    double2* hist = nullptr;
+    int32_t* its_log = nullptr;  // [batch][2] per-solve iterations, indexed by step % batch
+    // pinned host mirrors
+    StepClock* h_clock = nullptr;
+    SolveCtl* h_ctl = nullptr;
+    int32_t* h_its = nullptr;
+    int batch = 64;
     // host copies for cmx_get_iionic
     std::vector<cmx_stimulus> h_stim;
     std::vector<uint8_t> h_stim_id;
     double last_t_next = 0.0;
     bool stepped = false;
+    // CG iteration budget per solve in the enqueued sequence
+    int budget = 8;
     // stats
-    long long steps = 0, cg_total = 0, refine_total = 0;
+    long long steps = 0, cg_total = 0, refine_total = 0, resumes = 0;
     int last_its = 0;
     double last_residual = 0;
     double device_bytes = 0;
@@ -486,18 +602,62 @@ void collect_times(cmx_ctx* c)
     c->pending.clear();
 }
 
+MatVals mat_vals(cmx_ctx* c)
+{
+    MatVals m;
+    m.m64 = c->m64;
+    m.k64 = c->k64;
+    m.a64 = c->a64;
+    m.a32 = c->a32;
+    m.dinv64 = c->dinv64;
+    m.dinv32 = c->dinv32;
+    return m;
+}
+
+double* pbuf(cmx_ctx* c, int k) { return c->partials + (size_t)k * kMaxBlocks * 3; }
+
+template <class RV>
+CgVecs<RV> cg_vecs(cmx_ctx* c)
+{
+    CgVecs<RV> v;
+    v.r[0] = (RV*)c->vec[0]; v.r[1] = (RV*)c->vec[1];
+    v.w[0] = (RV*)c->vec[2]; v.w[1] = (RV*)c->vec[3];
+    v.s[0] = (RV*)c->vec[4]; v.s[1] = (RV*)c->vec[5];
+    v.p = (RV*)c->vec[6];
+    v.x = (RV*)c->vec[7];
+    return v;
+}
+
+// Partial-sum buffers: residual -> 0, init -> 1, iteration i reads 1 + (i & 1)
+// and writes 2 - (i & 1).
+template <class RV>
+CgArgs<RV> cg_args(cmx_ctx* c, int solve, bool first)
+{
+    CgArgs<RV> a;
+    a.n = c->n;
+    a.first = first ? 1 : 0;
+    a.rtol2 = c->rtol * c->rtol;
+    a.atol2 = c->atol * c->atol;
+    a.clock = c->clock;
+    a.ctl = c->ctl + solve;
+    a.hist = c->hist;
+    a.pin = pbuf(c, 0);
+    a.pout = pbuf(c, 1);
+    a.v = cg_vecs<RV>(c);
+    return a;
+}
+
 // ----- typed launches -------------------------------------------------------
 template <class RS, class RV>
-int launch_cells(cmx_ctx* c, double t, double t_next)
+int launch_cells(cmx_ctx* c)
 {
     CellArgs<RS, RV> a;
     a.n = c->n;
     a.n_sub = c->n_sub;
-    a.t = t;
-    a.t_next = t_next;
     a.dt_ode = c->dt_ode;
     a.am = c->am;
     a.cm = c->cm;
+    a.clock = c->clock;
     a.V = c->V;
     a.state = (RS*)c->state;
     a.params = (const RS*)c->params;
@@ -520,149 +680,156 @@ int launch_cells(cmx_ctx* c, double t, double t_next)
     return CMX_OK;
 }
 
-MatVals mat_vals(cmx_ctx* c)
-{
-    MatVals m;
-    m.m64 = c->m64;
-    m.k64 = c->k64;
-    m.a64 = c->a64;
-    m.a32 = c->a32;
-    m.dinv64 = c->dinv64;
-    m.dinv32 = c->dinv32;
-    return m;
-}
-
 template <class Rows, class RV>
-int launch_residual(cmx_ctx* c, const Rows& R, bool first, bool use_dacc)
+int launch_residual(cmx_ctx* c, const Rows& R, int solve, bool use_dacc)
 {
     ResidualArgs<RV> a;
     a.n = c->n;
     a.c = c->am * c->cm / c->dt;
-    a.rtol2 = c->rtol * c->rtol;
-    a.atol2 = c->atol * c->atol;
-    a.first = first ? 1 : 0;
+    a.clock = c->clock;
     a.V = c->V;
     a.Dacc = use_dacc ? c->Dacc : nullptr;
     a.F = (const RV*)c->F;
-    a.r = (RV*)c->r;
-    a.p = (RV*)c->p;
-    a.d = (RV*)c->d;
-    a.partials = c->partials;
-    a.sc = c->sc;
+    a.r0 = (RV*)c->vec[0];
+    a.partials = pbuf(c, 0);
+    a.ctl = c->ctl + solve;
     KernelTimer timer(c, CMX_K_RESIDUAL);
-    residual_kernel<Rows, RV><<<grid_of(c->n), kBlock, 0, c->stream>>>(R, mat_vals(c), a);
+    residual_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mat_vals(c), a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
 
 template <class Rows, class RV>
-int launch_cg_iters(cmx_ctx* c, const Rows& R, int iters)
+int launch_cg_init(cmx_ctx* c, const Rows& R, int solve, bool first)
 {
-    const int G = grid_of(c->n);
+    CgArgs<RV> a = cg_args<RV>(c, solve, first);
+    KernelTimer timer(c, CMX_K_SPMV);
+    cg_init_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mat_vals(c), a);
+    CMX_HIP(c, hipGetLastError());
+    return CMX_OK;
+}
+
+template <class Rows, class RV>
+int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, int from, int count)
+{
+    CgArgs<RV> a = cg_args<RV>(c, solve, false);
     const MatVals mv = mat_vals(c);
-    for (int k = 0; k < iters; ++k) {
-        {
-            KernelTimer timer(c, CMX_K_SPMV);
-            cg_spmv_kernel<Rows, RV><<<G, kBlock, 0, c->stream>>>(R, mv, c->n, (const RV*)c->p, (RV*)c->q, c->partials, c->sc);
-        }
-        {
-            KernelTimer timer(c, CMX_K_UPDATE);
-            cg_update_kernel<Rows, RV><<<G, kBlock, 0, c->stream>>>(R, mv, c->n, (RV*)c->d, (RV*)c->r, (const RV*)c->p,
-                                                                   (const RV*)c->q, c->partials, c->sc);
-        }
-        {
-            KernelTimer timer(c, CMX_K_PUPDATE);
-            cg_pupdate_kernel<Rows, RV><<<G, kBlock, 0, c->stream>>>(R, mv, c->n, (const RV*)c->r, (RV*)c->p, c->sc);
-        }
+    for (int it = from; it < from + count; ++it) {
+        a.pin = pbuf(c, 1 + (it & 1));
+        a.pout = pbuf(c, 2 - (it & 1));
+        KernelTimer timer(c, CMX_K_UPDATE);
+        cg_iter_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mv, a, it);
     }
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
 
-int read_scalars(cmx_ctx* c)
+template <class RV>
+int launch_finish(cmx_ctx* c, bool refined)
 {
-    CMX_HIP(c, hipMemcpyAsync(c->h_sc, c->sc, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
+    KernelTimer timer(c, CMX_K_FINISH);
+    finish_kernel<RV><<<grid_of(c->n), kBlock, 0, c->stream>>>(
+        c->n, c->clock, c->ctl + (refined ? 1 : 0), refined ? c->ctl : nullptr, c->its_log, c->batch, c->V,
+        refined ? c->Dacc : nullptr, (const RV*)c->vec[7]);
+    CMX_HIP(c, hipGetLastError());
+    return CMX_OK;
+}
+
+// Enqueue one whole step with an iteration budget of c->budget per solve.
+template <class Rows, class RS, class RV>
+int enqueue_step(cmx_ctx* c, const Rows& R)
+{
+    int rc;
+    if ((rc = launch_cells<RS, RV>(c))) return rc;
+    if ((rc = launch_residual<Rows, RV>(c, R, 0, false))) return rc;
+    if ((rc = launch_cg_init<Rows, RV>(c, R, 0, true))) return rc;
+    if ((rc = launch_cg_iters<Rows, RV>(c, R, 0, 0, c->budget))) return rc;
+    const bool refined = c->precision == CMX_MIXED && c->refine > 0;
+    if (refined) {
+        // FP64 refinement: true residual of V^n + d against the step's b,
+        // then a second CG solve on it.
+        accumulate_kernel<RV><<<grid_of(c->n), kBlock, 0, c->stream>>>(c->n, c->clock, c->Dacc, (const RV*)c->vec[7]);
+        if ((rc = launch_residual<Rows, RV>(c, R, 1, true))) return rc;
+        if ((rc = launch_cg_init<Rows, RV>(c, R, 1, false))) return rc;
+        if ((rc = launch_cg_iters<Rows, RV>(c, R, 1, 0, c->budget))) return rc;
+    }
+    return launch_finish<RV>(c, refined);
+}
+
+int sync_state(cmx_ctx* c)
+{
+    CMX_HIP(c, hipMemcpyAsync(c->h_clock, c->clock, sizeof(StepClock), hipMemcpyDeviceToHost, c->stream));
+    CMX_HIP(c, hipMemcpyAsync(c->h_ctl, c->ctl, 2 * sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
     CMX_HIP(c, hipStreamSynchronize(c->stream));
     if (!c->pending.empty()) collect_times(c);
     return CMX_OK;
 }
 
-// One linear solve on the residual already in r/p (d = 0): enqueue chunks of
-// iterations, each a no-op once converged, until the control block says done.
-template <class Rows, class RV>
-int solve(cmx_ctx* c, const Rows& R)
-{
-    // The first chunk is sized from the previous step's count, so a step
-    // normally costs one host synchronisation.
-    int chunk = std::min(std::max(c->last_its + 1, 2), c->max_its);
-    int launched = 0;
-    int rc;
-    do {
-        rc = launch_cg_iters<Rows, RV>(c, R, chunk);
-        if (rc) return rc;
-        launched += chunk;
-        rc = read_scalars(c);
-        if (rc) return rc;
-        chunk = std::min(4, std::max(c->max_its - launched, 1));
-    } while (c->h_sc->done == 0);
-    if (c->h_sc->nonfinite) return fail(c, CMX_E_NONFINITE, "non-finite residual in the diffusion solve");
-    if (c->h_sc->done == 2)
-        return fail(c, CMX_E_DIVERGED, "CG reached the iteration limit (" + std::to_string(c->max_its) +
-                                           ") without converging: KSP_DIVERGED_ITS");
-    return CMX_OK;
-}
-
+// Runs the clock's steps k .. n_steps-1 in batches, resuming poisoned steps.
 template <class Rows, class RS, class RV>
-int step_once(cmx_ctx* c, const Rows& R, double t, double t_next)
+int run_steps(cmx_ctx* c, const Rows& R)
 {
-    int rc = launch_cells<RS, RV>(c, t, t_next);
-    if (rc) return rc;
-    const int G = grid_of(c->n);
-    const int sweeps = (c->precision == CMX_MIXED) ? std::max(c->refine, 0) : 0;
-    rc = launch_residual<Rows, RV>(c, R, true, false);
-    if (rc) return rc;
-    rc = solve<Rows, RV>(c, R);
-    if (rc) return rc;
-    int its = c->h_sc->it;
-    if (sweeps == 0) {
-        KernelTimer timer(c, CMX_K_FINISH);
-        accumulate_kernel<RV><<<G, kBlock, 0, c->stream>>>(c->n, c->V, (const RV*)c->d, c->sc);
-    } else {
-        // FP64 refinement: true residual of V^n + Dacc against the step's b.
-        accumulate_kernel<RV><<<G, kBlock, 0, c->stream>>>(c->n, c->Dacc, (const RV*)c->d, c->sc);
-        for (int s = 0; s < sweeps; ++s) {
-            rc = launch_residual<Rows, RV>(c, R, false, true);
+    const bool refined = c->precision == CMX_MIXED && c->refine > 0;
+    const int solve = refined ? 1 : 0;
+    while (c->h_clock->k < c->h_clock->n_steps) {
+        const int k_start = c->h_clock->k;
+        const int todo = std::min(c->h_clock->n_steps - k_start, c->batch);
+        for (int s = 0; s < todo; ++s) {
+            const int rc = enqueue_step<Rows, RS, RV>(c, R);
             if (rc) return rc;
-            ++c->refine_total;
-            rc = read_scalars(c);
-            if (rc) return rc;
-            if (c->h_sc->done) break;   // true residual already below the threshold
-            rc = solve<Rows, RV>(c, R);
-            if (rc) return rc;
-            its += c->h_sc->it;
-            accumulate_kernel<RV><<<G, kBlock, 0, c->stream>>>(c->n, c->Dacc, (const RV*)c->d, c->sc);
         }
-        add_kernel<<<G, kBlock, 0, c->stream>>>(c->n, c->V, c->Dacc, c->sc);
+        int rc = sync_state(c);
+        if (rc) return rc;
+        while (c->h_clock->poison) {
+            // Resume the failed step's final solve where it stopped.
+            const SolveCtl& ctl = c->h_ctl[solve];
+            if (ctl.done == 2)
+                return fail(c, CMX_E_DIVERGED, "CG reached the iteration limit (" + std::to_string(c->max_its) +
+                                                   ") without converging: KSP_DIVERGED_ITS");
+            if (ctl.done == 3) return fail(c, CMX_E_NONFINITE, "non-finite residual in the diffusion solve");
+            ++c->resumes;
+            c->budget = std::min(c->budget + 2, c->max_its + 1);
+            if (ctl.done == 0) {
+                const int from = ctl.it;
+                const int chunk = std::max(1, std::min(4, c->max_its + 1 - from));
+                if ((rc = launch_cg_iters<Rows, RV>(c, R, solve, from, chunk))) return rc;
+            }
+            CMX_HIP(c, hipMemsetAsync(&c->clock->poison, 0, sizeof(int32_t), c->stream));
+            if ((rc = launch_finish<RV>(c, refined))) return rc;
+            if ((rc = sync_state(c))) return rc;
+        }
+        const int got = c->h_clock->k - k_start;
+        if (got <= 0) return fail(c, CMX_E_STATE, "no step completed in a batch");
+        // iteration statistics and the next budget
+        CMX_HIP(c, hipMemcpy(c->h_its, c->its_log, 2 * c->batch * sizeof(int32_t), hipMemcpyDeviceToHost));
+        int mx = 0;
+        for (int s = 0; s < got; ++s) {
+            const int slot = 2 * ((k_start + s) % c->batch);
+            const int a = c->h_its[slot], b = c->h_its[slot + 1];
+            c->cg_total += a + b;
+            mx = std::max(mx, std::max(a, b));
+            c->last_its = a + b;
+        }
+        if (refined) c->refine_total += got;
+        // budget: the largest per-solve count of the batch plus one
+        c->budget = std::min(std::max(mx + 1, 2), c->max_its + 1);
+        c->steps += got;
+        if (c->h_clock->nonfinite)
+            return fail(c, CMX_E_NONFINITE, "non-finite transmembrane potential (cell ODE or solve blew up)");
     }
-    CMX_HIP(c, hipGetLastError());
-    c->last_its = its;
-    c->cg_total += its;
-    c->last_residual = std::sqrt(std::max(c->h_sc->zz, 0.0));
-    c->steps += 1;
-    c->stepped = true;
-    c->last_t_next = t_next;
+    c->last_residual = std::sqrt(std::max(c->h_ctl[solve].zz, 0.0));
     return CMX_OK;
 }
 
 template <class RS, class RV>
-int step_typed(cmx_ctx* c, double t, double t_next)
+int steps_typed(cmx_ctx* c)
 {
     if (c->format == CMX_FMT_STENCIL) {
         StencilRows R{c->pid, c->off, c->width};
-        return step_once<StencilRows, RS, RV>(c, R, t, t_next);
+        return run_steps<StencilRows, RS, RV>(c, R);
     }
     EllRows R{c->ell_col, c->width, c->n};
-    return step_once<EllRows, RS, RV>(c, R, t, t_next);
+    return run_steps<EllRows, RS, RV>(c, R);
 }
 
 template <class RS>
@@ -707,6 +874,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
         return fail(c, CMX_E_ARG, "dt_ode must divide dt_pde");
     if (!(opts->am > 0) || !(opts->cm > 0)) return fail(c, CMX_E_ARG, "Am and Cm must be positive");
     if (opts->ksp_max_its <= 0) return fail(c, CMX_E_ARG, "ksp_max_its must be positive");
+    if (opts->ksp_rtol < 0 || opts->ksp_atol < 0) return fail(c, CMX_E_ARG, "negative KSP tolerance");
     if (tissue->n_params < 0 || tissue->n_params > c->P || (tissue->n_params > 0 && !tissue->params))
         return fail(c, CMX_E_ARG, "bad per-node parameter planes");
     if (tissue->n_stimuli < 0 || tissue->n_stimuli > 255 || (tissue->n_stimuli > 0 && !tissue->stimuli))
@@ -727,6 +895,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->refine = opts->refine_sweeps;
     c->use_graph = opts->use_graph;
     c->n_param_planes = tissue->n_params;
+    c->budget = std::min(8, c->max_its + 1);
 
     HostCsr csr;
     std::string err;
@@ -734,7 +903,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->nnz = csr.row_ptr[c->n];
     c->width = csr.max_row;
     const int n = c->n;
-    const double cA = c->am * c->cm / c->dt;
+    c->G = std::min(grid_of(n), kMaxBlocks);
 
     CMX_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
@@ -747,6 +916,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     }
     c->format = fmt;
     int rc;
+    const double cA = c->am * c->cm / c->dt;
     if (fmt == CMX_FMT_STENCIL) {
         const int W = st.width, NPt = st.n_patterns;
         c->n_patterns = NPt;
@@ -783,7 +953,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
         }
         for (int i = 0; i < n; ++i) {
             for (int p = csr.row_ptr[i]; p < csr.row_ptr[i + 1]; ++p)
-                if (csr.col[p] == i) dinv64[i] = 1.0 / ((1.0 / c->dt) * (c->am * c->cm * csr.mass[p]) + csr.stiff[p]);
+                if (csr.col[p] == i) dinv64[i] = 1.0 / (cA * csr.mass[p] + csr.stiff[p]);
             dinv32[i] = (float)dinv64[i];
         }
         if ((rc = upload(c, &c->ell_col, ell.col.data(), E))) return rc;
@@ -794,7 +964,6 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
         if ((rc = upload(c, &c->dinv64, dinv64.data(), n))) return rc;
         if ((rc = upload(c, &c->dinv32, dinv32.data(), n))) return rc;
     }
-    (void)cA;
 
     // Cells: default initial state (AbstractCardiacCell::Init ->
     // ResetToInitialConditions), V from the model's initial voltage.
@@ -848,20 +1017,30 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
 
     // Work vectors.
     if ((rc = dalloc(c, (char**)&c->F, n * rv))) return rc;
-    if ((rc = dalloc(c, (char**)&c->r, n * rv))) return rc;
-    if ((rc = dalloc(c, (char**)&c->p, n * rv))) return rc;
-    if ((rc = dalloc(c, (char**)&c->q, n * rv))) return rc;
-    if ((rc = dalloc(c, (char**)&c->d, n * rv))) return rc;
+    for (int v = 0; v < 8; ++v) {
+        if ((rc = dalloc(c, (char**)&c->vec[v], n * rv))) return rc;
+        CMX_HIP(c, hipMemset(c->vec[v], 0, n * rv));
+    }
     if ((rc = dalloc(c, &c->Dacc, n))) return rc;
     CMX_HIP(c, hipMemset(c->Dacc, 0, n * sizeof(double)));
     CMX_HIP(c, hipMemset(c->F, 0, n * rv));
-    if ((rc = dalloc(c, &c->partials, 3 * (size_t)grid_of(n)))) return rc;
-    if ((rc = dalloc(c, &c->sc, 1))) return rc;
-    CgScalars z{};
+    if ((rc = dalloc(c, &c->partials, 3 * (size_t)kMaxBlocks * 3))) return rc;
+    CMX_HIP(c, hipMemset(c->partials, 0, 3 * (size_t)kMaxBlocks * 3 * sizeof(double)));
+    if ((rc = dalloc(c, &c->clock, 1))) return rc;
+    if ((rc = dalloc(c, &c->ctl, 2))) return rc;
+    if ((rc = dalloc(c, &c->hist, (size_t)c->max_its + 2))) return rc;
+    if ((rc = dalloc(c, &c->its_log, 2 * (size_t)c->batch))) return rc;
+    CMX_HIP(c, hipMemset(c->its_log, 0, 2 * c->batch * sizeof(int32_t)));
+    SolveCtl z{};
     z.max_its = c->max_its;
-    CMX_HIP(c, hipMemcpy(c->sc, &z, sizeof(z), hipMemcpyHostToDevice));
-    CMX_HIP(c, hipHostMalloc((void**)&c->h_sc, sizeof(CgScalars), hipHostMallocDefault));
-    *c->h_sc = z;
+    SolveCtl z2[2] = {z, z};
+    CMX_HIP(c, hipMemcpy(c->ctl, z2, sizeof(z2), hipMemcpyHostToDevice));
+    StepClock ck{};
+    CMX_HIP(c, hipMemcpy(c->clock, &ck, sizeof(ck), hipMemcpyHostToDevice));
+    CMX_HIP(c, hipHostMalloc((void**)&c->h_clock, sizeof(StepClock), hipHostMallocDefault));
+    CMX_HIP(c, hipHostMalloc((void**)&c->h_ctl, 2 * sizeof(SolveCtl), hipHostMallocDefault));
+    CMX_HIP(c, hipHostMalloc((void**)&c->h_its, 2 * c->batch * sizeof(int32_t), hipHostMallocDefault));
+    *c->h_clock = ck;
     CMX_HIP(c, hipDeviceSynchronize());
     return CMX_OK;
 }
@@ -875,8 +1054,12 @@ void release(cmx_ctx* c)
     c->event_pool.clear();
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
-    if (c->h_sc) (void)hipHostFree(c->h_sc);
-    c->h_sc = nullptr;
+    if (c->h_clock) (void)hipHostFree(c->h_clock);
+    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    if (c->h_its) (void)hipHostFree(c->h_its);
+    c->h_clock = nullptr;
+    c->h_ctl = nullptr;
+    c->h_its = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
 }
@@ -914,20 +1097,23 @@ int cmx_step(cmx_ctx* c, double t0, int32_t n_steps)
 {
     if (!c) return fail(nullptr, CMX_E_ARG, "null context");
     if (n_steps < 0) return fail(c, CMX_E_ARG, "negative step count");
+    if (n_steps == 0) return CMX_OK;
     // TimeStepper(t0, t0 + n dt, dt): t_k = t0 + k dt, the end snapped
-    // (global/src/TimeStepper.cpp:106-121).
-    const double t_end = t0 + n_steps * c->dt;
-    const double eps = t_end > 1.0 ? DBL_EPSILON * t_end : DBL_EPSILON;
-    for (int k = 0; k < n_steps; ++k) {
-        const double t = t0 + k * c->dt;
-        double tn = t0 + (k + 1) * c->dt;
-        if (t_end - tn <= eps) tn = t_end;
-        const int rc = (c->precision == CMX_MIXED) ? step_typed<float, float>(c, t, tn) : step_typed<double, double>(c, t, tn);
-        if (rc) return rc;
-    }
-    int rc = read_scalars(c);
+    // (global/src/TimeStepper.cpp:106-121); evaluated on the device.
+    StepClock ck{};
+    ck.t0 = t0;
+    ck.dt = c->dt;
+    ck.t_end = t0 + n_steps * c->dt;
+    ck.eps = ck.t_end > 1.0 ? DBL_EPSILON * ck.t_end : DBL_EPSILON;
+    ck.k = 0;
+    ck.n_steps = n_steps;
+    *c->h_clock = ck;
+    CMX_HIP(c, hipMemcpyAsync(c->clock, c->h_clock, sizeof(StepClock), hipMemcpyHostToDevice, c->stream));
+    CMX_HIP(c, hipStreamSynchronize(c->stream));
+    const int rc = (c->precision == CMX_MIXED) ? steps_typed<float, float>(c) : steps_typed<double, double>(c);
     if (rc) return rc;
-    if (c->h_sc->nonfinite) return fail(c, CMX_E_NONFINITE, "non-finite transmembrane potential (cell ODE or solve blew up)");
+    c->stepped = true;
+    c->last_t_next = ck.t_end;
     return CMX_OK;
 }
 
@@ -963,6 +1149,7 @@ int cmx_set_state(cmx_ctx* c, const double* states)
     int rc = cmx_set_voltage(c, states);
     if (rc) return rc;
     const size_t cnt = (size_t)(c->S - 1) * c->n;
+    c->stepped = false;
     return c->precision == CMX_MIXED ? to_device_planes<float>(c, c->state, states + c->n, cnt)
                                      : to_device_planes<double>(c, c->state, states + c->n, cnt);
 }
@@ -971,7 +1158,7 @@ int cmx_get_iionic(cmx_ctx* c, double* out)
 {
     if (!c || !out) return fail(c, CMX_E_ARG, "null argument");
     if (!c->stepped) {
-        // No step yet: evaluate at the current state.
+        // No step since the state was set: evaluate at the current state.
         if (!c->iion) {
             int rc = dalloc(c, &c->iion, c->n);
             if (rc) return rc;

@@ -1,7 +1,13 @@
-// gfx950 kernels of the monodomain time-step.  One thread per node/row,
-// 256-thread blocks (four 64-lane wavefronts), block indices remapped so each
-// XCD sweeps a contiguous slice of rows (its private L2 then holds the
-// neighbouring planes the stencil gathers).
+// gfx950 kernels of the monodomain time-step: shared device helpers.
+// 256-thread blocks (four 64-lane wavefronts); each block owns a contiguous
+// slice of rows and block indices are remapped so that the blocks one XCD
+// runs cover a contiguous range (its private L2 then holds the neighbouring
+// planes the stencil gathers).
+//
+// Reductions never use fences or atomics: every producing block writes its
+// partial sums to its own slot, and every block of the *next* kernel sums all
+// slots in the same fixed order (the kernel boundary orders the writes).  The
+// totals are therefore bitwise identical in every block and on every run.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -13,22 +19,43 @@ namespace cmx {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kNumXcd = 8;
+constexpr int kMaxBlocks = 1024;   // partial-sum slots per reduction
 
-// Device-side CG control block; written only by the last block of a
-// reduction kernel, read by every later kernel of the step.
-struct CgScalars {
-    double rz;       // (r, z) of the current iterate
-    double alpha, beta;
-    double zz;       // ||D^-1 r||^2 of the current iterate
-    double tol2;     // squared stopping threshold
-    double bb;       // ||D^-1 b||^2
-    int32_t it;      // iterations done in this solve
-    int32_t max_its;
-    int32_t done;    // 0 running, 1 converged, 2 iteration limit
+// Time of the step being computed, advanced on the device by the finish
+// kernel so a step's launches have fixed arguments (and can be replayed).
+struct StepClock {
+    double t0, dt, t_end, eps;   // TimeStepper(t0, t_end, dt) of one cmx_step call
+    int32_t k;                   // steps completed
+    int32_t n_steps;
+    int32_t poison;              // a step's final solve missed its iteration budget:
+    int32_t failed_step;         //   every later kernel is a no-op until the host resumes it
     int32_t nonfinite;
-    uint32_t ticket; // last-block counter (reset by the last block)
-    uint32_t pad;
+    int32_t pad;
 };
+
+// One linear solve's control block.
+struct SolveCtl {
+    double tol2;                 // squared stopping threshold on ||D^-1 r||
+    double bb;                   // ||D^-1 b||^2 of the step
+    double zz;                   // ||D^-1 r||^2 at exit
+    int32_t it;                  // iterations applied
+    int32_t done;                // 0 running, 1 converged, 2 iteration limit, 3 non-finite
+    int32_t max_its;
+    int32_t pad;
+};
+
+__device__ __forceinline__ double step_time(const StepClock& c, int k)
+{
+    return c.t0 + k * c.dt;
+}
+
+// TimeStepper::CalculateNextTime (global/src/TimeStepper.cpp:106-121).
+__device__ __forceinline__ double step_next_time(const StepClock& c, int k)
+{
+    double n = c.t0 + (k + 1) * c.dt;
+    if (c.t_end - n <= c.eps) n = c.t_end;
+    return n;
+}
 
 // Hardware dispatches block b to XCD b % 8; give XCD x the contiguous range
 // of logical blocks [x*q + min(x, r), ...) of its q (+1) blocks.
@@ -48,7 +75,7 @@ struct StencilRows {
     __device__ __forceinline__ int base(int i) const { return (int)pid[i] * width; }
     __device__ __forceinline__ int col(int i, int b, int k) const { return i + off[b + k]; }
     __device__ __forceinline__ int slot(int, int b, int k) const { return b + k; }
-    __device__ __forceinline__ int dslot(int i, int b) const { return b / width; }
+    __device__ __forceinline__ int dslot(int, int b) const { return b / width; }
 };
 
 struct EllRows {
@@ -70,45 +97,40 @@ __device__ __forceinline__ void wave_sum(double (&v)[N])
         for (int j = 0; j < N; ++j) v[j] += __shfl_xor(v[j], s, kWave);
 }
 
-// Block-reduces N doubles; the last block to finish (ticket) sums the
-// per-block partials in block order (deterministic) and returns true on all
-// of its threads with the totals in tot.
+// Block-sum of N doubles into out[blk*N + j] (plain stores, thread 0).
 template <int N>
-__device__ __forceinline__ bool reduce_last_block(double (&v)[N], double* __restrict__ partials,
-                                                  uint32_t* ticket, int blk, int G, double (&tot)[N])
+__device__ __forceinline__ void write_partials(double (&v)[N], double* __restrict__ out, int blk)
 {
     __shared__ double red[kBlock / kWave][N];
-    __shared__ bool last;
     wave_sum<N>(v);
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     if (lane == 0)
 #pragma unroll
         for (int j = 0; j < N; ++j) red[wid][j] = v[j];
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int j = 0; j < N; ++j) {
             double s = 0;
 #pragma unroll
             for (int w = 0; w < kBlock / kWave; ++w) s += red[w][j];
-            partials[(size_t)blk * N + j] = s;
+            out[(size_t)blk * N + j] = s;
         }
-        __threadfence();
-        const uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == (uint32_t)(G - 1));
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+}
+
+// Sum of the G slots written by the previous kernel, identical in every block.
+template <int N>
+__device__ __forceinline__ void read_partials(const double* __restrict__ in, int G, double (&tot)[N])
+{
+    __shared__ double red[kBlock / kWave][N];
     double acc[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) acc[j] = 0;
-    // strided per-thread sums, then a fixed-order tree: deterministic for a given G
     for (int b = threadIdx.x; b < G; b += kBlock)
 #pragma unroll
-        for (int j = 0; j < N; ++j) acc[j] += partials[(size_t)b * N + j];
-    __syncthreads();
+        for (int j = 0; j < N; ++j) acc[j] += in[(size_t)b * N + j];
     wave_sum<N>(acc);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     if (lane == 0)
 #pragma unroll
         for (int j = 0; j < N; ++j) red[wid][j] = acc[j];
@@ -120,8 +142,7 @@ __device__ __forceinline__ bool reduce_last_block(double (&v)[N], double* __rest
         for (int w = 0; w < kBlock / kWave; ++w) s += red[w][j];
         tot[j] = s;
     }
-    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
+    __syncthreads();
 }
 
 // SimpleStimulus / RegularStimulus at time t (uA/cm^3); duration already padded.

@@ -1,0 +1,409 @@
+/*
+ * C++ host API for the MI355X monodomain path, mirroring the reference's
+ * user-facing classes so that code written against them keeps its shape:
+ *
+ *   HeartConfig                 heart/src/problem/HeartConfig.hpp:203 (singleton;
+ *                               defaults of heart/src/problem/HeartConfigDefaults.hpp:52-152)
+ *   SimpleStimulus,
+ *   RegularStimulus, ZeroStimulus
+ *                               heart/src/stimulus/SimpleStimulus.cpp:44-76,
+ *                               heart/src/stimulus/RegularStimulus.cpp:41-80
+ *   CellLuoRudy1991FromCellML,
+ *   CellTenTusscher2006EpiFromCellML[RushLarsen]
+ *                               the classes chaste_codegen generates from
+ *                               heart/src/odes/cellml/{LuoRudy1991,TenTusscher2006Epi}.cellml
+ *   AbstractCardiacCellFactory, PlaneStimulusCellFactory
+ *                               heart/src/problem/cell_factories/AbstractCardiacCellFactory.hpp:100-108,
+ *                               heart/src/problem/cell_factories/PlaneStimulusCellFactory.hpp:55-86
+ *   TetrahedralMesh::ConstructRegularSlabMesh
+ *                               mesh/src/common/AbstractTetrahedralMesh.cpp:569-614
+ *   MonodomainProblem           heart/src/problem/MonodomainProblem.cpp:44-137 with
+ *                               AbstractCardiacProblem::Initialise / Solve
+ *                               (heart/src/problem/AbstractCardiacProblem.cpp:117-214, 371-584)
+ *
+ * A cell factory returns a small descriptor (model, integrator, stimulus)
+ * per node instead of a heap-allocated cell object: the whole tissue lives in
+ * device memory as structure-of-arrays and is advanced by the gfx950 kernels
+ * behind include/cmx.h.  Errors throw chaste_amd::Exception (the reference's
+ * EXCEPTION / KSPEXCEPT, global/src/Exception.hpp).
+ */
+#ifndef CHASTE_AMD_MONODOMAIN_HPP_
+#define CHASTE_AMD_MONODOMAIN_HPP_
+
+#include <cfloat>
+#include <cmath>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../cmx.h"
+
+namespace chaste_amd {
+
+class Exception : public std::runtime_error {
+public:
+    Exception(int code, const std::string& msg) : std::runtime_error(msg), mCode(code) {}
+    int GetCode() const { return mCode; }
+private:
+    int mCode;
+};
+
+inline void Check(int rc, const cmx_ctx* ctx)
+{
+    if (rc != CMX_OK) throw Exception(rc, cmx_last_error(ctx));
+}
+
+// ---------------------------------------------------------------------------
+// HeartConfig subset (same names, units and defaults)
+// ---------------------------------------------------------------------------
+class HeartConfig {
+public:
+    static HeartConfig* Instance()
+    {
+        static HeartConfig instance;
+        return &instance;
+    }
+    static void Reset() { *Instance() = HeartConfig(); }
+
+    void SetSpaceDimension(unsigned d) { mDim = d; }
+    unsigned GetSpaceDimension() const { return mDim; }
+    void SetSimulationDuration(double ms) { mDuration = ms; }
+    double GetSimulationDuration() const { return mDuration; }
+    void SetSlabDimensions(double x, double y, double z, double h) { mDim = 3; mSize[0] = x; mSize[1] = y; mSize[2] = z; mStep = h; }
+    void SetSheetDimensions(double x, double y, double h) { mDim = 2; mSize[0] = x; mSize[1] = y; mSize[2] = 0; mStep = h; }
+    void SetFibreLength(double x, double h) { mDim = 1; mSize[0] = x; mSize[1] = mSize[2] = 0; mStep = h; }
+    const double* GetSlabSize() const { return mSize; }
+    double GetInterNodeSpace() const { return mStep; }
+
+    void SetIntracellularConductivities(const std::vector<double>& s) { mSigma = s; }
+    std::vector<double> GetIntracellularConductivities() const { return mSigma; }
+    void SetSurfaceAreaToVolumeRatio(double am) { mAm = am; }
+    double GetSurfaceAreaToVolumeRatio() const { return mAm; }
+    void SetCapacitance(double cm) { mCm = cm; }
+    double GetCapacitance() const { return mCm; }
+
+    void SetOdePdeAndPrintingTimeSteps(double ode, double pde, double print)
+    {
+        if (ode <= 0 || pde <= 0 || print <= 0) throw Exception(CMX_E_ARG, "Time steps must be positive");
+        if (ode > pde) throw Exception(CMX_E_ARG, "Ode time-step should not be greater than PDE time-step");
+        mOdeDt = ode; mPdeDt = pde; mPrintDt = print;
+    }
+    void SetOdeTimeStep(double dt) { SetOdePdeAndPrintingTimeSteps(dt, mPdeDt, mPrintDt); }
+    void SetPdeTimeStep(double dt) { SetOdePdeAndPrintingTimeSteps(mOdeDt, dt, mPrintDt); }
+    void SetPrintingTimeStep(double dt) { SetOdePdeAndPrintingTimeSteps(mOdeDt, mPdeDt, dt); }
+    double GetOdeTimeStep() const { return mOdeDt; }
+    double GetPdeTimeStep() const { return mPdeDt; }
+    double GetPrintingTimeStep() const { return mPrintDt; }
+
+    void SetUseAbsoluteTolerance(double atol) { mUseAbs = true; mAtol = atol; }
+    void SetUseRelativeTolerance(double rtol) { mUseAbs = false; mRtol = rtol; }
+    bool GetUseAbsoluteTolerance() const { return mUseAbs; }
+    bool GetUseRelativeTolerance() const { return !mUseAbs; }
+    double GetAbsoluteTolerance() const
+    {
+        if (!mUseAbs) throw Exception(CMX_E_STATE, "Absolute tolerance is not set in Chaste parameters");
+        return mAtol;
+    }
+    double GetRelativeTolerance() const
+    {
+        if (mUseAbs) throw Exception(CMX_E_STATE, "Relative tolerance is not set in Chaste parameters");
+        return mRtol;
+    }
+    void SetKSPSolver(const char* s) { if (std::string(s) != "cg") throw Exception(CMX_E_ARG, "only the cg KSP solver is provided"); }
+    void SetKSPPreconditioner(const char* pc) { mPc = pc; }  // Jacobi on the device for every PC name
+    std::string GetKSPPreconditioner() const { return mPc; }
+    void SetUseMassLumping(bool on = true) { mLumping = on; }
+    bool GetUseMassLumping() const { return mLumping; }
+
+    // MI355X-specific knobs (no reference counterpart).
+    void SetGpuPrecision(int precision) { mPrecision = precision; }
+    int GetGpuPrecision() const { return mPrecision; }
+    void SetGpuRefinementSweeps(int sweeps) { mRefine = sweeps; }
+    int GetGpuRefinementSweeps() const { return mRefine; }
+    void SetKspMaxIterations(int its) { mMaxIts = its; }
+    int GetKspMaxIterations() const { return mMaxIts; }
+
+private:
+    unsigned mDim = 3;
+    double mDuration = 10.0;
+    double mSize[3] = {0.2, 0.2, 0.2};
+    double mStep = 0.01;
+    std::vector<double> mSigma = {1.75, 1.75, 1.75};
+    double mAm = 1400.0, mCm = 1.0;
+    double mOdeDt = 0.01, mPdeDt = 0.01, mPrintDt = 0.01;
+    bool mUseAbs = true;
+    double mAtol = 2e-4, mRtol = 1e-6;
+    std::string mPc = "bjacobi";
+    bool mLumping = false;
+    int mPrecision = CMX_FP64;
+    int mRefine = 1;
+    int mMaxIts = 1000;
+};
+
+// ---------------------------------------------------------------------------
+// Stimuli
+// ---------------------------------------------------------------------------
+struct AbstractStimulusFunction {
+    virtual ~AbstractStimulusFunction() = default;
+    virtual cmx_stimulus Describe() const = 0;
+};
+
+struct ZeroStimulus : AbstractStimulusFunction {
+    cmx_stimulus Describe() const override { return cmx_stimulus{0.0, 0.0, 0.0, -1.0, 0.0}; }
+};
+
+struct SimpleStimulus : AbstractStimulusFunction {
+    SimpleStimulus(double magnitude, double duration, double timeOfStimulus = 0.0)
+        : mMagnitude(magnitude), mDuration(duration), mStart(timeOfStimulus) {}
+    cmx_stimulus Describe() const override { return cmx_stimulus{mMagnitude, mStart, mDuration, -1.0, 0.0}; }
+    double mMagnitude, mDuration, mStart;
+};
+
+struct RegularStimulus : AbstractStimulusFunction {
+    RegularStimulus(double magnitude, double duration, double period, double startTime, double stopTime = DBL_MAX)
+        : mMagnitude(magnitude), mDuration(duration), mPeriod(period), mStart(startTime), mStop(stopTime) {}
+    cmx_stimulus Describe() const override { return cmx_stimulus{mMagnitude, mStart, mDuration, mPeriod, mStop}; }
+    double mMagnitude, mDuration, mPeriod, mStart, mStop;
+};
+
+// ---------------------------------------------------------------------------
+// Cell types (descriptors of the generated models)
+// ---------------------------------------------------------------------------
+struct CellDescriptor {
+    int model = CMX_MODEL_LR91;
+    int integrator = CMX_FORWARD_EULER;
+    std::shared_ptr<AbstractStimulusFunction> stimulus;
+};
+
+struct CellLuoRudy1991FromCellML {
+    static constexpr int kModel = CMX_MODEL_LR91;
+    static constexpr int kIntegrator = CMX_FORWARD_EULER;
+};
+struct CellLuoRudy1991FromCellMLRushLarsen {
+    static constexpr int kModel = CMX_MODEL_LR91;
+    static constexpr int kIntegrator = CMX_RUSH_LARSEN;
+};
+struct CellTenTusscher2006EpiFromCellML {
+    static constexpr int kModel = CMX_MODEL_TT06_EPI;
+    static constexpr int kIntegrator = CMX_FORWARD_EULER;
+};
+struct CellTenTusscher2006EpiFromCellMLRushLarsen {
+    static constexpr int kModel = CMX_MODEL_TT06_EPI;
+    static constexpr int kIntegrator = CMX_RUSH_LARSEN;
+};
+
+// ---------------------------------------------------------------------------
+// Mesh (regular slabs; general simplices through SetNodesAndElements)
+// ---------------------------------------------------------------------------
+template <unsigned DIM>
+class TetrahedralMesh {
+public:
+    void ConstructRegularSlabMesh(double spaceStep, double width, double height = 0, double depth = 0)
+    {
+        // element counts snapped as the reference does (AbstractTetrahedralMesh.cpp:580-600)
+        const int ex = (int)((width + 0.5 * spaceStep) / spaceStep);
+        const int ey = DIM > 1 ? (int)((height + 0.5 * spaceStep) / spaceStep) : 0;
+        const int ez = DIM > 2 ? (int)((depth + 0.5 * spaceStep) / spaceStep) : 0;
+        if (std::fabs(ex * spaceStep - width) > DBL_EPSILON * width ||
+            (DIM > 1 && std::fabs(ey * spaceStep - height) > DBL_EPSILON * height) ||
+            (DIM > 2 && std::fabs(ez * spaceStep - depth) > DBL_EPSILON * depth))
+            throw Exception(CMX_E_ARG, "Space step does not divide the size of the mesh");
+        ConstructRegularSlabMeshWithDimensions(spaceStep, ex, ey, ez);
+    }
+    void ConstructRegularSlabMeshWithDimensions(double spaceStep, int ex, int ey = 0, int ez = 0)
+    {
+        int32_t n = 0, ne = 0;
+        Check(cmx_slab_mesh(DIM, ex, ey, ez, spaceStep, &n, &ne, nullptr, nullptr), nullptr);
+        mCoords.resize((size_t)n * DIM);
+        mElems.resize((size_t)ne * (DIM + 1));
+        Check(cmx_slab_mesh(DIM, ex, ey, ez, spaceStep, nullptr, nullptr, mCoords.data(), mElems.data()), nullptr);
+    }
+    void SetNodesAndElements(std::vector<double> coords, std::vector<int32_t> elems)
+    {
+        mCoords = std::move(coords);
+        mElems = std::move(elems);
+    }
+    unsigned GetNumNodes() const { return (unsigned)(mCoords.size() / DIM); }
+    unsigned GetNumElements() const { return (unsigned)(mElems.size() / (DIM + 1)); }
+    const double* GetNodeLocation(unsigned i) const { return &mCoords[(size_t)i * DIM]; }
+    const std::vector<double>& rGetCoords() const { return mCoords; }
+    const std::vector<int32_t>& rGetElements() const { return mElems; }
+
+private:
+    std::vector<double> mCoords;
+    std::vector<int32_t> mElems;
+};
+
+// ---------------------------------------------------------------------------
+// Cell factories
+// ---------------------------------------------------------------------------
+template <unsigned DIM>
+class AbstractCardiacCellFactory {
+public:
+    AbstractCardiacCellFactory() : mpZeroStimulus(new ZeroStimulus) {}
+    virtual ~AbstractCardiacCellFactory() = default;
+    virtual CellDescriptor CreateCardiacCellForTissueNode(const double* x) = 0;
+    void SetMesh(TetrahedralMesh<DIM>* pMesh) { mpMesh = pMesh; }
+    TetrahedralMesh<DIM>* GetMesh() { return mpMesh; }
+protected:
+    template <class CELL>
+    CellDescriptor Make(std::shared_ptr<AbstractStimulusFunction> stim) const
+    {
+        return CellDescriptor{CELL::kModel, CELL::kIntegrator, std::move(stim)};
+    }
+    std::shared_ptr<AbstractStimulusFunction> mpZeroStimulus;
+    TetrahedralMesh<DIM>* mpMesh = nullptr;
+};
+
+// Stimulated on the x = 0 plane (x^2 <= 1e-10), zero stimulus elsewhere.
+template <class CELL, unsigned DIM>
+class PlaneStimulusCellFactory : public AbstractCardiacCellFactory<DIM> {
+public:
+    explicit PlaneStimulusCellFactory(double stimulusMagnitude = -600, double stimulusDuration = 0.5)
+        : mpStimulus(new SimpleStimulus(stimulusMagnitude, stimulusDuration)) {}
+    CellDescriptor CreateCardiacCellForTissueNode(const double* x) override
+    {
+        if (x[0] * x[0] <= 1e-10) return this->template Make<CELL>(mpStimulus);
+        return this->template Make<CELL>(this->mpZeroStimulus);
+    }
+private:
+    std::shared_ptr<AbstractStimulusFunction> mpStimulus;
+};
+
+// ---------------------------------------------------------------------------
+// MonodomainProblem
+// ---------------------------------------------------------------------------
+template <unsigned DIM>
+class MonodomainProblem {
+public:
+    explicit MonodomainProblem(AbstractCardiacCellFactory<DIM>* pCellFactory) : mpFactory(pCellFactory) {}
+    ~MonodomainProblem()
+    {
+        if (mpCtx) cmx_destroy(mpCtx);
+    }
+    MonodomainProblem(const MonodomainProblem&) = delete;
+    MonodomainProblem& operator=(const MonodomainProblem&) = delete;
+
+    void SetMesh(TetrahedralMesh<DIM>* pMesh) { mpMesh = pMesh; }
+    TetrahedralMesh<DIM>& rGetMesh() { return *mpMesh; }
+
+    // Builds the mesh from HeartConfig when none was set, creates one cell
+    // per node through the factory and assembles M, K on the device.
+    void Initialise()
+    {
+        HeartConfig* cfg = HeartConfig::Instance();
+        if (!mpMesh) {
+            if (cfg->GetSpaceDimension() != DIM) throw Exception(CMX_E_ARG, "Space dimension does not match");
+            const double* sz = cfg->GetSlabSize();
+            mOwnMesh.ConstructRegularSlabMesh(cfg->GetInterNodeSpace(), sz[0], sz[1], sz[2]);
+            mpMesh = &mOwnMesh;
+        }
+        mpFactory->SetMesh(mpMesh);
+        const unsigned n = mpMesh->GetNumNodes();
+        std::vector<int32_t> stim_index(n, -1);
+        std::vector<cmx_stimulus> table;
+        std::vector<const AbstractStimulusFunction*> seen;
+        int model = -1, integrator = -1;
+        for (unsigned i = 0; i < n; ++i) {
+            CellDescriptor d = mpFactory->CreateCardiacCellForTissueNode(mpMesh->GetNodeLocation(i));
+            if (model < 0) { model = d.model; integrator = d.integrator; }
+            if (d.model != model || d.integrator != integrator)
+                throw Exception(CMX_E_ARG, "every node must use the same cell model and solver");
+            const cmx_stimulus s = d.stimulus ? d.stimulus->Describe() : cmx_stimulus{0, 0, 0, -1, 0};
+            if (s.magnitude == 0.0) continue;
+            int k = -1;
+            for (size_t q = 0; q < seen.size(); ++q)
+                if (seen[q] == d.stimulus.get()) k = (int)q;
+            if (k < 0) {
+                if (table.size() >= 255) throw Exception(CMX_E_ARG, "more than 255 distinct stimuli");
+                k = (int)table.size();
+                table.push_back(s);
+                seen.push_back(d.stimulus.get());
+            }
+            stim_index[i] = k;
+        }
+        cmx_mesh mesh{};
+        mesh.dim = DIM;
+        mesh.n_nodes = (int32_t)n;
+        mesh.coords = mpMesh->rGetCoords().data();
+        mesh.n_elems = (int32_t)mpMesh->GetNumElements();
+        mesh.elems = mpMesh->rGetElements().data();
+        const std::vector<double> sig = cfg->GetIntracellularConductivities();
+        for (unsigned r = 0; r < DIM; ++r) mesh.sigma[r * DIM + r] = sig[r];
+        cmx_tissue tissue{};
+        tissue.model = model;
+        tissue.integrator = integrator;
+        tissue.n_stimuli = (int32_t)table.size();
+        tissue.stimuli = table.data();
+        tissue.stim_index = stim_index.data();
+        cmx_opts o{};
+        o.precision = cfg->GetGpuPrecision();
+        o.format = CMX_FMT_AUTO;
+        o.dt_pde = cfg->GetPdeTimeStep();
+        o.dt_ode = cfg->GetOdeTimeStep();
+        o.am = cfg->GetSurfaceAreaToVolumeRatio();
+        o.cm = cfg->GetCapacitance();
+        if (cfg->GetUseAbsoluteTolerance()) { o.ksp_atol = cfg->GetAbsoluteTolerance(); o.ksp_rtol = 0.0; }
+        else { o.ksp_rtol = cfg->GetRelativeTolerance(); o.ksp_atol = 0.0; }
+        o.ksp_max_its = cfg->GetKspMaxIterations();
+        o.refine_sweeps = cfg->GetGpuRefinementSweeps();
+        o.mass_lumping = cfg->GetUseMassLumping() ? 1 : 0;
+        if (mpCtx) { cmx_destroy(mpCtx); mpCtx = nullptr; }
+        Check(cmx_create(&mesh, &tissue, &o, &mpCtx), nullptr);
+        mCurrentTime = 0.0;
+    }
+
+    // Advances to the simulation duration in printing steps; each printing
+    // interval is one TimeStepper of PDE steps (t_k = t_print + k dt).
+    void Solve()
+    {
+        if (!mpCtx) throw Exception(CMX_E_STATE, "Initialise() must be called before Solve()");
+        HeartConfig* cfg = HeartConfig::Instance();
+        const double end = cfg->GetSimulationDuration(), print = cfg->GetPrintingTimeStep(), dt = cfg->GetPdeTimeStep();
+        if (end <= mCurrentTime) throw Exception(CMX_E_ARG, "The simulation duration must be positive");
+        const long n_print = (long)std::floor((end - mCurrentTime) / print + 0.5);
+        const long per = (long)std::floor(print / dt + 0.5);
+        if (std::fabs(per * dt - print) > 1e-10 * print)
+            throw Exception(CMX_E_ARG, "Printing time-step should be a multiple of PDE time step");
+        const double t_start = mCurrentTime;
+        for (long k = 0; k < n_print; ++k) {
+            const double t = t_start + k * print;   // printing TimeStepper
+            Check(cmx_step(mpCtx, t, (int32_t)per), mpCtx);
+        }
+        mCurrentTime = end;
+    }
+
+    std::vector<double> GetSolution() const
+    {
+        std::vector<double> V(mpMesh->GetNumNodes());
+        Check(cmx_get_voltage(mpCtx, V.data()), mpCtx);
+        return V;
+    }
+    void SetInitialCondition(const std::vector<double>& V) { Check(cmx_set_voltage(mpCtx, V.data()), mpCtx); }
+    std::vector<double> GetIionicCache() const
+    {
+        std::vector<double> out(mpMesh->GetNumNodes());
+        Check(cmx_get_iionic(mpCtx, out.data()), mpCtx);
+        return out;
+    }
+    cmx_stats GetStats() const
+    {
+        cmx_stats s{};
+        Check(cmx_get_stats(mpCtx, &s), mpCtx);
+        return s;
+    }
+    double GetCurrentTime() const { return mCurrentTime; }
+    cmx_ctx* GetContext() { return mpCtx; }
+
+private:
+    AbstractCardiacCellFactory<DIM>* mpFactory;
+    TetrahedralMesh<DIM>* mpMesh = nullptr;
+    TetrahedralMesh<DIM> mOwnMesh;
+    cmx_ctx* mpCtx = nullptr;
+    double mCurrentTime = 0.0;
+};
+
+}  // namespace chaste_amd
+
+#endif  // CHASTE_AMD_MONODOMAIN_HPP_

@@ -1,0 +1,68 @@
+// C++ API test on the GPU path, in the shape of the reference's
+// heart/test/monodomain/TestMonodomainProblem.hpp:784-880: the same
+// HeartConfig calls, PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, DIM>
+// (-600*5000 uA/cm^3) and the same golden voltages at nodes 1, 3, 5, 7
+// (atol 5e-3 mV).  Prints one JSON line per case; exit status 0 iff all pass.
+#include <cstdio>
+#include <vector>
+
+#include "chaste_amd/Monodomain.hpp"
+
+using namespace chaste_amd;
+
+static int failures = 0;
+
+template <unsigned DIM>
+static void RunCase(const char* name, const double golden[4], double duration)
+{
+    PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, DIM> cell_factory(-600 * 5000);
+    MonodomainProblem<DIM> monodomain_problem(&cell_factory);
+    HeartConfig::Instance()->SetSimulationDuration(duration);
+    monodomain_problem.Initialise();
+    monodomain_problem.Solve();
+    std::vector<double> V = monodomain_problem.GetSolution();
+    const unsigned nodes[4] = {1, 3, 5, 7};
+    double worst = 0;
+    for (int q = 0; q < 4; ++q) worst = std::max(worst, std::fabs(V[nodes[q]] - golden[q]));
+    const bool ok = worst < 5e-3;
+    if (!ok) ++failures;
+    std::printf("{\"case\": \"%s\", \"V\": [%.6f, %.6f, %.6f, %.6f], \"max_err\": %.3e, \"cg_iterations\": %lld, \"pass\": %s}\n",
+                name, V[1], V[3], V[5], V[7], worst, (long long)monodomain_problem.GetStats().cg_iterations,
+                ok ? "true" : "false");
+}
+
+int main()
+{
+    HeartConfig::Reset();
+    HeartConfig::Instance()->SetSpaceDimension(1);
+    HeartConfig::Instance()->SetFibreLength(1, 0.01);
+    const double g1[4] = {13.9682, 13.9149, 13.8216, 13.7275};
+    RunCase<1>("1d", g1, 5.0);
+
+    HeartConfig::Reset();
+    HeartConfig::Instance()->SetSpaceDimension(2);
+    HeartConfig::Instance()->SetSheetDimensions(0.3, 0.3, 0.01);
+    const double g2[4] = {17.5728, 17.4562, 17.2559, 17.0440};
+    RunCase<2>("2d", g2, 4.0);
+
+    HeartConfig::Reset();
+    HeartConfig::Instance()->SetSpaceDimension(3);
+    HeartConfig::Instance()->SetSlabDimensions(0.1, 0.1, 0.1, 0.01);
+    const double g3[4] = {31.8958, 32.1109, 32.7315, 33.7011};
+    RunCase<3>("3d", g3, 2.0);
+
+    // Error behaviour: Solve before Initialise, bad time steps, bad slab step.
+    HeartConfig::Reset();
+    HeartConfig::Instance()->SetSpaceDimension(2);
+    HeartConfig::Instance()->SetSheetDimensions(0.05, 0.05, 0.01);
+    PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, 2> factory(-600 * 5000);
+    MonodomainProblem<2> problem(&factory);
+    int thrown = 0;
+    try { problem.Solve(); } catch (const Exception& e) { ++thrown; }
+    try { HeartConfig::Instance()->SetOdePdeAndPrintingTimeSteps(0.02, 0.01, 0.01); } catch (const Exception& e) { ++thrown; }
+    try { TetrahedralMesh<2> m; m.ConstructRegularSlabMesh(0.03, 0.1, 0.1); } catch (const Exception& e) { ++thrown; }
+    const bool errors_ok = thrown == 3;
+    if (!errors_ok) ++failures;
+    std::printf("{\"case\": \"errors\", \"thrown\": %d, \"pass\": %s}\n", thrown, errors_ok ? "true" : "false");
+    return failures == 0 ? 0 : 1;
+}
