@@ -41,11 +41,27 @@ def build_problem(precision, coords, elems, stim_index, cmx):
                           refine_sweeps=1 if precision == cmx.MIXED else 0)
 
 
-def cell_bytes_per_node(precision, n_states):
-    """Algorithmic HBM bytes of one cell-step launch per node: V in (8),
-    non-V states in and out, stimulus id (1), forcing F out."""
+def bytes_per_row(kind, precision, n_states, refined):
+    """Algorithmic HBM bytes per row of one launch of a kernel kind that does
+    its work.  s = CG-vector / forcing size, ss = cell-state size; V is FP64.
+      cell      V in, non-V states in+out, stimulus id, F out
+      residual  V, F (+ first correction) in, pattern id, r z p s x out
+      spmv      z (gathered once from HBM), r in, pattern id, w out
+      update    z p w s x r in, pattern id, p s x r z out
+      finish    V, x1 (+ x2) in, V out"""
     s = 4 if precision == 1 else 8
-    return 8 + 2 * (n_states - 1) * s + 1 + s
+    ss = s
+    if kind == 'cell':
+        return 8 + 2 * (n_states - 1) * ss + 1 + s
+    if kind == 'residual':
+        return 8 + s + 1 + 5 * s + (s / 2 if refined else 0)   # refinement sweep every other launch
+    if kind == 'spmv':
+        return 3 * s + 1
+    if kind == 'update':
+        return 11 * s + 1
+    if kind == 'finish':
+        return 16 + s * (2 if refined else 1)
+    return 0
 
 
 def sync():
@@ -57,31 +73,52 @@ def sync():
         pass
 
 
-def time_path(m, steps, warmup, profile_kind):
-    """W untimed steps, then exactly K timed steps; events on profile_kind."""
+KINDS = ['cell', 'residual', 'spmv', 'update', 'finish']
+
+
+def time_path(m, steps, warmup, roof_steps):
+    """W untimed steps; an untimed 5-step pass with events on every kernel
+    kind ranks them; exactly K timed steps with no instrumentation (value);
+    then roof_steps more steps with HIP events around every launch of the
+    dominant kind only (its average launch duration for the roofline)."""
     t = 0.0
     if warmup:
         m.step(t, warmup)
         t += warmup * DT
-    # untimed profiling pass over every kernel kind to name the dominant one
-    m.set_profiling(['cell', 'residual', 'spmv', 'update', 'pupdate', 'finish'])
+    m.set_profiling(KINDS)
     m.kernel_times(reset=True)
     m.step(t, 5)
     t += 5 * DT
-    all_times = m.kernel_times(reset=True)
-    m.set_profiling([profile_kind])
+    ranking = {k: v for k, v in m.kernel_times(reset=True).items() if k in KINDS}
+    dominant = max(ranking, key=lambda k: ranking[k][0])
+    m.set_profiling([])
     st0 = m.stats()
     sync()
     t0 = time.perf_counter()
     m.step(t, steps)
     sync()
     elapsed = time.perf_counter() - t0
+    t += steps * DT
     st1 = m.stats()
+    m.set_profiling([dominant])
+    m.kernel_times(reset=True)
+    m.step(t, roof_steps)
+    st2 = m.stats()
     ktimes = m.kernel_times(reset=True)
     m.set_profiling([])
     its = (st1['cg_iterations'] - st0['cg_iterations']) / steps
     ref = (st1['refine_sweeps'] - st0['refine_sweeps']) / steps
-    return elapsed, its, ref, all_times, ktimes, t + steps * DT
+    roof_its = (st2['cg_iterations'] - st1['cg_iterations']) / roof_steps
+    return {'elapsed': elapsed, 'its': its, 'refine': ref, 'ranking': ranking, 'dominant': dominant,
+            'dominant_ms': ktimes[dominant][0], 'dominant_launches': ktimes[dominant][1], 'roof_its': roof_its}
+
+
+def working_launches_per_step(kind, its, refined):
+    if kind in ('spmv', 'update'):
+        return its
+    if kind == 'residual':
+        return 2 if refined else 1
+    return 1
 
 
 def cpu_baseline(n_steps, threads):
@@ -101,6 +138,7 @@ def main():
     ap.add_argument('--steps', type=int, default=300)
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--cpu-steps', type=int, default=20)
+    ap.add_argument('--roof-steps', type=int, default=50)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
@@ -118,18 +156,28 @@ def main():
     volts = {}
     for name, prec in (('fp64', cmx.FP64), ('mixed', cmx.MIXED)):
         m = build_problem(prec, coords, elems, stim_index, cmx)
-        elapsed, its, ref, all_t, kt, t_end = time_path(m, args.steps, args.warmup, 'cell')
-        dominant = max(all_t, key=lambda k: all_t[k][0])
-        cell_ms, cell_n = kt['cell']
+        r = time_path(m, args.steps, args.warmup, args.roof_steps)
+        refined = prec == cmx.MIXED
         st = m.stats()
+        kinds = {}
+        for k, (ms, cnt) in r['ranking'].items():
+            work = working_launches_per_step(k, r['its'], refined) * 5
+            b = bytes_per_row(k, prec, n_states, refined) * n
+            kinds[k] = {'ms_per_step': ms / 5, 'launches_per_step': cnt / 5,
+                        'working_launches_per_step': work / 5,
+                        'achieved_GBs': (work * b / (ms * 1e-3) / 1e9) if ms > 0 else None}
+        dom = r['dominant']
+        dom_work = working_launches_per_step(dom, r['roof_its'], refined) * args.roof_steps
+        dom_bytes = dom_work * bytes_per_row(dom, prec, n_states, refined) * n
         results[name] = {
-            'steps_per_s': args.steps / elapsed,
-            'ms_per_step': 1e3 * elapsed / args.steps,
-            'cg_its_per_step': its,
-            'refine_sweeps_per_step': ref,
-            'kernel_ms_per_step_profiled': {k: v[0] / 5 for k, v in all_t.items()},
-            'dominant_kernel': dominant,
-            'cell_avg_ms': cell_ms / max(cell_n, 1),
+            'steps_per_s': args.steps / r['elapsed'],
+            'ms_per_step': 1e3 * r['elapsed'] / args.steps,
+            'cg_its_per_step': r['its'],
+            'refine_sweeps_per_step': r['refine'],
+            'kernels_profiled': kinds,
+            'dominant_kernel': dom,
+            'dominant_avg_launch_ms': r['dominant_ms'] / max(r['dominant_launches'], 1),
+            'dominant_bytes_per_launch': dom_bytes / max(r['dominant_launches'], 1),
             'format': 'stencil' if st['format'] == cmx.FMT_STENCIL else 'ell',
             'n_patterns': st['n_patterns'],
         }
@@ -139,8 +187,12 @@ def main():
     dv = np.abs(volts['mixed'] - volts['fp64'])
     mixed = results['mixed']
     steps_s = mixed['steps_per_s']
-    cell_b = cell_bytes_per_node(1, n_states) * n
-    achieved = cell_b / (mixed['cell_avg_ms'] * 1e-3) / 1e9
+    dom = mixed['dominant_kernel']
+    achieved = mixed['dominant_bytes_per_launch'] / (mixed['dominant_avg_launch_ms'] * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, 'profiles', 'hbm_traffic_slab442k.json')
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get('mixed', {}).get(dom)
 
     line = {
         'metric': 'monodomain steps/s + node-ODE updates/s, FP32-mixed vs FP64 at <=0.5 mV',
@@ -163,11 +215,13 @@ def main():
         'fp64_node_ode_updates_per_s': round(results['fp64']['steps_per_s'] * n, 1),
         'mixed_over_fp64': round(steps_s / results['fp64']['steps_per_s'], 3),
         'accuracy_mixed_vs_fp64': {'max_dV_mV': float(dv.max()), 'rms_dV_mV': float(np.sqrt((dv ** 2).mean())),
-                                   'after_steps': args.warmup + 5 + args.steps},
+                                   'after_steps': args.warmup + 5 + args.steps + args.roof_steps},
         'paths': results,
-        'roofline': {'bound': 'hbm', 'kernel': 'cell_step (mixed)', 'achieved': round(achieved, 1),
+        'roofline': {'bound': 'hbm', 'kernel': dom + ' (mixed)', 'achieved': round(achieved, 1),
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
-                     'traffic': None, 'bytes_per_launch': cell_b},
+                     'traffic': traffic,
+                     'bytes_per_launch': round(mixed['dominant_bytes_per_launch']),
+                     'avg_launch_ms': round(mixed['dominant_avg_launch_ms'], 5)},
     }
     if not args.no_cpu_baseline:
         threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))

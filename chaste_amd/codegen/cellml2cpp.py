@@ -158,9 +158,22 @@ def cident(key):
 class Printer:
     """Translates MathML expression trees to C++ source text."""
 
-    def __init__(self, model, names):
+    def __init__(self, model, names, is_const=None):
         self.m = model
         self.names = names      # canonical var -> C expression
+        self.is_const = is_const or (lambda key: False)
+
+    def const_expr(self, el, cname):
+        """True when el depends on constants only (no state, parameter or time)."""
+        t = _tag(el)
+        if t == 'ci':
+            return self.is_const(self.ref(cname, el))
+        if t in ('cn', 'pi', 'exponentiale'):
+            return True
+        if t == 'apply':
+            return all(self.const_expr(c, cname) for c in list(el)[1:]
+                       if _tag(c) not in ('degree', 'logbase', 'bvar'))
+        return False
 
     def ref(self, cname, el):
         key = self.m.canon(cname, el.text.strip())
@@ -208,7 +221,12 @@ class Printer:
             if op == 'times':
                 return '(' + ' * '.join(a) + ')'
             if op == 'divide':
-                return '(%s / %s)' % (a[0], a[1])
+                # constant denominators become a multiplication by their
+                # reciprocal (folded at compile time); others go through
+                # cmx_div (exact in double, fast reciprocal in float)
+                if self.const_expr(args[1], cname):
+                    return '(%s * (R(1) / %s))' % (a[0], a[1])
+                return 'cmx_div(%s, %s)' % (a[0], a[1])
             if op == 'power':
                 ex = args[1]
                 if _tag(ex) == 'cn':
@@ -222,9 +240,9 @@ class Printer:
                     return 'sqrt(%s)' % a[0]
                 return 'pow(%s, R(1) / %s)' % (a[0], self.expr(list(deg)[0], cname))
             if op == 'exp':
-                return 'exp(%s)' % a[0]
+                return 'cmx_exp(%s)' % a[0]
             if op == 'ln':
-                return 'log(%s)' % a[0]
+                return 'cmx_log(%s)' % a[0]
             if op == 'log':
                 return 'log10(%s)' % a[0]
             if op == 'abs':
@@ -280,6 +298,22 @@ class Generator:
         for k, (c, rhs) in model.defs.items():
             self.deps[k] = deps_of(rhs, c, model, set())
         self._state_deps = {}
+
+    def is_const(self, key):
+        """Constants and algebraic variables built from constants only."""
+        m = self.m
+        memo = self.__dict__.setdefault('_const_memo', {})
+        if key in memo:
+            return memo[key]
+        if key in m.odes or key == m.stim or key == m.time or key in self.params or key in self.scales:
+            r = False
+        elif key in m.defs:
+            memo[key] = False   # guards cycles
+            r = all(self.is_const(d) for d in self.deps[key])
+        else:
+            r = m.vars[key]['init'] is not None
+        memo[key] = r
+        return r
 
     def state_deps(self, key, stack=()):
         """States a variable depends on, transitively (through algebraic defs)."""
@@ -363,6 +397,23 @@ class Generator:
             return (_div(na, da), _div(nb, da))
         return None
 
+    def rl_coeffs(self, rhs, cname, y, pr, aff):
+        """(g_inf, g_k) source for a gate dy/dt = a + b y.  For the usual
+        (y_inf - y) / tau form the steady state is y_inf itself and the rate
+        -1/tau, without the a/b round trip."""
+        if _tag(rhs) == 'apply' and _tag(list(rhs)[0]) == 'divide':
+            num, den = list(rhs)[1:]
+            if not self._depends(den, cname, y):
+                part = self.affine(num, cname, y, pr)
+                if part is not None and part[1] is not None:
+                    na, nb = part
+                    d = pr.expr(den, cname)
+                    if nb == '(-R(1))':
+                        return (na if na is not None else 'R(0)'), 'cmx_div(R(-1), %s)' % d
+                    return ('cmx_div(-(%s), %s)' % (na or 'R(0)', nb)), 'cmx_div(%s, %s)' % (nb, d)
+        a, b = aff
+        return 'cmx_div(-(%s), %s)' % (a if a is not None else 'R(0)', b), b
+
     def _depends(self, el, cname, y):
         for k in deps_of(el, cname, self.m, set()):
             if k == y or y in self.state_deps(k):
@@ -383,7 +434,7 @@ class Generator:
         names[m.time] = 't'
         for k in self.params:
             names[k] = 'p[%d]' % self.param_index[k]
-        pr = Printer(m, names)
+        pr = Printer(m, names, self.is_const)
 
         constants = [k for k, info in m.vars.items()
                      if m.canon_map[k] == k and info['init'] is not None
@@ -497,11 +548,10 @@ class Generator:
             i = state_ix[k]
             c, rhs = m.odes[k]
             if k in rl:
-                a, b = rl[k]
+                ginf, gk = self.rl_coeffs(rhs, c, k, pr, rl[k])
                 w('    if (RL) {')
-                w('        const R gk = %s;' % b)
-                w('        g_k[%d] = gk;' % i)
-                w('        g_inf[%d] = -(%s) / gk;' % (i, a if a is not None else 'R(0)'))
+                w('        g_k[%d] = %s;' % (i, gk))
+                w('        g_inf[%d] = %s;' % (i, ginf))
                 w('    } else {')
                 w('        dy[%d] = %s;' % (i, pr.expr(rhs, c)))
                 w('    }')

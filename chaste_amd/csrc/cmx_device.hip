@@ -35,6 +35,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         const RS hh = (RS)h;
 #pragma unroll
         for (int s = 1; s < NS; ++s) {
-            if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * exp(gk[s] * hh);
+            if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_exp(gk[s] * hh);
             else y[s] += hh * dy[s];
         }
     }
@@ -189,13 +190,14 @@ struct RowRange {
     }
 };
 
-// CG vectors (RV precision).  r, w, s are gathered by the stencil and
-// therefore double-buffered by iteration parity; p and x are row-private.
+// CG vectors (RV precision).  Only z is gathered by the stencil (w = A z);
+// every other vector is read and written by its own row only.
 template <class RV>
 struct CgVecs {
-    RV* r[2];
-    RV* w[2];
-    RV* s[2];
+    RV* r;
+    RV* z;
+    RV* w;
+    RV* s;
     RV* p;
     RV* x;
 };
@@ -206,44 +208,95 @@ struct ResidualArgs {
     double c;                          // Am Cm / dt
     const StepClock* clock;
     const double* __restrict__ V;
-    const double* __restrict__ Dacc;   // accumulated correction (refinement) or null
     const RV* __restrict__ F;
-    RV* __restrict__ r0;
-    double* __restrict__ partials;     // [G]: ||D^-1 b||^2
+    double* __restrict__ r0_64;        // FP64 residual of V^n, kept for the refinement sweep
+    CgVecs<RV> v;
+    double* __restrict__ partials;     // [G][2]: ||D^-1 b||^2, ||D^-1 r0||^2
     SolveCtl* ctl;
 };
 
-// r0 = M F - K (V + D) - c M D ; b = c M V + M F.  The stiffness term uses
-// differences (x_j - x_i): K's rows sum to zero, and the differences keep the
-// diffusion flux free of cancellation against the ~85 mV resting level.
+// r0 = M F - K V ; b = c M V + M F (only ||D^-1 b|| is needed); z0 = D^-1 r0
+// and p = s = x = 0 for the solve that follows.  r0 is also kept in FP64 for
+// the refinement sweep.  The stiffness term uses differences (V_j - V_i): K's
+// rows sum to zero, and the differences keep the diffusion flux free of
+// cancellation against the ~85 mV resting level.
 template <class Rows, class RV>
 __global__ __launch_bounds__(kBlock) void residual_kernel(Rows R, MatVals mv, ResidualArgs<RV> a)
 {
     if (a.clock->poison) return;
+    __shared__ double sM[kMaxPatterns * kMaxStencil], sK[kMaxPatterns * kMaxStencil], sD[kMaxPatterns];
+    const double* __restrict__ Mv = stage_table(R, mv.m64, sM, R.value_count());
+    const double* __restrict__ Kv = stage_table(R, mv.k64, sK, R.value_count());
+    const double* __restrict__ Dv = stage_table(R, mv.dinv64, sD, R.diag_count());
+    if constexpr (Rows::kStaged) __syncthreads();
+    const RV* __restrict__ D32 = dinv_vals<RV>(mv);
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const RowRange rr(a.n, blk, gridDim.x);
-    double part[1] = {0.0};
+    double part[2] = {0.0, 0.0};
     for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kBlock) {
-        const int b = R.base(i);
-        const double xi = a.V[i] + (a.Dacc ? a.Dacc[i] : 0.0);
-        double mf = 0, mvv = 0, kx = 0, md = 0;
-        for (int k = 0; k < R.width; ++k) {
-            const int j = R.col(i, b, k);
-            const int s = R.slot(i, b, k);
-            const double m = mv.m64[s], kk = mv.k64[s];
+        const auto row = R.row(i);
+        const double vi = a.V[i];
+        double mf = 0, mvv = 0, kx = 0;
+#pragma unroll
+        for (int k = 0; k < R.width(); ++k) {
+            const int j = R.col(row, k);
+            const int s = R.vidx(row, k);
             const double vj = a.V[j];
-            const double dj = a.Dacc ? a.Dacc[j] : 0.0;
-            mf += m * (double)a.F[j];
-            mvv += m * vj;
-            md += m * dj;
-            kx += kk * ((vj + dj) - xi);
+            mf += Mv[s] * (double)a.F[j];
+            mvv += Mv[s] * vj;
+            kx += Kv[s] * (vj - vi);
         }
-        const double di = mv.dinv64[R.dslot(i, b)];
-        const double bi = di * (a.c * mvv + mf);
-        a.r0[i] = (RV)(mf - kx - a.c * md);
+        const int d = R.didx(row);
+        const double bi = Dv[d] * (a.c * mvv + mf);
+        const double r64 = mf - kx;
+        if (a.r0_64) a.r0_64[i] = r64;
+        const RV ri = (RV)r64;
+        const RV zi = D32[d] * ri;
+        a.v.r[i] = ri;
+        a.v.z[i] = zi;
+        a.v.p[i] = RV(0);
+        a.v.s[i] = RV(0);
+        a.v.x[i] = RV(0);
         part[0] += bi * bi;
+        part[1] += (double)zi * (double)zi;
     }
-    write_partials<1>(part, a.partials, blk);
+    write_partials<2>(part, a.partials, blk);
+    if (blk == 0 && threadIdx.x == 0) {
+        a.ctl->done = 0;
+        a.ctl->it = 0;
+    }
+}
+
+// Refinement sweep: the true FP64 residual of V^n + x1 against the step's b,
+// r = r0 - A x1 (A in FP64), then z = D^-1 r and p = s = x2 = 0 for the
+// second solve.  Partials: slot 0 unused, slot 1 ||D^-1 r||^2.
+template <class Rows, class RV>
+__global__ __launch_bounds__(kBlock) void refine_residual_kernel(Rows R, MatVals mv, ResidualArgs<RV> a,
+                                                                 const RV* __restrict__ x1)
+{
+    if (a.clock->poison) return;
+    __shared__ double sA[kMaxPatterns * kMaxStencil];
+    const double* __restrict__ Av = stage_table(R, mv.a64, sA, R.value_count());
+    if constexpr (Rows::kStaged) __syncthreads();
+    const RV* __restrict__ D32 = dinv_vals<RV>(mv);
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const RowRange rr(a.n, blk, gridDim.x);
+    double part[2] = {0.0, 0.0};
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kBlock) {
+        const auto row = R.row(i);
+        double ax = 0;
+#pragma unroll
+        for (int k = 0; k < R.width(); ++k) ax += Av[R.vidx(row, k)] * (double)x1[R.col(row, k)];
+        const RV ri = (RV)(a.r0_64[i] - ax);
+        const RV zi = D32[R.didx(row)] * ri;
+        a.v.r[i] = ri;
+        a.v.z[i] = zi;
+        a.v.p[i] = RV(0);
+        a.v.s[i] = RV(0);
+        a.v.x[i] = RV(0);
+        part[1] += (double)zi * (double)zi;
+    }
+    write_partials<2>(part, a.partials, blk);
     if (blk == 0 && threadIdx.x == 0) {
         a.ctl->done = 0;
         a.ctl->it = 0;
@@ -254,77 +307,110 @@ template <class RV>
 struct CgArgs {
     int n;
     int first;                         // first solve of the step: set the tolerance
+    int it0;                           // first iteration of a solve
     double rtol2, atol2;
     const StepClock* clock;
-    SolveCtl* ctl;
+    SolveCtl* ctl;                     // this solve
+    SolveCtl* ctl_all;                 // both solves of the step (tolerance is shared)
     double2* hist;                     // [max_its + 1]: (gamma_i, alpha_i)
     const double* __restrict__ pin;    // partials read
     double* __restrict__ pout;         // partials written
     CgVecs<RV> v;
 };
 
-// z0 = D^-1 r0 (gathered), w0 = A z0, p = s_{-1} = x = 0; (r0,z0), (w0,z0), (z0,z0).
+// w = A z with gamma = (r, z), delta = (w, z), zeta = (z, z).  The first call
+// of a step also turns ||D^-1 b||^2 into the stopping threshold.
 template <class Rows, class RV>
-__global__ __launch_bounds__(kBlock) void cg_init_kernel(Rows R, MatVals mv, CgArgs<RV> a)
+__global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Rows R, MatVals mv, CgArgs<RV> a)
 {
     if (a.clock->poison) return;
+    if (a.ctl->done) return;
+    __shared__ RV sA[kMaxPatterns * kMaxStencil];
+    const RV* __restrict__ A = stage_table(R, a_vals<RV>(mv), sA, R.value_count());
+    if constexpr (Rows::kStaged) __syncthreads();
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    if (a.first) {
-        double bb[1];
-        read_partials<1>(a.pin, gridDim.x, bb);
-        if (blk == 0 && threadIdx.x == 0) {
-            a.ctl->bb = bb[0];
-            a.ctl->tol2 = fmax(a.rtol2 * bb[0], a.atol2);
+    if (a.it0) {
+        // First iteration of a solve: the residual kernel left ||D^-1 b||^2
+        // and ||D^-1 r0||^2; a residual already below the threshold (the
+        // usual case of a refinement sweep) ends the solve here.
+        double g0[2];
+        read_partials<2>(a.pin, gridDim.x, g0);
+        double tol2 = a.first ? fmax(a.rtol2 * g0[0], a.atol2) : a.ctl->tol2;
+        const bool lead = blk == 0 && threadIdx.x == 0;
+        if (lead && a.first) {
+            a.ctl_all[0].bb = g0[0];
+            a.ctl_all[0].tol2 = tol2;
+            a.ctl_all[1].bb = g0[0];
+            a.ctl_all[1].tol2 = tol2;
+        }
+        if (g0[1] <= tol2) {
+            if (lead) { a.ctl->done = 1; a.ctl->it = 0; a.ctl->zz = g0[1]; }
+            return;
         }
     }
-    const RV* __restrict__ A = a_vals<RV>(mv);
-    const RV* __restrict__ D = dinv_vals<RV>(mv);
-    const RV* __restrict__ r0 = a.v.r[0];
+    const RV* __restrict__ z = a.v.z;
     const RowRange rr(a.n, blk, gridDim.x);
     double part[3] = {0.0, 0.0, 0.0};
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kBlock) {
-        const int b = R.base(i);
-        RV w = 0;
-        for (int k = 0; k < R.width; ++k) {
-            const int j = R.col(i, b, k);
-            w += A[R.slot(i, b, k)] * (D[R.dslot(j, R.base(j))] * r0[j]);
+    // two rows per trip so both rows' gathers are in flight together
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += 2 * kBlock) {
+        const int i2 = i + kBlock;
+        const bool two = i2 < rr.hi;
+        const auto row = R.row(i);
+        const auto row2 = R.row(two ? i2 : i);
+        RV w = 0, w2 = 0;
+#pragma unroll
+        for (int k = 0; k < R.width(); ++k) {
+            w += A[R.vidx(row, k)] * z[R.col(row, k)];
+            w2 += A[R.vidx(row2, k)] * z[R.col(row2, k)];
         }
-        const RV ri = r0[i];
-        const RV zi = D[R.dslot(i, b)] * ri;
-        a.v.w[0][i] = w;
-        a.v.s[1][i] = RV(0);
-        a.v.p[i] = RV(0);
-        a.v.x[i] = RV(0);
+        const RV ri = a.v.r[i], zi = z[i];
+        a.v.w[i] = w;
         part[0] += (double)ri * (double)zi;
         part[1] += (double)w * (double)zi;
         part[2] += (double)zi * (double)zi;
+        if (two) {
+            const RV ri2 = a.v.r[i2], zi2 = z[i2];
+            a.v.w[i2] = w2;
+            part[0] += (double)ri2 * (double)zi2;
+            part[1] += (double)w2 * (double)zi2;
+            part[2] += (double)zi2 * (double)zi2;
+        }
     }
     write_partials<3>(part, a.pout, blk);
 }
 
-// z_{i+1} at a row from iteration-i data: D^-1 (r_i - alpha (w_i + beta s_{i-1})).
-template <class RV>
-__device__ __forceinline__ RV next_z(RV r, RV w, RV s_old, RV dinv, RV alpha, RV beta)
-{
-    const RV s = w + beta * s_old;
-    return dinv * (r - alpha * s);
-}
-
-// Iteration i (Chronopoulos & Gear, J. Comput. Appl. Math. 25:153, 1989):
-//   p_i = z_i + beta_i p_{i-1};  s_i = w_i + beta_i s_{i-1}   (s_i = A p_i)
-//   x += alpha_i p_i;  r_{i+1} = r_i - alpha_i s_i;  z_{i+1} = D^-1 r_{i+1}
-//   w_{i+1} = A z_{i+1}  (z_{i+1} of each neighbour recomputed from its
-//   iteration-i values, so one kernel and one reduction per iteration)
-//   gamma = (r,z), delta = (w,z), zeta = (z,z) of iteration i+1
-//   beta_{i+1} = gamma_{i+1}/gamma_i,
-//   alpha_{i+1} = gamma_{i+1} / (delta_{i+1} - beta_{i+1} gamma_{i+1}/alpha_i).
+// Iteration i of Chronopoulos & Gear's CG (J. Comput. Appl. Math. 25:153,
+// 1989): one reduction per iteration, taken from the preceding spmv:
+//   beta_i = gamma_i / gamma_{i-1},
+//   alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1})   (i > 0)
+//   p = z + beta p;  s = w + beta s  (= A p);  x += alpha p;
+//   r -= alpha s;  z = D^-1 r.
+// Stops (x final) when zeta_i = ||D^-1 r_i||^2 <= tol2.
 template <class Rows, class RV>
-__global__ __launch_bounds__(kBlock) void cg_iter_kernel(Rows R, MatVals mv, CgArgs<RV> a, int it)
+__global__ __launch_bounds__(kBlock) void cg_update_kernel(Rows R, MatVals mv, CgArgs<RV> a, int it)
 {
     if (a.clock->poison) return;
     SolveCtl* ctl = a.ctl;
     if (ctl->done) return;
+    __shared__ RV sD[kMaxPatterns];
+    const RV* __restrict__ D = stage_table(R, dinv_vals<RV>(mv), sD, R.diag_count());
+    if constexpr (Rows::kStaged) __syncthreads();
     const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const RowRange rr(a.n, blk, gridDim.x);
+    // Own-row operands of the first kPre rows are loaded before the partial
+    // sums are reduced, so the two latencies overlap.
+    constexpr int kPre = 2;
+    RV pz[kPre], pp[kPre], pw[kPre], ps[kPre], px[kPre], pr[kPre];
+    int pd[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+        const int i = rr.lo + threadIdx.x + u * kBlock;
+        if (i < rr.hi) {
+            pz[u] = a.v.z[i]; pp[u] = a.v.p[i]; pw[u] = a.v.w[i];
+            ps[u] = a.v.s[i]; px[u] = a.v.x[i]; pr[u] = a.v.r[i];
+            pd[u] = R.didx(R.row(i));
+        }
+    }
     double g[3];
     read_partials<3>(a.pin, gridDim.x, g);
     const double gamma = g[0], delta = g[1], zeta = g[2];
@@ -356,60 +442,40 @@ __global__ __launch_bounds__(kBlock) void cg_iter_kernel(Rows R, MatVals mv, CgA
         ctl->zz = zeta;
     }
     const RV al = (RV)alpha, be = (RV)beta;
-    const RV* __restrict__ A = a_vals<RV>(mv);
-    const RV* __restrict__ D = dinv_vals<RV>(mv);
-    const int cur = it & 1, nxt = cur ^ 1;
-    const RV* __restrict__ r = a.v.r[cur];
-    const RV* __restrict__ w = a.v.w[cur];
-    const RV* __restrict__ s_old = a.v.s[nxt];   // s_{i-1}
-    RV* __restrict__ r_out = a.v.r[nxt];
-    RV* __restrict__ w_out = a.v.w[nxt];
-    RV* __restrict__ s_out = a.v.s[cur];
-    const RowRange rr(a.n, blk, gridDim.x);
-    double part[3] = {0.0, 0.0, 0.0};
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kBlock) {
-        const int b = R.base(i);
-        const RV di = D[R.dslot(i, b)];
-        const RV ri = r[i], wi = w[i], so = s_old[i];
-        const RV pi = di * ri + be * a.v.p[i];
-        const RV si = wi + be * so;
-        a.v.x[i] += al * pi;
-        a.v.p[i] = pi;
-        s_out[i] = si;
-        const RV rn = ri - al * si;
-        r_out[i] = rn;
-        RV wn = 0;
-        for (int k = 0; k < R.width; ++k) {
-            const int j = R.col(i, b, k);
-            const RV zj = next_z<RV>(r[j], w[j], s_old[j], D[R.dslot(j, R.base(j))], al, be);
-            wn += A[R.slot(i, b, k)] * zj;
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+        const int i = rr.lo + threadIdx.x + u * kBlock;
+        if (i < rr.hi) {
+            const RV pi = pz[u] + be * pp[u];
+            const RV si = pw[u] + be * ps[u];
+            a.v.p[i] = pi;
+            a.v.s[i] = si;
+            a.v.x[i] = px[u] + al * pi;
+            const RV rn = pr[u] - al * si;
+            a.v.r[i] = rn;
+            a.v.z[i] = D[pd[u]] * rn;
         }
-        w_out[i] = wn;
-        const RV zn = di * rn;
-        part[0] += (double)rn * (double)zn;
-        part[1] += (double)wn * (double)zn;
-        part[2] += (double)zn * (double)zn;
     }
-    write_partials<3>(part, a.pout, blk);
+    for (int i = rr.lo + threadIdx.x + kPre * kBlock; i < rr.hi; i += kBlock) {
+        const RV pi = a.v.z[i] + be * a.v.p[i];
+        const RV si = a.v.w[i] + be * a.v.s[i];
+        a.v.p[i] = pi;
+        a.v.s[i] = si;
+        a.v.x[i] += al * pi;
+        const RV rn = a.v.r[i] - al * si;
+        a.v.r[i] = rn;
+        a.v.z[i] = D[R.didx(R.row(i))] * rn;
+    }
 }
 
-// Dacc = x (first solve of a refined step).
-template <class RV>
-__global__ __launch_bounds__(kBlock) void accumulate_kernel(int n, const StepClock* clock, double* __restrict__ Dacc,
-                                                            const RV* __restrict__ x)
-{
-    if (clock->poison) return;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) Dacc[i] = (double)x[i];
-}
-
-// V += (Dacc) + x when the step's final solve converged; otherwise poison the
+// V += x1 (+ x2 of the refinement solve) when the step's final solve
+// converged; otherwise poison the
 // clock so the rest of the batch is skipped until the host resumes the solve.
 template <class RV>
 __global__ __launch_bounds__(kBlock) void finish_kernel(int n, StepClock* clock, const SolveCtl* ctl,
                                                         const SolveCtl* ctl_first, int32_t* its_log, int log_len,
-                                                        double* __restrict__ V, const double* __restrict__ Dacc,
-                                                        const RV* __restrict__ x)
+                                                        double* __restrict__ V, const RV* __restrict__ x1,
+                                                        const RV* __restrict__ x2)
 {
     if (clock->poison) return;
     const int k = clock->k;
@@ -422,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void finish_kernel(int n, StepClock* clock,
     }
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
-        const double v = V[i] + (Dacc ? Dacc[i] : 0.0) + (double)x[i];
+        const double v = V[i] + (double)x1[i] + (x2 ? (double)x2[i] : 0.0);
         V[i] = v;
         if (!isfinite(v)) clock->nonfinite = 1;
     }
@@ -459,12 +525,12 @@ struct cmx_ctx {
     uint8_t* stim_id = nullptr;
     cmx_stimulus* stim = nullptr;
     void* F = nullptr;
-    void* vec[8] = {};           // r0 r1 w0 w1 s0 s1 p x
-    double* Dacc = nullptr;
+    void* vec[7] = {};           // r z w s p x1 x2
+    double* r0_64 = nullptr;     // FP64 step residual (refinement)
     double* iion = nullptr;      // scratch for cmx_get_iionic
     // matrix
-    uint16_t* pid = nullptr;
-    int32_t* off = nullptr;
+    uint8_t* pid = nullptr;
+    std::vector<int32_t> h_off;  // stencil offsets, padded to the kernel width
     int32_t* ell_col = nullptr;
     double *m64 = nullptr, *k64 = nullptr, *a64 = nullptr, *dinv64 = nullptr;
     float *a32 = nullptr, *dinv32 = nullptr;
@@ -483,8 +549,9 @@ struct cmx_ctx {
     std::vector<uint8_t> h_stim_id;
     double last_t_next = 0.0;
     bool stepped = false;
-    // CG iteration budget per solve in the enqueued sequence
+    // CG iteration budgets (first solve, refinement solve) per enqueued step
     int budget = 8;
+    int refine_budget = 1;
     // stats
     long long steps = 0, cg_total = 0, refine_total = 0, resumes = 0;
     int last_its = 0;
@@ -617,33 +684,36 @@ MatVals mat_vals(cmx_ctx* c)
 double* pbuf(cmx_ctx* c, int k) { return c->partials + (size_t)k * kMaxBlocks * 3; }
 
 template <class RV>
-CgVecs<RV> cg_vecs(cmx_ctx* c)
+CgVecs<RV> cg_vecs(cmx_ctx* c, int solve)
 {
     CgVecs<RV> v;
-    v.r[0] = (RV*)c->vec[0]; v.r[1] = (RV*)c->vec[1];
-    v.w[0] = (RV*)c->vec[2]; v.w[1] = (RV*)c->vec[3];
-    v.s[0] = (RV*)c->vec[4]; v.s[1] = (RV*)c->vec[5];
-    v.p = (RV*)c->vec[6];
-    v.x = (RV*)c->vec[7];
+    v.r = (RV*)c->vec[0];
+    v.z = (RV*)c->vec[1];
+    v.w = (RV*)c->vec[2];
+    v.s = (RV*)c->vec[3];
+    v.p = (RV*)c->vec[4];
+    v.x = (RV*)c->vec[5 + solve];
     return v;
 }
 
-// Partial-sum buffers: residual -> 0, init -> 1, iteration i reads 1 + (i & 1)
-// and writes 2 - (i & 1).
+// Partial-sum buffers: residual -> 0 (||D^-1 b||^2), spmv -> 1 (dot products,
+// read by the update that follows).
 template <class RV>
 CgArgs<RV> cg_args(cmx_ctx* c, int solve, bool first)
 {
     CgArgs<RV> a;
     a.n = c->n;
     a.first = first ? 1 : 0;
+    a.it0 = 0;
     a.rtol2 = c->rtol * c->rtol;
     a.atol2 = c->atol * c->atol;
     a.clock = c->clock;
     a.ctl = c->ctl + solve;
+    a.ctl_all = c->ctl;
     a.hist = c->hist;
     a.pin = pbuf(c, 0);
     a.pout = pbuf(c, 1);
-    a.v = cg_vecs<RV>(c);
+    a.v = cg_vecs<RV>(c, solve);
     return a;
 }
 
@@ -681,44 +751,48 @@ int launch_cells(cmx_ctx* c)
 }
 
 template <class Rows, class RV>
-int launch_residual(cmx_ctx* c, const Rows& R, int solve, bool use_dacc)
+int launch_residual(cmx_ctx* c, const Rows& R, int solve, bool refine_sweep)
 {
     ResidualArgs<RV> a;
     a.n = c->n;
     a.c = c->am * c->cm / c->dt;
     a.clock = c->clock;
     a.V = c->V;
-    a.Dacc = use_dacc ? c->Dacc : nullptr;
     a.F = (const RV*)c->F;
-    a.r0 = (RV*)c->vec[0];
+    a.r0_64 = c->r0_64;
+    a.v = cg_vecs<RV>(c, solve);
     a.partials = pbuf(c, 0);
     a.ctl = c->ctl + solve;
     KernelTimer timer(c, CMX_K_RESIDUAL);
-    residual_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mat_vals(c), a);
+    if (refine_sweep)
+        refine_residual_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mat_vals(c), a, (const RV*)c->vec[5]);
+    else
+        residual_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mat_vals(c), a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
 
+// CG iterations from..from+count-1 of a solve: each is (spmv, update); the
+// very first spmv of a step's first solve sets the tolerance.
 template <class Rows, class RV>
-int launch_cg_init(cmx_ctx* c, const Rows& R, int solve, bool first)
-{
-    CgArgs<RV> a = cg_args<RV>(c, solve, first);
-    KernelTimer timer(c, CMX_K_SPMV);
-    cg_init_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mat_vals(c), a);
-    CMX_HIP(c, hipGetLastError());
-    return CMX_OK;
-}
-
-template <class Rows, class RV>
-int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, int from, int count)
+int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, bool first, int from, int count)
 {
     CgArgs<RV> a = cg_args<RV>(c, solve, false);
     const MatVals mv = mat_vals(c);
     for (int it = from; it < from + count; ++it) {
-        a.pin = pbuf(c, 1 + (it & 1));
-        a.pout = pbuf(c, 2 - (it & 1));
-        KernelTimer timer(c, CMX_K_UPDATE);
-        cg_iter_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mv, a, it);
+        a.first = (first && it == 0) ? 1 : 0;
+        a.it0 = it == 0 ? 1 : 0;
+        a.pin = pbuf(c, 0);
+        a.pout = pbuf(c, 1);
+        {
+            KernelTimer timer(c, CMX_K_SPMV);
+            cg_spmv_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mv, a);
+        }
+        a.pin = pbuf(c, 1);
+        {
+            KernelTimer timer(c, CMX_K_UPDATE);
+            cg_update_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mv, a, it);
+        }
     }
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
@@ -730,28 +804,26 @@ int launch_finish(cmx_ctx* c, bool refined)
     KernelTimer timer(c, CMX_K_FINISH);
     finish_kernel<RV><<<grid_of(c->n), kBlock, 0, c->stream>>>(
         c->n, c->clock, c->ctl + (refined ? 1 : 0), refined ? c->ctl : nullptr, c->its_log, c->batch, c->V,
-        refined ? c->Dacc : nullptr, (const RV*)c->vec[7]);
+        (const RV*)c->vec[5], refined ? (const RV*)c->vec[6] : nullptr);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
 
-// Enqueue one whole step with an iteration budget of c->budget per solve.
+// Enqueue one whole step with an iteration budget of c->budget per solve:
+// budget + 1 (spmv, update) pairs, the last update detecting convergence.
 template <class Rows, class RS, class RV>
 int enqueue_step(cmx_ctx* c, const Rows& R)
 {
     int rc;
     if ((rc = launch_cells<RS, RV>(c))) return rc;
     if ((rc = launch_residual<Rows, RV>(c, R, 0, false))) return rc;
-    if ((rc = launch_cg_init<Rows, RV>(c, R, 0, true))) return rc;
-    if ((rc = launch_cg_iters<Rows, RV>(c, R, 0, 0, c->budget))) return rc;
+    if ((rc = launch_cg_iters<Rows, RV>(c, R, 0, true, 0, c->budget + 1))) return rc;
     const bool refined = c->precision == CMX_MIXED && c->refine > 0;
     if (refined) {
-        // FP64 refinement: true residual of V^n + d against the step's b,
-        // then a second CG solve on it.
-        accumulate_kernel<RV><<<grid_of(c->n), kBlock, 0, c->stream>>>(c->n, c->clock, c->Dacc, (const RV*)c->vec[7]);
+        // FP64 refinement: true residual of V^n + x1 against the step's b,
+        // then a second CG solve on it into x2.
         if ((rc = launch_residual<Rows, RV>(c, R, 1, true))) return rc;
-        if ((rc = launch_cg_init<Rows, RV>(c, R, 1, false))) return rc;
-        if ((rc = launch_cg_iters<Rows, RV>(c, R, 1, 0, c->budget))) return rc;
+        if ((rc = launch_cg_iters<Rows, RV>(c, R, 1, false, 0, c->refine_budget + 1))) return rc;
     }
     return launch_finish<RV>(c, refined);
 }
@@ -780,7 +852,9 @@ int run_steps(cmx_ctx* c, const Rows& R)
         }
         int rc = sync_state(c);
         if (rc) return rc;
+        int guard = 0;
         while (c->h_clock->poison) {
+            if (++guard > c->max_its + 16) return fail(c, CMX_E_STATE, "resuming a step made no progress");
             // Resume the failed step's final solve where it stopped.
             const SolveCtl& ctl = c->h_ctl[solve];
             if (ctl.done == 2)
@@ -788,13 +862,15 @@ int run_steps(cmx_ctx* c, const Rows& R)
                                                    ") without converging: KSP_DIVERGED_ITS");
             if (ctl.done == 3) return fail(c, CMX_E_NONFINITE, "non-finite residual in the diffusion solve");
             ++c->resumes;
-            c->budget = std::min(c->budget + 2, c->max_its + 1);
+            if (solve == 1) c->refine_budget = std::min(c->refine_budget + 2, c->max_its);
+            else c->budget = std::min(c->budget + 2, c->max_its);
+            // Clear the poison first: every kernel launched while it is set is a no-op.
+            CMX_HIP(c, hipMemsetAsync(&c->clock->poison, 0, sizeof(int32_t), c->stream));
             if (ctl.done == 0) {
                 const int from = ctl.it;
                 const int chunk = std::max(1, std::min(4, c->max_its + 1 - from));
-                if ((rc = launch_cg_iters<Rows, RV>(c, R, solve, from, chunk))) return rc;
+                if ((rc = launch_cg_iters<Rows, RV>(c, R, solve, false, from, chunk))) return rc;
             }
-            CMX_HIP(c, hipMemsetAsync(&c->clock->poison, 0, sizeof(int32_t), c->stream));
             if ((rc = launch_finish<RV>(c, refined))) return rc;
             if ((rc = sync_state(c))) return rc;
         }
@@ -802,17 +878,19 @@ int run_steps(cmx_ctx* c, const Rows& R)
         if (got <= 0) return fail(c, CMX_E_STATE, "no step completed in a batch");
         // iteration statistics and the next budget
         CMX_HIP(c, hipMemcpy(c->h_its, c->its_log, 2 * c->batch * sizeof(int32_t), hipMemcpyDeviceToHost));
-        int mx = 0;
+        int mx0 = 0, mx1 = 0;
         for (int s = 0; s < got; ++s) {
             const int slot = 2 * ((k_start + s) % c->batch);
             const int a = c->h_its[slot], b = c->h_its[slot + 1];
             c->cg_total += a + b;
-            mx = std::max(mx, std::max(a, b));
+            mx0 = std::max(mx0, a);
+            mx1 = std::max(mx1, b);
             c->last_its = a + b;
         }
         if (refined) c->refine_total += got;
-        // budget: the largest per-solve count of the batch plus one
-        c->budget = std::min(std::max(mx + 1, 2), c->max_its + 1);
+        // budgets: the largest count of the batch (the pair after it detects convergence)
+        c->budget = std::min(std::max(mx0, 1), c->max_its);
+        c->refine_budget = std::min(mx1, c->max_its);
         c->steps += got;
         if (c->h_clock->nonfinite)
             return fail(c, CMX_E_NONFINITE, "non-finite transmembrane potential (cell ODE or solve blew up)");
@@ -821,12 +899,24 @@ int run_steps(cmx_ctx* c, const Rows& R)
     return CMX_OK;
 }
 
+template <int W>
+StencilRows<W> stencil_rows(cmx_ctx* c)
+{
+    StencilRows<W> R;
+    R.pid = c->pid;
+    R.n = c->n;
+    R.n_patterns = c->n_patterns;
+    for (int k = 0; k < W; ++k) R.off[k] = c->h_off[k];
+    return R;
+}
+
 template <class RS, class RV>
 int steps_typed(cmx_ctx* c)
 {
     if (c->format == CMX_FMT_STENCIL) {
-        StencilRows R{c->pid, c->off, c->width};
-        return run_steps<StencilRows, RS, RV>(c, R);
+        if (c->width == 4) return run_steps<StencilRows<4>, RS, RV>(c, stencil_rows<4>(c));
+        if (c->width == 8) return run_steps<StencilRows<8>, RS, RV>(c, stencil_rows<8>(c));
+        return run_steps<StencilRows<16>, RS, RV>(c, stencil_rows<16>(c));
     }
     EllRows R{c->ell_col, c->width, c->n};
     return run_steps<EllRows, RS, RV>(c, R);
@@ -895,7 +985,8 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->refine = opts->refine_sweeps;
     c->use_graph = opts->use_graph;
     c->n_param_planes = tissue->n_params;
-    c->budget = std::min(8, c->max_its + 1);
+    c->budget = std::min(8, c->max_its);
+    c->refine_budget = 1;
 
     HostCsr csr;
     std::string err;
@@ -903,40 +994,50 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->nnz = csr.row_ptr[c->n];
     c->width = csr.max_row;
     const int n = c->n;
-    c->G = std::min(grid_of(n), kMaxBlocks);
+    c->G = std::min(grid_of(n), 1024);
+    if (const char* e = std::getenv("CMX_CG_BLOCKS")) c->G = std::max(1, std::min(grid_of(n), std::min(atoi(e), kMaxBlocks)));
 
     CMX_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
     int fmt = opts->format;
     StencilTable st;
     if (fmt == CMX_FMT_AUTO || fmt == CMX_FMT_STENCIL) {
-        if (build_stencil(csr, 4096, st)) fmt = CMX_FMT_STENCIL;
-        else if (fmt == CMX_FMT_STENCIL) return fail(c, CMX_E_ARG, "mesh has too many distinct row patterns for the stencil format");
+        if (build_stencil(csr, kMaxPatterns, kMaxStencil, st)) fmt = CMX_FMT_STENCIL;
+        else if (fmt == CMX_FMT_STENCIL)
+            return fail(c, CMX_E_ARG, "mesh needs more than 64 row patterns or 16 stencil offsets: use the ELL format");
         else fmt = CMX_FMT_ELL;
     }
     c->format = fmt;
     int rc;
     const double cA = c->am * c->cm / c->dt;
     if (fmt == CMX_FMT_STENCIL) {
-        const int W = st.width, NPt = st.n_patterns;
+        // pad the width to the kernel's compile-time width (4, 8 or 16)
+        const int W = st.width <= 4 ? 4 : (st.width <= 8 ? 8 : 16);
+        const int NPt = st.n_patterns;
+        c->width = W;
         c->n_patterns = NPt;
-        std::vector<double> a64(st.m.size()), dinv64(NPt);
-        std::vector<float> a32(st.m.size()), dinv32(NPt);
-        for (size_t s = 0; s < st.m.size(); ++s) {
-            a64[s] = (1.0 / c->dt) * (c->am * c->cm * st.m[s]) + st.k[s];
-            a32[s] = (float)a64[s];
-        }
+        c->h_off.assign(W, 0);
+        std::copy(st.off.begin(), st.off.end(), c->h_off.begin());
+        std::vector<double> m64((size_t)NPt * W, 0.0), k64((size_t)NPt * W, 0.0), a64((size_t)NPt * W, 0.0);
+        std::vector<float> a32((size_t)NPt * W, 0.0f);
+        std::vector<double> dinv64(NPt);
+        std::vector<float> dinv32(NPt);
         for (int q = 0; q < NPt; ++q) {
             double diag = 0;
-            for (int k = 0; k < W; ++k)
-                if (st.off[(size_t)q * W + k] == 0) diag += a64[(size_t)q * W + k];
+            for (int k = 0; k < st.width; ++k) {
+                const size_t src = (size_t)q * st.width + k, dst = (size_t)q * W + k;
+                m64[dst] = st.m[src];
+                k64[dst] = st.k[src];
+                a64[dst] = (1.0 / c->dt) * (c->am * c->cm * st.m[src]) + st.k[src];
+                a32[dst] = (float)a64[dst];
+                if (st.off[k] == 0) diag = a64[dst];
+            }
             dinv64[q] = 1.0 / diag;
             dinv32[q] = (float)dinv64[q];
         }
         if ((rc = upload(c, &c->pid, st.pid.data(), n))) return rc;
-        if ((rc = upload(c, &c->off, st.off.data(), st.off.size()))) return rc;
-        if ((rc = upload(c, &c->m64, st.m.data(), st.m.size()))) return rc;
-        if ((rc = upload(c, &c->k64, st.k.data(), st.k.size()))) return rc;
+        if ((rc = upload(c, &c->m64, m64.data(), m64.size()))) return rc;
+        if ((rc = upload(c, &c->k64, k64.data(), k64.size()))) return rc;
         if ((rc = upload(c, &c->a64, a64.data(), a64.size()))) return rc;
         if ((rc = upload(c, &c->a32, a32.data(), a32.size()))) return rc;
         if ((rc = upload(c, &c->dinv64, dinv64.data(), NPt))) return rc;
@@ -1017,12 +1118,13 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
 
     // Work vectors.
     if ((rc = dalloc(c, (char**)&c->F, n * rv))) return rc;
-    for (int v = 0; v < 8; ++v) {
+    for (int v = 0; v < 7; ++v) {
         if ((rc = dalloc(c, (char**)&c->vec[v], n * rv))) return rc;
         CMX_HIP(c, hipMemset(c->vec[v], 0, n * rv));
     }
-    if ((rc = dalloc(c, &c->Dacc, n))) return rc;
-    CMX_HIP(c, hipMemset(c->Dacc, 0, n * sizeof(double)));
+    if (c->precision == CMX_MIXED && c->refine > 0) {
+        if ((rc = dalloc(c, &c->r0_64, n))) return rc;
+    }
     CMX_HIP(c, hipMemset(c->F, 0, n * rv));
     if ((rc = dalloc(c, &c->partials, 3 * (size_t)kMaxBlocks * 3))) return rc;
     CMX_HIP(c, hipMemset(c->partials, 0, 3 * (size_t)kMaxBlocks * 3 * sizeof(double)));

@@ -143,35 +143,47 @@ bool assemble_p1(const cmx_mesh& mesh, bool lumped, HostCsr& out, std::string& e
     return true;
 }
 
-bool build_stencil(const HostCsr& A, int max_patterns, StencilTable& out)
+bool build_stencil(const HostCsr& A, int max_patterns, int max_width, StencilTable& out)
 {
-    const int n = A.n, W = A.max_row;
+    const int n = A.n;
+    std::vector<int32_t> uni;
+    for (int i = 0; i < n; ++i)
+        for (int q = A.row_ptr[i]; q < A.row_ptr[i + 1]; ++q) uni.push_back(A.col[q] - i);
+    std::sort(uni.begin(), uni.end());
+    uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+    if ((int)uni.size() > max_width) return false;
+    const int W = (int)uni.size();
     out.width = W;
+    out.off = uni;
     out.pid.assign(n, 0);
-    out.off.clear();
     out.m.clear();
     out.k.clear();
-    // Patterns keyed by their relative offsets; candidates compared by value.
-    std::map<std::vector<int32_t>, std::vector<int>> by_offsets;
+    // candidates bucketed by the set of offsets a row actually has
+    std::map<std::vector<int>, std::vector<int>> by_shape;
+    std::vector<double> rm(W), rk(W);
+    std::vector<int> shape;
     int count = 0;
-    std::vector<int32_t> offs;
     for (int i = 0; i < n; ++i) {
-        const int b = A.row_ptr[i], e = A.row_ptr[i + 1];
-        offs.assign(e - b, 0);
+        std::fill(rm.begin(), rm.end(), 0.0);
+        std::fill(rk.begin(), rk.end(), 0.0);
+        shape.clear();
         double scale_m = 0, scale_k = 0;
-        for (int q = b; q < e; ++q) {
-            offs[q - b] = A.col[q] - i;
+        for (int q = A.row_ptr[i]; q < A.row_ptr[i + 1]; ++q) {
+            const int s = (int)(std::lower_bound(uni.begin(), uni.end(), A.col[q] - i) - uni.begin());
+            rm[s] = A.mass[q];
+            rk[s] = A.stiff[q];
+            shape.push_back(s);
             scale_m = std::max(scale_m, std::fabs(A.mass[q]));
             scale_k = std::max(scale_k, std::fabs(A.stiff[q]));
         }
         const double tm = 1e-12 * scale_m, tk = 1e-12 * scale_k;
-        auto& cands = by_offsets[offs];
+        auto& cands = by_shape[shape];
         int found = -1;
         for (int c : cands) {
             bool same = true;
-            for (int q = b; q < e && same; ++q) {
-                const size_t s = (size_t)c * W + (q - b);
-                same = std::fabs(out.m[s] - A.mass[q]) <= tm && std::fabs(out.k[s] - A.stiff[q]) <= tk;
+            for (int s = 0; s < W && same; ++s) {
+                const size_t d = (size_t)c * W + s;
+                same = std::fabs(out.m[d] - rm[s]) <= tm && std::fabs(out.k[d] - rk[s]) <= tk;
             }
             if (same) { found = c; break; }
         }
@@ -179,17 +191,10 @@ bool build_stencil(const HostCsr& A, int max_patterns, StencilTable& out)
             if (count >= max_patterns) return false;
             found = count++;
             cands.push_back(found);
-            out.off.resize((size_t)count * W, 0);
-            out.m.resize((size_t)count * W, 0.0);
-            out.k.resize((size_t)count * W, 0.0);
-            for (int q = b; q < e; ++q) {
-                const size_t s = (size_t)found * W + (q - b);
-                out.off[s] = offs[q - b];
-                out.m[s] = A.mass[q];
-                out.k[s] = A.stiff[q];
-            }
+            out.m.insert(out.m.end(), rm.begin(), rm.end());
+            out.k.insert(out.k.end(), rk.begin(), rk.end());
         }
-        out.pid[i] = (uint16_t)found;
+        out.pid[i] = (uint8_t)found;
     }
     out.n_patterns = count;
     return true;

@@ -26,19 +26,21 @@ struct HostCsr {
 //  heart/src/solver/electrics/monodomain/MonodomainStiffnessMatrixAssembler.hpp:68-84).
 bool assemble_p1(const cmx_mesh& mesh, bool lumped, HostCsr& out, std::string& err);
 
-// Row-pattern table: rows whose relative column offsets match and whose
-// values agree to a relative 1e-12 of the row scale share one entry.  On a
-// regular slab there are at most 27 (3-D) or a few dozen (2-D) patterns, so the
-// SpMV reads a 16-bit pattern id per row instead of the row's indices and
-// values.  Padding slots have offset 0 and value 0.
+// Uniform-offset stencil: every row reads the same column offsets (the union
+// of all rows' offsets, i.e. the mesh stencil) and only the values depend on
+// the row.  Rows whose value vectors over that union agree to a relative
+// 1e-12 of the row scale share one pattern.  On a regular slab there are at
+// most 27 (3-D) or a few dozen (2-D) patterns, so the SpMV reads an 8-bit
+// pattern id per row, the offsets come from kernel arguments and the values
+// from a table staged in LDS.  Entries absent from a row hold value 0.
 struct StencilTable {
-    int width = 0;                 // slots per pattern
+    int width = 0;                 // |offset union|
     int n_patterns = 0;
-    std::vector<uint16_t> pid;     // [n]
-    std::vector<int32_t> off;      // [n_patterns][width]
+    std::vector<int32_t> off;      // [width], sorted
+    std::vector<uint8_t> pid;      // [n]
     std::vector<double> m, k;      // [n_patterns][width]
 };
-bool build_stencil(const HostCsr& A, int max_patterns, StencilTable& out);
+bool build_stencil(const HostCsr& A, int max_patterns, int max_width, StencilTable& out);
 
 // ELL: width slots per row, column-major ([slot][n]) for coalesced loads.
 struct EllMatrix {

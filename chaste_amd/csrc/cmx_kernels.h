@@ -19,7 +19,7 @@ namespace cmx {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kNumXcd = 8;
-constexpr int kMaxBlocks = 1024;   // partial-sum slots per reduction
+constexpr int kMaxBlocks = 2048;   // partial-sum slots per reduction
 
 // Time of the step being computed, advanced on the device by the finish
 // kernel so a step's launches have fixed arguments (and can be replayed).
@@ -66,27 +66,70 @@ __device__ __forceinline__ int xcd_block(int b, int G)
     return x * q + (x < r ? x : r) + k;
 }
 
-// Row accessors.  slot() indexes the value arrays, dslot() the diagonal
-// inverse, col() gives the column of slot k of row i.
+constexpr int kMaxPatterns = 64;   // stencil format: distinct row patterns
+constexpr int kMaxStencil = 16;    // stencil format: offsets per row (padded width)
+
+// Row accessors.  row(i) caches what a row needs; col(r, k) is the column of
+// slot k, vidx(r, k) the index into the value tables, didx(r) / didx_of(j)
+// the index into the diagonal-inverse table of row i / of any row j.
+//
+// Uniform-offset stencil: offsets are kernel arguments (held in SGPRs),
+// values and diagonal inverses are per-pattern tables that the kernels stage
+// in LDS (kStaged).  Out-of-range columns of boundary rows are clamped to a
+// valid row; their values are 0.
+template <int W>
 struct StencilRows {
-    const uint16_t* __restrict__ pid;
-    const int32_t* __restrict__ off;
-    int width;
-    __device__ __forceinline__ int base(int i) const { return (int)pid[i] * width; }
-    __device__ __forceinline__ int col(int i, int b, int k) const { return i + off[b + k]; }
-    __device__ __forceinline__ int slot(int, int b, int k) const { return b + k; }
-    __device__ __forceinline__ int dslot(int, int b) const { return b / width; }
+    static constexpr bool kStaged = true;
+    const uint8_t* __restrict__ pid;
+    int n;
+    int n_patterns;
+    int32_t off[W];
+    struct Row { int i; int p; };
+    __device__ __forceinline__ constexpr int width() const { return W; }
+    __device__ __forceinline__ Row row(int i) const { return Row{i, (int)pid[i]}; }
+    __device__ __forceinline__ int col(const Row& r, int k) const
+    {
+        const int j = r.i + off[k];
+        return j < 0 ? 0 : (j >= n ? n - 1 : j);
+    }
+    __device__ __forceinline__ int vidx(const Row& r, int k) const { return r.p * W + k; }
+    __device__ __forceinline__ int didx(const Row& r) const { return r.p; }
+    __device__ __forceinline__ int didx_of(int j) const { return (int)pid[j]; }
+    __device__ __forceinline__ int value_count() const { return n_patterns * W; }
+    __device__ __forceinline__ int diag_count() const { return n_patterns; }
 };
 
+// ELL: [slot][n] column-major indices and values in global memory.
 struct EllRows {
+    static constexpr bool kStaged = false;
     const int32_t* __restrict__ cols;
-    int width;
+    int w;
     int n;
-    __device__ __forceinline__ int base(int i) const { return i; }
-    __device__ __forceinline__ int col(int i, int, int k) const { return cols[(size_t)k * n + i]; }
-    __device__ __forceinline__ int slot(int i, int, int k) const { return k * n + i; }
-    __device__ __forceinline__ int dslot(int i, int) const { return i; }
+    struct Row { int i; };
+    __device__ __forceinline__ int width() const { return w; }
+    __device__ __forceinline__ Row row(int i) const { return Row{i}; }
+    __device__ __forceinline__ int col(const Row& r, int k) const { return cols[(size_t)k * n + r.i]; }
+    __device__ __forceinline__ int vidx(const Row& r, int k) const { return k * n + r.i; }
+    __device__ __forceinline__ int didx(const Row& r) const { return r.i; }
+    __device__ __forceinline__ int didx_of(int j) const { return j; }
+    __device__ __forceinline__ int value_count() const { return 0; }
+    __device__ __forceinline__ int diag_count() const { return 0; }
 };
+
+// Copies a per-pattern table into LDS for the stencil format (the caller
+// synchronises); the ELL format reads its global arrays directly.
+template <class Rows, class T>
+__device__ __forceinline__ const T* stage_table(const Rows&, const T* __restrict__ g, T* lds, int count)
+{
+    if constexpr (Rows::kStaged) {
+        for (int q = threadIdx.x; q < count; q += kBlock) lds[q] = g[q];
+        return lds;
+    } else {
+        (void)lds;
+        (void)count;
+        return g;
+    }
+}
 
 template <int N>
 __device__ __forceinline__ void wave_sum(double (&v)[N])

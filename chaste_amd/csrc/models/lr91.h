@@ -53,63 +53,57 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R slow_inward_current__P_si = R(0.09);
     const R i_stim = i_stim_area;
     (void)i_stim;
-    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.135) * exp(((R(80.0) + V) / (-R(6.8)))))) : (R(0.0)));
-    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(3.56) * exp((R(0.079) * V))) + (R(310000.0) * exp((R(0.35) * V))))) : ((R(1.0) / (R(0.13) * (R(1.0) + exp(((V + R(10.66)) / (-R(11.1)))))))));
-    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? ((((((-R(127140.0)) * exp((R(0.2444) * V))) - (R(3.474e-05) * exp(((-R(0.04391)) * V)))) * (V + R(37.78))) / (R(1.0) + exp((R(0.311) * (V + R(79.23))))))) : (R(0.0)));
-    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (((R(0.1212) * exp(((-R(0.01052)) * V))) / (R(1.0) + exp(((-R(0.1378)) * (V + R(40.14))))))) : (((R(0.3) * exp(((-R(2.535e-07)) * V))) / (R(1.0) + exp(((-R(0.1)) * (V + R(32.0))))))));
-    const R fast_sodium_current_m_gate__alpha_m = ((R(0.32) * (V + R(47.13))) / (R(1.0) - exp(((-R(0.1)) * (V + R(47.13))))));
-    const R fast_sodium_current_m_gate__beta_m = (R(0.08) * exp(((-V) / R(11.0))));
-    const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * log((y[7] / R(1.0)))));
+    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.135) * cmx_exp(((R(80.0) + V) * (R(1) / (-R(6.8))))))) : (R(0.0)));
+    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(3.56) * cmx_exp((R(0.079) * V))) + (R(310000.0) * cmx_exp((R(0.35) * V))))) : (cmx_div(R(1.0), (R(0.13) * (R(1.0) + cmx_exp(((V + R(10.66)) * (R(1) / (-R(11.1))))))))));
+    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(127140.0)) * cmx_exp((R(0.2444) * V))) - (R(3.474e-05) * cmx_exp(((-R(0.04391)) * V)))) * (V + R(37.78))), (R(1.0) + cmx_exp((R(0.311) * (V + R(79.23))))))) : (R(0.0)));
+    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.1212) * cmx_exp(((-R(0.01052)) * V))), (R(1.0) + cmx_exp(((-R(0.1378)) * (V + R(40.14))))))) : (cmx_div((R(0.3) * cmx_exp(((-R(2.535e-07)) * V))), (R(1.0) + cmx_exp(((-R(0.1)) * (V + R(32.0))))))));
+    const R fast_sodium_current_m_gate__alpha_m = cmx_div((R(0.32) * (V + R(47.13))), (R(1.0) - cmx_exp(((-R(0.1)) * (V + R(47.13))))));
+    const R fast_sodium_current_m_gate__beta_m = (R(0.08) * cmx_exp(((-V) * (R(1) / R(11.0)))));
+    const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((y[7] * (R(1) / R(1.0))))));
     const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
-    const R slow_inward_current_d_gate__alpha_d = ((R(0.095) * exp(((-R(0.01)) * (V - R(5.0))))) / (R(1.0) + exp(((-R(0.072)) * (V - R(5.0))))));
-    const R slow_inward_current_d_gate__beta_d = ((R(0.07) * exp(((-R(0.017)) * (V + R(44.0))))) / (R(1.0) + exp((R(0.05) * (V + R(44.0))))));
-    const R slow_inward_current_f_gate__alpha_f = ((R(0.012) * exp(((-R(0.008)) * (V + R(28.0))))) / (R(1.0) + exp((R(0.15) * (V + R(28.0))))));
-    const R slow_inward_current_f_gate__beta_f = ((R(0.0065) * exp(((-R(0.02)) * (V + R(30.0))))) / (R(1.0) + exp(((-R(0.2)) * (V + R(30.0))))));
-    const R time_dependent_potassium_current_X_gate__alpha_X = ((R(0.0005) * exp((R(0.083) * (V + R(50.0))))) / (R(1.0) + exp((R(0.057) * (V + R(50.0))))));
-    const R time_dependent_potassium_current_X_gate__beta_X = ((R(0.0013) * exp(((-R(0.06)) * (V + R(20.0))))) / (R(1.0) + exp(((-R(0.04)) * (V + R(20.0))))));
+    const R slow_inward_current_d_gate__alpha_d = cmx_div((R(0.095) * cmx_exp(((-R(0.01)) * (V - R(5.0))))), (R(1.0) + cmx_exp(((-R(0.072)) * (V - R(5.0))))));
+    const R slow_inward_current_d_gate__beta_d = cmx_div((R(0.07) * cmx_exp(((-R(0.017)) * (V + R(44.0))))), (R(1.0) + cmx_exp((R(0.05) * (V + R(44.0))))));
+    const R slow_inward_current_f_gate__alpha_f = cmx_div((R(0.012) * cmx_exp(((-R(0.008)) * (V + R(28.0))))), (R(1.0) + cmx_exp((R(0.15) * (V + R(28.0))))));
+    const R slow_inward_current_f_gate__beta_f = cmx_div((R(0.0065) * cmx_exp(((-R(0.02)) * (V + R(30.0))))), (R(1.0) + cmx_exp(((-R(0.2)) * (V + R(30.0))))));
+    const R time_dependent_potassium_current_X_gate__alpha_X = cmx_div((R(0.0005) * cmx_exp((R(0.083) * (V + R(50.0))))), (R(1.0) + cmx_exp((R(0.057) * (V + R(50.0))))));
+    const R time_dependent_potassium_current_X_gate__beta_X = cmx_div((R(0.0013) * cmx_exp(((-R(0.06)) * (V + R(20.0))))), (R(1.0) + cmx_exp(((-R(0.04)) * (V + R(20.0))))));
     if (RL) {
-        const R gk = ((fast_sodium_current_m_gate__alpha_m * (-R(1))) - (fast_sodium_current_m_gate__beta_m * R(1)));
-        g_k[1] = gk;
-        g_inf[1] = -((fast_sodium_current_m_gate__alpha_m * R(1.0))) / gk;
+        g_k[1] = ((fast_sodium_current_m_gate__alpha_m * (-R(1))) - (fast_sodium_current_m_gate__beta_m * R(1)));
+        g_inf[1] = cmx_div(-((fast_sodium_current_m_gate__alpha_m * R(1.0))), ((fast_sodium_current_m_gate__alpha_m * (-R(1))) - (fast_sodium_current_m_gate__beta_m * R(1))));
     } else {
         dy[1] = ((fast_sodium_current_m_gate__alpha_m * (R(1.0) - y[1])) - (fast_sodium_current_m_gate__beta_m * y[1]));
     }
     if (RL) {
-        const R gk = ((fast_sodium_current_h_gate__alpha_h * (-R(1))) - (fast_sodium_current_h_gate__beta_h * R(1)));
-        g_k[2] = gk;
-        g_inf[2] = -((fast_sodium_current_h_gate__alpha_h * R(1.0))) / gk;
+        g_k[2] = ((fast_sodium_current_h_gate__alpha_h * (-R(1))) - (fast_sodium_current_h_gate__beta_h * R(1)));
+        g_inf[2] = cmx_div(-((fast_sodium_current_h_gate__alpha_h * R(1.0))), ((fast_sodium_current_h_gate__alpha_h * (-R(1))) - (fast_sodium_current_h_gate__beta_h * R(1))));
     } else {
         dy[2] = ((fast_sodium_current_h_gate__alpha_h * (R(1.0) - y[2])) - (fast_sodium_current_h_gate__beta_h * y[2]));
     }
     if (RL) {
-        const R gk = ((fast_sodium_current_j_gate__alpha_j * (-R(1))) - (fast_sodium_current_j_gate__beta_j * R(1)));
-        g_k[3] = gk;
-        g_inf[3] = -((fast_sodium_current_j_gate__alpha_j * R(1.0))) / gk;
+        g_k[3] = ((fast_sodium_current_j_gate__alpha_j * (-R(1))) - (fast_sodium_current_j_gate__beta_j * R(1)));
+        g_inf[3] = cmx_div(-((fast_sodium_current_j_gate__alpha_j * R(1.0))), ((fast_sodium_current_j_gate__alpha_j * (-R(1))) - (fast_sodium_current_j_gate__beta_j * R(1))));
     } else {
         dy[3] = ((fast_sodium_current_j_gate__alpha_j * (R(1.0) - y[3])) - (fast_sodium_current_j_gate__beta_j * y[3]));
     }
     if (RL) {
-        const R gk = ((slow_inward_current_d_gate__alpha_d * (-R(1))) - (slow_inward_current_d_gate__beta_d * R(1)));
-        g_k[4] = gk;
-        g_inf[4] = -((slow_inward_current_d_gate__alpha_d * R(1.0))) / gk;
+        g_k[4] = ((slow_inward_current_d_gate__alpha_d * (-R(1))) - (slow_inward_current_d_gate__beta_d * R(1)));
+        g_inf[4] = cmx_div(-((slow_inward_current_d_gate__alpha_d * R(1.0))), ((slow_inward_current_d_gate__alpha_d * (-R(1))) - (slow_inward_current_d_gate__beta_d * R(1))));
     } else {
         dy[4] = ((slow_inward_current_d_gate__alpha_d * (R(1.0) - y[4])) - (slow_inward_current_d_gate__beta_d * y[4]));
     }
     if (RL) {
-        const R gk = ((slow_inward_current_f_gate__alpha_f * (-R(1))) - (slow_inward_current_f_gate__beta_f * R(1)));
-        g_k[5] = gk;
-        g_inf[5] = -((slow_inward_current_f_gate__alpha_f * R(1.0))) / gk;
+        g_k[5] = ((slow_inward_current_f_gate__alpha_f * (-R(1))) - (slow_inward_current_f_gate__beta_f * R(1)));
+        g_inf[5] = cmx_div(-((slow_inward_current_f_gate__alpha_f * R(1.0))), ((slow_inward_current_f_gate__alpha_f * (-R(1))) - (slow_inward_current_f_gate__beta_f * R(1))));
     } else {
         dy[5] = ((slow_inward_current_f_gate__alpha_f * (R(1.0) - y[5])) - (slow_inward_current_f_gate__beta_f * y[5]));
     }
     if (RL) {
-        const R gk = ((time_dependent_potassium_current_X_gate__alpha_X * (-R(1))) - (time_dependent_potassium_current_X_gate__beta_X * R(1)));
-        g_k[6] = gk;
-        g_inf[6] = -((time_dependent_potassium_current_X_gate__alpha_X * R(1.0))) / gk;
+        g_k[6] = ((time_dependent_potassium_current_X_gate__alpha_X * (-R(1))) - (time_dependent_potassium_current_X_gate__beta_X * R(1)));
+        g_inf[6] = cmx_div(-((time_dependent_potassium_current_X_gate__alpha_X * R(1.0))), ((time_dependent_potassium_current_X_gate__alpha_X * (-R(1))) - (time_dependent_potassium_current_X_gate__beta_X * R(1))));
     } else {
         dy[6] = ((time_dependent_potassium_current_X_gate__alpha_X * (R(1.0) - y[6])) - (time_dependent_potassium_current_X_gate__beta_X * y[6]));
     }
-    dy[7] = ((((-R(0.0001)) / R(1.0)) * slow_inward_current__i_si) + (R(0.07) * (R(0.0001) - y[7])));
+    dy[7] = ((((-R(0.0001)) * (R(1) / R(1.0))) * slow_inward_current__i_si) + (R(0.07) * (R(0.0001) - y[7])));
 }
 
 // dV/dt including the stimulus (single-cell integration).
@@ -136,24 +130,24 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     const R i_stim = i_stim_area;
     (void)i_stim;
     const R background_current__i_b = (background_current__g_b * (V - background_current__E_b));
-    const R fast_sodium_current__E_Na = (((membrane__R * membrane__T) / membrane__F) * log((ionic_concentrations__Nao / ionic_concentrations__Nai)));
+    const R fast_sodium_current__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log((ionic_concentrations__Nao * (R(1) / ionic_concentrations__Nai))));
     const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[1]) * y[2] * y[3] * (V - fast_sodium_current__E_Na));
-    const R time_independent_potassium_current__E_K1 = (((membrane__R * membrane__T) / membrane__F) * log((ionic_concentrations__Ko / ionic_concentrations__Ki)));
+    const R time_independent_potassium_current__E_K1 = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log((ionic_concentrations__Ko * (R(1) / ionic_concentrations__Ki))));
     const R plateau_potassium_current__E_Kp = time_independent_potassium_current__E_K1;
-    const R plateau_potassium_current__Kp = (R(1.0) / (R(1.0) + exp(((R(7.488) - V) / R(5.98)))));
+    const R plateau_potassium_current__Kp = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(7.488) - V) * (R(1) / R(5.98))))));
     const R plateau_potassium_current__i_Kp = (plateau_potassium_current__g_Kp * plateau_potassium_current__Kp * (V - plateau_potassium_current__E_Kp));
-    const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * log((y[7] / R(1.0)))));
+    const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((y[7] * (R(1) / R(1.0))))));
     const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
-    const R time_dependent_potassium_current__E_K = (((membrane__R * membrane__T) / membrane__F) * log(((ionic_concentrations__Ko + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nao)) / (ionic_concentrations__Ki + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nai)))));
-    const R time_dependent_potassium_current__g_K = (time_dependent_potassium_current__g_Kmax * sqrt((ionic_concentrations__Ko / R(5.4))));
-    const R time_dependent_potassium_current_Xi_gate__Xi = (((V > (-R(100.0)))) ? (((R(2.837) * (exp((R(0.04) * (V + R(77.0)))) - R(1.0))) / ((V + R(77.0)) * exp((R(0.04) * (V + R(35.0))))))) : (R(1.0)));
+    const R time_dependent_potassium_current__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(((ionic_concentrations__Ko + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nao)) * (R(1) / (ionic_concentrations__Ki + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nai))))));
+    const R time_dependent_potassium_current__g_K = (time_dependent_potassium_current__g_Kmax * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
+    const R time_dependent_potassium_current_Xi_gate__Xi = (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * (cmx_exp((R(0.04) * (V + R(77.0)))) - R(1.0))), ((V + R(77.0)) * cmx_exp((R(0.04) * (V + R(35.0))))))) : (R(1.0)));
     const R time_dependent_potassium_current__i_K = (time_dependent_potassium_current__g_K * y[6] * time_dependent_potassium_current_Xi_gate__Xi * (V - time_dependent_potassium_current__E_K));
-    const R time_independent_potassium_current__g_K1 = (R(0.6047) * sqrt((ionic_concentrations__Ko / R(5.4))));
-    const R time_independent_potassium_current_K1_gate__alpha_K1 = (R(1.02) / (R(1.0) + exp((R(0.2385) * ((V - time_independent_potassium_current__E_K1) - R(59.215))))));
-    const R time_independent_potassium_current_K1_gate__beta_K1 = (((R(0.49124) * exp((R(0.08032) * ((V + R(5.476)) - time_independent_potassium_current__E_K1)))) + (R(1.0) * exp((R(0.06175) * (V - (time_independent_potassium_current__E_K1 + R(594.31))))))) / (R(1.0) + exp(((-R(0.5143)) * ((V - time_independent_potassium_current__E_K1) + R(4.753))))));
-    const R time_independent_potassium_current_K1_gate__K1_infinity = (time_independent_potassium_current_K1_gate__alpha_K1 / (time_independent_potassium_current_K1_gate__alpha_K1 + time_independent_potassium_current_K1_gate__beta_K1));
+    const R time_independent_potassium_current__g_K1 = (R(0.6047) * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
+    const R time_independent_potassium_current_K1_gate__alpha_K1 = cmx_div(R(1.02), (R(1.0) + cmx_exp((R(0.2385) * ((V - time_independent_potassium_current__E_K1) - R(59.215))))));
+    const R time_independent_potassium_current_K1_gate__beta_K1 = cmx_div(((R(0.49124) * cmx_exp((R(0.08032) * ((V + R(5.476)) - time_independent_potassium_current__E_K1)))) + (R(1.0) * cmx_exp((R(0.06175) * (V - (time_independent_potassium_current__E_K1 + R(594.31))))))), (R(1.0) + cmx_exp(((-R(0.5143)) * ((V - time_independent_potassium_current__E_K1) + R(4.753))))));
+    const R time_independent_potassium_current_K1_gate__K1_infinity = cmx_div(time_independent_potassium_current_K1_gate__alpha_K1, (time_independent_potassium_current_K1_gate__alpha_K1 + time_independent_potassium_current_K1_gate__beta_K1));
     const R time_independent_potassium_current__i_K1 = (time_independent_potassium_current__g_K1 * time_independent_potassium_current_K1_gate__K1_infinity * (V - time_independent_potassium_current__E_K1));
-    return (((-R(1.0)) / membrane__C) * (i_stim + fast_sodium_current__i_Na + slow_inward_current__i_si + time_dependent_potassium_current__i_K + time_independent_potassium_current__i_K1 + plateau_potassium_current__i_Kp + background_current__i_b));
+    return (((-R(1.0)) * (R(1) / membrane__C)) * (i_stim + fast_sodium_current__i_Na + slow_inward_current__i_si + time_dependent_potassium_current__i_K + time_independent_potassium_current__i_K1 + plateau_potassium_current__i_Kp + background_current__i_b));
 }
 
 // Total ionic current in uA/cm^2 (GetIIonic): minus dV/dt at zero stimulus,
@@ -181,24 +175,24 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     const R i_stim = R(0);
     (void)i_stim;
     const R background_current__i_b = (background_current__g_b * (V - background_current__E_b));
-    const R fast_sodium_current__E_Na = (((membrane__R * membrane__T) / membrane__F) * log((ionic_concentrations__Nao / ionic_concentrations__Nai)));
+    const R fast_sodium_current__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log((ionic_concentrations__Nao * (R(1) / ionic_concentrations__Nai))));
     const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[1]) * y[2] * y[3] * (V - fast_sodium_current__E_Na));
-    const R time_independent_potassium_current__E_K1 = (((membrane__R * membrane__T) / membrane__F) * log((ionic_concentrations__Ko / ionic_concentrations__Ki)));
+    const R time_independent_potassium_current__E_K1 = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log((ionic_concentrations__Ko * (R(1) / ionic_concentrations__Ki))));
     const R plateau_potassium_current__E_Kp = time_independent_potassium_current__E_K1;
-    const R plateau_potassium_current__Kp = (R(1.0) / (R(1.0) + exp(((R(7.488) - V) / R(5.98)))));
+    const R plateau_potassium_current__Kp = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(7.488) - V) * (R(1) / R(5.98))))));
     const R plateau_potassium_current__i_Kp = (plateau_potassium_current__g_Kp * plateau_potassium_current__Kp * (V - plateau_potassium_current__E_Kp));
-    const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * log((y[7] / R(1.0)))));
+    const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((y[7] * (R(1) / R(1.0))))));
     const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
-    const R time_dependent_potassium_current__E_K = (((membrane__R * membrane__T) / membrane__F) * log(((ionic_concentrations__Ko + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nao)) / (ionic_concentrations__Ki + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nai)))));
-    const R time_dependent_potassium_current__g_K = (time_dependent_potassium_current__g_Kmax * sqrt((ionic_concentrations__Ko / R(5.4))));
-    const R time_dependent_potassium_current_Xi_gate__Xi = (((V > (-R(100.0)))) ? (((R(2.837) * (exp((R(0.04) * (V + R(77.0)))) - R(1.0))) / ((V + R(77.0)) * exp((R(0.04) * (V + R(35.0))))))) : (R(1.0)));
+    const R time_dependent_potassium_current__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(((ionic_concentrations__Ko + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nao)) * (R(1) / (ionic_concentrations__Ki + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nai))))));
+    const R time_dependent_potassium_current__g_K = (time_dependent_potassium_current__g_Kmax * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
+    const R time_dependent_potassium_current_Xi_gate__Xi = (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * (cmx_exp((R(0.04) * (V + R(77.0)))) - R(1.0))), ((V + R(77.0)) * cmx_exp((R(0.04) * (V + R(35.0))))))) : (R(1.0)));
     const R time_dependent_potassium_current__i_K = (time_dependent_potassium_current__g_K * y[6] * time_dependent_potassium_current_Xi_gate__Xi * (V - time_dependent_potassium_current__E_K));
-    const R time_independent_potassium_current__g_K1 = (R(0.6047) * sqrt((ionic_concentrations__Ko / R(5.4))));
-    const R time_independent_potassium_current_K1_gate__alpha_K1 = (R(1.02) / (R(1.0) + exp((R(0.2385) * ((V - time_independent_potassium_current__E_K1) - R(59.215))))));
-    const R time_independent_potassium_current_K1_gate__beta_K1 = (((R(0.49124) * exp((R(0.08032) * ((V + R(5.476)) - time_independent_potassium_current__E_K1)))) + (R(1.0) * exp((R(0.06175) * (V - (time_independent_potassium_current__E_K1 + R(594.31))))))) / (R(1.0) + exp(((-R(0.5143)) * ((V - time_independent_potassium_current__E_K1) + R(4.753))))));
-    const R time_independent_potassium_current_K1_gate__K1_infinity = (time_independent_potassium_current_K1_gate__alpha_K1 / (time_independent_potassium_current_K1_gate__alpha_K1 + time_independent_potassium_current_K1_gate__beta_K1));
+    const R time_independent_potassium_current__g_K1 = (R(0.6047) * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
+    const R time_independent_potassium_current_K1_gate__alpha_K1 = cmx_div(R(1.02), (R(1.0) + cmx_exp((R(0.2385) * ((V - time_independent_potassium_current__E_K1) - R(59.215))))));
+    const R time_independent_potassium_current_K1_gate__beta_K1 = cmx_div(((R(0.49124) * cmx_exp((R(0.08032) * ((V + R(5.476)) - time_independent_potassium_current__E_K1)))) + (R(1.0) * cmx_exp((R(0.06175) * (V - (time_independent_potassium_current__E_K1 + R(594.31))))))), (R(1.0) + cmx_exp(((-R(0.5143)) * ((V - time_independent_potassium_current__E_K1) + R(4.753))))));
+    const R time_independent_potassium_current_K1_gate__K1_infinity = cmx_div(time_independent_potassium_current_K1_gate__alpha_K1, (time_independent_potassium_current_K1_gate__alpha_K1 + time_independent_potassium_current_K1_gate__beta_K1));
     const R time_independent_potassium_current__i_K1 = (time_independent_potassium_current__g_K1 * time_independent_potassium_current_K1_gate__K1_infinity * (V - time_independent_potassium_current__E_K1));
-    const R dv = (((-R(1.0)) / membrane__C) * (i_stim + fast_sodium_current__i_Na + slow_inward_current__i_si + time_dependent_potassium_current__i_K + time_independent_potassium_current__i_K1 + plateau_potassium_current__i_Kp + background_current__i_b));
+    const R dv = (((-R(1.0)) * (R(1) / membrane__C)) * (i_stim + fast_sodium_current__i_Na + slow_inward_current__i_si + time_dependent_potassium_current__i_K + time_independent_potassium_current__i_K1 + plateau_potassium_current__i_Kp + background_current__i_b));
     return -dv * membrane__C;
 }
 

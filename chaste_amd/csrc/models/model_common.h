@@ -13,6 +13,22 @@
     using std::exp; using std::log; using std::log10; using std::pow; \
     using std::sqrt; using std::floor; using std::ceil; using std::fabs;
 
+// exp, log and division of the generated right-hand sides.  Double precision
+// keeps the correctly rounded library forms; single precision (the FP32 cell
+// path) uses the hardware exp2/log2/reciprocal sequences.
+CMX_HD inline double cmx_exp(double x) { return std::exp(x); }
+CMX_HD inline double cmx_log(double x) { return std::log(x); }
+CMX_HD inline double cmx_div(double a, double b) { return a / b; }
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline float cmx_exp(float x) { return __expf(x); }
+__device__ inline float cmx_log(float x) { return __logf(x); }
+__device__ inline float cmx_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+#else
+inline float cmx_exp(float x) { return std::exp(x); }
+inline float cmx_log(float x) { return std::log(x); }
+inline float cmx_div(float a, float b) { return a / b; }
+#endif
+
 // Integer powers as products: pow(x, 3.0) in the CellML becomes x*x*x.
 template <int N, class R>
 CMX_HD inline R cmx_ipow(R x)

@@ -126,159 +126,148 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = ((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) / (membrane__R * membrane__T)) * ((R(0.25) * y[15] * exp(((R(2.0) * (V - R(15.0)) * membrane__F) / (membrane__R * membrane__T)))) - calcium_dynamics__Ca_o)) / (exp(((R(2.0) * (V - R(15.0)) * membrane__F) / (membrane__R * membrane__T))) - R(1.0)));
-    const R L_type_Ca_current_d_gate__d_inf = (R(1.0) / (R(1.0) + exp((((-R(8.0)) - V) / R(7.5)))));
-    const R L_type_Ca_current_d_gate__alpha_d = ((R(1.4) / (R(1.0) + exp((((-R(35.0)) - V) / R(13.0))))) + R(0.25));
-    const R L_type_Ca_current_d_gate__beta_d = (R(1.4) / (R(1.0) + exp(((V + R(5.0)) / R(5.0)))));
-    const R L_type_Ca_current_d_gate__gamma_d = (R(1.0) / (R(1.0) + exp(((R(50.0) - V) / R(20.0)))));
+    const R L_type_Ca_current__i_CaL = cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), (cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T)))) - R(1.0)));
+    const R L_type_Ca_current_d_gate__d_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(8.0)) - V) * (R(1) / R(7.5))))));
+    const R L_type_Ca_current_d_gate__alpha_d = (cmx_div(R(1.4), (R(1.0) + cmx_exp((((-R(35.0)) - V) * (R(1) / R(13.0)))))) + R(0.25));
+    const R L_type_Ca_current_d_gate__beta_d = cmx_div(R(1.4), (R(1.0) + cmx_exp(((V + R(5.0)) * (R(1) / R(5.0))))));
+    const R L_type_Ca_current_d_gate__gamma_d = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(50.0) - V) * (R(1) / R(20.0))))));
     const R L_type_Ca_current_d_gate__tau_d = ((L_type_Ca_current_d_gate__alpha_d * L_type_Ca_current_d_gate__beta_d) + L_type_Ca_current_d_gate__gamma_d);
-    const R L_type_Ca_current_f2_gate__f2_inf = ((R(0.67) / (R(1.0) + exp(((V + R(35.0)) / R(7.0))))) + R(0.33));
-    const R L_type_Ca_current_f2_gate__tau_f2 = ((R(562.0) * exp(((-cmx_ipow<2>((V + R(27.0)))) / R(240.0)))) + (R(31.0) / (R(1.0) + exp(((R(25.0) - V) / R(10.0))))) + (R(80.0) / (R(1.0) + exp(((V + R(30.0)) / R(10.0))))));
-    const R L_type_Ca_current_fCass_gate__fCass_inf = ((R(0.6) / (R(1.0) + cmx_ipow<2>((y[15] / R(0.05))))) + R(0.4));
-    const R L_type_Ca_current_fCass_gate__tau_fCass = ((R(80.0) / (R(1.0) + cmx_ipow<2>((y[15] / R(0.05))))) + R(2.0));
-    const R L_type_Ca_current_f_gate__f_inf = (R(1.0) / (R(1.0) + exp(((V + R(20.0)) / R(7.0)))));
-    const R L_type_Ca_current_f_gate__tau_f = ((R(1102.5) * exp(((-cmx_ipow<2>((V + R(27.0)))) / R(225.0)))) + (R(200.0) / (R(1.0) + exp(((R(13.0) - V) / R(10.0))))) + (R(180.0) / (R(1.0) + exp(((V + R(30.0)) / R(10.0))))) + R(20.0));
-    const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) / membrane__F) * log((calcium_dynamics__Ca_o / y[13])));
+    const R L_type_Ca_current_f2_gate__f2_inf = (cmx_div(R(0.67), (R(1.0) + cmx_exp(((V + R(35.0)) * (R(1) / R(7.0)))))) + R(0.33));
+    const R L_type_Ca_current_f2_gate__tau_f2 = ((R(562.0) * cmx_exp(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(240.0))))) + cmx_div(R(31.0), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(10.0)))))) + cmx_div(R(80.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(10.0)))))));
+    const R L_type_Ca_current_fCass_gate__fCass_inf = (cmx_div(R(0.6), (R(1.0) + cmx_ipow<2>((y[15] * (R(1) / R(0.05)))))) + R(0.4));
+    const R L_type_Ca_current_fCass_gate__tau_fCass = (cmx_div(R(80.0), (R(1.0) + cmx_ipow<2>((y[15] * (R(1) / R(0.05)))))) + R(2.0));
+    const R L_type_Ca_current_f_gate__f_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(20.0)) * (R(1) / R(7.0))))));
+    const R L_type_Ca_current_f_gate__tau_f = ((R(1102.5) * cmx_exp(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(225.0))))) + cmx_div(R(200.0), (R(1.0) + cmx_exp(((R(13.0) - V) * (R(1) / R(10.0)))))) + cmx_div(R(180.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(10.0)))))) + R(20.0));
+    const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, y[13])));
     const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
-    const R calcium_dynamics__Ca_i_bufc = (R(1.0) / (R(1.0) + ((calcium_dynamics__Buf_c * calcium_dynamics__K_buf_c) / cmx_ipow<2>((y[13] + calcium_dynamics__K_buf_c)))));
-    const R calcium_dynamics__Ca_sr_bufsr = (R(1.0) / (R(1.0) + ((calcium_dynamics__Buf_sr * calcium_dynamics__K_buf_sr) / cmx_ipow<2>((y[14] + calcium_dynamics__K_buf_sr)))));
-    const R calcium_dynamics__Ca_ss_bufss = (R(1.0) / (R(1.0) + ((calcium_dynamics__Buf_ss * calcium_dynamics__K_buf_ss) / cmx_ipow<2>((y[15] + calcium_dynamics__K_buf_ss)))));
+    const R calcium_dynamics__Ca_i_bufc = cmx_div(R(1.0), (R(1.0) + cmx_div((calcium_dynamics__Buf_c * calcium_dynamics__K_buf_c), cmx_ipow<2>((y[13] + calcium_dynamics__K_buf_c)))));
+    const R calcium_dynamics__Ca_sr_bufsr = cmx_div(R(1.0), (R(1.0) + cmx_div((calcium_dynamics__Buf_sr * calcium_dynamics__K_buf_sr), cmx_ipow<2>((y[14] + calcium_dynamics__K_buf_sr)))));
+    const R calcium_dynamics__Ca_ss_bufss = cmx_div(R(1.0), (R(1.0) + cmx_div((calcium_dynamics__Buf_ss * calcium_dynamics__K_buf_ss), cmx_ipow<2>((y[15] + calcium_dynamics__K_buf_ss)))));
     const R calcium_dynamics__i_leak = (calcium_dynamics__V_leak * (y[14] - y[13]));
-    const R calcium_dynamics__kcasr = (calcium_dynamics__max_sr - ((calcium_dynamics__max_sr - calcium_dynamics__min_sr) / (R(1.0) + cmx_ipow<2>((calcium_dynamics__EC / y[14])))));
-    const R calcium_dynamics__k1 = (calcium_dynamics__k1_prime / calcium_dynamics__kcasr);
-    const R calcium_dynamics__O = ((calcium_dynamics__k1 * cmx_ipow<2>(y[15]) * y[16]) / (calcium_dynamics__k3 + (calcium_dynamics__k1 * cmx_ipow<2>(y[15]))));
+    const R calcium_dynamics__kcasr = (calcium_dynamics__max_sr - cmx_div((calcium_dynamics__max_sr - calcium_dynamics__min_sr), (R(1.0) + cmx_ipow<2>(cmx_div(calcium_dynamics__EC, y[14])))));
+    const R calcium_dynamics__k1 = cmx_div(calcium_dynamics__k1_prime, calcium_dynamics__kcasr);
+    const R calcium_dynamics__O = cmx_div((calcium_dynamics__k1 * cmx_ipow<2>(y[15]) * y[16]), (calcium_dynamics__k3 + (calcium_dynamics__k1 * cmx_ipow<2>(y[15]))));
     const R calcium_dynamics__i_rel = (calcium_dynamics__V_rel * calcium_dynamics__O * (y[14] - y[15]));
-    const R calcium_dynamics__i_up = (calcium_dynamics__Vmax_up / (R(1.0) + (cmx_ipow<2>(calcium_dynamics__K_up) / cmx_ipow<2>(y[13]))));
+    const R calcium_dynamics__i_up = cmx_div(calcium_dynamics__Vmax_up, (R(1.0) + cmx_div(cmx_ipow<2>(calcium_dynamics__K_up), cmx_ipow<2>(y[13]))));
     const R calcium_dynamics__i_xfer = (calcium_dynamics__V_xfer * (y[15] - y[13]));
     const R calcium_dynamics__k2 = (calcium_dynamics__k2_prime * calcium_dynamics__kcasr);
-    const R calcium_pump_current__i_p_Ca = ((calcium_pump_current__g_pCa * y[13]) / (y[13] + calcium_pump_current__K_pCa));
-    const R reversal_potentials__E_Na = (((membrane__R * membrane__T) / membrane__F) * log((sodium_dynamics__Na_o / y[17])));
+    const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * y[13]), (y[13] + calcium_pump_current__K_pCa));
+    const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, y[17])));
     const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
-    const R fast_sodium_current_h_gate__h_inf = (R(1.0) / cmx_ipow<2>((R(1.0) + exp(((V + R(71.55)) / R(7.43))))));
-    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.057) * exp(((-(V + R(80.0))) / R(6.8))))) : (R(0.0)));
-    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(2.7) * exp((R(0.079) * V))) + (R(310000.0) * exp((R(0.3485) * V))))) : ((R(0.77) / (R(0.13) * (R(1.0) + exp(((V + R(10.66)) / (-R(11.1)))))))));
-    const R fast_sodium_current_h_gate__tau_h = (R(1.0) / (fast_sodium_current_h_gate__alpha_h + fast_sodium_current_h_gate__beta_h));
-    const R fast_sodium_current_j_gate__j_inf = (R(1.0) / cmx_ipow<2>((R(1.0) + exp(((V + R(71.55)) / R(7.43))))));
-    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? ((((((-R(25428.0)) * exp((R(0.2444) * V))) - (R(6.948e-06) * exp(((-R(0.04391)) * V)))) * (V + R(37.78))) / (R(1.0) + exp((R(0.311) * (V + R(79.23))))))) : (R(0.0)));
-    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (((R(0.02424) * exp(((-R(0.01052)) * V))) / (R(1.0) + exp(((-R(0.1378)) * (V + R(40.14))))))) : (((R(0.6) * exp((R(0.057) * V))) / (R(1.0) + exp(((-R(0.1)) * (V + R(32.0))))))));
-    const R fast_sodium_current_j_gate__tau_j = (R(1.0) / (fast_sodium_current_j_gate__alpha_j + fast_sodium_current_j_gate__beta_j));
-    const R fast_sodium_current_m_gate__m_inf = (R(1.0) / cmx_ipow<2>((R(1.0) + exp((((-R(56.86)) - V) / R(9.03))))));
-    const R fast_sodium_current_m_gate__alpha_m = (R(1.0) / (R(1.0) + exp((((-R(60.0)) - V) / R(5.0)))));
-    const R fast_sodium_current_m_gate__beta_m = ((R(0.1) / (R(1.0) + exp(((V + R(35.0)) / R(5.0))))) + (R(0.1) / (R(1.0) + exp(((V - R(50.0)) / R(200.0))))));
+    const R fast_sodium_current_h_gate__h_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp(((V + R(71.55)) * (R(1) / R(7.43)))))));
+    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.057) * cmx_exp(((-(V + R(80.0))) * (R(1) / R(6.8)))))) : (R(0.0)));
+    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(2.7) * cmx_exp((R(0.079) * V))) + (R(310000.0) * cmx_exp((R(0.3485) * V))))) : (cmx_div(R(0.77), (R(0.13) * (R(1.0) + cmx_exp(((V + R(10.66)) * (R(1) / (-R(11.1))))))))));
+    const R fast_sodium_current_h_gate__tau_h = cmx_div(R(1.0), (fast_sodium_current_h_gate__alpha_h + fast_sodium_current_h_gate__beta_h));
+    const R fast_sodium_current_j_gate__j_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp(((V + R(71.55)) * (R(1) / R(7.43)))))));
+    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(25428.0)) * cmx_exp((R(0.2444) * V))) - (R(6.948e-06) * cmx_exp(((-R(0.04391)) * V)))) * (V + R(37.78))), (R(1.0) + cmx_exp((R(0.311) * (V + R(79.23))))))) : (R(0.0)));
+    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.02424) * cmx_exp(((-R(0.01052)) * V))), (R(1.0) + cmx_exp(((-R(0.1378)) * (V + R(40.14))))))) : (cmx_div((R(0.6) * cmx_exp((R(0.057) * V))), (R(1.0) + cmx_exp(((-R(0.1)) * (V + R(32.0))))))));
+    const R fast_sodium_current_j_gate__tau_j = cmx_div(R(1.0), (fast_sodium_current_j_gate__alpha_j + fast_sodium_current_j_gate__beta_j));
+    const R fast_sodium_current_m_gate__m_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp((((-R(56.86)) - V) * (R(1) / R(9.03)))))));
+    const R fast_sodium_current_m_gate__alpha_m = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(60.0)) - V) * (R(1) / R(5.0))))));
+    const R fast_sodium_current_m_gate__beta_m = (cmx_div(R(0.1), (R(1.0) + cmx_exp(((V + R(35.0)) * (R(1) / R(5.0)))))) + cmx_div(R(0.1), (R(1.0) + cmx_exp(((V - R(50.0)) * (R(1) / R(200.0)))))));
     const R fast_sodium_current_m_gate__tau_m = (fast_sodium_current_m_gate__alpha_m * fast_sodium_current_m_gate__beta_m);
-    const R reversal_potentials__E_K = (((membrane__R * membrane__T) / membrane__F) * log((potassium_dynamics__K_o / y[18])));
-    const R inward_rectifier_potassium_current__alpha_K1 = (R(0.1) / (R(1.0) + exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
-    const R inward_rectifier_potassium_current__beta_K1 = (((R(3.0) * exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))) / (R(1.0) + exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
-    const R inward_rectifier_potassium_current__xK1_inf = (inward_rectifier_potassium_current__alpha_K1 / (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
+    const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, y[18])));
+    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
+    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
+    const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
     const R inward_rectifier_potassium_current__i_K1 = (inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
-    const R potassium_pump_current__i_p_K = ((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)) / (R(1.0) + exp(((R(25.0) - V) / R(5.98)))));
-    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o / R(5.4))) * y[1] * y[2] * (V - reversal_potentials__E_K));
-    const R rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 = (R(450.0) / (R(1.0) + exp((((-R(45.0)) - V) / R(10.0)))));
-    const R rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1 = (R(6.0) / (R(1.0) + exp(((V + R(30.0)) / R(11.5)))));
+    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
+    const R rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 = cmx_div(R(450.0), (R(1.0) + cmx_exp((((-R(45.0)) - V) * (R(1) / R(10.0))))));
+    const R rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1 = cmx_div(R(6.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(11.5))))));
     const R rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1 = (rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 * rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1);
-    const R rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf = (R(1.0) / (R(1.0) + exp((((-R(26.0)) - V) / R(7.0)))));
-    const R rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 = (R(3.0) / (R(1.0) + exp((((-R(60.0)) - V) / R(20.0)))));
-    const R rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2 = (R(1.12) / (R(1.0) + exp(((V - R(60.0)) / R(20.0)))));
+    const R rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(26.0)) - V) * (R(1) / R(7.0))))));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 = cmx_div(R(3.0), (R(1.0) + cmx_exp((((-R(60.0)) - V) * (R(1) / R(20.0))))));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2 = cmx_div(R(1.12), (R(1.0) + cmx_exp(((V - R(60.0)) * (R(1) / R(20.0))))));
     const R rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2 = (rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 * rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2);
-    const R rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf = (R(1.0) / (R(1.0) + exp(((V + R(88.0)) / R(24.0)))));
-    const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) / membrane__F) * log(((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)) / (y[18] + (reversal_potentials__P_kna * y[17])))));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(88.0)) * (R(1) / R(24.0))))));
+    const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (y[18] + (reversal_potentials__P_kna * y[17])))));
     const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
-    const R slow_time_dependent_potassium_current_Xs_gate__alpha_xs = (R(1400.0) / sqrt((R(1.0) + exp(((R(5.0) - V) / R(6.0))))));
-    const R slow_time_dependent_potassium_current_Xs_gate__beta_xs = (R(1.0) / (R(1.0) + exp(((V - R(35.0)) / R(15.0)))));
+    const R slow_time_dependent_potassium_current_Xs_gate__alpha_xs = cmx_div(R(1400.0), sqrt((R(1.0) + cmx_exp(((R(5.0) - V) * (R(1) / R(6.0)))))));
+    const R slow_time_dependent_potassium_current_Xs_gate__beta_xs = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V - R(35.0)) * (R(1) / R(15.0))))));
     const R slow_time_dependent_potassium_current_Xs_gate__tau_xs = ((slow_time_dependent_potassium_current_Xs_gate__alpha_xs * slow_time_dependent_potassium_current_Xs_gate__beta_xs) + R(80.0));
-    const R slow_time_dependent_potassium_current_Xs_gate__xs_inf = (R(1.0) / (R(1.0) + exp((((-R(5.0)) - V) / R(14.0)))));
+    const R slow_time_dependent_potassium_current_Xs_gate__xs_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(5.0)) - V) * (R(1) / R(14.0))))));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
-    const R sodium_calcium_exchanger_current__i_NaCa = ((sodium_calcium_exchanger_current__K_NaCa * ((exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) / (membrane__R * membrane__T))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) / (membrane__R * membrane__T))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))) / ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) / (membrane__R * membrane__T)))))));
-    const R sodium_potassium_pump_current__i_NaK = (((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk)) * y[17]) / (y[17] + sodium_potassium_pump_current__K_mNa)) / (R(1.0) + (R(0.1245) * exp((((-R(0.1)) * V * membrane__F) / (membrane__R * membrane__T)))) + (R(0.0353) * exp((((-V) * membrane__F) / (membrane__R * membrane__T))))));
+    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
+    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
     const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11] * (V - reversal_potentials__E_K));
-    const R transient_outward_current_r_gate__r_inf = (R(1.0) / (R(1.0) + exp(((R(20.0) - V) / R(6.0)))));
-    const R transient_outward_current_r_gate__tau_r = ((R(9.5) * exp(((-cmx_ipow<2>((V + R(40.0)))) / R(1800.0)))) + R(0.8));
-    const R transient_outward_current_s_gate__s_inf = (R(1.0) / (R(1.0) + exp(((V + R(20.0)) / R(5.0)))));
-    const R transient_outward_current_s_gate__tau_s = ((R(85.0) * exp(((-cmx_ipow<2>((V + R(45.0)))) / R(320.0)))) + (R(5.0) / (R(1.0) + exp(((V - R(20.0)) / R(5.0))))) + R(3.0));
+    const R transient_outward_current_r_gate__r_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(20.0) - V) * (R(1) / R(6.0))))));
+    const R transient_outward_current_r_gate__tau_r = ((R(9.5) * cmx_exp(((-cmx_ipow<2>((V + R(40.0)))) * (R(1) / R(1800.0))))) + R(0.8));
+    const R transient_outward_current_s_gate__s_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(20.0)) * (R(1) / R(5.0))))));
+    const R transient_outward_current_s_gate__tau_s = ((R(85.0) * cmx_exp(((-cmx_ipow<2>((V + R(45.0)))) * (R(1) / R(320.0))))) + cmx_div(R(5.0), (R(1.0) + cmx_exp(((V - R(20.0)) * (R(1) / R(5.0)))))) + R(3.0));
     if (RL) {
-        const R gk = ((-R(1)) / rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1);
-        g_k[1] = gk;
-        g_inf[1] = -((rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf / rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1)) / gk;
+        g_k[1] = cmx_div(R(-1), rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1);
+        g_inf[1] = rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf;
     } else {
-        dy[1] = ((rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf - y[1]) / rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1);
+        dy[1] = cmx_div((rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf - y[1]), rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1);
     }
     if (RL) {
-        const R gk = ((-R(1)) / rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2);
-        g_k[2] = gk;
-        g_inf[2] = -((rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf / rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2)) / gk;
+        g_k[2] = cmx_div(R(-1), rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2);
+        g_inf[2] = rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf;
     } else {
-        dy[2] = ((rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf - y[2]) / rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2);
+        dy[2] = cmx_div((rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf - y[2]), rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2);
     }
     if (RL) {
-        const R gk = ((-R(1)) / slow_time_dependent_potassium_current_Xs_gate__tau_xs);
-        g_k[3] = gk;
-        g_inf[3] = -((slow_time_dependent_potassium_current_Xs_gate__xs_inf / slow_time_dependent_potassium_current_Xs_gate__tau_xs)) / gk;
+        g_k[3] = cmx_div(R(-1), slow_time_dependent_potassium_current_Xs_gate__tau_xs);
+        g_inf[3] = slow_time_dependent_potassium_current_Xs_gate__xs_inf;
     } else {
-        dy[3] = ((slow_time_dependent_potassium_current_Xs_gate__xs_inf - y[3]) / slow_time_dependent_potassium_current_Xs_gate__tau_xs);
+        dy[3] = cmx_div((slow_time_dependent_potassium_current_Xs_gate__xs_inf - y[3]), slow_time_dependent_potassium_current_Xs_gate__tau_xs);
     }
     if (RL) {
-        const R gk = ((-R(1)) / fast_sodium_current_m_gate__tau_m);
-        g_k[4] = gk;
-        g_inf[4] = -((fast_sodium_current_m_gate__m_inf / fast_sodium_current_m_gate__tau_m)) / gk;
+        g_k[4] = cmx_div(R(-1), fast_sodium_current_m_gate__tau_m);
+        g_inf[4] = fast_sodium_current_m_gate__m_inf;
     } else {
-        dy[4] = ((fast_sodium_current_m_gate__m_inf - y[4]) / fast_sodium_current_m_gate__tau_m);
+        dy[4] = cmx_div((fast_sodium_current_m_gate__m_inf - y[4]), fast_sodium_current_m_gate__tau_m);
     }
     if (RL) {
-        const R gk = ((-R(1)) / fast_sodium_current_h_gate__tau_h);
-        g_k[5] = gk;
-        g_inf[5] = -((fast_sodium_current_h_gate__h_inf / fast_sodium_current_h_gate__tau_h)) / gk;
+        g_k[5] = cmx_div(R(-1), fast_sodium_current_h_gate__tau_h);
+        g_inf[5] = fast_sodium_current_h_gate__h_inf;
     } else {
-        dy[5] = ((fast_sodium_current_h_gate__h_inf - y[5]) / fast_sodium_current_h_gate__tau_h);
+        dy[5] = cmx_div((fast_sodium_current_h_gate__h_inf - y[5]), fast_sodium_current_h_gate__tau_h);
     }
     if (RL) {
-        const R gk = ((-R(1)) / fast_sodium_current_j_gate__tau_j);
-        g_k[6] = gk;
-        g_inf[6] = -((fast_sodium_current_j_gate__j_inf / fast_sodium_current_j_gate__tau_j)) / gk;
+        g_k[6] = cmx_div(R(-1), fast_sodium_current_j_gate__tau_j);
+        g_inf[6] = fast_sodium_current_j_gate__j_inf;
     } else {
-        dy[6] = ((fast_sodium_current_j_gate__j_inf - y[6]) / fast_sodium_current_j_gate__tau_j);
+        dy[6] = cmx_div((fast_sodium_current_j_gate__j_inf - y[6]), fast_sodium_current_j_gate__tau_j);
     }
     if (RL) {
-        const R gk = ((-R(1)) / L_type_Ca_current_d_gate__tau_d);
-        g_k[7] = gk;
-        g_inf[7] = -((L_type_Ca_current_d_gate__d_inf / L_type_Ca_current_d_gate__tau_d)) / gk;
+        g_k[7] = cmx_div(R(-1), L_type_Ca_current_d_gate__tau_d);
+        g_inf[7] = L_type_Ca_current_d_gate__d_inf;
     } else {
-        dy[7] = ((L_type_Ca_current_d_gate__d_inf - y[7]) / L_type_Ca_current_d_gate__tau_d);
+        dy[7] = cmx_div((L_type_Ca_current_d_gate__d_inf - y[7]), L_type_Ca_current_d_gate__tau_d);
     }
     if (RL) {
-        const R gk = ((-R(1)) / L_type_Ca_current_f_gate__tau_f);
-        g_k[8] = gk;
-        g_inf[8] = -((L_type_Ca_current_f_gate__f_inf / L_type_Ca_current_f_gate__tau_f)) / gk;
+        g_k[8] = cmx_div(R(-1), L_type_Ca_current_f_gate__tau_f);
+        g_inf[8] = L_type_Ca_current_f_gate__f_inf;
     } else {
-        dy[8] = ((L_type_Ca_current_f_gate__f_inf - y[8]) / L_type_Ca_current_f_gate__tau_f);
+        dy[8] = cmx_div((L_type_Ca_current_f_gate__f_inf - y[8]), L_type_Ca_current_f_gate__tau_f);
     }
     if (RL) {
-        const R gk = ((-R(1)) / L_type_Ca_current_f2_gate__tau_f2);
-        g_k[9] = gk;
-        g_inf[9] = -((L_type_Ca_current_f2_gate__f2_inf / L_type_Ca_current_f2_gate__tau_f2)) / gk;
+        g_k[9] = cmx_div(R(-1), L_type_Ca_current_f2_gate__tau_f2);
+        g_inf[9] = L_type_Ca_current_f2_gate__f2_inf;
     } else {
-        dy[9] = ((L_type_Ca_current_f2_gate__f2_inf - y[9]) / L_type_Ca_current_f2_gate__tau_f2);
+        dy[9] = cmx_div((L_type_Ca_current_f2_gate__f2_inf - y[9]), L_type_Ca_current_f2_gate__tau_f2);
     }
-    dy[10] = ((L_type_Ca_current_fCass_gate__fCass_inf - y[10]) / L_type_Ca_current_fCass_gate__tau_fCass);
+    dy[10] = cmx_div((L_type_Ca_current_fCass_gate__fCass_inf - y[10]), L_type_Ca_current_fCass_gate__tau_fCass);
     if (RL) {
-        const R gk = ((-R(1)) / transient_outward_current_s_gate__tau_s);
-        g_k[11] = gk;
-        g_inf[11] = -((transient_outward_current_s_gate__s_inf / transient_outward_current_s_gate__tau_s)) / gk;
+        g_k[11] = cmx_div(R(-1), transient_outward_current_s_gate__tau_s);
+        g_inf[11] = transient_outward_current_s_gate__s_inf;
     } else {
-        dy[11] = ((transient_outward_current_s_gate__s_inf - y[11]) / transient_outward_current_s_gate__tau_s);
+        dy[11] = cmx_div((transient_outward_current_s_gate__s_inf - y[11]), transient_outward_current_s_gate__tau_s);
     }
     if (RL) {
-        const R gk = ((-R(1)) / transient_outward_current_r_gate__tau_r);
-        g_k[12] = gk;
-        g_inf[12] = -((transient_outward_current_r_gate__r_inf / transient_outward_current_r_gate__tau_r)) / gk;
+        g_k[12] = cmx_div(R(-1), transient_outward_current_r_gate__tau_r);
+        g_inf[12] = transient_outward_current_r_gate__r_inf;
     } else {
-        dy[12] = ((transient_outward_current_r_gate__r_inf - y[12]) / transient_outward_current_r_gate__tau_r);
+        dy[12] = cmx_div((transient_outward_current_r_gate__r_inf - y[12]), transient_outward_current_r_gate__tau_r);
     }
-    dy[13] = (calcium_dynamics__Ca_i_bufc * (((((calcium_dynamics__i_leak - calcium_dynamics__i_up) * calcium_dynamics__V_sr) / membrane__V_c) + calcium_dynamics__i_xfer) - ((((calcium_background_current__i_b_Ca + calcium_pump_current__i_p_Ca) - (R(2.0) * sodium_calcium_exchanger_current__i_NaCa)) * membrane__Cm) / (R(2.0) * membrane__V_c * membrane__F))));
+    dy[13] = (calcium_dynamics__Ca_i_bufc * (((((calcium_dynamics__i_leak - calcium_dynamics__i_up) * calcium_dynamics__V_sr) * (R(1) / membrane__V_c)) + calcium_dynamics__i_xfer) - ((((calcium_background_current__i_b_Ca + calcium_pump_current__i_p_Ca) - (R(2.0) * sodium_calcium_exchanger_current__i_NaCa)) * membrane__Cm) * (R(1) / (R(2.0) * membrane__V_c * membrane__F)))));
     dy[14] = (calcium_dynamics__Ca_sr_bufsr * (calcium_dynamics__i_up - (calcium_dynamics__i_rel + calcium_dynamics__i_leak)));
-    dy[15] = (calcium_dynamics__Ca_ss_bufss * (((((-L_type_Ca_current__i_CaL) * membrane__Cm) / (R(2.0) * calcium_dynamics__V_ss * membrane__F)) + ((calcium_dynamics__i_rel * calcium_dynamics__V_sr) / calcium_dynamics__V_ss)) - ((calcium_dynamics__i_xfer * membrane__V_c) / calcium_dynamics__V_ss)));
+    dy[15] = (calcium_dynamics__Ca_ss_bufss * (((((-L_type_Ca_current__i_CaL) * membrane__Cm) * (R(1) / (R(2.0) * calcium_dynamics__V_ss * membrane__F))) + ((calcium_dynamics__i_rel * calcium_dynamics__V_sr) * (R(1) / calcium_dynamics__V_ss))) - ((calcium_dynamics__i_xfer * membrane__V_c) * (R(1) / calcium_dynamics__V_ss))));
     dy[16] = (((-calcium_dynamics__k2) * y[15] * y[16]) + (calcium_dynamics__k4 * (R(1.0) - y[16])));
-    dy[17] = (((-(fast_sodium_current__i_Na + sodium_background_current__i_b_Na + (R(3.0) * sodium_potassium_pump_current__i_NaK) + (R(3.0) * sodium_calcium_exchanger_current__i_NaCa))) / (membrane__V_c * membrane__F)) * membrane__Cm);
-    dy[18] = (((-((inward_rectifier_potassium_current__i_K1 + transient_outward_current__i_to + rapid_time_dependent_potassium_current__i_Kr + slow_time_dependent_potassium_current__i_Ks + potassium_pump_current__i_p_K + i_stim) - (R(2.0) * sodium_potassium_pump_current__i_NaK))) / (membrane__V_c * membrane__F)) * membrane__Cm);
+    dy[17] = (((-(fast_sodium_current__i_Na + sodium_background_current__i_b_Na + (R(3.0) * sodium_potassium_pump_current__i_NaK) + (R(3.0) * sodium_calcium_exchanger_current__i_NaCa))) * (R(1) / (membrane__V_c * membrane__F))) * membrane__Cm);
+    dy[18] = (((-((inward_rectifier_potassium_current__i_K1 + transient_outward_current__i_to + rapid_time_dependent_potassium_current__i_Kr + slow_time_dependent_potassium_current__i_Ks + potassium_pump_current__i_p_K + i_stim) - (R(2.0) * sodium_potassium_pump_current__i_NaK))) * (R(1) / (membrane__V_c * membrane__F))) * membrane__Cm);
 }
 
 // dV/dt including the stimulus (single-cell integration).
@@ -319,26 +308,26 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = ((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) / (membrane__R * membrane__T)) * ((R(0.25) * y[15] * exp(((R(2.0) * (V - R(15.0)) * membrane__F) / (membrane__R * membrane__T)))) - calcium_dynamics__Ca_o)) / (exp(((R(2.0) * (V - R(15.0)) * membrane__F) / (membrane__R * membrane__T))) - R(1.0)));
-    const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) / membrane__F) * log((calcium_dynamics__Ca_o / y[13])));
+    const R L_type_Ca_current__i_CaL = cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), (cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T)))) - R(1.0)));
+    const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, y[13])));
     const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
-    const R calcium_pump_current__i_p_Ca = ((calcium_pump_current__g_pCa * y[13]) / (y[13] + calcium_pump_current__K_pCa));
-    const R reversal_potentials__E_Na = (((membrane__R * membrane__T) / membrane__F) * log((sodium_dynamics__Na_o / y[17])));
+    const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * y[13]), (y[13] + calcium_pump_current__K_pCa));
+    const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, y[17])));
     const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
-    const R reversal_potentials__E_K = (((membrane__R * membrane__T) / membrane__F) * log((potassium_dynamics__K_o / y[18])));
-    const R inward_rectifier_potassium_current__alpha_K1 = (R(0.1) / (R(1.0) + exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
-    const R inward_rectifier_potassium_current__beta_K1 = (((R(3.0) * exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))) / (R(1.0) + exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
-    const R inward_rectifier_potassium_current__xK1_inf = (inward_rectifier_potassium_current__alpha_K1 / (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
+    const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, y[18])));
+    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
+    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
+    const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
     const R inward_rectifier_potassium_current__i_K1 = (inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
-    const R potassium_pump_current__i_p_K = ((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)) / (R(1.0) + exp(((R(25.0) - V) / R(5.98)))));
-    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o / R(5.4))) * y[1] * y[2] * (V - reversal_potentials__E_K));
-    const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) / membrane__F) * log(((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)) / (y[18] + (reversal_potentials__P_kna * y[17])))));
+    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
+    const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (y[18] + (reversal_potentials__P_kna * y[17])))));
     const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
-    const R sodium_calcium_exchanger_current__i_NaCa = ((sodium_calcium_exchanger_current__K_NaCa * ((exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) / (membrane__R * membrane__T))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) / (membrane__R * membrane__T))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))) / ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) / (membrane__R * membrane__T)))))));
-    const R sodium_potassium_pump_current__i_NaK = (((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk)) * y[17]) / (y[17] + sodium_potassium_pump_current__K_mNa)) / (R(1.0) + (R(0.1245) * exp((((-R(0.1)) * V * membrane__F) / (membrane__R * membrane__T)))) + (R(0.0353) * exp((((-V) * membrane__F) / (membrane__R * membrane__T))))));
+    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
+    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
     const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11] * (V - reversal_potentials__E_K));
-    return (((-R(1.0)) / R(1.0)) * (inward_rectifier_potassium_current__i_K1 + transient_outward_current__i_to + rapid_time_dependent_potassium_current__i_Kr + slow_time_dependent_potassium_current__i_Ks + L_type_Ca_current__i_CaL + sodium_potassium_pump_current__i_NaK + fast_sodium_current__i_Na + sodium_background_current__i_b_Na + sodium_calcium_exchanger_current__i_NaCa + calcium_background_current__i_b_Ca + potassium_pump_current__i_p_K + calcium_pump_current__i_p_Ca + i_stim));
+    return (((-R(1.0)) * (R(1) / R(1.0))) * (inward_rectifier_potassium_current__i_K1 + transient_outward_current__i_to + rapid_time_dependent_potassium_current__i_Kr + slow_time_dependent_potassium_current__i_Ks + L_type_Ca_current__i_CaL + sodium_potassium_pump_current__i_NaK + fast_sodium_current__i_Na + sodium_background_current__i_b_Na + sodium_calcium_exchanger_current__i_NaCa + calcium_background_current__i_b_Ca + potassium_pump_current__i_p_K + calcium_pump_current__i_p_Ca + i_stim));
 }
 
 // Total ionic current in uA/cm^2 (GetIIonic): minus dV/dt at zero stimulus,
@@ -380,26 +369,26 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = R(0);
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = ((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) / (membrane__R * membrane__T)) * ((R(0.25) * y[15] * exp(((R(2.0) * (V - R(15.0)) * membrane__F) / (membrane__R * membrane__T)))) - calcium_dynamics__Ca_o)) / (exp(((R(2.0) * (V - R(15.0)) * membrane__F) / (membrane__R * membrane__T))) - R(1.0)));
-    const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) / membrane__F) * log((calcium_dynamics__Ca_o / y[13])));
+    const R L_type_Ca_current__i_CaL = cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), (cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T)))) - R(1.0)));
+    const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, y[13])));
     const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
-    const R calcium_pump_current__i_p_Ca = ((calcium_pump_current__g_pCa * y[13]) / (y[13] + calcium_pump_current__K_pCa));
-    const R reversal_potentials__E_Na = (((membrane__R * membrane__T) / membrane__F) * log((sodium_dynamics__Na_o / y[17])));
+    const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * y[13]), (y[13] + calcium_pump_current__K_pCa));
+    const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, y[17])));
     const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
-    const R reversal_potentials__E_K = (((membrane__R * membrane__T) / membrane__F) * log((potassium_dynamics__K_o / y[18])));
-    const R inward_rectifier_potassium_current__alpha_K1 = (R(0.1) / (R(1.0) + exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
-    const R inward_rectifier_potassium_current__beta_K1 = (((R(3.0) * exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))) / (R(1.0) + exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
-    const R inward_rectifier_potassium_current__xK1_inf = (inward_rectifier_potassium_current__alpha_K1 / (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
+    const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, y[18])));
+    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
+    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
+    const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
     const R inward_rectifier_potassium_current__i_K1 = (inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
-    const R potassium_pump_current__i_p_K = ((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)) / (R(1.0) + exp(((R(25.0) - V) / R(5.98)))));
-    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o / R(5.4))) * y[1] * y[2] * (V - reversal_potentials__E_K));
-    const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) / membrane__F) * log(((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)) / (y[18] + (reversal_potentials__P_kna * y[17])))));
+    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
+    const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (y[18] + (reversal_potentials__P_kna * y[17])))));
     const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
-    const R sodium_calcium_exchanger_current__i_NaCa = ((sodium_calcium_exchanger_current__K_NaCa * ((exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) / (membrane__R * membrane__T))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) / (membrane__R * membrane__T))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))) / ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) / (membrane__R * membrane__T)))))));
-    const R sodium_potassium_pump_current__i_NaK = (((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk)) * y[17]) / (y[17] + sodium_potassium_pump_current__K_mNa)) / (R(1.0) + (R(0.1245) * exp((((-R(0.1)) * V * membrane__F) / (membrane__R * membrane__T)))) + (R(0.0353) * exp((((-V) * membrane__F) / (membrane__R * membrane__T))))));
+    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
+    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
     const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11] * (V - reversal_potentials__E_K));
-    const R dv = (((-R(1.0)) / R(1.0)) * (inward_rectifier_potassium_current__i_K1 + transient_outward_current__i_to + rapid_time_dependent_potassium_current__i_Kr + slow_time_dependent_potassium_current__i_Ks + L_type_Ca_current__i_CaL + sodium_potassium_pump_current__i_NaK + fast_sodium_current__i_Na + sodium_background_current__i_b_Na + sodium_calcium_exchanger_current__i_NaCa + calcium_background_current__i_b_Ca + potassium_pump_current__i_p_K + calcium_pump_current__i_p_Ca + i_stim));
+    const R dv = (((-R(1.0)) * (R(1) / R(1.0))) * (inward_rectifier_potassium_current__i_K1 + transient_outward_current__i_to + rapid_time_dependent_potassium_current__i_Kr + slow_time_dependent_potassium_current__i_Ks + L_type_Ca_current__i_CaL + sodium_potassium_pump_current__i_NaK + fast_sodium_current__i_Na + sodium_background_current__i_b_Na + sodium_calcium_exchanger_current__i_NaCa + calcium_background_current__i_b_Ca + potassium_pump_current__i_p_K + calcium_pump_current__i_p_Ca + i_stim));
     return -dv * cm;
 }
 

@@ -29,6 +29,7 @@ static void RunCase(const char* name, const double golden[4], double duration)
     std::printf("{\"case\": \"%s\", \"V\": [%.6f, %.6f, %.6f, %.6f], \"max_err\": %.3e, \"cg_iterations\": %lld, \"pass\": %s}\n",
                 name, V[1], V[3], V[5], V[7], worst, (long long)monodomain_problem.GetStats().cg_iterations,
                 ok ? "true" : "false");
+    std::fflush(stdout);
 }
 
 int main()

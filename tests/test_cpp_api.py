@@ -22,7 +22,7 @@ def test_cpp_program_links_libcmx():
 @pytest.mark.gpu
 def test_cpp_monodomain_problem_goldens():
     print('case cpp TestMonodomainProblem', flush=True)
-    r = subprocess.run([_binary()], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=240)
+    r = subprocess.run([_binary()], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     print(r.stdout)
     cases = [json.loads(l) for l in r.stdout.splitlines() if l.startswith('{')]
     assert {c['case'] for c in cases} == {'1d', '2d', '3d', 'errors'}

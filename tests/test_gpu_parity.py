@@ -79,6 +79,23 @@ def test_tt06_3d_fp64(oracle, integrator):
     assert serr.max() <= 1e-8
 
 
+def test_budget_resume_path(oracle):
+    """rtol 1e-14 needs more CG iterations than the first steps' budget: those
+    steps stop, poison the clock and are resumed on the host; the result must
+    still match the oracle to 1e-9."""
+    from chaste_amd import cmx
+    print('case resume path', flush=True)
+    m = make(2, 40, 40, 0, cmx.LR91, 0, -400000.0, 0.5, 0.02, rtol=1e-14)
+    m.step(0.0, 150)
+    ref = oracle.mono_run(2, 40, 40, 0, model=oracle.LR91_HAND, stim_mag=-400000.0, stim_dur=0.5, stim_xlim=0.02,
+                          t_end=1.5, ksp_rtol=1e-14)
+    V = m.voltage()
+    its = m.stats()['cg_iterations'] / 150.0
+    print('resume case its/step', its)
+    assert its > 8
+    assert np.abs(V - ref['V']).max() / np.abs(ref['V']).max() <= FP64_REL
+
+
 def test_ell_matches_stencil():
     from chaste_amd import cmx
     print('case ell vs stencil', flush=True)
