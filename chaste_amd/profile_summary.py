@@ -27,7 +27,10 @@ def summarise(path, command, title):
         idx = [i for i, e in enumerate(ev) if 'cell_step' in e[2] and prec in e[2]]
         if len(idx) < 3:
             continue
-        i0, i1 = idx[-3], idx[-2]
+        # bench.py runs warmup, a 5-step ranking pass, the timed steps, then an
+        # event-instrumented pass: take a step from the middle third (timed).
+        k = len(idx) // 3
+        i0, i1 = idx[k], idx[k + 1]
         out += ['', 'One %s step (start offset, duration, gap before; microseconds):' % ('FP64' if prec == 'double' else 'FP32-mixed'),
                 '', '| t | duration | gap | kernel |', '|---|---|---|---|']
         t0 = ev[i0][0]
