@@ -32,13 +32,15 @@ SIGMA = (1.75, 0.19, 0.19)
 STIM = (-50000.0, 0.0, 1.0)       # magnitude uA/cm^3, start ms, duration ms
 STIM_XLIM = 0.03
 DT = 0.01
+EXTRAPOLATE = True   # CG initial guess V^n + previous correction, in both paths
 
 
 def build_problem(precision, coords, elems, stim_index, cmx):
     return cmx.Monodomain(coords, elems, np.array(SIGMA), cmx.TT06_EPI, cmx.RUSH_LARSEN,
                           stimuli=[STIM], stim_index=stim_index, precision=precision,
                           dt_pde=DT, dt_ode=DT, ksp_rtol=1e-8, ksp_atol=0.0, ksp_max_its=500,
-                          refine_sweeps=1 if precision == cmx.MIXED else 0)
+                          refine_sweeps=1 if precision == cmx.MIXED else 0, use_graph=True,
+                          extrapolate=EXTRAPOLATE)
 
 
 def bytes_per_row(kind, precision, n_states, refined):
@@ -208,7 +210,8 @@ def main():
         'dtype': 'f32 (f64 V, dots, residual)',
         'data': 'synthetic (regular slab, default cell state)',
         'config': {'workload': 'TT06-epi RL monodomain slab 2.0x0.7x0.3 cm h=0.01 (201x71x31 = 442,401 nodes), '
-                               'dt 0.01 ms, stim -50000 uA/cm^3 1 ms x<0.03, CG rtol 1e-8',
+                               'dt 0.01 ms, stim -50000 uA/cm^3 1 ms x<0.03, Jacobi-CG rtol 1e-8'
+                               + (', initial guess V^n + previous correction' if EXTRAPOLATE else ''),
                    'n_nodes': n, 'n_elems': int(elems.shape[0])},
         'node_ode_updates_per_s': round(steps_s * n, 1),
         'fp64_steps_per_s': round(results['fp64']['steps_per_s'], 2),

@@ -210,13 +210,17 @@ struct ResidualArgs {
     const double* __restrict__ V;
     const RV* __restrict__ F;
     double* __restrict__ r0_64;        // FP64 residual of V^n, kept for the refinement sweep
+    int guess;                         // x holds an initial guess (previous correction)
     CgVecs<RV> v;
     double* __restrict__ partials;     // [G][2]: ||D^-1 b||^2, ||D^-1 r0||^2
     SolveCtl* ctl;
 };
 
-// r0 = M F - K V ; b = c M V + M F (only ||D^-1 b|| is needed); z0 = D^-1 r0
-// and p = s = x = 0 for the solve that follows.  r0 is also kept in FP64 for
+// r0 = M F - K V - A x0 with x0 = 0 (the reference's warm start from V^n) or,
+// when extrapolating, x0 = the previous step's correction, which x already
+// holds (read here by neighbours, so never written by this kernel);
+// b = c M V + M F (only ||D^-1 b|| is needed); z0 = D^-1 r0 and p = s = 0 for
+// the solve that follows.  r0 is also kept in FP64 for
 // the refinement sweep.  The stiffness term uses differences (V_j - V_i): K's
 // rows sum to zero, and the differences keep the diffusion flux free of
 // cancellation against the ~85 mV resting level.
@@ -236,7 +240,8 @@ __global__ __launch_bounds__(kBlock) void residual_kernel(Rows R, MatVals mv, Re
     for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kBlock) {
         const auto row = R.row(i);
         const double vi = a.V[i];
-        double mf = 0, mvv = 0, kx = 0;
+        const double gi = a.guess ? (double)a.v.x[i] : 0.0;
+        double mf = 0, mvv = 0, kx = 0, mg = 0, kg = 0;
 #pragma unroll
         for (int k = 0; k < R.width(); ++k) {
             const int j = R.col(row, k);
@@ -245,18 +250,23 @@ __global__ __launch_bounds__(kBlock) void residual_kernel(Rows R, MatVals mv, Re
             mf += Mv[s] * (double)a.F[j];
             mvv += Mv[s] * vj;
             kx += Kv[s] * (vj - vi);
+            if (a.guess) {
+                const double gj = (double)a.v.x[j];
+                mg += Mv[s] * gj;
+                kg += Kv[s] * (gj - gi);
+            }
         }
         const int d = R.didx(row);
         const double bi = Dv[d] * (a.c * mvv + mf);
-        const double r64 = mf - kx;
+        const double r64 = mf - kx;                 // b - A V^n
         if (a.r0_64) a.r0_64[i] = r64;
-        const RV ri = (RV)r64;
+        const RV ri = (RV)(r64 - a.c * mg - kg);    // b - A (V^n + x0)
         const RV zi = D32[d] * ri;
         a.v.r[i] = ri;
         a.v.z[i] = zi;
         a.v.p[i] = RV(0);
         a.v.s[i] = RV(0);
-        a.v.x[i] = RV(0);
+        if (!a.guess) a.v.x[i] = RV(0);
         part[0] += bi * bi;
         part[1] += (double)zi * (double)zi;
     }
@@ -512,7 +522,7 @@ struct cmx_ctx {
     int n = 0, dim = 0, model = 0, integrator = 0, precision = 0, format = 0;
     int S = 0, P = 0, n_param_planes = 0;
     double dt = 0, dt_ode = 0, am = 0, cm = 0, rtol = 0, atol = 0;
-    int max_its = 0, refine = 0, n_sub = 1, use_graph = 0;
+    int max_its = 0, refine = 0, n_sub = 1, use_graph = 0, extrapolate = 0;
     int width = 0, n_patterns = 0;
     int G = 1;                   // blocks of the reduction kernels
     long long nnz = 0;
@@ -558,6 +568,10 @@ struct cmx_ctx {
     double last_residual = 0;
     double device_bytes = 0;
     std::vector<void*> allocs;
+    // hipGraphs of whole steps, keyed by (budget, refinement budget, steps)
+    struct StepGraph { int budget, refine_budget, steps; hipGraph_t graph; hipGraphExec_t exec; };
+    std::vector<StepGraph> graphs;
+    long long graph_launches = 0;
     // kernel timing (cmx_set_profiling)
     int prof_mask = 0;
     struct Pending { int kind; hipEvent_t a, b; };
@@ -760,6 +774,7 @@ int launch_residual(cmx_ctx* c, const Rows& R, int solve, bool refine_sweep)
     a.V = c->V;
     a.F = (const RV*)c->F;
     a.r0_64 = c->r0_64;
+    a.guess = (!refine_sweep && c->extrapolate) ? 1 : 0;
     a.v = cg_vecs<RV>(c, solve);
     a.partials = pbuf(c, 0);
     a.ctl = c->ctl + solve;
@@ -837,6 +852,67 @@ int sync_state(cmx_ctx* c)
     return CMX_OK;
 }
 
+// Instantiated graph of `steps` consecutive steps at the current budgets
+// (captured once; kernel arguments are fixed because time lives in the
+// device step clock).  Returns nullptr when capture is unavailable.
+template <class Rows, class RS, class RV>
+hipGraphExec_t step_graph(cmx_ctx* c, const Rows& R, int steps)
+{
+    for (auto& g : c->graphs)
+        if (g.budget == c->budget && g.refine_budget == c->refine_budget && g.steps == steps) return g.exec;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    int rc = CMX_OK;
+    for (int s = 0; s < steps && rc == CMX_OK; ++s) rc = enqueue_step<Rows, RS, RV>(c, R);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc != CMX_OK || e != hipSuccess || !graph) {
+        (void)hipGetLastError();
+        if (graph) (void)hipGraphDestroy(graph);
+        return nullptr;
+    }
+    if (c->graphs.size() >= 16) {   // bounded cache: drop the oldest
+        (void)hipGraphExecDestroy(c->graphs.front().exec);
+        (void)hipGraphDestroy(c->graphs.front().graph);
+        c->graphs.erase(c->graphs.begin());
+    }
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipGraphDestroy(graph);
+        return nullptr;
+    }
+    c->graphs.push_back({c->budget, c->refine_budget, steps, graph, exec});
+    return exec;
+}
+
+// Enqueues `todo` steps: replays of captured multi-step graphs when enabled
+// (and no kernel is being timed), plain launches otherwise.
+template <class Rows, class RS, class RV>
+int enqueue_steps(cmx_ctx* c, const Rows& R, int todo)
+{
+    constexpr int kGraphSteps = 8;
+    if (c->use_graph && c->prof_mask == 0) {
+        int left = todo;
+        while (left > 0) {
+            const int g = left >= kGraphSteps ? kGraphSteps : 1;
+            hipGraphExec_t exec = step_graph<Rows, RS, RV>(c, R, g);
+            if (!exec) break;
+            CMX_HIP(c, hipGraphLaunch(exec, c->stream));
+            ++c->graph_launches;
+            left -= g;
+        }
+        todo = left;
+    }
+    for (int s = 0; s < todo; ++s) {
+        const int rc = enqueue_step<Rows, RS, RV>(c, R);
+        if (rc) return rc;
+    }
+    return CMX_OK;
+}
+
 // Runs the clock's steps k .. n_steps-1 in batches, resuming poisoned steps.
 template <class Rows, class RS, class RV>
 int run_steps(cmx_ctx* c, const Rows& R)
@@ -846,11 +922,9 @@ int run_steps(cmx_ctx* c, const Rows& R)
     while (c->h_clock->k < c->h_clock->n_steps) {
         const int k_start = c->h_clock->k;
         const int todo = std::min(c->h_clock->n_steps - k_start, c->batch);
-        for (int s = 0; s < todo; ++s) {
-            const int rc = enqueue_step<Rows, RS, RV>(c, R);
-            if (rc) return rc;
-        }
-        int rc = sync_state(c);
+        int rc = enqueue_steps<Rows, RS, RV>(c, R, todo);
+        if (rc) return rc;
+        rc = sync_state(c);
         if (rc) return rc;
         int guard = 0;
         while (c->h_clock->poison) {
@@ -984,6 +1058,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->max_its = opts->ksp_max_its;
     c->refine = opts->refine_sweeps;
     c->use_graph = opts->use_graph;
+    c->extrapolate = opts->extrapolate;
     c->n_param_planes = tissue->n_params;
     c->budget = std::min(8, c->max_its);
     c->refine_budget = 1;
@@ -1154,6 +1229,11 @@ void release(cmx_ctx* c)
     collect_times(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     c->event_pool.clear();
+    for (auto& g : c->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
+    c->graphs.clear();
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     if (c->h_clock) (void)hipHostFree(c->h_clock);
@@ -1232,6 +1312,9 @@ int cmx_set_voltage(cmx_ctx* c, const double* V)
     if (!c || !V) return fail(c, CMX_E_ARG, "null argument");
     CMX_HIP(c, hipStreamSynchronize(c->stream));
     CMX_HIP(c, hipMemcpy(c->V, V, c->n * sizeof(double), hipMemcpyHostToDevice));
+    // the previous correction no longer predicts the next one
+    const size_t rv = c->precision == CMX_MIXED ? sizeof(float) : sizeof(double);
+    CMX_HIP(c, hipMemset(c->vec[5], 0, c->n * rv));
     return CMX_OK;
 }
 

@@ -349,6 +349,8 @@ public:
         o.ksp_max_its = cfg->GetKspMaxIterations();
         o.refine_sweeps = cfg->GetGpuRefinementSweeps();
         o.mass_lumping = cfg->GetUseMassLumping() ? 1 : 0;
+        o.use_graph = 1;
+        o.extrapolate = 0;
         if (mpCtx) { cmx_destroy(mpCtx); mpCtx = nullptr; }
         Check(cmx_create(&mesh, &tissue, &o, &mpCtx), nullptr);
         mCurrentTime = 0.0;

@@ -88,8 +88,10 @@ typedef struct {
     double ksp_rtol, ksp_atol; /* stop when ||D^-1 r|| <= max(rtol ||D^-1 b||, atol) */
     int32_t ksp_max_its;
     int32_t refine_sweeps;     /* mixed: FP64 true-residual checks per step (0 = none) */
-    int32_t use_graph;         /* capture each step as a hipGraph */
+    int32_t use_graph;         /* replay steps from captured hipGraphs */
     int32_t mass_lumping;      /* HeartConfig::SetUseMassLumping */
+    int32_t extrapolate;       /* CG initial guess V^n + (previous step's correction)
+                                  instead of the reference's V^n; same stopping test */
 } cmx_opts;
 
 typedef struct {

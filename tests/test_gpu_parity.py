@@ -20,13 +20,14 @@ MIXED_ACT_MS = 0.1
 
 
 def make(dim, ex, ey, ez, model, integrator, stim_mag, stim_dur, stim_xlim, precision=0, sigma=(1.75, 1.75, 1.75),
-         rtol=1e-12, atol=0.0, fmt=0, refine=0, dt=0.01, dt_ode=None):
+         rtol=1e-12, atol=0.0, fmt=0, refine=0, dt=0.01, dt_ode=None, use_graph=False, extrapolate=False):
     from chaste_amd import cmx
     coords, elems = cmx.slab_mesh(dim, ex, ey, ez, 0.01)
     stim_index = np.where(coords[:, 0] < stim_xlim, 0, -1).astype(np.int32)
     return cmx.Monodomain(coords, elems, np.array(sigma[:dim]), model, integrator,
                           stimuli=[(stim_mag, 0.0, stim_dur)], stim_index=stim_index, precision=precision,
-                          ksp_rtol=rtol, ksp_atol=atol, fmt=fmt, refine_sweeps=refine, dt_pde=dt, dt_ode=dt if dt_ode is None else dt_ode)
+                          ksp_rtol=rtol, ksp_atol=atol, fmt=fmt, refine_sweeps=refine, dt_pde=dt, dt_ode=dt if dt_ode is None else dt_ode,
+                          use_graph=use_graph, extrapolate=extrapolate)
 
 
 def first_crossing(trace, dt, thr=0.0):
@@ -94,6 +95,53 @@ def test_budget_resume_path(oracle):
     print('resume case its/step', its)
     assert its > 8
     assert np.abs(V - ref['V']).max() / np.abs(ref['V']).max() <= FP64_REL
+
+
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_graph_replay_matches_eager(precision):
+    """Steps replayed from captured hipGraphs give bitwise the same voltages as
+    plain launches, over several calls (graph replays with eager work and
+    synchronisations in between), including steps that hit the resume path."""
+    from chaste_amd import cmx
+    print('case graph replay', precision, flush=True)
+    kw = dict(precision=precision, sigma=(1.75, 0.19, 0.19), rtol=1e-10, refine=1)
+    a = make(3, 24, 8, 5, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, use_graph=True, **kw)
+    b = make(3, 24, 8, 5, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, use_graph=False, **kw)
+    t = 0.0
+    for n in (40, 17, 64, 9):
+        a.step(t, n)
+        b.step(t, n)
+        t += n * 0.01
+        Va, Vb = a.voltage(), b.voltage()
+        assert np.array_equal(Va, Vb)
+    assert np.array_equal(a.state(), b.state())
+    assert a.stats()['cg_iterations'] == b.stats()['cg_iterations']
+
+
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_extrapolated_initial_guess(oracle, precision):
+    """Starting CG from V^n + (previous correction) changes the iterates, not
+    the converged voltages: FP64 stays within 1e-9 of the oracle (warm start
+    V^n, as the reference), mixed within 0.5 mV, with fewer iterations."""
+    from chaste_amd import cmx
+    print('case extrapolate', precision, flush=True)
+    dims = (3, 20, 7, 4)
+    rtol = 1e-12 if precision == 0 else 1e-8
+    kw = dict(precision=precision, sigma=(1.75, 0.19, 0.19), rtol=rtol, refine=precision, use_graph=True)
+    e = make(*dims, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, extrapolate=True, **kw)
+    p = make(*dims, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, extrapolate=False, **kw)
+    e.step(0.0, 400)
+    p.step(0.0, 400)
+    ref = oracle.mono_run(*dims, model=oracle.TT06_EPI, rush_larsen=True, sigma=(1.75, 0.19, 0.19),
+                          stim_mag=-100000.0, stim_dur=1.0, stim_xlim=0.03, t_end=4.0, ksp_rtol=1e-12)['V']
+    Ve = e.voltage()
+    its_e, its_p = e.stats()['cg_iterations'], p.stats()['cg_iterations']
+    print('extrapolate its', its_e, 'plain its', its_p)
+    if precision == 0:
+        assert np.abs(Ve - ref).max() / np.abs(ref).max() <= FP64_REL
+    else:
+        assert np.abs(Ve - ref).max() <= MIXED_MV
+    assert its_e <= its_p
 
 
 def test_ell_matches_stencil():
