@@ -43,24 +43,30 @@ def build_problem(precision, coords, elems, stim_index, cmx):
                           extrapolate=EXTRAPOLATE)
 
 
-def bytes_per_row(kind, precision, n_states, refined):
+def bytes_per_row(kind, precision, n_states, refined, guess=True):
     """Algorithmic HBM bytes per row of one launch of a kernel kind that does
-    its work.  s = CG-vector / forcing size, ss = cell-state size; V is FP64.
+    its work.  s = CG-vector / forcing size, ss = cell-state size; V is FP64;
+    the pattern id is one byte.
       cell      V in, non-V states in+out, stimulus id, F out
-      residual  V, F (+ first correction) in, pattern id, r z p s x out
-      spmv      z (gathered once from HBM), r in, pattern id, w out
-      update    z p w s x r in, pattern id, p s x r z out
-      finish    V, x1 (+ x2) in, V out"""
+      residual  V, F, pattern id (+ x0 guess) in; r, u0 (+ x = 0) out, + FP64 r0
+                when refined; the refinement sweep reads r0, x1 and writes
+                r, u, x2 (averaged over the two launches of a refined step)
+      spmv      (solve start) u0 gathered, r in; w, m out
+      update    (pipelined CG iteration) m gathered, r w x z s p in; z s p x r w m out
+      finish    V in/out, x1 (+ x2) in"""
     s = 4 if precision == 1 else 8
     ss = s
     if kind == 'cell':
         return 8 + 2 * (n_states - 1) * ss + 1 + s
     if kind == 'residual':
-        return 8 + s + 1 + 5 * s + (s / 2 if refined else 0)   # refinement sweep every other launch
+        first = 8 + s + 1 + (s if guess else 0) + 2 * s + (0 if guess else s) + (8 if refined else 0)
+        if not refined:
+            return first
+        return (first + (8 + s + 1 + 3 * s)) / 2
     if kind == 'spmv':
-        return 3 * s + 1
+        return 4 * s + 1
     if kind == 'update':
-        return 11 * s + 1
+        return 14 * s + 1
     if kind == 'finish':
         return 16 + s * (2 if refined else 1)
     return 0
@@ -116,8 +122,10 @@ def time_path(m, steps, warmup, roof_steps):
 
 
 def working_launches_per_step(kind, its, refined):
-    if kind in ('spmv', 'update'):
+    if kind == 'update':
         return its
+    if kind == 'spmv':
+        return 1     # the first solve's start; a refinement solve rarely iterates
     if kind == 'residual':
         return 2 if refined else 1
     return 1

@@ -179,27 +179,31 @@ template <class RV> __device__ __forceinline__ const RV* dinv_vals(const MatVals
 template <> __device__ __forceinline__ const double* dinv_vals<double>(const MatVals& m) { return m.dinv64; }
 template <> __device__ __forceinline__ const float* dinv_vals<float>(const MatVals& m) { return m.dinv32; }
 
-// Rows of one block: a contiguous chunk [lo, hi) walked with stride kBlock.
+// Rows of one block: a contiguous chunk [lo, hi) of ceil(n/G) rows walked
+// with stride kCgBlock.
 struct RowRange {
     int lo, hi;
     __device__ __forceinline__ RowRange(int n, int blk, int G)
     {
-        const int per = ((n + G - 1) / G + kBlock - 1) / kBlock * kBlock;
+        const int per = (n + G - 1) / G;
         lo = blk * per;
         hi = min(n, lo + per);
     }
 };
 
-// CG vectors (RV precision).  Only z is gathered by the stencil (w = A z);
-// every other vector is read and written by its own row only.
+// CG work vectors (RV precision).  The stencil gathers only p (which holds
+// u0 = D^-1 r0 before the first iteration) and m; every other vector is read
+// and written by its own row only.
 template <class RV>
 struct CgVecs {
-    RV* r;
-    RV* z;
-    RV* w;
-    RV* s;
-    RV* p;
-    RV* x;
+    RV* r;       // residual
+    RV* w;       // A u with u = D^-1 r
+    RV* z;       // A q with q = D^-1 s
+    RV* s;       // A p
+    RV* p;       // search direction
+    RV* m_in;    // D^-1 w of the previous iteration (gathered)
+    RV* m_out;   // D^-1 w after this iteration
+    RV* x;       // correction being solved for
 };
 
 template <class RV>
@@ -210,22 +214,58 @@ struct ResidualArgs {
     const double* __restrict__ V;
     const RV* __restrict__ F;
     double* __restrict__ r0_64;        // FP64 residual of V^n, kept for the refinement sweep
-    int guess;                         // x holds an initial guess (previous correction)
+    const RV* __restrict__ x1;         // refinement sweep: the first solve's correction
     CgVecs<RV> v;
-    double* __restrict__ partials;     // [G][2]: ||D^-1 b||^2, ||D^-1 r0||^2
+    double* __restrict__ partials;     // [G][3]: ||D^-1 b||^2, 0, ||D^-1 r0||^2
     SolveCtl* ctl;
 };
 
 // r0 = M F - K V - A x0 with x0 = 0 (the reference's warm start from V^n) or,
-// when extrapolating, x0 = the previous step's correction, which x already
-// holds (read here by neighbours, so never written by this kernel);
-// b = c M V + M F (only ||D^-1 b|| is needed); z0 = D^-1 r0 and p = s = 0 for
-// the solve that follows.  r0 is also kept in FP64 for
-// the refinement sweep.  The stiffness term uses differences (V_j - V_i): K's
-// rows sum to zero, and the differences keep the diffusion flux free of
-// cancellation against the ~85 mV resting level.
-template <class Rows, class RV>
-__global__ __launch_bounds__(kBlock) void residual_kernel(Rows R, MatVals mv, ResidualArgs<RV> a)
+// with GUESS, x0 = the previous step's correction, which x already holds
+// (read here by neighbours, so never written by this kernel);
+// b = c M V + M F (only ||D^-1 b|| is needed); u0 = D^-1 r0 goes to p for the
+// solve that follows, r0 is also kept in FP64 for the refinement sweep.  The
+// stiffness term uses differences (V_j - V_i): K's rows sum to zero, and the
+// differences keep the diffusion flux free of cancellation against the
+// ~85 mV resting level.
+template <class Rows, class RV, bool GUESS>
+__device__ __forceinline__ void residual_row(const Rows& R, int i, const double* __restrict__ Mv,
+                                             const double* __restrict__ Kv, const double* __restrict__ Dv,
+                                             const RV* __restrict__ D32, const ResidualArgs<RV>& a, double (&part)[3])
+{
+    const auto row = R.row(i);
+    const double vi = a.V[i];
+    double mf = 0, mvv = 0, kx = 0, mg = 0, kg = 0, gi = 0;
+    if constexpr (GUESS) gi = (double)a.v.x[i];
+#pragma unroll
+    for (int k = 0; k < R.width(); ++k) {
+        const int j = R.col(row, k);
+        const int s = R.vidx(row, k);
+        const double vj = a.V[j];
+        mf += Mv[s] * (double)a.F[j];
+        mvv += Mv[s] * vj;
+        kx += Kv[s] * (vj - vi);
+        if constexpr (GUESS) {
+            const double gj = (double)a.v.x[j];
+            mg += Mv[s] * gj;
+            kg += Kv[s] * (gj - gi);
+        }
+    }
+    const int d = R.didx(row);
+    const double bi = Dv[d] * (a.c * mvv + mf);
+    const double r64 = mf - kx;                 // b - A V^n
+    if (a.r0_64) a.r0_64[i] = r64;
+    const RV ri = (RV)(r64 - a.c * mg - kg);    // b - A (V^n + x0)
+    const RV ui = D32[d] * ri;
+    a.v.r[i] = ri;
+    a.v.p[i] = ui;
+    if constexpr (!GUESS) a.v.x[i] = RV(0);
+    part[0] += bi * bi;
+    part[2] += (double)ui * (double)ui;
+}
+
+template <class Rows, class RV, bool GUESS>
+__global__ __launch_bounds__(kCgBlock) void residual_kernel(Rows R, MatVals mv, ResidualArgs<RV> a)
 {
     if (a.clock->poison) return;
     __shared__ double sM[kMaxPatterns * kMaxStencil], sK[kMaxPatterns * kMaxStencil], sD[kMaxPatterns];
@@ -236,41 +276,10 @@ __global__ __launch_bounds__(kBlock) void residual_kernel(Rows R, MatVals mv, Re
     const RV* __restrict__ D32 = dinv_vals<RV>(mv);
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const RowRange rr(a.n, blk, gridDim.x);
-    double part[2] = {0.0, 0.0};
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kBlock) {
-        const auto row = R.row(i);
-        const double vi = a.V[i];
-        const double gi = a.guess ? (double)a.v.x[i] : 0.0;
-        double mf = 0, mvv = 0, kx = 0, mg = 0, kg = 0;
-#pragma unroll
-        for (int k = 0; k < R.width(); ++k) {
-            const int j = R.col(row, k);
-            const int s = R.vidx(row, k);
-            const double vj = a.V[j];
-            mf += Mv[s] * (double)a.F[j];
-            mvv += Mv[s] * vj;
-            kx += Kv[s] * (vj - vi);
-            if (a.guess) {
-                const double gj = (double)a.v.x[j];
-                mg += Mv[s] * gj;
-                kg += Kv[s] * (gj - gi);
-            }
-        }
-        const int d = R.didx(row);
-        const double bi = Dv[d] * (a.c * mvv + mf);
-        const double r64 = mf - kx;                 // b - A V^n
-        if (a.r0_64) a.r0_64[i] = r64;
-        const RV ri = (RV)(r64 - a.c * mg - kg);    // b - A (V^n + x0)
-        const RV zi = D32[d] * ri;
-        a.v.r[i] = ri;
-        a.v.z[i] = zi;
-        a.v.p[i] = RV(0);
-        a.v.s[i] = RV(0);
-        if (!a.guess) a.v.x[i] = RV(0);
-        part[0] += bi * bi;
-        part[1] += (double)zi * (double)zi;
-    }
-    write_partials<2>(part, a.partials, blk);
+    double part[3] = {0.0, 0.0, 0.0};
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock)
+        residual_row<Rows, RV, GUESS>(R, i, Mv, Kv, Dv, D32, a, part);
+    write_partials<kCgBlock, 3>(part, a.partials, blk);
     if (blk == 0 && threadIdx.x == 0) {
         a.ctl->done = 0;
         a.ctl->it = 0;
@@ -278,11 +287,10 @@ __global__ __launch_bounds__(kBlock) void residual_kernel(Rows R, MatVals mv, Re
 }
 
 // Refinement sweep: the true FP64 residual of V^n + x1 against the step's b,
-// r = r0 - A x1 (A in FP64), then z = D^-1 r and p = s = x2 = 0 for the
-// second solve.  Partials: slot 0 unused, slot 1 ||D^-1 r||^2.
+// r = r0 - A x1 (A in FP64), then u = D^-1 r to p and x2 = 0 for the second
+// solve.  Partials: (0, 0, ||D^-1 r||^2).
 template <class Rows, class RV>
-__global__ __launch_bounds__(kBlock) void refine_residual_kernel(Rows R, MatVals mv, ResidualArgs<RV> a,
-                                                                 const RV* __restrict__ x1)
+__global__ __launch_bounds__(kCgBlock) void refine_residual_kernel(Rows R, MatVals mv, ResidualArgs<RV> a)
 {
     if (a.clock->poison) return;
     __shared__ double sA[kMaxPatterns * kMaxStencil];
@@ -291,22 +299,20 @@ __global__ __launch_bounds__(kBlock) void refine_residual_kernel(Rows R, MatVals
     const RV* __restrict__ D32 = dinv_vals<RV>(mv);
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const RowRange rr(a.n, blk, gridDim.x);
-    double part[2] = {0.0, 0.0};
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kBlock) {
+    double part[3] = {0.0, 0.0, 0.0};
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
         const auto row = R.row(i);
         double ax = 0;
 #pragma unroll
-        for (int k = 0; k < R.width(); ++k) ax += Av[R.vidx(row, k)] * (double)x1[R.col(row, k)];
+        for (int k = 0; k < R.width(); ++k) ax += Av[R.vidx(row, k)] * (double)a.x1[R.col(row, k)];
         const RV ri = (RV)(a.r0_64[i] - ax);
-        const RV zi = D32[R.didx(row)] * ri;
+        const RV ui = D32[R.didx(row)] * ri;
         a.v.r[i] = ri;
-        a.v.z[i] = zi;
-        a.v.p[i] = RV(0);
-        a.v.s[i] = RV(0);
+        a.v.p[i] = ui;
         a.v.x[i] = RV(0);
-        part[1] += (double)zi * (double)zi;
+        part[2] += (double)ui * (double)ui;
     }
-    write_partials<2>(part, a.partials, blk);
+    write_partials<kCgBlock, 3>(part, a.partials, blk);
     if (blk == 0 && threadIdx.x == 0) {
         a.ctl->done = 0;
         a.ctl->it = 0;
@@ -317,7 +323,6 @@ template <class RV>
 struct CgArgs {
     int n;
     int first;                         // first solve of the step: set the tolerance
-    int it0;                           // first iteration of a solve
     double rtol2, atol2;
     const StepClock* clock;
     SolveCtl* ctl;                     // this solve
@@ -328,101 +333,141 @@ struct CgArgs {
     CgVecs<RV> v;
 };
 
-// w = A z with gamma = (r, z), delta = (w, z), zeta = (z, z).  The first call
-// of a step also turns ||D^-1 b||^2 into the stopping threshold.
+// Start of a solve: w0 = A u0, m0 = D^-1 w0 and the dot products
+// (r0, u0), (w0, u0), (u0, u0) for iteration 0.  The residual kernel left
+// ||D^-1 b||^2 and ||D^-1 r0||^2: the first solve of a step turns the former
+// into the stopping threshold, and a residual already below it (the usual
+// case of a refinement sweep) ends the solve here with no iteration.
 template <class Rows, class RV>
-__global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Rows R, MatVals mv, CgArgs<RV> a)
+__global__ __launch_bounds__(kCgBlock) void cg_start_kernel(Rows R, MatVals mv, CgArgs<RV> a)
 {
     if (a.clock->poison) return;
-    if (a.ctl->done) return;
-    __shared__ RV sA[kMaxPatterns * kMaxStencil];
+    __shared__ RV sA[kMaxPatterns * kMaxStencil], sD[kMaxPatterns];
     const RV* __restrict__ A = stage_table(R, a_vals<RV>(mv), sA, R.value_count());
+    const RV* __restrict__ D = stage_table(R, dinv_vals<RV>(mv), sD, R.diag_count());
     if constexpr (Rows::kStaged) __syncthreads();
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    if (a.it0) {
-        // First iteration of a solve: the residual kernel left ||D^-1 b||^2
-        // and ||D^-1 r0||^2; a residual already below the threshold (the
-        // usual case of a refinement sweep) ends the solve here.
-        double g0[2];
-        read_partials<2>(a.pin, gridDim.x, g0);
-        double tol2 = a.first ? fmax(a.rtol2 * g0[0], a.atol2) : a.ctl->tol2;
-        const bool lead = blk == 0 && threadIdx.x == 0;
-        if (lead && a.first) {
-            a.ctl_all[0].bb = g0[0];
-            a.ctl_all[0].tol2 = tol2;
-            a.ctl_all[1].bb = g0[0];
-            a.ctl_all[1].tol2 = tol2;
-        }
-        if (g0[1] <= tol2) {
-            if (lead) { a.ctl->done = 1; a.ctl->it = 0; a.ctl->zz = g0[1]; }
-            return;
-        }
+    double g0[3];
+    read_partials<kCgBlock, 3>(a.pin, gridDim.x, g0);
+    const double tol2 = a.first ? fmax(a.rtol2 * g0[0], a.atol2) : a.ctl->tol2;
+    const bool lead = blk == 0 && threadIdx.x == 0;
+    if (lead && a.first) {
+        a.ctl_all[0].bb = g0[0];
+        a.ctl_all[0].tol2 = tol2;
+        a.ctl_all[1].bb = g0[0];
+        a.ctl_all[1].tol2 = tol2;
     }
-    const RV* __restrict__ z = a.v.z;
+    if (g0[2] <= tol2) {
+        if (lead) { a.ctl->done = 1; a.ctl->it = 0; a.ctl->zz = g0[2]; }
+        return;
+    }
+    const RV* __restrict__ u = a.v.p;
     const RowRange rr(a.n, blk, gridDim.x);
     double part[3] = {0.0, 0.0, 0.0};
-    // two rows per trip so both rows' gathers are in flight together
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += 2 * kBlock) {
-        const int i2 = i + kBlock;
-        const bool two = i2 < rr.hi;
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
         const auto row = R.row(i);
-        const auto row2 = R.row(two ? i2 : i);
-        RV w = 0, w2 = 0;
+        RV w = 0;
 #pragma unroll
-        for (int k = 0; k < R.width(); ++k) {
-            w += A[R.vidx(row, k)] * z[R.col(row, k)];
-            w2 += A[R.vidx(row2, k)] * z[R.col(row2, k)];
-        }
-        const RV ri = a.v.r[i], zi = z[i];
+        for (int k = 0; k < R.width(); ++k) w += A[R.vidx(row, k)] * u[R.col(row, k)];
+        const RV ri = a.v.r[i], ui = u[i];
         a.v.w[i] = w;
-        part[0] += (double)ri * (double)zi;
-        part[1] += (double)w * (double)zi;
-        part[2] += (double)zi * (double)zi;
-        if (two) {
-            const RV ri2 = a.v.r[i2], zi2 = z[i2];
-            a.v.w[i2] = w2;
-            part[0] += (double)ri2 * (double)zi2;
-            part[1] += (double)w2 * (double)zi2;
-            part[2] += (double)zi2 * (double)zi2;
-        }
+        a.v.m_out[i] = D[R.didx(row)] * w;
+        part[0] += (double)ri * (double)ui;
+        part[1] += (double)w * (double)ui;
+        part[2] += (double)ui * (double)ui;
     }
-    write_partials<3>(part, a.pout, blk);
+    write_partials<kCgBlock, 3>(part, a.pout, blk);
 }
 
-// Iteration i of Chronopoulos & Gear's CG (J. Comput. Appl. Math. 25:153,
-// 1989): one reduction per iteration, taken from the preceding spmv:
+// Iteration i of the pipelined preconditioned CG of Ghysels & Vanroose
+// (Parallel Computing 40:224, 2014, Alg. 4) with the Jacobi preconditioner
+// applied pointwise (u = D^-1 r, q = D^-1 s, m = D^-1 w need no vectors of
+// their own except m, which the stencil gathers).  One kernel per iteration:
+// the dot products gamma_i = (r_i, u_i), delta_i = (w_i, u_i) and
+// zeta_i = ||u_i||^2 come reduced from the previous kernel, and the one
+// matrix product n_i = A m_i does not depend on them, so the gathers overlap
+// the reduction:
 //   beta_i = gamma_i / gamma_{i-1},
 //   alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1})   (i > 0)
-//   p = z + beta p;  s = w + beta s  (= A p);  x += alpha p;
-//   r -= alpha s;  z = D^-1 r.
-// Stops (x final) when zeta_i = ||D^-1 r_i||^2 <= tol2.
+//   z = n + beta z;  s = w + beta s;  p = u + beta p;
+//   x += alpha p;  r -= alpha s;  w -= alpha z;  m = D^-1 w
+// and the new dot products go to the next kernel.  The iteration stops (x
+// final) when zeta_i <= tol2, PETSc's preconditioned-norm test; the kernel
+// that consumes the solve applies the same test to the last reduction.
+template <class RV>
+struct IterRow {
+    RV n, d, r, w, z, s, p, x;
+};
+
 template <class Rows, class RV>
-__global__ __launch_bounds__(kBlock) void cg_update_kernel(Rows R, MatVals mv, CgArgs<RV> a, int it)
+__device__ __forceinline__ void iter_load(const Rows& R, const RV* __restrict__ A, const RV* __restrict__ D,
+                                          const CgVecs<RV>& v, int i, bool first_it, IterRow<RV>& q)
+{
+    const auto row = R.row(i);
+    RV n = 0;
+#pragma unroll
+    for (int k = 0; k < R.width(); ++k) n += A[R.vidx(row, k)] * v.m_in[R.col(row, k)];
+    q.n = n;
+    q.d = D[R.didx(row)];
+    q.r = v.r[i];
+    q.w = v.w[i];
+    q.x = v.x[i];
+    if (!first_it) {
+        q.z = v.z[i];
+        q.s = v.s[i];
+        q.p = v.p[i];
+    }
+}
+
+template <class RV>
+__device__ __forceinline__ void iter_store(const CgVecs<RV>& v, int i, bool first_it, RV al, RV be,
+                                           const IterRow<RV>& q, double (&part)[3])
+{
+    RV zn, sn, pn;
+    if (first_it) {
+        zn = q.n;
+        sn = q.w;
+        pn = q.d * q.r;
+    } else {
+        zn = q.n + be * q.z;
+        sn = q.w + be * q.s;
+        pn = q.d * q.r + be * q.p;
+    }
+    const RV rn = q.r - al * sn;
+    const RV wn = q.w - al * zn;
+    const RV un = q.d * rn;
+    v.z[i] = zn;
+    v.s[i] = sn;
+    v.p[i] = pn;
+    v.x[i] = q.x + al * pn;
+    v.r[i] = rn;
+    v.w[i] = wn;
+    v.m_out[i] = q.d * wn;
+    part[0] += (double)rn * (double)un;
+    part[1] += (double)wn * (double)un;
+    part[2] += (double)un * (double)un;
+}
+
+template <class Rows, class RV>
+__global__ __launch_bounds__(kCgBlock) void cg_iter_kernel(Rows R, MatVals mv, CgArgs<RV> a, int it)
 {
     if (a.clock->poison) return;
     SolveCtl* ctl = a.ctl;
     if (ctl->done) return;
-    __shared__ RV sD[kMaxPatterns];
+    __shared__ RV sA[kMaxPatterns * kMaxStencil], sD[kMaxPatterns];
+    const RV* __restrict__ A = stage_table(R, a_vals<RV>(mv), sA, R.value_count());
     const RV* __restrict__ D = stage_table(R, dinv_vals<RV>(mv), sD, R.diag_count());
     if constexpr (Rows::kStaged) __syncthreads();
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const RowRange rr(a.n, blk, gridDim.x);
-    // Own-row operands of the first kPre rows are loaded before the partial
-    // sums are reduced, so the two latencies overlap.
-    constexpr int kPre = 2;
-    RV pz[kPre], pp[kPre], pw[kPre], ps[kPre], px[kPre], pr[kPre];
-    int pd[kPre];
-#pragma unroll
-    for (int u = 0; u < kPre; ++u) {
-        const int i = rr.lo + threadIdx.x + u * kBlock;
-        if (i < rr.hi) {
-            pz[u] = a.v.z[i]; pp[u] = a.v.p[i]; pw[u] = a.v.w[i];
-            ps[u] = a.v.s[i]; px[u] = a.v.x[i]; pr[u] = a.v.r[i];
-            pd[u] = R.didx(R.row(i));
-        }
-    }
+    const bool first_it = it == 0;
+    // The first two rows of every thread are loaded before the reduction.
+    const int i0 = rr.lo + threadIdx.x, i1 = i0 + kCgBlock;
+    IterRow<RV> q0, q1;
+    if (i0 < rr.hi) iter_load(R, A, D, a.v, i0, first_it, q0);
+    if (i1 < rr.hi) iter_load(R, A, D, a.v, i1, first_it, q1);
     double g[3];
-    read_partials<3>(a.pin, gridDim.x, g);
+    read_partials<kCgBlock, 3>(a.pin, gridDim.x, g);
     const double gamma = g[0], delta = g[1], zeta = g[2];
     const bool lead = blk == 0 && threadIdx.x == 0;
     if (!(zeta == zeta) || !(gamma == gamma) || !(delta == delta) || fabs(zeta) > DBL_MAX) {
@@ -438,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Rows R, MatVals mv, C
         return;
     }
     double alpha, beta;
-    if (it == 0) {
+    if (first_it) {
         beta = 0.0;
         alpha = gamma / delta;
     } else {
@@ -452,60 +497,72 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Rows R, MatVals mv, C
         ctl->zz = zeta;
     }
     const RV al = (RV)alpha, be = (RV)beta;
-#pragma unroll
-    for (int u = 0; u < kPre; ++u) {
-        const int i = rr.lo + threadIdx.x + u * kBlock;
-        if (i < rr.hi) {
-            const RV pi = pz[u] + be * pp[u];
-            const RV si = pw[u] + be * ps[u];
-            a.v.p[i] = pi;
-            a.v.s[i] = si;
-            a.v.x[i] = px[u] + al * pi;
-            const RV rn = pr[u] - al * si;
-            a.v.r[i] = rn;
-            a.v.z[i] = D[pd[u]] * rn;
-        }
+    double part[3] = {0.0, 0.0, 0.0};
+    if (i0 < rr.hi) iter_store(a.v, i0, first_it, al, be, q0, part);
+    if (i1 < rr.hi) iter_store(a.v, i1, first_it, al, be, q1, part);
+    for (int i = i1 + kCgBlock; i < rr.hi; i += kCgBlock) {
+        IterRow<RV> q;
+        iter_load(R, A, D, a.v, i, first_it, q);
+        iter_store(a.v, i, first_it, al, be, q, part);
     }
-    for (int i = rr.lo + threadIdx.x + kPre * kBlock; i < rr.hi; i += kBlock) {
-        const RV pi = a.v.z[i] + be * a.v.p[i];
-        const RV si = a.v.w[i] + be * a.v.s[i];
-        a.v.p[i] = pi;
-        a.v.s[i] = si;
-        a.v.x[i] += al * pi;
-        const RV rn = a.v.r[i] - al * si;
-        a.v.r[i] = rn;
-        a.v.z[i] = D[R.didx(R.row(i))] * rn;
-    }
+    write_partials<kCgBlock, 3>(part, a.pout, blk);
 }
 
-// V += x1 (+ x2 of the refinement solve) when the step's final solve
-// converged; otherwise poison the
-// clock so the rest of the batch is skipped until the host resumes the solve.
+// Ends a step: decides the final solve (converged earlier, or its last
+// reduction passes the test), then V += x1 (+ x2 of the refinement solve) and
+// advances the clock; a solve still running poisons the clock so the rest of
+// the batch is skipped until the host resumes it.
 template <class RV>
-__global__ __launch_bounds__(kBlock) void finish_kernel(int n, StepClock* clock, const SolveCtl* ctl,
-                                                        const SolveCtl* ctl_first, int32_t* its_log, int log_len,
-                                                        double* __restrict__ V, const RV* __restrict__ x1,
-                                                        const RV* __restrict__ x2)
+struct FinishArgs {
+    int n;
+    StepClock* clock;
+    SolveCtl* ctl;                     // final solve
+    const SolveCtl* ctl_first;         // first solve when refined, else null
+    const double* __restrict__ pin;    // last reduction of the final solve
+    int32_t* its_log;                  // [log_len][4]: its first, converged first, its refinement
+    int log_len;
+    double* __restrict__ V;
+    const RV* __restrict__ x1;
+    const RV* __restrict__ x2;
+};
+
+template <class RV>
+__global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
 {
+    StepClock* clock = a.clock;
     if (clock->poison) return;
     const int k = clock->k;
-    if (ctl->done != 1) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    int done = a.ctl->done;
+    if (done == 0) {
+        double g[3];
+        read_partials<kCgBlock, 3>(a.pin, gridDim.x, g);
+        if (g[2] <= a.ctl->tol2) {
+            done = 1;
+            if (lead) { a.ctl->done = 1; a.ctl->zz = g[2]; }
+        }
+    }
+    if (done != 1) {
+        if (lead) {
             clock->poison = 1;
             clock->failed_step = k;
         }
         return;
     }
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) {
-        const double v = V[i] + (double)x1[i] + (x2 ? (double)x2[i] : 0.0);
-        V[i] = v;
-        if (!isfinite(v)) clock->nonfinite = 1;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const RowRange rr(a.n, blk, gridDim.x);
+    bool bad = false;
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
+        const double v = a.V[i] + (double)a.x1[i] + (a.x2 ? (double)a.x2[i] : 0.0);
+        a.V[i] = v;
+        bad |= !isfinite(v);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const int slot = 2 * (k % log_len);
-        its_log[slot] = ctl_first ? ctl_first->it : ctl->it;
-        its_log[slot + 1] = ctl_first ? ctl->it : 0;
+    if (bad) clock->nonfinite = 1;
+    if (lead) {
+        int32_t* slot = a.its_log + 4 * (k % a.log_len);
+        slot[0] = a.ctl_first ? a.ctl_first->it : a.ctl->it;
+        slot[1] = a.ctl_first ? (a.ctl_first->done == 1 ? 1 : 0) : 1;
+        slot[2] = a.ctl_first ? a.ctl->it : 0;
         clock->k = k + 1;
     }
 }
@@ -535,7 +592,7 @@ struct cmx_ctx {
     uint8_t* stim_id = nullptr;
     cmx_stimulus* stim = nullptr;
     void* F = nullptr;
-    void* vec[7] = {};           // r z w s p x1 x2
+    void* vec[9] = {};           // r w z s p m_a m_b x1 x2
     double* r0_64 = nullptr;     // FP64 step residual (refinement)
     double* iion = nullptr;      // scratch for cmx_get_iionic
     // matrix
@@ -548,7 +605,7 @@ struct cmx_ctx {
     StepClock* clock = nullptr;
     SolveCtl* ctl = nullptr;     // [2]: first solve, refinement solve
     double2* hist = nullptr;
-    int32_t* its_log = nullptr;  // [batch][2] per-solve iterations, indexed by step % batch
+    int32_t* its_log = nullptr;  // [batch][4] per-solve iterations, indexed by step % batch
     // pinned host mirrors
     StepClock* h_clock = nullptr;
     SolveCtl* h_ctl = nullptr;
@@ -695,39 +752,47 @@ MatVals mat_vals(cmx_ctx* c)
     return m;
 }
 
+// Partial-sum buffers: kResPart for the residual kernels, kIterPart + (it & 1)
+// for the reduction that feeds iteration it of a solve.
+constexpr int kIterPart = 0;
+constexpr int kResPart = 2;
 double* pbuf(cmx_ctx* c, int k) { return c->partials + (size_t)k * kMaxBlocks * 3; }
 
+// Vector slots of cmx_ctx::vec.
+enum { V_R = 0, V_W, V_Z, V_S, V_P, V_MA, V_MB, V_X1, V_X2, V_COUNT };
+
 template <class RV>
-CgVecs<RV> cg_vecs(cmx_ctx* c, int solve)
+CgVecs<RV> cg_vecs(cmx_ctx* c, int solve, int it)
 {
     CgVecs<RV> v;
-    v.r = (RV*)c->vec[0];
-    v.z = (RV*)c->vec[1];
-    v.w = (RV*)c->vec[2];
-    v.s = (RV*)c->vec[3];
-    v.p = (RV*)c->vec[4];
-    v.x = (RV*)c->vec[5 + solve];
+    v.r = (RV*)c->vec[V_R];
+    v.w = (RV*)c->vec[V_W];
+    v.z = (RV*)c->vec[V_Z];
+    v.s = (RV*)c->vec[V_S];
+    v.p = (RV*)c->vec[V_P];
+    // iteration it reads the m written by iteration it-1 (the start kernel
+    // writes it for iteration 0)
+    v.m_in = (RV*)c->vec[(it & 1) ? V_MB : V_MA];
+    v.m_out = (RV*)c->vec[(it & 1) ? V_MA : V_MB];
+    v.x = (RV*)c->vec[solve ? V_X2 : V_X1];
     return v;
 }
 
-// Partial-sum buffers: residual -> 0 (||D^-1 b||^2), spmv -> 1 (dot products,
-// read by the update that follows).
 template <class RV>
-CgArgs<RV> cg_args(cmx_ctx* c, int solve, bool first)
+CgArgs<RV> cg_args(cmx_ctx* c, int solve)
 {
     CgArgs<RV> a;
     a.n = c->n;
-    a.first = first ? 1 : 0;
-    a.it0 = 0;
+    a.first = 0;
     a.rtol2 = c->rtol * c->rtol;
     a.atol2 = c->atol * c->atol;
     a.clock = c->clock;
     a.ctl = c->ctl + solve;
     a.ctl_all = c->ctl;
     a.hist = c->hist;
-    a.pin = pbuf(c, 0);
-    a.pout = pbuf(c, 1);
-    a.v = cg_vecs<RV>(c, solve);
+    a.pin = pbuf(c, kResPart);
+    a.pout = pbuf(c, kIterPart);
+    a.v = cg_vecs<RV>(c, solve, 0);
     return a;
 }
 
@@ -765,7 +830,7 @@ int launch_cells(cmx_ctx* c)
 }
 
 template <class Rows, class RV>
-int launch_residual(cmx_ctx* c, const Rows& R, int solve, bool refine_sweep)
+int launch_residual(cmx_ctx* c, const Rows& R, int solve)
 {
     ResidualArgs<RV> a;
     a.n = c->n;
@@ -774,73 +839,100 @@ int launch_residual(cmx_ctx* c, const Rows& R, int solve, bool refine_sweep)
     a.V = c->V;
     a.F = (const RV*)c->F;
     a.r0_64 = c->r0_64;
-    a.guess = (!refine_sweep && c->extrapolate) ? 1 : 0;
-    a.v = cg_vecs<RV>(c, solve);
-    a.partials = pbuf(c, 0);
+    a.x1 = (const RV*)c->vec[V_X1];
+    a.v = cg_vecs<RV>(c, solve, 0);
+    a.partials = pbuf(c, kResPart);
     a.ctl = c->ctl + solve;
     KernelTimer timer(c, CMX_K_RESIDUAL);
-    if (refine_sweep)
-        refine_residual_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mat_vals(c), a, (const RV*)c->vec[5]);
+    if (solve == 1)
+        refine_residual_kernel<Rows, RV><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
+    else if (c->extrapolate)
+        residual_kernel<Rows, RV, true><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
     else
-        residual_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mat_vals(c), a);
+        residual_kernel<Rows, RV, false><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
 
-// CG iterations from..from+count-1 of a solve: each is (spmv, update); the
-// very first spmv of a step's first solve sets the tolerance.
+// Start of a solve (w0 = A u0 and its dot products; ends a solve whose
+// residual is already below the threshold).
 template <class Rows, class RV>
-int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, bool first, int from, int count)
+int launch_cg_start(cmx_ctx* c, const Rows& R, int solve)
 {
-    CgArgs<RV> a = cg_args<RV>(c, solve, false);
+    CgArgs<RV> a = cg_args<RV>(c, solve);
+    a.first = solve == 0 ? 1 : 0;
+    a.v.m_out = (RV*)c->vec[V_MA];   // read by iteration 0
+    KernelTimer timer(c, CMX_K_SPMV);
+    cg_start_kernel<Rows, RV><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
+    CMX_HIP(c, hipGetLastError());
+    return CMX_OK;
+}
+
+// CG iterations from .. from+count-1 of a solve, one kernel each.
+template <class Rows, class RV>
+int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, int from, int count)
+{
+    CgArgs<RV> a = cg_args<RV>(c, solve);
     const MatVals mv = mat_vals(c);
     for (int it = from; it < from + count; ++it) {
-        a.first = (first && it == 0) ? 1 : 0;
-        a.it0 = it == 0 ? 1 : 0;
-        a.pin = pbuf(c, 0);
-        a.pout = pbuf(c, 1);
-        {
-            KernelTimer timer(c, CMX_K_SPMV);
-            cg_spmv_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mv, a);
-        }
-        a.pin = pbuf(c, 1);
-        {
-            KernelTimer timer(c, CMX_K_UPDATE);
-            cg_update_kernel<Rows, RV><<<c->G, kBlock, 0, c->stream>>>(R, mv, a, it);
-        }
+        a.pin = pbuf(c, kIterPart + (it & 1));
+        a.pout = pbuf(c, kIterPart + ((it + 1) & 1));
+        a.v = cg_vecs<RV>(c, solve, it);
+        KernelTimer timer(c, CMX_K_UPDATE);
+        cg_iter_kernel<Rows, RV><<<c->G, kCgBlock, 0, c->stream>>>(R, mv, a, it);
     }
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
 
+// verdict: the reduction the final solve's last launched kernel wrote.
 template <class RV>
-int launch_finish(cmx_ctx* c, bool refined)
+int launch_finish(cmx_ctx* c, bool refined, const double* verdict)
 {
+    FinishArgs<RV> a;
+    a.n = c->n;
+    a.clock = c->clock;
+    a.ctl = c->ctl + (refined ? 1 : 0);
+    a.ctl_first = refined ? c->ctl : nullptr;
+    a.pin = verdict;
+    a.its_log = c->its_log;
+    a.log_len = c->batch;
+    a.V = c->V;
+    a.x1 = (const RV*)c->vec[V_X1];
+    a.x2 = refined ? (const RV*)c->vec[V_X2] : nullptr;
     KernelTimer timer(c, CMX_K_FINISH);
-    finish_kernel<RV><<<grid_of(c->n), kBlock, 0, c->stream>>>(
-        c->n, c->clock, c->ctl + (refined ? 1 : 0), refined ? c->ctl : nullptr, c->its_log, c->batch, c->V,
-        (const RV*)c->vec[5], refined ? (const RV*)c->vec[6] : nullptr);
+    finish_kernel<RV><<<c->G, kCgBlock, 0, c->stream>>>(a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
 
-// Enqueue one whole step with an iteration budget of c->budget per solve:
-// budget + 1 (spmv, update) pairs, the last update detecting convergence.
+// Enqueue one whole step with iteration budgets B (first solve) and B2
+// (refinement solve): cells, residual, start, B iterations, [refinement
+// residual, (start, B2 iterations)], finish.  A solve that needs more than its
+// budget is either continued by the refinement solve (first solve) or
+// resumed by the host after the finish kernel poisons the clock.
 template <class Rows, class RS, class RV>
 int enqueue_step(cmx_ctx* c, const Rows& R)
 {
     int rc;
     if ((rc = launch_cells<RS, RV>(c))) return rc;
-    if ((rc = launch_residual<Rows, RV>(c, R, 0, false))) return rc;
-    if ((rc = launch_cg_iters<Rows, RV>(c, R, 0, true, 0, c->budget + 1))) return rc;
+    if ((rc = launch_residual<Rows, RV>(c, R, 0))) return rc;
+    if ((rc = launch_cg_start<Rows, RV>(c, R, 0))) return rc;
+    if ((rc = launch_cg_iters<Rows, RV>(c, R, 0, 0, c->budget))) return rc;
     const bool refined = c->precision == CMX_MIXED && c->refine > 0;
+    const double* verdict = pbuf(c, kIterPart + (c->budget & 1));
     if (refined) {
         // FP64 refinement: true residual of V^n + x1 against the step's b,
-        // then a second CG solve on it into x2.
-        if ((rc = launch_residual<Rows, RV>(c, R, 1, true))) return rc;
-        if ((rc = launch_cg_iters<Rows, RV>(c, R, 1, false, 0, c->refine_budget + 1))) return rc;
+        // then a second CG solve on it into x2 when the budget allows one.
+        if ((rc = launch_residual<Rows, RV>(c, R, 1))) return rc;
+        verdict = pbuf(c, kResPart);
+        if (c->refine_budget > 0) {
+            if ((rc = launch_cg_start<Rows, RV>(c, R, 1))) return rc;
+            if ((rc = launch_cg_iters<Rows, RV>(c, R, 1, 0, c->refine_budget))) return rc;
+            verdict = pbuf(c, kIterPart + (c->refine_budget & 1));
+        }
     }
-    return launch_finish<RV>(c, refined);
+    return launch_finish<RV>(c, refined, verdict);
 }
 
 int sync_state(cmx_ctx* c)
@@ -940,30 +1032,37 @@ int run_steps(cmx_ctx* c, const Rows& R)
             else c->budget = std::min(c->budget + 2, c->max_its);
             // Clear the poison first: every kernel launched while it is set is a no-op.
             CMX_HIP(c, hipMemsetAsync(&c->clock->poison, 0, sizeof(int32_t), c->stream));
+            const double* verdict = pbuf(c, kResPart);
             if (ctl.done == 0) {
                 const int from = ctl.it;
+                // the start kernel is idempotent; a refinement solve with a
+                // zero budget has not run it yet
+                if (from == 0 && (rc = launch_cg_start<Rows, RV>(c, R, solve))) return rc;
                 const int chunk = std::max(1, std::min(4, c->max_its + 1 - from));
-                if ((rc = launch_cg_iters<Rows, RV>(c, R, solve, false, from, chunk))) return rc;
+                if ((rc = launch_cg_iters<Rows, RV>(c, R, solve, from, chunk))) return rc;
+                verdict = pbuf(c, kIterPart + ((from + chunk) & 1));
             }
-            if ((rc = launch_finish<RV>(c, refined))) return rc;
+            if ((rc = launch_finish<RV>(c, refined, verdict))) return rc;
             if ((rc = sync_state(c))) return rc;
         }
         const int got = c->h_clock->k - k_start;
         if (got <= 0) return fail(c, CMX_E_STATE, "no step completed in a batch");
-        // iteration statistics and the next budget
-        CMX_HIP(c, hipMemcpy(c->h_its, c->its_log, 2 * c->batch * sizeof(int32_t), hipMemcpyDeviceToHost));
+        // iteration statistics and the next budgets
+        CMX_HIP(c, hipMemcpy(c->h_its, c->its_log, 4 * c->batch * sizeof(int32_t), hipMemcpyDeviceToHost));
         int mx0 = 0, mx1 = 0;
+        bool short0 = false;
         for (int s = 0; s < got; ++s) {
-            const int slot = 2 * ((k_start + s) % c->batch);
-            const int a = c->h_its[slot], b = c->h_its[slot + 1];
-            c->cg_total += a + b;
-            mx0 = std::max(mx0, a);
-            mx1 = std::max(mx1, b);
-            c->last_its = a + b;
+            const int32_t* slot = c->h_its + 4 * ((k_start + s) % c->batch);
+            c->cg_total += slot[0] + slot[2];
+            mx0 = std::max(mx0, slot[0]);
+            mx1 = std::max(mx1, slot[2]);
+            short0 |= slot[1] == 0;
+            c->last_its = slot[0] + slot[2];
         }
         if (refined) c->refine_total += got;
-        // budgets: the largest count of the batch (the pair after it detects convergence)
-        c->budget = std::min(std::max(mx0, 1), c->max_its);
+        // budgets: the largest count of the batch; a first solve that the
+        // refinement had to finish gets one more iteration
+        c->budget = std::min(short0 ? std::max(mx0, c->budget) + 1 : mx0, c->max_its);
         c->refine_budget = std::min(mx1, c->max_its);
         c->steps += got;
         if (c->h_clock->nonfinite)
@@ -1069,8 +1168,9 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->nnz = csr.row_ptr[c->n];
     c->width = csr.max_row;
     const int n = c->n;
-    c->G = std::min(grid_of(n), 1024);
-    if (const char* e = std::getenv("CMX_CG_BLOCKS")) c->G = std::max(1, std::min(grid_of(n), std::min(atoi(e), kMaxBlocks)));
+    const int cg_rows = (n + kCgBlock - 1) / kCgBlock;
+    c->G = std::min(cg_rows, kCgGrid);
+    if (const char* e = std::getenv("CMX_CG_BLOCKS")) c->G = std::max(1, std::min(cg_rows, std::min(atoi(e), kMaxBlocks)));
 
     CMX_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
@@ -1193,7 +1293,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
 
     // Work vectors.
     if ((rc = dalloc(c, (char**)&c->F, n * rv))) return rc;
-    for (int v = 0; v < 7; ++v) {
+    for (int v = 0; v < V_COUNT; ++v) {
         if ((rc = dalloc(c, (char**)&c->vec[v], n * rv))) return rc;
         CMX_HIP(c, hipMemset(c->vec[v], 0, n * rv));
     }
@@ -1206,8 +1306,8 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     if ((rc = dalloc(c, &c->clock, 1))) return rc;
     if ((rc = dalloc(c, &c->ctl, 2))) return rc;
     if ((rc = dalloc(c, &c->hist, (size_t)c->max_its + 2))) return rc;
-    if ((rc = dalloc(c, &c->its_log, 2 * (size_t)c->batch))) return rc;
-    CMX_HIP(c, hipMemset(c->its_log, 0, 2 * c->batch * sizeof(int32_t)));
+    if ((rc = dalloc(c, &c->its_log, 4 * (size_t)c->batch))) return rc;
+    CMX_HIP(c, hipMemset(c->its_log, 0, 4 * c->batch * sizeof(int32_t)));
     SolveCtl z{};
     z.max_its = c->max_its;
     SolveCtl z2[2] = {z, z};
@@ -1216,7 +1316,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     CMX_HIP(c, hipMemcpy(c->clock, &ck, sizeof(ck), hipMemcpyHostToDevice));
     CMX_HIP(c, hipHostMalloc((void**)&c->h_clock, sizeof(StepClock), hipHostMallocDefault));
     CMX_HIP(c, hipHostMalloc((void**)&c->h_ctl, 2 * sizeof(SolveCtl), hipHostMallocDefault));
-    CMX_HIP(c, hipHostMalloc((void**)&c->h_its, 2 * c->batch * sizeof(int32_t), hipHostMallocDefault));
+    CMX_HIP(c, hipHostMalloc((void**)&c->h_its, 4 * c->batch * sizeof(int32_t), hipHostMallocDefault));
     *c->h_clock = ck;
     CMX_HIP(c, hipDeviceSynchronize());
     return CMX_OK;
@@ -1314,7 +1414,7 @@ int cmx_set_voltage(cmx_ctx* c, const double* V)
     CMX_HIP(c, hipMemcpy(c->V, V, c->n * sizeof(double), hipMemcpyHostToDevice));
     // the previous correction no longer predicts the next one
     const size_t rv = c->precision == CMX_MIXED ? sizeof(float) : sizeof(double);
-    CMX_HIP(c, hipMemset(c->vec[5], 0, c->n * rv));
+    CMX_HIP(c, hipMemset(c->vec[V_X1], 0, c->n * rv));
     return CMX_OK;
 }
 

@@ -1,13 +1,15 @@
 // gfx950 kernels of the monodomain time-step: shared device helpers.
-// 256-thread blocks (four 64-lane wavefronts); each block owns a contiguous
-// slice of rows and block indices are remapped so that the blocks one XCD
-// runs cover a contiguous range (its private L2 then holds the neighbouring
-// planes the stencil gathers).
+// The cell kernel runs 256-thread blocks, one node per thread.  The solver
+// kernels run 1024-thread blocks (sixteen 64-lane wavefronts), by default one
+// block per CU; each block owns a contiguous slice of rows and block indices
+// are remapped so that the blocks one XCD runs cover a contiguous range (its
+// private L2 then holds the neighbouring planes the stencil gathers).
 //
 // Reductions never use fences or atomics: every producing block writes its
 // partial sums to its own slot, and every block of the *next* kernel sums all
 // slots in the same fixed order (the kernel boundary orders the writes).  The
 // totals are therefore bitwise identical in every block and on every run.
+// With G blocks a reduction costs G^2 slot reads, hence few, large blocks.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -16,7 +18,9 @@
 
 namespace cmx {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;       // cell kernel
+constexpr int kCgBlock = 512;     // solver kernels
+constexpr int kCgGrid = 512;      // default solver blocks (two per CU)
 constexpr int kWave = 64;
 constexpr int kNumXcd = 8;
 constexpr int kMaxBlocks = 2048;   // partial-sum slots per reduction
@@ -122,7 +126,7 @@ template <class Rows, class T>
 __device__ __forceinline__ const T* stage_table(const Rows&, const T* __restrict__ g, T* lds, int count)
 {
     if constexpr (Rows::kStaged) {
-        for (int q = threadIdx.x; q < count; q += kBlock) lds[q] = g[q];
+        for (int q = threadIdx.x; q < count; q += blockDim.x) lds[q] = g[q];
         return lds;
     } else {
         (void)lds;
@@ -140,36 +144,38 @@ __device__ __forceinline__ void wave_sum(double (&v)[N])
         for (int j = 0; j < N; ++j) v[j] += __shfl_xor(v[j], s, kWave);
 }
 
-// Block-sum of N doubles into out[blk*N + j] (plain stores, thread 0).
-template <int N>
+// Block-sum of N doubles into out[blk*N + j] (BS threads; plain stores).
+template <int BS, int N>
 __device__ __forceinline__ void write_partials(double (&v)[N], double* __restrict__ out, int blk)
 {
-    __shared__ double red[kBlock / kWave][N];
+    constexpr int NW = BS / kWave;
+    __shared__ double red[NW][N];
     wave_sum<N>(v);
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     if (lane == 0)
 #pragma unroll
         for (int j = 0; j < N; ++j) red[wid][j] = v[j];
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (threadIdx.x < N) {
+        const int j = threadIdx.x;
+        double s = 0;
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            double s = 0;
-#pragma unroll
-            for (int w = 0; w < kBlock / kWave; ++w) s += red[w][j];
-            out[(size_t)blk * N + j] = s;
-        }
+        for (int w = 0; w < NW; ++w) s += red[w][j];
+        out[(size_t)blk * N + j] = s;
+    }
 }
 
-// Sum of the G slots written by the previous kernel, identical in every block.
-template <int N>
+// Sum of the G slots written by the previous kernel, identical in every
+// block (BS threads, fixed order).
+template <int BS, int N>
 __device__ __forceinline__ void read_partials(const double* __restrict__ in, int G, double (&tot)[N])
 {
-    __shared__ double red[kBlock / kWave][N];
+    constexpr int NW = BS / kWave;
+    __shared__ double red[NW][N];
     double acc[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) acc[j] = 0;
-    for (int b = threadIdx.x; b < G; b += kBlock)
+    for (int b = threadIdx.x; b < G; b += BS)
 #pragma unroll
         for (int j = 0; j < N; ++j) acc[j] += in[(size_t)b * N + j];
     wave_sum<N>(acc);
@@ -182,7 +188,7 @@ __device__ __forceinline__ void read_partials(const double* __restrict__ in, int
     for (int j = 0; j < N; ++j) {
         double s = 0;
 #pragma unroll
-        for (int w = 0; w < kBlock / kWave; ++w) s += red[w][j];
+        for (int w = 0; w < NW; ++w) s += red[w][j];
         tot[j] = s;
     }
     __syncthreads();
