@@ -91,6 +91,11 @@ struct CellArgs {
     const uint8_t* __restrict__ stim_id;  // 0 = none, k = row k-1
     const cmx_stimulus* __restrict__ stim;
     RV* __restrict__ F;        // out: -Am Iion - Istim(t_next)
+    // CG initial guess V^n + x0 (x0 = the previous step's correction), when set:
+    const RV* __restrict__ guess;   // x0
+    double c;                       // Am Cm / dt
+    double* __restrict__ Y;         // out: V^n + x0
+    RV* __restrict__ G1;            // out: F - c x0
 };
 
 // One thread per node; the state lives in registers for all substeps.
@@ -134,7 +139,13 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
     }
     const RS iion = Model::template iion<RS>(y, V, p, cm);
     const double istim = sid ? stimulus_at(a.stim[sid - 1], t_next) : 0.0;
-    a.F[i] = (RV)(-a.am * (double)iion - istim);
+    const RV f = (RV)(-a.am * (double)iion - istim);
+    a.F[i] = f;
+    if (a.guess) {
+        const double x0 = (double)a.guess[i];
+        a.Y[i] = a.V[i] + x0;
+        a.G1[i] = (RV)((double)f - a.c * x0);
+    }
 #pragma unroll
     for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.n + i] = y[s];
 }
@@ -206,79 +217,102 @@ struct CgVecs {
     RV* x;       // correction being solved for
 };
 
+// Every solver kernel below issues all of its global loads (tables, the
+// previous reduction, the rows' own and gathered values) before its first
+// barrier, so one memory round trip covers them; the iteration kernel is the
+// exception (it decides from the reduction first, so that launches after
+// convergence cost no row traffic).
+
 template <class RV>
 struct ResidualArgs {
     int n;
     double c;                          // Am Cm / dt
     const StepClock* clock;
-    const double* __restrict__ V;
-    const RV* __restrict__ F;
-    double* __restrict__ r0_64;        // FP64 residual of V^n, kept for the refinement sweep
+    const double* __restrict__ Z;      // V^n, or V^n + x0 with an initial guess
+    const RV* __restrict__ G;          // F, or F - c x0 with an initial guess
+    double* __restrict__ r0_64;        // FP64 residual, kept for the refinement sweep
     const RV* __restrict__ x1;         // refinement sweep: the first solve's correction
     CgVecs<RV> v;
     double* __restrict__ partials;     // [G][3]: ||D^-1 b||^2, 0, ||D^-1 r0||^2
     SolveCtl* ctl;
 };
 
-// r0 = M F - K V - A x0 with x0 = 0 (the reference's warm start from V^n) or,
-// with GUESS, x0 = the previous step's correction, which x already holds
-// (read here by neighbours, so never written by this kernel);
-// b = c M V + M F (only ||D^-1 b|| is needed); u0 = D^-1 r0 goes to p for the
-// solve that follows, r0 is also kept in FP64 for the refinement sweep.  The
-// stiffness term uses differences (V_j - V_i): K's rows sum to zero, and the
-// differences keep the diffusion flux free of cancellation against the
-// ~85 mV resting level.
-template <class Rows, class RV, bool GUESS>
-__device__ __forceinline__ void residual_row(const Rows& R, int i, const double* __restrict__ Mv,
-                                             const double* __restrict__ Kv, const double* __restrict__ Dv,
-                                             const RV* __restrict__ D32, const ResidualArgs<RV>& a, double (&part)[3])
-{
-    const auto row = R.row(i);
-    const double vi = a.V[i];
-    double mf = 0, mvv = 0, kx = 0, mg = 0, kg = 0, gi = 0;
-    if constexpr (GUESS) gi = (double)a.v.x[i];
-#pragma unroll
-    for (int k = 0; k < R.width(); ++k) {
-        const int j = R.col(row, k);
-        const int s = R.vidx(row, k);
-        const double vj = a.V[j];
-        mf += Mv[s] * (double)a.F[j];
-        mvv += Mv[s] * vj;
-        kx += Kv[s] * (vj - vi);
-        if constexpr (GUESS) {
-            const double gj = (double)a.v.x[j];
-            mg += Mv[s] * gj;
-            kg += Kv[s] * (gj - gi);
-        }
+// r0 = b - A Z for the solve A d = r0 (V^{n+1} = Z + d) with
+// b = M (c V^n + F) and A = c M + K (c = Am Cm / dt).  Z = V^n is the
+// reference's warm start (KSPSetInitialGuessNonzero from V^n); with an
+// initial guess Z = V^n + x0 and G = F - c x0, otherwise G = F, and in both
+// cases r0 = M G - K Z and b = M (G + c Z).  u0 = D^-1 r0 goes to p for the
+// solve that follows, d = 0, and r0 is also kept in FP64 for the refinement
+// sweep.  The stiffness term uses differences (Z_j - Z_i): K's rows sum to
+// zero, and the differences keep the diffusion flux free of cancellation
+// against the ~85 mV resting level.
+template <class Rows, class RV>
+struct ResidualRow {
+    typename Rows::Row row;
+    int i;
+    double zi;
+    Gathered<Rows, double> Z;
+    Gathered<Rows, RV> G;
+    __device__ __forceinline__ void load(const Rows& R, int i_, const ResidualArgs<RV>& a)
+    {
+        i = i_;
+        row = R.row(i);
+        zi = a.Z[i];
+        Z.load(R, row, a.Z);
+        G.load(R, row, a.G);
     }
-    const int d = R.didx(row);
-    const double bi = Dv[d] * (a.c * mvv + mf);
-    const double r64 = mf - kx;                 // b - A V^n
-    if (a.r0_64) a.r0_64[i] = r64;
-    const RV ri = (RV)(r64 - a.c * mg - kg);    // b - A (V^n + x0)
-    const RV ui = D32[d] * ri;
-    a.v.r[i] = ri;
-    a.v.p[i] = ui;
-    if constexpr (!GUESS) a.v.x[i] = RV(0);
-    part[0] += bi * bi;
-    part[2] += (double)ui * (double)ui;
-}
+    __device__ __forceinline__ void apply(const Rows& R, const double* __restrict__ Mv, const double* __restrict__ Kv,
+                                          const double* __restrict__ Dv, const RV* __restrict__ D32,
+                                          const ResidualArgs<RV>& a, double (&part)[3]) const
+    {
+        double mg = 0, mz = 0, kz = 0;
+#pragma unroll
+        for (int k = 0; k < R.width(); ++k) {
+            const int s = R.vidx(row, k);
+            const double zj = Z.at(R, row, a.Z, k);
+            mg += Mv[s] * (double)G.at(R, row, a.G, k);
+            mz += Mv[s] * zj;
+            kz += Kv[s] * (zj - zi);
+        }
+        const int d = R.didx(row);
+        const double bi = Dv[d] * (a.c * mz + mg);
+        const double r64 = mg - kz;
+        if (a.r0_64) a.r0_64[i] = r64;
+        const RV ri = (RV)r64;
+        const RV ui = D32[d] * ri;
+        a.v.r[i] = ri;
+        a.v.p[i] = ui;
+        a.v.x[i] = RV(0);
+        part[0] += bi * bi;
+        part[2] += (double)ui * (double)ui;
+    }
+};
 
-template <class Rows, class RV, bool GUESS>
+template <class Rows, class RV>
 __global__ __launch_bounds__(kCgBlock) void residual_kernel(Rows R, MatVals mv, ResidualArgs<RV> a)
 {
-    if (a.clock->poison) return;
     __shared__ double sM[kMaxPatterns * kMaxStencil], sK[kMaxPatterns * kMaxStencil], sD[kMaxPatterns];
-    const double* __restrict__ Mv = stage_table(R, mv.m64, sM, R.value_count());
-    const double* __restrict__ Kv = stage_table(R, mv.k64, sK, R.value_count());
-    const double* __restrict__ Dv = stage_table(R, mv.dinv64, sD, R.diag_count());
-    if constexpr (Rows::kStaged) __syncthreads();
-    const RV* __restrict__ D32 = dinv_vals<RV>(mv);
+    const TableLoad<Rows, double> tm(mv.m64, R.value_count()), tk(mv.k64, R.value_count()),
+        td(mv.dinv64, R.diag_count());
+    const int poison = a.clock->poison;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const RowRange rr(a.n, blk, gridDim.x);
+    const int i0 = rr.lo + threadIdx.x;
+    ResidualRow<Rows, RV> q;
+    if (i0 < rr.hi) q.load(R, i0, a);
+    const double* __restrict__ Mv = tm.store(sM, mv.m64);
+    const double* __restrict__ Kv = tk.store(sK, mv.k64);
+    const double* __restrict__ Dv = td.store(sD, mv.dinv64);
+    if constexpr (Rows::kStaged) __syncthreads();
+    if (poison) return;
+    const RV* __restrict__ D32 = dinv_vals<RV>(mv);
     double part[3] = {0.0, 0.0, 0.0};
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock)
-        residual_row<Rows, RV, GUESS>(R, i, Mv, Kv, Dv, D32, a, part);
+    if (i0 < rr.hi) q.apply(R, Mv, Kv, Dv, D32, a, part);
+    for (int i = i0 + kCgBlock; i < rr.hi; i += kCgBlock) {
+        ResidualRow<Rows, RV> q2;
+        q2.load(R, i, a);
+        q2.apply(R, Mv, Kv, Dv, D32, a, part);
+    }
     write_partials<kCgBlock, 3>(part, a.partials, blk);
     if (blk == 0 && threadIdx.x == 0) {
         a.ctl->done = 0;
@@ -290,27 +324,56 @@ __global__ __launch_bounds__(kCgBlock) void residual_kernel(Rows R, MatVals mv, 
 // r = r0 - A x1 (A in FP64), then u = D^-1 r to p and x2 = 0 for the second
 // solve.  Partials: (0, 0, ||D^-1 r||^2).
 template <class Rows, class RV>
-__global__ __launch_bounds__(kCgBlock) void refine_residual_kernel(Rows R, MatVals mv, ResidualArgs<RV> a)
-{
-    if (a.clock->poison) return;
-    __shared__ double sA[kMaxPatterns * kMaxStencil];
-    const double* __restrict__ Av = stage_table(R, mv.a64, sA, R.value_count());
-    if constexpr (Rows::kStaged) __syncthreads();
-    const RV* __restrict__ D32 = dinv_vals<RV>(mv);
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const RowRange rr(a.n, blk, gridDim.x);
-    double part[3] = {0.0, 0.0, 0.0};
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
-        const auto row = R.row(i);
+struct RefineRow {
+    typename Rows::Row row;
+    int i;
+    double r0;
+    Gathered<Rows, RV> X;
+    __device__ __forceinline__ void load(const Rows& R, int i_, const ResidualArgs<RV>& a)
+    {
+        i = i_;
+        row = R.row(i);
+        r0 = a.r0_64[i];
+        X.load(R, row, a.x1);
+    }
+    __device__ __forceinline__ void apply(const Rows& R, const double* __restrict__ Av, const RV* __restrict__ D32,
+                                          const ResidualArgs<RV>& a, double (&part)[3]) const
+    {
         double ax = 0;
 #pragma unroll
-        for (int k = 0; k < R.width(); ++k) ax += Av[R.vidx(row, k)] * (double)a.x1[R.col(row, k)];
-        const RV ri = (RV)(a.r0_64[i] - ax);
+        for (int k = 0; k < R.width(); ++k) ax += Av[R.vidx(row, k)] * (double)X.at(R, row, a.x1, k);
+        const RV ri = (RV)(r0 - ax);
         const RV ui = D32[R.didx(row)] * ri;
         a.v.r[i] = ri;
         a.v.p[i] = ui;
         a.v.x[i] = RV(0);
         part[2] += (double)ui * (double)ui;
+    }
+};
+
+template <class Rows, class RV>
+__global__ __launch_bounds__(kCgBlock) void refine_residual_kernel(Rows R, MatVals mv, ResidualArgs<RV> a)
+{
+    __shared__ double sA[kMaxPatterns * kMaxStencil];
+    const TableLoad<Rows, double> ta(mv.a64, R.value_count());
+    const int poison = a.clock->poison;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const RowRange rr(a.n, blk, gridDim.x);
+    const int i0 = rr.lo + threadIdx.x, i1 = i0 + kCgBlock;
+    RefineRow<Rows, RV> q0, q1;
+    if (i0 < rr.hi) q0.load(R, i0, a);
+    if (i1 < rr.hi) q1.load(R, i1, a);
+    const double* __restrict__ Av = ta.store(sA, mv.a64);
+    if constexpr (Rows::kStaged) __syncthreads();
+    if (poison) return;
+    const RV* __restrict__ D32 = dinv_vals<RV>(mv);
+    double part[3] = {0.0, 0.0, 0.0};
+    if (i0 < rr.hi) q0.apply(R, Av, D32, a, part);
+    if (i1 < rr.hi) q1.apply(R, Av, D32, a, part);
+    for (int i = i1 + kCgBlock; i < rr.hi; i += kCgBlock) {
+        RefineRow<Rows, RV> q;
+        q.load(R, i, a);
+        q.apply(R, Av, D32, a, part);
     }
     write_partials<kCgBlock, 3>(part, a.partials, blk);
     if (blk == 0 && threadIdx.x == 0) {
@@ -339,17 +402,55 @@ struct CgArgs {
 // into the stopping threshold, and a residual already below it (the usual
 // case of a refinement sweep) ends the solve here with no iteration.
 template <class Rows, class RV>
+struct StartRow {
+    typename Rows::Row row;
+    int i;
+    RV r, u;
+    Gathered<Rows, RV> U;
+    __device__ __forceinline__ void load(const Rows& R, int i_, const CgVecs<RV>& v)
+    {
+        i = i_;
+        row = R.row(i);
+        r = v.r[i];
+        u = v.p[i];
+        U.load(R, row, v.p);
+    }
+    __device__ __forceinline__ void apply(const Rows& R, const RV* __restrict__ A, const RV* __restrict__ D,
+                                          const CgVecs<RV>& v, double (&part)[3]) const
+    {
+        RV w = 0;
+#pragma unroll
+        for (int k = 0; k < R.width(); ++k) w += A[R.vidx(row, k)] * U.at(R, row, v.p, k);
+        v.w[i] = w;
+        v.m_out[i] = D[R.didx(row)] * w;
+        part[0] += (double)r * (double)u;
+        part[1] += (double)w * (double)u;
+        part[2] += (double)u * (double)u;
+    }
+};
+
+template <class Rows, class RV>
 __global__ __launch_bounds__(kCgBlock) void cg_start_kernel(Rows R, MatVals mv, CgArgs<RV> a)
 {
-    if (a.clock->poison) return;
     __shared__ RV sA[kMaxPatterns * kMaxStencil], sD[kMaxPatterns];
-    const RV* __restrict__ A = stage_table(R, a_vals<RV>(mv), sA, R.value_count());
-    const RV* __restrict__ D = stage_table(R, dinv_vals<RV>(mv), sD, R.diag_count());
-    if constexpr (Rows::kStaged) __syncthreads();
+    const TableLoad<Rows, RV> ta(a_vals<RV>(mv), R.value_count()), td(dinv_vals<RV>(mv), R.diag_count());
+    PartialLoad<3> pl(a.pin, gridDim.x);
+    const int poison = a.clock->poison;
+    const double tol2_set = a.ctl->tol2;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const RowRange rr(a.n, blk, gridDim.x);
+    // FP32 rows: two rows per thread in registers; FP64: one (register budget)
+    constexpr bool kTwo = sizeof(RV) == 4;
+    const int i0 = rr.lo + threadIdx.x, i1 = i0 + kCgBlock;
+    StartRow<Rows, RV> q0, q1;
+    if (i0 < rr.hi) q0.load(R, i0, a.v);
+    if (kTwo && i1 < rr.hi) q1.load(R, i1, a.v);
+    const RV* __restrict__ A = ta.store(sA, a_vals<RV>(mv));
+    const RV* __restrict__ D = td.store(sD, dinv_vals<RV>(mv));
     double g0[3];
-    read_partials<kCgBlock, 3>(a.pin, gridDim.x, g0);
-    const double tol2 = a.first ? fmax(a.rtol2 * g0[0], a.atol2) : a.ctl->tol2;
+    pl.total(g0);   // its barriers also cover the table stores
+    if (poison) return;
+    const double tol2 = a.first ? fmax(a.rtol2 * g0[0], a.atol2) : tol2_set;
     const bool lead = blk == 0 && threadIdx.x == 0;
     if (lead && a.first) {
         a.ctl_all[0].bb = g0[0];
@@ -361,20 +462,13 @@ __global__ __launch_bounds__(kCgBlock) void cg_start_kernel(Rows R, MatVals mv, 
         if (lead) { a.ctl->done = 1; a.ctl->it = 0; a.ctl->zz = g0[2]; }
         return;
     }
-    const RV* __restrict__ u = a.v.p;
-    const RowRange rr(a.n, blk, gridDim.x);
     double part[3] = {0.0, 0.0, 0.0};
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
-        const auto row = R.row(i);
-        RV w = 0;
-#pragma unroll
-        for (int k = 0; k < R.width(); ++k) w += A[R.vidx(row, k)] * u[R.col(row, k)];
-        const RV ri = a.v.r[i], ui = u[i];
-        a.v.w[i] = w;
-        a.v.m_out[i] = D[R.didx(row)] * w;
-        part[0] += (double)ri * (double)ui;
-        part[1] += (double)w * (double)ui;
-        part[2] += (double)ui * (double)ui;
+    if (i0 < rr.hi) q0.apply(R, A, D, a.v, part);
+    if (kTwo && i1 < rr.hi) q1.apply(R, A, D, a.v, part);
+    for (int i = (kTwo ? i1 : i0) + kCgBlock; i < rr.hi; i += kCgBlock) {
+        StartRow<Rows, RV> q;
+        q.load(R, i, a.v);
+        q.apply(R, A, D, a.v, part);
     }
     write_partials<kCgBlock, 3>(part, a.pout, blk);
 }
@@ -385,8 +479,7 @@ __global__ __launch_bounds__(kCgBlock) void cg_start_kernel(Rows R, MatVals mv, 
 // their own except m, which the stencil gathers).  One kernel per iteration:
 // the dot products gamma_i = (r_i, u_i), delta_i = (w_i, u_i) and
 // zeta_i = ||u_i||^2 come reduced from the previous kernel, and the one
-// matrix product n_i = A m_i does not depend on them, so the gathers overlap
-// the reduction:
+// matrix product n_i = A m_i does not depend on them:
 //   beta_i = gamma_i / gamma_{i-1},
 //   alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1})   (i > 0)
 //   z = n + beta z;  s = w + beta s;  p = u + beta p;
@@ -394,94 +487,97 @@ __global__ __launch_bounds__(kCgBlock) void cg_start_kernel(Rows R, MatVals mv, 
 // and the new dot products go to the next kernel.  The iteration stops (x
 // final) when zeta_i <= tol2, PETSc's preconditioned-norm test; the kernel
 // that consumes the solve applies the same test to the last reduction.
-template <class RV>
-struct IterRow {
-    RV n, d, r, w, z, s, p, x;
-};
-
 template <class Rows, class RV>
-__device__ __forceinline__ void iter_load(const Rows& R, const RV* __restrict__ A, const RV* __restrict__ D,
-                                          const CgVecs<RV>& v, int i, bool first_it, IterRow<RV>& q)
-{
-    const auto row = R.row(i);
-    RV n = 0;
+struct IterRow {
+    typename Rows::Row row;
+    int i;
+    RV r, w, x, z, s, p;
+    Gathered<Rows, RV> M;
+    __device__ __forceinline__ void load(const Rows& R, int i_, const CgVecs<RV>& v, bool first_it)
+    {
+        i = i_;
+        row = R.row(i);
+        M.load(R, row, v.m_in);
+        r = v.r[i];
+        w = v.w[i];
+        x = v.x[i];
+        if (!first_it) {
+            z = v.z[i];
+            s = v.s[i];
+            p = v.p[i];
+        }
+    }
+    __device__ __forceinline__ void apply(const Rows& R, const RV* __restrict__ A, const RV* __restrict__ D,
+                                          const CgVecs<RV>& v, bool first_it, RV al, RV be, double (&part)[3]) const
+    {
+        RV n = 0;
 #pragma unroll
-    for (int k = 0; k < R.width(); ++k) n += A[R.vidx(row, k)] * v.m_in[R.col(row, k)];
-    q.n = n;
-    q.d = D[R.didx(row)];
-    q.r = v.r[i];
-    q.w = v.w[i];
-    q.x = v.x[i];
-    if (!first_it) {
-        q.z = v.z[i];
-        q.s = v.s[i];
-        q.p = v.p[i];
+        for (int k = 0; k < R.width(); ++k) n += A[R.vidx(row, k)] * M.at(R, row, v.m_in, k);
+        const RV d = D[R.didx(row)];
+        RV zn, sn, pn;
+        if (first_it) {
+            zn = n;
+            sn = w;
+            pn = d * r;
+        } else {
+            zn = n + be * z;
+            sn = w + be * s;
+            pn = d * r + be * p;
+        }
+        const RV rn = r - al * sn;
+        const RV wn = w - al * zn;
+        const RV un = d * rn;
+        v.z[i] = zn;
+        v.s[i] = sn;
+        v.p[i] = pn;
+        v.x[i] = x + al * pn;
+        v.r[i] = rn;
+        v.w[i] = wn;
+        v.m_out[i] = d * wn;
+        part[0] += (double)rn * (double)un;
+        part[1] += (double)wn * (double)un;
+        part[2] += (double)un * (double)un;
     }
-}
-
-template <class RV>
-__device__ __forceinline__ void iter_store(const CgVecs<RV>& v, int i, bool first_it, RV al, RV be,
-                                           const IterRow<RV>& q, double (&part)[3])
-{
-    RV zn, sn, pn;
-    if (first_it) {
-        zn = q.n;
-        sn = q.w;
-        pn = q.d * q.r;
-    } else {
-        zn = q.n + be * q.z;
-        sn = q.w + be * q.s;
-        pn = q.d * q.r + be * q.p;
-    }
-    const RV rn = q.r - al * sn;
-    const RV wn = q.w - al * zn;
-    const RV un = q.d * rn;
-    v.z[i] = zn;
-    v.s[i] = sn;
-    v.p[i] = pn;
-    v.x[i] = q.x + al * pn;
-    v.r[i] = rn;
-    v.w[i] = wn;
-    v.m_out[i] = q.d * wn;
-    part[0] += (double)rn * (double)un;
-    part[1] += (double)wn * (double)un;
-    part[2] += (double)un * (double)un;
-}
+};
 
 template <class Rows, class RV>
 __global__ __launch_bounds__(kCgBlock) void cg_iter_kernel(Rows R, MatVals mv, CgArgs<RV> a, int it)
 {
-    if (a.clock->poison) return;
-    SolveCtl* ctl = a.ctl;
-    if (ctl->done) return;
     __shared__ RV sA[kMaxPatterns * kMaxStencil], sD[kMaxPatterns];
-    const RV* __restrict__ A = stage_table(R, a_vals<RV>(mv), sA, R.value_count());
-    const RV* __restrict__ D = stage_table(R, dinv_vals<RV>(mv), sD, R.diag_count());
-    if constexpr (Rows::kStaged) __syncthreads();
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const RowRange rr(a.n, blk, gridDim.x);
-    const bool first_it = it == 0;
-    // The first two rows of every thread are loaded before the reduction.
-    const int i0 = rr.lo + threadIdx.x, i1 = i0 + kCgBlock;
-    IterRow<RV> q0, q1;
-    if (i0 < rr.hi) iter_load(R, A, D, a.v, i0, first_it, q0);
-    if (i1 < rr.hi) iter_load(R, A, D, a.v, i1, first_it, q1);
+    const TableLoad<Rows, RV> ta(a_vals<RV>(mv), R.value_count()), td(dinv_vals<RV>(mv), R.diag_count());
+    PartialLoad<3> pl(a.pin, gridDim.x);
+    SolveCtl* ctl = a.ctl;
+    const int poison = a.clock->poison;
+    const int done = ctl->done;
+    const double tol2 = ctl->tol2;
+    const int max_its = ctl->max_its;
+    const RV* __restrict__ A = ta.store(sA, a_vals<RV>(mv));
+    const RV* __restrict__ D = td.store(sD, dinv_vals<RV>(mv));
     double g[3];
-    read_partials<kCgBlock, 3>(a.pin, gridDim.x, g);
+    pl.total(g);    // its barriers also cover the table stores
+    if (poison || done) return;
     const double gamma = g[0], delta = g[1], zeta = g[2];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
     const bool lead = blk == 0 && threadIdx.x == 0;
     if (!(zeta == zeta) || !(gamma == gamma) || !(delta == delta) || fabs(zeta) > DBL_MAX) {
         if (lead) { ctl->done = 3; ctl->it = it; ctl->zz = zeta; }
         return;
     }
-    if (zeta <= ctl->tol2) {
+    if (zeta <= tol2) {
         if (lead) { ctl->done = 1; ctl->it = it; ctl->zz = zeta; }
         return;
     }
-    if (it >= ctl->max_its) {
+    if (it >= max_its) {
         if (lead) { ctl->done = 2; ctl->it = it; ctl->zz = zeta; }
         return;
     }
+    const RowRange rr(a.n, blk, gridDim.x);
+    const bool first_it = it == 0;
+    constexpr bool kTwo = sizeof(RV) == 4;   // rows per thread held in registers: 2 (FP32) or 1
+    const int i0 = rr.lo + threadIdx.x, i1 = i0 + kCgBlock;
+    IterRow<Rows, RV> q0, q1;
+    if (i0 < rr.hi) q0.load(R, i0, a.v, first_it);
+    if (kTwo && i1 < rr.hi) q1.load(R, i1, a.v, first_it);
     double alpha, beta;
     if (first_it) {
         beta = 0.0;
@@ -498,12 +594,12 @@ __global__ __launch_bounds__(kCgBlock) void cg_iter_kernel(Rows R, MatVals mv, C
     }
     const RV al = (RV)alpha, be = (RV)beta;
     double part[3] = {0.0, 0.0, 0.0};
-    if (i0 < rr.hi) iter_store(a.v, i0, first_it, al, be, q0, part);
-    if (i1 < rr.hi) iter_store(a.v, i1, first_it, al, be, q1, part);
-    for (int i = i1 + kCgBlock; i < rr.hi; i += kCgBlock) {
-        IterRow<RV> q;
-        iter_load(R, A, D, a.v, i, first_it, q);
-        iter_store(a.v, i, first_it, al, be, q, part);
+    if (i0 < rr.hi) q0.apply(R, A, D, a.v, first_it, al, be, part);
+    if (kTwo && i1 < rr.hi) q1.apply(R, A, D, a.v, first_it, al, be, part);
+    for (int i = (kTwo ? i1 : i0) + kCgBlock; i < rr.hi; i += kCgBlock) {
+        IterRow<Rows, RV> q;
+        q.load(R, i, a.v, first_it);
+        q.apply(R, A, D, a.v, first_it, al, be, part);
     }
     write_partials<kCgBlock, 3>(part, a.pout, blk);
 }
@@ -521,7 +617,9 @@ struct FinishArgs {
     const double* __restrict__ pin;    // last reduction of the final solve
     int32_t* its_log;                  // [log_len][4]: its first, converged first, its refinement
     int log_len;
-    double* __restrict__ V;
+    double* __restrict__ V;            // out: V^{n+1} = Z + x1 (+ x2)
+    const double* __restrict__ Z;      // V^n, or V^n + x0 with an initial guess
+    RV* __restrict__ guess;            // x0 += x1 (+ x2) when set: the next step's guess
     const RV* __restrict__ x1;
     const RV* __restrict__ x2;
 };
@@ -530,17 +628,25 @@ template <class RV>
 __global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
 {
     StepClock* clock = a.clock;
-    if (clock->poison) return;
+    PartialLoad<3> pl(a.pin, gridDim.x);
+    const int poison = clock->poison;
     const int k = clock->k;
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     int done = a.ctl->done;
-    if (done == 0) {
-        double g[3];
-        read_partials<kCgBlock, 3>(a.pin, gridDim.x, g);
-        if (g[2] <= a.ctl->tol2) {
-            done = 1;
-            if (lead) { a.ctl->done = 1; a.ctl->zz = g[2]; }
-        }
+    const double tol2 = a.ctl->tol2;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const RowRange rr(a.n, blk, gridDim.x);
+    const int i0 = rr.lo + threadIdx.x, i1 = i0 + kCgBlock;
+    RV d0 = 0, d1 = 0;
+    double v0 = 0, v1 = 0;
+    if (i0 < rr.hi) { d0 = a.x1[i0] + (a.x2 ? a.x2[i0] : RV(0)); v0 = a.Z[i0] + (double)d0; }
+    if (i1 < rr.hi) { d1 = a.x1[i1] + (a.x2 ? a.x2[i1] : RV(0)); v1 = a.Z[i1] + (double)d1; }
+    double g[3];
+    pl.total(g);
+    if (poison) return;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (done == 0 && g[2] <= tol2) {
+        done = 1;
+        if (lead) { a.ctl->done = 1; a.ctl->zz = g[2]; }
     }
     if (done != 1) {
         if (lead) {
@@ -549,12 +655,22 @@ __global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
         }
         return;
     }
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const RowRange rr(a.n, blk, gridDim.x);
     bool bad = false;
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
-        const double v = a.V[i] + (double)a.x1[i] + (a.x2 ? (double)a.x2[i] : 0.0);
+    if (i0 < rr.hi) {
+        a.V[i0] = v0;
+        if (a.guess) a.guess[i0] += d0;
+        bad |= !isfinite(v0);
+    }
+    if (i1 < rr.hi) {
+        a.V[i1] = v1;
+        if (a.guess) a.guess[i1] += d1;
+        bad |= !isfinite(v1);
+    }
+    for (int i = i1 + kCgBlock; i < rr.hi; i += kCgBlock) {
+        const RV d = a.x1[i] + (a.x2 ? a.x2[i] : RV(0));
+        const double v = a.Z[i] + (double)d;
         a.V[i] = v;
+        if (a.guess) a.guess[i] += d;
         bad |= !isfinite(v);
     }
     if (bad) clock->nonfinite = 1;
@@ -592,7 +708,9 @@ struct cmx_ctx {
     uint8_t* stim_id = nullptr;
     cmx_stimulus* stim = nullptr;
     void* F = nullptr;
-    void* vec[9] = {};           // r w z s p m_a m_b x1 x2
+    void* vec[10] = {};          // r w z s p m_a m_b x1 x2 x0
+    double* Y = nullptr;         // V^n + x0 (initial guess)
+    void* G1 = nullptr;          // F - c x0
     double* r0_64 = nullptr;     // FP64 step residual (refinement)
     double* iion = nullptr;      // scratch for cmx_get_iionic
     // matrix
@@ -759,7 +877,7 @@ constexpr int kResPart = 2;
 double* pbuf(cmx_ctx* c, int k) { return c->partials + (size_t)k * kMaxBlocks * 3; }
 
 // Vector slots of cmx_ctx::vec.
-enum { V_R = 0, V_W, V_Z, V_S, V_P, V_MA, V_MB, V_X1, V_X2, V_COUNT };
+enum { V_R = 0, V_W, V_Z, V_S, V_P, V_MA, V_MB, V_X1, V_X2, V_X0, V_COUNT };
 
 template <class RV>
 CgVecs<RV> cg_vecs(cmx_ctx* c, int solve, int it)
@@ -815,6 +933,10 @@ int launch_cells(cmx_ctx* c)
     a.stim_id = c->stim_id;
     a.stim = c->stim;
     a.F = (RV*)c->F;
+    a.guess = c->extrapolate ? (const RV*)c->vec[V_X0] : nullptr;
+    a.c = c->am * c->cm / c->dt;
+    a.Y = c->Y;
+    a.G1 = (RV*)c->G1;
     const int G = grid_of(c->n);
     const bool rl = c->integrator == CMX_RUSH_LARSEN;
     KernelTimer timer(c, CMX_K_CELL);
@@ -836,8 +958,8 @@ int launch_residual(cmx_ctx* c, const Rows& R, int solve)
     a.n = c->n;
     a.c = c->am * c->cm / c->dt;
     a.clock = c->clock;
-    a.V = c->V;
-    a.F = (const RV*)c->F;
+    a.Z = c->extrapolate ? c->Y : c->V;
+    a.G = (const RV*)(c->extrapolate ? c->G1 : c->F);
     a.r0_64 = c->r0_64;
     a.x1 = (const RV*)c->vec[V_X1];
     a.v = cg_vecs<RV>(c, solve, 0);
@@ -846,10 +968,8 @@ int launch_residual(cmx_ctx* c, const Rows& R, int solve)
     KernelTimer timer(c, CMX_K_RESIDUAL);
     if (solve == 1)
         refine_residual_kernel<Rows, RV><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
-    else if (c->extrapolate)
-        residual_kernel<Rows, RV, true><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
     else
-        residual_kernel<Rows, RV, false><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
+        residual_kernel<Rows, RV><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
@@ -898,6 +1018,8 @@ int launch_finish(cmx_ctx* c, bool refined, const double* verdict)
     a.its_log = c->its_log;
     a.log_len = c->batch;
     a.V = c->V;
+    a.Z = c->extrapolate ? c->Y : c->V;
+    a.guess = c->extrapolate ? (RV*)c->vec[V_X0] : nullptr;
     a.x1 = (const RV*)c->vec[V_X1];
     a.x2 = refined ? (const RV*)c->vec[V_X2] : nullptr;
     KernelTimer timer(c, CMX_K_FINISH);
@@ -1300,6 +1422,10 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     if (c->precision == CMX_MIXED && c->refine > 0) {
         if ((rc = dalloc(c, &c->r0_64, n))) return rc;
     }
+    if (c->extrapolate) {
+        if ((rc = dalloc(c, &c->Y, n))) return rc;
+        if ((rc = dalloc(c, (char**)&c->G1, n * rv))) return rc;
+    }
     CMX_HIP(c, hipMemset(c->F, 0, n * rv));
     if ((rc = dalloc(c, &c->partials, 3 * (size_t)kMaxBlocks * 3))) return rc;
     CMX_HIP(c, hipMemset(c->partials, 0, 3 * (size_t)kMaxBlocks * 3 * sizeof(double)));
@@ -1414,7 +1540,7 @@ int cmx_set_voltage(cmx_ctx* c, const double* V)
     CMX_HIP(c, hipMemcpy(c->V, V, c->n * sizeof(double), hipMemcpyHostToDevice));
     // the previous correction no longer predicts the next one
     const size_t rv = c->precision == CMX_MIXED ? sizeof(float) : sizeof(double);
-    CMX_HIP(c, hipMemset(c->vec[V_X1], 0, c->n * rv));
+    CMX_HIP(c, hipMemset(c->vec[V_X0], 0, c->n * rv));
     return CMX_OK;
 }
 

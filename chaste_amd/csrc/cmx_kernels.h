@@ -19,11 +19,11 @@
 namespace cmx {
 
 constexpr int kBlock = 256;       // cell kernel
-constexpr int kCgBlock = 512;     // solver kernels
-constexpr int kCgGrid = 512;      // default solver blocks (two per CU)
+constexpr int kCgBlock = 1024;    // solver kernels
+constexpr int kCgGrid = 256;      // default solver blocks (one per CU)
 constexpr int kWave = 64;
 constexpr int kNumXcd = 8;
-constexpr int kMaxBlocks = 2048;   // partial-sum slots per reduction
+constexpr int kMaxBlocks = kCgBlock;   // partial-sum slots per reduction (one per thread of a consumer)
 
 // Time of the step being computed, advanced on the device by the finish
 // kernel so a step's launches have fixed arguments (and can be replayed).
@@ -84,6 +84,7 @@ constexpr int kMaxStencil = 16;    // stencil format: offsets per row (padded wi
 template <int W>
 struct StencilRows {
     static constexpr bool kStaged = true;
+    static constexpr int kPreW = W;      // gathers a kernel can hold in registers
     const uint8_t* __restrict__ pid;
     int n;
     int n_patterns;
@@ -106,6 +107,7 @@ struct StencilRows {
 // ELL: [slot][n] column-major indices and values in global memory.
 struct EllRows {
     static constexpr bool kStaged = false;
+    static constexpr int kPreW = 1;      // gathers are loaded where they are used
     const int32_t* __restrict__ cols;
     int w;
     int n;
@@ -120,20 +122,51 @@ struct EllRows {
     __device__ __forceinline__ int diag_count() const { return 0; }
 };
 
-// Copies a per-pattern table into LDS for the stencil format (the caller
-// synchronises); the ELL format reads its global arrays directly.
+static_assert(kMaxPatterns * kMaxStencil <= kCgBlock, "one table entry per solver thread");
+
+// One entry of a per-pattern table per thread, loaded into a register so the
+// load is issued together with the kernel's other loads; store() then puts it
+// in LDS (the caller synchronises before reading).  The ELL format reads its
+// global arrays directly.
 template <class Rows, class T>
-__device__ __forceinline__ const T* stage_table(const Rows&, const T* __restrict__ g, T* lds, int count)
-{
-    if constexpr (Rows::kStaged) {
-        for (int q = threadIdx.x; q < count; q += blockDim.x) lds[q] = g[q];
-        return lds;
-    } else {
-        (void)lds;
-        (void)count;
-        return g;
+struct TableLoad {
+    T v;
+    bool on;
+    __device__ __forceinline__ TableLoad(const T* __restrict__ g, int count)
+    {
+        on = Rows::kStaged && (int)threadIdx.x < count;
+        v = on ? g[threadIdx.x] : T(0);
     }
-}
+    __device__ __forceinline__ const T* store(T* lds, const T* g) const
+    {
+        if constexpr (Rows::kStaged) {
+            if (on) lds[threadIdx.x] = v;
+            return lds;
+        } else {
+            (void)lds;
+            return g;
+        }
+    }
+};
+
+// The gathered neighbour values of one row: held in registers for the
+// stencil format (loaded before they are needed), read in place for ELL.
+template <class Rows, class T>
+struct Gathered {
+    T g[Rows::kPreW];
+    __device__ __forceinline__ void load(const Rows& R, const typename Rows::Row& row, const T* __restrict__ x)
+    {
+        if constexpr (Rows::kStaged) {
+#pragma unroll
+            for (int k = 0; k < Rows::kPreW; ++k) g[k] = x[R.col(row, k)];
+        }
+    }
+    __device__ __forceinline__ T at(const Rows& R, const typename Rows::Row& row, const T* __restrict__ x, int k) const
+    {
+        if constexpr (Rows::kStaged) return g[k];
+        else return x[R.col(row, k)];
+    }
+};
 
 template <int N>
 __device__ __forceinline__ void wave_sum(double (&v)[N])
@@ -165,34 +198,38 @@ __device__ __forceinline__ void write_partials(double (&v)[N], double* __restric
     }
 }
 
-// Sum of the G slots written by the previous kernel, identical in every
-// block (BS threads, fixed order).
-template <int BS, int N>
-__device__ __forceinline__ void read_partials(const double* __restrict__ in, int G, double (&tot)[N])
-{
-    constexpr int NW = BS / kWave;
-    __shared__ double red[NW][N];
-    double acc[N];
+// The G <= kCgBlock slots of the previous kernel's reduction, one per
+// thread, loaded with the kernel's other loads; total() sums them in a fixed
+// order (identical in every block) and contains barriers.
+template <int N>
+struct PartialLoad {
+    double v[N];
+    __device__ __forceinline__ PartialLoad(const double* __restrict__ in, int G)
+    {
+        const bool on = (int)threadIdx.x < G;
 #pragma unroll
-    for (int j = 0; j < N; ++j) acc[j] = 0;
-    for (int b = threadIdx.x; b < G; b += BS)
-#pragma unroll
-        for (int j = 0; j < N; ++j) acc[j] += in[(size_t)b * N + j];
-    wave_sum<N>(acc);
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    if (lane == 0)
-#pragma unroll
-        for (int j = 0; j < N; ++j) red[wid][j] = acc[j];
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        double s = 0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) s += red[w][j];
-        tot[j] = s;
+        for (int j = 0; j < N; ++j) v[j] = on ? in[(size_t)threadIdx.x * N + j] : 0.0;
     }
-    __syncthreads();
-}
+    __device__ __forceinline__ void total(double (&tot)[N])
+    {
+        constexpr int NW = kCgBlock / kWave;
+        __shared__ double red[NW][N];
+        wave_sum<N>(v);
+        const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+        if (lane == 0)
+#pragma unroll
+            for (int j = 0; j < N; ++j) red[wid][j] = v[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            double s = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s += red[w][j];
+            tot[j] = s;
+        }
+        __syncthreads();
+    }
+};
 
 // SimpleStimulus / RegularStimulus at time t (uA/cm^3); duration already padded.
 __device__ __forceinline__ double stimulus_at(const cmx_stimulus& s, double t)
