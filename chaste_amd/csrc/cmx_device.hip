@@ -76,14 +76,50 @@ struct ModelTt06 {
     static constexpr bool gate(int i) { return tt06_epi::kIsGate[i] != 0; }
 };
 
+// What settles a pending step (its final solve decided, V^{n+1} written):
+// the cell kernel of the next step, or the finish kernel at a batch's end.
+template <class RV>
+struct SettleArgs {
+    SolveCtl* ctl;                     // final solve of the pending step
+    const double* __restrict__ pin;    // the last reduction of that solve
+    int gin;                           // its slot count
+    const double* __restrict__ Z;      // V^n, or V^n + x0 with an initial guess
+    const RV* __restrict__ x1;         // first solve's correction
+    const RV* __restrict__ x2;         // refinement solve's correction, or null
+    RV* __restrict__ guess;            // x0 (+= x1 + x2), or null
+};
+
+// The final solve converged earlier, or its last reduction passes the test
+// (identical in every block).  Lead records a convergence found here.
+__device__ __forceinline__ bool solve_converged(SolveCtl* ctl, int done, double tol2, double zeta, bool lead)
+{
+    if (done == 1) return true;
+    if (done == 0 && zeta <= tol2) {
+        if (lead) { ctl->done = 1; ctl->zz = zeta; }
+        return true;
+    }
+    return false;
+}
+
+// Iteration counts of step k for the host: first solve, whether it
+// converged (a refined step's first solve may hand over unconverged),
+// refinement solve.
+__device__ __forceinline__ void log_step(int32_t* its_log, int log_len, int k, const SolveCtl* ctl, bool refined)
+{
+    int32_t* slot = its_log + 4 * (k % log_len);
+    slot[0] = ctl[0].it;
+    slot[1] = refined ? (ctl[0].done == 1 ? 1 : 0) : 1;
+    slot[2] = refined ? ctl[1].it : 0;
+}
+
 template <class RS, class RV>
 struct CellArgs {
     int n;
     int n_sub;                 // ODE substeps per PDE step
     double dt_ode;             // ms
     double am, cm;
-    const StepClock* clock;
-    const double* __restrict__ V;
+    StepClock* clock;
+    double* __restrict__ V;
     RS* __restrict__ state;    // [NS-1][n]: plane s-1 holds state s (V excluded)
     const RS* __restrict__ params;  // [NP][n] or null
     const RS* __restrict__ pdef;    // [NP] defaults
@@ -91,42 +127,81 @@ struct CellArgs {
     const uint8_t* __restrict__ stim_id;  // 0 = none, k = row k-1
     const cmx_stimulus* __restrict__ stim;
     RV* __restrict__ F;        // out: -Am Iion - Istim(t_next)
-    // CG initial guess V^n + x0 (x0 = the previous step's correction), when set:
-    const RV* __restrict__ guess;   // x0
+    SettleArgs<RV> s;          // the previous step, when pending
+    // CG initial guess V^n + x0 (x0 = the previous step's correction), when s.guess is set:
     double c;                       // Am Cm / dt
     double* __restrict__ Y;         // out: V^n + x0
     RV* __restrict__ G1;            // out: F - c x0
 };
 
-// One thread per node; the state lives in registers for all substeps.
+// One thread per node; the state lives in registers for all substeps.  When
+// the previous step is pending, the kernel first settles it: the verdict on
+// its final solve (a failed one poisons the clock and nothing is written),
+// V^n = Z + x1 (+ x2) and the next initial guess.
 template <class Model, class RS, class RV, bool RL>
 __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
 {
-    const StepClock clk = *a.clock;
-    if (clk.poison) return;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (i >= a.n) return;
-    const double t = step_time(clk, clk.k), t_next = step_next_time(clk, clk.k);
     constexpr int NS = Model::NS;
     constexpr int NP = Model::NP > 0 ? Model::NP : 1;
-    const RS V = (RS)a.V[i];
+    const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const bool live = i < a.n;
+    PartialLoad<3, kBlock> pl(a.s.pin, a.s.gin);
+    const StepClock clk = *a.clock;
+    const int done = a.s.ctl->done;
+    const double tol2 = a.s.ctl->tol2;
     RS y[NS];
-    y[0] = V;
-#pragma unroll
-    for (int s = 1; s < NS; ++s) y[s] = a.state[(size_t)(s - 1) * a.n + i];
     RS p[NP];
+    int sid = 0;
+    double z = 0;
+    RV dx = 0, x0 = 0;
+    if (live) {
 #pragma unroll
-    for (int q = 0; q < NP; ++q)
-        p[q] = (q < a.n_param_planes) ? a.params[(size_t)q * a.n + i] : (Model::NP > 0 ? a.pdef[q] : RS(0));
-    const int sid = a.stim_id ? (int)a.stim_id[i] : 0;
+        for (int s = 1; s < NS; ++s) y[s] = a.state[(size_t)(s - 1) * a.n + i];
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+            p[q] = (q < a.n_param_planes) ? a.params[(size_t)q * a.n + i] : (Model::NP > 0 ? a.pdef[q] : RS(0));
+        sid = a.stim_id ? (int)a.stim_id[i] : 0;
+        z = a.s.Z[i];
+        dx = a.s.x1[i] + (a.s.x2 ? a.s.x2[i] : RV(0));
+        if (a.s.guess) x0 = a.s.guess[i];
+    }
+    double g[3];
+    pl.total(g);
+    if (clk.poison) return;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    int k = clk.k;
+    double vn = 0;
+    if (clk.pending) {
+        if (!solve_converged(a.s.ctl, done, tol2, g[2], lead)) {
+            if (lead) {
+                a.clock->poison = 1;
+                a.clock->failed_step = clk.k;
+            }
+            return;
+        }
+        vn = z + (double)dx;
+        x0 += dx;
+        k = clk.k + 1;
+    } else if (live) {
+        vn = a.V[i];
+    }
+    if (!live) return;
+    if (clk.pending) {
+        a.V[i] = vn;
+        if (a.s.guess) a.s.guess[i] = x0;
+        if (!isfinite(vn)) a.clock->nonfinite = 1;
+    }
+    const double t = step_time(clk, k), t_next = step_next_time(clk, k);
+    const RS V = (RS)vn;
+    y[0] = V;
     const RS cm = (RS)a.cm;
     // TimeStepper(t, t_next, dt_ode): substep k starts at t + k*dt_ode
     // (global/src/TimeStepper.cpp:106-121); forward Euler takes the remaining
     // interval on the last substep, Rush-Larsen always dt_ode
     // (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:106-128).
-    for (int k = 0; k < a.n_sub; ++k) {
-        const double ts = t + k * a.dt_ode;
-        const double h = (!RL && k + 1 == a.n_sub) ? t_next - ts : a.dt_ode;
+    for (int ks = 0; ks < a.n_sub; ++ks) {
+        const double ts = t + ks * a.dt_ode;
+        const double h = (!RL && ks + 1 == a.n_sub) ? t_next - ts : a.dt_ode;
         const RS st = sid ? (RS)(stimulus_at(a.stim[sid - 1], ts) / a.am) : RS(0);
         RS dy[NS], gi[NS], gk[NS];
         Model::template eval<RS, RL>(y, V, st, p, cm, dy, gi, gk);
@@ -141,10 +216,9 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
     const double istim = sid ? stimulus_at(a.stim[sid - 1], t_next) : 0.0;
     const RV f = (RV)(-a.am * (double)iion - istim);
     a.F[i] = f;
-    if (a.guess) {
-        const double x0 = (double)a.guess[i];
-        a.Y[i] = a.V[i] + x0;
-        a.G1[i] = (RV)((double)f - a.c * x0);
+    if (a.s.guess) {
+        a.Y[i] = vn + (double)x0;
+        a.G1[i] = (RV)((double)f - a.c * (double)x0);
     }
 #pragma unroll
     for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.n + i] = y[s];
@@ -227,7 +301,10 @@ template <class RV>
 struct ResidualArgs {
     int n;
     double c;                          // Am Cm / dt
-    const StepClock* clock;
+    StepClock* clock;
+    int32_t* its_log;                  // step statistics (first solve's residual kernel)
+    int log_len;
+    int refined;
     const double* __restrict__ Z;      // V^n, or V^n + x0 with an initial guess
     const RV* __restrict__ G;          // F, or F - c x0 with an initial guess
     double* __restrict__ r0_64;        // FP64 residual, kept for the refinement sweep
@@ -315,6 +392,13 @@ __global__ __launch_bounds__(kCgBlock) void residual_kernel(Rows R, MatVals mv, 
     }
     write_partials<kCgBlock, 3>(part, a.partials, blk);
     if (blk == 0 && threadIdx.x == 0) {
+        // the previous step was settled by this step's cell kernel: count it
+        StepClock* ck = a.clock;
+        if (ck->pending) {
+            log_step(a.its_log, a.log_len, ck->k, a.ctl, a.refined != 0);
+            ck->k += 1;
+        }
+        ck->pending = 1;
         a.ctl->done = 0;
         a.ctl->it = 0;
     }
@@ -604,51 +688,43 @@ __global__ __launch_bounds__(kCgBlock) void cg_iter_kernel(Rows R, MatVals mv, C
     write_partials<kCgBlock, 3>(part, a.pout, blk);
 }
 
-// Ends a step: decides the final solve (converged earlier, or its last
-// reduction passes the test), then V += x1 (+ x2 of the refinement solve) and
-// advances the clock; a solve still running poisons the clock so the rest of
-// the batch is skipped until the host resumes it.
+// Settles the last step of a batch (or a resumed step): the verdict on its
+// final solve, V^{n+1} = Z + x1 (+ x2), the next initial guess, and the
+// clock advance.  A solve still running poisons the clock until the host
+// resumes it.  Launched only after a step, so that one is pending.
 template <class RV>
 struct FinishArgs {
     int n;
     StepClock* clock;
-    SolveCtl* ctl;                     // final solve
-    const SolveCtl* ctl_first;         // first solve when refined, else null
-    const double* __restrict__ pin;    // last reduction of the final solve
-    int32_t* its_log;                  // [log_len][4]: its first, converged first, its refinement
+    SolveCtl* ctl_all;                 // both solves (statistics)
+    int refined;
+    int32_t* its_log;                  // [log_len][4]
     int log_len;
-    double* __restrict__ V;            // out: V^{n+1} = Z + x1 (+ x2)
-    const double* __restrict__ Z;      // V^n, or V^n + x0 with an initial guess
-    RV* __restrict__ guess;            // x0 += x1 (+ x2) when set: the next step's guess
-    const RV* __restrict__ x1;
-    const RV* __restrict__ x2;
+    double* __restrict__ V;
+    SettleArgs<RV> s;
 };
 
 template <class RV>
 __global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
 {
     StepClock* clock = a.clock;
-    PartialLoad<3> pl(a.pin, gridDim.x);
+    PartialLoad<3> pl(a.s.pin, a.s.gin);
     const int poison = clock->poison;
     const int k = clock->k;
-    int done = a.ctl->done;
-    const double tol2 = a.ctl->tol2;
+    const int done = a.s.ctl->done;
+    const double tol2 = a.s.ctl->tol2;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const RowRange rr(a.n, blk, gridDim.x);
     const int i0 = rr.lo + threadIdx.x, i1 = i0 + kCgBlock;
     RV d0 = 0, d1 = 0;
-    double v0 = 0, v1 = 0;
-    if (i0 < rr.hi) { d0 = a.x1[i0] + (a.x2 ? a.x2[i0] : RV(0)); v0 = a.Z[i0] + (double)d0; }
-    if (i1 < rr.hi) { d1 = a.x1[i1] + (a.x2 ? a.x2[i1] : RV(0)); v1 = a.Z[i1] + (double)d1; }
+    double z0 = 0, z1 = 0;
+    if (i0 < rr.hi) { d0 = a.s.x1[i0] + (a.s.x2 ? a.s.x2[i0] : RV(0)); z0 = a.s.Z[i0]; }
+    if (i1 < rr.hi) { d1 = a.s.x1[i1] + (a.s.x2 ? a.s.x2[i1] : RV(0)); z1 = a.s.Z[i1]; }
     double g[3];
     pl.total(g);
     if (poison) return;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (done == 0 && g[2] <= tol2) {
-        done = 1;
-        if (lead) { a.ctl->done = 1; a.ctl->zz = g[2]; }
-    }
-    if (done != 1) {
+    if (!solve_converged(a.s.ctl, done, tol2, g[2], lead)) {
         if (lead) {
             clock->poison = 1;
             clock->failed_step = k;
@@ -657,29 +733,29 @@ __global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
     }
     bool bad = false;
     if (i0 < rr.hi) {
-        a.V[i0] = v0;
-        if (a.guess) a.guess[i0] += d0;
-        bad |= !isfinite(v0);
+        const double v = z0 + (double)d0;
+        a.V[i0] = v;
+        if (a.s.guess) a.s.guess[i0] += d0;
+        bad |= !isfinite(v);
     }
     if (i1 < rr.hi) {
-        a.V[i1] = v1;
-        if (a.guess) a.guess[i1] += d1;
-        bad |= !isfinite(v1);
+        const double v = z1 + (double)d1;
+        a.V[i1] = v;
+        if (a.s.guess) a.s.guess[i1] += d1;
+        bad |= !isfinite(v);
     }
     for (int i = i1 + kCgBlock; i < rr.hi; i += kCgBlock) {
-        const RV d = a.x1[i] + (a.x2 ? a.x2[i] : RV(0));
-        const double v = a.Z[i] + (double)d;
+        const RV d = a.s.x1[i] + (a.s.x2 ? a.s.x2[i] : RV(0));
+        const double v = a.s.Z[i] + (double)d;
         a.V[i] = v;
-        if (a.guess) a.guess[i] += d;
+        if (a.s.guess) a.s.guess[i] += d;
         bad |= !isfinite(v);
     }
     if (bad) clock->nonfinite = 1;
     if (lead) {
-        int32_t* slot = a.its_log + 4 * (k % a.log_len);
-        slot[0] = a.ctl_first ? a.ctl_first->it : a.ctl->it;
-        slot[1] = a.ctl_first ? (a.ctl_first->done == 1 ? 1 : 0) : 1;
-        slot[2] = a.ctl_first ? a.ctl->it : 0;
+        log_step(a.its_log, a.log_len, k, a.ctl_all, a.refined != 0);
         clock->k = k + 1;
+        clock->pending = 0;
     }
 }
 
@@ -914,6 +990,33 @@ CgArgs<RV> cg_args(cmx_ctx* c, int solve)
     return a;
 }
 
+bool refined_of(const cmx_ctx* c) { return c->precision == CMX_MIXED && c->refine > 0; }
+
+// The reduction that decides a step's final solve at the current budgets: the
+// last iteration's (or the start kernel's when the budget is 0), or the
+// refinement residual's when the refinement solve has no budget.
+const double* verdict_buf(cmx_ctx* c)
+{
+    if (refined_of(c))
+        return c->refine_budget > 0 ? pbuf(c, kIterPart + (c->refine_budget & 1)) : pbuf(c, kResPart);
+    return pbuf(c, kIterPart + (c->budget & 1));
+}
+
+template <class RV>
+SettleArgs<RV> settle_args(cmx_ctx* c, const double* verdict)
+{
+    const bool refined = refined_of(c);
+    SettleArgs<RV> s;
+    s.ctl = c->ctl + (refined ? 1 : 0);
+    s.pin = verdict;
+    s.gin = c->G;
+    s.Z = c->extrapolate ? c->Y : c->V;
+    s.x1 = (const RV*)c->vec[V_X1];
+    s.x2 = refined ? (const RV*)c->vec[V_X2] : nullptr;
+    s.guess = c->extrapolate ? (RV*)c->vec[V_X0] : nullptr;
+    return s;
+}
+
 // ----- typed launches -------------------------------------------------------
 template <class RS, class RV>
 int launch_cells(cmx_ctx* c)
@@ -933,7 +1036,7 @@ int launch_cells(cmx_ctx* c)
     a.stim_id = c->stim_id;
     a.stim = c->stim;
     a.F = (RV*)c->F;
-    a.guess = c->extrapolate ? (const RV*)c->vec[V_X0] : nullptr;
+    a.s = settle_args<RV>(c, verdict_buf(c));
     a.c = c->am * c->cm / c->dt;
     a.Y = c->Y;
     a.G1 = (RV*)c->G1;
@@ -958,6 +1061,9 @@ int launch_residual(cmx_ctx* c, const Rows& R, int solve)
     a.n = c->n;
     a.c = c->am * c->cm / c->dt;
     a.clock = c->clock;
+    a.its_log = c->its_log;
+    a.log_len = c->batch;
+    a.refined = refined_of(c) ? 1 : 0;
     a.Z = c->extrapolate ? c->Y : c->V;
     a.G = (const RV*)(c->extrapolate ? c->G1 : c->F);
     a.r0_64 = c->r0_64;
@@ -1005,23 +1111,20 @@ int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, int from, int count)
     return CMX_OK;
 }
 
-// verdict: the reduction the final solve's last launched kernel wrote.
+// Settles the pending step; verdict: the reduction that decides its final
+// solve.
 template <class RV>
-int launch_finish(cmx_ctx* c, bool refined, const double* verdict)
+int launch_finish(cmx_ctx* c, const double* verdict)
 {
     FinishArgs<RV> a;
     a.n = c->n;
     a.clock = c->clock;
-    a.ctl = c->ctl + (refined ? 1 : 0);
-    a.ctl_first = refined ? c->ctl : nullptr;
-    a.pin = verdict;
+    a.ctl_all = c->ctl;
+    a.refined = refined_of(c) ? 1 : 0;
     a.its_log = c->its_log;
     a.log_len = c->batch;
     a.V = c->V;
-    a.Z = c->extrapolate ? c->Y : c->V;
-    a.guess = c->extrapolate ? (RV*)c->vec[V_X0] : nullptr;
-    a.x1 = (const RV*)c->vec[V_X1];
-    a.x2 = refined ? (const RV*)c->vec[V_X2] : nullptr;
+    a.s = settle_args<RV>(c, verdict);
     KernelTimer timer(c, CMX_K_FINISH);
     finish_kernel<RV><<<c->G, kCgBlock, 0, c->stream>>>(a);
     CMX_HIP(c, hipGetLastError());
@@ -1029,10 +1132,12 @@ int launch_finish(cmx_ctx* c, bool refined, const double* verdict)
 }
 
 // Enqueue one whole step with iteration budgets B (first solve) and B2
-// (refinement solve): cells, residual, start, B iterations, [refinement
-// residual, (start, B2 iterations)], finish.  A solve that needs more than its
-// budget is either continued by the refinement solve (first solve) or
-// resumed by the host after the finish kernel poisons the clock.
+// (refinement solve): cells (which settle the previous step), residual,
+// start, B iterations, [refinement residual, (start, B2 iterations)].  The
+// step stays pending until the next step's cell kernel or the batch's finish
+// kernel settles it.  A solve that needs more than its budget is either
+// continued by the refinement solve (first solve) or resumed by the host
+// after the settling kernel poisons the clock.
 template <class Rows, class RS, class RV>
 int enqueue_step(cmx_ctx* c, const Rows& R)
 {
@@ -1041,20 +1146,16 @@ int enqueue_step(cmx_ctx* c, const Rows& R)
     if ((rc = launch_residual<Rows, RV>(c, R, 0))) return rc;
     if ((rc = launch_cg_start<Rows, RV>(c, R, 0))) return rc;
     if ((rc = launch_cg_iters<Rows, RV>(c, R, 0, 0, c->budget))) return rc;
-    const bool refined = c->precision == CMX_MIXED && c->refine > 0;
-    const double* verdict = pbuf(c, kIterPart + (c->budget & 1));
-    if (refined) {
+    if (refined_of(c)) {
         // FP64 refinement: true residual of V^n + x1 against the step's b,
         // then a second CG solve on it into x2 when the budget allows one.
         if ((rc = launch_residual<Rows, RV>(c, R, 1))) return rc;
-        verdict = pbuf(c, kResPart);
         if (c->refine_budget > 0) {
             if ((rc = launch_cg_start<Rows, RV>(c, R, 1))) return rc;
             if ((rc = launch_cg_iters<Rows, RV>(c, R, 1, 0, c->refine_budget))) return rc;
-            verdict = pbuf(c, kIterPart + (c->refine_budget & 1));
         }
     }
-    return launch_finish<RV>(c, refined, verdict);
+    return CMX_OK;
 }
 
 int sync_state(cmx_ctx* c)
@@ -1138,6 +1239,7 @@ int run_steps(cmx_ctx* c, const Rows& R)
         const int todo = std::min(c->h_clock->n_steps - k_start, c->batch);
         int rc = enqueue_steps<Rows, RS, RV>(c, R, todo);
         if (rc) return rc;
+        if ((rc = launch_finish<RV>(c, verdict_buf(c)))) return rc;
         rc = sync_state(c);
         if (rc) return rc;
         int guard = 0;
@@ -1164,7 +1266,7 @@ int run_steps(cmx_ctx* c, const Rows& R)
                 if ((rc = launch_cg_iters<Rows, RV>(c, R, solve, from, chunk))) return rc;
                 verdict = pbuf(c, kIterPart + ((from + chunk) & 1));
             }
-            if ((rc = launch_finish<RV>(c, refined, verdict))) return rc;
+            if ((rc = launch_finish<RV>(c, verdict))) return rc;
             if ((rc = sync_state(c))) return rc;
         }
         const int got = c->h_clock->k - k_start;

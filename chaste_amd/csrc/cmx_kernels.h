@@ -25,8 +25,11 @@ constexpr int kWave = 64;
 constexpr int kNumXcd = 8;
 constexpr int kMaxBlocks = kCgBlock;   // partial-sum slots per reduction (one per thread of a consumer)
 
-// Time of the step being computed, advanced on the device by the finish
-// kernel so a step's launches have fixed arguments (and can be replayed).
+// Time of the step being computed, advanced on the device so that a step's
+// launches have fixed arguments (and can be replayed).  Step k ends in two
+// parts: the kernel that starts step k+1 (or the batch's finish kernel)
+// decides step k's final solve and writes V^{k+1}; the residual kernel of
+// step k+1 then advances k.
 struct StepClock {
     double t0, dt, t_end, eps;   // TimeStepper(t0, t_end, dt) of one cmx_step call
     int32_t k;                   // steps completed
@@ -34,7 +37,7 @@ struct StepClock {
     int32_t poison;              // a step's final solve missed its iteration budget:
     int32_t failed_step;         //   every later kernel is a no-op until the host resumes it
     int32_t nonfinite;
-    int32_t pad;
+    int32_t pending;             // step k's solves are done, its voltage update is not
 };
 
 // One linear solve's control block.
@@ -198,21 +201,29 @@ __device__ __forceinline__ void write_partials(double (&v)[N], double* __restric
     }
 }
 
-// The G <= kCgBlock slots of the previous kernel's reduction, one per
-// thread, loaded with the kernel's other loads; total() sums them in a fixed
-// order (identical in every block) and contains barriers.
-template <int N>
+// The G <= kMaxBlocks slots of the previous kernel's reduction, loaded with
+// the kernel's other loads (BS threads, at most kMaxBlocks / BS slots each);
+// total() sums them in a fixed order (identical in every block) and contains
+// barriers.
+template <int N, int BS = kCgBlock>
 struct PartialLoad {
+    static constexpr int kTrips = (kMaxBlocks + BS - 1) / BS;
     double v[N];
     __device__ __forceinline__ PartialLoad(const double* __restrict__ in, int G)
     {
-        const bool on = (int)threadIdx.x < G;
 #pragma unroll
-        for (int j = 0; j < N; ++j) v[j] = on ? in[(size_t)threadIdx.x * N + j] : 0.0;
+        for (int j = 0; j < N; ++j) v[j] = 0.0;
+#pragma unroll
+        for (int t = 0; t < kTrips; ++t) {
+            const int b = (int)threadIdx.x + t * BS;
+            if (b < G)
+#pragma unroll
+                for (int j = 0; j < N; ++j) v[j] += in[(size_t)b * N + j];
+        }
     }
     __device__ __forceinline__ void total(double (&tot)[N])
     {
-        constexpr int NW = kCgBlock / kWave;
+        constexpr int NW = BS / kWave;
         __shared__ double red[NW][N];
         wave_sum<N>(v);
         const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
