@@ -32,7 +32,7 @@ SIGMA = (1.75, 0.19, 0.19)
 STIM = (-50000.0, 0.0, 1.0)       # magnitude uA/cm^3, start ms, duration ms
 STIM_XLIM = 0.03
 DT = 0.01
-EXTRAPOLATE = True   # CG initial guess V^n + previous correction, in both paths
+EXTRAPOLATE = 2      # CG initial guess V^n + 2 x_{n-1} - x_{n-2} (corrections x), in both paths
 
 
 def build_problem(precision, coords, elems, stim_index, cmx):
@@ -219,7 +219,8 @@ def main():
         'data': 'synthetic (regular slab, default cell state)',
         'config': {'workload': 'TT06-epi RL monodomain slab 2.0x0.7x0.3 cm h=0.01 (201x71x31 = 442,401 nodes), '
                                'dt 0.01 ms, stim -50000 uA/cm^3 1 ms x<0.03, Jacobi-CG rtol 1e-8'
-                               + (', initial guess V^n + previous correction' if EXTRAPOLATE else ''),
+                               + {0: '', 1: ', initial guess V^n + x_{n-1}',
+                                  2: ', initial guess V^n + 2 x_{n-1} - x_{n-2}'}[EXTRAPOLATE],
                    'n_nodes': n, 'n_elems': int(elems.shape[0])},
         'node_ode_updates_per_s': round(steps_s * n, 1),
         'fp64_steps_per_s': round(results['fp64']['steps_per_s'], 2),

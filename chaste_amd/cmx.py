@@ -190,7 +190,7 @@ class Monodomain:
             self._keep.append(pa)
             t.n_params, t.params = pa.shape[0], _p(pa)
         o = CmxOpts(precision, fmt, dt_pde, dt_ode, am, cm, ksp_rtol, ksp_atol, ksp_max_its,
-                    refine_sweeps, 1 if use_graph else 0, 1 if mass_lumping else 0, 1 if extrapolate else 0)
+                    refine_sweeps, 1 if use_graph else 0, 1 if mass_lumping else 0, int(extrapolate))
         self.dt = dt_pde
         ctx = ctypes.c_void_p()
         rc = L.cmx_create(ctypes.byref(m), ctypes.byref(t), ctypes.byref(o), ctypes.byref(ctx))

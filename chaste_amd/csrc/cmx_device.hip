@@ -86,8 +86,20 @@ struct SettleArgs {
     const double* __restrict__ Z;      // V^n, or V^n + x0 with an initial guess
     const RV* __restrict__ x1;         // first solve's correction
     const RV* __restrict__ x2;         // refinement solve's correction, or null
-    RV* __restrict__ guess;            // x0 (+= x1 + x2), or null
+    RV* __restrict__ guess;            // x0 of the pending step, replaced by the next step's, or null
+    RV* __restrict__ prev;             // second order: x_{n-2}, replaced by x_{n-1}; or null
 };
+
+// The next step's initial-guess correction from the settled step's total
+// correction xt = x0 + x1 (+ x2): xt itself, or 2 xt - x_{n-2} (linear
+// extrapolation of the corrections); updates the stored x_{n-2}.
+template <class RV>
+__device__ __forceinline__ RV next_guess(const SettleArgs<RV>& s, int i, RV xt, RV prev)
+{
+    if (!s.prev) return xt;
+    s.prev[i] = xt;
+    return RV(2) * xt - prev;
+}
 
 // The final solve converged earlier, or its last reduction passes the test
 // (identical in every block).  Lead records a convergence found here.
@@ -153,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
     RS p[NP];
     int sid = 0;
     double z = 0;
-    RV dx = 0, x0 = 0;
+    RV dx = 0, x0 = 0, xp = 0;
     if (live) {
 #pragma unroll
         for (int s = 1; s < NS; ++s) y[s] = a.state[(size_t)(s - 1) * a.n + i];
@@ -164,6 +176,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         z = a.s.Z[i];
         dx = a.s.x1[i] + (a.s.x2 ? a.s.x2[i] : RV(0));
         if (a.s.guess) x0 = a.s.guess[i];
+        if (a.s.prev) xp = a.s.prev[i];
     }
     double g[3];
     pl.total(g);
@@ -180,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
             return;
         }
         vn = z + (double)dx;
-        x0 += dx;
+        if (live && a.s.guess) x0 = next_guess(a.s, i, x0 + dx, xp);
         k = clk.k + 1;
     } else if (live) {
         vn = a.V[i];
@@ -720,6 +733,9 @@ __global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
     double z0 = 0, z1 = 0;
     if (i0 < rr.hi) { d0 = a.s.x1[i0] + (a.s.x2 ? a.s.x2[i0] : RV(0)); z0 = a.s.Z[i0]; }
     if (i1 < rr.hi) { d1 = a.s.x1[i1] + (a.s.x2 ? a.s.x2[i1] : RV(0)); z1 = a.s.Z[i1]; }
+    auto settle_guess = [&](int i, RV d) {
+        if (a.s.guess) a.s.guess[i] = next_guess(a.s, i, a.s.guess[i] + d, a.s.prev ? a.s.prev[i] : RV(0));
+    };
     double g[3];
     pl.total(g);
     if (poison) return;
@@ -735,20 +751,20 @@ __global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
     if (i0 < rr.hi) {
         const double v = z0 + (double)d0;
         a.V[i0] = v;
-        if (a.s.guess) a.s.guess[i0] += d0;
+        settle_guess(i0, d0);
         bad |= !isfinite(v);
     }
     if (i1 < rr.hi) {
         const double v = z1 + (double)d1;
         a.V[i1] = v;
-        if (a.s.guess) a.s.guess[i1] += d1;
+        settle_guess(i1, d1);
         bad |= !isfinite(v);
     }
     for (int i = i1 + kCgBlock; i < rr.hi; i += kCgBlock) {
         const RV d = a.s.x1[i] + (a.s.x2 ? a.s.x2[i] : RV(0));
         const double v = a.s.Z[i] + (double)d;
         a.V[i] = v;
-        if (a.s.guess) a.s.guess[i] += d;
+        settle_guess(i, d);
         bad |= !isfinite(v);
     }
     if (bad) clock->nonfinite = 1;
@@ -784,7 +800,7 @@ struct cmx_ctx {
     uint8_t* stim_id = nullptr;
     cmx_stimulus* stim = nullptr;
     void* F = nullptr;
-    void* vec[10] = {};          // r w z s p m_a m_b x1 x2 x0
+    void* vec[11] = {};          // r w z s p m_a m_b x1 x2 x0 x_{n-2}
     double* Y = nullptr;         // V^n + x0 (initial guess)
     void* G1 = nullptr;          // F - c x0
     double* r0_64 = nullptr;     // FP64 step residual (refinement)
@@ -792,6 +808,7 @@ struct cmx_ctx {
     // matrix
     uint8_t* pid = nullptr;
     std::vector<int32_t> h_off;  // stencil offsets, padded to the kernel width
+    int pad = 0;                 // guard elements on each side of the gathered vectors
     int32_t* ell_col = nullptr;
     double *m64 = nullptr, *k64 = nullptr, *a64 = nullptr, *dinv64 = nullptr;
     float *a32 = nullptr, *dinv32 = nullptr;
@@ -862,6 +879,19 @@ int dalloc(cmx_ctx* c, T** ptr, size_t count)
     c->allocs.push_back(p);
     c->device_bytes += (double)bytes;
     *ptr = (T*)p;
+    return CMX_OK;
+}
+
+// A vector the stencil may gather: c->pad zeroed guard elements on each side.
+template <class T>
+int dalloc_vec(cmx_ctx* c, T** ptr, size_t count)
+{
+    T* base = nullptr;
+    const size_t total = count + 2 * (size_t)c->pad;
+    int rc = dalloc(c, &base, total);
+    if (rc) return rc;
+    CMX_HIP(c, hipMemset(base, 0, total * sizeof(T)));
+    *ptr = base + c->pad;
     return CMX_OK;
 }
 
@@ -953,7 +983,7 @@ constexpr int kResPart = 2;
 double* pbuf(cmx_ctx* c, int k) { return c->partials + (size_t)k * kMaxBlocks * 3; }
 
 // Vector slots of cmx_ctx::vec.
-enum { V_R = 0, V_W, V_Z, V_S, V_P, V_MA, V_MB, V_X1, V_X2, V_X0, V_COUNT };
+enum { V_R = 0, V_W, V_Z, V_S, V_P, V_MA, V_MB, V_X1, V_X2, V_X0, V_XP, V_COUNT };
 
 template <class RV>
 CgVecs<RV> cg_vecs(cmx_ctx* c, int solve, int it)
@@ -1014,6 +1044,7 @@ SettleArgs<RV> settle_args(cmx_ctx* c, const double* verdict)
     s.x1 = (const RV*)c->vec[V_X1];
     s.x2 = refined ? (const RV*)c->vec[V_X2] : nullptr;
     s.guess = c->extrapolate ? (RV*)c->vec[V_X0] : nullptr;
+    s.prev = c->extrapolate >= 2 ? (RV*)c->vec[V_XP] : nullptr;
     return s;
 }
 
@@ -1303,6 +1334,7 @@ StencilRows<W> stencil_rows(cmx_ctx* c)
     R.pid = c->pid;
     R.n = c->n;
     R.n_patterns = c->n_patterns;
+    R.pad = c->pad;
     for (int k = 0; k < W; ++k) R.off[k] = c->h_off[k];
     return R;
 }
@@ -1381,6 +1413,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->max_its = opts->ksp_max_its;
     c->refine = opts->refine_sweeps;
     c->use_graph = opts->use_graph;
+    if (opts->extrapolate < 0 || opts->extrapolate > 2) return fail(c, CMX_E_ARG, "extrapolate must be 0, 1 or 2");
     c->extrapolate = opts->extrapolate;
     c->n_param_planes = tissue->n_params;
     c->budget = std::min(8, c->max_its);
@@ -1417,6 +1450,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
         c->n_patterns = NPt;
         c->h_off.assign(W, 0);
         std::copy(st.off.begin(), st.off.end(), c->h_off.begin());
+        for (int32_t o : c->h_off) c->pad = std::max(c->pad, o < 0 ? -o : o);
         std::vector<double> m64((size_t)NPt * W, 0.0), k64((size_t)NPt * W, 0.0), a64((size_t)NPt * W, 0.0);
         std::vector<float> a32((size_t)NPt * W, 0.0f);
         std::vector<double> dinv64(NPt);
@@ -1474,7 +1508,8 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     std::vector<double> planes((size_t)(c->S - 1) * n);
     for (int s = 1; s < c->S; ++s) std::fill(planes.begin() + (size_t)(s - 1) * n, planes.begin() + (size_t)s * n, y0[s]);
     std::vector<double> V0(n, y0[0]);
-    if ((rc = upload(c, &c->V, V0.data(), n))) return rc;
+    if ((rc = dalloc_vec(c, &c->V, n))) return rc;
+    CMX_HIP(c, hipMemcpy(c->V, V0.data(), n * sizeof(double), hipMemcpyHostToDevice));
     if ((rc = dalloc(c, (char**)&c->state, planes.size() * rs))) return rc;
     if ((rc = mixed ? to_device_planes<float>(c, c->state, planes.data(), planes.size())
                     : to_device_planes<double>(c, c->state, planes.data(), planes.size())))
@@ -1516,19 +1551,19 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     if ((rc = upload(c, &c->stim, c->h_stim.data(), c->h_stim.size()))) return rc;
 
     // Work vectors.
-    if ((rc = dalloc(c, (char**)&c->F, n * rv))) return rc;
-    for (int v = 0; v < V_COUNT; ++v) {
-        if ((rc = dalloc(c, (char**)&c->vec[v], n * rv))) return rc;
-        CMX_HIP(c, hipMemset(c->vec[v], 0, n * rv));
-    }
+    auto rvec = [&](void** ptr) {
+        return mixed ? dalloc_vec(c, (float**)ptr, n) : dalloc_vec(c, (double**)ptr, n);
+    };
+    if ((rc = rvec(&c->F))) return rc;
+    for (int v = 0; v < V_COUNT; ++v)
+        if ((rc = rvec(&c->vec[v]))) return rc;
     if (c->precision == CMX_MIXED && c->refine > 0) {
         if ((rc = dalloc(c, &c->r0_64, n))) return rc;
     }
     if (c->extrapolate) {
-        if ((rc = dalloc(c, &c->Y, n))) return rc;
-        if ((rc = dalloc(c, (char**)&c->G1, n * rv))) return rc;
+        if ((rc = dalloc_vec(c, &c->Y, n))) return rc;
+        if ((rc = rvec(&c->G1))) return rc;
     }
-    CMX_HIP(c, hipMemset(c->F, 0, n * rv));
     if ((rc = dalloc(c, &c->partials, 3 * (size_t)kMaxBlocks * 3))) return rc;
     CMX_HIP(c, hipMemset(c->partials, 0, 3 * (size_t)kMaxBlocks * 3 * sizeof(double)));
     if ((rc = dalloc(c, &c->clock, 1))) return rc;
@@ -1643,6 +1678,7 @@ int cmx_set_voltage(cmx_ctx* c, const double* V)
     // the previous correction no longer predicts the next one
     const size_t rv = c->precision == CMX_MIXED ? sizeof(float) : sizeof(double);
     CMX_HIP(c, hipMemset(c->vec[V_X0], 0, c->n * rv));
+    CMX_HIP(c, hipMemset(c->vec[V_XP], 0, c->n * rv));
     return CMX_OK;
 }
 

@@ -91,6 +91,7 @@ struct StencilRows {
     const uint8_t* __restrict__ pid;
     int n;
     int n_patterns;
+    int pad;                             // guard elements each side of a gathered vector (>= max |off|)
     int32_t off[W];
     struct Row { int i; int p; };
     __device__ __forceinline__ constexpr int width() const { return W; }
@@ -152,16 +153,47 @@ struct TableLoad {
     }
 };
 
+// Raw buffer load of one element: address = descriptor base + voff + soff
+// (bytes), so that all the gathers of a row share one 32-bit VGPR offset and
+// take their per-neighbour offsets from SGPRs.
+template <class T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
+// Descriptor of a gathered vector x[-pad .. n+pad) (data format 32-bit, the
+// gfx9 raw-buffer word 3).
+template <class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vec_rsrc(const T* x, int n, int pad)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(x - pad), (short)0, (int)((uint32_t)(n + 2 * pad) * sizeof(T)),
+                                             0x00020000);
+}
+
 // The gathered neighbour values of one row: held in registers for the
 // stencil format (loaded before they are needed), read in place for ELL.
+// Stencil gathers read x[i + off[k]] with no clamping: gathered vectors carry
+// pad >= max |off| zero guard elements on each side, and the boundary rows'
+// out-of-mesh slots have zero matrix values.
 template <class Rows, class T>
 struct Gathered {
     T g[Rows::kPreW];
     __device__ __forceinline__ void load(const Rows& R, const typename Rows::Row& row, const T* __restrict__ x)
     {
         if constexpr (Rows::kStaged) {
+            const __amdgpu_buffer_rsrc_t rs = vec_rsrc(x, R.n, R.pad);
+            const uint32_t voff = (uint32_t)row.i * (uint32_t)sizeof(T);
 #pragma unroll
-            for (int k = 0; k < Rows::kPreW; ++k) g[k] = x[R.col(row, k)];
+            for (int k = 0; k < Rows::kPreW; ++k)
+                g[k] = buf_load<T>(rs, voff, (uint32_t)(R.pad + R.off[k]) * (uint32_t)sizeof(T));
         }
     }
     __device__ __forceinline__ T at(const Rows& R, const typename Rows::Row& row, const T* __restrict__ x, int k) const

@@ -90,8 +90,9 @@ typedef struct {
     int32_t refine_sweeps;     /* mixed: FP64 true-residual checks per step (0 = none) */
     int32_t use_graph;         /* replay steps from captured hipGraphs */
     int32_t mass_lumping;      /* HeartConfig::SetUseMassLumping */
-    int32_t extrapolate;       /* CG initial guess V^n + (previous step's correction)
-                                  instead of the reference's V^n; same stopping test */
+    int32_t extrapolate;       /* CG initial guess instead of the reference's V^n (same
+                                  stopping test): 1 = V^n + x_{n-1}, 2 = V^n + 2 x_{n-1} - x_{n-2},
+                                  with x_m = V^{m+1} - V^m the step corrections */
 } cmx_opts;
 
 typedef struct {

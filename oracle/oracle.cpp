@@ -318,7 +318,7 @@ int orc_slab_fill(int dim, int ex, int ey, int ez, double h, double* coords, int
 // (pde/src/solver/GaussianQuadratureRule.cpp): 1D 2 points, triangles 3
 // points, tetrahedra 4 points; P1 basis phi_0 = 1 - sum(xi), phi_k = xi_k.
 int orc_assemble(int dim, int n, const double* coords, int ne, const int* elems,
-                 const double* sigma, int lumped, int* nnz_out,
+                 const double* sigma_all, int sigma_per_elem, int lumped, int* nnz_out,
                  int* row_ptr, int* col, double* mass, double* stiff)
 {
     if (dim < 1 || dim > 3) return -1;
@@ -368,6 +368,7 @@ int orc_assemble(int dim, int n, const double* coords, int ne, const int* elems,
 
     for (int e = 0; e < ne; ++e) {
         const int* nd = elems + e * nb;
+        const double* sigma = sigma_per_elem ? sigma_all + (size_t)e * dim * dim : sigma_all;
         // Jacobian J[r][c] = d x_r / d xi_c = x_{c+1,r} - x_{0,r}
         double J[3][3] = {}, Ji[3][3] = {};
         for (int r = 0; r < dim; ++r)
@@ -514,11 +515,15 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
     orc_slab_fill(dim, cfg->ex, cfg->ey, cfg->ez, cfg->h, coords.data(), elems.data());
     double sigma[9] = {};
     for (int r = 0; r < dim; ++r) sigma[r * dim + r] = cfg->sigma[r];
+    const double* sig = cfg->sigma_elem ? cfg->sigma_elem : sigma;
+    const int per_elem = cfg->sigma_elem ? 1 : 0;
     int nnz;
-    orc_assemble(dim, n, coords.data(), ne, elems.data(), sigma, 0, &nnz, nullptr, nullptr, nullptr, nullptr);
+    orc_assemble(dim, n, coords.data(), ne, elems.data(), sig, per_elem, cfg->lumped, &nnz, nullptr, nullptr,
+                 nullptr, nullptr);
     std::vector<int> rp(n + 1), col(nnz);
     std::vector<double> M(nnz), K(nnz), A(nnz);
-    orc_assemble(dim, n, coords.data(), ne, elems.data(), sigma, 0, &nnz, rp.data(), col.data(), M.data(), K.data());
+    orc_assemble(dim, n, coords.data(), ne, elems.data(), sig, per_elem, cfg->lumped, &nnz, rp.data(), col.data(),
+                 M.data(), K.data());
     // A = (Am Cm / dt) M + K  (MonodomainAssembler.cpp:42-66)
     const double am = cfg->am, cm = cfg->cm, dt = cfg->dt_pde;
     for (int k = 0; k < nnz; ++k) A[k] = (1.0 / dt) * (am * cm * M[k]) + K[k];

@@ -57,11 +57,14 @@ int orc_slab_sizes(int dim, int ex, int ey, int ez, int* n_nodes, int* n_elems);
 /* coords[n*dim], elems[ne*(dim+1)] */
 int orc_slab_fill(int dim, int ex, int ey, int ez, double h, double* coords, int* elems);
 
-/* P1 assembly with the reference's quadrature. sigma: dim*dim tensor shared by
- * all elements.  Outputs CSR (row_ptr[n+1], col[nnz], sorted) of M (unscaled
- * mass) and K (stiffness).  Call with col == NULL to get nnz only. */
+/* P1 assembly with the reference's quadrature. sigma: dim*dim tensors, one
+ * shared by all elements (sigma_per_elem == 0) or one per element
+ * (sigma[e*dim*dim ...], AbstractCardiacTissue::rGetIntracellularConductivityTensor).
+ * Outputs CSR (row_ptr[n+1], col[nnz], sorted) of M (unscaled mass, row-sum
+ * lumped when lumped != 0) and K (stiffness).  Call with col == NULL to get
+ * nnz only. */
 int orc_assemble(int dim, int n, const double* coords, int ne, const int* elems,
-                 const double* sigma, int lumped, int* nnz_out,
+                 const double* sigma, int sigma_per_elem, int lumped, int* nnz_out,
                  int* row_ptr, int* col, double* mass, double* stiff);
 
 /* Jacobi-PCG on CSR (x in/out = initial guess).  Stops when ||D^-1 r||_2 <=
@@ -85,6 +88,8 @@ typedef struct {
     int threads;              /* OpenMP threads (<=0: library default) */
     int n_param;              /* per-node parameters (0: model defaults) */
     const double* params;     /* [n_param][n] node-major planes, or NULL */
+    int lumped;               /* HeartConfig::SetUseMassLumping */
+    const double* sigma_elem; /* [n_elems][dim*dim] per-element tensors, or NULL for sigma[] */
 } orc_mono_cfg;
 
 typedef struct {

@@ -25,7 +25,8 @@ class MonoCfg(ctypes.Structure):
                 ('dt_pde', ctypes.c_double), ('dt_ode', ctypes.c_double), ('stim_mag', ctypes.c_double),
                 ('stim_dur', ctypes.c_double), ('stim_start', ctypes.c_double), ('stim_xlim', ctypes.c_double),
                 ('t_end', ctypes.c_double), ('ksp_rtol', ctypes.c_double), ('ksp_atol', ctypes.c_double),
-                ('max_its', ctypes.c_int), ('threads', ctypes.c_int), ('n_param', ctypes.c_int), ('params', _dp)]
+                ('max_its', ctypes.c_int), ('threads', ctypes.c_int), ('n_param', ctypes.c_int), ('params', _dp),
+                ('lumped', ctypes.c_int), ('sigma_elem', _dp)]
 
 
 class MonoStats(ctypes.Structure):
@@ -56,8 +57,8 @@ def lib():
         L.orc_cell_iionic.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_slab_sizes.argtypes = [ctypes.c_int] * 4 + [_ip, _ip]
         L.orc_slab_fill.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double, _dp, _ip]
-        L.orc_assemble.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int, _ip,
-                                   _ip, _ip, _dp, _dp]
+        L.orc_assemble.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int,
+                                   ctypes.c_int, _ip, _ip, _ip, _dp, _dp]
         L.orc_pcg.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_mono_run.argtypes = [ctypes.POINTER(MonoCfg), _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp,
                                    ctypes.POINTER(MonoStats)]
@@ -103,20 +104,23 @@ def slab(dim, ex, ey=0, ez=0, h=0.01):
 
 
 def assemble(coords, elems, sigma, lumped=False):
-    """(row_ptr, col, M, K) with the reference's quadrature."""
+    """(row_ptr, col, M, K) with the reference's quadrature.  sigma: one
+    dim x dim tensor, or [n_elems, dim, dim] per element."""
     coords = np.ascontiguousarray(coords, dtype=np.float64)
     elems = np.ascontiguousarray(elems, dtype=np.int32)
     n, dim = coords.shape
-    sig = np.ascontiguousarray(np.asarray(sigma, dtype=np.float64).reshape(dim, dim))
+    sig = np.asarray(sigma, dtype=np.float64)
+    per_elem = int(sig.size == elems.shape[0] * dim * dim and sig.size != dim * dim)
+    sig = np.ascontiguousarray(sig.reshape(-1))
     nnz = ctypes.c_int()
-    lib().orc_assemble(dim, n, _p(coords), elems.shape[0], _p(elems, _ip), _p(sig), int(lumped), ctypes.byref(nnz),
-                       None, None, None, None)
+    lib().orc_assemble(dim, n, _p(coords), elems.shape[0], _p(elems, _ip), _p(sig), per_elem, int(lumped),
+                       ctypes.byref(nnz), None, None, None, None)
     rp = np.zeros(n + 1, dtype=np.int32)
     col = np.zeros(nnz.value, dtype=np.int32)
     M = np.zeros(nnz.value)
     K = np.zeros(nnz.value)
-    lib().orc_assemble(dim, n, _p(coords), elems.shape[0], _p(elems, _ip), _p(sig), int(lumped), ctypes.byref(nnz),
-                       _p(rp, _ip), _p(col, _ip), _p(M), _p(K))
+    lib().orc_assemble(dim, n, _p(coords), elems.shape[0], _p(elems, _ip), _p(sig), per_elem, int(lumped),
+                       ctypes.byref(nnz), _p(rp, _ip), _p(col, _ip), _p(M), _p(K))
     return rp, col, M, K
 
 
@@ -130,7 +134,7 @@ def pcg(rp, col, val, b, x0, rtol=1e-12, atol=0.0, max_its=10000):
 def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, sigma=(1.75, 1.75, 1.75),
              am=1400.0, cm=1.0, dt_pde=0.01, dt_ode=0.01, stim_mag=-3e6, stim_dur=0.5, stim_start=0.0,
              stim_xlim=1e-5, t_end=1.0, ksp_rtol=1e-12, ksp_atol=0.0, max_its=10000, threads=0,
-             params=None, trace_nodes=None, act_threshold=None, want_state=False):
+             params=None, trace_nodes=None, act_threshold=None, want_state=False, lumped=False, sigma_elem=None):
     """Full FP64 monodomain run (AbstractCardiacProblem::Solve equivalent)."""
     n = [ex + 1, (ex + 1) * (ey + 1), (ex + 1) * (ey + 1) * (ez + 1)][dim - 1]
     pa = None
@@ -138,9 +142,10 @@ def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, si
     if params is not None:
         pa = np.ascontiguousarray(params, dtype=np.float64)
         npar = pa.shape[0]
+    se = None if sigma_elem is None else np.ascontiguousarray(sigma_elem, dtype=np.float64).reshape(-1)
     cfg = MonoCfg(dim, ex, ey, ez, h, model, int(rush_larsen), (ctypes.c_double * 3)(*sigma), am, cm, dt_pde, dt_ode,
                   stim_mag, stim_dur, stim_start, stim_xlim, t_end, ksp_rtol, ksp_atol, max_its, threads, npar,
-                  _p(pa))
+                  _p(pa), int(lumped), _p(se))
     V = np.zeros(n)
     S = num_states(model)
     state = np.zeros((S, n)) if want_state else None

@@ -47,17 +47,17 @@ def first_crossing(trace, dt, thr=0.0):
 # supra-threshold variants below (-400000 and -100000) launch a propagating
 # wave so the comparison covers upstrokes and activation times.
 @pytest.mark.parametrize('stim', [-25000.0, -400000.0], ids=['configured', 'suprathreshold'])
-def test_cfg1_lr91_2d_fp64(oracle, stim):
-    """BASELINE config 1 geometry in full: 101x101 LR91 FE, 10 ms, stimulus x < 0.02."""
+def test_lr91_square_101x101_fp64(oracle, stim):
+    """2-D square 1.0 x 1.0 cm, 101x101 nodes, LR91 FE, 10 ms, stimulus on x < 0.02."""
     from chaste_amd import cmx
-    print('case cfg1 lr91 2d fp64', stim, flush=True)
+    print("case lr91 square 101x101 fp64", stim, flush=True)
     m = make(2, 100, 100, 0, cmx.LR91, cmx.FORWARD_EULER, stim, 0.5, 0.02)
     m.step(0.0, 1000)
     V = m.voltage()
     ref = oracle.mono_run(2, 100, 100, 0, model=oracle.LR91_HAND, stim_mag=stim, stim_dur=0.5, stim_xlim=0.02,
                           t_end=10.0)
     err = np.abs(V - ref['V']).max() / np.abs(ref['V']).max()
-    print('cfg1 rel err', err, 'its/step', m.stats()['cg_iterations'] / 1000.0)
+    print("lr91 square rel err", err, 'its/step', m.stats()['cg_iterations'] / 1000.0)
     assert err <= FP64_REL
 
 
@@ -97,14 +97,15 @@ def test_budget_resume_path(oracle):
     assert np.abs(V - ref['V']).max() / np.abs(ref['V']).max() <= FP64_REL
 
 
+@pytest.mark.parametrize('extrapolate', [0, 2], ids=['warm', 'guess2'])
 @pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
-def test_graph_replay_matches_eager(precision):
+def test_graph_replay_matches_eager(precision, extrapolate):
     """Steps replayed from captured hipGraphs give bitwise the same voltages as
     plain launches, over several calls (graph replays with eager work and
     synchronisations in between), including steps that hit the resume path."""
     from chaste_amd import cmx
-    print('case graph replay', precision, flush=True)
-    kw = dict(precision=precision, sigma=(1.75, 0.19, 0.19), rtol=1e-10, refine=1)
+    print('case graph replay', precision, extrapolate, flush=True)
+    kw = dict(precision=precision, sigma=(1.75, 0.19, 0.19), rtol=1e-10, refine=1, extrapolate=extrapolate)
     a = make(3, 24, 8, 5, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, use_graph=True, **kw)
     b = make(3, 24, 8, 5, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, use_graph=False, **kw)
     t = 0.0
@@ -118,17 +119,19 @@ def test_graph_replay_matches_eager(precision):
     assert a.stats()['cg_iterations'] == b.stats()['cg_iterations']
 
 
+@pytest.mark.parametrize('order', [1, 2], ids=['x_prev', 'linear'])
 @pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
-def test_extrapolated_initial_guess(oracle, precision):
-    """Starting CG from V^n + (previous correction) changes the iterates, not
-    the converged voltages: FP64 stays within 1e-9 of the oracle (warm start
-    V^n, as the reference), mixed within 0.5 mV, with fewer iterations."""
+def test_extrapolated_initial_guess(oracle, precision, order):
+    """Starting CG from V^n + x_{n-1} (order 1) or V^n + 2 x_{n-1} - x_{n-2}
+    (order 2; x = step corrections) changes the iterates, not the converged
+    voltages: FP64 stays within 1e-9 of the oracle (warm start V^n, as the
+    reference), mixed within 0.5 mV, with fewer iterations."""
     from chaste_amd import cmx
-    print('case extrapolate', precision, flush=True)
+    print('case extrapolate', precision, order, flush=True)
     dims = (3, 20, 7, 4)
     rtol = 1e-12 if precision == 0 else 1e-8
     kw = dict(precision=precision, sigma=(1.75, 0.19, 0.19), rtol=rtol, refine=precision, use_graph=True)
-    e = make(*dims, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, extrapolate=True, **kw)
+    e = make(*dims, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, extrapolate=order, **kw)
     p = make(*dims, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, extrapolate=False, **kw)
     e.step(0.0, 400)
     p.step(0.0, 400)
@@ -142,6 +145,57 @@ def test_extrapolated_initial_guess(oracle, precision):
     else:
         assert np.abs(Ve - ref).max() <= MIXED_MV
     assert its_e <= its_p
+
+
+def test_per_element_conductivity(oracle):
+    """Per-element conductivity tensors (two regions, one with rotated fibres:
+    off-diagonal terms) against the oracle at 1e-9 relative; the row patterns
+    differ across the region boundary."""
+    from chaste_amd import cmx
+    print('case per-element sigma', flush=True)
+    dims = (3, 12, 6, 5)
+    coords, elems = cmx.slab_mesh(*dims, 0.01)
+    centroid_x = coords[elems, 0].mean(axis=1)
+    s1 = np.diag([1.75, 0.19, 0.19])
+    c, s = np.cos(0.6), np.sin(0.6)
+    O = np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+    s2 = O @ np.diag([1.75, 0.19, 0.19]) @ O.T
+    per = np.where((centroid_x < 0.06)[:, None, None], s1[None], s2[None])
+    stim_index = np.where(coords[:, 0] < 0.03, 0, -1).astype(np.int32)
+    m = cmx.Monodomain(coords, elems, np.array([1.75, 0.19, 0.19]), cmx.TT06_EPI, cmx.RUSH_LARSEN,
+                       stimuli=[(-100000.0, 0.0, 1.0)], stim_index=stim_index, precision=cmx.FP64,
+                       ksp_rtol=1e-12, ksp_atol=0.0, sigma_elem=per, use_graph=True, extrapolate=2)
+    m.step(0.0, 300)
+    ref = oracle.mono_run(*dims, model=oracle.TT06_EPI, rush_larsen=True, stim_mag=-100000.0, stim_dur=1.0,
+                          stim_xlim=0.03, t_end=3.0, ksp_rtol=1e-12, sigma_elem=per)['V']
+    V = m.voltage()
+    print('per-element sigma format', m.stats()['format'], 'max rel', np.abs(V - ref).max() / np.abs(ref).max())
+    assert np.abs(V - ref).max() / np.abs(ref).max() <= FP64_REL
+
+
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_mass_lumping(oracle, precision):
+    """HeartConfig::SetUseMassLumping: lumped mass in the RHS and the system
+    matrix, against the oracle (1e-9 relative in FP64, 0.5 mV mixed)."""
+    from chaste_amd import cmx
+    print('case mass lumping', precision, flush=True)
+    coords, elems = cmx.slab_mesh(2, 30, 30, 0, 0.01)
+    stim_index = np.where(coords[:, 0] < 0.02, 0, -1).astype(np.int32)
+    m = cmx.Monodomain(coords, elems, np.array([1.75, 1.75]), cmx.LR91, cmx.FORWARD_EULER,
+                       stimuli=[(-25000.0, 0.0, 0.5)], stim_index=stim_index, precision=precision,
+                       ksp_rtol=1e-12 if precision == 0 else 1e-8, ksp_atol=0.0, mass_lumping=True,
+                       refine_sweeps=precision, use_graph=True)
+    m.step(0.0, 400)
+    ref = oracle.mono_run(2, 30, 30, model=oracle.LR91_HAND, sigma=(1.75, 1.75, 1.75), stim_mag=-25000.0,
+                          stim_dur=0.5, stim_xlim=0.02, t_end=4.0, ksp_rtol=1e-12, lumped=True)['V']
+    unl = oracle.mono_run(2, 30, 30, model=oracle.LR91_HAND, sigma=(1.75, 1.75, 1.75), stim_mag=-25000.0,
+                          stim_dur=0.5, stim_xlim=0.02, t_end=4.0, ksp_rtol=1e-12, lumped=False)['V']
+    V = m.voltage()
+    assert np.abs(ref - unl).max() > 1e-3       # lumping changes the answer
+    if precision == 0:
+        assert np.abs(V - ref).max() / np.abs(ref).max() <= FP64_REL
+    else:
+        assert np.abs(V - ref).max() <= MIXED_MV
 
 
 def test_ell_matches_stencil():

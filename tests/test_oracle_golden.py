@@ -146,6 +146,39 @@ def test_element_matrices_closed_form(oracle):
         assert np.abs(Kd.sum(axis=1)).max() < 1e-12 * np.abs(Kd).max()
 
 
+def _dense(rp, col, v, n):
+    out = np.zeros((n, n))
+    for i in range(n):
+        out[i, col[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]]
+    return out
+
+
+def test_per_element_conductivity_and_lumping(oracle):
+    """Per-element tensors: uniform ones reproduce the shared-tensor assembly,
+    and K is linear in the element tensors (K over a split = K_A(s1) + K_B(s2),
+    each part assembled on its own elements).  Lumped mass: diagonal, with the
+    consistent mass matrix's row sums (MassMatrixAssembler.hpp:88-101)."""
+    coords, elems = oracle.slab(3, 3, 2, 2, h=0.01)
+    n, ne = coords.shape[0], elems.shape[0]
+    s1 = np.diag([1.75, 0.19, 0.19])
+    s2 = np.array([[0.9, 0.2, 0.0], [0.2, 0.5, 0.1], [0.0, 0.1, 0.3]])
+    rp, col, M, K = oracle.assemble(coords, elems, s1)
+    rpe, cole, Me, Ke = oracle.assemble(coords, elems, np.repeat(s1[None], ne, axis=0))
+    assert np.array_equal(rp, rpe) and np.array_equal(col, cole)
+    assert np.array_equal(M, Me) and np.array_equal(K, Ke)
+    half = np.arange(ne) < ne // 2
+    per = np.where(half[:, None, None], s1[None], s2[None])
+    rp2, col2, _, K2 = oracle.assemble(coords, elems, per)
+    ka = oracle.assemble(coords, elems[half], s1)
+    kb = oracle.assemble(coords, elems[~half], s2)
+    assert np.allclose(_dense(rp2, col2, K2, n), _dense(ka[0], ka[1], ka[3], n) + _dense(kb[0], kb[1], kb[3], n),
+                       rtol=1e-13, atol=1e-16)
+    rpl, coll, Ml, _ = oracle.assemble(coords, elems, s1, lumped=True)
+    Dl = _dense(rpl, coll, Ml, n)
+    assert np.array_equal(Dl, np.diag(np.diag(Dl)))
+    assert np.allclose(np.diag(Dl), _dense(rp, col, M, n).sum(axis=1), rtol=1e-14)
+
+
 def test_stiffness_1d_pattern(oracle):
     """1-D K = sigma/h [-1 2 -1] (TestMonodomainStiffnessMatrixAssembler.hpp)."""
     coords, elems = oracle.slab(1, 10, h=0.1)
