@@ -68,6 +68,8 @@ SIGNATURES = {
     'cmx_set_state': (ctypes.c_int, [ctypes.c_void_p, _dp]),
     'cmx_get_iionic': (ctypes.c_int, [ctypes.c_void_p, _dp]),
     'cmx_get_stats': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CmxStats)]),
+    'cmx_set_activation': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_double]),
+    'cmx_get_activation': (ctypes.c_int, [ctypes.c_void_p, _dp]),
     'cmx_set_profiling': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     'cmx_get_kernel_times': (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
     'cmx_model_info': (ctypes.c_int, [ctypes.c_int32, _ip, _ip]),
@@ -232,6 +234,18 @@ class Monodomain:
     def iionic(self):
         out = np.zeros(self.n)
         _check(lib().cmx_get_iionic(self.ctx, _p(out)), self.ctx)
+        return out
+
+    def set_activation(self, threshold, every_steps=1, t_now=0.0):
+        """ActivationOutputModifier on the device: processes the current voltage
+        at t_now, then every every_steps-th PDE step of each step() call."""
+        _check(lib().cmx_set_activation(self.ctx, float(threshold), int(every_steps), float(t_now)), self.ctx)
+
+    def activation(self):
+        """[5, n]: first activation, first recovery, second activation, second
+        recovery (ms, -1 if not yet seen), interpolated first upward crossing."""
+        out = np.zeros((5, self.n))
+        _check(lib().cmx_get_activation(self.ctx, _p(out)), self.ctx)
         return out
 
     def set_profiling(self, kinds):

@@ -88,7 +88,33 @@ struct SettleArgs {
     const RV* __restrict__ x2;         // refinement solve's correction, or null
     RV* __restrict__ guess;            // x0 of the pending step, replaced by the next step's, or null
     RV* __restrict__ prev;             // second order: x_{n-2}, replaced by x_{n-1}; or null
+    double* __restrict__ act;          // [5][n] activation times, or null
+    double act_thr;
+    int act_every;
 };
+
+// ActivationOutputModifier::ProcessSolutionAtTimeStep for node i at time t
+// (heart/src/problem/ActivationOutputModifier.cpp:94-125).
+__device__ __forceinline__ void modifier_update(double* __restrict__ act, int n, int i, double t, double v, double thr)
+{
+    double* a0 = act + i;
+    const double fa = a0[0], fr = a0[n], sa = a0[2 * (size_t)n], sr = a0[3 * (size_t)n];
+    if (v > thr && fa < 0.0) a0[0] = t;
+    else if (fa >= 0.0 && fr < 0.0 && v < thr) a0[n] = t;
+    else if (fr >= 0.0 && sa < 0.0 && v > thr) a0[2 * (size_t)n] = t;
+    else if (sa >= 0.0 && sr < 0.0 && v < thr) a0[3 * (size_t)n] = t;
+}
+
+// Activation bookkeeping of a settled step k (V^k = v0 -> V^{k+1} = v1).
+template <class RV>
+__device__ __forceinline__ void track_activation(const SettleArgs<RV>& s, int n, int i, const StepClock& clk, int k,
+                                                 double v0, double v1)
+{
+    const double t0 = step_time(clk, k), t1 = step_next_time(clk, k);
+    double* cross = s.act + 4 * (size_t)n + i;
+    if (*cross < 0.0 && v0 < s.act_thr && v1 >= s.act_thr) *cross = t0 + (t1 - t0) * (s.act_thr - v0) / (v1 - v0);
+    if ((k + 1) % s.act_every == 0) modifier_update(s.act, n, i, t1, v1, s.act_thr);
+}
 
 // The next step's initial-guess correction from the settled step's total
 // correction xt = x0 + x1 (+ x2): xt itself, or 2 xt - x_{n-2} (linear
@@ -193,6 +219,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
             return;
         }
         vn = z + (double)dx;
+        if (live && a.s.act) track_activation(a.s, a.n, i, clk, clk.k, z - (double)x0, vn);
         if (live && a.s.guess) x0 = next_guess(a.s, i, x0 + dx, xp);
         k = clk.k + 1;
     } else if (live) {
@@ -702,9 +729,10 @@ __global__ __launch_bounds__(kCgBlock) void cg_iter_kernel(Rows R, MatVals mv, C
 }
 
 // Settles the last step of a batch (or a resumed step): the verdict on its
-// final solve, V^{n+1} = Z + x1 (+ x2), the next initial guess, and the
-// clock advance.  A solve still running poisons the clock until the host
-// resumes it.  Launched only after a step, so that one is pending.
+// final solve, V^{n+1} = Z + x1 (+ x2), the next initial guess and the
+// activation times; advance_kernel then counts the step.  A solve still
+// running poisons the clock until the host resumes it.  Launched only after a
+// step, so that one is pending.
 template <class RV>
 struct FinishArgs {
     int n;
@@ -733,8 +761,12 @@ __global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
     double z0 = 0, z1 = 0;
     if (i0 < rr.hi) { d0 = a.s.x1[i0] + (a.s.x2 ? a.s.x2[i0] : RV(0)); z0 = a.s.Z[i0]; }
     if (i1 < rr.hi) { d1 = a.s.x1[i1] + (a.s.x2 ? a.s.x2[i1] : RV(0)); z1 = a.s.Z[i1]; }
-    auto settle_guess = [&](int i, RV d) {
-        if (a.s.guess) a.s.guess[i] = next_guess(a.s, i, a.s.guess[i] + d, a.s.prev ? a.s.prev[i] : RV(0));
+    StepClock clk = *clock;
+    // activation bookkeeping (V^k = Z - x0) and the next guess
+    auto settle_guess = [&](int i, RV d, double z, double v) {
+        const RV x0 = a.s.guess ? a.s.guess[i] : RV(0);
+        if (a.s.act) track_activation(a.s, a.n, i, clk, k, z - (double)x0, v);
+        if (a.s.guess) a.s.guess[i] = next_guess(a.s, i, x0 + d, a.s.prev ? a.s.prev[i] : RV(0));
     };
     double g[3];
     pl.total(g);
@@ -751,28 +783,46 @@ __global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
     if (i0 < rr.hi) {
         const double v = z0 + (double)d0;
         a.V[i0] = v;
-        settle_guess(i0, d0);
+        settle_guess(i0, d0, z0, v);
         bad |= !isfinite(v);
     }
     if (i1 < rr.hi) {
         const double v = z1 + (double)d1;
         a.V[i1] = v;
-        settle_guess(i1, d1);
+        settle_guess(i1, d1, z1, v);
         bad |= !isfinite(v);
     }
     for (int i = i1 + kCgBlock; i < rr.hi; i += kCgBlock) {
         const RV d = a.s.x1[i] + (a.s.x2 ? a.s.x2[i] : RV(0));
-        const double v = a.s.Z[i] + (double)d;
+        const double z = a.s.Z[i];
+        const double v = z + (double)d;
         a.V[i] = v;
-        settle_guess(i, d);
+        settle_guess(i, d, z, v);
         bad |= !isfinite(v);
     }
     if (bad) clock->nonfinite = 1;
-    if (lead) {
-        log_step(a.its_log, a.log_len, k, a.ctl_all, a.refined != 0);
-        clock->k = k + 1;
-        clock->pending = 0;
-    }
+}
+
+// After a finish kernel: counts the step it settled (a poisoned clock means
+// it did not).  A kernel of its own, so that no block of the finish kernel
+// reads a clock field written in the same launch.
+__global__ void advance_kernel(StepClock* clock, const SolveCtl* ctl_all, int refined, int32_t* its_log, int log_len)
+{
+    if (clock->poison || !clock->pending) return;
+    log_step(its_log, log_len, clock->k, ctl_all, refined != 0);
+    clock->k += 1;
+    clock->pending = 0;
+}
+
+// Activation times reset to -1 and the voltage at t processed
+// (AbstractCardiacProblem.cpp:459-464: the modifier sees the initial condition).
+__global__ __launch_bounds__(kBlock) void activation_init_kernel(int n, const double* __restrict__ V,
+                                                                 double* __restrict__ act, double thr, double t)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < 5; ++c) act[(size_t)c * n + i] = -1.0;
+    modifier_update(act, n, i, t, V[i], thr);
 }
 
 }  // namespace cmx
@@ -805,6 +855,9 @@ struct cmx_ctx {
     void* G1 = nullptr;          // F - c x0
     double* r0_64 = nullptr;     // FP64 step residual (refinement)
     double* iion = nullptr;      // scratch for cmx_get_iionic
+    double* act = nullptr;       // [5][n] activation times (cmx_set_activation)
+    double act_thr = 0.0;
+    int act_every = 0;
     // matrix
     uint8_t* pid = nullptr;
     std::vector<int32_t> h_off;  // stencil offsets, padded to the kernel width
@@ -837,7 +890,7 @@ struct cmx_ctx {
     double device_bytes = 0;
     std::vector<void*> allocs;
     // hipGraphs of whole steps, keyed by (budget, refinement budget, steps)
-    struct StepGraph { int budget, refine_budget, steps; hipGraph_t graph; hipGraphExec_t exec; };
+    struct StepGraph { int budget, refine_budget, steps, act_every; double act_thr; hipGraph_t graph; hipGraphExec_t exec; };
     std::vector<StepGraph> graphs;
     long long graph_launches = 0;
     // kernel timing (cmx_set_profiling)
@@ -1045,6 +1098,9 @@ SettleArgs<RV> settle_args(cmx_ctx* c, const double* verdict)
     s.x2 = refined ? (const RV*)c->vec[V_X2] : nullptr;
     s.guess = c->extrapolate ? (RV*)c->vec[V_X0] : nullptr;
     s.prev = c->extrapolate >= 2 ? (RV*)c->vec[V_XP] : nullptr;
+    s.act = c->act_every > 0 ? c->act : nullptr;
+    s.act_thr = c->act_thr;
+    s.act_every = c->act_every > 0 ? c->act_every : 1;
     return s;
 }
 
@@ -1158,6 +1214,7 @@ int launch_finish(cmx_ctx* c, const double* verdict)
     a.s = settle_args<RV>(c, verdict);
     KernelTimer timer(c, CMX_K_FINISH);
     finish_kernel<RV><<<c->G, kCgBlock, 0, c->stream>>>(a);
+    advance_kernel<<<1, 1, 0, c->stream>>>(c->clock, c->ctl, a.refined, c->its_log, c->batch);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
@@ -1205,7 +1262,9 @@ template <class Rows, class RS, class RV>
 hipGraphExec_t step_graph(cmx_ctx* c, const Rows& R, int steps)
 {
     for (auto& g : c->graphs)
-        if (g.budget == c->budget && g.refine_budget == c->refine_budget && g.steps == steps) return g.exec;
+        if (g.budget == c->budget && g.refine_budget == c->refine_budget && g.steps == steps &&
+            g.act_every == c->act_every && g.act_thr == c->act_thr)
+            return g.exec;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;
@@ -1230,7 +1289,7 @@ hipGraphExec_t step_graph(cmx_ctx* c, const Rows& R, int steps)
         (void)hipGraphDestroy(graph);
         return nullptr;
     }
-    c->graphs.push_back({c->budget, c->refine_budget, steps, graph, exec});
+    c->graphs.push_back({c->budget, c->refine_budget, steps, c->act_every, c->act_thr, graph, exec});
     return exec;
 }
 
@@ -1740,6 +1799,36 @@ int cmx_get_iionic(cmx_ctx* c, double* out)
         const double is = sid ? host_stimulus(c->h_stim[sid - 1], c->last_t_next) : 0.0;
         out[i] = -(F[i] + is) / c->am;
     }
+    return CMX_OK;
+}
+
+int cmx_set_activation(cmx_ctx* c, double threshold, int32_t every_steps, double t_now)
+{
+    if (!c) return fail(nullptr, CMX_E_ARG, "null context");
+    CMX_HIP(c, hipStreamSynchronize(c->stream));
+    if (every_steps <= 0) {
+        c->act_every = 0;
+        return CMX_OK;
+    }
+    if (!std::isfinite(threshold)) return fail(c, CMX_E_ARG, "activation threshold must be finite");
+    if (!c->act) {
+        int rc = dalloc(c, &c->act, 5 * (size_t)c->n);
+        if (rc) return rc;
+    }
+    c->act_thr = threshold;
+    c->act_every = every_steps;
+    activation_init_kernel<<<grid_of(c->n), kBlock, 0, c->stream>>>(c->n, c->V, c->act, threshold, t_now);
+    CMX_HIP(c, hipGetLastError());
+    CMX_HIP(c, hipStreamSynchronize(c->stream));
+    return CMX_OK;
+}
+
+int cmx_get_activation(cmx_ctx* c, double* out)
+{
+    if (!c || !out) return fail(c, CMX_E_ARG, "null argument");
+    if (!c->act) return fail(c, CMX_E_STATE, "activation tracking was never enabled (cmx_set_activation)");
+    CMX_HIP(c, hipStreamSynchronize(c->stream));
+    CMX_HIP(c, hipMemcpy(out, c->act, 5 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost));
     return CMX_OK;
 }
 

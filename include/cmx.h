@@ -136,6 +136,19 @@ int cmx_set_state(cmx_ctx* ctx, const double* states);
 /* Ionic current cache (uA/cm^2) of the last step
  * (AbstractCardiacTissue::rGetIionicCacheReplicated, heart/src/tissue/AbstractCardiacTissue.cpp:730-734). */
 int cmx_get_iionic(cmx_ctx* ctx, double* iion);
+
+/* Activation and recovery times, ActivationOutputModifier
+ * (heart/src/problem/ActivationOutputModifier.cpp:40-128) driven as
+ * AbstractCardiacProblem::Solve drives it (AbstractCardiacProblem.cpp:459-464,
+ * 552-555): the voltage at t_now and then after every every_steps-th PDE step
+ * of a cmx_step call (counted from its t0; every_steps = printing dt / PDE dt)
+ * updates first activation (V > threshold), first recovery (V < threshold),
+ * second activation and second recovery, -1 until seen.  A fifth column holds
+ * the first upward crossing of the threshold linearly interpolated between
+ * PDE steps (checked every step).  every_steps <= 0 stops tracking. */
+int cmx_set_activation(cmx_ctx* ctx, double threshold, int32_t every_steps, double t_now);
+/* out[5][n_nodes]: first act., first rec., second act., second rec., interpolated first crossing (ms). */
+int cmx_get_activation(cmx_ctx* ctx, double* out);
 int cmx_get_stats(cmx_ctx* ctx, cmx_stats* stats);
 
 /* Per-kernel timing with HIP events recorded on the context's stream around

@@ -537,7 +537,26 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
     for (int i = 0; i < n; ++i) V[i] = y0[0];
     std::vector<char> stimulated(n);
     for (int i = 0; i < n; ++i) stimulated[i] = coords[(size_t)i * dim] < cfg->stim_xlim;
-    Stimulus stim(cfg->stim_mag, cfg->stim_dur, cfg->stim_start, -1.0);
+    Stimulus stim(cfg->stim_mag, cfg->stim_dur, cfg->stim_start, cfg->stim_period > 0 ? cfg->stim_period : -1.0);
+    // ActivationOutputModifier::ProcessSolutionAtTimeStep
+    // (heart/src/problem/ActivationOutputModifier.cpp:94-125), called with the
+    // initial condition and after every printing step
+    // (heart/src/problem/AbstractCardiacProblem.cpp:459-464, 552-555).
+    auto modifier = [&](double time) {
+        double* m = cfg->modifier_out;
+        const double thr = cfg->modifier_threshold;
+        for (int i = 0; i < n; ++i) {
+            const double v = V[i];
+            if (v > thr && m[i] < 0.0) m[i] = time;
+            else if (m[i] >= 0.0 && m[n + i] < 0.0 && v < thr) m[n + i] = time;
+            else if (m[n + i] >= 0.0 && m[2 * n + i] < 0.0 && v > thr) m[2 * n + i] = time;
+            else if (m[2 * n + i] >= 0.0 && m[3 * n + i] < 0.0 && v < thr) m[3 * n + i] = time;
+        }
+    };
+    if (cfg->modifier_out) {
+        std::fill(cfg->modifier_out, cfg->modifier_out + 4 * (size_t)n, -1.0);
+        modifier(0.0);
+    }
     const int P = num_params(model);
     const double* pdef = default_params(model);
     if (act_out) std::fill(act_out, act_out + n, -1.0);
@@ -594,6 +613,7 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
         if (its < 0) return -2;
         st.cg_iterations += its;
         pde.advance();
+        if (cfg->modifier_out && (step + 1) % std::max(cfg->modifier_every, 1) == 0) modifier(t_next);
         if (act_out) {
             for (int i = 0; i < n; ++i)
                 if (act_out[i] < 0 && Vprev[i] < act_threshold && V[i] >= act_threshold)

@@ -90,6 +90,10 @@ typedef struct {
     const double* params;     /* [n_param][n] node-major planes, or NULL */
     int lumped;               /* HeartConfig::SetUseMassLumping */
     const double* sigma_elem; /* [n_elems][dim*dim] per-element tensors, or NULL for sigma[] */
+    double stim_period;       /* > 0: RegularStimulus with this period (ms) */
+    double* modifier_out;     /* [4][n] ActivationOutputModifier times, or NULL */
+    double modifier_threshold;
+    int modifier_every;       /* process the voltage every this many PDE steps (and at t = 0) */
 } orc_mono_cfg;
 
 typedef struct {

@@ -26,7 +26,8 @@ class MonoCfg(ctypes.Structure):
                 ('stim_dur', ctypes.c_double), ('stim_start', ctypes.c_double), ('stim_xlim', ctypes.c_double),
                 ('t_end', ctypes.c_double), ('ksp_rtol', ctypes.c_double), ('ksp_atol', ctypes.c_double),
                 ('max_its', ctypes.c_int), ('threads', ctypes.c_int), ('n_param', ctypes.c_int), ('params', _dp),
-                ('lumped', ctypes.c_int), ('sigma_elem', _dp)]
+                ('lumped', ctypes.c_int), ('sigma_elem', _dp), ('stim_period', ctypes.c_double),
+                ('modifier_out', _dp), ('modifier_threshold', ctypes.c_double), ('modifier_every', ctypes.c_int)]
 
 
 class MonoStats(ctypes.Structure):
@@ -134,7 +135,8 @@ def pcg(rp, col, val, b, x0, rtol=1e-12, atol=0.0, max_its=10000):
 def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, sigma=(1.75, 1.75, 1.75),
              am=1400.0, cm=1.0, dt_pde=0.01, dt_ode=0.01, stim_mag=-3e6, stim_dur=0.5, stim_start=0.0,
              stim_xlim=1e-5, t_end=1.0, ksp_rtol=1e-12, ksp_atol=0.0, max_its=10000, threads=0,
-             params=None, trace_nodes=None, act_threshold=None, want_state=False, lumped=False, sigma_elem=None):
+             params=None, trace_nodes=None, act_threshold=None, want_state=False, lumped=False, sigma_elem=None,
+             stim_period=0.0, modifier_threshold=None, modifier_every=1):
     """Full FP64 monodomain run (AbstractCardiacProblem::Solve equivalent)."""
     n = [ex + 1, (ex + 1) * (ey + 1), (ex + 1) * (ey + 1) * (ez + 1)][dim - 1]
     pa = None
@@ -143,9 +145,11 @@ def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, si
         pa = np.ascontiguousarray(params, dtype=np.float64)
         npar = pa.shape[0]
     se = None if sigma_elem is None else np.ascontiguousarray(sigma_elem, dtype=np.float64).reshape(-1)
+    mod = None if modifier_threshold is None else np.zeros((4, n))
     cfg = MonoCfg(dim, ex, ey, ez, h, model, int(rush_larsen), (ctypes.c_double * 3)(*sigma), am, cm, dt_pde, dt_ode,
                   stim_mag, stim_dur, stim_start, stim_xlim, t_end, ksp_rtol, ksp_atol, max_its, threads, npar,
-                  _p(pa), int(lumped), _p(se))
+                  _p(pa), int(lumped), _p(se), float(stim_period), _p(mod),
+                  0.0 if modifier_threshold is None else float(modifier_threshold), int(modifier_every))
     V = np.zeros(n)
     S = num_states(model)
     state = np.zeros((S, n)) if want_state else None
@@ -169,6 +173,8 @@ def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, si
         out['state'] = state
     if act is not None:
         out['act'] = act
+    if mod is not None:
+        out['modifier'] = mod
     if trace is not None:
         out['trace'] = trace[:st.steps]
     return out

@@ -222,3 +222,27 @@ def test_slab_numbering(oracle):
     assert list(e2[0]) == [3, 4, 0] and list(e2[1]) == [1, 0, 4]
     # square (1,0): parity 1 -> upper triangle uses sw+1
     assert list(e2[2]) == [4, 5, 2] and list(e2[3]) == [2, 1, 4]
+
+
+def test_activation_modifier_semantics(oracle):
+    """ActivationOutputModifier times (first time V > threshold at a printing
+    step) against the interpolated first upward crossing of the same run:
+    the modifier's time is the first step after the crossing, so
+    0 <= modifier - crossing < dt; recovery (V < threshold) follows the
+    activation; with a printing step of 5 PDE steps the times land on
+    multiples of 0.05 ms and lie within 0.05 ms after the crossing."""
+    kw = dict(model=oracle.LR91_HAND, stim_mag=-400000.0, stim_dur=0.5, stim_xlim=0.02, t_end=6.0,
+              ksp_rtol=1e-12, act_threshold=-30.0)
+    r1 = oracle.mono_run(2, 12, 12, modifier_threshold=-30.0, modifier_every=1, **kw)
+    cross, m = r1['act'], r1['modifier']
+    hit = cross >= 0
+    assert hit.sum() > 50
+    assert np.array_equal(hit, m[0] >= 0)
+    d = m[0][hit] - cross[hit]
+    assert d.min() >= -1e-12 and d.max() < 0.01 + 1e-12
+    assert (m[1] < 0).all() and (m[2] < 0).all() and (m[3] < 0).all()   # no recovery within 6 ms
+    r5 = oracle.mono_run(2, 12, 12, modifier_threshold=-30.0, modifier_every=5, **kw)
+    m5 = r5['modifier'][0][hit]
+    assert np.allclose(np.round(m5 / 0.05) * 0.05, m5, atol=1e-9)
+    d5 = m5 - cross[hit]
+    assert d5.min() >= -1e-12 and d5.max() < 0.05 + 1e-12
