@@ -30,14 +30,15 @@ HBM_PEAK_GBS = 8000.0
 EX, EY, EZ, H = 200, 70, 30, 0.01
 SIGMA = (1.75, 0.19, 0.19)
 STIM = (-50000.0, 0.0, 1.0)       # magnitude uA/cm^3, start ms, duration ms
+STIM_PROPAGATING = -100000.0      # a magnitude above threshold for the second case
 STIM_XLIM = 0.03
 DT = 0.01
 EXTRAPOLATE = 2      # CG initial guess V^n + 2 x_{n-1} - x_{n-2} (corrections x), in both paths
 
 
-def build_problem(precision, coords, elems, stim_index, cmx):
+def build_problem(precision, coords, elems, stim_index, cmx, stim_mag=STIM[0]):
     return cmx.Monodomain(coords, elems, np.array(SIGMA), cmx.TT06_EPI, cmx.RUSH_LARSEN,
-                          stimuli=[STIM], stim_index=stim_index, precision=precision,
+                          stimuli=[(stim_mag, STIM[1], STIM[2])], stim_index=stim_index, precision=precision,
                           dt_pde=DT, dt_ode=DT, ksp_rtol=1e-8, ksp_atol=0.0, ksp_max_its=500,
                           refine_sweeps=1 if precision == cmx.MIXED else 0, use_graph=True,
                           extrapolate=EXTRAPOLATE)
@@ -85,21 +86,15 @@ KINDS = ['cell', 'residual', 'spmv', 'update', 'finish']
 
 
 def time_path(m, steps, warmup, roof_steps):
-    """W untimed steps; an untimed 5-step pass with events on every kernel
-    kind ranks them; exactly K timed steps with no instrumentation (value);
-    then roof_steps more steps with HIP events around every launch of the
-    dominant kind only (its average launch duration for the roofline)."""
+    """W untimed steps from t = 0, then exactly K timed steps with no
+    instrumentation (value); then an untimed 5-step pass with HIP events on
+    every kernel kind ranks them, and roof_steps more steps with events
+    around every launch of the dominant kind only (its average launch
+    duration for the roofline)."""
     t = 0.0
     if warmup:
         m.step(t, warmup)
         t += warmup * DT
-    m.set_profiling(KINDS)
-    m.kernel_times(reset=True)
-    m.step(t, 5)
-    t += 5 * DT
-    ranking = {k: v for k, v in m.kernel_times(reset=True).items() if k in KINDS}
-    dominant = max(ranking, key=lambda k: ranking[k][0])
-    m.set_profiling([])
     st0 = m.stats()
     sync()
     t0 = time.perf_counter()
@@ -108,6 +103,13 @@ def time_path(m, steps, warmup, roof_steps):
     elapsed = time.perf_counter() - t0
     t += steps * DT
     st1 = m.stats()
+    m.set_profiling(KINDS)
+    m.kernel_times(reset=True)
+    m.step(t, 5)
+    t += 5 * DT
+    ranking = {k: v for k, v in m.kernel_times(reset=True).items() if k in KINDS}
+    dominant = max(ranking, key=lambda k: ranking[k][0])
+    st_r = m.stats()
     m.set_profiling([dominant])
     m.kernel_times(reset=True)
     m.step(t, roof_steps)
@@ -116,9 +118,11 @@ def time_path(m, steps, warmup, roof_steps):
     m.set_profiling([])
     its = (st1['cg_iterations'] - st0['cg_iterations']) / steps
     ref = (st1['refine_sweeps'] - st0['refine_sweeps']) / steps
-    roof_its = (st2['cg_iterations'] - st1['cg_iterations']) / roof_steps
-    return {'elapsed': elapsed, 'its': its, 'refine': ref, 'ranking': ranking, 'dominant': dominant,
-            'dominant_ms': ktimes[dominant][0], 'dominant_launches': ktimes[dominant][1], 'roof_its': roof_its}
+    rank_its = (st_r['cg_iterations'] - st1['cg_iterations']) / 5
+    roof_its = (st2['cg_iterations'] - st_r['cg_iterations']) / roof_steps
+    return {'elapsed': elapsed, 'its': its, 'refine': ref, 'ranking': ranking, 'rank_its': rank_its,
+            'dominant': dominant, 'dominant_ms': ktimes[dominant][0], 'dominant_launches': ktimes[dominant][1],
+            'roof_its': roof_its}
 
 
 def working_launches_per_step(kind, its, refined):
@@ -145,7 +149,7 @@ def cpu_baseline(n_steps, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=300)
+    ap.add_argument('--steps', type=int, default=4980)   # with the warmup: the 50 ms run
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--cpu-steps', type=int, default=20)
     ap.add_argument('--roof-steps', type=int, default=50)
@@ -161,40 +165,59 @@ def main():
     n = coords.shape[0]
     stim_index = np.where(coords[:, 0] < STIM_XLIM, 0, -1).astype(np.int32)
     n_states = cmx.model_info(cmx.TT06_EPI)[0]
+    y0 = cmx.initial_state(cmx.TT06_EPI)
+    init = np.repeat(np.asarray(y0, dtype=np.float64)[:, None], n, axis=1)
 
-    results = {}
-    volts = {}
-    for name, prec in (('fp64', cmx.FP64), ('mixed', cmx.MIXED)):
-        m = build_problem(prec, coords, elems, stim_index, cmx)
-        r = time_path(m, args.steps, args.warmup, args.roof_steps)
-        refined = prec == cmx.MIXED
-        st = m.stats()
-        kinds = {}
-        for k, (ms, cnt) in r['ranking'].items():
-            work = working_launches_per_step(k, r['its'], refined) * 5
-            b = bytes_per_row(k, prec, n_states, refined) * n
-            kinds[k] = {'ms_per_step': ms / 5, 'launches_per_step': cnt / 5,
-                        'working_launches_per_step': work / 5,
-                        'achieved_GBs': (work * b / (ms * 1e-3) / 1e9) if ms > 0 else None}
-        dom = r['dominant']
-        dom_work = working_launches_per_step(dom, r['roof_its'], refined) * args.roof_steps
-        dom_bytes = dom_work * bytes_per_row(dom, prec, n_states, refined) * n
-        results[name] = {
-            'steps_per_s': args.steps / r['elapsed'],
-            'ms_per_step': 1e3 * r['elapsed'] / args.steps,
-            'cg_its_per_step': r['its'],
-            'refine_sweeps_per_step': r['refine'],
-            'kernels_profiled': kinds,
-            'dominant_kernel': dom,
-            'dominant_avg_launch_ms': r['dominant_ms'] / max(r['dominant_launches'], 1),
-            'dominant_bytes_per_launch': dom_bytes / max(r['dominant_launches'], 1),
-            'format': 'stencil' if st['format'] == cmx.FMT_STENCIL else 'ell',
-            'n_patterns': st['n_patterns'],
-        }
-        volts[name] = m.voltage()
-        m.close()
+    def run_case(stim_mag, with_roofline):
+        """Both paths: timing (and ranking / roofline passes), then the
+        accuracy run on the same context reset to the initial state."""
+        results, runs = {}, {}
+        for name, prec in (('fp64', cmx.FP64), ('mixed', cmx.MIXED)):
+            m = build_problem(prec, coords, elems, stim_index, cmx, stim_mag)
+            r = time_path(m, args.steps, args.warmup, args.roof_steps if with_roofline else 5)
+            refined = prec == cmx.MIXED
+            st = m.stats()
+            kinds = {}
+            for k, (ms, cnt) in r['ranking'].items():
+                work = working_launches_per_step(k, r['rank_its'], refined) * 5
+                b = bytes_per_row(k, prec, n_states, refined) * n
+                kinds[k] = {'ms_per_step': ms / 5, 'launches_per_step': cnt / 5,
+                            'working_launches_per_step': work / 5,
+                            'achieved_GBs': (work * b / (ms * 1e-3) / 1e9) if ms > 0 else None}
+            dom = r['dominant']
+            dom_work = working_launches_per_step(dom, r['roof_its'], refined) * args.roof_steps
+            dom_bytes = dom_work * bytes_per_row(dom, prec, n_states, refined) * n
+            results[name] = {
+                'steps_per_s': args.steps / r['elapsed'],
+                'ms_per_step': 1e3 * r['elapsed'] / args.steps,
+                'cg_its_per_step': r['its'],
+                'refine_sweeps_per_step': r['refine'],
+                'kernels_profiled': kinds,
+                'dominant_kernel': dom,
+                'dominant_avg_launch_ms': r['dominant_ms'] / max(r['dominant_launches'], 1),
+                'dominant_bytes_per_launch': dom_bytes / max(r['dominant_launches'], 1),
+                'format': 'stencil' if st['format'] == cmx.FMT_STENCIL else 'ell',
+                'n_patterns': st['n_patterns'],
+            }
+            # accuracy: the same simulation from t = 0 with activation tracking (0 mV)
+            m.set_state(init)
+            m.set_activation(0.0, 1, 0.0)
+            m.step(0.0, args.warmup + args.steps)
+            runs[name] = (m.voltage(), m.activation()[4])
+            m.close()
+        dv = np.abs(runs['mixed'][0] - runs['fp64'][0])
+        a64, a32 = runs['fp64'][1], runs['mixed'][1]
+        both = (a64 >= 0) & (a32 >= 0)
+        acc = {'max_dV_mV': float(dv.max()), 'rms_dV_mV': float(np.sqrt((dv ** 2).mean())),
+               'activation_max_err_ms': float(np.abs(a32[both] - a64[both]).max()) if both.any() else None,
+               'activated_nodes': int(both.sum()),
+               'activation_mismatched_nodes': int(((a64 >= 0) != (a32 >= 0)).sum()),
+               'after_steps': args.warmup + args.steps,
+               'run': 'the same run from t = 0 with activation tracking (first upward crossing of 0 mV)'}
+        return results, acc
 
-    dv = np.abs(volts['mixed'] - volts['fp64'])
+    results, accuracy = run_case(STIM[0], True)
+    prop_results, prop_accuracy = run_case(STIM_PROPAGATING, False)
     mixed = results['mixed']
     steps_s = mixed['steps_per_s']
     dom = mixed['dominant_kernel']
@@ -220,15 +243,24 @@ def main():
         'config': {'workload': 'TT06-epi RL monodomain slab 2.0x0.7x0.3 cm h=0.01 (201x71x31 = 442,401 nodes), '
                                'dt 0.01 ms, stim -50000 uA/cm^3 1 ms x<0.03, Jacobi-CG rtol 1e-8'
                                + {0: '', 1: ', initial guess V^n + x_{n-1}',
-                                  2: ', initial guess V^n + 2 x_{n-1} - x_{n-2}'}[EXTRAPOLATE],
+                                  2: ', initial guess V^n + 2 x_{n-1} - x_{n-2}'}[EXTRAPOLATE]
+                               + ', 50 ms (%d + %d steps)' % (args.warmup, args.steps),
                    'n_nodes': n, 'n_elems': int(elems.shape[0])},
         'node_ode_updates_per_s': round(steps_s * n, 1),
         'fp64_steps_per_s': round(results['fp64']['steps_per_s'], 2),
         'fp64_node_ode_updates_per_s': round(results['fp64']['steps_per_s'] * n, 1),
         'mixed_over_fp64': round(steps_s / results['fp64']['steps_per_s'], 3),
-        'accuracy_mixed_vs_fp64': {'max_dV_mV': float(dv.max()), 'rms_dV_mV': float(np.sqrt((dv ** 2).mean())),
-                                   'after_steps': args.warmup + 5 + args.steps + args.roof_steps},
+        'accuracy_mixed_vs_fp64': accuracy,
         'paths': results,
+        'propagating': {
+            'note': 'the named stimulus stays below threshold in this tissue (no node activates); the same '
+                    'run with %g uA/cm^3 launches a wave across the slab' % STIM_PROPAGATING,
+            'stim_uA_per_cm3': STIM_PROPAGATING,
+            'steps_per_s': round(prop_results['mixed']['steps_per_s'], 2),
+            'fp64_steps_per_s': round(prop_results['fp64']['steps_per_s'], 2),
+            'mixed_over_fp64': round(prop_results['mixed']['steps_per_s'] / prop_results['fp64']['steps_per_s'], 3),
+            'cg_its_per_step': {k: round(v['cg_its_per_step'], 3) for k, v in prop_results.items()},
+            'accuracy_mixed_vs_fp64': prop_accuracy},
         'roofline': {'bound': 'hbm', 'kernel': dom + ' (mixed)', 'achieved': round(achieved, 1),
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
                      'traffic': traffic,

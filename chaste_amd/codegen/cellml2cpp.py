@@ -217,6 +217,12 @@ class Printer:
             if op == 'minus':
                 if len(a) == 1:
                     return '(-%s)' % a[0]
+                # exp(E) - 1 and 1 - exp(E) without cancellation for small E
+                e0, e1 = _exp_arg(args[0]), _exp_arg(args[1])
+                if e0 is not None and _is_one(args[1]):
+                    return 'cmx_expm1(%s)' % self.expr(e0, cname)
+                if e1 is not None and _is_one(args[0]):
+                    return '(-cmx_expm1(%s))' % self.expr(e1, cname)
                 return '(%s - %s)' % (a[0], a[1])
             if op == 'times':
                 return '(' + ' * '.join(a) + ')'
@@ -277,6 +283,109 @@ class Printer:
         return 'R(%s)' % repr(f)
 
 
+def _exp_arg(el):
+    """E when el is exp(E), else None."""
+    if _tag(el) == 'apply':
+        ch = list(el)
+        if _tag(ch[0]) == 'exp':
+            return ch[1]
+    return None
+
+
+def _is_one(el):
+    return _tag(el) == 'cn' and float(Printer.number_value(el)) == 1.0
+
+
+class Evaluator:
+    """Numeric evaluation of model expressions with V given, states at their
+    initial values, parameters at their defaults and stimulus / time zero
+    (used to locate removable singularities)."""
+
+    def __init__(self, model):
+        self.m = model
+
+    def var(self, key, V, memo):
+        m = self.m
+        if key == m.V:
+            return V
+        if key == m.stim or key == m.time:
+            return 0.0
+        if key in memo:
+            return memo[key]
+        if key in m.odes or key not in m.defs:
+            val = m.init(key)
+        else:
+            c, rhs = m.defs[key]
+            val = self.eval(rhs, c, V, memo)
+        memo[key] = val
+        return val
+
+    def eval(self, el, cname, V, memo=None):
+        import math
+        memo = {} if memo is None else memo
+        t = _tag(el)
+        if t == 'ci':
+            return self.var(self.m.canon(cname, el.text.strip()), V, memo)
+        if t == 'cn':
+            return float(Printer.number_value(el))
+        if t == 'pi':
+            return math.pi
+        if t == 'exponentiale':
+            return math.e
+        if t == 'piecewise':
+            for p in el:
+                if _tag(p) == 'piece':
+                    val, cond = list(p)
+                    if self.eval(cond, cname, V, memo):
+                        return self.eval(val, cname, V, memo)
+                elif _tag(p) == 'otherwise':
+                    return self.eval(list(p)[0], cname, V, memo)
+            return float('nan')
+        if t == 'apply':
+            ch = list(el)
+            op = _tag(ch[0])
+            a = [self.eval(x, cname, V, memo) for x in ch[1:] if _tag(x) not in ('degree', 'logbase', 'bvar')]
+            if op == 'plus':
+                return sum(a)
+            if op == 'minus':
+                return -a[0] if len(a) == 1 else a[0] - a[1]
+            if op == 'times':
+                r = 1.0
+                for x in a:
+                    r *= x
+                return r
+            if op == 'divide':
+                return a[0] / a[1] if a[1] != 0 else (float('nan') if a[0] == 0 else math.copysign(float('inf'), a[0]))
+            if op == 'power':
+                return a[0] ** a[1]
+            if op == 'exp':
+                return math.exp(min(a[0], 700.0))
+            if op == 'ln':
+                return math.log(a[0]) if a[0] > 0 else float('nan')
+            if op == 'abs':
+                return abs(a[0])
+            rel = {'lt': lambda x, y: x < y, 'gt': lambda x, y: x > y, 'leq': lambda x, y: x <= y,
+                   'geq': lambda x, y: x >= y, 'eq': lambda x, y: x == y, 'neq': lambda x, y: x != y}
+            if op in rel:
+                return rel[op](a[0], a[1])
+            if op == 'and':
+                return all(a)
+            if op == 'or':
+                return any(a)
+            if op == 'not':
+                return not a[0]
+        raise NotImplementedError(t)
+
+
+# Removable singularities of GHK-type current and rate expressions, e.g.
+# (V - c) / (exp(k (V - c)) - 1): at V = c numerator and denominator both
+# vanish.  The reference's code generator (chaste_codegen, Chaste >= 2021.1)
+# removes these automatically; here an expression that has one is evaluated
+# by linear interpolation between V = c - dV and c + dV when |V - c| < dV,
+# with dV chosen so that the exponent k (V - c) spans +-1e-7.
+SINGULARITY_SPAN = 1e-7
+
+
 def deps_of(el, cname, model, acc):
     """Canonical variables referenced by an expression."""
     if _tag(el) == 'ci':
@@ -294,6 +403,7 @@ class Generator:
         self.param_index = {}
         for i, k in enumerate(params + scales):
             self.param_index[k] = i
+        self.fixed = []         # (variable, singular V, half width) of removed singularities
         self.deps = {}
         for k, (c, rhs) in model.defs.items():
             self.deps[k] = deps_of(rhs, c, model, set())
@@ -332,6 +442,58 @@ class Generator:
             r = set()
         self._state_deps[key] = r
         return r
+
+    def singular_points(self, key):
+        """(c, dV) for each removable singularity of definition key: a
+        division whose numerator or denominator holds exp(E) - 1 (or
+        1 - exp(E)) with E linear in V alone, and whose numerator and
+        denominator both vanish at the root c of E."""
+        m = self.m
+        cname, rhs = m.defs[key]
+        ev = Evaluator(m)
+        found = []
+
+        def v_only(el, c):
+            ds = deps_of(el, c, m, set())
+            return all(d == m.V or self.is_const(d) for d in ds)
+
+        def exp_minus_one(el, c, out):
+            if _tag(el) == 'apply':
+                ch = list(el)
+                if _tag(ch[0]) == 'minus' and len(ch) == 3:
+                    for e, one in ((_exp_arg(ch[1]), ch[2]), (_exp_arg(ch[2]), ch[1])):
+                        if e is not None and _is_one(one) and v_only(e, c):
+                            out.append(e)
+                for x in ch[1:]:
+                    exp_minus_one(x, c, out)
+
+        def walk(el, c):
+            if _tag(el) != 'apply':
+                for x in el:
+                    walk(x, c)
+                return
+            ch = list(el)
+            if _tag(ch[0]) == 'divide':
+                num, den = ch[1], ch[2]
+                cands = []
+                exp_minus_one(num, c, cands)
+                exp_minus_one(den, c, cands)
+                for e in cands:
+                    e0, e1, e2 = ev.eval(e, c, 0.0), ev.eval(e, c, 1.0), ev.eval(e, c, 2.0)
+                    slope = e1 - e0
+                    if slope == 0 or abs(e2 - e0 - 2 * slope) > 1e-9 * (abs(e0) + abs(slope)):
+                        continue
+                    vc = -e0 / slope
+                    n0, d0 = ev.eval(num, c, vc), ev.eval(den, c, vc)
+                    n1, d1 = ev.eval(num, c, vc + 1.0), ev.eval(den, c, vc + 1.0)
+                    if abs(n0) <= 1e-9 * abs(n1) and abs(d0) <= 1e-9 * abs(d1):
+                        pt = (vc, SINGULARITY_SPAN / abs(slope))
+                        if all(abs(pt[0] - q[0]) > 1e-9 for q in found):
+                            found.append(pt)
+            for x in ch[1:]:
+                walk(x, c)
+        walk(rhs, cname)
+        return found
 
     def closure(self, roots):
         """Algebraic variables needed by roots, in evaluation order."""
@@ -451,6 +613,10 @@ class Generator:
             for k in self.closure(roots):
                 c, rhs = m.defs[k]
                 e = pr.expr(rhs, c)
+                for vc, dv in self.singular_points(k):
+                    # the lambda's V shadows the clamped voltage
+                    e = 'cmx_remove_singularity(V, R(%r), R(%r), [&](const R V) { return %s; })' % (vc, dv, e)
+                    self.fixed.append((cident(k), vc, dv))
                 if k in self.scales:
                     e = '(%s) * p[%d]' % (e, self.param_index[k])
                 lines.append('%sconst R %s = %s;' % (indent, cident(k), e))
@@ -691,6 +857,8 @@ def main(argv=None):
         f.write(src)
     print('%s: %d states, %d Rush-Larsen gates: %s' % (
         args.out, len(model.states), len(rl), ', '.join(cident(k) for k in rl)))
+    for name, vc, dv in sorted(set(gen.fixed)):
+        print('  removable singularity in %s at V = %.9g (+-%.3g mV)' % (name, vc, dv))
 
 
 if __name__ == '__main__':

@@ -57,7 +57,7 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(3.56) * cmx_exp((R(0.079) * V))) + (R(310000.0) * cmx_exp((R(0.35) * V))))) : (cmx_div(R(1.0), (R(0.13) * (R(1.0) + cmx_exp(((V + R(10.66)) * (R(1) / (-R(11.1))))))))));
     const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(127140.0)) * cmx_exp((R(0.2444) * V))) - (R(3.474e-05) * cmx_exp(((-R(0.04391)) * V)))) * (V + R(37.78))), (R(1.0) + cmx_exp((R(0.311) * (V + R(79.23))))))) : (R(0.0)));
     const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.1212) * cmx_exp(((-R(0.01052)) * V))), (R(1.0) + cmx_exp(((-R(0.1378)) * (V + R(40.14))))))) : (cmx_div((R(0.3) * cmx_exp(((-R(2.535e-07)) * V))), (R(1.0) + cmx_exp(((-R(0.1)) * (V + R(32.0))))))));
-    const R fast_sodium_current_m_gate__alpha_m = cmx_div((R(0.32) * (V + R(47.13))), (R(1.0) - cmx_exp(((-R(0.1)) * (V + R(47.13))))));
+    const R fast_sodium_current_m_gate__alpha_m = cmx_remove_singularity(V, R(-47.12999999999975), R(9.999999999999947e-07), [&](const R V) { return cmx_div((R(0.32) * (V + R(47.13))), (-cmx_expm1(((-R(0.1)) * (V + R(47.13)))))); });
     const R fast_sodium_current_m_gate__beta_m = (R(0.08) * cmx_exp(((-V) * (R(1) / R(11.0)))));
     const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((y[7] * (R(1) / R(1.0))))));
     const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
@@ -140,7 +140,7 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
     const R time_dependent_potassium_current__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(((ionic_concentrations__Ko + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nao)) * (R(1) / (ionic_concentrations__Ki + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nai))))));
     const R time_dependent_potassium_current__g_K = (time_dependent_potassium_current__g_Kmax * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
-    const R time_dependent_potassium_current_Xi_gate__Xi = (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * (cmx_exp((R(0.04) * (V + R(77.0)))) - R(1.0))), ((V + R(77.0)) * cmx_exp((R(0.04) * (V + R(35.0))))))) : (R(1.0)));
+    const R time_dependent_potassium_current_Xi_gate__Xi = cmx_remove_singularity(V, R(-76.99999999999993), R(2.4999999999999977e-06), [&](const R V) { return (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * cmx_expm1((R(0.04) * (V + R(77.0))))), ((V + R(77.0)) * cmx_exp((R(0.04) * (V + R(35.0))))))) : (R(1.0))); });
     const R time_dependent_potassium_current__i_K = (time_dependent_potassium_current__g_K * y[6] * time_dependent_potassium_current_Xi_gate__Xi * (V - time_dependent_potassium_current__E_K));
     const R time_independent_potassium_current__g_K1 = (R(0.6047) * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
     const R time_independent_potassium_current_K1_gate__alpha_K1 = cmx_div(R(1.02), (R(1.0) + cmx_exp((R(0.2385) * ((V - time_independent_potassium_current__E_K1) - R(59.215))))));
@@ -185,7 +185,7 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
     const R time_dependent_potassium_current__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(((ionic_concentrations__Ko + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nao)) * (R(1) / (ionic_concentrations__Ki + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nai))))));
     const R time_dependent_potassium_current__g_K = (time_dependent_potassium_current__g_Kmax * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
-    const R time_dependent_potassium_current_Xi_gate__Xi = (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * (cmx_exp((R(0.04) * (V + R(77.0)))) - R(1.0))), ((V + R(77.0)) * cmx_exp((R(0.04) * (V + R(35.0))))))) : (R(1.0)));
+    const R time_dependent_potassium_current_Xi_gate__Xi = cmx_remove_singularity(V, R(-76.99999999999993), R(2.4999999999999977e-06), [&](const R V) { return (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * cmx_expm1((R(0.04) * (V + R(77.0))))), ((V + R(77.0)) * cmx_exp((R(0.04) * (V + R(35.0))))))) : (R(1.0))); });
     const R time_dependent_potassium_current__i_K = (time_dependent_potassium_current__g_K * y[6] * time_dependent_potassium_current_Xi_gate__Xi * (V - time_dependent_potassium_current__E_K));
     const R time_independent_potassium_current__g_K1 = (R(0.6047) * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
     const R time_independent_potassium_current_K1_gate__alpha_K1 = cmx_div(R(1.02), (R(1.0) + cmx_exp((R(0.2385) * ((V - time_independent_potassium_current__E_K1) - R(59.215))))));

@@ -126,7 +126,7 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), (cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T)))) - R(1.0)));
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R L_type_Ca_current_d_gate__d_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(8.0)) - V) * (R(1) / R(7.5))))));
     const R L_type_Ca_current_d_gate__alpha_d = (cmx_div(R(1.4), (R(1.0) + cmx_exp((((-R(35.0)) - V) * (R(1) / R(13.0)))))) + R(0.25));
     const R L_type_Ca_current_d_gate__beta_d = cmx_div(R(1.4), (R(1.0) + cmx_exp(((V + R(5.0)) * (R(1) / R(5.0))))));
@@ -308,7 +308,7 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), (cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T)))) - R(1.0)));
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, y[13])));
     const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
     const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * y[13]), (y[13] + calcium_pump_current__K_pCa));
@@ -369,7 +369,7 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = R(0);
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), (cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T)))) - R(1.0)));
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, y[13])));
     const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
     const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * y[13]), (y[13] + calcium_pump_current__K_pCa));

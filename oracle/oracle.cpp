@@ -317,6 +317,15 @@ int orc_slab_fill(int dim, int ex, int ey, int ez, double h, double* coords, int
 // Element integrals with the reference's 2nd-order Gauss rules
 // (pde/src/solver/GaussianQuadratureRule.cpp): 1D 2 points, triangles 3
 // points, tetrahedra 4 points; P1 basis phi_0 = 1 - sum(xi), phi_k = xi_k.
+int orc_model_initial_state(int model, double* y)
+{
+    const int S = num_states(model);
+    if (S < 0) return -1;
+    const double* y0 = initial_state(model);
+    for (int s = 0; s < S; ++s) y[s] = y0[s];
+    return S;
+}
+
 int orc_assemble(int dim, int n, const double* coords, int ne, const int* elems,
                  const double* sigma_all, int sigma_per_elem, int lumped, int* nnz_out,
                  int* row_ptr, int* col, double* mass, double* stiff)

@@ -49,6 +49,9 @@ int orc_cell_compute(int model, int rush_larsen, const double* params, double* y
                      double stim_mag, double stim_dur, double stim_start, double stim_period,
                      int steps_per_sample, double* samples, int max_samples);
 
+/* The model's default initial state (CellML order, V first); returns S. */
+int orc_model_initial_state(int model, double* y);
+
 /* GetIIonic at the given state (uA/cm^2). */
 double orc_cell_iionic(int model, const double* params, const double* y);
 

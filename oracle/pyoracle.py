@@ -54,6 +54,7 @@ def lib():
         L.orc_cell_compute.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
+        L.orc_model_initial_state.argtypes = [ctypes.c_int, _dp]
         L.orc_cell_iionic.restype = ctypes.c_double
         L.orc_cell_iionic.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_slab_sizes.argtypes = [ctypes.c_int] * 4 + [_ip, _ip]
@@ -88,6 +89,14 @@ def cell_compute(model, y, t0, t_end, dt, stim=(0.0, 0.0, 0.0, -1.0), rush_larse
     if ns < 0:
         raise RuntimeError('orc_cell_compute failed')
     return out[:ns]
+
+
+def model_initial_state(model):
+    y = np.zeros(64)
+    S = lib().orc_model_initial_state(model, _p(y))
+    if S < 0:
+        raise ValueError('unknown model %d' % model)
+    return y[:S].copy()
 
 
 def cell_iionic(model, y, params=None):

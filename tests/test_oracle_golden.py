@@ -246,3 +246,19 @@ def test_activation_modifier_semantics(oracle):
     assert np.allclose(np.round(m5 / 0.05) * 0.05, m5, atol=1e-9)
     d5 = m5 - cross[hit]
     assert d5.min() >= -1e-12 and d5.max() < 0.05 + 1e-12
+
+
+@pytest.mark.parametrize('model,vc', [('TT06_EPI', 15.0), ('LR91_CELLML', -77.0)], ids=['tt06_ical', 'lr91_xi'])
+def test_removable_singularity(oracle, model, vc):
+    """GHK-type 0/0 at V = vc (TT06 i_CaL: (V-15)/(exp(k(V-15)) - 1); LR91
+    Xi: (exp(k(V+77)) - 1)/(V+77)): the generated models interpolate across
+    it, as the reference's code generator does, so I_ion is finite and
+    continuous there (the raw expression gives NaN at exactly vc)."""
+    m = getattr(oracle, model)
+    y = oracle.model_initial_state(m)
+    vals = []
+    for v in (vc - 1e-3, vc, vc + 1e-3):
+        y[0] = v
+        vals.append(oracle.cell_iionic(m, y))
+    assert np.isfinite(vals).all()
+    assert abs(vals[1] - 0.5 * (vals[0] + vals[2])) <= 1e-6 * max(1.0, abs(vals[1]))
