@@ -36,31 +36,42 @@ DT = 0.01
 EXTRAPOLATE = 2      # CG initial guess V^n + 2 x_{n-1} - x_{n-2} (corrections x), in both paths
 
 
+USE_GRAPH = True
+
+
 def build_problem(precision, coords, elems, stim_index, cmx, stim_mag=STIM[0]):
     return cmx.Monodomain(coords, elems, np.array(SIGMA), cmx.TT06_EPI, cmx.RUSH_LARSEN,
                           stimuli=[(stim_mag, STIM[1], STIM[2])], stim_index=stim_index, precision=precision,
                           dt_pde=DT, dt_ode=DT, ksp_rtol=1e-8, ksp_atol=0.0, ksp_max_its=500,
-                          refine_sweeps=1 if precision == cmx.MIXED else 0, use_graph=True,
+                          refine_sweeps=1 if precision == cmx.MIXED else 0, use_graph=USE_GRAPH,
                           extrapolate=EXTRAPOLATE)
 
 
-def bytes_per_row(kind, precision, n_states, refined, guess=True):
+def bytes_per_row(kind, precision, n_states, refined, guess=EXTRAPOLATE):
     """Algorithmic HBM bytes per row of one launch of a kernel kind that does
-    its work.  s = CG-vector / forcing size, ss = cell-state size; V is FP64;
-    the pattern id is one byte.
-      cell      V in, non-V states in+out, stimulus id, F out
-      residual  V, F, pattern id (+ x0 guess) in; r, u0 (+ x = 0) out, + FP64 r0
-                when refined; the refinement sweep reads r0, x1 and writes
-                r, u, x2 (averaged over the two launches of a refined step)
+    its work.  s = CG-vector / forcing size, ss = cell-state size; V and Z are
+    FP64; the pattern id and the stimulus id are one byte each.
+      cell      non-V states in+out, stimulus id, F out; settling the previous
+                step: Z, x1 (+ x2) in, V out; with an initial guess x0 in/out,
+                Y = V + x0 and G = F - c x0 out (+ x_{n-2} in/out at order 2)
+      residual  Z, G gathered (read once), pattern id in; r, u0, x = 0 out,
+                + FP64 r0 when refined; the refinement sweep reads r0, x1 and
+                writes r, u, x2 (averaged over the two launches of a refined step)
       spmv      (solve start) u0 gathered, r in; w, m out
       update    (pipelined CG iteration) m gathered, r w x z s p in; z s p x r w m out
-      finish    V in/out, x1 (+ x2) in"""
+      finish    (batch end) Z, x1 (+ x2) in, V out"""
     s = 4 if precision == 1 else 8
     ss = s
     if kind == 'cell':
-        return 8 + 2 * (n_states - 1) * ss + 1 + s
+        b = 2 * (n_states - 1) * ss + 1 + s                  # states, stimulus id, F
+        b += 8 + s + (s if refined else 0) + 8               # Z, x1, x2 in; V out
+        if guess:
+            b += 2 * s + 8 + s                               # x0 in/out, Y, G out
+            if guess >= 2:
+                b += 2 * s                                   # x_{n-2} in/out
+        return b
     if kind == 'residual':
-        first = 8 + s + 1 + (s if guess else 0) + 2 * s + (0 if guess else s) + (8 if refined else 0)
+        first = 8 + s + 1 + 3 * s + (8 if refined else 0)
         if not refined:
             return first
         return (first + (8 + s + 1 + 3 * s)) / 2
@@ -85,7 +96,11 @@ def sync():
 KINDS = ['cell', 'residual', 'spmv', 'update', 'finish']
 
 
-def time_path(m, steps, warmup, roof_steps):
+def progress(msg):
+    print('[bench] ' + msg, file=sys.stderr, flush=True)
+
+
+def time_path(m, steps, warmup, roof_steps, events=True):
     """W untimed steps from t = 0, then exactly K timed steps with no
     instrumentation (value); then an untimed 5-step pass with HIP events on
     every kernel kind ranks them, and roof_steps more steps with events
@@ -101,8 +116,14 @@ def time_path(m, steps, warmup, roof_steps):
     m.step(t, steps)
     sync()
     elapsed = time.perf_counter() - t0
+    progress('timed %d steps: %.3f s' % (steps, elapsed))
     t += steps * DT
     st1 = m.stats()
+    its = (st1['cg_iterations'] - st0['cg_iterations']) / steps
+    ref = (st1['refine_sweeps'] - st0['refine_sweeps']) / steps
+    if not events:
+        return {'elapsed': elapsed, 'its': its, 'refine': ref, 'ranking': {}, 'rank_its': its,
+                'dominant': 'cell', 'dominant_ms': float('nan'), 'dominant_launches': 1, 'roof_its': its}
     m.set_profiling(KINDS)
     m.kernel_times(reset=True)
     m.step(t, 5)
@@ -116,8 +137,6 @@ def time_path(m, steps, warmup, roof_steps):
     st2 = m.stats()
     ktimes = m.kernel_times(reset=True)
     m.set_profiling([])
-    its = (st1['cg_iterations'] - st0['cg_iterations']) / steps
-    ref = (st1['refine_sweeps'] - st0['refine_sweeps']) / steps
     rank_its = (st_r['cg_iterations'] - st1['cg_iterations']) / 5
     roof_its = (st2['cg_iterations'] - st_r['cg_iterations']) / roof_steps
     return {'elapsed': elapsed, 'its': its, 'refine': ref, 'ranking': ranking, 'rank_its': rank_its,
@@ -154,7 +173,13 @@ def main():
     ap.add_argument('--cpu-steps', type=int, default=20)
     ap.add_argument('--roof-steps', type=int, default=50)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--skip-propagating', action='store_true', help='only the named-stimulus case')
+    ap.add_argument('--no-graph', action='store_true', help='plain launches instead of hipGraph replay')
+    ap.add_argument('--no-kernel-events', action='store_true',
+                    help='no per-kernel event timing (ranking / roofline passes); for runs under a profiler')
     args = ap.parse_args()
+    global USE_GRAPH
+    USE_GRAPH = not args.no_graph
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one GPU only: the monodomain slab runs on a single MI355X here '
               '(multi-GPU sharding is a design note, not built); run with --gpus 1', file=sys.stderr)
@@ -173,8 +198,10 @@ def main():
         accuracy run on the same context reset to the initial state."""
         results, runs = {}, {}
         for name, prec in (('fp64', cmx.FP64), ('mixed', cmx.MIXED)):
+            progress('%s stim %g: setup' % (name, stim_mag))
             m = build_problem(prec, coords, elems, stim_index, cmx, stim_mag)
-            r = time_path(m, args.steps, args.warmup, args.roof_steps if with_roofline else 5)
+            r = time_path(m, args.steps, args.warmup, args.roof_steps if with_roofline else 5,
+                          events=not args.no_kernel_events)
             refined = prec == cmx.MIXED
             st = m.stats()
             kinds = {}
@@ -200,6 +227,7 @@ def main():
                 'n_patterns': st['n_patterns'],
             }
             # accuracy: the same simulation from t = 0 with activation tracking (0 mV)
+            progress('%s stim %g: accuracy run' % (name, stim_mag))
             m.set_state(init)
             m.set_activation(0.0, 1, 0.0)
             m.step(0.0, args.warmup + args.steps)
@@ -217,15 +245,18 @@ def main():
         return results, acc
 
     results, accuracy = run_case(STIM[0], True)
-    prop_results, prop_accuracy = run_case(STIM_PROPAGATING, False)
+    prop_results, prop_accuracy = (None, None) if args.skip_propagating else run_case(STIM_PROPAGATING, False)
     mixed = results['mixed']
     steps_s = mixed['steps_per_s']
     dom = mixed['dominant_kernel']
     achieved = mixed['dominant_bytes_per_launch'] / (mixed['dominant_avg_launch_ms'] * 1e-3) / 1e9
+    timed_kernels = bool(np.isfinite(achieved))
     traffic = None
     tfile = os.path.join(ROOT, 'profiles', 'hbm_traffic_slab442k.json')
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get('mixed', {}).get(dom)
+        t = json.load(open(tfile)).get('mixed', {}).get(dom)
+        # bytes moved per launch that does its work (the roofline's launches)
+        traffic = round(t['bytes_per_working_dispatch']) if t else None
 
     line = {
         'metric': 'monodomain steps/s + node-ODE updates/s, FP32-mixed vs FP64 at <=0.5 mV',
@@ -252,7 +283,7 @@ def main():
         'mixed_over_fp64': round(steps_s / results['fp64']['steps_per_s'], 3),
         'accuracy_mixed_vs_fp64': accuracy,
         'paths': results,
-        'propagating': {
+        'propagating': None if prop_results is None else {
             'note': 'the named stimulus stays below threshold in this tissue (no node activates); the same '
                     'run with %g uA/cm^3 launches a wave across the slab' % STIM_PROPAGATING,
             'stim_uA_per_cm3': STIM_PROPAGATING,
@@ -261,13 +292,14 @@ def main():
             'mixed_over_fp64': round(prop_results['mixed']['steps_per_s'] / prop_results['fp64']['steps_per_s'], 3),
             'cg_its_per_step': {k: round(v['cg_its_per_step'], 3) for k, v in prop_results.items()},
             'accuracy_mixed_vs_fp64': prop_accuracy},
-        'roofline': {'bound': 'hbm', 'kernel': dom + ' (mixed)', 'achieved': round(achieved, 1),
+        'roofline': None if not timed_kernels else {'bound': 'hbm', 'kernel': dom + ' (mixed)', 'achieved': round(achieved, 1),
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
                      'traffic': traffic,
                      'bytes_per_launch': round(mixed['dominant_bytes_per_launch']),
                      'avg_launch_ms': round(mixed['dominant_avg_launch_ms'], 5)},
     }
     if not args.no_cpu_baseline:
+        progress('cpu baseline')
         threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
         threads = max(1, min(threads, os.cpu_count() or 1))
         cpu_sps, cpu_stats = cpu_baseline(args.cpu_steps, threads)

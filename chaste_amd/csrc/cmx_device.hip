@@ -29,6 +29,7 @@
 // finish kernel poisons the clock, every later kernel returns at once, and
 // the host resumes that solve where it stopped (all its buffers are intact),
 // then re-enqueues the rest of the batch.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -158,8 +159,9 @@ struct CellArgs {
     double am, cm;
     StepClock* clock;
     double* __restrict__ V;
-    RS* __restrict__ state;    // [NS-1][n]: plane s-1 holds state s (V excluded)
-    const RS* __restrict__ params;  // [NP][n] or null
+    int ps;                    // plane stride (n rounded up to whole 256-byte units)
+    RS* __restrict__ state;    // [NS-1][ps]: plane s-1 holds state s (V excluded)
+    const RS* __restrict__ params;  // [NP][ps] or null
     const RS* __restrict__ pdef;    // [NP] defaults
     int n_param_planes;
     const uint8_t* __restrict__ stim_id;  // 0 = none, k = row k-1
@@ -194,10 +196,10 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
     RV dx = 0, x0 = 0, xp = 0;
     if (live) {
 #pragma unroll
-        for (int s = 1; s < NS; ++s) y[s] = a.state[(size_t)(s - 1) * a.n + i];
+        for (int s = 1; s < NS; ++s) y[s] = a.state[(size_t)(s - 1) * a.ps + i];
 #pragma unroll
         for (int q = 0; q < NP; ++q)
-            p[q] = (q < a.n_param_planes) ? a.params[(size_t)q * a.n + i] : (Model::NP > 0 ? a.pdef[q] : RS(0));
+            p[q] = (q < a.n_param_planes) ? a.params[(size_t)q * a.ps + i] : (Model::NP > 0 ? a.pdef[q] : RS(0));
         sid = a.stim_id ? (int)a.stim_id[i] : 0;
         z = a.s.Z[i];
         dx = a.s.x1[i] + (a.s.x2 ? a.s.x2[i] : RV(0));
@@ -261,12 +263,12 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         a.G1[i] = (RV)((double)f - a.c * (double)x0);
     }
 #pragma unroll
-    for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.n + i] = y[s];
+    for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.ps + i] = y[s];
 }
 
 // Iion at the stored state (cmx_get_iionic before any step).
 template <class Model, class RS>
-__global__ __launch_bounds__(kBlock) void iionic_kernel(int n, const double* __restrict__ V, const RS* __restrict__ state,
+__global__ __launch_bounds__(kBlock) void iionic_kernel(int n, int ps, const double* __restrict__ V, const RS* __restrict__ state,
                                                         const RS* __restrict__ params, const RS* __restrict__ pdef,
                                                         int n_param_planes, double cm, double* __restrict__ out)
 {
@@ -277,11 +279,11 @@ __global__ __launch_bounds__(kBlock) void iionic_kernel(int n, const double* __r
     RS y[NS];
     y[0] = (RS)V[i];
 #pragma unroll
-    for (int s = 1; s < NS; ++s) y[s] = state[(size_t)(s - 1) * n + i];
+    for (int s = 1; s < NS; ++s) y[s] = state[(size_t)(s - 1) * ps + i];
     RS p[NP];
 #pragma unroll
     for (int q = 0; q < NP; ++q)
-        p[q] = (q < n_param_planes) ? params[(size_t)q * n + i] : (Model::NP > 0 ? pdef[q] : RS(0));
+        p[q] = (q < n_param_planes) ? params[(size_t)q * ps + i] : (Model::NP > 0 ? pdef[q] : RS(0));
     out[i] = (double)Model::template iion<RS>(y, y[0], p, (RS)cm);
 }
 
@@ -862,6 +864,7 @@ struct cmx_ctx {
     uint8_t* pid = nullptr;
     std::vector<int32_t> h_off;  // stencil offsets, padded to the kernel width
     int pad = 0;                 // guard elements on each side of the gathered vectors
+    int ps = 0;                  // cell-state / parameter plane stride (elements)
     int32_t* ell_col = nullptr;
     double *m64 = nullptr, *k64 = nullptr, *a64 = nullptr, *dinv64 = nullptr;
     float *a32 = nullptr, *dinv32 = nullptr;
@@ -981,26 +984,22 @@ hipEvent_t take_event(cmx_ctx* c)
     return e;
 }
 
-// Brackets one launch of the given kind with events when profiling it.
-struct KernelTimer {
-    cmx_ctx* c;
-    int kind;
-    hipEvent_t a = nullptr;
-    KernelTimer(cmx_ctx* ctx, int k) : c(ctx), kind(k)
-    {
-        if (c->prof_mask & (1 << kind)) {
-            a = take_event(c);
-            (void)hipEventRecord(a, c->stream);
-        }
-    }
-    ~KernelTimer()
-    {
-        if (!a) return;
-        hipEvent_t b = take_event(c);
-        (void)hipEventRecord(b, c->stream);
-        c->pending.push_back({kind, a, b});
-    }
-};
+// Launches a kernel (named directly, in parentheses) on the context's
+// stream.  When its kind is being profiled the launch goes through
+// hipExtLaunchKernelGGL with start/stop events that the dispatch itself
+// timestamps (the kernel's own begin and end, as a kernel trace records them,
+// without event packets in between).
+#define CMX_LAUNCH(c, kind, kernel, grid, block, ...)                                                   \
+    do {                                                                                                \
+        if ((kind) >= 0 && ((c)->prof_mask & (1 << (kind)))) {                                         \
+            hipEvent_t ev_a_ = take_event(c), ev_b_ = take_event(c);                                    \
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, (c)->stream, ev_a_, ev_b_, 0,     \
+                                  __VA_ARGS__);                                                         \
+            (c)->pending.push_back({(kind), ev_a_, ev_b_});                                             \
+        } else {                                                                                        \
+            kernel<<<dim3(grid), dim3(block), 0, (c)->stream>>>(__VA_ARGS__);                           \
+        }                                                                                               \
+    } while (0)
 
 // Folds completed event pairs into the totals (call after a stream sync).
 void collect_times(cmx_ctx* c)
@@ -1116,6 +1115,7 @@ int launch_cells(cmx_ctx* c)
     a.cm = c->cm;
     a.clock = c->clock;
     a.V = c->V;
+    a.ps = c->ps;
     a.state = (RS*)c->state;
     a.params = (const RS*)c->params;
     a.pdef = (const RS*)c->pdef;
@@ -1129,13 +1129,12 @@ int launch_cells(cmx_ctx* c)
     a.G1 = (RV*)c->G1;
     const int G = grid_of(c->n);
     const bool rl = c->integrator == CMX_RUSH_LARSEN;
-    KernelTimer timer(c, CMX_K_CELL);
     if (c->model == CMX_MODEL_LR91) {
-        if (rl) cell_step_kernel<ModelLr91, RS, RV, true><<<G, kBlock, 0, c->stream>>>(a);
-        else cell_step_kernel<ModelLr91, RS, RV, false><<<G, kBlock, 0, c->stream>>>(a);
+        if (rl) CMX_LAUNCH(c, CMX_K_CELL, (cell_step_kernel<ModelLr91, RS, RV, true>), G, kBlock, a);
+        else CMX_LAUNCH(c, CMX_K_CELL, (cell_step_kernel<ModelLr91, RS, RV, false>), G, kBlock, a);
     } else {
-        if (rl) cell_step_kernel<ModelTt06, RS, RV, true><<<G, kBlock, 0, c->stream>>>(a);
-        else cell_step_kernel<ModelTt06, RS, RV, false><<<G, kBlock, 0, c->stream>>>(a);
+        if (rl) CMX_LAUNCH(c, CMX_K_CELL, (cell_step_kernel<ModelTt06, RS, RV, true>), G, kBlock, a);
+        else CMX_LAUNCH(c, CMX_K_CELL, (cell_step_kernel<ModelTt06, RS, RV, false>), G, kBlock, a);
     }
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
@@ -1158,11 +1157,10 @@ int launch_residual(cmx_ctx* c, const Rows& R, int solve)
     a.v = cg_vecs<RV>(c, solve, 0);
     a.partials = pbuf(c, kResPart);
     a.ctl = c->ctl + solve;
-    KernelTimer timer(c, CMX_K_RESIDUAL);
     if (solve == 1)
-        refine_residual_kernel<Rows, RV><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
+        CMX_LAUNCH(c, CMX_K_RESIDUAL, (refine_residual_kernel<Rows, RV>), c->G, kCgBlock, R, mat_vals(c), a);
     else
-        residual_kernel<Rows, RV><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
+        CMX_LAUNCH(c, CMX_K_RESIDUAL, (residual_kernel<Rows, RV>), c->G, kCgBlock, R, mat_vals(c), a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
@@ -1175,8 +1173,7 @@ int launch_cg_start(cmx_ctx* c, const Rows& R, int solve)
     CgArgs<RV> a = cg_args<RV>(c, solve);
     a.first = solve == 0 ? 1 : 0;
     a.v.m_out = (RV*)c->vec[V_MA];   // read by iteration 0
-    KernelTimer timer(c, CMX_K_SPMV);
-    cg_start_kernel<Rows, RV><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
+    CMX_LAUNCH(c, CMX_K_SPMV, (cg_start_kernel<Rows, RV>), c->G, kCgBlock, R, mat_vals(c), a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
@@ -1191,8 +1188,7 @@ int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, int from, int count)
         a.pin = pbuf(c, kIterPart + (it & 1));
         a.pout = pbuf(c, kIterPart + ((it + 1) & 1));
         a.v = cg_vecs<RV>(c, solve, it);
-        KernelTimer timer(c, CMX_K_UPDATE);
-        cg_iter_kernel<Rows, RV><<<c->G, kCgBlock, 0, c->stream>>>(R, mv, a, it);
+        CMX_LAUNCH(c, CMX_K_UPDATE, (cg_iter_kernel<Rows, RV>), c->G, kCgBlock, R, mv, a, it);
     }
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
@@ -1212,8 +1208,7 @@ int launch_finish(cmx_ctx* c, const double* verdict)
     a.log_len = c->batch;
     a.V = c->V;
     a.s = settle_args<RV>(c, verdict);
-    KernelTimer timer(c, CMX_K_FINISH);
-    finish_kernel<RV><<<c->G, kCgBlock, 0, c->stream>>>(a);
+    CMX_LAUNCH(c, CMX_K_FINISH, (finish_kernel<RV>), c->G, kCgBlock, a);
     advance_kernel<<<1, 1, 0, c->stream>>>(c->clock, c->ctl, a.refined, c->its_log, c->batch);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
@@ -1410,20 +1405,23 @@ int steps_typed(cmx_ctx* c)
     return run_steps<EllRows, RS, RV>(c, R);
 }
 
+// Host [planes][n] doubles <-> device [planes][stride] RS (stride >= n; the
+// gap elements stay as they are on the device).
 template <class RS>
-int to_device_planes(cmx_ctx* c, void* dst, const double* src, size_t count)
+int to_device_planes(cmx_ctx* c, void* dst, const double* src, size_t planes, size_t n, size_t stride)
 {
-    std::vector<RS> tmp(src, src + count);
-    CMX_HIP(c, hipMemcpy(dst, tmp.data(), count * sizeof(RS), hipMemcpyHostToDevice));
+    std::vector<RS> tmp(planes * stride, RS(0));
+    for (size_t q = 0; q < planes; ++q) std::copy(src + q * n, src + (q + 1) * n, tmp.begin() + q * stride);
+    CMX_HIP(c, hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(RS), hipMemcpyHostToDevice));
     return CMX_OK;
 }
 
 template <class RS>
-int from_device_planes(cmx_ctx* c, double* dst, const void* src, size_t count)
+int from_device_planes(cmx_ctx* c, double* dst, const void* src, size_t planes, size_t n, size_t stride)
 {
-    std::vector<RS> tmp(count);
-    CMX_HIP(c, hipMemcpy(tmp.data(), src, count * sizeof(RS), hipMemcpyDeviceToHost));
-    std::copy(tmp.begin(), tmp.end(), dst);
+    std::vector<RS> tmp(planes * stride);
+    CMX_HIP(c, hipMemcpy(tmp.data(), src, tmp.size() * sizeof(RS), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < planes; ++q) std::copy(tmp.begin() + q * stride, tmp.begin() + q * stride + n, dst + q * n);
     return CMX_OK;
 }
 
@@ -1510,6 +1508,9 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
         c->h_off.assign(W, 0);
         std::copy(st.off.begin(), st.off.end(), c->h_off.begin());
         for (int32_t o : c->h_off) c->pad = std::max(c->pad, o < 0 ? -o : o);
+        // whole 256-byte units, so that every vector starts on a cache-line
+        // boundary (streamed stores then write whole lines)
+        c->pad = (c->pad + 63) / 64 * 64;
         std::vector<double> m64((size_t)NPt * W, 0.0), k64((size_t)NPt * W, 0.0), a64((size_t)NPt * W, 0.0);
         std::vector<float> a32((size_t)NPt * W, 0.0f);
         std::vector<double> dinv64(NPt);
@@ -1569,20 +1570,23 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     std::vector<double> V0(n, y0[0]);
     if ((rc = dalloc_vec(c, &c->V, n))) return rc;
     CMX_HIP(c, hipMemcpy(c->V, V0.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    if ((rc = dalloc(c, (char**)&c->state, planes.size() * rs))) return rc;
-    if ((rc = mixed ? to_device_planes<float>(c, c->state, planes.data(), planes.size())
-                    : to_device_planes<double>(c, c->state, planes.data(), planes.size())))
+    c->ps = (n + 63) / 64 * 64;
+    const size_t ns = (size_t)(c->S - 1);
+    if ((rc = dalloc(c, (char**)&c->state, ns * c->ps * rs))) return rc;
+    if ((rc = mixed ? to_device_planes<float>(c, c->state, planes.data(), ns, n, c->ps)
+                    : to_device_planes<double>(c, c->state, planes.data(), ns, n, c->ps)))
         return rc;
     const int P = std::max(c->P, 1);
     std::vector<double> pdef(model_pdef(c->model), model_pdef(c->model) + P);
     if ((rc = dalloc(c, (char**)&c->pdef, P * rs))) return rc;
-    if ((rc = mixed ? to_device_planes<float>(c, c->pdef, pdef.data(), P) : to_device_planes<double>(c, c->pdef, pdef.data(), P)))
+    if ((rc = mixed ? to_device_planes<float>(c, c->pdef, pdef.data(), 1, P, P)
+                    : to_device_planes<double>(c, c->pdef, pdef.data(), 1, P, P)))
         return rc;
     if (c->n_param_planes > 0) {
-        const size_t np = (size_t)c->n_param_planes * n;
-        if ((rc = dalloc(c, (char**)&c->params, np * rs))) return rc;
-        if ((rc = mixed ? to_device_planes<float>(c, c->params, tissue->params, np)
-                        : to_device_planes<double>(c, c->params, tissue->params, np)))
+        const size_t np = (size_t)c->n_param_planes;
+        if ((rc = dalloc(c, (char**)&c->params, np * c->ps * rs))) return rc;
+        if ((rc = mixed ? to_device_planes<float>(c, c->params, tissue->params, np, n, c->ps)
+                        : to_device_planes<double>(c, c->params, tissue->params, np, n, c->ps)))
             return rc;
     }
     // Stimulus table (durations padded as the reference's constructors do).
@@ -1746,9 +1750,9 @@ int cmx_get_state(cmx_ctx* c, double* states)
     if (!c || !states) return fail(c, CMX_E_ARG, "null argument");
     int rc = cmx_get_voltage(c, states);
     if (rc) return rc;
-    const size_t cnt = (size_t)(c->S - 1) * c->n;
-    return c->precision == CMX_MIXED ? from_device_planes<float>(c, states + c->n, c->state, cnt)
-                                     : from_device_planes<double>(c, states + c->n, c->state, cnt);
+    const size_t ns = (size_t)(c->S - 1);
+    return c->precision == CMX_MIXED ? from_device_planes<float>(c, states + c->n, c->state, ns, c->n, c->ps)
+                                     : from_device_planes<double>(c, states + c->n, c->state, ns, c->n, c->ps);
 }
 
 int cmx_set_state(cmx_ctx* c, const double* states)
@@ -1756,10 +1760,10 @@ int cmx_set_state(cmx_ctx* c, const double* states)
     if (!c || !states) return fail(c, CMX_E_ARG, "null argument");
     int rc = cmx_set_voltage(c, states);
     if (rc) return rc;
-    const size_t cnt = (size_t)(c->S - 1) * c->n;
+    const size_t ns = (size_t)(c->S - 1);
     c->stepped = false;
-    return c->precision == CMX_MIXED ? to_device_planes<float>(c, c->state, states + c->n, cnt)
-                                     : to_device_planes<double>(c, c->state, states + c->n, cnt);
+    return c->precision == CMX_MIXED ? to_device_planes<float>(c, c->state, states + c->n, ns, c->n, c->ps)
+                                     : to_device_planes<double>(c, c->state, states + c->n, ns, c->n, c->ps);
 }
 
 int cmx_get_iionic(cmx_ctx* c, double* out)
@@ -1774,14 +1778,14 @@ int cmx_get_iionic(cmx_ctx* c, double* out)
         const int G = grid_of(c->n);
         if (c->precision == CMX_MIXED) {
             if (c->model == CMX_MODEL_LR91)
-                iionic_kernel<ModelLr91, float><<<G, kBlock, 0, c->stream>>>(c->n, c->V, (const float*)c->state, (const float*)c->params, (const float*)c->pdef, c->n_param_planes, c->cm, c->iion);
+                iionic_kernel<ModelLr91, float><<<G, kBlock, 0, c->stream>>>(c->n, c->ps, c->V, (const float*)c->state, (const float*)c->params, (const float*)c->pdef, c->n_param_planes, c->cm, c->iion);
             else
-                iionic_kernel<ModelTt06, float><<<G, kBlock, 0, c->stream>>>(c->n, c->V, (const float*)c->state, (const float*)c->params, (const float*)c->pdef, c->n_param_planes, c->cm, c->iion);
+                iionic_kernel<ModelTt06, float><<<G, kBlock, 0, c->stream>>>(c->n, c->ps, c->V, (const float*)c->state, (const float*)c->params, (const float*)c->pdef, c->n_param_planes, c->cm, c->iion);
         } else {
             if (c->model == CMX_MODEL_LR91)
-                iionic_kernel<ModelLr91, double><<<G, kBlock, 0, c->stream>>>(c->n, c->V, (const double*)c->state, (const double*)c->params, (const double*)c->pdef, c->n_param_planes, c->cm, c->iion);
+                iionic_kernel<ModelLr91, double><<<G, kBlock, 0, c->stream>>>(c->n, c->ps, c->V, (const double*)c->state, (const double*)c->params, (const double*)c->pdef, c->n_param_planes, c->cm, c->iion);
             else
-                iionic_kernel<ModelTt06, double><<<G, kBlock, 0, c->stream>>>(c->n, c->V, (const double*)c->state, (const double*)c->params, (const double*)c->pdef, c->n_param_planes, c->cm, c->iion);
+                iionic_kernel<ModelTt06, double><<<G, kBlock, 0, c->stream>>>(c->n, c->ps, c->V, (const double*)c->state, (const double*)c->params, (const double*)c->pdef, c->n_param_planes, c->cm, c->iion);
         }
         CMX_HIP(c, hipGetLastError());
         CMX_HIP(c, hipStreamSynchronize(c->stream));
@@ -1791,8 +1795,8 @@ int cmx_get_iionic(cmx_ctx* c, double* out)
     // After a step the cache is recovered from F = -Am Iion - Istim(t_next).
     std::vector<double> F(c->n);
     CMX_HIP(c, hipStreamSynchronize(c->stream));
-    int rc = c->precision == CMX_MIXED ? from_device_planes<float>(c, F.data(), c->F, c->n)
-                                       : from_device_planes<double>(c, F.data(), c->F, c->n);
+    int rc = c->precision == CMX_MIXED ? from_device_planes<float>(c, F.data(), c->F, 1, c->n, c->n)
+                                       : from_device_planes<double>(c, F.data(), c->F, 1, c->n, c->n);
     if (rc) return rc;
     for (int i = 0; i < c->n; ++i) {
         const int sid = c->h_stim_id[i];

@@ -35,9 +35,11 @@ def summarise(path, command, title):
         idx = [i for i, e in enumerate(ev) if 'cell_step' in e[2] and prec in e[2]]
         if len(idx) < 3:
             continue
-        # bench.py runs warmup, a 5-step ranking pass, the timed steps, then an
-        # event-instrumented pass: take a step from the middle third (timed).
-        k = len(idx) // 3
+        # a typical step: the one of median length among the middle third
+        # (steps spanning a host synchronisation are the long tail)
+        third = range(len(idx) // 3, 2 * len(idx) // 3)
+        spans = sorted((ev[idx[j + 1]][0] - ev[idx[j]][0], j) for j in third if j + 1 < len(idx))
+        k = spans[len(spans) // 2][1]
         i0, i1 = idx[k], idx[k + 1]
         out += ['', 'One %s step (start offset, duration, gap before; microseconds):' % ('FP64' if prec == 'double' else 'FP32-mixed'),
                 '', '| t | duration | gap | kernel |', '|---|---|---|---|']
@@ -85,7 +87,7 @@ def traffic(fetch_dir, write_dir, json_out):
     out = {'scale': {'FETCH_SIZE': scale['FETCH_SIZE'], 'WRITE_SIZE': scale['WRITE_SIZE']}, 'mixed': {}, 'fp64': {}}
     lines = ['# HBM traffic per dispatch: TT06 slab, 442,401 nodes (bench workload)', '',
              'Counter passes (GPU box, repository root; one rocprofv3 run per counter):', '',
-             '    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o run -- python3 bench.py --steps 12 --warmup 2 --roof-steps 2 --no-cpu-baseline',
+             '    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o run -- python3 bench.py --steps 200 --warmup 20 --no-cpu-baseline --skip-propagating --no-kernel-events --no-graph',
              '    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o calib -- python3 -m chaste_amd.hbm_calib 24 1024',
              '    (the same two with WRITE_SIZE into <write_dir>)',
              '    python -m chaste_amd.profile_summary traffic <fetch_dir> <write_dir> profiles/hbm_traffic_slab442k.json', '',
