@@ -61,6 +61,8 @@ struct ModelLr91 {
     }
     template <class R>
     static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return lr91::iionic<R>(y, V, p, cm); }
+    template <class R>
+    static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return lr91::dvdt<R>(y, V, st, p, cm); }
     static constexpr bool gate(int i) { return lr91::kIsGate[i] != 0; }
 };
 
@@ -74,6 +76,8 @@ struct ModelTt06 {
     }
     template <class R>
     static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return tt06_epi::iionic<R>(y, V, p, cm); }
+    template <class R>
+    static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return tt06_epi::dvdt<R>(y, V, st, p, cm); }
     static constexpr bool gate(int i) { return tt06_epi::kIsGate[i] != 0; }
 };
 
@@ -106,14 +110,25 @@ __device__ __forceinline__ void modifier_update(double* __restrict__ act, int n,
     else if (sa >= 0.0 && sr < 0.0 && v < thr) a0[3 * (size_t)n] = t;
 }
 
-// Activation bookkeeping of a settled step k (V^k = v0 -> V^{k+1} = v1).
+// First upward crossing of the threshold, linear in time between (t0, v0)
+// and (t1, v1).
 template <class RV>
-__device__ __forceinline__ void track_activation(const SettleArgs<RV>& s, int n, int i, const StepClock& clk, int k,
-                                                 double v0, double v1)
+__device__ __forceinline__ void record_crossing(const SettleArgs<RV>& s, int n, int i, double v0, double v1,
+                                                double t0, double t1)
 {
-    const double t0 = step_time(clk, k), t1 = step_next_time(clk, k);
     double* cross = s.act + 4 * (size_t)n + i;
     if (*cross < 0.0 && v0 < s.act_thr && v1 >= s.act_thr) *cross = t0 + (t1 - t0) * (s.act_thr - v0) / (v1 - v0);
+}
+
+// Activation bookkeeping of a settled step k: v0 at time t0 (V^k at t_k, or
+// with operator splitting the voltage after the first half step, at the
+// midpoint) -> V^{k+1} = v1 at t_{k+1}.
+template <class RV>
+__device__ __forceinline__ void track_activation(const SettleArgs<RV>& s, int n, int i, const StepClock& clk, int k,
+                                                 double v0, double v1, double t0)
+{
+    const double t1 = step_next_time(clk, k);
+    record_crossing(s, n, i, v0, v1, t0, t1);
     if ((k + 1) % s.act_every == 0) modifier_update(s.act, n, i, t1, v1, s.act_thr);
 }
 
@@ -168,17 +183,57 @@ struct CellArgs {
     const cmx_stimulus* __restrict__ stim;
     RV* __restrict__ F;        // out: -Am Iion - Istim(t_next)
     SettleArgs<RV> s;          // the previous step, when pending
+    int n_half;                // operator splitting: ODE substeps per half step
     // CG initial guess V^n + x0 (x0 = the previous step's correction), when s.guess is set:
     double c;                       // Am Cm / dt
     double* __restrict__ Y;         // out: V^n + x0
     RV* __restrict__ G1;            // out: F - c x0
 };
 
+// Cell-kernel roles: the ICI step (V clamped over the step), the operator-
+// splitting step (second half of the previous step and first half of this
+// one, V updated), and the splitting step's last half only (end of a batch).
+enum CellMode { kModeIci = 0, kModeSplit = 1, kModeSplitFinal = 2 };
+
+// ODE substeps over [t0, t1] with the voltage updated too, as
+// SolveAndUpdateState does (heart/src/odes/AbstractCardiacCell.cpp:70-73,
+// ode/src/solver/AbstractOneStepIvpOdeSolver.cpp:93-153 for forward Euler,
+// heart/src/odes/AbstractRushLarsenCardiacCell.cpp:130-153 for Rush-Larsen:
+// every rate evaluated at the substep's start, V by forward Euler; Rush-Larsen
+// cells always step by dt_ode).  vd is the FP64 voltage.
+template <class Model, class RS, class RV, bool RL>
+__device__ __forceinline__ void integrate_with_voltage(const CellArgs<RS, RV>& a, RS* y, const RS* p, int sid,
+                                                       double& vd, double t0, double t1)
+{
+    constexpr int NS = Model::NS;
+    const RS cm = (RS)a.cm;
+    for (int ks = 0; ks < a.n_half; ++ks) {
+        const double ts = t0 + ks * a.dt_ode;
+        const double h = (!RL && ks + 1 == a.n_half) ? t1 - ts : a.dt_ode;
+        const RS st = sid ? (RS)(stimulus_at(a.stim[sid - 1], ts) / a.am) : RS(0);
+        const RS V = (RS)vd;
+        y[0] = V;
+        RS dy[NS], gi[NS], gk[NS];
+        Model::template eval<RS, RL>(y, V, st, p, cm, dy, gi, gk);
+        const RS dv = Model::template dvdt<RS>(y, V, st, p, cm);
+        const RS hh = (RS)h;
+#pragma unroll
+        for (int s = 1; s < NS; ++s) {
+            if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_exp(gk[s] * hh);
+            else y[s] += hh * dy[s];
+        }
+        vd += h * (double)dv;
+    }
+}
+
 // One thread per node; the state lives in registers for all substeps.  When
 // the previous step is pending, the kernel first settles it: the verdict on
 // its final solve (a failed one poisons the clock and nothing is written),
-// V^n = Z + x1 (+ x2) and the next initial guess.
-template <class Model, class RS, class RV, bool RL>
+// V = Z + x1 (+ x2) and the next initial guess; with operator splitting
+// (OperatorSplittingMonodomainSolver, heart/src/solver/electrics/monodomain/
+// OperatorSplittingMonodomainSolver.cpp:40-137) that voltage then runs the
+// previous step's second half-step of the cells.
+template <class Model, class RS, class RV, bool RL, int MODE>
 __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
 {
     constexpr int NS = Model::NS;
@@ -209,6 +264,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
     double g[3];
     pl.total(g);
     if (clk.poison) return;
+    if (MODE == kModeSplitFinal && !clk.pending) return;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     int k = clk.k;
     double vn = 0;
@@ -221,7 +277,17 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
             return;
         }
         vn = z + (double)dx;
-        if (live && a.s.act) track_activation(a.s, a.n, i, clk, clk.k, z - (double)x0, vn);
+        double v0 = z - (double)x0, t0 = step_time(clk, clk.k);
+        if constexpr (MODE != kModeIci) {
+            // second half-step of the pending step's cells, V updated
+            if (live) {
+                const double tp = step_time(clk, clk.k), tq = step_next_time(clk, clk.k);
+                const double th = tp + 0.5 * (tq - tp);
+                t0 = th;
+                integrate_with_voltage<Model, RS, RV, RL>(a, y, p, sid, vn, th, tq);
+            }
+        }
+        if (live && a.s.act) track_activation(a.s, a.n, i, clk, clk.k, v0, vn, t0);
         if (live && a.s.guess) x0 = next_guess(a.s, i, x0 + dx, xp);
         k = clk.k + 1;
     } else if (live) {
@@ -229,34 +295,51 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
     }
     if (!live) return;
     if (clk.pending) {
-        a.V[i] = vn;
         if (a.s.guess) a.s.guess[i] = x0;
         if (!isfinite(vn)) a.clock->nonfinite = 1;
     }
-    const double t = step_time(clk, k), t_next = step_next_time(clk, k);
-    const RS V = (RS)vn;
-    y[0] = V;
-    const RS cm = (RS)a.cm;
-    // TimeStepper(t, t_next, dt_ode): substep k starts at t + k*dt_ode
-    // (global/src/TimeStepper.cpp:106-121); forward Euler takes the remaining
-    // interval on the last substep, Rush-Larsen always dt_ode
-    // (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:106-128).
-    for (int ks = 0; ks < a.n_sub; ++ks) {
-        const double ts = t + ks * a.dt_ode;
-        const double h = (!RL && ks + 1 == a.n_sub) ? t_next - ts : a.dt_ode;
-        const RS st = sid ? (RS)(stimulus_at(a.stim[sid - 1], ts) / a.am) : RS(0);
-        RS dy[NS], gi[NS], gk[NS];
-        Model::template eval<RS, RL>(y, V, st, p, cm, dy, gi, gk);
-        const RS hh = (RS)h;
+    if constexpr (MODE == kModeSplitFinal) {
+        a.V[i] = vn;
 #pragma unroll
-        for (int s = 1; s < NS; ++s) {
-            if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_exp(gk[s] * hh);
-            else y[s] += hh * dy[s];
-        }
+        for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.ps + i] = y[s];
+        return;
     }
-    const RS iion = Model::template iion<RS>(y, V, p, cm);
-    const double istim = sid ? stimulus_at(a.stim[sid - 1], t_next) : 0.0;
-    const RV f = (RV)(-a.am * (double)iion - istim);
+    const double t = step_time(clk, k), t_next = step_next_time(clk, k);
+    RV f;
+    if constexpr (MODE == kModeSplit) {
+        // first half-step of this step's cells, V updated; the diffusion
+        // solve that follows has no reaction term
+        const double v_start = vn, th = t + 0.5 * (t_next - t);
+        integrate_with_voltage<Model, RS, RV, RL>(a, y, p, sid, vn, t, th);
+        if (a.s.act) record_crossing(a.s, a.n, i, v_start, vn, t, th);
+        a.V[i] = vn;
+        f = RV(0);
+    } else {
+        if (clk.pending) a.V[i] = vn;
+        const RS V = (RS)vn;
+        y[0] = V;
+        const RS cm = (RS)a.cm;
+        // TimeStepper(t, t_next, dt_ode): substep k starts at t + k*dt_ode
+        // (global/src/TimeStepper.cpp:106-121); forward Euler takes the remaining
+        // interval on the last substep, Rush-Larsen always dt_ode
+        // (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:106-128).
+        for (int ks = 0; ks < a.n_sub; ++ks) {
+            const double ts = t + ks * a.dt_ode;
+            const double h = (!RL && ks + 1 == a.n_sub) ? t_next - ts : a.dt_ode;
+            const RS st = sid ? (RS)(stimulus_at(a.stim[sid - 1], ts) / a.am) : RS(0);
+            RS dy[NS], gi[NS], gk[NS];
+            Model::template eval<RS, RL>(y, V, st, p, cm, dy, gi, gk);
+            const RS hh = (RS)h;
+#pragma unroll
+            for (int s = 1; s < NS; ++s) {
+                if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_exp(gk[s] * hh);
+                else y[s] += hh * dy[s];
+            }
+        }
+        const RS iion = Model::template iion<RS>(y, V, p, cm);
+        const double istim = sid ? stimulus_at(a.stim[sid - 1], t_next) : 0.0;
+        f = (RV)(-a.am * (double)iion - istim);
+    }
     a.F[i] = f;
     if (a.s.guess) {
         a.Y[i] = vn + (double)x0;
@@ -767,7 +850,7 @@ __global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
     // activation bookkeeping (V^k = Z - x0) and the next guess
     auto settle_guess = [&](int i, RV d, double z, double v) {
         const RV x0 = a.s.guess ? a.s.guess[i] : RV(0);
-        if (a.s.act) track_activation(a.s, a.n, i, clk, k, z - (double)x0, v);
+        if (a.s.act) track_activation(a.s, a.n, i, clk, k, z - (double)x0, v, step_time(clk, k));
         if (a.s.guess) a.s.guess[i] = next_guess(a.s, i, x0 + d, a.s.prev ? a.s.prev[i] : RV(0));
     };
     double g[3];
@@ -840,6 +923,7 @@ struct cmx_ctx {
     int S = 0, P = 0, n_param_planes = 0;
     double dt = 0, dt_ode = 0, am = 0, cm = 0, rtol = 0, atol = 0;
     int max_its = 0, refine = 0, n_sub = 1, use_graph = 0, extrapolate = 0;
+    int split = 0, n_half = 1;   // operator splitting: ODE substeps per half step
     int width = 0, n_patterns = 0;
     int G = 1;                   // blocks of the reduction kernels
     long long nnz = 0;
@@ -1104,12 +1188,23 @@ SettleArgs<RV> settle_args(cmx_ctx* c, const double* verdict)
 }
 
 // ----- typed launches -------------------------------------------------------
+template <class Model, class RS, class RV, bool RL>
+void launch_cell_mode(cmx_ctx* c, int mode, int kind, int G, const CellArgs<RS, RV>& a)
+{
+    if (mode == kModeIci) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, RL, kModeIci>), G, kBlock, a);
+    else if (mode == kModeSplit) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, RL, kModeSplit>), G, kBlock, a);
+    else CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, RL, kModeSplitFinal>), G, kBlock, a);
+}
+
+// The cell kernel in one of its roles; verdict: the reduction that decides
+// the pending step's final solve.
 template <class RS, class RV>
-int launch_cells(cmx_ctx* c)
+int launch_cells(cmx_ctx* c, int mode, const double* verdict, int kind)
 {
     CellArgs<RS, RV> a;
     a.n = c->n;
     a.n_sub = c->n_sub;
+    a.n_half = c->n_half;
     a.dt_ode = c->dt_ode;
     a.am = c->am;
     a.cm = c->cm;
@@ -1123,18 +1218,18 @@ int launch_cells(cmx_ctx* c)
     a.stim_id = c->stim_id;
     a.stim = c->stim;
     a.F = (RV*)c->F;
-    a.s = settle_args<RV>(c, verdict_buf(c));
+    a.s = settle_args<RV>(c, verdict);
     a.c = c->am * c->cm / c->dt;
     a.Y = c->Y;
     a.G1 = (RV*)c->G1;
     const int G = grid_of(c->n);
     const bool rl = c->integrator == CMX_RUSH_LARSEN;
     if (c->model == CMX_MODEL_LR91) {
-        if (rl) CMX_LAUNCH(c, CMX_K_CELL, (cell_step_kernel<ModelLr91, RS, RV, true>), G, kBlock, a);
-        else CMX_LAUNCH(c, CMX_K_CELL, (cell_step_kernel<ModelLr91, RS, RV, false>), G, kBlock, a);
+        if (rl) launch_cell_mode<ModelLr91, RS, RV, true>(c, mode, kind, G, a);
+        else launch_cell_mode<ModelLr91, RS, RV, false>(c, mode, kind, G, a);
     } else {
-        if (rl) CMX_LAUNCH(c, CMX_K_CELL, (cell_step_kernel<ModelTt06, RS, RV, true>), G, kBlock, a);
-        else CMX_LAUNCH(c, CMX_K_CELL, (cell_step_kernel<ModelTt06, RS, RV, false>), G, kBlock, a);
+        if (rl) launch_cell_mode<ModelTt06, RS, RV, true>(c, mode, kind, G, a);
+        else launch_cell_mode<ModelTt06, RS, RV, false>(c, mode, kind, G, a);
     }
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
@@ -1196,9 +1291,17 @@ int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, int from, int count)
 
 // Settles the pending step; verdict: the reduction that decides its final
 // solve.
-template <class RV>
+template <class RS, class RV>
 int launch_finish(cmx_ctx* c, const double* verdict)
 {
+    if (c->split) {
+        // operator splitting: the last half-step of the cells completes the step
+        int rc = launch_cells<RS, RV>(c, kModeSplitFinal, verdict, CMX_K_FINISH);
+        if (rc) return rc;
+        advance_kernel<<<1, 1, 0, c->stream>>>(c->clock, c->ctl, refined_of(c) ? 1 : 0, c->its_log, c->batch);
+        CMX_HIP(c, hipGetLastError());
+        return CMX_OK;
+    }
     FinishArgs<RV> a;
     a.n = c->n;
     a.clock = c->clock;
@@ -1225,7 +1328,7 @@ template <class Rows, class RS, class RV>
 int enqueue_step(cmx_ctx* c, const Rows& R)
 {
     int rc;
-    if ((rc = launch_cells<RS, RV>(c))) return rc;
+    if ((rc = launch_cells<RS, RV>(c, c->split ? kModeSplit : kModeIci, verdict_buf(c), CMX_K_CELL))) return rc;
     if ((rc = launch_residual<Rows, RV>(c, R, 0))) return rc;
     if ((rc = launch_cg_start<Rows, RV>(c, R, 0))) return rc;
     if ((rc = launch_cg_iters<Rows, RV>(c, R, 0, 0, c->budget))) return rc;
@@ -1324,7 +1427,7 @@ int run_steps(cmx_ctx* c, const Rows& R)
         const int todo = std::min(c->h_clock->n_steps - k_start, c->batch);
         int rc = enqueue_steps<Rows, RS, RV>(c, R, todo);
         if (rc) return rc;
-        if ((rc = launch_finish<RV>(c, verdict_buf(c)))) return rc;
+        if ((rc = launch_finish<RS, RV>(c, verdict_buf(c)))) return rc;
         rc = sync_state(c);
         if (rc) return rc;
         int guard = 0;
@@ -1351,7 +1454,7 @@ int run_steps(cmx_ctx* c, const Rows& R)
                 if ((rc = launch_cg_iters<Rows, RV>(c, R, solve, from, chunk))) return rc;
                 verdict = pbuf(c, kIterPart + ((from + chunk) & 1));
             }
-            if ((rc = launch_finish<RV>(c, verdict))) return rc;
+            if ((rc = launch_finish<RS, RV>(c, verdict))) return rc;
             if ((rc = sync_state(c))) return rc;
         }
         const int got = c->h_clock->k - k_start;
@@ -1470,6 +1573,20 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->max_its = opts->ksp_max_its;
     c->refine = opts->refine_sweeps;
     c->use_graph = opts->use_graph;
+    c->split = opts->operator_splitting ? 1 : 0;
+    {
+        // TimeStepper(0, dt/2, dt_ode) step count (global/src/TimeStepper.cpp:43-121)
+        const double end = 0.5 * opts->dt_pde, eps = end > 1.0 ? DBL_EPSILON * end : DBL_EPSILON;
+        int k = 0;
+        double t = 0.0;
+        while (t < end) {
+            double nx = (k + 1) * opts->dt_ode;
+            if (end - nx <= eps) nx = end;
+            t = nx;
+            ++k;
+        }
+        c->n_half = k;
+    }
     if (opts->extrapolate < 0 || opts->extrapolate > 2) return fail(c, CMX_E_ARG, "extrapolate must be 0, 1 or 2");
     c->extrapolate = opts->extrapolate;
     c->n_param_planes = tissue->n_params;
@@ -1720,7 +1837,8 @@ int cmx_step(cmx_ctx* c, double t0, int32_t n_steps)
     CMX_HIP(c, hipStreamSynchronize(c->stream));
     const int rc = (c->precision == CMX_MIXED) ? steps_typed<float, float>(c) : steps_typed<double, double>(c);
     if (rc) return rc;
-    c->stepped = true;
+    // with operator splitting F carries no ionic current: Iion is evaluated at the state
+    c->stepped = !c->split;
     c->last_t_next = ck.t_end;
     return CMX_OK;
 }

@@ -115,6 +115,9 @@ public:
     std::string GetKSPPreconditioner() const { return mPc; }
     void SetUseMassLumping(bool on = true) { mLumping = on; }
     bool GetUseMassLumping() const { return mLumping; }
+    // HeartConfig.cpp:2874-2882 (MonodomainProblem.cpp:65-69 picks the splitting solver)
+    void SetUseReactionDiffusionOperatorSplitting(bool on = true) { mSplitting = on; }
+    bool GetUseReactionDiffusionOperatorSplitting() const { return mSplitting; }
 
     // MI355X-specific knobs (no reference counterpart).
     void SetGpuPrecision(int precision) { mPrecision = precision; }
@@ -136,6 +139,7 @@ private:
     double mAtol = 2e-4, mRtol = 1e-6;
     std::string mPc = "bjacobi";
     bool mLumping = false;
+    bool mSplitting = false;
     int mPrecision = CMX_FP64;
     int mRefine = 1;
     int mMaxIts = 1000;
@@ -351,6 +355,7 @@ public:
         o.mass_lumping = cfg->GetUseMassLumping() ? 1 : 0;
         o.use_graph = 1;
         o.extrapolate = 0;
+        o.operator_splitting = cfg->GetUseReactionDiffusionOperatorSplitting() ? 1 : 0;
         if (mpCtx) { cmx_destroy(mpCtx); mpCtx = nullptr; }
         Check(cmx_create(&mesh, &tissue, &o, &mpCtx), nullptr);
         mCurrentTime = 0.0;

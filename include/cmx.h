@@ -93,6 +93,10 @@ typedef struct {
     int32_t extrapolate;       /* CG initial guess instead of the reference's V^n (same
                                   stopping test): 1 = V^n + x_{n-1}, 2 = V^n + 2 x_{n-1} - x_{n-2},
                                   with x_m = V^{m+1} - V^m the step corrections */
+    int32_t operator_splitting; /* HeartConfig::SetUseReactionDiffusionOperatorSplitting:
+                                  Strang splitting, cells over half steps with V updated and a
+                                  diffusion-only solve in between
+                                  (OperatorSplittingMonodomainSolver.cpp:40-137) */
 } cmx_opts;
 
 typedef struct {

@@ -573,10 +573,76 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
     orc_mono_stats st{};
     st.n_nodes = n;
     st.nnz = nnz;
+    // Operator splitting (heart/src/solver/electrics/monodomain/
+    // OperatorSplittingMonodomainSolver.cpp:100-116): SolveCellSystems with
+    // updateVoltage over [t0, t1], every node's full state (V included) stepped
+    // by SolveAndUpdateState (heart/src/tissue/AbstractCardiacTissue.cpp:531-536).
+    std::vector<double> Vhalf(n);
+    auto cells_with_voltage = [&](double t0, double t1) {
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            double y[64], p[64];
+            for (int s = 0; s < S; ++s) y[s] = Y[(size_t)s * n + i];
+            for (int q = 0; q < P; ++q)
+                p[q] = (cfg->params && q < cfg->n_param) ? cfg->params[(size_t)q * n + i] : pdef[q];
+            y[0] = V[i];
+            Stepper ode(t0, t1, cfg->dt_ode);
+            while (!ode.at_end()) {
+                const double ts = ode.time;
+                const double sa = stimulated[i] ? stim.at(ts) / am : 0.0;
+                cell_substep(model, cfg->rush_larsen != 0, p, cm, y, ts, cfg->rush_larsen ? cfg->dt_ode : ode.step_size(), sa, true);
+                ode.advance();
+            }
+            V[i] = y[0];
+            for (int s = 1; s < S; ++s) Y[(size_t)s * n + i] = y[s];
+        }
+    };
     Stepper pde(0.0, cfg->t_end, dt);
     int step = 0;
     while (!pde.at_end()) {
         const double t = pde.time, t_next = pde.next;
+        if (cfg->split) {
+            const double th = t + 0.5 * (t_next - t);
+            Vprev = V;
+            auto c0 = std::chrono::steady_clock::now();
+            cells_with_voltage(t, th);
+            Vhalf = V;
+            st.t_ode += seconds_since(c0);
+            // diffusion only: b = M (Am Cm V / dt), initial guess V
+            c0 = std::chrono::steady_clock::now();
+            const double c = am * cm / (t_next - t);
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < n; ++i) {
+                double s = 0;
+                for (int k = rp[i]; k < rp[i + 1]; ++k) s += M[k] * (c * V[col[k]]);
+                b[i] = s;
+            }
+            st.t_rhs += seconds_since(c0);
+            c0 = std::chrono::steady_clock::now();
+            const int its = orc_pcg(n, rp.data(), col.data(), A.data(), b.data(), V.data(),
+                                    cfg->ksp_rtol, cfg->ksp_atol, cfg->max_its);
+            st.t_ksp += seconds_since(c0);
+            if (its < 0) return -2;
+            st.cg_iterations += its;
+            c0 = std::chrono::steady_clock::now();
+            cells_with_voltage(th, t_next);
+            st.t_ode += seconds_since(c0);
+            pde.advance();
+            if (cfg->modifier_out && (step + 1) % std::max(cfg->modifier_every, 1) == 0) modifier(t_next);
+            if (act_out) {
+                for (int i = 0; i < n; ++i) {
+                    if (act_out[i] >= 0) continue;
+                    if (Vprev[i] < act_threshold && Vhalf[i] >= act_threshold)
+                        act_out[i] = t + (th - t) * (act_threshold - Vprev[i]) / (Vhalf[i] - Vprev[i]);
+                    else if (Vhalf[i] < act_threshold && V[i] >= act_threshold)
+                        act_out[i] = th + (t_next - th) * (act_threshold - Vhalf[i]) / (V[i] - Vhalf[i]);
+                }
+            }
+            if (v_trace)
+                for (int q = 0; q < n_trace; ++q) v_trace[(size_t)step * n_trace + q] = V[trace_nodes[q]];
+            ++step;
+            continue;
+        }
         // SolveCellSystems(V, t, t_next): every node integrated with V clamped
         // over [t, t_next] in ODE substeps, then Iion and Istim(t_next) cached.
         auto c0 = std::chrono::steady_clock::now();

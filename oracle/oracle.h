@@ -97,6 +97,9 @@ typedef struct {
     double* modifier_out;     /* [4][n] ActivationOutputModifier times, or NULL */
     double modifier_threshold;
     int modifier_every;       /* process the voltage every this many PDE steps (and at t = 0) */
+    int split;                /* HeartConfig::SetUseReactionDiffusionOperatorSplitting: Strang
+                                 splitting, cells (V updated) over each half step around a
+                                 diffusion-only solve */
 } orc_mono_cfg;
 
 typedef struct {
@@ -109,7 +112,9 @@ typedef struct {
 /* Full monodomain run from the model's default state.  Any output may be NULL.
  * V_out[n]; state_out[S*n] (state-major planes, V plane included);
  * act_out[n] first upward crossing of act_threshold (ms, linear interpolation,
- * -1 if none), checked after every PDE step; v_trace[n_trace_nodes * steps]
+ * -1 if none), checked after every PDE step (with splitting, along the
+ * polyline through V at t, after the first half step at the midpoint, and at
+ * t_next); v_trace[n_trace_nodes * steps]
  * records V at trace_nodes after each step. */
 int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
                  double* act_out, double act_threshold,

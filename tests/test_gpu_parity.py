@@ -20,14 +20,15 @@ MIXED_ACT_MS = 0.1
 
 
 def make(dim, ex, ey, ez, model, integrator, stim_mag, stim_dur, stim_xlim, precision=0, sigma=(1.75, 1.75, 1.75),
-         rtol=1e-12, atol=0.0, fmt=0, refine=0, dt=0.01, dt_ode=None, use_graph=False, extrapolate=False):
+         rtol=1e-12, atol=0.0, fmt=0, refine=0, dt=0.01, dt_ode=None, use_graph=False, extrapolate=False,
+         split=False):
     from chaste_amd import cmx
     coords, elems = cmx.slab_mesh(dim, ex, ey, ez, 0.01)
     stim_index = np.where(coords[:, 0] < stim_xlim, 0, -1).astype(np.int32)
     return cmx.Monodomain(coords, elems, np.array(sigma[:dim]), model, integrator,
                           stimuli=[(stim_mag, 0.0, stim_dur)], stim_index=stim_index, precision=precision,
                           ksp_rtol=rtol, ksp_atol=atol, fmt=fmt, refine_sweeps=refine, dt_pde=dt, dt_ode=dt if dt_ode is None else dt_ode,
-                          use_graph=use_graph, extrapolate=extrapolate)
+                          use_graph=use_graph, extrapolate=extrapolate, operator_splitting=split)
 
 
 def first_crossing(trace, dt, thr=0.0):
@@ -226,6 +227,84 @@ def test_activation_times(oracle, every):
     # times are printing steps; a node whose V sits within rounding of the
     # threshold at a step may flip by one printing step
     assert (d[:] <= 1e-9).mean() > 0.99 and d.max() <= 0.01 * every + 1e-9
+
+
+@pytest.mark.parametrize('case', ['lr91_fe_fp64', 'tt06_rl_fp64', 'lr91_fe_mixed', 'tt06_rl_mixed'])
+def test_operator_splitting(oracle, case):
+    """HeartConfig::SetUseReactionDiffusionOperatorSplitting: half step of the
+    cells with V updated, diffusion-only solve, second half step, against the
+    oracle's split run (1e-9 relative in FP64, 0.5 mV and 0.1 ms mixed), with
+    activation times (interpolated crossing along the half-step polyline and
+    the printing-step modifier).  The Rush-Larsen case keeps dt_ode = dt: each
+    half step advances a full dt_ode, as the reference's cells do."""
+    from chaste_amd import cmx
+    print('case operator splitting', case, flush=True)
+    mixed = case.endswith('mixed')
+    if case.startswith('lr91'):
+        dims, sigma, model, integ, stim, dur, xlim, dt_ode = (2, 30, 30, 0), (1.75, 1.75, 1.75), cmx.LR91, 0, \
+            -400000.0, 0.5, 0.02, 0.005
+        omodel = oracle.LR91_CELLML
+    else:
+        dims, sigma, model, integ, stim, dur, xlim, dt_ode = (3, 20, 7, 4), (1.75, 0.19, 0.19), cmx.TT06_EPI, 1, \
+            -100000.0, 1.0, 0.03, 0.01
+        omodel = oracle.TT06_EPI
+    steps = 500
+    m = make(*dims, model, integ, stim, dur, xlim, precision=1 if mixed else 0, sigma=sigma,
+             rtol=1e-8 if mixed else 1e-12, refine=1 if mixed else 0, dt_ode=dt_ode, use_graph=True,
+             extrapolate=2 if case.startswith('tt06') else False, split=True)
+    m.set_activation(-30.0, 1, 0.0)
+    m.step(0.0, steps)
+    ref = oracle.mono_run(*dims, model=omodel, rush_larsen=bool(integ), sigma=sigma, stim_mag=stim, stim_dur=dur,
+                          stim_xlim=xlim, t_end=steps * 0.01, dt_ode=dt_ode, ksp_rtol=1e-12, want_state=True,
+                          act_threshold=-30.0, modifier_threshold=-30.0, modifier_every=1, split=True)
+    V = m.voltage()
+    err = np.abs(V - ref['V']).max()
+    act = m.activation()
+    exp = np.vstack([ref['modifier'], ref['act'][None]])
+    hit = exp[4] >= 0
+    print('splitting max |dV|', err, 'rel', err / np.abs(ref['V']).max(), 'its/step',
+          m.stats()['cg_iterations'] / steps, 'activated', hit.sum(), 'of', hit.size)
+    assert hit.sum() > hit.size // 4
+    assert np.array_equal(act[4] >= 0, hit)
+    dact = np.abs(act[4] - exp[4])[hit].max()
+    print('splitting activation max diff', dact)
+    if mixed:
+        assert err <= MIXED_MV
+        assert dact <= MIXED_ACT_MS
+    else:
+        assert err / np.abs(ref['V']).max() <= FP64_REL
+        assert dact <= 1e-6
+        S = m.state()
+        serr = np.abs(S - ref['state']).max(axis=1) / np.maximum(np.abs(ref['state']).max(axis=1), 1e-30)
+        assert serr.max() <= 1e-8
+        d = np.abs(act[:4] - exp[:4])[exp[:4] >= 0]
+        assert np.array_equal(act[:4] < 0, exp[:4] < 0)
+        assert (d <= 1e-9).mean() > 0.99 and d.max() <= 0.01 + 1e-9
+
+
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_operator_splitting_graph_replay(precision):
+    """Split steps replayed from graphs equal plain launches bitwise across
+    calls of uneven length (the batch's last half step of the cells runs in
+    the finishing launch, the next call's first kernel resumes from it)."""
+    from chaste_amd import cmx
+    print('case splitting graph replay', precision, flush=True)
+    kw = dict(precision=precision, sigma=(1.75, 0.19, 0.19), rtol=1e-10, refine=1, extrapolate=2, split=True)
+    a = make(3, 24, 8, 5, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, use_graph=True, **kw)
+    b = make(3, 24, 8, 5, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, use_graph=False, **kw)
+    c = make(3, 24, 8, 5, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03, use_graph=False, **kw)
+    t = 0.0
+    for n in (40, 17, 64, 9):
+        a.step(t, n)
+        b.step(t, n)
+        t += n * 0.01
+        assert np.array_equal(a.voltage(), b.voltage())
+    c.step(0.0, 130)
+    assert np.array_equal(a.state(), b.state())
+    assert a.stats()['cg_iterations'] == b.stats()['cg_iterations']
+    # one call of 130 steps against four calls: same steps, same answer
+    tol = 0.0 if precision == 0 else 1e-3
+    assert np.abs(c.voltage() - a.voltage()).max() <= max(tol, 1e-9 * np.abs(a.voltage()).max())
 
 
 def test_ell_matches_stencil():
