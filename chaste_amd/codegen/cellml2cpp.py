@@ -25,6 +25,12 @@ oracle includes the same header).  It provides
 * ``iionic<R>(y, V, p, cm)``: total ionic current in uA/cm^2, the quantity
   ``AbstractCardiacCellInterface::GetIIonic`` returns
   (``heart/src/odes/AbstractCardiacCellInterface.hpp:227-256``).
+* ``be_linear``, ``be_residual``, ``be_jacobian``, ``be_step``: the
+  backward-Euler form of the reference's ``*BackwardEuler`` cell classes
+  (``heart/src/odes/AbstractBackwardEulerCardiacCell.hpp:116-128, 300-330``):
+  the linear gates in closed form, the remaining states by Newton's method on
+  the implicit step with an analytic Jacobian (symbolic differentiation of
+  the model equations, as the reference's generator does with sympy).
 
 Unit conventions follow the reference's tissue coupling: the stimulus passed
 in is an *area* stimulus in uA/cm^2 (``GetIntracellularAreaStimulus``,
@@ -608,11 +614,12 @@ class Generator:
         stim_units = m.vars[m.stim]['units']
         per_cap = 'per_picoF' in stim_units or 'A_per_F' in stim_units or 'uF' in stim_units and 'cm' not in stim_units
 
-        def body_lines(roots, indent='    '):
+        def body_lines(roots, indent='    ', printer=None):
+            printer = printer or pr
             lines = []
             for k in self.closure(roots):
                 c, rhs = m.defs[k]
-                e = pr.expr(rhs, c)
+                e = printer.expr(rhs, c)
                 for vc, dv in self.singular_points(k):
                     # the lambda's V shadows the clamped voltage
                     e = 'cmx_remove_singularity(V, R(%r), R(%r), [&](const R V) { return %s; })' % (vc, dv, e)
@@ -770,8 +777,228 @@ class Generator:
             w('    return -dv * %s;' % (cident(capname) if capname else 'R(1)'))
         w('}')
         w('')
+        self._backward_euler(w, rl, names, constants, state_ix, per_cap, body_lines)
         w('}}  // namespace cmx::%s' % ns)
         return '\n'.join(out) + '\n', rl
+
+    # --- Backward Euler (the reference's BackwardEuler cell classes) --------
+    def _backward_euler(self, w, rl, names, constants, state_ix, per_cap, body_lines):
+        """be_linear / be_residual / be_jacobian.  States whose derivative is
+        affine in themselves with V-only coefficients (the Rush-Larsen gates)
+        are 'linear' and updated in closed form, y <- (y + a dt) / (1 - b dt);
+        the others are solved together by Newton's method on
+        g - y_old - dt f(g) = 0 with the analytic Jacobian
+        (heart/src/odes/AbstractBackwardEulerCardiacCell.hpp:116-128,
+        heart/src/odes/CardiacNewtonSolver.hpp:66-160)."""
+        m = self.m
+        nonlin = [k for k in m.states[1:] if k not in rl]
+        lin = [k for k in m.states[1:] if k in rl]
+        N = len(nonlin)
+        nl_ix = {k: i for i, k in enumerate(nonlin)}
+        gnames = dict(names)
+        for k in nonlin:
+            gnames[k] = 'g[%d]' % nl_ix[k]
+        gpr = Printer(m, gnames, self.is_const)
+        w('// Backward Euler: the states solved by Newton (all but the linear gates).')
+        w('constexpr int kNumNonlinear = %d;' % N)
+        w('static constexpr int kNonlinear[%d] = {%s};' % (max(N, 1), ', '.join(str(state_ix[k]) for k in nonlin) or '0'))
+        w('')
+        w('// Linear states: y <- (y + a dt) / (1 - b dt) for dy/dt = a + b y, a and b')
+        w('// functions of V (ComputeOneStepExceptVoltage, closed-form part).')
+        w('template <class R>')
+        w('CMX_HD inline void be_linear(R* y, R V, const R* p, R cm, R dt)')
+        w('{')
+        w('    CMX_MATH_USING')
+        w('    (void)p; (void)cm; (void)V;')
+        roots = set()
+        for k in lin:
+            roots |= self.deps_rhs(k)
+        for k in self.used_constants(roots, constants):
+            w('    const R %s = R(%r);' % (cident(k), m.init(k)))
+        for line in body_lines(sorted(roots)):
+            w(line)
+        for k in lin:
+            a, b = rl[k]
+            w('    y[%d] = (y[%d] + (%s) * dt) / (R(1) - (%s) * dt);' % (state_ix[k], state_ix[k], a or 'R(0)', b))
+        w('}')
+        w('')
+        roots = set()
+        for k in nonlin:
+            roots |= self.deps_rhs(k)
+        consts = self.used_constants(roots, constants)
+        w('// Newton residual of the nonlinear states at guess g (y holds the other')
+        w('// states, linear ones already advanced): g - y_old - dt f(g).')
+        w('template <class R>')
+        w('CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const R* p, R cm, R dt, R* res)')
+        w('{')
+        w('    CMX_MATH_USING')
+        w('    (void)y; (void)p; (void)cm; (void)i_stim_area;')
+        for k in consts:
+            w('    const R %s = R(%r);' % (cident(k), m.init(k)))
+        w('    const R i_stim = %s;' % ('i_stim_area / cm' if per_cap else 'i_stim_area'))
+        w('    (void)i_stim;')
+        for line in body_lines(sorted(roots), printer=gpr):
+            w(line)
+        for k in nonlin:
+            c, rhs = m.odes[k]
+            i = nl_ix[k]
+            w('    res[%d] = g[%d] - y[%d] - dt * %s;' % (i, i, state_ix[k], gpr.expr(rhs, c)))
+        w('}')
+        w('')
+        w('// Jacobian of be_residual in g: J[i*N + j] = delta_ij - dt d f_i / d g_j.')
+        w('template <class R>')
+        w('CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const R* p, R cm, R dt, R* J)')
+        w('{')
+        w('    CMX_MATH_USING')
+        w('    (void)y; (void)p; (void)cm; (void)i_stim_area;')
+        for k in consts:
+            w('    const R %s = R(%r);' % (cident(k), m.init(k)))
+        w('    const R i_stim = %s;' % ('i_stim_area / cm' if per_cap else 'i_stim_area'))
+        w('    (void)i_stim;')
+        for line in body_lines(sorted(roots), printer=gpr):
+            w(line)
+        nlset = set(nonlin)
+        dnames = {}
+        for k in self.closure(sorted(roots)):
+            c, rhs = m.defs[k]
+            for j in sorted(self.state_deps(k) & nlset, key=lambda q: nl_ix[q]):
+                d = self.derivative(rhs, c, j, gpr, dnames)
+                if d is None:
+                    continue
+                for vc, dv in self.singular_points(k):
+                    d = 'cmx_remove_singularity(V, R(%r), R(%r), [&](const R V) { return %s; })' % (vc, dv, d)
+                if k in self.scales:
+                    d = '(%s) * p[%d]' % (d, self.param_index[k])
+                nm = 'd_%s__d%d' % (cident(k), nl_ix[j])
+                dnames[(k, j)] = nm
+                w('    const R %s = %s;' % (nm, d))
+        for k in nonlin:
+            c, rhs = m.odes[k]
+            i = nl_ix[k]
+            for j in nonlin:
+                jj = nl_ix[j]
+                d = self.derivative(rhs, c, j, gpr, dnames)
+                diag = 'R(1)' if i == jj else 'R(0)'
+                if d is None:
+                    w('    J[%d] = %s;' % (i * N + jj, diag))
+                else:
+                    w('    J[%d] = %s - dt * %s;' % (i * N + jj, diag, d))
+        w('}')
+        w('')
+        w('// One backward-Euler step of every state but V at the given V')
+        w('// (ComputeOneStepExceptVoltage): linear states in closed form, then the')
+        w('// nonlinear ones by Newton from their current values.  Returns the Newton')
+        w('// iteration count, -1 if it did not converge.')
+        w('template <class R>')
+        w('CMX_HD inline int be_step(R* y, R V, R i_stim_area, const R* p, R cm, R dt)')
+        w('{')
+        w('    be_linear<R>(y, V, p, cm, dt);')
+        w('    R g[kNumNonlinear];')
+        w('CMX_UNROLL')
+        w('    for (int i = 0; i < kNumNonlinear; ++i) g[i] = y[kNonlinear[i]];')
+        w('    const int its = cmx_newton<kNumNonlinear, R>(')
+        w('        g, [&](const R* x, R* r) { be_residual<R>(y, x, V, i_stim_area, p, cm, dt, r); },')
+        w('        [&](const R* x, R* J) { be_jacobian<R>(y, x, V, i_stim_area, p, cm, dt, J); });')
+        w('CMX_UNROLL')
+        w('    for (int i = 0; i < kNumNonlinear; ++i) y[kNonlinear[i]] = g[i];')
+        w('    return its;')
+        w('}')
+        w('')
+
+    def derivative(self, el, cname, j, pr, dnames):
+        """C++ source of d(el)/d(state j), or None when it is identically
+        zero.  Algebraic variables that depend on j refer to their own
+        derivative variables (dnames), emitted in evaluation order."""
+        m = self.m
+        t = _tag(el)
+        if t == 'ci':
+            k = m.canon(cname, el.text.strip())
+            if k == j:
+                return 'R(1)'
+            if k in m.defs and k != m.stim and j in self.state_deps(k):
+                return dnames.get((k, j))
+            return None
+        if t in ('cn', 'pi', 'exponentiale', 'true', 'false'):
+            return None
+        if t == 'piecewise':
+            pieces, other, any_d = [], 'R(0)', False
+            for pc in el:
+                if _tag(pc) == 'piece':
+                    val, cond = list(pc)
+                    d = self.derivative(val, cname, j, pr, dnames)
+                    any_d |= d is not None
+                    pieces.append((pr.expr(cond, cname), d or 'R(0)'))
+                elif _tag(pc) == 'otherwise':
+                    d = self.derivative(list(pc)[0], cname, j, pr, dnames)
+                    any_d |= d is not None
+                    other = d or 'R(0)'
+            if not any_d:
+                return None
+            out = other
+            for cond, d in reversed(pieces):
+                out = '((%s) ? (%s) : (%s))' % (cond, d, out)
+            return out
+        if t != 'apply':
+            raise NotImplementedError(t)
+        ch = list(el)
+        op = _tag(ch[0])
+        args = [a for a in ch[1:] if _tag(a) not in ('degree', 'logbase', 'bvar')]
+        ds = [self.derivative(a, cname, j, pr, dnames) for a in args]
+        if all(d is None for d in ds):
+            return None
+        ex = [pr.expr(a, cname) for a in args]
+        if op == 'plus':
+            return '(' + ' + '.join(d for d in ds if d is not None) + ')'
+        if op == 'minus':
+            if len(args) == 1:
+                return '(-%s)' % ds[0]
+            return _sub(ds[0], ds[1])
+        if op == 'times':
+            terms = []
+            for i, d in enumerate(ds):
+                if d is None:
+                    continue
+                terms.append('(' + ' * '.join([d] + [ex[l] for l in range(len(args)) if l != i]) + ')')
+            return '(' + ' + '.join(terms) + ')'
+        if op == 'divide':
+            da, db = ds
+            if db is None:
+                return 'cmx_div(%s, %s)' % (da, ex[1])
+            num = '(-(%s * %s))' % (ex[0], db) if da is None else '(%s * %s - %s * %s)' % (da, ex[1], ex[0], db)
+            return 'cmx_div(%s, (%s * %s))' % (num, ex[1], ex[1])
+        if op == 'power':
+            da, db = ds
+            if db is None:
+                b = args[1]
+                if _tag(b) == 'cn':
+                    v = float(Printer.number_value(b))
+                    if v == 1.0:
+                        return da
+                    if v == int(v) and 2 <= v <= 5:
+                        return '(R(%r) * cmx_ipow<%d>(%s) * %s)' % (v, int(v) - 1, ex[0], da)
+                return '(%s * pow(%s, %s - R(1)) * %s)' % (ex[1], ex[0], ex[1], da)
+            base = 'pow(%s, %s)' % (ex[0], ex[1])
+            inner = '%s * cmx_log(%s)' % (db, ex[0])
+            if da is not None:
+                inner += ' + cmx_div(%s * %s, %s)' % (ex[1], da, ex[0])
+            return '(%s * (%s))' % (base, inner)
+        if op == 'root':
+            deg = el.find(MML + 'degree')
+            if deg is None:
+                return 'cmx_div(%s, (R(2) * sqrt(%s)))' % (ds[0], ex[0])
+            dg = pr.expr(list(deg)[0], cname)
+            return '((R(1) / %s) * pow(%s, R(1) / %s - R(1)) * %s)' % (dg, ex[0], dg, ds[0])
+        if op == 'exp':
+            return '(cmx_exp(%s) * %s)' % (ex[0], ds[0])
+        if op == 'ln':
+            return 'cmx_div(%s, %s)' % (ds[0], ex[0])
+        if op == 'log':
+            return 'cmx_div(%s, (%s * R(2.302585092994045684)))' % (ds[0], ex[0])
+        if op == 'abs':
+            return '((%s < R(0)) ? (-%s) : (%s))' % (ex[0], ds[0], ds[0])
+        if op in ('floor', 'ceiling'):
+            return None
+        raise NotImplementedError(op)
 
     def _find_model_capacitance(self):
         m = self.m

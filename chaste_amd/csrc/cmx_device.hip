@@ -63,6 +63,11 @@ struct ModelLr91 {
     static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return lr91::iionic<R>(y, V, p, cm); }
     template <class R>
     static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return lr91::dvdt<R>(y, V, st, p, cm); }
+    template <class R>
+    static __device__ __forceinline__ int be_step(R* y, R V, R st, const R* p, R cm, R dt)
+    {
+        return lr91::be_step<R>(y, V, st, p, cm, dt);
+    }
     static constexpr bool gate(int i) { return lr91::kIsGate[i] != 0; }
 };
 
@@ -78,6 +83,11 @@ struct ModelTt06 {
     static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return tt06_epi::iionic<R>(y, V, p, cm); }
     template <class R>
     static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return tt06_epi::dvdt<R>(y, V, st, p, cm); }
+    template <class R>
+    static __device__ __forceinline__ int be_step(R* y, R V, R st, const R* p, R cm, R dt)
+    {
+        return tt06_epi::be_step<R>(y, V, st, p, cm, dt);
+    }
     static constexpr bool gate(int i) { return tt06_epi::kIsGate[i] != 0; }
 };
 
@@ -195,24 +205,58 @@ struct CellArgs {
 // one, V updated), and the splitting step's last half only (end of a batch).
 enum CellMode { kModeIci = 0, kModeSplit = 1, kModeSplitFinal = 2 };
 
+// One backward-Euler substep of every state but V at voltage V
+// (ComputeOneStepExceptVoltage of the reference's *BackwardEuler cells,
+// heart/src/odes/AbstractBackwardEulerCardiacCell.hpp:180-200), computed in
+// FP64 whatever the state precision: Newton's update test (max-norm <= 1e-6)
+// is below the FP32 spacing of the larger concentrations.  Returns false
+// when Newton did not converge.
+template <class Model, class RS>
+__device__ __forceinline__ bool be_substep(RS* y, const RS* p, double V, double st, double cm, double h)
+{
+    constexpr int NS = Model::NS;
+    constexpr int NP = Model::NP > 0 ? Model::NP : 1;
+    double yd[NS], pd[NP];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) yd[s] = (double)y[s];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) pd[q] = (double)p[q];
+    yd[0] = V;
+    const int its = Model::template be_step<double>(yd, V, st, pd, cm, h);
+#pragma unroll
+    for (int s = 1; s < NS; ++s) y[s] = (RS)yd[s];
+    return its >= 0;
+}
+
 // ODE substeps over [t0, t1] with the voltage updated too, as
 // SolveAndUpdateState does (heart/src/odes/AbstractCardiacCell.cpp:70-73,
 // ode/src/solver/AbstractOneStepIvpOdeSolver.cpp:93-153 for forward Euler,
 // heart/src/odes/AbstractRushLarsenCardiacCell.cpp:130-153 for Rush-Larsen:
-// every rate evaluated at the substep's start, V by forward Euler; Rush-Larsen
-// cells always step by dt_ode).  vd is the FP64 voltage.
-template <class Model, class RS, class RV, bool RL>
-__device__ __forceinline__ void integrate_with_voltage(const CellArgs<RS, RV>& a, RS* y, const RS* p, int sid,
+// every rate evaluated at the substep's start, V by forward Euler;
+// heart/src/odes/AbstractBackwardEulerCardiacCell.hpp:300-330 for backward
+// Euler: V by forward Euler first, the other states then implicit at the new
+// V; Rush-Larsen and backward-Euler cells always step by dt_ode).  vd is the
+// FP64 voltage.
+template <class Model, class RS, class RV, int IG>
+__device__ __forceinline__ bool integrate_with_voltage(const CellArgs<RS, RV>& a, RS* y, const RS* p, int sid,
                                                        double& vd, double t0, double t1)
 {
     constexpr int NS = Model::NS;
+    constexpr bool RL = IG == CMX_RUSH_LARSEN;
     const RS cm = (RS)a.cm;
+    bool ok = true;
     for (int ks = 0; ks < a.n_half; ++ks) {
         const double ts = t0 + ks * a.dt_ode;
-        const double h = (!RL && ks + 1 == a.n_half) ? t1 - ts : a.dt_ode;
-        const RS st = sid ? (RS)(stimulus_at(a.stim[sid - 1], ts) / a.am) : RS(0);
+        const double h = (IG == CMX_FORWARD_EULER && ks + 1 == a.n_half) ? t1 - ts : a.dt_ode;
+        const double std_ = sid ? stimulus_at(a.stim[sid - 1], ts) / a.am : 0.0;
+        const RS st = (RS)std_;
         const RS V = (RS)vd;
         y[0] = V;
+        if constexpr (IG == CMX_BACKWARD_EULER) {
+            vd += h * (double)Model::template dvdt<RS>(y, V, st, p, cm);
+            ok &= be_substep<Model, RS>(y, p, vd, std_, a.cm, h);
+            continue;
+        }
         RS dy[NS], gi[NS], gk[NS];
         Model::template eval<RS, RL>(y, V, st, p, cm, dy, gi, gk);
         const RS dv = Model::template dvdt<RS>(y, V, st, p, cm);
@@ -224,6 +268,7 @@ __device__ __forceinline__ void integrate_with_voltage(const CellArgs<RS, RV>& a
         }
         vd += h * (double)dv;
     }
+    return ok;
 }
 
 // One thread per node; the state lives in registers for all substeps.  When
@@ -233,9 +278,10 @@ __device__ __forceinline__ void integrate_with_voltage(const CellArgs<RS, RV>& a
 // (OperatorSplittingMonodomainSolver, heart/src/solver/electrics/monodomain/
 // OperatorSplittingMonodomainSolver.cpp:40-137) that voltage then runs the
 // previous step's second half-step of the cells.
-template <class Model, class RS, class RV, bool RL, int MODE>
+template <class Model, class RS, class RV, int IG, int MODE>
 __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
 {
+    constexpr bool RL = IG == CMX_RUSH_LARSEN;
     constexpr int NS = Model::NS;
     constexpr int NP = Model::NP > 0 ? Model::NP : 1;
     const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -284,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
                 const double tp = step_time(clk, clk.k), tq = step_next_time(clk, clk.k);
                 const double th = tp + 0.5 * (tq - tp);
                 t0 = th;
-                integrate_with_voltage<Model, RS, RV, RL>(a, y, p, sid, vn, th, tq);
+                if (!integrate_with_voltage<Model, RS, RV, IG>(a, y, p, sid, vn, th, tq)) a.clock->nonfinite = 2;
             }
         }
         if (live && a.s.act) track_activation(a.s, a.n, i, clk, clk.k, v0, vn, t0);
@@ -310,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         // first half-step of this step's cells, V updated; the diffusion
         // solve that follows has no reaction term
         const double v_start = vn, th = t + 0.5 * (t_next - t);
-        integrate_with_voltage<Model, RS, RV, RL>(a, y, p, sid, vn, t, th);
+        if (!integrate_with_voltage<Model, RS, RV, IG>(a, y, p, sid, vn, t, th)) a.clock->nonfinite = 2;
         if (a.s.act) record_crossing(a.s, a.n, i, v_start, vn, t, th);
         a.V[i] = vn;
         f = RV(0);
@@ -321,12 +367,19 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         const RS cm = (RS)a.cm;
         // TimeStepper(t, t_next, dt_ode): substep k starts at t + k*dt_ode
         // (global/src/TimeStepper.cpp:106-121); forward Euler takes the remaining
-        // interval on the last substep, Rush-Larsen always dt_ode
-        // (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:106-128).
+        // interval on the last substep, Rush-Larsen and backward Euler always
+        // dt_ode (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:106-128,
+        // heart/src/odes/AbstractBackwardEulerCardiacCell.hpp:204-230).
+        bool ok = true;
         for (int ks = 0; ks < a.n_sub; ++ks) {
             const double ts = t + ks * a.dt_ode;
-            const double h = (!RL && ks + 1 == a.n_sub) ? t_next - ts : a.dt_ode;
-            const RS st = sid ? (RS)(stimulus_at(a.stim[sid - 1], ts) / a.am) : RS(0);
+            const double h = (IG == CMX_FORWARD_EULER && ks + 1 == a.n_sub) ? t_next - ts : a.dt_ode;
+            const double std_ = sid ? stimulus_at(a.stim[sid - 1], ts) / a.am : 0.0;
+            const RS st = (RS)std_;
+            if constexpr (IG == CMX_BACKWARD_EULER) {
+                ok &= be_substep<Model, RS>(y, p, (double)V, std_, a.cm, h);
+                continue;
+            }
             RS dy[NS], gi[NS], gk[NS];
             Model::template eval<RS, RL>(y, V, st, p, cm, dy, gi, gk);
             const RS hh = (RS)h;
@@ -336,6 +389,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
                 else y[s] += hh * dy[s];
             }
         }
+        if (!ok) a.clock->nonfinite = 2;
         const RS iion = Model::template iion<RS>(y, V, p, cm);
         const double istim = sid ? stimulus_at(a.stim[sid - 1], t_next) : 0.0;
         f = (RV)(-a.am * (double)iion - istim);
@@ -1188,12 +1242,20 @@ SettleArgs<RV> settle_args(cmx_ctx* c, const double* verdict)
 }
 
 // ----- typed launches -------------------------------------------------------
-template <class Model, class RS, class RV, bool RL>
+template <class Model, class RS, class RV, int IG>
 void launch_cell_mode(cmx_ctx* c, int mode, int kind, int G, const CellArgs<RS, RV>& a)
 {
-    if (mode == kModeIci) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, RL, kModeIci>), G, kBlock, a);
-    else if (mode == kModeSplit) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, RL, kModeSplit>), G, kBlock, a);
-    else CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, RL, kModeSplitFinal>), G, kBlock, a);
+    if (mode == kModeIci) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeIci>), G, kBlock, a);
+    else if (mode == kModeSplit) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeSplit>), G, kBlock, a);
+    else CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeSplitFinal>), G, kBlock, a);
+}
+
+template <class Model, class RS, class RV>
+void launch_cell_integrator(cmx_ctx* c, int mode, int kind, int G, const CellArgs<RS, RV>& a)
+{
+    if (c->integrator == CMX_RUSH_LARSEN) launch_cell_mode<Model, RS, RV, CMX_RUSH_LARSEN>(c, mode, kind, G, a);
+    else if (c->integrator == CMX_BACKWARD_EULER) launch_cell_mode<Model, RS, RV, CMX_BACKWARD_EULER>(c, mode, kind, G, a);
+    else launch_cell_mode<Model, RS, RV, CMX_FORWARD_EULER>(c, mode, kind, G, a);
 }
 
 // The cell kernel in one of its roles; verdict: the reduction that decides
@@ -1223,14 +1285,8 @@ int launch_cells(cmx_ctx* c, int mode, const double* verdict, int kind)
     a.Y = c->Y;
     a.G1 = (RV*)c->G1;
     const int G = grid_of(c->n);
-    const bool rl = c->integrator == CMX_RUSH_LARSEN;
-    if (c->model == CMX_MODEL_LR91) {
-        if (rl) launch_cell_mode<ModelLr91, RS, RV, true>(c, mode, kind, G, a);
-        else launch_cell_mode<ModelLr91, RS, RV, false>(c, mode, kind, G, a);
-    } else {
-        if (rl) launch_cell_mode<ModelTt06, RS, RV, true>(c, mode, kind, G, a);
-        else launch_cell_mode<ModelTt06, RS, RV, false>(c, mode, kind, G, a);
-    }
+    if (c->model == CMX_MODEL_LR91) launch_cell_integrator<ModelLr91, RS, RV>(c, mode, kind, G, a);
+    else launch_cell_integrator<ModelTt06, RS, RV>(c, mode, kind, G, a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
@@ -1477,6 +1533,8 @@ int run_steps(cmx_ctx* c, const Rows& R)
         c->budget = std::min(short0 ? std::max(mx0, c->budget) + 1 : mx0, c->max_its);
         c->refine_budget = std::min(mx1, c->max_its);
         c->steps += got;
+        if (c->h_clock->nonfinite == 2)
+            return fail(c, CMX_E_NONFINITE, "backward-Euler cell: Newton iteration did not converge in 16 steps");
         if (c->h_clock->nonfinite)
             return fail(c, CMX_E_NONFINITE, "non-finite transmembrane potential (cell ODE or solve blew up)");
     }
@@ -1542,7 +1600,8 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
 {
     if (!mesh || !tissue || !opts) return fail(c, CMX_E_ARG, "null mesh, tissue or opts");
     if (model_sizes(tissue->model, &c->S, &c->P)) return fail(c, CMX_E_ARG, "unknown cell model");
-    if (tissue->integrator != CMX_FORWARD_EULER && tissue->integrator != CMX_RUSH_LARSEN)
+    if (tissue->integrator != CMX_FORWARD_EULER && tissue->integrator != CMX_RUSH_LARSEN &&
+        tissue->integrator != CMX_BACKWARD_EULER)
         return fail(c, CMX_E_ARG, "unknown integrator");
     if (opts->precision != CMX_FP64 && opts->precision != CMX_MIXED) return fail(c, CMX_E_ARG, "unknown precision");
     if (!(opts->dt_pde > 0) || !(opts->dt_ode > 0) || opts->dt_ode > opts->dt_pde * (1 + 1e-12))

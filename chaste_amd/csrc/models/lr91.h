@@ -196,4 +196,85 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     return -dv * membrane__C;
 }
 
+// Backward Euler: the states solved by Newton (all but the linear gates).
+constexpr int kNumNonlinear = 1;
+static constexpr int kNonlinear[1] = {7};
+
+// Linear states: y <- (y + a dt) / (1 - b dt) for dy/dt = a + b y, a and b
+// functions of V (ComputeOneStepExceptVoltage, closed-form part).
+template <class R>
+CMX_HD inline void be_linear(R* y, R V, const R* p, R cm, R dt)
+{
+    CMX_MATH_USING
+    (void)p; (void)cm; (void)V;
+    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.135) * cmx_exp(((R(80.0) + V) * (R(1) / (-R(6.8))))))) : (R(0.0)));
+    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(3.56) * cmx_exp((R(0.079) * V))) + (R(310000.0) * cmx_exp((R(0.35) * V))))) : (cmx_div(R(1.0), (R(0.13) * (R(1.0) + cmx_exp(((V + R(10.66)) * (R(1) / (-R(11.1))))))))));
+    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(127140.0)) * cmx_exp((R(0.2444) * V))) - (R(3.474e-05) * cmx_exp(((-R(0.04391)) * V)))) * (V + R(37.78))), (R(1.0) + cmx_exp((R(0.311) * (V + R(79.23))))))) : (R(0.0)));
+    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.1212) * cmx_exp(((-R(0.01052)) * V))), (R(1.0) + cmx_exp(((-R(0.1378)) * (V + R(40.14))))))) : (cmx_div((R(0.3) * cmx_exp(((-R(2.535e-07)) * V))), (R(1.0) + cmx_exp(((-R(0.1)) * (V + R(32.0))))))));
+    const R fast_sodium_current_m_gate__alpha_m = cmx_remove_singularity(V, R(-47.12999999999975), R(9.999999999999947e-07), [&](const R V) { return cmx_div((R(0.32) * (V + R(47.13))), (-cmx_expm1(((-R(0.1)) * (V + R(47.13)))))); });
+    const R fast_sodium_current_m_gate__beta_m = (R(0.08) * cmx_exp(((-V) * (R(1) / R(11.0)))));
+    const R slow_inward_current_d_gate__alpha_d = cmx_div((R(0.095) * cmx_exp(((-R(0.01)) * (V - R(5.0))))), (R(1.0) + cmx_exp(((-R(0.072)) * (V - R(5.0))))));
+    const R slow_inward_current_d_gate__beta_d = cmx_div((R(0.07) * cmx_exp(((-R(0.017)) * (V + R(44.0))))), (R(1.0) + cmx_exp((R(0.05) * (V + R(44.0))))));
+    const R slow_inward_current_f_gate__alpha_f = cmx_div((R(0.012) * cmx_exp(((-R(0.008)) * (V + R(28.0))))), (R(1.0) + cmx_exp((R(0.15) * (V + R(28.0))))));
+    const R slow_inward_current_f_gate__beta_f = cmx_div((R(0.0065) * cmx_exp(((-R(0.02)) * (V + R(30.0))))), (R(1.0) + cmx_exp(((-R(0.2)) * (V + R(30.0))))));
+    const R time_dependent_potassium_current_X_gate__alpha_X = cmx_div((R(0.0005) * cmx_exp((R(0.083) * (V + R(50.0))))), (R(1.0) + cmx_exp((R(0.057) * (V + R(50.0))))));
+    const R time_dependent_potassium_current_X_gate__beta_X = cmx_div((R(0.0013) * cmx_exp(((-R(0.06)) * (V + R(20.0))))), (R(1.0) + cmx_exp(((-R(0.04)) * (V + R(20.0))))));
+    y[1] = (y[1] + ((fast_sodium_current_m_gate__alpha_m * R(1.0))) * dt) / (R(1) - (((fast_sodium_current_m_gate__alpha_m * (-R(1))) - (fast_sodium_current_m_gate__beta_m * R(1)))) * dt);
+    y[2] = (y[2] + ((fast_sodium_current_h_gate__alpha_h * R(1.0))) * dt) / (R(1) - (((fast_sodium_current_h_gate__alpha_h * (-R(1))) - (fast_sodium_current_h_gate__beta_h * R(1)))) * dt);
+    y[3] = (y[3] + ((fast_sodium_current_j_gate__alpha_j * R(1.0))) * dt) / (R(1) - (((fast_sodium_current_j_gate__alpha_j * (-R(1))) - (fast_sodium_current_j_gate__beta_j * R(1)))) * dt);
+    y[4] = (y[4] + ((slow_inward_current_d_gate__alpha_d * R(1.0))) * dt) / (R(1) - (((slow_inward_current_d_gate__alpha_d * (-R(1))) - (slow_inward_current_d_gate__beta_d * R(1)))) * dt);
+    y[5] = (y[5] + ((slow_inward_current_f_gate__alpha_f * R(1.0))) * dt) / (R(1) - (((slow_inward_current_f_gate__alpha_f * (-R(1))) - (slow_inward_current_f_gate__beta_f * R(1)))) * dt);
+    y[6] = (y[6] + ((time_dependent_potassium_current_X_gate__alpha_X * R(1.0))) * dt) / (R(1) - (((time_dependent_potassium_current_X_gate__alpha_X * (-R(1))) - (time_dependent_potassium_current_X_gate__beta_X * R(1)))) * dt);
+}
+
+// Newton residual of the nonlinear states at guess g (y holds the other
+// states, linear ones already advanced): g - y_old - dt f(g).
+template <class R>
+CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const R* p, R cm, R dt, R* res)
+{
+    CMX_MATH_USING
+    (void)y; (void)p; (void)cm; (void)i_stim_area;
+    const R slow_inward_current__P_si = R(0.09);
+    const R i_stim = i_stim_area;
+    (void)i_stim;
+    const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((g[0] * (R(1) / R(1.0))))));
+    const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
+    res[0] = g[0] - y[7] - dt * ((((-R(0.0001)) * (R(1) / R(1.0))) * slow_inward_current__i_si) + (R(0.07) * (R(0.0001) - g[0])));
+}
+
+// Jacobian of be_residual in g: J[i*N + j] = delta_ij - dt d f_i / d g_j.
+template <class R>
+CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const R* p, R cm, R dt, R* J)
+{
+    CMX_MATH_USING
+    (void)y; (void)p; (void)cm; (void)i_stim_area;
+    const R slow_inward_current__P_si = R(0.09);
+    const R i_stim = i_stim_area;
+    (void)i_stim;
+    const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((g[0] * (R(1) / R(1.0))))));
+    const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
+    const R d_slow_inward_current__E_si__d0 = (-((cmx_div(cmx_div(R(1), R(1.0)), (g[0] * (R(1) / R(1.0)))) * R(13.0287))));
+    const R d_slow_inward_current__i_si__d0 = (((-d_slow_inward_current__E_si__d0) * slow_inward_current__P_si * y[4] * y[5]));
+    J[0] = R(1) - dt * (((d_slow_inward_current__i_si__d0 * ((-R(0.0001)) * (R(1) / R(1.0))))) + (((-R(1)) * R(0.07))));
+}
+
+// One backward-Euler step of every state but V at the given V
+// (ComputeOneStepExceptVoltage): linear states in closed form, then the
+// nonlinear ones by Newton from their current values.  Returns the Newton
+// iteration count, -1 if it did not converge.
+template <class R>
+CMX_HD inline int be_step(R* y, R V, R i_stim_area, const R* p, R cm, R dt)
+{
+    be_linear<R>(y, V, p, cm, dt);
+    R g[kNumNonlinear];
+CMX_UNROLL
+    for (int i = 0; i < kNumNonlinear; ++i) g[i] = y[kNonlinear[i]];
+    const int its = cmx_newton<kNumNonlinear, R>(
+        g, [&](const R* x, R* r) { be_residual<R>(y, x, V, i_stim_area, p, cm, dt, r); },
+        [&](const R* x, R* J) { be_jacobian<R>(y, x, V, i_stim_area, p, cm, dt, J); });
+CMX_UNROLL
+    for (int i = 0; i < kNumNonlinear; ++i) y[kNonlinear[i]] = g[i];
+    return its;
+}
+
 }}  // namespace cmx::lr91

@@ -10,6 +10,12 @@
 #define CMX_HD
 #endif
 
+#if defined(__clang__)
+#define CMX_UNROLL _Pragma("unroll")
+#else
+#define CMX_UNROLL _Pragma("GCC unroll 16")
+#endif
+
 #define CMX_MATH_USING \
     using std::exp; using std::log; using std::log10; using std::pow; \
     using std::sqrt; using std::floor; using std::ceil; using std::fabs;
@@ -58,4 +64,74 @@ CMX_HD inline R cmx_ipow(R x)
     else if constexpr (N == 2) return x * x;
     else if constexpr (N == 3) return x * x * x;
     else return (x * x) * (x * x);
+}
+
+// Newton's method of the reference's backward-Euler cells
+// (heart/src/odes/CardiacNewtonSolver.hpp:66-160): Jacobian solve by Gaussian
+// elimination without pivoting, stop when the update's max-norm is <= 1e-6,
+// and when the residual grows while the update is not yet small, walk only
+// 0.2 of the way in the direction whose relative change exceeds 1.  Returns
+// the iteration count, or -1 after 16 iterations (the reference throws).
+template <int N, class R, class Res, class Jac>
+CMX_HD inline int cmx_newton(R* g, Res residual, Jac jacobian)
+{
+    constexpr double eps = 1e-6;
+    R r[N], J[N * N], u[N];
+    residual(g, r);
+    R norm_r = R(0);
+CMX_UNROLL
+    for (int i = 0; i < N; ++i) norm_r = std::fabs(r[i]) > norm_r ? std::fabs(r[i]) : norm_r;
+    R norm_u = R(0);
+    int counter = 0;
+    do {
+        jacobian(g, J);
+CMX_UNROLL
+        for (int i = 0; i < N; ++i) {
+CMX_UNROLL
+            for (int ii = i + 1; ii < N; ++ii) {
+                const R fact = J[ii * N + i] / J[i * N + i];
+CMX_UNROLL
+                for (int j = i; j < N; ++j) J[ii * N + j] -= fact * J[i * N + j];
+                r[ii] -= fact * r[i];
+            }
+        }
+CMX_UNROLL
+        for (int i = N - 1; i >= 0; --i) {
+            R v = r[i];
+CMX_UNROLL
+            for (int j = i + 1; j < N; ++j) v -= J[i * N + j] * u[j];
+            u[i] = v / J[i * N + i];
+        }
+        norm_u = R(0);
+CMX_UNROLL
+        for (int i = 0; i < N; ++i) norm_u = std::fabs(u[i]) > norm_u ? std::fabs(u[i]) : norm_u;
+CMX_UNROLL
+        for (int i = 0; i < N; ++i) g[i] -= u[i];
+        const R prev = norm_r;
+        residual(g, r);
+        norm_r = R(0);
+CMX_UNROLL
+        for (int i = 0; i < N; ++i) norm_r = std::fabs(r[i]) > norm_r ? std::fabs(r[i]) : norm_r;
+        if (norm_r > prev && norm_u > R(eps)) {
+            R rel_max = R(0);
+            int dir = 0;
+CMX_UNROLL
+            for (int i = 0; i < N; ++i) {
+                const R rel = std::fabs(u[i] / g[i]);
+                if (rel > rel_max) { rel_max = rel; dir = i; }
+            }
+            if (rel_max > R(1)) {
+CMX_UNROLL
+                for (int i = 0; i < N; ++i)
+                    if (i == dir) g[i] += R(0.8) * u[i];
+                residual(g, r);
+                norm_r = R(0);
+CMX_UNROLL
+                for (int i = 0; i < N; ++i) norm_r = std::fabs(r[i]) > norm_r ? std::fabs(r[i]) : norm_r;
+            }
+        }
+        ++counter;
+        if (counter > 15) return -1;
+    } while (norm_u > R(eps));
+    return counter;
 }

@@ -392,4 +392,371 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     return -dv * cm;
 }
 
+// Backward Euler: the states solved by Newton (all but the linear gates).
+constexpr int kNumNonlinear = 7;
+static constexpr int kNonlinear[7] = {10, 13, 14, 15, 16, 17, 18};
+
+// Linear states: y <- (y + a dt) / (1 - b dt) for dy/dt = a + b y, a and b
+// functions of V (ComputeOneStepExceptVoltage, closed-form part).
+template <class R>
+CMX_HD inline void be_linear(R* y, R V, const R* p, R cm, R dt)
+{
+    CMX_MATH_USING
+    (void)p; (void)cm; (void)V;
+    const R L_type_Ca_current_d_gate__d_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(8.0)) - V) * (R(1) / R(7.5))))));
+    const R L_type_Ca_current_d_gate__alpha_d = (cmx_div(R(1.4), (R(1.0) + cmx_exp((((-R(35.0)) - V) * (R(1) / R(13.0)))))) + R(0.25));
+    const R L_type_Ca_current_d_gate__beta_d = cmx_div(R(1.4), (R(1.0) + cmx_exp(((V + R(5.0)) * (R(1) / R(5.0))))));
+    const R L_type_Ca_current_d_gate__gamma_d = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(50.0) - V) * (R(1) / R(20.0))))));
+    const R L_type_Ca_current_d_gate__tau_d = ((L_type_Ca_current_d_gate__alpha_d * L_type_Ca_current_d_gate__beta_d) + L_type_Ca_current_d_gate__gamma_d);
+    const R L_type_Ca_current_f2_gate__f2_inf = (cmx_div(R(0.67), (R(1.0) + cmx_exp(((V + R(35.0)) * (R(1) / R(7.0)))))) + R(0.33));
+    const R L_type_Ca_current_f2_gate__tau_f2 = ((R(562.0) * cmx_exp(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(240.0))))) + cmx_div(R(31.0), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(10.0)))))) + cmx_div(R(80.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(10.0)))))));
+    const R L_type_Ca_current_f_gate__f_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(20.0)) * (R(1) / R(7.0))))));
+    const R L_type_Ca_current_f_gate__tau_f = ((R(1102.5) * cmx_exp(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(225.0))))) + cmx_div(R(200.0), (R(1.0) + cmx_exp(((R(13.0) - V) * (R(1) / R(10.0)))))) + cmx_div(R(180.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(10.0)))))) + R(20.0));
+    const R fast_sodium_current_h_gate__h_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp(((V + R(71.55)) * (R(1) / R(7.43)))))));
+    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.057) * cmx_exp(((-(V + R(80.0))) * (R(1) / R(6.8)))))) : (R(0.0)));
+    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(2.7) * cmx_exp((R(0.079) * V))) + (R(310000.0) * cmx_exp((R(0.3485) * V))))) : (cmx_div(R(0.77), (R(0.13) * (R(1.0) + cmx_exp(((V + R(10.66)) * (R(1) / (-R(11.1))))))))));
+    const R fast_sodium_current_h_gate__tau_h = cmx_div(R(1.0), (fast_sodium_current_h_gate__alpha_h + fast_sodium_current_h_gate__beta_h));
+    const R fast_sodium_current_j_gate__j_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp(((V + R(71.55)) * (R(1) / R(7.43)))))));
+    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(25428.0)) * cmx_exp((R(0.2444) * V))) - (R(6.948e-06) * cmx_exp(((-R(0.04391)) * V)))) * (V + R(37.78))), (R(1.0) + cmx_exp((R(0.311) * (V + R(79.23))))))) : (R(0.0)));
+    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.02424) * cmx_exp(((-R(0.01052)) * V))), (R(1.0) + cmx_exp(((-R(0.1378)) * (V + R(40.14))))))) : (cmx_div((R(0.6) * cmx_exp((R(0.057) * V))), (R(1.0) + cmx_exp(((-R(0.1)) * (V + R(32.0))))))));
+    const R fast_sodium_current_j_gate__tau_j = cmx_div(R(1.0), (fast_sodium_current_j_gate__alpha_j + fast_sodium_current_j_gate__beta_j));
+    const R fast_sodium_current_m_gate__m_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp((((-R(56.86)) - V) * (R(1) / R(9.03)))))));
+    const R fast_sodium_current_m_gate__alpha_m = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(60.0)) - V) * (R(1) / R(5.0))))));
+    const R fast_sodium_current_m_gate__beta_m = (cmx_div(R(0.1), (R(1.0) + cmx_exp(((V + R(35.0)) * (R(1) / R(5.0)))))) + cmx_div(R(0.1), (R(1.0) + cmx_exp(((V - R(50.0)) * (R(1) / R(200.0)))))));
+    const R fast_sodium_current_m_gate__tau_m = (fast_sodium_current_m_gate__alpha_m * fast_sodium_current_m_gate__beta_m);
+    const R rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 = cmx_div(R(450.0), (R(1.0) + cmx_exp((((-R(45.0)) - V) * (R(1) / R(10.0))))));
+    const R rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1 = cmx_div(R(6.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(11.5))))));
+    const R rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1 = (rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 * rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1);
+    const R rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(26.0)) - V) * (R(1) / R(7.0))))));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 = cmx_div(R(3.0), (R(1.0) + cmx_exp((((-R(60.0)) - V) * (R(1) / R(20.0))))));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2 = cmx_div(R(1.12), (R(1.0) + cmx_exp(((V - R(60.0)) * (R(1) / R(20.0))))));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2 = (rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 * rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2);
+    const R rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(88.0)) * (R(1) / R(24.0))))));
+    const R slow_time_dependent_potassium_current_Xs_gate__alpha_xs = cmx_div(R(1400.0), sqrt((R(1.0) + cmx_exp(((R(5.0) - V) * (R(1) / R(6.0)))))));
+    const R slow_time_dependent_potassium_current_Xs_gate__beta_xs = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V - R(35.0)) * (R(1) / R(15.0))))));
+    const R slow_time_dependent_potassium_current_Xs_gate__tau_xs = ((slow_time_dependent_potassium_current_Xs_gate__alpha_xs * slow_time_dependent_potassium_current_Xs_gate__beta_xs) + R(80.0));
+    const R slow_time_dependent_potassium_current_Xs_gate__xs_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(5.0)) - V) * (R(1) / R(14.0))))));
+    const R transient_outward_current_r_gate__r_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(20.0) - V) * (R(1) / R(6.0))))));
+    const R transient_outward_current_r_gate__tau_r = ((R(9.5) * cmx_exp(((-cmx_ipow<2>((V + R(40.0)))) * (R(1) / R(1800.0))))) + R(0.8));
+    const R transient_outward_current_s_gate__s_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(20.0)) * (R(1) / R(5.0))))));
+    const R transient_outward_current_s_gate__tau_s = ((R(85.0) * cmx_exp(((-cmx_ipow<2>((V + R(45.0)))) * (R(1) / R(320.0))))) + cmx_div(R(5.0), (R(1.0) + cmx_exp(((V - R(20.0)) * (R(1) / R(5.0)))))) + R(3.0));
+    y[1] = (y[1] + ((rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf / rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1)) * dt) / (R(1) - (((-R(1)) / rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1)) * dt);
+    y[2] = (y[2] + ((rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf / rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2)) * dt) / (R(1) - (((-R(1)) / rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2)) * dt);
+    y[3] = (y[3] + ((slow_time_dependent_potassium_current_Xs_gate__xs_inf / slow_time_dependent_potassium_current_Xs_gate__tau_xs)) * dt) / (R(1) - (((-R(1)) / slow_time_dependent_potassium_current_Xs_gate__tau_xs)) * dt);
+    y[4] = (y[4] + ((fast_sodium_current_m_gate__m_inf / fast_sodium_current_m_gate__tau_m)) * dt) / (R(1) - (((-R(1)) / fast_sodium_current_m_gate__tau_m)) * dt);
+    y[5] = (y[5] + ((fast_sodium_current_h_gate__h_inf / fast_sodium_current_h_gate__tau_h)) * dt) / (R(1) - (((-R(1)) / fast_sodium_current_h_gate__tau_h)) * dt);
+    y[6] = (y[6] + ((fast_sodium_current_j_gate__j_inf / fast_sodium_current_j_gate__tau_j)) * dt) / (R(1) - (((-R(1)) / fast_sodium_current_j_gate__tau_j)) * dt);
+    y[7] = (y[7] + ((L_type_Ca_current_d_gate__d_inf / L_type_Ca_current_d_gate__tau_d)) * dt) / (R(1) - (((-R(1)) / L_type_Ca_current_d_gate__tau_d)) * dt);
+    y[8] = (y[8] + ((L_type_Ca_current_f_gate__f_inf / L_type_Ca_current_f_gate__tau_f)) * dt) / (R(1) - (((-R(1)) / L_type_Ca_current_f_gate__tau_f)) * dt);
+    y[9] = (y[9] + ((L_type_Ca_current_f2_gate__f2_inf / L_type_Ca_current_f2_gate__tau_f2)) * dt) / (R(1) - (((-R(1)) / L_type_Ca_current_f2_gate__tau_f2)) * dt);
+    y[11] = (y[11] + ((transient_outward_current_s_gate__s_inf / transient_outward_current_s_gate__tau_s)) * dt) / (R(1) - (((-R(1)) / transient_outward_current_s_gate__tau_s)) * dt);
+    y[12] = (y[12] + ((transient_outward_current_r_gate__r_inf / transient_outward_current_r_gate__tau_r)) * dt) / (R(1) - (((-R(1)) / transient_outward_current_r_gate__tau_r)) * dt);
+}
+
+// Newton residual of the nonlinear states at guess g (y holds the other
+// states, linear ones already advanced): g - y_old - dt f(g).
+template <class R>
+CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const R* p, R cm, R dt, R* res)
+{
+    CMX_MATH_USING
+    (void)y; (void)p; (void)cm; (void)i_stim_area;
+    const R membrane__R = R(8314.472);
+    const R membrane__T = R(310.0);
+    const R membrane__F = R(96485.3415);
+    const R membrane__Cm = R(0.185);
+    const R membrane__V_c = R(0.016404);
+    const R reversal_potentials__P_kna = R(0.03);
+    const R inward_rectifier_potassium_current__g_K1 = R(5.405);
+    const R rapid_time_dependent_potassium_current__ScaleFactorGkr = R(1.0);
+    const R rapid_time_dependent_potassium_current__g_Kr = R(0.153);
+    const R slow_time_dependent_potassium_current__ScaleFactorGks = R(1.0);
+    const R slow_time_dependent_potassium_current__g_Ks = R(0.392);
+    const R fast_sodium_current__g_Na = R(14.838);
+    const R sodium_background_current__g_bna = R(0.00029);
+    const R L_type_Ca_current__g_CaL = R(3.98e-05);
+    const R calcium_background_current__g_bca = R(0.000592);
+    const R transient_outward_current__ScaleFactorIto = R(1.0);
+    const R transient_outward_current__g_to = R(0.294);
+    const R sodium_potassium_pump_current__P_NaK = R(2.724);
+    const R sodium_potassium_pump_current__K_mk = R(1.0);
+    const R sodium_potassium_pump_current__K_mNa = R(40.0);
+    const R sodium_calcium_exchanger_current__K_NaCa = R(1000.0);
+    const R sodium_calcium_exchanger_current__K_sat = R(0.1);
+    const R sodium_calcium_exchanger_current__alpha = R(2.5);
+    const R sodium_calcium_exchanger_current__gamma = R(0.35);
+    const R sodium_calcium_exchanger_current__Km_Ca = R(1.38);
+    const R sodium_calcium_exchanger_current__Km_Nai = R(87.5);
+    const R calcium_pump_current__g_pCa = R(0.1238);
+    const R calcium_pump_current__K_pCa = R(0.0005);
+    const R potassium_pump_current__g_pK = R(0.0146);
+    const R calcium_dynamics__Ca_o = R(2.0);
+    const R calcium_dynamics__k1_prime = R(0.15);
+    const R calcium_dynamics__k2_prime = R(0.045);
+    const R calcium_dynamics__k3 = R(0.06);
+    const R calcium_dynamics__k4 = R(0.005);
+    const R calcium_dynamics__EC = R(1.5);
+    const R calcium_dynamics__max_sr = R(2.5);
+    const R calcium_dynamics__min_sr = R(1.0);
+    const R calcium_dynamics__V_rel = R(0.102);
+    const R calcium_dynamics__V_xfer = R(0.0038);
+    const R calcium_dynamics__K_up = R(0.00025);
+    const R calcium_dynamics__V_leak = R(0.00036);
+    const R calcium_dynamics__Vmax_up = R(0.006375);
+    const R calcium_dynamics__Buf_c = R(0.2);
+    const R calcium_dynamics__K_buf_c = R(0.001);
+    const R calcium_dynamics__Buf_sr = R(10.0);
+    const R calcium_dynamics__K_buf_sr = R(0.3);
+    const R calcium_dynamics__Buf_ss = R(0.4);
+    const R calcium_dynamics__K_buf_ss = R(0.00025);
+    const R calcium_dynamics__V_sr = R(0.001094);
+    const R calcium_dynamics__V_ss = R(5.468e-05);
+    const R sodium_dynamics__Na_o = R(140.0);
+    const R potassium_dynamics__K_o = R(5.4);
+    const R i_stim = i_stim_area / cm;
+    (void)i_stim;
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current_fCass_gate__fCass_inf = (cmx_div(R(0.6), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(0.4));
+    const R L_type_Ca_current_fCass_gate__tau_fCass = (cmx_div(R(80.0), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(2.0));
+    const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, g[1])));
+    const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
+    const R calcium_dynamics__Ca_i_bufc = cmx_div(R(1.0), (R(1.0) + cmx_div((calcium_dynamics__Buf_c * calcium_dynamics__K_buf_c), cmx_ipow<2>((g[1] + calcium_dynamics__K_buf_c)))));
+    const R calcium_dynamics__Ca_sr_bufsr = cmx_div(R(1.0), (R(1.0) + cmx_div((calcium_dynamics__Buf_sr * calcium_dynamics__K_buf_sr), cmx_ipow<2>((g[2] + calcium_dynamics__K_buf_sr)))));
+    const R calcium_dynamics__Ca_ss_bufss = cmx_div(R(1.0), (R(1.0) + cmx_div((calcium_dynamics__Buf_ss * calcium_dynamics__K_buf_ss), cmx_ipow<2>((g[3] + calcium_dynamics__K_buf_ss)))));
+    const R calcium_dynamics__i_leak = (calcium_dynamics__V_leak * (g[2] - g[1]));
+    const R calcium_dynamics__kcasr = (calcium_dynamics__max_sr - cmx_div((calcium_dynamics__max_sr - calcium_dynamics__min_sr), (R(1.0) + cmx_ipow<2>(cmx_div(calcium_dynamics__EC, g[2])))));
+    const R calcium_dynamics__k1 = cmx_div(calcium_dynamics__k1_prime, calcium_dynamics__kcasr);
+    const R calcium_dynamics__O = cmx_div((calcium_dynamics__k1 * cmx_ipow<2>(g[3]) * g[4]), (calcium_dynamics__k3 + (calcium_dynamics__k1 * cmx_ipow<2>(g[3]))));
+    const R calcium_dynamics__i_rel = (calcium_dynamics__V_rel * calcium_dynamics__O * (g[2] - g[3]));
+    const R calcium_dynamics__i_up = cmx_div(calcium_dynamics__Vmax_up, (R(1.0) + cmx_div(cmx_ipow<2>(calcium_dynamics__K_up), cmx_ipow<2>(g[1]))));
+    const R calcium_dynamics__i_xfer = (calcium_dynamics__V_xfer * (g[3] - g[1]));
+    const R calcium_dynamics__k2 = (calcium_dynamics__k2_prime * calcium_dynamics__kcasr);
+    const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * g[1]), (g[1] + calcium_pump_current__K_pCa));
+    const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, g[5])));
+    const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
+    const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, g[6])));
+    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
+    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
+    const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
+    const R inward_rectifier_potassium_current__i_K1 = (inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
+    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
+    const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))));
+    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
+    const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
+    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(g[5]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * g[1] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
+    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]), (g[5] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
+    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11] * (V - reversal_potentials__E_K));
+    res[0] = g[0] - y[10] - dt * cmx_div((L_type_Ca_current_fCass_gate__fCass_inf - g[0]), L_type_Ca_current_fCass_gate__tau_fCass);
+    res[1] = g[1] - y[13] - dt * (calcium_dynamics__Ca_i_bufc * (((((calcium_dynamics__i_leak - calcium_dynamics__i_up) * calcium_dynamics__V_sr) * (R(1) / membrane__V_c)) + calcium_dynamics__i_xfer) - ((((calcium_background_current__i_b_Ca + calcium_pump_current__i_p_Ca) - (R(2.0) * sodium_calcium_exchanger_current__i_NaCa)) * membrane__Cm) * (R(1) / (R(2.0) * membrane__V_c * membrane__F)))));
+    res[2] = g[2] - y[14] - dt * (calcium_dynamics__Ca_sr_bufsr * (calcium_dynamics__i_up - (calcium_dynamics__i_rel + calcium_dynamics__i_leak)));
+    res[3] = g[3] - y[15] - dt * (calcium_dynamics__Ca_ss_bufss * (((((-L_type_Ca_current__i_CaL) * membrane__Cm) * (R(1) / (R(2.0) * calcium_dynamics__V_ss * membrane__F))) + ((calcium_dynamics__i_rel * calcium_dynamics__V_sr) * (R(1) / calcium_dynamics__V_ss))) - ((calcium_dynamics__i_xfer * membrane__V_c) * (R(1) / calcium_dynamics__V_ss))));
+    res[4] = g[4] - y[16] - dt * (((-calcium_dynamics__k2) * g[3] * g[4]) + (calcium_dynamics__k4 * (R(1.0) - g[4])));
+    res[5] = g[5] - y[17] - dt * (((-(fast_sodium_current__i_Na + sodium_background_current__i_b_Na + (R(3.0) * sodium_potassium_pump_current__i_NaK) + (R(3.0) * sodium_calcium_exchanger_current__i_NaCa))) * (R(1) / (membrane__V_c * membrane__F))) * membrane__Cm);
+    res[6] = g[6] - y[18] - dt * (((-((inward_rectifier_potassium_current__i_K1 + transient_outward_current__i_to + rapid_time_dependent_potassium_current__i_Kr + slow_time_dependent_potassium_current__i_Ks + potassium_pump_current__i_p_K + i_stim) - (R(2.0) * sodium_potassium_pump_current__i_NaK))) * (R(1) / (membrane__V_c * membrane__F))) * membrane__Cm);
+}
+
+// Jacobian of be_residual in g: J[i*N + j] = delta_ij - dt d f_i / d g_j.
+template <class R>
+CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const R* p, R cm, R dt, R* J)
+{
+    CMX_MATH_USING
+    (void)y; (void)p; (void)cm; (void)i_stim_area;
+    const R membrane__R = R(8314.472);
+    const R membrane__T = R(310.0);
+    const R membrane__F = R(96485.3415);
+    const R membrane__Cm = R(0.185);
+    const R membrane__V_c = R(0.016404);
+    const R reversal_potentials__P_kna = R(0.03);
+    const R inward_rectifier_potassium_current__g_K1 = R(5.405);
+    const R rapid_time_dependent_potassium_current__ScaleFactorGkr = R(1.0);
+    const R rapid_time_dependent_potassium_current__g_Kr = R(0.153);
+    const R slow_time_dependent_potassium_current__ScaleFactorGks = R(1.0);
+    const R slow_time_dependent_potassium_current__g_Ks = R(0.392);
+    const R fast_sodium_current__g_Na = R(14.838);
+    const R sodium_background_current__g_bna = R(0.00029);
+    const R L_type_Ca_current__g_CaL = R(3.98e-05);
+    const R calcium_background_current__g_bca = R(0.000592);
+    const R transient_outward_current__ScaleFactorIto = R(1.0);
+    const R transient_outward_current__g_to = R(0.294);
+    const R sodium_potassium_pump_current__P_NaK = R(2.724);
+    const R sodium_potassium_pump_current__K_mk = R(1.0);
+    const R sodium_potassium_pump_current__K_mNa = R(40.0);
+    const R sodium_calcium_exchanger_current__K_NaCa = R(1000.0);
+    const R sodium_calcium_exchanger_current__K_sat = R(0.1);
+    const R sodium_calcium_exchanger_current__alpha = R(2.5);
+    const R sodium_calcium_exchanger_current__gamma = R(0.35);
+    const R sodium_calcium_exchanger_current__Km_Ca = R(1.38);
+    const R sodium_calcium_exchanger_current__Km_Nai = R(87.5);
+    const R calcium_pump_current__g_pCa = R(0.1238);
+    const R calcium_pump_current__K_pCa = R(0.0005);
+    const R potassium_pump_current__g_pK = R(0.0146);
+    const R calcium_dynamics__Ca_o = R(2.0);
+    const R calcium_dynamics__k1_prime = R(0.15);
+    const R calcium_dynamics__k2_prime = R(0.045);
+    const R calcium_dynamics__k3 = R(0.06);
+    const R calcium_dynamics__k4 = R(0.005);
+    const R calcium_dynamics__EC = R(1.5);
+    const R calcium_dynamics__max_sr = R(2.5);
+    const R calcium_dynamics__min_sr = R(1.0);
+    const R calcium_dynamics__V_rel = R(0.102);
+    const R calcium_dynamics__V_xfer = R(0.0038);
+    const R calcium_dynamics__K_up = R(0.00025);
+    const R calcium_dynamics__V_leak = R(0.00036);
+    const R calcium_dynamics__Vmax_up = R(0.006375);
+    const R calcium_dynamics__Buf_c = R(0.2);
+    const R calcium_dynamics__K_buf_c = R(0.001);
+    const R calcium_dynamics__Buf_sr = R(10.0);
+    const R calcium_dynamics__K_buf_sr = R(0.3);
+    const R calcium_dynamics__Buf_ss = R(0.4);
+    const R calcium_dynamics__K_buf_ss = R(0.00025);
+    const R calcium_dynamics__V_sr = R(0.001094);
+    const R calcium_dynamics__V_ss = R(5.468e-05);
+    const R sodium_dynamics__Na_o = R(140.0);
+    const R potassium_dynamics__K_o = R(5.4);
+    const R i_stim = i_stim_area / cm;
+    (void)i_stim;
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current_fCass_gate__fCass_inf = (cmx_div(R(0.6), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(0.4));
+    const R L_type_Ca_current_fCass_gate__tau_fCass = (cmx_div(R(80.0), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(2.0));
+    const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, g[1])));
+    const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
+    const R calcium_dynamics__Ca_i_bufc = cmx_div(R(1.0), (R(1.0) + cmx_div((calcium_dynamics__Buf_c * calcium_dynamics__K_buf_c), cmx_ipow<2>((g[1] + calcium_dynamics__K_buf_c)))));
+    const R calcium_dynamics__Ca_sr_bufsr = cmx_div(R(1.0), (R(1.0) + cmx_div((calcium_dynamics__Buf_sr * calcium_dynamics__K_buf_sr), cmx_ipow<2>((g[2] + calcium_dynamics__K_buf_sr)))));
+    const R calcium_dynamics__Ca_ss_bufss = cmx_div(R(1.0), (R(1.0) + cmx_div((calcium_dynamics__Buf_ss * calcium_dynamics__K_buf_ss), cmx_ipow<2>((g[3] + calcium_dynamics__K_buf_ss)))));
+    const R calcium_dynamics__i_leak = (calcium_dynamics__V_leak * (g[2] - g[1]));
+    const R calcium_dynamics__kcasr = (calcium_dynamics__max_sr - cmx_div((calcium_dynamics__max_sr - calcium_dynamics__min_sr), (R(1.0) + cmx_ipow<2>(cmx_div(calcium_dynamics__EC, g[2])))));
+    const R calcium_dynamics__k1 = cmx_div(calcium_dynamics__k1_prime, calcium_dynamics__kcasr);
+    const R calcium_dynamics__O = cmx_div((calcium_dynamics__k1 * cmx_ipow<2>(g[3]) * g[4]), (calcium_dynamics__k3 + (calcium_dynamics__k1 * cmx_ipow<2>(g[3]))));
+    const R calcium_dynamics__i_rel = (calcium_dynamics__V_rel * calcium_dynamics__O * (g[2] - g[3]));
+    const R calcium_dynamics__i_up = cmx_div(calcium_dynamics__Vmax_up, (R(1.0) + cmx_div(cmx_ipow<2>(calcium_dynamics__K_up), cmx_ipow<2>(g[1]))));
+    const R calcium_dynamics__i_xfer = (calcium_dynamics__V_xfer * (g[3] - g[1]));
+    const R calcium_dynamics__k2 = (calcium_dynamics__k2_prime * calcium_dynamics__kcasr);
+    const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * g[1]), (g[1] + calcium_pump_current__K_pCa));
+    const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, g[5])));
+    const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
+    const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, g[6])));
+    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
+    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
+    const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
+    const R inward_rectifier_potassium_current__i_K1 = (inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
+    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
+    const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))));
+    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
+    const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
+    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(g[5]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * g[1] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
+    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]), (g[5] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
+    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11] * (V - reversal_potentials__E_K));
+    const R d_L_type_Ca_current__i_CaL__d0 = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div(((cmx_div(((R(1) * L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F))), (membrane__R * membrane__T)) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o))), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R d_L_type_Ca_current__i_CaL__d3 = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div(((((R(1) * R(0.25) * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))) * ((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))))), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R d_L_type_Ca_current_fCass_gate__fCass_inf__d3 = (cmx_div((-(R(0.6) * ((R(2.0) * cmx_ipow<1>((g[3] * (R(1) / R(0.05)))) * cmx_div(R(1), R(0.05)))))), ((R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))) * (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))))));
+    const R d_L_type_Ca_current_fCass_gate__tau_fCass__d3 = (cmx_div((-(R(80.0) * ((R(2.0) * cmx_ipow<1>((g[3] * (R(1) / R(0.05)))) * cmx_div(R(1), R(0.05)))))), ((R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))) * (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))))));
+    const R d_reversal_potentials__E_Ca__d1 = ((cmx_div(cmx_div((-(calcium_dynamics__Ca_o * R(1))), (g[1] * g[1])), cmx_div(calcium_dynamics__Ca_o, g[1])) * ((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F))));
+    const R d_calcium_background_current__i_b_Ca__d1 = (((-d_reversal_potentials__E_Ca__d1) * calcium_background_current__g_bca));
+    const R d_calcium_dynamics__Ca_i_bufc__d1 = cmx_div((-(R(1.0) * (cmx_div((-((calcium_dynamics__Buf_c * calcium_dynamics__K_buf_c) * (R(2.0) * cmx_ipow<1>((g[1] + calcium_dynamics__K_buf_c)) * (R(1))))), (cmx_ipow<2>((g[1] + calcium_dynamics__K_buf_c)) * cmx_ipow<2>((g[1] + calcium_dynamics__K_buf_c))))))), ((R(1.0) + cmx_div((calcium_dynamics__Buf_c * calcium_dynamics__K_buf_c), cmx_ipow<2>((g[1] + calcium_dynamics__K_buf_c)))) * (R(1.0) + cmx_div((calcium_dynamics__Buf_c * calcium_dynamics__K_buf_c), cmx_ipow<2>((g[1] + calcium_dynamics__K_buf_c))))));
+    const R d_calcium_dynamics__Ca_sr_bufsr__d2 = cmx_div((-(R(1.0) * (cmx_div((-((calcium_dynamics__Buf_sr * calcium_dynamics__K_buf_sr) * (R(2.0) * cmx_ipow<1>((g[2] + calcium_dynamics__K_buf_sr)) * (R(1))))), (cmx_ipow<2>((g[2] + calcium_dynamics__K_buf_sr)) * cmx_ipow<2>((g[2] + calcium_dynamics__K_buf_sr))))))), ((R(1.0) + cmx_div((calcium_dynamics__Buf_sr * calcium_dynamics__K_buf_sr), cmx_ipow<2>((g[2] + calcium_dynamics__K_buf_sr)))) * (R(1.0) + cmx_div((calcium_dynamics__Buf_sr * calcium_dynamics__K_buf_sr), cmx_ipow<2>((g[2] + calcium_dynamics__K_buf_sr))))));
+    const R d_calcium_dynamics__Ca_ss_bufss__d3 = cmx_div((-(R(1.0) * (cmx_div((-((calcium_dynamics__Buf_ss * calcium_dynamics__K_buf_ss) * (R(2.0) * cmx_ipow<1>((g[3] + calcium_dynamics__K_buf_ss)) * (R(1))))), (cmx_ipow<2>((g[3] + calcium_dynamics__K_buf_ss)) * cmx_ipow<2>((g[3] + calcium_dynamics__K_buf_ss))))))), ((R(1.0) + cmx_div((calcium_dynamics__Buf_ss * calcium_dynamics__K_buf_ss), cmx_ipow<2>((g[3] + calcium_dynamics__K_buf_ss)))) * (R(1.0) + cmx_div((calcium_dynamics__Buf_ss * calcium_dynamics__K_buf_ss), cmx_ipow<2>((g[3] + calcium_dynamics__K_buf_ss))))));
+    const R d_calcium_dynamics__i_leak__d1 = (((-R(1)) * calcium_dynamics__V_leak));
+    const R d_calcium_dynamics__i_leak__d2 = ((R(1) * calcium_dynamics__V_leak));
+    const R d_calcium_dynamics__kcasr__d2 = (-cmx_div((-((calcium_dynamics__max_sr - calcium_dynamics__min_sr) * ((R(2.0) * cmx_ipow<1>(cmx_div(calcium_dynamics__EC, g[2])) * cmx_div((-(calcium_dynamics__EC * R(1))), (g[2] * g[2])))))), ((R(1.0) + cmx_ipow<2>(cmx_div(calcium_dynamics__EC, g[2]))) * (R(1.0) + cmx_ipow<2>(cmx_div(calcium_dynamics__EC, g[2]))))));
+    const R d_calcium_dynamics__k1__d2 = cmx_div((-(calcium_dynamics__k1_prime * d_calcium_dynamics__kcasr__d2)), (calcium_dynamics__kcasr * calcium_dynamics__kcasr));
+    const R d_calcium_dynamics__O__d2 = cmx_div((((d_calcium_dynamics__k1__d2 * cmx_ipow<2>(g[3]) * g[4])) * (calcium_dynamics__k3 + (calcium_dynamics__k1 * cmx_ipow<2>(g[3]))) - (calcium_dynamics__k1 * cmx_ipow<2>(g[3]) * g[4]) * (((d_calcium_dynamics__k1__d2 * cmx_ipow<2>(g[3]))))), ((calcium_dynamics__k3 + (calcium_dynamics__k1 * cmx_ipow<2>(g[3]))) * (calcium_dynamics__k3 + (calcium_dynamics__k1 * cmx_ipow<2>(g[3])))));
+    const R d_calcium_dynamics__O__d3 = cmx_div(((((R(2.0) * cmx_ipow<1>(g[3]) * R(1)) * calcium_dynamics__k1 * g[4])) * (calcium_dynamics__k3 + (calcium_dynamics__k1 * cmx_ipow<2>(g[3]))) - (calcium_dynamics__k1 * cmx_ipow<2>(g[3]) * g[4]) * ((((R(2.0) * cmx_ipow<1>(g[3]) * R(1)) * calcium_dynamics__k1)))), ((calcium_dynamics__k3 + (calcium_dynamics__k1 * cmx_ipow<2>(g[3]))) * (calcium_dynamics__k3 + (calcium_dynamics__k1 * cmx_ipow<2>(g[3])))));
+    const R d_calcium_dynamics__O__d4 = cmx_div(((R(1) * calcium_dynamics__k1 * cmx_ipow<2>(g[3]))), (calcium_dynamics__k3 + (calcium_dynamics__k1 * cmx_ipow<2>(g[3]))));
+    const R d_calcium_dynamics__i_rel__d2 = ((d_calcium_dynamics__O__d2 * calcium_dynamics__V_rel * (g[2] - g[3])) + (R(1) * calcium_dynamics__V_rel * calcium_dynamics__O));
+    const R d_calcium_dynamics__i_rel__d3 = ((d_calcium_dynamics__O__d3 * calcium_dynamics__V_rel * (g[2] - g[3])) + ((-R(1)) * calcium_dynamics__V_rel * calcium_dynamics__O));
+    const R d_calcium_dynamics__i_rel__d4 = ((d_calcium_dynamics__O__d4 * calcium_dynamics__V_rel * (g[2] - g[3])));
+    const R d_calcium_dynamics__i_up__d1 = cmx_div((-(calcium_dynamics__Vmax_up * (cmx_div((-(cmx_ipow<2>(calcium_dynamics__K_up) * (R(2.0) * cmx_ipow<1>(g[1]) * R(1)))), (cmx_ipow<2>(g[1]) * cmx_ipow<2>(g[1])))))), ((R(1.0) + cmx_div(cmx_ipow<2>(calcium_dynamics__K_up), cmx_ipow<2>(g[1]))) * (R(1.0) + cmx_div(cmx_ipow<2>(calcium_dynamics__K_up), cmx_ipow<2>(g[1])))));
+    const R d_calcium_dynamics__i_xfer__d1 = (((-R(1)) * calcium_dynamics__V_xfer));
+    const R d_calcium_dynamics__i_xfer__d3 = ((R(1) * calcium_dynamics__V_xfer));
+    const R d_calcium_dynamics__k2__d2 = ((d_calcium_dynamics__kcasr__d2 * calcium_dynamics__k2_prime));
+    const R d_calcium_pump_current__i_p_Ca__d1 = cmx_div((((R(1) * calcium_pump_current__g_pCa)) * (g[1] + calcium_pump_current__K_pCa) - (calcium_pump_current__g_pCa * g[1]) * (R(1))), ((g[1] + calcium_pump_current__K_pCa) * (g[1] + calcium_pump_current__K_pCa)));
+    const R d_reversal_potentials__E_Na__d5 = ((cmx_div(cmx_div((-(sodium_dynamics__Na_o * R(1))), (g[5] * g[5])), cmx_div(sodium_dynamics__Na_o, g[5])) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
+    const R d_fast_sodium_current__i_Na__d5 = (((-d_reversal_potentials__E_Na__d5) * fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6]));
+    const R d_reversal_potentials__E_K__d6 = ((cmx_div(cmx_div((-(potassium_dynamics__K_o * R(1))), (g[6] * g[6])), cmx_div(potassium_dynamics__K_o, g[6])) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
+    const R d_inward_rectifier_potassium_current__alpha_K1__d6 = cmx_div((-(R(0.1) * ((cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0)))) * (((-d_reversal_potentials__E_K__d6) * R(0.06))))))), ((R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))) * (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0)))))));
+    const R d_inward_rectifier_potassium_current__beta_K1__d6 = cmx_div((((((cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0)))) * ((((-d_reversal_potentials__E_K__d6)) * R(0.0002)))) * R(3.0))) + (cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0)))) * (((-d_reversal_potentials__E_K__d6) * R(0.1))))) * (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))) - ((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))) * ((cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K))) * (((-d_reversal_potentials__E_K__d6) * (-R(0.5))))))), ((R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))) * (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K))))));
+    const R d_inward_rectifier_potassium_current__xK1_inf__d6 = cmx_div((d_inward_rectifier_potassium_current__alpha_K1__d6 * (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1) - inward_rectifier_potassium_current__alpha_K1 * (d_inward_rectifier_potassium_current__alpha_K1__d6 + d_inward_rectifier_potassium_current__beta_K1__d6)), ((inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1) * (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1)));
+    const R d_inward_rectifier_potassium_current__i_K1__d6 = ((d_inward_rectifier_potassium_current__xK1_inf__d6 * inward_rectifier_potassium_current__g_K1 * (V - reversal_potentials__E_K)) + ((-d_reversal_potentials__E_K__d6) * inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf));
+    const R d_potassium_pump_current__i_p_K__d6 = cmx_div((((-d_reversal_potentials__E_K__d6) * potassium_pump_current__g_pK)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R d_rapid_time_dependent_potassium_current__i_Kr__d6 = (((-d_reversal_potentials__E_K__d6) * rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2]));
+    const R d_reversal_potentials__E_Ks__d5 = ((cmx_div(cmx_div((-((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)) * (((R(1) * reversal_potentials__P_kna))))), ((g[6] + (reversal_potentials__P_kna * g[5])) * (g[6] + (reversal_potentials__P_kna * g[5])))), cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
+    const R d_reversal_potentials__E_Ks__d6 = ((cmx_div(cmx_div((-((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)) * (R(1)))), ((g[6] + (reversal_potentials__P_kna * g[5])) * (g[6] + (reversal_potentials__P_kna * g[5])))), cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
+    const R d_slow_time_dependent_potassium_current__i_Ks__d5 = (((-d_reversal_potentials__E_Ks__d5) * slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3])));
+    const R d_slow_time_dependent_potassium_current__i_Ks__d6 = (((-d_reversal_potentials__E_Ks__d6) * slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3])));
+    const R d_sodium_background_current__i_b_Na__d5 = (((-d_reversal_potentials__E_Na__d5) * sodium_background_current__g_bna));
+    const R d_sodium_calcium_exchanger_current__i_NaCa__d1 = cmx_div((((-((R(1) * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * sodium_calcium_exchanger_current__alpha))) * sodium_calcium_exchanger_current__K_NaCa)), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
+    const R d_sodium_calcium_exchanger_current__i_NaCa__d5 = cmx_div((((((R(3.0) * cmx_ipow<2>(g[5]) * R(1)) * cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * calcium_dynamics__Ca_o)) * sodium_calcium_exchanger_current__K_NaCa)), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
+    const R d_sodium_potassium_pump_current__i_NaK__d5 = cmx_div(cmx_div((((R(1) * ((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))))) * (g[5] + sodium_potassium_pump_current__K_mNa) - (((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]) * (R(1))), ((g[5] + sodium_potassium_pump_current__K_mNa) * (g[5] + sodium_potassium_pump_current__K_mNa))), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
+    const R d_transient_outward_current__i_to__d6 = (((-d_reversal_potentials__E_K__d6) * transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11]));
+    J[0] = R(1) - dt * cmx_div((-R(1)), L_type_Ca_current_fCass_gate__tau_fCass);
+    J[1] = R(0);
+    J[2] = R(0);
+    J[3] = R(0) - dt * cmx_div((d_L_type_Ca_current_fCass_gate__fCass_inf__d3 * L_type_Ca_current_fCass_gate__tau_fCass - (L_type_Ca_current_fCass_gate__fCass_inf - g[0]) * d_L_type_Ca_current_fCass_gate__tau_fCass__d3), (L_type_Ca_current_fCass_gate__tau_fCass * L_type_Ca_current_fCass_gate__tau_fCass));
+    J[4] = R(0);
+    J[5] = R(0);
+    J[6] = R(0);
+    J[7] = R(0);
+    J[8] = R(1) - dt * ((d_calcium_dynamics__Ca_i_bufc__d1 * (((((calcium_dynamics__i_leak - calcium_dynamics__i_up) * calcium_dynamics__V_sr) * (R(1) / membrane__V_c)) + calcium_dynamics__i_xfer) - ((((calcium_background_current__i_b_Ca + calcium_pump_current__i_p_Ca) - (R(2.0) * sodium_calcium_exchanger_current__i_NaCa)) * membrane__Cm) * (R(1) / (R(2.0) * membrane__V_c * membrane__F))))) + (((cmx_div((((d_calcium_dynamics__i_leak__d1 - d_calcium_dynamics__i_up__d1) * calcium_dynamics__V_sr)), membrane__V_c) + d_calcium_dynamics__i_xfer__d1) - cmx_div(((((d_calcium_background_current__i_b_Ca__d1 + d_calcium_pump_current__i_p_Ca__d1) - ((d_sodium_calcium_exchanger_current__i_NaCa__d1 * R(2.0)))) * membrane__Cm)), (R(2.0) * membrane__V_c * membrane__F))) * calcium_dynamics__Ca_i_bufc));
+    J[9] = R(0) - dt * (((cmx_div(((d_calcium_dynamics__i_leak__d2 * calcium_dynamics__V_sr)), membrane__V_c)) * calcium_dynamics__Ca_i_bufc));
+    J[10] = R(0) - dt * (((d_calcium_dynamics__i_xfer__d3) * calcium_dynamics__Ca_i_bufc));
+    J[11] = R(0);
+    J[12] = R(0) - dt * (((-cmx_div((((-((d_sodium_calcium_exchanger_current__i_NaCa__d5 * R(2.0)))) * membrane__Cm)), (R(2.0) * membrane__V_c * membrane__F))) * calcium_dynamics__Ca_i_bufc));
+    J[13] = R(0);
+    J[14] = R(0);
+    J[15] = R(0) - dt * (((d_calcium_dynamics__i_up__d1 - (d_calcium_dynamics__i_leak__d1)) * calcium_dynamics__Ca_sr_bufsr));
+    J[16] = R(1) - dt * ((d_calcium_dynamics__Ca_sr_bufsr__d2 * (calcium_dynamics__i_up - (calcium_dynamics__i_rel + calcium_dynamics__i_leak))) + ((-(d_calcium_dynamics__i_rel__d2 + d_calcium_dynamics__i_leak__d2)) * calcium_dynamics__Ca_sr_bufsr));
+    J[17] = R(0) - dt * (((-(d_calcium_dynamics__i_rel__d3)) * calcium_dynamics__Ca_sr_bufsr));
+    J[18] = R(0) - dt * (((-(d_calcium_dynamics__i_rel__d4)) * calcium_dynamics__Ca_sr_bufsr));
+    J[19] = R(0);
+    J[20] = R(0);
+    J[21] = R(0) - dt * (((cmx_div((((-d_L_type_Ca_current__i_CaL__d0) * membrane__Cm)), (R(2.0) * calcium_dynamics__V_ss * membrane__F))) * calcium_dynamics__Ca_ss_bufss));
+    J[22] = R(0) - dt * (((-cmx_div(((d_calcium_dynamics__i_xfer__d1 * membrane__V_c)), calcium_dynamics__V_ss)) * calcium_dynamics__Ca_ss_bufss));
+    J[23] = R(0) - dt * (((cmx_div(((d_calcium_dynamics__i_rel__d2 * calcium_dynamics__V_sr)), calcium_dynamics__V_ss)) * calcium_dynamics__Ca_ss_bufss));
+    J[24] = R(1) - dt * ((d_calcium_dynamics__Ca_ss_bufss__d3 * (((((-L_type_Ca_current__i_CaL) * membrane__Cm) * (R(1) / (R(2.0) * calcium_dynamics__V_ss * membrane__F))) + ((calcium_dynamics__i_rel * calcium_dynamics__V_sr) * (R(1) / calcium_dynamics__V_ss))) - ((calcium_dynamics__i_xfer * membrane__V_c) * (R(1) / calcium_dynamics__V_ss)))) + (((cmx_div((((-d_L_type_Ca_current__i_CaL__d3) * membrane__Cm)), (R(2.0) * calcium_dynamics__V_ss * membrane__F)) + cmx_div(((d_calcium_dynamics__i_rel__d3 * calcium_dynamics__V_sr)), calcium_dynamics__V_ss)) - cmx_div(((d_calcium_dynamics__i_xfer__d3 * membrane__V_c)), calcium_dynamics__V_ss)) * calcium_dynamics__Ca_ss_bufss));
+    J[25] = R(0) - dt * (((cmx_div(((d_calcium_dynamics__i_rel__d4 * calcium_dynamics__V_sr)), calcium_dynamics__V_ss)) * calcium_dynamics__Ca_ss_bufss));
+    J[26] = R(0);
+    J[27] = R(0);
+    J[28] = R(0);
+    J[29] = R(0);
+    J[30] = R(0) - dt * ((((-d_calcium_dynamics__k2__d2) * g[3] * g[4])));
+    J[31] = R(0) - dt * (((R(1) * (-calcium_dynamics__k2) * g[4])));
+    J[32] = R(1) - dt * (((R(1) * (-calcium_dynamics__k2) * g[3])) + (((-R(1)) * calcium_dynamics__k4)));
+    J[33] = R(0);
+    J[34] = R(0);
+    J[35] = R(0);
+    J[36] = R(0) - dt * ((cmx_div((-(((d_sodium_calcium_exchanger_current__i_NaCa__d1 * R(3.0))))), (membrane__V_c * membrane__F)) * membrane__Cm));
+    J[37] = R(0);
+    J[38] = R(0);
+    J[39] = R(0);
+    J[40] = R(1) - dt * ((cmx_div((-(d_fast_sodium_current__i_Na__d5 + d_sodium_background_current__i_b_Na__d5 + ((d_sodium_potassium_pump_current__i_NaK__d5 * R(3.0))) + ((d_sodium_calcium_exchanger_current__i_NaCa__d5 * R(3.0))))), (membrane__V_c * membrane__F)) * membrane__Cm));
+    J[41] = R(0);
+    J[42] = R(0);
+    J[43] = R(0);
+    J[44] = R(0);
+    J[45] = R(0);
+    J[46] = R(0);
+    J[47] = R(0) - dt * ((cmx_div((-((d_slow_time_dependent_potassium_current__i_Ks__d5) - ((d_sodium_potassium_pump_current__i_NaK__d5 * R(2.0))))), (membrane__V_c * membrane__F)) * membrane__Cm));
+    J[48] = R(1) - dt * ((cmx_div((-(d_inward_rectifier_potassium_current__i_K1__d6 + d_transient_outward_current__i_to__d6 + d_rapid_time_dependent_potassium_current__i_Kr__d6 + d_slow_time_dependent_potassium_current__i_Ks__d6 + d_potassium_pump_current__i_p_K__d6)), (membrane__V_c * membrane__F)) * membrane__Cm));
+}
+
+// One backward-Euler step of every state but V at the given V
+// (ComputeOneStepExceptVoltage): linear states in closed form, then the
+// nonlinear ones by Newton from their current values.  Returns the Newton
+// iteration count, -1 if it did not converge.
+template <class R>
+CMX_HD inline int be_step(R* y, R V, R i_stim_area, const R* p, R cm, R dt)
+{
+    be_linear<R>(y, V, p, cm, dt);
+    R g[kNumNonlinear];
+CMX_UNROLL
+    for (int i = 0; i < kNumNonlinear; ++i) g[i] = y[kNonlinear[i]];
+    const int its = cmx_newton<kNumNonlinear, R>(
+        g, [&](const R* x, R* r) { be_residual<R>(y, x, V, i_stim_area, p, cm, dt, r); },
+        [&](const R* x, R* J) { be_jacobian<R>(y, x, V, i_stim_area, p, cm, dt, J); });
+CMX_UNROLL
+    for (int i = 0; i < kNumNonlinear; ++i) y[kNonlinear[i]] = g[i];
+    return its;
+}
+
 }}  // namespace cmx::tt06_epi

@@ -196,6 +196,19 @@ struct CellTenTusscher2006EpiFromCellMLRushLarsen {
     static constexpr int kModel = CMX_MODEL_TT06_EPI;
     static constexpr int kIntegrator = CMX_RUSH_LARSEN;
 };
+// Backward-Euler classes (LuoRudy1991BackwardEuler[Opt].hpp,
+// TenTusscher2006EpiBackwardEuler[Opt].hpp; the Opt variants' lookup tables
+// only change round-off, so both names map to the same cell).
+struct CellLuoRudy1991FromCellMLBackwardEuler {
+    static constexpr int kModel = CMX_MODEL_LR91;
+    static constexpr int kIntegrator = CMX_BACKWARD_EULER;
+};
+struct CellLuoRudy1991FromCellMLBackwardEulerOpt : CellLuoRudy1991FromCellMLBackwardEuler {};
+struct CellTenTusscher2006EpiFromCellMLBackwardEuler {
+    static constexpr int kModel = CMX_MODEL_TT06_EPI;
+    static constexpr int kIntegrator = CMX_BACKWARD_EULER;
+};
+struct CellTenTusscher2006EpiFromCellMLBackwardEulerOpt : CellTenTusscher2006EpiFromCellMLBackwardEuler {};
 
 // ---------------------------------------------------------------------------
 // Mesh (regular slabs; general simplices through SetNodesAndElements)

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -123,14 +124,20 @@ const double* initial_state(int model)
     return model == ORC_TT06_EPI ? cmx::tt06_epi::kInitialState : cmx::lr91::kInitialState;
 }
 
+// Newton failures of backward-Euler substeps (the reference throws).
+std::atomic<int> g_newton_failures{0};
+
 // One cell substep over [t, t+dt] with V clamped unless update_v.  stim_area
 // is the area stimulus (uA/cm^2) at time t.  Returns dV/dt when update_v.
-double cell_substep(int model, bool rl, const double* p, double cm, double* y, double t, double dt,
+// Backward Euler (AbstractBackwardEulerCardiacCell.hpp:300-330) advances V
+// first and then steps the other states at the new V.
+double cell_substep(int model, int integ, const double* p, double cm, double* y, double t, double dt,
                     double stim_area, bool update_v)
 {
     (void)t;
-    double dy[64], ginf[64], gk[64];
+    double dy[64] = {}, ginf[64], gk[64];
     const int S = num_states(model);
+    const bool rl = integ == ORC_RL;
     double dv = 0.0;
     if (model == ORC_LR91_HAND) {
         const double iion = Lr91Hand::currents(y, dy);
@@ -139,6 +146,18 @@ double cell_substep(int model, bool rl, const double* p, double cm, double* y, d
         // reference class is an AbstractCardiacCell).
         if (update_v) y[0] += dt * dv;
         for (int i = 1; i < S; ++i) y[i] += dt * dy[i];
+        return dv;
+    }
+    if (integ == ORC_BE) {
+        int its;
+        if (model == ORC_LR91_CELLML) {
+            if (update_v) y[0] += dt * (dv = cmx::lr91::dvdt<double>(y, y[0], stim_area, p, cm));
+            its = cmx::lr91::be_step<double>(y, y[0], stim_area, p, cm, dt);
+        } else {
+            if (update_v) y[0] += dt * (dv = cmx::tt06_epi::dvdt<double>(y, y[0], stim_area, p, cm));
+            its = cmx::tt06_epi::be_step<double>(y, y[0], stim_area, p, cm, dt);
+        }
+        if (its < 0) ++g_newton_failures;
         return dv;
     }
     if (model == ORC_LR91_CELLML) {
@@ -213,14 +232,15 @@ extern "C" {
 
 int orc_model_num_states(int model) { return num_states(model); }
 
-int orc_cell_compute(int model, int rush_larsen, const double* params, double* y,
+int orc_cell_compute(int model, int integrator, const double* params, double* y,
                      double t0, double t_end, double dt,
                      double stim_mag, double stim_dur, double stim_start, double stim_period,
                      int steps_per_sample, double* samples, int max_samples)
 {
     const int S = num_states(model);
     if (S < 0 || dt <= 0 || t_end <= t0 || steps_per_sample <= 0) return -1;
-    if (model == ORC_LR91_HAND && rush_larsen) return -1;
+    if (integrator < ORC_FE || integrator > ORC_BE || (model == ORC_LR91_HAND && integrator != ORC_FE)) return -1;
+    const int fails0 = g_newton_failures.load();
     const double* p = params ? params : default_params(model);
     Stimulus stim(stim_mag, stim_dur, stim_start, stim_period);
     int ns = 0;
@@ -230,18 +250,22 @@ int orc_cell_compute(int model, int rush_larsen, const double* params, double* y
     };
     record();
     // AbstractCardiacCell::Compute with tSamp = dt: an outer TimeStepper at dt,
-    // each sample interval integrated by one inner step.  Rush-Larsen cells
-    // use curr_time = tStart + i*dt (AbstractRushLarsenCardiacCell.cpp:89-103).
+    // each sample interval integrated by one inner step.  Rush-Larsen and
+    // backward-Euler cells use curr_time = tStart + i*dt
+    // (AbstractRushLarsenCardiacCell.cpp:89-103,
+    // AbstractBackwardEulerCardiacCell.hpp:146-200).
+    const bool fixed_dt = integrator != ORC_FE;
     Stepper st(t0, t_end, dt);
     unsigned long long count = 0;
     while (!st.at_end()) {
-        const double t = rush_larsen ? t0 + st.k * dt : st.time;
-        const double h = rush_larsen ? dt : st.step_size();
-        cell_substep(model, rush_larsen != 0, p, 1.0, y, t, h, stim.at(t), true);
+        const double t = fixed_dt ? t0 + st.k * dt : st.time;
+        const double h = fixed_dt ? dt : st.step_size();
+        cell_substep(model, integrator, p, 1.0, y, t, h, stim.at(t), true);
         st.advance();
         ++count;
         if (count % steps_per_sample == 0) record();
     }
+    if (g_newton_failures.load() != fails0) return -1;
     return ns;
 }
 
@@ -516,7 +540,14 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
 #endif
     const int dim = cfg->dim, model = cfg->model;
     const int S = num_states(model);
-    if (S < 0 || (model == ORC_LR91_HAND && cfg->rush_larsen)) return -1;
+    const int integ = cfg->integrator;
+    if (S < 0 || integ < ORC_FE || integ > ORC_BE || (model == ORC_LR91_HAND && integ != ORC_FE)) return -1;
+    const int fails0 = g_newton_failures.load();
+    // Rush-Larsen and backward-Euler cells step by dt_ode always
+    // (AbstractRushLarsenCardiacCell.cpp:106-153,
+    // AbstractBackwardEulerCardiacCell.hpp:204-245), forward Euler by the
+    // stepper's next step (AbstractOneStepIvpOdeSolver.cpp:115-153).
+    const bool fixed_dt = integ != ORC_FE;
     int n, ne;
     if (orc_slab_sizes(dim, cfg->ex, cfg->ey, cfg->ez, &n, &ne)) return -1;
     std::vector<double> coords((size_t)n * dim);
@@ -590,7 +621,7 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
             while (!ode.at_end()) {
                 const double ts = ode.time;
                 const double sa = stimulated[i] ? stim.at(ts) / am : 0.0;
-                cell_substep(model, cfg->rush_larsen != 0, p, cm, y, ts, cfg->rush_larsen ? cfg->dt_ode : ode.step_size(), sa, true);
+                cell_substep(model, integ, p, cm, y, ts, fixed_dt ? cfg->dt_ode : ode.step_size(), sa, true);
                 ode.advance();
             }
             V[i] = y[0];
@@ -627,6 +658,7 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
             c0 = std::chrono::steady_clock::now();
             cells_with_voltage(th, t_next);
             st.t_ode += seconds_since(c0);
+            if (g_newton_failures.load() != fails0) return -3;
             pde.advance();
             if (cfg->modifier_out && (step + 1) % std::max(cfg->modifier_every, 1) == 0) modifier(t_next);
             if (act_out) {
@@ -657,9 +689,7 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
             while (!ode.at_end()) {
                 const double ts = ode.time;
                 const double sa = stimulated[i] ? stim.at(ts) / am : 0.0;
-                // Rush-Larsen cells step by dt_ode (AbstractRushLarsenCardiacCell.cpp:106-128),
-                // forward Euler by the stepper's next step (AbstractOneStepIvpOdeSolver.cpp:115-153).
-                cell_substep(model, cfg->rush_larsen != 0, p, cm, y, ts, cfg->rush_larsen ? cfg->dt_ode : ode.step_size(), sa, false);
+                cell_substep(model, integ, p, cm, y, ts, fixed_dt ? cfg->dt_ode : ode.step_size(), sa, false);
                 ode.advance();
             }
             y[0] = V[i];
@@ -687,6 +717,7 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
         st.t_ksp += seconds_since(c0);
         if (its < 0) return -2;
         st.cg_iterations += its;
+        if (g_newton_failures.load() != fails0) return -3;
         pde.advance();
         if (cfg->modifier_out && (step + 1) % std::max(cfg->modifier_every, 1) == 0) modifier(t_next);
         if (act_out) {

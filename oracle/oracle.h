@@ -35,16 +35,20 @@ extern "C" {
 #endif
 
 enum orc_model { ORC_LR91_HAND = 0, ORC_LR91_CELLML = 1, ORC_TT06_EPI = 2 };
+/* Cell integrators: forward Euler, Rush-Larsen, backward Euler (the
+ * reference's *BackwardEuler classes; generated models only). */
+enum orc_integrator { ORC_FE = 0, ORC_RL = 1, ORC_BE = 2 };
 
 int orc_model_num_states(int model);
 
 /* Single cell (AbstractCardiacCell::Compute): V integrated by forward Euler,
- * gates by Rush-Larsen when rush_larsen != 0.  y is in/out, CellML order
+ * the other states by the integrator (orc_integrator; backward Euler advances
+ * V first and steps the rest at the new V).  y is in/out, CellML order
  * (V first).  Stimulus: magnitude for t in [start, start+dur] (SimpleStimulus)
  * or every period if period > 0 (RegularStimulus).  Writes the state every
  * steps_per_sample steps (including t0) into samples[(k)*S + i]; returns the
  * number of samples written, or -1 on error. */
-int orc_cell_compute(int model, int rush_larsen, const double* params, double* y,
+int orc_cell_compute(int model, int integrator, const double* params, double* y,
                      double t0, double t_end, double dt,
                      double stim_mag, double stim_dur, double stim_start, double stim_period,
                      int steps_per_sample, double* samples, int max_samples);
@@ -79,7 +83,7 @@ typedef struct {
     int dim, ex, ey, ez;      /* slab: elements per axis */
     double h;                 /* space step (cm) */
     int model;                /* orc_model */
-    int rush_larsen;
+    int integrator;           /* orc_integrator */
     double sigma[3];          /* diagonal intracellular conductivity, fibres along x (mS/cm) */
     double am, cm;            /* 1/cm, uF/cm^2 */
     double dt_pde, dt_ode;    /* ms */

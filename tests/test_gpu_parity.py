@@ -307,6 +307,61 @@ def test_operator_splitting_graph_replay(precision):
     assert np.abs(c.voltage() - a.voltage()).max() <= max(tol, 1e-9 * np.abs(a.voltage()).max())
 
 
+@pytest.mark.parametrize('case', ['lr91_ici_fp64', 'lr91_split_fp64', 'tt06_ici_fp64', 'tt06_ici_mixed',
+                                  'tt06_split_fp64'])
+def test_backward_euler_cells(oracle, case):
+    """Backward-Euler cells (the reference's *BackwardEuler classes: linear
+    gates in closed form, the rest by Newton with the analytic Jacobian) in
+    the ICI step (V clamped) and with operator splitting (V by forward Euler
+    first), against the oracle: 1e-9 relative in FP64, 0.5 mV mixed."""
+    from chaste_amd import cmx
+    print('case backward euler', case, flush=True)
+    mixed = case.endswith('mixed')
+    split = '_split_' in case
+    if case.startswith('lr91'):
+        dims, sigma, model, stim, dur, xlim = (2, 24, 24, 0), (1.75, 1.75, 1.75), cmx.LR91, -400000.0, 0.5, 0.02
+        omodel = oracle.LR91_CELLML
+    else:
+        dims, sigma, model, stim, dur, xlim = (3, 16, 6, 4), (1.75, 0.19, 0.19), cmx.TT06_EPI, -100000.0, 1.0, 0.03
+        omodel = oracle.TT06_EPI
+    steps = 300
+    m = make(*dims, model, cmx.BACKWARD_EULER, stim, dur, xlim, precision=1 if mixed else 0, sigma=sigma,
+             rtol=1e-8 if mixed else 1e-12, refine=1 if mixed else 0, dt_ode=0.005, use_graph=True,
+             extrapolate=2, split=split)
+    m.step(0.0, steps)
+    ref = oracle.mono_run(*dims, model=omodel, backward_euler=True, sigma=sigma, stim_mag=stim, stim_dur=dur,
+                          stim_xlim=xlim, t_end=steps * 0.01, dt_ode=0.005, ksp_rtol=1e-12, want_state=True,
+                          split=split)
+    V = m.voltage()
+    err = np.abs(V - ref['V']).max()
+    print('backward euler max |dV|', err, 'rel', err / np.abs(ref['V']).max(), 'V range', ref['V'].min(),
+          ref['V'].max())
+    assert ref['V'].max() > 0.0          # the wave is under way
+    if mixed:
+        assert err <= MIXED_MV
+    else:
+        assert err / np.abs(ref['V']).max() <= FP64_REL
+        S = m.state()
+        serr = np.abs(S - ref['state']).max(axis=1) / np.maximum(np.abs(ref['state']).max(axis=1), 1e-30)
+        assert serr.max() <= 1e-8
+
+
+def test_backward_euler_splitting_reference_value():
+    """TestOperatorSplittingMonodomainSolver.hpp:97-146 on the device: the
+    1 cm cable with backward-Euler LR91 cells, operator splitting and the
+    default absolute KSP tolerance 2e-4 gives V = 5.0567 +- 1e-3 mV at node
+    30 after 4 ms."""
+    from chaste_amd import cmx
+    print('case backward euler splitting reference value', flush=True)
+    for precision in (cmx.FP64, cmx.MIXED):
+        m = make(1, 100, 0, 0, cmx.LR91, cmx.BACKWARD_EULER, -1e6, 0.5, 0.02 + 1e-6 + 1e-12, precision=precision,
+                 rtol=0.0, atol=2e-4, dt_ode=0.005, use_graph=True, split=True, refine=precision)
+        m.step(0.0, 400)
+        v30 = m.voltage()[30]
+        print('precision', precision, 'V[30]', v30)
+        assert abs(v30 - 5.0567) <= 1e-3
+
+
 def test_ell_matches_stencil():
     from chaste_amd import cmx
     print('case ell vs stencil', flush=True)

@@ -316,18 +316,96 @@ def test_operator_splitting_closed_form(oracle, model, rl, dt_ode, lumped):
     assert np.abs(ici['V'] - r['V']).max() > 1e-6
 
 
-def test_operator_splitting_near_ici(oracle):
+@pytest.mark.parametrize('be', [True, False], ids=['backward_euler', 'forward_euler'])
+def test_operator_splitting_reference_cable(oracle, be):
     """heart/test/monodomain/TestOperatorSplittingMonodomainSolver.hpp:97-172
-    criteria on its cable (1 cm, h 0.01, stimulus -1e6 uA/cm^3 for 0.5 ms on
-    x <= 0.02, dt_ode 0.005, dt 0.01, 4 ms): the split and ICI wavefronts lie
-    within 25 mV of each other everywhere, differ wherever V > -80 mV, and
-    some node is depolarised above 0 mV.  (The reference runs the
-    backward-Euler LR91 variant; this runs forward Euler.)"""
+    on its cable (1 cm, h 0.01, stimulus -1e6 uA/cm^3 for 0.5 ms on
+    x <= 0.02, dt_ode 0.005, dt 0.01, 4 ms, default absolute KSP tolerance
+    2e-4): with the backward-Euler LR91 cells of the reference the split
+    voltage at node 30 is 5.0567 +- 1e-3 mV (a forward-Euler cell gives
+    about 10.8); split and ICI wavefronts lie within 25 mV of each other,
+    differ wherever V > -80 mV, and some node is depolarised above 0 mV."""
     kw = dict(h=0.01, model=oracle.LR91_CELLML, sigma=(1.75, 1.75, 1.75), dt_pde=0.01, dt_ode=0.005,
-              stim_mag=-1e6, stim_dur=0.5, stim_xlim=0.02 + 1e-6 + 1e-12, t_end=4.0, ksp_rtol=1e-10)
+              stim_mag=-1e6, stim_dur=0.5, stim_xlim=0.02 + 1e-6 + 1e-12, t_end=4.0, ksp_rtol=0.0, ksp_atol=2e-4,
+              backward_euler=be)
     ici = oracle.mono_run(1, 100, **kw)['V']
     split = oracle.mono_run(1, 100, split=True, **kw)['V']
+    if be:
+        assert abs(split[30] - 5.0567) <= 1e-3
+    else:
+        assert abs(split[30] - 5.0567) > 1.0
     assert np.abs(ici - split).max() <= 25.0
     up = ici > -80.0
     assert up.any() and (ici[up] != split[up]).all()
     assert (ici > 0.0).any()
+
+
+def _compare_like_reference(v_fine, v_coarse, tolerance):
+    """CompareCellModelResults (heart/src/fortests/RunAndCheckIonicModels.hpp:
+    224-283) on voltages: tolerance, x25 where the coarse trace moved by more
+    than 0.05 mV since its previous sample."""
+    last = v_coarse[0]
+    for a, b in zip(v_fine, v_coarse):
+        tol = tolerance * 25 if abs(b - last) > 0.05 else tolerance
+        last = b
+        assert abs(a - b) <= tol, (a, b, tol)
+
+
+@pytest.mark.parametrize('dt,tolerance', [(0.01, 0.01), (0.1, 0.25)], ids=['dt0.01', 'dt0.1'])
+def test_lr91_backward_euler(oracle, dt, tolerance):
+    """TestIonicModels.hpp:503-570: backward-Euler LR91 against the forward
+    Euler trace (here the Lr91DelayedStim ValidData file, which the
+    forward-Euler oracle matches to 1e-3 mV) at the reference's tolerances
+    and sampling: every 1 ms at dt 0.01 (a 1 ms sampling interval), every
+    100th step (10 ms) at dt 0.1."""
+    _, ref = load_dat('Lr91DelayedStimValidData.dat')
+    y = np.array(LR91_INIT)
+    out = oracle.cell_compute(oracle.LR91_CELLML, y, 0.0, 1000.0, dt, stim=(-25.5, 2.0, 50.0, -1.0),
+                              steps_per_sample=100, backward_euler=True)
+    ref = ref[::int(round(dt * 100))]
+    assert out.shape[0] == ref.shape[0]
+    _compare_like_reference(ref[:, 1], out[:, 0], tolerance)
+    # the implicit step differs from forward Euler by more than round-off
+    assert np.abs(out[:, 0] - ref[:, 1]).max() > 1e-3
+    fe = np.array(LR91_INIT)
+    be = np.array(LR91_INIT)
+    assert abs(oracle.cell_iionic(oracle.LR91_CELLML, fe) - oracle.cell_iionic(oracle.LR91_CELLML, be)) < 1e-3
+
+
+def test_tt06_backward_euler(oracle):
+    """TestIonicModels.hpp:838-882: TT06 backward Euler at dt 0.001 for 50 ms
+    (stimulus -1800 pA/pF for 0.05 ms at 5 ms): GetIIonic -0.0413 +- 1e-3 at
+    the end, and V within 0.03 mV (x25 where it moves) of forward Euler, both
+    written every 100th step."""
+    stim = (-1800.0, 0.05, 5.0, -1.0)
+    yb = tt06_initial()
+    be = oracle.cell_compute(oracle.TT06_EPI, yb, 0.0, 50.0, 0.001, stim=stim, backward_euler=True,
+                             steps_per_sample=100)
+    assert abs(oracle.cell_iionic(oracle.TT06_EPI, yb) - (-0.0413)) < 1e-3
+    yf = tt06_initial()
+    fe = oracle.cell_compute(oracle.TT06_EPI, yf, 0.0, 50.0, 0.001, stim=stim, steps_per_sample=100)
+    assert be.shape == fe.shape
+    _compare_like_reference(fe[:, 0], be[:, 0], 0.03)
+    assert be[:, 0].max() > 20.0          # it fired
+
+
+def test_backward_euler_newton_converges(oracle):
+    """One backward-Euler step from the TT06 initial state with dt large
+    enough to need several Newton iterations satisfies the implicit
+    equation: the nonlinear states equal y0 + dt f(y1) at the converged
+    point (checked against the forward-Euler right-hand side evaluated at
+    the result through a zero-length FE step difference)."""
+    y0 = tt06_initial()
+    y1 = y0.copy()
+    oracle.cell_compute(oracle.TT06_EPI, y1, 0.0, 0.5, 0.5, backward_euler=True)
+    # f(y1) from a tiny forward-Euler step at y1 with V and gates as in y1
+    h = 1e-7
+    yh = y1.copy()
+    oracle.cell_compute(oracle.TT06_EPI, yh, 0.5, 0.5 + h, h)
+    f = (yh - y1) / h
+    names = tt06_state_names()
+    for nm in ('calcium_dynamics__Ca_i', 'calcium_dynamics__Ca_SR', 'calcium_dynamics__Ca_ss',
+               'sodium_dynamics__Na_i', 'potassium_dynamics__K_i', 'calcium_dynamics__R_prime'):
+        i = names.index(nm)
+        resid = y1[i] - y0[i] - 0.5 * f[i]
+        assert abs(resid) <= 1e-6 * max(1.0, abs(y1[i])) + 1e-9 * abs(f[i]), nm
