@@ -14,7 +14,7 @@ FP64, MIXED = 0, 1
 LR91, TT06_EPI = 0, 1
 FORWARD_EULER, RUSH_LARSEN, BACKWARD_EULER = 0, 1, 2
 FMT_AUTO, FMT_STENCIL, FMT_ELL = 0, 1, 2
-KERNEL_KINDS = ['cell', 'residual', 'spmv', 'update', 'pupdate', 'finish']
+KERNEL_KINDS = ['cell', 'residual', 'spmv', 'update', 'pupdate', 'finish', 'svi']
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -44,7 +44,8 @@ class CmxOpts(ctypes.Structure):
                 ('ksp_rtol', ctypes.c_double), ('ksp_atol', ctypes.c_double),
                 ('ksp_max_its', ctypes.c_int32), ('refine_sweeps', ctypes.c_int32),
                 ('use_graph', ctypes.c_int32), ('mass_lumping', ctypes.c_int32),
-                ('extrapolate', ctypes.c_int32), ('operator_splitting', ctypes.c_int32)]
+                ('extrapolate', ctypes.c_int32), ('operator_splitting', ctypes.c_int32),
+                ('state_variable_interpolation', ctypes.c_int32)]
 
 
 class CmxStats(ctypes.Structure):
@@ -153,7 +154,7 @@ class Monodomain:
                  stimuli=(), stim_index=None, params=None, precision=FP64,
                  dt_pde=0.01, dt_ode=0.01, am=1400.0, cm=1.0, ksp_rtol=1e-8, ksp_atol=0.0,
                  ksp_max_its=1000, refine_sweeps=0, fmt=FMT_AUTO, mass_lumping=False,
-                 sigma_elem=None, use_graph=False, extrapolate=False, operator_splitting=False):
+                 sigma_elem=None, use_graph=False, extrapolate=False, operator_splitting=False, svi=False):
         L = lib()
         self._keep = []
         coords = np.ascontiguousarray(coords, dtype=np.float64)
@@ -192,7 +193,8 @@ class Monodomain:
             self._keep.append(pa)
             t.n_params, t.params = pa.shape[0], _p(pa)
         o = CmxOpts(precision, fmt, dt_pde, dt_ode, am, cm, ksp_rtol, ksp_atol, ksp_max_its,
-                    refine_sweeps, 1 if use_graph else 0, 1 if mass_lumping else 0, int(extrapolate), 1 if operator_splitting else 0)
+                    refine_sweeps, 1 if use_graph else 0, 1 if mass_lumping else 0, int(extrapolate), 1 if operator_splitting else 0,
+                    1 if svi else 0)
         self.dt = dt_pde
         ctx = ctypes.c_void_p()
         rc = L.cmx_create(ctypes.byref(m), ctypes.byref(t), ctypes.byref(o), ctypes.byref(ctx))

@@ -192,6 +192,7 @@ struct CellArgs {
     const uint8_t* __restrict__ stim_id;  // 0 = none, k = row k-1
     const cmx_stimulus* __restrict__ stim;
     RV* __restrict__ F;        // out: -Am Iion - Istim(t_next)
+    RV* __restrict__ iion_out; // out (state-variable interpolation): Iion, or null
     SettleArgs<RV> s;          // the previous step, when pending
     int n_half;                // operator splitting: ODE substeps per half step
     // CG initial guess V^n + x0 (x0 = the previous step's correction), when s.guess is set:
@@ -393,6 +394,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         const RS iion = Model::template iion<RS>(y, V, p, cm);
         const double istim = sid ? stimulus_at(a.stim[sid - 1], t_next) : 0.0;
         f = (RV)(-a.am * (double)iion - istim);
+        if (a.iion_out) a.iion_out[i] = (RV)iion;
     }
     a.F[i] = f;
     if (a.s.guess) {
@@ -476,6 +478,109 @@ struct CgVecs {
 // exception (it decides from the reduction first, so that launches after
 // convergence cost no row traffic).
 
+// State-variable interpolation (AbstractCorrectionTermAssembler.cpp:98-156,
+// MonodomainCorrectionTermAssembler.cpp:48-74).  Every element whose nodal
+// ionic currents differ pairwise by more than 1 uA/cm^2 adds, per quadrature
+// point, w |J| phi_a (-Am) (Iion(sum_j phi_j y_j) - sum_j phi_j Iion_j) to the
+// right-hand side of its node a (the interpolated state includes V; the cell
+// parameters are those of the element's first node).
+template <class RS, class RV>
+struct SviArgs {
+    int n, nb, nq;
+    double am, cm;
+    const StepClock* clock;
+    const int32_t* __restrict__ elems;     // [ne][nb]
+    const double* __restrict__ edet;       // [ne] |det J|
+    const int32_t* __restrict__ inc_ptr;   // [n+1] node -> incident (element, local) list
+    const int32_t* __restrict__ inc;       //   entries 4 e + local, ascending e
+    const RV* __restrict__ iion;           // nodal Iion after the cell step
+    const double* __restrict__ V;          // V^n
+    const RS* __restrict__ state;          // [NS-1][ps]
+    int ps;
+    const RS* __restrict__ params;
+    const RS* __restrict__ pdef;
+    int n_param_planes;
+    double qw[4];
+    double qp[4][3];
+    RV* __restrict__ out;                  // correction added to b
+};
+
+// Node a's share of element e's correction (0 when the element is below the
+// criterion), in the reference's accumulation order.
+template <class Model, class RS, class RV>
+__device__ __forceinline__ double svi_element_term(const SviArgs<RS, RV>& a, int e, int loc)
+{
+    constexpr int NS = Model::NS;
+    constexpr int NP = Model::NP > 0 ? Model::NP : 1;
+    int nd[4];
+    double ii[4];
+    for (int j = 0; j < a.nb; ++j) {
+        nd[j] = a.elems[(size_t)e * a.nb + j];
+        ii[j] = (double)a.iion[nd[j]];
+    }
+    double di = 0;
+    for (int j = 0; j < a.nb; ++j)
+        for (int k = j + 1; k < a.nb; ++k) di = fmax(di, fabs(ii[j] - ii[k]));
+    if (!(di > 1.0)) return 0.0;
+    const double det = a.edet[e];
+    RS p[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+        p[q] = (q < a.n_param_planes) ? a.params[(size_t)q * a.ps + nd[0]] : (Model::NP > 0 ? a.pdef[q] : RS(0));
+    double be = 0.0;
+    for (int q = 0; q < a.nq; ++q) {
+        double phi[4], sum = 0;
+        for (int c = 0; c < a.nb - 1; ++c) {
+            phi[c + 1] = a.qp[q][c];
+            sum += a.qp[q][c];
+        }
+        phi[0] = 1.0 - sum;
+        double yq[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) yq[s] = 0.0;
+        double iq = 0.0;
+        for (int j = 0; j < a.nb; ++j) {
+            iq += phi[j] * ii[j];
+            yq[0] += phi[j] * a.V[nd[j]];
+#pragma unroll
+            for (int s = 1; s < NS; ++s) yq[s] += phi[j] * (double)a.state[(size_t)(s - 1) * a.ps + nd[j]];
+        }
+        RS yr[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) yr[s] = (RS)yq[s];
+        const double isv = (double)Model::template iion<RS>(yr, yr[0], p, (RS)a.cm);
+        const double wJ = a.qw[q] * det;
+        be += phi[loc] * (-a.am) * (isv - iq) * wJ;
+    }
+    return be;
+}
+
+// One thread per node.  A node takes part only if some element through it
+// passes the criterion, so the range of Iion over its matrix row (every
+// node it shares an element with) is checked first.
+template <class Model, class RS, class RV, class Rows>
+__global__ __launch_bounds__(kBlock) void svi_kernel(Rows R, SviArgs<RS, RV> a)
+{
+    const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (i >= a.n || a.clock->poison) return;
+    const auto row = R.row(i);
+    double lo = (double)a.iion[i], hi = lo;
+    for (int k = 0; k < R.width(); ++k) {
+        const double v = (double)a.iion[R.col(row, k)];
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+    }
+    double acc = 0.0;
+    if (hi - lo > 1.0) {
+        const int q1 = a.inc_ptr[i + 1];
+        for (int q = a.inc_ptr[i]; q < q1; ++q) {
+            const int code = a.inc[q];
+            acc += svi_element_term<Model, RS, RV>(a, code >> 2, code & 3);
+        }
+    }
+    a.out[i] = (RV)acc;
+}
+
 template <class RV>
 struct ResidualArgs {
     int n;
@@ -486,6 +591,7 @@ struct ResidualArgs {
     int refined;
     const double* __restrict__ Z;      // V^n, or V^n + x0 with an initial guess
     const RV* __restrict__ G;          // F, or F - c x0 with an initial guess
+    const RV* __restrict__ svi;        // state-variable interpolation correction of b, or null
     double* __restrict__ r0_64;        // FP64 residual, kept for the refinement sweep
     const RV* __restrict__ x1;         // refinement sweep: the first solve's correction
     CgVecs<RV> v;
@@ -494,7 +600,8 @@ struct ResidualArgs {
 };
 
 // r0 = b - A Z for the solve A d = r0 (V^{n+1} = Z + d) with
-// b = M (c V^n + F) and A = c M + K (c = Am Cm / dt).  Z = V^n is the
+// b = M (c V^n + F) (+ the state-variable interpolation correction S,
+// MonodomainSolver.cpp:122-127) and A = c M + K (c = Am Cm / dt).  Z = V^n is the
 // reference's warm start (KSPSetInitialGuessNonzero from V^n); with an
 // initial guess Z = V^n + x0 and G = F - c x0, otherwise G = F, and in both
 // cases r0 = M G - K Z and b = M (G + c Z).  u0 = D^-1 r0 goes to p for the
@@ -506,7 +613,7 @@ template <class Rows, class RV>
 struct ResidualRow {
     typename Rows::Row row;
     int i;
-    double zi;
+    double zi, si;
     Gathered<Rows, double> Z;
     Gathered<Rows, RV> G;
     __device__ __forceinline__ void load(const Rows& R, int i_, const ResidualArgs<RV>& a)
@@ -514,6 +621,7 @@ struct ResidualRow {
         i = i_;
         row = R.row(i);
         zi = a.Z[i];
+        si = a.svi ? (double)a.svi[i] : 0.0;
         Z.load(R, row, a.Z);
         G.load(R, row, a.G);
     }
@@ -531,8 +639,8 @@ struct ResidualRow {
             kz += Kv[s] * (zj - zi);
         }
         const int d = R.didx(row);
-        const double bi = Dv[d] * (a.c * mz + mg);
-        const double r64 = mg - kz;
+        const double bi = Dv[d] * (a.c * mz + mg + si);
+        const double r64 = mg - kz + si;
         if (a.r0_64) a.r0_64[i] = r64;
         const RV ri = (RV)r64;
         const RV ui = D32[d] * ri;
@@ -995,6 +1103,16 @@ struct cmx_ctx {
     void* G1 = nullptr;          // F - c x0
     double* r0_64 = nullptr;     // FP64 step residual (refinement)
     double* iion = nullptr;      // scratch for cmx_get_iionic
+    // state-variable interpolation
+    int svi = 0;
+    void* svi_iion = nullptr;    // nodal Iion of the step (RV)
+    void* svi_out = nullptr;     // correction of b (RV)
+    int32_t* elems = nullptr;
+    double* edet = nullptr;
+    int32_t* inc_ptr = nullptr;
+    int32_t* inc = nullptr;
+    double qw[4] = {}, qp[4][3] = {};
+    int nq = 0;
     double* act = nullptr;       // [5][n] activation times (cmx_set_activation)
     double act_thr = 0.0;
     int act_every = 0;
@@ -1280,6 +1398,7 @@ int launch_cells(cmx_ctx* c, int mode, const double* verdict, int kind)
     a.stim_id = c->stim_id;
     a.stim = c->stim;
     a.F = (RV*)c->F;
+    a.iion_out = (c->svi && mode == kModeIci) ? (RV*)c->svi_iion : nullptr;
     a.s = settle_args<RV>(c, verdict);
     a.c = c->am * c->cm / c->dt;
     a.Y = c->Y;
@@ -1287,6 +1406,39 @@ int launch_cells(cmx_ctx* c, int mode, const double* verdict, int kind)
     const int G = grid_of(c->n);
     if (c->model == CMX_MODEL_LR91) launch_cell_integrator<ModelLr91, RS, RV>(c, mode, kind, G, a);
     else launch_cell_integrator<ModelTt06, RS, RV>(c, mode, kind, G, a);
+    CMX_HIP(c, hipGetLastError());
+    return CMX_OK;
+}
+
+template <class Rows, class RS, class RV>
+int launch_svi(cmx_ctx* c, const Rows& R)
+{
+    SviArgs<RS, RV> a;
+    a.n = c->n;
+    a.nb = c->dim + 1;
+    a.nq = c->nq;
+    a.am = c->am;
+    a.cm = c->cm;
+    a.clock = c->clock;
+    a.elems = c->elems;
+    a.edet = c->edet;
+    a.inc_ptr = c->inc_ptr;
+    a.inc = c->inc;
+    a.iion = (const RV*)c->svi_iion;
+    a.V = c->V;
+    a.state = (const RS*)c->state;
+    a.ps = c->ps;
+    a.params = (const RS*)c->params;
+    a.pdef = (const RS*)c->pdef;
+    a.n_param_planes = c->n_param_planes;
+    for (int q = 0; q < 4; ++q) {
+        a.qw[q] = c->qw[q];
+        for (int k = 0; k < 3; ++k) a.qp[q][k] = c->qp[q][k];
+    }
+    a.out = (RV*)c->svi_out;
+    const int G = grid_of(c->n);
+    if (c->model == CMX_MODEL_LR91) CMX_LAUNCH(c, CMX_K_SVI, (svi_kernel<ModelLr91, RS, RV, Rows>), G, kBlock, R, a);
+    else CMX_LAUNCH(c, CMX_K_SVI, (svi_kernel<ModelTt06, RS, RV, Rows>), G, kBlock, R, a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
@@ -1303,6 +1455,7 @@ int launch_residual(cmx_ctx* c, const Rows& R, int solve)
     a.refined = refined_of(c) ? 1 : 0;
     a.Z = c->extrapolate ? c->Y : c->V;
     a.G = (const RV*)(c->extrapolate ? c->G1 : c->F);
+    a.svi = (c->svi && !c->split) ? (const RV*)c->svi_out : nullptr;
     a.r0_64 = c->r0_64;
     a.x1 = (const RV*)c->vec[V_X1];
     a.v = cg_vecs<RV>(c, solve, 0);
@@ -1385,6 +1538,7 @@ int enqueue_step(cmx_ctx* c, const Rows& R)
 {
     int rc;
     if ((rc = launch_cells<RS, RV>(c, c->split ? kModeSplit : kModeIci, verdict_buf(c), CMX_K_CELL))) return rc;
+    if (c->svi && !c->split && (rc = launch_svi<Rows, RS, RV>(c, R))) return rc;
     if ((rc = launch_residual<Rows, RV>(c, R, 0))) return rc;
     if ((rc = launch_cg_start<Rows, RV>(c, R, 0))) return rc;
     if ((rc = launch_cg_iters<Rows, RV>(c, R, 0, 0, c->budget))) return rc;
@@ -1596,6 +1750,67 @@ double host_stimulus(const cmx_stimulus& s, double t)
     return (s.start <= t && t <= s.duration + s.start) ? s.magnitude : 0.0;
 }
 
+// State-variable interpolation tables: element connectivity, |det J| per
+// element, each node's incident (element, local index) list in ascending
+// element order, and the second-order quadrature rule of the element
+// dimension (GaussianQuadratureRule(2), pde/src/solver/GaussianQuadratureRule.cpp:
+// 81-91, 126-135, 184-202).
+template <class VecAlloc>
+int setup_svi(cmx_ctx* c, const cmx_mesh* mesh, VecAlloc& rvec)
+{
+    int rc;
+    const int n = c->n, dim = c->dim, nb = dim + 1, ne = mesh->n_elems;
+    if ((rc = rvec(&c->svi_iion))) return rc;
+    if ((rc = rvec(&c->svi_out))) return rc;
+    std::vector<int32_t> ptr(n + 1, 0);
+    for (int e = 0; e < ne; ++e)
+        for (int k = 0; k < nb; ++k) ptr[mesh->elems[(size_t)e * nb + k] + 1] += 1;
+    for (int i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int32_t> inc(ptr[n]), fill(ptr.begin(), ptr.end() - 1);
+    std::vector<double> det(ne);
+    for (int e = 0; e < ne; ++e) {
+        const int32_t* nd = mesh->elems + (size_t)e * nb;
+        for (int k = 0; k < nb; ++k) inc[fill[nd[k]]++] = 4 * e + k;
+        double J[3][3] = {};
+        for (int r = 0; r < dim; ++r)
+            for (int q = 0; q < dim; ++q)
+                J[r][q] = mesh->coords[(size_t)nd[q + 1] * dim + r] - mesh->coords[(size_t)nd[0] * dim + r];
+        double d;
+        if (dim == 1) d = J[0][0];
+        else if (dim == 2) d = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        else
+            d = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        det[e] = std::fabs(d);
+    }
+    if ((rc = upload(c, &c->elems, mesh->elems, (size_t)ne * nb))) return rc;
+    if ((rc = upload(c, &c->edet, det.data(), det.size()))) return rc;
+    if ((rc = upload(c, &c->inc_ptr, ptr.data(), ptr.size()))) return rc;
+    if ((rc = upload(c, &c->inc, inc.data(), inc.size()))) return rc;
+    if (dim == 1) {
+        const double s = std::sqrt(1.0 / 3.0);
+        c->nq = 2;
+        c->qw[0] = c->qw[1] = 0.5;
+        c->qp[0][0] = (-s + 1.0) / 2.0;
+        c->qp[1][0] = (s + 1.0) / 2.0;
+    } else if (dim == 2) {
+        c->nq = 3;
+        c->qw[0] = c->qw[1] = c->qw[2] = 1.0 / 6.0;
+        const double pts[3][2] = {{2.0 / 3.0, 1.0 / 6.0}, {1.0 / 6.0, 1.0 / 6.0}, {1.0 / 6.0, 2.0 / 3.0}};
+        for (int q = 0; q < 3; ++q)
+            for (int k = 0; k < 2; ++k) c->qp[q][k] = pts[q][k];
+    } else {
+        const double s = std::sqrt(0.2);
+        const double a = (1.0 + 3.0 * s) / 4.0, b = (1.0 - s) / 4.0;
+        c->nq = 4;
+        for (int q = 0; q < 4; ++q) c->qw[q] = 1.0 / 24;
+        const double pts[4][3] = {{a, b, b}, {b, a, b}, {b, b, a}, {b, b, b}};
+        for (int q = 0; q < 4; ++q)
+            for (int k = 0; k < 3; ++k) c->qp[q][k] = pts[q][k];
+    }
+    return CMX_OK;
+}
+
 int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* opts, cmx_ctx* c)
 {
     if (!mesh || !tissue || !opts) return fail(c, CMX_E_ARG, "null mesh, tissue or opts");
@@ -1633,6 +1848,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->refine = opts->refine_sweeps;
     c->use_graph = opts->use_graph;
     c->split = opts->operator_splitting ? 1 : 0;
+    c->svi = opts->state_variable_interpolation ? 1 : 0;
     {
         // TimeStepper(0, dt/2, dt_ode) step count (global/src/TimeStepper.cpp:43-121)
         const double end = 0.5 * opts->dt_pde, eps = end > 1.0 ? DBL_EPSILON * end : DBL_EPSILON;
@@ -1803,6 +2019,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
         if ((rc = dalloc_vec(c, &c->Y, n))) return rc;
         if ((rc = rvec(&c->G1))) return rc;
     }
+    if (c->svi && !c->split && (rc = setup_svi(c, mesh, rvec))) return rc;
     if ((rc = dalloc(c, &c->partials, 3 * (size_t)kMaxBlocks * 3))) return rc;
     CMX_HIP(c, hipMemset(c->partials, 0, 3 * (size_t)kMaxBlocks * 3 * sizeof(double)));
     if ((rc = dalloc(c, &c->clock, 1))) return rc;

@@ -118,6 +118,9 @@ public:
     // HeartConfig.cpp:2874-2882 (MonodomainProblem.cpp:65-69 picks the splitting solver)
     void SetUseReactionDiffusionOperatorSplitting(bool on = true) { mSplitting = on; }
     bool GetUseReactionDiffusionOperatorSplitting() const { return mSplitting; }
+    // HeartConfig.cpp:2756-2779 (MonodomainSolver.cpp:200-209 builds the correction assembler)
+    void SetUseStateVariableInterpolation(bool on = true) { mSvi = on; }
+    bool GetUseStateVariableInterpolation() const { return mSvi; }
 
     // MI355X-specific knobs (no reference counterpart).
     void SetGpuPrecision(int precision) { mPrecision = precision; }
@@ -140,6 +143,7 @@ private:
     std::string mPc = "bjacobi";
     bool mLumping = false;
     bool mSplitting = false;
+    bool mSvi = false;
     int mPrecision = CMX_FP64;
     int mRefine = 1;
     int mMaxIts = 1000;
@@ -369,6 +373,7 @@ public:
         o.use_graph = 1;
         o.extrapolate = 0;
         o.operator_splitting = cfg->GetUseReactionDiffusionOperatorSplitting() ? 1 : 0;
+        o.state_variable_interpolation = cfg->GetUseStateVariableInterpolation() ? 1 : 0;
         if (mpCtx) { cmx_destroy(mpCtx); mpCtx = nullptr; }
         Check(cmx_create(&mesh, &tissue, &o, &mpCtx), nullptr);
         mCurrentTime = 0.0;

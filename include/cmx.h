@@ -99,6 +99,10 @@ typedef struct {
                                   Strang splitting, cells over half steps with V updated and a
                                   diffusion-only solve in between
                                   (OperatorSplittingMonodomainSolver.cpp:40-137) */
+    int32_t state_variable_interpolation; /* HeartConfig::SetUseStateVariableInterpolation:
+                                  SVI correction of the ICI right-hand side
+                                  (MonodomainCorrectionTermAssembler.cpp:48-74); not applied with
+                                  operator splitting, whose solver has none */
 } cmx_opts;
 
 typedef struct {
@@ -162,7 +166,7 @@ int cmx_get_stats(cmx_ctx* ctx, cmx_stats* stats);
  * of the reference's HeartEventHandler events Ode / AssRhs / Ksp
  * (global/src/timing/HeartEventHandler.hpp).  Times accumulate until reset. */
 enum { CMX_K_CELL = 0, CMX_K_RESIDUAL = 1, CMX_K_SPMV = 2, CMX_K_UPDATE = 3, CMX_K_PUPDATE = 4,
-       CMX_K_FINISH = 5, CMX_K_COUNT = 6 };
+       CMX_K_FINISH = 5, CMX_K_SVI = 6, CMX_K_COUNT = 7 };
 int cmx_set_profiling(cmx_ctx* ctx, int32_t mask);
 /* total_ms[CMX_K_COUNT], launches[CMX_K_COUNT]; reset != 0 clears them afterwards. */
 int cmx_get_kernel_times(cmx_ctx* ctx, double* total_ms, int64_t* launches, int32_t reset);

@@ -221,6 +221,87 @@ struct Stepper {
     void advance() { ++k; time = next; next = calc(); }
 };
 
+// Second-order Gaussian rules on the reference simplex (GaussianQuadratureRule
+// with order 2, pde/src/solver/GaussianQuadratureRule.cpp:81-91, 126-135,
+// 184-202): 2 points in 1-D, 3 in 2-D, 4 in 3-D.
+void quad_rule(int dim, std::vector<std::vector<double>>& qp, std::vector<double>& qw)
+{
+    if (dim == 1) {
+        const double s = std::sqrt(1.0 / 3.0);
+        qw = {0.5, 0.5};
+        qp = {{(-s + 1.0) / 2.0}, {(s + 1.0) / 2.0}};
+    } else if (dim == 2) {
+        qw = {1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0};
+        qp = {{2.0 / 3.0, 1.0 / 6.0}, {1.0 / 6.0, 1.0 / 6.0}, {1.0 / 6.0, 2.0 / 3.0}};
+    } else {
+        const double s = std::sqrt(0.2);
+        const double a = (1.0 + 3.0 * s) / 4.0, b = (1.0 - s) / 4.0;
+        qw = {1.0 / 24, 1.0 / 24, 1.0 / 24, 1.0 / 24};
+        qp = {{a, b, b}, {b, a, b}, {b, b, a}, {b, b, b}};
+    }
+}
+
+// |det J| of a simplex (J[r][c] = x_{c+1,r} - x_{0,r}).
+double simplex_det(int dim, const double* coords, const int* nd)
+{
+    double J[3][3] = {};
+    for (int r = 0; r < dim; ++r)
+        for (int c = 0; c < dim; ++c) J[r][c] = coords[nd[c + 1] * dim + r] - coords[nd[0] * dim + r];
+    double det;
+    if (dim == 1) det = J[0][0];
+    else if (dim == 2) det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    else
+        det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+              J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    return std::fabs(det);
+}
+
+// State-variable interpolation correction of the right-hand side
+// (AbstractCorrectionTermAssembler.cpp:98-156, MonodomainCorrectionTermAssembler.cpp:
+// 48-74): on every element whose nodal ionic currents differ pairwise by more
+// than 1 uA/cm^2, each quadrature point adds
+//   w |J| phi_a (-Am) (Iion(sum_j phi_j y_j) - sum_j phi_j Iion_j)
+// to node a; the interpolated state includes V and takes the cell parameters
+// of the element's first node.  Y: state-major planes (plane 0 unused), V.
+void svi_correction(int dim, int n, const double* coords, int ne, const int* elems, int model, double am, double cm,
+                    const double* Y, const double* V, const double* iion, const double* params, int n_param,
+                    double* out)
+{
+    const int nb = dim + 1;
+    const int S = num_states(model), P = num_params(model);
+    const double* pdef = default_params(model);
+    std::vector<std::vector<double>> qp;
+    std::vector<double> qw;
+    quad_rule(dim, qp, qw);
+    std::fill(out, out + n, 0.0);
+    for (int e = 0; e < ne; ++e) {
+        const int* nd = elems + (size_t)e * nb;
+        double di = 0;
+        for (int a = 0; a < nb; ++a)
+            for (int b = a + 1; b < nb; ++b) di = std::max(di, std::fabs(iion[nd[a]] - iion[nd[b]]));
+        if (!(di > 1.0)) continue;
+        const double adet = simplex_det(dim, coords, nd);
+        double p[64];
+        for (int q = 0; q < P; ++q) p[q] = (params && q < n_param) ? params[(size_t)q * n + nd[0]] : pdef[q];
+        double be[4] = {};
+        for (size_t q = 0; q < qw.size(); ++q) {
+            double phi[4], sum = 0;
+            for (int c = 0; c < dim; ++c) { phi[c + 1] = qp[q][c]; sum += qp[q][c]; }
+            phi[0] = 1.0 - sum;
+            double yq[64] = {}, ii = 0;
+            for (int a = 0; a < nb; ++a) {
+                ii += phi[a] * iion[nd[a]];
+                yq[0] += phi[a] * V[nd[a]];
+                for (int s2 = 1; s2 < S; ++s2) yq[s2] += phi[a] * Y[(size_t)s2 * n + nd[a]];
+            }
+            const double isv = cell_iionic(model, p, cm, yq);
+            const double wJ = qw[q] * adet;
+            for (int a = 0; a < nb; ++a) be[a] += phi[a] * (-am) * (isv - ii) * wJ;
+        }
+        for (int a = 0; a < nb; ++a) out[nd[a]] += be[a];
+    }
+}
+
 double seconds_since(std::chrono::steady_clock::time_point t0)
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -379,19 +460,7 @@ int orc_assemble(int dim, int n, const double* coords, int ne, const int* elems,
 
     std::vector<std::vector<double>> qp;
     std::vector<double> qw;
-    if (dim == 1) {
-        const double s = std::sqrt(1.0 / 3.0);
-        qw = {0.5, 0.5};
-        qp = {{(-s + 1.0) / 2.0}, {(s + 1.0) / 2.0}};
-    } else if (dim == 2) {
-        qw = {1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0};
-        qp = {{2.0 / 3.0, 1.0 / 6.0}, {1.0 / 6.0, 1.0 / 6.0}, {1.0 / 6.0, 2.0 / 3.0}};
-    } else {
-        const double s = std::sqrt(0.2);
-        const double a = (1.0 + 3.0 * s) / 4.0, b = (1.0 - s) / 4.0;
-        qw = {1.0 / 24, 1.0 / 24, 1.0 / 24, 1.0 / 24};
-        qp = {{a, b, b}, {b, a, b}, {b, b, a}, {b, b, b}};
-    }
+    quad_rule(dim, qp, qw);
 
     auto find = [&](int r, int c) {
         const int* b = col + row_ptr[r];
@@ -573,10 +642,11 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
     const double* y0 = initial_state(model);
     for (int s = 0; s < S; ++s)
         for (int i = 0; i < n; ++i) Y[(size_t)s * n + i] = y0[s];
-    std::vector<double> V(n), Vprev(n), iion(n), istim(n), z(n), b(n);
+    std::vector<double> V(n), Vprev(n), iion(n), istim(n), z(n), b(n), svi(cfg->svi ? n : 0);
     for (int i = 0; i < n; ++i) V[i] = y0[0];
     std::vector<char> stimulated(n);
-    for (int i = 0; i < n; ++i) stimulated[i] = coords[(size_t)i * dim] < cfg->stim_xlim;
+    for (int i = 0; i < n; ++i)
+        stimulated[i] = cfg->stim_mask ? cfg->stim_mask[i] != 0 : coords[(size_t)i * dim] < cfg->stim_xlim;
     Stimulus stim(cfg->stim_mag, cfg->stim_dur, cfg->stim_start, cfg->stim_period > 0 ? cfg->stim_period : -1.0);
     // ActivationOutputModifier::ProcessSolutionAtTimeStep
     // (heart/src/problem/ActivationOutputModifier.cpp:94-125), called with the
@@ -708,6 +778,12 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
             double s = 0;
             for (int k = rp[i]; k < rp[i + 1]; ++k) s += M[k] * z[col[k]];
             b[i] = s;
+        }
+        if (cfg->svi) {
+            // MonodomainSolver.cpp:122-127: the correction is added to b = M z
+            svi_correction(dim, n, coords.data(), ne, elems.data(), model, am, cm, Y.data(), V.data(), iion.data(),
+                           cfg->params, cfg->n_param, svi.data());
+            for (int i = 0; i < n; ++i) b[i] += svi[i];
         }
         st.t_rhs += seconds_since(c0);
         c0 = std::chrono::steady_clock::now();

@@ -104,6 +104,9 @@ typedef struct {
     int split;                /* HeartConfig::SetUseReactionDiffusionOperatorSplitting: Strang
                                  splitting, cells (V updated) over each half step around a
                                  diffusion-only solve */
+    int svi;                  /* HeartConfig::SetUseStateVariableInterpolation (ICI step only,
+                                 as the reference's splitting solver has no correction) */
+    const unsigned char* stim_mask;  /* [n] stimulated nodes, or NULL for x < stim_xlim */
 } orc_mono_cfg;
 
 typedef struct {

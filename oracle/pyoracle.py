@@ -29,7 +29,7 @@ class MonoCfg(ctypes.Structure):
                 ('max_its', ctypes.c_int), ('threads', ctypes.c_int), ('n_param', ctypes.c_int), ('params', _dp),
                 ('lumped', ctypes.c_int), ('sigma_elem', _dp), ('stim_period', ctypes.c_double),
                 ('modifier_out', _dp), ('modifier_threshold', ctypes.c_double), ('modifier_every', ctypes.c_int),
-                ('split', ctypes.c_int)]
+                ('split', ctypes.c_int), ('svi', ctypes.c_int), ('stim_mask', ctypes.POINTER(ctypes.c_ubyte))]
 
 
 class MonoStats(ctypes.Structure):
@@ -147,7 +147,8 @@ def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, si
              am=1400.0, cm=1.0, dt_pde=0.01, dt_ode=0.01, stim_mag=-3e6, stim_dur=0.5, stim_start=0.0,
              stim_xlim=1e-5, t_end=1.0, ksp_rtol=1e-12, ksp_atol=0.0, max_its=10000, threads=0,
              params=None, trace_nodes=None, act_threshold=None, want_state=False, lumped=False, sigma_elem=None,
-             stim_period=0.0, modifier_threshold=None, modifier_every=1, split=False, backward_euler=False):
+             stim_period=0.0, modifier_threshold=None, modifier_every=1, split=False, backward_euler=False,
+             svi=False, stim_mask=None):
     """Full FP64 monodomain run (AbstractCardiacProblem::Solve equivalent)."""
     n = [ex + 1, (ex + 1) * (ey + 1), (ex + 1) * (ey + 1) * (ez + 1)][dim - 1]
     pa = None
@@ -157,11 +158,12 @@ def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, si
         npar = pa.shape[0]
     se = None if sigma_elem is None else np.ascontiguousarray(sigma_elem, dtype=np.float64).reshape(-1)
     mod = None if modifier_threshold is None else np.zeros((4, n))
+    mask = None if stim_mask is None else np.ascontiguousarray(stim_mask, dtype=np.uint8)
     cfg = MonoCfg(dim, ex, ey, ez, h, model, BE if backward_euler else int(rush_larsen), (ctypes.c_double * 3)(*sigma), am, cm, dt_pde, dt_ode,
                   stim_mag, stim_dur, stim_start, stim_xlim, t_end, ksp_rtol, ksp_atol, max_its, threads, npar,
                   _p(pa), int(lumped), _p(se), float(stim_period), _p(mod),
                   0.0 if modifier_threshold is None else float(modifier_threshold), int(modifier_every),
-                  int(split))
+                  int(split), int(svi), _p(mask, ctypes.POINTER(ctypes.c_ubyte)))
     V = np.zeros(n)
     S = num_states(model)
     state = np.zeros((S, n)) if want_state else None

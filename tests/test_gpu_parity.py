@@ -362,6 +362,68 @@ def test_backward_euler_splitting_reference_value():
         assert abs(v30 - 5.0567) <= 1e-3
 
 
+def _niederer(precision, t_ms, rtol, atol, print_every=None, extrapolate=2):
+    """The Niederer et al. 2011 benchmark at h 0.05 cm on the device (see
+    test_oracle_golden.niederer_setup): returns the final V and, with
+    print_every, V every print_every steps from t = 0."""
+    from chaste_amd import cmx
+    from test_oracle_golden import niederer_setup
+    from oracle import pyoracle
+    coords, elems = cmx.slab_mesh(3, 40, 14, 6, 0.05)
+    _, mask, sigma = niederer_setup(pyoracle)
+    stim_index = np.where(mask, 0, -1).astype(np.int32)
+    m = cmx.Monodomain(coords, elems, np.array(sigma), cmx.TT06_EPI, cmx.BACKWARD_EULER,
+                       stimuli=[(-50000.0, 0.0, 2.0)], stim_index=stim_index, precision=precision, ksp_rtol=rtol,
+                       ksp_atol=atol, refine_sweeps=precision, dt_pde=0.01, dt_ode=0.005, use_graph=True,
+                       extrapolate=extrapolate, svi=True)
+    steps = int(round(t_ms / 0.01))
+    if print_every is None:
+        m.step(0.0, steps)
+        return m, m.voltage(), None
+    printed = [m.voltage()]
+    for k in range(0, steps, print_every):
+        m.step(k * 0.01, print_every)
+        printed.append(m.voltage())
+    return m, printed[-1], np.array(printed)
+
+
+def test_svi_matches_oracle_fp64(oracle):
+    """State-variable interpolation on the device against the oracle over
+    the first 10 ms of the benchmark (the wave crosses the coarse mesh, so
+    many elements pass the criterion every step): 1e-9 relative, both
+    solvers warm-started from V^n as the reference is (measured: ~1e-13; an
+    extrapolated start moves the converged voltages within the rtol 1e-12
+    band, ~1e-9 at the wavefront)."""
+    from test_oracle_golden import niederer_setup
+    print('case svi fp64 parity', flush=True)
+    m, V, _ = _niederer(0, 10.0, 1e-12, 0.0, extrapolate=0)
+    coords, mask, sigma = niederer_setup(oracle)
+    ref = oracle.mono_run(3, 40, 14, 6, h=0.05, model=oracle.TT06_EPI, backward_euler=True, sigma=sigma, am=1400.0,
+                          cm=1.0, dt_pde=0.01, dt_ode=0.005, stim_mag=-50000.0, stim_dur=2.0, t_end=10.0,
+                          ksp_rtol=1e-12, svi=True, stim_mask=mask)['V']
+    err = np.abs(V - ref).max() / np.abs(ref).max()
+    print('svi rel err', err, 'its/step', m.stats()['cg_iterations'] / 1000.0, 'kernel ms', m.kernel_times())
+    assert (ref > 0).sum() > 100
+    assert err <= FP64_REL
+
+
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_niederer_benchmark_gpu(precision):
+    """TestNiederer2011BenchmarkSimulation.hpp:282-292 on the device: the
+    activation map over 40 ms (V printed every 0.1 ms) against the
+    reference's data file, 1.5e-3 ms in FP64 (its tolerance) and 0.1 ms
+    mixed."""
+    from test_oracle_golden import activation_map
+    print('case niederer benchmark', precision, flush=True)
+    _, _, printed = _niederer(precision, 40.0, 0.0, 2e-4, print_every=10)
+    act = activation_map(printed, 0.1)
+    gold = np.loadtxt(os.path.join(GOLDEN, 'niederer_activation_h0.05_dt0.01.dat'))
+    d = np.abs(act - gold)
+    print('niederer activated', (act >= 0).sum(), 'of', act.size, 'max diff', d.max(), 'mean', d.mean())
+    assert (act >= 0).all()
+    assert d.max() <= (1.5e-3 if precision == 0 else MIXED_ACT_MS)
+
+
 def test_ell_matches_stencil():
     from chaste_amd import cmx
     print('case ell vs stencil', flush=True)

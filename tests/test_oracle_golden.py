@@ -409,3 +409,56 @@ def test_backward_euler_newton_converges(oracle):
         i = names.index(nm)
         resid = y1[i] - y0[i] - 0.5 * f[i]
         assert abs(resid) <= 1e-6 * max(1.0, abs(y1[i])) + 1e-9 * abs(f[i]), nm
+
+
+def niederer_setup(oracle):
+    """TestNiederer2011BenchmarkSimulation.hpp:95-175 at h = 0.05 cm: slab
+    2.0 x 0.7 x 0.3 cm (41 x 15 x 7 nodes), TT06 backward Euler at dt_ode
+    0.005, dt 0.01, conductivities the harmonic means of 0.17/0.62 and
+    0.019/0.24 S/m in mS/cm, stimulus -50000 uA/cm^3 for 2 ms on the corner
+    cube x, y, z < 0.15 cm, SVI, default KSP absolute tolerance 2e-4."""
+    h = 0.05
+    coords, _ = oracle.slab(3, 40, 14, 6, h)
+    c = 0.15 + 1e-6
+    mask = (coords[:, 0] < c) & (coords[:, 1] < c) & (coords[:, 2] < c)
+    lc = 0.17 * 0.62 / (0.17 + 0.62) * 10
+    tc = 0.019 * 0.24 / (0.019 + 0.24) * 10
+    return coords, mask, (lc, tc, tc)
+
+
+def activation_map(printed, printing_dt):
+    """ConvertToActivationMap (TestNiederer2011BenchmarkSimulation.hpp:
+    177-215): first printed V > 0, interpolated against the last printed
+    V <= 0 as (timestep - V/(V - V_old)) * printing_dt."""
+    n = printed.shape[1]
+    act = np.full(n, -1.0)
+    last = np.ones(n)
+    for ts in range(printed.shape[0]):
+        V = printed[ts]
+        hit = (V > 0) & (act < 0)
+        act[hit] = (ts - V[hit] / (V[hit] - last[hit])) * printing_dt
+        neg = V <= 0
+        last[neg] = V[neg]
+    return act
+
+
+def test_niederer_benchmark_activation_times(oracle):
+    """The benchmark's activation map at h 0.05, dt 0.01 over 40 ms against
+    heart/test/data/benchmark_data_orig/activation_time_h0.05_dt0.01.dat to
+    1.5e-3 ms (the reference's tolerance).  This pins the backward-Euler
+    TT06 cells, state-variable interpolation and the slab together; without
+    SVI the coarse-mesh wave is too slow and a quarter of the nodes never
+    activate within 40 ms."""
+    coords, mask, sigma = niederer_setup(oracle)
+    n = coords.shape[0]
+    gold = np.loadtxt(os.path.join(GOLDEN, 'niederer_activation_h0.05_dt0.01.dat'))
+    assert gold.shape == (n,)
+    r = oracle.mono_run(3, 40, 14, 6, h=0.05, model=oracle.TT06_EPI, backward_euler=True, sigma=sigma, am=1400.0,
+                        cm=1.0, dt_pde=0.01, dt_ode=0.005, stim_mag=-50000.0, stim_dur=2.0, t_end=40.0, ksp_rtol=0.0,
+                        ksp_atol=2e-4, svi=True, stim_mask=mask, trace_nodes=np.arange(n))
+    tr = r['trace']
+    v0 = oracle.model_initial_state(oracle.TT06_EPI)[0]
+    printed = np.vstack([np.full((1, n), v0), tr[9::10]])    # t = 0, 0.1, ..., 40
+    act = activation_map(printed, 0.1)
+    assert (act >= 0).all()
+    assert np.abs(act - gold).max() <= 1.5e-3
