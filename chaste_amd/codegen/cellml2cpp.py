@@ -25,6 +25,9 @@ oracle includes the same header).  It provides
 * ``iionic<R>(y, V, p, cm)``: total ionic current in uA/cm^2, the quantity
   ``AbstractCardiacCellInterface::GetIIonic`` returns
   (``heart/src/odes/AbstractCardiacCellInterface.hpp:227-256``).
+* ``grl_step<R, ORDER>``: the generalized Rush-Larsen step of the
+  reference's ``--grl1`` / ``--grl2`` cells (forward-difference diagonal
+  Jacobian, ``heart/src/odes/AbstractGeneralizedRushLarsenCardiacCell.cpp``).
 * ``be_linear``, ``be_residual``, ``be_jacobian``, ``be_step``: the
   backward-Euler form of the reference's ``*BackwardEuler`` cell classes
   (``heart/src/odes/AbstractBackwardEulerCardiacCell.hpp:116-128, 300-330``):
@@ -778,6 +781,7 @@ class Generator:
         w('}')
         w('')
         self._backward_euler(w, rl, names, constants, state_ix, per_cap, body_lines)
+        self._generalized_rush_larsen(w)
         w('}}  // namespace cmx::%s' % ns)
         return '\n'.join(out) + '\n', rl
 
@@ -902,6 +906,82 @@ class Generator:
         w('CMX_UNROLL')
         w('    for (int i = 0; i < kNumNonlinear; ++i) y[kNonlinear[i]] = g[i];')
         w('    return its;')
+        w('}')
+        w('')
+
+    # --- Generalized Rush-Larsen (first and second order) ------------------
+    def _generalized_rush_larsen(self, w):
+        """grl_step<R, ORDER>: every state (and V when it is free) advanced by
+        y + f/a (exp(a dt) - 1), a the diagonal Jacobian entry by a forward
+        difference of step 1e-8, forward Euler where |a| < 1e-8
+        (heart/src/odes/AbstractGeneralizedRushLarsenCardiacCell.cpp:70-170,
+        ode/src/solver/GRL1IvpOdeSolver.cpp:53-90 and GRL2IvpOdeSolver.cpp:
+        53-118: the second-order form takes a half step, then re-evaluates
+        each state's rate and diagonal term at the midpoint state with that
+        state itself at its initial value)."""
+        w('// Rates of the full system: dV/dt (0 when V is clamped) and the states.')
+        w('template <class R>')
+        w('CMX_HD inline void grl_rates(const R* y, R i_stim_area, const R* p, R cm, bool update_v, R* f)')
+        w('{')
+        w('    R gi[kNumStates], gk[kNumStates];')
+        w('    evaluate<R, false>(y, y[0], i_stim_area, p, cm, f, gi, gk);')
+        w('    f[0] = update_v ? dvdt<R>(y, y[0], i_stim_area, p, cm) : R(0);')
+        w('}')
+        w('')
+        w('template <class R>')
+        w('CMX_HD inline R grl_advance(R y0, R f, R a, R dt)')
+        w('{')
+        w('    using std::fabs; using std::exp;')
+        w('    return fabs(a) < R(1e-8) ? y0 + f * dt : y0 + (f / a) * (exp(a * dt) - R(1));')
+        w('}')
+        w('')
+        w('// One generalized Rush-Larsen step of length dt (ORDER 1 or 2); V is')
+        w('// advanced too when update_v, else held.')
+        w('template <class R, int ORDER>')
+        w('CMX_HD inline void grl_step(R* y, R i_stim_area, const R* p, R cm, R dt, bool update_v)')
+        w('{')
+        w('    constexpr int S = kNumStates;')
+        w('    const R delta = R(1e-8);')
+        w('    R f[S], a[S], y0[S], t[S];')
+        w('    const int first = update_v ? 0 : 1;')
+        w('CMX_UNROLL')
+        w('    for (int i = 0; i < S; ++i) y0[i] = y[i];')
+        w('    grl_rates<R>(y, i_stim_area, p, cm, update_v, f);')
+        w('CMX_UNROLL')
+        w('    for (int i = 0; i < S; ++i) {')
+        w('        if (i < first) continue;')
+        w('        const R ys = y[i];')
+        w('        y[i] = ys + delta;')
+        w('        grl_rates<R>(y, i_stim_area, p, cm, update_v, t);')
+        w('        a[i] = (t[i] - f[i]) / delta;')
+        w('        y[i] = ys;')
+        w('    }')
+        w('    if constexpr (ORDER == 1) {')
+        w('CMX_UNROLL')
+        w('        for (int i = 0; i < S; ++i)')
+        w('            if (i >= first) y[i] = grl_advance<R>(y0[i], f[i], a[i], dt);')
+        w('        return;')
+        w('    }')
+        w('    R half = dt * R(0.5);')
+        w('CMX_UNROLL')
+        w('    for (int i = 0; i < S; ++i)')
+        w('        if (i >= first) y[i] = grl_advance<R>(y0[i], f[i], a[i], half);')
+        w('    R g[S];')
+        w('CMX_UNROLL')
+        w('    for (int i = 0; i < S; ++i) {')
+        w('        if (i < first) continue;')
+        w('        const R ym = y[i];')
+        w('        y[i] = y0[i];')
+        w('        grl_rates<R>(y, i_stim_area, p, cm, update_v, t);')
+        w('        g[i] = t[i];')
+        w('        y[i] = y0[i] + delta;')
+        w('        grl_rates<R>(y, i_stim_area, p, cm, update_v, t);')
+        w('        a[i] = (t[i] - g[i]) / delta;')
+        w('        y[i] = ym;')
+        w('    }')
+        w('CMX_UNROLL')
+        w('    for (int i = 0; i < S; ++i)')
+        w('        if (i >= first) y[i] = grl_advance<R>(y0[i], g[i], a[i], dt);')
         w('}')
         w('')
 

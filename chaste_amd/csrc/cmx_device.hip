@@ -68,6 +68,11 @@ struct ModelLr91 {
     {
         return lr91::be_step<R>(y, V, st, p, cm, dt);
     }
+    template <class R, int ORDER>
+    static __device__ __forceinline__ void grl_step(R* y, R st, const R* p, R cm, R dt, bool update_v)
+    {
+        lr91::grl_step<R, ORDER>(y, st, p, cm, dt, update_v);
+    }
     static constexpr bool gate(int i) { return lr91::kIsGate[i] != 0; }
 };
 
@@ -87,6 +92,11 @@ struct ModelTt06 {
     static __device__ __forceinline__ int be_step(R* y, R V, R st, const R* p, R cm, R dt)
     {
         return tt06_epi::be_step<R>(y, V, st, p, cm, dt);
+    }
+    template <class R, int ORDER>
+    static __device__ __forceinline__ void grl_step(R* y, R st, const R* p, R cm, R dt, bool update_v)
+    {
+        tt06_epi::grl_step<R, ORDER>(y, st, p, cm, dt, update_v);
     }
     static constexpr bool gate(int i) { return tt06_epi::kIsGate[i] != 0; }
 };
@@ -229,6 +239,28 @@ __device__ __forceinline__ bool be_substep(RS* y, const RS* p, double V, double 
     return its >= 0;
 }
 
+// One generalized Rush-Larsen substep in FP64 whatever the state precision:
+// the forward difference of step 1e-8 that gives the diagonal Jacobian
+// entries vanishes below the FP32 spacing of most states.  V is y[0] on entry
+// (FP64 vd); with update_v it is advanced into vd.
+template <class Model, class RS, int ORDER>
+__device__ __forceinline__ void grl_substep(RS* y, const RS* p, double& vd, double st, double cm, double h,
+                                            bool update_v)
+{
+    constexpr int NS = Model::NS;
+    constexpr int NP = Model::NP > 0 ? Model::NP : 1;
+    double yd[NS], pd[NP];
+#pragma unroll
+    for (int s = 1; s < NS; ++s) yd[s] = (double)y[s];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) pd[q] = (double)p[q];
+    yd[0] = vd;
+    Model::template grl_step<double, ORDER>(yd, st, pd, cm, h, update_v);
+#pragma unroll
+    for (int s = 1; s < NS; ++s) y[s] = (RS)yd[s];
+    if (update_v) vd = yd[0];
+}
+
 // ODE substeps over [t0, t1] with the voltage updated too, as
 // SolveAndUpdateState does (heart/src/odes/AbstractCardiacCell.cpp:70-73,
 // ode/src/solver/AbstractOneStepIvpOdeSolver.cpp:93-153 for forward Euler,
@@ -256,6 +288,10 @@ __device__ __forceinline__ bool integrate_with_voltage(const CellArgs<RS, RV>& a
         if constexpr (IG == CMX_BACKWARD_EULER) {
             vd += h * (double)Model::template dvdt<RS>(y, V, st, p, cm);
             ok &= be_substep<Model, RS>(y, p, vd, std_, a.cm, h);
+            continue;
+        }
+        if constexpr (IG == CMX_GRL1 || IG == CMX_GRL2) {
+            grl_substep<Model, RS, IG == CMX_GRL1 ? 1 : 2>(y, p, vd, std_, a.cm, h, true);
             continue;
         }
         RS dy[NS], gi[NS], gk[NS];
@@ -379,6 +415,11 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
             const RS st = (RS)std_;
             if constexpr (IG == CMX_BACKWARD_EULER) {
                 ok &= be_substep<Model, RS>(y, p, (double)V, std_, a.cm, h);
+                continue;
+            }
+            if constexpr (IG == CMX_GRL1 || IG == CMX_GRL2) {
+                double vclamp = vn;
+                grl_substep<Model, RS, IG == CMX_GRL1 ? 1 : 2>(y, p, vclamp, std_, a.cm, h, false);
                 continue;
             }
             RS dy[NS], gi[NS], gk[NS];
@@ -1373,6 +1414,8 @@ void launch_cell_integrator(cmx_ctx* c, int mode, int kind, int G, const CellArg
 {
     if (c->integrator == CMX_RUSH_LARSEN) launch_cell_mode<Model, RS, RV, CMX_RUSH_LARSEN>(c, mode, kind, G, a);
     else if (c->integrator == CMX_BACKWARD_EULER) launch_cell_mode<Model, RS, RV, CMX_BACKWARD_EULER>(c, mode, kind, G, a);
+    else if (c->integrator == CMX_GRL1) launch_cell_mode<Model, RS, RV, CMX_GRL1>(c, mode, kind, G, a);
+    else if (c->integrator == CMX_GRL2) launch_cell_mode<Model, RS, RV, CMX_GRL2>(c, mode, kind, G, a);
     else launch_cell_mode<Model, RS, RV, CMX_FORWARD_EULER>(c, mode, kind, G, a);
 }
 
@@ -1815,8 +1858,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
 {
     if (!mesh || !tissue || !opts) return fail(c, CMX_E_ARG, "null mesh, tissue or opts");
     if (model_sizes(tissue->model, &c->S, &c->P)) return fail(c, CMX_E_ARG, "unknown cell model");
-    if (tissue->integrator != CMX_FORWARD_EULER && tissue->integrator != CMX_RUSH_LARSEN &&
-        tissue->integrator != CMX_BACKWARD_EULER)
+    if (tissue->integrator < CMX_FORWARD_EULER || tissue->integrator > CMX_GRL2)
         return fail(c, CMX_E_ARG, "unknown integrator");
     if (opts->precision != CMX_FP64 && opts->precision != CMX_MIXED) return fail(c, CMX_E_ARG, "unknown precision");
     if (!(opts->dt_pde > 0) || !(opts->dt_ode > 0) || opts->dt_ode > opts->dt_pde * (1 + 1e-12))

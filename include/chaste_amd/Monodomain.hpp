@@ -213,6 +213,23 @@ struct CellTenTusscher2006EpiFromCellMLBackwardEuler {
     static constexpr int kIntegrator = CMX_BACKWARD_EULER;
 };
 struct CellTenTusscher2006EpiFromCellMLBackwardEulerOpt : CellTenTusscher2006EpiFromCellMLBackwardEuler {};
+// Generalized Rush-Larsen cells (chaste_codegen --grl1 / --grl2).
+struct CellLuoRudy1991FromCellMLGRL1 {
+    static constexpr int kModel = CMX_MODEL_LR91;
+    static constexpr int kIntegrator = CMX_GRL1;
+};
+struct CellLuoRudy1991FromCellMLGRL2 {
+    static constexpr int kModel = CMX_MODEL_LR91;
+    static constexpr int kIntegrator = CMX_GRL2;
+};
+struct CellTenTusscher2006EpiFromCellMLGRL1 {
+    static constexpr int kModel = CMX_MODEL_TT06_EPI;
+    static constexpr int kIntegrator = CMX_GRL1;
+};
+struct CellTenTusscher2006EpiFromCellMLGRL2 {
+    static constexpr int kModel = CMX_MODEL_TT06_EPI;
+    static constexpr int kIntegrator = CMX_GRL2;
+};
 
 // ---------------------------------------------------------------------------
 // Mesh (regular slabs; general simplices through SetNodesAndElements)

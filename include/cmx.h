@@ -38,10 +38,13 @@ enum {
 enum { CMX_MODEL_LR91 = 0, CMX_MODEL_TT06_EPI = 1 };
 
 /* Cell integrators: EulerIvpOdeSolver (ode/src/solver/EulerIvpOdeSolver.cpp:38-57),
- * AbstractRushLarsenCardiacCell (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:57-153)
- * and AbstractBackwardEulerCardiacCell (heart/src/odes/AbstractBackwardEulerCardiacCell.hpp:
- * 116-330 with CardiacNewtonSolver.hpp:66-160; dt_pde must be a whole number of dt_ode). */
-enum { CMX_FORWARD_EULER = 0, CMX_RUSH_LARSEN = 1, CMX_BACKWARD_EULER = 2 };
+ * AbstractRushLarsenCardiacCell (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:57-153),
+ * AbstractBackwardEulerCardiacCell (heart/src/odes/AbstractBackwardEulerCardiacCell.hpp:
+ * 116-330 with CardiacNewtonSolver.hpp:66-160) and AbstractGeneralizedRushLarsenCardiacCell
+ * of first / second order (heart/src/odes/AbstractGeneralizedRushLarsenCardiacCell.cpp:70-170,
+ * ode/src/solver/GRL1IvpOdeSolver.cpp, GRL2IvpOdeSolver.cpp).  dt_pde must be a whole number
+ * of dt_ode. */
+enum { CMX_FORWARD_EULER = 0, CMX_RUSH_LARSEN = 1, CMX_BACKWARD_EULER = 2, CMX_GRL1 = 3, CMX_GRL2 = 4 };
 
 /* Precision: plain FP64, or FP32 cells + FP32 CG on the correction with FP64
  * voltage, FP64 dot reductions and FP64 residual refinement. */
@@ -73,7 +76,7 @@ typedef struct {
 /* Tissue: one cell model for every node (AbstractCardiacCellFactory). */
 typedef struct {
     int32_t model;             /* CMX_MODEL_* */
-    int32_t integrator;        /* CMX_FORWARD_EULER / CMX_RUSH_LARSEN / CMX_BACKWARD_EULER */
+    int32_t integrator;        /* CMX_FORWARD_EULER ... CMX_GRL2 */
     int32_t n_stimuli;         /* rows in stimuli (<= 255) */
     const cmx_stimulus* stimuli;
     const int32_t* stim_index; /* [n_nodes]: -1 = no stimulus, else row; NULL = none */

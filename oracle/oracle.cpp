@@ -148,6 +148,16 @@ double cell_substep(int model, int integ, const double* p, double cm, double* y,
         for (int i = 1; i < S; ++i) y[i] += dt * dy[i];
         return dv;
     }
+    if (integ == ORC_GRL1 || integ == ORC_GRL2) {
+        if (model == ORC_LR91_CELLML) {
+            if (integ == ORC_GRL1) cmx::lr91::grl_step<double, 1>(y, stim_area, p, cm, dt, update_v);
+            else cmx::lr91::grl_step<double, 2>(y, stim_area, p, cm, dt, update_v);
+        } else {
+            if (integ == ORC_GRL1) cmx::tt06_epi::grl_step<double, 1>(y, stim_area, p, cm, dt, update_v);
+            else cmx::tt06_epi::grl_step<double, 2>(y, stim_area, p, cm, dt, update_v);
+        }
+        return 0.0;
+    }
     if (integ == ORC_BE) {
         int its;
         if (model == ORC_LR91_CELLML) {
@@ -320,7 +330,7 @@ int orc_cell_compute(int model, int integrator, const double* params, double* y,
 {
     const int S = num_states(model);
     if (S < 0 || dt <= 0 || t_end <= t0 || steps_per_sample <= 0) return -1;
-    if (integrator < ORC_FE || integrator > ORC_BE || (model == ORC_LR91_HAND && integrator != ORC_FE)) return -1;
+    if (integrator < ORC_FE || integrator > ORC_GRL2 || (model == ORC_LR91_HAND && integrator != ORC_FE)) return -1;
     const int fails0 = g_newton_failures.load();
     const double* p = params ? params : default_params(model);
     Stimulus stim(stim_mag, stim_dur, stim_start, stim_period);
@@ -610,7 +620,7 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
     const int dim = cfg->dim, model = cfg->model;
     const int S = num_states(model);
     const int integ = cfg->integrator;
-    if (S < 0 || integ < ORC_FE || integ > ORC_BE || (model == ORC_LR91_HAND && integ != ORC_FE)) return -1;
+    if (S < 0 || integ < ORC_FE || integ > ORC_GRL2 || (model == ORC_LR91_HAND && integ != ORC_FE)) return -1;
     const int fails0 = g_newton_failures.load();
     // Rush-Larsen and backward-Euler cells step by dt_ode always
     // (AbstractRushLarsenCardiacCell.cpp:106-153,

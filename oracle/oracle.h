@@ -36,8 +36,9 @@ extern "C" {
 
 enum orc_model { ORC_LR91_HAND = 0, ORC_LR91_CELLML = 1, ORC_TT06_EPI = 2 };
 /* Cell integrators: forward Euler, Rush-Larsen, backward Euler (the
- * reference's *BackwardEuler classes; generated models only). */
-enum orc_integrator { ORC_FE = 0, ORC_RL = 1, ORC_BE = 2 };
+ * reference's *BackwardEuler classes), generalized Rush-Larsen of first and
+ * second order (--grl1 / --grl2 cells); all but FE on generated models only. */
+enum orc_integrator { ORC_FE = 0, ORC_RL = 1, ORC_BE = 2, ORC_GRL1 = 3, ORC_GRL2 = 4 };
 
 int orc_model_num_states(int model);
 

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'liboracle.so')
 
 LR91_HAND, LR91_CELLML, TT06_EPI = 0, 1, 2
-FE, RL, BE = 0, 1, 2     # orc_integrator
+FE, RL, BE, GRL1, GRL2 = 0, 1, 2, 3, 4     # orc_integrator
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -78,15 +78,21 @@ def num_states(model):
     return lib().orc_model_num_states(model)
 
 
+def _integrator(rush_larsen, backward_euler, integrator):
+    if integrator is not None:
+        return int(integrator)
+    return BE if backward_euler else int(rush_larsen)
+
+
 def cell_compute(model, y, t0, t_end, dt, stim=(0.0, 0.0, 0.0, -1.0), rush_larsen=False,
-                 steps_per_sample=1, params=None, backward_euler=False):
+                 steps_per_sample=1, params=None, backward_euler=False, integrator=None):
     """AbstractCardiacCell::Compute: returns samples[k, S]; y updated in place."""
     S = num_states(model)
     nsteps = int(round((t_end - t0) / dt))
     cap = nsteps // steps_per_sample + 2
     out = np.zeros((cap, S))
     pa = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
-    ns = lib().orc_cell_compute(model, BE if backward_euler else int(rush_larsen), _p(pa), _p(y), t0, t_end, dt, stim[0], stim[1], stim[2],
+    ns = lib().orc_cell_compute(model, _integrator(rush_larsen, backward_euler, integrator), _p(pa), _p(y), t0, t_end, dt, stim[0], stim[1], stim[2],
                                 stim[3], steps_per_sample, _p(out), cap)
     if ns < 0:
         raise RuntimeError('orc_cell_compute failed')
@@ -148,7 +154,7 @@ def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, si
              stim_xlim=1e-5, t_end=1.0, ksp_rtol=1e-12, ksp_atol=0.0, max_its=10000, threads=0,
              params=None, trace_nodes=None, act_threshold=None, want_state=False, lumped=False, sigma_elem=None,
              stim_period=0.0, modifier_threshold=None, modifier_every=1, split=False, backward_euler=False,
-             svi=False, stim_mask=None):
+             svi=False, stim_mask=None, integrator=None):
     """Full FP64 monodomain run (AbstractCardiacProblem::Solve equivalent)."""
     n = [ex + 1, (ex + 1) * (ey + 1), (ex + 1) * (ey + 1) * (ez + 1)][dim - 1]
     pa = None
@@ -159,7 +165,7 @@ def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, si
     se = None if sigma_elem is None else np.ascontiguousarray(sigma_elem, dtype=np.float64).reshape(-1)
     mod = None if modifier_threshold is None else np.zeros((4, n))
     mask = None if stim_mask is None else np.ascontiguousarray(stim_mask, dtype=np.uint8)
-    cfg = MonoCfg(dim, ex, ey, ez, h, model, BE if backward_euler else int(rush_larsen), (ctypes.c_double * 3)(*sigma), am, cm, dt_pde, dt_ode,
+    cfg = MonoCfg(dim, ex, ey, ez, h, model, _integrator(rush_larsen, backward_euler, integrator), (ctypes.c_double * 3)(*sigma), am, cm, dt_pde, dt_ode,
                   stim_mag, stim_dur, stim_start, stim_xlim, t_end, ksp_rtol, ksp_atol, max_its, threads, npar,
                   _p(pa), int(lumped), _p(se), float(stim_period), _p(mod),
                   0.0 if modifier_threshold is None else float(modifier_threshold), int(modifier_every),

@@ -462,3 +462,42 @@ def test_niederer_benchmark_activation_times(oracle):
     act = activation_map(printed, 0.1)
     assert (act >= 0).all()
     assert np.abs(act - gold).max() <= 1.5e-3
+
+
+def _lr91_run(oracle, integ, dt, T=1.0, stim=(0.0, 0.0, 0.0, -1.0), v0=None):
+    y = oracle.model_initial_state(oracle.LR91_CELLML)
+    if v0 is not None:
+        y[0] = v0
+    oracle.cell_compute(oracle.LR91_CELLML, y, 0.0, T, dt, stim=stim, integrator=integ,
+                        steps_per_sample=int(round(T / dt)))
+    return y
+
+
+def test_generalized_rush_larsen_first_order(oracle):
+    """TestGeneralizedRushLarsen.hpp:133-250 (LR91, --grl1): stimulated for
+    1 ms at dt 0.01 every state within 1e-2 of an accurate solution, and from
+    V = -30 mV without stimulus the error ratio between dt 0.002 and 0.001
+    is 2 +- 0.1 for every state (the reference's accurate solution is Heun at
+    1e-6; forward Euler at 1e-6 here)."""
+    stim = (-25.5, 1.99, 0.0, -1.0)
+    fine = _lr91_run(oracle, oracle.FE, 1e-5, stim=stim)
+    assert np.abs(_lr91_run(oracle, oracle.GRL1, 0.01, stim=stim) - fine).max() <= 1e-2
+    ref = _lr91_run(oracle, oracle.FE, 1e-6, v0=-30.0)
+    e1 = _lr91_run(oracle, oracle.GRL1, 0.002, v0=-30.0) - ref
+    e2 = _lr91_run(oracle, oracle.GRL1, 0.001, v0=-30.0) - ref
+    assert np.all(np.abs(e1 / e2 - 2.0) <= 0.1)
+
+
+def test_generalized_rush_larsen_second_order(oracle):
+    """TestGeneralizedRushLarsen.hpp:252-330 (LR91, --grl2): stimulated for
+    1 ms at dt 0.01 within 1e-2 of an accurate solution; and the method is
+    second order (error ratio between dt 0.002 and 0.001 near 4 against its
+    own dt 2.5e-5 solution), which the first-order form is not."""
+    stim = (-25.5, 1.99, 0.0, -1.0)
+    fine = _lr91_run(oracle, oracle.FE, 1e-5, stim=stim)
+    assert np.abs(_lr91_run(oracle, oracle.GRL2, 0.01, stim=stim) - fine).max() <= 1e-2
+    ref = _lr91_run(oracle, oracle.GRL2, 2.5e-5, v0=-30.0)
+    e1 = _lr91_run(oracle, oracle.GRL2, 0.002, v0=-30.0) - ref
+    e2 = _lr91_run(oracle, oracle.GRL2, 0.001, v0=-30.0) - ref
+    assert np.all(np.abs(e1 / e2 - 4.0) <= 0.6)
+    assert np.median(np.abs(e1 / e2 - 4.0)) <= 0.1
