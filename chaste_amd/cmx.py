@@ -61,6 +61,11 @@ SIGNATURES = {
     'cmx_create': (ctypes.c_int, [ctypes.POINTER(CmxMesh), ctypes.POINTER(CmxTissue),
                                   ctypes.POINTER(CmxOpts), ctypes.POINTER(ctypes.c_void_p)]),
     'cmx_destroy': (ctypes.c_int, [ctypes.c_void_p]),
+    'cmx_read_mesh': (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _dp, _ip, _dp]),
+    'cmx_read_fibres': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _dp]),
+    'cmx_conductivity_tensors': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp, _dp]),
+    'cmx_io_last_error': (ctypes.c_char_p, []),
     'cmx_last_error': (ctypes.c_char_p, [ctypes.c_void_p]),
     'cmx_step': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
     'cmx_get_voltage': (ctypes.c_int, [ctypes.c_void_p, _dp]),
@@ -145,6 +150,49 @@ def slab_mesh(dim, ex, ey=0, ez=0, h=0.01):
     elems = np.zeros((ne.value, dim + 1), dtype=np.int32)
     _check(lib().cmx_slab_mesh(dim, ex, ey, ez, h, None, None, _p(coords), _p(elems, _ip)))
     return coords, elems
+
+
+def _io_check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().cmx_io_last_error().decode())
+
+
+def read_mesh(base):
+    """TrianglesMeshReader: <base>.node/.ele (ASCII or binary) ->
+    (coords[n, dim], elems[ne, dim+1] from zero, elem_attr[ne, n_attr])."""
+    d, n, ne, na = (ctypes.c_int32() for _ in range(4))
+    b = os.fsencode(base)
+    _io_check(lib().cmx_read_mesh(b, ctypes.byref(d), ctypes.byref(n), ctypes.byref(ne), ctypes.byref(na),
+                                  None, None, None))
+    coords = np.zeros((n.value, d.value))
+    elems = np.zeros((ne.value, d.value + 1), dtype=np.int32)
+    attr = np.zeros((ne.value, na.value))
+    _io_check(lib().cmx_read_mesh(b, ctypes.byref(d), ctypes.byref(n), ctypes.byref(ne), ctypes.byref(na),
+                                  _p(coords), _p(elems, _ip), _p(attr) if na.value else None))
+    return coords, elems, attr
+
+
+def read_fibres(path, dim, axisymmetric=False):
+    """FibreReader: [n, dim, dim] (.ortho: fibre, sheet, normal rows) or
+    [n, dim] (.axi)."""
+    per = dim if axisymmetric else dim * dim
+    n = ctypes.c_int32()
+    b = os.fsencode(path)
+    _io_check(lib().cmx_read_fibres(b, per, ctypes.byref(n), None))
+    out = np.zeros((n.value, per))
+    _io_check(lib().cmx_read_fibres(b, per, ctypes.byref(n), _p(out)))
+    return out.reshape((n.value, dim) if axisymmetric else (n.value, dim, dim))
+
+
+def conductivity_tensors(fibres, sigma, axisymmetric=False):
+    """Per-element conductivity tensors [n, dim, dim] from fibre directions."""
+    f = np.ascontiguousarray(fibres, dtype=np.float64)
+    n = f.shape[0]
+    dim = f.shape[1]
+    sg = np.ascontiguousarray(np.resize(np.asarray(sigma, dtype=np.float64), 3))
+    out = np.zeros((n, dim, dim))
+    _io_check(lib().cmx_conductivity_tensors(dim, n, 1 if axisymmetric else 0, _p(f), _p(sg), _p(out)))
+    return out
 
 
 class Monodomain:

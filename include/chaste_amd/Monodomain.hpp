@@ -71,6 +71,16 @@ public:
     void SetSimulationDuration(double ms) { mDuration = ms; }
     double GetSimulationDuration() const { return mDuration; }
     void SetSlabDimensions(double x, double y, double z, double h) { mDim = 3; mSize[0] = x; mSize[1] = y; mSize[2] = z; mStep = h; }
+    // HeartConfig.cpp SetMeshFileName (mesh read by TrianglesMeshReader; the
+    // fibre file, when given, is <prefix>.ortho or <prefix>.axi)
+    enum FibreOrientation { NoFibreOrientation = 0, Orthotropic = 1, Axisymmetric = 2 };
+    void SetMeshFileName(const std::string& prefix, FibreOrientation fibres = NoFibreOrientation)
+    {
+        mMeshFile = prefix;
+        mFibres = fibres;
+    }
+    const std::string& GetMeshName() const { return mMeshFile; }
+    FibreOrientation GetConductivityMedia() const { return mFibres; }
     void SetSheetDimensions(double x, double y, double h) { mDim = 2; mSize[0] = x; mSize[1] = y; mSize[2] = 0; mStep = h; }
     void SetFibreLength(double x, double h) { mDim = 1; mSize[0] = x; mSize[1] = mSize[2] = 0; mStep = h; }
     const double* GetSlabSize() const { return mSize; }
@@ -143,6 +153,8 @@ private:
     std::string mPc = "bjacobi";
     bool mLumping = false;
     bool mSplitting = false;
+    std::string mMeshFile;
+    FibreOrientation mFibres = NoFibreOrientation;
     bool mSvi = false;
     int mPrecision = CMX_FP64;
     int mRefine = 1;
@@ -257,6 +269,19 @@ public:
         mElems.resize((size_t)ne * (DIM + 1));
         Check(cmx_slab_mesh(DIM, ex, ey, ez, spaceStep, nullptr, nullptr, mCoords.data(), mElems.data()), nullptr);
     }
+    // TetrahedralMesh::ConstructFromMeshReader with a TrianglesMeshReader on
+    // <base>.node / <base>.ele.
+    void ConstructFromMeshFile(const std::string& base)
+    {
+        int32_t d = 0, n = 0, ne = 0, na = 0;
+        CheckIo(cmx_read_mesh(base.c_str(), &d, &n, &ne, &na, nullptr, nullptr, nullptr));
+        if ((unsigned)d != DIM) throw Exception(CMX_E_ARG, "SPACE_DIM  != dimension read from file ");
+        mCoords.resize((size_t)n * DIM);
+        mElems.resize((size_t)ne * (DIM + 1));
+        mAttributes.resize((size_t)ne * na);
+        CheckIo(cmx_read_mesh(base.c_str(), &d, &n, &ne, &na, mCoords.data(), mElems.data(),
+                              na ? mAttributes.data() : nullptr));
+    }
     void SetNodesAndElements(std::vector<double> coords, std::vector<int32_t> elems)
     {
         mCoords = std::move(coords);
@@ -269,8 +294,13 @@ public:
     const std::vector<int32_t>& rGetElements() const { return mElems; }
 
 private:
+    static void CheckIo(int rc)
+    {
+        if (rc) throw Exception(rc, cmx_io_last_error());
+    }
     std::vector<double> mCoords;
     std::vector<int32_t> mElems;
+    std::vector<double> mAttributes;
 };
 
 // ---------------------------------------------------------------------------
@@ -332,9 +362,13 @@ public:
     {
         HeartConfig* cfg = HeartConfig::Instance();
         if (!mpMesh) {
-            if (cfg->GetSpaceDimension() != DIM) throw Exception(CMX_E_ARG, "Space dimension does not match");
-            const double* sz = cfg->GetSlabSize();
-            mOwnMesh.ConstructRegularSlabMesh(cfg->GetInterNodeSpace(), sz[0], sz[1], sz[2]);
+            if (!cfg->GetMeshName().empty()) {
+                mOwnMesh.ConstructFromMeshFile(cfg->GetMeshName());
+            } else {
+                if (cfg->GetSpaceDimension() != DIM) throw Exception(CMX_E_ARG, "Space dimension does not match");
+                const double* sz = cfg->GetSlabSize();
+                mOwnMesh.ConstructRegularSlabMesh(cfg->GetInterNodeSpace(), sz[0], sz[1], sz[2]);
+            }
             mpMesh = &mOwnMesh;
         }
         mpFactory->SetMesh(mpMesh);
@@ -369,6 +403,25 @@ public:
         mesh.elems = mpMesh->rGetElements().data();
         const std::vector<double> sig = cfg->GetIntracellularConductivities();
         for (unsigned r = 0; r < DIM; ++r) mesh.sigma[r * DIM + r] = sig[r];
+        // Fibre orientation file: one tensor per element
+        // (OrthotropicConductivityTensors / AxisymmetricConductivityTensors).
+        std::vector<double> tensors;
+        if (cfg->GetConductivityMedia() != HeartConfig::NoFibreOrientation && !cfg->GetMeshName().empty()) {
+            const bool axi = cfg->GetConductivityMedia() == HeartConfig::Axisymmetric;
+            const std::string path = cfg->GetMeshName() + (axi ? ".axi" : ".ortho");
+            const int per = axi ? (int)DIM : (int)(DIM * DIM);
+            int32_t items = 0;
+            if (cmx_read_fibres(path.c_str(), per, &items, nullptr)) throw Exception(CMX_E_ARG, cmx_io_last_error());
+            if ((unsigned)items != mpMesh->GetNumElements())
+                throw Exception(CMX_E_ARG, "The size of the fibre file does not match the number of elements in the mesh");
+            std::vector<double> fib((size_t)items * per);
+            if (cmx_read_fibres(path.c_str(), per, &items, fib.data())) throw Exception(CMX_E_ARG, cmx_io_last_error());
+            double s3[3] = {sig[0], DIM > 1 ? sig[1] : sig[0], DIM > 2 ? sig[2] : (DIM > 1 ? sig[1] : sig[0])};
+            tensors.resize((size_t)items * DIM * DIM);
+            if (cmx_conductivity_tensors(DIM, items, axi ? 1 : 0, fib.data(), s3, tensors.data()))
+                throw Exception(CMX_E_ARG, cmx_io_last_error());
+            mesh.sigma_elem = tensors.data();
+        }
         cmx_tissue tissue{};
         tissue.model = model;
         tissue.integrator = integrator;
@@ -394,6 +447,8 @@ public:
         if (mpCtx) { cmx_destroy(mpCtx); mpCtx = nullptr; }
         Check(cmx_create(&mesh, &tissue, &o, &mpCtx), nullptr);
         mCurrentTime = 0.0;
+        mAm = o.am;
+        mCm = o.cm;
     }
 
     // Advances to the simulation duration in printing steps; each printing
@@ -402,6 +457,13 @@ public:
     {
         if (!mpCtx) throw Exception(CMX_E_STATE, "Initialise() must be called before Solve()");
         HeartConfig* cfg = HeartConfig::Instance();
+        // The reference reads Am and Cm when it assembles, so values set
+        // between Initialise() and the first Solve() apply: rebuild then.
+        if (cfg->GetSurfaceAreaToVolumeRatio() != mAm || cfg->GetCapacitance() != mCm) {
+            if (mCurrentTime != 0.0)
+                throw Exception(CMX_E_STATE, "Am and Cm cannot change once the simulation has started");
+            Initialise();
+        }
         const double end = cfg->GetSimulationDuration(), print = cfg->GetPrintingTimeStep(), dt = cfg->GetPdeTimeStep();
         if (end <= mCurrentTime) throw Exception(CMX_E_ARG, "The simulation duration must be positive");
         const long n_print = (long)std::floor((end - mCurrentTime) / print + 0.5);
@@ -444,6 +506,7 @@ private:
     TetrahedralMesh<DIM> mOwnMesh;
     cmx_ctx* mpCtx = nullptr;
     double mCurrentTime = 0.0;
+    double mAm = 0.0, mCm = 0.0;
 };
 
 }  // namespace chaste_amd

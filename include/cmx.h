@@ -179,6 +179,32 @@ int cmx_model_info(int32_t model, int32_t* n_states, int32_t* n_params);
 int cmx_model_state_name(int32_t model, int32_t index, const char** name);
 int cmx_model_initial_state(int32_t model, double* y);
 
+/* Reads a Triangle/TetGen mesh <base>.node / <base>.ele, ASCII (items numbered
+ * from 0 or 1, '#' comments) or binary (BIN header tag), as
+ * TrianglesMeshReader does (mesh/src/reader/TrianglesMeshReader.cpp:593-990);
+ * linear simplices of the node dimension only.  Node indices come back from
+ * zero.  With coords == NULL only the sizes are returned; elem_attr gets
+ * n_elem_attr doubles per element (may be NULL).  Errors: CMX_E_ARG with the
+ * message in cmx_io_last_error(). */
+int cmx_read_mesh(const char* base, int32_t* dim, int32_t* n_nodes, int32_t* n_elems, int32_t* n_elem_attr,
+                  double* coords, int32_t* elems, double* elem_attr);
+
+/* Reads a fibre orientation file (FibreReader, heart/src/io/FibreReader.cpp:
+ * 246-335): a header with the number of items (and optionally BIN), then
+ * per_item values per element -- dim*dim for .ortho (fibre, sheet, normal
+ * directions, one after another), dim for .axi.  out == NULL: count only. */
+int cmx_read_fibres(const char* path, int32_t per_item, int32_t* n_items, double* out);
+
+/* Per-element intracellular conductivity tensors [n_elems][dim][dim] from
+ * fibre directions: A diag(sigma) A^T with A's columns the fibre, sheet and
+ * normal directions (OrthotropicConductivityTensors.cpp:40-130), or
+ * sigma[1] I + (sigma[0] - sigma[1]) f f^T when axisymmetric
+ * (AxisymmetricConductivityTensors.cpp:100-170). */
+int cmx_conductivity_tensors(int32_t dim, int32_t n_elems, int32_t axisymmetric, const double* fibres,
+                             const double* sigma, double* out);
+
+const char* cmx_io_last_error(void);
+
 /* Builds a regular slab mesh exactly as ConstructRegularSlabMesh does
  * (mesh/src/common/AbstractTetrahedralMesh.cpp:277-614): nodes
  * (k*(ey+1)+j)*(ex+1)+i, 2-D staggered triangles, 3-D six tetrahedra per cube.

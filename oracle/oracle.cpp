@@ -628,10 +628,19 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
     // stepper's next step (AbstractOneStepIvpOdeSolver.cpp:115-153).
     const bool fixed_dt = integ != ORC_FE;
     int n, ne;
-    if (orc_slab_sizes(dim, cfg->ex, cfg->ey, cfg->ez, &n, &ne)) return -1;
-    std::vector<double> coords((size_t)n * dim);
-    std::vector<int> elems((size_t)ne * (dim + 1));
-    orc_slab_fill(dim, cfg->ex, cfg->ey, cfg->ez, cfg->h, coords.data(), elems.data());
+    std::vector<double> coords;
+    std::vector<int> elems;
+    if (cfg->mesh_coords) {
+        n = cfg->mesh_n_nodes;
+        ne = cfg->mesh_n_elems;
+        coords.assign(cfg->mesh_coords, cfg->mesh_coords + (size_t)n * dim);
+        elems.assign(cfg->mesh_elems, cfg->mesh_elems + (size_t)ne * (dim + 1));
+    } else {
+        if (orc_slab_sizes(dim, cfg->ex, cfg->ey, cfg->ez, &n, &ne)) return -1;
+        coords.resize((size_t)n * dim);
+        elems.resize((size_t)ne * (dim + 1));
+        orc_slab_fill(dim, cfg->ex, cfg->ey, cfg->ez, cfg->h, coords.data(), elems.data());
+    }
     double sigma[9] = {};
     for (int r = 0; r < dim; ++r) sigma[r * dim + r] = cfg->sigma[r];
     const double* sig = cfg->sigma_elem ? cfg->sigma_elem : sigma;

@@ -108,6 +108,10 @@ typedef struct {
     int svi;                  /* HeartConfig::SetUseStateVariableInterpolation (ICI step only,
                                  as the reference's splitting solver has no correction) */
     const unsigned char* stim_mask;  /* [n] stimulated nodes, or NULL for x < stim_xlim */
+    /* a general mesh instead of the slab (dim = the mesh dimension): */
+    int mesh_n_nodes, mesh_n_elems;
+    const double* mesh_coords;       /* [n][dim], or NULL for the slab */
+    const int* mesh_elems;           /* [ne][dim+1] from zero */
 } orc_mono_cfg;
 
 typedef struct {

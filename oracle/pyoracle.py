@@ -29,7 +29,9 @@ class MonoCfg(ctypes.Structure):
                 ('max_its', ctypes.c_int), ('threads', ctypes.c_int), ('n_param', ctypes.c_int), ('params', _dp),
                 ('lumped', ctypes.c_int), ('sigma_elem', _dp), ('stim_period', ctypes.c_double),
                 ('modifier_out', _dp), ('modifier_threshold', ctypes.c_double), ('modifier_every', ctypes.c_int),
-                ('split', ctypes.c_int), ('svi', ctypes.c_int), ('stim_mask', ctypes.POINTER(ctypes.c_ubyte))]
+                ('split', ctypes.c_int), ('svi', ctypes.c_int), ('stim_mask', ctypes.POINTER(ctypes.c_ubyte)),
+                ('mesh_n_nodes', ctypes.c_int), ('mesh_n_elems', ctypes.c_int), ('mesh_coords', _dp),
+                ('mesh_elems', _ip)]
 
 
 class MonoStats(ctypes.Structure):
@@ -112,6 +114,36 @@ def cell_iionic(model, y, params=None):
     return lib().orc_cell_iionic(model, _p(pa), _p(np.ascontiguousarray(y, dtype=np.float64)))
 
 
+def read_triangles(base):
+    """ASCII Triangle/TetGen mesh (<base>.node, <base>.ele): the oracle's own
+    reading of the format TrianglesMeshReader reads (header, then one item
+    per line numbered from 0 or 1, '#' comments) -> (coords, elems from 0)."""
+    def lines(path):
+        with open(path) as f:
+            for raw in f:
+                raw = raw.split('#', 1)[0].split()
+                if raw:
+                    yield raw
+    nl = lines(base + '.node')
+    head = next(nl)
+    n, dim = int(head[0]), int(head[1])
+    coords = np.zeros((n, dim))
+    first = None
+    for i in range(n):
+        row = next(nl)
+        if first is None:
+            first = int(row[0])
+        coords[i] = [float(x) for x in row[1:1 + dim]]
+    el = lines(base + '.ele')
+    head = next(el)
+    ne, npe = int(head[0]), int(head[1])
+    elems = np.zeros((ne, npe), dtype=np.int32)
+    for e in range(ne):
+        row = next(el)
+        elems[e] = [int(x) - first for x in row[1:1 + npe]]
+    return coords, elems
+
+
 def slab(dim, ex, ey=0, ez=0, h=0.01):
     n, ne = ctypes.c_int(), ctypes.c_int()
     lib().orc_slab_sizes(dim, ex, ey, ez, ctypes.byref(n), ctypes.byref(ne))
@@ -154,9 +186,16 @@ def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, si
              stim_xlim=1e-5, t_end=1.0, ksp_rtol=1e-12, ksp_atol=0.0, max_its=10000, threads=0,
              params=None, trace_nodes=None, act_threshold=None, want_state=False, lumped=False, sigma_elem=None,
              stim_period=0.0, modifier_threshold=None, modifier_every=1, split=False, backward_euler=False,
-             svi=False, stim_mask=None, integrator=None):
-    """Full FP64 monodomain run (AbstractCardiacProblem::Solve equivalent)."""
-    n = [ex + 1, (ex + 1) * (ey + 1), (ex + 1) * (ey + 1) * (ez + 1)][dim - 1]
+             svi=False, stim_mask=None, integrator=None, mesh=None):
+    """Full FP64 monodomain run (AbstractCardiacProblem::Solve equivalent)
+    on the slab, or on mesh = (coords[n, dim], elems[ne, dim+1])."""
+    if mesh is not None:
+        mc = np.ascontiguousarray(mesh[0], dtype=np.float64)
+        me = np.ascontiguousarray(mesh[1], dtype=np.int32)
+        n = mc.shape[0]
+    else:
+        mc = me = None
+        n = [ex + 1, (ex + 1) * (ey + 1), (ex + 1) * (ey + 1) * (ez + 1)][dim - 1]
     pa = None
     npar = 0
     if params is not None:
@@ -169,7 +208,8 @@ def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, si
                   stim_mag, stim_dur, stim_start, stim_xlim, t_end, ksp_rtol, ksp_atol, max_its, threads, npar,
                   _p(pa), int(lumped), _p(se), float(stim_period), _p(mod),
                   0.0 if modifier_threshold is None else float(modifier_threshold), int(modifier_every),
-                  int(split), int(svi), _p(mask, ctypes.POINTER(ctypes.c_ubyte)))
+                  int(split), int(svi), _p(mask, ctypes.POINTER(ctypes.c_ubyte)),
+                  0 if mc is None else n, 0 if me is None else me.shape[0], _p(mc), _p(me, _ip))
     V = np.zeros(n)
     S = num_states(model)
     state = np.zeros((S, n)) if want_state else None

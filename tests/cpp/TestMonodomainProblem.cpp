@@ -4,6 +4,7 @@
 // (-600*5000 uA/cm^3) and the same golden voltages at nodes 1, 3, 5, 7
 // (atol 5e-3 mV).  Prints one JSON line per case; exit status 0 iff all pass.
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "chaste_amd/Monodomain.hpp"
@@ -32,7 +33,34 @@ static void RunCase(const char* name, const double golden[4], double duration)
     std::fflush(stdout);
 }
 
-int main()
+// heart/test/monodomain/TestMonodomainProblem.hpp:226-290 on its mesh file
+// (mesh/test/data/1D_0_to_1mm_10_elements): sigma 0.0005, Am = Cm = 1 set
+// between Initialise() and Solve(), 2 ms; nodes 1, 3, 5, 7, 9, 10 (atol 5e-3).
+static void RunMeshFileCase(const std::string& meshes)
+{
+    HeartConfig::Reset();
+    HeartConfig::Instance()->SetIntracellularConductivities({0.0005});
+    HeartConfig::Instance()->SetSimulationDuration(2.0);
+    HeartConfig::Instance()->SetMeshFileName(meshes + "/1D_0_to_1mm_10_elements");
+    PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, 1> cell_factory;
+    MonodomainProblem<1> monodomain_problem(&cell_factory);
+    monodomain_problem.Initialise();
+    HeartConfig::Instance()->SetSurfaceAreaToVolumeRatio(1.0);
+    HeartConfig::Instance()->SetCapacitance(1.0);
+    monodomain_problem.Solve();
+    std::vector<double> V = monodomain_problem.GetSolution();
+    const unsigned nodes[6] = {1, 3, 5, 7, 9, 10};
+    const double golden[6] = {20.7710232, 21.5319692, 22.9280817, 24.0611303, -0.770330519, -19.2234919};
+    double worst = 0;
+    for (int q = 0; q < 6; ++q) worst = std::max(worst, std::fabs(V[nodes[q]] - golden[q]));
+    const bool ok = worst < 5e-3;
+    if (!ok) ++failures;
+    std::printf("{\"case\": \"mesh_file_1d\", \"V9\": %.6f, \"max_err\": %.3e, \"pass\": %s}\n", V[9], worst,
+                ok ? "true" : "false");
+    std::fflush(stdout);
+}
+
+int main(int argc, char** argv)
 {
     HeartConfig::Reset();
     HeartConfig::Instance()->SetSpaceDimension(1);
@@ -63,6 +91,7 @@ int main()
     try { HeartConfig::Instance()->SetOdePdeAndPrintingTimeSteps(0.02, 0.01, 0.01); } catch (const Exception& e) { ++thrown; }
     try { TetrahedralMesh<2> m; m.ConstructRegularSlabMesh(0.03, 0.1, 0.1); } catch (const Exception& e) { ++thrown; }
     const bool errors_ok = thrown == 3;
+    if (argc > 1) RunMeshFileCase(argv[1]);
     if (!errors_ok) ++failures;
     std::printf("{\"case\": \"errors\", \"thrown\": %d, \"pass\": %s}\n", thrown, errors_ok ? "true" : "false");
     return failures == 0 ? 0 : 1;

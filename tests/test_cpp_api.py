@@ -22,9 +22,11 @@ def test_cpp_program_links_libcmx():
 @pytest.mark.gpu
 def test_cpp_monodomain_problem_goldens():
     print('case cpp TestMonodomainProblem', flush=True)
-    r = subprocess.run([_binary()], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    from conftest import GOLDEN
+    r = subprocess.run([_binary(), os.path.join(GOLDEN, 'meshes')], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=120)
     print(r.stdout)
     cases = [json.loads(l) for l in r.stdout.splitlines() if l.startswith('{')]
-    assert {c['case'] for c in cases} == {'1d', '2d', '3d', 'errors'}
+    assert {c['case'] for c in cases} == {'1d', '2d', '3d', 'errors', 'mesh_file_1d'}
     assert all(c['pass'] for c in cases)
     assert r.returncode == 0

@@ -463,6 +463,63 @@ def test_niederer_benchmark_gpu(precision):
     assert d.max() <= (1.5e-3 if precision == 0 else MIXED_ACT_MS)
 
 
+@pytest.mark.parametrize('case', ['1d', '2d_edge', '2d_point'])
+def test_mesh_file_goldens_gpu(oracle, case):
+    """The reference's mesh-file monodomain tests (read through the C ABI
+    mesh reader) on the device: its hard-coded voltages, and 1e-9 relative
+    against the oracle at a tight tolerance."""
+    from chaste_amd import cmx
+    from test_oracle_golden import MESH_CASES, mesh_case_inputs, check_mesh_case
+    print('case mesh file golden', case, flush=True)
+    c = MESH_CASES[case]
+    coords, elems, _ = cmx.read_mesh(os.path.join(GOLDEN, 'meshes', c['mesh']))
+    dim = coords.shape[1]
+    mask = mesh_case_inputs(case, coords)
+    stim_index = np.where(mask, 0, -1).astype(np.int32)
+    steps = int(round(c['t_end'] / 0.01))
+    for atol, rtol in ((2e-4, 0.0), (0.0, 1e-12)):
+        m = cmx.Monodomain(coords, elems, np.full(dim, c['sigma']), cmx.LR91, cmx.FORWARD_EULER,
+                           stimuli=[(c['stim'], 0.0, 0.5)], stim_index=stim_index, precision=cmx.FP64,
+                           ksp_rtol=rtol, ksp_atol=atol, am=1.0, cm=1.0, dt_pde=0.01, dt_ode=0.01, use_graph=True)
+        m.step(0.0, steps)
+        V = m.voltage()
+        if atol > 0:
+            check_mesh_case(case, coords, V)
+        else:
+            ref = oracle.mono_run(dim, 0, mesh=(coords, elems), model=oracle.LR91_CELLML, sigma=(c['sigma'],) * 3,
+                                  am=1.0, cm=1.0, dt_pde=0.01, dt_ode=0.01, stim_mag=c['stim'], stim_dur=0.5,
+                                  t_end=c['t_end'], ksp_rtol=1e-12, stim_mask=mask.astype(np.uint8))['V']
+            err = np.abs(V - ref).max() / np.abs(ref).max()
+            print('mesh file', case, 'rel err', err, 'format', m.stats()['format'])
+            assert err <= FP64_REL
+
+
+def test_fibre_file_conductivities_gpu(oracle):
+    """Per-element tensors built from 2D_0_to_1mm_800_elements.ortho (kinked
+    fibres) drive the device run; against the oracle with the same tensors at
+    1e-9 relative, and the kink changes the answer."""
+    from chaste_amd import cmx
+    print('case fibre file conductivities', flush=True)
+    base = os.path.join(GOLDEN, 'meshes', '2D_0_to_1mm_800_elements')
+    coords, elems, _ = cmx.read_mesh(base)
+    T = cmx.conductivity_tensors(cmx.read_fibres(base + '.ortho', 2), (1.75, 0.19))
+    mask = coords[:, 0] ** 2 <= 1e-10
+    stim_index = np.where(mask, 0, -1).astype(np.int32)
+    m = cmx.Monodomain(coords, elems, np.array([1.75, 0.19]), cmx.LR91, cmx.FORWARD_EULER,
+                       stimuli=[(-1.2e6, 0.0, 0.5)], stim_index=stim_index, precision=cmx.FP64, ksp_rtol=1e-12,
+                       ksp_atol=0.0, sigma_elem=T, use_graph=True)
+    m.step(0.0, 200)
+    kw = dict(mesh=(coords, elems), model=oracle.LR91_CELLML, sigma=(1.75, 0.19, 0.19), stim_mag=-1.2e6,
+              stim_dur=0.5, t_end=2.0, ksp_rtol=1e-12, stim_mask=mask.astype(np.uint8))
+    ref = oracle.mono_run(2, 0, sigma_elem=T, **kw)['V']
+    plain = oracle.mono_run(2, 0, **kw)['V']
+    V = m.voltage()
+    err = np.abs(V - ref).max() / np.abs(ref).max()
+    print('fibre tensors rel err', err, 'kink effect', np.abs(ref - plain).max())
+    assert np.abs(ref - plain).max() > 10.0 and (ref > 0).sum() > 100
+    assert err <= FP64_REL
+
+
 def test_ell_matches_stencil():
     from chaste_amd import cmx
     print('case ell vs stencil', flush=True)

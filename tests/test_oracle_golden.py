@@ -501,3 +501,62 @@ def test_generalized_rush_larsen_second_order(oracle):
     e2 = _lr91_run(oracle, oracle.GRL2, 0.001, v0=-30.0) - ref
     assert np.all(np.abs(e1 / e2 - 4.0) <= 0.6)
     assert np.median(np.abs(e1 / e2 - 4.0)) <= 0.1
+
+
+MESH_CASES = {
+    # TestMonodomainProblem.hpp:226-290: nodes 1, 3, 5, 7, 9, 10 at 2 ms, atol 5e-3
+    '1d': dict(mesh='1D_0_to_1mm_10_elements', sigma=0.0005, t_end=2.0, stim=-600.0, point=None),
+    # :445-530: every node on x = 0.1 at -59.6488 +- 5e-4 after 2 ms
+    '2d_edge': dict(mesh='2D_0_to_1mm_400_elements', sigma=0.0005, t_end=2.0, stim=-600.0, point=None),
+    # :534-650: stimulus -6000 at the centre; corners -34.3481, edge centres
+    # 34.6692 (+- 1e-3) after 1.3 ms
+    '2d_point': dict(mesh='2D_0_to_1mm_400_elements', sigma=0.0005, t_end=1.3, stim=-6000.0, point=(0.05, 0.05)),
+}
+
+
+def mesh_case_inputs(case, coords):
+    """Stimulated nodes of a mesh-file case: PlaneStimulusCellFactory's
+    x^2 <= 1e-10 (heart/src/problem/cell_factories/PlaneStimulusCellFactory.hpp:
+    73-85), or the centre node of PointStimulus2dCellFactory."""
+    c = MESH_CASES[case]
+    if c['point'] is None:
+        return coords[:, 0] ** 2 <= 1e-10
+    px, py = c['point']
+    m = (np.abs(coords[:, 0] - px) < 1e-6) & (np.abs(coords[:, 1] - py) < 1e-6)
+    assert m.sum() == 1
+    return m
+
+
+def check_mesh_case(case, coords, V):
+    """The reference's assertions for the case."""
+    if case == '1d':
+        expect = {1: 20.7710232, 3: 21.5319692, 5: 22.9280817, 7: 24.0611303, 9: -0.770330519, 10: -19.2234919}
+        for i, v in expect.items():
+            assert abs(V[i] - v) <= 5e-3, (i, V[i], v)
+    elif case == '2d_edge':
+        right = coords[:, 0] == 0.1
+        assert right.sum() == 11
+        assert np.abs(V[right] - V[right][0]).max() <= 7e-4
+        assert np.abs(V[right] - (-59.6488)).max() <= 5e-4
+    else:
+        def at(x, y):
+            return int(np.where((np.abs(coords[:, 0] - x) < 1e-6) & (np.abs(coords[:, 1] - y) < 1e-6))[0][0])
+        for x, y in ((0.0, 0.0), (0.1, 0.0)):
+            assert abs(V[at(x, y)] - (-34.3481)) <= 1e-3
+        for x, y in ((0.05, 0.0), (0.0, 0.05), (0.1, 0.05), (0.05, 0.1)):
+            assert abs(V[at(x, y)] - 34.6692) <= 1e-3
+
+
+@pytest.mark.parametrize('case', sorted(MESH_CASES))
+def test_mesh_file_goldens(oracle, case):
+    """Monodomain runs on the reference's own mesh files (LR91 forward Euler,
+    sigma 0.0005 mS/cm, Am = 1 /cm, Cm = 1 uF/cm^2, dt 0.01, default absolute
+    KSP tolerance 2e-4) against the values its tests hard-code."""
+    c = MESH_CASES[case]
+    coords, elems = oracle.read_triangles(os.path.join(GOLDEN, 'meshes', c['mesh']))
+    dim = coords.shape[1]
+    mask = mesh_case_inputs(case, coords)
+    r = oracle.mono_run(dim, 0, mesh=(coords, elems), model=oracle.LR91_CELLML, sigma=(c['sigma'],) * 3, am=1.0,
+                        cm=1.0, dt_pde=0.01, dt_ode=0.01, stim_mag=c['stim'], stim_dur=0.5, t_end=c['t_end'],
+                        ksp_rtol=0.0, ksp_atol=2e-4, stim_mask=mask.astype(np.uint8))
+    check_mesh_case(case, coords, r['V'])
