@@ -635,6 +635,9 @@ struct ResidualArgs {
     const RV* __restrict__ svi;        // state-variable interpolation correction of b, or null
     double* __restrict__ r0_64;        // FP64 residual, kept for the refinement sweep
     const RV* __restrict__ x1;         // refinement sweep: the first solve's correction
+    const double* __restrict__ pin0;   // refinement sweep: the first solve's last reduction
+    int gin0;                          //   its slot count
+    SolveCtl* ctl0;                    //   and the first solve (its convergence is recorded)
     CgVecs<RV> v;
     double* __restrict__ partials;     // [G][3]: ||D^-1 b||^2, 0, ||D^-1 r0||^2
     SolveCtl* ctl;
@@ -775,6 +778,12 @@ __global__ __launch_bounds__(kCgBlock) void refine_residual_kernel(Rows R, MatVa
     RefineRow<Rows, RV> q0, q1;
     if (i0 < rr.hi) q0.load(R, i0, a);
     if (i1 < rr.hi) q1.load(R, i1, a);
+    // block 0 also decides whether the first solve met its own test, for the
+    // iteration-budget statistics (the iteration that would have detected it
+    // is not launched when the budget is exact)
+    PartialLoad<3, kCgBlock> pl0(a.pin0, blk == 0 ? a.gin0 : 0);
+    const int done0 = (blk == 0 && threadIdx.x == 0) ? a.ctl0->done : 1;
+    const double tol0 = (blk == 0 && threadIdx.x == 0) ? a.ctl0->tol2 : 0.0;
     const double* __restrict__ Av = ta.store(sA, mv.a64);
     if constexpr (Rows::kStaged) __syncthreads();
     if (poison) return;
@@ -788,9 +797,14 @@ __global__ __launch_bounds__(kCgBlock) void refine_residual_kernel(Rows R, MatVa
         q.apply(R, Av, D32, a, part);
     }
     write_partials<kCgBlock, 3>(part, a.partials, blk);
-    if (blk == 0 && threadIdx.x == 0) {
-        a.ctl->done = 0;
-        a.ctl->it = 0;
+    if (blk == 0) {
+        double g0[3];
+        pl0.total(g0);
+        if (threadIdx.x == 0) {
+            (void)solve_converged(a.ctl0, done0, tol0, g0[2], true);
+            a.ctl->done = 0;
+            a.ctl->it = 0;
+        }
     }
 }
 
@@ -1501,6 +1515,9 @@ int launch_residual(cmx_ctx* c, const Rows& R, int solve)
     a.svi = (c->svi && !c->split) ? (const RV*)c->svi_out : nullptr;
     a.r0_64 = c->r0_64;
     a.x1 = (const RV*)c->vec[V_X1];
+    a.pin0 = pbuf(c, kIterPart + (c->budget & 1));
+    a.gin0 = c->G;
+    a.ctl0 = c->ctl;
     a.v = cg_vecs<RV>(c, solve, 0);
     a.partials = pbuf(c, kResPart);
     a.ctl = c->ctl + solve;
