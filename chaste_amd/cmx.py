@@ -66,6 +66,10 @@ SIGNATURES = {
     'cmx_read_fibres': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _dp]),
     'cmx_conductivity_tensors': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp, _dp]),
     'cmx_io_last_error': (ctypes.c_char_p, []),
+    'cmx_h5_open': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p),
+                                   ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    'cmx_h5_write_step': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _dp]),
+    'cmx_h5_close': (ctypes.c_int, [ctypes.c_void_p]),
     'cmx_last_error': (ctypes.c_char_p, [ctypes.c_void_p]),
     'cmx_step': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
     'cmx_get_voltage': (ctypes.c_int, [ctypes.c_void_p, _dp]),
@@ -193,6 +197,39 @@ def conductivity_tensors(fibres, sigma, axisymmetric=False):
     out = np.zeros((n, dim, dim))
     _io_check(lib().cmx_conductivity_tensors(dim, n, 1 if axisymmetric else 0, _p(f), _p(sg), _p(out)))
     return out
+
+
+class H5Writer:
+    """Hdf5DataWriter-layout results file ("Data" [t][node][var], "Time"),
+    written one time step at a time:
+
+        with cmx.H5Writer('results.h5', n_nodes, [('V', 'mV')]) as w:
+            w.write(t, V)
+    """
+
+    def __init__(self, path, n_nodes, variables=(('V', 'mV'),), time_unit='msecs'):
+        self.n_nodes, self.n_vars = int(n_nodes), len(variables)
+        names = (ctypes.c_char_p * self.n_vars)(*[v[0].encode() for v in variables])
+        units = (ctypes.c_char_p * self.n_vars)(*[v[1].encode() for v in variables])
+        h = ctypes.c_void_p()
+        _io_check(lib().cmx_h5_open(os.fsencode(path), self.n_nodes, self.n_vars, names, units, time_unit.encode(),
+                                    ctypes.byref(h)))
+        self.h = h
+
+    def write(self, t, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(self.n_nodes, self.n_vars)
+        _io_check(lib().cmx_h5_write_step(self.h, float(t), _p(v)))
+
+    def close(self):
+        if self.h:
+            h, self.h = self.h, None
+            _io_check(lib().cmx_h5_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class Monodomain:

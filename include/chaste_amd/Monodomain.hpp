@@ -32,6 +32,8 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
+#include <sys/stat.h>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -74,6 +76,14 @@ public:
     // HeartConfig.cpp SetMeshFileName (mesh read by TrianglesMeshReader; the
     // fibre file, when given, is <prefix>.ortho or <prefix>.axi)
     enum FibreOrientation { NoFibreOrientation = 0, Orthotropic = 1, Axisymmetric = 2 };
+    // Results file <output directory>/<prefix>.h5 (the directory is taken
+    // under $CHASTE_TEST_OUTPUT when that is set), V at t = 0 and after every
+    // printing step, as AbstractCardiacProblem::Solve writes it
+    // (heart/src/problem/AbstractCardiacProblem.cpp:459-464, 552-555).
+    void SetOutputDirectory(const std::string& dir) { mOutputDir = dir; }
+    const std::string& GetOutputDirectory() const { return mOutputDir; }
+    void SetOutputFilenamePrefix(const std::string& prefix) { mOutputPrefix = prefix; }
+    const std::string& GetOutputFilenamePrefix() const { return mOutputPrefix; }
     void SetMeshFileName(const std::string& prefix, FibreOrientation fibres = NoFibreOrientation)
     {
         mMeshFile = prefix;
@@ -154,6 +164,7 @@ private:
     bool mLumping = false;
     bool mSplitting = false;
     std::string mMeshFile;
+    std::string mOutputDir, mOutputPrefix;
     FibreOrientation mFibres = NoFibreOrientation;
     bool mSvi = false;
     int mPrecision = CMX_FP64;
@@ -471,11 +482,32 @@ public:
         if (std::fabs(per * dt - print) > 1e-10 * print)
             throw Exception(CMX_E_ARG, "Printing time-step should be a multiple of PDE time step");
         const double t_start = mCurrentTime;
-        for (long k = 0; k < n_print; ++k) {
-            const double t = t_start + k * print;   // printing TimeStepper
-            Check(cmx_step(mpCtx, t, (int32_t)per), mpCtx);
+        cmx_h5* out = OpenResults();
+        std::vector<double> V;
+        try {
+            if (out) WriteResults(out, t_start, V);
+            for (long k = 0; k < n_print; ++k) {
+                const double t = t_start + k * print;   // printing TimeStepper
+                Check(cmx_step(mpCtx, t, (int32_t)per), mpCtx);
+                if (out) WriteResults(out, t_start + (k + 1) * print, V);
+            }
+        } catch (...) {
+            if (out) cmx_h5_close(out);
+            throw;
         }
+        if (out && cmx_h5_close(out)) throw Exception(CMX_E_ARG, cmx_io_last_error());
         mCurrentTime = end;
+    }
+
+    // Path of the results file, or empty when no output directory is set.
+    std::string GetResultsPath() const
+    {
+        HeartConfig* cfg = HeartConfig::Instance();
+        if (cfg->GetOutputDirectory().empty()) return "";
+        const char* root = std::getenv("CHASTE_TEST_OUTPUT");
+        std::string dir = root ? std::string(root) + "/" + cfg->GetOutputDirectory() : cfg->GetOutputDirectory();
+        const std::string prefix = cfg->GetOutputFilenamePrefix().empty() ? "results" : cfg->GetOutputFilenamePrefix();
+        return dir + "/" + prefix + ".h5";
     }
 
     std::vector<double> GetSolution() const
@@ -501,6 +533,31 @@ public:
     cmx_ctx* GetContext() { return mpCtx; }
 
 private:
+    cmx_h5* OpenResults()
+    {
+        const std::string path = GetResultsPath();
+        if (path.empty()) return nullptr;
+        const std::string dir = path.substr(0, path.rfind('/'));
+        std::string partial;
+        for (size_t p = 0; p <= dir.size(); ++p) {   // mkdir -p
+            if (p == dir.size() || dir[p] == '/') {
+                partial = dir.substr(0, p);
+                if (!partial.empty()) ::mkdir(partial.c_str(), 0755);
+            }
+        }
+        const char* names[1] = {"V"};
+        const char* units[1] = {"mV"};
+        cmx_h5* h = nullptr;
+        if (cmx_h5_open(path.c_str(), (int32_t)mpMesh->GetNumNodes(), 1, names, units, "msecs", &h))
+            throw Exception(CMX_E_ARG, cmx_io_last_error());
+        return h;
+    }
+    void WriteResults(cmx_h5* h, double t, std::vector<double>& V)
+    {
+        V = GetSolution();
+        if (cmx_h5_write_step(h, t, V.data())) throw Exception(CMX_E_ARG, cmx_io_last_error());
+    }
+
     AbstractCardiacCellFactory<DIM>* mpFactory;
     TetrahedralMesh<DIM>* mpMesh = nullptr;
     TetrahedralMesh<DIM> mOwnMesh;

@@ -205,6 +205,20 @@ int cmx_conductivity_tensors(int32_t dim, int32_t n_elems, int32_t axisymmetric,
 
 const char* cmx_io_last_error(void);
 
+/* HDF5 results file in the layout of the reference's Hdf5DataWriter
+ * (io/src/writer/Hdf5DataWriter.cpp:544-663, read by Hdf5DataReader): dataset
+ * "Data" [time][node][variable] of doubles with the attributes
+ * "Variable Details" (one "name(unit)" string per variable) and
+ * "IsDataComplete" = 1, dataset "Time" [time] with attribute "Unit"
+ * (time_unit, NULL for "msecs").  Rows are streamed to disk one time step at
+ * a time (values[node][variable]); cmx_h5_close writes the time column and the
+ * final sizes.  Host code: no device work. */
+typedef struct cmx_h5 cmx_h5;
+int cmx_h5_open(const char* path, int32_t n_nodes, int32_t n_vars, const char* const* names,
+                const char* const* units, const char* time_unit, cmx_h5** out);
+int cmx_h5_write_step(cmx_h5* h, double time, const double* values);
+int cmx_h5_close(cmx_h5* h);
+
 /* Builds a regular slab mesh exactly as ConstructRegularSlabMesh does
  * (mesh/src/common/AbstractTetrahedralMesh.cpp:277-614): nodes
  * (k*(ey+1)+j)*(ex+1)+i, 2-D staggered triangles, 3-D six tetrahedra per cube.

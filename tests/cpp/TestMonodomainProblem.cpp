@@ -36,12 +36,16 @@ static void RunCase(const char* name, const double golden[4], double duration)
 // heart/test/monodomain/TestMonodomainProblem.hpp:226-290 on its mesh file
 // (mesh/test/data/1D_0_to_1mm_10_elements): sigma 0.0005, Am = Cm = 1 set
 // between Initialise() and Solve(), 2 ms; nodes 1, 3, 5, 7, 9, 10 (atol 5e-3).
-static void RunMeshFileCase(const std::string& meshes)
+static void RunMeshFileCase(const std::string& meshes, const std::string& out_dir)
 {
     HeartConfig::Reset();
     HeartConfig::Instance()->SetIntracellularConductivities({0.0005});
     HeartConfig::Instance()->SetSimulationDuration(2.0);
     HeartConfig::Instance()->SetMeshFileName(meshes + "/1D_0_to_1mm_10_elements");
+    if (!out_dir.empty()) {
+        HeartConfig::Instance()->SetOutputDirectory(out_dir);
+        HeartConfig::Instance()->SetOutputFilenamePrefix("MonodomainLR91_1d");
+    }
     PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, 1> cell_factory;
     MonodomainProblem<1> monodomain_problem(&cell_factory);
     monodomain_problem.Initialise();
@@ -91,7 +95,7 @@ int main(int argc, char** argv)
     try { HeartConfig::Instance()->SetOdePdeAndPrintingTimeSteps(0.02, 0.01, 0.01); } catch (const Exception& e) { ++thrown; }
     try { TetrahedralMesh<2> m; m.ConstructRegularSlabMesh(0.03, 0.1, 0.1); } catch (const Exception& e) { ++thrown; }
     const bool errors_ok = thrown == 3;
-    if (argc > 1) RunMeshFileCase(argv[1]);
+    if (argc > 1) RunMeshFileCase(argv[1], argc > 2 ? argv[2] : "");
     if (!errors_ok) ++failures;
     std::printf("{\"case\": \"errors\", \"thrown\": %d, \"pass\": %s}\n", thrown, errors_ok ? "true" : "false");
     return failures == 0 ? 0 : 1;

@@ -11,7 +11,7 @@ from conftest import ROOT
 
 def declared_symbols():
     src = open(os.path.join(ROOT, 'include', 'cmx.h')).read()
-    return sorted(set(re.findall(r'\b(cmx_[a-z_]+)\s*\(', src)))
+    return sorted(set(re.findall(r'\b(cmx_[a-z0-9_]+)\s*\(', src)))
 
 
 def test_exports_every_declared_symbol():

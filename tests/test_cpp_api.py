@@ -22,11 +22,22 @@ def test_cpp_program_links_libcmx():
 @pytest.mark.gpu
 def test_cpp_monodomain_problem_goldens():
     print('case cpp TestMonodomainProblem', flush=True)
+    import tempfile
+
+    import numpy as np
+
+    import h5lite
     from conftest import GOLDEN
-    r = subprocess.run([_binary(), os.path.join(GOLDEN, 'meshes')], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                       text=True, timeout=120)
+    out = tempfile.mkdtemp()
+    r = subprocess.run([_binary(), os.path.join(GOLDEN, 'meshes'), os.path.join(out, 'MonoProblem1d')],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     print(r.stdout)
     cases = [json.loads(l) for l in r.stdout.splitlines() if l.startswith('{')]
     assert {c['case'] for c in cases} == {'1d', '2d', '3d', 'errors', 'mesh_file_1d'}
+    # the mesh-file case wrote its results file: V at t = 0 and every 0.01 ms to 2 ms
+    f = h5lite.File(os.path.join(out, 'MonoProblem1d', 'MonodomainLR91_1d.h5'))
+    data, times = f.read('Data'), f.read('Time')
+    assert data.shape == (201, 11, 1) and np.allclose(times, np.arange(201) * 0.01)
+    assert abs(data[-1, 9, 0] - (-0.770330519)) <= 5e-3 and abs(data[-1, 10, 0] - (-19.2234919)) <= 5e-3
     assert all(c['pass'] for c in cases)
     assert r.returncode == 0

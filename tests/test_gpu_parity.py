@@ -520,6 +520,41 @@ def test_fibre_file_conductivities_gpu(oracle):
     assert err <= FP64_REL
 
 
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_reference_hdf5_time_series_gpu(precision, tmp_path):
+    """The reference's own results file heart/test/data/Monodomain1d/
+    MonodomainLR91_1d.h5 (101-node cable, 10 ms printed every step) against
+    the device run written through cmx_h5_* and read back: every printed
+    voltage within 0.1 mV and the crossing times of the same 49 nodes within
+    1e-3 ms in FP64; 0.5 mV / 0.1 ms mixed."""
+    import h5lite
+    from chaste_amd import cmx
+    from test_oracle_golden import check_reference_series, crossing_times
+    print('case reference hdf5 series', precision, flush=True)
+    coords, elems = cmx.slab_mesh(1, 100, 0, 0, 0.01)
+    stim_index = np.where(coords[:, 0] ** 2 <= 1e-10, 0, -1).astype(np.int32)
+    m = cmx.Monodomain(coords, elems, np.array([0.0005]), cmx.LR91, cmx.FORWARD_EULER,
+                       stimuli=[(-600.0, 0.0, 0.5)], stim_index=stim_index, precision=precision, ksp_rtol=0.0,
+                       ksp_atol=2e-4, am=1.0, cm=1.0, refine_sweeps=precision, use_graph=True)
+    path = str(tmp_path / 'MonodomainLR91_1d.h5')
+    with cmx.H5Writer(path, coords.shape[0]) as w:
+        w.write(0.0, m.voltage())
+        for k in range(1000):
+            m.step(k * 0.01, 1)
+            w.write((k + 1) * 0.01, m.voltage())
+    got = h5lite.File(path)
+    ref = h5lite.File(os.path.join(GOLDEN, 'MonodomainLR91_1d.h5'))
+    g, r, times = got.read('Data')[:, :, 0], ref.read('Data')[:, :, 0], ref.read('Time')
+    assert np.allclose(got.read('Time'), times, atol=1e-12)
+    print('hdf5 series max |dV|', np.abs(g - r).max())
+    if precision == 0:
+        check_reference_series(g, r, times)
+    else:
+        a, b = crossing_times(g, times), crossing_times(r, times)
+        assert np.abs(g - r).max() <= MIXED_MV
+        assert np.array_equal(a >= 0, b >= 0) and np.abs(a - b)[b >= 0].max() <= MIXED_ACT_MS
+
+
 def test_ell_matches_stencil():
     from chaste_amd import cmx
     print('case ell vs stencil', flush=True)

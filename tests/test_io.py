@@ -106,3 +106,46 @@ def test_fibre_file_errors(tmp_path):
         fh.write('1 2 3\n')
     with pytest.raises(RuntimeError, match='First \\(non comment\\) line'):
         cmx.read_fibres(p, 2)
+
+
+def test_hdf5_reader_reads_reference_file():
+    """The test reader against a file the real HDF5 library wrote (the
+    reference's heart/test/data/Monodomain1d/MonodomainLR91_1d.h5: chunked
+    layout, v1 messages) and the text twin the reference keeps beside it."""
+    import h5lite
+    f = h5lite.File(os.path.join(GOLDEN, 'MonodomainLR91_1d.h5'))
+    d = f.dataset('Data')
+    assert d.shape == (1001, 101, 1)
+    assert list(d.attrs['Variable Details']) == ['V(mV)']
+    assert list(f.dataset('Time').attrs['Unit']) == ['msecs']
+    t = f.read('Time')
+    assert np.allclose(t, np.arange(1001) * 0.01)
+    x = f.read('Data')
+    assert x[0, 0, 0] == -83.853 and abs(x.max() - 48.90776852039833) < 1e-12
+
+
+def test_hdf5_writer_round_trip(tmp_path):
+    """cmx_h5_* writes the reference's layout: read back by the reader that
+    reads libhdf5's files, values bit for bit, attributes, and a file that
+    ends where the superblock says."""
+    import h5lite
+    from chaste_amd import cmx
+    rng = np.random.default_rng(3)
+    data = rng.standard_normal((9, 17, 2))
+    path = str(tmp_path / 'r.h5')
+    with cmx.H5Writer(path, 17, [('V', 'mV'), ('Cai', 'mM')]) as w:
+        for k in range(9):
+            w.write(0.25 * k, data[k])
+    f = h5lite.File(path)
+    assert sorted(f.root) == ['Data', 'Time']
+    assert np.array_equal(f.read('Data'), data)
+    assert np.array_equal(f.read('Time'), 0.25 * np.arange(9))
+    assert list(f.dataset('Data').attrs['Variable Details']) == ['V(mV)', 'Cai(mM)']
+    assert int(f.dataset('Data').attrs['IsDataComplete'][0]) == 1
+    assert list(f.dataset('Time').attrs['Unit']) == ['msecs']
+    assert f.eof == os.path.getsize(path)
+    # an empty file is still well formed
+    with cmx.H5Writer(str(tmp_path / 'e.h5'), 3) as w:
+        pass
+    e = h5lite.File(str(tmp_path / 'e.h5'))
+    assert e.dataset('Data').shape == (0, 3, 1)

@@ -560,3 +560,38 @@ def test_mesh_file_goldens(oracle, case):
                         cm=1.0, dt_pde=0.01, dt_ode=0.01, stim_mag=c['stim'], stim_dur=0.5, t_end=c['t_end'],
                         ksp_rtol=0.0, ksp_atol=2e-4, stim_mask=mask.astype(np.uint8))
     check_mesh_case(case, coords, r['V'])
+
+
+def test_reference_hdf5_time_series(oracle):
+    """heart/test/data/Monodomain1d/MonodomainLR91_1d.h5 (written by the
+    reference: 1 cm cable, 101 nodes, LR91, sigma 0.0005, Am = Cm = 1,
+    stimulus -600 uA/cm^3 for 0.5 ms at x = 0, 10 ms printed every 0.01 ms):
+    every printed voltage within 0.1 mV (measured 0.06; median 3e-7), and the
+    same 49 nodes activated with crossing times within 1e-3 ms (measured
+    2e-4)."""
+    import h5lite
+    f = h5lite.File(os.path.join(GOLDEN, 'MonodomainLR91_1d.h5'))
+    ref = f.read('Data')[:, :, 0]
+    times = f.read('Time')
+    r = oracle.mono_run(1, 100, h=0.01, model=oracle.LR91_CELLML, sigma=(0.0005,) * 3, am=1.0, cm=1.0, dt_pde=0.01,
+                        dt_ode=0.01, stim_mag=-600.0, stim_dur=0.5, stim_xlim=1e-5, t_end=10.0, ksp_rtol=0.0,
+                        ksp_atol=2e-4, trace_nodes=np.arange(101))
+    got = np.vstack([ref[:1], r['trace']])
+    check_reference_series(got, ref, times)
+
+
+def crossing_times(v, times, thr=0.0):
+    act = np.full(v.shape[1], -1.0)
+    for i in range(v.shape[1]):
+        k = int(np.argmax(v[:, i] > thr))
+        if v[k, i] > thr and k > 0:
+            act[i] = times[k - 1] + (times[k] - times[k - 1]) * (thr - v[k - 1, i]) / (v[k, i] - v[k - 1, i])
+    return act
+
+
+def check_reference_series(got, ref, times):
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() <= 0.1
+    a, b = crossing_times(got, times), crossing_times(ref, times)
+    assert np.array_equal(a >= 0, b >= 0) and (b >= 0).sum() == 49
+    assert np.abs(a - b)[b >= 0].max() <= 1e-3
