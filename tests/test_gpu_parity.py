@@ -555,6 +555,22 @@ def test_reference_hdf5_time_series_gpu(precision, tmp_path):
         assert np.array_equal(a >= 0, b >= 0) and np.abs(a - b)[b >= 0].max() <= MIXED_ACT_MS
 
 
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_two_node_cable_gpu(precision):
+    """The smallest mesh (TestMonodomainProblem.hpp:166-218): one element,
+    two nodes, one block per kernel, Cm = 10: V = -52.1698, -83.8381 (5e-3)."""
+    from chaste_amd import cmx
+    print('case two node cable', precision, flush=True)
+    coords, elems = cmx.slab_mesh(1, 1, 0, 0, 1.0)
+    m = cmx.Monodomain(coords, elems, np.array([0.0005]), cmx.LR91, cmx.FORWARD_EULER,
+                       stimuli=[(-600.0, 0.0, 0.5)], stim_index=np.array([0, -1], dtype=np.int32), precision=precision,
+                       ksp_rtol=0.0, ksp_atol=2e-4, am=1.0, cm=10.0, refine_sweeps=precision, use_graph=True)
+    m.step(0.0, 200)
+    V = m.voltage()
+    print('two node V', V)
+    assert abs(V[0] - (-52.1698)) <= 5e-3 and abs(V[1] - (-83.8381)) <= 5e-3
+
+
 def test_ell_matches_stencil():
     from chaste_amd import cmx
     print('case ell vs stencil', flush=True)

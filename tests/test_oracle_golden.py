@@ -595,3 +595,13 @@ def check_reference_series(got, ref, times):
     a, b = crossing_times(got, times), crossing_times(ref, times)
     assert np.array_equal(a >= 0, b >= 0) and (b >= 0).sum() == 49
     assert np.abs(a - b)[b >= 0].max() <= 1e-3
+
+
+def test_two_node_cable(oracle):
+    """TestMonodomainProblem.hpp:166-218: the smallest mesh (one element, two
+    nodes 1 cm apart), sigma 0.0005, Am = 1, Cm = 10, stimulus -600 for 0.5 ms
+    at x = 0, 2 ms: V = -52.1698 and -83.8381 (atol 5e-3)."""
+    r = oracle.mono_run(1, 1, h=1.0, model=oracle.LR91_CELLML, sigma=(0.0005,) * 3, am=1.0, cm=10.0, dt_pde=0.01,
+                        dt_ode=0.01, stim_mag=-600.0, stim_dur=0.5, stim_xlim=1e-5, t_end=2.0, ksp_rtol=0.0,
+                        ksp_atol=2e-4)
+    assert abs(r['V'][0] - (-52.1698)) <= 5e-3 and abs(r['V'][1] - (-83.8381)) <= 5e-3
