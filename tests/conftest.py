@@ -27,3 +27,36 @@ def oracle():
     from oracle import pyoracle
     pyoracle.lib()
     return pyoracle
+
+
+def load_exnode(name):
+    """Values of a cmgui .exnode data file (one field, one component), in node order."""
+    vals = {}
+    lines = open(os.path.join(GOLDEN, name)).read().split('\n')
+    for i, line in enumerate(lines):
+        if line.startswith('Node:'):
+            vals[int(line.split()[1])] = float(lines[i + 1])
+    import numpy as np
+    return np.array([vals[k] for k in range(1, len(vals) + 1)])
+
+
+# TestMonodomainProblem.hpp:851-975 (the 3-D slab 0.1 cm, h 0.01, LR91,
+# -600*5000 uA/cm^3 for 0.5 ms at x = 0, 2 ms): the cmgui data file of printing
+# step 43 (heart/test/data/CmguiData/monodomain/monodomain3dValidData.exnode,
+# six significant digits) and V at nodes 0, 665, 1330 read back from its VTK
+# output at printing steps 100 and 200 (atol 1e-3).
+SLAB3D_VTK = {100: ([0, 665, 1330], [48.0637, 26.5404, -55.3058]),
+              200: ([0, 665, 1330], [31.8730, 32.6531, 34.5152])}
+SLAB3D_EXNODE = (43, 'monodomain3d_43.exnode')
+
+
+def check_slab3d_geometry(v_by_step):
+    """v_by_step: {printing step: V over all 1331 nodes}."""
+    import numpy as np
+    step, name = SLAB3D_EXNODE
+    ref = load_exnode(name)
+    # six significant digits: a half unit in the last place is at most 5e-4 mV
+    assert ref.shape == v_by_step[step].shape == (1331,)
+    assert np.abs(v_by_step[step] - ref).max() <= 1e-3
+    for k, (nodes, vals) in SLAB3D_VTK.items():
+        assert np.abs(v_by_step[k][nodes] - np.array(vals)).max() <= 1e-3

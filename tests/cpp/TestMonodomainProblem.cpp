@@ -14,8 +14,13 @@ using namespace chaste_amd;
 static int failures = 0;
 
 template <unsigned DIM>
-static void RunCase(const char* name, const double golden[4], double duration)
+static void RunCase(const char* name, const double golden[4], double duration, const std::string& out_dir = "",
+                    const char* prefix = "")
 {
+    if (!out_dir.empty()) {
+        HeartConfig::Instance()->SetOutputDirectory(out_dir);
+        HeartConfig::Instance()->SetOutputFilenamePrefix(prefix);
+    }
     PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, DIM> cell_factory(-600 * 5000);
     MonodomainProblem<DIM> monodomain_problem(&cell_factory);
     HeartConfig::Instance()->SetSimulationDuration(duration);
@@ -64,8 +69,34 @@ static void RunMeshFileCase(const std::string& meshes, const std::string& out_di
     std::fflush(stdout);
 }
 
+// heart/test/monodomain/TestMonodomainProblem.hpp:658-691: printing time
+// step 0.1 ms over 0.3 ms on the same mesh file writes exactly t = 0, 0.1,
+// 0.2, 0.3 (the Python side reads the results file back).
+static void RunPrintingTimesCase(const std::string& meshes, const std::string& out_dir)
+{
+    HeartConfig::Reset();
+    HeartConfig::Instance()->SetPrintingTimeStep(0.1);
+    HeartConfig::Instance()->SetSimulationDuration(0.3);
+    HeartConfig::Instance()->SetMeshFileName(meshes + "/1D_0_to_1mm_10_elements");
+    HeartConfig::Instance()->SetOutputDirectory(out_dir);
+    HeartConfig::Instance()->SetOutputFilenamePrefix("mono_testPrintTimes");
+    PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, 1> cell_factory;
+    MonodomainProblem<1> monodomain_problem(&cell_factory);
+    monodomain_problem.Initialise();
+    monodomain_problem.Solve();
+    std::printf("{\"case\": \"printing_times\", \"results\": \"%s\", \"pass\": true}\n",
+                monodomain_problem.GetResultsPath().c_str());
+    std::fflush(stdout);
+}
+
+// argv[1]: folder holding the mesh files; argv[2]: output root.  Results
+// files go to <root>/MonodomainCreatesGeometry/monodomain3d.h5,
+// <root>/MonoProblem1d/MonodomainLR91_1d.h5 and
+// <root>/MonoProblem1dRequestedTimes/mono_testPrintTimes.h5.
 int main(int argc, char** argv)
 {
+    const std::string out_root = argc > 2 ? argv[2] : "";
+    const auto out = [&](const char* sub) { return out_root.empty() ? out_root : out_root + "/" + sub; };
     HeartConfig::Reset();
     HeartConfig::Instance()->SetSpaceDimension(1);
     HeartConfig::Instance()->SetFibreLength(1, 0.01);
@@ -82,7 +113,7 @@ int main(int argc, char** argv)
     HeartConfig::Instance()->SetSpaceDimension(3);
     HeartConfig::Instance()->SetSlabDimensions(0.1, 0.1, 0.1, 0.01);
     const double g3[4] = {31.8958, 32.1109, 32.7315, 33.7011};
-    RunCase<3>("3d", g3, 2.0);
+    RunCase<3>("3d", g3, 2.0, out("MonodomainCreatesGeometry"), "monodomain3d");
 
     // Error behaviour: Solve before Initialise, bad time steps, bad slab step.
     HeartConfig::Reset();
@@ -95,7 +126,8 @@ int main(int argc, char** argv)
     try { HeartConfig::Instance()->SetOdePdeAndPrintingTimeSteps(0.02, 0.01, 0.01); } catch (const Exception& e) { ++thrown; }
     try { TetrahedralMesh<2> m; m.ConstructRegularSlabMesh(0.03, 0.1, 0.1); } catch (const Exception& e) { ++thrown; }
     const bool errors_ok = thrown == 3;
-    if (argc > 1) RunMeshFileCase(argv[1], argc > 2 ? argv[2] : "");
+    if (argc > 1) RunMeshFileCase(argv[1], out("MonoProblem1d"));
+    if (argc > 2) RunPrintingTimesCase(argv[1], out("MonoProblem1dRequestedTimes"));
     if (!errors_ok) ++failures;
     std::printf("{\"case\": \"errors\", \"thrown\": %d, \"pass\": %s}\n", thrown, errors_ok ? "true" : "false");
     return failures == 0 ? 0 : 1;

@@ -29,15 +29,27 @@ def test_cpp_monodomain_problem_goldens():
     import h5lite
     from conftest import GOLDEN
     out = tempfile.mkdtemp()
-    r = subprocess.run([_binary(), os.path.join(GOLDEN, 'meshes'), os.path.join(out, 'MonoProblem1d')],
+    r = subprocess.run([_binary(), os.path.join(GOLDEN, 'meshes'), out],
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     print(r.stdout)
     cases = [json.loads(l) for l in r.stdout.splitlines() if l.startswith('{')]
-    assert {c['case'] for c in cases} == {'1d', '2d', '3d', 'errors', 'mesh_file_1d'}
+    assert {c['case'] for c in cases} == {'1d', '2d', '3d', 'errors', 'mesh_file_1d', 'printing_times'}
     # the mesh-file case wrote its results file: V at t = 0 and every 0.01 ms to 2 ms
     f = h5lite.File(os.path.join(out, 'MonoProblem1d', 'MonodomainLR91_1d.h5'))
     data, times = f.read('Data'), f.read('Time')
     assert data.shape == (201, 11, 1) and np.allclose(times, np.arange(201) * 0.01)
     assert abs(data[-1, 9, 0] - (-0.770330519)) <= 5e-3 and abs(data[-1, 10, 0] - (-19.2234919)) <= 5e-3
+    # TestMonodomainProblem.hpp:658-691: printing step 0.1 ms over 0.3 ms
+    f = h5lite.File(os.path.join(out, 'MonoProblem1dRequestedTimes', 'mono_testPrintTimes.h5'))
+    times = f.read('Time')
+    assert times.shape == (4,) and np.abs(times - [0.0, 0.1, 0.2, 0.3]).max() <= 1e-12
+    assert f.read('Data').shape == (4, 11, 1)
+    # TestMonodomainProblem.hpp:851-975: the 3-D slab's results file against the
+    # reference's cmgui data file (step 43) and VTK values (steps 100, 200)
+    from conftest import check_slab3d_geometry
+    f = h5lite.File(os.path.join(out, 'MonodomainCreatesGeometry', 'monodomain3d.h5'))
+    data, times = f.read('Data'), f.read('Time')
+    assert data.shape == (201, 1331, 1) and np.allclose(times, np.arange(201) * 0.01)
+    check_slab3d_geometry({k: data[k, :, 0] for k in (43, 100, 200)})
     assert all(c['pass'] for c in cases)
     assert r.returncode == 0

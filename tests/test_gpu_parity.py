@@ -651,3 +651,20 @@ def test_divergence_reported():
     with pytest.raises(cmx.CmxError) as e:
         m.step(0.0, 1)
     assert e.value.code == -4
+
+
+@pytest.mark.parametrize('use_graph', [False, True], ids=['eager', 'graph'])
+def test_slab3d_geometry_outputs_gpu(use_graph):
+    """TestMonodomainProblem.hpp:851-975 on the GPU path (FP64, KSP atol 2e-4):
+    all 1331 nodes at printing step 43 against the reference's cmgui data file,
+    nodes 0, 665, 1330 at steps 100 and 200 against its VTK checks; the run is
+    split into three cmx_step calls at those printing times."""
+    from chaste_amd import cmx
+    from conftest import SLAB3D_EXNODE, SLAB3D_VTK, check_slab3d_geometry
+    print('case slab3d geometry graph=%s' % use_graph, flush=True)
+    m = make(3, 10, 10, 10, cmx.LR91, 0, -3e6, 0.5, 1e-5, rtol=0.0, atol=2e-4, use_graph=use_graph)
+    got, done = {}, 0
+    for k in sorted([SLAB3D_EXNODE[0]] + list(SLAB3D_VTK)):
+        m.step(done * 0.01, k - done)
+        got[k], done = m.voltage(), k
+    check_slab3d_geometry(got)

@@ -605,3 +605,19 @@ def test_two_node_cable(oracle):
                         dt_ode=0.01, stim_mag=-600.0, stim_dur=0.5, stim_xlim=1e-5, t_end=2.0, ksp_rtol=0.0,
                         ksp_atol=2e-4)
     assert abs(r['V'][0] - (-52.1698)) <= 5e-3 and abs(r['V'][1] - (-83.8381)) <= 5e-3
+
+
+def test_slab3d_geometry_outputs(oracle):
+    """TestMonodomainProblem.hpp:851-975 (MonodomainCreatesGeometry): every
+    node's V at printing step 43 against the reference's cmgui data file, and
+    nodes 0, 665, 1330 at steps 100 and 200 against its VTK checks."""
+    from conftest import SLAB3D_EXNODE, SLAB3D_VTK, check_slab3d_geometry
+    r = oracle.mono_run(3, 10, 10, 10, h=0.01, model=oracle.LR91_CELLML, sigma=(1.75,) * 3, am=1400.0, cm=1.0,
+                        dt_pde=0.01, dt_ode=0.01, stim_mag=-3e6, stim_dur=0.5, stim_xlim=1e-5, t_end=2.0,
+                        ksp_rtol=0.0, ksp_atol=2e-4, trace_nodes=np.arange(1331))
+    steps = [SLAB3D_EXNODE[0]] + list(SLAB3D_VTK)
+    check_slab3d_geometry({k: r['trace'][k - 1] for k in steps})
+    # a step either side of 43 is over 3 mV away at the upstroke: the file pins the step
+    from conftest import load_exnode
+    ref = load_exnode(SLAB3D_EXNODE[1])
+    assert np.abs(r['trace'][41] - ref).max() > 1.0 and np.abs(r['trace'][43] - ref).max() > 1.0
