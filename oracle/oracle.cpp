@@ -360,6 +360,28 @@ int orc_cell_compute(int model, int integrator, const double* params, double* y,
     return ns;
 }
 
+double orc_cell_compute_except_voltage(int model, int integrator, const double* params, double* y,
+                                       double t0, double t_end, double dt)
+{
+    // AbstractCardiacCell::ComputeExceptVoltage (AbstractCardiacCell.cpp:90-105):
+    // the same stepping as Compute with V clamped (so the stimulus, which
+    // only enters dV/dt, has no effect), then GetIIonic at the result.
+    const int S = num_states(model);
+    if (S < 0 || dt <= 0 || t_end <= t0) return NAN;
+    if (integrator < ORC_FE || integrator > ORC_GRL2 || (model == ORC_LR91_HAND && integrator != ORC_FE)) return NAN;
+    const int fails0 = g_newton_failures.load();
+    const double* p = params ? params : default_params(model);
+    const bool fixed_dt = integrator != ORC_FE;
+    Stepper st(t0, t_end, dt);
+    while (!st.at_end()) {
+        const double t = fixed_dt ? t0 + st.k * dt : st.time;
+        cell_substep(model, integrator, p, 1.0, y, t, fixed_dt ? dt : st.step_size(), 0.0, false);
+        st.advance();
+    }
+    if (g_newton_failures.load() != fails0) return NAN;
+    return cell_iionic(model, p, 1.0, y);
+}
+
 double orc_cell_iionic(int model, const double* params, const double* y)
 {
     return cell_iionic(model, params ? params : default_params(model), 1.0, y);

@@ -54,6 +54,13 @@ int orc_cell_compute(int model, int integrator, const double* params, double* y,
                      double stim_mag, double stim_dur, double stim_start, double stim_period,
                      int steps_per_sample, double* samples, int max_samples);
 
+/* AbstractCardiacCell::ComputeExceptVoltage over [t0, t_end] in steps of dt
+ * (V clamped, the other states by the integrator), then GetIIonic at the
+ * result (what SolveCellSystems caches per node).  y is in/out; returns the
+ * ionic current, NaN on error. */
+double orc_cell_compute_except_voltage(int model, int integrator, const double* params, double* y,
+                                       double t0, double t_end, double dt);
+
 /* The model's default initial state (CellML order, V first); returns S. */
 int orc_model_initial_state(int model, double* y);
 

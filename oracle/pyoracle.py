@@ -59,6 +59,9 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
         L.orc_model_initial_state.argtypes = [ctypes.c_int, _dp]
+        L.orc_cell_compute_except_voltage.restype = ctypes.c_double
+        L.orc_cell_compute_except_voltage.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                                      ctypes.c_double, ctypes.c_double]
         L.orc_cell_iionic.restype = ctypes.c_double
         L.orc_cell_iionic.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_slab_sizes.argtypes = [ctypes.c_int] * 4 + [_ip, _ip]
@@ -107,6 +110,15 @@ def model_initial_state(model):
     if S < 0:
         raise ValueError('unknown model %d' % model)
     return y[:S].copy()
+
+
+def cell_compute_except_voltage(model, y, t0, t_end, dt, integrator=0, params=None):
+    """AbstractCardiacCell::ComputeExceptVoltage then GetIIonic; y updated in place."""
+    pa = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    r = lib().orc_cell_compute_except_voltage(model, integrator, _p(pa), _p(y), t0, t_end, dt)
+    if not np.isfinite(r):
+        raise RuntimeError('orc_cell_compute_except_voltage failed')
+    return r
 
 
 def cell_iionic(model, y, params=None):

@@ -60,3 +60,35 @@ def check_slab3d_geometry(v_by_step):
     assert np.abs(v_by_step[step] - ref).max() <= 1e-3
     for k, (nodes, vals) in SLAB3D_VTK.items():
         assert np.abs(v_by_step[k][nodes] - np.array(vals)).max() <= 1e-3
+
+
+# TestMonodomainConductionVelocity.hpp:69-127 (TestMonodomainDg01DWith100elements):
+# mesh/test/data/1D_0_to_1_100_elements, LR91, sigma 0.0005, Am = Cm = 1 (set
+# after Initialise), the default plane stimulus (-600 uA/cm^3 for 0.5 ms at
+# x = 0), 30 ms, printing every 0.01 ms, ActivationOutputModifier at 0 and
+# -70 mV; its files heart/test/data/MonoConductionVel/activation_map_*.txt
+# (first activation, first recovery, second activation, second recovery).
+ACTIVATION_MAPS = {0.0: 'activation/activation_map_0.0.txt', -70.0: 'activation/activation_map_-70.0.txt'}
+
+
+def load_activation_map(threshold):
+    import numpy as np
+    rows = [l.replace(',', ' ').split() for l in open(os.path.join(GOLDEN, ACTIVATION_MAPS[threshold])) if l.strip()]
+    return np.array(rows, dtype=float)
+
+
+def check_activation_map(got, threshold, dt=0.01, max_flips=2):
+    """got[4][n] against the file.  Times are printing steps: a node whose
+    voltage lies within a few uV of the threshold at a printing step may land
+    one step either side (the reference's iterates differ from any other
+    solver's at the 1e-3 mV level), so at most max_flips nodes may differ, by
+    exactly one step.  Returns the nodes that did."""
+    import numpy as np
+    ref = load_activation_map(threshold).T
+    assert got.shape == ref.shape
+    assert np.array_equal(got < 0, ref < 0)
+    d = np.abs(got - ref)
+    flipped = np.where(d.max(axis=0) > 1e-9)[0]
+    assert len(flipped) <= max_flips, (threshold, flipped)
+    assert np.all(np.abs(d[:, flipped] - dt * (d[:, flipped] > 1e-9)) <= 1e-9)
+    return flipped

@@ -668,3 +668,59 @@ def test_slab3d_geometry_outputs_gpu(use_graph):
         m.step(done * 0.01, k - done)
         got[k], done = m.voltage(), k
     check_slab3d_geometry(got)
+
+
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_conduction_activation_maps_gpu(precision):
+    """TestMonodomainConductionVelocity.hpp:69-127 on the device: the mesh file
+    read through the C ABI, ActivationOutputModifier at 0 and -70 mV tracked on
+    the device every printing step over 30 ms, against the reference's
+    activation map files (FP64: equal up to one printing-step tie per map;
+    mixed: within 0.1 ms)."""
+    from chaste_amd import cmx
+    from conftest import check_activation_map, load_activation_map
+    print('case conduction activation maps precision', precision, flush=True)
+    coords, elems, _ = cmx.read_mesh(os.path.join(GOLDEN, 'meshes', '1D_0_to_1_100_elements'))
+    stim_index = np.where(coords[:, 0] ** 2 <= 1e-10, 0, -1).astype(np.int32)
+    for threshold in (0.0, -70.0):
+        m = cmx.Monodomain(coords, elems, np.array([0.0005]), cmx.LR91, cmx.FORWARD_EULER,
+                           stimuli=[(-600.0, 0.0, 0.5)], stim_index=stim_index, precision=precision,
+                           ksp_rtol=0.0, ksp_atol=2e-4, am=1.0, cm=1.0, dt_pde=0.01, dt_ode=0.01, use_graph=True)
+        m.set_activation(threshold, 1, 0.0)
+        m.step(0.0, 3000)
+        got = m.activation()[:4]
+        if precision == 0:
+            check_activation_map(got, threshold)
+        else:
+            ref = load_activation_map(threshold).T
+            assert np.array_equal(got < 0, ref < 0)
+            print('mixed activation map', threshold, 'max diff', np.abs(got - ref).max())
+            assert np.abs(got - ref).max() <= MIXED_ACT_MS
+
+
+@pytest.mark.parametrize('v_clamp', [-83.853, -20.0], ids=['rest', 'depolarised'])
+def test_tissue_iionic_cache_matches_single_cell(oracle, v_clamp):
+    """TestMonodomainTissue.hpp:133-237 on the device: a two-node cable (h = 1,
+    node 0 stimulated), one PDE step of 0.5 ms over 50 cell steps of 0.01 ms;
+    the Iion cache at both nodes equals a directly integrated cell
+    (ComputeExceptVoltage + GetIIonic) within 1e-10, and again after the
+    voltage is reset and a second step taken (the reference's tolerances are
+    1e-6, then 1e-10)."""
+    from chaste_amd import cmx
+    print('case tissue iionic cache', v_clamp, flush=True)
+    coords = np.array([[0.0], [1.0]])
+    elems = np.array([[0, 1]], dtype=np.int32)
+    m = cmx.Monodomain(coords, elems, np.array([1.75]), cmx.LR91, cmx.FORWARD_EULER,
+                       stimuli=[(-600.0 * 1400, 0.0, 1.0)], stim_index=np.array([0, -1], dtype=np.int32),
+                       precision=cmx.FP64, ksp_rtol=1e-12, ksp_atol=0.0, dt_pde=0.5, dt_ode=0.01)
+    y = oracle.model_initial_state(oracle.LR91_CELLML)
+    y[0] = v_clamp
+    for k in range(2):
+        m.set_voltage(np.full(2, v_clamp))
+        m.step(0.5 * k, 1)
+        ref = oracle.cell_compute_except_voltage(oracle.LR91_CELLML, y, 0.5 * k, 0.5 * (k + 1), 0.01)
+        got = m.iionic()
+        print('step', k, 'cache', got, 'cell', ref)
+        assert np.abs(got - ref).max() <= 1e-10
+        # the PDE step moved V (stimulus and diffusion): the cache is at V^n
+        assert np.abs(m.voltage() - v_clamp).max() > 1e-3

@@ -621,3 +621,53 @@ def test_slab3d_geometry_outputs(oracle):
     from conftest import load_exnode
     ref = load_exnode(SLAB3D_EXNODE[1])
     assert np.abs(r['trace'][41] - ref).max() > 1.0 and np.abs(r['trace'][43] - ref).max() > 1.0
+
+
+@pytest.mark.parametrize('threshold', [0.0, -70.0])
+def test_conduction_activation_maps(oracle, threshold):
+    """TestMonodomainConductionVelocity.hpp:69-127: the reference's activation
+    map files at 0 and -70 mV (101 nodes, 30 ms).  One node per map lands one
+    printing step away; the test shows that its voltage there is within
+    0.01 mV of the threshold, i.e. a tie below the solvers' agreement."""
+    from conftest import check_activation_map, load_activation_map
+    coords, elems = oracle.read_triangles(os.path.join(GOLDEN, 'meshes', '1D_0_to_1_100_elements'))
+    r = oracle.mono_run(1, 0, mesh=(coords, elems), model=oracle.LR91_CELLML, sigma=(0.0005,) * 3, am=1.0, cm=1.0,
+                        dt_pde=0.01, dt_ode=0.01, stim_mag=-600.0, stim_dur=0.5, t_end=30.0, ksp_rtol=0.0,
+                        ksp_atol=2e-4, stim_mask=(coords[:, 0] ** 2 <= 1e-10).astype(np.uint8),
+                        modifier_threshold=threshold, modifier_every=1, trace_nodes=np.arange(101))
+    flipped = check_activation_map(r['modifier'], threshold)
+    trace = np.vstack([np.full(101, -83.853), r['trace']])
+    for i in flipped:
+        k = int(round(min(r['modifier'][0, i], load_activation_map(threshold)[i, 0]) / 0.01))
+        assert abs(trace[k, i] - threshold) < 0.01
+    # activation spreads monotonically along the cable at ~0.05 cm/ms
+    # (the reference's conduction-velocity check, 0.05 +- 0.003)
+    t = r['modifier'][0]
+    assert np.all(np.diff(t) >= 0)
+    assert abs((coords[95, 0] - coords[5, 0]) / (t[95] - t[5]) - 0.05) <= 0.003
+
+
+@pytest.mark.parametrize('v_clamp', [-83.853, -20.0], ids=['rest', 'depolarised'])
+def test_compute_except_voltage(oracle, v_clamp):
+    """AbstractCardiacCell::ComputeExceptVoltage + GetIIonic, the per-node work
+    SolveCellSystems caches (TestMonodomainTissue.hpp:133-237): V stays at the
+    clamp, the generated LR91 agrees with the hand restatement (pinned to the
+    ValidData files) to 1e-8, and two half-intervals equal one interval."""
+    ys = []
+    for model in (oracle.LR91_HAND, oracle.LR91_CELLML):
+        y = np.array(LR91_INIT)
+        y[0] = v_clamp
+        i1 = oracle.cell_compute_except_voltage(model, y, 0.0, 0.5, 0.01)
+        assert y[0] == v_clamp
+        ys.append((y.copy(), i1))
+    assert np.abs(ys[0][0] - ys[1][0]).max() <= 1e-8 and abs(ys[0][1] - ys[1][1]) <= 1e-8
+    y2 = np.array(LR91_INIT)
+    y2[0] = v_clamp
+    oracle.cell_compute_except_voltage(oracle.LR91_CELLML, y2, 0.0, 0.25, 0.01)
+    i2 = oracle.cell_compute_except_voltage(oracle.LR91_CELLML, y2, 0.25, 0.5, 0.01)
+    assert np.abs(y2 - ys[1][0]).max() <= 1e-12 and abs(i2 - ys[1][1]) <= 1e-12
+    if v_clamp > -50:
+        # the gates move: the current is far from its value at the initial state
+        y0 = np.array(LR91_INIT)
+        y0[0] = v_clamp
+        assert abs(ys[1][1] - oracle.cell_iionic(oracle.LR91_CELLML, y0)) > 1.0
