@@ -165,6 +165,104 @@ def cpu_baseline(n_steps, threads):
     return s['steps'] / loop, s
 
 
+def adjoint_case(args):
+    """The adjoint workload: a 2-D LR91 square of 256 x 256 nodes (h 0.01 cm,
+    forward Euler, FP64, Jacobi-PCG rtol 1e-12), per-node scalings of g_Na and
+    g_Kmax; ground truth = seeded smooth Gaussian random fields in [0.6, 1.4]
+    with correlation length 0.1 cm; loss = squared V mismatch at 64 seeded
+    observation nodes every 0.1 ms over 20 ms (2000 steps) against the
+    ground-truth run, evaluated at scalings 1; checkpoints every 100 steps.
+    Prints one JSON line: the gradient's wall time next to the forward run's,
+    and its agreement with FP64 central differences of the device forward run
+    along random directions."""
+    from chaste_amd import cmx
+    from chaste_amd.autograd import smooth_random_field
+    N, steps, every, n_obs = 255, 2000, 10, 64
+    coords, elems = cmx.slab_mesh(2, N, N, 0, H)
+    n = coords.shape[0]
+    stim_index = np.where(coords[:, 0] < 0.02, 0, -1).astype(np.int32)
+    m = cmx.Monodomain(coords, elems, np.array([1.75, 1.75]), cmx.LR91, cmx.FORWARD_EULER,
+                       stimuli=[(-400000.0, 0.0, 0.5)], stim_index=stim_index, precision=cmx.FP64,
+                       ksp_rtol=1e-12, ksp_atol=0.0, use_graph=True)
+    y0 = m.state()
+    defaults = np.array([d for _, d in cmx.param_info(cmx.LR91)])
+    rng = np.random.default_rng(2024)
+    truth = np.stack([smooth_random_field((N + 1, N + 1), H, 0.1, rng).ravel() for _ in range(2)])
+    obs = np.sort(rng.choice(n, n_obs, replace=False)).astype(np.int32)
+
+    def params(s):
+        p = np.repeat(defaults[:, None], n, axis=1)
+        p[:2] *= s
+        return p
+
+    def observe(s):
+        m.set_state(y0)
+        m.set_params(params(s))
+        out = []
+        for j in range(steps // every):
+            m.step(j * every * DT, every)
+            out.append(m.voltage()[obs])
+        return np.array(out)
+
+    progress('adjoint: ground truth')
+    data = observe(truth)
+    s1 = np.ones((2, n))
+    m.set_state(y0)
+    m.set_params(params(s1))
+    m.step(0.0, 50)
+    sync()
+    m.set_state(y0)
+    t = time.perf_counter()
+    m.step(0.0, steps)
+    sync()
+    t_fwd = time.perf_counter() - t
+    progress('adjoint: gradient')
+    m.set_state(y0)
+    t = time.perf_counter()
+    loss, gp, _ = m.adjoint(0.0, steps, obs, every, data, 100)
+    t_adj = time.perf_counter() - t
+    g = gp[:2] * defaults[:2, None]
+
+    def central(d, eps):
+        return (((observe(s1 + eps * d) - data) ** 2).sum() - ((observe(s1 - eps * d) - data) ** 2).sum()) / (2 * eps)
+
+    # Central differences at eps and eps/2, Richardson-combined.  LR91's h and
+    # j gate rates jump at V = -40 mV, so the loss jumps wherever a
+    # perturbation moves some node's voltage at some step across -40 mV; the
+    # two differences then disagree far beyond their O(eps^2) gap, and the
+    # direction is differenced again at a 100 times smaller eps.
+    rel, retried = [], 0
+    for k in range(args.adjoint_directions):
+        d = rng.standard_normal((2, n))
+        ad = float((g * d).sum())
+        for eps in (1e-4, 1e-6):
+            d1, d2 = central(d, eps), central(d, eps / 2)
+            if abs(d1 - d2) <= 1e-5 * abs(d2):
+                break
+            retried += eps == 1e-4
+        fd = (4 * d2 - d1) / 3
+        rel.append(abs(ad - fd) / abs(fd))
+        progress('adjoint: direction %d eps %g rel %.2e' % (k, eps, rel[-1]))
+    line = {
+        'metric': 'adjoint gradient wall time (loss + dL/d 2x65536 conductance scalings)',
+        'value': round(t_adj, 4), 'unit': 's', 'higher_is_better': False, 'n_gpus': 1, 'dtype': 'f64',
+        'data': 'synthetic (seeded Gaussian random-field scalings, seeded observation nodes)',
+        'config': {'workload': 'LR91 FE monodomain square 256x256 nodes h=0.01, 20 ms (2000 steps), '
+                               'stim -400000 uA/cm^3 0.5 ms x<0.02, Jacobi-CG rtol 1e-12, 64 observation '
+                               'nodes every 0.1 ms, checkpoints every 100 steps', 'n_nodes': n},
+        'forward_s': round(t_fwd, 4), 'gradient_over_forward': round(t_adj / t_fwd, 2), 'loss': loss,
+        'fd_directions': len(rel), 'fd_eps': [1e-4, 1e-6],
+        'fd_directions_redifferenced': retried,
+        'fd_within_1e-4': int(sum(r <= 1e-4 for r in rel)),
+        'fd_max_rel_err': max(rel) if rel else None,
+        'fd_median_rel_err': float(np.median(rel)) if rel else None,
+        'fd_note': 'Richardson-combined central differences of the device forward run at eps and eps/2; a '
+                   'direction whose two differences disagree (the loss jumps where a node crosses LR91\'s '
+                   'h/j-gate branch at -40 mV) is differenced again at eps 1e-6',
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -177,6 +275,9 @@ def main():
     ap.add_argument('--no-graph', action='store_true', help='plain launches instead of hipGraph replay')
     ap.add_argument('--no-kernel-events', action='store_true',
                     help='no per-kernel event timing (ranking / roofline passes); for runs under a profiler')
+    ap.add_argument('--adjoint', action='store_true',
+                    help='the adjoint workload instead (gradient time and finite-difference check)')
+    ap.add_argument('--adjoint-directions', type=int, default=32)
     args = ap.parse_args()
     global USE_GRAPH
     USE_GRAPH = not args.no_graph
@@ -185,6 +286,9 @@ def main():
               '(multi-GPU sharding is a design note, not built); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
 
+    if args.adjoint:
+        adjoint_case(args)
+        return
     from chaste_amd import cmx
     coords, elems = cmx.slab_mesh(3, EX, EY, EZ, H)
     n = coords.shape[0]

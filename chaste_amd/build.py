@@ -13,7 +13,7 @@ HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
              '-I' + os.path.join(ROOT, 'include'), '-I' + CSRC]
 
-SOURCES = ['cmx_device.hip', 'cmx_host.cpp', 'cmx_io.cpp']
+SOURCES = ['cmx_device.hip', 'cmx_adjoint.hip', 'cmx_host.cpp', 'cmx_io.cpp']
 
 
 CPP_TESTS = ['TestMonodomainProblem']
@@ -35,12 +35,25 @@ def _run(cmd):
 
 
 def build_library(verbose=False):
-    """Compile every HIP/C++ source into chaste_amd/lib/libcmx.so."""
-    os.makedirs(LIBDIR, exist_ok=True)
+    """Compile every HIP/C++ source (in parallel, objects under
+    chaste_amd/lib/_build) and link chaste_amd/lib/libcmx.so."""
+    obj_dir = os.path.join(LIBDIR, '_build')
+    os.makedirs(obj_dir, exist_ok=True)
     out = library_path()
     tmp = out + '.tmp'
-    cmd = [HIPCC] + HIP_FLAGS + ['-shared', '-o', tmp] + [os.path.join(CSRC, s) for s in SOURCES]
-    log = _run(cmd)
+    objs, procs = [], []
+    for s in SOURCES:
+        obj = os.path.join(obj_dir, os.path.splitext(s)[0] + '.o')
+        cmd = [HIPCC] + HIP_FLAGS + ['-c', '-o', obj, os.path.join(CSRC, s)]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    log = ''
+    for cmd, p in procs:
+        text = p.communicate()[0]
+        if p.returncode != 0:
+            raise RuntimeError('build failed: %s\n%s' % (' '.join(cmd), text))
+        log += text
+    log += _run([HIPCC, '--offload-arch=gfx950', '-shared', '-o', tmp] + objs)
     os.replace(tmp, out)
     if verbose:
         print(log)

@@ -85,6 +85,10 @@ SIGNATURES = {
     'cmx_model_info': (ctypes.c_int, [ctypes.c_int32, _ip, _ip]),
     'cmx_model_state_name': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p)]),
     'cmx_model_initial_state': (ctypes.c_int, [ctypes.c_int32, _dp]),
+    'cmx_model_param': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), _dp]),
+    'cmx_set_params': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp]),
+    'cmx_adjoint': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                   _ip, ctypes.c_int32, _dp, _dp, _dp, _dp]),
     'cmx_slab_mesh': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                      ctypes.c_double, _ip, _ip, _dp, _ip]),
 }
@@ -144,6 +148,17 @@ def initial_state(model):
     y = np.zeros(S)
     _check(lib().cmx_model_initial_state(model, _p(y)))
     return y
+
+
+def param_info(model):
+    """[(name, default)] of the model's per-node parameters."""
+    _, P = model_info(model)
+    out = []
+    for q in range(P):
+        c, d = ctypes.c_char_p(), np.zeros(1)
+        _check(lib().cmx_model_param(model, q, ctypes.byref(c), _p(d)))
+        out.append((c.value.decode(), float(d[0])))
+    return out
 
 
 def slab_mesh(dim, ex, ey=0, ez=0, h=0.01):
@@ -322,6 +337,27 @@ class Monodomain:
         out = np.zeros(self.n)
         _check(lib().cmx_get_iionic(self.ctx, _p(out)), self.ctx)
         return out
+
+    def set_params(self, params):
+        """Per-node parameters [n_params, n] (AbstractParameterisedSystem::SetParameter
+        on every node); fewer planes leave the rest at the model defaults."""
+        pa = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        _check(lib().cmx_set_params(self.ctx, pa.shape[0], _p(pa)), self.ctx)
+
+    def adjoint(self, t0, n_steps, obs_nodes, obs_every, obs_values, checkpoint_every=100):
+        """Loss sum_j sum_o (V_o(t0 + j obs_every dt) - obs_values[j-1, o])^2 over
+        the n_steps steps from the current state, and its gradient with respect
+        to the per-node parameters [n_params, n] and the initial state
+        [n_states, n].  The context ends at t0 + n_steps dt."""
+        nodes = np.ascontiguousarray(obs_nodes, dtype=np.int32)
+        vals = np.ascontiguousarray(obs_values, dtype=np.float64).reshape(n_steps // obs_every, len(nodes))
+        loss = np.zeros(1)
+        gp = np.zeros((max(self.n_params, 1), self.n))
+        gs = np.zeros((self.n_states, self.n))
+        _check(lib().cmx_adjoint(self.ctx, float(t0), int(n_steps), int(checkpoint_every), len(nodes),
+                                 _p(nodes, _ip), int(obs_every), _p(vals), _p(loss),
+                                 _p(gp) if self.n_params else None, _p(gs)), self.ctx)
+        return float(loss[0]), gp[:self.n_params], gs
 
     def set_activation(self, threshold, every_steps=1, t_now=0.0):
         """ActivationOutputModifier on the device: processes the current voltage

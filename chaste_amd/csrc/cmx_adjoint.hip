@@ -1,0 +1,606 @@
+// Discrete adjoint of the FP64 monodomain time-step (ICI form, forward-Euler
+// or Rush-Larsen cells) behind cmx_adjoint.  The reference has no adjoint;
+// the forward map differentiated here is exactly the step the runtime takes
+// (cmx_device.hip), restated per node as
+//   y^{n+1} = Phi(y^n, V^n, p)          cell substeps from t_n to t_{n+1}, V clamped
+//   I^n     = Iion(y^{n+1}, V^n, p)     (AbstractCardiacTissue::SolveCellSystems)
+//   A V^{n+1} = M (c V^n - Am I^n - Istim(t_{n+1})),  c = Am Cm / dt,  A = c M + K
+// (MonodomainSolver::SetupLinearSystem, LinearSystem::Solve).  For the loss
+// L = sum over observation steps j and nodes o of (V^j_o - d^j_o)^2, with
+// lambda_V^n = dL/dV^n and lambda_y^n = dL/dy^n through the later steps:
+//   mu       = A^{-1} lambda_V^{n+1}     (A is symmetric: the transposed solve is a solve)
+//   l_z      = M mu
+//   lambda_y^n = J_y^T lambda_y^{n+1} - Am l_z dI/dy^n
+//   lambda_V^n = J_V^T lambda_y^{n+1} + l_z (c - Am dI/dV^n) + 2 (V^n - d^n) at observed nodes
+//   dL/dp   += J_p^T lambda_y^{n+1} - Am l_z dI/dp
+// The Jacobian columns of (Phi, Iion) come from forward-mode differentiation
+// of the generated cell model (one seeded input per pass, cmx_dual); the
+// states of each step are recomputed from checkpoints.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cmx_context.h"
+#include "models/model_dual.h"
+
+namespace cmx {
+namespace {
+
+constexpr int kAdjBlock = 256;
+constexpr int kAdjChunk = 16;       // CG iterations launched between convergence checks
+constexpr int kAdjMaxIts = 2000;
+
+// ---------------------------------------------------------------------------
+// Cell step of one node with V clamped, as cell_step_kernel's ICI branch
+// computes it for forward Euler and Rush-Larsen cells; returns Iion at the
+// new state.  R = double or cmx_dual.
+// ---------------------------------------------------------------------------
+template <class Model, class R, int IG>
+__device__ __forceinline__ R ici_cell_map(R* y, R V, const R* p, double cm, double am, const cmx_stimulus* stim,
+                                          int sid, double t, double t_next, double dt_ode, int n_sub)
+{
+    constexpr int NS = Model::NS;
+    constexpr bool RL = IG == CMX_RUSH_LARSEN;
+    const R cmr = R(cm);
+    y[0] = V;
+#pragma unroll 1
+    for (int ks = 0; ks < n_sub; ++ks) {
+        const double ts = t + ks * dt_ode;
+        const double h = (IG == CMX_FORWARD_EULER && ks + 1 == n_sub) ? t_next - ts : dt_ode;
+        const R st = R(sid ? stimulus_at(stim[sid - 1], ts) / am : 0.0);
+        R dy[NS], gi[NS], gk[NS];
+        Model::template eval<R, RL>(y, V, st, p, cmr, dy, gi, gk);
+        const R hh = R(h);
+#pragma unroll
+        for (int s = 1; s < NS; ++s) {
+            if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_exp(gk[s] * hh);
+            else y[s] += hh * dy[s];
+        }
+    }
+    return Model::template iion<R>(y, V, p, cmr);
+}
+
+struct CellAdjArgs {
+    int n, ps, n_sub;
+    double dt_ode, am, cm, c;          // c = Am Cm / dt
+    StepClock clock;                   // the run's clock (t0, dt, end)
+    int k;                             // the step k -> k+1
+    const double* __restrict__ Vn;     // V^n
+    const double* __restrict__ Yn;     // [S-1][ps] y^n
+    const double* __restrict__ params; // [P][ps] or null
+    const double* __restrict__ pdef;   // [P]
+    int n_param_planes;
+    const uint8_t* __restrict__ stim_id;
+    const cmx_stimulus* __restrict__ stim;
+    const double* __restrict__ lz;     // M mu
+    double* __restrict__ lam_y;        // in: lambda_y^{n+1}, out: lambda_y^n   [S-1][ps]
+    double* __restrict__ lam_v;        // out: lambda_V^n
+    double* __restrict__ grad;         // [P][ps] accumulated dL/dp, or null
+    const int32_t* __restrict__ obs_slot;  // [n]: observation column of node i, or -1
+    const double* __restrict__ obs_d;  // observed values at step n, or null when n is not observed
+};
+
+// One thread per node: the Jacobian columns of (Phi, Iion) with respect to
+// each state, V and each parameter, one forward-mode pass per column,
+// contracted with (lambda_y^{n+1}, -Am l_z).
+template <class Model, int IG>
+__global__ __launch_bounds__(kAdjBlock) void cell_adjoint_kernel(CellAdjArgs a)
+{
+    constexpr int NS = Model::NS;
+    constexpr int NP = Model::NP;
+    constexpr int NPd = NP > 0 ? NP : 1;
+    const int i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= a.n) return;
+    double y0[NS], ly[NS], p0[NPd];
+    y0[0] = 0.0;
+    ly[0] = 0.0;
+#pragma unroll
+    for (int s = 1; s < NS; ++s) {
+        y0[s] = a.Yn[(size_t)(s - 1) * a.ps + i];
+        ly[s] = a.lam_y[(size_t)(s - 1) * a.ps + i];
+    }
+#pragma unroll
+    for (int q = 0; q < NPd; ++q)
+        p0[q] = (q < a.n_param_planes) ? a.params[(size_t)q * a.ps + i] : (NP > 0 ? a.pdef[q] : 0.0);
+    const double vn = a.Vn[i];
+    const int sid = a.stim_id ? (int)a.stim_id[i] : 0;
+    // the step times exactly as the forward cell kernel evaluates them
+    const double t = step_time(a.clock, a.k), t_next = step_next_time(a.clock, a.k);
+    const double lz = a.lz[i];
+    const double w_iion = -a.am * lz;
+    double adj[NS + NPd];    // [1, NS): states, 0: V, NS + q: parameter q
+#pragma unroll 1
+    for (int k = 0; k < NS + NP; ++k) {
+        cmx_dual y[NS], p[NPd];
+#pragma unroll
+        for (int s = 1; s < NS; ++s) y[s] = cmx_dual(y0[s], s == k ? 1.0 : 0.0);
+#pragma unroll
+        for (int q = 0; q < NPd; ++q) p[q] = cmx_dual(p0[q], NS + q == k ? 1.0 : 0.0);
+        const cmx_dual V(vn, k == 0 ? 1.0 : 0.0);
+        const cmx_dual I = ici_cell_map<Model, cmx_dual, IG>(y, V, p, a.cm, a.am, a.stim, sid, t, t_next,
+                                                              a.dt_ode, a.n_sub);
+        double s_ = w_iion * I.d;
+#pragma unroll
+        for (int s = 1; s < NS; ++s) s_ += ly[s] * y[s].d;
+#pragma unroll
+        for (int kk = 0; kk < NS + NPd; ++kk)
+            if (kk == k) adj[kk] = s_;
+    }
+#pragma unroll
+    for (int s = 1; s < NS; ++s) a.lam_y[(size_t)(s - 1) * a.ps + i] = adj[s];
+    double lv = adj[0] + a.c * lz;
+    if (a.obs_d) {
+        const int o = a.obs_slot[i];
+        if (o >= 0) lv += 2.0 * (vn - a.obs_d[o]);
+    }
+    a.lam_v[i] = lv;
+    if (a.grad)
+#pragma unroll
+        for (int q = 0; q < NP; ++q) a.grad[(size_t)q * a.ps + i] += adj[NS + q];
+}
+
+// lambda_V at the last step: the observation term only; lambda_y = 0.
+__global__ void adjoint_final_kernel(int n, int ps, int ns1, const double* __restrict__ V,
+                                     const int32_t* __restrict__ obs_slot, const double* __restrict__ obs_d,
+                                     double* __restrict__ lam_v, double* __restrict__ lam_y)
+{
+    const int i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    double lv = 0.0;
+    if (obs_d && obs_slot[i] >= 0) lv = 2.0 * (V[i] - obs_d[obs_slot[i]]);
+    lam_v[i] = lv;
+    for (int s = 0; s < ns1; ++s) lam_y[(size_t)s * ps + i] = 0.0;
+}
+
+__global__ void gather_kernel(int m, const int32_t* __restrict__ idx, const double* __restrict__ V,
+                              double* __restrict__ out)
+{
+    const int o = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (o < m) out[o] = V[idx[o]];
+}
+
+// ---------------------------------------------------------------------------
+// FP64 Jacobi-PCG for A mu = b from mu = 0, stopping on the preconditioned
+// norm (||D^-1 r|| <= rtol ||D^-1 b||), two kernels per iteration with the
+// same fence-free slot reductions as the forward solver.  Iteration it:
+//   spmv:   beta = (r,z)_it / (r,z)_{it-1} (0 at it = 0),
+//           p' = z + beta p, q = A p' (p' gathered as z_j + beta p_j), (p', q)
+//   update: alpha = (r,z) / (p', q), x += alpha p', r -= alpha q, z = D^-1 r,
+//           (r,z), (z,z) for the next iteration
+// with p and p' in alternating buffers.
+// ---------------------------------------------------------------------------
+struct AdjCtl {
+    double tol2;
+    int32_t done;      // 1 converged
+    int32_t it;
+};
+
+template <class Rows>
+__global__ __launch_bounds__(kCgBlock) void adj_spmv_kernel(Rows R, const double* __restrict__ vals, int n,
+                                                            const double* __restrict__ x, double* __restrict__ y)
+{
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const RowRange rr(n, blk, gridDim.x);
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
+        const typename Rows::Row row = R.row(i);
+        double s = 0.0;
+        for (int k = 0; k < R.width(); ++k) s += vals[R.vidx(row, k)] * x[R.col(row, k)];
+        y[i] = s;
+    }
+}
+
+struct AdjCgVecs {
+    const double* __restrict__ b;
+    double* __restrict__ x;
+    double* __restrict__ r;
+    double* __restrict__ z;
+    double* __restrict__ q;
+    double* __restrict__ p_in;
+    double* __restrict__ p_out;
+};
+
+template <class Rows>
+__global__ __launch_bounds__(kCgBlock) void adj_cg_init_kernel(Rows R, const double* __restrict__ dinv, int n,
+                                                               AdjCgVecs v, double* __restrict__ pout)
+{
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const RowRange rr(n, blk, gridDim.x);
+    double part[3] = {0.0, 0.0, 0.0};
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
+        const double d = dinv[R.didx_of(i)];
+        const double bi = v.b[i], zi = d * bi;
+        v.x[i] = 0.0;
+        v.r[i] = bi;
+        v.z[i] = zi;
+        part[0] += bi * zi;
+        part[1] += zi * zi;
+        part[2] += zi * zi;
+    }
+    write_partials<kCgBlock, 3>(part, pout, blk);
+}
+
+template <class Rows>
+__global__ __launch_bounds__(kCgBlock) void adj_cg_spmv_kernel(Rows R, const double* __restrict__ avals, int n,
+                                                               AdjCgVecs v, int it, double rtol2, AdjCtl* ctl,
+                                                               double* __restrict__ hist,
+                                                               const double* __restrict__ pin, int G,
+                                                               double* __restrict__ pout)
+{
+    PartialLoad<3> pl(pin, G);
+    const int done = ctl->done;
+    const double tol2_set = ctl->tol2;
+    const double prev = it > 0 ? hist[it - 1] : 1.0;
+    double g[3];
+    pl.total(g);
+    if (done) return;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const bool lead = blk == 0 && threadIdx.x == 0;
+    const double tol2 = it == 0 ? rtol2 * g[2] : tol2_set;
+    if (lead) {
+        if (it == 0) ctl->tol2 = tol2;
+        hist[it] = g[0];
+        ctl->it = it;
+    }
+    if (g[1] <= tol2) {
+        if (lead) ctl->done = 1;
+        return;
+    }
+    const double beta = it == 0 ? 0.0 : g[0] / prev;
+    const RowRange rr(n, blk, gridDim.x);
+    double part[1] = {0.0};
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
+        const typename Rows::Row row = R.row(i);
+        double q = 0.0;
+        for (int k = 0; k < R.width(); ++k) {
+            const int j = R.col(row, k);
+            q += avals[R.vidx(row, k)] * (v.z[j] + beta * v.p_in[j]);
+        }
+        const double pi = v.z[i] + beta * v.p_in[i];
+        v.p_out[i] = pi;
+        v.q[i] = q;
+        part[0] += pi * q;
+    }
+    write_partials<kCgBlock, 1>(part, pout, blk);
+}
+
+template <class Rows>
+__global__ __launch_bounds__(kCgBlock) void adj_cg_update_kernel(Rows R, const double* __restrict__ dinv, int n,
+                                                                 AdjCgVecs v, const AdjCtl* ctl,
+                                                                 const double* __restrict__ prz,
+                                                                 const double* __restrict__ ppq, int G,
+                                                                 double* __restrict__ pout)
+{
+    PartialLoad<3> pl(prz, G);
+    PartialLoad<1> pq(ppq, G);
+    const int done = ctl->done;
+    double g[3], h[1];
+    pl.total(g);
+    pq.total(h);
+    if (done) return;
+    const double alpha = g[0] / h[0];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const RowRange rr(n, blk, gridDim.x);
+    double part[3] = {0.0, 0.0, 0.0};
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
+        const double pi = v.p_out[i];
+        v.x[i] += alpha * pi;
+        const double ri = v.r[i] - alpha * v.q[i];
+        const double zi = dinv[R.didx_of(i)] * ri;
+        v.r[i] = ri;
+        v.z[i] = zi;
+        part[0] += ri * zi;
+        part[1] += zi * zi;
+    }
+    write_partials<kCgBlock, 3>(part, pout, blk);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+int afail(cmx_ctx* c, int code, const std::string& msg)
+{
+    c->err = msg;
+    return code;
+}
+
+#define ADJ_HIP(ctx, call)                                                                        \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return afail(ctx, CMX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+// Device buffers of one cmx_adjoint call, freed on every return path.
+struct Scratch {
+    std::vector<void*> ptrs;
+    ~Scratch()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    int alloc(cmx_ctx* c, T** out, size_t count)
+    {
+        void* p = nullptr;
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return afail(c, CMX_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed (adjoint)");
+        }
+        ptrs.push_back(p);
+        *out = (T*)p;
+        return CMX_OK;
+    }
+};
+
+template <int W>
+StencilRows<W> adj_stencil_rows(cmx_ctx* c)
+{
+    StencilRows<W> R;
+    R.pid = c->pid;
+    R.n = c->n;
+    R.n_patterns = c->n_patterns;
+    R.pad = c->pad;
+    for (int k = 0; k < W; ++k) R.off[k] = c->h_off[k];
+    return R;
+}
+
+struct AdjBuffers {
+    double *lam_v, *lam_y, *lz, *grad;
+    double *b, *x, *r, *z, *q, *p[2];
+    double* partials;            // 3 buffers of [kMaxBlocks * 3]
+    double* hist;
+    AdjCtl* ctl;
+    AdjCtl* h_ctl;               // pinned mirror
+};
+
+// mu = A^{-1} b (b = lambda_V^{n+1}) into B.x, then l_z = M mu.
+template <class Rows>
+int adjoint_solve(cmx_ctx* c, const Rows& R, AdjBuffers& B, double rtol2)
+{
+    const int n = c->n, G = c->G;
+    double* prz[2] = {B.partials, B.partials + (size_t)kMaxBlocks * 3};
+    double* ppq = B.partials + 2 * (size_t)kMaxBlocks * 3;
+    ADJ_HIP(c, hipMemsetAsync(B.ctl, 0, sizeof(AdjCtl), c->stream));
+    AdjCgVecs v{B.b, B.x, B.r, B.z, B.q, B.p[0], B.p[1]};
+    adj_cg_init_kernel<Rows><<<G, kCgBlock, 0, c->stream>>>(R, c->dinv64, n, v, prz[0]);
+    ADJ_HIP(c, hipGetLastError());
+    int it = 0;
+    for (;;) {
+        for (int k = 0; k < kAdjChunk; ++k, ++it) {
+            AdjCgVecs vi{B.b, B.x, B.r, B.z, B.q, B.p[it & 1], B.p[(it + 1) & 1]};
+            adj_cg_spmv_kernel<Rows><<<G, kCgBlock, 0, c->stream>>>(R, c->a64, n, vi, it, rtol2, B.ctl, B.hist,
+                                                                     prz[it & 1], G, ppq);
+            adj_cg_update_kernel<Rows><<<G, kCgBlock, 0, c->stream>>>(R, c->dinv64, n, vi, B.ctl, prz[it & 1], ppq,
+                                                                       G, prz[(it + 1) & 1]);
+        }
+        ADJ_HIP(c, hipGetLastError());
+        ADJ_HIP(c, hipMemcpyAsync(B.h_ctl, B.ctl, sizeof(AdjCtl), hipMemcpyDeviceToHost, c->stream));
+        ADJ_HIP(c, hipStreamSynchronize(c->stream));
+        if (B.h_ctl->done) break;
+        if (it >= kAdjMaxIts)
+            return afail(c, CMX_E_DIVERGED, "adjoint CG did not converge in " + std::to_string(kAdjMaxIts) +
+                                                 " iterations");
+    }
+    adj_spmv_kernel<Rows><<<G, kCgBlock, 0, c->stream>>>(R, c->m64, n, B.x, B.lz);
+    ADJ_HIP(c, hipGetLastError());
+    return CMX_OK;
+}
+
+template <class Model, int IG>
+void launch_cell_adjoint(cmx_ctx* c, const CellAdjArgs& a)
+{
+    cell_adjoint_kernel<Model, IG><<<(c->n + kAdjBlock - 1) / kAdjBlock, kAdjBlock, 0, c->stream>>>(a);
+}
+
+// Slot layout of a stored step: V at [0, ps), the S-1 state planes after it.
+int save_slot(cmx_ctx* c, double* dst)
+{
+    ADJ_HIP(c, hipMemcpyAsync(dst, c->V, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    ADJ_HIP(c, hipMemcpyAsync(dst + c->ps, c->state, (size_t)(c->S - 1) * c->ps * sizeof(double),
+                              hipMemcpyDeviceToDevice, c->stream));
+    return CMX_OK;
+}
+
+int restore_slot(cmx_ctx* c, const double* src)
+{
+    ADJ_HIP(c, hipMemcpyAsync(c->V, src, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    ADJ_HIP(c, hipMemcpyAsync(c->state, src + c->ps, (size_t)(c->S - 1) * c->ps * sizeof(double),
+                              hipMemcpyDeviceToDevice, c->stream));
+    ADJ_HIP(c, hipStreamSynchronize(c->stream));
+    c->stepped = false;
+    return CMX_OK;
+}
+
+template <class Rows>
+int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n_obs, const int32_t* obs_nodes,
+                 int every, const double* obs_values, double* loss, double* grad_params, double* grad_state0)
+{
+    const int n = c->n, ps = c->ps, S = c->S, P = c->P;
+    const size_t slot = (size_t)S * ps;
+    const int n_ck = (n_steps + L - 1) / L;
+    const int n_obs_steps = n_steps / every;
+    Scratch sc;
+    int rc;
+    double *ck, *seg, *final_slot, *obs_v, *obs_d;
+    int32_t *d_obs_nodes, *obs_slot;
+    AdjBuffers B{};
+    if ((rc = sc.alloc(c, &ck, (size_t)n_ck * slot))) return rc;
+    if ((rc = sc.alloc(c, &seg, (size_t)L * slot))) return rc;
+    if ((rc = sc.alloc(c, &final_slot, slot))) return rc;
+    if ((rc = sc.alloc(c, &obs_v, (size_t)std::max(n_obs_steps, 1) * n_obs))) return rc;
+    if ((rc = sc.alloc(c, &obs_d, (size_t)std::max(n_obs_steps, 1) * n_obs))) return rc;
+    if ((rc = sc.alloc(c, &d_obs_nodes, n_obs))) return rc;
+    if ((rc = sc.alloc(c, &obs_slot, n))) return rc;
+    if ((rc = sc.alloc(c, &B.lam_v, n))) return rc;
+    if ((rc = sc.alloc(c, &B.lam_y, (size_t)(S - 1) * ps))) return rc;
+    if ((rc = sc.alloc(c, &B.lz, n))) return rc;
+    if ((rc = sc.alloc(c, &B.grad, (size_t)std::max(P, 1) * ps))) return rc;
+    if ((rc = sc.alloc(c, &B.x, n))) return rc;
+    if ((rc = sc.alloc(c, &B.r, n))) return rc;
+    if ((rc = sc.alloc(c, &B.z, n))) return rc;
+    if ((rc = sc.alloc(c, &B.q, n))) return rc;
+    if ((rc = sc.alloc(c, &B.p[0], n))) return rc;
+    if ((rc = sc.alloc(c, &B.p[1], n))) return rc;
+    if ((rc = sc.alloc(c, &B.partials, 3 * (size_t)kMaxBlocks * 3))) return rc;
+    if ((rc = sc.alloc(c, &B.hist, kAdjMaxIts + kAdjChunk + 1))) return rc;
+    if ((rc = sc.alloc(c, &B.ctl, 1))) return rc;
+    B.b = B.lam_v;
+    std::vector<int32_t> h_slot(n, -1);
+    for (int o = 0; o < n_obs; ++o) h_slot[obs_nodes[o]] = o;
+    if (n_obs > 0) ADJ_HIP(c, hipMemcpy(d_obs_nodes, obs_nodes, n_obs * sizeof(int32_t), hipMemcpyHostToDevice));
+    ADJ_HIP(c, hipMemcpy(obs_slot, h_slot.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (n_obs_steps > 0 && n_obs > 0)
+        ADJ_HIP(c, hipMemcpy(obs_d, obs_values, (size_t)n_obs_steps * n_obs * sizeof(double), hipMemcpyHostToDevice));
+    ADJ_HIP(c, hipMemset(B.grad, 0, (size_t)std::max(P, 1) * ps * sizeof(double)));
+    ADJ_HIP(c, hipMemset(B.p[0], 0, n * sizeof(double)));
+    ADJ_HIP(c, hipMemset(B.p[1], 0, n * sizeof(double)));
+    AdjCtl* h_ctl = nullptr;
+    ADJ_HIP(c, hipHostMalloc((void**)&h_ctl, sizeof(AdjCtl), hipHostMallocDefault));
+    struct PinnedFree {
+        AdjCtl* p;
+        ~PinnedFree() { (void)hipHostFree(p); }
+    } pinned{h_ctl};
+    B.h_ctl = h_ctl;
+
+    // Step m of the run cmx_step(ctx, t0, n_steps) takes.
+    auto step_at = [&](int m) { return step_range(c, t0, n_steps, m, m + 1); };
+    const int go = (n_obs + kAdjBlock - 1) / kAdjBlock;
+
+    // Forward: checkpoints every L steps and the observed voltages.
+    for (int m = 0; m < n_steps; ++m) {
+        if (m % L == 0 && (rc = save_slot(c, ck + (size_t)(m / L) * slot))) return rc;
+        if ((rc = step_at(m))) return rc;
+        if (n_obs > 0 && (m + 1) % every == 0) {
+            gather_kernel<<<go, kAdjBlock, 0, c->stream>>>(n_obs, d_obs_nodes, c->V,
+                                                           obs_v + (size_t)((m + 1) / every - 1) * n_obs);
+            ADJ_HIP(c, hipGetLastError());
+        }
+    }
+    if ((rc = save_slot(c, final_slot))) return rc;
+    std::vector<double> h_obs((size_t)n_obs_steps * n_obs);
+    if (!h_obs.empty())
+        ADJ_HIP(c, hipMemcpyAsync(h_obs.data(), obs_v, h_obs.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+    ADJ_HIP(c, hipStreamSynchronize(c->stream));
+    double L2 = 0.0;
+    for (size_t k = 0; k < h_obs.size(); ++k) {
+        const double e = h_obs[k] - obs_values[k];
+        L2 += e * e;
+    }
+    *loss = L2;
+
+    // Backward, one checkpoint segment at a time.
+    const double* d_at_end = (n_steps % every == 0 && n_obs_steps > 0)
+                                 ? obs_d + (size_t)(n_steps / every - 1) * n_obs : nullptr;
+    const int gc = (n + kAdjBlock - 1) / kAdjBlock;
+    adjoint_final_kernel<<<gc, kAdjBlock, 0, c->stream>>>(n, ps, S - 1, c->V, obs_slot, d_at_end, B.lam_v, B.lam_y);
+    ADJ_HIP(c, hipGetLastError());
+    const double rtol = c->rtol > 0.0 ? c->rtol : 1e-10;
+    CellAdjArgs a{};
+    a.n = n;
+    a.ps = ps;
+    a.n_sub = c->n_sub;
+    a.dt_ode = c->dt_ode;
+    a.am = c->am;
+    a.cm = c->cm;
+    a.c = c->am * c->cm / c->dt;
+    a.clock = run_clock(t0, c->dt, n_steps);
+    a.params = (const double*)c->params;
+    a.pdef = (const double*)c->pdef;
+    a.n_param_planes = c->n_param_planes;
+    a.stim_id = c->stim_id;
+    a.stim = c->stim;
+    a.lz = B.lz;
+    a.lam_y = B.lam_y;
+    a.lam_v = B.lam_v;
+    a.grad = P > 0 ? B.grad : nullptr;
+    a.obs_slot = obs_slot;
+    for (int s0 = (n_ck - 1) * L; s0 >= 0; s0 -= L) {
+        const int s1 = std::min(s0 + L, n_steps);
+        if ((rc = restore_slot(c, ck + (size_t)(s0 / L) * slot))) return rc;
+        for (int m = s0; m < s1; ++m) {
+            if ((rc = save_slot(c, seg + (size_t)(m - s0) * slot))) return rc;
+            if (m + 1 < s1 && (rc = step_at(m))) return rc;
+        }
+        for (int m = s1 - 1; m >= s0; --m) {
+            if ((rc = adjoint_solve<Rows>(c, R, B, rtol * rtol))) return rc;
+            const double* st = seg + (size_t)(m - s0) * slot;
+            a.Vn = st;
+            a.Yn = st + ps;
+            a.k = m;
+            a.obs_d = (m > 0 && m % every == 0) ? obs_d + (size_t)(m / every - 1) * n_obs : nullptr;
+            if (c->model == CMX_MODEL_LR91) {
+                if (c->integrator == CMX_RUSH_LARSEN) launch_cell_adjoint<ModelLr91, CMX_RUSH_LARSEN>(c, a);
+                else launch_cell_adjoint<ModelLr91, CMX_FORWARD_EULER>(c, a);
+            } else {
+                if (c->integrator == CMX_RUSH_LARSEN) launch_cell_adjoint<ModelTt06, CMX_RUSH_LARSEN>(c, a);
+                else launch_cell_adjoint<ModelTt06, CMX_FORWARD_EULER>(c, a);
+            }
+            ADJ_HIP(c, hipGetLastError());
+        }
+    }
+    ADJ_HIP(c, hipStreamSynchronize(c->stream));
+    if (grad_params && P > 0) {
+        std::vector<double> tmp((size_t)P * ps);
+        ADJ_HIP(c, hipMemcpy(tmp.data(), B.grad, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int q = 0; q < P; ++q) std::copy(tmp.begin() + (size_t)q * ps, tmp.begin() + (size_t)q * ps + n,
+                                              grad_params + (size_t)q * n);
+    }
+    if (grad_state0) {
+        ADJ_HIP(c, hipMemcpy(grad_state0, B.lam_v, n * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<double> tmp((size_t)(S - 1) * ps);
+        ADJ_HIP(c, hipMemcpy(tmp.data(), B.lam_y, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int s = 1; s < S; ++s) std::copy(tmp.begin() + (size_t)(s - 1) * ps, tmp.begin() + (size_t)(s - 1) * ps + n,
+                                              grad_state0 + (size_t)s * n);
+    }
+    // leave the context at the end of the run, as cmx_step(ctx, t0, n_steps) would
+    return restore_slot(c, final_slot);
+}
+
+}  // namespace
+}  // namespace cmx
+
+using namespace cmx;
+
+extern "C" {
+
+int cmx_adjoint(cmx_ctx* c, double t0, int32_t n_steps, int32_t checkpoint_every, int32_t n_obs,
+                const int32_t* obs_nodes, int32_t obs_every, const double* obs_values, double* loss,
+                double* grad_params, double* grad_state0)
+{
+    if (!c) return CMX_E_ARG;
+    if (!loss) return afail(c, CMX_E_ARG, "null loss pointer");
+    if (n_steps <= 0 || checkpoint_every <= 0 || obs_every <= 0)
+        return afail(c, CMX_E_ARG, "n_steps, checkpoint_every and obs_every must be positive");
+    if (n_obs < 0 || (n_obs > 0 && (!obs_nodes || !obs_values)))
+        return afail(c, CMX_E_ARG, "observation nodes and values are required");
+    for (int o = 0; o < n_obs; ++o)
+        if (obs_nodes[o] < 0 || obs_nodes[o] >= c->n) return afail(c, CMX_E_ARG, "observation node out of range");
+    for (int o = 0; o < n_obs; ++o)
+        for (int o2 = 0; o2 < o; ++o2)
+            if (obs_nodes[o2] == obs_nodes[o]) return afail(c, CMX_E_ARG, "observation nodes must be distinct");
+    if (c->precision != CMX_FP64) return afail(c, CMX_E_ARG, "the adjoint needs precision CMX_FP64");
+    if (c->integrator != CMX_FORWARD_EULER && c->integrator != CMX_RUSH_LARSEN)
+        return afail(c, CMX_E_ARG, "the adjoint supports forward-Euler and Rush-Larsen cells");
+    if (c->split || c->svi) return afail(c, CMX_E_ARG, "the adjoint supports the ICI step only (no splitting, no SVI)");
+    if (c->extrapolate) return afail(c, CMX_E_ARG, "the adjoint needs the reference's initial guess (extrapolate = 0)");
+    if (c->act_every > 0) return afail(c, CMX_E_STATE, "activation tracking must be off during cmx_adjoint");
+    if (c->format == CMX_FMT_STENCIL) {
+        if (c->width == 4) return adjoint_impl(c, adj_stencil_rows<4>(c), t0, n_steps, checkpoint_every, n_obs,
+                                               obs_nodes, obs_every, obs_values, loss, grad_params, grad_state0);
+        if (c->width == 8) return adjoint_impl(c, adj_stencil_rows<8>(c), t0, n_steps, checkpoint_every, n_obs,
+                                               obs_nodes, obs_every, obs_values, loss, grad_params, grad_state0);
+        return adjoint_impl(c, adj_stencil_rows<16>(c), t0, n_steps, checkpoint_every, n_obs, obs_nodes, obs_every,
+                            obs_values, loss, grad_params, grad_state0);
+    }
+    EllRows R{c->ell_col, c->width, c->n};
+    return adjoint_impl(c, R, t0, n_steps, checkpoint_every, n_obs, obs_nodes, obs_every, obs_values, loss,
+                        grad_params, grad_state0);
+}
+
+}  // extern "C"

@@ -1,0 +1,174 @@
+// The context behind the C ABI's opaque cmx_ctx and the cell-model wrappers,
+// shared by the time-step runtime (cmx_device.hip) and the adjoint
+// (cmx_adjoint.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <string>
+#include <vector>
+
+#include "cmx_internal.h"
+#include "cmx_kernels.h"
+#include "models/lr91.h"
+#include "models/tt06_epi.h"
+
+namespace cmx {
+
+// ---------------------------------------------------------------------------
+// Cell models
+// ---------------------------------------------------------------------------
+struct ModelLr91 {
+    static constexpr int NS = lr91::kNumStates;
+    static constexpr int NP = lr91::kNumParams;
+    template <class R, bool RL>
+    static __device__ __forceinline__ void eval(const R* y, R V, R st, const R* p, R cm, R* dy, R* gi, R* gk)
+    {
+        lr91::evaluate<R, RL>(y, V, st, p, cm, dy, gi, gk);
+    }
+    template <class R>
+    static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return lr91::iionic<R>(y, V, p, cm); }
+    template <class R>
+    static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return lr91::dvdt<R>(y, V, st, p, cm); }
+    template <class R>
+    static __device__ __forceinline__ int be_step(R* y, R V, R st, const R* p, R cm, R dt)
+    {
+        return lr91::be_step<R>(y, V, st, p, cm, dt);
+    }
+    template <class R, int ORDER>
+    static __device__ __forceinline__ void grl_step(R* y, R st, const R* p, R cm, R dt, bool update_v)
+    {
+        lr91::grl_step<R, ORDER>(y, st, p, cm, dt, update_v);
+    }
+    static constexpr bool gate(int i) { return lr91::kIsGate[i] != 0; }
+};
+
+struct ModelTt06 {
+    static constexpr int NS = tt06_epi::kNumStates;
+    static constexpr int NP = tt06_epi::kNumParams;
+    template <class R, bool RL>
+    static __device__ __forceinline__ void eval(const R* y, R V, R st, const R* p, R cm, R* dy, R* gi, R* gk)
+    {
+        tt06_epi::evaluate<R, RL>(y, V, st, p, cm, dy, gi, gk);
+    }
+    template <class R>
+    static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return tt06_epi::iionic<R>(y, V, p, cm); }
+    template <class R>
+    static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return tt06_epi::dvdt<R>(y, V, st, p, cm); }
+    template <class R>
+    static __device__ __forceinline__ int be_step(R* y, R V, R st, const R* p, R cm, R dt)
+    {
+        return tt06_epi::be_step<R>(y, V, st, p, cm, dt);
+    }
+    template <class R, int ORDER>
+    static __device__ __forceinline__ void grl_step(R* y, R st, const R* p, R cm, R dt, bool update_v)
+    {
+        tt06_epi::grl_step<R, ORDER>(y, st, p, cm, dt, update_v);
+    }
+    static constexpr bool gate(int i) { return tt06_epi::kIsGate[i] != 0; }
+};
+
+}  // namespace cmx
+
+struct cmx_ctx {
+    std::string err;
+    int n = 0, dim = 0, model = 0, integrator = 0, precision = 0, format = 0;
+    int S = 0, P = 0, n_param_planes = 0, param_planes_alloc = 0;
+    double dt = 0, dt_ode = 0, am = 0, cm = 0, rtol = 0, atol = 0;
+    int max_its = 0, refine = 0, n_sub = 1, use_graph = 0, extrapolate = 0;
+    int split = 0, n_half = 1;   // operator splitting: ODE substeps per half step
+    int width = 0, n_patterns = 0;
+    int G = 1;                   // blocks of the reduction kernels
+    long long nnz = 0;
+    hipStream_t stream = nullptr;
+    // device state
+    double* V = nullptr;
+    void* state = nullptr;       // [S-1][n] RS
+    void* params = nullptr;      // [P][n] RS
+    void* pdef = nullptr;        // [P] RS
+    uint8_t* stim_id = nullptr;
+    cmx_stimulus* stim = nullptr;
+    void* F = nullptr;
+    void* vec[11] = {};          // r w z s p m_a m_b x1 x2 x0 x_{n-2}
+    double* Y = nullptr;         // V^n + x0 (initial guess)
+    void* G1 = nullptr;          // F - c x0
+    double* r0_64 = nullptr;     // FP64 step residual (refinement)
+    double* iion = nullptr;      // scratch for cmx_get_iionic
+    // state-variable interpolation
+    int svi = 0;
+    void* svi_iion = nullptr;    // nodal Iion of the step (RV)
+    void* svi_out = nullptr;     // correction of b (RV)
+    int32_t* elems = nullptr;
+    double* edet = nullptr;
+    int32_t* inc_ptr = nullptr;
+    int32_t* inc = nullptr;
+    double qw[4] = {}, qp[4][3] = {};
+    int nq = 0;
+    double* act = nullptr;       // [5][n] activation times (cmx_set_activation)
+    double act_thr = 0.0;
+    int act_every = 0;
+    // matrix
+    uint8_t* pid = nullptr;
+    std::vector<int32_t> h_off;  // stencil offsets, padded to the kernel width
+    int pad = 0;                 // guard elements on each side of the gathered vectors
+    int ps = 0;                  // cell-state / parameter plane stride (elements)
+    int32_t* ell_col = nullptr;
+    double *m64 = nullptr, *k64 = nullptr, *a64 = nullptr, *dinv64 = nullptr;
+    float *a32 = nullptr, *dinv32 = nullptr;
+    double* partials = nullptr;  // 3 buffers of [kMaxBlocks * 3]
+    cmx::StepClock* clock = nullptr;
+    cmx::SolveCtl* ctl = nullptr;     // [2]: first solve, refinement solve
+    double2* hist = nullptr;
+    int32_t* its_log = nullptr;  // [batch][4] per-solve iterations, indexed by step % batch
+    // pinned host mirrors
+    cmx::StepClock* h_clock = nullptr;
+    cmx::SolveCtl* h_ctl = nullptr;
+    int32_t* h_its = nullptr;
+    int batch = 64;
+    // host copies for cmx_get_iionic
+    std::vector<cmx_stimulus> h_stim;
+    std::vector<uint8_t> h_stim_id;
+    double last_t_next = 0.0;
+    bool stepped = false;
+    // CG iteration budgets (first solve, refinement solve) per enqueued step
+    int budget = 8;
+    int refine_budget = 1;
+    // stats
+    long long steps = 0, cg_total = 0, refine_total = 0, resumes = 0;
+    int last_its = 0;
+    double last_residual = 0;
+    double device_bytes = 0;
+    std::vector<void*> allocs;
+    // hipGraphs of whole steps, keyed by (budget, refinement budget, steps)
+    struct StepGraph { int budget, refine_budget, steps, act_every; double act_thr; hipGraph_t graph; hipGraphExec_t exec; };
+    std::vector<StepGraph> graphs;
+    long long graph_launches = 0;
+    // kernel timing (cmx_set_profiling)
+    int prof_mask = 0;
+    struct Pending { int kind; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    double k_ms[CMX_K_COUNT] = {};
+    long long k_launches[CMX_K_COUNT] = {};
+};
+
+namespace cmx {
+
+// The clock of one run of n_total steps from t0 (TimeStepper(t0, t0 + n_total dt, dt)).
+inline StepClock run_clock(double t0, double dt, int n_total)
+{
+    StepClock ck{};
+    ck.t0 = t0;
+    ck.dt = dt;
+    ck.t_end = t0 + n_total * dt;
+    ck.eps = ck.t_end > 1.0 ? DBL_EPSILON * ck.t_end : DBL_EPSILON;
+    ck.k = 0;
+    ck.n_steps = n_total;
+    return ck;
+}
+
+// Steps k0 .. k1-1 of the run that cmx_step(c, t0, n_total) takes, with the
+// same step times (the adjoint replays a run step by step).
+int step_range(cmx_ctx* c, double t0, int n_total, int k0, int k1);
+
+}  // namespace cmx

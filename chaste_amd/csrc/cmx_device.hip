@@ -41,65 +41,9 @@
 #include <string>
 #include <vector>
 
-#include "cmx_internal.h"
-#include "cmx_kernels.h"
-#include "models/lr91.h"
-#include "models/tt06_epi.h"
+#include "cmx_context.h"
 
 namespace cmx {
-
-// ---------------------------------------------------------------------------
-// Cell models
-// ---------------------------------------------------------------------------
-struct ModelLr91 {
-    static constexpr int NS = lr91::kNumStates;
-    static constexpr int NP = lr91::kNumParams;
-    template <class R, bool RL>
-    static __device__ __forceinline__ void eval(const R* y, R V, R st, const R* p, R cm, R* dy, R* gi, R* gk)
-    {
-        lr91::evaluate<R, RL>(y, V, st, p, cm, dy, gi, gk);
-    }
-    template <class R>
-    static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return lr91::iionic<R>(y, V, p, cm); }
-    template <class R>
-    static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return lr91::dvdt<R>(y, V, st, p, cm); }
-    template <class R>
-    static __device__ __forceinline__ int be_step(R* y, R V, R st, const R* p, R cm, R dt)
-    {
-        return lr91::be_step<R>(y, V, st, p, cm, dt);
-    }
-    template <class R, int ORDER>
-    static __device__ __forceinline__ void grl_step(R* y, R st, const R* p, R cm, R dt, bool update_v)
-    {
-        lr91::grl_step<R, ORDER>(y, st, p, cm, dt, update_v);
-    }
-    static constexpr bool gate(int i) { return lr91::kIsGate[i] != 0; }
-};
-
-struct ModelTt06 {
-    static constexpr int NS = tt06_epi::kNumStates;
-    static constexpr int NP = tt06_epi::kNumParams;
-    template <class R, bool RL>
-    static __device__ __forceinline__ void eval(const R* y, R V, R st, const R* p, R cm, R* dy, R* gi, R* gk)
-    {
-        tt06_epi::evaluate<R, RL>(y, V, st, p, cm, dy, gi, gk);
-    }
-    template <class R>
-    static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return tt06_epi::iionic<R>(y, V, p, cm); }
-    template <class R>
-    static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return tt06_epi::dvdt<R>(y, V, st, p, cm); }
-    template <class R>
-    static __device__ __forceinline__ int be_step(R* y, R V, R st, const R* p, R cm, R dt)
-    {
-        return tt06_epi::be_step<R>(y, V, st, p, cm, dt);
-    }
-    template <class R, int ORDER>
-    static __device__ __forceinline__ void grl_step(R* y, R st, const R* p, R cm, R dt, bool update_v)
-    {
-        tt06_epi::grl_step<R, ORDER>(y, st, p, cm, dt, update_v);
-    }
-    static constexpr bool gate(int i) { return tt06_epi::kIsGate[i] != 0; }
-};
 
 // What settles a pending step (its final solve decided, V^{n+1} written):
 // the cell kernel of the next step, or the finish kernel at a batch's end.
@@ -486,17 +430,6 @@ template <class RV> __device__ __forceinline__ const RV* dinv_vals(const MatVals
 template <> __device__ __forceinline__ const double* dinv_vals<double>(const MatVals& m) { return m.dinv64; }
 template <> __device__ __forceinline__ const float* dinv_vals<float>(const MatVals& m) { return m.dinv32; }
 
-// Rows of one block: a contiguous chunk [lo, hi) of ceil(n/G) rows walked
-// with stride kCgBlock.
-struct RowRange {
-    int lo, hi;
-    __device__ __forceinline__ RowRange(int n, int blk, int G)
-    {
-        const int per = (n + G - 1) / G;
-        lo = blk * per;
-        hi = min(n, lo + per);
-    }
-};
 
 // CG work vectors (RV precision).  The stencil gathers only p (which holds
 // u0 = D^-1 r0 before the first iteration) and m; every other vector is read
@@ -1133,88 +1066,6 @@ __global__ __launch_bounds__(kBlock) void activation_init_kernel(int n, const do
 // Host runtime
 // ---------------------------------------------------------------------------
 using namespace cmx;
-
-struct cmx_ctx {
-    std::string err;
-    int n = 0, dim = 0, model = 0, integrator = 0, precision = 0, format = 0;
-    int S = 0, P = 0, n_param_planes = 0;
-    double dt = 0, dt_ode = 0, am = 0, cm = 0, rtol = 0, atol = 0;
-    int max_its = 0, refine = 0, n_sub = 1, use_graph = 0, extrapolate = 0;
-    int split = 0, n_half = 1;   // operator splitting: ODE substeps per half step
-    int width = 0, n_patterns = 0;
-    int G = 1;                   // blocks of the reduction kernels
-    long long nnz = 0;
-    hipStream_t stream = nullptr;
-    // device state
-    double* V = nullptr;
-    void* state = nullptr;       // [S-1][n] RS
-    void* params = nullptr;      // [P][n] RS
-    void* pdef = nullptr;        // [P] RS
-    uint8_t* stim_id = nullptr;
-    cmx_stimulus* stim = nullptr;
-    void* F = nullptr;
-    void* vec[11] = {};          // r w z s p m_a m_b x1 x2 x0 x_{n-2}
-    double* Y = nullptr;         // V^n + x0 (initial guess)
-    void* G1 = nullptr;          // F - c x0
-    double* r0_64 = nullptr;     // FP64 step residual (refinement)
-    double* iion = nullptr;      // scratch for cmx_get_iionic
-    // state-variable interpolation
-    int svi = 0;
-    void* svi_iion = nullptr;    // nodal Iion of the step (RV)
-    void* svi_out = nullptr;     // correction of b (RV)
-    int32_t* elems = nullptr;
-    double* edet = nullptr;
-    int32_t* inc_ptr = nullptr;
-    int32_t* inc = nullptr;
-    double qw[4] = {}, qp[4][3] = {};
-    int nq = 0;
-    double* act = nullptr;       // [5][n] activation times (cmx_set_activation)
-    double act_thr = 0.0;
-    int act_every = 0;
-    // matrix
-    uint8_t* pid = nullptr;
-    std::vector<int32_t> h_off;  // stencil offsets, padded to the kernel width
-    int pad = 0;                 // guard elements on each side of the gathered vectors
-    int ps = 0;                  // cell-state / parameter plane stride (elements)
-    int32_t* ell_col = nullptr;
-    double *m64 = nullptr, *k64 = nullptr, *a64 = nullptr, *dinv64 = nullptr;
-    float *a32 = nullptr, *dinv32 = nullptr;
-    double* partials = nullptr;  // 3 buffers of [kMaxBlocks * 3]
-    StepClock* clock = nullptr;
-    SolveCtl* ctl = nullptr;     // [2]: first solve, refinement solve
-    double2* hist = nullptr;
-    int32_t* its_log = nullptr;  // [batch][4] per-solve iterations, indexed by step % batch
-    // pinned host mirrors
-    StepClock* h_clock = nullptr;
-    SolveCtl* h_ctl = nullptr;
-    int32_t* h_its = nullptr;
-    int batch = 64;
-    // host copies for cmx_get_iionic
-    std::vector<cmx_stimulus> h_stim;
-    std::vector<uint8_t> h_stim_id;
-    double last_t_next = 0.0;
-    bool stepped = false;
-    // CG iteration budgets (first solve, refinement solve) per enqueued step
-    int budget = 8;
-    int refine_budget = 1;
-    // stats
-    long long steps = 0, cg_total = 0, refine_total = 0, resumes = 0;
-    int last_its = 0;
-    double last_residual = 0;
-    double device_bytes = 0;
-    std::vector<void*> allocs;
-    // hipGraphs of whole steps, keyed by (budget, refinement budget, steps)
-    struct StepGraph { int budget, refine_budget, steps, act_every; double act_thr; hipGraph_t graph; hipGraphExec_t exec; };
-    std::vector<StepGraph> graphs;
-    long long graph_launches = 0;
-    // kernel timing (cmx_set_profiling)
-    int prof_mask = 0;
-    struct Pending { int kind; hipEvent_t a, b; };
-    std::vector<Pending> pending;
-    std::vector<hipEvent_t> event_pool;
-    double k_ms[CMX_K_COUNT] = {};
-    long long k_launches[CMX_K_COUNT] = {};
-};
 
 namespace {
 
@@ -2036,6 +1887,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     if (c->n_param_planes > 0) {
         const size_t np = (size_t)c->n_param_planes;
         if ((rc = dalloc(c, (char**)&c->params, np * c->ps * rs))) return rc;
+        c->param_planes_alloc = c->n_param_planes;
         if ((rc = mixed ? to_device_planes<float>(c, c->params, tissue->params, np, n, c->ps)
                         : to_device_planes<double>(c, c->params, tissue->params, np, n, c->ps)))
             return rc;
@@ -2158,15 +2010,19 @@ int cmx_step(cmx_ctx* c, double t0, int32_t n_steps)
     if (!c) return fail(nullptr, CMX_E_ARG, "null context");
     if (n_steps < 0) return fail(c, CMX_E_ARG, "negative step count");
     if (n_steps == 0) return CMX_OK;
-    // TimeStepper(t0, t0 + n dt, dt): t_k = t0 + k dt, the end snapped
-    // (global/src/TimeStepper.cpp:106-121); evaluated on the device.
-    StepClock ck{};
-    ck.t0 = t0;
-    ck.dt = c->dt;
-    ck.t_end = t0 + n_steps * c->dt;
-    ck.eps = ck.t_end > 1.0 ? DBL_EPSILON * ck.t_end : DBL_EPSILON;
-    ck.k = 0;
-    ck.n_steps = n_steps;
+    return cmx::step_range(c, t0, n_steps, 0, n_steps);
+}
+
+}  // extern "C"
+
+// Steps k0 .. k1-1 of the run cmx_step(c, t0, n_total) takes, with its step
+// times: TimeStepper(t0, t0 + n_total dt, dt), t_k = t0 + k dt, the end
+// snapped (global/src/TimeStepper.cpp:106-121), evaluated on the device.
+int cmx::step_range(cmx_ctx* c, double t0, int n_total, int k0, int k1)
+{
+    StepClock ck = run_clock(t0, c->dt, n_total);
+    ck.k = k0;
+    ck.n_steps = k1;
     *c->h_clock = ck;
     CMX_HIP(c, hipMemcpyAsync(c->clock, c->h_clock, sizeof(StepClock), hipMemcpyHostToDevice, c->stream));
     CMX_HIP(c, hipStreamSynchronize(c->stream));
@@ -2174,9 +2030,11 @@ int cmx_step(cmx_ctx* c, double t0, int32_t n_steps)
     if (rc) return rc;
     // with operator splitting F carries no ionic current: Iion is evaluated at the state
     c->stepped = !c->split;
-    c->last_t_next = ck.t_end;
+    c->last_t_next = k1 == n_total ? ck.t_end : t0 + k1 * c->dt;
     return CMX_OK;
 }
+
+extern "C" {
 
 int cmx_get_voltage(cmx_ctx* c, double* V)
 {
@@ -2333,6 +2191,42 @@ int cmx_model_info(int32_t model, int32_t* n_states, int32_t* n_params)
     if (model_sizes(model, &S, &P)) return fail(nullptr, CMX_E_ARG, "unknown cell model");
     if (n_states) *n_states = S;
     if (n_params) *n_params = P;
+    return CMX_OK;
+}
+
+int cmx_model_param(int32_t model, int32_t index, const char** name, double* default_value)
+{
+    int S, P;
+    if (model_sizes(model, &S, &P) || index < 0 || index >= P) return fail(nullptr, CMX_E_ARG, "bad model/parameter index");
+    if (name) *name = model == CMX_MODEL_LR91 ? lr91::kParamNames[index] : tt06_epi::kParamNames[index];
+    if (default_value) *default_value = model_pdef(model)[index];
+    return CMX_OK;
+}
+
+int cmx_set_params(cmx_ctx* c, int32_t n_params, const double* params)
+{
+    if (!c) return fail(nullptr, CMX_E_ARG, "null context");
+    if (n_params < 0 || n_params > c->P || (n_params > 0 && !params))
+        return fail(c, CMX_E_ARG, "n_params must be in [0, the model's parameter count] with params given");
+    CMX_HIP(c, hipStreamSynchronize(c->stream));
+    const bool mixed = c->precision == CMX_MIXED;
+    if (n_params > c->param_planes_alloc) {
+        int rc = dalloc(c, (char**)&c->params, (size_t)n_params * c->ps * (mixed ? sizeof(float) : sizeof(double)));
+        if (rc) return rc;
+        c->param_planes_alloc = n_params;
+    }
+    if (n_params > 0) {
+        int rc = mixed ? to_device_planes<float>(c, c->params, params, n_params, c->n, c->ps)
+                       : to_device_planes<double>(c, c->params, params, n_params, c->n, c->ps);
+        if (rc) return rc;
+    }
+    c->n_param_planes = n_params;
+    // captured steps hold the parameter planes' address and count
+    for (auto& g : c->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
+    c->graphs.clear();
     return CMX_OK;
 }
 

@@ -73,6 +73,18 @@ __device__ __forceinline__ int xcd_block(int b, int G)
     return x * q + (x < r ? x : r) + k;
 }
 
+// Rows of one block: a contiguous chunk [lo, hi) of ceil(n/G) rows walked
+// with stride kCgBlock.
+struct RowRange {
+    int lo, hi;
+    __device__ __forceinline__ RowRange(int n, int blk, int G)
+    {
+        const int per = (n + G - 1) / G;
+        lo = blk * per;
+        hi = min(n, lo + per);
+    }
+};
+
 constexpr int kMaxPatterns = 64;   // stencil format: distinct row patterns
 constexpr int kMaxStencil = 16;    // stencil format: offsets per row (padded width)
 

@@ -7,7 +7,7 @@ namespace cmx { namespace lr91 {
 
 constexpr int kNumStates = 8;
 constexpr int kVIndex = 0;
-constexpr int kNumParams = 0;
+constexpr int kNumParams = 3;
 constexpr bool kPerCapacitanceUnits = false;
 
 // State variables (CellML component__variable), V first then document order.
@@ -31,11 +31,15 @@ static constexpr double kInitialState[kNumStates] = {
     0.16647703,  // time_dependent_potassium_current_X_gate__X
     0.0002,  // intracellular_calcium_concentration__Cai
 };
-static const char* const kParamNames[1] = {
-    "",
+static const char* const kParamNames[3] = {
+    "fast_sodium_current__g_Na",
+    "time_dependent_potassium_current__g_Kmax",
+    "slow_inward_current__P_si",
 };
-static constexpr double kParamDefaults[1] = {
-    0.0,
+static constexpr double kParamDefaults[3] = {
+    23.0,
+    0.282,
+    0.09,
 };
 // 1 = integrated by Rush-Larsen (affine gate, V-dependent coefficients only)
 static constexpr int kIsGate[kNumStates] = {0, 1, 1, 1, 1, 1, 1, 0};
@@ -50,7 +54,6 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
 {
     CMX_MATH_USING
     (void)p; (void)cm; (void)g_inf; (void)g_k; (void)i_stim_area;
-    const R slow_inward_current__P_si = R(0.09);
     const R i_stim = i_stim_area;
     (void)i_stim;
     const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.135) * cmx_exp(((R(80.0) + V) * (R(1) / (-R(6.8))))))) : (R(0.0)));
@@ -60,7 +63,7 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R fast_sodium_current_m_gate__alpha_m = cmx_remove_singularity(V, R(-47.12999999999975), R(9.999999999999947e-07), [&](const R V) { return cmx_div((R(0.32) * (V + R(47.13))), (-cmx_expm1(((-R(0.1)) * (V + R(47.13)))))); });
     const R fast_sodium_current_m_gate__beta_m = (R(0.08) * cmx_exp(((-V) * (R(1) / R(11.0)))));
     const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((y[7] * (R(1) / R(1.0))))));
-    const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
+    const R slow_inward_current__i_si = (p[2] * y[4] * y[5] * (V - slow_inward_current__E_si));
     const R slow_inward_current_d_gate__alpha_d = cmx_div((R(0.095) * cmx_exp(((-R(0.01)) * (V - R(5.0))))), (R(1.0) + cmx_exp(((-R(0.072)) * (V - R(5.0))))));
     const R slow_inward_current_d_gate__beta_d = cmx_div((R(0.07) * cmx_exp(((-R(0.017)) * (V + R(44.0))))), (R(1.0) + cmx_exp((R(0.05) * (V + R(44.0))))));
     const R slow_inward_current_f_gate__alpha_f = cmx_div((R(0.012) * cmx_exp(((-R(0.008)) * (V + R(28.0))))), (R(1.0) + cmx_exp((R(0.15) * (V + R(28.0))))));
@@ -116,9 +119,6 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     const R membrane__T = R(310.0);
     const R membrane__F = R(96484.6);
     const R membrane__C = R(1.0);
-    const R fast_sodium_current__g_Na = R(23.0);
-    const R slow_inward_current__P_si = R(0.09);
-    const R time_dependent_potassium_current__g_Kmax = R(0.282);
     const R time_dependent_potassium_current__PR_NaK = R(0.01833);
     const R plateau_potassium_current__g_Kp = R(0.0183);
     const R background_current__E_b = R(-59.87);
@@ -131,15 +131,15 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     (void)i_stim;
     const R background_current__i_b = (background_current__g_b * (V - background_current__E_b));
     const R fast_sodium_current__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log((ionic_concentrations__Nao * (R(1) / ionic_concentrations__Nai))));
-    const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[1]) * y[2] * y[3] * (V - fast_sodium_current__E_Na));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[1]) * y[2] * y[3] * (V - fast_sodium_current__E_Na));
     const R time_independent_potassium_current__E_K1 = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log((ionic_concentrations__Ko * (R(1) / ionic_concentrations__Ki))));
     const R plateau_potassium_current__E_Kp = time_independent_potassium_current__E_K1;
     const R plateau_potassium_current__Kp = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(7.488) - V) * (R(1) / R(5.98))))));
     const R plateau_potassium_current__i_Kp = (plateau_potassium_current__g_Kp * plateau_potassium_current__Kp * (V - plateau_potassium_current__E_Kp));
     const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((y[7] * (R(1) / R(1.0))))));
-    const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
+    const R slow_inward_current__i_si = (p[2] * y[4] * y[5] * (V - slow_inward_current__E_si));
     const R time_dependent_potassium_current__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(((ionic_concentrations__Ko + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nao)) * (R(1) / (ionic_concentrations__Ki + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nai))))));
-    const R time_dependent_potassium_current__g_K = (time_dependent_potassium_current__g_Kmax * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
+    const R time_dependent_potassium_current__g_K = (p[1] * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
     const R time_dependent_potassium_current_Xi_gate__Xi = cmx_remove_singularity(V, R(-76.99999999999993), R(2.4999999999999977e-06), [&](const R V) { return (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * cmx_expm1((R(0.04) * (V + R(77.0))))), ((V + R(77.0)) * cmx_exp((R(0.04) * (V + R(35.0))))))) : (R(1.0))); });
     const R time_dependent_potassium_current__i_K = (time_dependent_potassium_current__g_K * y[6] * time_dependent_potassium_current_Xi_gate__Xi * (V - time_dependent_potassium_current__E_K));
     const R time_independent_potassium_current__g_K1 = (R(0.6047) * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
@@ -161,9 +161,6 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     const R membrane__T = R(310.0);
     const R membrane__F = R(96484.6);
     const R membrane__C = R(1.0);
-    const R fast_sodium_current__g_Na = R(23.0);
-    const R slow_inward_current__P_si = R(0.09);
-    const R time_dependent_potassium_current__g_Kmax = R(0.282);
     const R time_dependent_potassium_current__PR_NaK = R(0.01833);
     const R plateau_potassium_current__g_Kp = R(0.0183);
     const R background_current__E_b = R(-59.87);
@@ -176,15 +173,15 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     (void)i_stim;
     const R background_current__i_b = (background_current__g_b * (V - background_current__E_b));
     const R fast_sodium_current__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log((ionic_concentrations__Nao * (R(1) / ionic_concentrations__Nai))));
-    const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[1]) * y[2] * y[3] * (V - fast_sodium_current__E_Na));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[1]) * y[2] * y[3] * (V - fast_sodium_current__E_Na));
     const R time_independent_potassium_current__E_K1 = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log((ionic_concentrations__Ko * (R(1) / ionic_concentrations__Ki))));
     const R plateau_potassium_current__E_Kp = time_independent_potassium_current__E_K1;
     const R plateau_potassium_current__Kp = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(7.488) - V) * (R(1) / R(5.98))))));
     const R plateau_potassium_current__i_Kp = (plateau_potassium_current__g_Kp * plateau_potassium_current__Kp * (V - plateau_potassium_current__E_Kp));
     const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((y[7] * (R(1) / R(1.0))))));
-    const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
+    const R slow_inward_current__i_si = (p[2] * y[4] * y[5] * (V - slow_inward_current__E_si));
     const R time_dependent_potassium_current__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(((ionic_concentrations__Ko + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nao)) * (R(1) / (ionic_concentrations__Ki + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nai))))));
-    const R time_dependent_potassium_current__g_K = (time_dependent_potassium_current__g_Kmax * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
+    const R time_dependent_potassium_current__g_K = (p[1] * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
     const R time_dependent_potassium_current_Xi_gate__Xi = cmx_remove_singularity(V, R(-76.99999999999993), R(2.4999999999999977e-06), [&](const R V) { return (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * cmx_expm1((R(0.04) * (V + R(77.0))))), ((V + R(77.0)) * cmx_exp((R(0.04) * (V + R(35.0))))))) : (R(1.0))); });
     const R time_dependent_potassium_current__i_K = (time_dependent_potassium_current__g_K * y[6] * time_dependent_potassium_current_Xi_gate__Xi * (V - time_dependent_potassium_current__E_K));
     const R time_independent_potassium_current__g_K1 = (R(0.6047) * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
@@ -234,11 +231,10 @@ CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const
 {
     CMX_MATH_USING
     (void)y; (void)p; (void)cm; (void)i_stim_area;
-    const R slow_inward_current__P_si = R(0.09);
     const R i_stim = i_stim_area;
     (void)i_stim;
     const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((g[0] * (R(1) / R(1.0))))));
-    const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
+    const R slow_inward_current__i_si = (p[2] * y[4] * y[5] * (V - slow_inward_current__E_si));
     res[0] = g[0] - y[7] - dt * ((((-R(0.0001)) * (R(1) / R(1.0))) * slow_inward_current__i_si) + (R(0.07) * (R(0.0001) - g[0])));
 }
 
@@ -248,13 +244,12 @@ CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const
 {
     CMX_MATH_USING
     (void)y; (void)p; (void)cm; (void)i_stim_area;
-    const R slow_inward_current__P_si = R(0.09);
     const R i_stim = i_stim_area;
     (void)i_stim;
     const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((g[0] * (R(1) / R(1.0))))));
-    const R slow_inward_current__i_si = (slow_inward_current__P_si * y[4] * y[5] * (V - slow_inward_current__E_si));
+    const R slow_inward_current__i_si = (p[2] * y[4] * y[5] * (V - slow_inward_current__E_si));
     const R d_slow_inward_current__E_si__d0 = (-((cmx_div(cmx_div(R(1), R(1.0)), (g[0] * (R(1) / R(1.0)))) * R(13.0287))));
-    const R d_slow_inward_current__i_si__d0 = (((-d_slow_inward_current__E_si__d0) * slow_inward_current__P_si * y[4] * y[5]));
+    const R d_slow_inward_current__i_si__d0 = (((-d_slow_inward_current__E_si__d0) * p[2] * y[4] * y[5]));
     J[0] = R(1) - dt * (((d_slow_inward_current__i_si__d0 * ((-R(0.0001)) * (R(1) / R(1.0))))) + (((-R(1)) * R(0.07))));
 }
 

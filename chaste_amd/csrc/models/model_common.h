@@ -47,9 +47,10 @@ CMX_HD inline float cmx_expm1(float x) { return std::expm1(x); }
 template <class R, class F>
 CMX_HD inline R cmx_remove_singularity(R V, R c, R dv, F f)
 {
-    const R ulp4 = R(4) * std::fabs(c) * std::numeric_limits<R>::epsilon();
+    using std::fabs;
+    const R ulp4 = R(4) * fabs(c) * std::numeric_limits<R>::epsilon();
     const R d = dv > ulp4 ? dv : ulp4;
-    if (std::fabs(V - c) < d) {
+    if (fabs(V - c) < d) {
         const R f0 = f(c - d), f1 = f(c + d);
         return f0 + (V - (c - d)) * (f1 - f0) / (R(2) * d);
     }

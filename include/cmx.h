@@ -178,6 +178,36 @@ int cmx_get_kernel_times(cmx_ctx* ctx, double* total_ms, int64_t* launches, int3
 int cmx_model_info(int32_t model, int32_t* n_states, int32_t* n_params);
 int cmx_model_state_name(int32_t model, int32_t index, const char** name);
 int cmx_model_initial_state(int32_t model, double* y);
+/* Parameter `index` of a model: its CellML name (component__variable) and
+ * default value.  LR91's parameters are the three conductances its CellML
+ * annotates with the metadata the reference's generator turns into modifiable
+ * parameters: g_Na, g_Kmax, P_si (in that order). */
+int cmx_model_param(int32_t model, int32_t index, const char** name, double* default_value);
+
+/* Per-node parameter planes [n_params][n_nodes] in place of those given at
+ * create, as AbstractParameterisedSystem::SetParameter on every node's cell
+ * (ode/src/common/AbstractParameterisedSystem.cpp:256-275); planes beyond
+ * n_params take the model defaults. */
+int cmx_set_params(cmx_ctx* ctx, int32_t n_params, const double* params);
+
+/* Gradient of the voltage-mismatch loss
+ *   L = sum_{j=1..n_steps/obs_every} sum_o (V_{obs_nodes[o]}(t0 + j obs_every dt) - obs_values[j-1][o])^2
+ * over the n_steps PDE steps the context takes from its current state at t0
+ * (as cmx_step(ctx, t0, n_steps) takes them) with respect to every per-node
+ * cell parameter, grad_params[n_params][n_nodes] (may be NULL), and to the
+ * initial state, grad_state0[n_states][n_nodes] (V plane first; may be NULL).
+ * The reference has no adjoint: this is the discrete adjoint of its FP64 ICI
+ * step (cell-step Jacobian columns by forward-mode differentiation of the
+ * generated model, one symmetric diffusion solve per step, FP64 Jacobi-PCG
+ * to the context's rtol, 1e-10 when it is 0), recomputing the forward states
+ * from checkpoints saved every checkpoint_every steps (device memory about
+ * (n_steps/checkpoint_every + checkpoint_every) x n_states x n_nodes doubles).
+ * Needs CMX_FP64, forward-Euler or Rush-Larsen cells, no splitting, no SVI,
+ * extrapolate = 0 and activation tracking off.  On return the context holds
+ * the state at t0 + n_steps dt. */
+int cmx_adjoint(cmx_ctx* ctx, double t0, int32_t n_steps, int32_t checkpoint_every, int32_t n_obs,
+                const int32_t* obs_nodes, int32_t obs_every, const double* obs_values, double* loss,
+                double* grad_params, double* grad_state0);
 
 /* Reads a Triangle/TetGen mesh <base>.node / <base>.ele, ASCII (items numbered
  * from 0 or 1, '#' comments) or binary (BIN header tag), as

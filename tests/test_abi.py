@@ -41,7 +41,11 @@ def test_slab_mesh_matches_oracle(oracle):
 
 def test_model_metadata():
     from chaste_amd import cmx
-    assert cmx.model_info(cmx.LR91) == (8, 0)
+    assert cmx.model_info(cmx.LR91) == (8, 3)
+    # the conductances LuoRudy1991.cellml annotates as modifiable parameters
+    assert cmx.param_info(cmx.LR91) == [('fast_sodium_current__g_Na', 23.0),
+                                        ('time_dependent_potassium_current__g_Kmax', 0.282),
+                                        ('slow_inward_current__P_si', 0.09)]
     assert cmx.model_info(cmx.TT06_EPI)[0] == 19
     names = cmx.state_names(cmx.LR91)
     assert names[0] == 'membrane__V' and names[-1] == 'intracellular_calcium_concentration__Cai'
