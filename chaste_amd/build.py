@@ -13,7 +13,7 @@ HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
              '-I' + os.path.join(ROOT, 'include'), '-I' + CSRC]
 
-SOURCES = ['cmx_device.hip', 'cmx_adjoint.hip', 'cmx_host.cpp', 'cmx_io.cpp']
+SOURCES = ['cmx_device.hip', 'cmx_adjoint.hip', 'cmx_cells.hip', 'cmx_host.cpp', 'cmx_io.cpp']
 
 
 CPP_TESTS = ['TestMonodomainProblem']

@@ -48,6 +48,18 @@ class CmxOpts(ctypes.Structure):
                 ('state_variable_interpolation', ctypes.c_int32)]
 
 
+class CmxCells(ctypes.Structure):
+    _fields_ = [('model', ctypes.c_int32), ('integrator', ctypes.c_int32), ('precision', ctypes.c_int32),
+                ('n_cells', ctypes.c_int32), ('dt', ctypes.c_double), ('t_end', _dp),
+                ('stimuli', ctypes.POINTER(CmxStimulus)), ('n_params', ctypes.c_int32), ('params', _dp),
+                ('threshold', ctypes.c_double), ('steps_per_launch', ctypes.c_int32)]
+
+
+class CmxCellsStats(ctypes.Structure):
+    _fields_ = [('kernel_ms', ctypes.c_double), ('cell_steps', ctypes.c_int64), ('launches', ctypes.c_int32),
+                ('block', ctypes.c_int32)]
+
+
 class CmxStats(ctypes.Structure):
     _fields_ = [('steps', ctypes.c_int64), ('cg_iterations', ctypes.c_int64),
                 ('refine_sweeps', ctypes.c_int64), ('last_iterations', ctypes.c_int32),
@@ -87,6 +99,8 @@ SIGNATURES = {
     'cmx_model_initial_state': (ctypes.c_int, [ctypes.c_int32, _dp]),
     'cmx_model_param': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), _dp]),
     'cmx_set_params': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp]),
+    'cmx_cells_run': (ctypes.c_int, [ctypes.POINTER(CmxCells), _dp, _dp, _dp, ctypes.POINTER(CmxCellsStats)]),
+    'cmx_cells_last_error': (ctypes.c_char_p, []),
     'cmx_adjoint': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                    _ip, ctypes.c_int32, _dp, _dp, _dp, _dp]),
     'cmx_slab_mesh': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -159,6 +173,42 @@ def param_info(model):
         _check(lib().cmx_model_param(model, q, ctypes.byref(c), _p(d)))
         out.append((c.value.decode(), float(d[0])))
     return out
+
+
+def cells_run(model, integrator, t_end, dt, stimuli=None, params=None, y0=None, precision=FP64,
+              threshold=-30.0, steps_per_launch=0):
+    """Batched single cells (cmx_cells_run).  t_end [n]; stimuli: one
+    (magnitude, start, duration[, period[, stop]]) per cell or None; params
+    [n_params, n].  Returns (final states [S, n], props [4, n]: completed APs,
+    last onset, last peak, last end; stats dict)."""
+    t_end = np.ascontiguousarray(t_end, dtype=np.float64)
+    n = t_end.shape[0]
+    S, _ = model_info(model)
+    c = CmxCells()
+    c.model, c.integrator, c.precision, c.n_cells, c.dt = model, integrator, precision, n, dt
+    c.t_end = _p(t_end)
+    keep = [t_end]
+    if stimuli is not None:
+        st = (CmxStimulus * n)()
+        for k, row in enumerate(stimuli):
+            st[k] = CmxStimulus(row[0], row[1], row[2], row[3] if len(row) > 3 else -1.0,
+                                row[4] if len(row) > 4 else 0.0)
+        c.stimuli = st
+        keep.append(st)
+    if params is not None:
+        pa = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        c.n_params, c.params = pa.shape[0], _p(pa)
+        keep.append(pa)
+    c.threshold, c.steps_per_launch = threshold, steps_per_launch
+    yi = None if y0 is None else np.ascontiguousarray(y0, dtype=np.float64)
+    y = np.zeros((S, n))
+    props = np.zeros((4, n))
+    stats = CmxCellsStats()
+    rc = lib().cmx_cells_run(ctypes.byref(c), _p(yi) if yi is not None else None, _p(y), _p(props),
+                             ctypes.byref(stats))
+    if rc != 0:
+        raise CmxError(rc, lib().cmx_cells_last_error().decode())
+    return y, props, {k: getattr(stats, k) for k, _ in CmxCellsStats._fields_}
 
 
 def slab_mesh(dim, ex, ey=0, ez=0, h=0.01):

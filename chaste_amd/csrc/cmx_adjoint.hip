@@ -471,13 +471,16 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
     auto step_at = [&](int m) { return step_range(c, t0, n_steps, m, m + 1); };
     const int go = (n_obs + kAdjBlock - 1) / kAdjBlock;
 
-    // Forward: checkpoints every L steps and the observed voltages.
-    for (int m = 0; m < n_steps; ++m) {
+    // Forward: checkpoints every L steps and the observed voltages, stepping
+    // from one of those points to the next in one call.
+    for (int m = 0; m < n_steps;) {
         if (m % L == 0 && (rc = save_slot(c, ck + (size_t)(m / L) * slot))) return rc;
-        if ((rc = step_at(m))) return rc;
-        if (n_obs > 0 && (m + 1) % every == 0) {
+        const int next = std::min({(m / L + 1) * L, (m / every + 1) * every, n_steps});
+        if ((rc = step_range(c, t0, n_steps, m, next))) return rc;
+        m = next;
+        if (n_obs > 0 && m % every == 0) {
             gather_kernel<<<go, kAdjBlock, 0, c->stream>>>(n_obs, d_obs_nodes, c->V,
-                                                           obs_v + (size_t)((m + 1) / every - 1) * n_obs);
+                                                           obs_v + (size_t)(m / every - 1) * n_obs);
             ADJ_HIP(c, hipGetLastError());
         }
     }

@@ -167,6 +167,22 @@ inline StepClock run_clock(double t0, double dt, int n_total)
     return ck;
 }
 
+// SimpleStimulus pads its duration by (duration + start) eps and
+// RegularStimulus by period eps (heart/src/stimulus/SimpleStimulus.cpp:44-52,
+// RegularStimulus.cpp:40-56); a stop time <= 0 means none.  False when a
+// regular stimulus lasts longer than its period.
+inline bool pad_stimulus(cmx_stimulus& s)
+{
+    if (s.period > 0) {
+        if (s.duration > s.period) return false;
+        s.duration += s.period * DBL_EPSILON;
+    } else {
+        s.duration += (s.duration + s.start) * DBL_EPSILON;
+    }
+    if (!(s.stop > 0)) s.stop = DBL_MAX;
+    return true;
+}
+
 // Steps k0 .. k1-1 of the run that cmx_step(c, t0, n_total) takes, with the
 // same step times (the adjoint replays a run step by step).
 int step_range(cmx_ctx* c, double t0, int n_total, int k0, int k1);

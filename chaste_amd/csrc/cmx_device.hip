@@ -1896,13 +1896,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->h_stim.resize(tissue->n_stimuli);
     for (int k = 0; k < tissue->n_stimuli; ++k) {
         cmx_stimulus s = tissue->stimuli[k];
-        if (s.period > 0) {
-            if (s.duration > s.period) return fail(c, CMX_E_ARG, "stimulus duration exceeds its period");
-            s.duration += s.period * DBL_EPSILON;
-        } else {
-            s.duration += (s.duration + s.start) * DBL_EPSILON;
-        }
-        if (!(s.stop > 0)) s.stop = DBL_MAX;
+        if (!pad_stimulus(s)) return fail(c, CMX_E_ARG, "stimulus duration exceeds its period");
         c->h_stim[k] = s;
     }
     c->h_stim_id.assign(n, 0);

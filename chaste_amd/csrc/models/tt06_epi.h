@@ -7,7 +7,7 @@ namespace cmx { namespace tt06_epi {
 
 constexpr int kNumStates = 19;
 constexpr int kVIndex = 0;
-constexpr int kNumParams = 0;
+constexpr int kNumParams = 6;
 constexpr bool kPerCapacitanceUnits = true;
 
 // State variables (CellML component__variable), V first then document order.
@@ -53,11 +53,21 @@ static constexpr double kInitialState[kNumStates] = {
     8.604,  // sodium_dynamics__Na_i
     136.89,  // potassium_dynamics__K_i
 };
-static const char* const kParamNames[1] = {
-    "",
+static const char* const kParamNames[6] = {
+    "fast_sodium_current__g_Na",
+    "L_type_Ca_current__g_CaL",
+    "rapid_time_dependent_potassium_current__g_Kr",
+    "slow_time_dependent_potassium_current__g_Ks",
+    "inward_rectifier_potassium_current__g_K1",
+    "transient_outward_current__g_to",
 };
-static constexpr double kParamDefaults[1] = {
-    0.0,
+static constexpr double kParamDefaults[6] = {
+    14.838,
+    3.98e-05,
+    0.153,
+    0.392,
+    5.405,
+    0.294,
 };
 // 1 = integrated by Rush-Larsen (affine gate, V-dependent coefficients only)
 static constexpr int kIsGate[kNumStates] = {0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 0, 0, 0, 0, 0, 0};
@@ -78,17 +88,11 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R membrane__Cm = R(0.185);
     const R membrane__V_c = R(0.016404);
     const R reversal_potentials__P_kna = R(0.03);
-    const R inward_rectifier_potassium_current__g_K1 = R(5.405);
     const R rapid_time_dependent_potassium_current__ScaleFactorGkr = R(1.0);
-    const R rapid_time_dependent_potassium_current__g_Kr = R(0.153);
     const R slow_time_dependent_potassium_current__ScaleFactorGks = R(1.0);
-    const R slow_time_dependent_potassium_current__g_Ks = R(0.392);
-    const R fast_sodium_current__g_Na = R(14.838);
     const R sodium_background_current__g_bna = R(0.00029);
-    const R L_type_Ca_current__g_CaL = R(3.98e-05);
     const R calcium_background_current__g_bca = R(0.000592);
     const R transient_outward_current__ScaleFactorIto = R(1.0);
-    const R transient_outward_current__g_to = R(0.294);
     const R sodium_potassium_pump_current__P_NaK = R(2.724);
     const R sodium_potassium_pump_current__K_mk = R(1.0);
     const R sodium_potassium_pump_current__K_mNa = R(40.0);
@@ -126,7 +130,7 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R L_type_Ca_current_d_gate__d_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(8.0)) - V) * (R(1) / R(7.5))))));
     const R L_type_Ca_current_d_gate__alpha_d = (cmx_div(R(1.4), (R(1.0) + cmx_exp((((-R(35.0)) - V) * (R(1) / R(13.0)))))) + R(0.25));
     const R L_type_Ca_current_d_gate__beta_d = cmx_div(R(1.4), (R(1.0) + cmx_exp(((V + R(5.0)) * (R(1) / R(5.0))))));
@@ -153,7 +157,7 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R calcium_dynamics__k2 = (calcium_dynamics__k2_prime * calcium_dynamics__kcasr);
     const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * y[13]), (y[13] + calcium_pump_current__K_pCa));
     const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, y[17])));
-    const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
     const R fast_sodium_current_h_gate__h_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp(((V + R(71.55)) * (R(1) / R(7.43)))))));
     const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.057) * cmx_exp(((-(V + R(80.0))) * (R(1) / R(6.8)))))) : (R(0.0)));
     const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(2.7) * cmx_exp((R(0.079) * V))) + (R(310000.0) * cmx_exp((R(0.3485) * V))))) : (cmx_div(R(0.77), (R(0.13) * (R(1.0) + cmx_exp(((V + R(10.66)) * (R(1) / (-R(11.1))))))))));
@@ -170,9 +174,9 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
     const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
     const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
-    const R inward_rectifier_potassium_current__i_K1 = (inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
+    const R inward_rectifier_potassium_current__i_K1 = (p[4] * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
     const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
-    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
+    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
     const R rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 = cmx_div(R(450.0), (R(1.0) + cmx_exp((((-R(45.0)) - V) * (R(1) / R(10.0))))));
     const R rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1 = cmx_div(R(6.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(11.5))))));
     const R rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1 = (rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 * rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1);
@@ -182,7 +186,7 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2 = (rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 * rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2);
     const R rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(88.0)) * (R(1) / R(24.0))))));
     const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (y[18] + (reversal_potentials__P_kna * y[17])))));
-    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
+    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
     const R slow_time_dependent_potassium_current_Xs_gate__alpha_xs = cmx_div(R(1400.0), sqrt((R(1.0) + cmx_exp(((R(5.0) - V) * (R(1) / R(6.0)))))));
     const R slow_time_dependent_potassium_current_Xs_gate__beta_xs = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V - R(35.0)) * (R(1) / R(15.0))))));
     const R slow_time_dependent_potassium_current_Xs_gate__tau_xs = ((slow_time_dependent_potassium_current_Xs_gate__alpha_xs * slow_time_dependent_potassium_current_Xs_gate__beta_xs) + R(80.0));
@@ -190,7 +194,7 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
     const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
     const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
-    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11] * (V - reversal_potentials__E_K));
+    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11] * (V - reversal_potentials__E_K));
     const R transient_outward_current_r_gate__r_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(20.0) - V) * (R(1) / R(6.0))))));
     const R transient_outward_current_r_gate__tau_r = ((R(9.5) * cmx_exp(((-cmx_ipow<2>((V + R(40.0)))) * (R(1) / R(1800.0))))) + R(0.8));
     const R transient_outward_current_s_gate__s_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(20.0)) * (R(1) / R(5.0))))));
@@ -280,17 +284,11 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     const R membrane__T = R(310.0);
     const R membrane__F = R(96485.3415);
     const R reversal_potentials__P_kna = R(0.03);
-    const R inward_rectifier_potassium_current__g_K1 = R(5.405);
     const R rapid_time_dependent_potassium_current__ScaleFactorGkr = R(1.0);
-    const R rapid_time_dependent_potassium_current__g_Kr = R(0.153);
     const R slow_time_dependent_potassium_current__ScaleFactorGks = R(1.0);
-    const R slow_time_dependent_potassium_current__g_Ks = R(0.392);
-    const R fast_sodium_current__g_Na = R(14.838);
     const R sodium_background_current__g_bna = R(0.00029);
-    const R L_type_Ca_current__g_CaL = R(3.98e-05);
     const R calcium_background_current__g_bca = R(0.000592);
     const R transient_outward_current__ScaleFactorIto = R(1.0);
-    const R transient_outward_current__g_to = R(0.294);
     const R sodium_potassium_pump_current__P_NaK = R(2.724);
     const R sodium_potassium_pump_current__K_mk = R(1.0);
     const R sodium_potassium_pump_current__K_mNa = R(40.0);
@@ -308,25 +306,25 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, y[13])));
     const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
     const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * y[13]), (y[13] + calcium_pump_current__K_pCa));
     const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, y[17])));
-    const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
     const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, y[18])));
     const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
     const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
     const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
-    const R inward_rectifier_potassium_current__i_K1 = (inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
+    const R inward_rectifier_potassium_current__i_K1 = (p[4] * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
     const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
-    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
+    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
     const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (y[18] + (reversal_potentials__P_kna * y[17])))));
-    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
+    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
     const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
     const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
-    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11] * (V - reversal_potentials__E_K));
+    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11] * (V - reversal_potentials__E_K));
     return (((-R(1.0)) * (R(1) / R(1.0))) * (inward_rectifier_potassium_current__i_K1 + transient_outward_current__i_to + rapid_time_dependent_potassium_current__i_Kr + slow_time_dependent_potassium_current__i_Ks + L_type_Ca_current__i_CaL + sodium_potassium_pump_current__i_NaK + fast_sodium_current__i_Na + sodium_background_current__i_b_Na + sodium_calcium_exchanger_current__i_NaCa + calcium_background_current__i_b_Ca + potassium_pump_current__i_p_K + calcium_pump_current__i_p_Ca + i_stim));
 }
 
@@ -341,17 +339,11 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     const R membrane__T = R(310.0);
     const R membrane__F = R(96485.3415);
     const R reversal_potentials__P_kna = R(0.03);
-    const R inward_rectifier_potassium_current__g_K1 = R(5.405);
     const R rapid_time_dependent_potassium_current__ScaleFactorGkr = R(1.0);
-    const R rapid_time_dependent_potassium_current__g_Kr = R(0.153);
     const R slow_time_dependent_potassium_current__ScaleFactorGks = R(1.0);
-    const R slow_time_dependent_potassium_current__g_Ks = R(0.392);
-    const R fast_sodium_current__g_Na = R(14.838);
     const R sodium_background_current__g_bna = R(0.00029);
-    const R L_type_Ca_current__g_CaL = R(3.98e-05);
     const R calcium_background_current__g_bca = R(0.000592);
     const R transient_outward_current__ScaleFactorIto = R(1.0);
-    const R transient_outward_current__g_to = R(0.294);
     const R sodium_potassium_pump_current__P_NaK = R(2.724);
     const R sodium_potassium_pump_current__K_mk = R(1.0);
     const R sodium_potassium_pump_current__K_mNa = R(40.0);
@@ -369,25 +361,25 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = R(0);
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, y[13])));
     const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
     const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * y[13]), (y[13] + calcium_pump_current__K_pCa));
     const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, y[17])));
-    const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
     const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, y[18])));
     const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
     const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
     const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
-    const R inward_rectifier_potassium_current__i_K1 = (inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
+    const R inward_rectifier_potassium_current__i_K1 = (p[4] * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
     const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
-    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
+    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
     const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (y[18] + (reversal_potentials__P_kna * y[17])))));
-    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
+    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
     const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
     const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
-    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11] * (V - reversal_potentials__E_K));
+    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11] * (V - reversal_potentials__E_K));
     const R dv = (((-R(1.0)) * (R(1) / R(1.0))) * (inward_rectifier_potassium_current__i_K1 + transient_outward_current__i_to + rapid_time_dependent_potassium_current__i_Kr + slow_time_dependent_potassium_current__i_Ks + L_type_Ca_current__i_CaL + sodium_potassium_pump_current__i_NaK + fast_sodium_current__i_Na + sodium_background_current__i_b_Na + sodium_calcium_exchanger_current__i_NaCa + calcium_background_current__i_b_Ca + potassium_pump_current__i_p_K + calcium_pump_current__i_p_Ca + i_stim));
     return -dv * cm;
 }
@@ -466,17 +458,11 @@ CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const
     const R membrane__Cm = R(0.185);
     const R membrane__V_c = R(0.016404);
     const R reversal_potentials__P_kna = R(0.03);
-    const R inward_rectifier_potassium_current__g_K1 = R(5.405);
     const R rapid_time_dependent_potassium_current__ScaleFactorGkr = R(1.0);
-    const R rapid_time_dependent_potassium_current__g_Kr = R(0.153);
     const R slow_time_dependent_potassium_current__ScaleFactorGks = R(1.0);
-    const R slow_time_dependent_potassium_current__g_Ks = R(0.392);
-    const R fast_sodium_current__g_Na = R(14.838);
     const R sodium_background_current__g_bna = R(0.00029);
-    const R L_type_Ca_current__g_CaL = R(3.98e-05);
     const R calcium_background_current__g_bca = R(0.000592);
     const R transient_outward_current__ScaleFactorIto = R(1.0);
-    const R transient_outward_current__g_to = R(0.294);
     const R sodium_potassium_pump_current__P_NaK = R(2.724);
     const R sodium_potassium_pump_current__K_mk = R(1.0);
     const R sodium_potassium_pump_current__K_mNa = R(40.0);
@@ -514,7 +500,7 @@ CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R L_type_Ca_current_fCass_gate__fCass_inf = (cmx_div(R(0.6), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(0.4));
     const R L_type_Ca_current_fCass_gate__tau_fCass = (cmx_div(R(80.0), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(2.0));
     const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, g[1])));
@@ -532,20 +518,20 @@ CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const
     const R calcium_dynamics__k2 = (calcium_dynamics__k2_prime * calcium_dynamics__kcasr);
     const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * g[1]), (g[1] + calcium_pump_current__K_pCa));
     const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, g[5])));
-    const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
     const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, g[6])));
     const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
     const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
     const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
-    const R inward_rectifier_potassium_current__i_K1 = (inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
+    const R inward_rectifier_potassium_current__i_K1 = (p[4] * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
     const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
-    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
+    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
     const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))));
-    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
+    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
     const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(g[5]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * g[1] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
     const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]), (g[5] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
-    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11] * (V - reversal_potentials__E_K));
+    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11] * (V - reversal_potentials__E_K));
     res[0] = g[0] - y[10] - dt * cmx_div((L_type_Ca_current_fCass_gate__fCass_inf - g[0]), L_type_Ca_current_fCass_gate__tau_fCass);
     res[1] = g[1] - y[13] - dt * (calcium_dynamics__Ca_i_bufc * (((((calcium_dynamics__i_leak - calcium_dynamics__i_up) * calcium_dynamics__V_sr) * (R(1) / membrane__V_c)) + calcium_dynamics__i_xfer) - ((((calcium_background_current__i_b_Ca + calcium_pump_current__i_p_Ca) - (R(2.0) * sodium_calcium_exchanger_current__i_NaCa)) * membrane__Cm) * (R(1) / (R(2.0) * membrane__V_c * membrane__F)))));
     res[2] = g[2] - y[14] - dt * (calcium_dynamics__Ca_sr_bufsr * (calcium_dynamics__i_up - (calcium_dynamics__i_rel + calcium_dynamics__i_leak)));
@@ -567,17 +553,11 @@ CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const
     const R membrane__Cm = R(0.185);
     const R membrane__V_c = R(0.016404);
     const R reversal_potentials__P_kna = R(0.03);
-    const R inward_rectifier_potassium_current__g_K1 = R(5.405);
     const R rapid_time_dependent_potassium_current__ScaleFactorGkr = R(1.0);
-    const R rapid_time_dependent_potassium_current__g_Kr = R(0.153);
     const R slow_time_dependent_potassium_current__ScaleFactorGks = R(1.0);
-    const R slow_time_dependent_potassium_current__g_Ks = R(0.392);
-    const R fast_sodium_current__g_Na = R(14.838);
     const R sodium_background_current__g_bna = R(0.00029);
-    const R L_type_Ca_current__g_CaL = R(3.98e-05);
     const R calcium_background_current__g_bca = R(0.000592);
     const R transient_outward_current__ScaleFactorIto = R(1.0);
-    const R transient_outward_current__g_to = R(0.294);
     const R sodium_potassium_pump_current__P_NaK = R(2.724);
     const R sodium_potassium_pump_current__K_mk = R(1.0);
     const R sodium_potassium_pump_current__K_mNa = R(40.0);
@@ -615,7 +595,7 @@ CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R L_type_Ca_current_fCass_gate__fCass_inf = (cmx_div(R(0.6), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(0.4));
     const R L_type_Ca_current_fCass_gate__tau_fCass = (cmx_div(R(80.0), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(2.0));
     const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, g[1])));
@@ -633,22 +613,22 @@ CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const
     const R calcium_dynamics__k2 = (calcium_dynamics__k2_prime * calcium_dynamics__kcasr);
     const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * g[1]), (g[1] + calcium_pump_current__K_pCa));
     const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, g[5])));
-    const R fast_sodium_current__i_Na = (fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
     const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, g[6])));
     const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
     const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
     const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
-    const R inward_rectifier_potassium_current__i_K1 = (inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
+    const R inward_rectifier_potassium_current__i_K1 = (p[4] * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
     const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
-    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
+    const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
     const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))));
-    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
+    const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
     const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(g[5]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * g[1] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
     const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]), (g[5] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
-    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11] * (V - reversal_potentials__E_K));
-    const R d_L_type_Ca_current__i_CaL__d0 = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div(((cmx_div(((R(1) * L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F))), (membrane__R * membrane__T)) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o))), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
-    const R d_L_type_Ca_current__i_CaL__d3 = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div(((((R(1) * R(0.25) * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))) * ((L_type_Ca_current__g_CaL * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))))), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11] * (V - reversal_potentials__E_K));
+    const R d_L_type_Ca_current__i_CaL__d0 = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div(((cmx_div(((R(1) * p[1] * y[7] * y[8] * y[9] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F))), (membrane__R * membrane__T)) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o))), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R d_L_type_Ca_current__i_CaL__d3 = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div(((((R(1) * R(0.25) * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))) * ((p[1] * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))))), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R d_L_type_Ca_current_fCass_gate__fCass_inf__d3 = (cmx_div((-(R(0.6) * ((R(2.0) * cmx_ipow<1>((g[3] * (R(1) / R(0.05)))) * cmx_div(R(1), R(0.05)))))), ((R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))) * (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))))));
     const R d_L_type_Ca_current_fCass_gate__tau_fCass__d3 = (cmx_div((-(R(80.0) * ((R(2.0) * cmx_ipow<1>((g[3] * (R(1) / R(0.05)))) * cmx_div(R(1), R(0.05)))))), ((R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))) * (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))))));
     const R d_reversal_potentials__E_Ca__d1 = ((cmx_div(cmx_div((-(calcium_dynamics__Ca_o * R(1))), (g[1] * g[1])), cmx_div(calcium_dynamics__Ca_o, g[1])) * ((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F))));
@@ -672,23 +652,23 @@ CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const
     const R d_calcium_dynamics__k2__d2 = ((d_calcium_dynamics__kcasr__d2 * calcium_dynamics__k2_prime));
     const R d_calcium_pump_current__i_p_Ca__d1 = cmx_div((((R(1) * calcium_pump_current__g_pCa)) * (g[1] + calcium_pump_current__K_pCa) - (calcium_pump_current__g_pCa * g[1]) * (R(1))), ((g[1] + calcium_pump_current__K_pCa) * (g[1] + calcium_pump_current__K_pCa)));
     const R d_reversal_potentials__E_Na__d5 = ((cmx_div(cmx_div((-(sodium_dynamics__Na_o * R(1))), (g[5] * g[5])), cmx_div(sodium_dynamics__Na_o, g[5])) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
-    const R d_fast_sodium_current__i_Na__d5 = (((-d_reversal_potentials__E_Na__d5) * fast_sodium_current__g_Na * cmx_ipow<3>(y[4]) * y[5] * y[6]));
+    const R d_fast_sodium_current__i_Na__d5 = (((-d_reversal_potentials__E_Na__d5) * p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6]));
     const R d_reversal_potentials__E_K__d6 = ((cmx_div(cmx_div((-(potassium_dynamics__K_o * R(1))), (g[6] * g[6])), cmx_div(potassium_dynamics__K_o, g[6])) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
     const R d_inward_rectifier_potassium_current__alpha_K1__d6 = cmx_div((-(R(0.1) * ((cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0)))) * (((-d_reversal_potentials__E_K__d6) * R(0.06))))))), ((R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))) * (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0)))))));
     const R d_inward_rectifier_potassium_current__beta_K1__d6 = cmx_div((((((cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0)))) * ((((-d_reversal_potentials__E_K__d6)) * R(0.0002)))) * R(3.0))) + (cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0)))) * (((-d_reversal_potentials__E_K__d6) * R(0.1))))) * (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))) - ((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))) * ((cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K))) * (((-d_reversal_potentials__E_K__d6) * (-R(0.5))))))), ((R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))) * (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K))))));
     const R d_inward_rectifier_potassium_current__xK1_inf__d6 = cmx_div((d_inward_rectifier_potassium_current__alpha_K1__d6 * (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1) - inward_rectifier_potassium_current__alpha_K1 * (d_inward_rectifier_potassium_current__alpha_K1__d6 + d_inward_rectifier_potassium_current__beta_K1__d6)), ((inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1) * (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1)));
-    const R d_inward_rectifier_potassium_current__i_K1__d6 = ((d_inward_rectifier_potassium_current__xK1_inf__d6 * inward_rectifier_potassium_current__g_K1 * (V - reversal_potentials__E_K)) + ((-d_reversal_potentials__E_K__d6) * inward_rectifier_potassium_current__g_K1 * inward_rectifier_potassium_current__xK1_inf));
+    const R d_inward_rectifier_potassium_current__i_K1__d6 = ((d_inward_rectifier_potassium_current__xK1_inf__d6 * p[4] * (V - reversal_potentials__E_K)) + ((-d_reversal_potentials__E_K__d6) * p[4] * inward_rectifier_potassium_current__xK1_inf));
     const R d_potassium_pump_current__i_p_K__d6 = cmx_div((((-d_reversal_potentials__E_K__d6) * potassium_pump_current__g_pK)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
-    const R d_rapid_time_dependent_potassium_current__i_Kr__d6 = (((-d_reversal_potentials__E_K__d6) * rapid_time_dependent_potassium_current__ScaleFactorGkr * rapid_time_dependent_potassium_current__g_Kr * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2]));
+    const R d_rapid_time_dependent_potassium_current__i_Kr__d6 = (((-d_reversal_potentials__E_K__d6) * rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2]));
     const R d_reversal_potentials__E_Ks__d5 = ((cmx_div(cmx_div((-((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)) * (((R(1) * reversal_potentials__P_kna))))), ((g[6] + (reversal_potentials__P_kna * g[5])) * (g[6] + (reversal_potentials__P_kna * g[5])))), cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
     const R d_reversal_potentials__E_Ks__d6 = ((cmx_div(cmx_div((-((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)) * (R(1)))), ((g[6] + (reversal_potentials__P_kna * g[5])) * (g[6] + (reversal_potentials__P_kna * g[5])))), cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
-    const R d_slow_time_dependent_potassium_current__i_Ks__d5 = (((-d_reversal_potentials__E_Ks__d5) * slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3])));
-    const R d_slow_time_dependent_potassium_current__i_Ks__d6 = (((-d_reversal_potentials__E_Ks__d6) * slow_time_dependent_potassium_current__ScaleFactorGks * slow_time_dependent_potassium_current__g_Ks * cmx_ipow<2>(y[3])));
+    const R d_slow_time_dependent_potassium_current__i_Ks__d5 = (((-d_reversal_potentials__E_Ks__d5) * slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3])));
+    const R d_slow_time_dependent_potassium_current__i_Ks__d6 = (((-d_reversal_potentials__E_Ks__d6) * slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3])));
     const R d_sodium_background_current__i_b_Na__d5 = (((-d_reversal_potentials__E_Na__d5) * sodium_background_current__g_bna));
     const R d_sodium_calcium_exchanger_current__i_NaCa__d1 = cmx_div((((-((R(1) * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * sodium_calcium_exchanger_current__alpha))) * sodium_calcium_exchanger_current__K_NaCa)), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
     const R d_sodium_calcium_exchanger_current__i_NaCa__d5 = cmx_div((((((R(3.0) * cmx_ipow<2>(g[5]) * R(1)) * cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * calcium_dynamics__Ca_o)) * sodium_calcium_exchanger_current__K_NaCa)), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
     const R d_sodium_potassium_pump_current__i_NaK__d5 = cmx_div(cmx_div((((R(1) * ((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))))) * (g[5] + sodium_potassium_pump_current__K_mNa) - (((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]) * (R(1))), ((g[5] + sodium_potassium_pump_current__K_mNa) * (g[5] + sodium_potassium_pump_current__K_mNa))), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
-    const R d_transient_outward_current__i_to__d6 = (((-d_reversal_potentials__E_K__d6) * transient_outward_current__ScaleFactorIto * transient_outward_current__g_to * y[12] * y[11]));
+    const R d_transient_outward_current__i_to__d6 = (((-d_reversal_potentials__E_K__d6) * transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11]));
     J[0] = R(1) - dt * cmx_div((-R(1)), L_type_Ca_current_fCass_gate__tau_fCass);
     J[1] = R(0);
     J[2] = R(0);

@@ -249,6 +249,47 @@ int cmx_h5_open(const char* path, int32_t n_nodes, int32_t n_vars, const char* c
 int cmx_h5_write_step(cmx_h5* h, double time, const double* values);
 int cmx_h5_close(cmx_h5* h);
 
+/* Batched single cells: n_cells independent cells (no tissue), each
+ * integrated from t = 0 to its own t_end as AbstractCardiacCell::Compute
+ * integrates one cell (heart/src/odes/AbstractCardiacCell.cpp:64-88,
+ * ode/src/solver/AbstractOneStepIvpOdeSolver.cpp:93-153,
+ * heart/src/odes/AbstractRushLarsenCardiacCell.cpp:57-104): rates at the
+ * step's start, V by forward Euler, the rest by the integrator, the cell's
+ * own area stimulus (uA/cm^2; Cm = 1) and parameters.  CMX_MIXED keeps the
+ * states in FP32 and V in FP64. */
+typedef struct {
+    int32_t model;               /* CMX_MODEL_* */
+    int32_t integrator;          /* CMX_FORWARD_EULER or CMX_RUSH_LARSEN */
+    int32_t precision;           /* CMX_FP64 or CMX_MIXED */
+    int32_t n_cells;
+    double dt;                   /* ms; every t_end a whole number of dt */
+    const double* t_end;         /* [n_cells] ms */
+    const cmx_stimulus* stimuli; /* [n_cells] one protocol per cell, or NULL */
+    int32_t n_params;            /* per-cell parameter planes given (<= model's count) */
+    const double* params;        /* [n_params][n_cells] or NULL */
+    double threshold;            /* mV, for the action-potential summary */
+    int32_t steps_per_launch;    /* steps per kernel launch (0: 2000) */
+} cmx_cells;
+
+typedef struct {
+    double kernel_ms;            /* device time of all launches */
+    int64_t cell_steps;          /* sum of the cells' step counts */
+    int32_t launches;
+    int32_t block;               /* threads per block (one cell per thread) */
+} cmx_cells_stats;
+
+/* y0: [n_states][n_cells] initial states (NULL: the model's initial state);
+ * y_out: [n_states][n_cells] final states (may be NULL); props: [4][n_cells]
+ * (may be NULL) the action-potential summary of each cell's voltage trace as
+ * CellProperties computes it at the given threshold
+ * (heart/src/postprocessing/CellProperties.cpp:54-200): completed action
+ * potentials, the last onset (linear interpolation), the peak of the last
+ * completed one, and the time (interpolated) at which it fell back below the
+ * threshold; -1 where none.  stats may be NULL.  Errors: CMX_E_* with the
+ * message in cmx_cells_last_error(). */
+int cmx_cells_run(const cmx_cells* cells, const double* y0, double* y_out, double* props, cmx_cells_stats* stats);
+const char* cmx_cells_last_error(void);
+
 /* Builds a regular slab mesh exactly as ConstructRegularSlabMesh does
  * (mesh/src/common/AbstractTetrahedralMesh.cpp:277-614): nodes
  * (k*(ey+1)+j)*(ex+1)+i, 2-D staggered triangles, 3-D six tetrahedra per cube.

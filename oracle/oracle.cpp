@@ -360,6 +360,31 @@ int orc_cell_compute(int model, int integrator, const double* params, double* y,
     return ns;
 }
 
+int orc_cell_compute_v(int model, int integrator, const double* params, double* y, double t_end, double dt,
+                       double stim_mag, double stim_dur, double stim_start, double stim_period, double* v_out)
+{
+    // orc_cell_compute from t = 0 in steps of dt, keeping only V (v_out[0] at
+    // t = 0, then after every step)
+    const int S = num_states(model);
+    if (S < 0 || dt <= 0 || t_end <= 0) return -1;
+    if (integrator < ORC_FE || integrator > ORC_GRL2 || (model == ORC_LR91_HAND && integrator != ORC_FE)) return -1;
+    const int fails0 = g_newton_failures.load();
+    const double* p = params ? params : default_params(model);
+    Stimulus stim(stim_mag, stim_dur, stim_start, stim_period);
+    const bool fixed_dt = integrator != ORC_FE;
+    Stepper st(0.0, t_end, dt);
+    int ns = 0;
+    v_out[ns++] = y[0];
+    while (!st.at_end()) {
+        const double t = fixed_dt ? st.k * dt : st.time;
+        cell_substep(model, integrator, p, 1.0, y, t, fixed_dt ? dt : st.step_size(), stim.at(t), true);
+        st.advance();
+        v_out[ns++] = y[0];
+    }
+    if (g_newton_failures.load() != fails0) return -1;
+    return ns;
+}
+
 double orc_cell_compute_except_voltage(int model, int integrator, const double* params, double* y,
                                        double t0, double t_end, double dt)
 {

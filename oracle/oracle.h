@@ -54,6 +54,12 @@ int orc_cell_compute(int model, int integrator, const double* params, double* y,
                      double stim_mag, double stim_dur, double stim_start, double stim_period,
                      int steps_per_sample, double* samples, int max_samples);
 
+/* orc_cell_compute from t = 0 to t_end keeping V only: v_out[0] = V(0), then
+ * V after every step (v_out holds round(t_end/dt) + 1 values); returns the
+ * count written, or -1 on error. */
+int orc_cell_compute_v(int model, int integrator, const double* params, double* y, double t_end, double dt,
+                       double stim_mag, double stim_dur, double stim_start, double stim_period, double* v_out);
+
 /* AbstractCardiacCell::ComputeExceptVoltage over [t0, t_end] in steps of dt
  * (V clamped, the other states by the integrator), then GetIIonic at the
  * result (what SolveCellSystems caches per node).  y is in/out; returns the

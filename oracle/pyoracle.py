@@ -59,6 +59,9 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
         L.orc_model_initial_state.argtypes = [ctypes.c_int, _dp]
+        L.orc_cell_compute_v.restype = ctypes.c_int
+        L.orc_cell_compute_v.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_cell_compute_except_voltage.restype = ctypes.c_double
         L.orc_cell_compute_except_voltage.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double,
                                                       ctypes.c_double, ctypes.c_double]
@@ -110,6 +113,46 @@ def model_initial_state(model):
     if S < 0:
         raise ValueError('unknown model %d' % model)
     return y[:S].copy()
+
+
+def cell_trace_v(model, y, t_end, dt, stim=(0.0, 0.0, 0.0, -1.0), integrator=0, params=None):
+    """Single cell from t = 0: V at t = 0 and after every step; y updated in place."""
+    n = int(round(t_end / dt)) + 2
+    out = np.zeros(n)
+    pa = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    ns = lib().orc_cell_compute_v(model, integrator, _p(pa), _p(y), t_end, dt, stim[0], stim[1], stim[2], stim[3],
+                                  _p(out))
+    if ns < 0:
+        raise RuntimeError('orc_cell_compute_v failed')
+    return out[:ns]
+
+
+def cell_properties(v, dt, threshold):
+    """CellProperties' action-potential bookkeeping on a uniformly sampled
+    trace (heart/src/postprocessing/CellProperties.cpp:54-200): completed
+    APs, last onset (linear interpolation), peak of the last completed AP and
+    the interpolated time it fell back below the threshold (-1 if none)."""
+    v = np.asarray(v)
+    ups = np.where((v[:-1] <= threshold) & (v[1:] > threshold))[0]
+    downs = np.where((v[:-1] >= threshold) & (v[1:] < threshold))[0]
+
+    def cross(i):     # the interval from sample i to i + 1
+        return i * dt + dt / (v[i + 1] - v[i]) * (threshold - v[i])
+    count, onset, last_peak, end = 0, -1.0, -1.0, -1.0
+    i = -1
+    while True:
+        u = ups[ups > i]
+        if len(u) == 0:
+            break
+        onset = cross(u[0])
+        d = downs[downs > u[0]]
+        if len(d) == 0:
+            break
+        last_peak = v[u[0] + 1:d[0] + 2].max()
+        end = cross(d[0])
+        count += 1
+        i = d[0]
+    return np.array([count, onset, last_peak, end], dtype=float)
 
 
 def cell_compute_except_voltage(model, y, t0, t_end, dt, integrator=0, params=None):

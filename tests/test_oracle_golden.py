@@ -671,3 +671,40 @@ def test_compute_except_voltage(oracle, v_clamp):
         y0 = np.array(LR91_INIT)
         y0[0] = v_clamp
         assert abs(ys[1][1] - oracle.cell_iionic(oracle.LR91_CELLML, y0)) > 1.0
+
+
+def test_cell_properties_restatement(oracle):
+    """The oracle's CellProperties bookkeeping (CellProperties.cpp:54-200) on
+    a piecewise-linear trace with known crossings, and against a step-by-step
+    transcription of the reference's loop on a paced TT06 trace."""
+    dt = 0.5
+    # rest -80 -> +20 over 4 samples, plateau, back to -80: threshold -30
+    beat = [-80, -55, -30, -5, 20, 20, 10, -20, -40, -80, -80, -80]
+    v = np.array(beat * 3 + [-80, -60, -10])     # 3 completed APs, a 4th onset
+    props = oracle.cell_properties(v, dt, -30.0)
+    n = len(beat)
+    # the onset of the 4th AP: between -60 (sample 3n+1) and -10 (3n+2), at 60 % of the step
+    assert props[0] == 3 and props[1] == pytest.approx((3 * n + 1 + 0.6) * dt)
+    assert props[2] == 20
+    # the last end: between -20 and -40 (samples 2n+7 and 2n+8), halfway
+    assert props[3] == pytest.approx((2 * n + 7.5) * dt)
+
+    def loop(v, dt, thr):
+        count, onset, peak, last_peak, end, above = 0, -1.0, -np.inf, -1.0, -1.0, False
+        for i in range(1, len(v)):
+            v0, v1 = v[i - 1], v[i]
+            t0, t1 = (i - 1) * dt, i * dt
+            if not above and v1 > thr and v0 <= thr:
+                onset, above = t0 + (t1 - t0) / (v1 - v0) * (thr - v0), True
+            if above:
+                peak = max(peak, v1)
+                if v1 < thr and v0 >= thr:
+                    last_peak, peak = peak, thr
+                    end = t0 + (t1 - t0) / (v1 - v0) * (thr - v0)
+                    count += 1
+                    above = False
+        return np.array([count, onset, last_peak, end])
+    y = oracle.model_initial_state(oracle.TT06_EPI)
+    tr = oracle.cell_trace_v(oracle.TT06_EPI, y, 900.0, 0.005, stim=(-80.0, 2.0, 0.0, 450.0), integrator=1)
+    assert np.array_equal(oracle.cell_properties(tr, 0.005, -30.0), loop(tr, 0.005, -30.0))
+    assert oracle.cell_properties(tr, 0.005, -30.0)[0] == 2
