@@ -52,7 +52,7 @@ struct CellsArgs {
     double dt, cm, thr;
     const int32_t* __restrict__ n_steps;   // [n] steps of each cell (sorted, longest first)
     const cmx_stimulus* __restrict__ stim; // [n] padded protocols, or null
-    RS* __restrict__ state;                // [S-1][ps]
+    double* __restrict__ state;            // [S-1][ps]
     double* __restrict__ V;                // [n]
     const RS* __restrict__ params;         // [P][ps] or null
     const RS* __restrict__ pdef;           // [P]
@@ -76,9 +76,17 @@ __global__ __launch_bounds__(kBlock) void cells_kernel(CellsArgs<RS> a, int k0, 
     if (i >= a.n) return;
     const int kend = min(k1, a.n_steps[i]);
     if (k0 >= kend) return;
+    // Rates in RS; the states that integrate small increments over long runs
+    // (V and every non-gate: concentrations) accumulate in FP64 whatever RS
+    // is -- an FP32 K_i of ~138 mM cannot take a per-step change of ~1e-6 mM.
+    // Gates (bounded, Rush-Larsen or Euler) stay in RS.
     RS y[NS], p[NPd];
+    double yd[NS];
 #pragma unroll
-    for (int s = 1; s < NS; ++s) y[s] = a.state[(size_t)(s - 1) * a.ps + i];
+    for (int s = 1; s < NS; ++s) {
+        yd[s] = a.state[(size_t)(s - 1) * a.ps + i];
+        y[s] = (RS)yd[s];
+    }
 #pragma unroll
     for (int q = 0; q < NPd; ++q)
         p[q] = (q < a.n_param_planes) ? a.params[(size_t)q * a.ps + i] : (Model::NP > 0 ? a.pdef[q] : RS(0));
@@ -99,8 +107,13 @@ __global__ __launch_bounds__(kBlock) void cells_kernel(CellsArgs<RS> a, int k0, 
         const RS dv = Model::template dvdt<RS>(y, V, is, p, cm);
 #pragma unroll
         for (int s = 1; s < NS; ++s) {
-            if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_exp(gk[s] * h);
-            else y[s] += h * dy[s];
+            if (Model::gate(s)) {
+                if (RL) y[s] = gi[s] + (y[s] - gi[s]) * cmx_exp(gk[s] * h);
+                else y[s] += h * dy[s];
+            } else {
+                yd[s] += a.dt * (double)dy[s];
+                y[s] = (RS)yd[s];
+            }
         }
         const double v0 = vd;
         vd += a.dt * (double)dv;
@@ -123,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void cells_kernel(CellsArgs<RS> a, int k0, 
         }
     }
 #pragma unroll
-    for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.ps + i] = y[s];
+    for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.ps + i] = Model::gate(s) ? (double)y[s] : yd[s];
     a.V[i] = vd;
     a.phase[i] = phase;
     a.ap_count[i] = count;
@@ -187,7 +200,8 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return steps[a] > steps[b]; });
     std::vector<int32_t> s_steps(n);
     std::vector<cmx_stimulus> s_stim;
-    std::vector<RS> s_state((size_t)(S - 1) * ps, RS(0)), s_params((size_t)std::max(c.n_params, 0) * ps, RS(0));
+    std::vector<double> s_state((size_t)(S - 1) * ps, 0.0);
+    std::vector<RS> s_params((size_t)std::max(c.n_params, 0) * ps, RS(0));
     std::vector<double> s_v(n);
     const double* yi = y0 ? y0 : nullptr;
     std::vector<double> def(S);
@@ -200,7 +214,7 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
         const int i = order[j];
         s_steps[j] = steps[i];
         s_v[j] = yi ? yi[i] : def[0];
-        for (int s = 1; s < S; ++s) s_state[(size_t)(s - 1) * ps + j] = (RS)(yi ? yi[(size_t)s * n + i] : def[s]);
+        for (int s = 1; s < S; ++s) s_state[(size_t)(s - 1) * ps + j] = yi ? yi[(size_t)s * n + i] : def[s];
         for (int q = 0; q < c.n_params; ++q) s_params[(size_t)q * ps + j] = (RS)c.params[(size_t)q * n + i];
         if (c.stimuli) {
             cmx_stimulus st = c.stimuli[i];
@@ -217,8 +231,8 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
     int rc;
     int32_t* d_steps;
     cmx_stimulus* d_stim = nullptr;
-    RS *d_state, *d_params = nullptr, *d_pdef;
-    double* d_v;
+    RS *d_params = nullptr, *d_pdef;
+    double *d_state, *d_v;
     if ((rc = B.upload(&d_steps, s_steps))) return rc;
     if (c.stimuli && (rc = B.upload(&d_stim, s_stim))) return rc;
     if ((rc = B.upload(&d_state, s_state))) return rc;
@@ -299,7 +313,7 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
     CELLS_HIP(hipEventElapsedTime(&ms, e0, e1));
 
     // back to the caller's order
-    CELLS_HIP(hipMemcpy(s_state.data(), d_state, s_state.size() * sizeof(RS), hipMemcpyDeviceToHost));
+    CELLS_HIP(hipMemcpy(s_state.data(), d_state, s_state.size() * sizeof(double), hipMemcpyDeviceToHost));
     CELLS_HIP(hipMemcpy(s_v.data(), d_v, n * sizeof(double), hipMemcpyDeviceToHost));
     std::vector<int32_t> cnt(ps);
     std::vector<double> on(ps), lp(ps), en(ps);
@@ -313,7 +327,7 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
         cell_steps += s_steps[j];
         if (y_out) {
             y_out[i] = s_v[j];
-            for (int s = 1; s < S; ++s) y_out[(size_t)s * n + i] = (double)s_state[(size_t)(s - 1) * ps + j];
+            for (int s = 1; s < S; ++s) y_out[(size_t)s * n + i] = s_state[(size_t)(s - 1) * ps + j];
         }
         if (props) {
             props[i] = (double)cnt[j];

@@ -255,8 +255,9 @@ int cmx_h5_close(cmx_h5* h);
  * ode/src/solver/AbstractOneStepIvpOdeSolver.cpp:93-153,
  * heart/src/odes/AbstractRushLarsenCardiacCell.cpp:57-104): rates at the
  * step's start, V by forward Euler, the rest by the integrator, the cell's
- * own area stimulus (uA/cm^2; Cm = 1) and parameters.  CMX_MIXED keeps the
- * states in FP32 and V in FP64. */
+ * own area stimulus (uA/cm^2; Cm = 1) and parameters.  CMX_MIXED computes
+ * the rates and the gates in FP32 and accumulates V and the other states
+ * (concentrations) in FP64. */
 typedef struct {
     int32_t model;               /* CMX_MODEL_* */
     int32_t integrator;          /* CMX_FORWARD_EULER or CMX_RUSH_LARSEN */
