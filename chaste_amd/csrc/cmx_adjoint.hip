@@ -32,7 +32,8 @@ namespace cmx {
 namespace {
 
 constexpr int kAdjBlock = 256;
-constexpr int kAdjChunk = 16;       // CG iterations launched between convergence checks
+constexpr int kAdjChunk = 16;       // CG iterations of a solve's first launch batch (no history yet)
+constexpr int kAdjFirstMax = 64;    // cap on the first batch taken from the previous solve's count
 constexpr int kAdjMaxIts = 2000;
 
 // ---------------------------------------------------------------------------
@@ -231,12 +232,15 @@ __global__ __launch_bounds__(kCgBlock) void adj_cg_spmv_kernel(Rows R, const dou
                                                                const double* __restrict__ pin, int G,
                                                                double* __restrict__ pout)
 {
+    __shared__ double sA[kMaxPatterns * kMaxStencil];
+    const TableLoad<Rows, double> ta(avals, R.value_count());
     PartialLoad<3> pl(pin, G);
     const int done = ctl->done;
     const double tol2_set = ctl->tol2;
     const double prev = it > 0 ? hist[it - 1] : 1.0;
+    const double* __restrict__ A = ta.store(sA, avals);
     double g[3];
-    pl.total(g);
+    pl.total(g);    // its barriers also cover the table stores
     if (done) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const bool lead = blk == 0 && threadIdx.x == 0;
@@ -255,11 +259,17 @@ __global__ __launch_bounds__(kCgBlock) void adj_cg_spmv_kernel(Rows R, const dou
     double part[1] = {0.0};
     for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
         const typename Rows::Row row = R.row(i);
-        double q = 0.0;
-        for (int k = 0; k < R.width(); ++k) {
-            const int j = R.col(row, k);
-            q += avals[R.vidx(row, k)] * (v.z[j] + beta * v.p_in[j]);
-        }
+        // q = A z + beta A p, each gathered through the row's stencil
+        Gathered<Rows, double> Zg;
+        Zg.load(R, row, v.z);
+        double qz = 0.0, qp = 0.0;
+#pragma unroll
+        for (int k = 0; k < R.width(); ++k) qz += A[R.vidx(row, k)] * Zg.at(R, row, v.z, k);
+        Gathered<Rows, double> Pg;
+        Pg.load(R, row, v.p_in);
+#pragma unroll
+        for (int k = 0; k < R.width(); ++k) qp += A[R.vidx(row, k)] * Pg.at(R, row, v.p_in, k);
+        const double q = qz + beta * qp;
         const double pi = v.z[i] + beta * v.p_in[i];
         v.p_out[i] = pi;
         v.q[i] = q;
@@ -335,6 +345,18 @@ struct Scratch {
         *out = (T*)p;
         return CMX_OK;
     }
+    // A vector the stencil gathers: c->pad zeroed guard elements on each side.
+    int alloc_gathered(cmx_ctx* c, double** out, size_t count)
+    {
+        double* base = nullptr;
+        const size_t total = count + 2 * (size_t)c->pad;
+        int rc = alloc(c, &base, total);
+        if (rc) return rc;
+        if (hipMemset(base, 0, total * sizeof(double)) != hipSuccess)
+            return afail(c, CMX_E_HIP, "hipMemset of an adjoint vector failed");
+        *out = base + c->pad;
+        return CMX_OK;
+    }
 };
 
 template <int W>
@@ -356,6 +378,7 @@ struct AdjBuffers {
     double* hist;
     AdjCtl* ctl;
     AdjCtl* h_ctl;               // pinned mirror
+    int last_its;                // iterations of the previous solve
 };
 
 // mu = A^{-1} b (b = lambda_V^{n+1}) into B.x, then l_z = M mu.
@@ -369,9 +392,11 @@ int adjoint_solve(cmx_ctx* c, const Rows& R, AdjBuffers& B, double rtol2)
     AdjCgVecs v{B.b, B.x, B.r, B.z, B.q, B.p[0], B.p[1]};
     adj_cg_init_kernel<Rows><<<G, kCgBlock, 0, c->stream>>>(R, c->dinv64, n, v, prz[0]);
     ADJ_HIP(c, hipGetLastError());
-    int it = 0;
+    // launch as many iterations as the previous solve took (plus the launch
+    // that detects convergence), then a few at a time
+    int it = 0, chunk = B.last_its > 0 ? std::min(B.last_its + 1, kAdjFirstMax) : kAdjChunk;
     for (;;) {
-        for (int k = 0; k < kAdjChunk; ++k, ++it) {
+        for (int k = 0; k < chunk; ++k, ++it) {
             AdjCgVecs vi{B.b, B.x, B.r, B.z, B.q, B.p[it & 1], B.p[(it + 1) & 1]};
             adj_cg_spmv_kernel<Rows><<<G, kCgBlock, 0, c->stream>>>(R, c->a64, n, vi, it, rtol2, B.ctl, B.hist,
                                                                      prz[it & 1], G, ppq);
@@ -381,7 +406,11 @@ int adjoint_solve(cmx_ctx* c, const Rows& R, AdjBuffers& B, double rtol2)
         ADJ_HIP(c, hipGetLastError());
         ADJ_HIP(c, hipMemcpyAsync(B.h_ctl, B.ctl, sizeof(AdjCtl), hipMemcpyDeviceToHost, c->stream));
         ADJ_HIP(c, hipStreamSynchronize(c->stream));
-        if (B.h_ctl->done) break;
+        if (B.h_ctl->done) {
+            B.last_its = B.h_ctl->it;
+            break;
+        }
+        chunk = 4;
         if (it >= kAdjMaxIts)
             return afail(c, CMX_E_DIVERGED, "adjoint CG did not converge in " + std::to_string(kAdjMaxIts) +
                                                  " iterations");
@@ -442,12 +471,12 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
     if ((rc = sc.alloc(c, &B.grad, (size_t)std::max(P, 1) * ps))) return rc;
     if ((rc = sc.alloc(c, &B.x, n))) return rc;
     if ((rc = sc.alloc(c, &B.r, n))) return rc;
-    if ((rc = sc.alloc(c, &B.z, n))) return rc;
+    if ((rc = sc.alloc_gathered(c, &B.z, n))) return rc;
     if ((rc = sc.alloc(c, &B.q, n))) return rc;
-    if ((rc = sc.alloc(c, &B.p[0], n))) return rc;
-    if ((rc = sc.alloc(c, &B.p[1], n))) return rc;
+    if ((rc = sc.alloc_gathered(c, &B.p[0], n))) return rc;
+    if ((rc = sc.alloc_gathered(c, &B.p[1], n))) return rc;
     if ((rc = sc.alloc(c, &B.partials, 3 * (size_t)kMaxBlocks * 3))) return rc;
-    if ((rc = sc.alloc(c, &B.hist, kAdjMaxIts + kAdjChunk + 1))) return rc;
+    if ((rc = sc.alloc(c, &B.hist, kAdjMaxIts + kAdjFirstMax + 1))) return rc;
     if ((rc = sc.alloc(c, &B.ctl, 1))) return rc;
     B.b = B.lam_v;
     std::vector<int32_t> h_slot(n, -1);
@@ -457,8 +486,6 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
     if (n_obs_steps > 0 && n_obs > 0)
         ADJ_HIP(c, hipMemcpy(obs_d, obs_values, (size_t)n_obs_steps * n_obs * sizeof(double), hipMemcpyHostToDevice));
     ADJ_HIP(c, hipMemset(B.grad, 0, (size_t)std::max(P, 1) * ps * sizeof(double)));
-    ADJ_HIP(c, hipMemset(B.p[0], 0, n * sizeof(double)));
-    ADJ_HIP(c, hipMemset(B.p[1], 0, n * sizeof(double)));
     AdjCtl* h_ctl = nullptr;
     ADJ_HIP(c, hipHostMalloc((void**)&h_ctl, sizeof(AdjCtl), hipHostMallocDefault));
     struct PinnedFree {
@@ -467,8 +494,6 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
     } pinned{h_ctl};
     B.h_ctl = h_ctl;
 
-    // Step m of the run cmx_step(ctx, t0, n_steps) takes.
-    auto step_at = [&](int m) { return step_range(c, t0, n_steps, m, m + 1); };
     const int go = (n_obs + kAdjBlock - 1) / kAdjBlock;
 
     // Forward: checkpoints every L steps and the observed voltages, stepping
@@ -526,10 +551,18 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
     for (int s0 = (n_ck - 1) * L; s0 >= 0; s0 -= L) {
         const int s1 = std::min(s0 + L, n_steps);
         if ((rc = restore_slot(c, ck + (size_t)(s0 / L) * slot))) return rc;
-        for (int m = s0; m < s1; ++m) {
-            if ((rc = save_slot(c, seg + (size_t)(m - s0) * slot))) return rc;
-            if (m + 1 < s1 && (rc = step_at(m))) return rc;
-        }
+        // replay the segment in one call; its cell kernels write each step's
+        // starting state into seg (plain launches: a captured step has no trace)
+        c->trace = seg;
+        c->trace_k0 = s0;
+        c->trace_len = s1 - s0;
+        c->trace_slot = slot;
+        const int graphs = c->use_graph;
+        c->use_graph = 0;
+        rc = step_range(c, t0, n_steps, s0, s1);
+        c->use_graph = graphs;
+        c->trace = nullptr;
+        if (rc) return rc;
         for (int m = s1 - 1; m >= s0; --m) {
             if ((rc = adjoint_solve<Rows>(c, R, B, rtol * rtol))) return rc;
             const double* st = seg + (size_t)(m - s0) * slot;

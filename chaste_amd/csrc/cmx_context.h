@@ -105,6 +105,11 @@ struct cmx_ctx {
     double qw[4] = {}, qp[4][3] = {};
     int nq = 0;
     double* act = nullptr;       // [5][n] activation times (cmx_set_activation)
+    // the adjoint's replay: ICI steps [trace_k0, trace_k0 + trace_len) write the
+    // state they start from to trace (slots of trace_slot doubles); plain launches only
+    double* trace = nullptr;
+    int trace_k0 = 0, trace_len = 0;
+    size_t trace_slot = 0;
     double act_thr = 0.0;
     int act_every = 0;
     // matrix

@@ -153,6 +153,12 @@ struct CellArgs {
     double c;                       // Am Cm / dt
     double* __restrict__ Y;         // out: V^n + x0
     RV* __restrict__ G1;            // out: F - c x0
+    // ICI steps k in [trace_k0, trace_k0 + trace_len): V^k and the states y^k
+    // the step starts from, to trace + (k - trace_k0) * trace_slot as V at
+    // [0, ps) and state s at [s*ps, (s+1)*ps) (the adjoint's replay), or null
+    double* __restrict__ trace;
+    int trace_k0, trace_len;
+    size_t trace_slot;
 };
 
 // Cell-kernel roles: the ICI step (V clamped over the step), the operator-
@@ -343,6 +349,12 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         f = RV(0);
     } else {
         if (clk.pending) a.V[i] = vn;
+        if (a.trace && k >= a.trace_k0 && k < a.trace_k0 + a.trace_len) {
+            double* tr = a.trace + (size_t)(k - a.trace_k0) * a.trace_slot;
+            tr[i] = vn;
+#pragma unroll
+            for (int s = 1; s < NS; ++s) tr[(size_t)s * a.ps + i] = (double)y[s];
+        }
         const RS V = (RS)vn;
         y[0] = V;
         const RS cm = (RS)a.cm;
@@ -1311,6 +1323,10 @@ int launch_cells(cmx_ctx* c, int mode, const double* verdict, int kind)
     a.c = c->am * c->cm / c->dt;
     a.Y = c->Y;
     a.G1 = (RV*)c->G1;
+    a.trace = mode == kModeIci ? c->trace : nullptr;
+    a.trace_k0 = c->trace_k0;
+    a.trace_len = c->trace_len;
+    a.trace_slot = c->trace_slot;
     const int G = grid_of(c->n);
     if (c->model == CMX_MODEL_LR91) launch_cell_integrator<ModelLr91, RS, RV>(c, mode, kind, G, a);
     else launch_cell_integrator<ModelTt06, RS, RV>(c, mode, kind, G, a);
