@@ -37,6 +37,7 @@ EXTRAPOLATE = 2      # CG initial guess V^n + 2 x_{n-1} - x_{n-2} (corrections x
 
 
 USE_GRAPH = True
+AUTOTUNE = True      # cmx_autotune picks the solver block count after the warmup
 
 
 def build_problem(precision, coords, elems, stim_index, cmx, stim_mag=STIM[0]):
@@ -110,6 +111,12 @@ def time_path(m, steps, warmup, roof_steps, events=True):
     if warmup:
         m.step(t, warmup)
         t += warmup * DT
+    tuned = None
+    if AUTOTUNE:
+        # solver block count picked from the warmed-up state (state restored)
+        chosen, times = m.autotune(t, 20)
+        tuned = {'blocks': chosen, 'ms_per_step': {str(k): round(v, 5) for k, v in times.items()}}
+        progress('autotune: %d blocks (%s)' % (chosen, tuned['ms_per_step']))
     st0 = m.stats()
     sync()
     t0 = time.perf_counter()
@@ -123,7 +130,8 @@ def time_path(m, steps, warmup, roof_steps, events=True):
     ref = (st1['refine_sweeps'] - st0['refine_sweeps']) / steps
     if not events:
         return {'elapsed': elapsed, 'its': its, 'refine': ref, 'ranking': {}, 'rank_its': its,
-                'dominant': 'cell', 'dominant_ms': float('nan'), 'dominant_launches': 1, 'roof_its': its}
+                'dominant': 'cell', 'dominant_ms': float('nan'), 'dominant_launches': 1, 'roof_its': its,
+                'autotune': tuned}
     m.set_profiling(KINDS)
     m.kernel_times(reset=True)
     m.step(t, 5)
@@ -141,7 +149,7 @@ def time_path(m, steps, warmup, roof_steps, events=True):
     roof_its = (st2['cg_iterations'] - st_r['cg_iterations']) / roof_steps
     return {'elapsed': elapsed, 'its': its, 'refine': ref, 'ranking': ranking, 'rank_its': rank_its,
             'dominant': dominant, 'dominant_ms': ktimes[dominant][0], 'dominant_launches': ktimes[dominant][1],
-            'roof_its': roof_its}
+            'roof_its': roof_its, 'autotune': tuned}
 
 
 def working_launches_per_step(kind, its, refined):
@@ -325,6 +333,7 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--skip-propagating', action='store_true', help='only the named-stimulus case')
     ap.add_argument('--no-graph', action='store_true', help='plain launches instead of hipGraph replay')
+    ap.add_argument('--no-autotune', action='store_true', help='the default solver block count (256)')
     ap.add_argument('--no-kernel-events', action='store_true',
                     help='no per-kernel event timing (ranking / roofline passes); for runs under a profiler')
     ap.add_argument('--adjoint', action='store_true',
@@ -334,8 +343,9 @@ def main():
                     help='the batched-cell workload with this many cells instead (65536 for the named sweep)')
     ap.add_argument('--cells-check', type=int, default=256, help='cells also run in FP64 for the accuracy check')
     args = ap.parse_args()
-    global USE_GRAPH
+    global USE_GRAPH, AUTOTUNE
     USE_GRAPH = not args.no_graph
+    AUTOTUNE = not args.no_autotune
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one GPU only: the monodomain slab runs on a single MI355X here '
               '(multi-GPU sharding is a design note, not built); run with --gpus 1', file=sys.stderr)
@@ -387,6 +397,7 @@ def main():
                 'dominant_bytes_per_launch': dom_bytes / max(r['dominant_launches'], 1),
                 'format': 'stencil' if st['format'] == cmx.FMT_STENCIL else 'ell',
                 'n_patterns': st['n_patterns'],
+                'autotune': r['autotune'],
             }
             # accuracy: the same simulation from t = 0 with activation tracking (0 mV)
             progress('%s stim %g: accuracy run' % (name, stim_mag))

@@ -99,6 +99,8 @@ SIGNATURES = {
     'cmx_model_initial_state': (ctypes.c_int, [ctypes.c_int32, _dp]),
     'cmx_model_param': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), _dp]),
     'cmx_set_params': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp]),
+    'cmx_autotune': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _ip,
+                                    ctypes.POINTER(ctypes.c_int32), _dp]),
     'cmx_cells_run': (ctypes.c_int, [ctypes.POINTER(CmxCells), _dp, _dp, _dp, ctypes.POINTER(CmxCellsStats)]),
     'cmx_cells_last_error': (ctypes.c_char_p, []),
     'cmx_adjoint': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -387,6 +389,18 @@ class Monodomain:
         out = np.zeros(self.n)
         _check(lib().cmx_get_iionic(self.ctx, _p(out)), self.ctx)
         return out
+
+    def autotune(self, t0=0.0, n_steps=20, candidates=None):
+        """Time n_steps from the current state for each candidate solver block
+        count, keep the fastest; the state is restored.  Returns (chosen,
+        {blocks: ms per step})."""
+        cand = np.ascontiguousarray([128, 192, 256, 320, 384, 512] if candidates is None else candidates,
+                                    dtype=np.int32)
+        ms = np.zeros(len(cand))
+        chosen = ctypes.c_int32()
+        _check(lib().cmx_autotune(self.ctx, float(t0), int(n_steps), len(cand), _p(cand, _ip), ctypes.byref(chosen),
+                                  _p(ms)), self.ctx)
+        return chosen.value, dict(zip(cand.tolist(), ms.tolist()))
 
     def set_params(self, params):
         """Per-node parameters [n_params, n] (AbstractParameterisedSystem::SetParameter

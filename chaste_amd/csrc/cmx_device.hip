@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -1801,8 +1802,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->width = csr.max_row;
     const int n = c->n;
     const int cg_rows = (n + kCgBlock - 1) / kCgBlock;
-    c->G = std::min(cg_rows, kCgGrid);
-    if (const char* e = std::getenv("CMX_CG_BLOCKS")) c->G = std::max(1, std::min(cg_rows, std::min(atoi(e), kMaxBlocks)));
+    c->G = std::min(cg_rows, kCgGrid);   // cmx_autotune may pick another
 
     CMX_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
@@ -2045,6 +2045,100 @@ int cmx::step_range(cmx_ctx* c, double t0, int n_total, int k0, int k1)
 }
 
 extern "C" {
+
+int cmx_autotune(cmx_ctx* c, double t0, int32_t n_steps, int32_t n_candidates, const int32_t* candidates,
+                 int32_t* chosen, double* ms_per_step)
+{
+    if (!c) return fail(nullptr, CMX_E_ARG, "null context");
+    if (n_steps <= 0) return fail(c, CMX_E_ARG, "n_steps must be positive");
+    const int cg_rows = (c->n + kCgBlock - 1) / kCgBlock;
+    std::vector<int> cand;
+    if (candidates && n_candidates > 0) {
+        for (int k = 0; k < n_candidates; ++k) {
+            if (candidates[k] < 1 || candidates[k] > kMaxBlocks)
+                return fail(c, CMX_E_ARG, "candidate block counts must lie in [1, 1024]");
+            cand.push_back(std::min(candidates[k], cg_rows));
+        }
+    } else {
+        for (int g : {128, 192, 256, 320, 384, 512}) cand.push_back(std::min(g, cg_rows));
+    }
+    CMX_HIP(c, hipStreamSynchronize(c->stream));
+    // everything a trial run changes: the state, the initial-guess history,
+    // activation times, budgets and statistics
+    const size_t rv = c->precision == CMX_MIXED ? sizeof(float) : sizeof(double);
+    const size_t rs = rv;
+    std::vector<char> V(c->n * sizeof(double)), S((size_t)(c->S - 1) * c->ps * rs), X0(c->n * rv), XP(c->n * rv),
+        act(c->act ? 5 * (size_t)c->n * sizeof(double) : 0), Yv(c->Y ? c->n * sizeof(double) : 0), F(c->n * rv);
+    auto copy_out = [&](std::vector<char>& h, const void* d) {
+        return h.empty() ? hipSuccess : hipMemcpy(h.data(), d, h.size(), hipMemcpyDeviceToHost);
+    };
+    auto copy_in = [&](void* d, const std::vector<char>& h) {
+        return h.empty() ? hipSuccess : hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice);
+    };
+    CMX_HIP(c, copy_out(V, c->V));
+    CMX_HIP(c, copy_out(S, c->state));
+    CMX_HIP(c, copy_out(X0, c->vec[V_X0]));
+    CMX_HIP(c, copy_out(XP, c->vec[V_XP]));
+    CMX_HIP(c, copy_out(act, c->act));
+    CMX_HIP(c, copy_out(Yv, c->Y));
+    CMX_HIP(c, copy_out(F, c->F));   // the Iion cache is recovered from F
+    const double last_t_next = c->last_t_next;
+    const int budget = c->budget, refine_budget = c->refine_budget;
+    const long long steps = c->steps, cg_total = c->cg_total, refine_total = c->refine_total, resumes = c->resumes;
+    const bool stepped = c->stepped;
+    auto restore = [&]() -> int {
+        CMX_HIP(c, copy_in(c->V, V));
+        CMX_HIP(c, copy_in(c->state, S));
+        CMX_HIP(c, copy_in(c->vec[V_X0], X0));
+        CMX_HIP(c, copy_in(c->vec[V_XP], XP));
+        CMX_HIP(c, copy_in(c->act, act));
+        CMX_HIP(c, copy_in(c->Y, Yv));
+        CMX_HIP(c, copy_in(c->F, F));
+        c->budget = budget;
+        c->refine_budget = refine_budget;
+        for (auto& g : c->graphs) {
+            (void)hipGraphExecDestroy(g.exec);
+            (void)hipGraphDestroy(g.graph);
+        }
+        c->graphs.clear();
+        return CMX_OK;
+    };
+    const int G0 = c->G;
+    int best = G0;
+    double best_ms = 0.0;
+    int rc = CMX_OK;
+    for (size_t k = 0; k < cand.size() && rc == CMX_OK; ++k) {
+        c->G = cand[k];
+        // one untimed run (graph capture, caches), then the timed one
+        double ms = 0.0;
+        for (int pass = 0; pass < 2 && rc == CMX_OK; ++pass) {
+            if ((rc = restore())) break;
+            CMX_HIP(c, hipDeviceSynchronize());
+            const auto a = std::chrono::steady_clock::now();
+            rc = cmx::step_range(c, t0, n_steps, 0, n_steps);
+            CMX_HIP(c, hipStreamSynchronize(c->stream));
+            ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count() / n_steps;
+        }
+        if (rc) break;
+        if (ms_per_step) ms_per_step[k] = ms;
+        if (k == 0 || ms < best_ms) {
+            best_ms = ms;
+            best = cand[k];
+        }
+    }
+    c->G = rc == CMX_OK ? best : G0;
+    const int rc2 = restore();
+    c->steps = steps;
+    c->cg_total = cg_total;
+    c->refine_total = refine_total;
+    c->resumes = resumes;
+    c->stepped = stepped;
+    c->last_t_next = last_t_next;
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (chosen) *chosen = c->G;
+    return CMX_OK;
+}
 
 int cmx_get_voltage(cmx_ctx* c, double* V)
 {

@@ -137,6 +137,17 @@ const char* cmx_last_error(const cmx_ctx* ctx);
  * (AbstractDynamicLinearPdeSolver::Solve, pde/src/solver/AbstractDynamicLinearPdeSolver.hpp:328-497). */
 int cmx_step(cmx_ctx* ctx, double t0, int32_t n_steps);
 
+/* Launch-shape autotuner (no counterpart in the reference): times n_steps
+ * PDE steps from the current state at t0 for each candidate number of
+ * solver blocks (candidates == NULL: 128, 192, 256, 320, 384, 512, capped
+ * at one block per 1024 rows), keeps the fastest, and restores the state,
+ * the initial-guess history, activation times and statistics, so the next
+ * cmx_step starts where it would have.  The block count changes the
+ * reductions' summation order only (results agree to rounding).
+ * ms_per_step[n_candidates or 6] (may be NULL) gets each candidate's time. */
+int cmx_autotune(cmx_ctx* ctx, double t0, int32_t n_steps, int32_t n_candidates, const int32_t* candidates,
+                 int32_t* chosen, double* ms_per_step);
+
 /* Transmembrane potential (mV): AbstractCardiacProblem::GetSolution /
  * SetInitialCondition (heart/src/problem/AbstractCardiacProblem.cpp:265-292). */
 int cmx_get_voltage(cmx_ctx* ctx, double* V);

@@ -724,3 +724,35 @@ def test_tissue_iionic_cache_matches_single_cell(oracle, v_clamp):
         assert np.abs(got - ref).max() <= 1e-10
         # the PDE step moved V (stimulus and diffusion): the cache is at V^n
         assert np.abs(m.voltage() - v_clamp).max() > 1e-3
+
+
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_autotune_keeps_state_and_results(oracle, precision):
+    """cmx_autotune times candidate solver block counts from the current state
+    and restores it: the run that follows matches an untuned context to
+    rounding (FP64 1e-9 against the oracle; mixed against the untuned run)."""
+    from chaste_amd import cmx
+    print('case autotune', precision, flush=True)
+    args = (2, 40, 40, 0, cmx.LR91, 0, -400000.0, 0.5, 0.02)
+    # the mixed mode solves to the tolerance its FP32 CG reaches, with the FP64 refinement
+    kw = {'precision': precision, 'use_graph': True}
+    if precision:
+        kw.update(rtol=1e-8, refine=1)
+    m = make(*args, **kw)
+    ref = make(*args, **kw)
+    m.step(0.0, 30)
+    ref.step(0.0, 30)
+    chosen, times = m.autotune(0.3, 10, candidates=[1, 2])
+    print('autotune', chosen, times)
+    assert chosen in (1, 2) and all(t > 0 for t in times.values())
+    assert np.array_equal(m.voltage(), ref.voltage()) and np.array_equal(m.iionic(), ref.iionic())
+    m.step(0.3, 200)
+    ref.step(0.3, 200)
+    V, Vr = m.voltage(), ref.voltage()
+    print('after autotune max dV', np.abs(V - Vr).max())
+    if precision == 0:
+        o = oracle.mono_run(2, 40, 40, model=oracle.LR91_CELLML, stim_mag=-400000.0, stim_dur=0.5, stim_xlim=0.02,
+                            t_end=2.3, ksp_rtol=1e-12)['V']
+        assert np.abs(V - o).max() <= FP64_REL * np.abs(o).max()
+    else:
+        assert np.abs(V - Vr).max() <= 1e-3
