@@ -1,0 +1,104 @@
+"""The ellipsoid-shell meshes of the batched-tissue benchmark: a valid,
+conforming tetrahedralisation of the truncated shell (volume against the
+closed form), fibres in the wall's tangent plane rotating -60 -> +60 degrees
+through it; on the GPU, a shell with its fibre tensors against the oracle and
+a disjoint union of shells against the shells run one at a time."""
+import numpy as np
+import pytest
+
+
+def shell(h=0.25):
+    from chaste_amd.shells import ellipsoid_shell
+    return ellipsoid_shell(h)
+
+
+def test_shell_is_a_conforming_tetrahedralisation():
+    coords, elems, fibres, apex = shell(0.2)
+    x = coords[elems]
+    vol6 = np.einsum('ij,ij->i', np.cross(x[:, 1] - x[:, 0], x[:, 2] - x[:, 0]), x[:, 3] - x[:, 0])
+    assert np.abs(vol6).min() > 1e-6 * np.abs(vol6).max()
+    faces = np.sort(np.concatenate([elems[:, [0, 1, 2]], elems[:, [0, 1, 3]], elems[:, [0, 2, 3]],
+                                    elems[:, [1, 2, 3]]]), axis=1)
+    _, counts = np.unique(faces, axis=0, return_counts=True)
+    assert counts.max() == 2                       # every face shared by at most two tetrahedra
+    assert len(np.unique(elems)) == coords.shape[0]
+
+
+def test_shell_volume_and_mass_matrix(oracle):
+    """Truncated at z = 0 the shell is half of each ellipsoid: (2/3) pi (a_out
+    b_out^2 - a_in b_in^2), less the thin apical opening theta < 0.1 and the
+    chords' deficit; the P1 mass matrix sums to the mesh volume."""
+    coords, elems, _, _ = shell(0.1)
+    x = coords[elems]
+    vol = np.abs(np.einsum('ij,ij->i', np.cross(x[:, 1] - x[:, 0], x[:, 2] - x[:, 0]), x[:, 3] - x[:, 0])).sum() / 6
+    exact = 2.0 / 3.0 * np.pi * (3.5 * 2.0 ** 2 - 2.5 * 1.0 ** 2)
+    assert abs(vol - exact) <= 0.02 * exact
+    _, _, M, _ = oracle.assemble(coords, elems, np.eye(3))
+    assert abs(M.sum() - vol) <= 1e-9 * vol
+
+
+def test_shell_fibres_rotate_through_the_wall():
+    """h = 0.2 gives 6 node layers through the 1 cm wall: element layer l has
+    its centroid at s = (l + 1/2) / 5, so its fibre angle is -60 + 120 s:
+    -48, -24, 0, 24, 48 degrees from the circumferential direction."""
+    coords, elems, fibres, apex = shell(0.2)
+    assert np.allclose(np.linalg.norm(fibres, axis=1), 1.0)
+    x = coords[elems].mean(axis=1)
+    phi = np.arctan2(x[:, 1], x[:, 0])
+    e_phi = np.stack([-np.sin(phi), np.cos(phi), np.zeros_like(phi)], axis=1)
+    ang = np.degrees(np.arccos(np.clip(np.abs(np.einsum('ij,ij->i', fibres, e_phi)), 0, 1)))
+    ang *= np.sign(fibres[:, 2])                    # towards the base (+z) is positive
+    # (a tetrahedron's centroid sits up to ~2 degrees of azimuth from its
+    # hexahedron's centre, where the fibre was set: 3 degrees of slack)
+    levels = np.array([-48.0, -24.0, 0.0, 24.0, 48.0])
+    nearest = np.abs(ang[:, None] - levels[None, :]).min(axis=1)
+    assert nearest.max() <= 3.0
+    assert all((np.abs(ang - v) <= 1.0).any() for v in levels)
+    assert apex.sum() > 0
+    assert coords[apex, 2].max() <= -2.5 * np.cos(0.1) + 1e-9
+    assert coords[:, 2].argmin() in np.where(apex)[0]
+
+
+def run_shell_gpu(coords, elems, fibres, apex, steps, precision=0):
+    from chaste_amd import cmx
+    sig = cmx.conductivity_tensors(fibres, (1.75, 0.19), axisymmetric=True)
+    m = cmx.Monodomain(coords, elems, np.array([1.75, 0.19, 0.19]), cmx.TT06_EPI, cmx.RUSH_LARSEN,
+                       stimuli=[(-100000.0, 0.0, 1.0)], stim_index=np.where(apex, 0, -1).astype(np.int32),
+                       precision=precision, ksp_rtol=1e-12 if precision == 0 else 1e-8,
+                       refine_sweeps=0 if precision == 0 else 1, sigma_elem=sig, use_graph=True)
+    m.step(0.0, steps)
+    return m, sig
+
+
+@pytest.mark.gpu
+def test_shell_gpu_matches_oracle(oracle):
+    print('case shell vs oracle', flush=True)
+    coords, elems, fibres, apex = shell(0.25)
+    m, sig = run_shell_gpu(coords, elems, fibres, apex, 300)
+    V = m.voltage()
+    r = oracle.mono_run(3, 0, mesh=(coords, elems), model=oracle.TT06_EPI, rush_larsen=True, sigma=(1.75, 0.19, 0.19),
+                        sigma_elem=sig, stim_mag=-100000.0, stim_dur=1.0, t_end=3.0, ksp_rtol=1e-12,
+                        stim_mask=apex.astype(np.uint8))
+    err = np.abs(V - r['V']).max() / np.abs(r['V']).max()
+    print('shell nodes', len(V), 'format', m.stats()['format'], 'rel err', err, 'V max', V.max())
+    assert V.max() > 0                              # the apical stimulus launched a wave
+    assert err <= 1e-9
+
+
+@pytest.mark.gpu
+def test_shell_union_matches_shells_alone():
+    """Two shells of different resolution stepped as one context (block-diagonal
+    operators, one launch sequence) against each alone: the same solutions to
+    the solves' tolerance (the union's CG stops on the joint residual)."""
+    from chaste_amd.shells import disjoint_union, ellipsoid_shell
+    print('case shell union', flush=True)
+    a, b = ellipsoid_shell(0.3), ellipsoid_shell(0.22)
+    coords, elems, fibres, apex, off = disjoint_union([a, b])
+    mu, _ = run_shell_gpu(coords, elems, fibres, apex, 200)
+    Vu = mu.voltage()
+    for k, mesh in enumerate((a, b)):
+        ms, _ = run_shell_gpu(*mesh, 200)
+        Vs = ms.voltage()
+        d = np.abs(Vu[off[k]:off[k + 1]] - Vs).max() / np.abs(Vs).max()
+        print('shell', k, 'union vs alone', d)
+        assert d <= 1e-9
