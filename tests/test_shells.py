@@ -102,3 +102,62 @@ def test_shell_union_matches_shells_alone():
         d = np.abs(Vu[off[k]:off[k + 1]] - Vs).max() / np.abs(Vs).max()
         print('shell', k, 'union vs alone', d)
         assert d <= 1e-9
+
+
+@pytest.mark.gpu
+def test_shell_regional_scaling_gradient():
+    """The checkpointed adjoint on a shell (TT06 RL, FP64, fibre tensors, ELL):
+    the gradient of a voltage mismatch with respect to regional scalings of
+    g_Na and g_Kr (four regions by quadrant of azimuth) is the per-node
+    gradient summed over each region; against central differences of the
+    device's forward run, to 1e-4."""
+    from chaste_amd import cmx
+    print('case shell regional gradient', flush=True)
+    coords, elems, fibres, apex = shell(0.3)
+    n = coords.shape[0]
+    sig = cmx.conductivity_tensors(fibres, (1.75, 0.19), axisymmetric=True)
+    m = cmx.Monodomain(coords, elems, np.array([1.75, 0.19, 0.19]), cmx.TT06_EPI, cmx.RUSH_LARSEN,
+                       stimuli=[(-100000.0, 0.0, 1.0)], stim_index=np.where(apex, 0, -1).astype(np.int32),
+                       precision=cmx.FP64, ksp_rtol=1e-12, sigma_elem=sig)
+    y0 = m.state()
+    defaults = np.array([d for _, d in cmx.param_info(cmx.TT06_EPI)])
+    region = ((np.arctan2(coords[:, 1], coords[:, 0]) + np.pi) // (np.pi / 2)).astype(int).clip(0, 3)
+    rng = np.random.default_rng(4)
+    obs = np.sort(rng.choice(n, 24, replace=False)).astype(np.int32)
+    steps, every = 400, 20
+    ids = (0, 2)                                     # g_Na, g_Kr
+
+    def params(scal):                                # scal [2, 4]: per region
+        p = np.repeat(defaults[:, None], n, axis=1)
+        for k, q in enumerate(ids):
+            p[q] *= scal[k][region]
+        return p
+
+    def observe(scal):
+        m.set_state(y0)
+        m.set_params(params(scal))
+        out = []
+        for j in range(steps // every):
+            m.step(j * every * 0.01, every)
+            out.append(m.voltage()[obs])
+        return np.array(out)
+
+    truth = np.array([[0.7, 1.0, 1.3, 1.0], [1.0, 1.4, 0.8, 1.0]])
+    data = observe(truth)
+    s = np.ones((2, 4))
+    m.set_state(y0)
+    m.set_params(params(s))
+    loss, gp, _ = m.adjoint(0.0, steps, obs, every, data, checkpoint_every=50)
+    g = np.array([[(gp[q] * defaults[q])[region == r].sum() for r in range(4)] for q in ids])
+    print('loss', loss, 'regional gradient', g)
+    assert loss > 0
+    for k in range(2):
+        for r in range(4):
+            d = np.zeros((2, 4))
+            d[k, r] = 1.0
+            eps = 1e-4
+            d1 = (((observe(s + eps * d) - data) ** 2).sum() - ((observe(s - eps * d) - data) ** 2).sum()) / (2 * eps)
+            d2 = (((observe(s + eps / 2 * d) - data) ** 2).sum() - ((observe(s - eps / 2 * d) - data) ** 2).sum()) / eps
+            fd = (4 * d2 - d1) / 3
+            print('param', ids[k], 'region', r, 'adjoint', g[k, r], 'fd', fd)
+            assert abs(g[k, r] - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(g).max())
