@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from conftest import ROOT
 
@@ -51,3 +52,26 @@ def test_model_metadata():
     assert names[0] == 'membrane__V' and names[-1] == 'intracellular_calcium_concentration__Cai'
     y = cmx.initial_state(cmx.LR91)
     assert y[0] == -83.853 and y[7] == 0.0002
+
+
+def test_entry_points_validate_before_touching_the_device():
+    """Argument errors of the batched-cell, adjoint and autotune entry points
+    come back as CMX_E_ARG with a message, before any device work."""
+    import ctypes
+
+    from chaste_amd import cmx
+    for kw in ({'t_end': [1.0025], 'dt': 0.005}, {'t_end': [1.0], 'dt': 0.0}, {'t_end': [-1.0], 'dt': 0.01}):
+        with pytest.raises(cmx.CmxError) as e:
+            cmx.cells_run(cmx.TT06_EPI, cmx.RUSH_LARSEN, kw['t_end'], kw['dt'])
+        assert e.value.code == -1 and cmx.lib().cmx_cells_last_error().decode()
+    with pytest.raises(cmx.CmxError):
+        cmx.cells_run(cmx.TT06_EPI, cmx.BACKWARD_EULER, [1.0], 0.01)
+    with pytest.raises(cmx.CmxError):
+        cmx.cells_run(cmx.LR91, cmx.FORWARD_EULER, [1.0], 0.01, params=np.ones((4, 1)))
+    with pytest.raises(cmx.CmxError):
+        cmx.cells_run(7, cmx.FORWARD_EULER, [1.0], 0.01)
+    loss = np.zeros(1)
+    assert cmx.lib().cmx_adjoint(None, 0.0, 10, 5, 0, None, 5, None, loss.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 None, None) == -1
+    assert cmx.lib().cmx_autotune(None, 0.0, 10, 0, None, None, None) == -1
+    assert cmx.lib().cmx_set_params(None, 0, None) == -1
