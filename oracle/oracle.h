@@ -26,6 +26,11 @@
  *                                                  pde/src/solver/AbstractDynamicLinearPdeSolver.hpp:328-497
  *   stimulus   SimpleStimulus / RegularStimulus    heart/src/stimulus/SimpleStimulus.cpp:44-76,
  *                                                  heart/src/stimulus/RegularStimulus.cpp:58-80
+ *
+ * Pins beyond the reference's golden files (tests/test_oracle_golden.py):
+ *   orc_cell_compute_except_voltage   hand and generated LR91 agree, V held,
+ *                                     two half intervals equal one
+ *   orc_cell_compute_v                equals orc_cell_compute's V at every step
  */
 #ifndef CMX_ORACLE_H_
 #define CMX_ORACLE_H_

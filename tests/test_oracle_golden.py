@@ -708,3 +708,15 @@ def test_cell_properties_restatement(oracle):
     tr = oracle.cell_trace_v(oracle.TT06_EPI, y, 900.0, 0.005, stim=(-80.0, 2.0, 0.0, 450.0), integrator=1)
     assert np.array_equal(oracle.cell_properties(tr, 0.005, -30.0), loop(tr, 0.005, -30.0))
     assert oracle.cell_properties(tr, 0.005, -30.0)[0] == 2
+
+
+def test_cell_trace_v_matches_cell_compute(oracle):
+    """orc_cell_compute_v is orc_cell_compute (pinned above against the
+    ValidData files) keeping V only: the same voltages at every step."""
+    for model, integrator, dt, stim in ((oracle.LR91_CELLML, 0, 0.01, (-25.5, 2.0, 1.0, -1.0)),
+                                        (oracle.TT06_EPI, 1, 0.005, (-80.0, 2.0, 0.0, 300.0))):
+        y1 = oracle.model_initial_state(model)
+        y2 = y1.copy()
+        v = oracle.cell_trace_v(model, y1, 30.0, dt, stim=stim, integrator=integrator)
+        samples = oracle.cell_compute(model, y2, 0.0, 30.0, dt, stim=stim, integrator=integrator)
+        assert np.array_equal(v, samples[:, 0]) and np.array_equal(y1, y2)
