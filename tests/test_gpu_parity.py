@@ -756,3 +756,25 @@ def test_autotune_keeps_state_and_results(oracle, precision):
         assert np.abs(V - o).max() <= FP64_REL * np.abs(o).max()
     else:
         assert np.abs(V - Vr).max() <= 1e-3
+
+
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_svi_cable_goldens_gpu(precision):
+    """TestMonodomainWithSvi.hpp:175-307 on the device: V at x = 0.3 of the
+    1 cm LR91 cable with ICI and SVI on three meshes (FP64 to the reference's
+    tolerances at rtol 1e-12; mixed within 0.5 mV of them)."""
+    from chaste_amd import cmx
+    from test_oracle_golden import SVI_1D
+    print('case svi cable', precision, flush=True)
+    for h, (node, ici, ici_tol, svi, svi_tol) in sorted(SVI_1D.items()):
+        coords, elems = cmx.slab_mesh(1, int(round(1.0 / h)), 0, 0, h)
+        stim_index = np.where(coords[:, 0] < 0.02 + 2e-6, 0, -1).astype(np.int32)
+        for use_svi, want, tol in ((False, ici, ici_tol), (True, svi, svi_tol)):
+            m = cmx.Monodomain(coords, elems, np.array([1.75]), cmx.LR91, cmx.FORWARD_EULER,
+                               stimuli=[(-1e6, 0.0, 0.5)], stim_index=stim_index, precision=precision,
+                               ksp_rtol=1e-12 if precision == 0 else 1e-8, refine_sweeps=precision, svi=use_svi,
+                               use_graph=True)
+            m.step(0.0, 400)
+            v = m.voltage()[node]
+            print('h', h, 'svi' if use_svi else 'ici', v, 'golden', want)
+            assert abs(v - want) <= (tol if precision == 0 else max(tol, 0.5))

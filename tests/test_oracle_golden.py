@@ -720,3 +720,30 @@ def test_cell_trace_v_matches_cell_compute(oracle):
         v = oracle.cell_trace_v(model, y1, 30.0, dt, stim=stim, integrator=integrator)
         samples = oracle.cell_compute(model, y2, 0.0, 30.0, dt, stim=stim, integrator=integrator)
         assert np.array_equal(v, samples[:, 0]) and np.array_equal(y1, y2)
+
+
+# TestMonodomainWithSvi.hpp:175-307: a 1 cm LR91 cable, default conductivity
+# 1.75, Am 1400, Cm 1, -1e6 uA/cm^3 for 0.5 ms on x <= 0.02, 4 ms; V at
+# x = 0.3 on three meshes, ionic-current (ICI) and state-variable (SVI)
+# interpolation.  In 1-D the reference's block-Jacobi ILU(0) preconditioner
+# is the exact factorisation of the tridiagonal matrix, so its solve is exact
+# whatever the tolerance: the runs here solve to rtol 1e-12 (at the default
+# absolute 2e-4, Jacobi-PCG stops up to 0.02 mV short).
+SVI_1D = {0.001: (300, 11.0067, 0.12, 11.0067, 2e-4), 0.01: (30, 19.8924, 1e-3, 14.9579, 1e-3),
+          0.02: (15, 24.4938, 1e-3, 17.3131, 1e-3)}
+
+
+@pytest.mark.parametrize('h', sorted(SVI_1D))
+def test_svi_cable_goldens(oracle, h):
+    node, ici, ici_tol, svi, svi_tol = SVI_1D[h]
+    got = {}
+    for use_svi in (False, True):
+        r = oracle.mono_run(1, int(round(1.0 / h)), h=h, model=oracle.LR91_CELLML, sigma=(1.75,) * 3, stim_mag=-1e6,
+                            stim_dur=0.5, stim_xlim=0.02 + 2e-6, t_end=4.0, ksp_rtol=1e-12, svi=use_svi)
+        got[use_svi] = r['V']
+    assert abs(got[False][node] - ici) <= ici_tol and abs(got[True][node] - svi) <= svi_tol
+    if h == 0.001:
+        # on the finest mesh the two agree to 0.3 mV everywhere and differ away from rest
+        assert np.abs(got[False] - got[True]).max() <= 0.3
+        moving = got[False] > -80
+        assert np.all(got[False][moving] != got[True][moving])
