@@ -89,6 +89,33 @@ static void RunPrintingTimesCase(const std::string& meshes, const std::string& o
     std::fflush(stdout);
 }
 
+// Per-node cell parameters through SetCellParameter (the reference sets them
+// on each cell, AbstractParameterisedSystem::SetParameter): the 1-D case of
+// RunCase with g_Na = 0 everywhere cannot fire, so node 7 stays below -40 mV
+// while the unmodified run reaches 13.7 mV; an unknown name throws.
+static void RunParameterCase()
+{
+    HeartConfig::Reset();
+    HeartConfig::Instance()->SetSpaceDimension(1);
+    HeartConfig::Instance()->SetFibreLength(1, 0.01);
+    HeartConfig::Instance()->SetSimulationDuration(5.0);
+    PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, 1> cell_factory(-600 * 5000);
+    MonodomainProblem<1> monodomain_problem(&cell_factory);
+    monodomain_problem.Initialise();
+    const unsigned n = monodomain_problem.rGetMesh().GetNumNodes();
+    monodomain_problem.SetCellParameter("fast_sodium_current__g_Na", std::vector<double>(n, 0.0));
+    int thrown = 0;
+    try { monodomain_problem.SetCellParameter("no_such_parameter", std::vector<double>(n, 1.0)); }
+    catch (const Exception& e) { ++thrown; }
+    monodomain_problem.Solve();
+    const double v7 = monodomain_problem.GetSolution()[7];
+    const bool ok = v7 < -40.0 && thrown == 1;   // measured -60.5 mV; an action potential would be above 0
+    if (!ok) ++failures;
+    std::printf("{\"case\": \"cell_parameter\", \"V7\": %.4f, \"thrown\": %d, \"pass\": %s}\n", v7, thrown,
+                ok ? "true" : "false");
+    std::fflush(stdout);
+}
+
 // argv[1]: folder holding the mesh files; argv[2]: output root.  Results
 // files go to <root>/MonodomainCreatesGeometry/monodomain3d.h5,
 // <root>/MonoProblem1d/MonodomainLR91_1d.h5 and
@@ -128,6 +155,7 @@ int main(int argc, char** argv)
     const bool errors_ok = thrown == 3;
     if (argc > 1) RunMeshFileCase(argv[1], out("MonoProblem1d"));
     if (argc > 2) RunPrintingTimesCase(argv[1], out("MonoProblem1dRequestedTimes"));
+    RunParameterCase();
     if (!errors_ok) ++failures;
     std::printf("{\"case\": \"errors\", \"thrown\": %d, \"pass\": %s}\n", thrown, errors_ok ? "true" : "false");
     return failures == 0 ? 0 : 1;
