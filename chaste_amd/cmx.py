@@ -23,7 +23,7 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 class CmxMesh(ctypes.Structure):
     _fields_ = [('dim', ctypes.c_int32), ('n_nodes', ctypes.c_int32), ('coords', _dp),
                 ('n_elems', ctypes.c_int32), ('elems', _ip), ('sigma_elem', _dp),
-                ('sigma', ctypes.c_double * 9)]
+                ('sigma', ctypes.c_double * 9), ('elem_dim', ctypes.c_int32)]
 
 
 class CmxStimulus(ctypes.Structure):
@@ -75,6 +75,9 @@ SIGNATURES = {
     'cmx_destroy': (ctypes.c_int, [ctypes.c_void_p]),
     'cmx_read_mesh': (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _dp, _ip, _dp]),
+    'cmx_read_mesh_elements': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                              ctypes.POINTER(ctypes.c_int32), _dp, _ip, _dp]),
     'cmx_read_fibres': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _dp]),
     'cmx_conductivity_tensors': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp, _dp]),
     'cmx_io_last_error': (ctypes.c_char_p, []),
@@ -228,18 +231,20 @@ def _io_check(rc):
         raise RuntimeError(lib().cmx_io_last_error().decode())
 
 
-def read_mesh(base):
-    """TrianglesMeshReader: <base>.node/.ele (ASCII or binary) ->
-    (coords[n, dim], elems[ne, dim+1] from zero, elem_attr[ne, n_attr])."""
+def read_mesh(base, elem_dim=0):
+    """TrianglesMeshReader: <base>.node/.ele (ASCII or binary), or with
+    elem_dim = 1 the line elements of <base>.edge (a cable in 2-D or 3-D
+    space) -> (coords[n, dim], elems[ne, elem_dim+1] from zero, elem_attr[ne, n_attr])."""
     d, n, ne, na = (ctypes.c_int32() for _ in range(4))
     b = os.fsencode(base)
-    _io_check(lib().cmx_read_mesh(b, ctypes.byref(d), ctypes.byref(n), ctypes.byref(ne), ctypes.byref(na),
-                                  None, None, None))
+    _io_check(lib().cmx_read_mesh_elements(b, elem_dim, ctypes.byref(d), ctypes.byref(n), ctypes.byref(ne),
+                                           ctypes.byref(na), None, None, None))
+    ed = elem_dim if elem_dim > 0 else d.value
     coords = np.zeros((n.value, d.value))
-    elems = np.zeros((ne.value, d.value + 1), dtype=np.int32)
+    elems = np.zeros((ne.value, ed + 1), dtype=np.int32)
     attr = np.zeros((ne.value, na.value))
-    _io_check(lib().cmx_read_mesh(b, ctypes.byref(d), ctypes.byref(n), ctypes.byref(ne), ctypes.byref(na),
-                                  _p(coords), _p(elems, _ip), _p(attr) if na.value else None))
+    _io_check(lib().cmx_read_mesh_elements(b, elem_dim, ctypes.byref(d), ctypes.byref(n), ctypes.byref(ne),
+                                           ctypes.byref(na), _p(coords), _p(elems, _ip), _p(attr) if na.value else None))
     return coords, elems, attr
 
 
@@ -317,6 +322,7 @@ class Monodomain:
         m = CmxMesh()
         m.dim, m.n_nodes, m.coords = dim, n, _p(coords)
         m.n_elems, m.elems = elems.shape[0], _p(elems, _ip)
+        m.elem_dim = elems.shape[1] - 1      # line elements (cables) when below dim
         sig = np.zeros(9)
         s = np.asarray(sigma, dtype=np.float64)
         if s.ndim == 1:

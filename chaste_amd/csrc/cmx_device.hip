@@ -1761,6 +1761,8 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
         return fail(c, CMX_E_ARG, "bad stimulus table (at most 255 rows)");
     c->n = mesh->n_nodes;
     c->dim = mesh->dim;
+    if (mesh->elem_dim > 0 && mesh->elem_dim != mesh->dim && opts->state_variable_interpolation)
+        return fail(c, CMX_E_ARG, "state-variable interpolation needs elements of the node dimension");
     c->model = tissue->model;
     c->integrator = tissue->integrator;
     c->precision = opts->precision;

@@ -65,9 +65,19 @@ bool simplex_geometry(int dim, const double* X /*[(dim+1)][dim]*/, double grads[
 
 bool assemble_p1(const cmx_mesh& mesh, bool lumped, HostCsr& out, std::string& err)
 {
-    const int dim = mesh.dim, n = mesh.n_nodes, ne = mesh.n_elems, nb = dim + 1;
+    const int dim = mesh.dim, n = mesh.n_nodes, ne = mesh.n_elems;
+    // elements of the node dimension, or line elements (cables) in 2-D / 3-D
+    const int edim = mesh.elem_dim > 0 ? mesh.elem_dim : dim, nb = edim + 1;
     if (dim < 1 || dim > 3 || n <= 0 || ne <= 0 || !mesh.coords || !mesh.elems) {
         err = "mesh: bad dimension, sizes or null arrays";
+        return false;
+    }
+    if (edim != dim && edim != 1) {
+        err = "mesh: elements must have the node dimension or be lines";
+        return false;
+    }
+    if (edim != dim && mesh.sigma_elem) {
+        err = "mesh: line elements take the scalar conductivity sigma[0], not per-element tensors";
         return false;
     }
     for (long e = 0; e < (long)ne * nb; ++e)
@@ -101,18 +111,39 @@ bool assemble_p1(const cmx_mesh& mesh, bool lumped, HostCsr& out, std::string& e
     out.mass.assign(nnz, 0.0);
     out.stiff.assign(nnz, 0.0);
 
-    const double mass_scale = 1.0 / ((dim + 1) * (dim + 2));
+    const double mass_scale = 1.0 / ((edim + 1) * (edim + 2));
+    // a line's conductivity acts along it: sigma[0] I gives g_a . sigma g_b = sigma[0] g_a . g_b
+    double line_sigma[9] = {};
+    for (int r = 0; r < dim; ++r) line_sigma[r * dim + r] = mesh.sigma[0];
     for (int e = 0; e < ne; ++e) {
         const int32_t* nd = mesh.elems + (size_t)e * nb;
         double X[4 * 3];
         for (int a = 0; a < nb; ++a)
             for (int r = 0; r < dim; ++r) X[a * dim + r] = mesh.coords[(size_t)nd[a] * dim + r];
-        double g[4][3], vol;
-        if (!simplex_geometry(dim, X, g, vol)) {
-            err = "mesh: element " + std::to_string(e) + " is degenerate";
-            return false;
+        double g[4][3] = {}, vol;
+        if (edim == dim) {
+            if (!simplex_geometry(dim, X, g, vol)) {
+                err = "mesh: element " + std::to_string(e) + " is degenerate";
+                return false;
+            }
+        } else {
+            // line element: length L, basis gradients -t/L and t/L along its unit tangent t
+            double t[3] = {}, L2 = 0;
+            for (int r = 0; r < dim; ++r) {
+                t[r] = X[dim + r] - X[r];
+                L2 += t[r] * t[r];
+            }
+            vol = std::sqrt(L2);
+            if (!(vol > 0)) {
+                err = "mesh: element " + std::to_string(e) + " is degenerate";
+                return false;
+            }
+            for (int r = 0; r < dim; ++r) {
+                g[0][r] = -t[r] / L2;
+                g[1][r] = t[r] / L2;
+            }
         }
-        const double* sig = mesh.sigma_elem ? mesh.sigma_elem + (size_t)e * dim * dim : mesh.sigma;
+        const double* sig = edim != dim ? line_sigma : mesh.sigma_elem ? mesh.sigma_elem + (size_t)e * dim * dim : mesh.sigma;
         // (sigma G^T) columns
         double sg[4][3];
         for (int b = 0; b < nb; ++b)

@@ -48,7 +48,7 @@ struct MeshFile {
 
 // Header "n dim n_attributes n_boundary_markers [BIN]" of a .node file, and
 // "n nodes_per_element n_attributes [BIN]" of a .ele file.
-int read_mesh(const std::string& base, MeshFile& m)
+int read_mesh(const std::string& base, MeshFile& m, int elem_dim = 0)
 {
     std::ifstream nf(base + ".node", std::ios::binary);
     if (!nf) return io_fail("cannot open " + base + ".node");
@@ -87,20 +87,32 @@ int read_mesh(const std::string& base, MeshFile& m)
                 if (!(s >> m.coords[(size_t)i * m.dim + d])) return io_fail(base + ".node: bad coordinate");
         }
     }
-    std::ifstream ef(base + ".ele", std::ios::binary);
-    if (!ef) return io_fail("cannot open " + base + ".ele");
-    if (!next_data_line(ef, line)) return io_fail(base + ".ele: no header");
+    // Elements of the node dimension come from <base>.ele ("n nodes_per_element
+    // n_attributes [BIN]"); line elements of a mesh in 2-D or 3-D space
+    // (TrianglesMeshReader<1, SPACE_DIM>) from <base>.edge, whose header is
+    // "n n_attributes [BIN]" with two nodes per item.
+    if (elem_dim <= 0) elem_dim = m.dim;
+    if (elem_dim != m.dim && elem_dim != 1) return io_fail(base + ": only volume or line elements are supported");
+    const std::string ext = elem_dim == m.dim ? ".ele" : ".edge";
+    std::ifstream ef(base + ext, std::ios::binary);
+    if (!ef) return io_fail("cannot open " + base + ext);
+    if (!next_data_line(ef, line)) return io_fail(base + ext + ": no header");
     bool ebin = false;
     {
         std::istringstream h(line);
         std::string extra;
-        if (!(h >> m.n_elems >> m.nodes_per_elem >> m.n_elem_attr)) return io_fail(base + ".ele: bad header");
+        if (elem_dim == m.dim) {
+            if (!(h >> m.n_elems >> m.nodes_per_elem >> m.n_elem_attr)) return io_fail(base + ext + ": bad header");
+        } else {
+            if (!(h >> m.n_elems >> m.n_elem_attr)) return io_fail(base + ext + ": bad header");
+            m.nodes_per_elem = 2;
+        }
         h >> extra;
         ebin = extra == "BIN";
     }
-    if (ebin != m.binary) return io_fail(base + ": .node and .ele disagree on BIN");
-    if (m.n_elems <= 0 || m.nodes_per_elem != m.dim + 1 || m.n_elem_attr < 0)
-        return io_fail(base + ".ele: only linear simplices of the node dimension are supported");
+    if (ebin != m.binary) return io_fail(base + ": .node and " + ext + " disagree on BIN");
+    if (m.n_elems <= 0 || m.nodes_per_elem != elem_dim + 1 || m.n_elem_attr < 0)
+        return io_fail(base + ext + ": only linear simplices are supported");
     m.elems.resize((size_t)m.n_elems * m.nodes_per_elem);
     m.elem_attr.resize((size_t)m.n_elems * m.n_elem_attr);
     for (int e = 0; e < m.n_elems; ++e) {
@@ -109,26 +121,26 @@ int read_mesh(const std::string& base, MeshFile& m)
         if (m.binary) {
             // node indices as 32-bit unsigned, then the attributes as doubles
             std::vector<uint32_t> idx(m.nodes_per_elem);
-            if (!ef.read((char*)idx.data(), idx.size() * sizeof(uint32_t))) return io_fail(base + ".ele: truncated");
+            if (!ef.read((char*)idx.data(), idx.size() * sizeof(uint32_t))) return io_fail(base + ext + ": truncated");
             for (int k = 0; k < m.nodes_per_elem; ++k) nd[k] = (int32_t)idx[k];
             if (m.n_elem_attr && !ef.read((char*)at, m.n_elem_attr * sizeof(double)))
-                return io_fail(base + ".ele: truncated");
+                return io_fail(base + ext + ": truncated");
         } else {
-            if (!next_data_line(ef, line)) return io_fail(base + ".ele: unexpected end of file");
+            if (!next_data_line(ef, line)) return io_fail(base + ext + ": unexpected end of file");
             std::istringstream s(line);
             long idx;
             if (!(s >> idx) || idx != e + base_index)
-                return io_fail(base + ".ele: data for item " + std::to_string(e) + " missing");
+                return io_fail(base + ext + ": data for item " + std::to_string(e) + " missing");
             for (int k = 0; k < m.nodes_per_elem; ++k) {
                 long v;
-                if (!(s >> v)) return io_fail(base + ".ele: bad node index");
+                if (!(s >> v)) return io_fail(base + ext + ": bad node index");
                 nd[k] = (int32_t)(v - base_index);
             }
             for (int a = 0; a < m.n_elem_attr; ++a)
-                if (!(s >> at[a])) return io_fail(base + ".ele: error in reading attribute index " + std::to_string(a));
+                if (!(s >> at[a])) return io_fail(base + ext + ": error in reading attribute index " + std::to_string(a));
         }
         for (int k = 0; k < m.nodes_per_elem; ++k)
-            if (nd[k] < 0 || nd[k] >= m.n_nodes) return io_fail(base + ".ele: node index out of range");
+            if (nd[k] < 0 || nd[k] >= m.n_nodes) return io_fail(base + ext + ": node index out of range");
     }
     return CMX_OK;
 }
@@ -142,9 +154,15 @@ const char* cmx_io_last_error(void) { return g_io_error.c_str(); }
 int cmx_read_mesh(const char* base, int32_t* dim, int32_t* n_nodes, int32_t* n_elems, int32_t* n_elem_attr,
                   double* coords, int32_t* elems, double* elem_attr)
 {
+    return cmx_read_mesh_elements(base, 0, dim, n_nodes, n_elems, n_elem_attr, coords, elems, elem_attr);
+}
+
+int cmx_read_mesh_elements(const char* base, int32_t elem_dim, int32_t* dim, int32_t* n_nodes, int32_t* n_elems,
+                           int32_t* n_elem_attr, double* coords, int32_t* elems, double* elem_attr)
+{
     if (!base || !dim || !n_nodes || !n_elems) return io_fail("null argument");
     MeshFile m;
-    const int rc = read_mesh(base, m);
+    const int rc = read_mesh(base, m, elem_dim);
     if (rc) return rc;
     *dim = m.dim;
     *n_nodes = m.n_nodes;

@@ -64,6 +64,9 @@ typedef struct {
     const int32_t* elems;      /* [n_elems][dim+1] node indices */
     const double* sigma_elem;  /* [n_elems][dim][dim] intracellular conductivity, or NULL */
     double sigma[9];           /* [dim][dim] used when sigma_elem == NULL (mS/cm) */
+    int32_t elem_dim;          /* 0 or dim: simplices of the node dimension; 1: line elements
+                                  (cables) in 2-D or 3-D space, elems [n_elems][2], conductivity
+                                  sigma[0] along the line (MonodomainProblem<1, SPACE_DIM>) */
 } cmx_mesh;
 
 /* Stimulus protocol, one row of a table (SimpleStimulus when period <= 0,
@@ -229,6 +232,13 @@ int cmx_adjoint(cmx_ctx* ctx, double t0, int32_t n_steps, int32_t checkpoint_eve
  * message in cmx_io_last_error(). */
 int cmx_read_mesh(const char* base, int32_t* dim, int32_t* n_nodes, int32_t* n_elems, int32_t* n_elem_attr,
                   double* coords, int32_t* elems, double* elem_attr);
+/* The same for a mesh whose elements have dimension elem_dim (0 = the node
+ * dimension, as cmx_read_mesh): line elements (elem_dim 1) of a mesh in 2-D
+ * or 3-D space come from <base>.edge, header "n n_attributes [BIN]", as
+ * TrianglesMeshReader<1, SPACE_DIM> reads them
+ * (mesh/src/reader/TrianglesMeshReader.cpp:690-712); elems [n_elems][elem_dim+1]. */
+int cmx_read_mesh_elements(const char* base, int32_t elem_dim, int32_t* dim, int32_t* n_nodes, int32_t* n_elems,
+                           int32_t* n_elem_attr, double* coords, int32_t* elems, double* elem_attr);
 
 /* Reads a fibre orientation file (FibreReader, heart/src/io/FibreReader.cpp:
  * 246-335): a header with the number of items (and optionally BIN), then

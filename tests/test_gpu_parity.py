@@ -778,3 +778,28 @@ def test_svi_cable_goldens_gpu(precision):
             v = m.voltage()[node]
             print('h', h, 'svi' if use_svi else 'ici', v, 'golden', want)
             assert abs(v - want) <= (tol if precision == 0 else max(tol, 0.5))
+
+
+def test_cable_in_3d_mesh_file_gpu():
+    """TestMonodomainProblem.hpp:348-394 (MonodomainProblem<1, 3>): the 1 mm
+    cable of mesh/test/data/1D_in_3D_0_to_1mm_10_elements (line elements in
+    3-D space) gives the 1-D case's voltages at nodes 1, 3, 5, 7, 9, 10 (atol
+    5e-3), and the same solution as the 1-D mesh file to rounding."""
+    from chaste_amd import cmx
+    from test_oracle_golden import MESH_CASES, check_mesh_case
+    print('case cable in 3-D', flush=True)
+    c = MESH_CASES['1d']
+    out = {}
+    for name, base, ed in (('3d', '1D_in_3D_0_to_1mm_10_elements', 1), ('1d', c['mesh'], 0)):
+        coords, elems, _ = cmx.read_mesh(os.path.join(GOLDEN, 'meshes', base), elem_dim=ed)
+        stim_index = np.where(coords[:, 0] ** 2 <= 1e-10, 0, -1).astype(np.int32)
+        dim = coords.shape[1]
+        m = cmx.Monodomain(coords, elems, np.full(dim, c['sigma']), cmx.LR91, cmx.FORWARD_EULER,
+                           stimuli=[(c['stim'], 0.0, 0.5)], stim_index=stim_index, precision=cmx.FP64,
+                           ksp_rtol=0.0, ksp_atol=2e-4, am=1.0, cm=1.0, dt_pde=0.01, dt_ode=0.01)
+        m.step(0.0, 200)
+        out[name] = (coords, m.voltage())
+    check_mesh_case('1d', out['1d'][0], out['3d'][1])
+    d = np.abs(out['3d'][1] - out['1d'][1]).max()
+    print('cable in 3-D vs 1-D max |dV|', d)
+    assert d <= 1e-9

@@ -149,3 +149,17 @@ def test_hdf5_writer_round_trip(tmp_path):
         pass
     e = h5lite.File(str(tmp_path / 'e.h5'))
     assert e.dataset('Data').shape == (0, 3, 1)
+
+
+def test_read_cable_in_3d():
+    """mesh/test/data/1D_in_3D_0_to_1mm_10_elements: 11 nodes on the line
+    y = 1, z = 0.5 and 10 line elements from its .edge file."""
+    import numpy as np
+    from chaste_amd import cmx
+    base = os.path.join(GOLDEN, 'meshes', '1D_in_3D_0_to_1mm_10_elements')
+    coords, elems, attr = cmx.read_mesh(base, elem_dim=1)
+    assert coords.shape == (11, 3) and elems.shape == (10, 2)
+    assert np.allclose(coords[:, 0], np.arange(11) * 0.01) and np.all(coords[:, 1] == 1.0) and np.all(coords[:, 2] == 0.5)
+    assert np.array_equal(elems, np.stack([np.arange(10), np.arange(1, 11)], axis=1))
+    with pytest.raises(RuntimeError):
+        cmx.read_mesh(base)              # no .ele: volume elements are absent
