@@ -41,6 +41,8 @@ static void RunCase(const char* name, const double golden[4], double duration, c
 // heart/test/monodomain/TestMonodomainProblem.hpp:226-290 on its mesh file
 // (mesh/test/data/1D_0_to_1mm_10_elements): sigma 0.0005, Am = Cm = 1 set
 // between Initialise() and Solve(), 2 ms; nodes 1, 3, 5, 7, 9, 10 (atol 5e-3).
+static std::vector<double> g_mesh_file_V;   // the single 2 ms run, for the two-halves case
+
 static void RunMeshFileCase(const std::string& meshes, const std::string& out_dir)
 {
     HeartConfig::Reset();
@@ -64,8 +66,40 @@ static void RunMeshFileCase(const std::string& meshes, const std::string& out_di
     for (int q = 0; q < 6; ++q) worst = std::max(worst, std::fabs(V[nodes[q]] - golden[q]));
     const bool ok = worst < 5e-3;
     if (!ok) ++failures;
+    g_mesh_file_V = V;
     std::printf("{\"case\": \"mesh_file_1d\", \"V9\": %.6f, \"max_err\": %.3e, \"pass\": %s}\n", V[9], worst,
                 ok ? "true" : "false");
+    std::fflush(stdout);
+}
+
+// heart/test/monodomain/TestMonodomainProblem.hpp:1178-1225: the same problem
+// solved to 1 ms, then (SetSimulationDuration(2.0), Solve() again) to 2 ms:
+// the goldens (atol 5e-3) and the single run's voltages (5e-11).
+static void RunTwoHalvesCase(const std::string& meshes)
+{
+    HeartConfig::Reset();
+    HeartConfig::Instance()->SetIntracellularConductivities({0.0005});
+    HeartConfig::Instance()->SetMeshFileName(meshes + "/1D_0_to_1mm_10_elements");
+    HeartConfig::Instance()->SetSurfaceAreaToVolumeRatio(1.0);
+    HeartConfig::Instance()->SetCapacitance(1.0);
+    PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, 1> cell_factory;
+    MonodomainProblem<1> monodomain_problem(&cell_factory);
+    monodomain_problem.Initialise();
+    HeartConfig::Instance()->SetSimulationDuration(1.0);
+    monodomain_problem.Solve();
+    HeartConfig::Instance()->SetSimulationDuration(2.0);
+    monodomain_problem.Solve();
+    std::vector<double> V = monodomain_problem.GetSolution();
+    const unsigned nodes[6] = {1, 3, 5, 7, 9, 10};
+    const double golden[6] = {20.7710232, 21.5319692, 22.9280817, 24.0611303, -0.770330519, -19.2234919};
+    double worst = 0, vs_single = 0;
+    for (int q = 0; q < 6; ++q) worst = std::max(worst, std::fabs(V[nodes[q]] - golden[q]));
+    for (size_t i = 0; i < V.size() && i < g_mesh_file_V.size(); ++i)
+        vs_single = std::max(vs_single, std::fabs(V[i] - g_mesh_file_V[i]));
+    const bool ok = worst < 5e-3 && g_mesh_file_V.size() == V.size() && vs_single <= 5e-11;
+    if (!ok) ++failures;
+    std::printf("{\"case\": \"two_halves\", \"max_err\": %.3e, \"vs_single\": %.3e, \"pass\": %s}\n", worst,
+                vs_single, ok ? "true" : "false");
     std::fflush(stdout);
 }
 
@@ -154,6 +188,7 @@ int main(int argc, char** argv)
     try { TetrahedralMesh<2> m; m.ConstructRegularSlabMesh(0.03, 0.1, 0.1); } catch (const Exception& e) { ++thrown; }
     const bool errors_ok = thrown == 3;
     if (argc > 1) RunMeshFileCase(argv[1], out("MonoProblem1d"));
+    if (argc > 1) RunTwoHalvesCase(argv[1]);
     if (argc > 2) RunPrintingTimesCase(argv[1], out("MonoProblem1dRequestedTimes"));
     RunParameterCase();
     if (!errors_ok) ++failures;
