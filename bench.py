@@ -351,7 +351,7 @@ def shells_case(args):
         m = cmx.Monodomain(coords, elems, np.array([1.75, 0.19, 0.19]), cmx.TT06_EPI, cmx.RUSH_LARSEN,
                            stimuli=[(-100000.0, 0.0, 1.0)], stim_index=stim_index, precision=prec, ksp_rtol=1e-8,
                            ksp_max_its=500, refine_sweeps=1 if prec == cmx.MIXED else 0, sigma_elem=sig,
-                           use_graph=True, extrapolate=EXTRAPOLATE)
+                           use_graph=USE_GRAPH, extrapolate=EXTRAPOLATE)
         setup = time.perf_counter() - t
         m.set_activation(0.0, 1, 0.0)
         m.step(0.0, warm)
