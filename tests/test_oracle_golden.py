@@ -747,3 +747,44 @@ def test_svi_cable_goldens(oracle, h):
         assert np.abs(got[False] - got[True]).max() <= 0.3
         moving = got[False] > -80
         assert np.all(got[False][moving] != got[True][moving])
+
+
+# TestMonodomainExactSolution.hpp:48-395: with a toy cell whose ionic current
+# is alpha exp(-t) Phi(x) and stimulus beta exp(-t) Phi(x) (both at the step's
+# end time, as SolveCellSystems evaluates them), Phi = prod cos(pi x_i), the
+# monodomain equation has the solution V = exp(-t) Phi when
+# -Am Cm + Am alpha = -(sum sigma_ii) pi^2 - beta.  Am = 2, Cm = 1.5,
+# alpha = 1.5 - pi^2/2, beta = -pi^2; every printed V within 0.1% (1-D),
+# 0.2% (2-D) or 3% (3-D) of exp(-t).  Run on the oracle's assembly and PCG.
+EXACT_CASES = {1: (0.01, (2.0,), 0.001), 2: (0.02, (1.2, 0.8), 0.002), 3: (0.05, (0.8, 0.7, 0.5), 0.03)}
+
+
+@pytest.mark.parametrize('dim', [1, 2, 3])
+def test_monodomain_exact_solution(oracle, dim):
+    h, sig, tol = EXACT_CASES[dim]
+    ne = int(round(1.0 / h))
+    coords, elems = oracle.slab(dim, ne, ne if dim > 1 else 0, ne if dim > 2 else 0, h=h)
+    rp, col, M, K = oracle.assemble(coords, elems, np.diag(sig))
+    am, cm, dt = 2.0, 1.5, 0.01
+    alpha, beta = 1.5 - 0.5 * np.pi ** 2, -np.pi ** 2
+    assert abs((-am * cm + am * alpha) - (-sum(sig) * np.pi ** 2 - beta)) < 1e-12
+    phi = np.prod(np.cos(np.pi * coords), axis=1)
+    A = am * cm / dt * M + K
+    n = coords.shape[0]
+    rows = np.repeat(np.arange(n), np.diff(rp))
+
+    def mat(vals, x):
+        return np.bincount(rows, weights=vals * x[col], minlength=n)
+    V = phi.copy()
+    for k in range(100):
+        t_next = (k + 1) * dt
+        z = am * cm / dt * V - am * alpha * np.exp(-t_next) * phi - beta * np.exp(-t_next) * phi
+        V, its = oracle.pcg(rp, col, A, mat(M, z), V, rtol=1e-12)
+        assert its >= 0
+        exact = np.exp(-t_next) * phi
+        assert np.abs(V - exact).max() <= tol * np.exp(-t_next)
+    # a wrong sign on the stimulus breaks it
+    if dim == 1:
+        z = am * cm / dt * phi - am * alpha * np.exp(-dt) * phi + beta * np.exp(-dt) * phi
+        Vb, _ = oracle.pcg(rp, col, A, mat(M, z), phi, rtol=1e-12)
+        assert np.abs(Vb - np.exp(-dt) * phi).max() > tol
