@@ -803,3 +803,40 @@ def test_cable_in_3d_mesh_file_gpu():
     d = np.abs(out['3d'][1] - out['1d'][1]).max()
     print('cable in 3-D vs 1-D max |dV|', d)
     assert d <= 1e-9
+
+
+def test_face_stimulus_3d_mesh_file_gpu():
+    """TestMonodomainSlab.hpp:59-114 on the device (see the oracle test)."""
+    from chaste_amd import cmx
+    print('case face stimulus 3-D', flush=True)
+    coords, elems, _ = cmx.read_mesh(os.path.join(GOLDEN, 'meshes', '3D_0_to_1mm_6000_elements'))
+    m = cmx.Monodomain(coords, elems, np.full(3, 1.75), cmx.LR91, cmx.FORWARD_EULER,
+                       stimuli=[(-600000.0, 0.0, 0.5)],
+                       stim_index=np.where(coords[:, 0] ** 2 <= 1e-10, 0, -1).astype(np.int32), precision=cmx.FP64,
+                       ksp_rtol=0.0, ksp_atol=2e-4, use_graph=True)
+    m.step(0.0, 400)
+    v = m.voltage()[coords[:, 0] == 0.1]
+    print('face x = 0.1:', v.min(), v.max())
+    assert np.abs(v - 21.3).max() <= 1.0 and np.abs(v - v[0]).max() <= 1.1
+
+
+def test_mass_lumping_against_consistent_mass_gpu():
+    """TestMonodomainMassLumping.hpp:51-103: a 3 mm cube at h 0.05 with
+    backward-Euler LR91 cells (ODE step 0.01, PDE step 0.1 ms), -3e5 uA/cm^3
+    for 1 ms on x = 0, 20 ms: the lumped-mass solution within 30 h = 1.5 mV of
+    the consistent one at every node."""
+    from chaste_amd import cmx
+    print('case mass lumping cube', flush=True)
+    coords, elems = cmx.slab_mesh(3, 6, 6, 6, 0.05)
+    stim_index = np.where(coords[:, 0] ** 2 <= 1e-10, 0, -1).astype(np.int32)
+    V = {}
+    for lumped in (False, True):
+        m = cmx.Monodomain(coords, elems, np.full(3, 1.75), cmx.LR91, cmx.BACKWARD_EULER,
+                           stimuli=[(-3e5, 0.0, 1.0)], stim_index=stim_index, precision=cmx.FP64, ksp_rtol=0.0,
+                           ksp_atol=2e-4, dt_pde=0.1, dt_ode=0.01, mass_lumping=lumped, use_graph=True)
+        m.step(0.0, 200)
+        V[lumped] = m.voltage()
+    d = np.abs(V[True] - V[False]).max()
+    print('lumped vs consistent max |dV|', d, 'V range', V[False].min(), V[False].max())
+    assert d <= 30 * 0.05
+    assert V[False].min() > 0          # the whole cube is in its plateau at 20 ms (measured 3.3-4.4 mV)

@@ -788,3 +788,17 @@ def test_monodomain_exact_solution(oracle, dim):
         z = am * cm / dt * phi - am * alpha * np.exp(-dt) * phi + beta * np.exp(-dt) * phi
         Vb, _ = oracle.pcg(rp, col, A, mat(M, z), phi, rtol=1e-12)
         assert np.abs(Vb - np.exp(-dt) * phi).max() > tol
+
+
+def test_face_stimulus_3d_mesh_file(oracle):
+    """TestMonodomainSlab.hpp:59-114: mesh/test/data/3D_0_to_1mm_6000_elements,
+    sigma 1.75, -600000 uA/cm^3 for 0.5 ms on x = 0, 4 ms: every node on
+    x = 0.1 at 21.3 +- 1 mV and within 1.1 mV of the first of them."""
+    coords, elems = oracle.read_triangles(os.path.join(GOLDEN, 'meshes', '3D_0_to_1mm_6000_elements'))
+    r = oracle.mono_run(3, 0, mesh=(coords, elems), model=oracle.LR91_CELLML, sigma=(1.75,) * 3, stim_mag=-600000.0,
+                        stim_dur=0.5, t_end=4.0, ksp_rtol=0.0, ksp_atol=2e-4,
+                        stim_mask=(coords[:, 0] ** 2 <= 1e-10).astype(np.uint8))
+    face = coords[:, 0] == 0.1
+    assert face.sum() > 0
+    v = r['V'][face]
+    assert np.abs(v - 21.3).max() <= 1.0 and np.abs(v - v[0]).max() <= 1.1
