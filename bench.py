@@ -326,7 +326,8 @@ def cells_case(args):
 def shells_case(args):
     """The batched-tissue workload: 8 truncated prolate-ellipsoid shells
     (inner semi-axes 2.5/1.0 cm, outer 3.5/2.0 cm, cut at the base), meshed as
-    tetrahedra at h = 0.06 ... 0.04 cm (0.16M - 0.52M nodes each, uneven),
+    tetrahedra at h = 0.04 ... 0.02 cm (0.52M - 4.0M nodes each, uneven;
+    --shell-h sets the range),
     fibres rotating -60 -> +60 degrees through the wall, conductivities
     1.75/0.19 mS/cm, TT06-epi RL, -100000 uA/cm^3 for 1 ms on each apical rim.
     The shells form one context (a disjoint union: block-diagonal operators,
@@ -335,7 +336,8 @@ def shells_case(args):
     from chaste_amd import cmx
     from chaste_amd.shells import disjoint_union, ellipsoid_shell
     progress('shells: meshing')
-    meshes = [ellipsoid_shell(h) for h in np.linspace(0.06, 0.04, 8)]
+    h_max, h_min = args.shell_h
+    meshes = [ellipsoid_shell(h) for h in np.linspace(h_max, h_min, 8)]
     sizes = [int(m[0].shape[0]) for m in meshes]
     coords, elems, fibres, apex, _ = disjoint_union(meshes)
     del meshes
@@ -363,12 +365,16 @@ def shells_case(args):
         el = time.perf_counter() - t
         runs[name] = (m.voltage(), m.activation()[4])
         steps = first - warm
-        if prec == cmx.MIXED and total > first:
+        k = first
+        while prec == cmx.MIXED and k < total:
+            chunk = min(2000, total - k)
             t = time.perf_counter()
-            m.step(first * DT, total - first)
+            m.step(k * DT, chunk)
             sync()
             el += time.perf_counter() - t
-            steps += total - first
+            steps += chunk
+            k += chunk
+            progress('shells: mixed at %g ms' % (k * DT))
         st1 = m.stats()
         out[name] = {'steps_per_s': round(steps / el, 2), 'node_steps_per_s': round(steps * n / el, 1),
                      'timed_steps': steps, 'setup_s': round(setup, 2),
@@ -376,6 +382,9 @@ def shells_case(args):
                      'format': 'stencil' if st1['format'] == cmx.FMT_STENCIL else 'ell',
                      'row_width': st1['row_width']}
         m.close()
+    n_elems = int(elems.shape[0])
+    del coords, elems, sig, stim_index
+    adj = shell_adjoint(h_max)
     dv = np.abs(runs['mixed'][0] - runs['fp64'][0])
     a64, a32 = runs['fp64'][1], runs['mixed'][1]
     both = (a64 >= 0) & (a32 >= 0)
@@ -384,9 +393,9 @@ def shells_case(args):
         'value': out['mixed']['steps_per_s'], 'unit': 'steps/s', 'higher_is_better': True, 'n_gpus': 1,
         'dtype': 'f32 (f64 V, dots, residual)', 'data': 'synthetic (generated shells, default cell state)',
         'config': {'workload': '8 truncated prolate-ellipsoid shells (2.5/1.0 -> 3.5/2.0 cm), tetrahedra at h '
-                               '0.06..0.04 cm, fibres -60 -> +60 deg, sigma 1.75/0.19, TT06-epi RL dt 0.01 ms, '
+                               '%g..%g cm, fibres -60 -> +60 deg, sigma 1.75/0.19, TT06-epi RL dt 0.01 ms, '
                                'apical stimulus -100000 uA/cm^3 1 ms, Jacobi-CG rtol 1e-8; mixed to %g ms'
-                               % (total * DT), 'n_nodes': n, 'n_elems': int(elems.shape[0]), 'shell_nodes': sizes},
+                               % (h_max, h_min, total * DT), 'n_nodes': n, 'n_elems': n_elems, 'shell_nodes': sizes},
         'paths': out,
         'mixed_over_fp64': round(out['mixed']['steps_per_s'] / out['fp64']['steps_per_s'], 3),
         'accuracy_mixed_vs_fp64': {
@@ -395,8 +404,88 @@ def shells_case(args):
             'activation_mismatched_nodes': int(((a64 >= 0) != (a32 >= 0)).sum()),
             'activation_max_err_ms': float(np.abs(a32[both] - a64[both]).max()) if both.any() else None},
         'note': 'activation tracking is on in the timed steps',
+        'regional_adjoint': adj,
     }
     print(json.dumps(line))
+
+
+def shell_adjoint(h):
+    """The batched-tissue workload's gradient: on the coarsest shell (h), TT06
+    RL in FP64 (Jacobi-PCG rtol 1e-12), the gradient of the squared voltage
+    mismatch at 64 seeded nodes every 0.1 ms over 20 ms with respect to
+    regional scalings of g_Na and g_Kr (four regions: quadrants of azimuth),
+    by the checkpointed adjoint (checkpoints every 100 steps), against
+    Richardson-combined central differences of the device forward run."""
+    from chaste_amd import cmx
+    from chaste_amd.shells import ellipsoid_shell
+    coords, elems, fibres, apex = ellipsoid_shell(h)
+    n = coords.shape[0]
+    steps, every = 2000, 10
+    progress('shells: adjoint setup (%d nodes)' % n)
+    sig = cmx.conductivity_tensors(fibres, (1.75, 0.19), axisymmetric=True)
+    m = cmx.Monodomain(coords, elems, np.array([1.75, 0.19, 0.19]), cmx.TT06_EPI, cmx.RUSH_LARSEN,
+                       stimuli=[(-100000.0, 0.0, 1.0)], stim_index=np.where(apex, 0, -1).astype(np.int32),
+                       precision=cmx.FP64, ksp_rtol=1e-12, ksp_atol=0.0, ksp_max_its=2000, sigma_elem=sig,
+                       use_graph=True)
+    y0 = m.state()
+    defaults = np.array([d for _, d in cmx.param_info(cmx.TT06_EPI)])
+    names = [nm.split('__')[-1] for nm, _ in cmx.param_info(cmx.TT06_EPI)]
+    ids = (names.index('g_Na'), names.index('g_Kr'))
+    region = ((np.arctan2(coords[:, 1], coords[:, 0]) + np.pi) // (np.pi / 2)).astype(int).clip(0, 3)
+    rng = np.random.default_rng(5)
+    obs = np.sort(rng.choice(n, 64, replace=False)).astype(np.int32)
+
+    def params(scal):                                # scal [2, 4]: per parameter and region
+        p = np.repeat(defaults[:, None], n, axis=1)
+        for k, q in enumerate(ids):
+            p[q] *= scal[k][region]
+        return p
+
+    def observe(scal):
+        m.set_state(y0)
+        m.set_params(params(scal))
+        out = []
+        for j in range(steps // every):
+            m.step(j * every * DT, every)
+            out.append(m.voltage()[obs])
+        return np.array(out)
+
+    data = observe(np.array([[0.7, 1.0, 1.3, 1.0], [1.0, 1.4, 0.8, 1.0]]))
+    s1 = np.ones((2, 4))
+    m.set_state(y0)
+    m.set_params(params(s1))
+    t = time.perf_counter()
+    m.step(0.0, steps)
+    sync()
+    t_fwd = time.perf_counter() - t
+    progress('shells: adjoint gradient')
+    m.set_state(y0)
+    t = time.perf_counter()
+    loss, gp, _ = m.adjoint(0.0, steps, obs, every, data, 100)
+    t_adj = time.perf_counter() - t
+    g = np.array([[(gp[q] * defaults[q])[region == r].sum() for r in range(4)] for q in ids])
+
+    def central(d, eps):
+        return (((observe(s1 + eps * d) - data) ** 2).sum() - ((observe(s1 - eps * d) - data) ** 2).sum()) / (2 * eps)
+
+    rel = []
+    for k in range(2):
+        for r in range(4):
+            d = np.zeros((2, 4))
+            d[k, r] = 1.0
+            for eps in (1e-4, 1e-6):                 # as in the adjoint workload: re-difference across a jump
+                d1, d2 = central(d, eps), central(d, eps / 2)
+                if abs(d1 - d2) <= 1e-5 * abs(d2):
+                    break
+            fd = (4 * d2 - d1) / 3
+            rel.append(abs(g[k, r] - fd) / max(abs(fd), 1e-3 * np.abs(g).max()))
+            progress('shells: adjoint %s region %d: %.6e vs %.6e (eps %g)' % (names[ids[k]], r, g[k, r], fd, eps))
+    m.close()
+    return {'n_nodes': n, 'steps': steps, 'forward_s': round(t_fwd, 4), 'gradient_s': round(t_adj, 4),
+            'gradient_over_forward': round(t_adj / t_fwd, 2), 'loss': loss,
+            'regional_gradient': {names[q]: [float(x) for x in g[k]] for k, q in enumerate(ids)},
+            'fd_max_rel_err': float(max(rel)), 'fd_within_1e-4': int(sum(r <= 1e-4 for r in rel)),
+            'fd_checks': len(rel)}
 
 
 def main():
@@ -420,6 +509,8 @@ def main():
     ap.add_argument('--cells-check', type=int, default=256, help='cells also run in FP64 for the accuracy check')
     ap.add_argument('--shells', action='store_true', help='the batched ellipsoid-shell tissue workload instead')
     ap.add_argument('--shell-steps', type=int, default=10000)
+    ap.add_argument('--shell-h', type=float, nargs=2, default=(0.04, 0.02), metavar=('H_MAX', 'H_MIN'),
+                    help='element sizes of the first and last shell (cm)')
     args = ap.parse_args()
     global USE_GRAPH, AUTOTUNE
     USE_GRAPH = not args.no_graph
