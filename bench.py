@@ -376,11 +376,29 @@ def shells_case(args):
             k += chunk
             progress('shells: mixed at %g ms' % (k * DT))
         st1 = m.stats()
+        # 20 more steps with HIP events on every launch (plain launches): each
+        # kernel kind's time per step, and the ELL iteration's bandwidth over
+        # the launches that iterated (the others exit on the converged test)
+        m.set_profiling(KINDS)
+        m.kernel_times(reset=True)
+        m.step(k * DT, 20)
+        kt = m.kernel_times(reset=True)
+        m.set_profiling([])
+        st2 = m.stats()
+        s_b = 4 if prec == cmx.MIXED else 8
+        w = st2['row_width']
+        its = st2['cg_iterations'] - st1['cg_iterations']
+        upd_bytes = n * (14 * s_b + w * (4 + s_b))          # vectors + one ELL row of A (column, value)
+        kernels = {kk: {'ms_per_step': round(kt[kk][0] / 20, 4), 'launches_per_step': kt[kk][1] / 20}
+                   for kk in KINDS if kt[kk][1]}
+        if kt['update'][1] and its:
+            kernels['update']['working_launch_GBps'] = round(upd_bytes * its / (kt['update'][0] * 1e-3) / 1e9, 1)
+            kernels['update']['bytes_per_row'] = 14 * s_b + w * (4 + s_b)
         out[name] = {'steps_per_s': round(steps / el, 2), 'node_steps_per_s': round(steps * n / el, 1),
                      'timed_steps': steps, 'setup_s': round(setup, 2),
                      'cg_its_per_step': round((st1['cg_iterations'] - st0['cg_iterations']) / steps, 3),
                      'format': 'stencil' if st1['format'] == cmx.FMT_STENCIL else 'ell',
-                     'row_width': st1['row_width']}
+                     'row_width': st1['row_width'], 'kernels_20_steps': kernels}
         m.close()
     n_elems = int(elems.shape[0])
     del coords, elems, sig, stim_index
