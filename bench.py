@@ -191,7 +191,7 @@ def adjoint_case(args):
     stim_index = np.where(coords[:, 0] < 0.02, 0, -1).astype(np.int32)
     m = cmx.Monodomain(coords, elems, np.array([1.75, 1.75]), cmx.LR91, cmx.FORWARD_EULER,
                        stimuli=[(-400000.0, 0.0, 0.5)], stim_index=stim_index, precision=cmx.FP64,
-                       ksp_rtol=1e-12, ksp_atol=0.0, use_graph=True)
+                       ksp_rtol=1e-12, ksp_atol=0.0, use_graph=USE_GRAPH)
     y0 = m.state()
     defaults = np.array([d for _, d in cmx.param_info(cmx.LR91)])
     rng = np.random.default_rng(2024)
@@ -444,7 +444,7 @@ def shell_adjoint(h):
     m = cmx.Monodomain(coords, elems, np.array([1.75, 0.19, 0.19]), cmx.TT06_EPI, cmx.RUSH_LARSEN,
                        stimuli=[(-100000.0, 0.0, 1.0)], stim_index=np.where(apex, 0, -1).astype(np.int32),
                        precision=cmx.FP64, ksp_rtol=1e-12, ksp_atol=0.0, ksp_max_its=2000, sigma_elem=sig,
-                       use_graph=True)
+                       use_graph=USE_GRAPH)
     y0 = m.state()
     defaults = np.array([d for _, d in cmx.param_info(cmx.TT06_EPI)])
     names = [nm.split('__')[-1] for nm, _ in cmx.param_info(cmx.TT06_EPI)]

@@ -166,21 +166,10 @@ __global__ void gather_kernel(int m, const int32_t* __restrict__ idx, const doub
 }
 
 // ---------------------------------------------------------------------------
-// FP64 Jacobi-PCG for A mu = b from mu = 0, stopping on the preconditioned
-// norm (||D^-1 r|| <= rtol ||D^-1 b||), two kernels per iteration with the
-// same fence-free slot reductions as the forward solver.  Iteration it:
-//   spmv:   beta = (r,z)_it / (r,z)_{it-1} (0 at it = 0),
-//           p' = z + beta p, q = A p' (p' gathered as z_j + beta p_j), (p', q)
-//   update: alpha = (r,z) / (p', q), x += alpha p', r -= alpha q, z = D^-1 r,
-//           (r,z), (z,z) for the next iteration
-// with p and p' in alternating buffers.
+// The adjoint's linear solves A mu = b (A symmetric) run on the forward
+// step's pipelined Jacobi-PCG kernels (cmx::plain_cg_start / plain_cg_iters,
+// one kernel per iteration); l_z = M mu is one more product.
 // ---------------------------------------------------------------------------
-struct AdjCtl {
-    double tol2;
-    int32_t done;      // 1 converged
-    int32_t it;
-};
-
 template <class Rows>
 __global__ __launch_bounds__(kCgBlock) void adj_spmv_kernel(Rows R, const double* __restrict__ vals, int n,
                                                             const double* __restrict__ x, double* __restrict__ y)
@@ -193,120 +182,6 @@ __global__ __launch_bounds__(kCgBlock) void adj_spmv_kernel(Rows R, const double
         for (int k = 0; k < R.width(); ++k) s += vals[R.vidx(row, k)] * x[R.col(row, k)];
         y[i] = s;
     }
-}
-
-struct AdjCgVecs {
-    const double* __restrict__ b;
-    double* __restrict__ x;
-    double* __restrict__ r;
-    double* __restrict__ z;
-    double* __restrict__ q;
-    double* __restrict__ p_in;
-    double* __restrict__ p_out;
-};
-
-template <class Rows>
-__global__ __launch_bounds__(kCgBlock) void adj_cg_init_kernel(Rows R, const double* __restrict__ dinv, int n,
-                                                               AdjCgVecs v, double* __restrict__ pout)
-{
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const RowRange rr(n, blk, gridDim.x);
-    double part[3] = {0.0, 0.0, 0.0};
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
-        const double d = dinv[R.didx_of(i)];
-        const double bi = v.b[i], zi = d * bi;
-        v.x[i] = 0.0;
-        v.r[i] = bi;
-        v.z[i] = zi;
-        part[0] += bi * zi;
-        part[1] += zi * zi;
-        part[2] += zi * zi;
-    }
-    write_partials<kCgBlock, 3>(part, pout, blk);
-}
-
-template <class Rows>
-__global__ __launch_bounds__(kCgBlock) void adj_cg_spmv_kernel(Rows R, const double* __restrict__ avals, int n,
-                                                               AdjCgVecs v, int it, double rtol2, AdjCtl* ctl,
-                                                               double* __restrict__ hist,
-                                                               const double* __restrict__ pin, int G,
-                                                               double* __restrict__ pout)
-{
-    __shared__ double sA[kMaxPatterns * kMaxStencil];
-    const TableLoad<Rows, double> ta(avals, R.value_count());
-    PartialLoad<3> pl(pin, G);
-    const int done = ctl->done;
-    const double tol2_set = ctl->tol2;
-    const double prev = it > 0 ? hist[it - 1] : 1.0;
-    const double* __restrict__ A = ta.store(sA, avals);
-    double g[3];
-    pl.total(g);    // its barriers also cover the table stores
-    if (done) return;
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const bool lead = blk == 0 && threadIdx.x == 0;
-    const double tol2 = it == 0 ? rtol2 * g[2] : tol2_set;
-    if (lead) {
-        if (it == 0) ctl->tol2 = tol2;
-        hist[it] = g[0];
-        ctl->it = it;
-    }
-    if (g[1] <= tol2) {
-        if (lead) ctl->done = 1;
-        return;
-    }
-    const double beta = it == 0 ? 0.0 : g[0] / prev;
-    const RowRange rr(n, blk, gridDim.x);
-    double part[1] = {0.0};
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
-        const typename Rows::Row row = R.row(i);
-        // q = A z + beta A p, each gathered through the row's stencil
-        Gathered<Rows, double> Zg;
-        Zg.load(R, row, v.z);
-        double qz = 0.0, qp = 0.0;
-#pragma unroll
-        for (int k = 0; k < R.width(); ++k) qz += A[R.vidx(row, k)] * Zg.at(R, row, v.z, k);
-        Gathered<Rows, double> Pg;
-        Pg.load(R, row, v.p_in);
-#pragma unroll
-        for (int k = 0; k < R.width(); ++k) qp += A[R.vidx(row, k)] * Pg.at(R, row, v.p_in, k);
-        const double q = qz + beta * qp;
-        const double pi = v.z[i] + beta * v.p_in[i];
-        v.p_out[i] = pi;
-        v.q[i] = q;
-        part[0] += pi * q;
-    }
-    write_partials<kCgBlock, 1>(part, pout, blk);
-}
-
-template <class Rows>
-__global__ __launch_bounds__(kCgBlock) void adj_cg_update_kernel(Rows R, const double* __restrict__ dinv, int n,
-                                                                 AdjCgVecs v, const AdjCtl* ctl,
-                                                                 const double* __restrict__ prz,
-                                                                 const double* __restrict__ ppq, int G,
-                                                                 double* __restrict__ pout)
-{
-    PartialLoad<3> pl(prz, G);
-    PartialLoad<1> pq(ppq, G);
-    const int done = ctl->done;
-    double g[3], h[1];
-    pl.total(g);
-    pq.total(h);
-    if (done) return;
-    const double alpha = g[0] / h[0];
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const RowRange rr(n, blk, gridDim.x);
-    double part[3] = {0.0, 0.0, 0.0};
-    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
-        const double pi = v.p_out[i];
-        v.x[i] += alpha * pi;
-        const double ri = v.r[i] - alpha * v.q[i];
-        const double zi = dinv[R.didx_of(i)] * ri;
-        v.r[i] = ri;
-        v.z[i] = zi;
-        part[0] += ri * zi;
-        part[1] += zi * zi;
-    }
-    write_partials<kCgBlock, 3>(part, pout, blk);
 }
 
 // ---------------------------------------------------------------------------
@@ -373,49 +248,37 @@ StencilRows<W> adj_stencil_rows(cmx_ctx* c)
 
 struct AdjBuffers {
     double *lam_v, *lam_y, *lz, *grad;
-    double *b, *x, *r, *z, *q, *p[2];
-    double* partials;            // 3 buffers of [kMaxBlocks * 3]
-    double* hist;
-    AdjCtl* ctl;
-    AdjCtl* h_ctl;               // pinned mirror
+    const double* b;             // right-hand side of the solve: lambda_V^{n+1}
+    PlainCg cg;                  // the solve's vectors, reductions and control
+    SolveCtl* h_ctl;             // pinned mirror of cg.ctl[0]
     int last_its;                // iterations of the previous solve
 };
 
-// mu = A^{-1} b (b = lambda_V^{n+1}) into B.x, then l_z = M mu.
+// mu = A^{-1} b (b = lambda_V^{n+1}) into B.cg.x, then l_z = M mu.
 template <class Rows>
-int adjoint_solve(cmx_ctx* c, const Rows& R, AdjBuffers& B, double rtol2)
+int adjoint_solve(cmx_ctx* c, const Rows& R, AdjBuffers& B)
 {
-    const int n = c->n, G = c->G;
-    double* prz[2] = {B.partials, B.partials + (size_t)kMaxBlocks * 3};
-    double* ppq = B.partials + 2 * (size_t)kMaxBlocks * 3;
-    ADJ_HIP(c, hipMemsetAsync(B.ctl, 0, sizeof(AdjCtl), c->stream));
-    AdjCgVecs v{B.b, B.x, B.r, B.z, B.q, B.p[0], B.p[1]};
-    adj_cg_init_kernel<Rows><<<G, kCgBlock, 0, c->stream>>>(R, c->dinv64, n, v, prz[0]);
-    ADJ_HIP(c, hipGetLastError());
+    int rc = plain_cg_start(c, B.cg, B.b);
+    if (rc) return rc;
     // launch as many iterations as the previous solve took (plus the launch
     // that detects convergence), then a few at a time
     int it = 0, chunk = B.last_its > 0 ? std::min(B.last_its + 1, kAdjFirstMax) : kAdjChunk;
     for (;;) {
-        for (int k = 0; k < chunk; ++k, ++it) {
-            AdjCgVecs vi{B.b, B.x, B.r, B.z, B.q, B.p[it & 1], B.p[(it + 1) & 1]};
-            adj_cg_spmv_kernel<Rows><<<G, kCgBlock, 0, c->stream>>>(R, c->a64, n, vi, it, rtol2, B.ctl, B.hist,
-                                                                     prz[it & 1], G, ppq);
-            adj_cg_update_kernel<Rows><<<G, kCgBlock, 0, c->stream>>>(R, c->dinv64, n, vi, B.ctl, prz[it & 1], ppq,
-                                                                       G, prz[(it + 1) & 1]);
-        }
-        ADJ_HIP(c, hipGetLastError());
-        ADJ_HIP(c, hipMemcpyAsync(B.h_ctl, B.ctl, sizeof(AdjCtl), hipMemcpyDeviceToHost, c->stream));
+        if ((rc = plain_cg_iters(c, B.cg, it, chunk))) return rc;
+        it += chunk;
+        ADJ_HIP(c, hipMemcpyAsync(B.h_ctl, B.cg.ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, c->stream));
         ADJ_HIP(c, hipStreamSynchronize(c->stream));
-        if (B.h_ctl->done) {
+        if (B.h_ctl->done == 1) {
             B.last_its = B.h_ctl->it;
             break;
         }
-        chunk = 4;
-        if (it >= kAdjMaxIts)
+        if (B.h_ctl->done == 2)
             return afail(c, CMX_E_DIVERGED, "adjoint CG did not converge in " + std::to_string(kAdjMaxIts) +
                                                  " iterations");
+        if (B.h_ctl->done == 3) return afail(c, CMX_E_DIVERGED, "adjoint CG: non-finite residual");
+        chunk = 4;
     }
-    adj_spmv_kernel<Rows><<<G, kCgBlock, 0, c->stream>>>(R, c->m64, n, B.x, B.lz);
+    adj_spmv_kernel<Rows><<<c->G, kCgBlock, 0, c->stream>>>(R, c->m64, c->n, B.cg.x, B.lz);
     ADJ_HIP(c, hipGetLastError());
     return CMX_OK;
 }
@@ -469,15 +332,21 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
     if ((rc = sc.alloc(c, &B.lam_y, (size_t)(S - 1) * ps))) return rc;
     if ((rc = sc.alloc(c, &B.lz, n))) return rc;
     if ((rc = sc.alloc(c, &B.grad, (size_t)std::max(P, 1) * ps))) return rc;
-    if ((rc = sc.alloc(c, &B.x, n))) return rc;
-    if ((rc = sc.alloc(c, &B.r, n))) return rc;
-    if ((rc = sc.alloc_gathered(c, &B.z, n))) return rc;
-    if ((rc = sc.alloc(c, &B.q, n))) return rc;
-    if ((rc = sc.alloc_gathered(c, &B.p[0], n))) return rc;
-    if ((rc = sc.alloc_gathered(c, &B.p[1], n))) return rc;
-    if ((rc = sc.alloc(c, &B.partials, 3 * (size_t)kMaxBlocks * 3))) return rc;
-    if ((rc = sc.alloc(c, &B.hist, kAdjMaxIts + kAdjFirstMax + 1))) return rc;
-    if ((rc = sc.alloc(c, &B.ctl, 1))) return rc;
+    PlainCg& cg = B.cg;
+    if ((rc = sc.alloc(c, &cg.r, n))) return rc;
+    if ((rc = sc.alloc(c, &cg.w, n))) return rc;
+    if ((rc = sc.alloc(c, &cg.z, n))) return rc;
+    if ((rc = sc.alloc(c, &cg.s, n))) return rc;
+    if ((rc = sc.alloc(c, &cg.x, n))) return rc;
+    if ((rc = sc.alloc_gathered(c, &cg.p, n))) return rc;
+    if ((rc = sc.alloc_gathered(c, &cg.m[0], n))) return rc;
+    if ((rc = sc.alloc_gathered(c, &cg.m[1], n))) return rc;
+    for (int q = 0; q < 3; ++q)
+        if ((rc = sc.alloc(c, &cg.part[q], (size_t)kMaxBlocks * 3))) return rc;
+    if ((rc = sc.alloc(c, &cg.hist, kAdjMaxIts + kAdjFirstMax + 1))) return rc;
+    if ((rc = sc.alloc(c, &cg.ctl, 2))) return rc;
+    cg.rtol = c->rtol > 0.0 ? c->rtol : 1e-10;
+    cg.max_its = kAdjMaxIts;
     B.b = B.lam_v;
     std::vector<int32_t> h_slot(n, -1);
     for (int o = 0; o < n_obs; ++o) h_slot[obs_nodes[o]] = o;
@@ -486,10 +355,10 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
     if (n_obs_steps > 0 && n_obs > 0)
         ADJ_HIP(c, hipMemcpy(obs_d, obs_values, (size_t)n_obs_steps * n_obs * sizeof(double), hipMemcpyHostToDevice));
     ADJ_HIP(c, hipMemset(B.grad, 0, (size_t)std::max(P, 1) * ps * sizeof(double)));
-    AdjCtl* h_ctl = nullptr;
-    ADJ_HIP(c, hipHostMalloc((void**)&h_ctl, sizeof(AdjCtl), hipHostMallocDefault));
+    SolveCtl* h_ctl = nullptr;
+    ADJ_HIP(c, hipHostMalloc((void**)&h_ctl, sizeof(SolveCtl), hipHostMallocDefault));
     struct PinnedFree {
-        AdjCtl* p;
+        SolveCtl* p;
         ~PinnedFree() { (void)hipHostFree(p); }
     } pinned{h_ctl};
     B.h_ctl = h_ctl;
@@ -528,7 +397,6 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
     const int gc = (n + kAdjBlock - 1) / kAdjBlock;
     adjoint_final_kernel<<<gc, kAdjBlock, 0, c->stream>>>(n, ps, S - 1, c->V, obs_slot, d_at_end, B.lam_v, B.lam_y);
     ADJ_HIP(c, hipGetLastError());
-    const double rtol = c->rtol > 0.0 ? c->rtol : 1e-10;
     CellAdjArgs a{};
     a.n = n;
     a.ps = ps;
@@ -564,7 +432,7 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
         c->trace = nullptr;
         if (rc) return rc;
         for (int m = s1 - 1; m >= s0; --m) {
-            if ((rc = adjoint_solve<Rows>(c, R, B, rtol * rtol))) return rc;
+            if ((rc = adjoint_solve<Rows>(c, R, B))) return rc;
             const double* st = seg + (size_t)(m - s0) * slot;
             a.Vn = st;
             a.Yn = st + ps;

@@ -192,4 +192,23 @@ inline bool pad_stimulus(cmx_stimulus& s)
 // same step times (the adjoint replays a run step by step).
 int step_range(cmx_ctx* c, double t0, int n_total, int k0, int k1);
 
+// A x = b from x = 0 with the step's pipelined Jacobi-PCG kernels, on the
+// caller's buffers (FP64 contexts; the adjoint's solves).  Stopping test
+// ||D^-1 r|| <= rtol ||D^-1 b||.  plain_cg_start launches the start, which
+// ends a solve whose b is already below the test; plain_cg_iters launches
+// iterations from .. from+count-1, each returning at once once ctl[0].done
+// is set (1 converged, 2 max_its reached, 3 non-finite).
+struct PlainCg {
+    double *r, *w, *z, *s, *x;
+    double* p;           // gathered at the start: c->pad zeroed guard elements on each side
+    double* m[2];        // gathered, likewise
+    double* part[3];     // [kMaxBlocks * 3] each: iteration ping-pong, start
+    double2* hist;       // [max_its + 1]
+    SolveCtl* ctl;       // [2]
+    double rtol;
+    int max_its;
+};
+int plain_cg_start(cmx_ctx* c, const PlainCg& s, const double* b);
+int plain_cg_iters(cmx_ctx* c, const PlainCg& s, int from, int count);
+
 }  // namespace cmx

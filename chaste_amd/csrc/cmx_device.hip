@@ -976,6 +976,41 @@ __global__ __launch_bounds__(kCgBlock) void cg_iter_kernel(Rows R, MatVals mv, C
     write_partials<kCgBlock, 3>(part, a.pout, blk);
 }
 
+// Start of a solve A x = b from x = 0 outside a step (the adjoint's solves):
+// stands in for the residual kernel, with r0 = b, u0 = D^-1 b to p, x = 0 and
+// the reductions cg_start_kernel reads (||D^-1 b||^2, 0, ||u0||^2); it also
+// resets the solve's two control records.
+template <class Rows>
+__global__ __launch_bounds__(kCgBlock) void plain_init_kernel(Rows R, MatVals mv, int n,
+                                                              const double* __restrict__ b, CgVecs<double> v,
+                                                              double* __restrict__ pout, SolveCtl* ctl, int max_its)
+{
+    const double* __restrict__ D = mv.dinv64;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    if (blk == 0 && threadIdx.x < 2) {
+        SolveCtl& t = ctl[threadIdx.x];
+        t.tol2 = 0.0;
+        t.bb = 0.0;
+        t.zz = 0.0;
+        t.it = 0;
+        t.done = 0;
+        t.max_its = max_its;
+        t.pad = 0;
+    }
+    const RowRange rr(n, blk, gridDim.x);
+    double part[3] = {0.0, 0.0, 0.0};
+    for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
+        const double bi = b[i];
+        const double ui = D[R.didx(R.row(i))] * bi;
+        v.r[i] = bi;
+        v.p[i] = ui;
+        v.x[i] = 0.0;
+        part[0] += ui * ui;
+        part[2] += ui * ui;
+    }
+    write_partials<kCgBlock, 3>(part, pout, blk);
+}
+
 // Settles the last step of a batch (or a resumed step): the verdict on its
 // final solve, V^{n+1} = Z + x1 (+ x2), the next initial guess and the
 // activation times; advance_kernel then counts the step.  A solve still
@@ -1648,6 +1683,58 @@ int steps_typed(cmx_ctx* c)
     return run_steps<EllRows, RS, RV>(c, R);
 }
 
+template <class Rows>
+CgArgs<double> plain_args(cmx_ctx* c, const PlainCg& s, int it)
+{
+    CgArgs<double> a;
+    a.n = c->n;
+    a.first = 0;
+    a.rtol2 = s.rtol * s.rtol;
+    a.atol2 = 0.0;
+    a.clock = c->clock;
+    a.ctl = s.ctl;
+    a.ctl_all = s.ctl;
+    a.hist = s.hist;
+    a.pin = s.part[it & 1];
+    a.pout = s.part[(it + 1) & 1];
+    a.v.r = s.r;
+    a.v.w = s.w;
+    a.v.z = s.z;
+    a.v.s = s.s;
+    a.v.p = s.p;
+    a.v.m_in = s.m[it & 1];
+    a.v.m_out = s.m[(it + 1) & 1];
+    a.v.x = s.x;
+    return a;
+}
+
+template <class Rows>
+int plain_start_rows(cmx_ctx* c, const Rows& R, const PlainCg& s, const double* b)
+{
+    CgArgs<double> a = plain_args<Rows>(c, s, 0);
+    plain_init_kernel<Rows><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), c->n, b, a.v, s.part[2], s.ctl,
+                                                              s.max_its);
+    a.first = 1;
+    a.pin = s.part[2];
+    a.pout = s.part[0];
+    a.v.m_out = s.m[0];   // read by iteration 0
+    cg_start_kernel<Rows, double><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), a);
+    CMX_HIP(c, hipGetLastError());
+    return CMX_OK;
+}
+
+template <class Rows>
+int plain_iters_rows(cmx_ctx* c, const Rows& R, const PlainCg& s, int from, int count)
+{
+    const MatVals mv = mat_vals(c);
+    for (int it = from; it < from + count; ++it) {
+        const CgArgs<double> a = plain_args<Rows>(c, s, it);
+        cg_iter_kernel<Rows, double><<<c->G, kCgBlock, 0, c->stream>>>(R, mv, a, it);
+    }
+    CMX_HIP(c, hipGetLastError());
+    return CMX_OK;
+}
+
 // Host [planes][n] doubles <-> device [planes][stride] RS (stride >= n; the
 // gap elements stay as they are on the device).
 template <class RS>
@@ -2030,6 +2117,28 @@ int cmx_step(cmx_ctx* c, double t0, int32_t n_steps)
 // Steps k0 .. k1-1 of the run cmx_step(c, t0, n_total) takes, with its step
 // times: TimeStepper(t0, t0 + n_total dt, dt), t_k = t0 + k dt, the end
 // snapped (global/src/TimeStepper.cpp:106-121), evaluated on the device.
+int cmx::plain_cg_start(cmx_ctx* c, const PlainCg& s, const double* b)
+{
+    if (c->precision != CMX_FP64) return fail(c, CMX_E_ARG, "plain solves need an FP64 context");
+    if (c->format == CMX_FMT_STENCIL) {
+        if (c->width == 4) return plain_start_rows(c, stencil_rows<4>(c), s, b);
+        if (c->width == 8) return plain_start_rows(c, stencil_rows<8>(c), s, b);
+        return plain_start_rows(c, stencil_rows<16>(c), s, b);
+    }
+    return plain_start_rows(c, EllRows{c->ell_col, c->width, c->n}, s, b);
+}
+
+int cmx::plain_cg_iters(cmx_ctx* c, const PlainCg& s, int from, int count)
+{
+    if (c->precision != CMX_FP64) return fail(c, CMX_E_ARG, "plain solves need an FP64 context");
+    if (c->format == CMX_FMT_STENCIL) {
+        if (c->width == 4) return plain_iters_rows(c, stencil_rows<4>(c), s, from, count);
+        if (c->width == 8) return plain_iters_rows(c, stencil_rows<8>(c), s, from, count);
+        return plain_iters_rows(c, stencil_rows<16>(c), s, from, count);
+    }
+    return plain_iters_rows(c, EllRows{c->ell_col, c->width, c->n}, s, from, count);
+}
+
 int cmx::step_range(cmx_ctx* c, double t0, int n_total, int k0, int k1)
 {
     StepClock ck = run_clock(t0, c->dt, n_total);
