@@ -14,7 +14,7 @@
 //   lambda_V^n = J_V^T lambda_y^{n+1} + l_z (c - Am dI/dV^n) + 2 (V^n - d^n) at observed nodes
 //   dL/dp   += J_p^T lambda_y^{n+1} - Am l_z dI/dp
 // The Jacobian columns of (Phi, Iion) come from forward-mode differentiation
-// of the generated cell model (one seeded input per pass, cmx_dual); the
+// of the generated cell model (four seeded inputs per pass, cmx_dualn); the
 // states of each step are recomputed from checkpoints.
 #include <hip/hip_runtime.h>
 
@@ -39,7 +39,7 @@ constexpr int kAdjMaxIts = 2000;
 // ---------------------------------------------------------------------------
 // Cell step of one node with V clamped, as cell_step_kernel's ICI branch
 // computes it for forward Euler and Rush-Larsen cells; returns Iion at the
-// new state.  R = double or cmx_dual.
+// new state.  R = double or cmx_dualn<K>.
 // ---------------------------------------------------------------------------
 template <class Model, class R, int IG>
 __device__ __forceinline__ R ici_cell_map(R* y, R V, const R* p, double cm, double am, const cmx_stimulus* stim,
@@ -87,14 +87,15 @@ struct CellAdjArgs {
 };
 
 // One thread per node: the Jacobian columns of (Phi, Iion) with respect to
-// each state, V and each parameter, one forward-mode pass per column,
+// each state, V and each parameter, K columns per forward-mode pass,
 // contracted with (lambda_y^{n+1}, -Am l_z).
-template <class Model, int IG>
+template <class Model, int IG, int K>
 __global__ __launch_bounds__(kAdjBlock) void cell_adjoint_kernel(CellAdjArgs a)
 {
     constexpr int NS = Model::NS;
     constexpr int NP = Model::NP;
     constexpr int NPd = NP > 0 ? NP : 1;
+    using Dual = cmx_dualn<K>;
     const int i = blockIdx.x * kAdjBlock + threadIdx.x;
     if (i >= a.n) return;
     double y0[NS], ly[NS], p0[NPd];
@@ -116,21 +117,35 @@ __global__ __launch_bounds__(kAdjBlock) void cell_adjoint_kernel(CellAdjArgs a)
     const double w_iion = -a.am * lz;
     double adj[NS + NPd];    // [1, NS): states, 0: V, NS + q: parameter q
 #pragma unroll 1
-    for (int k = 0; k < NS + NP; ++k) {
-        cmx_dual y[NS], p[NPd];
+    for (int k0 = 0; k0 < NS + NP; k0 += K) {
+        // inputs k0 .. k0 + K - 1 (0: V, [1, NS): states, NS + q: parameters)
+        Dual y[NS], p[NPd];
 #pragma unroll
-        for (int s = 1; s < NS; ++s) y[s] = cmx_dual(y0[s], s == k ? 1.0 : 0.0);
+        for (int s = 1; s < NS; ++s) {
+            y[s] = Dual(y0[s]);
 #pragma unroll
-        for (int q = 0; q < NPd; ++q) p[q] = cmx_dual(p0[q], NS + q == k ? 1.0 : 0.0);
-        const cmx_dual V(vn, k == 0 ? 1.0 : 0.0);
-        const cmx_dual I = ici_cell_map<Model, cmx_dual, IG>(y, V, p, a.cm, a.am, a.stim, sid, t, t_next,
-                                                              a.dt_ode, a.n_sub);
-        double s_ = w_iion * I.d;
+            for (int j = 0; j < K; ++j) y[s].d[j] = s == k0 + j ? 1.0 : 0.0;
+        }
 #pragma unroll
-        for (int s = 1; s < NS; ++s) s_ += ly[s] * y[s].d;
+        for (int q = 0; q < NPd; ++q) {
+            p[q] = Dual(p0[q]);
 #pragma unroll
-        for (int kk = 0; kk < NS + NPd; ++kk)
-            if (kk == k) adj[kk] = s_;
+            for (int j = 0; j < K; ++j) p[q].d[j] = NS + q == k0 + j ? 1.0 : 0.0;
+        }
+        Dual V(vn);
+#pragma unroll
+        for (int j = 0; j < K; ++j) V.d[j] = k0 + j == 0 ? 1.0 : 0.0;
+        const Dual I = ici_cell_map<Model, Dual, IG>(y, V, p, a.cm, a.am, a.stim, sid, t, t_next, a.dt_ode,
+                                                     a.n_sub);
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            double s_ = w_iion * I.d[j];
+#pragma unroll
+            for (int s = 1; s < NS; ++s) s_ += ly[s] * y[s].d[j];
+#pragma unroll
+            for (int kk = 0; kk < NS + NPd; ++kk)
+                if (kk == k0 + j) adj[kk] = s_;
+        }
     }
 #pragma unroll
     for (int s = 1; s < NS; ++s) a.lam_y[(size_t)(s - 1) * a.ps + i] = adj[s];
@@ -286,7 +301,8 @@ int adjoint_solve(cmx_ctx* c, const Rows& R, AdjBuffers& B)
 template <class Model, int IG>
 void launch_cell_adjoint(cmx_ctx* c, const CellAdjArgs& a)
 {
-    cell_adjoint_kernel<Model, IG><<<(c->n + kAdjBlock - 1) / kAdjBlock, kAdjBlock, 0, c->stream>>>(a);
+    // four Jacobian columns per pass: LR91's 11 in 3 passes, TT06's 25 in 7
+    cell_adjoint_kernel<Model, IG, 4><<<(c->n + kAdjBlock - 1) / kAdjBlock, kAdjBlock, 0, c->stream>>>(a);
 }
 
 // Slot layout of a stored step: V at [0, ps), the S-1 state planes after it.
