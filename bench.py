@@ -34,6 +34,8 @@ STIM_PROPAGATING = -100000.0      # a magnitude above threshold for the second c
 STIM_XLIM = 0.03
 DT = 0.01
 EXTRAPOLATE = 2      # CG initial guess V^n + 2 x_{n-1} - x_{n-2} (corrections x), in both paths
+ACC_STEPS = 5000     # accuracy runs: the whole 50 ms
+ORACLE_STEPS = 200   # ... with the voltage after this many steps checked against the oracle
 
 
 USE_GRAPH = True
@@ -128,10 +130,13 @@ def time_path(m, steps, warmup, roof_steps, events=True):
     st1 = m.stats()
     its = (st1['cg_iterations'] - st0['cg_iterations']) / steps
     ref = (st1['refine_sweeps'] - st0['refine_sweeps']) / steps
+    graphs = {'mode': ['plain launches', 'graph replay', 'plain launches (profiler attached)'][st1['graph_mode']],
+              'captures_in_timed_region': st1['graph_captures'] - st0['graph_captures'],
+              'replays_in_timed_region': st1['graph_launches'] - st0['graph_launches']}
     if not events:
         return {'elapsed': elapsed, 'its': its, 'refine': ref, 'ranking': {}, 'rank_its': its,
                 'dominant': 'cell', 'dominant_ms': float('nan'), 'dominant_launches': 1, 'roof_its': its,
-                'autotune': tuned}
+                'autotune': tuned, 'graphs': graphs}
     m.set_profiling(KINDS)
     m.kernel_times(reset=True)
     m.step(t, 5)
@@ -149,17 +154,38 @@ def time_path(m, steps, warmup, roof_steps, events=True):
     roof_its = (st2['cg_iterations'] - st_r['cg_iterations']) / roof_steps
     return {'elapsed': elapsed, 'its': its, 'refine': ref, 'ranking': ranking, 'rank_its': rank_its,
             'dominant': dominant, 'dominant_ms': ktimes[dominant][0], 'dominant_launches': ktimes[dominant][1],
-            'roof_its': roof_its, 'autotune': tuned}
+            'roof_its': roof_its, 'autotune': tuned, 'graphs': graphs}
 
 
-def working_launches_per_step(kind, its, refined):
+def working_launches_per_step(kind, its, refined, launches_per_step):
+    """Launches per step that do their work: every iteration launch past a
+    solve's convergence exits at once; the finish kernel runs once per
+    cmx_step batch, not once per step."""
     if kind == 'update':
         return its
     if kind == 'spmv':
         return 1     # the first solve's start; a refinement solve rarely iterates
     if kind == 'residual':
         return 2 if refined else 1
+    if kind == 'finish':
+        return launches_per_step
     return 1
+
+
+def survey_bytes(n, nnz, n_states, precision, its, refine):
+    """The survey's algorithmic bytes per PDE step (Jacobi-PCG textbook
+    minimum; s = 8 in FP64, 4 in the mixed path): cells n (8 V + 8 Iion +
+    2 S s + 4 stimulus), right-hand side nnz (s + 4) + 4 (n + 1) + 24 n, one
+    CG iteration nnz (s + 4) + 4 (n + 1) + 18 n s, one FP64 refinement sweep
+    12 nnz + 4 (n + 1) + 48 n."""
+    s = 4 if precision == 1 else 8
+    S = n_states - 1
+    cell = n * (16 + 2 * S * s + 4)
+    rhs = nnz * (s + 4) + 4 * (n + 1) + 24 * n
+    it = nnz * (s + 4) + 4 * (n + 1) + 2 * n * s + 13 * n * s + 3 * n * s
+    ref = 12 * nnz + 4 * (n + 1) + 48 * n
+    return {'cell': cell, 'rhs': rhs, 'cg_iteration': it, 'refinement': ref,
+            'step': cell + rhs + its * it + refine * ref}
 
 
 def cpu_baseline(n_steps, threads):
@@ -171,6 +197,66 @@ def cpu_baseline(n_steps, threads):
     s = r['stats']
     loop = s['t_ode'] + s['t_rhs'] + s['t_ksp']
     return s['steps'] / loop, s
+
+
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def oracle_slab_run(stim_mag, n_steps, threads):
+    """The oracle's FP64 slab run (V^n warm start, Jacobi-PCG rtol 1e-8) to
+    step n_steps: the accuracy leg's reference for the first steps."""
+    from oracle import pyoracle
+    return pyoracle.mono_run(3, EX, EY, EZ, H, model=pyoracle.TT06_EPI, rush_larsen=True, sigma=SIGMA,
+                             stim_mag=stim_mag, stim_dur=STIM[2], stim_start=STIM[1], stim_xlim=STIM_XLIM,
+                             t_end=n_steps * DT, ksp_rtol=1e-8, threads=threads)['V']
+
+
+def cfg1_case(cmx, steps=1000):
+    """The reference's CPU-runnable case: LR91 FE on the 101 x 101 staggered
+    square (h 0.01 cm), sigma 1.75, -25000 uA/cm^3 for 0.5 ms on x < 0.02,
+    10 ms = 1000 steps, CG rtol 1e-8; launch-latency bound at 10,201 nodes.
+    Both paths, 1000 steps after a 20-step warmup on the same context, then
+    the mixed voltage against the FP64 one after the same 1000 steps from
+    t = 0."""
+    coords, elems = cmx.slab_mesh(2, 100, 100, 0, H)
+    stim_index = np.where(coords[:, 0] < 0.02, 0, -1).astype(np.int32)
+    init = np.repeat(np.asarray(cmx.initial_state(cmx.LR91), dtype=np.float64)[:, None], coords.shape[0], axis=1)
+    out, V = {}, {}
+    for name, prec in (('fp64', cmx.FP64), ('mixed', cmx.MIXED)):
+        m = cmx.Monodomain(coords, elems, np.array([1.75, 1.75]), cmx.LR91, cmx.FORWARD_EULER,
+                           stimuli=[(-25000.0, 0.0, 0.5)], stim_index=stim_index, precision=prec, ksp_rtol=1e-8,
+                           ksp_atol=0.0, refine_sweeps=1 if prec == cmx.MIXED else 0, use_graph=USE_GRAPH)
+        m.step(0.0, 20)
+        m.step(0.2, steps)     # graphs of the budgets this run uses
+        m.set_state(init)
+        m.step(0.0, 20)
+        sync()
+        st0 = m.stats()
+        t = time.perf_counter()
+        m.step(0.2, steps)
+        sync()
+        el = time.perf_counter() - t
+        st1 = m.stats()
+        m.set_state(init)
+        m.step(0.0, steps)
+        V[name] = m.voltage()
+        out[name] = {'steps_per_s': round(steps / el, 1), 'ms_per_step': round(1e3 * el / steps, 4),
+                     'cg_its_per_step': round((st1['cg_iterations'] - st0['cg_iterations']) / steps, 3),
+                     'graph_captures_in_timed_region': st1['graph_captures'] - st0['graph_captures']}
+        m.close()
+    dv = np.abs(V['mixed'] - V['fp64'])
+    return {'workload': 'LR91 FE monodomain square 101x101 nodes h=0.01, stim -25000 uA/cm^3 0.5 ms x<0.02, '
+                        'dt 0.01 ms, Jacobi-CG rtol 1e-8, %d timed steps after 20' % steps,
+            'n_nodes': int(coords.shape[0]), 'paths': out,
+            'mixed_over_fp64': round(out['mixed']['steps_per_s'] / out['fp64']['steps_per_s'], 3),
+            'max_dV_mixed_vs_fp64_mV': float(dv.max())}
 
 
 def adjoint_case(args):
@@ -365,6 +451,7 @@ def shells_case(args):
         el = time.perf_counter() - t
         runs[name] = (m.voltage(), m.activation()[4])
         steps = first - warm
+        common = {'steps_per_s': round(steps / el, 2), 'window_ms': [warm * DT, first * DT]}
         k = first
         while prec == cmx.MIXED and k < total:
             chunk = min(2000, total - k)
@@ -395,7 +482,8 @@ def shells_case(args):
             kernels['update']['working_launch_GBps'] = round(upd_bytes * its / (kt['update'][0] * 1e-3) / 1e9, 1)
             kernels['update']['bytes_per_row'] = 14 * s_b + w * (4 + s_b)
         out[name] = {'steps_per_s': round(steps / el, 2), 'node_steps_per_s': round(steps * n / el, 1),
-                     'timed_steps': steps, 'setup_s': round(setup, 2),
+                     'timed_steps': steps, 'timed_window_ms': [warm * DT, k * DT], 'setup_s': round(setup, 2),
+                     'common_window': common,
                      'cg_its_per_step': round((st1['cg_iterations'] - st0['cg_iterations']) / steps, 3),
                      'format': 'stencil' if st1['format'] == cmx.FMT_STENCIL else 'ell',
                      'row_width': st1['row_width'], 'kernels_20_steps': kernels}
@@ -415,7 +503,9 @@ def shells_case(args):
                                'apical stimulus -100000 uA/cm^3 1 ms, Jacobi-CG rtol 1e-8; mixed to %g ms'
                                % (h_max, h_min, total * DT), 'n_nodes': n, 'n_elems': n_elems, 'shell_nodes': sizes},
         'paths': out,
-        'mixed_over_fp64': round(out['mixed']['steps_per_s'] / out['fp64']['steps_per_s'], 3),
+        'mixed_over_fp64': round(out['mixed']['common_window']['steps_per_s'] /
+                                 out['fp64']['common_window']['steps_per_s'], 3),
+        'mixed_over_fp64_window_ms': [warm * DT, first * DT],
         'accuracy_mixed_vs_fp64': {
             'after_steps': first, 'max_dV_mV': float(dv.max()), 'rms_dV_mV': float(np.sqrt((dv ** 2).mean())),
             'activated_nodes': int(both.sum()),
@@ -514,6 +604,9 @@ def main():
     ap.add_argument('--cpu-steps', type=int, default=20)
     ap.add_argument('--roof-steps', type=int, default=50)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-oracle-check', action='store_true',
+                    help='skip the oracle comparison of the first %d steps of each accuracy run' % ORACLE_STEPS)
+    ap.add_argument('--skip-cfg1', action='store_true', help='skip the LR91 101x101 square (launch-bound case)')
     ap.add_argument('--skip-propagating', action='store_true', help='only the named-stimulus case')
     ap.add_argument('--no-graph', action='store_true', help='plain launches instead of hipGraph replay')
     ap.add_argument('--no-autotune', action='store_true', help='the default solver block count (256)')
@@ -555,9 +648,12 @@ def main():
     y0 = cmx.initial_state(cmx.TT06_EPI)
     init = np.repeat(np.asarray(y0, dtype=np.float64)[:, None], n, axis=1)
 
-    def run_case(stim_mag, with_roofline):
+    def run_case(stim_mag, with_roofline, oracle_v):
         """Both paths: timing (and ranking / roofline passes), then the
-        accuracy run on the same context reset to the initial state."""
+        accuracy run on the same context reset to the initial state: the
+        whole 50 ms (ACC_STEPS steps from t = 0, whatever --steps is) with
+        activation tracking, the voltage after ORACLE_STEPS steps compared
+        with the oracle's FP64 run."""
         results, runs = {}, {}
         for name, prec in (('fp64', cmx.FP64), ('mixed', cmx.MIXED)):
             progress('%s stim %g: setup' % (name, stim_mag))
@@ -568,52 +664,83 @@ def main():
             st = m.stats()
             kinds = {}
             for k, (ms, cnt) in r['ranking'].items():
-                work = working_launches_per_step(k, r['rank_its'], refined) * 5
+                work = working_launches_per_step(k, r['rank_its'], refined, cnt / 5) * 5
                 b = bytes_per_row(k, prec, n_states, refined) * n
                 kinds[k] = {'ms_per_step': ms / 5, 'launches_per_step': cnt / 5,
                             'working_launches_per_step': work / 5,
                             'achieved_GBs': (work * b / (ms * 1e-3) / 1e9) if ms > 0 else None}
             dom = r['dominant']
-            dom_work = working_launches_per_step(dom, r['roof_its'], refined) * args.roof_steps
+            dom_launches = max(r['dominant_launches'], 1)
+            dom_work = working_launches_per_step(dom, r['roof_its'], refined, dom_launches / args.roof_steps) \
+                * args.roof_steps
             dom_bytes = dom_work * bytes_per_row(dom, prec, n_states, refined) * n
+            sb = survey_bytes(n, st['nnz'], n_states, prec, r['its'], r['refine'])
             results[name] = {
                 'steps_per_s': args.steps / r['elapsed'],
                 'ms_per_step': 1e3 * r['elapsed'] / args.steps,
                 'cg_its_per_step': r['its'],
                 'refine_sweeps_per_step': r['refine'],
+                'graphs': r['graphs'],
+                'survey_bytes_per_step': sb['step'],
+                'survey_step_frac_of_hbm_peak': round(sb['step'] / (r['elapsed'] / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
                 'kernels_profiled': kinds,
                 'dominant_kernel': dom,
-                'dominant_avg_launch_ms': r['dominant_ms'] / max(r['dominant_launches'], 1),
-                'dominant_bytes_per_launch': dom_bytes / max(r['dominant_launches'], 1),
+                'dominant_avg_launch_ms': r['dominant_ms'] / dom_launches,
+                'dominant_bytes_per_launch': dom_bytes / dom_launches,
+                'dominant_survey_bytes_per_launch': sb['cell'] if dom == 'cell' else None,
                 'format': 'stencil' if st['format'] == cmx.FMT_STENCIL else 'ell',
                 'n_patterns': st['n_patterns'],
                 'autotune': r['autotune'],
             }
             # accuracy: the same simulation from t = 0 with activation tracking (0 mV)
-            progress('%s stim %g: accuracy run' % (name, stim_mag))
+            progress('%s stim %g: accuracy run (%d steps)' % (name, stim_mag, ACC_STEPS))
             m.set_state(init)
             m.set_activation(0.0, 1, 0.0)
-            m.step(0.0, args.warmup + args.steps)
-            runs[name] = (m.voltage(), m.activation()[4])
+            m.step(0.0, ORACLE_STEPS)
+            v_early = m.voltage()
+            m.step(ORACLE_STEPS * DT, ACC_STEPS - ORACLE_STEPS)
+            runs[name] = (m.voltage(), m.activation()[4], v_early)
             m.close()
         dv = np.abs(runs['mixed'][0] - runs['fp64'][0])
         a64, a32 = runs['fp64'][1], runs['mixed'][1]
         both = (a64 >= 0) & (a32 >= 0)
         acc = {'max_dV_mV': float(dv.max()), 'rms_dV_mV': float(np.sqrt((dv ** 2).mean())),
                'activation_max_err_ms': float(np.abs(a32[both] - a64[both]).max()) if both.any() else None,
+               'activation_rms_err_ms': float(np.sqrt(((a32[both] - a64[both]) ** 2).mean())) if both.any() else None,
                'activated_nodes': int(both.sum()),
                'activation_mismatched_nodes': int(((a64 >= 0) != (a32 >= 0)).sum()),
-               'after_steps': args.warmup + args.steps,
-               'run': 'the same run from t = 0 with activation tracking (first upward crossing of 0 mV)'}
+               'after_steps': ACC_STEPS,
+               'run': 'the whole 50 ms from t = 0 on each path with activation tracking (first upward crossing '
+                      'of 0 mV, interpolated between steps); max/RMS dV over all nodes at 50 ms'}
+        if oracle_v is not None:
+            for name in ('fp64', 'mixed'):
+                d = np.abs(runs[name][2] - oracle_v)
+                acc['vs_oracle_%s' % name] = {'after_steps': ORACLE_STEPS, 'max_dV_mV': float(d.max()),
+                                              'rms_dV_mV': float(np.sqrt((d ** 2).mean())),
+                                              'max_rel': float(d.max() / np.abs(oracle_v).max())}
         return results, acc
 
-    results, accuracy = run_case(STIM[0], True)
-    prop_results, prop_accuracy = (None, None) if args.skip_propagating else run_case(STIM_PROPAGATING, False)
+    cpu_threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    cpu_threads = max(1, min(cpu_threads, os.cpu_count() or 1))
+    oracle_named = oracle_prop = None
+    if not args.no_oracle_check:
+        progress('oracle: %d FP64 steps of each stimulus on %d threads' % (ORACLE_STEPS, cpu_threads))
+        oracle_named = oracle_slab_run(STIM[0], ORACLE_STEPS, cpu_threads)
+        if not args.skip_propagating:
+            oracle_prop = oracle_slab_run(STIM_PROPAGATING, ORACLE_STEPS, cpu_threads)
+    results, accuracy = run_case(STIM[0], True, oracle_named)
+    prop_results, prop_accuracy = (None, None) if args.skip_propagating else \
+        run_case(STIM_PROPAGATING, False, oracle_prop)
+    cfg1 = None if args.skip_cfg1 else cfg1_case(cmx)
     mixed = results['mixed']
     steps_s = mixed['steps_per_s']
     dom = mixed['dominant_kernel']
-    achieved = mixed['dominant_bytes_per_launch'] / (mixed['dominant_avg_launch_ms'] * 1e-3) / 1e9
-    timed_kernels = bool(np.isfinite(achieved))
+    dom_ms = mixed['dominant_avg_launch_ms']
+    timed_kernels = bool(np.isfinite(dom_ms))
+    # roofline.achieved: the survey's algorithmic bytes of one launch of the
+    # dominant kernel (cells: n (16 + 2 S s + 4)) over its average duration
+    dom_survey = mixed['dominant_survey_bytes_per_launch'] or mixed['dominant_bytes_per_launch']
+    achieved = dom_survey / (dom_ms * 1e-3) / 1e9 if timed_kernels else float('nan')
     traffic = None
     tfile = os.path.join(ROOT, 'profiles', 'hbm_traffic_slab442k.json')
     if os.path.exists(tfile):
@@ -638,7 +765,9 @@ def main():
                                'dt 0.01 ms, stim -50000 uA/cm^3 1 ms x<0.03, Jacobi-CG rtol 1e-8'
                                + {0: '', 1: ', initial guess V^n + x_{n-1}',
                                   2: ', initial guess V^n + 2 x_{n-1} - x_{n-2}'}[EXTRAPOLATE]
-                               + ', 50 ms (%d + %d steps)' % (args.warmup, args.steps),
+                               + '; timed: steps %d..%d (t = %g..%g ms) after %d untimed'
+                               % (args.warmup + 1, args.warmup + args.steps, args.warmup * DT,
+                                  (args.warmup + args.steps) * DT, args.warmup),
                    'n_nodes': n, 'n_elems': int(elems.shape[0])},
         'node_ode_updates_per_s': round(steps_s * n, 1),
         'fp64_steps_per_s': round(results['fp64']['steps_per_s'], 2),
@@ -654,23 +783,36 @@ def main():
             'fp64_steps_per_s': round(prop_results['fp64']['steps_per_s'], 2),
             'mixed_over_fp64': round(prop_results['mixed']['steps_per_s'] / prop_results['fp64']['steps_per_s'], 3),
             'cg_its_per_step': {k: round(v['cg_its_per_step'], 3) for k, v in prop_results.items()},
+            'graphs': {k: v['graphs'] for k, v in prop_results.items()},
             'accuracy_mixed_vs_fp64': prop_accuracy},
-        'roofline': None if not timed_kernels else {'bound': 'hbm', 'kernel': dom + ' (mixed)', 'achieved': round(achieved, 1),
-                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
-                     'traffic': traffic,
-                     'bytes_per_launch': round(mixed['dominant_bytes_per_launch']),
-                     'avg_launch_ms': round(mixed['dominant_avg_launch_ms'], 5)},
+        'cfg1_lr91_square': cfg1,
+        'roofline': None if not timed_kernels else {
+            'bound': 'hbm', 'kernel': dom + ' (mixed)', 'achieved': round(achieved, 1),
+            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
+            'traffic': traffic,
+            'traffic_source': 'profiles/hbm_traffic_slab442k.json (FETCH_SIZE/WRITE_SIZE counters, copy-calibrated)',
+            'bytes_per_launch': round(dom_survey),
+            'bytes_basis': 'survey algorithmic bytes: n (8 V + 8 Iion + 2 S s + 4 stimulus)',
+            'bytes_per_launch_fused': round(mixed['dominant_bytes_per_launch']),
+            'fused_basis': 'the kernel\'s own traffic: also settles the previous step (Z, x1, x2 in, V out) and '
+                           'forms the extrapolated guess (x0, x_{n-2}, Y, G)',
+            'avg_launch_ms': round(dom_ms, 5)},
     }
     if not args.no_cpu_baseline:
-        progress('cpu baseline')
-        threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
-        threads = max(1, min(threads, os.cpu_count() or 1))
-        cpu_sps, cpu_stats = cpu_baseline(args.cpu_steps, threads)
-        line['cpu_baseline'] = {'value': round(cpu_sps, 3), 'unit': 'steps/s', 'cores': threads, 'kind': 'port',
-                                'sample': '%d FP64 steps of the same slab (oracle: Jacobi-PCG, OpenMP), setup '
-                                          'excluded; Ode %.2fs AssRhs %.2fs Ksp %.2fs' %
+        progress('cpu baseline: %d threads, then 1' % cpu_threads)
+        cpu_sps, cpu_stats = cpu_baseline(args.cpu_steps, cpu_threads)
+        one_sps, one_stats = cpu_baseline(args.cpu_steps, 1)
+        line['cpu_baseline'] = {'value': round(cpu_sps, 3), 'unit': 'steps/s', 'cores': cpu_threads, 'kind': 'port',
+                                'sample': '%d FP64 steps of the same slab from t = 0 (oracle: Jacobi-PCG, OpenMP over '
+                                          'nodes and rows), setup excluded; Ode %.2fs AssRhs %.2fs Ksp %.2fs' %
                                           (cpu_stats['steps'], cpu_stats['t_ode'], cpu_stats['t_rhs'],
-                                           cpu_stats['t_ksp'])}
+                                           cpu_stats['t_ksp']),
+                                'cpu_model': cpu_model(), 'host_cpus': os.cpu_count(),
+                                'single_thread': {'value': round(one_sps, 3), 'unit': 'steps/s', 'cores': 1,
+                                                  'Ode_s': round(one_stats['t_ode'], 3),
+                                                  'AssRhs_s': round(one_stats['t_rhs'], 3),
+                                                  'Ksp_s': round(one_stats['t_ksp'], 3)},
+                                'preconditioner': 'Jacobi (the reference default is bjacobi/ILU(0))'}
     print(json.dumps(line))
 
 

@@ -65,7 +65,9 @@ class CmxStats(ctypes.Structure):
                 ('refine_sweeps', ctypes.c_int64), ('last_iterations', ctypes.c_int32),
                 ('format', ctypes.c_int32), ('n_nodes', ctypes.c_int32), ('n_states', ctypes.c_int32),
                 ('row_width', ctypes.c_int32), ('n_patterns', ctypes.c_int32), ('nnz', ctypes.c_int64),
-                ('last_residual', ctypes.c_double), ('device_bytes', ctypes.c_double)]
+                ('last_residual', ctypes.c_double), ('device_bytes', ctypes.c_double),
+                ('graph_launches', ctypes.c_int64), ('graph_captures', ctypes.c_int64),
+                ('graph_mode', ctypes.c_int32), ('solver_blocks', ctypes.c_int32)]
 
 
 # Every symbol include/cmx.h declares, with its ctypes signature.

@@ -144,10 +144,12 @@ struct cmx_ctx {
     double last_residual = 0;
     double device_bytes = 0;
     std::vector<void*> allocs;
-    // hipGraphs of whole steps, keyed by (budget, refinement budget, steps)
-    struct StepGraph { int budget, refine_budget, steps, act_every; double act_thr; hipGraph_t graph; hipGraphExec_t exec; };
+    // hipGraphs of whole steps, keyed by (budget, refinement budget, steps,
+    // solver blocks, activation tracking)
+    struct StepGraph { int budget, refine_budget, steps, G, act_every; double act_thr; hipGraph_t graph; hipGraphExec_t exec; };
     std::vector<StepGraph> graphs;
-    long long graph_launches = 0;
+    long long graph_launches = 0, graph_captures = 0, graph_evictions = 0;
+    int graph_profiler_off = 0;  // a profiler's dispatch interception was detected: plain launches
     // kernel timing (cmx_set_profiling)
     int prof_mask = 0;
     struct Pending { int kind; hipEvent_t a, b; };

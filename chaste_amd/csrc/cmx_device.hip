@@ -1182,6 +1182,32 @@ const double* model_pdef(int model) { return model == CMX_MODEL_LR91 ? lr91::kPa
 
 int grid_of(int n) { return (n + kBlock - 1) / kBlock; }
 
+constexpr int kGraphSteps = 8;        // steps in the largest captured graph
+constexpr size_t kGraphCache = 64;    // instantiated graphs kept per context
+
+// Destroys every captured graph once the stream has drained (a graph still
+// executing must not be destroyed).
+void drop_graphs(cmx_ctx* c)
+{
+    if (c->graphs.empty()) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& g : c->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
+    c->graphs.clear();
+}
+
+// rocprofv3 preloads its tool library and exports ROCPROF_* settings; under
+// its dispatch interception the context runs plain launches (graph replay
+// under the tracer is not used).
+bool profiler_attached()
+{
+    const char* pre = std::getenv("LD_PRELOAD");
+    if (pre && std::strstr(pre, "rocprof")) return true;
+    return std::getenv("ROCPROF_OUTPUT_PATH") || std::getenv("ROCPROFILER_LIBRARY_CTOR");
+}
+
 hipEvent_t take_event(cmx_ctx* c)
 {
     if (!c->event_pool.empty()) {
@@ -1528,14 +1554,27 @@ int sync_state(cmx_ctx* c)
 
 // Instantiated graph of `steps` consecutive steps at the current budgets
 // (captured once; kernel arguments are fixed because time lives in the
-// device step clock).  Returns nullptr when capture is unavailable.
+// device step clock).  Returns nullptr when capture is unavailable.  The
+// cache is bounded; an evicted graph may still be executing on the stream
+// (it can be the one launched just before this capture), so the stream is
+// drained before the executable is destroyed.
 template <class Rows, class RS, class RV>
 hipGraphExec_t step_graph(cmx_ctx* c, const Rows& R, int steps)
 {
     for (auto& g : c->graphs)
-        if (g.budget == c->budget && g.refine_budget == c->refine_budget && g.steps == steps &&
+        if (g.budget == c->budget && g.refine_budget == c->refine_budget && g.steps == steps && g.G == c->G &&
             g.act_every == c->act_every && g.act_thr == c->act_thr)
             return g.exec;
+    if (c->graphs.size() >= kGraphCache) {
+        if (hipStreamSynchronize(c->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        (void)hipGraphExecDestroy(c->graphs.front().exec);
+        (void)hipGraphDestroy(c->graphs.front().graph);
+        c->graphs.erase(c->graphs.begin());
+        ++c->graph_evictions;
+    }
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;
@@ -1549,31 +1588,29 @@ hipGraphExec_t step_graph(cmx_ctx* c, const Rows& R, int steps)
         if (graph) (void)hipGraphDestroy(graph);
         return nullptr;
     }
-    if (c->graphs.size() >= 16) {   // bounded cache: drop the oldest
-        (void)hipGraphExecDestroy(c->graphs.front().exec);
-        (void)hipGraphDestroy(c->graphs.front().graph);
-        c->graphs.erase(c->graphs.begin());
-    }
     hipGraphExec_t exec = nullptr;
     if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipGraphDestroy(graph);
         return nullptr;
     }
-    c->graphs.push_back({c->budget, c->refine_budget, steps, c->act_every, c->act_thr, graph, exec});
+    c->graphs.push_back({c->budget, c->refine_budget, steps, c->G, c->act_every, c->act_thr, graph, exec});
+    ++c->graph_captures;
     return exec;
 }
 
-// Enqueues `todo` steps: replays of captured multi-step graphs when enabled
-// (and no kernel is being timed), plain launches otherwise.
+bool graphs_on(const cmx_ctx* c) { return c->use_graph && !c->graph_profiler_off && c->prof_mask == 0; }
+
+// Enqueues `todo` steps: replays of captured graphs of 8, 4, 2 and 1 steps
+// when enabled (and no kernel is being timed), plain launches otherwise.
 template <class Rows, class RS, class RV>
 int enqueue_steps(cmx_ctx* c, const Rows& R, int todo)
 {
-    constexpr int kGraphSteps = 8;
-    if (c->use_graph && c->prof_mask == 0) {
+    if (graphs_on(c)) {
         int left = todo;
         while (left > 0) {
-            const int g = left >= kGraphSteps ? kGraphSteps : 1;
+            int g = kGraphSteps;
+            while (g > left) g >>= 1;
             hipGraphExec_t exec = step_graph<Rows, RS, RV>(c, R, g);
             if (!exec) break;
             CMX_HIP(c, hipGraphLaunch(exec, c->stream));
@@ -1586,6 +1623,17 @@ int enqueue_steps(cmx_ctx* c, const Rows& R, int todo)
         const int rc = enqueue_step<Rows, RS, RV>(c, R);
         if (rc) return rc;
     }
+    return CMX_OK;
+}
+
+// Captures the graphs a batch of `todo` steps replays at the current budgets
+// (outside any timed region: cmx_autotune calls it for the next batch).
+template <class Rows, class RS, class RV>
+int prime_graphs(cmx_ctx* c, const Rows& R, int todo)
+{
+    if (!graphs_on(c)) return CMX_OK;
+    for (int g = kGraphSteps; g >= 1; g >>= 1)
+        if (todo >= g && !step_graph<Rows, RS, RV>(c, R, g)) break;
     return CMX_OK;
 }
 
@@ -1681,6 +1729,18 @@ int steps_typed(cmx_ctx* c)
     }
     EllRows R{c->ell_col, c->width, c->n};
     return run_steps<EllRows, RS, RV>(c, R);
+}
+
+template <class RS, class RV>
+int prime_typed(cmx_ctx* c, int todo)
+{
+    if (c->format == CMX_FMT_STENCIL) {
+        if (c->width == 4) return prime_graphs<StencilRows<4>, RS, RV>(c, stencil_rows<4>(c), todo);
+        if (c->width == 8) return prime_graphs<StencilRows<8>, RS, RV>(c, stencil_rows<8>(c), todo);
+        return prime_graphs<StencilRows<16>, RS, RV>(c, stencil_rows<16>(c), todo);
+    }
+    EllRows R{c->ell_col, c->width, c->n};
+    return prime_graphs<EllRows, RS, RV>(c, R, todo);
 }
 
 template <class Rows>
@@ -1863,6 +1923,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->max_its = opts->ksp_max_its;
     c->refine = opts->refine_sweeps;
     c->use_graph = opts->use_graph;
+    c->graph_profiler_off = (opts->use_graph && profiler_attached()) ? 1 : 0;
     c->split = opts->operator_splitting ? 1 : 0;
     c->svi = opts->state_variable_interpolation ? 1 : 0;
     {
@@ -2058,11 +2119,7 @@ void release(cmx_ctx* c)
     collect_times(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     c->event_pool.clear();
-    for (auto& g : c->graphs) {
-        (void)hipGraphExecDestroy(g.exec);
-        (void)hipGraphDestroy(g.graph);
-    }
-    c->graphs.clear();
+    drop_graphs(c);
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     if (c->h_clock) (void)hipHostFree(c->h_clock);
@@ -2207,11 +2264,7 @@ int cmx_autotune(cmx_ctx* c, double t0, int32_t n_steps, int32_t n_candidates, c
         CMX_HIP(c, copy_in(c->F, F));
         c->budget = budget;
         c->refine_budget = refine_budget;
-        for (auto& g : c->graphs) {
-            (void)hipGraphExecDestroy(g.exec);
-            (void)hipGraphDestroy(g.graph);
-        }
-        c->graphs.clear();
+        drop_graphs(c);
         return CMX_OK;
     };
     const int G0 = c->G;
@@ -2238,7 +2291,11 @@ int cmx_autotune(cmx_ctx* c, double t0, int32_t n_steps, int32_t n_candidates, c
         }
     }
     c->G = rc == CMX_OK ? best : G0;
-    const int rc2 = restore();
+    int rc2 = restore();
+    // the graphs of the next batch at the restored budgets, captured here
+    // rather than inside the caller's next (possibly timed) cmx_step
+    if (rc == CMX_OK && rc2 == CMX_OK)
+        rc2 = c->precision == CMX_MIXED ? prime_typed<float, float>(c, c->batch) : prime_typed<double, double>(c, c->batch);
     c->steps = steps;
     c->cg_total = cg_total;
     c->refine_total = refine_total;
@@ -2377,6 +2434,10 @@ int cmx_get_stats(cmx_ctx* c, cmx_stats* s)
     s->nnz = c->nnz;
     s->last_residual = c->last_residual;
     s->device_bytes = c->device_bytes;
+    s->graph_launches = c->graph_launches;
+    s->graph_captures = c->graph_captures;
+    s->graph_mode = !c->use_graph ? 0 : (c->graph_profiler_off ? 2 : 1);
+    s->solver_blocks = c->G;
     return CMX_OK;
 }
 
@@ -2437,11 +2498,7 @@ int cmx_set_params(cmx_ctx* c, int32_t n_params, const double* params)
     }
     c->n_param_planes = n_params;
     // captured steps hold the parameter planes' address and count
-    for (auto& g : c->graphs) {
-        (void)hipGraphExecDestroy(g.exec);
-        (void)hipGraphDestroy(g.graph);
-    }
-    c->graphs.clear();
+    drop_graphs(c);
     return CMX_OK;
 }
 

@@ -469,8 +469,13 @@ public:
         mCm = o.cm;
     }
 
-    // Advances to the simulation duration in printing steps; each printing
-    // interval is one TimeStepper of PDE steps (t_k = t_print + k dt).
+    // Advances to the simulation duration in printing steps, as
+    // AbstractCardiacProblem::Solve does (heart/src/problem/AbstractCardiacProblem.cpp:
+    // 371-584): a TimeStepper(current time, duration, printing dt) whose last
+    // interval ends exactly at the duration (global/src/TimeStepper.cpp:43-121),
+    // each interval one PDE TimeStepper with a constant dt
+    // (pde/src/solver/AbstractDynamicLinearPdeSolver.hpp:367), which must
+    // divide the interval.
     void Solve()
     {
         if (!mpCtx) throw Exception(CMX_E_STATE, "Initialise() must be called before Solve()");
@@ -484,19 +489,30 @@ public:
         }
         const double end = cfg->GetSimulationDuration(), print = cfg->GetPrintingTimeStep(), dt = cfg->GetPdeTimeStep();
         if (end <= mCurrentTime) throw Exception(CMX_E_ARG, "The simulation duration must be positive");
-        const long n_print = (long)std::floor((end - mCurrentTime) / print + 0.5);
         const long per = (long)std::floor(print / dt + 0.5);
         if (std::fabs(per * dt - print) > 1e-10 * print)
             throw Exception(CMX_E_ARG, "Printing time-step should be a multiple of PDE time step");
         const double t_start = mCurrentTime;
+        const double eps_print = end > 1.0 ? DBL_EPSILON * end : DBL_EPSILON;
         cmx_h5* out = OpenResults();
         std::vector<double> V;
         try {
             if (out) WriteResults(out, t_start, V);
-            for (long k = 0; k < n_print; ++k) {
-                const double t = t_start + k * print;   // printing TimeStepper
-                Check(cmx_step(mpCtx, t, (int32_t)per), mpCtx);
-                if (out) WriteResults(out, t_start + (k + 1) * print, V);
+            double t = t_start;
+            for (long k = 0; t < end; ++k) {
+                double t_next = t_start + (k + 1) * print;   // printing TimeStepper
+                if (end - t_next <= eps_print) t_next = end;
+                // the interval's PDE TimeStepper(t, t_next, dt) with a constant step
+                const long n_pde = (long)std::floor((t_next - t) / dt + 0.5);
+                const double eps_pde = t_next > 1.0 ? DBL_EPSILON * t_next : DBL_EPSILON;
+                if (n_pde < 1 || std::fabs(t_next - (t + dt * n_pde)) > eps_pde)
+                    throw Exception(CMX_E_ARG, "TimeStepper estimates non-constant timesteps will need to be used: "
+                                               "check timestep divides (end_time-start_time) (or divides printing "
+                                               "timestep)");
+                Check(cmx_step(mpCtx, t, (int32_t)n_pde), mpCtx);
+                mCurrentTime = t_next;
+                if (out) WriteResults(out, t_next, V);
+                t = t_next;
             }
         } catch (...) {
             if (out) cmx_h5_close(out);

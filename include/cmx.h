@@ -121,6 +121,11 @@ typedef struct {
     int64_t nnz;
     double last_residual;      /* ||D^-1 r|| at the end of the last solve */
     double device_bytes;       /* resident device memory (bytes) */
+    int64_t graph_launches;    /* captured multi-step graphs replayed */
+    int64_t graph_captures;    /* graphs captured and instantiated */
+    int32_t graph_mode;        /* 0 plain launches (use_graph = 0), 1 graph replay, 2 plain launches
+                                  because a profiler (rocprofv3) is attached */
+    int32_t solver_blocks;     /* blocks of the solver kernels (cmx_autotune) */
 } cmx_stats;
 
 typedef struct cmx_ctx cmx_ctx;
@@ -311,7 +316,6 @@ typedef struct {
  * message in cmx_cells_last_error(). */
 int cmx_cells_run(const cmx_cells* cells, const double* y0, double* y_out, double* props, cmx_cells_stats* stats);
 const char* cmx_cells_last_error(void);
-
 /* Builds a regular slab mesh exactly as ConstructRegularSlabMesh does
  * (mesh/src/common/AbstractTetrahedralMesh.cpp:277-614): nodes
  * (k*(ey+1)+j)*(ex+1)+i, 2-D staggered triangles, 3-D six tetrahedra per cube.

@@ -123,6 +123,47 @@ static void RunPrintingTimesCase(const std::string& meshes, const std::string& o
     std::fflush(stdout);
 }
 
+// A duration that is not a multiple of the printing step: the printing
+// TimeStepper's last interval ends exactly at the duration
+// (global/src/TimeStepper.cpp:106-121), so printing 0.1 ms over 0.35 ms
+// writes t = 0, 0.1, 0.2, 0.3, 0.35 and ends where a run printed every PDE
+// step ends (5e-11); a duration the PDE step does not divide throws.
+static void RunShortLastIntervalCase(const std::string& meshes, const std::string& out_dir)
+{
+    std::vector<double> V[2];
+    for (int pass = 0; pass < 2; ++pass) {
+        HeartConfig::Reset();
+        HeartConfig::Instance()->SetIntracellularConductivities({0.0005});
+        HeartConfig::Instance()->SetMeshFileName(meshes + "/1D_0_to_1mm_10_elements");
+        HeartConfig::Instance()->SetPrintingTimeStep(pass == 0 ? 0.1 : 0.01);
+        HeartConfig::Instance()->SetSimulationDuration(0.35);
+        if (pass == 0) {
+            HeartConfig::Instance()->SetOutputDirectory(out_dir);
+            HeartConfig::Instance()->SetOutputFilenamePrefix("mono_shortLastInterval");
+        }
+        PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, 1> cell_factory;
+        MonodomainProblem<1> monodomain_problem(&cell_factory);
+        monodomain_problem.Initialise();
+        monodomain_problem.Solve();
+        V[pass] = monodomain_problem.GetSolution();
+    }
+    double diff = 0;
+    for (size_t i = 0; i < V[0].size(); ++i) diff = std::max(diff, std::fabs(V[0][i] - V[1][i]));
+    HeartConfig::Reset();
+    HeartConfig::Instance()->SetMeshFileName(meshes + "/1D_0_to_1mm_10_elements");
+    HeartConfig::Instance()->SetSimulationDuration(0.355);
+    PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, 1> cell_factory;
+    MonodomainProblem<1> monodomain_problem(&cell_factory);
+    monodomain_problem.Initialise();
+    int thrown = 0;
+    try { monodomain_problem.Solve(); } catch (const Exception& e) { ++thrown; }
+    const bool ok = diff <= 5e-11 && thrown == 1;
+    if (!ok) ++failures;
+    std::printf("{\"case\": \"short_last_interval\", \"vs_every_step\": %.3e, \"thrown\": %d, \"pass\": %s}\n",
+                diff, thrown, ok ? "true" : "false");
+    std::fflush(stdout);
+}
+
 // Per-node cell parameters through SetCellParameter (the reference sets them
 // on each cell, AbstractParameterisedSystem::SetParameter): the 1-D case of
 // RunCase with g_Na = 0 everywhere cannot fire, so node 7 stays below -40 mV
@@ -152,8 +193,9 @@ static void RunParameterCase()
 
 // argv[1]: folder holding the mesh files; argv[2]: output root.  Results
 // files go to <root>/MonodomainCreatesGeometry/monodomain3d.h5,
-// <root>/MonoProblem1d/MonodomainLR91_1d.h5 and
-// <root>/MonoProblem1dRequestedTimes/mono_testPrintTimes.h5.
+// <root>/MonoProblem1d/MonodomainLR91_1d.h5,
+// <root>/MonoProblem1dRequestedTimes/mono_testPrintTimes.h5 and
+// <root>/MonoProblem1dShortLastInterval/mono_shortLastInterval.h5.
 int main(int argc, char** argv)
 {
     const std::string out_root = argc > 2 ? argv[2] : "";
@@ -190,6 +232,7 @@ int main(int argc, char** argv)
     if (argc > 1) RunMeshFileCase(argv[1], out("MonoProblem1d"));
     if (argc > 1) RunTwoHalvesCase(argv[1]);
     if (argc > 2) RunPrintingTimesCase(argv[1], out("MonoProblem1dRequestedTimes"));
+    if (argc > 2) RunShortLastIntervalCase(argv[1], out("MonoProblem1dShortLastInterval"));
     RunParameterCase();
     if (!errors_ok) ++failures;
     std::printf("{\"case\": \"errors\", \"thrown\": %d, \"pass\": %s}\n", thrown, errors_ok ? "true" : "false");

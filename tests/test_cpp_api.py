@@ -34,7 +34,7 @@ def test_cpp_monodomain_problem_goldens():
     print(r.stdout)
     cases = [json.loads(l) for l in r.stdout.splitlines() if l.startswith('{')]
     assert {c['case'] for c in cases} == {'1d', '2d', '3d', 'errors', 'mesh_file_1d', 'printing_times',
-                                          'cell_parameter', 'two_halves'}
+                                          'cell_parameter', 'two_halves', 'short_last_interval'}
     # the mesh-file case wrote its results file: V at t = 0 and every 0.01 ms to 2 ms
     f = h5lite.File(os.path.join(out, 'MonoProblem1d', 'MonodomainLR91_1d.h5'))
     data, times = f.read('Data'), f.read('Time')
@@ -45,6 +45,10 @@ def test_cpp_monodomain_problem_goldens():
     times = f.read('Time')
     assert times.shape == (4,) and np.abs(times - [0.0, 0.1, 0.2, 0.3]).max() <= 1e-12
     assert f.read('Data').shape == (4, 11, 1)
+    # a 0.35 ms run printed every 0.1 ms: the last interval is cut at 0.35
+    f = h5lite.File(os.path.join(out, 'MonoProblem1dShortLastInterval', 'mono_shortLastInterval.h5'))
+    times = f.read('Time')
+    assert times.shape == (5,) and np.abs(times - [0.0, 0.1, 0.2, 0.3, 0.35]).max() <= 1e-12
     # TestMonodomainProblem.hpp:851-975: the 3-D slab's results file against the
     # reference's cmgui data file (step 43) and VTK values (steps 100, 200)
     from conftest import check_slab3d_geometry

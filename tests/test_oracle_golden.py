@@ -93,19 +93,64 @@ def test_tt06_all_states(oracle):
     assert abs(oracle.cell_iionic(2, y) - (-0.1843)) < 1e-3
 
 
-def test_rush_larsen_first_order(oracle):
-    """RL error halves with dt (TestRushLarsen.hpp:179-196 checks ratio 2 +- 0.1)."""
-    y0 = tt06_initial()
+# TestRushLarsen.hpp:108-196 on LR91: ComputeExceptVoltage at dt 0.01 over
+# 1 ms equal to forward Euler's within 1e-6 for every state; the stimulated
+# cell (-25.5 uA/cm^2 for 1.99 ms from 0) within 1e-2 of forward Euler at
+# the same dt; every state's error against forward Euler at dt 1e-6 halves
+# with dt (ratio 2 +- 0.1 between dt 0.01 and 0.005).  TT06 gets the same
+# three checks; its forward-Euler cell needs dt <= 0.002 (at 0.005 it blows
+# up even with V clamped), so its first two comparisons run at 0.002 and
+# 0.001.  Forward Euler is pinned by the ValidData files above, and every
+# Rush-Larsen gate update y_inf + (y - y_inf) exp(dt/tau) is a separate
+# branch of the generated model: a wrong y_inf moves the clamped states
+# away from forward Euler's, and a wrong tau converges at first order to a
+# wrong limit, which the error ratio against the dt 1e-6 forward-Euler
+# solution exposes.
+RL_CASES = {'lr91': dict(model=1, cev_dt=0.01, stim_dt=0.01), 'tt06': dict(model=2, cev_dt=0.002, stim_dt=0.001)}
+RL_STIM = (-25.5, 1.99, 0.0, -1.0)
 
-    def v_at(dt):
+
+def _cell_run(oracle, model, integ, dt, T=1.0, stim=RL_STIM):
+    y = oracle.model_initial_state(model)
+    oracle.cell_compute(model, y, 0.0, T, dt, stim=stim, integrator=integ, steps_per_sample=int(round(T / dt)))
+    return y
+
+
+@pytest.mark.parametrize('name', sorted(RL_CASES))
+def test_rush_larsen_against_forward_euler(oracle, name):
+    c = RL_CASES[name]
+    model = c['model']
+    y0 = oracle.model_initial_state(model)
+    a, b = y0.copy(), y0.copy()
+    ia = oracle.cell_compute_except_voltage(model, a, 0.0, 1.0, c['cev_dt'], integrator=oracle.RL)
+    ib = oracle.cell_compute_except_voltage(model, b, 0.0, 1.0, c['cev_dt'], integrator=oracle.FE)
+    assert np.abs(a - b).max() <= 1e-6
+    assert a[0] == y0[0] and abs(ia - ib) <= 1e-6
+    rl = _cell_run(oracle, model, oracle.RL, c['stim_dt'])
+    fe = _cell_run(oracle, model, oracle.FE, c['stim_dt'])
+    assert np.abs(rl - fe).max() <= 1e-2
+    assert rl[0] > y0[0] + 10.0          # the stimulus has depolarised the cell
+    ref = _cell_run(oracle, model, oracle.FE, 1e-6)
+    e1 = _cell_run(oracle, model, oracle.RL, 0.01) - ref
+    e2 = _cell_run(oracle, model, oracle.RL, 0.005) - ref
+    assert np.all(np.abs(e1 / e2 - 2.0) <= 0.1), e1 / e2
+
+
+def test_rush_larsen_first_order_tt06_stimulated(oracle):
+    """TT06 Rush-Larsen through an upstroke (-38 uA/cm^2 for 1 ms from 5 ms,
+    the ValidData protocol, 20 ms): V's error against forward Euler at
+    dt 1e-5 halves with dt (2 +- 0.2 between dt 0.004 and 0.002)."""
+    y0 = tt06_initial()
+    stim = (-38.0, 1.0, 5.0, -1.0)
+
+    def v_at(integ, dt):
         y = y0.copy()
-        oracle.cell_compute(2, y, 0.0, 20.0, dt, stim=(-38.0, 1.0, 5.0, -1.0), rush_larsen=True,
-                            steps_per_sample=int(round(20.0 / dt)))
+        oracle.cell_compute(2, y, 0.0, 20.0, dt, stim=stim, integrator=integ, steps_per_sample=int(round(20.0 / dt)))
         return y[0]
 
-    fine = v_at(0.0005 / 4)
-    e1 = abs(v_at(0.004) - fine)
-    e2 = abs(v_at(0.002) - fine)
+    fine = v_at(oracle.FE, 1e-5)
+    e1 = abs(v_at(oracle.RL, 0.004) - fine)
+    e2 = abs(v_at(oracle.RL, 0.002) - fine)
     assert 1.8 < e1 / e2 < 2.2
 
 
