@@ -108,6 +108,8 @@ SIGNATURES = {
                                     ctypes.POINTER(ctypes.c_int32), _dp]),
     'cmx_cells_run': (ctypes.c_int, [ctypes.POINTER(CmxCells), _dp, _dp, _dp, ctypes.POINTER(CmxCellsStats)]),
     'cmx_cells_last_error': (ctypes.c_char_p, []),
+    'cmx_cells_run_noalloc': (ctypes.c_int, [ctypes.POINTER(CmxCells), ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p]),
     'cmx_adjoint': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                    _ip, ctypes.c_int32, _dp, _dp, _dp, _dp]),
     'cmx_slab_mesh': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,

@@ -22,7 +22,9 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -345,6 +347,211 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
     return CMX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The allocation-free variant (cmx_cells_run_noalloc): every buffer the
+// kernels work in -- state and parameter planes, step counts, padded
+// stimuli, the CellProperties accumulators -- is carved out of one static
+// device array instead of hipMalloc'd storage.  Its device address is looked
+// up once per device (hipGetSymbolAddress); host code never passes the
+// array's name itself to a kernel.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = CMX_CELLS_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's address on the current device (looked up on the
+// device's first call).
+int scratch_ptr(void** out)
+{
+    int dev = 0;
+    CELLS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return cfail(CMX_E_ARG, "device index beyond the supported range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        CELLS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return cfail(CMX_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CMX_OK;
+}
+
+// Layout of n cells in the scratch array: 256-byte aligned regions of
+// ps = n rounded up to 64 elements each.
+struct ScratchLayout {
+    size_t state, V, params, pdef, steps, stim, phase, count, onset, peak, last_peak, end, bytes;
+};
+
+ScratchLayout scratch_layout(int n, int S, int P, size_t rs)
+{
+    const size_t ps = ((size_t)n + 63) / 64 * 64;
+    ScratchLayout L{};
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        const size_t off = at;
+        at += (bytes + 255) / 256 * 256;
+        return off;
+    };
+    L.state = take((size_t)(S - 1) * ps * sizeof(double));
+    L.V = take(ps * sizeof(double));
+    L.params = take((size_t)P * ps * rs);
+    L.pdef = take((size_t)std::max(P, 1) * rs);
+    L.steps = take(ps * sizeof(int32_t));
+    L.stim = take(ps * sizeof(cmx_stimulus));
+    L.phase = take(ps * sizeof(int32_t));
+    L.count = take(ps * sizeof(int32_t));
+    L.onset = take(ps * sizeof(double));
+    L.peak = take(ps * sizeof(double));
+    L.last_peak = take(ps * sizeof(double));
+    L.end = take(ps * sizeof(double));
+    L.bytes = at;
+    return L;
+}
+
+constexpr int kMaxModelStates = 64;
+struct ModelConsts {
+    double init[kMaxModelStates];
+    double pdef[kMaxModelStates];
+};
+
+// One thread per cell: step count from t_end (0, and the cell marked invalid
+// with -1, when t_end is not a whole number of dt), initial state (y0 or the
+// model's), parameters, padded stimulus, and every CellProperties
+// accumulator reset -- nothing is left from an earlier call.
+template <class RS>
+__global__ __launch_bounds__(kBlock) void noalloc_prepare_kernel(CellsArgs<RS> a, int S, int P, const double* t_end,
+                                                                 const cmx_stimulus* stim_in, const double* params_in,
+                                                                 const double* y0, ModelConsts mc, RS* pdef,
+                                                                 int32_t* steps, cmx_stimulus* stim)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < P) pdef[i] = (RS)mc.pdef[i];
+    if (i >= a.n) return;
+    const int n = a.n;
+    const double r = t_end[i] / a.dt;
+    const double k = rint(r);
+    bool ok = t_end[i] >= 0.0 && fabs(r - k) <= 1e-9 * fmax(1.0, r) && k <= 2147483647.0;
+    if (stim_in) {
+        cmx_stimulus s = stim_in[i];
+        if (s.period > 0) {
+            ok = ok && s.duration <= s.period;
+            s.duration += s.period * DBL_EPSILON;
+        } else {
+            s.duration += (s.duration + s.start) * DBL_EPSILON;
+        }
+        if (!(s.stop > 0)) s.stop = DBL_MAX;
+        stim[i] = s;
+    }
+    steps[i] = ok ? (int32_t)k : 0;
+    a.V[i] = y0 ? y0[i] : mc.init[0];
+    for (int s = 1; s < S; ++s) a.state[(size_t)(s - 1) * a.ps + i] = y0 ? y0[(size_t)s * n + i] : mc.init[s];
+    for (int q = 0; q < a.n_param_planes; ++q) ((RS*)a.params)[(size_t)q * a.ps + i] = (RS)params_in[(size_t)q * n + i];
+    a.phase[i] = ok ? 0 : -1;
+    a.ap_count[i] = 0;
+    a.onset[i] = -1.0;
+    a.peak[i] = -DBL_MAX;
+    a.last_peak[i] = -1.0;
+    a.end[i] = -1.0;
+}
+
+// Final states [S][n] and the summary [4][n] in the caller's buffers; an
+// invalid cell (phase -1 before the run, left untouched by it since it takes
+// no step) gets NaN everywhere.
+__global__ __launch_bounds__(kBlock) void noalloc_finish_kernel(CellsArgs<double> a, int S, double* y_out, double* props)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const int n = a.n;
+    const bool bad = a.phase[i] < 0;
+    const double nan = __builtin_nan("");
+    if (y_out) {
+        y_out[i] = bad ? nan : a.V[i];
+        for (int s = 1; s < S; ++s) y_out[(size_t)s * n + i] = bad ? nan : a.state[(size_t)(s - 1) * a.ps + i];
+    }
+    if (props) {
+        props[i] = bad ? nan : (double)a.ap_count[i];
+        props[(size_t)n + i] = bad ? nan : a.onset[i];
+        props[2 * (size_t)n + i] = bad ? nan : a.last_peak[i];
+        props[3 * (size_t)n + i] = bad ? nan : a.end[i];
+    }
+}
+
+template <class RS>
+int run_cells_noalloc(const cmx_cells& c, int S, int P, const double* y0, double* y_out, double* props, hipStream_t stream)
+{
+    const int n = c.n_cells;
+    const ScratchLayout L = scratch_layout(n, S, P, sizeof(RS));
+    if (L.bytes > kScratchBytes) {
+        int cap = 0;
+        for (int step = 1 << 30; step > 0; step >>= 1)
+            if (scratch_layout(cap + step, S, P, sizeof(RS)).bytes <= kScratchBytes) cap += step;
+        return cfail(CMX_E_ARG, "n_cells = " + std::to_string(n) + " exceeds the static scratch array (" +
+                                    std::to_string(kScratchBytes) + " bytes holds at most " + std::to_string(cap) +
+                                    " cells of this model and precision)");
+    }
+    void* base = nullptr;
+    int rc = scratch_ptr(&base);
+    if (rc) return rc;
+    char* b = (char*)base;
+    CellsArgs<RS> a{};
+    a.n = n;
+    a.ps = (n + 63) / 64 * 64;
+    a.dt = c.dt;
+    a.cm = 1.0;
+    a.thr = c.threshold;
+    a.n_steps = (const int32_t*)(b + L.steps);
+    a.stim = c.stimuli ? (const cmx_stimulus*)(b + L.stim) : nullptr;
+    a.state = (double*)(b + L.state);
+    a.V = (double*)(b + L.V);
+    a.params = c.n_params > 0 ? (const RS*)(b + L.params) : nullptr;
+    a.pdef = (const RS*)(b + L.pdef);
+    a.n_param_planes = c.n_params;
+    a.phase = (int32_t*)(b + L.phase);
+    a.ap_count = (int32_t*)(b + L.count);
+    a.onset = (double*)(b + L.onset);
+    a.peak = (double*)(b + L.peak);
+    a.last_peak = (double*)(b + L.last_peak);
+    a.end = (double*)(b + L.end);
+    ModelConsts mc{};
+    const double* init = c.model == CMX_MODEL_LR91 ? lr91::kInitialState : tt06_epi::kInitialState;
+    const double* pd = c.model == CMX_MODEL_LR91 ? lr91::kParamDefaults : tt06_epi::kParamDefaults;
+    for (int s = 0; s < S; ++s) mc.init[s] = init[s];
+    for (int q = 0; q < P; ++q) mc.pdef[q] = pd[q];
+    const int grid = (n + kBlock - 1) / kBlock;
+    noalloc_prepare_kernel<RS><<<grid, kBlock, 0, stream>>>(a, S, P, c.t_end, c.stimuli, c.params, y0, mc,
+                                                            (RS*)(b + L.pdef), (int32_t*)(b + L.steps),
+                                                            (cmx_stimulus*)(b + L.stim));
+    CELLS_HIP(hipGetLastError());
+    // every cell runs its own step count in one launch (k1 beyond any count)
+    const bool rl = c.integrator == CMX_RUSH_LARSEN;
+    if (c.model == CMX_MODEL_LR91) {
+        if (rl) launch_cells<ModelLr91, RS, CMX_RUSH_LARSEN>(a, 0, INT32_MAX, stream);
+        else launch_cells<ModelLr91, RS, CMX_FORWARD_EULER>(a, 0, INT32_MAX, stream);
+    } else {
+        if (rl) launch_cells<ModelTt06, RS, CMX_RUSH_LARSEN>(a, 0, INT32_MAX, stream);
+        else launch_cells<ModelTt06, RS, CMX_FORWARD_EULER>(a, 0, INT32_MAX, stream);
+    }
+    CELLS_HIP(hipGetLastError());
+    CellsArgs<double> f{};
+    f.n = a.n;
+    f.ps = a.ps;
+    f.state = a.state;
+    f.V = a.V;
+    f.phase = a.phase;
+    f.ap_count = a.ap_count;
+    f.onset = a.onset;
+    f.last_peak = a.last_peak;
+    f.end = a.end;
+    noalloc_finish_kernel<<<grid, kBlock, 0, stream>>>(f, S, y_out, props);
+    CELLS_HIP(hipGetLastError());
+    return CMX_OK;
+}
+
 }  // namespace
 }  // namespace cmx
 
@@ -369,6 +576,27 @@ int cmx_cells_run(const cmx_cells* c, const double* y0, double* y_out, double* p
     if (c->n_cells == 0) return CMX_OK;
     return c->precision == CMX_MIXED ? run_cells<float>(*c, S, P, y0, y_out, props, stats)
                                      : run_cells<double>(*c, S, P, y0, y_out, props, stats);
+}
+
+int cmx_cells_run_noalloc(const cmx_cells* c, const double* y0, double* y_out, double* props, void* stream)
+{
+    if (!c) return cfail(CMX_E_ARG, "null description");
+    int S, P;
+    if (c->model == CMX_MODEL_LR91) { S = lr91::kNumStates; P = lr91::kNumParams; }
+    else if (c->model == CMX_MODEL_TT06_EPI) { S = tt06_epi::kNumStates; P = tt06_epi::kNumParams; }
+    else return cfail(CMX_E_ARG, "unknown cell model");
+    if (S > kMaxModelStates || P > kMaxModelStates) return cfail(CMX_E_ARG, "model too large for the variant");
+    if (c->integrator != CMX_FORWARD_EULER && c->integrator != CMX_RUSH_LARSEN)
+        return cfail(CMX_E_ARG, "batched cells support forward-Euler and Rush-Larsen integration");
+    if (c->precision != CMX_FP64 && c->precision != CMX_MIXED) return cfail(CMX_E_ARG, "unknown precision");
+    if (c->n_cells < 0 || (c->n_cells > 0 && !c->t_end)) return cfail(CMX_E_ARG, "n_cells and t_end are required");
+    if (!(c->dt > 0)) return cfail(CMX_E_ARG, "dt must be positive");
+    if (c->n_params < 0 || c->n_params > P || (c->n_params > 0 && !c->params))
+        return cfail(CMX_E_ARG, "n_params must be in [0, the model's parameter count] with params given");
+    if (c->n_cells == 0) return CMX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    return c->precision == CMX_MIXED ? run_cells_noalloc<float>(*c, S, P, y0, y_out, props, s)
+                                     : run_cells_noalloc<double>(*c, S, P, y0, y_out, props, s);
 }
 
 const char* cmx_cells_last_error(void) { return g_cells_error.c_str(); }

@@ -316,6 +316,29 @@ typedef struct {
  * message in cmx_cells_last_error(). */
 int cmx_cells_run(const cmx_cells* cells, const double* y0, double* y_out, double* props, cmx_cells_stats* stats);
 const char* cmx_cells_last_error(void);
+
+/* No counterpart in the reference.  cmx_cells_run for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): it calls no hipMalloc, hipFree or other
+ * allocating routine and never synchronises.  Every array is a DEVICE
+ * pointer -- cells->t_end, cells->stimuli, cells->params, y0 [n_states][n]
+ * (NULL: the model's initial state), y_out [n_states][n] and props [4][n]
+ * (either may be NULL) -- and the work is enqueued on `stream` (a
+ * hipStream_t; NULL = the null stream); the caller synchronises before
+ * reading y_out / props.  The kernels' working buffers (state and parameter
+ * planes, step counts, padded stimuli, the CellProperties accumulators) live
+ * in one static device array of CMX_CELLS_SCRATCH_BYTES per GPU, held until
+ * the process ends; n_cells beyond what it holds (118,144 TT06 or 195,072
+ * LR91 cells in FP64) is rejected with CMX_E_ARG.  Results are those
+ * of cmx_cells_run on the same inputs (cells are not sorted by run length,
+ * and every cell runs in one launch).  A cell whose t_end is not a whole
+ * number of dt, or whose regular stimulus lasts longer than its period,
+ * comes back as NaN in y_out and props (the host cannot see device inputs
+ * to reject them).  Calls that overlap on one GPU share the array and race:
+ * serialise them (one stream, or host ordering). */
+#define CMX_CELLS_SCRATCH_BYTES (32u << 20)
+int cmx_cells_run_noalloc(const cmx_cells* cells, const double* y0, double* y_out, double* props, void* stream);
+
 /* Builds a regular slab mesh exactly as ConstructRegularSlabMesh does
  * (mesh/src/common/AbstractTetrahedralMesh.cpp:277-614): nodes
  * (k*(ey+1)+j)*(ex+1)+i, 2-D staggered triangles, 3-D six tetrahedra per cube.
