@@ -607,6 +607,8 @@ def main():
     ap.add_argument('--no-oracle-check', action='store_true',
                     help='skip the oracle comparison of the first %d steps of each accuracy run' % ORACLE_STEPS)
     ap.add_argument('--skip-cfg1', action='store_true', help='skip the LR91 101x101 square (launch-bound case)')
+    ap.add_argument('--no-accuracy', action='store_true',
+                    help='skip the 50 ms accuracy runs (for runs under a profiler)')
     ap.add_argument('--skip-propagating', action='store_true', help='only the named-stimulus case')
     ap.add_argument('--no-graph', action='store_true', help='plain launches instead of hipGraph replay')
     ap.add_argument('--no-autotune', action='store_true', help='the default solver block count (256)')
@@ -692,6 +694,9 @@ def main():
                 'n_patterns': st['n_patterns'],
                 'autotune': r['autotune'],
             }
+            if args.no_accuracy:
+                m.close()
+                continue
             # accuracy: the same simulation from t = 0 with activation tracking (0 mV)
             progress('%s stim %g: accuracy run (%d steps)' % (name, stim_mag, ACC_STEPS))
             m.set_state(init)
@@ -701,6 +706,8 @@ def main():
             m.step(ORACLE_STEPS * DT, ACC_STEPS - ORACLE_STEPS)
             runs[name] = (m.voltage(), m.activation()[4], v_early)
             m.close()
+        if args.no_accuracy:
+            return results, None
         dv = np.abs(runs['mixed'][0] - runs['fp64'][0])
         a64, a32 = runs['fp64'][1], runs['mixed'][1]
         both = (a64 >= 0) & (a32 >= 0)
@@ -723,7 +730,7 @@ def main():
     cpu_threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
     cpu_threads = max(1, min(cpu_threads, os.cpu_count() or 1))
     oracle_named = oracle_prop = None
-    if not args.no_oracle_check:
+    if not args.no_oracle_check and not args.no_accuracy:
         progress('oracle: %d FP64 steps of each stimulus on %d threads' % (ORACLE_STEPS, cpu_threads))
         oracle_named = oracle_slab_run(STIM[0], ORACLE_STEPS, cpu_threads)
         if not args.skip_propagating:

@@ -34,7 +34,8 @@ class CmxStimulus(ctypes.Structure):
 class CmxTissue(ctypes.Structure):
     _fields_ = [('model', ctypes.c_int32), ('integrator', ctypes.c_int32),
                 ('n_stimuli', ctypes.c_int32), ('stimuli', ctypes.POINTER(CmxStimulus)),
-                ('stim_index', _ip), ('n_params', ctypes.c_int32), ('params', _dp)]
+                ('stim_index', _ip), ('n_params', ctypes.c_int32), ('params', _dp),
+                ('n_groups', ctypes.c_int32), ('group_of', _ip), ('group_model', _ip), ('group_integrator', _ip)]
 
 
 class CmxOpts(ctypes.Structure):
@@ -315,7 +316,10 @@ class Monodomain:
                  stimuli=(), stim_index=None, params=None, precision=FP64,
                  dt_pde=0.01, dt_ode=0.01, am=1400.0, cm=1.0, ksp_rtol=1e-8, ksp_atol=0.0,
                  ksp_max_its=1000, refine_sweeps=0, fmt=FMT_AUTO, mass_lumping=False,
-                 sigma_elem=None, use_graph=False, extrapolate=False, operator_splitting=False, svi=False):
+                 sigma_elem=None, use_graph=False, extrapolate=False, operator_splitting=False, svi=False,
+                 cell_types=None):
+        """cell_types = (group_of [n], [(model, integrator), ...]) gives node i
+        the cell type group_of[i] (model and integrator are then ignored)."""
         L = lib()
         self._keep = []
         coords = np.ascontiguousarray(coords, dtype=np.float64)
@@ -323,6 +327,8 @@ class Monodomain:
         n, dim = coords.shape
         self.n, self.dim, self.model = n, dim, model
         self.n_states, self.n_params = model_info(model)
+        if cell_types is not None:
+            self.n_states = max(model_info(mo)[0] for mo, _ in cell_types[1])
         m = CmxMesh()
         m.dim, m.n_nodes, m.coords = dim, n, _p(coords)
         m.n_elems, m.elems = elems.shape[0], _p(elems, _ip)
@@ -354,6 +360,12 @@ class Monodomain:
             pa = np.ascontiguousarray(params, dtype=np.float64)
             self._keep.append(pa)
             t.n_params, t.params = pa.shape[0], _p(pa)
+        if cell_types is not None:
+            go = np.ascontiguousarray(cell_types[0], dtype=np.int32)
+            gm = np.array([mo for mo, _ in cell_types[1]], dtype=np.int32)
+            gi = np.array([ig for _, ig in cell_types[1]], dtype=np.int32)
+            self._keep += [go, gm, gi]
+            t.n_groups, t.group_of, t.group_model, t.group_integrator = len(gm), _p(go, _ip), _p(gm, _ip), _p(gi, _ip)
         o = CmxOpts(precision, fmt, dt_pde, dt_ode, am, cm, ksp_rtol, ksp_atol, ksp_max_its,
                     refine_sweeps, 1 if use_graph else 0, 1 if mass_lumping else 0, int(extrapolate), 1 if operator_splitting else 0,
                     1 if svi else 0)

@@ -70,6 +70,18 @@ struct ModelTt06 {
 
 }  // namespace cmx
 
+// Cells of one (model, integrator) pair when a tissue mixes cell types
+// (AbstractCardiacCellFactory::CreateCardiacCellForTissueNode returning
+// different classes per node): cell j of the group sits at node h_node[j]
+// (ascending), its state planes are indexed by j.
+struct CellGroup {
+    int model = 0, integrator = 0, S = 0, P = 0, n = 0, ps = 0;
+    int32_t* node = nullptr;     // device copy of h_node
+    void* state = nullptr;       // [S-1][ps] RS
+    void* pdef = nullptr;        // [P] RS
+    std::vector<int32_t> h_node;
+};
+
 struct cmx_ctx {
     std::string err;
     int n = 0, dim = 0, model = 0, integrator = 0, precision = 0, format = 0;
@@ -86,6 +98,10 @@ struct cmx_ctx {
     void* state = nullptr;       // [S-1][n] RS
     void* params = nullptr;      // [P][n] RS
     void* pdef = nullptr;        // [P] RS
+    // more than one cell type: one group per (model, integrator); state,
+    // params and pdef above are then unused (model / integrator / S / P are
+    // the first group's)
+    std::vector<CellGroup> cell_groups;
     uint8_t* stim_id = nullptr;
     cmx_stimulus* stim = nullptr;
     void* F = nullptr;

@@ -39,6 +39,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -133,15 +134,17 @@ __device__ __forceinline__ void log_step(int32_t* its_log, int log_len, int k, c
 
 template <class RS, class RV>
 struct CellArgs {
-    int n;
+    int n;                     // nodes (V, F, activation planes)
+    int n_cells;               // cells of this launch: thread j steps node node[j] (node == null: node j)
+    const int32_t* __restrict__ node;
     int n_sub;                 // ODE substeps per PDE step
     double dt_ode;             // ms
     double am, cm;
     StepClock* clock;
     double* __restrict__ V;
     int ps;                    // plane stride (n rounded up to whole 256-byte units)
-    RS* __restrict__ state;    // [NS-1][ps]: plane s-1 holds state s (V excluded)
-    const RS* __restrict__ params;  // [NP][ps] or null
+    RS* __restrict__ state;    // [NS-1][ps]: plane s-1 holds state s (V excluded), indexed by cell j
+    const RS* __restrict__ params;  // [NP][ps] or null, indexed by cell j
     const RS* __restrict__ pdef;    // [NP] defaults
     int n_param_planes;
     const uint8_t* __restrict__ stim_id;  // 0 = none, k = row k-1
@@ -272,8 +275,11 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
     constexpr bool RL = IG == CMX_RUSH_LARSEN;
     constexpr int NS = Model::NS;
     constexpr int NP = Model::NP > 0 ? Model::NP : 1;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    const bool live = i < a.n;
+    // thread j steps cell j of the launch, which sits at node i; with one
+    // cell group (node == null) i = j
+    const int j = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const bool live = j < a.n_cells;
+    const int i = (live && a.node) ? a.node[j] : j;
     PartialLoad<3, kBlock> pl(a.s.pin, a.s.gin);
     const StepClock clk = *a.clock;
     const int done = a.s.ctl->done;
@@ -285,10 +291,10 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
     RV dx = 0, x0 = 0, xp = 0;
     if (live) {
 #pragma unroll
-        for (int s = 1; s < NS; ++s) y[s] = a.state[(size_t)(s - 1) * a.ps + i];
+        for (int s = 1; s < NS; ++s) y[s] = a.state[(size_t)(s - 1) * a.ps + j];
 #pragma unroll
         for (int q = 0; q < NP; ++q)
-            p[q] = (q < a.n_param_planes) ? a.params[(size_t)q * a.ps + i] : (Model::NP > 0 ? a.pdef[q] : RS(0));
+            p[q] = (q < a.n_param_planes) ? a.params[(size_t)q * a.ps + j] : (Model::NP > 0 ? a.pdef[q] : RS(0));
         sid = a.stim_id ? (int)a.stim_id[i] : 0;
         z = a.s.Z[i];
         dx = a.s.x1[i] + (a.s.x2 ? a.s.x2[i] : RV(0));
@@ -335,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
     if constexpr (MODE == kModeSplitFinal) {
         a.V[i] = vn;
 #pragma unroll
-        for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.ps + i] = y[s];
+        for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.ps + j] = y[s];
         return;
     }
     const double t = step_time(clk, k), t_next = step_next_time(clk, k);
@@ -400,27 +406,30 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         a.G1[i] = (RV)((double)f - a.c * (double)x0);
     }
 #pragma unroll
-    for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.ps + i] = y[s];
+    for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.ps + j] = y[s];
 }
 
-// Iion at the stored state (cmx_get_iionic before any step).
+// Iion at the stored state (cmx_get_iionic before any step); cell j of the
+// group sits at node node[j] (node == null: node j).
 template <class Model, class RS>
-__global__ __launch_bounds__(kBlock) void iionic_kernel(int n, int ps, const double* __restrict__ V, const RS* __restrict__ state,
+__global__ __launch_bounds__(kBlock) void iionic_kernel(int n_cells, const int32_t* __restrict__ node, int ps,
+                                                        const double* __restrict__ V, const RS* __restrict__ state,
                                                         const RS* __restrict__ params, const RS* __restrict__ pdef,
                                                         int n_param_planes, double cm, double* __restrict__ out)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n_cells) return;
+    const int i = node ? node[j] : j;
     constexpr int NS = Model::NS;
     constexpr int NP = Model::NP > 0 ? Model::NP : 1;
     RS y[NS];
     y[0] = (RS)V[i];
 #pragma unroll
-    for (int s = 1; s < NS; ++s) y[s] = state[(size_t)(s - 1) * ps + i];
+    for (int s = 1; s < NS; ++s) y[s] = state[(size_t)(s - 1) * ps + j];
     RS p[NP];
 #pragma unroll
     for (int q = 0; q < NP; ++q)
-        p[q] = (q < n_param_planes) ? params[(size_t)q * ps + i] : (Model::NP > 0 ? pdef[q] : RS(0));
+        p[q] = (q < n_param_planes) ? params[(size_t)q * ps + j] : (Model::NP > 0 ? pdef[q] : RS(0));
     out[i] = (double)Model::template iion<RS>(y, y[0], p, (RS)cm);
 }
 
@@ -1349,13 +1358,20 @@ void launch_cell_mode(cmx_ctx* c, int mode, int kind, int G, const CellArgs<RS, 
 }
 
 template <class Model, class RS, class RV>
-void launch_cell_integrator(cmx_ctx* c, int mode, int kind, int G, const CellArgs<RS, RV>& a)
+void launch_cell_integrator(cmx_ctx* c, int integrator, int mode, int kind, int G, const CellArgs<RS, RV>& a)
 {
-    if (c->integrator == CMX_RUSH_LARSEN) launch_cell_mode<Model, RS, RV, CMX_RUSH_LARSEN>(c, mode, kind, G, a);
-    else if (c->integrator == CMX_BACKWARD_EULER) launch_cell_mode<Model, RS, RV, CMX_BACKWARD_EULER>(c, mode, kind, G, a);
-    else if (c->integrator == CMX_GRL1) launch_cell_mode<Model, RS, RV, CMX_GRL1>(c, mode, kind, G, a);
-    else if (c->integrator == CMX_GRL2) launch_cell_mode<Model, RS, RV, CMX_GRL2>(c, mode, kind, G, a);
+    if (integrator == CMX_RUSH_LARSEN) launch_cell_mode<Model, RS, RV, CMX_RUSH_LARSEN>(c, mode, kind, G, a);
+    else if (integrator == CMX_BACKWARD_EULER) launch_cell_mode<Model, RS, RV, CMX_BACKWARD_EULER>(c, mode, kind, G, a);
+    else if (integrator == CMX_GRL1) launch_cell_mode<Model, RS, RV, CMX_GRL1>(c, mode, kind, G, a);
+    else if (integrator == CMX_GRL2) launch_cell_mode<Model, RS, RV, CMX_GRL2>(c, mode, kind, G, a);
     else launch_cell_mode<Model, RS, RV, CMX_FORWARD_EULER>(c, mode, kind, G, a);
+}
+
+template <class RS, class RV>
+void launch_cell_model(cmx_ctx* c, int model, int integrator, int mode, int kind, int G, const CellArgs<RS, RV>& a)
+{
+    if (model == CMX_MODEL_LR91) launch_cell_integrator<ModelLr91, RS, RV>(c, integrator, mode, kind, G, a);
+    else launch_cell_integrator<ModelTt06, RS, RV>(c, integrator, mode, kind, G, a);
 }
 
 // The cell kernel in one of its roles; verdict: the reduction that decides
@@ -1365,6 +1381,8 @@ int launch_cells(cmx_ctx* c, int mode, const double* verdict, int kind)
 {
     CellArgs<RS, RV> a;
     a.n = c->n;
+    a.n_cells = c->n;
+    a.node = nullptr;
     a.n_sub = c->n_sub;
     a.n_half = c->n_half;
     a.dt_ode = c->dt_ode;
@@ -1389,9 +1407,25 @@ int launch_cells(cmx_ctx* c, int mode, const double* verdict, int kind)
     a.trace_k0 = c->trace_k0;
     a.trace_len = c->trace_len;
     a.trace_slot = c->trace_slot;
-    const int G = grid_of(c->n);
-    if (c->model == CMX_MODEL_LR91) launch_cell_integrator<ModelLr91, RS, RV>(c, mode, kind, G, a);
-    else launch_cell_integrator<ModelTt06, RS, RV>(c, mode, kind, G, a);
+    if (c->cell_groups.empty()) {
+        launch_cell_model<RS, RV>(c, c->model, c->integrator, mode, kind, grid_of(c->n), a);
+    } else {
+        // one launch per cell type; each settles the pending step at its own
+        // nodes (a failed verdict poisons the clock in the first launch and
+        // the others return at once)
+        for (const CellGroup& g : c->cell_groups) {
+            CellArgs<RS, RV> b = a;
+            b.n_cells = g.n;
+            b.node = g.node;
+            b.ps = g.ps;
+            b.state = (RS*)g.state;
+            b.params = nullptr;
+            b.pdef = (const RS*)g.pdef;
+            b.n_param_planes = 0;
+            b.trace = nullptr;
+            launch_cell_model<RS, RV>(c, g.model, g.integrator, mode, kind, grid_of(g.n), b);
+        }
+    }
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
@@ -1889,9 +1923,59 @@ int setup_svi(cmx_ctx* c, const cmx_mesh* mesh, VecAlloc& rvec)
 int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* opts, cmx_ctx* c)
 {
     if (!mesh || !tissue || !opts) return fail(c, CMX_E_ARG, "null mesh, tissue or opts");
-    if (model_sizes(tissue->model, &c->S, &c->P)) return fail(c, CMX_E_ARG, "unknown cell model");
-    if (tissue->integrator < CMX_FORWARD_EULER || tissue->integrator > CMX_GRL2)
-        return fail(c, CMX_E_ARG, "unknown integrator");
+    // Cell types: one (model, integrator) for every node, or per-node groups.
+    const bool grouped = tissue->n_groups > 1;
+    std::vector<int32_t> group_of;
+    if (grouped) {
+        if (!tissue->group_of || !tissue->group_model || !tissue->group_integrator)
+            return fail(c, CMX_E_ARG, "n_groups > 1 needs group_of, group_model and group_integrator");
+        if (tissue->n_params > 0) return fail(c, CMX_E_ARG, "per-node parameters need a single cell type");
+        if (opts->state_variable_interpolation)
+            return fail(c, CMX_E_ARG, "state-variable interpolation needs a single cell type");
+        std::vector<int> count(tissue->n_groups, 0);
+        group_of.assign(tissue->group_of, tissue->group_of + std::max(mesh->n_nodes, 0));
+        for (int32_t g : group_of) {
+            if (g < 0 || g >= tissue->n_groups) return fail(c, CMX_E_ARG, "group_of out of range");
+            ++count[g];
+        }
+        c->S = 0;
+        for (int g = 0; g < tissue->n_groups; ++g) {
+            CellGroup cg;
+            cg.model = tissue->group_model[g];
+            cg.integrator = tissue->group_integrator[g];
+            if (model_sizes(cg.model, &cg.S, &cg.P)) return fail(c, CMX_E_ARG, "unknown cell model in group_model");
+            if (cg.integrator < CMX_FORWARD_EULER || cg.integrator > CMX_GRL2)
+                return fail(c, CMX_E_ARG, "unknown integrator in group_integrator");
+            cg.n = count[g];
+            if (cg.n == 0) continue;
+            c->S = std::max(c->S, cg.S);
+            c->cell_groups.push_back(cg);
+            c->cell_groups.back().h_node.reserve(cg.n);
+        }
+        // group indices of the non-empty groups, in order
+        std::vector<int> slot(tissue->n_groups, -1);
+        for (int g = 0, k = 0; g < tissue->n_groups; ++g)
+            if (count[g]) slot[g] = k++;
+        for (int i = 0; i < mesh->n_nodes; ++i) c->cell_groups[slot[group_of[i]]].h_node.push_back(i);
+        if (c->cell_groups.size() == 1) {   // every node in one group: the plain layout
+            const CellGroup only = c->cell_groups[0];
+            c->cell_groups.clear();
+            c->model = only.model;
+            c->integrator = only.integrator;
+        } else {
+            c->model = c->cell_groups[0].model;
+            c->integrator = c->cell_groups[0].integrator;
+        }
+        int S0;
+        model_sizes(c->model, &S0, &c->P);
+        if (c->cell_groups.empty()) c->S = S0;
+    } else {
+        if (model_sizes(tissue->model, &c->S, &c->P)) return fail(c, CMX_E_ARG, "unknown cell model");
+        if (tissue->integrator < CMX_FORWARD_EULER || tissue->integrator > CMX_GRL2)
+            return fail(c, CMX_E_ARG, "unknown integrator");
+        c->model = tissue->model;
+        c->integrator = tissue->integrator;
+    }
     if (opts->precision != CMX_FP64 && opts->precision != CMX_MIXED) return fail(c, CMX_E_ARG, "unknown precision");
     if (!(opts->dt_pde > 0) || !(opts->dt_ode > 0) || opts->dt_ode > opts->dt_pde * (1 + 1e-12))
         return fail(c, CMX_E_ARG, "need 0 < dt_ode <= dt_pde");
@@ -1910,8 +1994,6 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->dim = mesh->dim;
     if (mesh->elem_dim > 0 && mesh->elem_dim != mesh->dim && opts->state_variable_interpolation)
         return fail(c, CMX_E_ARG, "state-variable interpolation needs elements of the node dimension");
-    c->model = tissue->model;
-    c->integrator = tissue->integrator;
     c->precision = opts->precision;
     c->dt = opts->dt_pde;
     c->dt_ode = opts->dt_ode;
@@ -2033,17 +2115,39 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     const size_t rs = mixed ? sizeof(float) : sizeof(double);
     const size_t rv = rs;
     const double* y0 = model_init(c->model);
-    std::vector<double> planes((size_t)(c->S - 1) * n);
-    for (int s = 1; s < c->S; ++s) std::fill(planes.begin() + (size_t)(s - 1) * n, planes.begin() + (size_t)s * n, y0[s]);
     std::vector<double> V0(n, y0[0]);
+    for (const CellGroup& g : c->cell_groups)
+        for (int32_t i : g.h_node) V0[i] = model_init(g.model)[0];
     if ((rc = dalloc_vec(c, &c->V, n))) return rc;
     CMX_HIP(c, hipMemcpy(c->V, V0.data(), n * sizeof(double), hipMemcpyHostToDevice));
     c->ps = (n + 63) / 64 * 64;
-    const size_t ns = (size_t)(c->S - 1);
-    if ((rc = dalloc(c, (char**)&c->state, ns * c->ps * rs))) return rc;
-    if ((rc = mixed ? to_device_planes<float>(c, c->state, planes.data(), ns, n, c->ps)
-                    : to_device_planes<double>(c, c->state, planes.data(), ns, n, c->ps)))
-        return rc;
+    for (CellGroup& g : c->cell_groups) {
+        g.ps = (g.n + 63) / 64 * 64;
+        const double* gi = model_init(g.model);
+        std::vector<double> gp((size_t)(g.S - 1) * g.n);
+        for (int s = 1; s < g.S; ++s) std::fill(gp.begin() + (size_t)(s - 1) * g.n, gp.begin() + (size_t)s * g.n, gi[s]);
+        if ((rc = dalloc(c, (char**)&g.state, (size_t)(g.S - 1) * g.ps * rs))) return rc;
+        if ((rc = mixed ? to_device_planes<float>(c, g.state, gp.data(), g.S - 1, g.n, g.ps)
+                        : to_device_planes<double>(c, g.state, gp.data(), g.S - 1, g.n, g.ps)))
+            return rc;
+        const int Pg = std::max(g.P, 1);
+        std::vector<double> pd(model_pdef(g.model), model_pdef(g.model) + Pg);
+        if ((rc = dalloc(c, (char**)&g.pdef, Pg * rs))) return rc;
+        if ((rc = mixed ? to_device_planes<float>(c, g.pdef, pd.data(), 1, Pg, Pg)
+                        : to_device_planes<double>(c, g.pdef, pd.data(), 1, Pg, Pg)))
+            return rc;
+        if ((rc = upload(c, &g.node, g.h_node.data(), g.h_node.size()))) return rc;
+    }
+    std::vector<double> planes(c->cell_groups.empty() ? (size_t)(c->S - 1) * n : 0);
+    for (int s = 1; s < c->S && c->cell_groups.empty(); ++s)
+        std::fill(planes.begin() + (size_t)(s - 1) * n, planes.begin() + (size_t)s * n, y0[s]);
+    const size_t ns = c->cell_groups.empty() ? (size_t)(c->S - 1) : 0;
+    if (ns) {
+        if ((rc = dalloc(c, (char**)&c->state, ns * c->ps * rs))) return rc;
+        if ((rc = mixed ? to_device_planes<float>(c, c->state, planes.data(), ns, n, c->ps)
+                        : to_device_planes<double>(c, c->state, planes.data(), ns, n, c->ps)))
+            return rc;
+    }
     const int P = std::max(c->P, 1);
     std::vector<double> pdef(model_pdef(c->model), model_pdef(c->model) + P);
     if ((rc = dalloc(c, (char**)&c->pdef, P * rs))) return rc;
@@ -2110,6 +2214,28 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     *c->h_clock = ck;
     CMX_HIP(c, hipDeviceSynchronize());
     return CMX_OK;
+}
+
+// Iion at the stored state of n_cells cells (cell j at node node[j], or j).
+void launch_iionic(cmx_ctx* c, int model, int n_cells, const int32_t* node, int ps, const void* state,
+                   const void* params, const void* pdef, int n_param_planes)
+{
+    const int G = grid_of(n_cells);
+    if (c->precision == CMX_MIXED) {
+        if (model == CMX_MODEL_LR91)
+            iionic_kernel<ModelLr91, float><<<G, kBlock, 0, c->stream>>>(n_cells, node, ps, c->V, (const float*)state,
+                (const float*)params, (const float*)pdef, n_param_planes, c->cm, c->iion);
+        else
+            iionic_kernel<ModelTt06, float><<<G, kBlock, 0, c->stream>>>(n_cells, node, ps, c->V, (const float*)state,
+                (const float*)params, (const float*)pdef, n_param_planes, c->cm, c->iion);
+    } else {
+        if (model == CMX_MODEL_LR91)
+            iionic_kernel<ModelLr91, double><<<G, kBlock, 0, c->stream>>>(n_cells, node, ps, c->V, (const double*)state,
+                (const double*)params, (const double*)pdef, n_param_planes, c->cm, c->iion);
+        else
+            iionic_kernel<ModelTt06, double><<<G, kBlock, 0, c->stream>>>(n_cells, node, ps, c->V, (const double*)state,
+                (const double*)params, (const double*)pdef, n_param_planes, c->cm, c->iion);
+    }
 }
 
 void release(cmx_ctx* c)
@@ -2235,7 +2361,8 @@ int cmx_autotune(cmx_ctx* c, double t0, int32_t n_steps, int32_t n_candidates, c
     // activation times, budgets and statistics
     const size_t rv = c->precision == CMX_MIXED ? sizeof(float) : sizeof(double);
     const size_t rs = rv;
-    std::vector<char> V(c->n * sizeof(double)), S((size_t)(c->S - 1) * c->ps * rs), X0(c->n * rv), XP(c->n * rv),
+    std::vector<char> V(c->n * sizeof(double)), S(c->cell_groups.empty() ? (size_t)(c->S - 1) * c->ps * rs : 0),
+        X0(c->n * rv), XP(c->n * rv),
         act(c->act ? 5 * (size_t)c->n * sizeof(double) : 0), Yv(c->Y ? c->n * sizeof(double) : 0), F(c->n * rv);
     auto copy_out = [&](std::vector<char>& h, const void* d) {
         return h.empty() ? hipSuccess : hipMemcpy(h.data(), d, h.size(), hipMemcpyDeviceToHost);
@@ -2243,8 +2370,11 @@ int cmx_autotune(cmx_ctx* c, double t0, int32_t n_steps, int32_t n_candidates, c
     auto copy_in = [&](void* d, const std::vector<char>& h) {
         return h.empty() ? hipSuccess : hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice);
     };
+    std::vector<std::vector<char>> GS;
+    for (const CellGroup& g : c->cell_groups) GS.emplace_back((size_t)(g.S - 1) * g.ps * rs);
     CMX_HIP(c, copy_out(V, c->V));
-    CMX_HIP(c, copy_out(S, c->state));
+    if (c->cell_groups.empty()) CMX_HIP(c, copy_out(S, c->state));
+    for (size_t q = 0; q < GS.size(); ++q) CMX_HIP(c, copy_out(GS[q], c->cell_groups[q].state));
     CMX_HIP(c, copy_out(X0, c->vec[V_X0]));
     CMX_HIP(c, copy_out(XP, c->vec[V_XP]));
     CMX_HIP(c, copy_out(act, c->act));
@@ -2256,7 +2386,8 @@ int cmx_autotune(cmx_ctx* c, double t0, int32_t n_steps, int32_t n_candidates, c
     const bool stepped = c->stepped;
     auto restore = [&]() -> int {
         CMX_HIP(c, copy_in(c->V, V));
-        CMX_HIP(c, copy_in(c->state, S));
+        if (c->cell_groups.empty()) CMX_HIP(c, copy_in(c->state, S));
+        for (size_t q = 0; q < GS.size(); ++q) CMX_HIP(c, copy_in(c->cell_groups[q].state, GS[q]));
         CMX_HIP(c, copy_in(c->vec[V_X0], X0));
         CMX_HIP(c, copy_in(c->vec[V_XP], XP));
         CMX_HIP(c, copy_in(c->act, act));
@@ -2333,9 +2464,23 @@ int cmx_get_state(cmx_ctx* c, double* states)
     if (!c || !states) return fail(c, CMX_E_ARG, "null argument");
     int rc = cmx_get_voltage(c, states);
     if (rc) return rc;
-    const size_t ns = (size_t)(c->S - 1);
-    return c->precision == CMX_MIXED ? from_device_planes<float>(c, states + c->n, c->state, ns, c->n, c->ps)
-                                     : from_device_planes<double>(c, states + c->n, c->state, ns, c->n, c->ps);
+    if (c->cell_groups.empty()) {
+        const size_t ns = (size_t)(c->S - 1);
+        return c->precision == CMX_MIXED ? from_device_planes<float>(c, states + c->n, c->state, ns, c->n, c->ps)
+                                         : from_device_planes<double>(c, states + c->n, c->state, ns, c->n, c->ps);
+    }
+    // mixed cell types: node i's column holds its model's states, NaN past them
+    const size_t n = (size_t)c->n;
+    std::fill(states + n, states + (size_t)c->S * n, std::numeric_limits<double>::quiet_NaN());
+    for (const CellGroup& g : c->cell_groups) {
+        std::vector<double> planes((size_t)(g.S - 1) * g.n);
+        rc = c->precision == CMX_MIXED ? from_device_planes<float>(c, planes.data(), g.state, g.S - 1, g.n, g.ps)
+                                       : from_device_planes<double>(c, planes.data(), g.state, g.S - 1, g.n, g.ps);
+        if (rc) return rc;
+        for (int s = 1; s < g.S; ++s)
+            for (int j = 0; j < g.n; ++j) states[(size_t)s * n + g.h_node[j]] = planes[(size_t)(s - 1) * g.n + j];
+    }
+    return CMX_OK;
 }
 
 int cmx_set_state(cmx_ctx* c, const double* states)
@@ -2343,10 +2488,22 @@ int cmx_set_state(cmx_ctx* c, const double* states)
     if (!c || !states) return fail(c, CMX_E_ARG, "null argument");
     int rc = cmx_set_voltage(c, states);
     if (rc) return rc;
-    const size_t ns = (size_t)(c->S - 1);
     c->stepped = false;
-    return c->precision == CMX_MIXED ? to_device_planes<float>(c, c->state, states + c->n, ns, c->n, c->ps)
-                                     : to_device_planes<double>(c, c->state, states + c->n, ns, c->n, c->ps);
+    if (c->cell_groups.empty()) {
+        const size_t ns = (size_t)(c->S - 1);
+        return c->precision == CMX_MIXED ? to_device_planes<float>(c, c->state, states + c->n, ns, c->n, c->ps)
+                                         : to_device_planes<double>(c, c->state, states + c->n, ns, c->n, c->ps);
+    }
+    const size_t n = (size_t)c->n;
+    for (const CellGroup& g : c->cell_groups) {
+        std::vector<double> planes((size_t)(g.S - 1) * g.n);
+        for (int s = 1; s < g.S; ++s)
+            for (int j = 0; j < g.n; ++j) planes[(size_t)(s - 1) * g.n + j] = states[(size_t)s * n + g.h_node[j]];
+        rc = c->precision == CMX_MIXED ? to_device_planes<float>(c, g.state, planes.data(), g.S - 1, g.n, g.ps)
+                                       : to_device_planes<double>(c, g.state, planes.data(), g.S - 1, g.n, g.ps);
+        if (rc) return rc;
+    }
+    return CMX_OK;
 }
 
 int cmx_get_iionic(cmx_ctx* c, double* out)
@@ -2358,17 +2515,11 @@ int cmx_get_iionic(cmx_ctx* c, double* out)
             int rc = dalloc(c, &c->iion, c->n);
             if (rc) return rc;
         }
-        const int G = grid_of(c->n);
-        if (c->precision == CMX_MIXED) {
-            if (c->model == CMX_MODEL_LR91)
-                iionic_kernel<ModelLr91, float><<<G, kBlock, 0, c->stream>>>(c->n, c->ps, c->V, (const float*)c->state, (const float*)c->params, (const float*)c->pdef, c->n_param_planes, c->cm, c->iion);
-            else
-                iionic_kernel<ModelTt06, float><<<G, kBlock, 0, c->stream>>>(c->n, c->ps, c->V, (const float*)c->state, (const float*)c->params, (const float*)c->pdef, c->n_param_planes, c->cm, c->iion);
+        if (c->cell_groups.empty()) {
+            launch_iionic(c, c->model, c->n, nullptr, c->ps, c->state, c->params, c->pdef, c->n_param_planes);
         } else {
-            if (c->model == CMX_MODEL_LR91)
-                iionic_kernel<ModelLr91, double><<<G, kBlock, 0, c->stream>>>(c->n, c->ps, c->V, (const double*)c->state, (const double*)c->params, (const double*)c->pdef, c->n_param_planes, c->cm, c->iion);
-            else
-                iionic_kernel<ModelTt06, double><<<G, kBlock, 0, c->stream>>>(c->n, c->ps, c->V, (const double*)c->state, (const double*)c->params, (const double*)c->pdef, c->n_param_planes, c->cm, c->iion);
+            for (const CellGroup& g : c->cell_groups)
+                launch_iionic(c, g.model, g.n, g.node, g.ps, g.state, nullptr, g.pdef, 0);
         }
         CMX_HIP(c, hipGetLastError());
         CMX_HIP(c, hipStreamSynchronize(c->stream));
@@ -2484,6 +2635,7 @@ int cmx_set_params(cmx_ctx* c, int32_t n_params, const double* params)
     if (!c) return fail(nullptr, CMX_E_ARG, "null context");
     if (n_params < 0 || n_params > c->P || (n_params > 0 && !params))
         return fail(c, CMX_E_ARG, "n_params must be in [0, the model's parameter count] with params given");
+    if (!c->cell_groups.empty()) return fail(c, CMX_E_ARG, "per-node parameters need a single cell type");
     CMX_HIP(c, hipStreamSynchronize(c->stream));
     const bool mixed = c->precision == CMX_MIXED;
     if (n_params > c->param_planes_alloc) {

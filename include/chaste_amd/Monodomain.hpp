@@ -388,12 +388,19 @@ public:
         std::vector<int32_t> stim_index(n, -1);
         std::vector<cmx_stimulus> table;
         std::vector<const AbstractStimulusFunction*> seen;
-        int model = -1, integrator = -1;
+        // one cell group per distinct (model, integrator) the factory returns
+        std::vector<int32_t> group_of(n), group_model, group_integrator;
         for (unsigned i = 0; i < n; ++i) {
             CellDescriptor d = mpFactory->CreateCardiacCellForTissueNode(mpMesh->GetNodeLocation(i));
-            if (model < 0) { model = d.model; integrator = d.integrator; }
-            if (d.model != model || d.integrator != integrator)
-                throw Exception(CMX_E_ARG, "every node must use the same cell model and solver");
+            int g = -1;
+            for (size_t q = 0; q < group_model.size(); ++q)
+                if (group_model[q] == d.model && group_integrator[q] == d.integrator) g = (int)q;
+            if (g < 0) {
+                g = (int)group_model.size();
+                group_model.push_back(d.model);
+                group_integrator.push_back(d.integrator);
+            }
+            group_of[i] = g;
             const cmx_stimulus s = d.stimulus ? d.stimulus->Describe() : cmx_stimulus{0, 0, 0, -1, 0};
             if (s.magnitude == 0.0) continue;
             int k = -1;
@@ -435,9 +442,16 @@ public:
             mesh.sigma_elem = tensors.data();
         }
         cmx_tissue tissue{};
-        mModel = model;
-        tissue.model = model;
-        tissue.integrator = integrator;
+        mModel = group_model.empty() ? CMX_MODEL_LR91 : group_model[0];
+        mCellTypes = (int)group_model.size();
+        tissue.model = mModel;
+        tissue.integrator = group_integrator.empty() ? CMX_FORWARD_EULER : group_integrator[0];
+        if (group_model.size() > 1) {
+            tissue.n_groups = (int32_t)group_model.size();
+            tissue.group_of = group_of.data();
+            tissue.group_model = group_model.data();
+            tissue.group_integrator = group_integrator.data();
+        }
         tissue.n_stimuli = (int32_t)table.size();
         tissue.stimuli = table.data();
         tissue.stim_index = stim_index.data();
@@ -459,7 +473,7 @@ public:
         o.state_variable_interpolation = cfg->GetUseStateVariableInterpolation() ? 1 : 0;
         if (mpCtx) { cmx_destroy(mpCtx); mpCtx = nullptr; }
         Check(cmx_create(&mesh, &tissue, &o, &mpCtx), nullptr);
-        if (!mParams.empty()) {   // parameters set before a re-initialisation
+        if (!mParams.empty() && mCellTypes == 1) {   // parameters set before a re-initialisation
             int32_t S = 0, P = 0;
             Check(cmx_model_info(mModel, &S, &P), nullptr);
             Check(cmx_set_params(mpCtx, P, mParams.data()), mpCtx);
@@ -563,6 +577,7 @@ public:
     void SetCellParameter(const std::string& rName, const std::vector<double>& values)
     {
         if (!mpCtx) throw Exception(CMX_E_STATE, "SetCellParameter needs Initialise() first");
+        if (mCellTypes > 1) throw Exception(CMX_E_ARG, "SetCellParameter needs a single cell type");
         const unsigned n = mpMesh->GetNumNodes();
         if (values.size() != n) throw Exception(CMX_E_ARG, "one value per node is required");
         int32_t S = 0, P = 0;
@@ -620,6 +635,7 @@ private:
     double mCurrentTime = 0.0;
     double mAm = 0.0, mCm = 0.0;
     int mModel = CMX_MODEL_LR91;
+    int mCellTypes = 1;             // distinct (model, integrator) pairs the factory returned
     std::vector<double> mParams;    // [P][n] planes once a parameter is set
 };
 

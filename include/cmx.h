@@ -76,15 +76,25 @@ typedef struct {
     double magnitude, start, duration, period, stop;
 } cmx_stimulus;
 
-/* Tissue: one cell model for every node (AbstractCardiacCellFactory). */
+/* Tissue: the cells the factory creates for every node
+ * (AbstractCardiacCellFactory::CreateCardiacCellForTissueNode,
+ * heart/src/problem/cell_factories/AbstractCardiacCellFactory.hpp:100): one
+ * model and integrator for every node, or, with n_groups > 1, node i gets
+ * group_model[group_of[i]] and group_integrator[group_of[i]] (each group is
+ * stepped by its own launch of the cell kernel). */
 typedef struct {
-    int32_t model;             /* CMX_MODEL_* */
-    int32_t integrator;        /* CMX_FORWARD_EULER ... CMX_GRL2 */
+    int32_t model;             /* CMX_MODEL_* (ignored when n_groups > 1) */
+    int32_t integrator;        /* CMX_FORWARD_EULER ... CMX_GRL2 (ignored when n_groups > 1) */
     int32_t n_stimuli;         /* rows in stimuli (<= 255) */
     const cmx_stimulus* stimuli;
     const int32_t* stim_index; /* [n_nodes]: -1 = no stimulus, else row; NULL = none */
-    int32_t n_params;          /* per-node parameter planes given (<= model's count) */
+    int32_t n_params;          /* per-node parameter planes given (<= model's count; 0 with n_groups > 1) */
     const double* params;      /* [n_params][n_nodes] or NULL (model defaults) */
+    int32_t n_groups;          /* 0 or 1: model / integrator everywhere; > 1: per-node cell types
+                                  (no per-node parameters, no state-variable interpolation, no adjoint) */
+    const int32_t* group_of;   /* [n_nodes] group of each node */
+    const int32_t* group_model;       /* [n_groups] CMX_MODEL_* */
+    const int32_t* group_integrator;  /* [n_groups] CMX_FORWARD_EULER ... */
 } cmx_tissue;
 
 /* Options (HeartConfig subset, heart/src/problem/HeartConfigDefaults.hpp:52-152). */
@@ -162,7 +172,9 @@ int cmx_get_voltage(cmx_ctx* ctx, double* V);
 int cmx_set_voltage(cmx_ctx* ctx, const double* V);
 /* Cell states as [n_states][n_nodes] planes in the model's state order; the
  * V plane (index 0) is read from / written to the voltage vector
- * (AbstractCardiacCellInterface::GetStdVecStateVariables / SetStateVariables). */
+ * (AbstractCardiacCellInterface::GetStdVecStateVariables / SetStateVariables).
+ * With several cell types n_states is the largest state count and node i's
+ * column holds its own model's states, NaN past them (ignored by set). */
 int cmx_get_state(cmx_ctx* ctx, double* states);
 int cmx_set_state(cmx_ctx* ctx, const double* states);
 /* Ionic current cache (uA/cm^2) of the last step

@@ -665,15 +665,8 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
     if (cfg->threads > 0) omp_set_num_threads(cfg->threads);
 #endif
     const int dim = cfg->dim, model = cfg->model;
-    const int S = num_states(model);
     const int integ = cfg->integrator;
-    if (S < 0 || integ < ORC_FE || integ > ORC_GRL2 || (model == ORC_LR91_HAND && integ != ORC_FE)) return -1;
     const int fails0 = g_newton_failures.load();
-    // Rush-Larsen and backward-Euler cells step by dt_ode always
-    // (AbstractRushLarsenCardiacCell.cpp:106-153,
-    // AbstractBackwardEulerCardiacCell.hpp:204-245), forward Euler by the
-    // stepper's next step (AbstractOneStepIvpOdeSolver.cpp:115-153).
-    const bool fixed_dt = integ != ORC_FE;
     int n, ne;
     std::vector<double> coords;
     std::vector<int> elems;
@@ -688,6 +681,24 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
         elems.resize((size_t)ne * (dim + 1));
         orc_slab_fill(dim, cfg->ex, cfg->ey, cfg->ez, cfg->h, coords.data(), elems.data());
     }
+    // Cell type of each node; S = the largest state count.  Rush-Larsen and
+    // backward-Euler cells step by dt_ode always
+    // (AbstractRushLarsenCardiacCell.cpp:106-153,
+    // AbstractBackwardEulerCardiacCell.hpp:204-245), forward Euler by the
+    // stepper's next step (AbstractOneStepIvpOdeSolver.cpp:115-153).
+    std::vector<int> nmodel(n, model), ninteg(n, integ);
+    if (cfg->node_model) nmodel.assign(cfg->node_model, cfg->node_model + n);
+    if (cfg->node_integrator) ninteg.assign(cfg->node_integrator, cfg->node_integrator + n);
+    int S = 0;
+    for (int i = 0; i < n; ++i) {
+        const int Si = num_states(nmodel[i]);
+        if (Si < 0 || ninteg[i] < ORC_FE || ninteg[i] > ORC_GRL2 || (nmodel[i] == ORC_LR91_HAND && ninteg[i] != ORC_FE))
+            return -1;
+        S = std::max(S, Si);
+    }
+    if (S <= 0) return -1;
+    const bool mixed_cells = cfg->node_model || cfg->node_integrator;
+    if (mixed_cells && (cfg->svi || cfg->params)) return -1;
     double sigma[9] = {};
     for (int r = 0; r < dim; ++r) sigma[r * dim + r] = cfg->sigma[r];
     const double* sig = cfg->sigma_elem ? cfg->sigma_elem : sigma;
@@ -705,11 +716,12 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
 
     // Cells: state-major planes y[s*n + i]; V lives in its own vector.
     std::vector<double> Y((size_t)S * n);
-    const double* y0 = initial_state(model);
-    for (int s = 0; s < S; ++s)
-        for (int i = 0; i < n; ++i) Y[(size_t)s * n + i] = y0[s];
     std::vector<double> V(n), Vprev(n), iion(n), istim(n), z(n), b(n), svi(cfg->svi ? n : 0);
-    for (int i = 0; i < n; ++i) V[i] = y0[0];
+    for (int i = 0; i < n; ++i) {
+        const double* y0 = initial_state(nmodel[i]);
+        for (int s = 0; s < num_states(nmodel[i]); ++s) Y[(size_t)s * n + i] = y0[s];
+        V[i] = y0[0];
+    }
     std::vector<char> stimulated(n);
     for (int i = 0; i < n; ++i)
         stimulated[i] = cfg->stim_mask ? cfg->stim_mask[i] != 0 : coords[(size_t)i * dim] < cfg->stim_xlim;
@@ -733,8 +745,6 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
         std::fill(cfg->modifier_out, cfg->modifier_out + 4 * (size_t)n, -1.0);
         modifier(0.0);
     }
-    const int P = num_params(model);
-    const double* pdef = default_params(model);
     if (act_out) std::fill(act_out, act_out + n, -1.0);
 
     orc_mono_stats st{};
@@ -748,8 +758,11 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
     auto cells_with_voltage = [&](double t0, double t1) {
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) {
+            const int mi = nmodel[i], ii = ninteg[i], Si = num_states(mi), P = num_params(mi);
+            const double* pdef = default_params(mi);
+            const bool fixed_dt = ii != ORC_FE;
             double y[64], p[64];
-            for (int s = 0; s < S; ++s) y[s] = Y[(size_t)s * n + i];
+            for (int s = 0; s < Si; ++s) y[s] = Y[(size_t)s * n + i];
             for (int q = 0; q < P; ++q)
                 p[q] = (cfg->params && q < cfg->n_param) ? cfg->params[(size_t)q * n + i] : pdef[q];
             y[0] = V[i];
@@ -757,11 +770,11 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
             while (!ode.at_end()) {
                 const double ts = ode.time;
                 const double sa = stimulated[i] ? stim.at(ts) / am : 0.0;
-                cell_substep(model, integ, p, cm, y, ts, fixed_dt ? cfg->dt_ode : ode.step_size(), sa, true);
+                cell_substep(mi, ii, p, cm, y, ts, fixed_dt ? cfg->dt_ode : ode.step_size(), sa, true);
                 ode.advance();
             }
             V[i] = y[0];
-            for (int s = 1; s < S; ++s) Y[(size_t)s * n + i] = y[s];
+            for (int s = 1; s < Si; ++s) Y[(size_t)s * n + i] = y[s];
         }
     };
     Stepper pde(0.0, cfg->t_end, dt);
@@ -816,8 +829,11 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
         auto c0 = std::chrono::steady_clock::now();
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) {
+            const int mi = nmodel[i], ii = ninteg[i], Si = num_states(mi), P = num_params(mi);
+            const double* pdef = default_params(mi);
+            const bool fixed_dt = ii != ORC_FE;
             double y[64], p[64];
-            for (int s = 0; s < S; ++s) y[s] = Y[(size_t)s * n + i];
+            for (int s = 0; s < Si; ++s) y[s] = Y[(size_t)s * n + i];
             for (int q = 0; q < P; ++q)
                 p[q] = (cfg->params && q < cfg->n_param) ? cfg->params[(size_t)q * n + i] : pdef[q];
             y[0] = V[i];
@@ -825,12 +841,12 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
             while (!ode.at_end()) {
                 const double ts = ode.time;
                 const double sa = stimulated[i] ? stim.at(ts) / am : 0.0;
-                cell_substep(model, integ, p, cm, y, ts, fixed_dt ? cfg->dt_ode : ode.step_size(), sa, false);
+                cell_substep(mi, ii, p, cm, y, ts, fixed_dt ? cfg->dt_ode : ode.step_size(), sa, false);
                 ode.advance();
             }
             y[0] = V[i];
-            for (int s = 1; s < S; ++s) Y[(size_t)s * n + i] = y[s];
-            iion[i] = cell_iionic(model, p, cm, y);
+            for (int s = 1; s < Si; ++s) Y[(size_t)s * n + i] = y[s];
+            iion[i] = cell_iionic(mi, p, cm, y);
             istim[i] = stimulated[i] ? stim.at(t_next) : 0.0;
         }
         st.t_ode += seconds_since(c0);

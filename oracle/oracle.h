@@ -130,6 +130,11 @@ typedef struct {
     int mesh_n_nodes, mesh_n_elems;
     const double* mesh_coords;       /* [n][dim], or NULL for the slab */
     const int* mesh_elems;           /* [ne][dim+1] from zero */
+    /* mixed cell types (a factory returning different cells per node,
+       AbstractCardiacCellFactory::CreateCardiacCellForTissueNode): per-node
+       orc_model / orc_integrator, or NULL for model / integrator everywhere */
+    const int* node_model;
+    const int* node_integrator;
 } orc_mono_cfg;
 
 typedef struct {

@@ -31,7 +31,7 @@ class MonoCfg(ctypes.Structure):
                 ('modifier_out', _dp), ('modifier_threshold', ctypes.c_double), ('modifier_every', ctypes.c_int),
                 ('split', ctypes.c_int), ('svi', ctypes.c_int), ('stim_mask', ctypes.POINTER(ctypes.c_ubyte)),
                 ('mesh_n_nodes', ctypes.c_int), ('mesh_n_elems', ctypes.c_int), ('mesh_coords', _dp),
-                ('mesh_elems', _ip)]
+                ('mesh_elems', _ip), ('node_model', _ip), ('node_integrator', _ip)]
 
 
 class MonoStats(ctypes.Structure):
@@ -241,9 +241,10 @@ def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, si
              stim_xlim=1e-5, t_end=1.0, ksp_rtol=1e-12, ksp_atol=0.0, max_its=10000, threads=0,
              params=None, trace_nodes=None, act_threshold=None, want_state=False, lumped=False, sigma_elem=None,
              stim_period=0.0, modifier_threshold=None, modifier_every=1, split=False, backward_euler=False,
-             svi=False, stim_mask=None, integrator=None, mesh=None):
+             svi=False, stim_mask=None, integrator=None, mesh=None, node_model=None, node_integrator=None):
     """Full FP64 monodomain run (AbstractCardiacProblem::Solve equivalent)
-    on the slab, or on mesh = (coords[n, dim], elems[ne, dim+1])."""
+    on the slab, or on mesh = (coords[n, dim], elems[ne, dim+1]); node_model /
+    node_integrator [n] give each node its own cell type."""
     if mesh is not None:
         mc = np.ascontiguousarray(mesh[0], dtype=np.float64)
         me = np.ascontiguousarray(mesh[1], dtype=np.int32)
@@ -265,8 +266,11 @@ def mono_run(dim, ex, ey=0, ez=0, h=0.01, model=LR91_HAND, rush_larsen=False, si
                   0.0 if modifier_threshold is None else float(modifier_threshold), int(modifier_every),
                   int(split), int(svi), _p(mask, ctypes.POINTER(ctypes.c_ubyte)),
                   0 if mc is None else n, 0 if me is None else me.shape[0], _p(mc), _p(me, _ip))
+    nm = None if node_model is None else np.ascontiguousarray(node_model, dtype=np.int32)
+    ni = None if node_integrator is None else np.ascontiguousarray(node_integrator, dtype=np.int32)
+    cfg.node_model, cfg.node_integrator = _p(nm, _ip), _p(ni, _ip)
     V = np.zeros(n)
-    S = num_states(model)
+    S = num_states(model) if nm is None else max(num_states(int(m)) for m in np.unique(nm))
     state = np.zeros((S, n)) if want_state else None
     act = np.zeros(n) if act_threshold is not None else None
     tn = None

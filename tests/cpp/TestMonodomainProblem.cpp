@@ -164,6 +164,48 @@ static void RunShortLastIntervalCase(const std::string& meshes, const std::strin
     std::fflush(stdout);
 }
 
+// A factory that returns different cell classes per node (the reference's
+// factories may, AbstractCardiacCellFactory.hpp:100): LR91 forward-Euler cells
+// for x < 0.15 cm and TT06 Rush-Larsen cells elsewhere on a 0.3 x 0.1 cm
+// sheet, stimulated on x = 0.  Prints every node's voltage after 4 ms for the
+// Python side to check against the oracle's per-node-type run.
+template <unsigned DIM>
+class MixedCellFactory : public AbstractCardiacCellFactory<DIM> {
+public:
+    MixedCellFactory() : mpStimulus(new SimpleStimulus(-600 * 5000, 0.5)) {}
+    CellDescriptor CreateCardiacCellForTissueNode(const double* x) override
+    {
+        auto stim = x[0] * x[0] <= 1e-10 ? mpStimulus : this->mpZeroStimulus;
+        if (x[0] < 0.15) return this->template Make<CellLuoRudy1991FromCellML>(stim);
+        return this->template Make<CellTenTusscher2006EpiFromCellMLRushLarsen>(stim);
+    }
+private:
+    std::shared_ptr<AbstractStimulusFunction> mpStimulus;
+};
+
+static void RunMixedCellTypesCase()
+{
+    HeartConfig::Reset();
+    HeartConfig::Instance()->SetSpaceDimension(2);
+    HeartConfig::Instance()->SetSheetDimensions(0.3, 0.1, 0.01);
+    HeartConfig::Instance()->SetSimulationDuration(4.0);
+    HeartConfig::Instance()->SetUseRelativeTolerance(1e-12);
+    MixedCellFactory<2> cell_factory;
+    MonodomainProblem<2> monodomain_problem(&cell_factory);
+    monodomain_problem.Initialise();
+    int thrown = 0;
+    const unsigned n = monodomain_problem.rGetMesh().GetNumNodes();
+    try { monodomain_problem.SetCellParameter("fast_sodium_current__g_Na", std::vector<double>(n, 1.0)); }
+    catch (const Exception& e) { ++thrown; }
+    monodomain_problem.Solve();
+    const std::vector<double> V = monodomain_problem.GetSolution();
+    std::printf("{\"case\": \"mixed_cell_types\", \"thrown\": %d, \"V\": [", thrown);
+    for (size_t i = 0; i < V.size(); ++i) std::printf("%s%.17g", i ? ", " : "", V[i]);
+    std::printf("], \"pass\": %s}\n", thrown == 1 ? "true" : "false");
+    if (thrown != 1) ++failures;
+    std::fflush(stdout);
+}
+
 // Per-node cell parameters through SetCellParameter (the reference sets them
 // on each cell, AbstractParameterisedSystem::SetParameter): the 1-D case of
 // RunCase with g_Na = 0 everywhere cannot fire, so node 7 stays below -40 mV
@@ -234,6 +276,7 @@ int main(int argc, char** argv)
     if (argc > 2) RunPrintingTimesCase(argv[1], out("MonoProblem1dRequestedTimes"));
     if (argc > 2) RunShortLastIntervalCase(argv[1], out("MonoProblem1dShortLastInterval"));
     RunParameterCase();
+    RunMixedCellTypesCase();
     if (!errors_ok) ++failures;
     std::printf("{\"case\": \"errors\", \"thrown\": %d, \"pass\": %s}\n", thrown, errors_ok ? "true" : "false");
     return failures == 0 ? 0 : 1;

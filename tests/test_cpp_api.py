@@ -34,7 +34,7 @@ def test_cpp_monodomain_problem_goldens():
     print(r.stdout)
     cases = [json.loads(l) for l in r.stdout.splitlines() if l.startswith('{')]
     assert {c['case'] for c in cases} == {'1d', '2d', '3d', 'errors', 'mesh_file_1d', 'printing_times',
-                                          'cell_parameter', 'two_halves', 'short_last_interval'}
+                                          'cell_parameter', 'two_halves', 'short_last_interval', 'mixed_cell_types'}
     # the mesh-file case wrote its results file: V at t = 0 and every 0.01 ms to 2 ms
     f = h5lite.File(os.path.join(out, 'MonoProblem1d', 'MonodomainLR91_1d.h5'))
     data, times = f.read('Data'), f.read('Time')
@@ -56,5 +56,18 @@ def test_cpp_monodomain_problem_goldens():
     data, times = f.read('Data'), f.read('Time')
     assert data.shape == (201, 1331, 1) and np.allclose(times, np.arange(201) * 0.01)
     check_slab3d_geometry({k: data[k, :, 0] for k in (43, 100, 200)})
+    # per-node cell types from the factory against the oracle's per-node-type run
+    mixed = [c for c in cases if c['case'] == 'mixed_cell_types'][0]
+    from oracle import pyoracle
+    x = np.tile(np.arange(31) * 0.01, 11)
+    left = x < 0.15
+    ref = pyoracle.mono_run(2, 30, 10, 0, model=pyoracle.LR91_CELLML, sigma=(1.75, 1.75, 1.75), stim_mag=-3e6,
+                            stim_dur=0.5, stim_xlim=1e-5, t_end=4.0, ksp_rtol=1e-12,
+                            node_model=np.where(left, pyoracle.LR91_CELLML, pyoracle.TT06_EPI),
+                            node_integrator=np.where(left, pyoracle.FE, pyoracle.RL))['V']
+    got = np.array(mixed['V'])
+    print('mixed cell types: max rel err %.3e' % (np.abs(got - ref).max() / np.abs(ref).max()))
+    assert got.shape == ref.shape and np.abs(got - ref).max() / np.abs(ref).max() <= 1e-9
+    assert got[~left].max() > 0.0            # the wave reached the TT06 cells
     assert all(c['pass'] for c in cases)
     assert r.returncode == 0

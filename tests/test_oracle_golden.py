@@ -847,3 +847,30 @@ def test_face_stimulus_3d_mesh_file(oracle):
     assert face.sum() > 0
     v = r['V'][face]
     assert np.abs(v - 21.3).max() <= 1.0 and np.abs(v - v[0]).max() <= 1.1
+
+
+def test_mixed_cell_types_decouple_without_diffusion(oracle):
+    """Per-node cell types (a factory returning different cells per node):
+    with zero conductivity the nodes do not interact (K = 0, A = c M, so the
+    solve returns each node's own ICI update), and every node of a tissue
+    that mixes LR91 forward-Euler and TT06 Rush-Larsen cells ends where the
+    same node ends in a tissue made of its own type only.  With diffusion
+    the mixed run differs from both."""
+    dims = (2, 12, 6, 0)
+    n = 13 * 7
+    x = np.tile(np.arange(13), 7)
+    node_model = np.where(x < 6, oracle.LR91_CELLML, oracle.TT06_EPI)
+    node_integ = np.where(x < 6, oracle.FE, oracle.RL)
+    kw = dict(stim_mag=-400000.0, stim_dur=0.5, stim_xlim=0.02, t_end=3.0, ksp_rtol=1e-13, want_state=True)
+    mixed = oracle.mono_run(*dims, sigma=(0.0, 0.0, 0.0), node_model=node_model, node_integrator=node_integ, **kw)
+    lr = oracle.mono_run(*dims, sigma=(0.0, 0.0, 0.0), model=oracle.LR91_CELLML, **kw)
+    tt = oracle.mono_run(*dims, sigma=(0.0, 0.0, 0.0), model=oracle.TT06_EPI, rush_larsen=True, **kw)
+    a, b = node_model == oracle.LR91_CELLML, node_model == oracle.TT06_EPI
+    assert np.abs(mixed['V'][a] - lr['V'][a]).max() <= 1e-9
+    assert np.abs(mixed['V'][b] - tt['V'][b]).max() <= 1e-9
+    assert np.abs(mixed['state'][:8, a] - lr['state'][:8, a]).max() <= 1e-9
+    assert np.abs(mixed['state'][:, b] - tt['state'][:, b]).max() <= 1e-9
+    assert mixed['V'][a].max() > 0.0          # the stimulated LR91 column fired
+    coupled = oracle.mono_run(*dims, node_model=node_model, node_integrator=node_integ,
+                              **{k: v for k, v in kw.items() if k != 'want_state'})
+    assert np.abs(coupled['V'] - mixed['V']).max() > 1.0
