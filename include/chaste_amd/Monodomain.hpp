@@ -352,6 +352,124 @@ private:
 };
 
 // ---------------------------------------------------------------------------
+// The PDE solver contract (pde/src/solver/AbstractDynamicLinearPdeSolver.hpp:
+// 160-182, 328-497) and the GPU solver behind it
+// ---------------------------------------------------------------------------
+
+// The reference's solvers pass PETSc vectors: SetInitialCondition keeps the
+// caller's vector without taking ownership, Solve() returns a new one that
+// the caller destroys (PetscTools::Destroy).  Here a Vec is a host vector of
+// nodal voltages with the same ownership rules (DestroyVec).
+typedef std::vector<double>* Vec;
+inline void DestroyVec(Vec& v)
+{
+    delete v;
+    v = nullptr;
+}
+
+template <unsigned DIM>
+class AbstractDynamicLinearPdeSolver {
+public:
+    virtual ~AbstractDynamicLinearPdeSolver() = default;
+    void SetTimes(double tStart, double tEnd)
+    {
+        mTstart = tStart;
+        mTend = tEnd;
+        if (mTstart >= mTend) throw Exception(CMX_E_ARG, "Start time has to be less than end time");
+        mTimesSet = true;
+    }
+    virtual void SetTimeStep(double dt)
+    {
+        if (dt <= 0) throw Exception(CMX_E_ARG, "Time step has to be greater than zero");
+        mIdealTimeStep = dt;
+    }
+    void SetInitialCondition(Vec initialCondition)
+    {
+        if (!initialCondition) throw Exception(CMX_E_ARG, "null initial condition");
+        mInitialCondition = initialCondition;
+    }
+    virtual Vec Solve() = 0;
+
+protected:
+    // AbstractDynamicLinearPdeSolver::Solve's preconditions (:330-343)
+    void CheckSetUp() const
+    {
+        if (!mTimesSet) throw Exception(CMX_E_STATE, "SetTimes() has not been called");
+        if (mIdealTimeStep <= 0.0) throw Exception(CMX_E_STATE, "SetTimeStep() has not been called");
+        if (!mInitialCondition) throw Exception(CMX_E_STATE, "SetInitialCondition() has not been called");
+    }
+    // TimeStepper(tStart, tEnd, dt, enforceConstantTimeStep = true) step count
+    // (global/src/TimeStepper.cpp:43-104): dt must divide the interval.
+    long ConstantSteps() const
+    {
+        const long k = (long)std::floor((mTend - mTstart) / mIdealTimeStep + 0.5);
+        const double eps = mTend > 1.0 ? DBL_EPSILON * mTend : DBL_EPSILON;
+        if (k < 1 || std::fabs(mTend - (mTstart + mIdealTimeStep * k)) > eps)
+            throw Exception(CMX_E_ARG, "TimeStepper estimates non-constant timesteps will need to be used: check "
+                                       "timestep divides (end_time-start_time) (or divides printing timestep)");
+        return k;
+    }
+    double mTstart = 0.0, mTend = 0.0, mIdealTimeStep = -1.0;
+    bool mTimesSet = false;
+    Vec mInitialCondition = nullptr;
+};
+
+// MonodomainSolver (heart/src/solver/electrics/monodomain/MonodomainSolver.cpp)
+// on the device: the tissue's cells and the assembled matrices live in the
+// cmx context; Solve() advances from the initial condition's voltage over
+// [tStart, tEnd] in steps of the PDE time step (cell sweep, ICI right-hand
+// side and CG solve per step) and returns the voltage at tEnd.  The matrix
+// A = (Am Cm / dt) M + K is assembled for the context's PDE time step, so
+// SetTimeStep accepts that step only.  The cell states persist between
+// Solve() calls, as the reference's tissue keeps its cells.  The initial
+// condition is uploaded unless it is the vector the device already holds
+// (the previous Solve()'s result, unchanged).
+template <unsigned DIM>
+class GpuMonodomainSolver : public AbstractDynamicLinearPdeSolver<DIM> {
+public:
+    GpuMonodomainSolver(cmx_ctx* pCtx, unsigned numNodes, double pdeTimeStep)
+        : mpCtx(pCtx), mNumNodes(numNodes), mPdeTimeStep(pdeTimeStep) {}
+    void SetTimeStep(double dt) override
+    {
+        if (dt > 0 && std::fabs(dt - mPdeTimeStep) > 1e-12 * mPdeTimeStep)
+            throw Exception(CMX_E_ARG, "the GPU solver's matrix is assembled for the PDE time step " +
+                                           std::to_string(mPdeTimeStep) + " ms");
+        AbstractDynamicLinearPdeSolver<DIM>::SetTimeStep(dt);
+    }
+    Vec Solve() override
+    {
+        this->CheckSetUp();
+        if (this->mInitialCondition->size() != mNumNodes)
+            throw Exception(CMX_E_ARG, "the initial condition must hold one value per node");
+        const long n_steps = this->ConstantSteps();
+        if (*this->mInitialCondition != mDeviceV) {
+            Check(cmx_set_voltage(mpCtx, this->mInitialCondition->data()), mpCtx);
+            ++mUploads;
+        }
+        Check(cmx_step(mpCtx, this->mTstart, (int32_t)n_steps), mpCtx);
+        Vec out = new std::vector<double>(mNumNodes);
+        try {
+            Check(cmx_get_voltage(mpCtx, out->data()), mpCtx);
+        } catch (...) {
+            DestroyVec(out);
+            throw;
+        }
+        mDeviceV = *out;
+        return out;
+    }
+    // Host-to-device uploads of an initial condition so far.
+    long GetNumberOfUploads() const { return mUploads; }
+    cmx_ctx* GetContext() { return mpCtx; }
+
+private:
+    cmx_ctx* mpCtx;
+    unsigned mNumNodes;
+    double mPdeTimeStep;
+    std::vector<double> mDeviceV;   // the voltage the device holds after the last Solve()
+    long mUploads = 0;
+};
+
+// ---------------------------------------------------------------------------
 // MonodomainProblem
 // ---------------------------------------------------------------------------
 template <unsigned DIM>
@@ -508,32 +626,47 @@ public:
             throw Exception(CMX_E_ARG, "Printing time-step should be a multiple of PDE time step");
         const double t_start = mCurrentTime;
         const double eps_print = end > 1.0 ? DBL_EPSILON * end : DBL_EPSILON;
+        // the solver the problem creates and deletes (AbstractCardiacProblem.cpp:
+        // 389-390, 573); the voltage passes through Vecs as in the reference's loop
+        std::unique_ptr<GpuMonodomainSolver<DIM>> solver(CreateSolver());
+        solver->SetTimeStep(dt);
         cmx_h5* out = OpenResults();
-        std::vector<double> V;
+        Vec initial_condition = nullptr;
         try {
-            if (out) WriteResults(out, t_start, V);
+            initial_condition = new std::vector<double>(GetSolution());
+            if (out && cmx_h5_write_step(out, t_start, initial_condition->data()))
+                throw Exception(CMX_E_ARG, cmx_io_last_error());
             double t = t_start;
             for (long k = 0; t < end; ++k) {
                 double t_next = t_start + (k + 1) * print;   // printing TimeStepper
                 if (end - t_next <= eps_print) t_next = end;
-                // the interval's PDE TimeStepper(t, t_next, dt) with a constant step
-                const long n_pde = (long)std::floor((t_next - t) / dt + 0.5);
-                const double eps_pde = t_next > 1.0 ? DBL_EPSILON * t_next : DBL_EPSILON;
-                if (n_pde < 1 || std::fabs(t_next - (t + dt * n_pde)) > eps_pde)
-                    throw Exception(CMX_E_ARG, "TimeStepper estimates non-constant timesteps will need to be used: "
-                                               "check timestep divides (end_time-start_time) (or divides printing "
-                                               "timestep)");
-                Check(cmx_step(mpCtx, t, (int32_t)n_pde), mpCtx);
+                solver->SetTimes(t, t_next);
+                solver->SetInitialCondition(initial_condition);
+                Vec solution = solver->Solve();
+                DestroyVec(initial_condition);
+                initial_condition = solution;
                 mCurrentTime = t_next;
-                if (out) WriteResults(out, t_next, V);
+                if (out && cmx_h5_write_step(out, t_next, solution->data()))
+                    throw Exception(CMX_E_ARG, cmx_io_last_error());
                 t = t_next;
             }
         } catch (...) {
+            DestroyVec(initial_condition);
             if (out) cmx_h5_close(out);
             throw;
         }
+        DestroyVec(initial_condition);
         if (out && cmx_h5_close(out)) throw Exception(CMX_E_ARG, cmx_io_last_error());
         mCurrentTime = end;
+    }
+
+    // AbstractCardiacProblem::CreateSolver (heart/src/problem/AbstractCardiacProblem.hpp:452,
+    // MonodomainProblem.cpp:52-77): the monodomain PDE solver over this
+    // problem's tissue; the caller owns it.  After Initialise().
+    GpuMonodomainSolver<DIM>* CreateSolver()
+    {
+        if (!mpCtx) throw Exception(CMX_E_STATE, "Initialise() must be called before CreateSolver()");
+        return new GpuMonodomainSolver<DIM>(mpCtx, mpMesh->GetNumNodes(), HeartConfig::Instance()->GetPdeTimeStep());
     }
 
     // Path of the results file, or empty when no output directory is set.
@@ -622,12 +755,6 @@ private:
             throw Exception(CMX_E_ARG, cmx_io_last_error());
         return h;
     }
-    void WriteResults(cmx_h5* h, double t, std::vector<double>& V)
-    {
-        V = GetSolution();
-        if (cmx_h5_write_step(h, t, V.data())) throw Exception(CMX_E_ARG, cmx_io_last_error());
-    }
-
     AbstractCardiacCellFactory<DIM>* mpFactory;
     TetrahedralMesh<DIM>* mpMesh = nullptr;
     TetrahedralMesh<DIM> mOwnMesh;

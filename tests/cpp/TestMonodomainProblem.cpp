@@ -4,6 +4,7 @@
 // (-600*5000 uA/cm^3) and the same golden voltages at nodes 1, 3, 5, 7
 // (atol 5e-3 mV).  Prints one JSON line per case; exit status 0 iff all pass.
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -164,6 +165,76 @@ static void RunShortLastIntervalCase(const std::string& meshes, const std::strin
     std::fflush(stdout);
 }
 
+// The solver contract of AbstractDynamicLinearPdeSolver
+// (pde/src/solver/AbstractDynamicLinearPdeSolver.hpp:160-182, 280-343) on the
+// solver MonodomainProblem::CreateSolver() returns, driven directly on the
+// 1-D mesh-file case: the preconditions throw in the reference's order, the
+// initial condition stays the caller's, Solve() returns a new vector, and two
+// solves (0-1 ms, 1-2 ms) end where the problem's printed run ends (5e-11).
+static void RunSolverContractCase(const std::string& meshes)
+{
+    HeartConfig::Reset();
+    HeartConfig::Instance()->SetIntracellularConductivities({0.0005});
+    HeartConfig::Instance()->SetMeshFileName(meshes + "/1D_0_to_1mm_10_elements");
+    HeartConfig::Instance()->SetSurfaceAreaToVolumeRatio(1.0);
+    HeartConfig::Instance()->SetCapacitance(1.0);
+    PlaneStimulusCellFactory<CellLuoRudy1991FromCellML, 1> cell_factory;
+    MonodomainProblem<1> problem(&cell_factory);
+    problem.Initialise();
+    GpuMonodomainSolver<1>* solver = problem.CreateSolver();
+    std::vector<std::string> messages;
+    auto expect_throw = [&](const std::function<void()>& f) {
+        try { f(); messages.push_back("(no exception)"); }
+        catch (const Exception& e) { messages.push_back(e.what()); }
+    };
+    expect_throw([&] { solver->Solve(); });
+    expect_throw([&] { solver->SetTimes(1.0, 1.0); });
+    solver->SetTimes(0.0, 1.0);
+    expect_throw([&] { solver->Solve(); });
+    expect_throw([&] { solver->SetTimeStep(0.02); });
+    solver->SetTimeStep(0.01);
+    expect_throw([&] { solver->Solve(); });
+    Vec ic = new std::vector<double>(problem.GetSolution());
+    const std::vector<double> ic_copy = *ic;
+    solver->SetInitialCondition(ic);
+    Vec v1 = solver->Solve();
+    solver->SetTimes(1.0, 2.0);
+    solver->SetInitialCondition(v1);
+    Vec v2 = solver->Solve();
+    const long uploads_two_solves = solver->GetNumberOfUploads();
+    double vs_problem = 0;
+    for (size_t i = 0; i < v2->size() && i < g_mesh_file_V.size(); ++i)
+        vs_problem = std::max(vs_problem, std::fabs((*v2)[i] - g_mesh_file_V[i]));
+    solver->SetTimes(2.0, 2.005);
+    expect_throw([&] { solver->Solve(); });
+    Vec shifted = new std::vector<double>(*v2);
+    for (double& v : *shifted) v += 1.0;
+    solver->SetTimes(2.0, 2.1);
+    solver->SetInitialCondition(shifted);
+    Vec v3 = solver->Solve();
+    const long uploads = solver->GetNumberOfUploads();
+    const std::vector<std::string> expected = {"SetTimes() has not been called", "Start time has to be less than end time",
+                                               "SetTimeStep() has not been called", "the GPU solver",
+                                               "SetInitialCondition() has not been called", "TimeStepper estimates"};
+    bool msgs_ok = messages.size() == expected.size();
+    for (size_t q = 0; msgs_ok && q < expected.size(); ++q) msgs_ok = messages[q].find(expected[q]) == 0;
+    const bool ok = msgs_ok && *ic == ic_copy && v1 != ic && v2 != v1 && v3 != v2 && uploads_two_solves == 1 &&
+                    uploads == 2 && g_mesh_file_V.size() == v2->size() && vs_problem <= 5e-11;
+    if (!ok) ++failures;
+    std::printf("{\"case\": \"solver_contract\", \"messages_ok\": %s, \"uploads\": [%ld, %ld], "
+                "\"vs_problem\": %.3e, \"pass\": %s}\n", msgs_ok ? "true" : "false", uploads_two_solves, uploads,
+                vs_problem, ok ? "true" : "false");
+    if (!msgs_ok)
+        for (const std::string& m : messages) std::printf("# %s\n", m.c_str());
+    std::fflush(stdout);
+    DestroyVec(ic);
+    DestroyVec(v1);
+    DestroyVec(v2);
+    DestroyVec(shifted);
+    DestroyVec(v3);
+    delete solver;
+}
+
 // A factory that returns different cell classes per node (the reference's
 // factories may, AbstractCardiacCellFactory.hpp:100): LR91 forward-Euler cells
 // for x < 0.15 cm and TT06 Rush-Larsen cells elsewhere on a 0.3 x 0.1 cm
@@ -273,6 +344,7 @@ int main(int argc, char** argv)
     const bool errors_ok = thrown == 3;
     if (argc > 1) RunMeshFileCase(argv[1], out("MonoProblem1d"));
     if (argc > 1) RunTwoHalvesCase(argv[1]);
+    if (argc > 1) RunSolverContractCase(argv[1]);
     if (argc > 2) RunPrintingTimesCase(argv[1], out("MonoProblem1dRequestedTimes"));
     if (argc > 2) RunShortLastIntervalCase(argv[1], out("MonoProblem1dShortLastInterval"));
     RunParameterCase();

@@ -34,7 +34,8 @@ def test_cpp_monodomain_problem_goldens():
     print(r.stdout)
     cases = [json.loads(l) for l in r.stdout.splitlines() if l.startswith('{')]
     assert {c['case'] for c in cases} == {'1d', '2d', '3d', 'errors', 'mesh_file_1d', 'printing_times',
-                                          'cell_parameter', 'two_halves', 'short_last_interval', 'mixed_cell_types'}
+                                          'cell_parameter', 'two_halves', 'short_last_interval', 'mixed_cell_types',
+                                          'solver_contract'}
     # the mesh-file case wrote its results file: V at t = 0 and every 0.01 ms to 2 ms
     f = h5lite.File(os.path.join(out, 'MonoProblem1d', 'MonodomainLR91_1d.h5'))
     data, times = f.read('Data'), f.read('Time')
