@@ -269,38 +269,49 @@ __device__ __forceinline__ bool integrate_with_voltage(const CellArgs<RS, RV>& a
 // (OperatorSplittingMonodomainSolver, heart/src/solver/electrics/monodomain/
 // OperatorSplittingMonodomainSolver.cpp:40-137) that voltage then runs the
 // previous step's second half-step of the cells.
-template <class Model, class RS, class RV, int IG, int MODE>
+template <class Model, class RS, class RV, int IG, int MODE, bool GROUPED>
 __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
 {
     constexpr bool RL = IG == CMX_RUSH_LARSEN;
     constexpr int NS = Model::NS;
     constexpr int NP = Model::NP > 0 ? Model::NP : 1;
-    // thread j steps cell j of the launch, which sits at node i; with one
-    // cell group (node == null) i = j
+    // Thread j steps cell j of the launch, which sits at node i (GROUPED:
+    // node[j]; one cell group: i = j).  The prologue is branch-free (indices
+    // clamped for the lanes past the last cell, absent vectors read through
+    // a present one and masked) and issues its loads in the order they are
+    // needed -- the reduction, the node-indexed settle vectors, the state
+    // planes, the parameters -- so each wait covers only what it needs.
     const int j = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const bool live = j < a.n_cells;
-    const int i = (live && a.node) ? a.node[j] : j;
+    const int jl = live ? j : a.n_cells - 1;
     PartialLoad<3, kBlock> pl(a.s.pin, a.s.gin);
     const StepClock clk = *a.clock;
     const int done = a.s.ctl->done;
     const double tol2 = a.s.ctl->tol2;
+    const int i = GROUPED ? a.node[jl] : jl;
+    const uint32_t ui = (uint32_t)i;
+    const bool has_x2 = a.s.x2 != nullptr, has_guess = a.s.guess != nullptr, has_prev = a.s.prev != nullptr;
+    const double z = ldu(a.s.Z, ui);
+    const RV d1 = ldu(a.s.x1, ui);
+    const RV d2 = ldu(has_x2 ? a.s.x2 : a.s.x1, ui);
+    const RV g0 = ldu(has_guess ? (const RV*)a.s.guess : a.s.x1, ui);
+    const RV gp = ldu(has_prev ? (const RV*)a.s.prev : a.s.x1, ui);
+    const int sid = (int)ldu(a.stim_id, ui);
     RS y[NS];
     RS p[NP];
-    int sid = 0;
-    double z = 0;
-    RV dx = 0, x0 = 0, xp = 0;
-    if (live) {
+    // state planes through one buffer descriptor: plane s at scalar offset
+    // (s-1) ps, every plane at the thread's one vector offset
+    const __amdgpu_buffer_rsrc_t st_rs = raw_rsrc(a.state, (uint32_t)((NS - 1) * a.ps * sizeof(RS)));
+    const uint32_t vj = (uint32_t)jl * (uint32_t)sizeof(RS);
 #pragma unroll
-        for (int s = 1; s < NS; ++s) y[s] = a.state[(size_t)(s - 1) * a.ps + j];
+    for (int s = 1; s < NS; ++s) y[s] = buf_load<RS>(st_rs, vj, (uint32_t)((s - 1) * a.ps * sizeof(RS)));
 #pragma unroll
-        for (int q = 0; q < NP; ++q)
-            p[q] = (q < a.n_param_planes) ? a.params[(size_t)q * a.ps + j] : (Model::NP > 0 ? a.pdef[q] : RS(0));
-        sid = a.stim_id ? (int)a.stim_id[i] : 0;
-        z = a.s.Z[i];
-        dx = a.s.x1[i] + (a.s.x2 ? a.s.x2[i] : RV(0));
-        if (a.s.guess) x0 = a.s.guess[i];
-        if (a.s.prev) xp = a.s.prev[i];
-    }
+    for (int q = 0; q < NP; ++q)
+        p[q] = (q < a.n_param_planes) ? ldu(a.params + (size_t)q * a.ps, (uint32_t)jl)
+                                      : (Model::NP > 0 ? a.pdef[q] : RS(0));
+    const RV dx = d1 + (has_x2 ? d2 : RV(0));
+    RV x0 = has_guess ? g0 : RV(0);
+    const RV xp = has_prev ? gp : RV(0);
     double g[3];
     pl.total(g);
     if (clk.poison) return;
@@ -331,17 +342,17 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         if (live && a.s.guess) x0 = next_guess(a.s, i, x0 + dx, xp);
         k = clk.k + 1;
     } else if (live) {
-        vn = a.V[i];
+        vn = ldu(a.V, ui);
     }
     if (!live) return;
     if (clk.pending) {
-        if (a.s.guess) a.s.guess[i] = x0;
+        if (a.s.guess) stu(a.s.guess, ui, x0);
         if (!isfinite(vn)) a.clock->nonfinite = 1;
     }
     if constexpr (MODE == kModeSplitFinal) {
-        a.V[i] = vn;
+        stu(a.V, ui, vn);
 #pragma unroll
-        for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.ps + j] = y[s];
+        for (int s = 1; s < NS; ++s) buf_store<RS>(st_rs, vj, (uint32_t)((s - 1) * a.ps * sizeof(RS)), y[s]);
         return;
     }
     const double t = step_time(clk, k), t_next = step_next_time(clk, k);
@@ -352,10 +363,10 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         const double v_start = vn, th = t + 0.5 * (t_next - t);
         if (!integrate_with_voltage<Model, RS, RV, IG>(a, y, p, sid, vn, t, th)) a.clock->nonfinite = 2;
         if (a.s.act) record_crossing(a.s, a.n, i, v_start, vn, t, th);
-        a.V[i] = vn;
+        stu(a.V, ui, vn);
         f = RV(0);
     } else {
-        if (clk.pending) a.V[i] = vn;
+        if (clk.pending) stu(a.V, ui, vn);
         if (a.trace && k >= a.trace_k0 && k < a.trace_k0 + a.trace_len) {
             double* tr = a.trace + (size_t)(k - a.trace_k0) * a.trace_slot;
             tr[i] = vn;
@@ -398,15 +409,15 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         const RS iion = Model::template iion<RS>(y, V, p, cm);
         const double istim = sid ? stimulus_at(a.stim[sid - 1], t_next) : 0.0;
         f = (RV)(-a.am * (double)iion - istim);
-        if (a.iion_out) a.iion_out[i] = (RV)iion;
+        if (a.iion_out) stu(a.iion_out, ui, (RV)iion);
     }
-    a.F[i] = f;
+    stu(a.F, ui, f);
     if (a.s.guess) {
-        a.Y[i] = vn + (double)x0;
-        a.G1[i] = (RV)((double)f - a.c * (double)x0);
+        stu(a.Y, ui, vn + (double)x0);
+        stu(a.G1, ui, (RV)((double)f - a.c * (double)x0));
     }
 #pragma unroll
-    for (int s = 1; s < NS; ++s) a.state[(size_t)(s - 1) * a.ps + j] = y[s];
+    for (int s = 1; s < NS; ++s) buf_store<RS>(st_rs, vj, (uint32_t)((s - 1) * a.ps * sizeof(RS)), y[s]);
 }
 
 // Iion at the stored state (cmx_get_iionic before any step); cell j of the
@@ -1349,12 +1360,19 @@ SettleArgs<RV> settle_args(cmx_ctx* c, const double* verdict)
 }
 
 // ----- typed launches -------------------------------------------------------
+template <class Model, class RS, class RV, int IG, bool GROUPED>
+void launch_cell_grouping(cmx_ctx* c, int mode, int kind, int G, const CellArgs<RS, RV>& a)
+{
+    if (mode == kModeIci) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeIci, GROUPED>), G, kBlock, a);
+    else if (mode == kModeSplit) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeSplit, GROUPED>), G, kBlock, a);
+    else CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeSplitFinal, GROUPED>), G, kBlock, a);
+}
+
 template <class Model, class RS, class RV, int IG>
 void launch_cell_mode(cmx_ctx* c, int mode, int kind, int G, const CellArgs<RS, RV>& a)
 {
-    if (mode == kModeIci) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeIci>), G, kBlock, a);
-    else if (mode == kModeSplit) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeSplit>), G, kBlock, a);
-    else CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeSplitFinal>), G, kBlock, a);
+    if (a.node) launch_cell_grouping<Model, RS, RV, IG, true>(c, mode, kind, G, a);
+    else launch_cell_grouping<Model, RS, RV, IG, false>(c, mode, kind, G, a);
 }
 
 template <class Model, class RS, class RV>

@@ -181,6 +181,27 @@ __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uin
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
 
+template <class T>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, T v);
+template <>
+__device__ __forceinline__ void buf_store<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+}
+template <>
+__device__ __forceinline__ void buf_store<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, double v)
+{
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 0);
+}
+
+// Descriptor over `bytes` bytes from p (planes of one array addressed by a
+// per-thread offset and a per-plane scalar offset).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
+}
+
 // Descriptor of a gathered vector x[-pad .. n+pad) (data format 32-bit, the
 // gfx9 raw-buffer word 3).
 template <class T>
@@ -248,27 +269,37 @@ __device__ __forceinline__ void write_partials(double (&v)[N], double* __restric
 // The G <= kMaxBlocks slots of the previous kernel's reduction, loaded with
 // the kernel's other loads (BS threads, at most kMaxBlocks / BS slots each);
 // total() sums them in a fixed order (identical in every block) and contains
-// barriers.
+// barriers.  The constructor issues the first slot of each thread without a
+// branch (index clamped, the value masked in total()), so the kernel's later
+// loads are issued before anything waits for it; slots past BS (G > BS) are
+// loaded in total().
 template <int N, int BS = kCgBlock>
 struct PartialLoad {
     static constexpr int kTrips = (kMaxBlocks + BS - 1) / BS;
+    const double* __restrict__ in;
+    int G;
     double v[N];
-    __device__ __forceinline__ PartialLoad(const double* __restrict__ in, int G)
+    __device__ __forceinline__ PartialLoad(const double* __restrict__ in_, int G_) : in(in_), G(G_)
     {
+        const int b = (int)threadIdx.x;
+        const int bc = b < G ? b : (G > 0 ? G - 1 : 0);
 #pragma unroll
-        for (int j = 0; j < N; ++j) v[j] = 0.0;
-#pragma unroll
-        for (int t = 0; t < kTrips; ++t) {
-            const int b = (int)threadIdx.x + t * BS;
-            if (b < G)
-#pragma unroll
-                for (int j = 0; j < N; ++j) v[j] += in[(size_t)b * N + j];
-        }
+        for (int j = 0; j < N; ++j) v[j] = in[(size_t)bc * N + j];
     }
     __device__ __forceinline__ void total(double (&tot)[N])
     {
         constexpr int NW = BS / kWave;
         __shared__ double red[NW][N];
+        if ((int)threadIdx.x >= G)
+#pragma unroll
+            for (int j = 0; j < N; ++j) v[j] = 0.0;
+#pragma unroll
+        for (int t = 1; t < kTrips; ++t) {
+            const int b = (int)threadIdx.x + t * BS;
+            if (b < G)
+#pragma unroll
+                for (int j = 0; j < N; ++j) v[j] += in[(size_t)b * N + j];
+        }
         wave_sum<N>(v);
         const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
         if (lane == 0)
@@ -285,6 +316,21 @@ struct PartialLoad {
         __syncthreads();
     }
 };
+
+// Element i of an array with a wave-uniform base, addressed as base + a
+// 32-bit byte offset (arrays below 4 GiB): the compiler then uses the
+// scalar-base + vector-offset form, one offset VGPR shared by every plane,
+// instead of a 64-bit vector address per array.
+template <class T>
+__device__ __forceinline__ T ldu(const T* __restrict__ base, uint32_t i)
+{
+    return *(const T*)((const char*)base + i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void stu(T* __restrict__ base, uint32_t i, T v)
+{
+    *(T*)((char*)base + i * (uint32_t)sizeof(T)) = v;
+}
 
 // SimpleStimulus / RegularStimulus at time t (uA/cm^3); duration already padded.
 __device__ __forceinline__ double stimulus_at(const cmx_stimulus& s, double t)
