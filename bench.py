@@ -358,28 +358,33 @@ def adjoint_case(args):
 
 
 def cells_case(args):
-    """The batched-cell workload: 65,536 independent ten Tusscher 2006 epi
-    cells (Rush-Larsen, dt 0.005 ms, FP32 states with FP64 V), per-cell
-    scalings of g_Na, g_CaL, g_Kr, g_Ks, g_K1, g_to log-uniform in [0.5, 2],
-    cycle lengths uniform in 300-1000 ms with a 2 ms -80 uA/cm^2 stimulus at
-    the start of each cycle, 2-10 cycles per cell (uneven run lengths), all
-    seeded.  Prints one JSON line: cell-steps/s over the launches, and the
-    mixed run's accuracy against the FP64 device run on a seeded subset."""
+    """The batched-cell workload: 65,536 independent O'Hara-Rudy 2011
+    endocardial cells (41 states; Rush-Larsen, dt 0.005 ms, FP32 rates and
+    gates with FP64 V and concentrations), per-cell scalings of the seven
+    conductances G_Na, P_CaL, G_Kr, G_Ks, G_K1, G_to, G_NaL log-uniform in
+    [0.5, 2], cycle lengths uniform in 300-1000 ms with a 2 ms -80 uA/uF
+    stimulus at the start of each cycle, 2-10 cycles per cell (uneven run
+    lengths), all seeded (--cells-model tt06 runs the same sweep on ten
+    Tusscher 2006 cells with its six conductances).  Prints one JSON line:
+    cell-steps/s over the launches, and the mixed run's accuracy against the
+    FP64 device run on a seeded subset."""
     from chaste_amd import cmx
+    model = cmx.ORD_ENDO if args.cells_model == 'ord' else cmx.TT06_EPI
+    name = {cmx.ORD_ENDO: "O'Hara-Rudy 2011 endo (41 states)", cmx.TT06_EPI: 'TT06-epi (19 states)'}[model]
     n = args.cells
     rng = np.random.default_rng(355)
-    defaults = np.array([d for _, d in cmx.param_info(cmx.TT06_EPI)])
+    defaults = np.array([d for _, d in cmx.param_info(model)])
     params = defaults[:, None] * np.exp(rng.uniform(np.log(0.5), np.log(2.0), (len(defaults), n)))
     period = np.round(rng.uniform(300.0, 1000.0, n))
     beats = rng.integers(2, 11, n)
     t_end = period * beats
     stimuli = [(-80.0, 0.0, 2.0, float(p)) for p in period]
-    progress('cells: %d mixed' % n)
-    y, props, st = cmx.cells_run(cmx.TT06_EPI, cmx.RUSH_LARSEN, t_end, 0.005, stimuli=stimuli, params=params,
+    progress('cells: %d %s mixed' % (n, name))
+    y, props, st = cmx.cells_run(model, cmx.RUSH_LARSEN, t_end, 0.005, stimuli=stimuli, params=params,
                                  precision=cmx.MIXED, threshold=-30.0)
     sub = np.sort(rng.choice(n, min(args.cells_check, n), replace=False))
     progress('cells: %d FP64 for the accuracy check' % len(sub))
-    y64, p64, st64 = cmx.cells_run(cmx.TT06_EPI, cmx.RUSH_LARSEN, t_end[sub], 0.005,
+    y64, p64, st64 = cmx.cells_run(model, cmx.RUSH_LARSEN, t_end[sub], 0.005,
                                    stimuli=[stimuli[i] for i in sub], params=params[:, sub], precision=cmx.FP64,
                                    threshold=-30.0)
     steps = np.round(t_end / 0.005)
@@ -389,9 +394,9 @@ def cells_case(args):
         'metric': 'batched single cells: cell ODE steps/s (FP32 rates and gates, FP64 V and concentrations)',
         'value': round(sps, 1), 'unit': 'cell-steps/s', 'higher_is_better': True, 'n_gpus': 1,
         'dtype': 'f32 (f64 V, concentrations)', 'data': 'synthetic (seeded conductance scalings and pacing)',
-        'config': {'workload': '65,536 TT06-epi cells (ORd is absent from the reference), Rush-Larsen dt 0.005 ms, '
-                               '6 conductance scalings log-uniform [0.5, 2], CL uniform 300-1000 ms, 2 ms -80 '
-                               'uA/cm^2 stimulus per cycle, 2-10 cycles per cell', 'n_cells': n},
+        'config': {'workload': '%d %s cells, Rush-Larsen dt 0.005 ms, %d conductance scalings log-uniform '
+                               '[0.5, 2], CL uniform 300-1000 ms, 2 ms -80 uA/uF stimulus per cycle, 2-10 cycles '
+                               'per cell' % (n, name, len(defaults)), 'n_cells': n},
         'kernel_s': round(st['kernel_ms'] * 1e-3, 3), 'cell_steps': st['cell_steps'], 'launches': st['launches'],
         'longest_cell_steps': int(steps.max()),
         'lane_balance': round(float(steps.sum() / (n * steps.max())), 3),
@@ -620,6 +625,7 @@ def main():
     ap.add_argument('--cells', type=int, default=0,
                     help='the batched-cell workload with this many cells instead (65536 for the named sweep)')
     ap.add_argument('--cells-check', type=int, default=256, help='cells also run in FP64 for the accuracy check')
+    ap.add_argument('--cells-model', choices=['ord', 'tt06'], default='ord', help='cell model of the batched-cell sweep')
     ap.add_argument('--shells', action='store_true', help='the batched ellipsoid-shell tissue workload instead')
     ap.add_argument('--shell-steps', type=int, default=10000)
     ap.add_argument('--shell-h', type=float, nargs=2, default=(0.04, 0.02), metavar=('H_MAX', 'H_MIN'),

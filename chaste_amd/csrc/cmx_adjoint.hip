@@ -506,6 +506,8 @@ int cmx_adjoint(cmx_ctx* c, double t0, int32_t n_steps, int32_t checkpoint_every
             if (obs_nodes[o2] == obs_nodes[o]) return afail(c, CMX_E_ARG, "observation nodes must be distinct");
     if (c->precision != CMX_FP64) return afail(c, CMX_E_ARG, "the adjoint needs precision CMX_FP64");
     if (!c->cell_groups.empty()) return afail(c, CMX_E_ARG, "the adjoint needs a single cell type");
+    if (c->model != CMX_MODEL_LR91 && c->model != CMX_MODEL_TT06_EPI)
+        return afail(c, CMX_E_ARG, "the adjoint supports the LR91 and TT06 cells");
     if (c->integrator != CMX_FORWARD_EULER && c->integrator != CMX_RUSH_LARSEN)
         return afail(c, CMX_E_ARG, "the adjoint supports forward-Euler and Rush-Larsen cells");
     if (c->split || c->svi) return afail(c, CMX_E_ARG, "the adjoint supports the ICI step only (no splitting, no SVI)");

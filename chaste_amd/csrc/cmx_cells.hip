@@ -208,8 +208,9 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
     const double* yi = y0 ? y0 : nullptr;
     std::vector<double> def(S);
     if (!yi) {
-        const double* init = c.model == CMX_MODEL_LR91 ? lr91::kInitialState : tt06_epi::kInitialState;
-        std::copy(init, init + S, def.begin());
+        ModelMeta mm{};
+        model_meta(c.model, &mm);
+        std::copy(mm.init, mm.init + S, def.begin());
     }
     if (c.stimuli) s_stim.resize(n);
     for (int j = 0; j < n; ++j) {
@@ -225,8 +226,9 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
         }
     }
     std::vector<RS> pdef(std::max(P, 1), RS(0));
-    const double* pd = c.model == CMX_MODEL_LR91 ? lr91::kParamDefaults : tt06_epi::kParamDefaults;
-    for (int q = 0; q < P; ++q) pdef[q] = (RS)pd[q];
+    ModelMeta mp{};
+    model_meta(c.model, &mp);
+    for (int q = 0; q < P; ++q) pdef[q] = (RS)mp.pdef[q];
 
     Buffers B;
     CellsArgs<RS> a{};
@@ -302,6 +304,9 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
         if (c.model == CMX_MODEL_LR91) {
             if (rl) launch_cells<ModelLr91, RS, CMX_RUSH_LARSEN>(ak, k0, k1, stream);
             else launch_cells<ModelLr91, RS, CMX_FORWARD_EULER>(ak, k0, k1, stream);
+        } else if (c.model == CMX_MODEL_ORD_ENDO) {
+            if (rl) launch_cells<ModelOrd, RS, CMX_RUSH_LARSEN>(ak, k0, k1, stream);
+            else launch_cells<ModelOrd, RS, CMX_FORWARD_EULER>(ak, k0, k1, stream);
         } else {
             if (rl) launch_cells<ModelTt06, RS, CMX_RUSH_LARSEN>(ak, k0, k1, stream);
             else launch_cells<ModelTt06, RS, CMX_FORWARD_EULER>(ak, k0, k1, stream);
@@ -518,8 +523,10 @@ int run_cells_noalloc(const cmx_cells& c, int S, int P, const double* y0, double
     a.last_peak = (double*)(b + L.last_peak);
     a.end = (double*)(b + L.end);
     ModelConsts mc{};
-    const double* init = c.model == CMX_MODEL_LR91 ? lr91::kInitialState : tt06_epi::kInitialState;
-    const double* pd = c.model == CMX_MODEL_LR91 ? lr91::kParamDefaults : tt06_epi::kParamDefaults;
+    ModelMeta mm{};
+    model_meta(c.model, &mm);
+    const double* init = mm.init;
+    const double* pd = mm.pdef;
     for (int s = 0; s < S; ++s) mc.init[s] = init[s];
     for (int q = 0; q < P; ++q) mc.pdef[q] = pd[q];
     const int grid = (n + kBlock - 1) / kBlock;
@@ -532,6 +539,9 @@ int run_cells_noalloc(const cmx_cells& c, int S, int P, const double* y0, double
     if (c.model == CMX_MODEL_LR91) {
         if (rl) launch_cells<ModelLr91, RS, CMX_RUSH_LARSEN>(a, 0, INT32_MAX, stream);
         else launch_cells<ModelLr91, RS, CMX_FORWARD_EULER>(a, 0, INT32_MAX, stream);
+    } else if (c.model == CMX_MODEL_ORD_ENDO) {
+        if (rl) launch_cells<ModelOrd, RS, CMX_RUSH_LARSEN>(a, 0, INT32_MAX, stream);
+        else launch_cells<ModelOrd, RS, CMX_FORWARD_EULER>(a, 0, INT32_MAX, stream);
     } else {
         if (rl) launch_cells<ModelTt06, RS, CMX_RUSH_LARSEN>(a, 0, INT32_MAX, stream);
         else launch_cells<ModelTt06, RS, CMX_FORWARD_EULER>(a, 0, INT32_MAX, stream);
@@ -562,10 +572,9 @@ extern "C" {
 int cmx_cells_run(const cmx_cells* c, const double* y0, double* y_out, double* props, cmx_cells_stats* stats)
 {
     if (!c) return cfail(CMX_E_ARG, "null description");
-    int S, P;
-    if (c->model == CMX_MODEL_LR91) { S = lr91::kNumStates; P = lr91::kNumParams; }
-    else if (c->model == CMX_MODEL_TT06_EPI) { S = tt06_epi::kNumStates; P = tt06_epi::kNumParams; }
-    else return cfail(CMX_E_ARG, "unknown cell model");
+    ModelMeta mm{};
+    if (!model_meta(c->model, &mm)) return cfail(CMX_E_ARG, "unknown cell model");
+    const int S = mm.S, P = mm.P;
     if (c->integrator != CMX_FORWARD_EULER && c->integrator != CMX_RUSH_LARSEN)
         return cfail(CMX_E_ARG, "batched cells support forward-Euler and Rush-Larsen integration");
     if (c->precision != CMX_FP64 && c->precision != CMX_MIXED) return cfail(CMX_E_ARG, "unknown precision");
@@ -581,10 +590,9 @@ int cmx_cells_run(const cmx_cells* c, const double* y0, double* y_out, double* p
 int cmx_cells_run_noalloc(const cmx_cells* c, const double* y0, double* y_out, double* props, void* stream)
 {
     if (!c) return cfail(CMX_E_ARG, "null description");
-    int S, P;
-    if (c->model == CMX_MODEL_LR91) { S = lr91::kNumStates; P = lr91::kNumParams; }
-    else if (c->model == CMX_MODEL_TT06_EPI) { S = tt06_epi::kNumStates; P = tt06_epi::kNumParams; }
-    else return cfail(CMX_E_ARG, "unknown cell model");
+    ModelMeta mm{};
+    if (!model_meta(c->model, &mm)) return cfail(CMX_E_ARG, "unknown cell model");
+    const int S = mm.S, P = mm.P;
     if (S > kMaxModelStates || P > kMaxModelStates) return cfail(CMX_E_ARG, "model too large for the variant");
     if (c->integrator != CMX_FORWARD_EULER && c->integrator != CMX_RUSH_LARSEN)
         return cfail(CMX_E_ARG, "batched cells support forward-Euler and Rush-Larsen integration");

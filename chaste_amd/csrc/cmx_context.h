@@ -11,6 +11,7 @@
 #include "cmx_internal.h"
 #include "cmx_kernels.h"
 #include "models/lr91.h"
+#include "models/ord_endo.h"
 #include "models/tt06_epi.h"
 
 namespace cmx {
@@ -67,6 +68,48 @@ struct ModelTt06 {
     }
     static constexpr bool gate(int i) { return tt06_epi::kIsGate[i] != 0; }
 };
+
+// O'Hara-Rudy 2011 endocardial cells: forward Euler and Rush-Larsen only (no
+// backward-Euler or generalized Rush-Larsen forms).
+struct ModelOrd {
+    static constexpr int NS = ord_endo::kNumStates;
+    static constexpr int NP = ord_endo::kNumParams;
+    template <class R, bool RL>
+    static __device__ __forceinline__ void eval(const R* y, R V, R st, const R* p, R cm, R* dy, R* gi, R* gk)
+    {
+        ord_endo::evaluate<R, RL>(y, V, st, p, cm, dy, gi, gk);
+    }
+    template <class R>
+    static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return ord_endo::iionic<R>(y, V, p, cm); }
+    template <class R>
+    static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return ord_endo::dvdt<R>(y, V, st, p, cm); }
+    static constexpr bool gate(int i) { return ord_endo::kIsGate[i] != 0; }
+};
+
+// Metadata of a model id (CMX_MODEL_*); false for an unknown id.
+struct ModelMeta {
+    int S, P;
+    const double* init;
+    const double* pdef;
+    const char* const* state_names;
+    const char* const* param_names;
+    bool implicit;       // backward-Euler and generalized Rush-Larsen forms available
+};
+inline bool model_meta(int model, ModelMeta* m)
+{
+    if (model == CMX_MODEL_LR91)
+        *m = {lr91::kNumStates, lr91::kNumParams, lr91::kInitialState, lr91::kParamDefaults, lr91::kStateNames,
+              lr91::kParamNames, true};
+    else if (model == CMX_MODEL_TT06_EPI)
+        *m = {tt06_epi::kNumStates, tt06_epi::kNumParams, tt06_epi::kInitialState, tt06_epi::kParamDefaults,
+              tt06_epi::kStateNames, tt06_epi::kParamNames, true};
+    else if (model == CMX_MODEL_ORD_ENDO)
+        *m = {ord_endo::kNumStates, ord_endo::kNumParams, ord_endo::kInitialState, ord_endo::kParamDefaults,
+              ord_endo::kStateNames, ord_endo::kParamNames, false};
+    else
+        return false;
+    return true;
+}
 
 }  // namespace cmx
 

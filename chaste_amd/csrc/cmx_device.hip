@@ -1192,13 +1192,24 @@ int upload(cmx_ctx* c, T** ptr, const T* host, size_t count)
 
 int model_sizes(int model, int* S, int* P)
 {
-    if (model == CMX_MODEL_LR91) { *S = lr91::kNumStates; *P = lr91::kNumParams; return CMX_OK; }
-    if (model == CMX_MODEL_TT06_EPI) { *S = tt06_epi::kNumStates; *P = tt06_epi::kNumParams; return CMX_OK; }
-    return CMX_E_ARG;
+    ModelMeta m;
+    if (!model_meta(model, &m)) return CMX_E_ARG;
+    *S = m.S;
+    *P = m.P;
+    return CMX_OK;
 }
 
-const double* model_init(int model) { return model == CMX_MODEL_LR91 ? lr91::kInitialState : tt06_epi::kInitialState; }
-const double* model_pdef(int model) { return model == CMX_MODEL_LR91 ? lr91::kParamDefaults : tt06_epi::kParamDefaults; }
+const double* model_init(int model) { ModelMeta m{}; model_meta(model, &m); return m.init; }
+const double* model_pdef(int model) { ModelMeta m{}; model_meta(model, &m); return m.pdef; }
+
+// A (model, integrator) pair the cell kernels are built for.
+bool model_supports(int model, int integrator)
+{
+    ModelMeta m;
+    if (!model_meta(model, &m)) return false;
+    return integrator == CMX_FORWARD_EULER || integrator == CMX_RUSH_LARSEN ||
+           (m.implicit && integrator >= CMX_BACKWARD_EULER && integrator <= CMX_GRL2);
+}
 
 int grid_of(int n) { return (n + kBlock - 1) / kBlock; }
 
@@ -1375,20 +1386,25 @@ void launch_cell_mode(cmx_ctx* c, int mode, int kind, int G, const CellArgs<RS, 
     else launch_cell_grouping<Model, RS, RV, IG, false>(c, mode, kind, G, a);
 }
 
-template <class Model, class RS, class RV>
+template <class Model, class RS, class RV, bool IMPLICIT = true>
 void launch_cell_integrator(cmx_ctx* c, int integrator, int mode, int kind, int G, const CellArgs<RS, RV>& a)
 {
     if (integrator == CMX_RUSH_LARSEN) launch_cell_mode<Model, RS, RV, CMX_RUSH_LARSEN>(c, mode, kind, G, a);
-    else if (integrator == CMX_BACKWARD_EULER) launch_cell_mode<Model, RS, RV, CMX_BACKWARD_EULER>(c, mode, kind, G, a);
-    else if (integrator == CMX_GRL1) launch_cell_mode<Model, RS, RV, CMX_GRL1>(c, mode, kind, G, a);
-    else if (integrator == CMX_GRL2) launch_cell_mode<Model, RS, RV, CMX_GRL2>(c, mode, kind, G, a);
-    else launch_cell_mode<Model, RS, RV, CMX_FORWARD_EULER>(c, mode, kind, G, a);
+    else if constexpr (IMPLICIT) {
+        if (integrator == CMX_BACKWARD_EULER) launch_cell_mode<Model, RS, RV, CMX_BACKWARD_EULER>(c, mode, kind, G, a);
+        else if (integrator == CMX_GRL1) launch_cell_mode<Model, RS, RV, CMX_GRL1>(c, mode, kind, G, a);
+        else if (integrator == CMX_GRL2) launch_cell_mode<Model, RS, RV, CMX_GRL2>(c, mode, kind, G, a);
+        else launch_cell_mode<Model, RS, RV, CMX_FORWARD_EULER>(c, mode, kind, G, a);
+    } else {
+        launch_cell_mode<Model, RS, RV, CMX_FORWARD_EULER>(c, mode, kind, G, a);   // create admits FE / RL only
+    }
 }
 
 template <class RS, class RV>
 void launch_cell_model(cmx_ctx* c, int model, int integrator, int mode, int kind, int G, const CellArgs<RS, RV>& a)
 {
     if (model == CMX_MODEL_LR91) launch_cell_integrator<ModelLr91, RS, RV>(c, integrator, mode, kind, G, a);
+    else if (model == CMX_MODEL_ORD_ENDO) launch_cell_integrator<ModelOrd, RS, RV, false>(c, integrator, mode, kind, G, a);
     else launch_cell_integrator<ModelTt06, RS, RV>(c, integrator, mode, kind, G, a);
 }
 
@@ -1476,6 +1492,7 @@ int launch_svi(cmx_ctx* c, const Rows& R)
     a.out = (RV*)c->svi_out;
     const int G = grid_of(c->n);
     if (c->model == CMX_MODEL_LR91) CMX_LAUNCH(c, CMX_K_SVI, (svi_kernel<ModelLr91, RS, RV, Rows>), G, kBlock, R, a);
+    else if (c->model == CMX_MODEL_ORD_ENDO) CMX_LAUNCH(c, CMX_K_SVI, (svi_kernel<ModelOrd, RS, RV, Rows>), G, kBlock, R, a);
     else CMX_LAUNCH(c, CMX_K_SVI, (svi_kernel<ModelTt06, RS, RV, Rows>), G, kBlock, R, a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
@@ -1964,6 +1981,8 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
             if (model_sizes(cg.model, &cg.S, &cg.P)) return fail(c, CMX_E_ARG, "unknown cell model in group_model");
             if (cg.integrator < CMX_FORWARD_EULER || cg.integrator > CMX_GRL2)
                 return fail(c, CMX_E_ARG, "unknown integrator in group_integrator");
+            if (!model_supports(cg.model, cg.integrator))
+                return fail(c, CMX_E_ARG, "this cell model has forward-Euler and Rush-Larsen forms only");
             cg.n = count[g];
             if (cg.n == 0) continue;
             c->S = std::max(c->S, cg.S);
@@ -1991,6 +2010,8 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
         if (model_sizes(tissue->model, &c->S, &c->P)) return fail(c, CMX_E_ARG, "unknown cell model");
         if (tissue->integrator < CMX_FORWARD_EULER || tissue->integrator > CMX_GRL2)
             return fail(c, CMX_E_ARG, "unknown integrator");
+        if (!model_supports(tissue->model, tissue->integrator))
+            return fail(c, CMX_E_ARG, "this cell model has forward-Euler and Rush-Larsen forms only");
         c->model = tissue->model;
         c->integrator = tissue->integrator;
     }
@@ -2243,12 +2264,18 @@ void launch_iionic(cmx_ctx* c, int model, int n_cells, const int32_t* node, int 
         if (model == CMX_MODEL_LR91)
             iionic_kernel<ModelLr91, float><<<G, kBlock, 0, c->stream>>>(n_cells, node, ps, c->V, (const float*)state,
                 (const float*)params, (const float*)pdef, n_param_planes, c->cm, c->iion);
+        else if (model == CMX_MODEL_ORD_ENDO)
+            iionic_kernel<ModelOrd, float><<<G, kBlock, 0, c->stream>>>(n_cells, node, ps, c->V, (const float*)state,
+                (const float*)params, (const float*)pdef, n_param_planes, c->cm, c->iion);
         else
             iionic_kernel<ModelTt06, float><<<G, kBlock, 0, c->stream>>>(n_cells, node, ps, c->V, (const float*)state,
                 (const float*)params, (const float*)pdef, n_param_planes, c->cm, c->iion);
     } else {
         if (model == CMX_MODEL_LR91)
             iionic_kernel<ModelLr91, double><<<G, kBlock, 0, c->stream>>>(n_cells, node, ps, c->V, (const double*)state,
+                (const double*)params, (const double*)pdef, n_param_planes, c->cm, c->iion);
+        else if (model == CMX_MODEL_ORD_ENDO)
+            iionic_kernel<ModelOrd, double><<<G, kBlock, 0, c->stream>>>(n_cells, node, ps, c->V, (const double*)state,
                 (const double*)params, (const double*)pdef, n_param_planes, c->cm, c->iion);
         else
             iionic_kernel<ModelTt06, double><<<G, kBlock, 0, c->stream>>>(n_cells, node, ps, c->V, (const double*)state,
@@ -2643,7 +2670,7 @@ int cmx_model_param(int32_t model, int32_t index, const char** name, double* def
 {
     int S, P;
     if (model_sizes(model, &S, &P) || index < 0 || index >= P) return fail(nullptr, CMX_E_ARG, "bad model/parameter index");
-    if (name) *name = model == CMX_MODEL_LR91 ? lr91::kParamNames[index] : tt06_epi::kParamNames[index];
+    if (name) { ModelMeta m{}; model_meta(model, &m); *name = m.param_names[index]; }
     if (default_value) *default_value = model_pdef(model)[index];
     return CMX_OK;
 }
@@ -2676,7 +2703,9 @@ int cmx_model_state_name(int32_t model, int32_t index, const char** name)
 {
     int S, P;
     if (model_sizes(model, &S, &P) || index < 0 || index >= S || !name) return fail(nullptr, CMX_E_ARG, "bad model/index");
-    *name = model == CMX_MODEL_LR91 ? lr91::kStateNames[index] : tt06_epi::kStateNames[index];
+    ModelMeta m{};
+    model_meta(model, &m);
+    *name = m.state_names[index];
     return CMX_OK;
 }
 

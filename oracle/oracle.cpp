@@ -14,6 +14,7 @@
 #endif
 
 #include "../chaste_amd/csrc/models/lr91.h"
+#include "../chaste_amd/csrc/models/ord_endo.h"
 #include "../chaste_amd/csrc/models/tt06_epi.h"
 
 namespace {
@@ -115,13 +116,24 @@ int num_states(int model)
     case ORC_LR91_HAND:
     case ORC_LR91_CELLML: return cmx::lr91::kNumStates;
     case ORC_TT06_EPI: return cmx::tt06_epi::kNumStates;
+    case ORC_ORD_ENDO: return cmx::ord_endo::kNumStates;
     default: return -1;
     }
 }
 
 const double* initial_state(int model)
 {
+    if (model == ORC_ORD_ENDO) return cmx::ord_endo::kInitialState;
     return model == ORC_TT06_EPI ? cmx::tt06_epi::kInitialState : cmx::lr91::kInitialState;
+}
+
+// The O'Hara-Rudy cell has forward-Euler and Rush-Larsen forms only, the hand
+// LR91 restatement forward Euler only.
+bool integrator_ok(int model, int integ)
+{
+    if (integ < ORC_FE || integ > ORC_GRL2) return false;
+    if (model == ORC_LR91_HAND || model == ORC_ORD_ENDO) return integ == ORC_FE || (model == ORC_ORD_ENDO && integ == ORC_RL);
+    return true;
 }
 
 // Newton failures of backward-Euler substeps (the reference throws).
@@ -170,6 +182,17 @@ double cell_substep(int model, int integ, const double* p, double cm, double* y,
         if (its < 0) ++g_newton_failures;
         return dv;
     }
+    if (model == ORC_ORD_ENDO) {
+        if (update_v) dv = cmx::ord_endo::dvdt<double>(y, y[0], stim_area, p, cm);
+        if (rl) cmx::ord_endo::evaluate<double, true>(y, y[0], stim_area, p, cm, dy, ginf, gk);
+        else cmx::ord_endo::evaluate<double, false>(y, y[0], stim_area, p, cm, dy, ginf, gk);
+        for (int i = 1; i < S; ++i) {
+            if (rl && cmx::ord_endo::kIsGate[i]) y[i] = ginf[i] + (y[i] - ginf[i]) * std::exp(gk[i] * dt);
+            else y[i] += dt * dy[i];
+        }
+        if (update_v) y[0] += dt * dv;
+        return dv;
+    }
     if (model == ORC_LR91_CELLML) {
         if (update_v) dv = cmx::lr91::dvdt<double>(y, y[0], stim_area, p, cm);
         if (rl) cmx::lr91::evaluate<double, true>(y, y[0], stim_area, p, cm, dy, ginf, gk);
@@ -196,17 +219,20 @@ double cell_iionic(int model, const double* p, double cm, const double* y)
     switch (model) {
     case ORC_LR91_HAND: return Lr91Hand::currents(y, nullptr);
     case ORC_LR91_CELLML: return cmx::lr91::iionic<double>(y, y[0], p, cm);
+    case ORC_ORD_ENDO: return cmx::ord_endo::iionic<double>(y, y[0], p, cm);
     default: return cmx::tt06_epi::iionic<double>(y, y[0], p, cm);
     }
 }
 
 const double* default_params(int model)
 {
+    if (model == ORC_ORD_ENDO) return cmx::ord_endo::kParamDefaults;
     return model == ORC_TT06_EPI ? cmx::tt06_epi::kParamDefaults : cmx::lr91::kParamDefaults;
 }
 
 int num_params(int model)
 {
+    if (model == ORC_ORD_ENDO) return cmx::ord_endo::kNumParams;
     return model == ORC_TT06_EPI ? cmx::tt06_epi::kNumParams : cmx::lr91::kNumParams;
 }
 
@@ -330,7 +356,7 @@ int orc_cell_compute(int model, int integrator, const double* params, double* y,
 {
     const int S = num_states(model);
     if (S < 0 || dt <= 0 || t_end <= t0 || steps_per_sample <= 0) return -1;
-    if (integrator < ORC_FE || integrator > ORC_GRL2 || (model == ORC_LR91_HAND && integrator != ORC_FE)) return -1;
+    if (!integrator_ok(model, integrator)) return -1;
     const int fails0 = g_newton_failures.load();
     const double* p = params ? params : default_params(model);
     Stimulus stim(stim_mag, stim_dur, stim_start, stim_period);
@@ -367,7 +393,7 @@ int orc_cell_compute_v(int model, int integrator, const double* params, double* 
     // t = 0, then after every step)
     const int S = num_states(model);
     if (S < 0 || dt <= 0 || t_end <= 0) return -1;
-    if (integrator < ORC_FE || integrator > ORC_GRL2 || (model == ORC_LR91_HAND && integrator != ORC_FE)) return -1;
+    if (!integrator_ok(model, integrator)) return -1;
     const int fails0 = g_newton_failures.load();
     const double* p = params ? params : default_params(model);
     Stimulus stim(stim_mag, stim_dur, stim_start, stim_period);
@@ -393,7 +419,7 @@ double orc_cell_compute_except_voltage(int model, int integrator, const double* 
     // only enters dV/dt, has no effect), then GetIIonic at the result.
     const int S = num_states(model);
     if (S < 0 || dt <= 0 || t_end <= t0) return NAN;
-    if (integrator < ORC_FE || integrator > ORC_GRL2 || (model == ORC_LR91_HAND && integrator != ORC_FE)) return NAN;
+    if (!integrator_ok(model, integrator)) return NAN;
     const int fails0 = g_newton_failures.load();
     const double* p = params ? params : default_params(model);
     const bool fixed_dt = integrator != ORC_FE;
@@ -692,7 +718,7 @@ int orc_mono_run(const orc_mono_cfg* cfg, double* V_out, double* state_out,
     int S = 0;
     for (int i = 0; i < n; ++i) {
         const int Si = num_states(nmodel[i]);
-        if (Si < 0 || ninteg[i] < ORC_FE || ninteg[i] > ORC_GRL2 || (nmodel[i] == ORC_LR91_HAND && ninteg[i] != ORC_FE))
+        if (Si < 0 || !integrator_ok(nmodel[i], ninteg[i]))
             return -1;
         S = std::max(S, Si);
     }
