@@ -224,6 +224,16 @@ struct CellTenTusscher2006EpiFromCellMLRushLarsen {
     static constexpr int kModel = CMX_MODEL_TT06_EPI;
     static constexpr int kIntegrator = CMX_RUSH_LARSEN;
 };
+// O'Hara-Rudy 2011 endocardial cells (not in the reference tree; written from
+// the published equations): forward Euler and Rush-Larsen.
+struct CellOHaraRudy2011EndoFromCellML {
+    static constexpr int kModel = CMX_MODEL_ORD_ENDO;
+    static constexpr int kIntegrator = CMX_FORWARD_EULER;
+};
+struct CellOHaraRudy2011EndoFromCellMLRushLarsen {
+    static constexpr int kModel = CMX_MODEL_ORD_ENDO;
+    static constexpr int kIntegrator = CMX_RUSH_LARSEN;
+};
 // Backward-Euler classes (LuoRudy1991BackwardEuler[Opt].hpp,
 // TenTusscher2006EpiBackwardEuler[Opt].hpp; the Opt variants' lookup tables
 // only change round-off, so both names map to the same cell).
