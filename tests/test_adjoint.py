@@ -215,3 +215,43 @@ def test_adjoint_with_mass_lumping_and_fibre_tensors():
         ad = float((g * d).sum())
         print('direction', k, 'adjoint', ad, 'fd', fd)
         assert abs(ad - fd) <= ADJ_REL * abs(fd)
+
+
+def test_adjoint_config_full_size():
+    """The adjoint workload at its full size: LR91 square of 256 x 256 nodes
+    (h 0.01 cm, forward Euler, FP64, KSP rtol 1e-12), per-node scalings of
+    g_Na and g_Kmax (2 x 65,536 parameters) with a seeded smooth Gaussian
+    random field in [0.6, 1.4] (correlation length 0.1 cm) as the ground
+    truth, 64 seeded observation nodes every 0.1 ms over 20 ms (2000 steps),
+    checkpoints every 100 steps; the gradient at scalings 1 along 8 seeded
+    random directions against Richardson-combined central differences of the
+    device forward run within 1e-4.  A direction whose differences at eps and
+    eps/2 disagree (LR91's h/j gate rates jump at V = -40 mV, so the loss jumps
+    where a perturbation moves a node across it) is differenced at 1e-6."""
+    from chaste_amd.autograd import smooth_random_field
+    print('case adjoint full size 256x256, 2000 steps', flush=True)
+    sq = Square(ex=255, steps=2000, n_obs=64, every=10, seed=2024)
+    assert sq.n == 65536
+    rng = np.random.default_rng(2024)
+    truth = np.stack([smooth_random_field((256, 256), 0.01, 0.1, rng).ravel() for _ in range(2)])
+    assert truth.min() >= 0.6 - 1e-12 and truth.max() <= 1.4 + 1e-12
+    data = sq.observe(truth)
+    s1 = np.ones((2, sq.n))
+    loss, g, _ = sq.adjoint(s1, data, checkpoint_every=100)
+    assert loss > 0
+
+    def loss_at(s):
+        return ((sq.observe(s) - data) ** 2).sum()
+    rel = []
+    for k in range(8):
+        d = rng.standard_normal((2, sq.n))
+        ad = float((g * d).sum())
+        for eps in (1e-4, 1e-6):
+            d1 = (loss_at(s1 + eps * d) - loss_at(s1 - eps * d)) / (2 * eps)
+            d2 = (loss_at(s1 + eps / 2 * d) - loss_at(s1 - eps / 2 * d)) / eps
+            if abs(d1 - d2) <= 1e-5 * abs(d2):
+                break
+        fd = (4 * d2 - d1) / 3
+        rel.append(abs(ad - fd) / abs(fd))
+        print('direction %d: adjoint %.9e  fd %.9e  rel %.2e (eps %g)' % (k, ad, fd, rel[-1], eps), flush=True)
+    assert max(rel) <= ADJ_REL

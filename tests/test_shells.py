@@ -59,12 +59,12 @@ def test_shell_fibres_rotate_through_the_wall():
     assert coords[:, 2].argmin() in np.where(apex)[0]
 
 
-def run_shell_gpu(coords, elems, fibres, apex, steps, precision=0):
+def run_shell_gpu(coords, elems, fibres, apex, steps, precision=0, rtol=None):
     from chaste_amd import cmx
     sig = cmx.conductivity_tensors(fibres, (1.75, 0.19), axisymmetric=True)
     m = cmx.Monodomain(coords, elems, np.array([1.75, 0.19, 0.19]), cmx.TT06_EPI, cmx.RUSH_LARSEN,
                        stimuli=[(-100000.0, 0.0, 1.0)], stim_index=np.where(apex, 0, -1).astype(np.int32),
-                       precision=precision, ksp_rtol=1e-12 if precision == 0 else 1e-8,
+                       precision=precision, ksp_rtol=rtol or (1e-12 if precision == 0 else 1e-8),
                        refine_sweeps=0 if precision == 0 else 1, sigma_elem=sig, use_graph=True)
     m.step(0.0, steps)
     return m, sig
@@ -161,3 +161,61 @@ def test_shell_regional_scaling_gradient():
             fd = (4 * d2 - d1) / 3
             print('param', ids[k], 'region', r, 'adjoint', g[k, r], 'fd', fd)
             assert abs(g[k, r] - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(g).max())
+
+
+@pytest.mark.gpu
+def test_shell_union_at_config_size(oracle):
+    """The batched-tissue configuration at its size: 8 shells at h from 0.04
+    to 0.02 cm (0.5 M - 4 M nodes each, 13 M in all) stepped as one context
+    for 200 steps (FP64, fibre tensors, ELL) against each shell run alone.
+
+    The union's CG stops on the joint preconditioned residual, ||D^-1 r|| <=
+    1e-12 ||D^-1 b||, so a shell's slice of it meets that shell's own test
+    only at rtol 1e-12 sqrt(N / n_k) <= 5e-12 (its share of ||D^-1 b|| is
+    about sqrt(n_k / N), N the union's nodes, n_k the shell's).  The bound
+    each slice is held to is therefore measured, not guessed: the distance
+    between the shell alone at rtol 1e-12 and at rtol 1e-11 (10x looser than
+    the union can be), doubled because CG stops at whole iterations, so the
+    distance is not monotone in the tolerance, and floored at 1e-9 relative
+    to max |V|.  A slice taken from the wrong nodes, or a union assembled
+    wrongly, differs by O(1).  The coarsest shell alone is also checked
+    against the oracle after 100 steps (1e-9)."""
+    from chaste_amd.shells import disjoint_union, ellipsoid_shell
+    print('case shell union at config size', flush=True)
+    hs = np.linspace(0.04, 0.02, 8)
+    meshes = [ellipsoid_shell(h) for h in hs]
+    coords, elems, fibres, apex, off = disjoint_union(meshes)
+    n = coords.shape[0]
+    print('union: %d nodes, %d tetrahedra' % (n, elems.shape[0]), flush=True)
+    assert n > 12_000_000 and all(m[0].shape[0] > 300_000 for m in meshes)
+    mu, _ = run_shell_gpu(coords, elems, fibres, apex, 200)
+    Vu = mu.voltage()
+    mu.close()
+    del coords, elems, fibres, apex
+    rows = []
+    for k, mesh in enumerate(meshes):
+        if k == 0:
+            ms, sig = run_shell_gpu(*mesh, 100)
+            c, e, f, a = mesh
+            r = oracle.mono_run(3, 0, mesh=(c, e), model=oracle.TT06_EPI, rush_larsen=True, sigma=(1.75, 0.19, 0.19),
+                                sigma_elem=sig, stim_mag=-100000.0, stim_dur=1.0, t_end=1.0, ksp_rtol=1e-12,
+                                stim_mask=a.astype(np.uint8))
+            err = np.abs(ms.voltage() - r['V']).max() / np.abs(r['V']).max()
+            print('shell 0 alone vs oracle at 1 ms: %.3e' % err, flush=True)
+            assert err <= 1e-9
+            ms.step(1.0, 100)
+        else:
+            ms, _ = run_shell_gpu(*mesh, 200)
+        Vs = ms.voltage()
+        ms.close()
+        ml, _ = run_shell_gpu(*mesh, 200, rtol=1e-11)
+        Vl = ml.voltage()
+        ml.close()
+        scale = np.abs(Vs).max()
+        d = np.abs(Vu[off[k]:off[k + 1]] - Vs).max() / scale
+        dl = np.abs(Vl - Vs).max() / scale
+        print('shell %d (h %.4f, %d nodes): union vs alone %.3e, alone rtol 1e-11 vs 1e-12 %.3e, V max %.1f' % (
+            k, hs[k], Vs.shape[0], d, dl, Vs.max()), flush=True)
+        rows.append((k, d, dl, Vs.max()))
+    assert all(vmax > 0 for _, _, _, vmax in rows)          # every shell's apical stimulus launched a wave
+    assert all(d <= max(1e-9, 2.0 * dl) for _, d, dl, _ in rows), rows
