@@ -132,7 +132,9 @@ def time_path(m, steps, warmup, roof_steps, events=True):
     ref = (st1['refine_sweeps'] - st0['refine_sweeps']) / steps
     graphs = {'mode': ['plain launches', 'graph replay', 'plain launches (profiler attached)'][st1['graph_mode']],
               'captures_in_timed_region': st1['graph_captures'] - st0['graph_captures'],
-              'replays_in_timed_region': st1['graph_launches'] - st0['graph_launches']}
+              'replays_in_timed_region': st1['graph_launches'] - st0['graph_launches'],
+              'resumes_in_timed_region': st1['resumes'] - st0['resumes'],
+              'folded_refinements_in_timed_region': st1['folded_refinements'] - st0['folded_refinements']}
     if not events:
         return {'elapsed': elapsed, 'its': its, 'refine': ref, 'ranking': {}, 'rank_its': its,
                 'dominant': 'cell', 'dominant_ms': float('nan'), 'dominant_launches': 1, 'roof_its': its,

@@ -46,7 +46,7 @@ class CmxOpts(ctypes.Structure):
                 ('ksp_max_its', ctypes.c_int32), ('refine_sweeps', ctypes.c_int32),
                 ('use_graph', ctypes.c_int32), ('mass_lumping', ctypes.c_int32),
                 ('extrapolate', ctypes.c_int32), ('operator_splitting', ctypes.c_int32),
-                ('state_variable_interpolation', ctypes.c_int32)]
+                ('state_variable_interpolation', ctypes.c_int32), ('separate_refinement', ctypes.c_int32)]
 
 
 class CmxCells(ctypes.Structure):
@@ -68,7 +68,8 @@ class CmxStats(ctypes.Structure):
                 ('row_width', ctypes.c_int32), ('n_patterns', ctypes.c_int32), ('nnz', ctypes.c_int64),
                 ('last_residual', ctypes.c_double), ('device_bytes', ctypes.c_double),
                 ('graph_launches', ctypes.c_int64), ('graph_captures', ctypes.c_int64),
-                ('graph_mode', ctypes.c_int32), ('solver_blocks', ctypes.c_int32)]
+                ('graph_mode', ctypes.c_int32), ('solver_blocks', ctypes.c_int32),
+                ('resumes', ctypes.c_int64), ('folded_refinements', ctypes.c_int64)]
 
 
 # Every symbol include/cmx.h declares, with its ctypes signature.
@@ -317,9 +318,11 @@ class Monodomain:
                  dt_pde=0.01, dt_ode=0.01, am=1400.0, cm=1.0, ksp_rtol=1e-8, ksp_atol=0.0,
                  ksp_max_its=1000, refine_sweeps=0, fmt=FMT_AUTO, mass_lumping=False,
                  sigma_elem=None, use_graph=False, extrapolate=False, operator_splitting=False, svi=False,
-                 cell_types=None):
+                 cell_types=None, separate_refinement=False):
         """cell_types = (group_of [n], [(model, integrator), ...]) gives node i
-        the cell type group_of[i] (model and integrator are then ignored)."""
+        the cell type group_of[i] (model and integrator are then ignored).
+        separate_refinement: mixed precision's FP64 true-residual check always
+        in a kernel of its own instead of in the last CG iteration."""
         L = lib()
         self._keep = []
         coords = np.ascontiguousarray(coords, dtype=np.float64)
@@ -368,7 +371,7 @@ class Monodomain:
             t.n_groups, t.group_of, t.group_model, t.group_integrator = len(gm), _p(go, _ip), _p(gm, _ip), _p(gi, _ip)
         o = CmxOpts(precision, fmt, dt_pde, dt_ode, am, cm, ksp_rtol, ksp_atol, ksp_max_its,
                     refine_sweeps, 1 if use_graph else 0, 1 if mass_lumping else 0, int(extrapolate), 1 if operator_splitting else 0,
-                    1 if svi else 0)
+                    1 if svi else 0, 1 if separate_refinement else 0)
         self.dt = dt_pde
         ctx = ctypes.c_void_p()
         rc = L.cmx_create(ctypes.byref(m), ctypes.byref(t), ctypes.byref(o), ctypes.byref(ctx))

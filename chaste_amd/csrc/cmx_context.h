@@ -148,7 +148,7 @@ struct cmx_ctx {
     uint8_t* stim_id = nullptr;
     cmx_stimulus* stim = nullptr;
     void* F = nullptr;
-    void* vec[11] = {};          // r w z s p m_a m_b x1 x2 x0 x_{n-2}
+    void* vec[15] = {};          // r w z s p m_a m_b x1 x2 x0 x_{n-2} r2 p2 xf u
     double* Y = nullptr;         // V^n + x0 (initial guess)
     void* G1 = nullptr;          // F - c x0
     double* r0_64 = nullptr;     // FP64 step residual (refinement)
@@ -197,8 +197,9 @@ struct cmx_ctx {
     // CG iteration budgets (first solve, refinement solve) per enqueued step
     int budget = 8;
     int refine_budget = 1;
+    bool fold_refine = true;     // the refinement residual in the last iteration when it can be
     // stats
-    long long steps = 0, cg_total = 0, refine_total = 0, resumes = 0;
+    long long steps = 0, cg_total = 0, refine_total = 0, resumes = 0, folded_total = 0;
     int last_its = 0;
     double last_residual = 0;
     double device_bytes = 0;

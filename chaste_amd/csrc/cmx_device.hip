@@ -11,22 +11,26 @@
 //                A = (Am Cm/dt) M + K, evaluated as r0 = M F - K V^n (the
 //                Am Cm V/dt terms cancel exactly), plus ||D^-1 b|| for the test
 //                (MonodomainSolver.cpp:107 MatMult, LinearSystem::Solve)
-//   cg_init      w0 = A D^-1 r0 and the first dot products
+//   cg_start     w0 = A D^-1 r0 and the first dot products
 //   cg_iter x B  Jacobi-preconditioned CG on A d = r0 from d = 0, which is the
 //                reference's warm start from V^n (KSPSetInitialGuessNonzero,
-//                linalg/src/LinearSystem.cpp:833-837).  Chronopoulos-Gear
-//                form: one kernel and one reduction per iteration; stops when
-//                ||D^-1 r|| <= max(rtol ||D^-1 b||, atol) (PETSc's default
-//                preconditioned-norm test); iterations past convergence are
-//                no-ops
-//   finish       V^{n+1} = V^n + d in FP64, advance the device step clock.
-// CMX_MIXED keeps cell states, F and the CG vectors in FP32 and adds an FP64
-// refinement sweep (true residual of V^n + d, second CG solve); V, residuals,
-// dot products and the voltage update are FP64 throughout.
+//                linalg/src/LinearSystem.cpp:833-837).  Pipelined form
+//                (Ghysels & Vanroose): one kernel and one reduction per
+//                iteration; stops when ||D^-1 r|| <= max(rtol ||D^-1 b||, atol)
+//                (PETSc's default preconditioned-norm test); iterations past
+//                convergence are no-ops
+// and the next step's cell kernel (or a batch's finish kernel) settles it:
+// the last reduction decides the solve, V^{n+1} = V^n + d in FP64, and the
+// device step clock advances.  CMX_MIXED keeps cell states, F and the CG
+// vectors in FP32 and adds an FP64 refinement (true residual of V^n + d, a
+// second CG solve when that fails the test); V, residuals, dot products and
+// the voltage update are FP64 throughout.  The true residual is computed by
+// the first solve's last budgeted iteration (cg_last) when the refinement
+// solve has no budget, else by a kernel of its own (refine_residual).
 //
 // Steps are enqueued in batches without host synchronisation.  If a step's
 // final solve is still running when its iteration budget B is used up, the
-// finish kernel poisons the clock, every later kernel returns at once, and
+// settling kernel poisons the clock, every later kernel returns at once, and
 // the host resumes that solve where it stopped (all its buffers are intact),
 // then re-enqueues the rest of the batch.
 #include <hip/hip_ext.h>
@@ -52,6 +56,8 @@ namespace cmx {
 template <class RV>
 struct SettleArgs {
     SolveCtl* ctl;                     // final solve of the pending step
+    SolveCtl* ctl0;                    // first solve, when the reduction's slot 1 tells whether its
+                                       // recursive residual converged (refinement folded in); or null
     const double* __restrict__ pin;    // the last reduction of that solve
     int gin;                           // its slot count
     const double* __restrict__ Z;      // V^n, or V^n + x0 with an initial guess
@@ -288,6 +294,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
     const StepClock clk = *a.clock;
     const int done = a.s.ctl->done;
     const double tol2 = a.s.ctl->tol2;
+    const int done0 = (a.s.ctl0 ? a.s.ctl0 : a.s.ctl)->done;
     const int i = GROUPED ? a.node[jl] : jl;
     const uint32_t ui = (uint32_t)i;
     const bool has_x2 = a.s.x2 != nullptr, has_guess = a.s.guess != nullptr, has_prev = a.s.prev != nullptr;
@@ -327,6 +334,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
             }
             return;
         }
+        if (lead && a.s.ctl0 && done0 == 0 && g[1] <= tol2) a.s.ctl0->done = 1;
         vn = z + (double)dx;
         double v0 = z - (double)x0, t0 = step_time(clk, clk.k);
         if constexpr (MODE != kModeIci) {
@@ -477,6 +485,7 @@ struct CgVecs {
     RV* m_in;    // D^-1 w of the previous iteration (gathered)
     RV* m_out;   // D^-1 w after this iteration
     RV* x;       // correction being solved for
+    RV* u_out = nullptr;   // D^-1 r after this iteration (for the folded refinement that follows), or null
 };
 
 // Every solver kernel below issues all of its global loads (tables, the
@@ -926,6 +935,7 @@ struct IterRow {
         v.r[i] = rn;
         v.w[i] = wn;
         v.m_out[i] = d * wn;
+        if (v.u_out) v.u_out[i] = un;
         part[0] += (double)rn * (double)un;
         part[1] += (double)wn * (double)un;
         part[2] += (double)un * (double)un;
@@ -996,6 +1006,172 @@ __global__ __launch_bounds__(kCgBlock) void cg_iter_kernel(Rows R, MatVals mv, C
     write_partials<kCgBlock, 3>(part, a.pout, blk);
 }
 
+// The last budgeted iteration of a refined step's first solve with the FP64
+// refinement residual folded in (FP32 vectors).  It makes the iteration's
+// x update -- or none, if the solve has already ended -- and in the same
+// pass the true residual of the result, r = r0 - A x (A and r0 in FP64),
+// which is what decides the step, as the separate refinement residual kernel
+// does.  A row needs its neighbours' new x: they are recomputed from values
+// this kernel does not write (x, u = D^-1 r from the previous iteration, p),
+//   x_new = x + alpha (u + beta p)      (x_new = alpha p at iteration 0),
+// with the same rounded operations as the row's own update, so the residual
+// is that of exactly the x that is stored.  The result goes to xf (the first
+// solve's answer); r, D^-1 r and x = 0 go to the refinement solve's vectors,
+// for the host to resume with if the residual fails the test.  Partials: (0,
+// ||D^-1 r_rec||^2 of the recursive residual after the update (statistics),
+// ||D^-1 r||^2 of the true one).
+template <class RV>
+struct LastArgs {
+    CgArgs<RV> cg;                     // this iteration (cg.pin: the reduction it reads)
+    const double* __restrict__ r0_64;  // the step's FP64 residual r0 = b - A Z
+    const RV* __restrict__ u_in;       // D^-1 r after the previous iteration (it > 0)
+    RV* __restrict__ xf;               // the first solve's final x
+    SolveCtl* ctl1;                    // refinement solve (reset here)
+    CgVecs<RV> v1;                     // refinement solve: r, p = D^-1 r, x = 0
+};
+
+__device__ __forceinline__ float fma_rn(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_rn(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// x + alpha (u + beta p) with its two roundings fixed (no contraction choice
+// left to the compiler), so every thread that forms it gets the same bits.
+template <class RV>
+__device__ __forceinline__ RV last_x(RV x, RV u, RV p, RV al, RV be)
+{
+    return fma_rn(al, fma_rn(be, p, u), x);
+}
+
+template <class Rows, class RV>
+struct LastRow {
+    typename Rows::Row row;
+    int i;
+    double r0;
+    RV r, w, s, x, u, p;
+    Gathered<Rows, RV> X, U, P;
+    __device__ __forceinline__ void load(const Rows& R, int i_, const LastArgs<RV>& a, bool first_it)
+    {
+        i = i_;
+        row = R.row(i);
+        r0 = a.r0_64[i];
+        const CgVecs<RV>& v = a.cg.v;
+        P.load(R, row, v.p);
+        r = v.r[i];
+        w = v.w[i];
+        p = v.p[i];
+        if (!first_it) {
+            X.load(R, row, v.x);
+            U.load(R, row, a.u_in);
+            s = v.s[i];
+            x = v.x[i];
+            u = a.u_in[i];
+        }
+    }
+    // upd: this launch makes the update (otherwise x is final as it stands)
+    __device__ __forceinline__ void apply(const Rows& R, const double* __restrict__ A64, const RV* __restrict__ D,
+                                          const LastArgs<RV>& a, bool first_it, bool upd, RV al, RV be,
+                                          double (&part)[3]) const
+    {
+        const CgVecs<RV>& v = a.cg.v;
+        double ax = 0;
+#pragma unroll
+        for (int k = 0; k < R.width(); ++k) {
+            RV xj;
+            if (first_it) {
+                xj = upd ? al * P.at(R, row, v.p, k) : RV(0);
+            } else {
+                const RV xo = X.at(R, row, v.x, k);
+                xj = upd ? last_x(xo, U.at(R, row, a.u_in, k), P.at(R, row, v.p, k), al, be) : xo;
+            }
+            ax += A64[R.vidx(row, k)] * (double)xj;
+        }
+        const RV d = D[R.didx(row)];
+        const RV xn = first_it ? (upd ? al * p : RV(0)) : (upd ? last_x(x, u, p, al, be) : x);
+        const RV rt = (RV)(r0 - ax);
+        const RV ut = d * rt;
+        a.xf[i] = xn;
+        a.v1.r[i] = rt;
+        a.v1.p[i] = ut;
+        a.v1.x[i] = RV(0);
+        part[2] += (double)ut * (double)ut;
+        if (upd) {
+            const RV sn = first_it ? w : w + be * s;
+            const RV un = d * (r - al * sn);
+            part[1] += (double)un * (double)un;
+        }
+    }
+};
+
+// FIRST (it == 0) gathers p only and issues its first row with the
+// reduction; later iterations gather x, u and p, after the reduction (held
+// across it, three gathers would not fit the register budget).
+template <class Rows, class RV, bool FIRST>
+__global__ __launch_bounds__(kCgBlock) void cg_last_kernel(Rows R, MatVals mv, LastArgs<RV> a, int it)
+{
+    __shared__ double sA[kMaxPatterns * kMaxStencil];
+    __shared__ RV sD[kMaxPatterns];
+    const TableLoad<Rows, double> ta(mv.a64, R.value_count());
+    const TableLoad<Rows, RV> td(dinv_vals<RV>(mv), R.diag_count());
+    PartialLoad<3> pl(a.cg.pin, gridDim.x);
+    SolveCtl* ctl = a.cg.ctl;
+    const int poison = a.cg.clock->poison;
+    const int done = ctl->done;
+    const double tol2 = ctl->tol2;
+    const int max_its = ctl->max_its;
+    constexpr bool first_it = FIRST;
+    const double2 h = a.cg.hist[first_it ? 0 : it - 1];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const RowRange rr(a.cg.n, blk, gridDim.x);
+    const int i0 = rr.lo + threadIdx.x;
+    // every outcome reads the row (the update's inputs are a superset of the
+    // final-x case's)
+    LastRow<Rows, RV> q0;
+    if (FIRST && i0 < rr.hi) q0.load(R, i0, a, first_it);
+    const double* __restrict__ A64 = ta.store(sA, mv.a64);
+    const RV* __restrict__ D = td.store(sD, dinv_vals<RV>(mv));
+    double g[3];
+    pl.total(g);    // its barriers also cover the table stores
+    if (poison) return;
+    const double gamma = g[0], delta = g[1], zeta = g[2];
+    const bool lead = blk == 0 && threadIdx.x == 0;
+    bool upd = false;
+    double alpha = 0.0, beta = 0.0;
+    if (done == 0) {
+        if (!(zeta == zeta) || !(gamma == gamma) || !(delta == delta) || fabs(zeta) > DBL_MAX) {
+            if (lead) { ctl->done = 3; ctl->it = it; ctl->zz = zeta; }
+        } else if (zeta <= tol2) {
+            if (lead) { ctl->done = 1; ctl->it = it; ctl->zz = zeta; }
+        } else if (it >= max_its) {
+            if (lead) { ctl->done = 2; ctl->it = it; ctl->zz = zeta; }
+        } else {
+            upd = true;
+            if (first_it) {
+                alpha = gamma / delta;
+            } else {
+                beta = gamma / h.x;
+                alpha = gamma / (delta - beta * gamma / h.y);
+            }
+            if (lead) {
+                a.cg.hist[it] = make_double2(gamma, alpha);
+                ctl->it = it + 1;
+                ctl->zz = zeta;
+            }
+        }
+    }
+    if (lead) {
+        a.ctl1->done = 0;
+        a.ctl1->it = 0;
+    }
+    const RV al = (RV)alpha, be = (RV)beta;
+    double part[3] = {0.0, 0.0, 0.0};
+    if (FIRST && i0 < rr.hi) q0.apply(R, A64, D, a, first_it, upd, al, be, part);
+    for (int i = FIRST ? i0 + kCgBlock : i0; i < rr.hi; i += kCgBlock) {
+        LastRow<Rows, RV> q;
+        q.load(R, i, a, first_it);
+        q.apply(R, A64, D, a, first_it, upd, al, be, part);
+    }
+    write_partials<kCgBlock, 3>(part, a.cg.pout, blk);
+}
+
 // Start of a solve A x = b from x = 0 outside a step (the adjoint's solves):
 // stands in for the residual kernel, with r0 = b, u0 = D^-1 b to p, x = 0 and
 // the reductions cg_start_kernel reads (||D^-1 b||^2, 0, ||u0||^2); it also
@@ -1057,6 +1233,7 @@ __global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
     const int k = clock->k;
     const int done = a.s.ctl->done;
     const double tol2 = a.s.ctl->tol2;
+    const int done0 = (a.s.ctl0 ? a.s.ctl0 : a.s.ctl)->done;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const RowRange rr(a.n, blk, gridDim.x);
     const int i0 = rr.lo + threadIdx.x, i1 = i0 + kCgBlock;
@@ -1082,6 +1259,7 @@ __global__ __launch_bounds__(kCgBlock) void finish_kernel(FinishArgs<RV> a)
         }
         return;
     }
+    if (lead && a.s.ctl0 && done0 == 0 && g[1] <= tol2) a.s.ctl0->done = 1;
     bool bad = false;
     if (i0 < rr.hi) {
         const double v = z0 + (double)d0;
@@ -1296,23 +1474,29 @@ MatVals mat_vals(cmx_ctx* c)
 }
 
 // Partial-sum buffers: kResPart for the residual kernels, kIterPart + (it & 1)
-// for the reduction that feeds iteration it of a solve.
+// for the reduction that feeds iteration it of a solve, kFoldPart for the
+// folded last iteration (read by the settling kernel, and by the refinement
+// solve's start when the host resumes it).
 constexpr int kIterPart = 0;
 constexpr int kResPart = 2;
+constexpr int kFoldPart = 3;
+constexpr int kPartBufs = 4;
 double* pbuf(cmx_ctx* c, int k) { return c->partials + (size_t)k * kMaxBlocks * 3; }
 
 // Vector slots of cmx_ctx::vec.
-enum { V_R = 0, V_W, V_Z, V_S, V_P, V_MA, V_MB, V_X1, V_X2, V_X0, V_XP, V_COUNT };
+// V_R2 / V_P2: the refinement solve's r and p; V_XF: the first solve's answer
+// and V_U: D^-1 r for the last iteration when the refinement is folded into it.
+enum { V_R = 0, V_W, V_Z, V_S, V_P, V_MA, V_MB, V_X1, V_X2, V_X0, V_XP, V_R2, V_P2, V_XF, V_U, V_COUNT };
 
 template <class RV>
 CgVecs<RV> cg_vecs(cmx_ctx* c, int solve, int it)
 {
     CgVecs<RV> v;
-    v.r = (RV*)c->vec[V_R];
+    v.r = (RV*)c->vec[solve ? V_R2 : V_R];
     v.w = (RV*)c->vec[V_W];
     v.z = (RV*)c->vec[V_Z];
     v.s = (RV*)c->vec[V_S];
-    v.p = (RV*)c->vec[V_P];
+    v.p = (RV*)c->vec[solve ? V_P2 : V_P];
     // iteration it reads the m written by iteration it-1 (the start kernel
     // writes it for iteration 0)
     v.m_in = (RV*)c->vec[(it & 1) ? V_MB : V_MA];
@@ -1341,26 +1525,37 @@ CgArgs<RV> cg_args(cmx_ctx* c, int solve)
 
 bool refined_of(const cmx_ctx* c) { return c->precision == CMX_MIXED && c->refine > 0; }
 
+// Steps at the current budgets fold the refinement residual into the first
+// solve's last iteration (cg_last_kernel): refined, the refinement solve
+// without a budget of its own, and at least one iteration budgeted.
+bool fused_of(const cmx_ctx* c) { return refined_of(c) && c->fold_refine && c->refine_budget == 0 && c->budget >= 1; }
+
 // The reduction that decides a step's final solve at the current budgets: the
 // last iteration's (or the start kernel's when the budget is 0), or the
-// refinement residual's when the refinement solve has no budget.
+// refinement residual's when the refinement solve has no budget -- written by
+// the last iteration itself when the refinement is folded in.
 const double* verdict_buf(cmx_ctx* c)
 {
+    if (fused_of(c)) return pbuf(c, kFoldPart);
     if (refined_of(c))
         return c->refine_budget > 0 ? pbuf(c, kIterPart + (c->refine_budget & 1)) : pbuf(c, kResPart);
     return pbuf(c, kIterPart + (c->budget & 1));
 }
 
+// fused: the pending step folded its refinement residual into the first
+// solve's last iteration (its answer is in V_XF); rec0: the verdict is that
+// iteration's reduction, whose slot 1 tells whether the first solve converged.
 template <class RV>
-SettleArgs<RV> settle_args(cmx_ctx* c, const double* verdict)
+SettleArgs<RV> settle_args(cmx_ctx* c, const double* verdict, bool fused, bool rec0)
 {
     const bool refined = refined_of(c);
     SettleArgs<RV> s;
     s.ctl = c->ctl + (refined ? 1 : 0);
+    s.ctl0 = fused && rec0 ? c->ctl : nullptr;
     s.pin = verdict;
     s.gin = c->G;
     s.Z = c->extrapolate ? c->Y : c->V;
-    s.x1 = (const RV*)c->vec[V_X1];
+    s.x1 = (const RV*)c->vec[fused ? V_XF : V_X1];
     s.x2 = refined ? (const RV*)c->vec[V_X2] : nullptr;
     s.guess = c->extrapolate ? (RV*)c->vec[V_X0] : nullptr;
     s.prev = c->extrapolate >= 2 ? (RV*)c->vec[V_XP] : nullptr;
@@ -1411,7 +1606,7 @@ void launch_cell_model(cmx_ctx* c, int model, int integrator, int mode, int kind
 // The cell kernel in one of its roles; verdict: the reduction that decides
 // the pending step's final solve.
 template <class RS, class RV>
-int launch_cells(cmx_ctx* c, int mode, const double* verdict, int kind)
+int launch_cells(cmx_ctx* c, int mode, const double* verdict, int kind, bool fused, bool rec0)
 {
     CellArgs<RS, RV> a;
     a.n = c->n;
@@ -1433,7 +1628,7 @@ int launch_cells(cmx_ctx* c, int mode, const double* verdict, int kind)
     a.stim = c->stim;
     a.F = (RV*)c->F;
     a.iion_out = (c->svi && mode == kModeIci) ? (RV*)c->svi_iion : nullptr;
-    a.s = settle_args<RV>(c, verdict);
+    a.s = settle_args<RV>(c, verdict, fused, rec0);
     a.c = c->am * c->cm / c->dt;
     a.Y = c->Y;
     a.G1 = (RV*)c->G1;
@@ -1530,19 +1725,21 @@ int launch_residual(cmx_ctx* c, const Rows& R, int solve)
 // Start of a solve (w0 = A u0 and its dot products; ends a solve whose
 // residual is already below the threshold).
 template <class Rows, class RV>
-int launch_cg_start(cmx_ctx* c, const Rows& R, int solve)
+int launch_cg_start(cmx_ctx* c, const Rows& R, int solve, const double* pin = nullptr)
 {
     CgArgs<RV> a = cg_args<RV>(c, solve);
     a.first = solve == 0 ? 1 : 0;
+    if (pin) a.pin = pin;   // the reduction holding ||D^-1 r0||^2 of this solve
     a.v.m_out = (RV*)c->vec[V_MA];   // read by iteration 0
     CMX_LAUNCH(c, CMX_K_SPMV, (cg_start_kernel<Rows, RV>), c->G, kCgBlock, R, mat_vals(c), a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
 
-// CG iterations from .. from+count-1 of a solve, one kernel each.
+// CG iterations from .. from+count-1 of a solve, one kernel each; u_last:
+// the last of them also stores D^-1 r for a folded last iteration.
 template <class Rows, class RV>
-int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, int from, int count)
+int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, int from, int count, bool u_last = false)
 {
     CgArgs<RV> a = cg_args<RV>(c, solve);
     const MatVals mv = mat_vals(c);
@@ -1550,20 +1747,46 @@ int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, int from, int count)
         a.pin = pbuf(c, kIterPart + (it & 1));
         a.pout = pbuf(c, kIterPart + ((it + 1) & 1));
         a.v = cg_vecs<RV>(c, solve, it);
+        if (u_last && it == from + count - 1) a.v.u_out = (RV*)c->vec[V_U];
         CMX_LAUNCH(c, CMX_K_UPDATE, (cg_iter_kernel<Rows, RV>), c->G, kCgBlock, R, mv, a, it);
     }
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
 
+// Iteration `it` of the first solve as its last, with the refinement
+// residual folded in (FP32 vectors only).
+template <class Rows, class RV>
+int launch_cg_last(cmx_ctx* c, const Rows& R, int it)
+{
+    if constexpr (std::is_same<RV, float>::value) {
+        LastArgs<RV> a;
+        a.cg = cg_args<RV>(c, 0);
+        a.cg.pin = pbuf(c, kIterPart + (it & 1));
+        a.cg.pout = pbuf(c, kFoldPart);
+        a.cg.v = cg_vecs<RV>(c, 0, it);
+        a.r0_64 = c->r0_64;
+        a.u_in = (const RV*)c->vec[V_U];
+        a.xf = (RV*)c->vec[V_XF];
+        a.ctl1 = c->ctl + 1;
+        a.v1 = cg_vecs<RV>(c, 1, 0);
+        if (it == 0) CMX_LAUNCH(c, CMX_K_UPDATE, (cg_last_kernel<Rows, RV, true>), c->G, kCgBlock, R, mat_vals(c), a, it);
+        else CMX_LAUNCH(c, CMX_K_UPDATE, (cg_last_kernel<Rows, RV, false>), c->G, kCgBlock, R, mat_vals(c), a, it);
+        CMX_HIP(c, hipGetLastError());
+        return CMX_OK;
+    } else {
+        return fail(c, CMX_E_STATE, "the folded refinement runs on FP32 vectors only");
+    }
+}
+
 // Settles the pending step; verdict: the reduction that decides its final
 // solve.
 template <class RS, class RV>
-int launch_finish(cmx_ctx* c, const double* verdict)
+int launch_finish(cmx_ctx* c, const double* verdict, bool fused, bool rec0)
 {
     if (c->split) {
         // operator splitting: the last half-step of the cells completes the step
-        int rc = launch_cells<RS, RV>(c, kModeSplitFinal, verdict, CMX_K_FINISH);
+        int rc = launch_cells<RS, RV>(c, kModeSplitFinal, verdict, CMX_K_FINISH, fused, rec0);
         if (rc) return rc;
         advance_kernel<<<1, 1, 0, c->stream>>>(c->clock, c->ctl, refined_of(c) ? 1 : 0, c->its_log, c->batch);
         CMX_HIP(c, hipGetLastError());
@@ -1577,7 +1800,7 @@ int launch_finish(cmx_ctx* c, const double* verdict)
     a.its_log = c->its_log;
     a.log_len = c->batch;
     a.V = c->V;
-    a.s = settle_args<RV>(c, verdict);
+    a.s = settle_args<RV>(c, verdict, fused, rec0);
     CMX_LAUNCH(c, CMX_K_FINISH, (finish_kernel<RV>), c->G, kCgBlock, a);
     advance_kernel<<<1, 1, 0, c->stream>>>(c->clock, c->ctl, a.refined, c->its_log, c->batch);
     CMX_HIP(c, hipGetLastError());
@@ -1586,19 +1809,26 @@ int launch_finish(cmx_ctx* c, const double* verdict)
 
 // Enqueue one whole step with iteration budgets B (first solve) and B2
 // (refinement solve): cells (which settle the previous step), residual,
-// start, B iterations, [refinement residual, (start, B2 iterations)].  The
-// step stays pending until the next step's cell kernel or the batch's finish
-// kernel settles it.  A solve that needs more than its budget is either
-// continued by the refinement solve (first solve) or resumed by the host
-// after the settling kernel poisons the clock.
+// start, B iterations, [refinement residual, (start, B2 iterations)].  With
+// B >= 1 and B2 = 0 the refinement residual is folded into the B-th
+// iteration.  The step stays pending until the next step's cell kernel or the
+// batch's finish kernel settles it.  A solve that needs more than its budget
+// is either continued by the refinement solve (first solve) or resumed by the
+// host after the settling kernel poisons the clock.
 template <class Rows, class RS, class RV>
 int enqueue_step(cmx_ctx* c, const Rows& R)
 {
     int rc;
-    if ((rc = launch_cells<RS, RV>(c, c->split ? kModeSplit : kModeIci, verdict_buf(c), CMX_K_CELL))) return rc;
+    const bool fused = fused_of(c);
+    if ((rc = launch_cells<RS, RV>(c, c->split ? kModeSplit : kModeIci, verdict_buf(c), CMX_K_CELL, fused, fused)))
+        return rc;
     if (c->svi && !c->split && (rc = launch_svi<Rows, RS, RV>(c, R))) return rc;
     if ((rc = launch_residual<Rows, RV>(c, R, 0))) return rc;
     if ((rc = launch_cg_start<Rows, RV>(c, R, 0))) return rc;
+    if (fused) {
+        if ((rc = launch_cg_iters<Rows, RV>(c, R, 0, 0, c->budget - 1, true))) return rc;
+        return launch_cg_last<Rows, RV>(c, R, c->budget - 1);
+    }
     if ((rc = launch_cg_iters<Rows, RV>(c, R, 0, 0, c->budget))) return rc;
     if (refined_of(c)) {
         // FP64 refinement: true residual of V^n + x1 against the step's b,
@@ -1663,6 +1893,9 @@ hipGraphExec_t step_graph(cmx_ctx* c, const Rows& R, int steps)
         (void)hipGraphDestroy(graph);
         return nullptr;
     }
+    // the executable's one-time device set-up now, on the stream, rather
+    // than in its first replay (a primed graph's first replay is often timed)
+    if (hipGraphUpload(exec, c->stream) != hipSuccess) (void)hipGetLastError();
     c->graphs.push_back({c->budget, c->refine_budget, steps, c->G, c->act_every, c->act_thr, graph, exec});
     ++c->graph_captures;
     return exec;
@@ -1715,9 +1948,10 @@ int run_steps(cmx_ctx* c, const Rows& R)
     while (c->h_clock->k < c->h_clock->n_steps) {
         const int k_start = c->h_clock->k;
         const int todo = std::min(c->h_clock->n_steps - k_start, c->batch);
+        const bool fused = fused_of(c);   // at the budgets the batch runs with
         int rc = enqueue_steps<Rows, RS, RV>(c, R, todo);
         if (rc) return rc;
-        if ((rc = launch_finish<RS, RV>(c, verdict_buf(c)))) return rc;
+        if ((rc = launch_finish<RS, RV>(c, verdict_buf(c), fused, fused))) return rc;
         rc = sync_state(c);
         if (rc) return rc;
         int guard = 0;
@@ -1739,12 +1973,14 @@ int run_steps(cmx_ctx* c, const Rows& R)
                 const int from = ctl.it;
                 // the start kernel is idempotent; a refinement solve with a
                 // zero budget has not run it yet
-                if (from == 0 && (rc = launch_cg_start<Rows, RV>(c, R, solve))) return rc;
+                // (after a folded step its residual's norm is in the folded reduction)
+                if (from == 0 && (rc = launch_cg_start<Rows, RV>(c, R, solve, fused ? pbuf(c, kFoldPart) : nullptr)))
+                    return rc;
                 const int chunk = std::max(1, std::min(4, c->max_its + 1 - from));
                 if ((rc = launch_cg_iters<Rows, RV>(c, R, solve, from, chunk))) return rc;
                 verdict = pbuf(c, kIterPart + ((from + chunk) & 1));
             }
-            if ((rc = launch_finish<RS, RV>(c, verdict))) return rc;
+            if ((rc = launch_finish<RS, RV>(c, verdict, fused, false))) return rc;
             if ((rc = sync_state(c))) return rc;
         }
         const int got = c->h_clock->k - k_start;
@@ -1762,6 +1998,7 @@ int run_steps(cmx_ctx* c, const Rows& R)
             c->last_its = slot[0] + slot[2];
         }
         if (refined) c->refine_total += got;
+        if (fused) c->folded_total += got;
         // budgets: the largest count of the batch; a first solve that the
         // refinement had to finish gets one more iteration
         c->budget = std::min(short0 ? std::max(mx0, c->budget) + 1 : mx0, c->max_its);
@@ -2043,6 +2280,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->atol = opts->ksp_atol;
     c->max_its = opts->ksp_max_its;
     c->refine = opts->refine_sweeps;
+    c->fold_refine = opts->separate_refinement == 0;
     c->use_graph = opts->use_graph;
     c->graph_profiler_off = (opts->use_graph && profiler_attached()) ? 1 : 0;
     c->split = opts->operator_splitting ? 1 : 0;
@@ -2234,8 +2472,8 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
         if ((rc = rvec(&c->G1))) return rc;
     }
     if (c->svi && !c->split && (rc = setup_svi(c, mesh, rvec))) return rc;
-    if ((rc = dalloc(c, &c->partials, 3 * (size_t)kMaxBlocks * 3))) return rc;
-    CMX_HIP(c, hipMemset(c->partials, 0, 3 * (size_t)kMaxBlocks * 3 * sizeof(double)));
+    if ((rc = dalloc(c, &c->partials, kPartBufs * (size_t)kMaxBlocks * 3))) return rc;
+    CMX_HIP(c, hipMemset(c->partials, 0, kPartBufs * (size_t)kMaxBlocks * 3 * sizeof(double)));
     if ((rc = dalloc(c, &c->clock, 1))) return rc;
     if ((rc = dalloc(c, &c->ctl, 2))) return rc;
     if ((rc = dalloc(c, &c->hist, (size_t)c->max_its + 2))) return rc;
@@ -2427,7 +2665,8 @@ int cmx_autotune(cmx_ctx* c, double t0, int32_t n_steps, int32_t n_candidates, c
     CMX_HIP(c, copy_out(F, c->F));   // the Iion cache is recovered from F
     const double last_t_next = c->last_t_next;
     const int budget = c->budget, refine_budget = c->refine_budget;
-    const long long steps = c->steps, cg_total = c->cg_total, refine_total = c->refine_total, resumes = c->resumes;
+    const long long steps = c->steps, cg_total = c->cg_total, refine_total = c->refine_total, resumes = c->resumes,
+                    folded_total = c->folded_total;
     const bool stepped = c->stepped;
     auto restore = [&]() -> int {
         CMX_HIP(c, copy_in(c->V, V));
@@ -2444,7 +2683,7 @@ int cmx_autotune(cmx_ctx* c, double t0, int32_t n_steps, int32_t n_candidates, c
         return CMX_OK;
     };
     const int G0 = c->G;
-    int best = G0;
+    int best = G0, best_budget = budget, best_refine_budget = refine_budget;
     double best_ms = 0.0;
     int rc = CMX_OK;
     for (size_t k = 0; k < cand.size() && rc == CMX_OK; ++k) {
@@ -2464,10 +2703,17 @@ int cmx_autotune(cmx_ctx* c, double t0, int32_t n_steps, int32_t n_candidates, c
         if (k == 0 || ms < best_ms) {
             best_ms = ms;
             best = cand[k];
+            best_budget = c->budget;
+            best_refine_budget = c->refine_budget;
         }
     }
     c->G = rc == CMX_OK ? best : G0;
     int rc2 = restore();
+    if (rc == CMX_OK) {
+        // the budgets the trial of those same steps settled on
+        c->budget = best_budget;
+        c->refine_budget = best_refine_budget;
+    }
     // the graphs of the next batch at the restored budgets, captured here
     // rather than inside the caller's next (possibly timed) cmx_step
     if (rc == CMX_OK && rc2 == CMX_OK)
@@ -2476,6 +2722,7 @@ int cmx_autotune(cmx_ctx* c, double t0, int32_t n_steps, int32_t n_candidates, c
     c->cg_total = cg_total;
     c->refine_total = refine_total;
     c->resumes = resumes;
+    c->folded_total = folded_total;
     c->stepped = stepped;
     c->last_t_next = last_t_next;
     if (rc) return rc;
@@ -2634,6 +2881,8 @@ int cmx_get_stats(cmx_ctx* c, cmx_stats* s)
     s->graph_captures = c->graph_captures;
     s->graph_mode = !c->use_graph ? 0 : (c->graph_profiler_off ? 2 : 1);
     s->solver_blocks = c->G;
+    s->resumes = c->resumes;
+    s->folded_refinements = c->folded_total;
     return CMX_OK;
 }
 

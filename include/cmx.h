@@ -121,6 +121,9 @@ typedef struct {
                                   SVI correction of the ICI right-hand side
                                   (MonodomainCorrectionTermAssembler.cpp:48-74); not applied with
                                   operator splitting, whose solver has none */
+    int32_t separate_refinement; /* mixed: 0 (default) computes a step's FP64 true residual in the
+                                  first solve's last CG iteration whenever the refinement solve needs
+                                  no iterations; 1 always in a kernel of its own (same test) */
 } cmx_opts;
 
 typedef struct {
@@ -138,6 +141,9 @@ typedef struct {
     int32_t graph_mode;        /* 0 plain launches (use_graph = 0), 1 graph replay, 2 plain launches
                                   because a profiler (rocprofv3) is attached */
     int32_t solver_blocks;     /* blocks of the solver kernels (cmx_autotune) */
+    int64_t resumes;           /* steps whose final solve the host resumed past its budget */
+    int64_t folded_refinements; /* mixed: steps whose FP64 true residual came from the last CG
+                                  iteration (no kernel of its own) */
 } cmx_stats;
 
 typedef struct cmx_ctx cmx_ctx;
@@ -162,8 +168,11 @@ int cmx_step(cmx_ctx* ctx, double t0, int32_t n_steps);
  * solver blocks (candidates == NULL: 128, 192, 256, 320, 384, 512, capped
  * at one block per 1024 rows), keeps the fastest, and restores the state,
  * the initial-guess history, activation times and statistics, so the next
- * cmx_step starts where it would have.  The block count changes the
- * reductions' summation order only (results agree to rounding).
+ * cmx_step starts where it would have.  The CG iteration budgets (kernels
+ * enqueued per solve, adapted after every batch) are those the fastest trial
+ * settled on, as a first batch of those steps would have left them.  The
+ * block count changes the reductions' summation order only (results agree to
+ * rounding).
  * ms_per_step[n_candidates or 6] (may be NULL) gets each candidate's time. */
 int cmx_autotune(cmx_ctx* ctx, double t0, int32_t n_steps, int32_t n_candidates, const int32_t* candidates,
                  int32_t* chosen, double* ms_per_step);
