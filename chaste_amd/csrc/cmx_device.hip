@@ -953,11 +953,13 @@ __global__ __launch_bounds__(kCgBlock) void cg_iter_kernel(Rows R, MatVals mv, C
     const int done = ctl->done;
     const double tol2 = ctl->tol2;
     const int max_its = ctl->max_its;
+    // a launch after the solve has ended returns before the reduction
+    // (uniform: every thread read the same flags)
+    if (poison || done) return;
     const RV* __restrict__ A = ta.store(sA, a_vals<RV>(mv));
     const RV* __restrict__ D = td.store(sD, dinv_vals<RV>(mv));
     double g[3];
     pl.total(g);    // its barriers also cover the table stores
-    if (poison || done) return;
     const double gamma = g[0], delta = g[1], zeta = g[2];
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const bool lead = blk == 0 && threadIdx.x == 0;
@@ -1048,22 +1050,31 @@ struct LastRow {
     double r0;
     RV r, w, s, x, u, p;
     Gathered<Rows, RV> X, U, P;
-    __device__ __forceinline__ void load(const Rows& R, int i_, const LastArgs<RV>& a, bool first_it)
+    // upd: the update's inputs too (the first iteration's are loaded
+    // before the decision, so always)
+    __device__ __forceinline__ void load(const Rows& R, int i_, const LastArgs<RV>& a, bool first_it, bool upd)
     {
         i = i_;
         row = R.row(i);
         r0 = a.r0_64[i];
         const CgVecs<RV>& v = a.cg.v;
-        P.load(R, row, v.p);
-        r = v.r[i];
-        w = v.w[i];
-        p = v.p[i];
-        if (!first_it) {
-            X.load(R, row, v.x);
+        if (first_it) {
+            P.load(R, row, v.p);
+            r = v.r[i];
+            w = v.w[i];
+            p = v.p[i];
+            return;
+        }
+        X.load(R, row, v.x);
+        x = v.x[i];
+        if (upd) {
             U.load(R, row, a.u_in);
+            P.load(R, row, v.p);
+            r = v.r[i];
+            w = v.w[i];
             s = v.s[i];
-            x = v.x[i];
             u = a.u_in[i];
+            p = v.p[i];
         }
     }
     // upd: this launch makes the update (otherwise x is final as it stands)
@@ -1125,7 +1136,7 @@ __global__ __launch_bounds__(kCgBlock) void cg_last_kernel(Rows R, MatVals mv, L
     // every outcome reads the row (the update's inputs are a superset of the
     // final-x case's)
     LastRow<Rows, RV> q0;
-    if (FIRST && i0 < rr.hi) q0.load(R, i0, a, first_it);
+    if (FIRST && i0 < rr.hi) q0.load(R, i0, a, true, true);
     const double* __restrict__ A64 = ta.store(sA, mv.a64);
     const RV* __restrict__ D = td.store(sD, dinv_vals<RV>(mv));
     double g[3];
@@ -1166,7 +1177,7 @@ __global__ __launch_bounds__(kCgBlock) void cg_last_kernel(Rows R, MatVals mv, L
     if (FIRST && i0 < rr.hi) q0.apply(R, A64, D, a, first_it, upd, al, be, part);
     for (int i = FIRST ? i0 + kCgBlock : i0; i < rr.hi; i += kCgBlock) {
         LastRow<Rows, RV> q;
-        q.load(R, i, a, first_it);
+        q.load(R, i, a, first_it, upd);
         q.apply(R, A64, D, a, first_it, upd, al, be, part);
     }
     write_partials<kCgBlock, 3>(part, a.cg.pout, blk);
