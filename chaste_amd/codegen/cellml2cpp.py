@@ -167,10 +167,12 @@ def cident(key):
 class Printer:
     """Translates MathML expression trees to C++ source text."""
 
-    def __init__(self, model, names, is_const=None):
+    def __init__(self, model, names, is_const=None, v_affine=None):
         self.m = model
         self.names = names      # canonical var -> C expression
         self.is_const = is_const or (lambda key: False)
+        # (k, c) when an expression is k V + c with constant k, c, else None
+        self.v_affine = v_affine or (lambda el, cname: None)
 
     def const_expr(self, el, cname):
         """True when el depends on constants only (no state, parameter or time)."""
@@ -183,6 +185,29 @@ class Printer:
             return all(self.const_expr(c, cname) for c in list(el)[1:]
                        if _tag(c) not in ('degree', 'logbase', 'bvar'))
         return False
+
+    def const_scaled(self, el, cname):
+        """(X source, k) when el is a product with constant factors k and
+        other factors X, or X divided by a constant, else None."""
+        if _tag(el) != 'apply':
+            return None
+        ch = list(el)
+        op = _tag(ch[0])
+        args = ch[1:]
+        ev = Evaluator(self.m)
+        if op == 'times' and len(args) >= 2:
+            cs = [x for x in args if self.const_expr(x, cname)]
+            xs = [x for x in args if not self.const_expr(x, cname)]
+            if not cs or not xs:
+                return None
+            k = 1.0
+            for x in cs:
+                k *= float(ev.eval(x, cname, 0.0))
+            src = ' * '.join(self.expr(x, cname) for x in xs)
+            return '(' + src + ')', k
+        if op == 'divide' and self.const_expr(args[1], cname) and not self.const_expr(args[0], cname):
+            return self.expr(args[0], cname), 1.0 / float(ev.eval(args[1], cname, 0.0))
+        return None
 
     def ref(self, cname, el):
         key = self.m.canon(cname, el.text.strip())
@@ -255,6 +280,17 @@ class Printer:
                     return 'sqrt(%s)' % a[0]
                 return 'pow(%s, R(1) / %s)' % (a[0], self.expr(list(deg)[0], cname))
             if op == 'exp':
+                aff = self.v_affine(args[0], cname)
+                if aff is not None:
+                    # exponent affine in V: single precision folds it to one
+                    # FMA in base 2 (cmx_exp_v), double keeps it as written
+                    k, c = aff
+                    return 'cmx_exp_v(%s, V, %r, %r)' % (a[0], k * LOG2E, c * LOG2E)
+                scaled = self.const_scaled(args[0], cname)
+                if scaled is not None:
+                    # exponent k * X with constant k: log2(e) folds into k
+                    x, k = scaled
+                    return 'cmx_exp_c(%s, %s, %r)' % (a[0], x, k * LOG2E)
                 return 'cmx_exp(%s)' % a[0]
             if op == 'ln':
                 return 'cmx_log(%s)' % a[0]
@@ -290,6 +326,9 @@ class Printer:
         v = self.number_value(el)
         f = float(v)
         return 'R(%s)' % repr(f)
+
+
+LOG2E = 1.4426950408889634074
 
 
 def _exp_arg(el):
@@ -504,6 +543,26 @@ class Generator:
         walk(rhs, cname)
         return found
 
+    def v_affine(self, el, cname):
+        """(k, c) when el depends on the voltage and constants only and
+        equals k V + c (checked at five voltages to 1e-12), else None."""
+        m = self.m
+        ds = deps_of(el, cname, m, set())
+        if m.V not in ds or not all(d == m.V or self.is_const(d) for d in ds):
+            return None
+        ev = Evaluator(m)
+        pts = (-90.0, -47.5, 0.0, 12.25, 60.0)
+        try:
+            vals = [float(ev.eval(el, cname, v)) for v in pts]
+        except (TypeError, ValueError, ZeroDivisionError, OverflowError):
+            return None
+        c = vals[2]
+        k = (vals[4] - vals[0]) / (pts[4] - pts[0])
+        for v, f in zip(pts, vals):
+            if not abs(f - (k * v + c)) <= 1e-12 * (abs(k * v) + abs(c) + abs(f)) + 1e-300:
+                return None
+        return k, c
+
     def closure(self, roots):
         """Algebraic variables needed by roots, in evaluation order."""
         m = self.m
@@ -605,7 +664,7 @@ class Generator:
         names[m.time] = 't'
         for k in self.params:
             names[k] = 'p[%d]' % self.param_index[k]
-        pr = Printer(m, names, self.is_const)
+        pr = Printer(m, names, self.is_const, self.v_affine)
 
         constants = [k for k, info in m.vars.items()
                      if m.canon_map[k] == k and info['init'] is not None
@@ -802,7 +861,7 @@ class Generator:
         gnames = dict(names)
         for k in nonlin:
             gnames[k] = 'g[%d]' % nl_ix[k]
-        gpr = Printer(m, gnames, self.is_const)
+        gpr = Printer(m, gnames, self.is_const, self.v_affine)
         w('// Backward Euler: the states solved by Newton (all but the linear gates).')
         w('constexpr int kNumNonlinear = %d;' % N)
         w('static constexpr int kNonlinear[%d] = {%s};' % (max(N, 1), ', '.join(str(state_ix[k]) for k in nonlin) or '0'))

@@ -1,0 +1,42 @@
+"""Recipe for the generated cell-model headers under chaste_amd/csrc/models.
+
+Each entry names the reference's CellML file (heart/src/odes/cellml/) and the
+constants exposed as run-time parameters, in the order of the header's
+parameter planes.  Regenerate every header with
+
+    python -m chaste_amd.codegen.models [<cellml directory>]
+
+(default: the reference tree's heart/src/odes/cellml under /root/reference).
+The O'Hara-Rudy header (ord_endo.h) is written by hand: the reference holds
+no CellML for it.
+"""
+import os
+import sys
+
+from chaste_amd.codegen import cellml2cpp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(os.path.dirname(HERE), 'csrc', 'models')
+DEFAULT_CELLML = '/root/reference/heart/src/odes/cellml'
+
+MODELS = {
+    'lr91': ('LuoRudy1991.cellml',
+             ['fast_sodium_current.g_Na', 'time_dependent_potassium_current.g_Kmax', 'slow_inward_current.P_si']),
+    'tt06_epi': ('TenTusscher2006Epi.cellml',
+                 ['fast_sodium_current.g_Na', 'L_type_Ca_current.g_CaL', 'rapid_time_dependent_potassium_current.g_Kr',
+                  'slow_time_dependent_potassium_current.g_Ks', 'inward_rectifier_potassium_current.g_K1',
+                  'transient_outward_current.g_to']),
+}
+
+
+def generate(cellml_dir=DEFAULT_CELLML, out_dir=OUT, names=None):
+    for ns in names or sorted(MODELS):
+        fname, params = MODELS[ns]
+        argv = [os.path.join(cellml_dir, fname), os.path.join(out_dir, ns + '.h'), ns]
+        for p in params:
+            argv += ['--param', p]
+        cellml2cpp.main(argv)
+
+
+if __name__ == '__main__':
+    generate(*(sys.argv[1:2] or [DEFAULT_CELLML]))

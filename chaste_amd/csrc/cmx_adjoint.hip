@@ -59,7 +59,7 @@ __device__ __forceinline__ R ici_cell_map(R* y, R V, const R* p, double cm, doub
         const R hh = R(h);
 #pragma unroll
         for (int s = 1; s < NS; ++s) {
-            if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_exp(gk[s] * hh);
+            if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_decay(gk[s], hh);
             else y[s] += hh * dy[s];
         }
     }

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void cells_kernel(CellsArgs<RS> a, int k0, 
 #pragma unroll
         for (int s = 1; s < NS; ++s) {
             if (Model::gate(s)) {
-                if (RL) y[s] = gi[s] + (y[s] - gi[s]) * cmx_exp(gk[s] * h);
+                if (RL) y[s] = gi[s] + (y[s] - gi[s]) * cmx_decay(gk[s], h);
                 else y[s] += h * dy[s];
             } else {
                 yd[s] += a.dt * (double)dy[s];

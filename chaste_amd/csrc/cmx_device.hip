@@ -260,7 +260,7 @@ __device__ __forceinline__ bool integrate_with_voltage(const CellArgs<RS, RV>& a
         const RS hh = (RS)h;
 #pragma unroll
         for (int s = 1; s < NS; ++s) {
-            if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_exp(gk[s] * hh);
+            if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_decay(gk[s], hh);
             else y[s] += hh * dy[s];
         }
         vd += h * (double)dv;
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
             const RS hh = (RS)h;
 #pragma unroll
             for (int s = 1; s < NS; ++s) {
-                if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_exp(gk[s] * hh);
+                if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_decay(gk[s], hh);
                 else y[s] += hh * dy[s];
             }
         }

@@ -56,20 +56,20 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     (void)p; (void)cm; (void)g_inf; (void)g_k; (void)i_stim_area;
     const R i_stim = i_stim_area;
     (void)i_stim;
-    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.135) * cmx_exp(((R(80.0) + V) * (R(1) / (-R(6.8))))))) : (R(0.0)));
-    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(3.56) * cmx_exp((R(0.079) * V))) + (R(310000.0) * cmx_exp((R(0.35) * V))))) : (cmx_div(R(1.0), (R(0.13) * (R(1.0) + cmx_exp(((V + R(10.66)) * (R(1) / (-R(11.1))))))))));
-    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(127140.0)) * cmx_exp((R(0.2444) * V))) - (R(3.474e-05) * cmx_exp(((-R(0.04391)) * V)))) * (V + R(37.78))), (R(1.0) + cmx_exp((R(0.311) * (V + R(79.23))))))) : (R(0.0)));
-    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.1212) * cmx_exp(((-R(0.01052)) * V))), (R(1.0) + cmx_exp(((-R(0.1378)) * (V + R(40.14))))))) : (cmx_div((R(0.3) * cmx_exp(((-R(2.535e-07)) * V))), (R(1.0) + cmx_exp(((-R(0.1)) * (V + R(32.0))))))));
+    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.135) * cmx_exp_v(((R(80.0) + V) * (R(1) / (-R(6.8)))), V, -0.2121610354248476, -16.972882833987807))) : (R(0.0)));
+    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(3.56) * cmx_exp_v((R(0.079) * V), V, 0.11397290823022813, 0.0)) + (R(310000.0) * cmx_exp_v((R(0.35) * V), V, 0.5049432643111371, 0.0)))) : (cmx_div(R(1.0), (R(0.13) * (R(1.0) + cmx_exp_v(((V + R(10.66)) * (R(1) / (-R(11.1)))), V, -0.12997252620621294, -1.3855071293582297))))));
+    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(127140.0)) * cmx_exp_v((R(0.2444) * V), V, 0.3525946679932627, 0.0)) - (R(3.474e-05) * cmx_exp_v(((-R(0.04391)) * V), V, -0.06334873924543438, -0.0))) * (V + R(37.78))), (R(1.0) + cmx_exp_v((R(0.311) * (V + R(79.23))), V, 0.44867815771646763, 35.54877043587573)))) : (R(0.0)));
+    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.1212) * cmx_exp_v(((-R(0.01052)) * V), V, -0.015177151830151895, -0.0)), (R(1.0) + cmx_exp_v(((-R(0.1378)) * (V + R(40.14))), V, -0.19880337663449915, -7.979967538108797)))) : (cmx_div((R(0.3) * cmx_exp_v(((-R(2.535e-07)) * V), V, -3.657231928653522e-07, -0.0)), (R(1.0) + cmx_exp_v(((-R(0.1)) * (V + R(32.0))), V, -0.14426950408889636, -4.616624130844683)))));
     const R fast_sodium_current_m_gate__alpha_m = cmx_remove_singularity(V, R(-47.12999999999975), R(9.999999999999947e-07), [&](const R V) { return cmx_div((R(0.32) * (V + R(47.13))), (-cmx_expm1(((-R(0.1)) * (V + R(47.13)))))); });
-    const R fast_sodium_current_m_gate__beta_m = (R(0.08) * cmx_exp(((-V) * (R(1) / R(11.0)))));
+    const R fast_sodium_current_m_gate__beta_m = (R(0.08) * cmx_exp_v(((-V) * (R(1) / R(11.0))), V, -0.1311540946262694, -0.0));
     const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((y[7] * (R(1) / R(1.0))))));
     const R slow_inward_current__i_si = (p[2] * y[4] * y[5] * (V - slow_inward_current__E_si));
-    const R slow_inward_current_d_gate__alpha_d = cmx_div((R(0.095) * cmx_exp(((-R(0.01)) * (V - R(5.0))))), (R(1.0) + cmx_exp(((-R(0.072)) * (V - R(5.0))))));
-    const R slow_inward_current_d_gate__beta_d = cmx_div((R(0.07) * cmx_exp(((-R(0.017)) * (V + R(44.0))))), (R(1.0) + cmx_exp((R(0.05) * (V + R(44.0))))));
-    const R slow_inward_current_f_gate__alpha_f = cmx_div((R(0.012) * cmx_exp(((-R(0.008)) * (V + R(28.0))))), (R(1.0) + cmx_exp((R(0.15) * (V + R(28.0))))));
-    const R slow_inward_current_f_gate__beta_f = cmx_div((R(0.0065) * cmx_exp(((-R(0.02)) * (V + R(30.0))))), (R(1.0) + cmx_exp(((-R(0.2)) * (V + R(30.0))))));
-    const R time_dependent_potassium_current_X_gate__alpha_X = cmx_div((R(0.0005) * cmx_exp((R(0.083) * (V + R(50.0))))), (R(1.0) + cmx_exp((R(0.057) * (V + R(50.0))))));
-    const R time_dependent_potassium_current_X_gate__beta_X = cmx_div((R(0.0013) * cmx_exp(((-R(0.06)) * (V + R(20.0))))), (R(1.0) + cmx_exp(((-R(0.04)) * (V + R(20.0))))));
+    const R slow_inward_current_d_gate__alpha_d = cmx_div((R(0.095) * cmx_exp_v(((-R(0.01)) * (V - R(5.0))), V, -0.014426950408889635, 0.07213475204444818)), (R(1.0) + cmx_exp_v(((-R(0.072)) * (V - R(5.0))), V, -0.10387404294400536, 0.5193702147200268)));
+    const R slow_inward_current_d_gate__beta_d = cmx_div((R(0.07) * cmx_exp_v(((-R(0.017)) * (V + R(44.0))), V, -0.02452581569511238, -1.0791358905849446)), (R(1.0) + cmx_exp_v((R(0.05) * (V + R(44.0))), V, 0.07213475204444818, 3.1739290899557195)));
+    const R slow_inward_current_f_gate__alpha_f = cmx_div((R(0.012) * cmx_exp_v(((-R(0.008)) * (V + R(28.0))), V, -0.011541560327111707, -0.3231636891591278)), (R(1.0) + cmx_exp_v((R(0.15) * (V + R(28.0))), V, 0.2164042561333445, 6.059319171733646)));
+    const R slow_inward_current_f_gate__beta_f = cmx_div((R(0.0065) * cmx_exp_v(((-R(0.02)) * (V + R(30.0))), V, -0.02885390081777927, -0.865617024533378)), (R(1.0) + cmx_exp_v(((-R(0.2)) * (V + R(30.0))), V, -0.2885390081777927, -8.65617024533378)));
+    const R time_dependent_potassium_current_X_gate__alpha_X = cmx_div((R(0.0005) * cmx_exp_v((R(0.083) * (V + R(50.0))), V, 0.11974368839378396, 5.987184419689199)), (R(1.0) + cmx_exp_v((R(0.057) * (V + R(50.0))), V, 0.08223361733067092, 4.111680866533546)));
+    const R time_dependent_potassium_current_X_gate__beta_X = cmx_div((R(0.0013) * cmx_exp_v(((-R(0.06)) * (V + R(20.0))), V, -0.0865617024533378, -1.731234049066756)), (R(1.0) + cmx_exp_v(((-R(0.04)) * (V + R(20.0))), V, -0.05770780163555854, -1.1541560327111708)));
     if (RL) {
         g_k[1] = ((fast_sodium_current_m_gate__alpha_m * (-R(1))) - (fast_sodium_current_m_gate__beta_m * R(1)));
         g_inf[1] = cmx_div(-((fast_sodium_current_m_gate__alpha_m * R(1.0))), ((fast_sodium_current_m_gate__alpha_m * (-R(1))) - (fast_sodium_current_m_gate__beta_m * R(1))));
@@ -134,17 +134,17 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[1]) * y[2] * y[3] * (V - fast_sodium_current__E_Na));
     const R time_independent_potassium_current__E_K1 = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log((ionic_concentrations__Ko * (R(1) / ionic_concentrations__Ki))));
     const R plateau_potassium_current__E_Kp = time_independent_potassium_current__E_K1;
-    const R plateau_potassium_current__Kp = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(7.488) - V) * (R(1) / R(5.98))))));
+    const R plateau_potassium_current__Kp = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((R(7.488) - V) * (R(1) / R(5.98))), V, -0.2412533513192246, 1.806505094678354)));
     const R plateau_potassium_current__i_Kp = (plateau_potassium_current__g_Kp * plateau_potassium_current__Kp * (V - plateau_potassium_current__E_Kp));
     const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((y[7] * (R(1) / R(1.0))))));
     const R slow_inward_current__i_si = (p[2] * y[4] * y[5] * (V - slow_inward_current__E_si));
     const R time_dependent_potassium_current__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(((ionic_concentrations__Ko + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nao)) * (R(1) / (ionic_concentrations__Ki + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nai))))));
     const R time_dependent_potassium_current__g_K = (p[1] * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
-    const R time_dependent_potassium_current_Xi_gate__Xi = cmx_remove_singularity(V, R(-76.99999999999993), R(2.4999999999999977e-06), [&](const R V) { return (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * cmx_expm1((R(0.04) * (V + R(77.0))))), ((V + R(77.0)) * cmx_exp((R(0.04) * (V + R(35.0))))))) : (R(1.0))); });
+    const R time_dependent_potassium_current_Xi_gate__Xi = cmx_remove_singularity(V, R(-76.99999999999993), R(2.4999999999999977e-06), [&](const R V) { return (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * cmx_expm1((R(0.04) * (V + R(77.0))))), ((V + R(77.0)) * cmx_exp_v((R(0.04) * (V + R(35.0))), V, 0.05770780163555854, 2.019773057244549)))) : (R(1.0))); });
     const R time_dependent_potassium_current__i_K = (time_dependent_potassium_current__g_K * y[6] * time_dependent_potassium_current_Xi_gate__Xi * (V - time_dependent_potassium_current__E_K));
     const R time_independent_potassium_current__g_K1 = (R(0.6047) * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
-    const R time_independent_potassium_current_K1_gate__alpha_K1 = cmx_div(R(1.02), (R(1.0) + cmx_exp((R(0.2385) * ((V - time_independent_potassium_current__E_K1) - R(59.215))))));
-    const R time_independent_potassium_current_K1_gate__beta_K1 = cmx_div(((R(0.49124) * cmx_exp((R(0.08032) * ((V + R(5.476)) - time_independent_potassium_current__E_K1)))) + (R(1.0) * cmx_exp((R(0.06175) * (V - (time_independent_potassium_current__E_K1 + R(594.31))))))), (R(1.0) + cmx_exp(((-R(0.5143)) * ((V - time_independent_potassium_current__E_K1) + R(4.753))))));
+    const R time_independent_potassium_current_K1_gate__alpha_K1 = cmx_div(R(1.02), (R(1.0) + cmx_exp_v((R(0.2385) * ((V - time_independent_potassium_current__E_K1) - R(59.215))), V, 0.34408276725201775, 9.867571780666546)));
+    const R time_independent_potassium_current_K1_gate__beta_K1 = cmx_div(((R(0.49124) * cmx_exp_v((R(0.08032) * ((V + R(5.476)) - time_independent_potassium_current__E_K1)), V, 0.11587726568420155, 10.819333030532393)) + (R(1.0) * cmx_exp_v((R(0.06175) * (V - (time_independent_potassium_current__E_K1 + R(594.31)))), V, 0.08908641877489352, -45.11488569269058))), (R(1.0) + cmx_exp_v(((-R(0.5143)) * ((V - time_independent_potassium_current__E_K1) + R(4.753))), V, -0.741978059529194, -68.7412263769396)));
     const R time_independent_potassium_current_K1_gate__K1_infinity = cmx_div(time_independent_potassium_current_K1_gate__alpha_K1, (time_independent_potassium_current_K1_gate__alpha_K1 + time_independent_potassium_current_K1_gate__beta_K1));
     const R time_independent_potassium_current__i_K1 = (time_independent_potassium_current__g_K1 * time_independent_potassium_current_K1_gate__K1_infinity * (V - time_independent_potassium_current__E_K1));
     return (((-R(1.0)) * (R(1) / membrane__C)) * (i_stim + fast_sodium_current__i_Na + slow_inward_current__i_si + time_dependent_potassium_current__i_K + time_independent_potassium_current__i_K1 + plateau_potassium_current__i_Kp + background_current__i_b));
@@ -176,17 +176,17 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[1]) * y[2] * y[3] * (V - fast_sodium_current__E_Na));
     const R time_independent_potassium_current__E_K1 = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log((ionic_concentrations__Ko * (R(1) / ionic_concentrations__Ki))));
     const R plateau_potassium_current__E_Kp = time_independent_potassium_current__E_K1;
-    const R plateau_potassium_current__Kp = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(7.488) - V) * (R(1) / R(5.98))))));
+    const R plateau_potassium_current__Kp = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((R(7.488) - V) * (R(1) / R(5.98))), V, -0.2412533513192246, 1.806505094678354)));
     const R plateau_potassium_current__i_Kp = (plateau_potassium_current__g_Kp * plateau_potassium_current__Kp * (V - plateau_potassium_current__E_Kp));
     const R slow_inward_current__E_si = (R(7.7) - (R(13.0287) * cmx_log((y[7] * (R(1) / R(1.0))))));
     const R slow_inward_current__i_si = (p[2] * y[4] * y[5] * (V - slow_inward_current__E_si));
     const R time_dependent_potassium_current__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(((ionic_concentrations__Ko + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nao)) * (R(1) / (ionic_concentrations__Ki + (time_dependent_potassium_current__PR_NaK * ionic_concentrations__Nai))))));
     const R time_dependent_potassium_current__g_K = (p[1] * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
-    const R time_dependent_potassium_current_Xi_gate__Xi = cmx_remove_singularity(V, R(-76.99999999999993), R(2.4999999999999977e-06), [&](const R V) { return (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * cmx_expm1((R(0.04) * (V + R(77.0))))), ((V + R(77.0)) * cmx_exp((R(0.04) * (V + R(35.0))))))) : (R(1.0))); });
+    const R time_dependent_potassium_current_Xi_gate__Xi = cmx_remove_singularity(V, R(-76.99999999999993), R(2.4999999999999977e-06), [&](const R V) { return (((V > (-R(100.0)))) ? (cmx_div((R(2.837) * cmx_expm1((R(0.04) * (V + R(77.0))))), ((V + R(77.0)) * cmx_exp_v((R(0.04) * (V + R(35.0))), V, 0.05770780163555854, 2.019773057244549)))) : (R(1.0))); });
     const R time_dependent_potassium_current__i_K = (time_dependent_potassium_current__g_K * y[6] * time_dependent_potassium_current_Xi_gate__Xi * (V - time_dependent_potassium_current__E_K));
     const R time_independent_potassium_current__g_K1 = (R(0.6047) * sqrt((ionic_concentrations__Ko * (R(1) / R(5.4)))));
-    const R time_independent_potassium_current_K1_gate__alpha_K1 = cmx_div(R(1.02), (R(1.0) + cmx_exp((R(0.2385) * ((V - time_independent_potassium_current__E_K1) - R(59.215))))));
-    const R time_independent_potassium_current_K1_gate__beta_K1 = cmx_div(((R(0.49124) * cmx_exp((R(0.08032) * ((V + R(5.476)) - time_independent_potassium_current__E_K1)))) + (R(1.0) * cmx_exp((R(0.06175) * (V - (time_independent_potassium_current__E_K1 + R(594.31))))))), (R(1.0) + cmx_exp(((-R(0.5143)) * ((V - time_independent_potassium_current__E_K1) + R(4.753))))));
+    const R time_independent_potassium_current_K1_gate__alpha_K1 = cmx_div(R(1.02), (R(1.0) + cmx_exp_v((R(0.2385) * ((V - time_independent_potassium_current__E_K1) - R(59.215))), V, 0.34408276725201775, 9.867571780666546)));
+    const R time_independent_potassium_current_K1_gate__beta_K1 = cmx_div(((R(0.49124) * cmx_exp_v((R(0.08032) * ((V + R(5.476)) - time_independent_potassium_current__E_K1)), V, 0.11587726568420155, 10.819333030532393)) + (R(1.0) * cmx_exp_v((R(0.06175) * (V - (time_independent_potassium_current__E_K1 + R(594.31)))), V, 0.08908641877489352, -45.11488569269058))), (R(1.0) + cmx_exp_v(((-R(0.5143)) * ((V - time_independent_potassium_current__E_K1) + R(4.753))), V, -0.741978059529194, -68.7412263769396)));
     const R time_independent_potassium_current_K1_gate__K1_infinity = cmx_div(time_independent_potassium_current_K1_gate__alpha_K1, (time_independent_potassium_current_K1_gate__alpha_K1 + time_independent_potassium_current_K1_gate__beta_K1));
     const R time_independent_potassium_current__i_K1 = (time_independent_potassium_current__g_K1 * time_independent_potassium_current_K1_gate__K1_infinity * (V - time_independent_potassium_current__E_K1));
     const R dv = (((-R(1.0)) * (R(1) / membrane__C)) * (i_stim + fast_sodium_current__i_Na + slow_inward_current__i_si + time_dependent_potassium_current__i_K + time_independent_potassium_current__i_K1 + plateau_potassium_current__i_Kp + background_current__i_b));
@@ -204,18 +204,18 @@ CMX_HD inline void be_linear(R* y, R V, const R* p, R cm, R dt)
 {
     CMX_MATH_USING
     (void)p; (void)cm; (void)V;
-    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.135) * cmx_exp(((R(80.0) + V) * (R(1) / (-R(6.8))))))) : (R(0.0)));
-    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(3.56) * cmx_exp((R(0.079) * V))) + (R(310000.0) * cmx_exp((R(0.35) * V))))) : (cmx_div(R(1.0), (R(0.13) * (R(1.0) + cmx_exp(((V + R(10.66)) * (R(1) / (-R(11.1))))))))));
-    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(127140.0)) * cmx_exp((R(0.2444) * V))) - (R(3.474e-05) * cmx_exp(((-R(0.04391)) * V)))) * (V + R(37.78))), (R(1.0) + cmx_exp((R(0.311) * (V + R(79.23))))))) : (R(0.0)));
-    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.1212) * cmx_exp(((-R(0.01052)) * V))), (R(1.0) + cmx_exp(((-R(0.1378)) * (V + R(40.14))))))) : (cmx_div((R(0.3) * cmx_exp(((-R(2.535e-07)) * V))), (R(1.0) + cmx_exp(((-R(0.1)) * (V + R(32.0))))))));
+    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.135) * cmx_exp_v(((R(80.0) + V) * (R(1) / (-R(6.8)))), V, -0.2121610354248476, -16.972882833987807))) : (R(0.0)));
+    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(3.56) * cmx_exp_v((R(0.079) * V), V, 0.11397290823022813, 0.0)) + (R(310000.0) * cmx_exp_v((R(0.35) * V), V, 0.5049432643111371, 0.0)))) : (cmx_div(R(1.0), (R(0.13) * (R(1.0) + cmx_exp_v(((V + R(10.66)) * (R(1) / (-R(11.1)))), V, -0.12997252620621294, -1.3855071293582297))))));
+    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(127140.0)) * cmx_exp_v((R(0.2444) * V), V, 0.3525946679932627, 0.0)) - (R(3.474e-05) * cmx_exp_v(((-R(0.04391)) * V), V, -0.06334873924543438, -0.0))) * (V + R(37.78))), (R(1.0) + cmx_exp_v((R(0.311) * (V + R(79.23))), V, 0.44867815771646763, 35.54877043587573)))) : (R(0.0)));
+    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.1212) * cmx_exp_v(((-R(0.01052)) * V), V, -0.015177151830151895, -0.0)), (R(1.0) + cmx_exp_v(((-R(0.1378)) * (V + R(40.14))), V, -0.19880337663449915, -7.979967538108797)))) : (cmx_div((R(0.3) * cmx_exp_v(((-R(2.535e-07)) * V), V, -3.657231928653522e-07, -0.0)), (R(1.0) + cmx_exp_v(((-R(0.1)) * (V + R(32.0))), V, -0.14426950408889636, -4.616624130844683)))));
     const R fast_sodium_current_m_gate__alpha_m = cmx_remove_singularity(V, R(-47.12999999999975), R(9.999999999999947e-07), [&](const R V) { return cmx_div((R(0.32) * (V + R(47.13))), (-cmx_expm1(((-R(0.1)) * (V + R(47.13)))))); });
-    const R fast_sodium_current_m_gate__beta_m = (R(0.08) * cmx_exp(((-V) * (R(1) / R(11.0)))));
-    const R slow_inward_current_d_gate__alpha_d = cmx_div((R(0.095) * cmx_exp(((-R(0.01)) * (V - R(5.0))))), (R(1.0) + cmx_exp(((-R(0.072)) * (V - R(5.0))))));
-    const R slow_inward_current_d_gate__beta_d = cmx_div((R(0.07) * cmx_exp(((-R(0.017)) * (V + R(44.0))))), (R(1.0) + cmx_exp((R(0.05) * (V + R(44.0))))));
-    const R slow_inward_current_f_gate__alpha_f = cmx_div((R(0.012) * cmx_exp(((-R(0.008)) * (V + R(28.0))))), (R(1.0) + cmx_exp((R(0.15) * (V + R(28.0))))));
-    const R slow_inward_current_f_gate__beta_f = cmx_div((R(0.0065) * cmx_exp(((-R(0.02)) * (V + R(30.0))))), (R(1.0) + cmx_exp(((-R(0.2)) * (V + R(30.0))))));
-    const R time_dependent_potassium_current_X_gate__alpha_X = cmx_div((R(0.0005) * cmx_exp((R(0.083) * (V + R(50.0))))), (R(1.0) + cmx_exp((R(0.057) * (V + R(50.0))))));
-    const R time_dependent_potassium_current_X_gate__beta_X = cmx_div((R(0.0013) * cmx_exp(((-R(0.06)) * (V + R(20.0))))), (R(1.0) + cmx_exp(((-R(0.04)) * (V + R(20.0))))));
+    const R fast_sodium_current_m_gate__beta_m = (R(0.08) * cmx_exp_v(((-V) * (R(1) / R(11.0))), V, -0.1311540946262694, -0.0));
+    const R slow_inward_current_d_gate__alpha_d = cmx_div((R(0.095) * cmx_exp_v(((-R(0.01)) * (V - R(5.0))), V, -0.014426950408889635, 0.07213475204444818)), (R(1.0) + cmx_exp_v(((-R(0.072)) * (V - R(5.0))), V, -0.10387404294400536, 0.5193702147200268)));
+    const R slow_inward_current_d_gate__beta_d = cmx_div((R(0.07) * cmx_exp_v(((-R(0.017)) * (V + R(44.0))), V, -0.02452581569511238, -1.0791358905849446)), (R(1.0) + cmx_exp_v((R(0.05) * (V + R(44.0))), V, 0.07213475204444818, 3.1739290899557195)));
+    const R slow_inward_current_f_gate__alpha_f = cmx_div((R(0.012) * cmx_exp_v(((-R(0.008)) * (V + R(28.0))), V, -0.011541560327111707, -0.3231636891591278)), (R(1.0) + cmx_exp_v((R(0.15) * (V + R(28.0))), V, 0.2164042561333445, 6.059319171733646)));
+    const R slow_inward_current_f_gate__beta_f = cmx_div((R(0.0065) * cmx_exp_v(((-R(0.02)) * (V + R(30.0))), V, -0.02885390081777927, -0.865617024533378)), (R(1.0) + cmx_exp_v(((-R(0.2)) * (V + R(30.0))), V, -0.2885390081777927, -8.65617024533378)));
+    const R time_dependent_potassium_current_X_gate__alpha_X = cmx_div((R(0.0005) * cmx_exp_v((R(0.083) * (V + R(50.0))), V, 0.11974368839378396, 5.987184419689199)), (R(1.0) + cmx_exp_v((R(0.057) * (V + R(50.0))), V, 0.08223361733067092, 4.111680866533546)));
+    const R time_dependent_potassium_current_X_gate__beta_X = cmx_div((R(0.0013) * cmx_exp_v(((-R(0.06)) * (V + R(20.0))), V, -0.0865617024533378, -1.731234049066756)), (R(1.0) + cmx_exp_v(((-R(0.04)) * (V + R(20.0))), V, -0.05770780163555854, -1.1541560327111708)));
     y[1] = (y[1] + ((fast_sodium_current_m_gate__alpha_m * R(1.0))) * dt) / (R(1) - (((fast_sodium_current_m_gate__alpha_m * (-R(1))) - (fast_sodium_current_m_gate__beta_m * R(1)))) * dt);
     y[2] = (y[2] + ((fast_sodium_current_h_gate__alpha_h * R(1.0))) * dt) / (R(1) - (((fast_sodium_current_h_gate__alpha_h * (-R(1))) - (fast_sodium_current_h_gate__beta_h * R(1)))) * dt);
     y[3] = (y[3] + ((fast_sodium_current_j_gate__alpha_j * R(1.0))) * dt) / (R(1) - (((fast_sodium_current_j_gate__alpha_j * (-R(1))) - (fast_sodium_current_j_gate__beta_j * R(1)))) * dt);

@@ -36,9 +36,82 @@ inline float cmx_log(float x) { return std::log(x); }
 inline float cmx_div(float a, float b) { return a / b; }
 #endif
 
-// exp(x) - 1 without cancellation for small x (GHK-type currents).
+// exp(x) - 1 without cancellation for small x (GHK-type currents).  Single
+// precision on the device uses Kahan's form (u - 1) x / log(u), u = exp(x),
+// exact to a few units in the last place with the hardware exp2/log2
+// (W. Kahan, as given in N. J. Higham, Accuracy and Stability of Numerical
+// Algorithms, 2nd ed., SIAM 2002, section 1.14.1).
 CMX_HD inline double cmx_expm1(double x) { return std::expm1(x); }
-CMX_HD inline float cmx_expm1(float x) { return std::expm1(x); }
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline float cmx_expm1(float x)
+{
+    const float u = __expf(x);
+    if (u == 1.0f) return x;
+    const float um1 = u - 1.0f;
+    if (um1 == -1.0f) return -1.0f;
+    return um1 * x * __builtin_amdgcn_rcpf(__logf(u));
+}
+#else
+inline float cmx_expm1(float x) { return std::expm1(x); }
+#endif
+
+// exp(E) for an exponent E = k V + c affine in the membrane voltage, with
+// k2 = k log2(e) and c2 = c log2(e) folded by the generator.  Double
+// precision evaluates the exponent e exactly as the model writes it; single
+// precision takes exp2(V k2 + c2), one FMA and the hardware exp2, and the
+// unused e is dead code.
+CMX_HD inline double cmx_exp_v(double e, double V, double k2, double c2)
+{
+    (void)V; (void)k2; (void)c2;
+    return cmx_exp(e);
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline float cmx_exp_v(float e, float V, double k2, double c2)
+{
+    (void)e;
+    return __builtin_amdgcn_exp2f(__builtin_fmaf(V, (float)k2, (float)c2));
+}
+#else
+inline float cmx_exp_v(float e, float V, double k2, double c2)
+{
+    (void)e;
+    return std::exp2(std::fma(V, (float)k2, (float)c2));
+}
+#endif
+
+// exp(k x) for a constant k, with k2 = k log2(e) folded by the generator:
+// double evaluates the exponent e as written, single precision exp2(x k2).
+CMX_HD inline double cmx_exp_c(double e, double x, double k2)
+{
+    (void)x; (void)k2;
+    return cmx_exp(e);
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline float cmx_exp_c(float e, float x, double k2)
+{
+    (void)e;
+    return __builtin_amdgcn_exp2f(x * (float)k2);
+}
+#else
+inline float cmx_exp_c(float e, float x, double k2)
+{
+    (void)e;
+    return std::exp2(x * (float)k2);
+}
+#endif
+
+// Rush-Larsen decay factor exp(g_k h) of a gate (g_k = -1/tau).  Single
+// precision folds log2(e) into the step length, which the unrolled gate loop
+// computes once.
+CMX_HD inline double cmx_decay(double gk, double h) { return cmx_exp(gk * h); }
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline float cmx_decay(float gk, float h)
+{
+    return __builtin_amdgcn_exp2f(gk * (h * 1.44269504088896340736f));
+}
+#else
+inline float cmx_decay(float gk, float h) { return std::exp(gk * h); }
+#endif
 
 // f(V) near a removable singularity c (0/0 at V = c): linear interpolation
 // between V = c - d and c + d inside that window, f(V) elsewhere.  d is dv,

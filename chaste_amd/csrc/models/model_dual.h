@@ -95,6 +95,17 @@ struct cmx_dualn {
     }
     friend CMX_HD cmx_dualn exp(const cmx_dualn& x) { return cmx_exp(x); }
     friend CMX_HD cmx_dualn cmx_expm1(const cmx_dualn& x) { return chain(x, std::expm1(x.v), std::exp(x.v)); }
+    friend CMX_HD cmx_dualn cmx_exp_v(const cmx_dualn& e, const cmx_dualn& V, double k2, double c2)
+    {
+        (void)V; (void)k2; (void)c2;
+        return cmx_exp(e);
+    }
+    friend CMX_HD cmx_dualn cmx_exp_c(const cmx_dualn& e, const cmx_dualn& x, double k2)
+    {
+        (void)x; (void)k2;
+        return cmx_exp(e);
+    }
+    friend CMX_HD cmx_dualn cmx_decay(const cmx_dualn& gk, const cmx_dualn& h) { return cmx_exp(gk * h); }
     friend CMX_HD cmx_dualn cmx_log(const cmx_dualn& x)
     {
         cmx_dualn r(std::log(x.v));

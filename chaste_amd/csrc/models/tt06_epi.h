@@ -130,18 +130,18 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
-    const R L_type_Ca_current_d_gate__d_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(8.0)) - V) * (R(1) / R(7.5))))));
-    const R L_type_Ca_current_d_gate__alpha_d = (cmx_div(R(1.4), (R(1.0) + cmx_exp((((-R(35.0)) - V) * (R(1) / R(13.0)))))) + R(0.25));
-    const R L_type_Ca_current_d_gate__beta_d = cmx_div(R(1.4), (R(1.0) + cmx_exp(((V + R(5.0)) * (R(1) / R(5.0))))));
-    const R L_type_Ca_current_d_gate__gamma_d = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(50.0) - V) * (R(1) / R(20.0))))));
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp_v(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.10801137730230755, -1.620170659534613)) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current_d_gate__d_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v((((-R(8.0)) - V) * (R(1) / R(7.5))), V, -0.19235933878519512, -1.538874710281561)));
+    const R L_type_Ca_current_d_gate__alpha_d = (cmx_div(R(1.4), (R(1.0) + cmx_exp_v((((-R(35.0)) - V) * (R(1) / R(13.0))), V, -0.11097654160684334, -3.8841789562395173))) + R(0.25));
+    const R L_type_Ca_current_d_gate__beta_d = cmx_div(R(1.4), (R(1.0) + cmx_exp_v(((V + R(5.0)) * (R(1) / R(5.0))), V, 0.2885390081777927, 1.4426950408889634)));
+    const R L_type_Ca_current_d_gate__gamma_d = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((R(50.0) - V) * (R(1) / R(20.0))), V, -0.07213475204444818, 3.6067376022224087)));
     const R L_type_Ca_current_d_gate__tau_d = ((L_type_Ca_current_d_gate__alpha_d * L_type_Ca_current_d_gate__beta_d) + L_type_Ca_current_d_gate__gamma_d);
-    const R L_type_Ca_current_f2_gate__f2_inf = (cmx_div(R(0.67), (R(1.0) + cmx_exp(((V + R(35.0)) * (R(1) / R(7.0)))))) + R(0.33));
-    const R L_type_Ca_current_f2_gate__tau_f2 = ((R(562.0) * cmx_exp(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(240.0))))) + cmx_div(R(31.0), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(10.0)))))) + cmx_div(R(80.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(10.0)))))));
+    const R L_type_Ca_current_f2_gate__f2_inf = (cmx_div(R(0.67), (R(1.0) + cmx_exp_v(((V + R(35.0)) * (R(1) / R(7.0))), V, 0.2060992915555662, 7.213475204444817))) + R(0.33));
+    const R L_type_Ca_current_f2_gate__tau_f2 = ((R(562.0) * cmx_exp_c(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(240.0))), (-cmx_ipow<2>((V + R(27.0)))), 0.0060112293370373476)) + cmx_div(R(31.0), (R(1.0) + cmx_exp_v(((R(25.0) - V) * (R(1) / R(10.0))), V, -0.14426950408889636, 3.6067376022224087))) + cmx_div(R(80.0), (R(1.0) + cmx_exp_v(((V + R(30.0)) * (R(1) / R(10.0))), V, 0.14426950408889636, 4.32808512266689))));
     const R L_type_Ca_current_fCass_gate__fCass_inf = (cmx_div(R(0.6), (R(1.0) + cmx_ipow<2>((y[15] * (R(1) / R(0.05)))))) + R(0.4));
     const R L_type_Ca_current_fCass_gate__tau_fCass = (cmx_div(R(80.0), (R(1.0) + cmx_ipow<2>((y[15] * (R(1) / R(0.05)))))) + R(2.0));
-    const R L_type_Ca_current_f_gate__f_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(20.0)) * (R(1) / R(7.0))))));
-    const R L_type_Ca_current_f_gate__tau_f = ((R(1102.5) * cmx_exp(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(225.0))))) + cmx_div(R(200.0), (R(1.0) + cmx_exp(((R(13.0) - V) * (R(1) / R(10.0)))))) + cmx_div(R(180.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(10.0)))))) + R(20.0));
+    const R L_type_Ca_current_f_gate__f_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((V + R(20.0)) * (R(1) / R(7.0))), V, 0.20609929155556622, 4.121985831111324)));
+    const R L_type_Ca_current_f_gate__tau_f = ((R(1102.5) * cmx_exp_c(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(225.0))), (-cmx_ipow<2>((V + R(27.0)))), 0.006411977959506504)) + cmx_div(R(200.0), (R(1.0) + cmx_exp_v(((R(13.0) - V) * (R(1) / R(10.0))), V, -0.14426950408889636, 1.8755035531556525))) + cmx_div(R(180.0), (R(1.0) + cmx_exp_v(((V + R(30.0)) * (R(1) / R(10.0))), V, 0.14426950408889636, 4.32808512266689))) + R(20.0));
     const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, y[13])));
     const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
     const R calcium_dynamics__Ca_i_bufc = cmx_div(R(1.0), (R(1.0) + cmx_div((calcium_dynamics__Buf_c * calcium_dynamics__K_buf_c), cmx_ipow<2>((y[13] + calcium_dynamics__K_buf_c)))));
@@ -158,47 +158,47 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * y[13]), (y[13] + calcium_pump_current__K_pCa));
     const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, y[17])));
     const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
-    const R fast_sodium_current_h_gate__h_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp(((V + R(71.55)) * (R(1) / R(7.43)))))));
-    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.057) * cmx_exp(((-(V + R(80.0))) * (R(1) / R(6.8)))))) : (R(0.0)));
-    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(2.7) * cmx_exp((R(0.079) * V))) + (R(310000.0) * cmx_exp((R(0.3485) * V))))) : (cmx_div(R(0.77), (R(0.13) * (R(1.0) + cmx_exp(((V + R(10.66)) * (R(1) / (-R(11.1))))))))));
+    const R fast_sodium_current_h_gate__h_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp_v(((V + R(71.55)) * (R(1) / R(7.43))), V, 0.19417160711829928, 13.89297848931431))));
+    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.057) * cmx_exp_v(((-(V + R(80.0))) * (R(1) / R(6.8))), V, -0.2121610354248476, -16.972882833987807))) : (R(0.0)));
+    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(2.7) * cmx_exp_v((R(0.079) * V), V, 0.11397290823022813, 0.0)) + (R(310000.0) * cmx_exp_v((R(0.3485) * V), V, 0.5027792217498037, 0.0)))) : (cmx_div(R(0.77), (R(0.13) * (R(1.0) + cmx_exp_v(((V + R(10.66)) * (R(1) / (-R(11.1)))), V, -0.12997252620621294, -1.3855071293582297))))));
     const R fast_sodium_current_h_gate__tau_h = cmx_div(R(1.0), (fast_sodium_current_h_gate__alpha_h + fast_sodium_current_h_gate__beta_h));
-    const R fast_sodium_current_j_gate__j_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp(((V + R(71.55)) * (R(1) / R(7.43)))))));
-    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(25428.0)) * cmx_exp((R(0.2444) * V))) - (R(6.948e-06) * cmx_exp(((-R(0.04391)) * V)))) * (V + R(37.78))), (R(1.0) + cmx_exp((R(0.311) * (V + R(79.23))))))) : (R(0.0)));
-    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.02424) * cmx_exp(((-R(0.01052)) * V))), (R(1.0) + cmx_exp(((-R(0.1378)) * (V + R(40.14))))))) : (cmx_div((R(0.6) * cmx_exp((R(0.057) * V))), (R(1.0) + cmx_exp(((-R(0.1)) * (V + R(32.0))))))));
+    const R fast_sodium_current_j_gate__j_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp_v(((V + R(71.55)) * (R(1) / R(7.43))), V, 0.19417160711829928, 13.89297848931431))));
+    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(25428.0)) * cmx_exp_v((R(0.2444) * V), V, 0.3525946679932627, 0.0)) - (R(6.948e-06) * cmx_exp_v(((-R(0.04391)) * V), V, -0.06334873924543438, -0.0))) * (V + R(37.78))), (R(1.0) + cmx_exp_v((R(0.311) * (V + R(79.23))), V, 0.44867815771646763, 35.54877043587573)))) : (R(0.0)));
+    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.02424) * cmx_exp_v(((-R(0.01052)) * V), V, -0.015177151830151895, -0.0)), (R(1.0) + cmx_exp_v(((-R(0.1378)) * (V + R(40.14))), V, -0.19880337663449915, -7.979967538108797)))) : (cmx_div((R(0.6) * cmx_exp_v((R(0.057) * V), V, 0.08223361733067092, 0.0)), (R(1.0) + cmx_exp_v(((-R(0.1)) * (V + R(32.0))), V, -0.14426950408889636, -4.616624130844683)))));
     const R fast_sodium_current_j_gate__tau_j = cmx_div(R(1.0), (fast_sodium_current_j_gate__alpha_j + fast_sodium_current_j_gate__beta_j));
-    const R fast_sodium_current_m_gate__m_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp((((-R(56.86)) - V) * (R(1) / R(9.03)))))));
-    const R fast_sodium_current_m_gate__alpha_m = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(60.0)) - V) * (R(1) / R(5.0))))));
-    const R fast_sodium_current_m_gate__beta_m = (cmx_div(R(0.1), (R(1.0) + cmx_exp(((V + R(35.0)) * (R(1) / R(5.0)))))) + cmx_div(R(0.1), (R(1.0) + cmx_exp(((V - R(50.0)) * (R(1) / R(200.0)))))));
+    const R fast_sodium_current_m_gate__m_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp_v((((-R(56.86)) - V) * (R(1) / R(9.03))), V, -0.1597668926787335, -9.084345517712785))));
+    const R fast_sodium_current_m_gate__alpha_m = cmx_div(R(1.0), (R(1.0) + cmx_exp_v((((-R(60.0)) - V) * (R(1) / R(5.0))), V, -0.2885390081777927, -17.31234049066756)));
+    const R fast_sodium_current_m_gate__beta_m = (cmx_div(R(0.1), (R(1.0) + cmx_exp_v(((V + R(35.0)) * (R(1) / R(5.0))), V, 0.2885390081777927, 10.098865286222743))) + cmx_div(R(0.1), (R(1.0) + cmx_exp_v(((V - R(50.0)) * (R(1) / R(200.0))), V, 0.007213475204444817, -0.36067376022224085))));
     const R fast_sodium_current_m_gate__tau_m = (fast_sodium_current_m_gate__alpha_m * fast_sodium_current_m_gate__beta_m);
     const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, y[18])));
-    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
-    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
+    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp_c((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))), (((V - reversal_potentials__E_K) - R(200.0))), 0.0865617024533378)));
+    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp_c((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))), (((V - reversal_potentials__E_K) + R(100.0))), 0.0002885390081777927)) + cmx_exp_c((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))), (((V - reversal_potentials__E_K) - R(10.0))), 0.14426950408889636)), (R(1.0) + cmx_exp_c(((-R(0.5)) * (V - reversal_potentials__E_K)), ((V - reversal_potentials__E_K)), -0.7213475204444817)));
     const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
     const R inward_rectifier_potassium_current__i_K1 = (p[4] * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
-    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp_v(((R(25.0) - V) * (R(1) / R(5.98))), V, -0.24125335131922465, 6.0313337829806155)));
     const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
-    const R rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 = cmx_div(R(450.0), (R(1.0) + cmx_exp((((-R(45.0)) - V) * (R(1) / R(10.0))))));
-    const R rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1 = cmx_div(R(6.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(11.5))))));
+    const R rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 = cmx_div(R(450.0), (R(1.0) + cmx_exp_v((((-R(45.0)) - V) * (R(1) / R(10.0))), V, -0.14426950408889636, -6.4921276840003355)));
+    const R rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1 = cmx_div(R(6.0), (R(1.0) + cmx_exp_v(((V + R(30.0)) * (R(1) / R(11.5))), V, 0.12545174268599682, 3.7635522805799044)));
     const R rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1 = (rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 * rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1);
-    const R rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(26.0)) - V) * (R(1) / R(7.0))))));
-    const R rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 = cmx_div(R(3.0), (R(1.0) + cmx_exp((((-R(60.0)) - V) * (R(1) / R(20.0))))));
-    const R rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2 = cmx_div(R(1.12), (R(1.0) + cmx_exp(((V - R(60.0)) * (R(1) / R(20.0))))));
+    const R rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v((((-R(26.0)) - V) * (R(1) / R(7.0))), V, -0.20609929155556622, -5.358581580444722)));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 = cmx_div(R(3.0), (R(1.0) + cmx_exp_v((((-R(60.0)) - V) * (R(1) / R(20.0))), V, -0.07213475204444818, -4.32808512266689)));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2 = cmx_div(R(1.12), (R(1.0) + cmx_exp_v(((V - R(60.0)) * (R(1) / R(20.0))), V, 0.07213475204444818, -4.32808512266689)));
     const R rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2 = (rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 * rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2);
-    const R rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(88.0)) * (R(1) / R(24.0))))));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((V + R(88.0)) * (R(1) / R(24.0))), V, 0.06011229337037347, 5.289881816592866)));
     const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (y[18] + (reversal_potentials__P_kna * y[17])))));
     const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
-    const R slow_time_dependent_potassium_current_Xs_gate__alpha_xs = cmx_div(R(1400.0), sqrt((R(1.0) + cmx_exp(((R(5.0) - V) * (R(1) / R(6.0)))))));
-    const R slow_time_dependent_potassium_current_Xs_gate__beta_xs = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V - R(35.0)) * (R(1) / R(15.0))))));
+    const R slow_time_dependent_potassium_current_Xs_gate__alpha_xs = cmx_div(R(1400.0), sqrt((R(1.0) + cmx_exp_v(((R(5.0) - V) * (R(1) / R(6.0))), V, -0.2404491734814939, 1.2022458674074696))));
+    const R slow_time_dependent_potassium_current_Xs_gate__beta_xs = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((V - R(35.0)) * (R(1) / R(15.0))), V, 0.09617966939259756, -3.3662884287409147)));
     const R slow_time_dependent_potassium_current_Xs_gate__tau_xs = ((slow_time_dependent_potassium_current_Xs_gate__alpha_xs * slow_time_dependent_potassium_current_Xs_gate__beta_xs) + R(80.0));
-    const R slow_time_dependent_potassium_current_Xs_gate__xs_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(5.0)) - V) * (R(1) / R(14.0))))));
+    const R slow_time_dependent_potassium_current_Xs_gate__xs_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v((((-R(5.0)) - V) * (R(1) / R(14.0))), V, -0.10304964577778311, -0.5152482288889155)));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
-    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
-    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
+    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp_v(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.01890199102790382, 0.0) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0)))));
+    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp_v((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.005400568865115377, -0.0)) + (R(0.0353) * cmx_exp_v((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.05400568865115376, -0.0))));
     const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11] * (V - reversal_potentials__E_K));
-    const R transient_outward_current_r_gate__r_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(20.0) - V) * (R(1) / R(6.0))))));
-    const R transient_outward_current_r_gate__tau_r = ((R(9.5) * cmx_exp(((-cmx_ipow<2>((V + R(40.0)))) * (R(1) / R(1800.0))))) + R(0.8));
-    const R transient_outward_current_s_gate__s_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(20.0)) * (R(1) / R(5.0))))));
-    const R transient_outward_current_s_gate__tau_s = ((R(85.0) * cmx_exp(((-cmx_ipow<2>((V + R(45.0)))) * (R(1) / R(320.0))))) + cmx_div(R(5.0), (R(1.0) + cmx_exp(((V - R(20.0)) * (R(1) / R(5.0)))))) + R(3.0));
+    const R transient_outward_current_r_gate__r_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((R(20.0) - V) * (R(1) / R(6.0))), V, -0.2404491734814939, 4.8089834696298785)));
+    const R transient_outward_current_r_gate__tau_r = ((R(9.5) * cmx_exp_c(((-cmx_ipow<2>((V + R(40.0)))) * (R(1) / R(1800.0))), (-cmx_ipow<2>((V + R(40.0)))), 0.000801497244938313)) + R(0.8));
+    const R transient_outward_current_s_gate__s_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((V + R(20.0)) * (R(1) / R(5.0))), V, 0.2885390081777927, 5.7707801635558535)));
+    const R transient_outward_current_s_gate__tau_s = ((R(85.0) * cmx_exp_c(((-cmx_ipow<2>((V + R(45.0)))) * (R(1) / R(320.0))), (-cmx_ipow<2>((V + R(45.0)))), 0.004508422002778011)) + cmx_div(R(5.0), (R(1.0) + cmx_exp_v(((V - R(20.0)) * (R(1) / R(5.0))), V, 0.2885390081777927, -5.7707801635558535))) + R(3.0));
     if (RL) {
         g_k[1] = cmx_div(R(-1), rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1);
         g_inf[1] = rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf;
@@ -306,24 +306,24 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp_v(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.10801137730230755, -1.620170659534613)) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, y[13])));
     const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
     const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * y[13]), (y[13] + calcium_pump_current__K_pCa));
     const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, y[17])));
     const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
     const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, y[18])));
-    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
-    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
+    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp_c((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))), (((V - reversal_potentials__E_K) - R(200.0))), 0.0865617024533378)));
+    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp_c((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))), (((V - reversal_potentials__E_K) + R(100.0))), 0.0002885390081777927)) + cmx_exp_c((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))), (((V - reversal_potentials__E_K) - R(10.0))), 0.14426950408889636)), (R(1.0) + cmx_exp_c(((-R(0.5)) * (V - reversal_potentials__E_K)), ((V - reversal_potentials__E_K)), -0.7213475204444817)));
     const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
     const R inward_rectifier_potassium_current__i_K1 = (p[4] * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
-    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp_v(((R(25.0) - V) * (R(1) / R(5.98))), V, -0.24125335131922465, 6.0313337829806155)));
     const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
     const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (y[18] + (reversal_potentials__P_kna * y[17])))));
     const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
-    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
-    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
+    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp_v(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.01890199102790382, 0.0) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0)))));
+    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp_v((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.005400568865115377, -0.0)) + (R(0.0353) * cmx_exp_v((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.05400568865115376, -0.0))));
     const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11] * (V - reversal_potentials__E_K));
     return (((-R(1.0)) * (R(1) / R(1.0))) * (inward_rectifier_potassium_current__i_K1 + transient_outward_current__i_to + rapid_time_dependent_potassium_current__i_Kr + slow_time_dependent_potassium_current__i_Ks + L_type_Ca_current__i_CaL + sodium_potassium_pump_current__i_NaK + fast_sodium_current__i_Na + sodium_background_current__i_b_Na + sodium_calcium_exchanger_current__i_NaCa + calcium_background_current__i_b_Ca + potassium_pump_current__i_p_K + calcium_pump_current__i_p_Ca + i_stim));
 }
@@ -361,24 +361,24 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = R(0);
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * y[10] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * y[15] * cmx_exp_v(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.10801137730230755, -1.620170659534613)) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, y[13])));
     const R calcium_background_current__i_b_Ca = (calcium_background_current__g_bca * (V - reversal_potentials__E_Ca));
     const R calcium_pump_current__i_p_Ca = cmx_div((calcium_pump_current__g_pCa * y[13]), (y[13] + calcium_pump_current__K_pCa));
     const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, y[17])));
     const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
     const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, y[18])));
-    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
-    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
+    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp_c((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))), (((V - reversal_potentials__E_K) - R(200.0))), 0.0865617024533378)));
+    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp_c((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))), (((V - reversal_potentials__E_K) + R(100.0))), 0.0002885390081777927)) + cmx_exp_c((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))), (((V - reversal_potentials__E_K) - R(10.0))), 0.14426950408889636)), (R(1.0) + cmx_exp_c(((-R(0.5)) * (V - reversal_potentials__E_K)), ((V - reversal_potentials__E_K)), -0.7213475204444817)));
     const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
     const R inward_rectifier_potassium_current__i_K1 = (p[4] * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
-    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp_v(((R(25.0) - V) * (R(1) / R(5.98))), V, -0.24125335131922465, 6.0313337829806155)));
     const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
     const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (y[18] + (reversal_potentials__P_kna * y[17])))));
     const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
-    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
-    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
+    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp_v(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.01890199102790382, 0.0) * cmx_ipow<3>(y[17]) * calcium_dynamics__Ca_o) - (cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0) * cmx_ipow<3>(sodium_dynamics__Na_o) * y[13] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0)))));
+    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * y[17]), (y[17] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp_v((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.005400568865115377, -0.0)) + (R(0.0353) * cmx_exp_v((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.05400568865115376, -0.0))));
     const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11] * (V - reversal_potentials__E_K));
     const R dv = (((-R(1.0)) * (R(1) / R(1.0))) * (inward_rectifier_potassium_current__i_K1 + transient_outward_current__i_to + rapid_time_dependent_potassium_current__i_Kr + slow_time_dependent_potassium_current__i_Ks + L_type_Ca_current__i_CaL + sodium_potassium_pump_current__i_NaK + fast_sodium_current__i_Na + sodium_background_current__i_b_Na + sodium_calcium_exchanger_current__i_NaCa + calcium_background_current__i_b_Ca + potassium_pump_current__i_p_K + calcium_pump_current__i_p_Ca + i_stim));
     return -dv * cm;
@@ -395,43 +395,43 @@ CMX_HD inline void be_linear(R* y, R V, const R* p, R cm, R dt)
 {
     CMX_MATH_USING
     (void)p; (void)cm; (void)V;
-    const R L_type_Ca_current_d_gate__d_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(8.0)) - V) * (R(1) / R(7.5))))));
-    const R L_type_Ca_current_d_gate__alpha_d = (cmx_div(R(1.4), (R(1.0) + cmx_exp((((-R(35.0)) - V) * (R(1) / R(13.0)))))) + R(0.25));
-    const R L_type_Ca_current_d_gate__beta_d = cmx_div(R(1.4), (R(1.0) + cmx_exp(((V + R(5.0)) * (R(1) / R(5.0))))));
-    const R L_type_Ca_current_d_gate__gamma_d = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(50.0) - V) * (R(1) / R(20.0))))));
+    const R L_type_Ca_current_d_gate__d_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v((((-R(8.0)) - V) * (R(1) / R(7.5))), V, -0.19235933878519512, -1.538874710281561)));
+    const R L_type_Ca_current_d_gate__alpha_d = (cmx_div(R(1.4), (R(1.0) + cmx_exp_v((((-R(35.0)) - V) * (R(1) / R(13.0))), V, -0.11097654160684334, -3.8841789562395173))) + R(0.25));
+    const R L_type_Ca_current_d_gate__beta_d = cmx_div(R(1.4), (R(1.0) + cmx_exp_v(((V + R(5.0)) * (R(1) / R(5.0))), V, 0.2885390081777927, 1.4426950408889634)));
+    const R L_type_Ca_current_d_gate__gamma_d = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((R(50.0) - V) * (R(1) / R(20.0))), V, -0.07213475204444818, 3.6067376022224087)));
     const R L_type_Ca_current_d_gate__tau_d = ((L_type_Ca_current_d_gate__alpha_d * L_type_Ca_current_d_gate__beta_d) + L_type_Ca_current_d_gate__gamma_d);
-    const R L_type_Ca_current_f2_gate__f2_inf = (cmx_div(R(0.67), (R(1.0) + cmx_exp(((V + R(35.0)) * (R(1) / R(7.0)))))) + R(0.33));
-    const R L_type_Ca_current_f2_gate__tau_f2 = ((R(562.0) * cmx_exp(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(240.0))))) + cmx_div(R(31.0), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(10.0)))))) + cmx_div(R(80.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(10.0)))))));
-    const R L_type_Ca_current_f_gate__f_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(20.0)) * (R(1) / R(7.0))))));
-    const R L_type_Ca_current_f_gate__tau_f = ((R(1102.5) * cmx_exp(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(225.0))))) + cmx_div(R(200.0), (R(1.0) + cmx_exp(((R(13.0) - V) * (R(1) / R(10.0)))))) + cmx_div(R(180.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(10.0)))))) + R(20.0));
-    const R fast_sodium_current_h_gate__h_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp(((V + R(71.55)) * (R(1) / R(7.43)))))));
-    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.057) * cmx_exp(((-(V + R(80.0))) * (R(1) / R(6.8)))))) : (R(0.0)));
-    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(2.7) * cmx_exp((R(0.079) * V))) + (R(310000.0) * cmx_exp((R(0.3485) * V))))) : (cmx_div(R(0.77), (R(0.13) * (R(1.0) + cmx_exp(((V + R(10.66)) * (R(1) / (-R(11.1))))))))));
+    const R L_type_Ca_current_f2_gate__f2_inf = (cmx_div(R(0.67), (R(1.0) + cmx_exp_v(((V + R(35.0)) * (R(1) / R(7.0))), V, 0.2060992915555662, 7.213475204444817))) + R(0.33));
+    const R L_type_Ca_current_f2_gate__tau_f2 = ((R(562.0) * cmx_exp_c(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(240.0))), (-cmx_ipow<2>((V + R(27.0)))), 0.0060112293370373476)) + cmx_div(R(31.0), (R(1.0) + cmx_exp_v(((R(25.0) - V) * (R(1) / R(10.0))), V, -0.14426950408889636, 3.6067376022224087))) + cmx_div(R(80.0), (R(1.0) + cmx_exp_v(((V + R(30.0)) * (R(1) / R(10.0))), V, 0.14426950408889636, 4.32808512266689))));
+    const R L_type_Ca_current_f_gate__f_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((V + R(20.0)) * (R(1) / R(7.0))), V, 0.20609929155556622, 4.121985831111324)));
+    const R L_type_Ca_current_f_gate__tau_f = ((R(1102.5) * cmx_exp_c(((-cmx_ipow<2>((V + R(27.0)))) * (R(1) / R(225.0))), (-cmx_ipow<2>((V + R(27.0)))), 0.006411977959506504)) + cmx_div(R(200.0), (R(1.0) + cmx_exp_v(((R(13.0) - V) * (R(1) / R(10.0))), V, -0.14426950408889636, 1.8755035531556525))) + cmx_div(R(180.0), (R(1.0) + cmx_exp_v(((V + R(30.0)) * (R(1) / R(10.0))), V, 0.14426950408889636, 4.32808512266689))) + R(20.0));
+    const R fast_sodium_current_h_gate__h_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp_v(((V + R(71.55)) * (R(1) / R(7.43))), V, 0.19417160711829928, 13.89297848931431))));
+    const R fast_sodium_current_h_gate__alpha_h = (((V < (-R(40.0)))) ? ((R(0.057) * cmx_exp_v(((-(V + R(80.0))) * (R(1) / R(6.8))), V, -0.2121610354248476, -16.972882833987807))) : (R(0.0)));
+    const R fast_sodium_current_h_gate__beta_h = (((V < (-R(40.0)))) ? (((R(2.7) * cmx_exp_v((R(0.079) * V), V, 0.11397290823022813, 0.0)) + (R(310000.0) * cmx_exp_v((R(0.3485) * V), V, 0.5027792217498037, 0.0)))) : (cmx_div(R(0.77), (R(0.13) * (R(1.0) + cmx_exp_v(((V + R(10.66)) * (R(1) / (-R(11.1)))), V, -0.12997252620621294, -1.3855071293582297))))));
     const R fast_sodium_current_h_gate__tau_h = cmx_div(R(1.0), (fast_sodium_current_h_gate__alpha_h + fast_sodium_current_h_gate__beta_h));
-    const R fast_sodium_current_j_gate__j_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp(((V + R(71.55)) * (R(1) / R(7.43)))))));
-    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(25428.0)) * cmx_exp((R(0.2444) * V))) - (R(6.948e-06) * cmx_exp(((-R(0.04391)) * V)))) * (V + R(37.78))), (R(1.0) + cmx_exp((R(0.311) * (V + R(79.23))))))) : (R(0.0)));
-    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.02424) * cmx_exp(((-R(0.01052)) * V))), (R(1.0) + cmx_exp(((-R(0.1378)) * (V + R(40.14))))))) : (cmx_div((R(0.6) * cmx_exp((R(0.057) * V))), (R(1.0) + cmx_exp(((-R(0.1)) * (V + R(32.0))))))));
+    const R fast_sodium_current_j_gate__j_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp_v(((V + R(71.55)) * (R(1) / R(7.43))), V, 0.19417160711829928, 13.89297848931431))));
+    const R fast_sodium_current_j_gate__alpha_j = (((V < (-R(40.0)))) ? (cmx_div(((((-R(25428.0)) * cmx_exp_v((R(0.2444) * V), V, 0.3525946679932627, 0.0)) - (R(6.948e-06) * cmx_exp_v(((-R(0.04391)) * V), V, -0.06334873924543438, -0.0))) * (V + R(37.78))), (R(1.0) + cmx_exp_v((R(0.311) * (V + R(79.23))), V, 0.44867815771646763, 35.54877043587573)))) : (R(0.0)));
+    const R fast_sodium_current_j_gate__beta_j = (((V < (-R(40.0)))) ? (cmx_div((R(0.02424) * cmx_exp_v(((-R(0.01052)) * V), V, -0.015177151830151895, -0.0)), (R(1.0) + cmx_exp_v(((-R(0.1378)) * (V + R(40.14))), V, -0.19880337663449915, -7.979967538108797)))) : (cmx_div((R(0.6) * cmx_exp_v((R(0.057) * V), V, 0.08223361733067092, 0.0)), (R(1.0) + cmx_exp_v(((-R(0.1)) * (V + R(32.0))), V, -0.14426950408889636, -4.616624130844683)))));
     const R fast_sodium_current_j_gate__tau_j = cmx_div(R(1.0), (fast_sodium_current_j_gate__alpha_j + fast_sodium_current_j_gate__beta_j));
-    const R fast_sodium_current_m_gate__m_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp((((-R(56.86)) - V) * (R(1) / R(9.03)))))));
-    const R fast_sodium_current_m_gate__alpha_m = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(60.0)) - V) * (R(1) / R(5.0))))));
-    const R fast_sodium_current_m_gate__beta_m = (cmx_div(R(0.1), (R(1.0) + cmx_exp(((V + R(35.0)) * (R(1) / R(5.0)))))) + cmx_div(R(0.1), (R(1.0) + cmx_exp(((V - R(50.0)) * (R(1) / R(200.0)))))));
+    const R fast_sodium_current_m_gate__m_inf = cmx_div(R(1.0), cmx_ipow<2>((R(1.0) + cmx_exp_v((((-R(56.86)) - V) * (R(1) / R(9.03))), V, -0.1597668926787335, -9.084345517712785))));
+    const R fast_sodium_current_m_gate__alpha_m = cmx_div(R(1.0), (R(1.0) + cmx_exp_v((((-R(60.0)) - V) * (R(1) / R(5.0))), V, -0.2885390081777927, -17.31234049066756)));
+    const R fast_sodium_current_m_gate__beta_m = (cmx_div(R(0.1), (R(1.0) + cmx_exp_v(((V + R(35.0)) * (R(1) / R(5.0))), V, 0.2885390081777927, 10.098865286222743))) + cmx_div(R(0.1), (R(1.0) + cmx_exp_v(((V - R(50.0)) * (R(1) / R(200.0))), V, 0.007213475204444817, -0.36067376022224085))));
     const R fast_sodium_current_m_gate__tau_m = (fast_sodium_current_m_gate__alpha_m * fast_sodium_current_m_gate__beta_m);
-    const R rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 = cmx_div(R(450.0), (R(1.0) + cmx_exp((((-R(45.0)) - V) * (R(1) / R(10.0))))));
-    const R rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1 = cmx_div(R(6.0), (R(1.0) + cmx_exp(((V + R(30.0)) * (R(1) / R(11.5))))));
+    const R rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 = cmx_div(R(450.0), (R(1.0) + cmx_exp_v((((-R(45.0)) - V) * (R(1) / R(10.0))), V, -0.14426950408889636, -6.4921276840003355)));
+    const R rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1 = cmx_div(R(6.0), (R(1.0) + cmx_exp_v(((V + R(30.0)) * (R(1) / R(11.5))), V, 0.12545174268599682, 3.7635522805799044)));
     const R rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1 = (rapid_time_dependent_potassium_current_Xr1_gate__alpha_xr1 * rapid_time_dependent_potassium_current_Xr1_gate__beta_xr1);
-    const R rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(26.0)) - V) * (R(1) / R(7.0))))));
-    const R rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 = cmx_div(R(3.0), (R(1.0) + cmx_exp((((-R(60.0)) - V) * (R(1) / R(20.0))))));
-    const R rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2 = cmx_div(R(1.12), (R(1.0) + cmx_exp(((V - R(60.0)) * (R(1) / R(20.0))))));
+    const R rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v((((-R(26.0)) - V) * (R(1) / R(7.0))), V, -0.20609929155556622, -5.358581580444722)));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 = cmx_div(R(3.0), (R(1.0) + cmx_exp_v((((-R(60.0)) - V) * (R(1) / R(20.0))), V, -0.07213475204444818, -4.32808512266689)));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2 = cmx_div(R(1.12), (R(1.0) + cmx_exp_v(((V - R(60.0)) * (R(1) / R(20.0))), V, 0.07213475204444818, -4.32808512266689)));
     const R rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2 = (rapid_time_dependent_potassium_current_Xr2_gate__alpha_xr2 * rapid_time_dependent_potassium_current_Xr2_gate__beta_xr2);
-    const R rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(88.0)) * (R(1) / R(24.0))))));
-    const R slow_time_dependent_potassium_current_Xs_gate__alpha_xs = cmx_div(R(1400.0), sqrt((R(1.0) + cmx_exp(((R(5.0) - V) * (R(1) / R(6.0)))))));
-    const R slow_time_dependent_potassium_current_Xs_gate__beta_xs = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V - R(35.0)) * (R(1) / R(15.0))))));
+    const R rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((V + R(88.0)) * (R(1) / R(24.0))), V, 0.06011229337037347, 5.289881816592866)));
+    const R slow_time_dependent_potassium_current_Xs_gate__alpha_xs = cmx_div(R(1400.0), sqrt((R(1.0) + cmx_exp_v(((R(5.0) - V) * (R(1) / R(6.0))), V, -0.2404491734814939, 1.2022458674074696))));
+    const R slow_time_dependent_potassium_current_Xs_gate__beta_xs = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((V - R(35.0)) * (R(1) / R(15.0))), V, 0.09617966939259756, -3.3662884287409147)));
     const R slow_time_dependent_potassium_current_Xs_gate__tau_xs = ((slow_time_dependent_potassium_current_Xs_gate__alpha_xs * slow_time_dependent_potassium_current_Xs_gate__beta_xs) + R(80.0));
-    const R slow_time_dependent_potassium_current_Xs_gate__xs_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp((((-R(5.0)) - V) * (R(1) / R(14.0))))));
-    const R transient_outward_current_r_gate__r_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((R(20.0) - V) * (R(1) / R(6.0))))));
-    const R transient_outward_current_r_gate__tau_r = ((R(9.5) * cmx_exp(((-cmx_ipow<2>((V + R(40.0)))) * (R(1) / R(1800.0))))) + R(0.8));
-    const R transient_outward_current_s_gate__s_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(((V + R(20.0)) * (R(1) / R(5.0))))));
-    const R transient_outward_current_s_gate__tau_s = ((R(85.0) * cmx_exp(((-cmx_ipow<2>((V + R(45.0)))) * (R(1) / R(320.0))))) + cmx_div(R(5.0), (R(1.0) + cmx_exp(((V - R(20.0)) * (R(1) / R(5.0)))))) + R(3.0));
+    const R slow_time_dependent_potassium_current_Xs_gate__xs_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v((((-R(5.0)) - V) * (R(1) / R(14.0))), V, -0.10304964577778311, -0.5152482288889155)));
+    const R transient_outward_current_r_gate__r_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((R(20.0) - V) * (R(1) / R(6.0))), V, -0.2404491734814939, 4.8089834696298785)));
+    const R transient_outward_current_r_gate__tau_r = ((R(9.5) * cmx_exp_c(((-cmx_ipow<2>((V + R(40.0)))) * (R(1) / R(1800.0))), (-cmx_ipow<2>((V + R(40.0)))), 0.000801497244938313)) + R(0.8));
+    const R transient_outward_current_s_gate__s_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((V + R(20.0)) * (R(1) / R(5.0))), V, 0.2885390081777927, 5.7707801635558535)));
+    const R transient_outward_current_s_gate__tau_s = ((R(85.0) * cmx_exp_c(((-cmx_ipow<2>((V + R(45.0)))) * (R(1) / R(320.0))), (-cmx_ipow<2>((V + R(45.0)))), 0.004508422002778011)) + cmx_div(R(5.0), (R(1.0) + cmx_exp_v(((V - R(20.0)) * (R(1) / R(5.0))), V, 0.2885390081777927, -5.7707801635558535))) + R(3.0));
     y[1] = (y[1] + ((rapid_time_dependent_potassium_current_Xr1_gate__xr1_inf / rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1)) * dt) / (R(1) - (((-R(1)) / rapid_time_dependent_potassium_current_Xr1_gate__tau_xr1)) * dt);
     y[2] = (y[2] + ((rapid_time_dependent_potassium_current_Xr2_gate__xr2_inf / rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2)) * dt) / (R(1) - (((-R(1)) / rapid_time_dependent_potassium_current_Xr2_gate__tau_xr2)) * dt);
     y[3] = (y[3] + ((slow_time_dependent_potassium_current_Xs_gate__xs_inf / slow_time_dependent_potassium_current_Xs_gate__tau_xs)) * dt) / (R(1) - (((-R(1)) / slow_time_dependent_potassium_current_Xs_gate__tau_xs)) * dt);
@@ -500,7 +500,7 @@ CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * g[3] * cmx_exp_v(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.10801137730230755, -1.620170659534613)) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R L_type_Ca_current_fCass_gate__fCass_inf = (cmx_div(R(0.6), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(0.4));
     const R L_type_Ca_current_fCass_gate__tau_fCass = (cmx_div(R(80.0), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(2.0));
     const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, g[1])));
@@ -520,17 +520,17 @@ CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const
     const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, g[5])));
     const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
     const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, g[6])));
-    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
-    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
+    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp_c((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))), (((V - reversal_potentials__E_K) - R(200.0))), 0.0865617024533378)));
+    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp_c((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))), (((V - reversal_potentials__E_K) + R(100.0))), 0.0002885390081777927)) + cmx_exp_c((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))), (((V - reversal_potentials__E_K) - R(10.0))), 0.14426950408889636)), (R(1.0) + cmx_exp_c(((-R(0.5)) * (V - reversal_potentials__E_K)), ((V - reversal_potentials__E_K)), -0.7213475204444817)));
     const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
     const R inward_rectifier_potassium_current__i_K1 = (p[4] * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
-    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp_v(((R(25.0) - V) * (R(1) / R(5.98))), V, -0.24125335131922465, 6.0313337829806155)));
     const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
     const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))));
     const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
-    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(g[5]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * g[1] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
-    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]), (g[5] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
+    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp_v(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.01890199102790382, 0.0) * cmx_ipow<3>(g[5]) * calcium_dynamics__Ca_o) - (cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0) * cmx_ipow<3>(sodium_dynamics__Na_o) * g[1] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0)))));
+    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]), (g[5] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp_v((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.005400568865115377, -0.0)) + (R(0.0353) * cmx_exp_v((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.05400568865115376, -0.0))));
     const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11] * (V - reversal_potentials__E_K));
     res[0] = g[0] - y[10] - dt * cmx_div((L_type_Ca_current_fCass_gate__fCass_inf - g[0]), L_type_Ca_current_fCass_gate__tau_fCass);
     res[1] = g[1] - y[13] - dt * (calcium_dynamics__Ca_i_bufc * (((((calcium_dynamics__i_leak - calcium_dynamics__i_up) * calcium_dynamics__V_sr) * (R(1) / membrane__V_c)) + calcium_dynamics__i_xfer) - ((((calcium_background_current__i_b_Ca + calcium_pump_current__i_p_Ca) - (R(2.0) * sodium_calcium_exchanger_current__i_NaCa)) * membrane__Cm) * (R(1) / (R(2.0) * membrane__V_c * membrane__F)))));
@@ -595,7 +595,7 @@ CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const
     const R potassium_dynamics__K_o = R(5.4);
     const R i_stim = i_stim_area / cm;
     (void)i_stim;
-    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R L_type_Ca_current__i_CaL = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div((((p[1] * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((R(0.25) * g[3] * cmx_exp_v(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.10801137730230755, -1.620170659534613)) - calcium_dynamics__Ca_o)), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R L_type_Ca_current_fCass_gate__fCass_inf = (cmx_div(R(0.6), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(0.4));
     const R L_type_Ca_current_fCass_gate__tau_fCass = (cmx_div(R(80.0), (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05)))))) + R(2.0));
     const R reversal_potentials__E_Ca = (((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(calcium_dynamics__Ca_o, g[1])));
@@ -615,20 +615,20 @@ CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const
     const R reversal_potentials__E_Na = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(sodium_dynamics__Na_o, g[5])));
     const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6] * (V - reversal_potentials__E_Na));
     const R reversal_potentials__E_K = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div(potassium_dynamics__K_o, g[6])));
-    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))));
-    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))), (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))));
+    const R inward_rectifier_potassium_current__alpha_K1 = cmx_div(R(0.1), (R(1.0) + cmx_exp_c((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))), (((V - reversal_potentials__E_K) - R(200.0))), 0.0865617024533378)));
+    const R inward_rectifier_potassium_current__beta_K1 = cmx_div(((R(3.0) * cmx_exp_c((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))), (((V - reversal_potentials__E_K) + R(100.0))), 0.0002885390081777927)) + cmx_exp_c((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))), (((V - reversal_potentials__E_K) - R(10.0))), 0.14426950408889636)), (R(1.0) + cmx_exp_c(((-R(0.5)) * (V - reversal_potentials__E_K)), ((V - reversal_potentials__E_K)), -0.7213475204444817)));
     const R inward_rectifier_potassium_current__xK1_inf = cmx_div(inward_rectifier_potassium_current__alpha_K1, (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1));
     const R inward_rectifier_potassium_current__i_K1 = (p[4] * inward_rectifier_potassium_current__xK1_inf * (V - reversal_potentials__E_K));
-    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R potassium_pump_current__i_p_K = cmx_div((potassium_pump_current__g_pK * (V - reversal_potentials__E_K)), (R(1.0) + cmx_exp_v(((R(25.0) - V) * (R(1) / R(5.98))), V, -0.24125335131922465, 6.0313337829806155)));
     const R rapid_time_dependent_potassium_current__i_Kr = (rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2] * (V - reversal_potentials__E_K));
     const R reversal_potentials__E_Ks = (((membrane__R * membrane__T) * (R(1) / membrane__F)) * cmx_log(cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))));
     const R slow_time_dependent_potassium_current__i_Ks = (slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3]) * (V - reversal_potentials__E_Ks));
     const R sodium_background_current__i_b_Na = (sodium_background_current__g_bna * (V - reversal_potentials__E_Na));
-    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(g[5]) * calcium_dynamics__Ca_o) - (cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * g[1] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
-    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]), (g[5] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
+    const R sodium_calcium_exchanger_current__i_NaCa = cmx_div((sodium_calcium_exchanger_current__K_NaCa * ((cmx_exp_v(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.01890199102790382, 0.0) * cmx_ipow<3>(g[5]) * calcium_dynamics__Ca_o) - (cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0) * cmx_ipow<3>(sodium_dynamics__Na_o) * g[1] * sodium_calcium_exchanger_current__alpha))), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0)))));
+    const R sodium_potassium_pump_current__i_NaK = cmx_div(cmx_div((((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]), (g[5] + sodium_potassium_pump_current__K_mNa)), (R(1.0) + (R(0.1245) * cmx_exp_v((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.005400568865115377, -0.0)) + (R(0.0353) * cmx_exp_v((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.05400568865115376, -0.0))));
     const R transient_outward_current__i_to = (transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11] * (V - reversal_potentials__E_K));
-    const R d_L_type_Ca_current__i_CaL__d0 = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div(((cmx_div(((R(1) * p[1] * y[7] * y[8] * y[9] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F))), (membrane__R * membrane__T)) * ((R(0.25) * g[3] * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))) - calcium_dynamics__Ca_o))), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
-    const R d_L_type_Ca_current__i_CaL__d3 = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div(((((R(1) * R(0.25) * cmx_exp(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))) * ((p[1] * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))))), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R d_L_type_Ca_current__i_CaL__d0 = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div(((cmx_div(((R(1) * p[1] * y[7] * y[8] * y[9] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F))), (membrane__R * membrane__T)) * ((R(0.25) * g[3] * cmx_exp_v(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.10801137730230755, -1.620170659534613)) - calcium_dynamics__Ca_o))), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
+    const R d_L_type_Ca_current__i_CaL__d3 = cmx_remove_singularity(V, R(15.00000000000003), R(1.335688032984785e-06), [&](const R V) { return cmx_div(((((R(1) * R(0.25) * cmx_exp_v(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.10801137730230755, -1.620170659534613))) * ((p[1] * y[7] * y[8] * y[9] * g[0] * R(4.0) * (V - R(15.0)) * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))))), cmx_expm1(((R(2.0) * (V - R(15.0)) * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R d_L_type_Ca_current_fCass_gate__fCass_inf__d3 = (cmx_div((-(R(0.6) * ((R(2.0) * cmx_ipow<1>((g[3] * (R(1) / R(0.05)))) * cmx_div(R(1), R(0.05)))))), ((R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))) * (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))))));
     const R d_L_type_Ca_current_fCass_gate__tau_fCass__d3 = (cmx_div((-(R(80.0) * ((R(2.0) * cmx_ipow<1>((g[3] * (R(1) / R(0.05)))) * cmx_div(R(1), R(0.05)))))), ((R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))) * (R(1.0) + cmx_ipow<2>((g[3] * (R(1) / R(0.05))))))));
     const R d_reversal_potentials__E_Ca__d1 = ((cmx_div(cmx_div((-(calcium_dynamics__Ca_o * R(1))), (g[1] * g[1])), cmx_div(calcium_dynamics__Ca_o, g[1])) * ((R(0.5) * membrane__R * membrane__T) * (R(1) / membrane__F))));
@@ -654,20 +654,20 @@ CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const
     const R d_reversal_potentials__E_Na__d5 = ((cmx_div(cmx_div((-(sodium_dynamics__Na_o * R(1))), (g[5] * g[5])), cmx_div(sodium_dynamics__Na_o, g[5])) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
     const R d_fast_sodium_current__i_Na__d5 = (((-d_reversal_potentials__E_Na__d5) * p[0] * cmx_ipow<3>(y[4]) * y[5] * y[6]));
     const R d_reversal_potentials__E_K__d6 = ((cmx_div(cmx_div((-(potassium_dynamics__K_o * R(1))), (g[6] * g[6])), cmx_div(potassium_dynamics__K_o, g[6])) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
-    const R d_inward_rectifier_potassium_current__alpha_K1__d6 = cmx_div((-(R(0.1) * ((cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0)))) * (((-d_reversal_potentials__E_K__d6) * R(0.06))))))), ((R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))))) * (R(1.0) + cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0)))))));
-    const R d_inward_rectifier_potassium_current__beta_K1__d6 = cmx_div((((((cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0)))) * ((((-d_reversal_potentials__E_K__d6)) * R(0.0002)))) * R(3.0))) + (cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0)))) * (((-d_reversal_potentials__E_K__d6) * R(0.1))))) * (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))) - ((R(3.0) * cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))))) + cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))))) * ((cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K))) * (((-d_reversal_potentials__E_K__d6) * (-R(0.5))))))), ((R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K)))) * (R(1.0) + cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K))))));
+    const R d_inward_rectifier_potassium_current__alpha_K1__d6 = cmx_div((-(R(0.1) * ((cmx_exp((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0)))) * (((-d_reversal_potentials__E_K__d6) * R(0.06))))))), ((R(1.0) + cmx_exp_c((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))), (((V - reversal_potentials__E_K) - R(200.0))), 0.0865617024533378)) * (R(1.0) + cmx_exp_c((R(0.06) * ((V - reversal_potentials__E_K) - R(200.0))), (((V - reversal_potentials__E_K) - R(200.0))), 0.0865617024533378))));
+    const R d_inward_rectifier_potassium_current__beta_K1__d6 = cmx_div((((((cmx_exp((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0)))) * ((((-d_reversal_potentials__E_K__d6)) * R(0.0002)))) * R(3.0))) + (cmx_exp((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0)))) * (((-d_reversal_potentials__E_K__d6) * R(0.1))))) * (R(1.0) + cmx_exp_c(((-R(0.5)) * (V - reversal_potentials__E_K)), ((V - reversal_potentials__E_K)), -0.7213475204444817)) - ((R(3.0) * cmx_exp_c((R(0.0002) * ((V - reversal_potentials__E_K) + R(100.0))), (((V - reversal_potentials__E_K) + R(100.0))), 0.0002885390081777927)) + cmx_exp_c((R(0.1) * ((V - reversal_potentials__E_K) - R(10.0))), (((V - reversal_potentials__E_K) - R(10.0))), 0.14426950408889636)) * ((cmx_exp(((-R(0.5)) * (V - reversal_potentials__E_K))) * (((-d_reversal_potentials__E_K__d6) * (-R(0.5))))))), ((R(1.0) + cmx_exp_c(((-R(0.5)) * (V - reversal_potentials__E_K)), ((V - reversal_potentials__E_K)), -0.7213475204444817)) * (R(1.0) + cmx_exp_c(((-R(0.5)) * (V - reversal_potentials__E_K)), ((V - reversal_potentials__E_K)), -0.7213475204444817))));
     const R d_inward_rectifier_potassium_current__xK1_inf__d6 = cmx_div((d_inward_rectifier_potassium_current__alpha_K1__d6 * (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1) - inward_rectifier_potassium_current__alpha_K1 * (d_inward_rectifier_potassium_current__alpha_K1__d6 + d_inward_rectifier_potassium_current__beta_K1__d6)), ((inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1) * (inward_rectifier_potassium_current__alpha_K1 + inward_rectifier_potassium_current__beta_K1)));
     const R d_inward_rectifier_potassium_current__i_K1__d6 = ((d_inward_rectifier_potassium_current__xK1_inf__d6 * p[4] * (V - reversal_potentials__E_K)) + ((-d_reversal_potentials__E_K__d6) * p[4] * inward_rectifier_potassium_current__xK1_inf));
-    const R d_potassium_pump_current__i_p_K__d6 = cmx_div((((-d_reversal_potentials__E_K__d6) * potassium_pump_current__g_pK)), (R(1.0) + cmx_exp(((R(25.0) - V) * (R(1) / R(5.98))))));
+    const R d_potassium_pump_current__i_p_K__d6 = cmx_div((((-d_reversal_potentials__E_K__d6) * potassium_pump_current__g_pK)), (R(1.0) + cmx_exp_v(((R(25.0) - V) * (R(1) / R(5.98))), V, -0.24125335131922465, 6.0313337829806155)));
     const R d_rapid_time_dependent_potassium_current__i_Kr__d6 = (((-d_reversal_potentials__E_K__d6) * rapid_time_dependent_potassium_current__ScaleFactorGkr * p[2] * sqrt((potassium_dynamics__K_o * (R(1) / R(5.4)))) * y[1] * y[2]));
     const R d_reversal_potentials__E_Ks__d5 = ((cmx_div(cmx_div((-((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)) * (((R(1) * reversal_potentials__P_kna))))), ((g[6] + (reversal_potentials__P_kna * g[5])) * (g[6] + (reversal_potentials__P_kna * g[5])))), cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
     const R d_reversal_potentials__E_Ks__d6 = ((cmx_div(cmx_div((-((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)) * (R(1)))), ((g[6] + (reversal_potentials__P_kna * g[5])) * (g[6] + (reversal_potentials__P_kna * g[5])))), cmx_div((potassium_dynamics__K_o + (reversal_potentials__P_kna * sodium_dynamics__Na_o)), (g[6] + (reversal_potentials__P_kna * g[5])))) * ((membrane__R * membrane__T) * (R(1) / membrane__F))));
     const R d_slow_time_dependent_potassium_current__i_Ks__d5 = (((-d_reversal_potentials__E_Ks__d5) * slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3])));
     const R d_slow_time_dependent_potassium_current__i_Ks__d6 = (((-d_reversal_potentials__E_Ks__d6) * slow_time_dependent_potassium_current__ScaleFactorGks * p[3] * cmx_ipow<2>(y[3])));
     const R d_sodium_background_current__i_b_Na__d5 = (((-d_reversal_potentials__E_Na__d5) * sodium_background_current__g_bna));
-    const R d_sodium_calcium_exchanger_current__i_NaCa__d1 = cmx_div((((-((R(1) * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * cmx_ipow<3>(sodium_dynamics__Na_o) * sodium_calcium_exchanger_current__alpha))) * sodium_calcium_exchanger_current__K_NaCa)), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
-    const R d_sodium_calcium_exchanger_current__i_NaCa__d5 = cmx_div((((((R(3.0) * cmx_ipow<2>(g[5]) * R(1)) * cmx_exp(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T)))) * calcium_dynamics__Ca_o)) * sodium_calcium_exchanger_current__K_NaCa)), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))))));
-    const R d_sodium_potassium_pump_current__i_NaK__d5 = cmx_div(cmx_div((((R(1) * ((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))))) * (g[5] + sodium_potassium_pump_current__K_mNa) - (((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]) * (R(1))), ((g[5] + sodium_potassium_pump_current__K_mNa) * (g[5] + sodium_potassium_pump_current__K_mNa))), (R(1.0) + (R(0.1245) * cmx_exp((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))) + (R(0.0353) * cmx_exp((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T)))))));
+    const R d_sodium_calcium_exchanger_current__i_NaCa__d1 = cmx_div((((-((R(1) * cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0) * cmx_ipow<3>(sodium_dynamics__Na_o) * sodium_calcium_exchanger_current__alpha))) * sodium_calcium_exchanger_current__K_NaCa)), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0)))));
+    const R d_sodium_calcium_exchanger_current__i_NaCa__d5 = cmx_div((((((R(3.0) * cmx_ipow<2>(g[5]) * R(1)) * cmx_exp_v(((sodium_calcium_exchanger_current__gamma * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.01890199102790382, 0.0) * calcium_dynamics__Ca_o)) * sodium_calcium_exchanger_current__K_NaCa)), ((cmx_ipow<3>(sodium_calcium_exchanger_current__Km_Nai) + cmx_ipow<3>(sodium_dynamics__Na_o)) * (sodium_calcium_exchanger_current__Km_Ca + calcium_dynamics__Ca_o) * (R(1.0) + (sodium_calcium_exchanger_current__K_sat * cmx_exp_v((((sodium_calcium_exchanger_current__gamma - R(1.0)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.03510369762324995, -0.0)))));
+    const R d_sodium_potassium_pump_current__i_NaK__d5 = cmx_div(cmx_div((((R(1) * ((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))))) * (g[5] + sodium_potassium_pump_current__K_mNa) - (((sodium_potassium_pump_current__P_NaK * potassium_dynamics__K_o) * (R(1) / (potassium_dynamics__K_o + sodium_potassium_pump_current__K_mk))) * g[5]) * (R(1))), ((g[5] + sodium_potassium_pump_current__K_mNa) * (g[5] + sodium_potassium_pump_current__K_mNa))), (R(1.0) + (R(0.1245) * cmx_exp_v((((-R(0.1)) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.005400568865115377, -0.0)) + (R(0.0353) * cmx_exp_v((((-V) * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, -0.05400568865115376, -0.0))));
     const R d_transient_outward_current__i_to__d6 = (((-d_reversal_potentials__E_K__d6) * transient_outward_current__ScaleFactorIto * p[5] * y[12] * y[11]));
     J[0] = R(1) - dt * cmx_div((-R(1)), L_type_Ca_current_fCass_gate__tau_fCass);
     J[1] = R(0);
