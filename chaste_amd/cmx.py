@@ -46,14 +46,16 @@ class CmxOpts(ctypes.Structure):
                 ('ksp_max_its', ctypes.c_int32), ('refine_sweeps', ctypes.c_int32),
                 ('use_graph', ctypes.c_int32), ('mass_lumping', ctypes.c_int32),
                 ('extrapolate', ctypes.c_int32), ('operator_splitting', ctypes.c_int32),
-                ('state_variable_interpolation', ctypes.c_int32), ('separate_refinement', ctypes.c_int32)]
+                ('state_variable_interpolation', ctypes.c_int32), ('separate_refinement', ctypes.c_int32),
+                ('lookup_tables', ctypes.c_int32)]
 
 
 class CmxCells(ctypes.Structure):
     _fields_ = [('model', ctypes.c_int32), ('integrator', ctypes.c_int32), ('precision', ctypes.c_int32),
                 ('n_cells', ctypes.c_int32), ('dt', ctypes.c_double), ('t_end', _dp),
                 ('stimuli', ctypes.POINTER(CmxStimulus)), ('n_params', ctypes.c_int32), ('params', _dp),
-                ('threshold', ctypes.c_double), ('steps_per_launch', ctypes.c_int32)]
+                ('threshold', ctypes.c_double), ('steps_per_launch', ctypes.c_int32),
+                ('lookup_tables', ctypes.c_int32)]
 
 
 class CmxCellsStats(ctypes.Structure):
@@ -106,6 +108,8 @@ SIGNATURES = {
     'cmx_model_initial_state': (ctypes.c_int, [ctypes.c_int32, _dp]),
     'cmx_model_param': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), _dp]),
     'cmx_set_params': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp]),
+    'cmx_gate_table': (ctypes.c_int, [ctypes.c_int32, ctypes.c_double, ctypes.POINTER(ctypes.c_float), _ip, _ip,
+                                      _dp, _dp]),
     'cmx_autotune': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _ip,
                                     ctypes.POINTER(ctypes.c_int32), _dp]),
     'cmx_cells_run': (ctypes.c_int, [ctypes.POINTER(CmxCells), _dp, _dp, _dp, ctypes.POINTER(CmxCellsStats)]),
@@ -187,7 +191,7 @@ def param_info(model):
 
 
 def cells_run(model, integrator, t_end, dt, stimuli=None, params=None, y0=None, precision=FP64,
-              threshold=-30.0, steps_per_launch=0):
+              threshold=-30.0, steps_per_launch=0, lookup_tables=False):
     """Batched single cells (cmx_cells_run).  t_end [n]; stimuli: one
     (magnitude, start, duration[, period[, stop]]) per cell or None; params
     [n_params, n].  Returns (final states [S, n], props [4, n]: completed APs,
@@ -211,6 +215,7 @@ def cells_run(model, integrator, t_end, dt, stimuli=None, params=None, y0=None, 
         c.n_params, c.params = pa.shape[0], _p(pa)
         keep.append(pa)
     c.threshold, c.steps_per_launch = threshold, steps_per_launch
+    c.lookup_tables = 1 if lookup_tables else 0
     yi = None if y0 is None else np.ascontiguousarray(y0, dtype=np.float64)
     y = np.zeros((S, n))
     props = np.zeros((4, n))
@@ -220,6 +225,20 @@ def cells_run(model, integrator, t_end, dt, stimuli=None, params=None, y0=None, 
     if rc != 0:
         raise CmxError(rc, lib().cmx_cells_last_error().decode())
     return y, props, {k: getattr(stats, k) for k, _ in CmxCellsStats._fields_}
+
+
+def gate_table(model, dt):
+    """The gate lookup table of a model for Rush-Larsen steps of length dt:
+    (table [rows, width] float32, vmin, dv); row r is V = vmin + r dv, gate j
+    has its steady state in column 2j and its decay factor in 2j + 1."""
+    L = lib()
+    rows, width = ctypes.c_int32(), ctypes.c_int32()
+    vmin, dv = ctypes.c_double(), ctypes.c_double()
+    _check(L.cmx_gate_table(model, dt, None, ctypes.byref(rows), ctypes.byref(width), ctypes.byref(vmin),
+                            ctypes.byref(dv)))
+    t = np.zeros((rows.value, width.value), dtype=np.float32)
+    _check(L.cmx_gate_table(model, dt, t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), None, None, None, None))
+    return t, vmin.value, dv.value
 
 
 def slab_mesh(dim, ex, ey=0, ez=0, h=0.01):
@@ -318,11 +337,13 @@ class Monodomain:
                  dt_pde=0.01, dt_ode=0.01, am=1400.0, cm=1.0, ksp_rtol=1e-8, ksp_atol=0.0,
                  ksp_max_its=1000, refine_sweeps=0, fmt=FMT_AUTO, mass_lumping=False,
                  sigma_elem=None, use_graph=False, extrapolate=False, operator_splitting=False, svi=False,
-                 cell_types=None, separate_refinement=False):
+                 cell_types=None, separate_refinement=False, lookup_tables=False):
         """cell_types = (group_of [n], [(model, integrator), ...]) gives node i
         the cell type group_of[i] (model and integrator are then ignored).
         separate_refinement: mixed precision's FP64 true-residual check always
-        in a kernel of its own instead of in the last CG iteration."""
+        in a kernel of its own instead of in the last CG iteration.
+        lookup_tables: mixed precision with Rush-Larsen cells reads each
+        gate's steady state and decay factor from a table in V."""
         L = lib()
         self._keep = []
         coords = np.ascontiguousarray(coords, dtype=np.float64)
@@ -371,7 +392,7 @@ class Monodomain:
             t.n_groups, t.group_of, t.group_model, t.group_integrator = len(gm), _p(go, _ip), _p(gm, _ip), _p(gi, _ip)
         o = CmxOpts(precision, fmt, dt_pde, dt_ode, am, cm, ksp_rtol, ksp_atol, ksp_max_its,
                     refine_sweeps, 1 if use_graph else 0, 1 if mass_lumping else 0, int(extrapolate), 1 if operator_splitting else 0,
-                    1 if svi else 0, 1 if separate_refinement else 0)
+                    1 if svi else 0, 1 if separate_refinement else 0, 1 if lookup_tables else 0)
         self.dt = dt_pde
         ctx = ctypes.c_void_p()
         rc = L.cmx_create(ctypes.byref(m), ctypes.byref(t), ctypes.byref(o), ctypes.byref(ctx))

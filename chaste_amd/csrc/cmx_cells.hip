@@ -27,6 +27,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "cmx_context.h"
@@ -66,14 +67,20 @@ struct CellsArgs {
     double* __restrict__ peak;             // running peak of the current action potential
     double* __restrict__ last_peak;        // peak of the last completed one
     double* __restrict__ end;              // threshold crossing that ended it
+    // gate lookup table (GateCols layout), rows at lut_vmin + r / lut_inv_dv, or null
+    const float* __restrict__ lut;
+    float lut_vmin, lut_inv_dv;
+    int lut_rows;
+    int32_t* __restrict__ lut_miss;        // 1 + a cell whose voltage fell outside the table
 };
 
-template <class Model, class RS, int IG>
+template <class Model, class RS, int IG, bool LUT = false>
 __global__ __launch_bounds__(kBlock) void cells_kernel(CellsArgs<RS> a, int k0, int k1)
 {
     constexpr int NS = Model::NS;
     constexpr int NPd = Model::NP > 0 ? Model::NP : 1;
     constexpr bool RL = IG == CMX_RUSH_LARSEN;
+    static_assert(!LUT || RL, "lookup tables serve Rush-Larsen cells");
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n) return;
     const int kend = min(k1, a.n_steps[i]);
@@ -107,10 +114,16 @@ __global__ __launch_bounds__(kBlock) void cells_kernel(CellsArgs<RS> a, int k0, 
         RS dy[NS], gi[NS], gk[NS];
         Model::template eval<RS, RL>(y, V, is, p, cm, dy, gi, gk);
         const RS dv = Model::template dvdt<RS>(y, V, is, p, cm);
+        // with a table the gate outputs above are unused (and not computed)
+        float tgi[NS], tgd[NS];
+        if constexpr (LUT) {
+            if (!lut_gates<Model>(a.lut, a.lut_vmin, a.lut_inv_dv, a.lut_rows, (float)V, tgi, tgd)) *a.lut_miss = i + 1;
+        }
 #pragma unroll
         for (int s = 1; s < NS; ++s) {
             if (Model::gate(s)) {
-                if (RL) y[s] = gi[s] + (y[s] - gi[s]) * cmx_decay(gk[s], h);
+                if (LUT) y[s] = tgi[s] + (y[s] - tgi[s]) * tgd[s];
+                else if (RL) y[s] = gi[s] + (y[s] - gi[s]) * cmx_decay(gk[s], h);
                 else y[s] += h * dy[s];
             } else {
                 yd[s] += a.dt * (double)dy[s];
@@ -180,6 +193,12 @@ struct Buffers {
 template <class Model, class RS, int IG>
 void launch_cells(const CellsArgs<RS>& a, int k0, int k1, hipStream_t s)
 {
+    if constexpr (IG == CMX_RUSH_LARSEN && std::is_same<RS, float>::value) {
+        if (a.lut) {
+            cells_kernel<Model, RS, IG, true><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a, k0, k1);
+            return;
+        }
+    }
     cells_kernel<Model, RS, IG><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a, k0, k1);
 }
 
@@ -271,6 +290,23 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
     a.params = d_params;
     a.pdef = d_pdef;
     a.n_param_planes = c.n_params;
+    int32_t* d_miss = nullptr;
+    if (c.lookup_tables) {
+        if (!std::is_same<RS, float>::value || c.integrator != CMX_RUSH_LARSEN)
+            return cfail(CMX_E_ARG, "lookup tables need mixed precision and Rush-Larsen cells");
+        std::vector<float> table;
+        int rows = 0;
+        const char* err = gate_table(c.model, c.dt, 1.0, table, &rows);
+        if (err) return cfail(CMX_E_ARG, err);
+        float* d_lut;
+        if ((rc = B.upload(&d_lut, table))) return rc;
+        if ((rc = B.upload(&d_miss, std::vector<int32_t>(1, 0)))) return rc;
+        a.lut = d_lut;
+        a.lut_vmin = (float)kLutVmin;
+        a.lut_inv_dv = (float)(1.0 / kLutDv);
+        a.lut_rows = rows;
+        a.lut_miss = d_miss;
+    }
 
     hipStream_t stream = nullptr;
     CELLS_HIP(hipStreamCreate(&stream));
@@ -318,6 +354,13 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
     CELLS_HIP(hipStreamSynchronize(stream));
     float ms = 0.f;
     CELLS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (d_miss) {
+        int32_t miss = 0;
+        CELLS_HIP(hipMemcpy(&miss, d_miss, sizeof(miss), hipMemcpyDeviceToHost));
+        if (miss)
+            return cfail(CMX_E_NONFINITE, "cell " + std::to_string(order[miss - 1]) +
+                                              ": membrane voltage outside the gate lookup table (-250..350 mV)");
+    }
 
     // back to the caller's order
     CELLS_HIP(hipMemcpy(s_state.data(), d_state, s_state.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -601,6 +644,7 @@ int cmx_cells_run_noalloc(const cmx_cells* c, const double* y0, double* y_out, d
     if (!(c->dt > 0)) return cfail(CMX_E_ARG, "dt must be positive");
     if (c->n_params < 0 || c->n_params > P || (c->n_params > 0 && !c->params))
         return cfail(CMX_E_ARG, "n_params must be in [0, the model's parameter count] with params given");
+    if (c->lookup_tables) return cfail(CMX_E_ARG, "lookup tables need a table allocation: use cmx_cells_run");
     if (c->n_cells == 0) return CMX_OK;
     hipStream_t s = (hipStream_t)stream;
     return c->precision == CMX_MIXED ? run_cells_noalloc<float>(*c, S, P, y0, y_out, props, s)

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -86,6 +87,116 @@ struct ModelOrd {
     static constexpr bool gate(int i) { return ord_endo::kIsGate[i] != 0; }
 };
 
+// Gate columns of a model's lookup table: gate state s (Model::gate(s)) has
+// its steady state in column 2 col(s) and its decay factor in 2 col(s) + 1;
+// rows are padded to whole float4s.
+template <class Model>
+struct GateCols {
+    static constexpr int count()
+    {
+        int c = 0;
+        for (int i = 1; i < Model::NS; ++i) c += Model::gate(i) ? 1 : 0;
+        return c;
+    }
+    static constexpr int col(int s)
+    {
+        int c = 0;
+        for (int i = 1; i < s; ++i) c += Model::gate(i) ? 1 : 0;
+        return c;
+    }
+    static constexpr int width() { return (2 * count() + 3) / 4 * 4; }
+};
+
+// Rush-Larsen coefficients of every gate at voltage V (host, FP64): the
+// model's right-hand side at its initial state with parameters p.
+inline bool gate_coefficients(int model, double V, const double* p, double cm, double* gi, double* gk)
+{
+    double dy[64] = {0.0};
+    if (model == CMX_MODEL_LR91)
+        lr91::evaluate<double, true>(lr91::kInitialState, V, 0.0, p, cm, dy, gi, gk);
+    else if (model == CMX_MODEL_TT06_EPI)
+        tt06_epi::evaluate<double, true>(tt06_epi::kInitialState, V, 0.0, p, cm, dy, gi, gk);
+    else if (model == CMX_MODEL_ORD_ENDO)
+        ord_endo::evaluate<double, true>(ord_endo::kInitialState, V, 0.0, p, cm, dy, gi, gk);
+    else
+        return false;
+    return true;
+}
+
+// Gate lookup tables (mixed-precision Rush-Larsen cells): for
+// V = kLutVmin + r kLutDv mV, each gate's steady state and its decay factor
+// exp(g_k dt) over one ODE step of length dt, evaluated in FP64 by the model
+// itself and stored as floats in the GateCols layout.  The tables hold
+// V-only quantities, so a model whose gate coefficients change with its
+// parameters is refused.
+constexpr double kLutVmin = -250.0, kLutVmax = 350.0, kLutDv = 0.02;
+
+template <class Model>
+const char* fill_gate_table(int model, double dt, double cm, const double* pdef, int P, std::vector<float>& table,
+                            int* rows_out)
+{
+    constexpr int W = GateCols<Model>::width();
+    const int rows = (int)((kLutVmax - kLutVmin) / kLutDv + 0.5) + 1;
+    table.assign((size_t)rows * W, 0.0f);
+    std::vector<double> p(pdef, pdef + P), p2(P + 1);
+    for (int q = 0; q < P; ++q) p2[q] = 2.0 * pdef[q] + 1.0;
+    double gi[64], gk[64], gi2[64], gk2[64];
+    for (int r = 0; r < rows; ++r) {
+        const double V = kLutVmin + r * kLutDv;
+        gate_coefficients(model, V, p.data(), cm, gi, gk);
+        if (r % 500 == 0) {
+            gate_coefficients(model, V, p2.data(), cm, gi2, gk2);
+            for (int s = 1; s < Model::NS; ++s)
+                if (Model::gate(s) && (gi2[s] != gi[s] || gk2[s] != gk[s]))
+                    return "lookup tables: this model's gate coefficients depend on its parameters";
+        }
+        for (int s = 1; s < Model::NS; ++s) {
+            if (!Model::gate(s)) continue;
+            const int col = 2 * GateCols<Model>::col(s);
+            table[(size_t)r * W + col] = (float)gi[s];
+            table[(size_t)r * W + col + 1] = (float)std::exp(gk[s] * dt);
+            if (!std::isfinite(table[(size_t)r * W + col]) || !std::isfinite(table[(size_t)r * W + col + 1]))
+                return "lookup tables: a gate coefficient is not finite inside the table's voltage range";
+        }
+    }
+    *rows_out = rows;
+    return nullptr;
+}
+
+// Every gate's Rush-Larsen steady state and decay factor at V from a table
+// (rows at vmin + r / inv_dv), linearly interpolated between the two rows
+// around V.  Returns false when V lies outside the table (the values then
+// come from its first rows).
+template <class Model>
+__device__ __forceinline__ bool lut_gates(const float* __restrict__ lut, float vmin, float inv_dv, int rows, float V,
+                                          float* ginf, float* gdec)
+{
+    constexpr int W = GateCols<Model>::width();
+    const float x = (V - vmin) * inv_dv;
+    const bool inside = x >= 0.0f && x <= (float)(rows - 1);
+    const float xs = inside ? x : 0.0f;
+    int r = (int)xs;
+    r = r > rows - 2 ? rows - 2 : r;
+    const float f = xs - (float)r;
+    const float4* t0 = reinterpret_cast<const float4*>(lut + (size_t)r * W);
+    const float4* t1 = reinterpret_cast<const float4*>(lut + (size_t)(r + 1) * W);
+    float lo[W], hi[W];
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) {
+        const float4 u = t0[q], v = t1[q];
+        lo[4 * q] = u.x; lo[4 * q + 1] = u.y; lo[4 * q + 2] = u.z; lo[4 * q + 3] = u.w;
+        hi[4 * q] = v.x; hi[4 * q + 1] = v.y; hi[4 * q + 2] = v.z; hi[4 * q + 3] = v.w;
+    }
+#pragma unroll
+    for (int s = 1; s < Model::NS; ++s) {
+        if (!Model::gate(s)) continue;
+        const int c = 2 * GateCols<Model>::col(s);
+        ginf[s] = lo[c] + f * (hi[c] - lo[c]);
+        gdec[s] = lo[c + 1] + f * (hi[c + 1] - lo[c + 1]);
+    }
+    return inside;
+}
+
 // Metadata of a model id (CMX_MODEL_*); false for an unknown id.
 struct ModelMeta {
     int S, P;
@@ -111,6 +222,17 @@ inline bool model_meta(int model, ModelMeta* m)
     return true;
 }
 
+// The gate table of a model id for ODE steps of length dt (see
+// fill_gate_table); returns an error message, or null.
+inline const char* gate_table(int model, double dt, double cm, std::vector<float>& table, int* rows)
+{
+    ModelMeta mm;
+    if (!model_meta(model, &mm)) return "unknown cell model";
+    if (model == CMX_MODEL_LR91) return fill_gate_table<ModelLr91>(model, dt, cm, mm.pdef, mm.P, table, rows);
+    if (model == CMX_MODEL_ORD_ENDO) return fill_gate_table<ModelOrd>(model, dt, cm, mm.pdef, mm.P, table, rows);
+    return fill_gate_table<ModelTt06>(model, dt, cm, mm.pdef, mm.P, table, rows);
+}
+
 }  // namespace cmx
 
 // Cells of one (model, integrator) pair when a tissue mixes cell types
@@ -134,6 +256,10 @@ struct cmx_ctx {
     int split = 0, n_half = 1;   // operator splitting: ODE substeps per half step
     int width = 0, n_patterns = 0;
     int G = 1;                   // blocks of the reduction kernels
+    // gate lookup table (mixed Rush-Larsen cells): rows at lut_vmin + r lut_dv
+    float* lut = nullptr;
+    int lut_rows = 0;
+    double lut_vmin = 0.0, lut_dv = 0.0;
     long long nnz = 0;
     hipStream_t stream = nullptr;
     // device state

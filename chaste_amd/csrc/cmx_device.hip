@@ -37,6 +37,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <chrono>
 #include <cfloat>
 #include <cmath>
@@ -169,6 +170,10 @@ struct CellArgs {
     double* __restrict__ trace;
     int trace_k0, trace_len;
     size_t trace_slot;
+    // gate lookup table (GateCols layout), rows at lut_vmin + r / lut_inv_dv
+    const float* __restrict__ lut;
+    float lut_vmin, lut_inv_dv;
+    int lut_rows;
 };
 
 // Cell-kernel roles: the ICI step (V clamped over the step), the operator-
@@ -275,10 +280,11 @@ __device__ __forceinline__ bool integrate_with_voltage(const CellArgs<RS, RV>& a
 // (OperatorSplittingMonodomainSolver, heart/src/solver/electrics/monodomain/
 // OperatorSplittingMonodomainSolver.cpp:40-137) that voltage then runs the
 // previous step's second half-step of the cells.
-template <class Model, class RS, class RV, int IG, int MODE, bool GROUPED>
+template <class Model, class RS, class RV, int IG, int MODE, bool GROUPED, bool LUT = false>
 __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
 {
     constexpr bool RL = IG == CMX_RUSH_LARSEN;
+    static_assert(!LUT || (RL && MODE == kModeIci), "lookup tables serve Rush-Larsen ICI steps");
     constexpr int NS = Model::NS;
     constexpr int NP = Model::NP > 0 ? Model::NP : 1;
     // Thread j steps cell j of the launch, which sits at node i (GROUPED:
@@ -390,6 +396,13 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
         // dt_ode (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:106-128,
         // heart/src/odes/AbstractBackwardEulerCardiacCell.hpp:204-230).
         bool ok = true;
+        // V is clamped over the substeps: one table lookup serves them all
+        // (the gate outputs of the model evaluation below are then unused)
+        float tgi[NS], tgd[NS];
+        if constexpr (LUT) {
+            if (!lut_gates<Model>(a.lut, a.lut_vmin, a.lut_inv_dv, a.lut_rows, (float)V, tgi, tgd))
+                a.clock->nonfinite = 4;
+        }
         for (int ks = 0; ks < a.n_sub; ++ks) {
             const double ts = t + ks * a.dt_ode;
             const double h = (IG == CMX_FORWARD_EULER && ks + 1 == a.n_sub) ? t_next - ts : a.dt_ode;
@@ -409,7 +422,8 @@ __global__ __launch_bounds__(kBlock) void cell_step_kernel(CellArgs<RS, RV> a)
             const RS hh = (RS)h;
 #pragma unroll
             for (int s = 1; s < NS; ++s) {
-                if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_decay(gk[s], hh);
+                if (LUT && Model::gate(s)) y[s] = tgi[s] + (y[s] - tgi[s]) * tgd[s];
+                else if (RL && Model::gate(s)) y[s] = gi[s] + (y[s] - gi[s]) * cmx_decay(gk[s], hh);
                 else y[s] += hh * dy[s];
             }
         }
@@ -1580,6 +1594,12 @@ SettleArgs<RV> settle_args(cmx_ctx* c, const double* verdict, bool fused, bool r
 template <class Model, class RS, class RV, int IG, bool GROUPED>
 void launch_cell_grouping(cmx_ctx* c, int mode, int kind, int G, const CellArgs<RS, RV>& a)
 {
+    if constexpr (IG == CMX_RUSH_LARSEN && std::is_same<RS, float>::value) {
+        if (mode == kModeIci && a.lut) {
+            CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeIci, GROUPED, true>), G, kBlock, a);
+            return;
+        }
+    }
     if (mode == kModeIci) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeIci, GROUPED>), G, kBlock, a);
     else if (mode == kModeSplit) CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeSplit, GROUPED>), G, kBlock, a);
     else CMX_LAUNCH(c, kind, (cell_step_kernel<Model, RS, RV, IG, kModeSplitFinal, GROUPED>), G, kBlock, a);
@@ -1647,6 +1667,10 @@ int launch_cells(cmx_ctx* c, int mode, const double* verdict, int kind, bool fus
     a.trace_k0 = c->trace_k0;
     a.trace_len = c->trace_len;
     a.trace_slot = c->trace_slot;
+    a.lut = (std::is_same<RS, float>::value && c->cell_groups.empty()) ? c->lut : nullptr;
+    a.lut_vmin = (float)c->lut_vmin;
+    a.lut_inv_dv = c->lut_dv > 0 ? (float)(1.0 / c->lut_dv) : 0.0f;
+    a.lut_rows = c->lut_rows;
     if (c->cell_groups.empty()) {
         launch_cell_model<RS, RV>(c, c->model, c->integrator, mode, kind, grid_of(c->n), a);
     } else {
@@ -1950,6 +1974,17 @@ int prime_graphs(cmx_ctx* c, const Rows& R, int todo)
     return CMX_OK;
 }
 
+// A cell-kernel failure flagged in the clock (a root cause that can also make
+// the following solve fail, so it is reported first), or CMX_OK.
+int cell_failure(cmx_ctx* c)
+{
+    if (c->h_clock->nonfinite == 2)
+        return fail(c, CMX_E_NONFINITE, "backward-Euler cell: Newton iteration did not converge in 16 steps");
+    if (c->h_clock->nonfinite == 4)
+        return fail(c, CMX_E_NONFINITE, "membrane voltage outside the gate lookup table (-250..350 mV)");
+    return CMX_OK;
+}
+
 // Runs the clock's steps k .. n_steps-1 in batches, resuming poisoned steps.
 template <class Rows, class RS, class RV>
 int run_steps(cmx_ctx* c, const Rows& R)
@@ -1965,6 +2000,7 @@ int run_steps(cmx_ctx* c, const Rows& R)
         if ((rc = launch_finish<RS, RV>(c, verdict_buf(c), fused, fused))) return rc;
         rc = sync_state(c);
         if (rc) return rc;
+        if ((rc = cell_failure(c))) return rc;
         int guard = 0;
         while (c->h_clock->poison) {
             if (++guard > c->max_its + 16) return fail(c, CMX_E_STATE, "resuming a step made no progress");
@@ -2015,8 +2051,7 @@ int run_steps(cmx_ctx* c, const Rows& R)
         c->budget = std::min(short0 ? std::max(mx0, c->budget) + 1 : mx0, c->max_its);
         c->refine_budget = std::min(mx1, c->max_its);
         c->steps += got;
-        if (c->h_clock->nonfinite == 2)
-            return fail(c, CMX_E_NONFINITE, "backward-Euler cell: Newton iteration did not converge in 16 steps");
+        if ((rc = cell_failure(c))) return rc;
         if (c->h_clock->nonfinite)
             return fail(c, CMX_E_NONFINITE, "non-finite transmembrane potential (cell ODE or solve blew up)");
     }
@@ -2201,6 +2236,18 @@ int setup_svi(cmx_ctx* c, const cmx_mesh* mesh, VecAlloc& rvec)
             for (int k = 0; k < 3; ++k) c->qp[q][k] = pts[q][k];
     }
     return CMX_OK;
+}
+
+int build_gate_table(cmx_ctx* c)
+{
+    if (c->precision != CMX_MIXED || c->integrator != CMX_RUSH_LARSEN || !c->cell_groups.empty() || c->split)
+        return fail(c, CMX_E_ARG, "lookup tables need mixed precision, Rush-Larsen cells of one model and the ICI step");
+    std::vector<float> table;
+    const char* err = gate_table(c->model, c->dt_ode, c->cm, table, &c->lut_rows);
+    if (err) return fail(c, CMX_E_ARG, err);
+    c->lut_vmin = kLutVmin;
+    c->lut_dv = kLutDv;
+    return upload(c, &c->lut, table.data(), table.size());
 }
 
 int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* opts, cmx_ctx* c)
@@ -2500,6 +2547,10 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     CMX_HIP(c, hipHostMalloc((void**)&c->h_ctl, 2 * sizeof(SolveCtl), hipHostMallocDefault));
     CMX_HIP(c, hipHostMalloc((void**)&c->h_its, 4 * c->batch * sizeof(int32_t), hipHostMallocDefault));
     *c->h_clock = ck;
+    if (opts->lookup_tables) {
+        rc = build_gate_table(c);
+        if (rc) return rc;
+    }
     CMX_HIP(c, hipDeviceSynchronize());
     return CMX_OK;
 }
@@ -2923,6 +2974,21 @@ int cmx_model_info(int32_t model, int32_t* n_states, int32_t* n_params)
     if (model_sizes(model, &S, &P)) return fail(nullptr, CMX_E_ARG, "unknown cell model");
     if (n_states) *n_states = S;
     if (n_params) *n_params = P;
+    return CMX_OK;
+}
+
+int cmx_gate_table(int32_t model, double dt, float* table, int32_t* rows, int32_t* width, double* vmin, double* dv)
+{
+    if (!(dt > 0)) return fail(nullptr, CMX_E_ARG, "dt must be positive");
+    std::vector<float> t;
+    int r = 0;
+    const char* err = gate_table(model, dt, 1.0, t, &r);
+    if (err) return fail(nullptr, CMX_E_ARG, err);
+    if (rows) *rows = r;
+    if (width) *width = (int32_t)(t.size() / (size_t)r);
+    if (vmin) *vmin = kLutVmin;
+    if (dv) *dv = kLutDv;
+    if (table) std::copy(t.begin(), t.end(), table);
     return CMX_OK;
 }
 

@@ -150,6 +150,10 @@ public:
     int GetGpuRefinementSweeps() const { return mRefine; }
     void SetKspMaxIterations(int its) { mMaxIts = its; }
     int GetKspMaxIterations() const { return mMaxIts; }
+    // mixed precision, Rush-Larsen cells: gate coefficients from a table in V
+    // (the reference selects its *Opt cell classes for this)
+    void SetGpuLookupTables(bool on = true) { mLut = on; }
+    bool GetGpuLookupTables() const { return mLut; }
 
 private:
     unsigned mDim = 3;
@@ -171,6 +175,7 @@ private:
     int mPrecision = CMX_FP64;
     int mRefine = 1;
     int mMaxIts = 1000;
+    bool mLut = false;
 };
 
 // ---------------------------------------------------------------------------
@@ -599,6 +604,7 @@ public:
         o.extrapolate = 0;
         o.operator_splitting = cfg->GetUseReactionDiffusionOperatorSplitting() ? 1 : 0;
         o.state_variable_interpolation = cfg->GetUseStateVariableInterpolation() ? 1 : 0;
+        o.lookup_tables = cfg->GetGpuLookupTables() ? 1 : 0;
         if (mpCtx) { cmx_destroy(mpCtx); mpCtx = nullptr; }
         Check(cmx_create(&mesh, &tissue, &o, &mpCtx), nullptr);
         if (!mParams.empty() && mCellTypes == 1) {   // parameters set before a re-initialisation

@@ -124,6 +124,12 @@ typedef struct {
     int32_t separate_refinement; /* mixed: 0 (default) computes a step's FP64 true residual in the
                                   first solve's last CG iteration whenever the refinement solve needs
                                   no iterations; 1 always in a kernel of its own (same test) */
+    int32_t lookup_tables;     /* mixed, Rush-Larsen, one cell model: each gate's steady state and
+                                  decay factor exp(-dt_ode/tau) read from a table in V (-250..350 mV,
+                                  0.02 mV rows, linear interpolation) instead of evaluated, as the
+                                  reference's *Opt cells tabulate V-only expressions
+                                  (heart/src/odes/AbstractLookupTableCollection.hpp:49); a voltage
+                                  outside the table fails the step (CMX_E_NONFINITE).  0 = evaluate */
 } cmx_opts;
 
 typedef struct {
@@ -225,6 +231,14 @@ int cmx_model_initial_state(int32_t model, double* y);
  * annotates with the metadata the reference's generator turns into modifiable
  * parameters: g_Na, g_Kmax, P_si (in that order). */
 int cmx_model_param(int32_t model, int32_t index, const char** name, double* default_value);
+/* The gate lookup table that cmx_opts.lookup_tables / cmx_cells.lookup_tables
+ * use for Rush-Larsen steps of length dt (the tabulated-expression idea of the
+ * reference's AbstractLookupTableCollection, heart/src/odes/
+ * AbstractLookupTableCollection.hpp:49): row r at V = vmin + r dv holds, for the
+ * model's j-th gate, its steady state at column 2j and its decay factor
+ * exp(-dt/tau) at column 2j + 1 (width floats per row).  With table NULL only
+ * the sizes are returned.  Host code only. */
+int cmx_gate_table(int32_t model, double dt, float* table, int32_t* rows, int32_t* width, double* vmin, double* dv);
 
 /* Per-node parameter planes [n_params][n_nodes] in place of those given at
  * create, as AbstractParameterisedSystem::SetParameter on every node's cell
@@ -319,6 +333,10 @@ typedef struct {
     const double* params;        /* [n_params][n_cells] or NULL */
     double threshold;            /* mV, for the action-potential summary */
     int32_t steps_per_launch;    /* steps per kernel launch (0: 2000) */
+    int32_t lookup_tables;       /* mixed Rush-Larsen: gate steady states and decay factors over dt
+                                    from a table in V (as cmx_opts.lookup_tables); a voltage
+                                    outside the table fails the run.  Not with the
+                                    allocation-free variant. */
 } cmx_cells;
 
 typedef struct {
