@@ -71,7 +71,8 @@ class CmxStats(ctypes.Structure):
                 ('last_residual', ctypes.c_double), ('device_bytes', ctypes.c_double),
                 ('graph_launches', ctypes.c_int64), ('graph_captures', ctypes.c_int64),
                 ('graph_mode', ctypes.c_int32), ('solver_blocks', ctypes.c_int32),
-                ('resumes', ctypes.c_int64), ('folded_refinements', ctypes.c_int64)]
+                ('resumes', ctypes.c_int64), ('folded_refinements', ctypes.c_int64),
+                ('adjoint_slots', ctypes.c_int32), ('adjoint_sub_interval', ctypes.c_int32)]
 
 
 # Every symbol include/cmx.h declares, with its ctypes signature.
@@ -118,6 +119,7 @@ SIGNATURES = {
                                              ctypes.c_void_p, ctypes.c_void_p]),
     'cmx_adjoint': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                    _ip, ctypes.c_int32, _dp, _dp, _dp, _dp]),
+    'cmx_set_adjoint_memory': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     'cmx_slab_mesh': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                      ctypes.c_double, _ip, _ip, _dp, _ip]),
 }
@@ -468,6 +470,11 @@ class Monodomain:
                                  _p(nodes, _ip), int(obs_every), _p(vals), _p(loss),
                                  _p(gp) if self.n_params else None, _p(gs)), self.ctx)
         return float(loss[0]), gp[:self.n_params], gs
+
+    def set_adjoint_memory(self, nbytes):
+        """Bound on the forward states adjoint() stores (0: the device's free
+        memory); past one level of checkpoints it nests a second."""
+        _check(lib().cmx_set_adjoint_memory(self.ctx, float(nbytes)), self.ctx)
 
     def set_activation(self, threshold, every_steps=1, t_now=0.0):
         """ActivationOutputModifier on the device: processes the current voltage

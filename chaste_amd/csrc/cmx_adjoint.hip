@@ -21,6 +21,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -324,21 +326,64 @@ int restore_slot(cmx_ctx* c, const double* src)
     return CMX_OK;
 }
 
+// Checkpoint plan: n_ck checkpoints every L steps, then per segment either
+// its L traced steps (q = 0) or checkpoints every q steps and q traced steps
+// at a time; q minimises the stored states when one level does not fit.
+struct AdjPlan {
+    int n_ck, q, slots;
+};
+
+AdjPlan adjoint_plan(int n_steps, int L, double slot_bytes, double limit)
+{
+    AdjPlan p{(n_steps + L - 1) / L, 0, 0};
+    p.slots = p.n_ck + L + 1;
+    if (p.slots * slot_bytes <= limit || L <= 2) return p;
+    int best = p.slots, best_q = 0;
+    for (int q = 1; q < L; ++q) {
+        const int slots = p.n_ck + (L + q - 1) / q + q + 1;
+        if (slots < best) {
+            best = slots;
+            best_q = q;
+        }
+    }
+    p.q = best_q;
+    p.slots = best;
+    return p;
+}
+
 template <class Rows>
 int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n_obs, const int32_t* obs_nodes,
                  int every, const double* obs_values, double* loss, double* grad_params, double* grad_state0)
 {
     const int n = c->n, ps = c->ps, S = c->S, P = c->P;
     const size_t slot = (size_t)S * ps;
-    const int n_ck = (n_steps + L - 1) / L;
     const int n_obs_steps = n_steps / every;
+    double limit = c->adj_mem_limit;
+    if (!(limit > 0)) {
+        size_t free_b = 0, total_b = 0;
+        ADJ_HIP(c, hipMemGetInfo(&free_b, &total_b));
+        limit = (double)free_b - (double)(1ull << 30);
+    }
+    const AdjPlan plan = adjoint_plan(n_steps, L, (double)slot * sizeof(double), limit);
+    if (plan.slots * (double)slot * sizeof(double) > limit)
+        return afail(c, CMX_E_NOMEM, "adjoint checkpoints: " + std::to_string(plan.slots) + " stored states of " +
+                                         std::to_string((double)slot * sizeof(double) / 1e9) +
+                                         " GB exceed the memory bound of " + std::to_string(limit / 1e9) + " GB");
+    // CMX_PROGRESS set: one line on stderr per forward pass and replayed piece
+    const bool verbose = std::getenv("CMX_PROGRESS") != nullptr;
+    const int n_ck = plan.n_ck;
+    const int sq = plan.q;                   // 0: one level
+    const int traced = sq > 0 ? sq : L;        // steps kept at a time
+    c->adj_slots = plan.slots;
+    c->adj_sub = sq;
     Scratch sc;
     int rc;
-    double *ck, *seg, *final_slot, *obs_v, *obs_d;
+    double *ck, *sub = nullptr, *seg, *final_slot, *obs_v, *obs_d;
     int32_t *d_obs_nodes, *obs_slot;
     AdjBuffers B{};
     if ((rc = sc.alloc(c, &ck, (size_t)n_ck * slot))) return rc;
-    if ((rc = sc.alloc(c, &seg, (size_t)L * slot))) return rc;
+    if (sq > 0 && (rc = sc.alloc(c, &sub, (size_t)((L + sq - 1) / sq) * slot))) return rc;
+    if ((rc = sc.alloc(c, &seg, (size_t)traced * slot))) return rc;
     if ((rc = sc.alloc(c, &final_slot, slot))) return rc;
     if ((rc = sc.alloc(c, &obs_v, (size_t)std::max(n_obs_steps, 1) * n_obs))) return rc;
     if ((rc = sc.alloc(c, &obs_d, (size_t)std::max(n_obs_steps, 1) * n_obs))) return rc;
@@ -395,6 +440,11 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
         }
     }
     if ((rc = save_slot(c, final_slot))) return rc;
+    if (verbose) {
+        std::fprintf(stderr, "cmx_adjoint: forward pass of %d steps done; %d stored states (segment checkpoints every "
+                             "%d steps)\n", n_steps, plan.slots, sq);
+        std::fflush(stderr);
+    }
     std::vector<double> h_obs((size_t)n_obs_steps * n_obs);
     if (!h_obs.empty())
         ADJ_HIP(c, hipMemcpyAsync(h_obs.data(), obs_v, h_obs.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -432,24 +482,23 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
     a.lam_v = B.lam_v;
     a.grad = P > 0 ? B.grad : nullptr;
     a.obs_slot = obs_slot;
-    for (int s0 = (n_ck - 1) * L; s0 >= 0; s0 -= L) {
-        const int s1 = std::min(s0 + L, n_steps);
-        if ((rc = restore_slot(c, ck + (size_t)(s0 / L) * slot))) return rc;
-        // replay the segment in one call; its cell kernels write each step's
-        // starting state into seg (plain launches: a captured step has no trace)
+    // Replays steps [r0, r1) from the state the context holds, its cell
+    // kernels writing each step's starting state into seg (plain launches: a
+    // captured step has no trace), then walks them backwards.
+    auto replay_and_walk = [&](int r0, int r1) -> int {
         c->trace = seg;
-        c->trace_k0 = s0;
-        c->trace_len = s1 - s0;
+        c->trace_k0 = r0;
+        c->trace_len = r1 - r0;
         c->trace_slot = slot;
         const int graphs = c->use_graph;
         c->use_graph = 0;
-        rc = step_range(c, t0, n_steps, s0, s1);
+        int rc2 = step_range(c, t0, n_steps, r0, r1);
         c->use_graph = graphs;
         c->trace = nullptr;
-        if (rc) return rc;
-        for (int m = s1 - 1; m >= s0; --m) {
-            if ((rc = adjoint_solve<Rows>(c, R, B))) return rc;
-            const double* st = seg + (size_t)(m - s0) * slot;
+        if (rc2) return rc2;
+        for (int m = r1 - 1; m >= r0; --m) {
+            if ((rc2 = adjoint_solve<Rows>(c, R, B))) return rc2;
+            const double* st = seg + (size_t)(m - r0) * slot;
             a.Vn = st;
             a.Yn = st + ps;
             a.k = m;
@@ -462,6 +511,30 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
                 else launch_cell_adjoint<ModelTt06, CMX_FORWARD_EULER>(c, a);
             }
             ADJ_HIP(c, hipGetLastError());
+        }
+        if (verbose) {
+            ADJ_HIP(c, hipStreamSynchronize(c->stream));
+            std::fprintf(stderr, "cmx_adjoint: steps %d..%d back-propagated\n", r0, r1 - 1);
+            std::fflush(stderr);
+        }
+        return CMX_OK;
+    };
+    for (int s0 = (n_ck - 1) * L; s0 >= 0; s0 -= L) {
+        const int s1 = std::min(s0 + L, n_steps);
+        if ((rc = restore_slot(c, ck + (size_t)(s0 / L) * slot))) return rc;
+        if (sq == 0) {
+            if ((rc = replay_and_walk(s0, s1))) return rc;
+            continue;
+        }
+        // two levels: the segment once more, saving its state every q steps,
+        // then each q-step piece replayed from its checkpoint, last first
+        for (int m = s0; m < s1; m += sq) {
+            if ((rc = save_slot(c, sub + (size_t)((m - s0) / sq) * slot))) return rc;
+            if ((rc = step_range(c, t0, n_steps, m, std::min(m + sq, s1)))) return rc;
+        }
+        for (int r0 = s0 + (s1 - 1 - s0) / sq * sq; r0 >= s0; r0 -= sq) {
+            if ((rc = restore_slot(c, sub + (size_t)((r0 - s0) / sq) * slot))) return rc;
+            if ((rc = replay_and_walk(r0, std::min(r0 + sq, s1)))) return rc;
         }
     }
     ADJ_HIP(c, hipStreamSynchronize(c->stream));
@@ -524,6 +597,14 @@ int cmx_adjoint(cmx_ctx* c, double t0, int32_t n_steps, int32_t checkpoint_every
     EllRows R{c->ell_col, c->width, c->n};
     return adjoint_impl(c, R, t0, n_steps, checkpoint_every, n_obs, obs_nodes, obs_every, obs_values, loss,
                         grad_params, grad_state0);
+}
+
+int cmx_set_adjoint_memory(cmx_ctx* c, double bytes)
+{
+    if (!c) return CMX_E_ARG;
+    if (!(bytes >= 0)) return afail(c, CMX_E_ARG, "the adjoint memory bound must be >= 0");
+    c->adj_mem_limit = bytes;
+    return CMX_OK;
 }
 
 }  // extern "C"

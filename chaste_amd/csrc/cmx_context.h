@@ -256,6 +256,10 @@ struct cmx_ctx {
     int split = 0, n_half = 1;   // operator splitting: ODE substeps per half step
     int width = 0, n_patterns = 0;
     int G = 1;                   // blocks of the reduction kernels
+    // adjoint: bound on its stored forward states (bytes, 0 = free memory) and
+    // the plan of its last call
+    double adj_mem_limit = 0.0;
+    int adj_slots = 0, adj_sub = 0;
     // gate lookup table (mixed Rush-Larsen cells): rows at lut_vmin + r lut_dv
     float* lut = nullptr;
     int lut_rows = 0;

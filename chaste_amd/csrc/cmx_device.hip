@@ -2945,6 +2945,8 @@ int cmx_get_stats(cmx_ctx* c, cmx_stats* s)
     s->solver_blocks = c->G;
     s->resumes = c->resumes;
     s->folded_refinements = c->folded_total;
+    s->adjoint_slots = c->adj_slots;
+    s->adjoint_sub_interval = c->adj_sub;
     return CMX_OK;
 }
 

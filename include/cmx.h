@@ -150,6 +150,9 @@ typedef struct {
     int64_t resumes;           /* steps whose final solve the host resumed past its budget */
     int64_t folded_refinements; /* mixed: steps whose FP64 true residual came from the last CG
                                   iteration (no kernel of its own) */
+    int32_t adjoint_slots;     /* stored forward states of the last cmx_adjoint (checkpoints, segment
+                                  checkpoints and traced steps; each n_states x n_nodes doubles) */
+    int32_t adjoint_sub_interval; /* its segment-checkpoint interval (0: one level) */
 } cmx_stats;
 
 typedef struct cmx_ctx cmx_ctx;
@@ -256,14 +259,22 @@ int cmx_set_params(cmx_ctx* ctx, int32_t n_params, const double* params);
  * step (cell-step Jacobian columns by forward-mode differentiation of the
  * generated model, one symmetric diffusion solve per step, FP64 Jacobi-PCG
  * to the context's rtol, 1e-10 when it is 0), recomputing the forward states
- * from checkpoints saved every checkpoint_every steps (device memory about
- * (n_steps/checkpoint_every + checkpoint_every) x n_states x n_nodes doubles).
- * Needs CMX_FP64, forward-Euler or Rush-Larsen cells, no splitting, no SVI,
+ * from checkpoints saved every checkpoint_every steps: a segment between two
+ * checkpoints is replayed with every step's state kept (device memory about
+ * (n_steps/checkpoint_every + checkpoint_every) x n_states x n_nodes
+ * doubles).  When that exceeds the memory bound (cmx_set_adjoint_memory), each
+ * segment is replayed once more first to save checkpoints every
+ * q ~ sqrt(checkpoint_every) steps and only q steps are kept at a time
+ * (n_steps/checkpoint_every + checkpoint_every/q + q states; one forward pass
+ * more).  The gradient does not depend on the schedule.  Needs CMX_FP64, forward-Euler or Rush-Larsen cells, no splitting, no SVI,
  * extrapolate = 0 and activation tracking off.  On return the context holds
  * the state at t0 + n_steps dt. */
 int cmx_adjoint(cmx_ctx* ctx, double t0, int32_t n_steps, int32_t checkpoint_every, int32_t n_obs,
                 const int32_t* obs_nodes, int32_t obs_every, const double* obs_values, double* loss,
                 double* grad_params, double* grad_state0);
+/* Bound (bytes) on the forward states cmx_adjoint stores; 0 (default): the
+ * device's free memory at the call less 1 GiB.  No reference counterpart. */
+int cmx_set_adjoint_memory(cmx_ctx* ctx, double bytes);
 
 /* Reads a Triangle/TetGen mesh <base>.node / <base>.ele, ASCII (items numbered
  * from 0 or 1, '#' comments) or binary (BIN header tag), as

@@ -112,6 +112,33 @@ def test_adjoint_checkpoint_interval_invariance():
     assert np.array_equal(sq.m.voltage(), V_end)
 
 
+def test_adjoint_two_level_checkpoints_match_one_level():
+    """Under a memory bound too small for one level of checkpoints, each
+    segment is replayed once more to save checkpoints every q steps and only q
+    steps are kept at a time: the loss and both gradients are bitwise those of
+    the one-level schedule; a bound below even that plan is refused."""
+    from chaste_amd import cmx
+    print('case adjoint two-level checkpoints', flush=True)
+    sq = Square(ex=12, steps=120, n_obs=6, every=10, seed=5)
+    data = sq.observe(np.full((2, sq.n), 0.9))
+    s = np.ones((2, sq.n))
+    ref = sq.adjoint(s, data, checkpoint_every=60)
+    st = sq.m.stats()
+    slot = st['n_states'] * ((sq.n + 63) // 64 * 64) * 8
+    assert st['adjoint_sub_interval'] == 0 and st['adjoint_slots'] == 2 + 60 + 1
+    # 2 checkpoints + 60/q + q + 1 states: q = 8 gives 2 + 8 + 8 + 1 = 19
+    sq.m.set_adjoint_memory(20 * slot)
+    got = sq.adjoint(s, data, checkpoint_every=60)
+    st = sq.m.stats()
+    assert st['adjoint_sub_interval'] > 0 and st['adjoint_slots'] <= 20
+    assert got[0] == ref[0]
+    assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2])
+    sq.m.set_adjoint_memory(10 * slot)
+    with pytest.raises(cmx.CmxError, match='exceed the memory bound'):
+        sq.adjoint(s, data, checkpoint_every=60)
+    sq.m.set_adjoint_memory(0)
+
+
 @pytest.mark.parametrize('model,integrator', [(0, 0), (0, 1), (1, 1)], ids=['lr91_fe', 'lr91_rl', 'tt06_rl'])
 def test_adjoint_initial_state_gradient(model, integrator):
     """dL/d(initial state) against central differences of the device's own
