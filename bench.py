@@ -117,9 +117,10 @@ def time_path(m, steps, warmup, roof_steps, events=True):
     tuned = None
     if AUTOTUNE:
         # solver block count picked from the warmed-up state (state restored)
-        chosen, times = m.autotune(t, 20)
-        tuned = {'blocks': chosen, 'ms_per_step': {str(k): round(v, 5) for k, v in times.items()}}
-        progress('autotune: %d blocks (%s)' % (chosen, tuned['ms_per_step']))
+        chosen, times = m.autotune(t, 64)      # one batch of steps per trial
+        tuned = {'blocks': chosen, 'ms_per_step': {str(k): round(v, 5) for k, v in times.items()},
+                 'chosen': m.launch_config()}
+        progress('autotune: %s (blocks %s)' % (tuned['chosen'], tuned['ms_per_step']))
     st0 = m.stats()
     sync()
     t0 = time.perf_counter()

@@ -72,7 +72,8 @@ class CmxStats(ctypes.Structure):
                 ('graph_launches', ctypes.c_int64), ('graph_captures', ctypes.c_int64),
                 ('graph_mode', ctypes.c_int32), ('solver_blocks', ctypes.c_int32),
                 ('resumes', ctypes.c_int64), ('folded_refinements', ctypes.c_int64),
-                ('adjoint_slots', ctypes.c_int32), ('adjoint_sub_interval', ctypes.c_int32)]
+                ('adjoint_slots', ctypes.c_int32), ('adjoint_sub_interval', ctypes.c_int32),
+                ('cg_rows_per_thread', ctypes.c_int32), ('graph_steps', ctypes.c_int32)]
 
 
 # Every symbol include/cmx.h declares, with its ctypes signature.
@@ -120,6 +121,7 @@ SIGNATURES = {
     'cmx_adjoint': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                    _ip, ctypes.c_int32, _dp, _dp, _dp, _dp]),
     'cmx_set_adjoint_memory': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    'cmx_set_launch_config': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     'cmx_slab_mesh': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                      ctypes.c_double, _ip, _ip, _dp, _ip]),
 }
@@ -440,8 +442,9 @@ class Monodomain:
 
     def autotune(self, t0=0.0, n_steps=20, candidates=None):
         """Time n_steps from the current state for each candidate solver block
-        count, keep the fastest; the state is restored.  Returns (chosen,
-        {blocks: ms per step})."""
+        count, then the CG rows per thread and the steps per captured graph at
+        the fastest; keep the fastest configuration (launch_config()); the state
+        is restored.  Returns (chosen blocks, {blocks: ms per step})."""
         cand = np.ascontiguousarray([128, 192, 256, 320, 384, 512] if candidates is None else candidates,
                                     dtype=np.int32)
         ms = np.zeros(len(cand))
@@ -470,6 +473,15 @@ class Monodomain:
                                  _p(nodes, _ip), int(obs_every), _p(vals), _p(loss),
                                  _p(gp) if self.n_params else None, _p(gs)), self.ctx)
         return float(loss[0]), gp[:self.n_params], gs
+
+    def set_launch_config(self, solver_blocks=0, cg_rows=0, graph_steps=0):
+        """Solver blocks, CG rows per thread, steps per captured graph (0 keeps)."""
+        _check(lib().cmx_set_launch_config(self.ctx, int(solver_blocks), int(cg_rows), int(graph_steps)), self.ctx)
+
+    def launch_config(self):
+        st = self.stats()
+        return {'solver_blocks': st['solver_blocks'], 'cg_rows': st['cg_rows_per_thread'],
+                'graph_steps': st['graph_steps']}
 
     def set_adjoint_memory(self, nbytes):
         """Bound on the forward states adjoint() stores (0: the device's free

@@ -256,6 +256,8 @@ struct cmx_ctx {
     int split = 0, n_half = 1;   // operator splitting: ODE substeps per half step
     int width = 0, n_patterns = 0;
     int G = 1;                   // blocks of the reduction kernels
+    int cg_rows = 0;             // CG kernels' rows per thread held in registers (0: precision default)
+    int graph_steps = 8;         // steps in the largest captured graph (a power of two)
     // adjoint: bound on its stored forward states (bytes, 0 = free memory) and
     // the plan of its last call
     double adj_mem_limit = 0.0;
@@ -335,8 +337,13 @@ struct cmx_ctx {
     double device_bytes = 0;
     std::vector<void*> allocs;
     // hipGraphs of whole steps, keyed by (budget, refinement budget, steps,
-    // solver blocks, activation tracking)
-    struct StepGraph { int budget, refine_budget, steps, G, act_every; double act_thr; hipGraph_t graph; hipGraphExec_t exec; };
+    // solver blocks, CG rows per thread, activation tracking)
+    struct StepGraph {
+        int budget, refine_budget, steps, G, rows, act_every;
+        double act_thr;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
     std::vector<StepGraph> graphs;
     long long graph_launches = 0, graph_captures = 0, graph_evictions = 0;
     int graph_profiler_off = 0;  // a profiler's dispatch interception was detected: plain launches

@@ -844,7 +844,7 @@ struct StartRow {
     }
 };
 
-template <class Rows, class RV>
+template <class Rows, class RV, int ROWS = (sizeof(RV) == 4 ? 2 : 1)>
 __global__ __launch_bounds__(kCgBlock) void cg_start_kernel(Rows R, MatVals mv, CgArgs<RV> a)
 {
     __shared__ RV sA[kMaxPatterns * kMaxStencil], sD[kMaxPatterns];
@@ -854,8 +854,10 @@ __global__ __launch_bounds__(kCgBlock) void cg_start_kernel(Rows R, MatVals mv, 
     const double tol2_set = a.ctl->tol2;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const RowRange rr(a.n, blk, gridDim.x);
-    // FP32 rows: two rows per thread in registers; FP64: one (register budget)
-    constexpr bool kTwo = sizeof(RV) == 4;
+    // rows per thread held in registers before the reduction (ROWS, a launch
+    // knob: 2 by default in FP32, 1 in FP64 for the register budget); the
+    // block's further rows follow in a loop
+    constexpr bool kTwo = ROWS == 2;
     const int i0 = rr.lo + threadIdx.x, i1 = i0 + kCgBlock;
     StartRow<Rows, RV> q0, q1;
     if (i0 < rr.hi) q0.load(R, i0, a.v);
@@ -956,7 +958,7 @@ struct IterRow {
     }
 };
 
-template <class Rows, class RV>
+template <class Rows, class RV, int ROWS = (sizeof(RV) == 4 ? 2 : 1)>
 __global__ __launch_bounds__(kCgBlock) void cg_iter_kernel(Rows R, MatVals mv, CgArgs<RV> a, int it)
 {
     __shared__ RV sA[kMaxPatterns * kMaxStencil], sD[kMaxPatterns];
@@ -991,7 +993,7 @@ __global__ __launch_bounds__(kCgBlock) void cg_iter_kernel(Rows R, MatVals mv, C
     }
     const RowRange rr(a.n, blk, gridDim.x);
     const bool first_it = it == 0;
-    constexpr bool kTwo = sizeof(RV) == 4;   // rows per thread held in registers: 2 (FP32) or 1
+    constexpr bool kTwo = ROWS == 2;   // rows per thread held in registers (launch knob)
     const int i0 = rr.lo + threadIdx.x, i1 = i0 + kCgBlock;
     IterRow<Rows, RV> q0, q1;
     if (i0 < rr.hi) q0.load(R, i0, a.v, first_it);
@@ -1416,7 +1418,7 @@ bool model_supports(int model, int integrator)
 
 int grid_of(int n) { return (n + kBlock - 1) / kBlock; }
 
-constexpr int kGraphSteps = 8;        // steps in the largest captured graph
+constexpr int kMaxGraphSteps = 32;    // cap on the steps of the largest captured graph
 constexpr size_t kGraphCache = 64;    // instantiated graphs kept per context
 
 // Destroys every captured graph once the stream has drained (a graph still
@@ -1757,6 +1759,14 @@ int launch_residual(cmx_ctx* c, const Rows& R, int solve)
     return CMX_OK;
 }
 
+// Rows per thread the CG kernels hold in registers: the context's knob, or
+// the precision's default (2 in FP32, 1 in FP64).
+template <class RV>
+int cg_rows_of(const cmx_ctx* c)
+{
+    return c->cg_rows > 0 ? c->cg_rows : (sizeof(RV) == 4 ? 2 : 1);
+}
+
 // Start of a solve (w0 = A u0 and its dot products; ends a solve whose
 // residual is already below the threshold).
 template <class Rows, class RV>
@@ -1766,7 +1776,8 @@ int launch_cg_start(cmx_ctx* c, const Rows& R, int solve, const double* pin = nu
     a.first = solve == 0 ? 1 : 0;
     if (pin) a.pin = pin;   // the reduction holding ||D^-1 r0||^2 of this solve
     a.v.m_out = (RV*)c->vec[V_MA];   // read by iteration 0
-    CMX_LAUNCH(c, CMX_K_SPMV, (cg_start_kernel<Rows, RV>), c->G, kCgBlock, R, mat_vals(c), a);
+    if (cg_rows_of<RV>(c) == 2) CMX_LAUNCH(c, CMX_K_SPMV, (cg_start_kernel<Rows, RV, 2>), c->G, kCgBlock, R, mat_vals(c), a);
+    else CMX_LAUNCH(c, CMX_K_SPMV, (cg_start_kernel<Rows, RV, 1>), c->G, kCgBlock, R, mat_vals(c), a);
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
 }
@@ -1783,7 +1794,8 @@ int launch_cg_iters(cmx_ctx* c, const Rows& R, int solve, int from, int count, b
         a.pout = pbuf(c, kIterPart + ((it + 1) & 1));
         a.v = cg_vecs<RV>(c, solve, it);
         if (u_last && it == from + count - 1) a.v.u_out = (RV*)c->vec[V_U];
-        CMX_LAUNCH(c, CMX_K_UPDATE, (cg_iter_kernel<Rows, RV>), c->G, kCgBlock, R, mv, a, it);
+        if (cg_rows_of<RV>(c) == 2) CMX_LAUNCH(c, CMX_K_UPDATE, (cg_iter_kernel<Rows, RV, 2>), c->G, kCgBlock, R, mv, a, it);
+        else CMX_LAUNCH(c, CMX_K_UPDATE, (cg_iter_kernel<Rows, RV, 1>), c->G, kCgBlock, R, mv, a, it);
     }
     CMX_HIP(c, hipGetLastError());
     return CMX_OK;
@@ -1897,7 +1909,7 @@ hipGraphExec_t step_graph(cmx_ctx* c, const Rows& R, int steps)
 {
     for (auto& g : c->graphs)
         if (g.budget == c->budget && g.refine_budget == c->refine_budget && g.steps == steps && g.G == c->G &&
-            g.act_every == c->act_every && g.act_thr == c->act_thr)
+            g.rows == c->cg_rows && g.act_every == c->act_every && g.act_thr == c->act_thr)
             return g.exec;
     if (c->graphs.size() >= kGraphCache) {
         if (hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -1931,22 +1943,22 @@ hipGraphExec_t step_graph(cmx_ctx* c, const Rows& R, int steps)
     // the executable's one-time device set-up now, on the stream, rather
     // than in its first replay (a primed graph's first replay is often timed)
     if (hipGraphUpload(exec, c->stream) != hipSuccess) (void)hipGetLastError();
-    c->graphs.push_back({c->budget, c->refine_budget, steps, c->G, c->act_every, c->act_thr, graph, exec});
+    c->graphs.push_back({c->budget, c->refine_budget, steps, c->G, c->cg_rows, c->act_every, c->act_thr, graph, exec});
     ++c->graph_captures;
     return exec;
 }
 
 bool graphs_on(const cmx_ctx* c) { return c->use_graph && !c->graph_profiler_off && c->prof_mask == 0; }
 
-// Enqueues `todo` steps: replays of captured graphs of 8, 4, 2 and 1 steps
-// when enabled (and no kernel is being timed), plain launches otherwise.
+// Enqueues `todo` steps: replays of captured graphs of graph_steps, then
+// halving numbers of steps down to 1, when enabled (and no kernel is being timed), plain launches otherwise.
 template <class Rows, class RS, class RV>
 int enqueue_steps(cmx_ctx* c, const Rows& R, int todo)
 {
     if (graphs_on(c)) {
         int left = todo;
         while (left > 0) {
-            int g = kGraphSteps;
+            int g = c->graph_steps;
             while (g > left) g >>= 1;
             hipGraphExec_t exec = step_graph<Rows, RS, RV>(c, R, g);
             if (!exec) break;
@@ -1969,7 +1981,7 @@ template <class Rows, class RS, class RV>
 int prime_graphs(cmx_ctx* c, const Rows& R, int todo)
 {
     if (!graphs_on(c)) return CMX_OK;
-    for (int g = kGraphSteps; g >= 1; g >>= 1)
+    for (int g = c->graph_steps; g >= 1; g >>= 1)
         if (todo >= g && !step_graph<Rows, RS, RV>(c, R, g)) break;
     return CMX_OK;
 }
@@ -2744,32 +2756,68 @@ int cmx_autotune(cmx_ctx* c, double t0, int32_t n_steps, int32_t n_candidates, c
         drop_graphs(c);
         return CMX_OK;
     };
-    const int G0 = c->G;
-    int best = G0, best_budget = budget, best_refine_budget = refine_budget;
+    const int G0 = c->G, rows0 = c->cg_rows, gs0 = c->graph_steps;
+    int best = G0, best_rows = rows0, best_gs = gs0, best_budget = budget, best_refine_budget = refine_budget;
     double best_ms = 0.0;
     int rc = CMX_OK;
-    for (size_t k = 0; k < cand.size() && rc == CMX_OK; ++k) {
-        c->G = cand[k];
-        // one untimed run (graph capture, caches), then the timed one
+    // one trial of the current launch configuration: an untimed run (graph
+    // capture, caches), then the timed one, from the saved state each time
+    auto trial = [&](double* out_ms) -> int {
         double ms = 0.0;
-        for (int pass = 0; pass < 2 && rc == CMX_OK; ++pass) {
-            if ((rc = restore())) break;
+        for (int pass = 0; pass < 2; ++pass) {
+            int r = restore();
+            if (r) return r;
             CMX_HIP(c, hipDeviceSynchronize());
             const auto a = std::chrono::steady_clock::now();
-            rc = cmx::step_range(c, t0, n_steps, 0, n_steps);
+            r = cmx::step_range(c, t0, n_steps, 0, n_steps);
+            if (r) return r;
             CMX_HIP(c, hipStreamSynchronize(c->stream));
             ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count() / n_steps;
         }
-        if (rc) break;
-        if (ms_per_step) ms_per_step[k] = ms;
-        if (k == 0 || ms < best_ms) {
+        *out_ms = ms;
+        return CMX_OK;
+    };
+    auto consider = [&](double ms, bool first) {
+        if (first || ms < best_ms) {
             best_ms = ms;
-            best = cand[k];
+            best = c->G;
+            best_rows = c->cg_rows;
+            best_gs = c->graph_steps;
             best_budget = c->budget;
             best_refine_budget = c->refine_budget;
         }
+    };
+    // coordinate search: solver blocks, then rows per thread in the CG
+    // kernels (tile shape), then steps per captured graph (unroll depth)
+    for (size_t k = 0; k < cand.size() && rc == CMX_OK; ++k) {
+        c->G = cand[k];
+        double ms = 0.0;
+        if ((rc = trial(&ms))) break;
+        if (ms_per_step) ms_per_step[k] = ms;
+        consider(ms, k == 0);
     }
+    for (int rows = 1; rows <= 2 && rc == CMX_OK; ++rows) {
+        c->G = best;
+        c->graph_steps = best_gs;
+        c->cg_rows = rows;
+        if (rows == (best_rows > 0 ? best_rows : (c->precision == CMX_MIXED ? 2 : 1))) continue;
+        double ms = 0.0;
+        if ((rc = trial(&ms))) break;
+        consider(ms, false);
+    }
+    if (graphs_on(c))
+        for (int gs = 4; gs <= kMaxGraphSteps && rc == CMX_OK; gs *= 2) {
+            c->G = best;
+            c->cg_rows = best_rows;
+            c->graph_steps = gs;
+            if (gs == best_gs) continue;
+            double ms = 0.0;
+            if ((rc = trial(&ms))) break;
+            consider(ms, false);
+        }
     c->G = rc == CMX_OK ? best : G0;
+    c->cg_rows = rc == CMX_OK ? best_rows : rows0;
+    c->graph_steps = rc == CMX_OK ? best_gs : gs0;
     int rc2 = restore();
     if (rc == CMX_OK) {
         // the budgets the trial of those same steps settled on
@@ -2947,6 +2995,23 @@ int cmx_get_stats(cmx_ctx* c, cmx_stats* s)
     s->folded_refinements = c->folded_total;
     s->adjoint_slots = c->adj_slots;
     s->adjoint_sub_interval = c->adj_sub;
+    s->cg_rows_per_thread = c->cg_rows > 0 ? c->cg_rows : (c->precision == CMX_MIXED ? 2 : 1);
+    s->graph_steps = c->graph_steps;
+    return CMX_OK;
+}
+
+int cmx_set_launch_config(cmx_ctx* c, int32_t solver_blocks, int32_t cg_rows, int32_t graph_steps)
+{
+    if (!c) return fail(nullptr, CMX_E_ARG, "null context");
+    const int cg_max = (c->n + kCgBlock - 1) / kCgBlock;
+    if (solver_blocks < 0 || solver_blocks > kMaxBlocks) return fail(c, CMX_E_ARG, "solver_blocks must lie in [0, 1024]");
+    if (cg_rows < 0 || cg_rows > 2) return fail(c, CMX_E_ARG, "cg_rows must be 0, 1 or 2");
+    if (graph_steps < 0 || graph_steps > kMaxGraphSteps || (graph_steps & (graph_steps - 1)))
+        return fail(c, CMX_E_ARG, "graph_steps must be 0 or a power of two up to 32");
+    CMX_HIP(c, hipStreamSynchronize(c->stream));
+    if (solver_blocks > 0) c->G = std::min(solver_blocks, cg_max);
+    if (cg_rows > 0) c->cg_rows = cg_rows;
+    if (graph_steps > 0) c->graph_steps = graph_steps;
     return CMX_OK;
 }
 

@@ -153,6 +153,8 @@ typedef struct {
     int32_t adjoint_slots;     /* stored forward states of the last cmx_adjoint (checkpoints, segment
                                   checkpoints and traced steps; each n_states x n_nodes doubles) */
     int32_t adjoint_sub_interval; /* its segment-checkpoint interval (0: one level) */
+    int32_t cg_rows_per_thread; /* rows each thread of the CG kernels holds in registers (1 or 2) */
+    int32_t graph_steps;       /* steps in the largest captured graph */
 } cmx_stats;
 
 typedef struct cmx_ctx cmx_ctx;
@@ -175,7 +177,10 @@ int cmx_step(cmx_ctx* ctx, double t0, int32_t n_steps);
 /* Launch-shape autotuner (no counterpart in the reference): times n_steps
  * PDE steps from the current state at t0 for each candidate number of
  * solver blocks (candidates == NULL: 128, 192, 256, 320, 384, 512, capped
- * at one block per 1024 rows), keeps the fastest, and restores the state,
+ * at one block per 1024 rows), then, at the fastest, the CG kernels' rows per
+ * thread held in registers (1, 2: the tile shape) and, with graph replay, the
+ * steps in the largest captured graph (4 .. 32: the unroll depth of the step
+ * loop), keeps the fastest configuration, and restores the state,
  * the initial-guess history, activation times and statistics, so the next
  * cmx_step starts where it would have.  The CG iteration budgets (kernels
  * enqueued per solve, adapted after every batch) are those the fastest trial
@@ -185,6 +190,10 @@ int cmx_step(cmx_ctx* ctx, double t0, int32_t n_steps);
  * ms_per_step[n_candidates or 6] (may be NULL) gets each candidate's time. */
 int cmx_autotune(cmx_ctx* ctx, double t0, int32_t n_steps, int32_t n_candidates, const int32_t* candidates,
                  int32_t* chosen, double* ms_per_step);
+/* Sets the launch configuration the autotuner picks: solver blocks, CG rows
+ * per thread (1 or 2) and steps per captured graph (a power of two up to 32);
+ * 0 keeps a value.  Read it back with cmx_get_stats. */
+int cmx_set_launch_config(cmx_ctx* ctx, int32_t solver_blocks, int32_t cg_rows, int32_t graph_steps);
 
 /* Transmembrane potential (mV): AbstractCardiacProblem::GetSolution /
  * SetInitialCondition (heart/src/problem/AbstractCardiacProblem.cpp:265-292). */

@@ -727,6 +727,39 @@ def test_tissue_iionic_cache_matches_single_cell(oracle, v_clamp):
 
 
 @pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_launch_config_knobs_are_bitwise_neutral(precision):
+    """The autotuner's tile-shape and unroll-depth knobs change how the work is
+    laid out, not what is computed: one or two CG rows per thread held in
+    registers (each thread visits its rows in the same order either way) and
+    4 or 32 steps per captured graph give bitwise the same run; the tuned
+    configuration is reported and a bad one refused."""
+    from chaste_amd import cmx
+    print('case launch config', precision, flush=True)
+    args = (3, 40, 12, 10, cmx.TT06_EPI, 1, -100000.0, 1.0, 0.03)     # 5,863 nodes: up to 5 solver blocks
+    kw = {'precision': precision, 'use_graph': True, 'sigma': (1.75, 0.19, 0.19), 'rtol': 1e-8,
+          'refine': 1 if precision else 0, 'extrapolate': 2}
+    runs = {}
+    for rows, gs in ((1, 4), (2, 32), (2, 8)):
+        m = make(*args, **kw)
+        m.set_launch_config(solver_blocks=4, cg_rows=rows, graph_steps=gs)
+        assert m.launch_config() == {'solver_blocks': 4, 'cg_rows': rows, 'graph_steps': gs}
+        m.step(0.0, 150)
+        runs[(rows, gs)] = (m.voltage(), m.state(), m.stats()['cg_iterations'])
+        if rows == 2 and gs == 8:
+            chosen, _ = m.autotune(1.5, 10, candidates=[2, 4])
+            cfg = m.launch_config()
+            assert cfg['solver_blocks'] == chosen and cfg['cg_rows'] in (1, 2) and cfg['graph_steps'] in (4, 8, 16, 32)
+            with pytest.raises(cmx.CmxError, match='power of two'):
+                m.set_launch_config(graph_steps=12)
+            with pytest.raises(cmx.CmxError, match='cg_rows'):
+                m.set_launch_config(cg_rows=3)
+        m.close()
+    ref = runs[(2, 8)]
+    for key, r in runs.items():
+        assert np.array_equal(r[0], ref[0]) and np.array_equal(r[1], ref[1]) and r[2] == ref[2], key
+
+
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
 def test_autotune_keeps_state_and_results(oracle, precision):
     """cmx_autotune times candidate solver block counts from the current state
     and restores it: the run that follows matches an untuned context to
