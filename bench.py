@@ -317,8 +317,11 @@ def adjoint_case(args):
     progress('adjoint: gradient')
     m.set_state(y0)
     t = time.perf_counter()
+    it0 = m.stats()['adjoint_iterations']
     loss, gp, _ = m.adjoint(0.0, steps, obs, every, data, 100)
     t_adj = time.perf_counter() - t
+    adj_its = m.stats()['adjoint_iterations'] - it0
+    progress('adjoint: %.3f s, %d CG iterations (%.1f per step)' % (t_adj, adj_its, adj_its / steps))
     g = gp[:2] * defaults[:2, None]
 
     def central(d, eps):
@@ -349,6 +352,7 @@ def adjoint_case(args):
                                'stim -400000 uA/cm^3 0.5 ms x<0.02, Jacobi-CG rtol 1e-12, 64 observation '
                                'nodes every 0.1 ms, checkpoints every 100 steps', 'n_nodes': n},
         'forward_s': round(t_fwd, 4), 'gradient_over_forward': round(t_adj / t_fwd, 2), 'loss': loss,
+        'adjoint_cg_iterations_per_step': round(adj_its / steps, 2),
         'fd_directions': len(rel), 'fd_eps': [1e-4, 1e-6],
         'fd_directions_redifferenced': retried,
         'fd_within_1e-4': int(sum(r <= 1e-4 for r in rel)),

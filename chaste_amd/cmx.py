@@ -73,7 +73,8 @@ class CmxStats(ctypes.Structure):
                 ('graph_mode', ctypes.c_int32), ('solver_blocks', ctypes.c_int32),
                 ('resumes', ctypes.c_int64), ('folded_refinements', ctypes.c_int64),
                 ('adjoint_slots', ctypes.c_int32), ('adjoint_sub_interval', ctypes.c_int32),
-                ('cg_rows_per_thread', ctypes.c_int32), ('graph_steps', ctypes.c_int32)]
+                ('cg_rows_per_thread', ctypes.c_int32), ('graph_steps', ctypes.c_int32),
+                ('adjoint_iterations', ctypes.c_int64)]
 
 
 # Every symbol include/cmx.h declares, with its ctypes signature.

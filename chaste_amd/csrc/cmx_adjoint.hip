@@ -287,6 +287,7 @@ int adjoint_solve(cmx_ctx* c, const Rows& R, AdjBuffers& B)
         ADJ_HIP(c, hipStreamSynchronize(c->stream));
         if (B.h_ctl->done == 1) {
             B.last_its = B.h_ctl->it;
+            c->adj_iterations += B.h_ctl->it;
             break;
         }
         if (B.h_ctl->done == 2)
@@ -398,7 +399,7 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
     if ((rc = sc.alloc(c, &cg.w, n))) return rc;
     if ((rc = sc.alloc(c, &cg.z, n))) return rc;
     if ((rc = sc.alloc(c, &cg.s, n))) return rc;
-    if ((rc = sc.alloc(c, &cg.x, n))) return rc;
+    if ((rc = sc.alloc_gathered(c, &cg.x, n))) return rc;   // zero: the first solve starts from 0
     if ((rc = sc.alloc_gathered(c, &cg.p, n))) return rc;
     if ((rc = sc.alloc_gathered(c, &cg.m[0], n))) return rc;
     if ((rc = sc.alloc_gathered(c, &cg.m[1], n))) return rc;
@@ -407,6 +408,9 @@ int adjoint_impl(cmx_ctx* c, const Rows& R, double t0, int n_steps, int L, int n
     if ((rc = sc.alloc(c, &cg.hist, kAdjMaxIts + kAdjFirstMax + 1))) return rc;
     if ((rc = sc.alloc(c, &cg.ctl, 2))) return rc;
     cg.rtol = c->rtol > 0.0 ? c->rtol : 1e-10;
+    // each solve starts from the previous step's mu (consecutive right-hand
+    // sides lambda_V^{n+1} vary smoothly between observation steps)
+    cg.warm = 1;
     cg.max_its = kAdjMaxIts;
     B.b = B.lam_v;
     std::vector<int32_t> h_slot(n, -1);

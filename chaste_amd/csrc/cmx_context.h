@@ -262,6 +262,7 @@ struct cmx_ctx {
     // the plan of its last call
     double adj_mem_limit = 0.0;
     int adj_slots = 0, adj_sub = 0;
+    long long adj_iterations = 0;  // CG iterations of the adjoint's solves (all calls)
     // gate lookup table (mixed Rush-Larsen cells): rows at lut_vmin + r lut_dv
     float* lut = nullptr;
     int lut_rows = 0;
@@ -398,7 +399,9 @@ int step_range(cmx_ctx* c, double t0, int n_total, int k0, int k1);
 // iterations from .. from+count-1, each returning at once once ctl[0].done
 // is set (1 converged, 2 max_its reached, 3 non-finite).
 struct PlainCg {
-    double *r, *w, *z, *s, *x;
+    double *r, *w, *z, *s;
+    double* x;           // the solution; gathered (guard zones) when a solve starts from it
+    int warm;            // start from the x the vector holds (r = b - A x) instead of x = 0
     double* p;           // gathered at the start: c->pad zeroed guard elements on each side
     double* m[2];        // gathered, likewise
     double* part[3];     // [kMaxBlocks * 3] each: iteration ping-pong, start

@@ -1199,16 +1199,19 @@ __global__ __launch_bounds__(kCgBlock) void cg_last_kernel(Rows R, MatVals mv, L
     write_partials<kCgBlock, 3>(part, a.cg.pout, blk);
 }
 
-// Start of a solve A x = b from x = 0 outside a step (the adjoint's solves):
-// stands in for the residual kernel, with r0 = b, u0 = D^-1 b to p, x = 0 and
-// the reductions cg_start_kernel reads (||D^-1 b||^2, 0, ||u0||^2); it also
-// resets the solve's two control records.
+// Start of a solve A x = b outside a step (the adjoint's solves), from x = 0
+// or (warm) from the x the vector holds: stands in for the residual kernel,
+// with r0 = b - A x, u0 = D^-1 r0 to p and the reductions cg_start_kernel
+// reads (||D^-1 b||^2, 0, ||u0||^2); it also resets the solve's two control
+// records.  A warm start gathers x, which then needs guard zones.
 template <class Rows>
 __global__ __launch_bounds__(kCgBlock) void plain_init_kernel(Rows R, MatVals mv, int n,
                                                               const double* __restrict__ b, CgVecs<double> v,
-                                                              double* __restrict__ pout, SolveCtl* ctl, int max_its)
+                                                              double* __restrict__ pout, SolveCtl* ctl, int max_its,
+                                                              int warm)
 {
     const double* __restrict__ D = mv.dinv64;
+    const double* __restrict__ A = mv.a64;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     if (blk == 0 && threadIdx.x < 2) {
         SolveCtl& t = ctl[threadIdx.x];
@@ -1223,12 +1226,26 @@ __global__ __launch_bounds__(kCgBlock) void plain_init_kernel(Rows R, MatVals mv
     const RowRange rr(n, blk, gridDim.x);
     double part[3] = {0.0, 0.0, 0.0};
     for (int i = rr.lo + threadIdx.x; i < rr.hi; i += kCgBlock) {
+        const typename Rows::Row row = R.row(i);
         const double bi = b[i];
-        const double ui = D[R.didx(R.row(i))] * bi;
-        v.r[i] = bi;
+        const double d = D[R.didx(row)];
+        double ri = bi;
+        if (warm) {
+            // r = b - A x from the x held (the previous solve's answer)
+            Gathered<Rows, double> X;
+            X.load(R, row, v.x);
+            double ax = 0.0;
+#pragma unroll
+            for (int k = 0; k < R.width(); ++k) ax += A[R.vidx(row, k)] * X.at(R, row, v.x, k);
+            ri = bi - ax;
+        } else {
+            v.x[i] = 0.0;
+        }
+        const double ui = d * ri;
+        const double ub = d * bi;
+        v.r[i] = ri;
         v.p[i] = ui;
-        v.x[i] = 0.0;
-        part[0] += ui * ui;
+        part[0] += ub * ub;
         part[2] += ui * ui;
     }
     write_partials<kCgBlock, 3>(part, pout, blk);
@@ -2137,7 +2154,7 @@ int plain_start_rows(cmx_ctx* c, const Rows& R, const PlainCg& s, const double* 
 {
     CgArgs<double> a = plain_args<Rows>(c, s, 0);
     plain_init_kernel<Rows><<<c->G, kCgBlock, 0, c->stream>>>(R, mat_vals(c), c->n, b, a.v, s.part[2], s.ctl,
-                                                              s.max_its);
+                                                              s.max_its, s.warm);
     a.first = 1;
     a.pin = s.part[2];
     a.pout = s.part[0];
@@ -2997,6 +3014,7 @@ int cmx_get_stats(cmx_ctx* c, cmx_stats* s)
     s->adjoint_sub_interval = c->adj_sub;
     s->cg_rows_per_thread = c->cg_rows > 0 ? c->cg_rows : (c->precision == CMX_MIXED ? 2 : 1);
     s->graph_steps = c->graph_steps;
+    s->adjoint_iterations = c->adj_iterations;
     return CMX_OK;
 }
 

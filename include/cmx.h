@@ -155,6 +155,7 @@ typedef struct {
     int32_t adjoint_sub_interval; /* its segment-checkpoint interval (0: one level) */
     int32_t cg_rows_per_thread; /* rows each thread of the CG kernels holds in registers (1 or 2) */
     int32_t graph_steps;       /* steps in the largest captured graph */
+    int64_t adjoint_iterations; /* CG iterations of cmx_adjoint's solves (all calls) */
 } cmx_stats;
 
 typedef struct cmx_ctx cmx_ctx;
