@@ -54,12 +54,12 @@ def summarise(path, command, title):
 
 
 KIND_OF = {'cell_step_kernel': 'cell', 'refine_residual_kernel': 'refine', 'residual_kernel': 'residual',
-           'cg_start_kernel': 'spmv', 'cg_iter_kernel': 'update', 'finish_kernel': 'finish'}
+           'cg_start_kernel': 'spmv', 'cg_iter_kernel': 'update', 'cg_last_kernel': 'last', 'finish_kernel': 'finish'}
 
 
 def _kind(name):
     for key in ('refine_residual_kernel', 'cell_step_kernel', 'residual_kernel', 'cg_start_kernel',
-                'cg_iter_kernel', 'finish_kernel'):
+                'cg_iter_kernel', 'cg_last_kernel', 'finish_kernel'):
         if key in name:
             return KIND_OF[key]
     return None
@@ -87,7 +87,7 @@ def traffic(fetch_dir, write_dir, json_out):
     out = {'scale': {'FETCH_SIZE': scale['FETCH_SIZE'], 'WRITE_SIZE': scale['WRITE_SIZE']}, 'mixed': {}, 'fp64': {}}
     lines = ['# HBM traffic per dispatch: TT06 slab, 442,401 nodes (bench workload)', '',
              'Counter passes (GPU box, repository root; one rocprofv3 run per counter):', '',
-             '    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o run -- python3 bench.py --steps 200 --warmup 20 --no-cpu-baseline --skip-propagating --no-kernel-events --no-graph --no-autotune',
+             '    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o run -- python3 bench.py --steps 200 --warmup 20 --no-cpu-baseline --skip-propagating --no-kernel-events --no-graph --no-autotune --no-accuracy --skip-cfg1 --no-tables',
              '    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch_dir> -o calib -- python3 -m chaste_amd.hbm_calib 24 1024',
              '    (the same two with WRITE_SIZE into <write_dir>)',
              '    python -m chaste_amd.profile_summary traffic <fetch_dir> <write_dir> profiles/hbm_traffic_slab442k.json', '',
