@@ -389,7 +389,8 @@ def cells_case(args):
     stimuli = [(-80.0, 0.0, 2.0, float(p)) for p in period]
     progress('cells: %d %s mixed' % (n, name))
     y, props, st = cmx.cells_run(model, cmx.RUSH_LARSEN, t_end, 0.005, stimuli=stimuli, params=params,
-                                 precision=cmx.MIXED, threshold=-30.0, lookup_tables=LOOKUP_TABLES)
+                                 precision=cmx.MIXED, threshold=-30.0, lookup_tables=LOOKUP_TABLES,
+                                 lanes_per_cell=args.lanes_per_cell)
     sub = np.sort(rng.choice(n, min(args.cells_check, n), replace=False))
     progress('cells: %d FP64 for the accuracy check' % len(sub))
     y64, p64, st64 = cmx.cells_run(model, cmx.RUSH_LARSEN, t_end[sub], 0.005,
@@ -408,6 +409,7 @@ def cells_case(args):
                                + (', gate lookup tables (0.02 mV rows)' if LOOKUP_TABLES else ''),
                    'n_cells': n, 'lookup_tables': LOOKUP_TABLES},
         'kernel_s': round(st['kernel_ms'] * 1e-3, 3), 'cell_steps': st['cell_steps'], 'launches': st['launches'],
+        'lanes_per_cell': args.lanes_per_cell, 'split_launches': st['split_launches'],
         'longest_cell_steps': int(steps.max()),
         'lane_balance': round(float(steps.sum() / (n * steps.max())), 3),
         'all_lanes_busy_rate': round(n * float(steps.max()) / (st['kernel_ms'] * 1e-3), 1),
@@ -760,6 +762,8 @@ def main():
                     help='the batched-cell workload with this many cells instead (65536 for the named sweep)')
     ap.add_argument('--cells-check', type=int, default=256, help='cells also run in FP64 for the accuracy check')
     ap.add_argument('--cells-model', choices=['ord', 'tt06'], default='ord', help='cell model of the batched-cell sweep')
+    ap.add_argument('--lanes-per-cell', type=int, default=0, choices=[0, 1, 2],
+                    help='batched cells: 1 or 2 wavefronts per 64 cells; 0 picks per launch by occupancy')
     ap.add_argument('--shells', action='store_true', help='the batched ellipsoid-shell tissue workload instead')
     ap.add_argument('--shell-steps', type=int, default=10000)
     ap.add_argument('--shell-h', type=float, nargs=2, default=(0.04, 0.02), metavar=('H_MAX', 'H_MIN'),

@@ -55,12 +55,12 @@ class CmxCells(ctypes.Structure):
                 ('n_cells', ctypes.c_int32), ('dt', ctypes.c_double), ('t_end', _dp),
                 ('stimuli', ctypes.POINTER(CmxStimulus)), ('n_params', ctypes.c_int32), ('params', _dp),
                 ('threshold', ctypes.c_double), ('steps_per_launch', ctypes.c_int32),
-                ('lookup_tables', ctypes.c_int32)]
+                ('lookup_tables', ctypes.c_int32), ('lanes_per_cell', ctypes.c_int32)]
 
 
 class CmxCellsStats(ctypes.Structure):
     _fields_ = [('kernel_ms', ctypes.c_double), ('cell_steps', ctypes.c_int64), ('launches', ctypes.c_int32),
-                ('block', ctypes.c_int32)]
+                ('block', ctypes.c_int32), ('split_launches', ctypes.c_int32)]
 
 
 class CmxStats(ctypes.Structure):
@@ -196,7 +196,7 @@ def param_info(model):
 
 
 def cells_run(model, integrator, t_end, dt, stimuli=None, params=None, y0=None, precision=FP64,
-              threshold=-30.0, steps_per_launch=0, lookup_tables=False):
+              threshold=-30.0, steps_per_launch=0, lookup_tables=False, lanes_per_cell=0):
     """Batched single cells (cmx_cells_run).  t_end [n]; stimuli: one
     (magnitude, start, duration[, period[, stop]]) per cell or None; params
     [n_params, n].  Returns (final states [S, n], props [4, n]: completed APs,
@@ -221,6 +221,7 @@ def cells_run(model, integrator, t_end, dt, stimuli=None, params=None, y0=None, 
         keep.append(pa)
     c.threshold, c.steps_per_launch = threshold, steps_per_launch
     c.lookup_tables = 1 if lookup_tables else 0
+    c.lanes_per_cell = int(lanes_per_cell)
     yi = None if y0 is None else np.ascontiguousarray(y0, dtype=np.float64)
     y = np.zeros((S, n))
     props = np.zeros((4, n))

@@ -161,6 +161,134 @@ __global__ __launch_bounds__(kBlock) void cells_kernel(CellsArgs<RS> a, int k0, 
     a.end[i] = end;
 }
 
+// Two wavefronts per 64 cells (lanes_per_cell = 2): wavefront 0 of the block
+// advances the cells' gates, wavefront 1 everything else (currents, the
+// non-gate states, V and the CellProperties summary).  A gate's Rush-Larsen
+// (or Euler) coefficients depend on V only, so the first wavefront needs only
+// V from the second, and the second needs the gates from the first: they
+// exchange both through LDS once per step (one barrier; buffers alternate
+// with the step's parity).  Each wavefront runs the model's right-hand side
+// and uses only its own outputs, so the other part is dead code it does not
+// execute.  The step's arithmetic is the one-lane kernel's, split: a cell's
+// critical path per step is the longer of the two parts, which is what bounds
+// a sweep whose wavefronts are fewer than the SIMDs can interleave.
+constexpr int kSplitCells = 64;
+
+template <class Model, class RS, int IG>
+__global__ __launch_bounds__(2 * kSplitCells) void cells_split_kernel(CellsArgs<RS> a, int k0, int k1)
+{
+    constexpr int NS = Model::NS;
+    constexpr int NG = GateCols<Model>::count();
+    constexpr int NPd = Model::NP > 0 ? Model::NP : 1;
+    constexpr bool RL = IG == CMX_RUSH_LARSEN;
+    __shared__ RS s_gate[2][NG > 0 ? NG : 1][kSplitCells];
+    __shared__ double s_v[2][kSplitCells];
+    const int lane = threadIdx.x & (kSplitCells - 1);
+    const bool gates_role = threadIdx.x < kSplitCells;
+    const int j = blockIdx.x * kSplitCells + lane;
+    const bool valid = j < a.n;
+    const int i = valid ? j : a.n - 1;
+    // the block's longest cell bounds its steps (both wavefronts hold the same
+    // 64 cells, so each reduces the same maximum): every thread of the block
+    // runs the same number of barriers
+    const int kend = valid ? min(k1, a.n_steps[i]) : k0;
+    int kmax = kend;
+#pragma unroll
+    for (int off = kSplitCells / 2; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off, kSplitCells));
+    if (k0 >= kmax) return;
+    RS y[NS], p[NPd];
+    double yd[NS];
+#pragma unroll
+    for (int s = 1; s < NS; ++s) {
+        yd[s] = a.state[(size_t)(s - 1) * a.ps + i];
+        y[s] = (RS)yd[s];
+    }
+#pragma unroll
+    for (int q = 0; q < NPd; ++q)
+        p[q] = (q < a.n_param_planes) ? a.params[(size_t)q * a.ps + i] : (Model::NP > 0 ? a.pdef[q] : RS(0));
+    double vd = a.V[i];
+    cmx_stimulus st{};
+    if (a.stim) st = a.stim[i];
+    int phase = a.phase[i], count = a.ap_count[i];
+    double onset = a.onset[i], peak = a.peak[i], last_peak = a.last_peak[i], end = a.end[i];
+    const RS cm = (RS)a.cm, h = (RS)a.dt;
+    const double thr = a.thr;
+    for (int k = k0; k < kmax; ++k) {
+        const int b = k & 1;
+        if (k < kend) {
+            const double t = k * a.dt;
+            const RS V = (RS)vd;
+            y[0] = V;
+            RS dy[NS], gi[NS], gk[NS];
+            if (gates_role) {
+                Model::template eval<RS, RL>(y, V, RS(0), p, cm, dy, gi, gk);
+#pragma unroll
+                for (int s = 1; s < NS; ++s) {
+                    if (!Model::gate(s)) continue;
+                    if (RL) y[s] = gi[s] + (y[s] - gi[s]) * cmx_decay(gk[s], h);
+                    else y[s] += h * dy[s];
+                    s_gate[b][GateCols<Model>::col(s)][lane] = y[s];
+                }
+            } else {
+                const RS is = (RS)(a.stim ? stimulus_at(st, t) : 0.0);
+                Model::template eval<RS, RL>(y, V, is, p, cm, dy, gi, gk);
+                const RS dv = Model::template dvdt<RS>(y, V, is, p, cm);
+#pragma unroll
+                for (int s = 1; s < NS; ++s) {
+                    if (Model::gate(s)) continue;
+                    yd[s] += a.dt * (double)dy[s];
+                    y[s] = (RS)yd[s];
+                }
+                const double v0 = vd;
+                vd += a.dt * (double)dv;
+                s_v[b][lane] = vd;
+                const double t1 = (k + 1) * a.dt;
+                if (phase == 0 && vd > thr && v0 <= thr) {
+                    onset = t + (t1 - t) / (vd - v0) * (thr - v0);
+                    phase = 1;
+                }
+                if (phase == 1) {
+                    if (vd > peak) peak = vd;
+                    if (vd < thr && v0 >= thr) {
+                        last_peak = peak;
+                        peak = thr;
+                        end = t + (t1 - t) / (vd - v0) * (thr - v0);
+                        ++count;
+                        phase = 0;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (k < kend) {
+            if (gates_role) {
+                vd = s_v[b][lane];
+            } else {
+#pragma unroll
+                for (int s = 1; s < NS; ++s)
+                    if (Model::gate(s)) y[s] = s_gate[b][GateCols<Model>::col(s)][lane];
+            }
+        }
+    }
+    if (!valid) return;
+    if (gates_role) {
+#pragma unroll
+        for (int s = 1; s < NS; ++s)
+            if (Model::gate(s)) a.state[(size_t)(s - 1) * a.ps + i] = (double)y[s];
+        return;
+    }
+#pragma unroll
+    for (int s = 1; s < NS; ++s)
+        if (!Model::gate(s)) a.state[(size_t)(s - 1) * a.ps + i] = yd[s];
+    a.V[i] = vd;
+    a.phase[i] = phase;
+    a.ap_count[i] = count;
+    a.onset[i] = onset;
+    a.peak[i] = peak;
+    a.last_peak[i] = last_peak;
+    a.end[i] = end;
+}
+
 struct Buffers {
     std::vector<void*> ptrs;
     ~Buffers()
@@ -191,8 +319,12 @@ struct Buffers {
 };
 
 template <class Model, class RS, int IG>
-void launch_cells(const CellsArgs<RS>& a, int k0, int k1, hipStream_t s)
+void launch_cells(const CellsArgs<RS>& a, int k0, int k1, hipStream_t s, int lanes = 1)
 {
+    if (lanes == 2) {
+        cells_split_kernel<Model, RS, IG><<<(a.n + kSplitCells - 1) / kSplitCells, 2 * kSplitCells, 0, s>>>(a, k0, k1);
+        return;
+    }
     if constexpr (IG == CMX_RUSH_LARSEN && std::is_same<RS, float>::value) {
         if (a.lut) {
             cells_kernel<Model, RS, IG, true><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a, k0, k1);
@@ -328,27 +460,51 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
     const int longest = n > 0 ? s_steps[0] : 0;
     const int chunk = std::max(1, c.steps_per_launch > 0 ? c.steps_per_launch : 2000);
     int launches = 0;
+    const bool rl = c.integrator == CMX_RUSH_LARSEN;
+    auto launch = [&](int k0, int k1, int lanes) -> int {
+        // only the cells still running: the sorted prefix with n_steps > k0
+        // (counts sorted descending: lower_bound with greater<> finds the first <= k0)
+        CellsArgs<RS> ak = a;
+        ak.n = (int)(std::lower_bound(s_steps.begin(), s_steps.end(), k0, std::greater<int32_t>()) - s_steps.begin());
+        if (ak.n <= 0) return 0;
+        if (c.model == CMX_MODEL_LR91) {
+            if (rl) launch_cells<ModelLr91, RS, CMX_RUSH_LARSEN>(ak, k0, k1, stream, lanes);
+            else launch_cells<ModelLr91, RS, CMX_FORWARD_EULER>(ak, k0, k1, stream, lanes);
+        } else if (c.model == CMX_MODEL_ORD_ENDO) {
+            if (rl) launch_cells<ModelOrd, RS, CMX_RUSH_LARSEN>(ak, k0, k1, stream, lanes);
+            else launch_cells<ModelOrd, RS, CMX_FORWARD_EULER>(ak, k0, k1, stream, lanes);
+        } else {
+            if (rl) launch_cells<ModelTt06, RS, CMX_RUSH_LARSEN>(ak, k0, k1, stream, lanes);
+            else launch_cells<ModelTt06, RS, CMX_FORWARD_EULER>(ak, k0, k1, stream, lanes);
+        }
+        return 1;
+    };
+    // lanes_per_cell 0: per launch, two wavefronts per 64 cells while the
+    // launch's one-lane wavefronts would fill at most half of the device's
+    // SIMDs (each then hosts one wavefront of a cell pair, and a cell's step
+    // takes the longer of its two halves); one while more run (a SIMD then
+    // interleaves wavefronts anyway and the split only adds the exchange)
+    const int lanes_cfg = c.lookup_tables ? 1 : c.lanes_per_cell;
+    int n_simd = 1024;
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            n_simd = 4 * prop.multiProcessorCount;
+        else
+            (void)hipGetLastError();
+    }
+    int split_launches = 0;
     CELLS_HIP(hipEventRecord(e0, stream));
     for (int k0 = 0; k0 < longest; k0 += chunk) {
-        const int k1 = std::min(longest, k0 + chunk);
-        // only the cells still running: the sorted prefix with n_steps > k0
-        CellsArgs<RS> ak = a;
-        // (counts sorted descending: lower_bound with greater<> finds the first <= k0)
-        ak.n = (int)(std::lower_bound(s_steps.begin(), s_steps.end(), k0, std::greater<int32_t>()) - s_steps.begin());
-        if (ak.n <= 0) break;
-        const bool rl = c.integrator == CMX_RUSH_LARSEN;
-        if (c.model == CMX_MODEL_LR91) {
-            if (rl) launch_cells<ModelLr91, RS, CMX_RUSH_LARSEN>(ak, k0, k1, stream);
-            else launch_cells<ModelLr91, RS, CMX_FORWARD_EULER>(ak, k0, k1, stream);
-        } else if (c.model == CMX_MODEL_ORD_ENDO) {
-            if (rl) launch_cells<ModelOrd, RS, CMX_RUSH_LARSEN>(ak, k0, k1, stream);
-            else launch_cells<ModelOrd, RS, CMX_FORWARD_EULER>(ak, k0, k1, stream);
-        } else {
-            if (rl) launch_cells<ModelTt06, RS, CMX_RUSH_LARSEN>(ak, k0, k1, stream);
-            else launch_cells<ModelTt06, RS, CMX_FORWARD_EULER>(ak, k0, k1, stream);
-        }
+        const int active = (int)(std::lower_bound(s_steps.begin(), s_steps.end(), k0, std::greater<int32_t>()) -
+                                 s_steps.begin());
+        const int waves = (active + 63) / 64;
+        const int lanes = lanes_cfg == 0 ? (2 * waves <= n_simd ? 2 : 1) : lanes_cfg;
+        if (!launch(k0, std::min(longest, k0 + chunk), lanes)) break;
         CELLS_HIP(hipGetLastError());
         ++launches;
+        split_launches += lanes == 2;
     }
     CELLS_HIP(hipEventRecord(e1, stream));
     CELLS_HIP(hipStreamSynchronize(stream));
@@ -391,6 +547,7 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
         stats->cell_steps = cell_steps;
         stats->launches = launches;
         stats->block = kBlock;
+        stats->split_launches = split_launches;
     }
     return CMX_OK;
 }
@@ -579,15 +736,16 @@ int run_cells_noalloc(const cmx_cells& c, int S, int P, const double* y0, double
     CELLS_HIP(hipGetLastError());
     // every cell runs its own step count in one launch (k1 beyond any count)
     const bool rl = c.integrator == CMX_RUSH_LARSEN;
+    const int lanes = c.lanes_per_cell == 2 ? 2 : 1;
     if (c.model == CMX_MODEL_LR91) {
-        if (rl) launch_cells<ModelLr91, RS, CMX_RUSH_LARSEN>(a, 0, INT32_MAX, stream);
-        else launch_cells<ModelLr91, RS, CMX_FORWARD_EULER>(a, 0, INT32_MAX, stream);
+        if (rl) launch_cells<ModelLr91, RS, CMX_RUSH_LARSEN>(a, 0, INT32_MAX, stream, lanes);
+        else launch_cells<ModelLr91, RS, CMX_FORWARD_EULER>(a, 0, INT32_MAX, stream, lanes);
     } else if (c.model == CMX_MODEL_ORD_ENDO) {
-        if (rl) launch_cells<ModelOrd, RS, CMX_RUSH_LARSEN>(a, 0, INT32_MAX, stream);
-        else launch_cells<ModelOrd, RS, CMX_FORWARD_EULER>(a, 0, INT32_MAX, stream);
+        if (rl) launch_cells<ModelOrd, RS, CMX_RUSH_LARSEN>(a, 0, INT32_MAX, stream, lanes);
+        else launch_cells<ModelOrd, RS, CMX_FORWARD_EULER>(a, 0, INT32_MAX, stream, lanes);
     } else {
-        if (rl) launch_cells<ModelTt06, RS, CMX_RUSH_LARSEN>(a, 0, INT32_MAX, stream);
-        else launch_cells<ModelTt06, RS, CMX_FORWARD_EULER>(a, 0, INT32_MAX, stream);
+        if (rl) launch_cells<ModelTt06, RS, CMX_RUSH_LARSEN>(a, 0, INT32_MAX, stream, lanes);
+        else launch_cells<ModelTt06, RS, CMX_FORWARD_EULER>(a, 0, INT32_MAX, stream, lanes);
     }
     CELLS_HIP(hipGetLastError());
     CellsArgs<double> f{};
@@ -625,6 +783,8 @@ int cmx_cells_run(const cmx_cells* c, const double* y0, double* y_out, double* p
     if (!(c->dt > 0)) return cfail(CMX_E_ARG, "dt must be positive");
     if (c->n_params < 0 || c->n_params > P || (c->n_params > 0 && !c->params))
         return cfail(CMX_E_ARG, "n_params must be in [0, the model's parameter count] with params given");
+    if (c->lanes_per_cell < 0 || c->lanes_per_cell > 2) return cfail(CMX_E_ARG, "lanes_per_cell must be 0, 1 or 2");
+    if (c->lookup_tables && c->lanes_per_cell == 2) return cfail(CMX_E_ARG, "lookup tables need lanes_per_cell 1");
     if (c->n_cells == 0) return CMX_OK;
     return c->precision == CMX_MIXED ? run_cells<float>(*c, S, P, y0, y_out, props, stats)
                                      : run_cells<double>(*c, S, P, y0, y_out, props, stats);
@@ -645,6 +805,7 @@ int cmx_cells_run_noalloc(const cmx_cells* c, const double* y0, double* y_out, d
     if (c->n_params < 0 || c->n_params > P || (c->n_params > 0 && !c->params))
         return cfail(CMX_E_ARG, "n_params must be in [0, the model's parameter count] with params given");
     if (c->lookup_tables) return cfail(CMX_E_ARG, "lookup tables need a table allocation: use cmx_cells_run");
+    if (c->lanes_per_cell < 0 || c->lanes_per_cell > 2) return cfail(CMX_E_ARG, "lanes_per_cell must be 0, 1 or 2");
     if (c->n_cells == 0) return CMX_OK;
     hipStream_t s = (hipStream_t)stream;
     return c->precision == CMX_MIXED ? run_cells_noalloc<float>(*c, S, P, y0, y_out, props, s)

@@ -358,13 +358,20 @@ typedef struct {
                                     from a table in V (as cmx_opts.lookup_tables); a voltage
                                     outside the table fails the run.  Not with the
                                     allocation-free variant. */
+    int32_t lanes_per_cell;      /* 1: one thread steps a cell; 2: two wavefronts share 64 cells, one
+                                    advancing the gates and one the rest (the step's critical path
+                                    halves when wavefronts are too few to interleave on a SIMD);
+                                    0: per launch, 2 while the launch's cells would fill at most half
+                                    of the device's SIMDs with one lane each, else 1 (the
+                                    allocation-free variant takes 0 as 1).  Lookup tables need 1. */
 } cmx_cells;
 
 typedef struct {
     double kernel_ms;            /* device time of all launches */
     int64_t cell_steps;          /* sum of the cells' step counts */
     int32_t launches;
-    int32_t block;               /* threads per block (one cell per thread) */
+    int32_t block;               /* threads per block of the one-lane launches */
+    int32_t split_launches;      /* launches with two wavefronts per 64 cells */
 } cmx_cells_stats;
 
 /* y0: [n_states][n_cells] initial states (NULL: the model's initial state);

@@ -108,3 +108,42 @@ def test_cells_reject_bad_input():
         cmx.cells_run(0, 0, [1.0], 0.01, params=np.ones((4, 1)))
     with pytest.raises(cmx.CmxError):
         cmx.cells_run(1, 1, [10.0], 0.005, stimuli=[(-80.0, 0.0, 5.0, 2.0)])   # duration > period
+
+
+@pytest.mark.parametrize('model_name', ['lr91', 'tt06', 'ord'])
+def test_two_wavefronts_per_cell_match_one_lane(model_name):
+    """lanes_per_cell = 2 (one wavefront advancing the gates, one the rest, the
+    gates and V exchanged through LDS each step) runs the one-lane step's
+    arithmetic split in two: FP64 final states within 1e-12 of each state's
+    scale and the same CellProperties summary; mixed within 1e-3 mV.  Uneven
+    lengths (cells ending inside a block of 64) and a partial last block are
+    covered; the automatic layout splits a sweep this small."""
+    from chaste_amd import cmx
+    model, integrator, dt = {'lr91': (cmx.LR91, 0, 0.01), 'tt06': (cmx.TT06_EPI, 1, 0.005),
+                             'ord': (cmx.ORD_ENDO, 1, 0.005)}[model_name]
+    print('case cells two wavefronts', model_name, flush=True)
+    params, stimuli, t_end = sweep(model, 150, seed=7, beats=(1, 2), cl=(300.0, 500.0))
+    if model == cmx.ORD_ENDO:
+        stimuli = [(-80.0, 0.0, 2.0, s[3]) for s in stimuli]
+    for prec, tol in ((cmx.FP64, 1e-12), (cmx.MIXED, None)):
+        y1, p1, s1 = cmx.cells_run(model, integrator, t_end, dt, stimuli=stimuli, params=params, precision=prec,
+                                   lanes_per_cell=1, steps_per_launch=3000)
+        y2, p2, s2 = cmx.cells_run(model, integrator, t_end, dt, stimuli=stimuli, params=params, precision=prec,
+                                   lanes_per_cell=2, steps_per_launch=3000)
+        assert s1['split_launches'] == 0 and s2['split_launches'] == s2['launches'] > 0
+        assert np.isfinite(y2).all()
+        assert np.array_equal(p1[0], p2[0])
+        if tol is not None:
+            scale = np.abs(y1).max(axis=1, keepdims=True) + 1e-300
+            print('two wavefronts fp64 rel', (np.abs(y2 - y1) / scale).max())
+            assert (np.abs(y2 - y1) / scale).max() <= tol
+            assert np.abs(p2[1:] - p1[1:]).max() <= 1e-9
+        else:
+            print('two wavefronts mixed dV', np.abs(y2[0] - y1[0]).max())
+            assert np.abs(y2[0] - y1[0]).max() <= 1e-3
+    # automatic: 150 cells are 3 wavefronts, far below half the SIMDs, so every launch splits
+    ya, pa, sa = cmx.cells_run(model, integrator, t_end, dt, stimuli=stimuli, params=params, precision=cmx.MIXED,
+                               steps_per_launch=3000)
+    assert sa['split_launches'] == sa['launches'] > 0 and np.array_equal(ya, y2) and np.array_equal(pa, p2)
+    with pytest.raises(cmx.CmxError, match='lanes_per_cell'):
+        cmx.cells_run(model, integrator, t_end, dt, precision=cmx.MIXED, lanes_per_cell=3)

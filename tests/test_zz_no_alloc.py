@@ -93,7 +93,7 @@ def test_noalloc_matches_cells_run(precision):
         t_end, stim, params, y0 = _inputs(cmx, n, seed)
         y_ref, p_ref, _ = cmx.cells_run(cmx.TT06_EPI, cmx.RUSH_LARSEN, t_end, 0.005,
                                         stimuli=[tuple(r[:4]) for r in stim], params=params, y0=y0,
-                                        precision=precision, threshold=-30.0)
+                                        precision=precision, threshold=-30.0, lanes_per_cell=1)
         y, props = _noalloc(cmx, hip, cmx.TT06_EPI, precision, t_end, stim, params, y0, 0.005, -30.0)
         print('n %d: max |dy| %.3e, props equal %s, APs %d' % (n, np.abs(y - y_ref).max(),
                                                                np.array_equal(props, p_ref), int(p_ref[0].sum())))
