@@ -23,6 +23,8 @@ import time
 
 import numpy as np
 
+FORMAT_NAMES = {1: 'stencil', 2: 'ell', 3: 'ell_offsets'}
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -492,17 +494,20 @@ def shells_case(args):
         s_b = 4 if prec == cmx.MIXED else 8
         w = st2['row_width']
         its = st2['cg_iterations'] - st1['cg_iterations']
-        upd_bytes = n * (14 * s_b + w * (4 + s_b))          # vectors + one ELL row of A (column, value)
+        # vectors + one ELL row of A: a value per slot, and a 4-byte column per
+        # slot (column-index ELL) or one set byte per row (offset-set ELL)
+        row_b = 14 * s_b + w * s_b + (1 if st2['format'] == cmx.FMT_ELL_OFFSETS else 4 * w)
+        upd_bytes = n * row_b
         kernels = {kk: {'ms_per_step': round(kt[kk][0] / 20, 4), 'launches_per_step': kt[kk][1] / 20}
                    for kk in KINDS if kt[kk][1]}
         if kt['update'][1] and its:
             kernels['update']['working_launch_GBps'] = round(upd_bytes * its / (kt['update'][0] * 1e-3) / 1e9, 1)
-            kernels['update']['bytes_per_row'] = 14 * s_b + w * (4 + s_b)
+            kernels['update']['bytes_per_row'] = row_b
         out[name] = {'steps_per_s': round(steps / el, 2), 'node_steps_per_s': round(steps * n / el, 1),
                      'timed_steps': steps, 'timed_window_ms': [warm * DT, k * DT], 'setup_s': round(setup, 2),
                      'common_window': common,
                      'cg_its_per_step': round((st1['cg_iterations'] - st0['cg_iterations']) / steps, 3),
-                     'format': 'stencil' if st1['format'] == cmx.FMT_STENCIL else 'ell',
+                     'format': FORMAT_NAMES[st1['format']], 'offset_sets': st1['n_patterns'],
                      'row_width': st1['row_width'], 'kernels_20_steps': kernels}
         m.close()
     n_elems = int(elems.shape[0])
@@ -845,7 +850,7 @@ def main():
                 'dominant_avg_launch_ms': r['dominant_ms'] / dom_launches,
                 'dominant_bytes_per_launch': dom_bytes / dom_launches,
                 'dominant_survey_bytes_per_launch': sb['cell'] if dom == 'cell' else None,
-                'format': 'stencil' if st['format'] == cmx.FMT_STENCIL else 'ell',
+                'format': FORMAT_NAMES[st['format']],
                 'n_patterns': st['n_patterns'],
                 'autotune': r['autotune'],
             }

@@ -13,7 +13,7 @@ from .build import library_path
 FP64, MIXED = 0, 1
 LR91, TT06_EPI, ORD_ENDO = 0, 1, 2
 FORWARD_EULER, RUSH_LARSEN, BACKWARD_EULER, GRL1, GRL2 = 0, 1, 2, 3, 4
-FMT_AUTO, FMT_STENCIL, FMT_ELL = 0, 1, 2
+FMT_AUTO, FMT_STENCIL, FMT_ELL, FMT_ELL_OFFSETS = 0, 1, 2, 3
 KERNEL_KINDS = ['cell', 'residual', 'spmv', 'update', 'pupdate', 'finish', 'svi']
 
 _dp = ctypes.POINTER(ctypes.c_double)

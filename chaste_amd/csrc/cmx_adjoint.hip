@@ -598,7 +598,7 @@ int cmx_adjoint(cmx_ctx* c, double t0, int32_t n_steps, int32_t checkpoint_every
         return adjoint_impl(c, adj_stencil_rows<16>(c), t0, n_steps, checkpoint_every, n_obs, obs_nodes, obs_every,
                             obs_values, loss, grad_params, grad_state0);
     }
-    EllRows R{c->ell_col, c->width, c->n};
+    const EllRows R = ell_rows(c);
     return adjoint_impl(c, R, t0, n_steps, checkpoint_every, n_obs, obs_nodes, obs_every, obs_values, loss,
                         grad_params, grad_state0);
 }

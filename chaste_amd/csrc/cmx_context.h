@@ -310,6 +310,8 @@ struct cmx_ctx {
     int pad = 0;                 // guard elements on each side of the gathered vectors
     int ps = 0;                  // cell-state / parameter plane stride (elements)
     int32_t* ell_col = nullptr;
+    uint8_t* ell_set = nullptr;      // offset-set ELL: set of each row
+    int32_t* ell_off = nullptr;      // [sets][width] offsets
     double *m64 = nullptr, *k64 = nullptr, *a64 = nullptr, *dinv64 = nullptr;
     float *a32 = nullptr, *dinv32 = nullptr;
     double* partials = nullptr;  // 3 buffers of [kMaxBlocks * 3]
@@ -398,6 +400,15 @@ int step_range(cmx_ctx* c, double t0, int n_total, int k0, int k1);
 // ends a solve whose b is already below the test; plain_cg_iters launches
 // iterations from .. from+count-1, each returning at once once ctl[0].done
 // is set (1 converged, 2 max_its reached, 3 non-finite).
+// The ELL rows of a context (column indices, or offset sets).
+inline EllRows ell_rows(const cmx_ctx* c)
+{
+    EllRows R{c->ell_col, c->width, c->n};
+    R.sets = c->format == CMX_FMT_ELL_OFFSETS ? c->ell_set : nullptr;
+    R.set_off = c->ell_off;
+    return R;
+}
+
 struct PlainCg {
     double *r, *w, *z, *s;
     double* x;           // the solution; gathered (guard zones) when a solve starts from it

@@ -2108,8 +2108,7 @@ int steps_typed(cmx_ctx* c)
         if (c->width == 8) return run_steps<StencilRows<8>, RS, RV>(c, stencil_rows<8>(c));
         return run_steps<StencilRows<16>, RS, RV>(c, stencil_rows<16>(c));
     }
-    EllRows R{c->ell_col, c->width, c->n};
-    return run_steps<EllRows, RS, RV>(c, R);
+    return run_steps<EllRows, RS, RV>(c, ell_rows(c));
 }
 
 template <class RS, class RV>
@@ -2120,8 +2119,7 @@ int prime_typed(cmx_ctx* c, int todo)
         if (c->width == 8) return prime_graphs<StencilRows<8>, RS, RV>(c, stencil_rows<8>(c), todo);
         return prime_graphs<StencilRows<16>, RS, RV>(c, stencil_rows<16>(c), todo);
     }
-    EllRows R{c->ell_col, c->width, c->n};
-    return prime_graphs<EllRows, RS, RV>(c, R, todo);
+    return prime_graphs<EllRows, RS, RV>(c, ell_rows(c), todo);
 }
 
 template <class Rows>
@@ -2404,12 +2402,24 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
 
     int fmt = opts->format;
     StencilTable st;
+    OffsetEll oell;
     if (fmt == CMX_FMT_AUTO || fmt == CMX_FMT_STENCIL) {
         if (build_stencil(csr, kMaxPatterns, kMaxStencil, st)) fmt = CMX_FMT_STENCIL;
         else if (fmt == CMX_FMT_STENCIL)
             return fail(c, CMX_E_ARG, "mesh needs more than 64 row patterns or 16 stencil offsets: use the ELL format");
-        else fmt = CMX_FMT_ELL;
+        else fmt = CMX_FMT_ELL_OFFSETS;
     }
+    if (fmt == CMX_FMT_ELL_OFFSETS) {
+        if (!build_offset_ell(csr, 255, oell)) {
+            if (opts->format == CMX_FMT_ELL_OFFSETS)
+                return fail(c, CMX_E_ARG, "mesh rows need more than 255 column-offset sets: use the ELL format");
+            fmt = CMX_FMT_ELL;
+        }
+    }
+    if (fmt != CMX_FMT_STENCIL && fmt != CMX_FMT_ELL && fmt != CMX_FMT_ELL_OFFSETS)
+        return fail(c, CMX_E_ARG, "unknown matrix format");
+    if (fmt != CMX_FMT_STENCIL && ell_size(n, csr.max_row) > (size_t)INT32_MAX)
+        return fail(c, CMX_E_ARG, "mesh too large for the ELL format's 32-bit value index");
     c->format = fmt;
     int rc;
     const double cA = c->am * c->cm / c->dt;
@@ -2449,6 +2459,32 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
         if ((rc = upload(c, &c->a32, a32.data(), a32.size()))) return rc;
         if ((rc = upload(c, &c->dinv64, dinv64.data(), NPt))) return rc;
         if ((rc = upload(c, &c->dinv32, dinv32.data(), NPt))) return rc;
+    } else if (fmt == CMX_FMT_ELL_OFFSETS) {
+        const int W = oell.width;
+        const size_t E = oell.m.size();
+        c->width = W;
+        c->n_patterns = oell.n_sets;
+        // guard zones as wide as the largest offset (whole 256-byte units)
+        c->pad = (oell.max_abs + 63) / 64 * 64;
+        std::vector<double> a64(E), dinv64(n);
+        std::vector<float> a32(E), dinv32(n);
+        for (size_t s = 0; s < E; ++s) {
+            a64[s] = (1.0 / c->dt) * (c->am * c->cm * oell.m[s]) + oell.k[s];
+            a32[s] = (float)a64[s];
+        }
+        for (int i = 0; i < n; ++i) {
+            for (int p = csr.row_ptr[i]; p < csr.row_ptr[i + 1]; ++p)
+                if (csr.col[p] == i) dinv64[i] = 1.0 / (cA * csr.mass[p] + csr.stiff[p]);
+            dinv32[i] = (float)dinv64[i];
+        }
+        if ((rc = upload(c, &c->ell_set, oell.set.data(), n))) return rc;
+        if ((rc = upload(c, &c->ell_off, oell.offsets.data(), oell.offsets.size()))) return rc;
+        if ((rc = upload(c, &c->m64, oell.m.data(), E))) return rc;
+        if ((rc = upload(c, &c->k64, oell.k.data(), E))) return rc;
+        if ((rc = upload(c, &c->a64, a64.data(), E))) return rc;
+        if ((rc = upload(c, &c->a32, a32.data(), E))) return rc;
+        if ((rc = upload(c, &c->dinv64, dinv64.data(), n))) return rc;
+        if ((rc = upload(c, &c->dinv32, dinv32.data(), n))) return rc;
     } else {
         EllMatrix ell;
         build_ell(csr, ell);
@@ -2682,7 +2718,7 @@ int cmx::plain_cg_start(cmx_ctx* c, const PlainCg& s, const double* b)
         if (c->width == 8) return plain_start_rows(c, stencil_rows<8>(c), s, b);
         return plain_start_rows(c, stencil_rows<16>(c), s, b);
     }
-    return plain_start_rows(c, EllRows{c->ell_col, c->width, c->n}, s, b);
+    return plain_start_rows(c, ell_rows(c), s, b);
 }
 
 int cmx::plain_cg_iters(cmx_ctx* c, const PlainCg& s, int from, int count)
@@ -2693,7 +2729,7 @@ int cmx::plain_cg_iters(cmx_ctx* c, const PlainCg& s, int from, int count)
         if (c->width == 8) return plain_iters_rows(c, stencil_rows<8>(c), s, from, count);
         return plain_iters_rows(c, stencil_rows<16>(c), s, from, count);
     }
-    return plain_iters_rows(c, EllRows{c->ell_col, c->width, c->n}, s, from, count);
+    return plain_iters_rows(c, ell_rows(c), s, from, count);
 }
 
 int cmx::step_range(cmx_ctx* c, double t0, int n_total, int k0, int k1)

@@ -235,13 +235,13 @@ void build_ell(const HostCsr& A, EllMatrix& out)
 {
     const int n = A.n, W = A.max_row;
     out.width = W;
-    out.col.assign((size_t)W * n, 0);
-    out.m.assign((size_t)W * n, 0.0);
-    out.k.assign((size_t)W * n, 0.0);
+    out.col.assign(ell_size(n, W), 0);
+    out.m.assign(ell_size(n, W), 0.0);
+    out.k.assign(ell_size(n, W), 0.0);
     for (int i = 0; i < n; ++i) {
         const int b = A.row_ptr[i], e = A.row_ptr[i + 1];
         for (int s = 0; s < W; ++s) {
-            const size_t d = (size_t)s * n + i;
+            const size_t d = ell_index(i, s, W);
             if (b + s < e) {
                 out.col[d] = A.col[b + s];
                 out.m[d] = A.mass[b + s];
@@ -251,6 +251,69 @@ void build_ell(const HostCsr& A, EllMatrix& out)
             }
         }
     }
+}
+
+bool build_offset_ell(const HostCsr& A, int max_sets, OffsetEll& out)
+{
+    constexpr size_t kMaxOffsetLists = 1 << 16;
+    const int n = A.n, W = A.max_row;
+    // distinct offset lists, and the rows that have each
+    std::map<std::vector<int32_t>, int> list_id;
+    std::vector<std::vector<int32_t>> lists;
+    std::vector<int> row_list(n);
+    std::vector<int32_t> offs;
+    for (int i = 0; i < n; ++i) {
+        offs.clear();
+        for (int p = A.row_ptr[i]; p < A.row_ptr[i + 1]; ++p) offs.push_back(A.col[p] - i);
+        std::sort(offs.begin(), offs.end());
+        auto it = list_id.find(offs);
+        if (it == list_id.end()) {
+            // a mesh with this many distinct rows will not fit the set table
+            if (lists.size() >= kMaxOffsetLists) return false;
+            it = list_id.emplace(offs, (int)lists.size()).first;
+            lists.push_back(offs);
+        }
+        row_list[i] = it->second;
+    }
+    // sets: longest lists first, each list into the first set containing it
+    std::vector<int> order(lists.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lists[a].size() > lists[b].size(); });
+    std::vector<std::vector<int32_t>> sets;
+    std::vector<int> list_set(lists.size(), -1);
+    for (int l : order) {
+        for (size_t q = 0; q < sets.size() && list_set[l] < 0; ++q)
+            if (std::includes(sets[q].begin(), sets[q].end(), lists[l].begin(), lists[l].end())) list_set[l] = (int)q;
+        if (list_set[l] < 0) {
+            if ((int)sets.size() >= max_sets) return false;
+            list_set[l] = (int)sets.size();
+            sets.push_back(lists[l]);
+        }
+    }
+    out.width = W;
+    out.n_sets = (int)sets.size();
+    out.offsets.assign((size_t)out.n_sets * W, 0);
+    out.max_abs = 0;
+    for (int q = 0; q < out.n_sets; ++q)
+        for (size_t k = 0; k < sets[q].size(); ++k) {
+            out.offsets[(size_t)q * W + k] = sets[q][k];
+            out.max_abs = std::max(out.max_abs, std::abs(sets[q][k]));
+        }
+    out.set.assign(n, 0);
+    out.m.assign(ell_size(n, W), 0.0);
+    out.k.assign(ell_size(n, W), 0.0);
+    for (int i = 0; i < n; ++i) {
+        const int q = list_set[row_list[i]];
+        out.set[i] = (uint8_t)q;
+        const std::vector<int32_t>& so = sets[q];
+        for (int p = A.row_ptr[i]; p < A.row_ptr[i + 1]; ++p) {
+            const int32_t o = A.col[p] - i;
+            const int k = (int)(std::lower_bound(so.begin(), so.end(), o) - so.begin());
+            out.m[ell_index(i, k, W)] = A.mass[p];
+            out.k[ell_index(i, k, W)] = A.stiff[p];
+        }
+    }
+    return true;
 }
 
 int slab_mesh(int dim, int ex, int ey, int ez, double h, int* n_nodes, int* n_elems,

@@ -120,18 +120,31 @@ struct StencilRows {
     __device__ __forceinline__ int diag_count() const { return n_patterns; }
 };
 
-// ELL: [slot][n] column-major indices and values in global memory.
+// ELL: values (and column indices) in global memory in 64-row slices, each
+// slice holding its w slots one after another ([n/64][w][64]): a wavefront's
+// rows read one contiguous block, slot by slot.  Either column indices (cols)
+// or, with offset sets (sets != null), one set id per row: slot k of row i is
+// column i + set_off[sets[i] * w + k], gathered vectors then carrying guard
+// zones as wide as the largest offset.
 struct EllRows {
     static constexpr bool kStaged = false;
     static constexpr int kPreW = 1;      // gathers are loaded where they are used
     const int32_t* __restrict__ cols;
     int w;
     int n;
-    struct Row { int i; };
+    const uint8_t* __restrict__ sets = nullptr;
+    const int32_t* __restrict__ set_off = nullptr;
+    struct Row { int i; const int32_t* off; };
     __device__ __forceinline__ int width() const { return w; }
-    __device__ __forceinline__ Row row(int i) const { return Row{i}; }
-    __device__ __forceinline__ int col(const Row& r, int k) const { return cols[(size_t)k * n + r.i]; }
-    __device__ __forceinline__ int vidx(const Row& r, int k) const { return k * n + r.i; }
+    __device__ __forceinline__ Row row(int i) const
+    {
+        return Row{i, sets ? set_off + (int)sets[i] * w : nullptr};
+    }
+    __device__ __forceinline__ int col(const Row& r, int k) const
+    {
+        return r.off ? r.i + r.off[k] : cols[vidx(r, k)];
+    }
+    __device__ __forceinline__ int vidx(const Row& r, int k) const { return ((r.i >> 6) * w + k) * 64 + (r.i & 63); }
     __device__ __forceinline__ int didx(const Row& r) const { return r.i; }
     __device__ __forceinline__ int didx_of(int j) const { return j; }
     __device__ __forceinline__ int value_count() const { return 0; }

@@ -52,9 +52,13 @@ enum { CMX_FORWARD_EULER = 0, CMX_RUSH_LARSEN = 1, CMX_BACKWARD_EULER = 2, CMX_G
  * voltage, FP64 dot reductions and FP64 residual refinement. */
 enum { CMX_FP64 = 0, CMX_MIXED = 1 };
 
-/* Sparse format for M, K, A (chosen at create time; CMX_FMT_AUTO picks the
- * stencil table when the mesh has few distinct row patterns). */
-enum { CMX_FMT_AUTO = 0, CMX_FMT_STENCIL = 1, CMX_FMT_ELL = 2 };
+/* Sparse format for M, K, A, chosen at create time.  CMX_FMT_STENCIL: one
+ * set of column offsets for all rows and per-pattern values (few distinct
+ * rows).  CMX_FMT_ELL_OFFSETS: ELL values per row with one byte per row naming
+ * its set of column offsets (at most 255 sets; no column indices).
+ * CMX_FMT_ELL: ELL with a column index per entry.  CMX_FMT_AUTO takes the first
+ * of those three that applies. */
+enum { CMX_FMT_AUTO = 0, CMX_FMT_STENCIL = 1, CMX_FMT_ELL = 2, CMX_FMT_ELL_OFFSETS = 3 };
 
 /* Mesh: P1 simplices (replaces AbstractTetrahedralMesh + the FE assemblers,
  * pde/src/solver/AbstractFeVolumeIntegralAssembler.hpp:257-459). */
@@ -137,8 +141,9 @@ typedef struct {
     int64_t cg_iterations;     /* total CG iterations */
     int64_t refine_sweeps;     /* mixed: FP64 residual sweeps */
     int32_t last_iterations;   /* CG iterations of the last step */
-    int32_t format;            /* CMX_FMT_STENCIL or CMX_FMT_ELL in use */
-    int32_t n_nodes, n_states, row_width, n_patterns;
+    int32_t format;            /* CMX_FMT_STENCIL, CMX_FMT_ELL or CMX_FMT_ELL_OFFSETS in use */
+    int32_t n_nodes, n_states, row_width, n_patterns;   /* n_patterns: value patterns (stencil) or
+                                                               offset sets (ELL_OFFSETS) */
     int64_t nnz;
     double last_residual;      /* ||D^-1 r|| at the end of the last solve */
     double device_bytes;       /* resident device memory (bytes) */

@@ -583,6 +583,47 @@ def test_ell_matches_stencil():
     assert np.abs(Va - Vb).max() / np.abs(Va).max() <= 1e-11
 
 
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+def test_offset_ell_matches_ell(precision):
+    """ELL with per-row offset sets (one byte per row, no column indices)
+    against ELL with a column index per entry, on a mesh the stencil format
+    cannot take (per-element conductivities from rotated fibres) and on a
+    two-shell ellipsoid union: the same entries in the same order, padding
+    slots holding zeros, so the runs agree bitwise; the sets are few."""
+    from chaste_amd import cmx
+    from chaste_amd.shells import disjoint_union, ellipsoid_shell
+    print('case offset ell', precision, flush=True)
+    coords, elems = cmx.slab_mesh(3, 16, 8, 6, 0.01)
+    rng = np.random.default_rng(3)
+    ang = rng.uniform(0, np.pi, elems.shape[0])
+    fib = np.stack([np.cos(ang), np.sin(ang), np.zeros_like(ang)], axis=1)
+    sig_slab = cmx.conductivity_tensors(fib, (1.75, 0.19), axisymmetric=True)
+    shells = [ellipsoid_shell(h) for h in (0.3, 0.25)]
+    sc, se, sf, sa, _ = disjoint_union(shells)
+    sig_shell = cmx.conductivity_tensors(sf, (1.75, 0.19), axisymmetric=True)
+    cases = [(coords, elems, sig_slab, np.where(coords[:, 0] < 0.03, 0, -1)),
+             (sc, se, sig_shell, np.where(sa, 0, -1))]
+    for co, el, sig, stim in cases:
+        runs = {}
+        for fmt in (cmx.FMT_ELL, cmx.FMT_ELL_OFFSETS, cmx.FMT_AUTO):
+            m = cmx.Monodomain(co, el, np.array([1.75, 0.19, 0.19]), cmx.TT06_EPI, cmx.RUSH_LARSEN,
+                               stimuli=[(-100000.0, 0.0, 1.0)], stim_index=stim.astype(np.int32), precision=precision,
+                               ksp_rtol=1e-10, refine_sweeps=1 if precision else 0, sigma_elem=sig, fmt=fmt,
+                               use_graph=True)
+            st = m.stats()
+            m.step(0.0, 150)
+            runs[fmt] = (m.voltage(), m.state(), st)
+            m.close()
+        assert runs[cmx.FMT_ELL][2]['format'] == cmx.FMT_ELL
+        assert runs[cmx.FMT_ELL_OFFSETS][2]['format'] == cmx.FMT_ELL_OFFSETS
+        assert runs[cmx.FMT_AUTO][2]['format'] == cmx.FMT_ELL_OFFSETS
+        print('offset sets', runs[cmx.FMT_ELL_OFFSETS][2]['n_patterns'], 'nodes', co.shape[0])
+        assert 1 <= runs[cmx.FMT_ELL_OFFSETS][2]['n_patterns'] <= 255
+        for fmt in (cmx.FMT_ELL_OFFSETS, cmx.FMT_AUTO):
+            assert np.array_equal(runs[fmt][0], runs[cmx.FMT_ELL][0])
+            assert np.array_equal(runs[fmt][1], runs[cmx.FMT_ELL][1])
+
+
 def test_tt06_mixed_precision(oracle):
     """FP32-bulk path vs the FP64 oracle: 0.5 mV / 0.1 ms."""
     from chaste_amd import cmx
