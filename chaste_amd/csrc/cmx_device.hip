@@ -959,7 +959,7 @@ struct IterRow {
 };
 
 template <class Rows, class RV, int ROWS = (sizeof(RV) == 4 ? 2 : 1)>
-__global__ __launch_bounds__(kCgBlock) void cg_iter_kernel(Rows R, MatVals mv, CgArgs<RV> a, int it)
+__global__ __launch_bounds__(kCgBlock, Rows::kIterWaves) void cg_iter_kernel(Rows R, MatVals mv, CgArgs<RV> a, int it)
 {
     __shared__ RV sA[kMaxPatterns * kMaxStencil], sD[kMaxPatterns];
     const TableLoad<Rows, RV> ta(a_vals<RV>(mv), R.value_count()), td(dinv_vals<RV>(mv), R.diag_count());
@@ -1784,6 +1784,47 @@ int cg_rows_of(const cmx_ctx* c)
     return c->cg_rows > 0 ? c->cg_rows : (sizeof(RV) == 4 ? 2 : 1);
 }
 
+// Resident blocks per CU of the CG iteration kernel the context will launch.
+template <class Rows, class RV>
+int iter_blocks_per_cu(const cmx_ctx* c)
+{
+    int b = 0;
+    const hipError_t e = cg_rows_of<RV>(c) == 2
+                             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, cg_iter_kernel<Rows, RV, 2>, kCgBlock, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, cg_iter_kernel<Rows, RV, 1>, kCgBlock, 0);
+    return e == hipSuccess ? std::max(b, 1) : 1;
+}
+
+template <class RV>
+int iter_blocks_per_cu_rv(const cmx_ctx* c)
+{
+    if (c->format != CMX_FMT_STENCIL) return iter_blocks_per_cu<EllRows, RV>(c);
+    if (c->width == 4) return iter_blocks_per_cu<StencilRows<4>, RV>(c);
+    if (c->width == 8) return iter_blocks_per_cu<StencilRows<8>, RV>(c);
+    return iter_blocks_per_cu<StencilRows<16>, RV>(c);
+}
+
+// Default solver blocks: one per CU, or on meshes with at least 8 rows per
+// thread at that count, as many as the CG iteration kernel keeps resident
+// (each thread walks its rows one after another, so the resident wavefronts
+// bound the loads in flight; the FP32 ELL iteration on a 13M-row shell union
+// runs 1.5x faster at two blocks per CU).  cmx_autotune and
+// cmx_set_launch_config override it.
+int default_solver_blocks(const cmx_ctx* c)
+{
+    const int cg_rows = (c->n + kCgBlock - 1) / kCgBlock;
+    int G = std::min(cg_rows, kCgGrid);
+    if ((long)c->n >= 8L * kCgGrid * kCgBlock) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            return G;
+        const int per_cu = c->precision == CMX_MIXED ? iter_blocks_per_cu_rv<float>(c) : iter_blocks_per_cu_rv<double>(c);
+        G = std::min({cg_rows, per_cu * cus, kMaxBlocks});
+    }
+    return G;
+}
+
 // Start of a solve (w0 = A u0 and its dot products; ends a solve whose
 // residual is already below the threshold).
 template <class Rows, class RV>
@@ -2395,8 +2436,6 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
     c->nnz = csr.row_ptr[c->n];
     c->width = csr.max_row;
     const int n = c->n;
-    const int cg_rows = (n + kCgBlock - 1) / kCgBlock;
-    c->G = std::min(cg_rows, kCgGrid);   // cmx_autotune may pick another
 
     CMX_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
@@ -2508,6 +2547,7 @@ int create_impl(const cmx_mesh* mesh, const cmx_tissue* tissue, const cmx_opts* 
         if ((rc = upload(c, &c->dinv64, dinv64.data(), n))) return rc;
         if ((rc = upload(c, &c->dinv32, dinv32.data(), n))) return rc;
     }
+    c->G = default_solver_blocks(c);
 
     // Cells: default initial state (AbstractCardiacCell::Init ->
     // ResetToInitialConditions), V from the model's initial voltage.
