@@ -13,7 +13,8 @@ HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
              '-I' + os.path.join(ROOT, 'include'), '-I' + CSRC]
 
-SOURCES = ['cmx_device.hip', 'cmx_adjoint.hip', 'cmx_cells.hip', 'cmx_host.cpp', 'cmx_io.cpp']
+SOURCES = ['cmx_device.hip', 'cmx_adjoint.hip', 'cmx_cells.hip', 'cmx_host.cpp', 'cmx_io.cpp',
+           'cmx_model_lr91.hip', 'cmx_model_tt06.hip', 'cmx_model_ord.hip']
 
 
 CPP_TESTS = ['TestMonodomainProblem']

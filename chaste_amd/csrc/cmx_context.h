@@ -20,71 +20,58 @@ namespace cmx {
 // ---------------------------------------------------------------------------
 // Cell models
 // ---------------------------------------------------------------------------
-struct ModelLr91 {
-    static constexpr int NS = lr91::kNumStates;
-    static constexpr int NP = lr91::kNumParams;
-    template <class R, bool RL>
-    static __device__ __forceinline__ void eval(const R* y, R V, R st, const R* p, R cm, R* dy, R* gi, R* gk)
-    {
-        lr91::evaluate<R, RL>(y, V, st, p, cm, dy, gi, gk);
-    }
-    template <class R>
-    static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return lr91::iionic<R>(y, V, p, cm); }
-    template <class R>
-    static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return lr91::dvdt<R>(y, V, st, p, cm); }
-    template <class R>
-    static __device__ __forceinline__ int be_step(R* y, R V, R st, const R* p, R cm, R dt)
-    {
-        return lr91::be_step<R>(y, V, st, p, cm, dt);
-    }
-    template <class R, int ORDER>
-    static __device__ __forceinline__ void grl_step(R* y, R st, const R* p, R cm, R dt, bool update_v)
-    {
-        lr91::grl_step<R, ORDER>(y, st, p, cm, dt, update_v);
-    }
-    static constexpr bool gate(int i) { return lr91::kIsGate[i] != 0; }
-};
+// A kernel-side model: the generated (or hand-written) functions of namespace
+// ns behind one interface.  NS states (state 0 is V), NP parameters; eval
+// gives the right-hand side, and with RL the Rush-Larsen steady states and
+// rates of the gate states (gate(i)); be_step / grl_step (generated models
+// only) are the backward-Euler and generalized Rush-Larsen substeps.
+#define CMX_MODEL_COMMON(ns)                                                                                      \
+    static constexpr int NS = ns::kNumStates;                                                                     \
+    static constexpr int NP = ns::kNumParams;                                                                     \
+    template <class R, bool RL>                                                                                   \
+    static CMX_HD __forceinline__ void eval(const R* y, R V, R st, const R* p, R cm, R* dy, R* gi, R* gk)         \
+    {                                                                                                             \
+        ns::evaluate<R, RL>(y, V, st, p, cm, dy, gi, gk);                                                         \
+    }                                                                                                             \
+    template <class R>                                                                                            \
+    static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return ns::iionic<R>(y, V, p, cm); } \
+    template <class R>                                                                                            \
+    static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm)                             \
+    {                                                                                                             \
+        return ns::dvdt<R>(y, V, st, p, cm);                                                                      \
+    }                                                                                                             \
+    static constexpr bool gate(int i) { return ns::kIsGate[i] != 0; }                                            \
+    static const double* initial_state() { return ns::kInitialState; }                                           \
+    static const double* param_defaults() { return ns::kParamDefaults; }                                         \
+    static const char* const* state_names() { return ns::kStateNames; }                                          \
+    static const char* const* param_names() { return ns::kParamNames; }
 
-struct ModelTt06 {
-    static constexpr int NS = tt06_epi::kNumStates;
-    static constexpr int NP = tt06_epi::kNumParams;
-    template <class R, bool RL>
-    static __device__ __forceinline__ void eval(const R* y, R V, R st, const R* p, R cm, R* dy, R* gi, R* gk)
-    {
-        tt06_epi::evaluate<R, RL>(y, V, st, p, cm, dy, gi, gk);
+#define CMX_MODEL_IMPLICIT(ns)                                                                                    \
+    template <class R>                                                                                            \
+    static __device__ __forceinline__ int be_step(R* y, R V, R st, const R* p, R cm, R dt)                        \
+    {                                                                                                             \
+        return ns::be_step<R>(y, V, st, p, cm, dt);                                                               \
+    }                                                                                                             \
+    template <class R, int ORDER>                                                                                 \
+    static __device__ __forceinline__ void grl_step(R* y, R st, const R* p, R cm, R dt, bool update_v)            \
+    {                                                                                                             \
+        ns::grl_step<R, ORDER>(y, st, p, cm, dt, update_v);                                                       \
     }
-    template <class R>
-    static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return tt06_epi::iionic<R>(y, V, p, cm); }
-    template <class R>
-    static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return tt06_epi::dvdt<R>(y, V, st, p, cm); }
-    template <class R>
-    static __device__ __forceinline__ int be_step(R* y, R V, R st, const R* p, R cm, R dt)
-    {
-        return tt06_epi::be_step<R>(y, V, st, p, cm, dt);
-    }
-    template <class R, int ORDER>
-    static __device__ __forceinline__ void grl_step(R* y, R st, const R* p, R cm, R dt, bool update_v)
-    {
-        tt06_epi::grl_step<R, ORDER>(y, st, p, cm, dt, update_v);
-    }
-    static constexpr bool gate(int i) { return tt06_epi::kIsGate[i] != 0; }
-};
+
+// A generated model (all four integrators).
+#define CMX_GENERATED_MODEL(Name, ns) \
+    struct Name {                     \
+        CMX_MODEL_COMMON(ns)          \
+        CMX_MODEL_IMPLICIT(ns)        \
+    };
+
+CMX_GENERATED_MODEL(ModelLr91, lr91)
+CMX_GENERATED_MODEL(ModelTt06, tt06_epi)
 
 // O'Hara-Rudy 2011 endocardial cells: forward Euler and Rush-Larsen only (no
 // backward-Euler or generalized Rush-Larsen forms).
 struct ModelOrd {
-    static constexpr int NS = ord_endo::kNumStates;
-    static constexpr int NP = ord_endo::kNumParams;
-    template <class R, bool RL>
-    static __device__ __forceinline__ void eval(const R* y, R V, R st, const R* p, R cm, R* dy, R* gi, R* gk)
-    {
-        ord_endo::evaluate<R, RL>(y, V, st, p, cm, dy, gi, gk);
-    }
-    template <class R>
-    static __device__ __forceinline__ R iion(const R* y, R V, const R* p, R cm) { return ord_endo::iionic<R>(y, V, p, cm); }
-    template <class R>
-    static __device__ __forceinline__ R dvdt(const R* y, R V, R st, const R* p, R cm) { return ord_endo::dvdt<R>(y, V, st, p, cm); }
-    static constexpr bool gate(int i) { return ord_endo::kIsGate[i] != 0; }
+    CMX_MODEL_COMMON(ord_endo)
 };
 
 // Gate columns of a model's lookup table: gate state s (Model::gate(s)) has
@@ -109,18 +96,11 @@ struct GateCols {
 
 // Rush-Larsen coefficients of every gate at voltage V (host, FP64): the
 // model's right-hand side at its initial state with parameters p.
-inline bool gate_coefficients(int model, double V, const double* p, double cm, double* gi, double* gk)
+template <class Model>
+void gate_coefficients(double V, const double* p, double cm, double* gi, double* gk)
 {
     double dy[64] = {0.0};
-    if (model == CMX_MODEL_LR91)
-        lr91::evaluate<double, true>(lr91::kInitialState, V, 0.0, p, cm, dy, gi, gk);
-    else if (model == CMX_MODEL_TT06_EPI)
-        tt06_epi::evaluate<double, true>(tt06_epi::kInitialState, V, 0.0, p, cm, dy, gi, gk);
-    else if (model == CMX_MODEL_ORD_ENDO)
-        ord_endo::evaluate<double, true>(ord_endo::kInitialState, V, 0.0, p, cm, dy, gi, gk);
-    else
-        return false;
-    return true;
+    Model::template eval<double, true>(Model::initial_state(), V, 0.0, p, cm, dy, gi, gk);
 }
 
 // Gate lookup tables (mixed-precision Rush-Larsen cells): for
@@ -132,9 +112,10 @@ inline bool gate_coefficients(int model, double V, const double* p, double cm, d
 constexpr double kLutVmin = -250.0, kLutVmax = 350.0, kLutDv = 0.02;
 
 template <class Model>
-const char* fill_gate_table(int model, double dt, double cm, const double* pdef, int P, std::vector<float>& table,
-                            int* rows_out)
+const char* fill_gate_table(double dt, double cm, std::vector<float>& table, int* rows_out)
 {
+    const double* pdef = Model::param_defaults();
+    constexpr int P = Model::NP;
     constexpr int W = GateCols<Model>::width();
     const int rows = (int)((kLutVmax - kLutVmin) / kLutDv + 0.5) + 1;
     table.assign((size_t)rows * W, 0.0f);
@@ -143,9 +124,9 @@ const char* fill_gate_table(int model, double dt, double cm, const double* pdef,
     double gi[64], gk[64], gi2[64], gk2[64];
     for (int r = 0; r < rows; ++r) {
         const double V = kLutVmin + r * kLutDv;
-        gate_coefficients(model, V, p.data(), cm, gi, gk);
+        gate_coefficients<Model>(V, p.data(), cm, gi, gk);
         if (r % 500 == 0) {
-            gate_coefficients(model, V, p2.data(), cm, gi2, gk2);
+            gate_coefficients<Model>(V, p2.data(), cm, gi2, gk2);
             for (int s = 1; s < Model::NS; ++s)
                 if (Model::gate(s) && (gi2[s] != gi[s] || gk2[s] != gk[s]))
                     return "lookup tables: this model's gate coefficients depend on its parameters";
@@ -195,42 +176,6 @@ __device__ __forceinline__ bool lut_gates(const float* __restrict__ lut, float v
         gdec[s] = lo[c + 1] + f * (hi[c + 1] - lo[c + 1]);
     }
     return inside;
-}
-
-// Metadata of a model id (CMX_MODEL_*); false for an unknown id.
-struct ModelMeta {
-    int S, P;
-    const double* init;
-    const double* pdef;
-    const char* const* state_names;
-    const char* const* param_names;
-    bool implicit;       // backward-Euler and generalized Rush-Larsen forms available
-};
-inline bool model_meta(int model, ModelMeta* m)
-{
-    if (model == CMX_MODEL_LR91)
-        *m = {lr91::kNumStates, lr91::kNumParams, lr91::kInitialState, lr91::kParamDefaults, lr91::kStateNames,
-              lr91::kParamNames, true};
-    else if (model == CMX_MODEL_TT06_EPI)
-        *m = {tt06_epi::kNumStates, tt06_epi::kNumParams, tt06_epi::kInitialState, tt06_epi::kParamDefaults,
-              tt06_epi::kStateNames, tt06_epi::kParamNames, true};
-    else if (model == CMX_MODEL_ORD_ENDO)
-        *m = {ord_endo::kNumStates, ord_endo::kNumParams, ord_endo::kInitialState, ord_endo::kParamDefaults,
-              ord_endo::kStateNames, ord_endo::kParamNames, false};
-    else
-        return false;
-    return true;
-}
-
-// The gate table of a model id for ODE steps of length dt (see
-// fill_gate_table); returns an error message, or null.
-inline const char* gate_table(int model, double dt, double cm, std::vector<float>& table, int* rows)
-{
-    ModelMeta mm;
-    if (!model_meta(model, &mm)) return "unknown cell model";
-    if (model == CMX_MODEL_LR91) return fill_gate_table<ModelLr91>(model, dt, cm, mm.pdef, mm.P, table, rows);
-    if (model == CMX_MODEL_ORD_ENDO) return fill_gate_table<ModelOrd>(model, dt, cm, mm.pdef, mm.P, table, rows);
-    return fill_gate_table<ModelTt06>(model, dt, cm, mm.pdef, mm.P, table, rows);
 }
 
 }  // namespace cmx
