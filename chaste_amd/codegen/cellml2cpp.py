@@ -956,6 +956,13 @@ class Generator:
         w('CMX_HD inline int be_step(R* y, R V, R i_stim_area, const R* p, R cm, R dt)')
         w('{')
         w('    be_linear<R>(y, V, p, cm, dt);')
+        if N == 0:
+            # every state is linear in itself: the closed form is the step
+            w('    (void)i_stim_area;')
+            w('    return 0;')
+            w('}')
+            w('')
+            return
         w('    R g[kNumNonlinear];')
         w('CMX_UNROLL')
         w('    for (int i = 0; i < kNumNonlinear; ++i) g[i] = y[kNonlinear[i]];')

@@ -22,6 +22,14 @@ DEFAULT_CELLML = '/root/reference/heart/src/odes/cellml'
 MODELS = {
     'lr91': ('LuoRudy1991.cellml',
              ['fast_sodium_current.g_Na', 'time_dependent_potassium_current.g_Kmax', 'slow_inward_current.P_si']),
+    'hh52': ('HodgkinHuxley1952.cellml',
+             ['sodium_channel.g_Na', 'potassium_channel.g_K', 'leakage_current.g_L']),
+    'fox2002': ('FoxModel2002.cellml',
+                ['fast_sodium_current.g_Na', 'rapid_activating_delayed_rectifiyer_K_current.g_Kr',
+                 'slow_activating_delayed_rectifiyer_K_current.g_Ks', 'time_independent_potassium_current.g_K1',
+                 'L_type_Ca_current.P_Ca']),
+    'mahajan2008': ('Mahajan2008.cellml',
+                    ['INa.gna', 'ICaL.gca', 'IKr.ScaleFactorGkr', 'IKs.ScaleFactorGks', 'Ito.ScaleFactorIto']),
     'tt06_epi': ('TenTusscher2006Epi.cellml',
                  ['fast_sodium_current.g_Na', 'L_type_Ca_current.g_CaL', 'rapid_time_dependent_potassium_current.g_Kr',
                   'slow_time_dependent_potassium_current.g_Ks', 'inward_rectifier_potassium_current.g_K1',

@@ -11,7 +11,10 @@
 
 #include "cmx_internal.h"
 #include "cmx_kernels.h"
+#include "models/fox2002.h"
+#include "models/hh52.h"
 #include "models/lr91.h"
+#include "models/mahajan2008.h"
 #include "models/ord_endo.h"
 #include "models/tt06_epi.h"
 
@@ -67,6 +70,9 @@ namespace cmx {
 
 CMX_GENERATED_MODEL(ModelLr91, lr91)
 CMX_GENERATED_MODEL(ModelTt06, tt06_epi)
+CMX_GENERATED_MODEL(ModelHh52, hh52)
+CMX_GENERATED_MODEL(ModelFox2002, fox2002)
+CMX_GENERATED_MODEL(ModelMahajan2008, mahajan2008)
 
 // O'Hara-Rudy 2011 endocardial cells: forward Euler and Rush-Larsen only (no
 // backward-Euler or generalized Rush-Larsen forms).

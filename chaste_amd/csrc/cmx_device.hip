@@ -855,6 +855,9 @@ hipEvent_t take_event(cmx_ctx* c)
 const ModelOps* model_ops_lr91();
 const ModelOps* model_ops_tt06();
 const ModelOps* model_ops_ord();
+const ModelOps* model_ops_hh52();
+const ModelOps* model_ops_fox2002();
+const ModelOps* model_ops_mahajan2008();
 
 const ModelOps* model_ops(int model)
 {
@@ -862,6 +865,9 @@ const ModelOps* model_ops(int model)
     case CMX_MODEL_LR91: return model_ops_lr91();
     case CMX_MODEL_TT06_EPI: return model_ops_tt06();
     case CMX_MODEL_ORD_ENDO: return model_ops_ord();
+    case CMX_MODEL_HH52: return model_ops_hh52();
+    case CMX_MODEL_FOX2002: return model_ops_fox2002();
+    case CMX_MODEL_MAHAJAN2008: return model_ops_mahajan2008();
     default: return nullptr;
     }
 }

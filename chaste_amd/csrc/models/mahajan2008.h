@@ -1,0 +1,1235 @@
+// Generated by chaste_amd/codegen/cellml2cpp.py from Mahajan2008.cellml -- do not edit.
+// Model: mahajan_shiferaw_model_2008
+#pragma once
+#include "model_common.h"
+
+namespace cmx { namespace mahajan2008 {
+
+constexpr int kNumStates = 26;
+constexpr int kVIndex = 0;
+constexpr int kNumParams = 5;
+constexpr bool kPerCapacitanceUnits = false;
+
+// State variables (CellML component__variable), V first then document order.
+static const char* const kStateNames[kNumStates] = {
+    "cell__V",
+    "INa__xm",
+    "INa__xh",
+    "INa__xj",
+    "ICaL__c1",
+    "ICaL__c2",
+    "ICaL__xi1ca",
+    "ICaL__xi1ba",
+    "ICaL__xi2ca",
+    "ICaL__xi2ba",
+    "IKr__xr",
+    "IKs__xs1",
+    "IKs__xs2",
+    "Ito__xtos",
+    "Ito__ytos",
+    "Ito__xtof",
+    "Ito__ytof",
+    "Irel__Ca_JSR",
+    "Irel__xir",
+    "Na__Na_i",
+    "Ca__Ca_dyad",
+    "Ca__Ca_submem",
+    "Ca__Ca_i",
+    "Ca__Ca_NSR",
+    "Ca__tropi",
+    "Ca__trops",
+};
+static constexpr double kInitialState[kNumStates] = {
+    -87.169816169406,  // cell__V
+    0.001075453357,  // INa__xm
+    0.990691306716,  // INa__xh
+    0.993888937283,  // INa__xj
+    1.8211252e-05,  // ICaL__c1
+    0.979322592773,  // ICaL__c2
+    0.001208153482,  // ICaL__xi1ca
+    3.3616596e-05,  // ICaL__xi1ba
+    0.004173008466,  // ICaL__xi2ca
+    0.015242594688,  // ICaL__xi2ba
+    0.007074239331,  // IKr__xr
+    0.048267587131,  // IKs__xs1
+    0.105468807033,  // IKs__xs2
+    0.00364776906,  // Ito__xtos
+    0.174403618112,  // Ito__ytos
+    0.003643592594,  // Ito__xtof
+    0.993331326442,  // Ito__ytof
+    97.505463697266,  // Irel__Ca_JSR
+    0.006679257264,  // Irel__xir
+    11.441712311614,  // Na__Na_i
+    1.716573130685,  // Ca__Ca_dyad
+    0.226941113355,  // Ca__Ca_submem
+    0.256752008084,  // Ca__Ca_i
+    104.450004990523,  // Ca__Ca_NSR
+    22.171689894953,  // Ca__tropi
+    19.864701949854,  // Ca__trops
+};
+static const char* const kParamNames[5] = {
+    "INa__gna",
+    "ICaL__gca",
+    "IKr__ScaleFactorGkr",
+    "IKs__ScaleFactorGks",
+    "Ito__ScaleFactorIto",
+};
+static constexpr double kParamDefaults[5] = {
+    12.0,
+    182.0,
+    1.0,
+    1.0,
+    1.0,
+};
+// 1 = integrated by Rush-Larsen (affine gate, V-dependent coefficients only)
+static constexpr int kIsGate[kNumStates] = {0, 1, 1, 1, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+// Right-hand side with V clamped.  Non-gates (and, with RL=false, every
+// state) get dy[i]; with RL=true gate i gets g_inf[i], g_k[i] such that
+// y_i(t+dt) = g_inf[i] + (y_i - g_inf[i]) * exp(g_k[i] * dt).
+// i_stim_area is in uA/cm^2; cm is the tissue capacitance in uF/cm^2.
+template <class R, bool RL>
+CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
+                            R* dy, R* g_inf, R* g_k)
+{
+    CMX_MATH_USING
+    (void)p; (void)cm; (void)g_inf; (void)g_k; (void)i_stim_area;
+    const R Environment__R = R(8.314472);
+    const R Environment__T = R(308.0);
+    const R Environment__F = R(96.4853415);
+    const R Environment__K_o = R(5.4);
+    const R Environment__Ca_o = R(1.8);
+    const R Environment__Na_o = R(136.0);
+    const R cell__wca = R(8.0);
+    const R ICaL__pca = R(0.00054);
+    const R ICaL__vth = R(0.0);
+    const R ICaL__s6 = R(8.0);
+    const R ICaL__vx = R(-40.0);
+    const R ICaL__sx = R(3.0);
+    const R ICaL__vy = R(-40.0);
+    const R ICaL__sy = R(4.0);
+    const R ICaL__vyr = R(-40.0);
+    const R ICaL__syr = R(11.32);
+    const R ICaL__cat = R(3.0);
+    const R ICaL__cpt = R(6.09365);
+    const R ICaL__k2 = R(0.000103615);
+    const R ICaL__k1t = R(0.00413);
+    const R ICaL__k2t = R(0.00224);
+    const R ICaL__r1 = R(0.3);
+    const R ICaL__r2 = R(3.0);
+    const R ICaL__s1t = R(0.00195);
+    const R ICaL__tca = R(78.0329);
+    const R ICaL__taupo = R(1.0);
+    const R ICaL__tau3 = R(3.0);
+    const R INaK__gNaK = R(1.5);
+    const R INaK__xkmko = R(1.5);
+    const R INaK__xkmnai = R(12.0);
+    const R INaCa__gNaCa = R(0.84);
+    const R INaCa__xkdna = R(0.3);
+    const R INaCa__xmcao = R(1.3);
+    const R INaCa__xmnao = R(87.5);
+    const R INaCa__xmnai = R(12.3);
+    const R INaCa__xmcai = R(0.0036);
+    const R Irel__cstar = R(90.0);
+    const R Irel__gryr = R(2.58079);
+    const R Irel__gbarsr = R(26841.8);
+    const R Irel__gdyad = R(9000.0);
+    const R Irel__ax = R(0.3576);
+    const R Irel__ay = R(0.05);
+    const R Irel__av = R(11.3);
+    const R Irel__taua = R(100.0);
+    const R Irel__taur = R(30.0);
+    const R Ileak_Iup_Ixfer__cup = R(0.5);
+    const R Ileak_Iup_Ixfer__kj = R(50.0);
+    const R Ileak_Iup_Ixfer__vup = R(0.4);
+    const R Ileak_Iup_Ixfer__gleak = R(2.069e-05);
+    const R Ca__bcal = R(24.0);
+    const R Ca__xkcal = R(7.0);
+    const R Ca__srmax = R(47.0);
+    const R Ca__srkd = R(0.6);
+    const R Ca__bmem = R(15.0);
+    const R Ca__kmem = R(0.3);
+    const R Ca__bsar = R(42.0);
+    const R Ca__ksar = R(13.0);
+    const R Ca__xkon = R(0.0327);
+    const R Ca__xkoff = R(0.0196);
+    const R Ca__btrop = R(70.0);
+    const R Ca__taud = R(4.0);
+    const R Ca__taups = R(0.5);
+    const R i_stim = i_stim_area;
+    (void)i_stim;
+    const R Ileak_Iup_Ixfer__jleak = (cmx_div((Ileak_Iup_Ixfer__gleak * y[23] * y[23]), ((y[23] * y[23]) + (Ileak_Iup_Ixfer__kj * Ileak_Iup_Ixfer__kj))) * ((y[23] * R(16.667)) - y[22]));
+    const R Ileak_Iup_Ixfer__jup = cmx_div((Ileak_Iup_Ixfer__vup * y[22] * y[22]), ((y[22] * y[22]) + (Ileak_Iup_Ixfer__cup * Ileak_Iup_Ixfer__cup)));
+    const R Ca__dCa_JSR = (((-y[18]) + Ileak_Iup_Ixfer__jup) - Ileak_Iup_Ixfer__jleak);
+    const R Ca__bpxi = cmx_div((Ca__bcal * Ca__xkcal), ((Ca__xkcal + y[22]) * (Ca__xkcal + y[22])));
+    const R Ca__mempxi = cmx_div((Ca__bmem * Ca__kmem), ((Ca__kmem + y[22]) * (Ca__kmem + y[22])));
+    const R Ca__sarpxi = cmx_div((Ca__bsar * Ca__ksar), ((Ca__ksar + y[22]) * (Ca__ksar + y[22])));
+    const R Ca__spxi = cmx_div((Ca__srmax * Ca__srkd), ((Ca__srkd + y[22]) * (Ca__srkd + y[22])));
+    const R Ca__dciib = cmx_div(R(1.0), (R(1.0) + Ca__bpxi + Ca__spxi + Ca__mempxi + Ca__sarpxi));
+    const R Ca__bpxs = cmx_div((Ca__bcal * Ca__xkcal), ((Ca__xkcal + y[21]) * (Ca__xkcal + y[21])));
+    const R Ca__mempxs = cmx_div((Ca__bmem * Ca__kmem), ((Ca__kmem + y[21]) * (Ca__kmem + y[21])));
+    const R Ca__sarpxs = cmx_div((Ca__bsar * Ca__ksar), ((Ca__ksar + y[21]) * (Ca__ksar + y[21])));
+    const R Ca__spxs = cmx_div((Ca__srmax * Ca__srkd), ((Ca__srkd + y[21]) * (Ca__srkd + y[21])));
+    const R Ca__dcsib = cmx_div(R(1.0), (R(1.0) + Ca__bpxs + Ca__spxs + Ca__mempxs + Ca__sarpxs));
+    const R Ca__jd = ((y[21] - y[22]) * (R(1) / Ca__taud));
+    const R Ca__xbi = ((Ca__xkon * y[22] * (Ca__btrop - y[24])) - (Ca__xkoff * y[24]));
+    const R Ca__xbs = ((Ca__xkon * y[21] * (Ca__btrop - y[25])) - (Ca__xkoff * y[25]));
+    const R ICaL__poinf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vth)) * (R(1) / ICaL__s6)), V, -0.18033688011112042, -0.0)));
+    const R ICaL__alpha = (ICaL__poinf * (R(1) / ICaL__taupo));
+    const R ICaL__beta = ((R(1.0) - ICaL__poinf) * (R(1) / ICaL__taupo));
+    const R ICaL__po = ((((((R(1.0) - y[6]) - y[8]) - y[7]) - y[9]) - y[4]) - y[5]);
+    const R Ca__csm = (y[21] * (R(1) / R(1000.0)));
+    const R Environment__FonRT = (Environment__F * (R(1) / (Environment__R * Environment__T)));
+    const R ICaL__za = (V * R(2.0) * Environment__FonRT);
+    const R ICaL__rxa = (((fabs(ICaL__za) < R(0.001))) ? (((R(4.0) * ICaL__pca * Environment__F * Environment__FonRT * ((Ca__csm * cmx_exp(ICaL__za)) - (R(0.341) * Environment__Ca_o))) * (R(1) / (R(2.0) * Environment__FonRT)))) : (cmx_div((R(4.0) * ICaL__pca * V * Environment__F * Environment__FonRT * ((Ca__csm * cmx_exp(ICaL__za)) - (R(0.341) * Environment__Ca_o))), cmx_expm1(ICaL__za))));
+    const R ICaL__jca = (p[1] * ICaL__po * ICaL__rxa);
+    const R ICaL__fca = cmx_div(R(1.0), (R(1.0) + cmx_ipow<3>(cmx_div(ICaL__cat, y[20]))));
+    const R ICaL__k1 = (R(0.024168) * ICaL__fca);
+    const R ICaL__poi = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vx)) * (R(1) / ICaL__sx)), V, -0.4808983469629878, -19.235933878519514)));
+    const R ICaL__k3 = ((R(1.0) - ICaL__poi) * (R(1) / ICaL__tau3));
+    const R ICaL__k3t = ICaL__k3;
+    const R ICaL__Ps = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vyr)) * (R(1) / ICaL__syr)), V, -0.12744655838241728, -5.09786233529669)));
+    const R ICaL__Pr = (R(1.0) - cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vy)) * (R(1) / ICaL__sy)), V, -0.36067376022224085, -14.426950408889635))));
+    const R ICaL__recov = (R(10.0) + (R(4954.0) * cmx_exp_v((V * (R(1) / R(15.6))), V, 0.09248045133903611, 0.0)));
+    const R ICaL__tau_ca = (cmx_div(ICaL__tca, (R(1.0) + cmx_ipow<4>((y[20] * (R(1) / ICaL__cpt))))) + R(0.1));
+    const R ICaL__tauca = (((ICaL__recov - ICaL__tau_ca) * ICaL__Pr) + ICaL__tau_ca);
+    const R ICaL__k5 = cmx_div((R(1.0) - ICaL__Ps), ICaL__tauca);
+    const R ICaL__k6 = cmx_div((ICaL__fca * ICaL__Ps), ICaL__tauca);
+    const R ICaL__k4 = cmx_div((((cmx_div((ICaL__k3 * ICaL__alpha), ICaL__beta) * ICaL__k1) * (R(1) / ICaL__k2)) * ICaL__k5), ICaL__k6);
+    const R ICaL__tauba = (((ICaL__recov - R(450.0)) * ICaL__Pr) + R(450.0));
+    const R ICaL__k5t = cmx_div((R(1.0) - ICaL__Ps), ICaL__tauba);
+    const R ICaL__k6t = cmx_div(ICaL__Ps, ICaL__tauba);
+    const R ICaL__k4t = cmx_div((((cmx_div((ICaL__k3t * ICaL__alpha), ICaL__beta) * ICaL__k1t) * (R(1) / ICaL__k2t)) * ICaL__k5t), ICaL__k6t);
+    const R ICaL__s1 = (R(0.0182688) * ICaL__fca);
+    const R ICaL__s2 = cmx_div((((ICaL__s1 * ICaL__r1) * (R(1) / ICaL__r2)) * ICaL__k2), ICaL__k1);
+    const R ICaL__s2t = ((((ICaL__s1t * ICaL__r1) * (R(1) / ICaL__r2)) * ICaL__k2t) * (R(1) / ICaL__k1t));
+    const R IKr__xkrv1 = cmx_remove_singularity(V, R(-7.0), R(8.130081300813008e-07), [&](const R V) { return (((fabs((V + R(7.0))) > R(0.001))) ? (cmx_div((R(0.00138) * R(1.0) * (V + R(7.0))), (-cmx_expm1(((-R(0.123)) * (V + R(7.0))))))) : ((R(0.00138) * (R(1) / R(0.123))))); });
+    const R IKr__xkrv2 = cmx_remove_singularity(V, R(-9.999999999999998), R(6.89655172413793e-07), [&](const R V) { return (((fabs((V + R(10.0))) > R(0.001))) ? (cmx_div((R(0.00061) * R(1.0) * (V + R(10.0))), cmx_expm1((R(0.145) * (V + R(10.0)))))) : ((R(0.00061) * (R(1) / R(0.145))))); });
+    const R IKr__taukr = cmx_div(R(1.0), (IKr__xkrv1 + IKr__xkrv2));
+    const R IKr__xkrinf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V + R(50.0))) * (R(1) / R(7.5))), V, -0.19235933878519512, -9.617966939259757)));
+    const R IKs__tauxs1 = cmx_remove_singularity(V, R(-29.99999999999988), R(6.75675675675673e-07), [&](const R V) { return (((fabs((V + R(30.0))) < (R(0.001) * (R(1) / R(0.0687))))) ? ((R(1.0) * (R(1) / ((R(7.19e-05) * (R(1) / R(0.148))) + (R(0.000131) * (R(1) / R(0.0687))))))) : (cmx_div(R(1.0), (cmx_div((R(7.19e-05) * (V + R(30.0))), (-cmx_expm1(((-R(0.148)) * (V + R(30.0)))))) + cmx_div((R(0.000131) * (V + R(30.0))), cmx_expm1((R(0.0687) * (V + R(30.0))))))))); });
+    const R IKs__tauxs2 = (R(4.0) * IKs__tauxs1);
+    const R IKs__xs1ss = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - R(1.5))) * (R(1) / R(16.7))), V, -0.08638892460412954, 0.1295833869061943)));
+    const R IKs__xs2ss = IKs__xs1ss;
+    const R INa__ah = (((V < (-R(40.0)))) ? ((R(0.135) * cmx_exp_v(((R(80.0) + V) * (R(1) / (-R(6.8)))), V, -0.2121610354248476, -16.972882833987807))) : (R(0.0)));
+    const R INa__aj = (((V < (-R(40.0)))) ? (cmx_div(((((-R(127140.0)) * cmx_exp_v((R(0.2444) * V), V, 0.3525946679932627, 0.0)) - (R(3.474e-05) * cmx_exp_v(((-R(0.04391)) * V), V, -0.06334873924543438, -0.0))) * R(1.0) * (V + R(37.78))), (R(1.0) + cmx_exp_v((R(0.311) * (V + R(79.23))), V, 0.44867815771646763, 35.54877043587573)))) : (R(0.0)));
+    const R INa__am = cmx_remove_singularity(V, R(-47.12999999999975), R(9.999999999999947e-07), [&](const R V) { return (((fabs((V + R(47.13))) > R(0.001))) ? (cmx_div((R(0.32) * R(1.0) * (V + R(47.13))), (-cmx_expm1(((-R(0.1)) * (V + R(47.13))))))) : (R(3.2))); });
+    const R INa__bh = (((V < (-R(40.0)))) ? (((R(3.56) * cmx_exp_v((R(0.079) * V), V, 0.11397290823022813, 0.0)) + (R(310000.0) * cmx_exp_v((R(0.35) * V), V, 0.5049432643111371, 0.0)))) : (cmx_div(R(1.0), (R(0.13) * (R(1.0) + cmx_exp_v(((V + R(10.66)) * (R(1) / (-R(11.1)))), V, -0.12997252620621294, -1.3855071293582297))))));
+    const R INa__bj = (((V < (-R(40.0)))) ? (cmx_div((R(0.1212) * cmx_exp_v(((-R(0.01052)) * V), V, -0.015177151830151895, -0.0)), (R(1.0) + cmx_exp_v(((-R(0.1378)) * (V + R(40.14))), V, -0.19880337663449915, -7.979967538108797)))) : (cmx_div((R(0.3) * cmx_exp_v(((-R(2.535e-07)) * V), V, -3.657231928653522e-07, -0.0)), (R(1.0) + cmx_exp_v(((-R(0.1)) * (V + R(32.0))), V, -0.14426950408889636, -4.616624130844683)))));
+    const R INa__bm = (R(0.08) * cmx_exp_v(((-V) * (R(1) / R(11.0))), V, -0.1311540946262694, -0.0));
+    const R reversal_potentials__ena = ((R(1.0) * (R(1) / Environment__FonRT)) * cmx_log(cmx_div(Environment__Na_o, y[19])));
+    const R INa__xina = (p[0] * y[2] * y[3] * y[1] * y[1] * y[1] * (V - reversal_potentials__ena));
+    const R INaCa__aloss = cmx_div(R(1.0), (R(1.0) + cmx_ipow<3>(cmx_div(INaCa__xkdna, y[21]))));
+    const R INaCa__zw3 = ((cmx_ipow<3>(y[19]) * Environment__Ca_o * cmx_exp_v((V * R(0.35) * Environment__FonRT), V, 0.019024731229383713, 0.0)) - (cmx_ipow<3>(Environment__Na_o) * Ca__csm * cmx_exp_v((V * (R(0.35) - R(1.0)) * Environment__FonRT), V, -0.035331643711712615, -0.0)));
+    const R INaCa__zw4 = (R(1.0) + (R(0.2) * cmx_exp_v((V * (R(0.35) - R(1.0)) * Environment__FonRT), V, -0.035331643711712615, -0.0)));
+    const R INaCa__yz1 = ((INaCa__xmcao * cmx_ipow<3>(y[19])) + (cmx_ipow<3>(INaCa__xmnao) * Ca__csm));
+    const R INaCa__yz2 = (cmx_ipow<3>(INaCa__xmnai) * Environment__Ca_o * (R(1.0) + (Ca__csm * (R(1) / INaCa__xmcai))));
+    const R INaCa__yz3 = (INaCa__xmcai * cmx_ipow<3>(Environment__Na_o) * (R(1.0) + cmx_ipow<3>((y[19] * (R(1) / INaCa__xmnai)))));
+    const R INaCa__yz4 = ((cmx_ipow<3>(y[19]) * Environment__Ca_o) + (cmx_ipow<3>(Environment__Na_o) * Ca__csm));
+    const R INaCa__zw8 = (INaCa__yz1 + INaCa__yz2 + INaCa__yz3 + INaCa__yz4);
+    const R INaCa__jNaCa = cmx_div((INaCa__gNaCa * INaCa__aloss * INaCa__zw3), (INaCa__zw4 * INaCa__zw8));
+    const R INaCa__xiNaCa = (cell__wca * INaCa__jNaCa);
+    const R INaK__sigma = (cmx_expm1((Environment__Na_o * (R(1) / R(67.3)))) * (R(1) / R(7.0)));
+    const R INaK__fNaK = cmx_div(R(1.0), (R(1.0) + (R(0.1245) * cmx_exp_v(((-R(0.1)) * V * Environment__FonRT), V, -0.005435637494109632, -0.0)) + (R(0.0365) * INaK__sigma * cmx_exp_v(((-V) * Environment__FonRT), V, -0.05435637494109633, -0.0))));
+    const R INaK__xiNaK = ((cmx_div((INaK__gNaK * INaK__fNaK * y[19]), (y[19] + INaK__xkmnai)) * Environment__K_o) * (R(1) / (Environment__K_o + INaK__xkmko)));
+    const R Irel__bv = (((R(1.0) - Irel__av) * Irel__cstar) - R(50.0));
+    const R Irel__Qr0 = ((((y[17] > R(50.0)) && (y[17] < Irel__cstar))) ? (((y[17] - R(50.0)) * (R(1) / R(1.0)))) : ((((y[17] >= Irel__cstar)) ? (((Irel__av * y[17]) + Irel__bv)) : (R(0.0)))));
+    const R Irel__Qr = ((y[23] * Irel__Qr0) * (R(1) / Irel__cstar));
+    const R Irel__sparkV = cmx_div(cmx_exp_v(((-Irel__ay) * (V + R(30.0))), V, -0.07213475204444818, -2.164042561333445), (R(1.0) + cmx_exp_v(((-Irel__ay) * (V + R(30.0))), V, -0.07213475204444818, -2.164042561333445)));
+    const R Irel__spark_rate = ((Irel__gryr * (R(1) / R(1.0))) * ICaL__po * fabs(ICaL__rxa) * Irel__sparkV);
+    const R Irel__xicap = (ICaL__po * Irel__gdyad * fabs(ICaL__rxa));
+    const R Irel__xirp = cmx_div((((ICaL__po * Irel__Qr * fabs(ICaL__rxa) * Irel__gbarsr) * (R(1) / R(1.0))) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R Irel__xiryr = (Irel__xirp + Irel__xicap);
+    const R Ito__rt4 = ((((-V) * (R(1) / R(30.0))) * V) * (R(1) / R(30.0)));
+    const R Ito__txf = ((R(3.5) * cmx_exp(Ito__rt4)) + R(1.5));
+    const R Ito__rt1 = ((-(V + R(3.0))) * (R(1) / R(15.0)));
+    const R Ito__txs = (cmx_div(R(9.0), (R(1.0) + cmx_exp((-Ito__rt1)))) + R(0.5));
+    const R Ito__rt5 = ((V + R(33.5)) * (R(1) / R(10.0)));
+    const R Ito__tyf = (cmx_div(R(20.0), (R(1.0) + cmx_exp(Ito__rt5))) + R(20.0));
+    const R Ito__rt3 = ((V + R(60.0)) * (R(1) / R(10.0)));
+    const R Ito__tys = (cmx_div(R(3000.0), (R(1.0) + cmx_exp(Ito__rt3))) + R(30.0));
+    const R Ito__xtos_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(Ito__rt1)));
+    const R Ito__xtof_inf = Ito__xtos_inf;
+    const R Ito__rt2 = ((V + R(33.5)) * (R(1) / R(10.0)));
+    const R Ito__ytos_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(Ito__rt2)));
+    const R Ito__ytof_inf = Ito__ytos_inf;
+    if (RL) {
+        g_k[1] = ((INa__am * (-R(1))) - (INa__bm * R(1)));
+        g_inf[1] = cmx_div(-((INa__am * R(1.0))), ((INa__am * (-R(1))) - (INa__bm * R(1))));
+    } else {
+        dy[1] = ((INa__am * (R(1.0) - y[1])) - (INa__bm * y[1]));
+    }
+    if (RL) {
+        g_k[2] = ((INa__ah * (-R(1))) - (INa__bh * R(1)));
+        g_inf[2] = cmx_div(-((INa__ah * R(1.0))), ((INa__ah * (-R(1))) - (INa__bh * R(1))));
+    } else {
+        dy[2] = ((INa__ah * (R(1.0) - y[2])) - (INa__bh * y[2]));
+    }
+    if (RL) {
+        g_k[3] = ((INa__aj * (-R(1))) - (INa__bj * R(1)));
+        g_inf[3] = cmx_div(-((INa__aj * R(1.0))), ((INa__aj * (-R(1))) - (INa__bj * R(1))));
+    } else {
+        dy[3] = ((INa__aj * (R(1.0) - y[3])) - (INa__bj * y[3]));
+    }
+    dy[4] = (((ICaL__alpha * y[5]) + (ICaL__k2 * y[6]) + (ICaL__k2t * y[7]) + (ICaL__r2 * ICaL__po)) - ((ICaL__beta + ICaL__r1 + ICaL__k1t + ICaL__k1) * y[4]));
+    dy[5] = (((ICaL__beta * y[4]) + (ICaL__k5 * y[8]) + (ICaL__k5t * y[9])) - ((ICaL__k6 + ICaL__k6t + ICaL__alpha) * y[5]));
+    dy[6] = (((ICaL__k1 * y[4]) + (ICaL__k4 * y[8]) + (ICaL__s1 * ICaL__po)) - ((ICaL__k3 + ICaL__k2 + ICaL__s2) * y[6]));
+    dy[7] = (((ICaL__k1t * y[4]) + (ICaL__k4t * y[9]) + (ICaL__s1t * ICaL__po)) - ((ICaL__k3t + ICaL__k2t + ICaL__s2t) * y[7]));
+    dy[8] = (((ICaL__k3 * y[6]) + (ICaL__k6 * y[5])) - ((ICaL__k5 + ICaL__k4) * y[8]));
+    dy[9] = (((ICaL__k3t * y[7]) + (ICaL__k6t * y[5])) - ((ICaL__k5t + ICaL__k4t) * y[9]));
+    if (RL) {
+        g_k[10] = cmx_div(R(-1), IKr__taukr);
+        g_inf[10] = IKr__xkrinf;
+    } else {
+        dy[10] = cmx_div((IKr__xkrinf - y[10]), IKr__taukr);
+    }
+    if (RL) {
+        g_k[11] = cmx_div(R(-1), IKs__tauxs1);
+        g_inf[11] = IKs__xs1ss;
+    } else {
+        dy[11] = cmx_div((IKs__xs1ss - y[11]), IKs__tauxs1);
+    }
+    if (RL) {
+        g_k[12] = cmx_div(R(-1), IKs__tauxs2);
+        g_inf[12] = IKs__xs2ss;
+    } else {
+        dy[12] = cmx_div((IKs__xs2ss - y[12]), IKs__tauxs2);
+    }
+    if (RL) {
+        g_k[13] = cmx_div(R(-1), Ito__txs);
+        g_inf[13] = Ito__xtos_inf;
+    } else {
+        dy[13] = cmx_div((Ito__xtos_inf - y[13]), Ito__txs);
+    }
+    if (RL) {
+        g_k[14] = cmx_div(R(-1), Ito__tys);
+        g_inf[14] = Ito__ytos_inf;
+    } else {
+        dy[14] = cmx_div((Ito__ytos_inf - y[14]), Ito__tys);
+    }
+    if (RL) {
+        g_k[15] = cmx_div(R(-1), Ito__txf);
+        g_inf[15] = Ito__xtof_inf;
+    } else {
+        dy[15] = cmx_div((Ito__xtof_inf - y[15]), Ito__txf);
+    }
+    if (RL) {
+        g_k[16] = cmx_div(R(-1), Ito__tyf);
+        g_inf[16] = Ito__ytof_inf;
+    } else {
+        dy[16] = cmx_div((Ito__ytof_inf - y[16]), Ito__tyf);
+    }
+    dy[17] = ((y[23] - y[17]) * (R(1) / Irel__taua));
+    dy[18] = ((Irel__spark_rate * Irel__Qr) - ((y[18] * (R(1.0) - cmx_div((Irel__taur * Ca__dCa_JSR), y[23]))) * (R(1) / Irel__taur)));
+    dy[19] = ((-(INa__xina + (R(3.0) * INaK__xiNaK) + (R(3.0) * INaCa__xiNaCa))) * (R(1) / (cell__wca * R(1000.0))));
+    dy[20] = (Irel__xiryr - ((y[20] - y[21]) * (R(1) / Ca__taups)));
+    dy[21] = (Ca__dcsib * ((R(50.0) * (((y[18] - Ca__jd) - ICaL__jca) + INaCa__jNaCa)) - Ca__xbs));
+    dy[22] = (Ca__dciib * (((Ca__jd - Ileak_Iup_Ixfer__jup) + Ileak_Iup_Ixfer__jleak) - Ca__xbi));
+    dy[23] = Ca__dCa_JSR;
+    dy[24] = Ca__xbi;
+    dy[25] = Ca__xbs;
+}
+
+// dV/dt including the stimulus (single-cell integration).
+template <class R>
+CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
+{
+    CMX_MATH_USING
+    (void)p; (void)cm;
+    const R Environment__R = R(8.314472);
+    const R Environment__T = R(308.0);
+    const R Environment__F = R(96.4853415);
+    const R Environment__K_o = R(5.4);
+    const R Environment__Ca_o = R(1.8);
+    const R Environment__Na_o = R(136.0);
+    const R cell__wca = R(8.0);
+    const R ICaL__pca = R(0.00054);
+    const R IK1__gkix = R(0.3);
+    const R IKr__gkr = R(0.0125);
+    const R IKs__gks = R(0.1386);
+    const R Ito__gtos = R(0.11);
+    const R Ito__gtof = R(0.04);
+    const R INaK__gNaK = R(1.5);
+    const R INaK__xkmko = R(1.5);
+    const R INaK__xkmnai = R(12.0);
+    const R INaCa__gNaCa = R(0.84);
+    const R INaCa__xkdna = R(0.3);
+    const R INaCa__xmcao = R(1.3);
+    const R INaCa__xmnao = R(87.5);
+    const R INaCa__xmnai = R(12.3);
+    const R INaCa__xmcai = R(0.0036);
+    const R reversal_potentials__K_i = R(140.0);
+    const R reversal_potentials__prNaK = R(0.01833);
+    const R i_stim = i_stim_area;
+    (void)i_stim;
+    const R ICaL__po = ((((((R(1.0) - y[6]) - y[8]) - y[7]) - y[9]) - y[4]) - y[5]);
+    const R Ca__csm = (y[21] * (R(1) / R(1000.0)));
+    const R Environment__FonRT = (Environment__F * (R(1) / (Environment__R * Environment__T)));
+    const R ICaL__za = (V * R(2.0) * Environment__FonRT);
+    const R ICaL__rxa = (((fabs(ICaL__za) < R(0.001))) ? (((R(4.0) * ICaL__pca * Environment__F * Environment__FonRT * ((Ca__csm * cmx_exp(ICaL__za)) - (R(0.341) * Environment__Ca_o))) * (R(1) / (R(2.0) * Environment__FonRT)))) : (cmx_div((R(4.0) * ICaL__pca * V * Environment__F * Environment__FonRT * ((Ca__csm * cmx_exp(ICaL__za)) - (R(0.341) * Environment__Ca_o))), cmx_expm1(ICaL__za))));
+    const R ICaL__jca = (p[1] * ICaL__po * ICaL__rxa);
+    const R ICaL__xica = (R(2.0) * cell__wca * ICaL__jca);
+    const R reversal_potentials__ek = ((R(1.0) * (R(1) / Environment__FonRT)) * cmx_log((Environment__K_o * (R(1) / reversal_potentials__K_i))));
+    const R IK1__aki = cmx_div(R(1.02), (R(1.0) + cmx_exp_v((R(0.2385) * ((V - reversal_potentials__ek) - R(59.215))), V, 0.3440827672520178, 9.353464736619463)));
+    const R IK1__bki = cmx_div(((R(0.49124) * cmx_exp_v((R(0.08032) * ((V - reversal_potentials__ek) + R(5.476))), V, 0.11587726568420156, 10.64619643607595)) + (R(1.0) * cmx_exp_v((R(0.06175) * ((V - reversal_potentials__ek) - R(594.31))), V, 0.08908641877489352, -45.24799307201933))), (R(1.0) + cmx_exp_v(((-R(0.5143)) * ((V - reversal_potentials__ek) + R(4.753))), V, -0.7419780595291938, -67.63260896497559)));
+    const R IK1__xkin = cmx_div(IK1__aki, (IK1__aki + IK1__bki));
+    const R IK1__xik1 = (IK1__gkix * sqrt((Environment__K_o * (R(1) / R(5.4)))) * IK1__xkin * (V - reversal_potentials__ek));
+    const R IKr__rg = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((V + R(33.0)) * (R(1) / R(22.4))), V, 0.06440602861111444, 2.1253989441667764)));
+    const R IKr__xikr = (p[2] * IKr__gkr * sqrt((Environment__K_o * (R(1) / R(5.4)))) * y[10] * IKr__rg * (V - reversal_potentials__ek));
+    const R IKs__gksx = (R(1.0) + cmx_div(R(0.8), (R(1.0) + cmx_ipow<3>(cmx_div(R(0.5), y[22])))));
+    const R reversal_potentials__eks = ((R(1.0) * (R(1) / Environment__FonRT)) * cmx_log(cmx_div((Environment__K_o + (reversal_potentials__prNaK * Environment__Na_o)), (reversal_potentials__K_i + (reversal_potentials__prNaK * y[19])))));
+    const R IKs__xiks = (p[3] * IKs__gks * IKs__gksx * y[11] * y[12] * (V - reversal_potentials__eks));
+    const R reversal_potentials__ena = ((R(1.0) * (R(1) / Environment__FonRT)) * cmx_log(cmx_div(Environment__Na_o, y[19])));
+    const R INa__xina = (p[0] * y[2] * y[3] * y[1] * y[1] * y[1] * (V - reversal_potentials__ena));
+    const R INaCa__aloss = cmx_div(R(1.0), (R(1.0) + cmx_ipow<3>(cmx_div(INaCa__xkdna, y[21]))));
+    const R INaCa__zw3 = ((cmx_ipow<3>(y[19]) * Environment__Ca_o * cmx_exp_v((V * R(0.35) * Environment__FonRT), V, 0.019024731229383713, 0.0)) - (cmx_ipow<3>(Environment__Na_o) * Ca__csm * cmx_exp_v((V * (R(0.35) - R(1.0)) * Environment__FonRT), V, -0.035331643711712615, -0.0)));
+    const R INaCa__zw4 = (R(1.0) + (R(0.2) * cmx_exp_v((V * (R(0.35) - R(1.0)) * Environment__FonRT), V, -0.035331643711712615, -0.0)));
+    const R INaCa__yz1 = ((INaCa__xmcao * cmx_ipow<3>(y[19])) + (cmx_ipow<3>(INaCa__xmnao) * Ca__csm));
+    const R INaCa__yz2 = (cmx_ipow<3>(INaCa__xmnai) * Environment__Ca_o * (R(1.0) + (Ca__csm * (R(1) / INaCa__xmcai))));
+    const R INaCa__yz3 = (INaCa__xmcai * cmx_ipow<3>(Environment__Na_o) * (R(1.0) + cmx_ipow<3>((y[19] * (R(1) / INaCa__xmnai)))));
+    const R INaCa__yz4 = ((cmx_ipow<3>(y[19]) * Environment__Ca_o) + (cmx_ipow<3>(Environment__Na_o) * Ca__csm));
+    const R INaCa__zw8 = (INaCa__yz1 + INaCa__yz2 + INaCa__yz3 + INaCa__yz4);
+    const R INaCa__jNaCa = cmx_div((INaCa__gNaCa * INaCa__aloss * INaCa__zw3), (INaCa__zw4 * INaCa__zw8));
+    const R INaCa__xiNaCa = (cell__wca * INaCa__jNaCa);
+    const R INaK__sigma = (cmx_expm1((Environment__Na_o * (R(1) / R(67.3)))) * (R(1) / R(7.0)));
+    const R INaK__fNaK = cmx_div(R(1.0), (R(1.0) + (R(0.1245) * cmx_exp_v(((-R(0.1)) * V * Environment__FonRT), V, -0.005435637494109632, -0.0)) + (R(0.0365) * INaK__sigma * cmx_exp_v(((-V) * Environment__FonRT), V, -0.05435637494109633, -0.0))));
+    const R INaK__xiNaK = ((cmx_div((INaK__gNaK * INaK__fNaK * y[19]), (y[19] + INaK__xkmnai)) * Environment__K_o) * (R(1) / (Environment__K_o + INaK__xkmko)));
+    const R Ito__xitof = (Ito__gtof * y[15] * y[16] * (V - reversal_potentials__ek));
+    const R Ito__rt2 = ((V + R(33.5)) * (R(1) / R(10.0)));
+    const R Ito__rs_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(Ito__rt2)));
+    const R Ito__xitos = (p[4] * Ito__gtos * y[13] * (y[14] + (R(0.5) * Ito__rs_inf)) * (V - reversal_potentials__ek));
+    const R Ito__xito = (Ito__xitos + Ito__xitof);
+    const R cell__Itotal = (-(INa__xina + IK1__xik1 + IKr__xikr + IKs__xiks + Ito__xito + INaCa__xiNaCa + ICaL__xica + INaK__xiNaK + i_stim));
+    return cell__Itotal;
+}
+
+// Total ionic current in uA/cm^2 (GetIIonic): minus dV/dt at zero stimulus,
+// times the membrane capacitance that converts it to a current density.
+template <class R>
+CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
+{
+    CMX_MATH_USING
+    (void)p; (void)cm;
+    const R Environment__R = R(8.314472);
+    const R Environment__T = R(308.0);
+    const R Environment__F = R(96.4853415);
+    const R Environment__K_o = R(5.4);
+    const R Environment__Ca_o = R(1.8);
+    const R Environment__Na_o = R(136.0);
+    const R cell__wca = R(8.0);
+    const R ICaL__pca = R(0.00054);
+    const R IK1__gkix = R(0.3);
+    const R IKr__gkr = R(0.0125);
+    const R IKs__gks = R(0.1386);
+    const R Ito__gtos = R(0.11);
+    const R Ito__gtof = R(0.04);
+    const R INaK__gNaK = R(1.5);
+    const R INaK__xkmko = R(1.5);
+    const R INaK__xkmnai = R(12.0);
+    const R INaCa__gNaCa = R(0.84);
+    const R INaCa__xkdna = R(0.3);
+    const R INaCa__xmcao = R(1.3);
+    const R INaCa__xmnao = R(87.5);
+    const R INaCa__xmnai = R(12.3);
+    const R INaCa__xmcai = R(0.0036);
+    const R reversal_potentials__K_i = R(140.0);
+    const R reversal_potentials__prNaK = R(0.01833);
+    const R i_stim = R(0);
+    (void)i_stim;
+    const R ICaL__po = ((((((R(1.0) - y[6]) - y[8]) - y[7]) - y[9]) - y[4]) - y[5]);
+    const R Ca__csm = (y[21] * (R(1) / R(1000.0)));
+    const R Environment__FonRT = (Environment__F * (R(1) / (Environment__R * Environment__T)));
+    const R ICaL__za = (V * R(2.0) * Environment__FonRT);
+    const R ICaL__rxa = (((fabs(ICaL__za) < R(0.001))) ? (((R(4.0) * ICaL__pca * Environment__F * Environment__FonRT * ((Ca__csm * cmx_exp(ICaL__za)) - (R(0.341) * Environment__Ca_o))) * (R(1) / (R(2.0) * Environment__FonRT)))) : (cmx_div((R(4.0) * ICaL__pca * V * Environment__F * Environment__FonRT * ((Ca__csm * cmx_exp(ICaL__za)) - (R(0.341) * Environment__Ca_o))), cmx_expm1(ICaL__za))));
+    const R ICaL__jca = (p[1] * ICaL__po * ICaL__rxa);
+    const R ICaL__xica = (R(2.0) * cell__wca * ICaL__jca);
+    const R reversal_potentials__ek = ((R(1.0) * (R(1) / Environment__FonRT)) * cmx_log((Environment__K_o * (R(1) / reversal_potentials__K_i))));
+    const R IK1__aki = cmx_div(R(1.02), (R(1.0) + cmx_exp_v((R(0.2385) * ((V - reversal_potentials__ek) - R(59.215))), V, 0.3440827672520178, 9.353464736619463)));
+    const R IK1__bki = cmx_div(((R(0.49124) * cmx_exp_v((R(0.08032) * ((V - reversal_potentials__ek) + R(5.476))), V, 0.11587726568420156, 10.64619643607595)) + (R(1.0) * cmx_exp_v((R(0.06175) * ((V - reversal_potentials__ek) - R(594.31))), V, 0.08908641877489352, -45.24799307201933))), (R(1.0) + cmx_exp_v(((-R(0.5143)) * ((V - reversal_potentials__ek) + R(4.753))), V, -0.7419780595291938, -67.63260896497559)));
+    const R IK1__xkin = cmx_div(IK1__aki, (IK1__aki + IK1__bki));
+    const R IK1__xik1 = (IK1__gkix * sqrt((Environment__K_o * (R(1) / R(5.4)))) * IK1__xkin * (V - reversal_potentials__ek));
+    const R IKr__rg = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((V + R(33.0)) * (R(1) / R(22.4))), V, 0.06440602861111444, 2.1253989441667764)));
+    const R IKr__xikr = (p[2] * IKr__gkr * sqrt((Environment__K_o * (R(1) / R(5.4)))) * y[10] * IKr__rg * (V - reversal_potentials__ek));
+    const R IKs__gksx = (R(1.0) + cmx_div(R(0.8), (R(1.0) + cmx_ipow<3>(cmx_div(R(0.5), y[22])))));
+    const R reversal_potentials__eks = ((R(1.0) * (R(1) / Environment__FonRT)) * cmx_log(cmx_div((Environment__K_o + (reversal_potentials__prNaK * Environment__Na_o)), (reversal_potentials__K_i + (reversal_potentials__prNaK * y[19])))));
+    const R IKs__xiks = (p[3] * IKs__gks * IKs__gksx * y[11] * y[12] * (V - reversal_potentials__eks));
+    const R reversal_potentials__ena = ((R(1.0) * (R(1) / Environment__FonRT)) * cmx_log(cmx_div(Environment__Na_o, y[19])));
+    const R INa__xina = (p[0] * y[2] * y[3] * y[1] * y[1] * y[1] * (V - reversal_potentials__ena));
+    const R INaCa__aloss = cmx_div(R(1.0), (R(1.0) + cmx_ipow<3>(cmx_div(INaCa__xkdna, y[21]))));
+    const R INaCa__zw3 = ((cmx_ipow<3>(y[19]) * Environment__Ca_o * cmx_exp_v((V * R(0.35) * Environment__FonRT), V, 0.019024731229383713, 0.0)) - (cmx_ipow<3>(Environment__Na_o) * Ca__csm * cmx_exp_v((V * (R(0.35) - R(1.0)) * Environment__FonRT), V, -0.035331643711712615, -0.0)));
+    const R INaCa__zw4 = (R(1.0) + (R(0.2) * cmx_exp_v((V * (R(0.35) - R(1.0)) * Environment__FonRT), V, -0.035331643711712615, -0.0)));
+    const R INaCa__yz1 = ((INaCa__xmcao * cmx_ipow<3>(y[19])) + (cmx_ipow<3>(INaCa__xmnao) * Ca__csm));
+    const R INaCa__yz2 = (cmx_ipow<3>(INaCa__xmnai) * Environment__Ca_o * (R(1.0) + (Ca__csm * (R(1) / INaCa__xmcai))));
+    const R INaCa__yz3 = (INaCa__xmcai * cmx_ipow<3>(Environment__Na_o) * (R(1.0) + cmx_ipow<3>((y[19] * (R(1) / INaCa__xmnai)))));
+    const R INaCa__yz4 = ((cmx_ipow<3>(y[19]) * Environment__Ca_o) + (cmx_ipow<3>(Environment__Na_o) * Ca__csm));
+    const R INaCa__zw8 = (INaCa__yz1 + INaCa__yz2 + INaCa__yz3 + INaCa__yz4);
+    const R INaCa__jNaCa = cmx_div((INaCa__gNaCa * INaCa__aloss * INaCa__zw3), (INaCa__zw4 * INaCa__zw8));
+    const R INaCa__xiNaCa = (cell__wca * INaCa__jNaCa);
+    const R INaK__sigma = (cmx_expm1((Environment__Na_o * (R(1) / R(67.3)))) * (R(1) / R(7.0)));
+    const R INaK__fNaK = cmx_div(R(1.0), (R(1.0) + (R(0.1245) * cmx_exp_v(((-R(0.1)) * V * Environment__FonRT), V, -0.005435637494109632, -0.0)) + (R(0.0365) * INaK__sigma * cmx_exp_v(((-V) * Environment__FonRT), V, -0.05435637494109633, -0.0))));
+    const R INaK__xiNaK = ((cmx_div((INaK__gNaK * INaK__fNaK * y[19]), (y[19] + INaK__xkmnai)) * Environment__K_o) * (R(1) / (Environment__K_o + INaK__xkmko)));
+    const R Ito__xitof = (Ito__gtof * y[15] * y[16] * (V - reversal_potentials__ek));
+    const R Ito__rt2 = ((V + R(33.5)) * (R(1) / R(10.0)));
+    const R Ito__rs_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(Ito__rt2)));
+    const R Ito__xitos = (p[4] * Ito__gtos * y[13] * (y[14] + (R(0.5) * Ito__rs_inf)) * (V - reversal_potentials__ek));
+    const R Ito__xito = (Ito__xitos + Ito__xitof);
+    const R cell__Itotal = (-(INa__xina + IK1__xik1 + IKr__xikr + IKs__xiks + Ito__xito + INaCa__xiNaCa + ICaL__xica + INaK__xiNaK + i_stim));
+    const R dv = cell__Itotal;
+    return -dv * R(1);
+}
+
+// Backward Euler: the states solved by Newton (all but the linear gates).
+constexpr int kNumNonlinear = 15;
+static constexpr int kNonlinear[15] = {4, 5, 6, 7, 8, 9, 17, 18, 19, 20, 21, 22, 23, 24, 25};
+
+// Linear states: y <- (y + a dt) / (1 - b dt) for dy/dt = a + b y, a and b
+// functions of V (ComputeOneStepExceptVoltage, closed-form part).
+template <class R>
+CMX_HD inline void be_linear(R* y, R V, const R* p, R cm, R dt)
+{
+    CMX_MATH_USING
+    (void)p; (void)cm; (void)V;
+    const R IKr__xkrv1 = cmx_remove_singularity(V, R(-7.0), R(8.130081300813008e-07), [&](const R V) { return (((fabs((V + R(7.0))) > R(0.001))) ? (cmx_div((R(0.00138) * R(1.0) * (V + R(7.0))), (-cmx_expm1(((-R(0.123)) * (V + R(7.0))))))) : ((R(0.00138) * (R(1) / R(0.123))))); });
+    const R IKr__xkrv2 = cmx_remove_singularity(V, R(-9.999999999999998), R(6.89655172413793e-07), [&](const R V) { return (((fabs((V + R(10.0))) > R(0.001))) ? (cmx_div((R(0.00061) * R(1.0) * (V + R(10.0))), cmx_expm1((R(0.145) * (V + R(10.0)))))) : ((R(0.00061) * (R(1) / R(0.145))))); });
+    const R IKr__taukr = cmx_div(R(1.0), (IKr__xkrv1 + IKr__xkrv2));
+    const R IKr__xkrinf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V + R(50.0))) * (R(1) / R(7.5))), V, -0.19235933878519512, -9.617966939259757)));
+    const R IKs__tauxs1 = cmx_remove_singularity(V, R(-29.99999999999988), R(6.75675675675673e-07), [&](const R V) { return (((fabs((V + R(30.0))) < (R(0.001) * (R(1) / R(0.0687))))) ? ((R(1.0) * (R(1) / ((R(7.19e-05) * (R(1) / R(0.148))) + (R(0.000131) * (R(1) / R(0.0687))))))) : (cmx_div(R(1.0), (cmx_div((R(7.19e-05) * (V + R(30.0))), (-cmx_expm1(((-R(0.148)) * (V + R(30.0)))))) + cmx_div((R(0.000131) * (V + R(30.0))), cmx_expm1((R(0.0687) * (V + R(30.0))))))))); });
+    const R IKs__tauxs2 = (R(4.0) * IKs__tauxs1);
+    const R IKs__xs1ss = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - R(1.5))) * (R(1) / R(16.7))), V, -0.08638892460412954, 0.1295833869061943)));
+    const R IKs__xs2ss = IKs__xs1ss;
+    const R INa__ah = (((V < (-R(40.0)))) ? ((R(0.135) * cmx_exp_v(((R(80.0) + V) * (R(1) / (-R(6.8)))), V, -0.2121610354248476, -16.972882833987807))) : (R(0.0)));
+    const R INa__aj = (((V < (-R(40.0)))) ? (cmx_div(((((-R(127140.0)) * cmx_exp_v((R(0.2444) * V), V, 0.3525946679932627, 0.0)) - (R(3.474e-05) * cmx_exp_v(((-R(0.04391)) * V), V, -0.06334873924543438, -0.0))) * R(1.0) * (V + R(37.78))), (R(1.0) + cmx_exp_v((R(0.311) * (V + R(79.23))), V, 0.44867815771646763, 35.54877043587573)))) : (R(0.0)));
+    const R INa__am = cmx_remove_singularity(V, R(-47.12999999999975), R(9.999999999999947e-07), [&](const R V) { return (((fabs((V + R(47.13))) > R(0.001))) ? (cmx_div((R(0.32) * R(1.0) * (V + R(47.13))), (-cmx_expm1(((-R(0.1)) * (V + R(47.13))))))) : (R(3.2))); });
+    const R INa__bh = (((V < (-R(40.0)))) ? (((R(3.56) * cmx_exp_v((R(0.079) * V), V, 0.11397290823022813, 0.0)) + (R(310000.0) * cmx_exp_v((R(0.35) * V), V, 0.5049432643111371, 0.0)))) : (cmx_div(R(1.0), (R(0.13) * (R(1.0) + cmx_exp_v(((V + R(10.66)) * (R(1) / (-R(11.1)))), V, -0.12997252620621294, -1.3855071293582297))))));
+    const R INa__bj = (((V < (-R(40.0)))) ? (cmx_div((R(0.1212) * cmx_exp_v(((-R(0.01052)) * V), V, -0.015177151830151895, -0.0)), (R(1.0) + cmx_exp_v(((-R(0.1378)) * (V + R(40.14))), V, -0.19880337663449915, -7.979967538108797)))) : (cmx_div((R(0.3) * cmx_exp_v(((-R(2.535e-07)) * V), V, -3.657231928653522e-07, -0.0)), (R(1.0) + cmx_exp_v(((-R(0.1)) * (V + R(32.0))), V, -0.14426950408889636, -4.616624130844683)))));
+    const R INa__bm = (R(0.08) * cmx_exp_v(((-V) * (R(1) / R(11.0))), V, -0.1311540946262694, -0.0));
+    const R Ito__rt4 = ((((-V) * (R(1) / R(30.0))) * V) * (R(1) / R(30.0)));
+    const R Ito__txf = ((R(3.5) * cmx_exp(Ito__rt4)) + R(1.5));
+    const R Ito__rt1 = ((-(V + R(3.0))) * (R(1) / R(15.0)));
+    const R Ito__txs = (cmx_div(R(9.0), (R(1.0) + cmx_exp((-Ito__rt1)))) + R(0.5));
+    const R Ito__rt5 = ((V + R(33.5)) * (R(1) / R(10.0)));
+    const R Ito__tyf = (cmx_div(R(20.0), (R(1.0) + cmx_exp(Ito__rt5))) + R(20.0));
+    const R Ito__rt3 = ((V + R(60.0)) * (R(1) / R(10.0)));
+    const R Ito__tys = (cmx_div(R(3000.0), (R(1.0) + cmx_exp(Ito__rt3))) + R(30.0));
+    const R Ito__xtos_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(Ito__rt1)));
+    const R Ito__xtof_inf = Ito__xtos_inf;
+    const R Ito__rt2 = ((V + R(33.5)) * (R(1) / R(10.0)));
+    const R Ito__ytos_inf = cmx_div(R(1.0), (R(1.0) + cmx_exp(Ito__rt2)));
+    const R Ito__ytof_inf = Ito__ytos_inf;
+    y[1] = (y[1] + ((INa__am * R(1.0))) * dt) / (R(1) - (((INa__am * (-R(1))) - (INa__bm * R(1)))) * dt);
+    y[2] = (y[2] + ((INa__ah * R(1.0))) * dt) / (R(1) - (((INa__ah * (-R(1))) - (INa__bh * R(1)))) * dt);
+    y[3] = (y[3] + ((INa__aj * R(1.0))) * dt) / (R(1) - (((INa__aj * (-R(1))) - (INa__bj * R(1)))) * dt);
+    y[10] = (y[10] + ((IKr__xkrinf / IKr__taukr)) * dt) / (R(1) - (((-R(1)) / IKr__taukr)) * dt);
+    y[11] = (y[11] + ((IKs__xs1ss / IKs__tauxs1)) * dt) / (R(1) - (((-R(1)) / IKs__tauxs1)) * dt);
+    y[12] = (y[12] + ((IKs__xs2ss / IKs__tauxs2)) * dt) / (R(1) - (((-R(1)) / IKs__tauxs2)) * dt);
+    y[13] = (y[13] + ((Ito__xtos_inf / Ito__txs)) * dt) / (R(1) - (((-R(1)) / Ito__txs)) * dt);
+    y[14] = (y[14] + ((Ito__ytos_inf / Ito__tys)) * dt) / (R(1) - (((-R(1)) / Ito__tys)) * dt);
+    y[15] = (y[15] + ((Ito__xtof_inf / Ito__txf)) * dt) / (R(1) - (((-R(1)) / Ito__txf)) * dt);
+    y[16] = (y[16] + ((Ito__ytof_inf / Ito__tyf)) * dt) / (R(1) - (((-R(1)) / Ito__tyf)) * dt);
+}
+
+// Newton residual of the nonlinear states at guess g (y holds the other
+// states, linear ones already advanced): g - y_old - dt f(g).
+template <class R>
+CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const R* p, R cm, R dt, R* res)
+{
+    CMX_MATH_USING
+    (void)y; (void)p; (void)cm; (void)i_stim_area;
+    const R Environment__R = R(8.314472);
+    const R Environment__T = R(308.0);
+    const R Environment__F = R(96.4853415);
+    const R Environment__K_o = R(5.4);
+    const R Environment__Ca_o = R(1.8);
+    const R Environment__Na_o = R(136.0);
+    const R cell__wca = R(8.0);
+    const R ICaL__pca = R(0.00054);
+    const R ICaL__vth = R(0.0);
+    const R ICaL__s6 = R(8.0);
+    const R ICaL__vx = R(-40.0);
+    const R ICaL__sx = R(3.0);
+    const R ICaL__vy = R(-40.0);
+    const R ICaL__sy = R(4.0);
+    const R ICaL__vyr = R(-40.0);
+    const R ICaL__syr = R(11.32);
+    const R ICaL__cat = R(3.0);
+    const R ICaL__cpt = R(6.09365);
+    const R ICaL__k2 = R(0.000103615);
+    const R ICaL__k1t = R(0.00413);
+    const R ICaL__k2t = R(0.00224);
+    const R ICaL__r1 = R(0.3);
+    const R ICaL__r2 = R(3.0);
+    const R ICaL__s1t = R(0.00195);
+    const R ICaL__tca = R(78.0329);
+    const R ICaL__taupo = R(1.0);
+    const R ICaL__tau3 = R(3.0);
+    const R INaK__gNaK = R(1.5);
+    const R INaK__xkmko = R(1.5);
+    const R INaK__xkmnai = R(12.0);
+    const R INaCa__gNaCa = R(0.84);
+    const R INaCa__xkdna = R(0.3);
+    const R INaCa__xmcao = R(1.3);
+    const R INaCa__xmnao = R(87.5);
+    const R INaCa__xmnai = R(12.3);
+    const R INaCa__xmcai = R(0.0036);
+    const R Irel__cstar = R(90.0);
+    const R Irel__gryr = R(2.58079);
+    const R Irel__gbarsr = R(26841.8);
+    const R Irel__gdyad = R(9000.0);
+    const R Irel__ax = R(0.3576);
+    const R Irel__ay = R(0.05);
+    const R Irel__av = R(11.3);
+    const R Irel__taua = R(100.0);
+    const R Irel__taur = R(30.0);
+    const R Ileak_Iup_Ixfer__cup = R(0.5);
+    const R Ileak_Iup_Ixfer__kj = R(50.0);
+    const R Ileak_Iup_Ixfer__vup = R(0.4);
+    const R Ileak_Iup_Ixfer__gleak = R(2.069e-05);
+    const R Ca__bcal = R(24.0);
+    const R Ca__xkcal = R(7.0);
+    const R Ca__srmax = R(47.0);
+    const R Ca__srkd = R(0.6);
+    const R Ca__bmem = R(15.0);
+    const R Ca__kmem = R(0.3);
+    const R Ca__bsar = R(42.0);
+    const R Ca__ksar = R(13.0);
+    const R Ca__xkon = R(0.0327);
+    const R Ca__xkoff = R(0.0196);
+    const R Ca__btrop = R(70.0);
+    const R Ca__taud = R(4.0);
+    const R Ca__taups = R(0.5);
+    const R i_stim = i_stim_area;
+    (void)i_stim;
+    const R Ileak_Iup_Ixfer__jleak = (cmx_div((Ileak_Iup_Ixfer__gleak * g[12] * g[12]), ((g[12] * g[12]) + (Ileak_Iup_Ixfer__kj * Ileak_Iup_Ixfer__kj))) * ((g[12] * R(16.667)) - g[11]));
+    const R Ileak_Iup_Ixfer__jup = cmx_div((Ileak_Iup_Ixfer__vup * g[11] * g[11]), ((g[11] * g[11]) + (Ileak_Iup_Ixfer__cup * Ileak_Iup_Ixfer__cup)));
+    const R Ca__dCa_JSR = (((-g[7]) + Ileak_Iup_Ixfer__jup) - Ileak_Iup_Ixfer__jleak);
+    const R Ca__bpxi = cmx_div((Ca__bcal * Ca__xkcal), ((Ca__xkcal + g[11]) * (Ca__xkcal + g[11])));
+    const R Ca__mempxi = cmx_div((Ca__bmem * Ca__kmem), ((Ca__kmem + g[11]) * (Ca__kmem + g[11])));
+    const R Ca__sarpxi = cmx_div((Ca__bsar * Ca__ksar), ((Ca__ksar + g[11]) * (Ca__ksar + g[11])));
+    const R Ca__spxi = cmx_div((Ca__srmax * Ca__srkd), ((Ca__srkd + g[11]) * (Ca__srkd + g[11])));
+    const R Ca__dciib = cmx_div(R(1.0), (R(1.0) + Ca__bpxi + Ca__spxi + Ca__mempxi + Ca__sarpxi));
+    const R Ca__bpxs = cmx_div((Ca__bcal * Ca__xkcal), ((Ca__xkcal + g[10]) * (Ca__xkcal + g[10])));
+    const R Ca__mempxs = cmx_div((Ca__bmem * Ca__kmem), ((Ca__kmem + g[10]) * (Ca__kmem + g[10])));
+    const R Ca__sarpxs = cmx_div((Ca__bsar * Ca__ksar), ((Ca__ksar + g[10]) * (Ca__ksar + g[10])));
+    const R Ca__spxs = cmx_div((Ca__srmax * Ca__srkd), ((Ca__srkd + g[10]) * (Ca__srkd + g[10])));
+    const R Ca__dcsib = cmx_div(R(1.0), (R(1.0) + Ca__bpxs + Ca__spxs + Ca__mempxs + Ca__sarpxs));
+    const R Ca__jd = ((g[10] - g[11]) * (R(1) / Ca__taud));
+    const R Ca__xbi = ((Ca__xkon * g[11] * (Ca__btrop - g[13])) - (Ca__xkoff * g[13]));
+    const R Ca__xbs = ((Ca__xkon * g[10] * (Ca__btrop - g[14])) - (Ca__xkoff * g[14]));
+    const R ICaL__poinf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vth)) * (R(1) / ICaL__s6)), V, -0.18033688011112042, -0.0)));
+    const R ICaL__alpha = (ICaL__poinf * (R(1) / ICaL__taupo));
+    const R ICaL__beta = ((R(1.0) - ICaL__poinf) * (R(1) / ICaL__taupo));
+    const R ICaL__po = ((((((R(1.0) - g[2]) - g[4]) - g[3]) - g[5]) - g[0]) - g[1]);
+    const R Ca__csm = (g[10] * (R(1) / R(1000.0)));
+    const R Environment__FonRT = (Environment__F * (R(1) / (Environment__R * Environment__T)));
+    const R ICaL__za = (V * R(2.0) * Environment__FonRT);
+    const R ICaL__rxa = (((fabs(ICaL__za) < R(0.001))) ? (((R(4.0) * ICaL__pca * Environment__F * Environment__FonRT * ((Ca__csm * cmx_exp(ICaL__za)) - (R(0.341) * Environment__Ca_o))) * (R(1) / (R(2.0) * Environment__FonRT)))) : (cmx_div((R(4.0) * ICaL__pca * V * Environment__F * Environment__FonRT * ((Ca__csm * cmx_exp(ICaL__za)) - (R(0.341) * Environment__Ca_o))), cmx_expm1(ICaL__za))));
+    const R ICaL__jca = (p[1] * ICaL__po * ICaL__rxa);
+    const R ICaL__fca = cmx_div(R(1.0), (R(1.0) + cmx_ipow<3>(cmx_div(ICaL__cat, g[9]))));
+    const R ICaL__k1 = (R(0.024168) * ICaL__fca);
+    const R ICaL__poi = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vx)) * (R(1) / ICaL__sx)), V, -0.4808983469629878, -19.235933878519514)));
+    const R ICaL__k3 = ((R(1.0) - ICaL__poi) * (R(1) / ICaL__tau3));
+    const R ICaL__k3t = ICaL__k3;
+    const R ICaL__Ps = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vyr)) * (R(1) / ICaL__syr)), V, -0.12744655838241728, -5.09786233529669)));
+    const R ICaL__Pr = (R(1.0) - cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vy)) * (R(1) / ICaL__sy)), V, -0.36067376022224085, -14.426950408889635))));
+    const R ICaL__recov = (R(10.0) + (R(4954.0) * cmx_exp_v((V * (R(1) / R(15.6))), V, 0.09248045133903611, 0.0)));
+    const R ICaL__tau_ca = (cmx_div(ICaL__tca, (R(1.0) + cmx_ipow<4>((g[9] * (R(1) / ICaL__cpt))))) + R(0.1));
+    const R ICaL__tauca = (((ICaL__recov - ICaL__tau_ca) * ICaL__Pr) + ICaL__tau_ca);
+    const R ICaL__k5 = cmx_div((R(1.0) - ICaL__Ps), ICaL__tauca);
+    const R ICaL__k6 = cmx_div((ICaL__fca * ICaL__Ps), ICaL__tauca);
+    const R ICaL__k4 = cmx_div((((cmx_div((ICaL__k3 * ICaL__alpha), ICaL__beta) * ICaL__k1) * (R(1) / ICaL__k2)) * ICaL__k5), ICaL__k6);
+    const R ICaL__tauba = (((ICaL__recov - R(450.0)) * ICaL__Pr) + R(450.0));
+    const R ICaL__k5t = cmx_div((R(1.0) - ICaL__Ps), ICaL__tauba);
+    const R ICaL__k6t = cmx_div(ICaL__Ps, ICaL__tauba);
+    const R ICaL__k4t = cmx_div((((cmx_div((ICaL__k3t * ICaL__alpha), ICaL__beta) * ICaL__k1t) * (R(1) / ICaL__k2t)) * ICaL__k5t), ICaL__k6t);
+    const R ICaL__s1 = (R(0.0182688) * ICaL__fca);
+    const R ICaL__s2 = cmx_div((((ICaL__s1 * ICaL__r1) * (R(1) / ICaL__r2)) * ICaL__k2), ICaL__k1);
+    const R ICaL__s2t = ((((ICaL__s1t * ICaL__r1) * (R(1) / ICaL__r2)) * ICaL__k2t) * (R(1) / ICaL__k1t));
+    const R reversal_potentials__ena = ((R(1.0) * (R(1) / Environment__FonRT)) * cmx_log(cmx_div(Environment__Na_o, g[8])));
+    const R INa__xina = (p[0] * y[2] * y[3] * y[1] * y[1] * y[1] * (V - reversal_potentials__ena));
+    const R INaCa__aloss = cmx_div(R(1.0), (R(1.0) + cmx_ipow<3>(cmx_div(INaCa__xkdna, g[10]))));
+    const R INaCa__zw3 = ((cmx_ipow<3>(g[8]) * Environment__Ca_o * cmx_exp_v((V * R(0.35) * Environment__FonRT), V, 0.019024731229383713, 0.0)) - (cmx_ipow<3>(Environment__Na_o) * Ca__csm * cmx_exp_v((V * (R(0.35) - R(1.0)) * Environment__FonRT), V, -0.035331643711712615, -0.0)));
+    const R INaCa__zw4 = (R(1.0) + (R(0.2) * cmx_exp_v((V * (R(0.35) - R(1.0)) * Environment__FonRT), V, -0.035331643711712615, -0.0)));
+    const R INaCa__yz1 = ((INaCa__xmcao * cmx_ipow<3>(g[8])) + (cmx_ipow<3>(INaCa__xmnao) * Ca__csm));
+    const R INaCa__yz2 = (cmx_ipow<3>(INaCa__xmnai) * Environment__Ca_o * (R(1.0) + (Ca__csm * (R(1) / INaCa__xmcai))));
+    const R INaCa__yz3 = (INaCa__xmcai * cmx_ipow<3>(Environment__Na_o) * (R(1.0) + cmx_ipow<3>((g[8] * (R(1) / INaCa__xmnai)))));
+    const R INaCa__yz4 = ((cmx_ipow<3>(g[8]) * Environment__Ca_o) + (cmx_ipow<3>(Environment__Na_o) * Ca__csm));
+    const R INaCa__zw8 = (INaCa__yz1 + INaCa__yz2 + INaCa__yz3 + INaCa__yz4);
+    const R INaCa__jNaCa = cmx_div((INaCa__gNaCa * INaCa__aloss * INaCa__zw3), (INaCa__zw4 * INaCa__zw8));
+    const R INaCa__xiNaCa = (cell__wca * INaCa__jNaCa);
+    const R INaK__sigma = (cmx_expm1((Environment__Na_o * (R(1) / R(67.3)))) * (R(1) / R(7.0)));
+    const R INaK__fNaK = cmx_div(R(1.0), (R(1.0) + (R(0.1245) * cmx_exp_v(((-R(0.1)) * V * Environment__FonRT), V, -0.005435637494109632, -0.0)) + (R(0.0365) * INaK__sigma * cmx_exp_v(((-V) * Environment__FonRT), V, -0.05435637494109633, -0.0))));
+    const R INaK__xiNaK = ((cmx_div((INaK__gNaK * INaK__fNaK * g[8]), (g[8] + INaK__xkmnai)) * Environment__K_o) * (R(1) / (Environment__K_o + INaK__xkmko)));
+    const R Irel__bv = (((R(1.0) - Irel__av) * Irel__cstar) - R(50.0));
+    const R Irel__Qr0 = ((((g[6] > R(50.0)) && (g[6] < Irel__cstar))) ? (((g[6] - R(50.0)) * (R(1) / R(1.0)))) : ((((g[6] >= Irel__cstar)) ? (((Irel__av * g[6]) + Irel__bv)) : (R(0.0)))));
+    const R Irel__Qr = ((g[12] * Irel__Qr0) * (R(1) / Irel__cstar));
+    const R Irel__sparkV = cmx_div(cmx_exp_v(((-Irel__ay) * (V + R(30.0))), V, -0.07213475204444818, -2.164042561333445), (R(1.0) + cmx_exp_v(((-Irel__ay) * (V + R(30.0))), V, -0.07213475204444818, -2.164042561333445)));
+    const R Irel__spark_rate = ((Irel__gryr * (R(1) / R(1.0))) * ICaL__po * fabs(ICaL__rxa) * Irel__sparkV);
+    const R Irel__xicap = (ICaL__po * Irel__gdyad * fabs(ICaL__rxa));
+    const R Irel__xirp = cmx_div((((ICaL__po * Irel__Qr * fabs(ICaL__rxa) * Irel__gbarsr) * (R(1) / R(1.0))) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R Irel__xiryr = (Irel__xirp + Irel__xicap);
+    res[0] = g[0] - y[4] - dt * (((ICaL__alpha * g[1]) + (ICaL__k2 * g[2]) + (ICaL__k2t * g[3]) + (ICaL__r2 * ICaL__po)) - ((ICaL__beta + ICaL__r1 + ICaL__k1t + ICaL__k1) * g[0]));
+    res[1] = g[1] - y[5] - dt * (((ICaL__beta * g[0]) + (ICaL__k5 * g[4]) + (ICaL__k5t * g[5])) - ((ICaL__k6 + ICaL__k6t + ICaL__alpha) * g[1]));
+    res[2] = g[2] - y[6] - dt * (((ICaL__k1 * g[0]) + (ICaL__k4 * g[4]) + (ICaL__s1 * ICaL__po)) - ((ICaL__k3 + ICaL__k2 + ICaL__s2) * g[2]));
+    res[3] = g[3] - y[7] - dt * (((ICaL__k1t * g[0]) + (ICaL__k4t * g[5]) + (ICaL__s1t * ICaL__po)) - ((ICaL__k3t + ICaL__k2t + ICaL__s2t) * g[3]));
+    res[4] = g[4] - y[8] - dt * (((ICaL__k3 * g[2]) + (ICaL__k6 * g[1])) - ((ICaL__k5 + ICaL__k4) * g[4]));
+    res[5] = g[5] - y[9] - dt * (((ICaL__k3t * g[3]) + (ICaL__k6t * g[1])) - ((ICaL__k5t + ICaL__k4t) * g[5]));
+    res[6] = g[6] - y[17] - dt * ((g[12] - g[6]) * (R(1) / Irel__taua));
+    res[7] = g[7] - y[18] - dt * ((Irel__spark_rate * Irel__Qr) - ((g[7] * (R(1.0) - cmx_div((Irel__taur * Ca__dCa_JSR), g[12]))) * (R(1) / Irel__taur)));
+    res[8] = g[8] - y[19] - dt * ((-(INa__xina + (R(3.0) * INaK__xiNaK) + (R(3.0) * INaCa__xiNaCa))) * (R(1) / (cell__wca * R(1000.0))));
+    res[9] = g[9] - y[20] - dt * (Irel__xiryr - ((g[9] - g[10]) * (R(1) / Ca__taups)));
+    res[10] = g[10] - y[21] - dt * (Ca__dcsib * ((R(50.0) * (((g[7] - Ca__jd) - ICaL__jca) + INaCa__jNaCa)) - Ca__xbs));
+    res[11] = g[11] - y[22] - dt * (Ca__dciib * (((Ca__jd - Ileak_Iup_Ixfer__jup) + Ileak_Iup_Ixfer__jleak) - Ca__xbi));
+    res[12] = g[12] - y[23] - dt * Ca__dCa_JSR;
+    res[13] = g[13] - y[24] - dt * Ca__xbi;
+    res[14] = g[14] - y[25] - dt * Ca__xbs;
+}
+
+// Jacobian of be_residual in g: J[i*N + j] = delta_ij - dt d f_i / d g_j.
+template <class R>
+CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const R* p, R cm, R dt, R* J)
+{
+    CMX_MATH_USING
+    (void)y; (void)p; (void)cm; (void)i_stim_area;
+    const R Environment__R = R(8.314472);
+    const R Environment__T = R(308.0);
+    const R Environment__F = R(96.4853415);
+    const R Environment__K_o = R(5.4);
+    const R Environment__Ca_o = R(1.8);
+    const R Environment__Na_o = R(136.0);
+    const R cell__wca = R(8.0);
+    const R ICaL__pca = R(0.00054);
+    const R ICaL__vth = R(0.0);
+    const R ICaL__s6 = R(8.0);
+    const R ICaL__vx = R(-40.0);
+    const R ICaL__sx = R(3.0);
+    const R ICaL__vy = R(-40.0);
+    const R ICaL__sy = R(4.0);
+    const R ICaL__vyr = R(-40.0);
+    const R ICaL__syr = R(11.32);
+    const R ICaL__cat = R(3.0);
+    const R ICaL__cpt = R(6.09365);
+    const R ICaL__k2 = R(0.000103615);
+    const R ICaL__k1t = R(0.00413);
+    const R ICaL__k2t = R(0.00224);
+    const R ICaL__r1 = R(0.3);
+    const R ICaL__r2 = R(3.0);
+    const R ICaL__s1t = R(0.00195);
+    const R ICaL__tca = R(78.0329);
+    const R ICaL__taupo = R(1.0);
+    const R ICaL__tau3 = R(3.0);
+    const R INaK__gNaK = R(1.5);
+    const R INaK__xkmko = R(1.5);
+    const R INaK__xkmnai = R(12.0);
+    const R INaCa__gNaCa = R(0.84);
+    const R INaCa__xkdna = R(0.3);
+    const R INaCa__xmcao = R(1.3);
+    const R INaCa__xmnao = R(87.5);
+    const R INaCa__xmnai = R(12.3);
+    const R INaCa__xmcai = R(0.0036);
+    const R Irel__cstar = R(90.0);
+    const R Irel__gryr = R(2.58079);
+    const R Irel__gbarsr = R(26841.8);
+    const R Irel__gdyad = R(9000.0);
+    const R Irel__ax = R(0.3576);
+    const R Irel__ay = R(0.05);
+    const R Irel__av = R(11.3);
+    const R Irel__taua = R(100.0);
+    const R Irel__taur = R(30.0);
+    const R Ileak_Iup_Ixfer__cup = R(0.5);
+    const R Ileak_Iup_Ixfer__kj = R(50.0);
+    const R Ileak_Iup_Ixfer__vup = R(0.4);
+    const R Ileak_Iup_Ixfer__gleak = R(2.069e-05);
+    const R Ca__bcal = R(24.0);
+    const R Ca__xkcal = R(7.0);
+    const R Ca__srmax = R(47.0);
+    const R Ca__srkd = R(0.6);
+    const R Ca__bmem = R(15.0);
+    const R Ca__kmem = R(0.3);
+    const R Ca__bsar = R(42.0);
+    const R Ca__ksar = R(13.0);
+    const R Ca__xkon = R(0.0327);
+    const R Ca__xkoff = R(0.0196);
+    const R Ca__btrop = R(70.0);
+    const R Ca__taud = R(4.0);
+    const R Ca__taups = R(0.5);
+    const R i_stim = i_stim_area;
+    (void)i_stim;
+    const R Ileak_Iup_Ixfer__jleak = (cmx_div((Ileak_Iup_Ixfer__gleak * g[12] * g[12]), ((g[12] * g[12]) + (Ileak_Iup_Ixfer__kj * Ileak_Iup_Ixfer__kj))) * ((g[12] * R(16.667)) - g[11]));
+    const R Ileak_Iup_Ixfer__jup = cmx_div((Ileak_Iup_Ixfer__vup * g[11] * g[11]), ((g[11] * g[11]) + (Ileak_Iup_Ixfer__cup * Ileak_Iup_Ixfer__cup)));
+    const R Ca__dCa_JSR = (((-g[7]) + Ileak_Iup_Ixfer__jup) - Ileak_Iup_Ixfer__jleak);
+    const R Ca__bpxi = cmx_div((Ca__bcal * Ca__xkcal), ((Ca__xkcal + g[11]) * (Ca__xkcal + g[11])));
+    const R Ca__mempxi = cmx_div((Ca__bmem * Ca__kmem), ((Ca__kmem + g[11]) * (Ca__kmem + g[11])));
+    const R Ca__sarpxi = cmx_div((Ca__bsar * Ca__ksar), ((Ca__ksar + g[11]) * (Ca__ksar + g[11])));
+    const R Ca__spxi = cmx_div((Ca__srmax * Ca__srkd), ((Ca__srkd + g[11]) * (Ca__srkd + g[11])));
+    const R Ca__dciib = cmx_div(R(1.0), (R(1.0) + Ca__bpxi + Ca__spxi + Ca__mempxi + Ca__sarpxi));
+    const R Ca__bpxs = cmx_div((Ca__bcal * Ca__xkcal), ((Ca__xkcal + g[10]) * (Ca__xkcal + g[10])));
+    const R Ca__mempxs = cmx_div((Ca__bmem * Ca__kmem), ((Ca__kmem + g[10]) * (Ca__kmem + g[10])));
+    const R Ca__sarpxs = cmx_div((Ca__bsar * Ca__ksar), ((Ca__ksar + g[10]) * (Ca__ksar + g[10])));
+    const R Ca__spxs = cmx_div((Ca__srmax * Ca__srkd), ((Ca__srkd + g[10]) * (Ca__srkd + g[10])));
+    const R Ca__dcsib = cmx_div(R(1.0), (R(1.0) + Ca__bpxs + Ca__spxs + Ca__mempxs + Ca__sarpxs));
+    const R Ca__jd = ((g[10] - g[11]) * (R(1) / Ca__taud));
+    const R Ca__xbi = ((Ca__xkon * g[11] * (Ca__btrop - g[13])) - (Ca__xkoff * g[13]));
+    const R Ca__xbs = ((Ca__xkon * g[10] * (Ca__btrop - g[14])) - (Ca__xkoff * g[14]));
+    const R ICaL__poinf = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vth)) * (R(1) / ICaL__s6)), V, -0.18033688011112042, -0.0)));
+    const R ICaL__alpha = (ICaL__poinf * (R(1) / ICaL__taupo));
+    const R ICaL__beta = ((R(1.0) - ICaL__poinf) * (R(1) / ICaL__taupo));
+    const R ICaL__po = ((((((R(1.0) - g[2]) - g[4]) - g[3]) - g[5]) - g[0]) - g[1]);
+    const R Ca__csm = (g[10] * (R(1) / R(1000.0)));
+    const R Environment__FonRT = (Environment__F * (R(1) / (Environment__R * Environment__T)));
+    const R ICaL__za = (V * R(2.0) * Environment__FonRT);
+    const R ICaL__rxa = (((fabs(ICaL__za) < R(0.001))) ? (((R(4.0) * ICaL__pca * Environment__F * Environment__FonRT * ((Ca__csm * cmx_exp(ICaL__za)) - (R(0.341) * Environment__Ca_o))) * (R(1) / (R(2.0) * Environment__FonRT)))) : (cmx_div((R(4.0) * ICaL__pca * V * Environment__F * Environment__FonRT * ((Ca__csm * cmx_exp(ICaL__za)) - (R(0.341) * Environment__Ca_o))), cmx_expm1(ICaL__za))));
+    const R ICaL__jca = (p[1] * ICaL__po * ICaL__rxa);
+    const R ICaL__fca = cmx_div(R(1.0), (R(1.0) + cmx_ipow<3>(cmx_div(ICaL__cat, g[9]))));
+    const R ICaL__k1 = (R(0.024168) * ICaL__fca);
+    const R ICaL__poi = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vx)) * (R(1) / ICaL__sx)), V, -0.4808983469629878, -19.235933878519514)));
+    const R ICaL__k3 = ((R(1.0) - ICaL__poi) * (R(1) / ICaL__tau3));
+    const R ICaL__k3t = ICaL__k3;
+    const R ICaL__Ps = cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vyr)) * (R(1) / ICaL__syr)), V, -0.12744655838241728, -5.09786233529669)));
+    const R ICaL__Pr = (R(1.0) - cmx_div(R(1.0), (R(1.0) + cmx_exp_v(((-(V - ICaL__vy)) * (R(1) / ICaL__sy)), V, -0.36067376022224085, -14.426950408889635))));
+    const R ICaL__recov = (R(10.0) + (R(4954.0) * cmx_exp_v((V * (R(1) / R(15.6))), V, 0.09248045133903611, 0.0)));
+    const R ICaL__tau_ca = (cmx_div(ICaL__tca, (R(1.0) + cmx_ipow<4>((g[9] * (R(1) / ICaL__cpt))))) + R(0.1));
+    const R ICaL__tauca = (((ICaL__recov - ICaL__tau_ca) * ICaL__Pr) + ICaL__tau_ca);
+    const R ICaL__k5 = cmx_div((R(1.0) - ICaL__Ps), ICaL__tauca);
+    const R ICaL__k6 = cmx_div((ICaL__fca * ICaL__Ps), ICaL__tauca);
+    const R ICaL__k4 = cmx_div((((cmx_div((ICaL__k3 * ICaL__alpha), ICaL__beta) * ICaL__k1) * (R(1) / ICaL__k2)) * ICaL__k5), ICaL__k6);
+    const R ICaL__tauba = (((ICaL__recov - R(450.0)) * ICaL__Pr) + R(450.0));
+    const R ICaL__k5t = cmx_div((R(1.0) - ICaL__Ps), ICaL__tauba);
+    const R ICaL__k6t = cmx_div(ICaL__Ps, ICaL__tauba);
+    const R ICaL__k4t = cmx_div((((cmx_div((ICaL__k3t * ICaL__alpha), ICaL__beta) * ICaL__k1t) * (R(1) / ICaL__k2t)) * ICaL__k5t), ICaL__k6t);
+    const R ICaL__s1 = (R(0.0182688) * ICaL__fca);
+    const R ICaL__s2 = cmx_div((((ICaL__s1 * ICaL__r1) * (R(1) / ICaL__r2)) * ICaL__k2), ICaL__k1);
+    const R ICaL__s2t = ((((ICaL__s1t * ICaL__r1) * (R(1) / ICaL__r2)) * ICaL__k2t) * (R(1) / ICaL__k1t));
+    const R reversal_potentials__ena = ((R(1.0) * (R(1) / Environment__FonRT)) * cmx_log(cmx_div(Environment__Na_o, g[8])));
+    const R INa__xina = (p[0] * y[2] * y[3] * y[1] * y[1] * y[1] * (V - reversal_potentials__ena));
+    const R INaCa__aloss = cmx_div(R(1.0), (R(1.0) + cmx_ipow<3>(cmx_div(INaCa__xkdna, g[10]))));
+    const R INaCa__zw3 = ((cmx_ipow<3>(g[8]) * Environment__Ca_o * cmx_exp_v((V * R(0.35) * Environment__FonRT), V, 0.019024731229383713, 0.0)) - (cmx_ipow<3>(Environment__Na_o) * Ca__csm * cmx_exp_v((V * (R(0.35) - R(1.0)) * Environment__FonRT), V, -0.035331643711712615, -0.0)));
+    const R INaCa__zw4 = (R(1.0) + (R(0.2) * cmx_exp_v((V * (R(0.35) - R(1.0)) * Environment__FonRT), V, -0.035331643711712615, -0.0)));
+    const R INaCa__yz1 = ((INaCa__xmcao * cmx_ipow<3>(g[8])) + (cmx_ipow<3>(INaCa__xmnao) * Ca__csm));
+    const R INaCa__yz2 = (cmx_ipow<3>(INaCa__xmnai) * Environment__Ca_o * (R(1.0) + (Ca__csm * (R(1) / INaCa__xmcai))));
+    const R INaCa__yz3 = (INaCa__xmcai * cmx_ipow<3>(Environment__Na_o) * (R(1.0) + cmx_ipow<3>((g[8] * (R(1) / INaCa__xmnai)))));
+    const R INaCa__yz4 = ((cmx_ipow<3>(g[8]) * Environment__Ca_o) + (cmx_ipow<3>(Environment__Na_o) * Ca__csm));
+    const R INaCa__zw8 = (INaCa__yz1 + INaCa__yz2 + INaCa__yz3 + INaCa__yz4);
+    const R INaCa__jNaCa = cmx_div((INaCa__gNaCa * INaCa__aloss * INaCa__zw3), (INaCa__zw4 * INaCa__zw8));
+    const R INaCa__xiNaCa = (cell__wca * INaCa__jNaCa);
+    const R INaK__sigma = (cmx_expm1((Environment__Na_o * (R(1) / R(67.3)))) * (R(1) / R(7.0)));
+    const R INaK__fNaK = cmx_div(R(1.0), (R(1.0) + (R(0.1245) * cmx_exp_v(((-R(0.1)) * V * Environment__FonRT), V, -0.005435637494109632, -0.0)) + (R(0.0365) * INaK__sigma * cmx_exp_v(((-V) * Environment__FonRT), V, -0.05435637494109633, -0.0))));
+    const R INaK__xiNaK = ((cmx_div((INaK__gNaK * INaK__fNaK * g[8]), (g[8] + INaK__xkmnai)) * Environment__K_o) * (R(1) / (Environment__K_o + INaK__xkmko)));
+    const R Irel__bv = (((R(1.0) - Irel__av) * Irel__cstar) - R(50.0));
+    const R Irel__Qr0 = ((((g[6] > R(50.0)) && (g[6] < Irel__cstar))) ? (((g[6] - R(50.0)) * (R(1) / R(1.0)))) : ((((g[6] >= Irel__cstar)) ? (((Irel__av * g[6]) + Irel__bv)) : (R(0.0)))));
+    const R Irel__Qr = ((g[12] * Irel__Qr0) * (R(1) / Irel__cstar));
+    const R Irel__sparkV = cmx_div(cmx_exp_v(((-Irel__ay) * (V + R(30.0))), V, -0.07213475204444818, -2.164042561333445), (R(1.0) + cmx_exp_v(((-Irel__ay) * (V + R(30.0))), V, -0.07213475204444818, -2.164042561333445)));
+    const R Irel__spark_rate = ((Irel__gryr * (R(1) / R(1.0))) * ICaL__po * fabs(ICaL__rxa) * Irel__sparkV);
+    const R Irel__xicap = (ICaL__po * Irel__gdyad * fabs(ICaL__rxa));
+    const R Irel__xirp = cmx_div((((ICaL__po * Irel__Qr * fabs(ICaL__rxa) * Irel__gbarsr) * (R(1) / R(1.0))) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R Irel__xiryr = (Irel__xirp + Irel__xicap);
+    const R d_Ileak_Iup_Ixfer__jleak__d11 = (((-R(1)) * cmx_div((Ileak_Iup_Ixfer__gleak * g[12] * g[12]), ((g[12] * g[12]) + (Ileak_Iup_Ixfer__kj * Ileak_Iup_Ixfer__kj)))));
+    const R d_Ileak_Iup_Ixfer__jleak__d12 = ((cmx_div((((R(1) * Ileak_Iup_Ixfer__gleak * g[12]) + (R(1) * Ileak_Iup_Ixfer__gleak * g[12])) * ((g[12] * g[12]) + (Ileak_Iup_Ixfer__kj * Ileak_Iup_Ixfer__kj)) - (Ileak_Iup_Ixfer__gleak * g[12] * g[12]) * (((R(1) * g[12]) + (R(1) * g[12])))), (((g[12] * g[12]) + (Ileak_Iup_Ixfer__kj * Ileak_Iup_Ixfer__kj)) * ((g[12] * g[12]) + (Ileak_Iup_Ixfer__kj * Ileak_Iup_Ixfer__kj)))) * ((g[12] * R(16.667)) - g[11])) + (((R(1) * R(16.667))) * cmx_div((Ileak_Iup_Ixfer__gleak * g[12] * g[12]), ((g[12] * g[12]) + (Ileak_Iup_Ixfer__kj * Ileak_Iup_Ixfer__kj)))));
+    const R d_Ileak_Iup_Ixfer__jup__d11 = cmx_div((((R(1) * Ileak_Iup_Ixfer__vup * g[11]) + (R(1) * Ileak_Iup_Ixfer__vup * g[11])) * ((g[11] * g[11]) + (Ileak_Iup_Ixfer__cup * Ileak_Iup_Ixfer__cup)) - (Ileak_Iup_Ixfer__vup * g[11] * g[11]) * (((R(1) * g[11]) + (R(1) * g[11])))), (((g[11] * g[11]) + (Ileak_Iup_Ixfer__cup * Ileak_Iup_Ixfer__cup)) * ((g[11] * g[11]) + (Ileak_Iup_Ixfer__cup * Ileak_Iup_Ixfer__cup))));
+    const R d_Ca__dCa_JSR__d7 = ((-R(1)));
+    const R d_Ca__dCa_JSR__d11 = ((d_Ileak_Iup_Ixfer__jup__d11) - d_Ileak_Iup_Ixfer__jleak__d11);
+    const R d_Ca__dCa_JSR__d12 = (-d_Ileak_Iup_Ixfer__jleak__d12);
+    const R d_Ca__bpxi__d11 = cmx_div((-((Ca__bcal * Ca__xkcal) * (((R(1)) * (Ca__xkcal + g[11])) + ((R(1)) * (Ca__xkcal + g[11]))))), (((Ca__xkcal + g[11]) * (Ca__xkcal + g[11])) * ((Ca__xkcal + g[11]) * (Ca__xkcal + g[11]))));
+    const R d_Ca__mempxi__d11 = cmx_div((-((Ca__bmem * Ca__kmem) * (((R(1)) * (Ca__kmem + g[11])) + ((R(1)) * (Ca__kmem + g[11]))))), (((Ca__kmem + g[11]) * (Ca__kmem + g[11])) * ((Ca__kmem + g[11]) * (Ca__kmem + g[11]))));
+    const R d_Ca__sarpxi__d11 = cmx_div((-((Ca__bsar * Ca__ksar) * (((R(1)) * (Ca__ksar + g[11])) + ((R(1)) * (Ca__ksar + g[11]))))), (((Ca__ksar + g[11]) * (Ca__ksar + g[11])) * ((Ca__ksar + g[11]) * (Ca__ksar + g[11]))));
+    const R d_Ca__spxi__d11 = cmx_div((-((Ca__srmax * Ca__srkd) * (((R(1)) * (Ca__srkd + g[11])) + ((R(1)) * (Ca__srkd + g[11]))))), (((Ca__srkd + g[11]) * (Ca__srkd + g[11])) * ((Ca__srkd + g[11]) * (Ca__srkd + g[11]))));
+    const R d_Ca__dciib__d11 = cmx_div((-(R(1.0) * (d_Ca__bpxi__d11 + d_Ca__spxi__d11 + d_Ca__mempxi__d11 + d_Ca__sarpxi__d11))), ((R(1.0) + Ca__bpxi + Ca__spxi + Ca__mempxi + Ca__sarpxi) * (R(1.0) + Ca__bpxi + Ca__spxi + Ca__mempxi + Ca__sarpxi)));
+    const R d_Ca__bpxs__d10 = cmx_div((-((Ca__bcal * Ca__xkcal) * (((R(1)) * (Ca__xkcal + g[10])) + ((R(1)) * (Ca__xkcal + g[10]))))), (((Ca__xkcal + g[10]) * (Ca__xkcal + g[10])) * ((Ca__xkcal + g[10]) * (Ca__xkcal + g[10]))));
+    const R d_Ca__mempxs__d10 = cmx_div((-((Ca__bmem * Ca__kmem) * (((R(1)) * (Ca__kmem + g[10])) + ((R(1)) * (Ca__kmem + g[10]))))), (((Ca__kmem + g[10]) * (Ca__kmem + g[10])) * ((Ca__kmem + g[10]) * (Ca__kmem + g[10]))));
+    const R d_Ca__sarpxs__d10 = cmx_div((-((Ca__bsar * Ca__ksar) * (((R(1)) * (Ca__ksar + g[10])) + ((R(1)) * (Ca__ksar + g[10]))))), (((Ca__ksar + g[10]) * (Ca__ksar + g[10])) * ((Ca__ksar + g[10]) * (Ca__ksar + g[10]))));
+    const R d_Ca__spxs__d10 = cmx_div((-((Ca__srmax * Ca__srkd) * (((R(1)) * (Ca__srkd + g[10])) + ((R(1)) * (Ca__srkd + g[10]))))), (((Ca__srkd + g[10]) * (Ca__srkd + g[10])) * ((Ca__srkd + g[10]) * (Ca__srkd + g[10]))));
+    const R d_Ca__dcsib__d10 = cmx_div((-(R(1.0) * (d_Ca__bpxs__d10 + d_Ca__spxs__d10 + d_Ca__mempxs__d10 + d_Ca__sarpxs__d10))), ((R(1.0) + Ca__bpxs + Ca__spxs + Ca__mempxs + Ca__sarpxs) * (R(1.0) + Ca__bpxs + Ca__spxs + Ca__mempxs + Ca__sarpxs)));
+    const R d_Ca__jd__d10 = cmx_div(R(1), Ca__taud);
+    const R d_Ca__jd__d11 = cmx_div((-R(1)), Ca__taud);
+    const R d_Ca__xbi__d11 = ((R(1) * Ca__xkon * (Ca__btrop - g[13])));
+    const R d_Ca__xbi__d13 = ((((-R(1)) * Ca__xkon * g[11])) - ((R(1) * Ca__xkoff)));
+    const R d_Ca__xbs__d10 = ((R(1) * Ca__xkon * (Ca__btrop - g[14])));
+    const R d_Ca__xbs__d14 = ((((-R(1)) * Ca__xkon * g[10])) - ((R(1) * Ca__xkoff)));
+    const R d_ICaL__po__d0 = (-R(1));
+    const R d_ICaL__po__d1 = (-R(1));
+    const R d_ICaL__po__d2 = (-R(1));
+    const R d_ICaL__po__d3 = (-R(1));
+    const R d_ICaL__po__d4 = (-R(1));
+    const R d_ICaL__po__d5 = (-R(1));
+    const R d_Ca__csm__d10 = cmx_div(R(1), R(1000.0));
+    const R d_ICaL__rxa__d10 = (((fabs(ICaL__za) < R(0.001))) ? (cmx_div(((((d_Ca__csm__d10 * cmx_exp(ICaL__za))) * R(4.0) * ICaL__pca * Environment__F * Environment__FonRT)), (R(2.0) * Environment__FonRT))) : (cmx_div(((((d_Ca__csm__d10 * cmx_exp(ICaL__za))) * R(4.0) * ICaL__pca * V * Environment__F * Environment__FonRT)), cmx_expm1(ICaL__za))));
+    const R d_ICaL__jca__d0 = ((d_ICaL__po__d0 * p[1] * ICaL__rxa));
+    const R d_ICaL__jca__d1 = ((d_ICaL__po__d1 * p[1] * ICaL__rxa));
+    const R d_ICaL__jca__d2 = ((d_ICaL__po__d2 * p[1] * ICaL__rxa));
+    const R d_ICaL__jca__d3 = ((d_ICaL__po__d3 * p[1] * ICaL__rxa));
+    const R d_ICaL__jca__d4 = ((d_ICaL__po__d4 * p[1] * ICaL__rxa));
+    const R d_ICaL__jca__d5 = ((d_ICaL__po__d5 * p[1] * ICaL__rxa));
+    const R d_ICaL__jca__d10 = ((d_ICaL__rxa__d10 * p[1] * ICaL__po));
+    const R d_ICaL__fca__d9 = cmx_div((-(R(1.0) * ((R(3.0) * cmx_ipow<2>(cmx_div(ICaL__cat, g[9])) * cmx_div((-(ICaL__cat * R(1))), (g[9] * g[9])))))), ((R(1.0) + cmx_ipow<3>(cmx_div(ICaL__cat, g[9]))) * (R(1.0) + cmx_ipow<3>(cmx_div(ICaL__cat, g[9])))));
+    const R d_ICaL__k1__d9 = ((d_ICaL__fca__d9 * R(0.024168)));
+    const R d_ICaL__tau_ca__d9 = (cmx_div((-(ICaL__tca * ((R(4.0) * cmx_ipow<3>((g[9] * (R(1) / ICaL__cpt))) * cmx_div(R(1), ICaL__cpt))))), ((R(1.0) + cmx_ipow<4>((g[9] * (R(1) / ICaL__cpt)))) * (R(1.0) + cmx_ipow<4>((g[9] * (R(1) / ICaL__cpt)))))));
+    const R d_ICaL__tauca__d9 = ((((-d_ICaL__tau_ca__d9) * ICaL__Pr)) + d_ICaL__tau_ca__d9);
+    const R d_ICaL__k5__d9 = cmx_div((-((R(1.0) - ICaL__Ps) * d_ICaL__tauca__d9)), (ICaL__tauca * ICaL__tauca));
+    const R d_ICaL__k6__d9 = cmx_div((((d_ICaL__fca__d9 * ICaL__Ps)) * ICaL__tauca - (ICaL__fca * ICaL__Ps) * d_ICaL__tauca__d9), (ICaL__tauca * ICaL__tauca));
+    const R d_ICaL__k4__d9 = cmx_div((((cmx_div(((d_ICaL__k1__d9 * cmx_div((ICaL__k3 * ICaL__alpha), ICaL__beta))), ICaL__k2) * ICaL__k5) + (d_ICaL__k5__d9 * ((cmx_div((ICaL__k3 * ICaL__alpha), ICaL__beta) * ICaL__k1) * (R(1) / ICaL__k2)))) * ICaL__k6 - (((cmx_div((ICaL__k3 * ICaL__alpha), ICaL__beta) * ICaL__k1) * (R(1) / ICaL__k2)) * ICaL__k5) * d_ICaL__k6__d9), (ICaL__k6 * ICaL__k6));
+    const R d_ICaL__s1__d9 = ((d_ICaL__fca__d9 * R(0.0182688)));
+    const R d_ICaL__s2__d9 = cmx_div((((cmx_div(((d_ICaL__s1__d9 * ICaL__r1)), ICaL__r2) * ICaL__k2)) * ICaL__k1 - (((ICaL__s1 * ICaL__r1) * (R(1) / ICaL__r2)) * ICaL__k2) * d_ICaL__k1__d9), (ICaL__k1 * ICaL__k1));
+    const R d_reversal_potentials__ena__d8 = ((cmx_div(cmx_div((-(Environment__Na_o * R(1))), (g[8] * g[8])), cmx_div(Environment__Na_o, g[8])) * (R(1.0) * (R(1) / Environment__FonRT))));
+    const R d_INa__xina__d8 = (((-d_reversal_potentials__ena__d8) * p[0] * y[2] * y[3] * y[1] * y[1] * y[1]));
+    const R d_INaCa__aloss__d10 = cmx_div((-(R(1.0) * ((R(3.0) * cmx_ipow<2>(cmx_div(INaCa__xkdna, g[10])) * cmx_div((-(INaCa__xkdna * R(1))), (g[10] * g[10])))))), ((R(1.0) + cmx_ipow<3>(cmx_div(INaCa__xkdna, g[10]))) * (R(1.0) + cmx_ipow<3>(cmx_div(INaCa__xkdna, g[10])))));
+    const R d_INaCa__zw3__d8 = (((R(3.0) * cmx_ipow<2>(g[8]) * R(1)) * Environment__Ca_o * cmx_exp_v((V * R(0.35) * Environment__FonRT), V, 0.019024731229383713, 0.0)));
+    const R d_INaCa__zw3__d10 = (-((d_Ca__csm__d10 * cmx_ipow<3>(Environment__Na_o) * cmx_exp_v((V * (R(0.35) - R(1.0)) * Environment__FonRT), V, -0.035331643711712615, -0.0))));
+    const R d_INaCa__yz1__d8 = ((((R(3.0) * cmx_ipow<2>(g[8]) * R(1)) * INaCa__xmcao)));
+    const R d_INaCa__yz1__d10 = (((d_Ca__csm__d10 * cmx_ipow<3>(INaCa__xmnao))));
+    const R d_INaCa__yz2__d10 = (((cmx_div(d_Ca__csm__d10, INaCa__xmcai)) * cmx_ipow<3>(INaCa__xmnai) * Environment__Ca_o));
+    const R d_INaCa__yz3__d8 = ((((R(3.0) * cmx_ipow<2>((g[8] * (R(1) / INaCa__xmnai))) * cmx_div(R(1), INaCa__xmnai))) * INaCa__xmcai * cmx_ipow<3>(Environment__Na_o)));
+    const R d_INaCa__yz4__d8 = ((((R(3.0) * cmx_ipow<2>(g[8]) * R(1)) * Environment__Ca_o)));
+    const R d_INaCa__yz4__d10 = (((d_Ca__csm__d10 * cmx_ipow<3>(Environment__Na_o))));
+    const R d_INaCa__zw8__d8 = (d_INaCa__yz1__d8 + d_INaCa__yz3__d8 + d_INaCa__yz4__d8);
+    const R d_INaCa__zw8__d10 = (d_INaCa__yz1__d10 + d_INaCa__yz2__d10 + d_INaCa__yz4__d10);
+    const R d_INaCa__jNaCa__d8 = cmx_div((((d_INaCa__zw3__d8 * INaCa__gNaCa * INaCa__aloss)) * (INaCa__zw4 * INaCa__zw8) - (INaCa__gNaCa * INaCa__aloss * INaCa__zw3) * ((d_INaCa__zw8__d8 * INaCa__zw4))), ((INaCa__zw4 * INaCa__zw8) * (INaCa__zw4 * INaCa__zw8)));
+    const R d_INaCa__jNaCa__d10 = cmx_div((((d_INaCa__aloss__d10 * INaCa__gNaCa * INaCa__zw3) + (d_INaCa__zw3__d10 * INaCa__gNaCa * INaCa__aloss)) * (INaCa__zw4 * INaCa__zw8) - (INaCa__gNaCa * INaCa__aloss * INaCa__zw3) * ((d_INaCa__zw8__d10 * INaCa__zw4))), ((INaCa__zw4 * INaCa__zw8) * (INaCa__zw4 * INaCa__zw8)));
+    const R d_INaCa__xiNaCa__d8 = ((d_INaCa__jNaCa__d8 * cell__wca));
+    const R d_INaCa__xiNaCa__d10 = ((d_INaCa__jNaCa__d10 * cell__wca));
+    const R d_INaK__xiNaK__d8 = cmx_div(((cmx_div((((R(1) * INaK__gNaK * INaK__fNaK)) * (g[8] + INaK__xkmnai) - (INaK__gNaK * INaK__fNaK * g[8]) * (R(1))), ((g[8] + INaK__xkmnai) * (g[8] + INaK__xkmnai))) * Environment__K_o)), (Environment__K_o + INaK__xkmko));
+    const R d_Irel__Qr0__d6 = ((((g[6] > R(50.0)) && (g[6] < Irel__cstar))) ? (cmx_div(R(1), R(1.0))) : ((((g[6] >= Irel__cstar)) ? ((((R(1) * Irel__av)))) : (R(0)))));
+    const R d_Irel__Qr__d6 = cmx_div(((d_Irel__Qr0__d6 * g[12])), Irel__cstar);
+    const R d_Irel__Qr__d12 = cmx_div(((R(1) * Irel__Qr0)), Irel__cstar);
+    const R d_Irel__spark_rate__d0 = ((d_ICaL__po__d0 * (Irel__gryr * (R(1) / R(1.0))) * fabs(ICaL__rxa) * Irel__sparkV));
+    const R d_Irel__spark_rate__d1 = ((d_ICaL__po__d1 * (Irel__gryr * (R(1) / R(1.0))) * fabs(ICaL__rxa) * Irel__sparkV));
+    const R d_Irel__spark_rate__d2 = ((d_ICaL__po__d2 * (Irel__gryr * (R(1) / R(1.0))) * fabs(ICaL__rxa) * Irel__sparkV));
+    const R d_Irel__spark_rate__d3 = ((d_ICaL__po__d3 * (Irel__gryr * (R(1) / R(1.0))) * fabs(ICaL__rxa) * Irel__sparkV));
+    const R d_Irel__spark_rate__d4 = ((d_ICaL__po__d4 * (Irel__gryr * (R(1) / R(1.0))) * fabs(ICaL__rxa) * Irel__sparkV));
+    const R d_Irel__spark_rate__d5 = ((d_ICaL__po__d5 * (Irel__gryr * (R(1) / R(1.0))) * fabs(ICaL__rxa) * Irel__sparkV));
+    const R d_Irel__spark_rate__d10 = ((((ICaL__rxa < R(0)) ? (-d_ICaL__rxa__d10) : (d_ICaL__rxa__d10)) * (Irel__gryr * (R(1) / R(1.0))) * ICaL__po * Irel__sparkV));
+    const R d_Irel__xicap__d0 = ((d_ICaL__po__d0 * Irel__gdyad * fabs(ICaL__rxa)));
+    const R d_Irel__xicap__d1 = ((d_ICaL__po__d1 * Irel__gdyad * fabs(ICaL__rxa)));
+    const R d_Irel__xicap__d2 = ((d_ICaL__po__d2 * Irel__gdyad * fabs(ICaL__rxa)));
+    const R d_Irel__xicap__d3 = ((d_ICaL__po__d3 * Irel__gdyad * fabs(ICaL__rxa)));
+    const R d_Irel__xicap__d4 = ((d_ICaL__po__d4 * Irel__gdyad * fabs(ICaL__rxa)));
+    const R d_Irel__xicap__d5 = ((d_ICaL__po__d5 * Irel__gdyad * fabs(ICaL__rxa)));
+    const R d_Irel__xicap__d10 = ((((ICaL__rxa < R(0)) ? (-d_ICaL__rxa__d10) : (d_ICaL__rxa__d10)) * ICaL__po * Irel__gdyad));
+    const R d_Irel__xirp__d0 = cmx_div(((cmx_div(((d_ICaL__po__d0 * Irel__Qr * fabs(ICaL__rxa) * Irel__gbarsr)), R(1.0)) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799))), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R d_Irel__xirp__d1 = cmx_div(((cmx_div(((d_ICaL__po__d1 * Irel__Qr * fabs(ICaL__rxa) * Irel__gbarsr)), R(1.0)) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799))), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R d_Irel__xirp__d2 = cmx_div(((cmx_div(((d_ICaL__po__d2 * Irel__Qr * fabs(ICaL__rxa) * Irel__gbarsr)), R(1.0)) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799))), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R d_Irel__xirp__d3 = cmx_div(((cmx_div(((d_ICaL__po__d3 * Irel__Qr * fabs(ICaL__rxa) * Irel__gbarsr)), R(1.0)) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799))), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R d_Irel__xirp__d4 = cmx_div(((cmx_div(((d_ICaL__po__d4 * Irel__Qr * fabs(ICaL__rxa) * Irel__gbarsr)), R(1.0)) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799))), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R d_Irel__xirp__d5 = cmx_div(((cmx_div(((d_ICaL__po__d5 * Irel__Qr * fabs(ICaL__rxa) * Irel__gbarsr)), R(1.0)) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799))), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R d_Irel__xirp__d6 = cmx_div(((cmx_div(((d_Irel__Qr__d6 * ICaL__po * fabs(ICaL__rxa) * Irel__gbarsr)), R(1.0)) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799))), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R d_Irel__xirp__d10 = cmx_div(((cmx_div(((((ICaL__rxa < R(0)) ? (-d_ICaL__rxa__d10) : (d_ICaL__rxa__d10)) * ICaL__po * Irel__Qr * Irel__gbarsr)), R(1.0)) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799))), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R d_Irel__xirp__d12 = cmx_div(((cmx_div(((d_Irel__Qr__d12 * ICaL__po * fabs(ICaL__rxa) * Irel__gbarsr)), R(1.0)) * cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799))), (R(1.0) + cmx_exp_v(((-Irel__ax) * (V + R(30.0))), V, -0.5159077466218933, -15.477232398656799)));
+    const R d_Irel__xiryr__d0 = (d_Irel__xirp__d0 + d_Irel__xicap__d0);
+    const R d_Irel__xiryr__d1 = (d_Irel__xirp__d1 + d_Irel__xicap__d1);
+    const R d_Irel__xiryr__d2 = (d_Irel__xirp__d2 + d_Irel__xicap__d2);
+    const R d_Irel__xiryr__d3 = (d_Irel__xirp__d3 + d_Irel__xicap__d3);
+    const R d_Irel__xiryr__d4 = (d_Irel__xirp__d4 + d_Irel__xicap__d4);
+    const R d_Irel__xiryr__d5 = (d_Irel__xirp__d5 + d_Irel__xicap__d5);
+    const R d_Irel__xiryr__d6 = (d_Irel__xirp__d6);
+    const R d_Irel__xiryr__d10 = (d_Irel__xirp__d10 + d_Irel__xicap__d10);
+    const R d_Irel__xiryr__d12 = (d_Irel__xirp__d12);
+    J[0] = R(1) - dt * ((((d_ICaL__po__d0 * ICaL__r2))) - ((R(1) * (ICaL__beta + ICaL__r1 + ICaL__k1t + ICaL__k1))));
+    J[1] = R(0) - dt * (((R(1) * ICaL__alpha)) + ((d_ICaL__po__d1 * ICaL__r2)));
+    J[2] = R(0) - dt * (((R(1) * ICaL__k2)) + ((d_ICaL__po__d2 * ICaL__r2)));
+    J[3] = R(0) - dt * (((R(1) * ICaL__k2t)) + ((d_ICaL__po__d3 * ICaL__r2)));
+    J[4] = R(0) - dt * (((d_ICaL__po__d4 * ICaL__r2)));
+    J[5] = R(0) - dt * (((d_ICaL__po__d5 * ICaL__r2)));
+    J[6] = R(0);
+    J[7] = R(0);
+    J[8] = R(0);
+    J[9] = R(0) - dt * (-(((d_ICaL__k1__d9) * g[0])));
+    J[10] = R(0);
+    J[11] = R(0);
+    J[12] = R(0);
+    J[13] = R(0);
+    J[14] = R(0);
+    J[15] = R(0) - dt * (((R(1) * ICaL__beta)));
+    J[16] = R(1) - dt * (-((R(1) * (ICaL__k6 + ICaL__k6t + ICaL__alpha))));
+    J[17] = R(0);
+    J[18] = R(0);
+    J[19] = R(0) - dt * (((R(1) * ICaL__k5)));
+    J[20] = R(0) - dt * (((R(1) * ICaL__k5t)));
+    J[21] = R(0);
+    J[22] = R(0);
+    J[23] = R(0);
+    J[24] = R(0) - dt * ((((d_ICaL__k5__d9 * g[4]))) - (((d_ICaL__k6__d9) * g[1])));
+    J[25] = R(0);
+    J[26] = R(0);
+    J[27] = R(0);
+    J[28] = R(0);
+    J[29] = R(0);
+    J[30] = R(0) - dt * (((R(1) * ICaL__k1)) + ((d_ICaL__po__d0 * ICaL__s1)));
+    J[31] = R(0) - dt * (((d_ICaL__po__d1 * ICaL__s1)));
+    J[32] = R(1) - dt * ((((d_ICaL__po__d2 * ICaL__s1))) - ((R(1) * (ICaL__k3 + ICaL__k2 + ICaL__s2))));
+    J[33] = R(0) - dt * (((d_ICaL__po__d3 * ICaL__s1)));
+    J[34] = R(0) - dt * (((R(1) * ICaL__k4)) + ((d_ICaL__po__d4 * ICaL__s1)));
+    J[35] = R(0) - dt * (((d_ICaL__po__d5 * ICaL__s1)));
+    J[36] = R(0);
+    J[37] = R(0);
+    J[38] = R(0);
+    J[39] = R(0) - dt * ((((d_ICaL__k1__d9 * g[0])) + ((d_ICaL__k4__d9 * g[4])) + ((d_ICaL__s1__d9 * ICaL__po))) - (((d_ICaL__s2__d9) * g[2])));
+    J[40] = R(0);
+    J[41] = R(0);
+    J[42] = R(0);
+    J[43] = R(0);
+    J[44] = R(0);
+    J[45] = R(0) - dt * (((R(1) * ICaL__k1t)) + ((d_ICaL__po__d0 * ICaL__s1t)));
+    J[46] = R(0) - dt * (((d_ICaL__po__d1 * ICaL__s1t)));
+    J[47] = R(0) - dt * (((d_ICaL__po__d2 * ICaL__s1t)));
+    J[48] = R(1) - dt * ((((d_ICaL__po__d3 * ICaL__s1t))) - ((R(1) * (ICaL__k3t + ICaL__k2t + ICaL__s2t))));
+    J[49] = R(0) - dt * (((d_ICaL__po__d4 * ICaL__s1t)));
+    J[50] = R(0) - dt * (((R(1) * ICaL__k4t)) + ((d_ICaL__po__d5 * ICaL__s1t)));
+    J[51] = R(0);
+    J[52] = R(0);
+    J[53] = R(0);
+    J[54] = R(0);
+    J[55] = R(0);
+    J[56] = R(0);
+    J[57] = R(0);
+    J[58] = R(0);
+    J[59] = R(0);
+    J[60] = R(0);
+    J[61] = R(0) - dt * (((R(1) * ICaL__k6)));
+    J[62] = R(0) - dt * (((R(1) * ICaL__k3)));
+    J[63] = R(0);
+    J[64] = R(1) - dt * (-((R(1) * (ICaL__k5 + ICaL__k4))));
+    J[65] = R(0);
+    J[66] = R(0);
+    J[67] = R(0);
+    J[68] = R(0);
+    J[69] = R(0) - dt * ((((d_ICaL__k6__d9 * g[1]))) - (((d_ICaL__k5__d9 + d_ICaL__k4__d9) * g[4])));
+    J[70] = R(0);
+    J[71] = R(0);
+    J[72] = R(0);
+    J[73] = R(0);
+    J[74] = R(0);
+    J[75] = R(0);
+    J[76] = R(0) - dt * (((R(1) * ICaL__k6t)));
+    J[77] = R(0);
+    J[78] = R(0) - dt * (((R(1) * ICaL__k3t)));
+    J[79] = R(0);
+    J[80] = R(1) - dt * (-((R(1) * (ICaL__k5t + ICaL__k4t))));
+    J[81] = R(0);
+    J[82] = R(0);
+    J[83] = R(0);
+    J[84] = R(0);
+    J[85] = R(0);
+    J[86] = R(0);
+    J[87] = R(0);
+    J[88] = R(0);
+    J[89] = R(0);
+    J[90] = R(0);
+    J[91] = R(0);
+    J[92] = R(0);
+    J[93] = R(0);
+    J[94] = R(0);
+    J[95] = R(0);
+    J[96] = R(1) - dt * cmx_div((-R(1)), Irel__taua);
+    J[97] = R(0);
+    J[98] = R(0);
+    J[99] = R(0);
+    J[100] = R(0);
+    J[101] = R(0);
+    J[102] = R(0) - dt * cmx_div(R(1), Irel__taua);
+    J[103] = R(0);
+    J[104] = R(0);
+    J[105] = R(0) - dt * ((d_Irel__spark_rate__d0 * Irel__Qr));
+    J[106] = R(0) - dt * ((d_Irel__spark_rate__d1 * Irel__Qr));
+    J[107] = R(0) - dt * ((d_Irel__spark_rate__d2 * Irel__Qr));
+    J[108] = R(0) - dt * ((d_Irel__spark_rate__d3 * Irel__Qr));
+    J[109] = R(0) - dt * ((d_Irel__spark_rate__d4 * Irel__Qr));
+    J[110] = R(0) - dt * ((d_Irel__spark_rate__d5 * Irel__Qr));
+    J[111] = R(0) - dt * ((d_Irel__Qr__d6 * Irel__spark_rate));
+    J[112] = R(1) - dt * (-cmx_div(((R(1) * (R(1.0) - cmx_div((Irel__taur * Ca__dCa_JSR), g[12]))) + ((-cmx_div(((d_Ca__dCa_JSR__d7 * Irel__taur)), g[12])) * g[7])), Irel__taur));
+    J[113] = R(0);
+    J[114] = R(0);
+    J[115] = R(0) - dt * ((d_Irel__spark_rate__d10 * Irel__Qr));
+    J[116] = R(0) - dt * (-cmx_div((((-cmx_div(((d_Ca__dCa_JSR__d11 * Irel__taur)), g[12])) * g[7])), Irel__taur));
+    J[117] = R(0) - dt * (((d_Irel__Qr__d12 * Irel__spark_rate)) - cmx_div((((-cmx_div((((d_Ca__dCa_JSR__d12 * Irel__taur)) * g[12] - (Irel__taur * Ca__dCa_JSR) * R(1)), (g[12] * g[12]))) * g[7])), Irel__taur));
+    J[118] = R(0);
+    J[119] = R(0);
+    J[120] = R(0);
+    J[121] = R(0);
+    J[122] = R(0);
+    J[123] = R(0);
+    J[124] = R(0);
+    J[125] = R(0);
+    J[126] = R(0);
+    J[127] = R(0);
+    J[128] = R(1) - dt * cmx_div((-(d_INa__xina__d8 + ((d_INaK__xiNaK__d8 * R(3.0))) + ((d_INaCa__xiNaCa__d8 * R(3.0))))), (cell__wca * R(1000.0)));
+    J[129] = R(0);
+    J[130] = R(0) - dt * cmx_div((-(((d_INaCa__xiNaCa__d10 * R(3.0))))), (cell__wca * R(1000.0)));
+    J[131] = R(0);
+    J[132] = R(0);
+    J[133] = R(0);
+    J[134] = R(0);
+    J[135] = R(0) - dt * d_Irel__xiryr__d0;
+    J[136] = R(0) - dt * d_Irel__xiryr__d1;
+    J[137] = R(0) - dt * d_Irel__xiryr__d2;
+    J[138] = R(0) - dt * d_Irel__xiryr__d3;
+    J[139] = R(0) - dt * d_Irel__xiryr__d4;
+    J[140] = R(0) - dt * d_Irel__xiryr__d5;
+    J[141] = R(0) - dt * d_Irel__xiryr__d6;
+    J[142] = R(0);
+    J[143] = R(0);
+    J[144] = R(1) - dt * (-cmx_div(R(1), Ca__taups));
+    J[145] = R(0) - dt * (d_Irel__xiryr__d10 - cmx_div((-R(1)), Ca__taups));
+    J[146] = R(0);
+    J[147] = R(0) - dt * d_Irel__xiryr__d12;
+    J[148] = R(0);
+    J[149] = R(0);
+    J[150] = R(0) - dt * ((((((-d_ICaL__jca__d0)) * R(50.0))) * Ca__dcsib));
+    J[151] = R(0) - dt * ((((((-d_ICaL__jca__d1)) * R(50.0))) * Ca__dcsib));
+    J[152] = R(0) - dt * ((((((-d_ICaL__jca__d2)) * R(50.0))) * Ca__dcsib));
+    J[153] = R(0) - dt * ((((((-d_ICaL__jca__d3)) * R(50.0))) * Ca__dcsib));
+    J[154] = R(0) - dt * ((((((-d_ICaL__jca__d4)) * R(50.0))) * Ca__dcsib));
+    J[155] = R(0) - dt * ((((((-d_ICaL__jca__d5)) * R(50.0))) * Ca__dcsib));
+    J[156] = R(0);
+    J[157] = R(0) - dt * (((((R(1)) * R(50.0))) * Ca__dcsib));
+    J[158] = R(0) - dt * (((((d_INaCa__jNaCa__d8) * R(50.0))) * Ca__dcsib));
+    J[159] = R(0);
+    J[160] = R(1) - dt * ((d_Ca__dcsib__d10 * ((R(50.0) * (((g[7] - Ca__jd) - ICaL__jca) + INaCa__jNaCa)) - Ca__xbs)) + (((((((-d_Ca__jd__d10) - d_ICaL__jca__d10) + d_INaCa__jNaCa__d10) * R(50.0))) - d_Ca__xbs__d10) * Ca__dcsib));
+    J[161] = R(0) - dt * ((((((-d_Ca__jd__d11)) * R(50.0))) * Ca__dcsib));
+    J[162] = R(0);
+    J[163] = R(0);
+    J[164] = R(0) - dt * (((-d_Ca__xbs__d14) * Ca__dcsib));
+    J[165] = R(0);
+    J[166] = R(0);
+    J[167] = R(0);
+    J[168] = R(0);
+    J[169] = R(0);
+    J[170] = R(0);
+    J[171] = R(0);
+    J[172] = R(0);
+    J[173] = R(0);
+    J[174] = R(0);
+    J[175] = R(0) - dt * (((d_Ca__jd__d10) * Ca__dciib));
+    J[176] = R(1) - dt * ((d_Ca__dciib__d11 * (((Ca__jd - Ileak_Iup_Ixfer__jup) + Ileak_Iup_Ixfer__jleak) - Ca__xbi)) + ((((d_Ca__jd__d11 - d_Ileak_Iup_Ixfer__jup__d11) + d_Ileak_Iup_Ixfer__jleak__d11) - d_Ca__xbi__d11) * Ca__dciib));
+    J[177] = R(0) - dt * (((d_Ileak_Iup_Ixfer__jleak__d12) * Ca__dciib));
+    J[178] = R(0) - dt * (((-d_Ca__xbi__d13) * Ca__dciib));
+    J[179] = R(0);
+    J[180] = R(0);
+    J[181] = R(0);
+    J[182] = R(0);
+    J[183] = R(0);
+    J[184] = R(0);
+    J[185] = R(0);
+    J[186] = R(0);
+    J[187] = R(0) - dt * d_Ca__dCa_JSR__d7;
+    J[188] = R(0);
+    J[189] = R(0);
+    J[190] = R(0);
+    J[191] = R(0) - dt * d_Ca__dCa_JSR__d11;
+    J[192] = R(1) - dt * d_Ca__dCa_JSR__d12;
+    J[193] = R(0);
+    J[194] = R(0);
+    J[195] = R(0);
+    J[196] = R(0);
+    J[197] = R(0);
+    J[198] = R(0);
+    J[199] = R(0);
+    J[200] = R(0);
+    J[201] = R(0);
+    J[202] = R(0);
+    J[203] = R(0);
+    J[204] = R(0);
+    J[205] = R(0);
+    J[206] = R(0) - dt * d_Ca__xbi__d11;
+    J[207] = R(0);
+    J[208] = R(1) - dt * d_Ca__xbi__d13;
+    J[209] = R(0);
+    J[210] = R(0);
+    J[211] = R(0);
+    J[212] = R(0);
+    J[213] = R(0);
+    J[214] = R(0);
+    J[215] = R(0);
+    J[216] = R(0);
+    J[217] = R(0);
+    J[218] = R(0);
+    J[219] = R(0);
+    J[220] = R(0) - dt * d_Ca__xbs__d10;
+    J[221] = R(0);
+    J[222] = R(0);
+    J[223] = R(0);
+    J[224] = R(1) - dt * d_Ca__xbs__d14;
+}
+
+// One backward-Euler step of every state but V at the given V
+// (ComputeOneStepExceptVoltage): linear states in closed form, then the
+// nonlinear ones by Newton from their current values.  Returns the Newton
+// iteration count, -1 if it did not converge.
+template <class R>
+CMX_HD inline int be_step(R* y, R V, R i_stim_area, const R* p, R cm, R dt)
+{
+    be_linear<R>(y, V, p, cm, dt);
+    R g[kNumNonlinear];
+CMX_UNROLL
+    for (int i = 0; i < kNumNonlinear; ++i) g[i] = y[kNonlinear[i]];
+    const int its = cmx_newton<kNumNonlinear, R>(
+        g, [&](const R* x, R* r) { be_residual<R>(y, x, V, i_stim_area, p, cm, dt, r); },
+        [&](const R* x, R* J) { be_jacobian<R>(y, x, V, i_stim_area, p, cm, dt, J); });
+CMX_UNROLL
+    for (int i = 0; i < kNumNonlinear; ++i) y[kNonlinear[i]] = g[i];
+    return its;
+}
+
+// Rates of the full system: dV/dt (0 when V is clamped) and the states.
+template <class R>
+CMX_HD inline void grl_rates(const R* y, R i_stim_area, const R* p, R cm, bool update_v, R* f)
+{
+    R gi[kNumStates], gk[kNumStates];
+    evaluate<R, false>(y, y[0], i_stim_area, p, cm, f, gi, gk);
+    f[0] = update_v ? dvdt<R>(y, y[0], i_stim_area, p, cm) : R(0);
+}
+
+template <class R>
+CMX_HD inline R grl_advance(R y0, R f, R a, R dt)
+{
+    using std::fabs; using std::exp;
+    return fabs(a) < R(1e-8) ? y0 + f * dt : y0 + (f / a) * (exp(a * dt) - R(1));
+}
+
+// One generalized Rush-Larsen step of length dt (ORDER 1 or 2); V is
+// advanced too when update_v, else held.
+template <class R, int ORDER>
+CMX_HD inline void grl_step(R* y, R i_stim_area, const R* p, R cm, R dt, bool update_v)
+{
+    constexpr int S = kNumStates;
+    const R delta = R(1e-8);
+    R f[S], a[S], y0[S], t[S];
+    const int first = update_v ? 0 : 1;
+CMX_UNROLL
+    for (int i = 0; i < S; ++i) y0[i] = y[i];
+    grl_rates<R>(y, i_stim_area, p, cm, update_v, f);
+CMX_UNROLL
+    for (int i = 0; i < S; ++i) {
+        if (i < first) continue;
+        const R ys = y[i];
+        y[i] = ys + delta;
+        grl_rates<R>(y, i_stim_area, p, cm, update_v, t);
+        a[i] = (t[i] - f[i]) / delta;
+        y[i] = ys;
+    }
+    if constexpr (ORDER == 1) {
+CMX_UNROLL
+        for (int i = 0; i < S; ++i)
+            if (i >= first) y[i] = grl_advance<R>(y0[i], f[i], a[i], dt);
+        return;
+    }
+    R half = dt * R(0.5);
+CMX_UNROLL
+    for (int i = 0; i < S; ++i)
+        if (i >= first) y[i] = grl_advance<R>(y0[i], f[i], a[i], half);
+    R g[S];
+CMX_UNROLL
+    for (int i = 0; i < S; ++i) {
+        if (i < first) continue;
+        const R ym = y[i];
+        y[i] = y0[i];
+        grl_rates<R>(y, i_stim_area, p, cm, update_v, t);
+        g[i] = t[i];
+        y[i] = y0[i] + delta;
+        grl_rates<R>(y, i_stim_area, p, cm, update_v, t);
+        a[i] = (t[i] - g[i]) / delta;
+        y[i] = ym;
+    }
+CMX_UNROLL
+    for (int i = 0; i < S; ++i)
+        if (i >= first) y[i] = grl_advance<R>(y0[i], g[i], a[i], dt);
+}
+
+}}  // namespace cmx::mahajan2008

@@ -9,9 +9,11 @@
  *                               heart/src/stimulus/SimpleStimulus.cpp:44-76,
  *                               heart/src/stimulus/RegularStimulus.cpp:41-80
  *   CellLuoRudy1991FromCellML,
- *   CellTenTusscher2006EpiFromCellML[RushLarsen]
+ *   CellTenTusscher2006EpiFromCellML[RushLarsen], CellHodgkinHuxley1952FromCellML,
+ *   CellFoxModel2002FromCellML, CellMahajan2008FromCellML
  *                               the classes chaste_codegen generates from
- *                               heart/src/odes/cellml/{LuoRudy1991,TenTusscher2006Epi}.cellml
+ *                               heart/src/odes/cellml/{LuoRudy1991,TenTusscher2006Epi,
+ *                               HodgkinHuxley1952,FoxModel2002,Mahajan2008}.cellml
  *   AbstractCardiacCellFactory, PlaneStimulusCellFactory
  *                               heart/src/problem/cell_factories/AbstractCardiacCellFactory.hpp:100-108,
  *                               heart/src/problem/cell_factories/PlaneStimulusCellFactory.hpp:55-86
@@ -269,6 +271,35 @@ struct CellTenTusscher2006EpiFromCellMLGRL2 {
     static constexpr int kModel = CMX_MODEL_TT06_EPI;
     static constexpr int kIntegrator = CMX_GRL2;
 };
+
+// Further generated models (chaste_codegen classes of
+// heart/src/odes/cellml/{HodgkinHuxley1952,FoxModel2002,Mahajan2008}.cellml).
+#define CHASTE_AMD_CELL_TAGS(Base, model)                                  \
+    struct Base {                                                          \
+        static constexpr int kModel = model;                               \
+        static constexpr int kIntegrator = CMX_FORWARD_EULER;              \
+    };                                                                     \
+    struct Base##RushLarsen {                                              \
+        static constexpr int kModel = model;                               \
+        static constexpr int kIntegrator = CMX_RUSH_LARSEN;                \
+    };                                                                     \
+    struct Base##BackwardEuler {                                           \
+        static constexpr int kModel = model;                               \
+        static constexpr int kIntegrator = CMX_BACKWARD_EULER;             \
+    };                                                                     \
+    struct Base##BackwardEulerOpt : Base##BackwardEuler {};                \
+    struct Base##GRL1 {                                                    \
+        static constexpr int kModel = model;                               \
+        static constexpr int kIntegrator = CMX_GRL1;                       \
+    };                                                                     \
+    struct Base##GRL2 {                                                    \
+        static constexpr int kModel = model;                               \
+        static constexpr int kIntegrator = CMX_GRL2;                       \
+    };
+CHASTE_AMD_CELL_TAGS(CellHodgkinHuxley1952FromCellML, CMX_MODEL_HH52)
+CHASTE_AMD_CELL_TAGS(CellFoxModel2002FromCellML, CMX_MODEL_FOX2002)
+CHASTE_AMD_CELL_TAGS(CellMahajan2008FromCellML, CMX_MODEL_MAHAJAN2008)
+#undef CHASTE_AMD_CELL_TAGS
 
 // ---------------------------------------------------------------------------
 // Mesh (regular slabs; general simplices through SetNodesAndElements)

@@ -13,7 +13,10 @@
 #include <omp.h>
 #endif
 
+#include "../chaste_amd/csrc/models/fox2002.h"
+#include "../chaste_amd/csrc/models/hh52.h"
 #include "../chaste_amd/csrc/models/lr91.h"
+#include "../chaste_amd/csrc/models/mahajan2008.h"
 #include "../chaste_amd/csrc/models/ord_endo.h"
 #include "../chaste_amd/csrc/models/tt06_epi.h"
 
@@ -110,131 +113,122 @@ struct Stimulus {
     }
 };
 
-int num_states(int model)
-{
-    switch (model) {
-    case ORC_LR91_HAND:
-    case ORC_LR91_CELLML: return cmx::lr91::kNumStates;
-    case ORC_TT06_EPI: return cmx::tt06_epi::kNumStates;
-    case ORC_ORD_ENDO: return cmx::ord_endo::kNumStates;
-    default: return -1;
+// A model generated from CellML (or written to the generator's interface):
+// its state and parameter tables and each integrator's substep with V
+// clamped unless update_v (the reference's ComputeExceptVoltage / Compute).
+// Forward Euler and Rush-Larsen: AbstractCardiacCell / AbstractRushLarsenCardiacCell
+// (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:57-153); backward Euler
+// advances V first and then steps the other states at the new V
+// (AbstractBackwardEulerCardiacCell.hpp:300-330).
+struct ModelEntry {
+    int S, P;
+    const double* init;
+    const double* pdef;
+    bool implicit;           // backward-Euler and generalized Rush-Larsen forms
+    double (*substep)(int integ, const double* p, double cm, double* y, double dt, double stim_area, bool update_v);
+    double (*iion)(const double* p, double cm, const double* y);
+};
+
+// Newton failures of backward-Euler substeps (the reference throws).
+std::atomic<int> g_newton_failures{0};
+
+// The backward-Euler and generalized Rush-Larsen substeps of a generated model.
+#define ORC_IMPLICIT_STEPS(ns)                                                                                  \
+    if (integ == ORC_GRL1 || integ == ORC_GRL2) {                                                               \
+        if (integ == ORC_GRL1) ns::grl_step<double, 1>(y, stim_area, p, cm, dt, update_v);                      \
+        else ns::grl_step<double, 2>(y, stim_area, p, cm, dt, update_v);                                        \
+        return 0.0;                                                                                             \
+    }                                                                                                           \
+    if (integ == ORC_BE) {                                                                                      \
+        if (update_v) y[0] += dt * (dv = ns::dvdt<double>(y, y[0], stim_area, p, cm));                          \
+        if (ns::be_step<double>(y, y[0], stim_area, p, cm, dt) < 0) ++g_newton_failures;                       \
+        return dv;                                                                                              \
     }
+
+#define ORC_MODEL_FNS(ns, IMPLICIT_STEPS)                                                                       \
+    [](int integ, const double* p, double cm, double* y, double dt, double stim_area, bool update_v) -> double { \
+        double dy[64] = {}, ginf[64], gk[64], dv = 0.0;                                                         \
+        IMPLICIT_STEPS                                                                                          \
+        const bool rl = integ == ORC_RL;                                                                        \
+        if (update_v) dv = ns::dvdt<double>(y, y[0], stim_area, p, cm);                                        \
+        if (rl) ns::evaluate<double, true>(y, y[0], stim_area, p, cm, dy, ginf, gk);                            \
+        else ns::evaluate<double, false>(y, y[0], stim_area, p, cm, dy, ginf, gk);                              \
+        for (int i = 1; i < ns::kNumStates; ++i) {                                                              \
+            if (rl && ns::kIsGate[i]) y[i] = ginf[i] + (y[i] - ginf[i]) * std::exp(gk[i] * dt);                \
+            else y[i] += dt * dy[i];                                                                            \
+        }                                                                                                       \
+        if (update_v) y[0] += dt * dv;                                                                          \
+        return dv;                                                                                              \
+    },                                                                                                          \
+        [](const double* p, double cm, const double* y) { return ns::iionic<double>(y, y[0], p, cm); }
+
+// A generated model (all integrators) and one with forward-Euler and
+// Rush-Larsen forms only.
+#define ORC_MODEL(ns) \
+    ModelEntry { ns::kNumStates, ns::kNumParams, ns::kInitialState, ns::kParamDefaults, true, ORC_MODEL_FNS(ns, ORC_IMPLICIT_STEPS(ns)) }
+#define ORC_MODEL_EXPLICIT(ns) \
+    ModelEntry { ns::kNumStates, ns::kNumParams, ns::kInitialState, ns::kParamDefaults, false, ORC_MODEL_FNS(ns, ) }
+
+const ModelEntry* model_entry(int model)
+{
+    static const ModelEntry table[] = {
+        ORC_MODEL(cmx::lr91),                // ORC_LR91_HAND: table values; the steps are Lr91Hand's
+        ORC_MODEL(cmx::lr91),                // ORC_LR91_CELLML
+        ORC_MODEL(cmx::tt06_epi),            // ORC_TT06_EPI
+        ORC_MODEL_EXPLICIT(cmx::ord_endo),   // ORC_ORD_ENDO
+        ORC_MODEL(cmx::hh52),                // ORC_HH52
+        ORC_MODEL(cmx::fox2002),             // ORC_FOX2002
+        ORC_MODEL(cmx::mahajan2008),         // ORC_MAHAJAN2008
+    };
+    if (model < 0 || model >= (int)(sizeof(table) / sizeof(table[0]))) return nullptr;
+    return &table[model];
 }
 
-const double* initial_state(int model)
+int num_states(int model)
 {
-    if (model == ORC_ORD_ENDO) return cmx::ord_endo::kInitialState;
-    return model == ORC_TT06_EPI ? cmx::tt06_epi::kInitialState : cmx::lr91::kInitialState;
+    const ModelEntry* m = model_entry(model);
+    return m ? m->S : -1;
 }
+
+const double* initial_state(int model) { return model_entry(model)->init; }
 
 // The O'Hara-Rudy cell has forward-Euler and Rush-Larsen forms only, the hand
 // LR91 restatement forward Euler only.
 bool integrator_ok(int model, int integ)
 {
-    if (integ < ORC_FE || integ > ORC_GRL2) return false;
-    if (model == ORC_LR91_HAND || model == ORC_ORD_ENDO) return integ == ORC_FE || (model == ORC_ORD_ENDO && integ == ORC_RL);
-    return true;
+    if (integ < ORC_FE || integ > ORC_GRL2 || !model_entry(model)) return false;
+    if (model == ORC_LR91_HAND) return integ == ORC_FE;
+    return integ == ORC_FE || integ == ORC_RL || model_entry(model)->implicit;
 }
-
-// Newton failures of backward-Euler substeps (the reference throws).
-std::atomic<int> g_newton_failures{0};
 
 // One cell substep over [t, t+dt] with V clamped unless update_v.  stim_area
 // is the area stimulus (uA/cm^2) at time t.  Returns dV/dt when update_v.
-// Backward Euler (AbstractBackwardEulerCardiacCell.hpp:300-330) advances V
-// first and then steps the other states at the new V.
 double cell_substep(int model, int integ, const double* p, double cm, double* y, double t, double dt,
                     double stim_area, bool update_v)
 {
     (void)t;
-    double dy[64] = {}, ginf[64], gk[64];
-    const int S = num_states(model);
-    const bool rl = integ == ORC_RL;
-    double dv = 0.0;
     if (model == ORC_LR91_HAND) {
+        double dy[64] = {};
         const double iion = Lr91Hand::currents(y, dy);
-        dv = -(iion + stim_area) / Lr91Hand::kC;
+        const double dv = -(iion + stim_area) / Lr91Hand::kC;
         // The hand restatement only has the forward-Euler form (the
         // reference class is an AbstractCardiacCell).
         if (update_v) y[0] += dt * dv;
-        for (int i = 1; i < S; ++i) y[i] += dt * dy[i];
+        for (int i = 1; i < num_states(model); ++i) y[i] += dt * dy[i];
         return dv;
     }
-    if (integ == ORC_GRL1 || integ == ORC_GRL2) {
-        if (model == ORC_LR91_CELLML) {
-            if (integ == ORC_GRL1) cmx::lr91::grl_step<double, 1>(y, stim_area, p, cm, dt, update_v);
-            else cmx::lr91::grl_step<double, 2>(y, stim_area, p, cm, dt, update_v);
-        } else {
-            if (integ == ORC_GRL1) cmx::tt06_epi::grl_step<double, 1>(y, stim_area, p, cm, dt, update_v);
-            else cmx::tt06_epi::grl_step<double, 2>(y, stim_area, p, cm, dt, update_v);
-        }
-        return 0.0;
-    }
-    if (integ == ORC_BE) {
-        int its;
-        if (model == ORC_LR91_CELLML) {
-            if (update_v) y[0] += dt * (dv = cmx::lr91::dvdt<double>(y, y[0], stim_area, p, cm));
-            its = cmx::lr91::be_step<double>(y, y[0], stim_area, p, cm, dt);
-        } else {
-            if (update_v) y[0] += dt * (dv = cmx::tt06_epi::dvdt<double>(y, y[0], stim_area, p, cm));
-            its = cmx::tt06_epi::be_step<double>(y, y[0], stim_area, p, cm, dt);
-        }
-        if (its < 0) ++g_newton_failures;
-        return dv;
-    }
-    if (model == ORC_ORD_ENDO) {
-        if (update_v) dv = cmx::ord_endo::dvdt<double>(y, y[0], stim_area, p, cm);
-        if (rl) cmx::ord_endo::evaluate<double, true>(y, y[0], stim_area, p, cm, dy, ginf, gk);
-        else cmx::ord_endo::evaluate<double, false>(y, y[0], stim_area, p, cm, dy, ginf, gk);
-        for (int i = 1; i < S; ++i) {
-            if (rl && cmx::ord_endo::kIsGate[i]) y[i] = ginf[i] + (y[i] - ginf[i]) * std::exp(gk[i] * dt);
-            else y[i] += dt * dy[i];
-        }
-        if (update_v) y[0] += dt * dv;
-        return dv;
-    }
-    if (model == ORC_LR91_CELLML) {
-        if (update_v) dv = cmx::lr91::dvdt<double>(y, y[0], stim_area, p, cm);
-        if (rl) cmx::lr91::evaluate<double, true>(y, y[0], stim_area, p, cm, dy, ginf, gk);
-        else cmx::lr91::evaluate<double, false>(y, y[0], stim_area, p, cm, dy, ginf, gk);
-        for (int i = 1; i < S; ++i) {
-            if (rl && cmx::lr91::kIsGate[i]) y[i] = ginf[i] + (y[i] - ginf[i]) * std::exp(gk[i] * dt);
-            else y[i] += dt * dy[i];
-        }
-    } else {
-        if (update_v) dv = cmx::tt06_epi::dvdt<double>(y, y[0], stim_area, p, cm);
-        if (rl) cmx::tt06_epi::evaluate<double, true>(y, y[0], stim_area, p, cm, dy, ginf, gk);
-        else cmx::tt06_epi::evaluate<double, false>(y, y[0], stim_area, p, cm, dy, ginf, gk);
-        for (int i = 1; i < S; ++i) {
-            if (rl && cmx::tt06_epi::kIsGate[i]) y[i] = ginf[i] + (y[i] - ginf[i]) * std::exp(gk[i] * dt);
-            else y[i] += dt * dy[i];
-        }
-    }
-    if (update_v) y[0] += dt * dv;
-    return dv;
+    return model_entry(model)->substep(integ, p, cm, y, dt, stim_area, update_v);
 }
 
 double cell_iionic(int model, const double* p, double cm, const double* y)
 {
-    switch (model) {
-    case ORC_LR91_HAND: return Lr91Hand::currents(y, nullptr);
-    case ORC_LR91_CELLML: return cmx::lr91::iionic<double>(y, y[0], p, cm);
-    case ORC_ORD_ENDO: return cmx::ord_endo::iionic<double>(y, y[0], p, cm);
-    default: return cmx::tt06_epi::iionic<double>(y, y[0], p, cm);
-    }
+    if (model == ORC_LR91_HAND) return Lr91Hand::currents(y, nullptr);
+    return model_entry(model)->iion(p, cm, y);
 }
 
-const double* default_params(int model)
-{
-    if (model == ORC_ORD_ENDO) return cmx::ord_endo::kParamDefaults;
-    return model == ORC_TT06_EPI ? cmx::tt06_epi::kParamDefaults : cmx::lr91::kParamDefaults;
-}
+const double* default_params(int model) { return model_entry(model)->pdef; }
 
-int num_params(int model)
-{
-    if (model == ORC_ORD_ENDO) return cmx::ord_endo::kNumParams;
-    return model == ORC_TT06_EPI ? cmx::tt06_epi::kNumParams : cmx::lr91::kNumParams;
-}
+int num_params(int model) { return model_entry(model)->P; }
 
 // TimeStepper (global/src/TimeStepper.cpp:43-200): t_k = start + k*dt, the last
 // step snapped to the end when within eps.

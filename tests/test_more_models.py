@@ -1,0 +1,239 @@
+"""Further generated cell models: Hodgkin-Huxley 1952, Fox 2002 and Mahajan
+2008 (chaste_amd/codegen/models.py, from the reference's CellML files).
+
+The oracle runs the generated FP64 model; it is pinned here against the
+reference's ValidData files and protocols (heart/test/ionicmodels/
+TestIonicModels.hpp:305-333 HH52, :700-748 Fox 2002, :910-983 Mahajan 2008;
+voltages within 1e-3 mV at every sample, RunAndCheckIonicModels.hpp:196-221),
+the GetIIonic values those tests hard-code, and backward Euler against
+forward Euler with the reference's CompareCellModelResults rule
+(RunAndCheckIonicModels.hpp:224-280).  Rush-Larsen is pinned as for LR91
+(test_oracle_golden.py): equal to forward Euler with V clamped, and first
+order against a dt 1e-6 forward-Euler solution.  On the GPU the batched
+cells and a propagating slab are compared with the oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_dat
+from tests.test_gpu_parity import MIXED_ACT_MS, MIXED_MV
+
+HH52, FOX, MAHAJAN = 4, 5, 6        # oracle model ids
+
+
+def _run(oracle, model, t_end, dt, stim, per, integ=0, y=None):
+    y = oracle.model_initial_state(model) if y is None else y
+    out = oracle.cell_compute(model, y, 0.0, t_end, dt, stim=stim, steps_per_sample=per, integrator=integ)
+    return y, out
+
+
+def test_hh52_valid_data(oracle):
+    _, ref = load_dat('HH52RegResultValidData.dat')
+    y, out = _run(oracle, HH52, 150.0, 0.01, (-20.0, 0.5, 10.0, -1.0), 100)
+    assert out.shape[0] == ref.shape[0] == 151
+    assert np.abs(out[:, 0] - ref[:, 1]).max() < 1e-3
+    # the same cell run again from t = 0 for 15 ms (TestIonicModels.hpp:332-333)
+    _run(oracle, HH52, 15.0, 0.01, (-20.0, 0.5, 10.0, -1.0), 100, y=y)
+    assert abs(oracle.cell_iionic(HH52, y) - 40.6341) < 1e-3
+
+
+def test_fox2002_valid_data(oracle):
+    hdr, ref = load_dat('FoxRegularStimValidData.dat')
+    y, out = _run(oracle, FOX, 200.0, 0.002, (-80.0, 1.0, 50.0, 500.0), 500)
+    assert out.shape[0] == ref.shape[0] == 201
+    assert np.abs(out[:, 0] - ref[:, 1]).max() < 1e-3
+    # every state of the file (5 significant figures), in the model's order
+    for j in range(1, 13):
+        rel = np.abs(out[:, j] - ref[:, j + 1]) / np.maximum(np.abs(ref[:, j + 1]), 1e-12)
+        assert rel.max() < 2e-3, hdr[j + 1]
+
+
+def test_mahajan2008_valid_data(oracle):
+    _, ref = load_dat('Mahajan2008ValidData.dat')
+    y0 = oracle.model_initial_state(MAHAJAN)
+    assert abs(oracle.cell_iionic(MAHAJAN, y0) - 0.0027) < 1e-3
+    y, out = _run(oracle, MAHAJAN, 80.0, 0.001, (-1800.0, 0.05, 10.0, 1000.0), 1000)
+    assert out.shape[0] == ref.shape[0] == 81
+    assert np.abs(out[:, 0] - ref[:, 1]).max() < 1e-3
+
+
+def _compare_cell_results(v1, v2, tol, extra=()):
+    """CompareCellModelResults on equal sample times: 25x the tolerance where
+    the second trace moved more than 0.05 mV since its previous sample."""
+    last = v2[0]
+    for i in range(len(v2)):
+        t = tol * 25 if abs(v2[i] - last) > 0.05 else tol
+        last = v2[i]
+        assert abs(v1[i] - v2[i]) <= t, (i, v1[i], v2[i])
+        for a, b, scale in extra:
+            assert abs(a[i] - b[i]) <= t * scale, (i, a[i], b[i])
+
+
+def test_fox2002_backward_euler_against_forward(oracle):
+    stim = (-80.0, 1.0, 50.0, 500.0)
+    _, fe = _run(oracle, FOX, 200.0, 0.002, stim, 500)
+    _, be = _run(oracle, FOX, 200.0, 0.01, stim, 100, integ=oracle.BE)
+    # calcium (Ca_i, state 11) at tol/100 and the h gate (state 2) at tol/10
+    _compare_cell_results(fe[:, 0], be[:, 0], 0.2, extra=((fe[:, 11], be[:, 11], 0.01), (fe[:, 2], be[:, 2], 0.1)))
+
+
+def test_mahajan2008_backward_euler_against_forward(oracle):
+    stim = (-1800.0, 0.05, 5.0, -1.0)
+    ybe, be = _run(oracle, MAHAJAN, 50.0, 0.005, stim, 100, integ=oracle.BE)
+    assert abs(oracle.cell_iionic(MAHAJAN, ybe) - 0.1244) < 1e-3
+    _, fe = _run(oracle, MAHAJAN, 50.0, 0.005, stim, 100)
+    _compare_cell_results(fe[:, 0], be[:, 0], 0.03)
+
+
+@pytest.mark.parametrize('model', [HH52, FOX, MAHAJAN], ids=['hh52', 'fox2002', 'mahajan2008'])
+def test_backward_euler_first_order_clamped(oracle, model):
+    """With V clamped, backward Euler over 1 ms converges at first order to
+    the dt 1e-6 forward-Euler solution (ratio 2 +- 0.2 between dt 0.01 and
+    0.005 for every state that moves)."""
+    y0 = oracle.model_initial_state(model)
+    y0[0] = -20.0
+    ref = y0.copy()
+    oracle.cell_compute_except_voltage(model, ref, 0.0, 1.0, 1e-6, integrator=oracle.FE)
+    errs = []
+    for dt in (0.01, 0.005):
+        y = y0.copy()
+        oracle.cell_compute_except_voltage(model, y, 0.0, 1.0, dt, integrator=oracle.BE)
+        errs.append(y - ref)
+    moved = np.abs(ref - y0) > 1e-6 * np.maximum(np.abs(y0), 1.0)
+    moved[0] = False
+    assert moved.sum() >= 3
+    ratio = errs[0][moved] / errs[1][moved]
+    assert np.all(np.abs(ratio - 2.0) <= 0.2), ratio
+
+
+@pytest.mark.parametrize('model', [HH52, FOX, MAHAJAN], ids=['hh52', 'fox2002', 'mahajan2008'])
+def test_rush_larsen_against_forward_euler_clamped(oracle, model):
+    """ComputeExceptVoltage over 1 ms at dt 0.001: Rush-Larsen and forward
+    Euler agree to 1e-6 relative in every state (TestRushLarsen.hpp:130-138),
+    and Rush-Larsen converges at first order to the dt 1e-6 forward-Euler
+    solution with V clamped at -20 mV."""
+    y0 = oracle.model_initial_state(model)
+    a, b = y0.copy(), y0.copy()
+    oracle.cell_compute_except_voltage(model, a, 0.0, 1.0, 0.001, integrator=oracle.RL)
+    oracle.cell_compute_except_voltage(model, b, 0.0, 1.0, 0.001, integrator=oracle.FE)
+    assert np.all(np.abs(a - b) <= 1e-6 * np.maximum(np.abs(b), 1.0))
+    y0[0] = -20.0
+    ref = y0.copy()
+    oracle.cell_compute_except_voltage(model, ref, 0.0, 1.0, 1e-6, integrator=oracle.FE)
+    errs = []
+    for dt in (0.002, 0.001):
+        y = y0.copy()
+        oracle.cell_compute_except_voltage(model, y, 0.0, 1.0, dt, integrator=oracle.RL)
+        errs.append(y - ref)
+    scale = np.maximum(np.abs(y0), 1.0)
+    # a gate with V clamped is linear with constant coefficients, where the
+    # Rush-Larsen update is the exact solution at any dt: its error is the
+    # reference's own (forward Euler at 1e-6, ~1e-7 relative)
+    exact = np.abs(errs[0] - errs[1]) <= 1e-12 * scale
+    exact[0] = False
+    assert np.all(np.abs(errs[0][exact]) <= 1e-5 * scale[exact])
+    moved = (np.abs(errs[1]) > 1e-9 * scale) & ~exact
+    moved[0] = False
+    assert exact.sum() + moved.sum() >= 3
+    ratio = errs[0][moved] / errs[1][moved]
+    assert np.all(np.abs(ratio - 2.0) <= 0.2), ratio
+
+
+CASES = {
+    # name: (cmx id, oracle id, dt of the single-cell runs (FE, RL), cell stimulus (mag, dur, period))
+    'hh52': ('HH52', HH52, (0.01, 0.01), (-20.0, 0.5, 20.0)),
+    'fox2002': ('FOX2002', FOX, (0.002, 0.002), (-80.0, 1.0, 300.0)),
+    'mahajan2008': ('MAHAJAN2008', MAHAJAN, (0.001, 0.002), (-1800.0, 0.05, 300.0)),
+}
+
+
+def test_library_model_table_matches_oracle(oracle):
+    """Each id of the library's model table carries the generated model the
+    oracle runs under the same name (initial state, parameter defaults)."""
+    from chaste_amd import cmx
+    for name, (cid, oid, _, _) in CASES.items():
+        y = np.asarray(cmx.initial_state(getattr(cmx, cid)))
+        np.testing.assert_array_equal(y, oracle.model_initial_state(oid), err_msg=name)
+        assert len(cmx.param_info(getattr(cmx, cid))) >= 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('integ', ['fe', 'rl'])
+@pytest.mark.parametrize('name', sorted(CASES))
+def test_batched_cells_fp64_matches_oracle(oracle, name, integ):
+    """cmx_cells_run in FP64 against the oracle's cells: every state within
+    1e-9 of its scale, the CellProperties summary within 1e-9 ms."""
+    from chaste_amd import cmx
+    cid, oid, dts, (mag, dur, per0) = CASES[name]
+    model = getattr(cmx, cid)
+    print('case more-model cells', name, integ, flush=True)
+    rng = np.random.default_rng(5)
+    n = 40
+    dt = dts[1] if integ == 'rl' else dts[0]
+    defaults = np.array([d for _, d in cmx.param_info(model)])
+    params = defaults[:, None] * np.exp(rng.uniform(np.log(0.8), np.log(1.25), (len(defaults), n)))
+    period = np.round(rng.uniform(per0, 2 * per0, n))
+    t_end = np.round(np.minimum(period * 1.5, 400.0) / dt) * dt
+    stimuli = [(mag, 0.0, dur, float(p)) for p in period]
+    ig = cmx.RUSH_LARSEN if integ == 'rl' else cmx.FORWARD_EULER
+    y, props, _ = cmx.cells_run(model, ig, t_end, dt, stimuli=stimuli, params=params, precision=cmx.FP64,
+                                threshold=-30.0)
+    oi = oracle.RL if integ == 'rl' else oracle.FE
+    scale = np.abs(np.asarray(cmx.initial_state(model))) + 1e-3
+    worst = 0.0
+    for c in range(0, n, 9):
+        yc = oracle.model_initial_state(oid)
+        v = oracle.cell_trace_v(oid, yc, t_end[c], dt, stim=(mag, dur, 0.0, period[c]), integrator=oi,
+                                params=params[:, c])
+        worst = max(worst, float((np.abs(y[:, c] - yc) / scale).max()))
+        pr = oracle.cell_properties(v, dt, -30.0)
+        assert pr[0] == props[0][c] and pr[0] >= 1
+        assert np.abs(pr[1:] - props[1:, c]).max() <= 1e-9
+    print('%s cells %s worst scaled state error %.3e' % (name, integ, worst))
+    assert worst <= 1e-9
+
+
+TISSUE = {
+    # name: (stimulus uA/cm^3, duration ms, dt_ode)
+    'hh52': (-100000.0, 0.5, 0.01),
+    'fox2002': (-400000.0, 1.0, 0.002),
+    'mahajan2008': (-2600000.0, 0.1, 0.002),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('precision', [0, 1], ids=['fp64', 'mixed'])
+@pytest.mark.parametrize('name', sorted(CASES))
+def test_tissue_matches_oracle(oracle, name, precision):
+    """A 2-D slab with a wave crossing it (Rush-Larsen cells), 4 ms: FP64
+    within 1e-9 relative of the oracle; mixed within 0.5 mV and 0.1 ms in
+    activation time of the oracle's FP64 run."""
+    from chaste_amd import cmx
+    cid, oid, _, _ = CASES[name]
+    mag, dur, dt_ode = TISSUE[name]
+    print('case more-model tissue', name, precision, flush=True)
+    dims = (2, 40, 8, 0)
+    coords, elems = cmx.slab_mesh(*dims, 0.01)
+    n = coords.shape[0]
+    stim_index = np.where(coords[:, 0] < 0.02, 0, -1).astype(np.int32)
+    m = cmx.Monodomain(coords, elems, np.array([1.75, 1.75]), getattr(cmx, cid), cmx.RUSH_LARSEN,
+                       stimuli=[(mag, 0.0, dur)], stim_index=stim_index, precision=precision,
+                       dt_pde=0.01, dt_ode=dt_ode, ksp_rtol=1e-12 if precision == 0 else 1e-8, ksp_atol=0.0,
+                       refine_sweeps=1 if precision else 0, use_graph=True)
+    m.set_activation(0.0, 1, 0.0)
+    m.step(0.0, 400)
+    V, act = m.voltage(), m.activation()[4]
+    m.close()
+    ref = oracle.mono_run(*dims, model=oid, rush_larsen=True, sigma=(1.75, 1.75, 1.75), stim_mag=mag, stim_dur=dur,
+                          stim_xlim=0.02, dt_ode=dt_ode, t_end=4.0, ksp_rtol=1e-12, act_threshold=0.0)
+    both = (act >= 0) & (ref['act'] >= 0)
+    print('%s tissue: rel err %.3e, max dV %.3e mV, activated %d of %d, act err %.3e' % (
+        name, np.abs(V - ref['V']).max() / np.abs(ref['V']).max(), np.abs(V - ref['V']).max(), both.sum(), n,
+        np.abs(act[both] - ref['act'][both]).max() if both.any() else -1))
+    assert np.array_equal(act >= 0, ref['act'] >= 0)
+    assert both.sum() > n // 4
+    if precision == 0:
+        assert np.abs(V - ref['V']).max() / np.abs(ref['V']).max() <= 1e-9
+    else:
+        assert np.abs(V - ref['V']).max() <= MIXED_MV
+        assert np.abs(act[both] - ref['act'][both]).max() <= MIXED_ACT_MS
