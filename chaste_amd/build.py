@@ -21,6 +21,27 @@ SOURCES = ['cmx_device.hip', 'cmx_adjoint.hip', 'cmx_cells.hip', 'cmx_host.cpp',
 CPP_TESTS = ['TestMonodomainProblem']
 
 
+# The plugin the tests load: the reference's LuoRudy1991.cellml generated
+# and built as a plugin (the built-in LR91 is the same model), found at run
+# time through a path relative to the plugin itself.
+TEST_PLUGIN_DIR = os.path.join(ROOT, 'tests', 'plugins')
+TEST_PLUGIN_CELLML = '/root/reference/heart/src/odes/cellml/LuoRudy1991.cellml'
+
+
+def test_plugin_path():
+    return os.path.join(TEST_PLUGIN_DIR, 'liblr91_plugin.so')
+
+
+def build_test_plugin():
+    """Builds tests/plugins/liblr91_plugin.so when the CellML source is present."""
+    if not os.path.exists(TEST_PLUGIN_CELLML):
+        return None
+    return build_model_plugin(TEST_PLUGIN_CELLML, TEST_PLUGIN_DIR, 'lr91_plugin',
+                              params=('fast_sodium_current.g_Na', 'time_dependent_potassium_current.g_Kmax',
+                                      'slow_inward_current.P_si'),
+                              rpath='$ORIGIN/../../chaste_amd/lib')
+
+
 def cpp_test_path(name):
     return os.path.join(ROOT, 'tests', 'cpp', 'bin', name)
 
@@ -70,3 +91,31 @@ def build_cpp_tests():
         src = os.path.join(ROOT, 'tests', 'cpp', name + '.cpp')
         _run(['g++', '-O2', '-std=c++17', '-Wall', '-I' + os.path.join(ROOT, 'include'), src,
               '-L' + LIBDIR, '-lcmx', '-Wl,-rpath,$ORIGIN/../../../chaste_amd/lib', '-o', cpp_test_path(name)])
+
+
+def build_model_plugin(cellml, out_dir, name, params=(), rpath=None):
+    """A cell model from a CellML file as a plugin for cmx_load_model: the
+    model is generated into <out_dir>/<name>.h (namespace cmx::<name>; params:
+    'component.variable' constants exposed as per-node parameters), a
+    one-line source instantiates every kernel through cmx_model_plugin.h, and
+    hipcc links it to libcmx.so as <out_dir>/lib<name>.so.  rpath: where the
+    plugin finds libcmx.so at load time (default: this library's directory).
+    Returns the plugin's path."""
+    from chaste_amd.codegen import cellml2cpp
+    if not name.isidentifier():
+        raise ValueError('plugin name must be a C identifier')
+    os.makedirs(out_dir, exist_ok=True)
+    header = os.path.join(out_dir, name + '.h')
+    argv = [cellml, header, name]
+    for p in params:
+        argv += ['--param', p]
+    cellml2cpp.main(argv)
+    src = os.path.join(out_dir, name + '_plugin.hip')
+    with open(src, 'w') as f:
+        f.write('#include "cmx_model_plugin.h"\n#include "%s.h"\n\nCMX_MODEL_PLUGIN(%s)\n' % (name, name))
+    out = os.path.join(out_dir, 'lib%s.so' % name)
+    lib_dir = rpath or LIBDIR
+    _run([HIPCC] + HIP_FLAGS + ['-I' + out_dir, '-I' + os.path.join(CSRC, 'models'), '-shared', '-o', out + '.tmp', src, '-L' + LIBDIR, '-lcmx',
+                                '-Wl,-rpath,' + lib_dir])
+    os.replace(out + '.tmp', out)
+    return out

@@ -107,6 +107,7 @@ SIGNATURES = {
     'cmx_set_profiling': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     'cmx_get_kernel_times': (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
     'cmx_model_info': (ctypes.c_int, [ctypes.c_int32, _ip, _ip]),
+    'cmx_load_model': (ctypes.c_int, [ctypes.c_char_p, _ip]),
     'cmx_model_state_name': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p)]),
     'cmx_model_initial_state': (ctypes.c_int, [ctypes.c_int32, _dp]),
     'cmx_model_param': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), _dp]),
@@ -159,6 +160,14 @@ def _check(rc, ctx=None):
 
 def _p(a, t=_dp):
     return a.ctypes.data_as(t)
+
+
+def load_model(path):
+    """Loads a cell-model plugin (build.build_model_plugin) and returns its
+    model id (cmx_load_model)."""
+    mid = ctypes.c_int32()
+    _check(lib().cmx_load_model(os.fsencode(os.path.abspath(path)), ctypes.byref(mid)))
+    return mid.value
 
 
 def model_info(model):

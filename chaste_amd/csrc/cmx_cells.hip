@@ -353,7 +353,6 @@ ScratchLayout scratch_layout(int n, int S, int P, size_t rs)
     return L;
 }
 
-constexpr int kMaxModelStates = 64;
 struct ModelConsts {
     double init[kMaxModelStates];
     double pdef[kMaxModelStates];

@@ -43,8 +43,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <atomic>
 #include <cstring>
+#include <dlfcn.h>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -859,8 +862,15 @@ const ModelOps* model_ops_hh52();
 const ModelOps* model_ops_fox2002();
 const ModelOps* model_ops_mahajan2008();
 
+// Plugin models (cmx_load_model), published once registered.
+std::atomic<const ModelOps*> g_plugin_models[kMaxPluginModels];
+std::mutex g_plugin_mutex;
+int g_plugin_count = 0;
+
 const ModelOps* model_ops(int model)
 {
+    if (model >= kPluginModelBase && model < kPluginModelBase + kMaxPluginModels)
+        return g_plugin_models[model - kPluginModelBase].load(std::memory_order_acquire);
     switch (model) {
     case CMX_MODEL_LR91: return model_ops_lr91();
     case CMX_MODEL_TT06_EPI: return model_ops_tt06();
@@ -2557,6 +2567,44 @@ int cmx_get_kernel_times(cmx_ctx* c, double* total_ms, int64_t* launches, int32_
         if (launches) launches[k] = c->k_launches[k];
         if (reset) { c->k_ms[k] = 0; c->k_launches[k] = 0; }
     }
+    return CMX_OK;
+}
+
+int cmx_load_model(const char* path, int32_t* model_id)
+{
+    if (!path || !model_id) return fail(nullptr, CMX_E_ARG, "null argument");
+    void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) {
+        const char* e = dlerror();
+        return fail(nullptr, CMX_E_ARG, std::string("cannot load the cell-model plugin: ") + (e ? e : path));
+    }
+    using AbiFn = uint32_t (*)();
+    using OpsFn = const ModelOps* (*)();
+    const AbiFn abi = (AbiFn)dlsym(h, "cmx_model_plugin_abi");
+    const OpsFn ops_of = (OpsFn)dlsym(h, "cmx_model_plugin_ops");
+    if (!abi || !ops_of) {
+        dlclose(h);
+        return fail(nullptr, CMX_E_ARG, std::string(path) + " is not a cell-model plugin (no cmx_model_plugin_ops)");
+    }
+    if (abi() != kModelPluginAbi) {
+        dlclose(h);
+        return fail(nullptr, CMX_E_ARG, std::string(path) + " was built against other headers than this library");
+    }
+    const ModelOps* ops = ops_of();
+    if (!ops || ops->meta.S < 2 || ops->meta.S > kMaxModelStates || ops->meta.P < 0 || !ops->cells64 || !ops->cells32) {
+        dlclose(h);
+        return fail(nullptr, CMX_E_ARG, std::string(path) + ": the model's table entry is incomplete or too large");
+    }
+    std::lock_guard<std::mutex> lock(g_plugin_mutex);
+    if (g_plugin_count >= kMaxPluginModels) {
+        dlclose(h);
+        return fail(nullptr, CMX_E_ARG, "too many cell-model plugins loaded");
+    }
+    // the plugin stays loaded until the process ends: its kernels may sit in
+    // captured graphs and in contexts that outlive this call
+    const int k = g_plugin_count++;
+    g_plugin_models[k].store(ops, std::memory_order_release);
+    *model_id = kPluginModelBase + k;
     return CMX_OK;
 }
 

@@ -56,6 +56,19 @@ struct ModelOps {
 // The entry of a model id; null for an unknown id.
 const ModelOps* model_ops(int model);
 
+// Models loaded at run time (cmx_load_model) take ids kPluginModelBase + k.
+constexpr int kPluginModelBase = 64;
+constexpr int kMaxPluginModels = 64;
+constexpr int kMaxModelStates = 64;
+
+// Stamp a plugin reports from cmx_model_plugin_abi(): a plugin built against
+// headers whose shared structures differ from the library's is refused.
+constexpr uint32_t kModelPluginAbi =
+    (1u << 24) | (uint32_t)((sizeof(ModelOps) * 131u + sizeof(cmx_ctx) * 7u + sizeof(CellArgs<double, double>) * 5u +
+                             sizeof(CellArgs<float, float>) * 3u + sizeof(SviArgs<double, double>) * 11u +
+                             sizeof(CellsArgs<double>) * 13u + sizeof(CellsArgs<float>) * 17u) &
+                            0xFFFFFFu);
+
 inline bool model_meta(int model, ModelMeta* m)
 {
     const ModelOps* ops = model_ops(model);

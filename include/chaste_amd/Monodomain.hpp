@@ -301,6 +301,18 @@ CHASTE_AMD_CELL_TAGS(CellFoxModel2002FromCellML, CMX_MODEL_FOX2002)
 CHASTE_AMD_CELL_TAGS(CellMahajan2008FromCellML, CMX_MODEL_MAHAJAN2008)
 #undef CHASTE_AMD_CELL_TAGS
 
+// A cell model loaded at run time from a plugin built from any CellML file
+// (chaste_amd.build.build_model_plugin): the model id for a CellDescriptor,
+// with any integrator.  The reference's DynamicCellModelLoader::CreateCell
+// (heart/src/odes/DynamicCellModelLoader.cpp:55-63) returns a cell object
+// instead; here the id stands for every node's cell of that type.
+inline int LoadCellModel(const std::string& pluginPath)
+{
+    int32_t id = -1;
+    Check(cmx_load_model(pluginPath.c_str(), &id), nullptr);
+    return id;
+}
+
 // ---------------------------------------------------------------------------
 // Mesh (regular slabs; general simplices through SetNodesAndElements)
 // ---------------------------------------------------------------------------

@@ -242,6 +242,16 @@ int cmx_set_profiling(cmx_ctx* ctx, int32_t mask);
 /* total_ms[CMX_K_COUNT], launches[CMX_K_COUNT]; reset != 0 clears them afterwards. */
 int cmx_get_kernel_times(cmx_ctx* ctx, double* total_ms, int64_t* launches, int32_t reset);
 
+/* Loads a cell model built as a plugin (a shared library from a generated
+ * CellML model, chaste_amd/csrc/cmx_model_plugin.h; chaste_amd.build.
+ * build_model_plugin makes one from a .cellml file) and returns its model id
+ * (64 and up), usable wherever a CMX_MODEL_* id is, with every integrator.
+ * Replaces DynamicCellModelLoader::CreateCell / CellMLLoader
+ * (heart/src/odes/DynamicCellModelLoader.cpp:55-63, heart/src/io/CellMLLoader.cpp):
+ * the library stays loaded until the process ends.  Errors: a file that
+ * does not load, is not a plugin, or was built against other headers. */
+int cmx_load_model(const char* path, int32_t* model_id);
+
 /* Model metadata (CellML order, V first). */
 int cmx_model_info(int32_t model, int32_t* n_states, int32_t* n_params);
 int cmx_model_state_name(int32_t model, int32_t index, const char** name);
