@@ -71,6 +71,23 @@ def _tag(e):
     return e.tag.split('}')[1]
 
 
+RDF = '{http://www.w3.org/1999/02/22-rdf-syntax-ns#}'
+BQBIOL = '{http://biomodels.net/biology-qualifiers/}'
+OXMETA = 'oxford-metadata#'
+
+# CellML 1.0 unit prefixes and the built-in units this translator resolves
+# (each built-in is its own dimension; only the ratios of like quantities --
+# time, currents, capacitances -- are ever taken).
+_PREFIX = {'yotta': 24, 'zetta': 21, 'exa': 18, 'peta': 15, 'tera': 12, 'giga': 9, 'mega': 6, 'kilo': 3,
+           'hecto': 2, 'deka': 1, 'deca': 1, 'deci': -1, 'centi': -2, 'milli': -3, 'micro': -6, 'nano': -9,
+           'pico': -12, 'femto': -15, 'atto': -18, 'zepto': -21, 'yocto': -24}
+_BUILTIN = {'ampere', 'farad', 'second', 'volt', 'metre', 'meter', 'litre', 'liter', 'mole', 'kelvin', 'coulomb',
+            'joule', 'siemens', 'ohm', 'gram', 'kilogram', 'newton', 'watt', 'hertz', 'tesla', 'weber', 'henry',
+            'pascal', 'becquerel', 'gray', 'sievert', 'katal', 'lux', 'lumen', 'candela', 'radian', 'steradian',
+            'celsius'}
+_ALIAS = {'meter': 'metre', 'liter': 'litre'}
+
+
 class CellMLModel:
     def __init__(self, path):
         root = ET.parse(path).getroot()
@@ -84,6 +101,15 @@ class CellMLModel:
                 k = parent[k]
             return k
 
+        # unit definitions (model level and inside components)
+        self.unit_defs = {}
+        for u in root.iter(CML + 'units'):
+            if u.get('name') and u.get('base_units') != 'yes':
+                self.unit_defs[u.get('name')] = [
+                    (x.get('units'), x.get('prefix'), float(x.get('exponent', '1')),
+                     float(x.get('multiplier', '1'))) for x in u.findall(CML + 'unit')]
+            elif u.get('name'):
+                self.unit_defs[u.get('name')] = 'base'
         self.components = []
         for comp in root.findall(CML + 'component'):
             cname = comp.get('name')
@@ -141,13 +167,35 @@ class CellMLModel:
             else:
                 self.odes[src] = (cname, rhs)
         self.time = self.canon_map[self.time]
-        # metadata
+        # metadata: a cmeta:id names its variable's oxmeta term, unless an RDF
+        # description of that id says which term (bqbiol:is oxford-metadata#...)
         self.meta = {}
+        by_id = {}
         for key, info in self.vars.items():
             if info['cmeta']:
                 self.meta[info['cmeta']] = self.canon_map[key]
+                by_id[info['cmeta']] = self.canon_map[key]
+        for d in root.iter(RDF + 'Description'):
+            about = (d.get(RDF + 'about') or '').lstrip('#')
+            if about not in by_id:
+                continue
+            for q in d.findall(BQBIOL + 'is'):
+                res = q.get(RDF + 'resource') or ''
+                if OXMETA in res:
+                    self.meta.setdefault(res.split(OXMETA)[1], by_id[about])
         self.V = self.meta['membrane_voltage']
         self.stim = self.meta['membrane_stimulus_current']
+        # rates per millisecond: a model in other time units has every
+        # derivative divided by the length of its time unit in ms
+        self.time_factor = self.unit_scale(self.vars[self.time]['units'], {'second': 1}) / 1e-3
+        if abs(self.time_factor - 1.0) > 1e-12:
+            for k, (c, rhs) in list(self.odes.items()):
+                ap = ET.Element(MML + 'apply')
+                ET.SubElement(ap, MML + 'divide')
+                ap.append(rhs)
+                cn = ET.SubElement(ap, MML + 'cn')
+                cn.text = repr(self.time_factor)
+                self.odes[k] = (c, ap)
         # state order: V first, then document order (component, variable)
         states = [k for k in self.vars if self.canon_map[k] == k and k in self.odes]
         states.remove(self.V)
@@ -155,6 +203,50 @@ class CellMLModel:
 
     def canon(self, cname, name):
         return self.canon_map[(cname, name)]
+
+    def unit(self, name):
+        """(factor, dimensions) of a units name: the quantity in SI-prefixed
+        built-in units, dimensions a dict built-in -> exponent."""
+        name = _ALIAS.get(name, name)
+        if name == 'dimensionless':
+            return 1.0, {}
+        d = self.unit_defs.get(name)
+        if d is None:
+            if name in _BUILTIN:
+                return 1.0, {name: 1}
+            raise KeyError('unknown units ' + name)
+        if d == 'base':
+            return 1.0, {name: 1}
+        f, dims = 1.0, {}
+        for u, prefix, e, mult in d:
+            fu, du = self.unit(u)
+            p = 0 if prefix is None else (_PREFIX[prefix] if prefix in _PREFIX else int(prefix))
+            f *= mult * (10.0 ** p * fu) ** e
+            for b, x in du.items():
+                dims[b] = dims.get(b, 0) + x * e
+        return f, {b: x for b, x in dims.items() if abs(x) > 1e-12}
+
+    def unit_scale(self, name, dims):
+        """The factor of a units name whose dimensions must equal dims."""
+        f, d = self.unit(name)
+        if d != dims:
+            raise NotImplementedError('units %s are not %s' % (name, dims))
+        return f
+
+    def capacitance(self):
+        """The membrane capacitance variable (oxmeta membrane_capacitance,
+        else a constant in farads in the membrane equation), or None."""
+        k = self.meta.get('membrane_capacitance')
+        if k is not None:
+            return k
+        for key, info in self.vars.items():
+            if self.canon_map[key] == key and info['init'] is not None and info['units']:
+                try:
+                    if self.unit(info['units'])[1] == {'farad': 1}:
+                        return key
+                except KeyError:
+                    pass
+        return None
 
     def init(self, key):
         return float(self.vars[key]['init'])
@@ -671,10 +763,35 @@ class Generator:
                      and k not in m.odes and k not in self.params]
         state_ix = {k: i for i, k in enumerate(m.states)}
 
-        # Per-capacitance current units (pA/pF == uA/uF) need the tissue cm.
-        v_units = None
-        stim_units = m.vars[m.stim]['units']
-        per_cap = 'per_picoF' in stim_units or 'A_per_F' in stim_units or 'uF' in stim_units and 'cm' not in stim_units
+        # The stimulus arrives as an area density i_stim_area (uA/cm^2) with
+        # the tissue capacitance cm (uF/cm^2), and the model's stimulus
+        # variable takes it in its own units: a density per area as it is, a
+        # current per capacitance as i_stim_area / cm, an absolute current as
+        # i_stim_area / cm times the model's membrane capacitance (the current
+        # per capacitance is what the conversion keeps).
+        if abs(m.unit_scale(m.vars[m.V]['units'], {'volt': 1}) - 1e-3) > 1e-15:
+            raise NotImplementedError('membrane voltage must be in millivolts')
+        f_stim, d_stim = m.unit(m.vars[m.stim]['units'])
+
+        def scaled(expr, k):
+            return expr if abs(k - 1.0) < 1e-12 else '(%s) * R(%r)' % (expr, k)
+        self.model_cap = None
+        if d_stim == {'ampere': 1, 'metre': -2}:
+            per_cap = False
+            self.stim_expr = scaled('i_stim_area', 1e-2 / f_stim)            # 1 uA/cm^2 = 1e-2 A/m^2
+        elif d_stim == {'ampere': 1, 'farad': -1}:
+            per_cap = True
+            self.stim_expr = scaled('i_stim_area / cm', 1.0 / f_stim)         # 1 uA/uF = 1 A/F
+        elif d_stim == {'ampere': 1}:
+            per_cap = True
+            ck = m.capacitance()
+            if ck is None:
+                raise NotImplementedError('absolute stimulus current without a membrane capacitance')
+            fc = m.unit_scale(m.vars[ck]['units'], {'farad': 1})
+            self.model_cap = ck
+            self.stim_expr = scaled('i_stim_area / cm', m.init(ck) * fc / f_stim)
+        else:
+            raise NotImplementedError('stimulus current in units %s' % m.vars[m.stim]['units'])
 
         def body_lines(roots, indent='    ', printer=None):
             printer = printer or pr
@@ -772,7 +889,7 @@ class Generator:
             roots |= self.deps_rhs(k)
         for k in self.used_constants(roots, constants):
             w('    const R %s = R(%r);' % (cident(k), m.init(k)))
-        w('    const R i_stim = %s;' % ('i_stim_area / cm' if per_cap else 'i_stim_area'))
+        w('    const R i_stim = %s;' % self.stim_expr)
         w('    (void)i_stim;')
         roots = set()
         for k in m.states[1:]:
@@ -803,7 +920,7 @@ class Generator:
         w('    (void)p; (void)cm;')
         for k in self.used_constants(self.deps_rhs(m.V), constants):
             w('    const R %s = R(%r);' % (cident(k), m.init(k)))
-        w('    const R i_stim = %s;' % ('i_stim_area / cm' if per_cap else 'i_stim_area'))
+        w('    const R i_stim = %s;' % self.stim_expr)
         w('    (void)i_stim;')
         for line in body_lines(sorted(self.deps_rhs(m.V))):
             w(line)
@@ -898,7 +1015,7 @@ class Generator:
         w('    (void)y; (void)p; (void)cm; (void)i_stim_area;')
         for k in consts:
             w('    const R %s = R(%r);' % (cident(k), m.init(k)))
-        w('    const R i_stim = %s;' % ('i_stim_area / cm' if per_cap else 'i_stim_area'))
+        w('    const R i_stim = %s;' % self.stim_expr)
         w('    (void)i_stim;')
         for line in body_lines(sorted(roots), printer=gpr):
             w(line)
@@ -916,7 +1033,7 @@ class Generator:
         w('    (void)y; (void)p; (void)cm; (void)i_stim_area;')
         for k in consts:
             w('    const R %s = R(%r);' % (cident(k), m.init(k)))
-        w('    const R i_stim = %s;' % ('i_stim_area / cm' if per_cap else 'i_stim_area'))
+        w('    const R i_stim = %s;' % self.stim_expr)
         w('    (void)i_stim;')
         for line in body_lines(sorted(roots), printer=gpr):
             w(line)

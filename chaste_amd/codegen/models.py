@@ -2,7 +2,9 @@
 
 Each entry names the reference's CellML file (heart/src/odes/cellml/) and the
 constants exposed as run-time parameters, in the order of the header's
-parameter planes.  Regenerate every header with
+parameter planes.  Models in seconds or with absolute currents are converted
+by the generator (rates per ms; the stimulus through the model's membrane
+capacitance).  Regenerate every header with
 
     python -m chaste_amd.codegen.models [<cellml directory>]
 
@@ -30,6 +32,17 @@ MODELS = {
                  'L_type_Ca_current.P_Ca']),
     'mahajan2008': ('Mahajan2008.cellml',
                     ['INa.gna', 'ICaL.gca', 'IKr.ScaleFactorGkr', 'IKs.ScaleFactorGks', 'Ito.ScaleFactorIto']),
+    'dn85': ('DiFrancescoNoble1985.cellml',
+             ['fast_sodium_current.g_Na', 'time_independent_potassium_current.g_K1', 'second_inward_current.P_si',
+              'transient_outward_current.g_to']),
+    'fr2000': ('FaberRudy2000.cellml',
+               ['slow_delayed_rectifier_potassium_current.ScaleFactorGks', 'transient_outward_current.ScaleFactorIto',
+                'rapid_delayed_rectifier_potassium_current.ScaleFactorGkr', 'fast_sodium_current.g_Na']),
+    'maleckar2008': ('Maleckar2008.cellml',
+                     ['sodium_current.ScaleFactorGna', 'L_type_Ca_channel.ScaleFactorGCaL',
+                      'delayed_rectifier_K_currents.ScaleFactorGkr', 'delayed_rectifier_K_currents.ScaleFactorGks',
+                      'Ca_independent_transient_outward_K_current.ScaleFactorIto',
+                      'ultra_rapid_K_current.ScaleFactorGKur']),
     'tt06_epi': ('TenTusscher2006Epi.cellml',
                  ['fast_sodium_current.g_Na', 'L_type_Ca_current.g_CaL', 'rapid_time_dependent_potassium_current.g_Kr',
                   'slow_time_dependent_potassium_current.g_Ks', 'inward_rectifier_potassium_current.g_K1',

@@ -11,10 +11,13 @@
 
 #include "cmx_internal.h"
 #include "cmx_kernels.h"
+#include "models/dn85.h"
 #include "models/fox2002.h"
+#include "models/fr2000.h"
 #include "models/hh52.h"
 #include "models/lr91.h"
 #include "models/mahajan2008.h"
+#include "models/maleckar2008.h"
 #include "models/ord_endo.h"
 #include "models/tt06_epi.h"
 
@@ -73,6 +76,9 @@ CMX_GENERATED_MODEL(ModelTt06, tt06_epi)
 CMX_GENERATED_MODEL(ModelHh52, hh52)
 CMX_GENERATED_MODEL(ModelFox2002, fox2002)
 CMX_GENERATED_MODEL(ModelMahajan2008, mahajan2008)
+CMX_GENERATED_MODEL(ModelDn85, dn85)
+CMX_GENERATED_MODEL(ModelFr2000, fr2000)
+CMX_GENERATED_MODEL(ModelMaleckar2008, maleckar2008)
 
 // O'Hara-Rudy 2011 endocardial cells: forward Euler and Rush-Larsen only (no
 // backward-Euler or generalized Rush-Larsen forms).

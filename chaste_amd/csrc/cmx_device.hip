@@ -861,6 +861,9 @@ const ModelOps* model_ops_ord();
 const ModelOps* model_ops_hh52();
 const ModelOps* model_ops_fox2002();
 const ModelOps* model_ops_mahajan2008();
+const ModelOps* model_ops_dn85();
+const ModelOps* model_ops_fr2000();
+const ModelOps* model_ops_maleckar2008();
 
 // Plugin models (cmx_load_model), published once registered.
 std::atomic<const ModelOps*> g_plugin_models[kMaxPluginModels];
@@ -878,6 +881,9 @@ const ModelOps* model_ops(int model)
     case CMX_MODEL_HH52: return model_ops_hh52();
     case CMX_MODEL_FOX2002: return model_ops_fox2002();
     case CMX_MODEL_MAHAJAN2008: return model_ops_mahajan2008();
+    case CMX_MODEL_DN85: return model_ops_dn85();
+    case CMX_MODEL_FR2000: return model_ops_fr2000();
+    case CMX_MODEL_MALECKAR2008: return model_ops_maleckar2008();
     default: return nullptr;
     }
 }

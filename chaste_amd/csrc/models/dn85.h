@@ -1,0 +1,713 @@
+// Generated by chaste_amd/codegen/cellml2cpp.py from DiFrancescoNoble1985.cellml -- do not edit.
+// Model: difrancesco_noble_model_1985
+#pragma once
+#include "model_common.h"
+
+namespace cmx { namespace dn85 {
+
+constexpr int kNumStates = 16;
+constexpr int kVIndex = 0;
+constexpr int kNumParams = 4;
+constexpr bool kPerCapacitanceUnits = true;
+
+// State variables (CellML component__variable), V first then document order.
+static const char* const kStateNames[kNumStates] = {
+    "membrane__V",
+    "hyperpolarising_activated_current_y_gate__y",
+    "time_dependent_potassium_current_x_gate__x",
+    "transient_outward_current_s_gate__s",
+    "fast_sodium_current_m_gate__m",
+    "fast_sodium_current_h_gate__h",
+    "second_inward_current_d_gate__d",
+    "second_inward_current_f_gate__f",
+    "second_inward_current_f2_gate__f2",
+    "intracellular_sodium_concentration__Nai",
+    "intracellular_calcium_concentration__Cai",
+    "intracellular_calcium_concentration__Ca_up",
+    "intracellular_calcium_concentration__Ca_rel",
+    "intracellular_calcium_concentration__p",
+    "extracellular_potassium_concentration__Kc",
+    "intracellular_potassium_concentration__Ki",
+};
+static constexpr double kInitialState[kNumStates] = {
+    -87.0,  // membrane__V
+    0.2,  // hyperpolarising_activated_current_y_gate__y
+    0.01,  // time_dependent_potassium_current_x_gate__x
+    1.0,  // transient_outward_current_s_gate__s
+    0.01,  // fast_sodium_current_m_gate__m
+    0.8,  // fast_sodium_current_h_gate__h
+    0.005,  // second_inward_current_d_gate__d
+    1.0,  // second_inward_current_f_gate__f
+    1.0,  // second_inward_current_f2_gate__f2
+    8.0,  // intracellular_sodium_concentration__Nai
+    5e-05,  // intracellular_calcium_concentration__Cai
+    2.0,  // intracellular_calcium_concentration__Ca_up
+    1.0,  // intracellular_calcium_concentration__Ca_rel
+    1.0,  // intracellular_calcium_concentration__p
+    4.0,  // extracellular_potassium_concentration__Kc
+    140.0,  // intracellular_potassium_concentration__Ki
+};
+static const char* const kParamNames[4] = {
+    "fast_sodium_current__g_Na",
+    "time_independent_potassium_current__g_K1",
+    "second_inward_current__P_si",
+    "transient_outward_current__g_to",
+};
+static constexpr double kParamDefaults[4] = {
+    750.0,
+    920.0,
+    15.0,
+    0.28,
+};
+// 1 = integrated by Rush-Larsen (affine gate, V-dependent coefficients only)
+static constexpr int kIsGate[kNumStates] = {0, 1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 1, 0, 0};
+
+// Right-hand side with V clamped.  Non-gates (and, with RL=false, every
+// state) get dy[i]; with RL=true gate i gets g_inf[i], g_k[i] such that
+// y_i(t+dt) = g_inf[i] + (y_i - g_inf[i]) * exp(g_k[i] * dt).
+// i_stim_area is in uA/cm^2; cm is the tissue capacitance in uF/cm^2.
+template <class R, bool RL>
+CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
+                            R* dy, R* g_inf, R* g_k)
+{
+    CMX_MATH_USING
+    (void)p; (void)cm; (void)g_inf; (void)g_k; (void)i_stim_area;
+    const R membrane__R = R(8314.472);
+    const R membrane__T = R(310.0);
+    const R membrane__F = R(96485.3415);
+    const R hyperpolarising_activated_current__g_f_Na = R(3.0);
+    const R hyperpolarising_activated_current__g_f_K = R(3.0);
+    const R hyperpolarising_activated_current__Km_f = R(45.0);
+    const R hyperpolarising_activated_current_y_gate__delta_y = R(1e-05);
+    const R time_dependent_potassium_current__i_K_max = R(180.0);
+    const R time_independent_potassium_current__Km_K1 = R(210.0);
+    const R transient_outward_current__Km_to = R(10.0);
+    const R transient_outward_current__Km_Ca = R(0.0005);
+    const R sodium_background_current__g_Nab = R(0.18);
+    const R calcium_background_current__g_Cab = R(0.02);
+    const R sodium_potassium_pump__I_p = R(125.0);
+    const R sodium_potassium_pump__K_mK = R(1.0);
+    const R sodium_potassium_pump__K_mNa = R(40.0);
+    const R Na_Ca_exchanger__n_NaCa = R(3.0);
+    const R Na_Ca_exchanger__K_NaCa = R(0.02);
+    const R Na_Ca_exchanger__d_NaCa = R(0.001);
+    const R Na_Ca_exchanger__gamma = R(0.5);
+    const R fast_sodium_current_m_gate__delta_m = R(1e-05);
+    const R second_inward_current_d_gate__delta_d = R(0.0001);
+    const R second_inward_current_f_gate__delta_f = R(0.0001);
+    const R second_inward_current_f2_gate__alpha_f2 = R(5.0);
+    const R second_inward_current_f2_gate__K_mf2 = R(0.001);
+    const R extracellular_sodium_concentration__Nao = R(140.0);
+    const R intracellular_sodium_concentration__radius = R(0.05);
+    const R intracellular_sodium_concentration__length = R(2.0);
+    const R intracellular_sodium_concentration__V_e_ratio = R(0.1);
+    const R extracellular_calcium_concentration__Cao = R(2.0);
+    const R intracellular_calcium_concentration__Ca_up_max = R(5.0);
+    const R intracellular_calcium_concentration__K_mCa = R(0.001);
+    const R intracellular_calcium_concentration__tau_up = R(0.025);
+    const R intracellular_calcium_concentration__tau_rep = R(2.0);
+    const R intracellular_calcium_concentration__tau_rel = R(0.05);
+    const R intracellular_calcium_concentration__rCa = R(2.0);
+    const R extracellular_potassium_concentration__Ve = R(0.00157);
+    const R extracellular_potassium_concentration__Kb = R(4.0);
+    const R extracellular_potassium_concentration__pf = R(0.7);
+    const R i_stim = (i_stim_area / cm) * R(74.99999999999999);
+    (void)i_stim;
+    const R membrane__RTONF = ((membrane__R * membrane__T) * (R(1) / membrane__F));
+    const R Na_Ca_exchanger__i_NaCa = cmx_div((Na_Ca_exchanger__K_NaCa * ((cmx_exp_v(((Na_Ca_exchanger__gamma * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0) * pow(y[9], Na_Ca_exchanger__n_NaCa) * extracellular_calcium_concentration__Cao) - (cmx_exp_v((((Na_Ca_exchanger__gamma - R(1.0)) * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0) * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa) * y[10]))), ((R(1.0) + (Na_Ca_exchanger__d_NaCa * ((y[10] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(y[9], Na_Ca_exchanger__n_NaCa))))) * (R(1.0) + (y[10] * (R(1) / R(0.0069))))));
+    const R calcium_background_current__E_Ca = (R(0.5) * membrane__RTONF * cmx_log(cmx_div(extracellular_calcium_concentration__Cao, y[10])));
+    const R calcium_background_current__i_Ca_b = (calcium_background_current__g_Cab * (V - calcium_background_current__E_Ca));
+    const R hyperpolarising_activated_current__E_K = (membrane__RTONF * cmx_log(cmx_div(y[14], y[15])));
+    const R hyperpolarising_activated_current__i_fK = (cmx_div((y[1] * y[14]), (y[14] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_K * (V - hyperpolarising_activated_current__E_K));
+    const R second_inward_current__i_siK = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return (cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((y[15] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (y[14] * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7] * y[8]); });
+    const R sodium_potassium_pump__i_p = cmx_div((cmx_div((sodium_potassium_pump__I_p * y[14]), (sodium_potassium_pump__K_mK + y[14])) * y[9]), (sodium_potassium_pump__K_mNa + y[9]));
+    const R time_dependent_potassium_current__I_K = ((time_dependent_potassium_current__i_K_max * (y[15] - (y[14] * cmx_exp_v(((-V) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, -0.0)))) * (R(1) / R(140.0)));
+    const R time_dependent_potassium_current__i_K = (y[2] * time_dependent_potassium_current__I_K);
+    const R time_independent_potassium_current__i_K1 = cmx_div((cmx_div((p[1] * y[14]), (y[14] + time_independent_potassium_current__Km_K1)) * (V - hyperpolarising_activated_current__E_K)), (R(1.0) + cmx_exp_c(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF)), (((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)), 0.05400568865115377)));
+    const R transient_outward_current__i_to = cmx_remove_singularity(V, R(-9.999999999999991), R(4.999999999999996e-07), [&](const R V) { return (cmx_div((cmx_div((y[3] * p[3] * (R(0.2) + cmx_div(y[14], (transient_outward_current__Km_to + y[14]))) * y[10]), (transient_outward_current__Km_Ca + y[10])) * (V + R(10.0))), (-cmx_expm1(((-R(0.2)) * (V + R(10.0)))))) * ((y[15] * cmx_exp_v(((R(0.5) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0)) - (y[14] * cmx_exp_v((((-R(0.5)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0)))); });
+    const R extracellular_potassium_concentration__i_mK = ((time_independent_potassium_current__i_K1 + time_dependent_potassium_current__i_K + hyperpolarising_activated_current__i_fK + second_inward_current__i_siK + transient_outward_current__i_to) - (R(2.0) * sodium_potassium_pump__i_p));
+    const R fast_sodium_current__E_mh = (membrane__RTONF * cmx_log(cmx_div((extracellular_sodium_concentration__Nao + (R(0.12) * y[14])), (y[9] + (R(0.12) * y[15])))));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * (V - fast_sodium_current__E_mh));
+    const R fast_sodium_current_h_gate__alpha_h = (R(20.0) * cmx_exp_v(((-R(0.125)) * (V + R(75.0))), V, -0.18033688011112042, -13.525266008334032));
+    const R fast_sodium_current_h_gate__beta_h = cmx_div(R(2000.0), ((R(320.0) * cmx_exp_v(((-R(0.1)) * (V + R(75.0))), V, -0.14426950408889636, -10.820212806667225)) + R(1.0)));
+    const R fast_sodium_current_m_gate__E0_m = (V + R(41.0));
+    const R fast_sodium_current_m_gate__alpha_m = (((fabs(fast_sodium_current_m_gate__E0_m) < fast_sodium_current_m_gate__delta_m)) ? (R(2000.0)) : (cmx_div((R(200.0) * fast_sodium_current_m_gate__E0_m), (-cmx_expm1(((-R(0.1)) * fast_sodium_current_m_gate__E0_m))))));
+    const R fast_sodium_current_m_gate__beta_m = (R(8000.0) * cmx_exp_v(((-R(0.056)) * (V + R(66.0))), V, -0.08079092228978195, -5.332200871125609));
+    const R hyperpolarising_activated_current__E_Na = (membrane__RTONF * cmx_log(cmx_div(extracellular_sodium_concentration__Nao, y[9])));
+    const R hyperpolarising_activated_current__i_fNa = (cmx_div((y[1] * y[14]), (y[14] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_Na * (V - hyperpolarising_activated_current__E_Na));
+    const R hyperpolarising_activated_current_y_gate__alpha_y = (R(0.05) * cmx_exp_v(((-R(0.067)) * ((V + R(52.0)) - R(10.0))), V, -0.09666056773956055, -4.059743845061543));
+    const R hyperpolarising_activated_current_y_gate__E0_y = ((V + R(52.0)) - R(10.0));
+    const R hyperpolarising_activated_current_y_gate__beta_y = (((fabs(hyperpolarising_activated_current_y_gate__E0_y) < hyperpolarising_activated_current_y_gate__delta_y)) ? (R(2.5)) : (cmx_div((R(1.0) * hyperpolarising_activated_current_y_gate__E0_y), (-cmx_expm1(((-R(0.2)) * hyperpolarising_activated_current_y_gate__E0_y))))));
+    const R intracellular_sodium_concentration__V_Cell = (R(3.141592654) * cmx_ipow<2>(intracellular_sodium_concentration__radius) * intracellular_sodium_concentration__length);
+    const R intracellular_sodium_concentration__Vi = (intracellular_sodium_concentration__V_Cell * (R(1.0) - intracellular_sodium_concentration__V_e_ratio));
+    const R intracellular_calcium_concentration__V_rel = (intracellular_sodium_concentration__Vi * R(0.02));
+    const R intracellular_calcium_concentration__V_up = (intracellular_sodium_concentration__Vi * R(0.05));
+    const R intracellular_calcium_concentration__alpha_p = cmx_remove_singularity(V, R(-34.0), R(4e-07), [&](const R V) { return cmx_div((R(0.625) * (V + R(34.0))), cmx_expm1(((V + R(34.0)) * (R(1) / R(4.0))))); });
+    const R intracellular_calcium_concentration__beta_p = cmx_div(R(5.0), (R(1.0) + cmx_exp_v((((-R(1.0)) * (V + R(34.0))) * (R(1) / R(4.0))), V, -0.36067376022224085, -12.262907847556189)));
+    const R intracellular_calcium_concentration__i_rel = cmx_div((((R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_rel))) * y[12] * pow(y[10], intracellular_calcium_concentration__rCa)), (pow(y[10], intracellular_calcium_concentration__rCa) + pow(intracellular_calcium_concentration__K_mCa, intracellular_calcium_concentration__rCa)));
+    const R intracellular_calcium_concentration__i_tr = (((R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_rep))) * y[13] * (y[11] - y[12]));
+    const R intracellular_calcium_concentration__i_up = (((R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_up * intracellular_calcium_concentration__Ca_up_max))) * y[10] * (intracellular_calcium_concentration__Ca_up_max - y[11]));
+    const R second_inward_current__i_siCa = cmx_remove_singularity(V, R(50.0000000000001), R(1.335688032984785e-06), [&](const R V) { return (cmx_div((R(4.0) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0)) * R(2.0)) * (R(1) / membrane__RTONF)))))) * ((y[10] * cmx_exp((R(100.0) * (R(1) / membrane__RTONF)))) - (extracellular_calcium_concentration__Cao * cmx_exp_v((((-R(2.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.10801137730230755, 5.400568865115377))) * y[6] * y[7] * y[8]); });
+    const R second_inward_current__i_siNa = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return (cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((y[9] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (extracellular_sodium_concentration__Nao * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7] * y[8]); });
+    const R second_inward_current_d_gate__E0_d = ((V + R(24.0)) - R(5.0));
+    const R second_inward_current_d_gate__alpha_d = (((fabs(second_inward_current_d_gate__E0_d) < second_inward_current_d_gate__delta_d)) ? (R(120.0)) : (cmx_div((R(30.0) * second_inward_current_d_gate__E0_d), (-cmx_expm1((((-R(1.0)) * second_inward_current_d_gate__E0_d) * (R(1) / R(4.0))))))));
+    const R second_inward_current_d_gate__beta_d = (((fabs(second_inward_current_d_gate__E0_d) < second_inward_current_d_gate__delta_d)) ? (R(120.0)) : (cmx_div((R(12.0) * second_inward_current_d_gate__E0_d), cmx_expm1((second_inward_current_d_gate__E0_d * (R(1) / R(10.0)))))));
+    const R second_inward_current_f2_gate__beta_f2 = ((y[10] * second_inward_current_f2_gate__alpha_f2) * (R(1) / second_inward_current_f2_gate__K_mf2));
+    const R second_inward_current_f_gate__E0_f = (V + R(34.0));
+    const R second_inward_current_f_gate__alpha_f = (((fabs(second_inward_current_f_gate__E0_f) < second_inward_current_f_gate__delta_f)) ? (R(25.0)) : (cmx_div((R(6.25) * second_inward_current_f_gate__E0_f), cmx_expm1((second_inward_current_f_gate__E0_f * (R(1) / R(4.0)))))));
+    const R second_inward_current_f_gate__beta_f = cmx_div(R(50.0), (R(1.0) + cmx_exp_v((((-R(1.0)) * (V + R(34.0))) * (R(1) / R(4.0))), V, -0.36067376022224085, -12.262907847556189)));
+    const R sodium_background_current__i_Na_b = (sodium_background_current__g_Nab * (V - hyperpolarising_activated_current__E_Na));
+    const R time_dependent_potassium_current_x_gate__alpha_x = cmx_div((R(0.5) * cmx_exp_v((R(0.0826) * (V + R(50.0))), V, 0.11916661037742839, 5.95833051887142)), (R(1.0) + cmx_exp_v((R(0.057) * (V + R(50.0))), V, 0.08223361733067092, 4.111680866533546)));
+    const R time_dependent_potassium_current_x_gate__beta_x = cmx_div((R(1.3) * cmx_exp_v(((-R(0.06)) * (V + R(20.0))), V, -0.0865617024533378, -1.731234049066756)), (R(1.0) + cmx_exp_v(((-R(0.04)) * (V + R(20.0))), V, -0.05770780163555854, -1.1541560327111708)));
+    const R transient_outward_current_s_gate__alpha_s = (R(0.033) * cmx_exp_v(((-V) * (R(1) / R(17.0))), V, -0.08486441416993903, -0.0));
+    const R transient_outward_current_s_gate__beta_s = cmx_div(R(33.0), (R(1.0) + cmx_exp_v(((-(V + R(10.0))) * (R(1) / R(8.0))), V, -0.18033688011112042, -1.8033688011112043)));
+    if (RL) {
+        g_k[1] = cmx_div(((hyperpolarising_activated_current_y_gate__alpha_y * (-R(1))) - (hyperpolarising_activated_current_y_gate__beta_y * R(1))), R(1000.0));
+        g_inf[1] = cmx_div(-((hyperpolarising_activated_current_y_gate__alpha_y * R(1.0))), ((hyperpolarising_activated_current_y_gate__alpha_y * (-R(1))) - (hyperpolarising_activated_current_y_gate__beta_y * R(1))));
+    } else {
+        dy[1] = (((hyperpolarising_activated_current_y_gate__alpha_y * (R(1.0) - y[1])) - (hyperpolarising_activated_current_y_gate__beta_y * y[1])) * (R(1) / R(1000.0)));
+    }
+    if (RL) {
+        g_k[2] = cmx_div(((time_dependent_potassium_current_x_gate__alpha_x * (-R(1))) - (time_dependent_potassium_current_x_gate__beta_x * R(1))), R(1000.0));
+        g_inf[2] = cmx_div(-((time_dependent_potassium_current_x_gate__alpha_x * R(1.0))), ((time_dependent_potassium_current_x_gate__alpha_x * (-R(1))) - (time_dependent_potassium_current_x_gate__beta_x * R(1))));
+    } else {
+        dy[2] = (((time_dependent_potassium_current_x_gate__alpha_x * (R(1.0) - y[2])) - (time_dependent_potassium_current_x_gate__beta_x * y[2])) * (R(1) / R(1000.0)));
+    }
+    if (RL) {
+        g_k[3] = cmx_div(((transient_outward_current_s_gate__alpha_s * (-R(1))) - (transient_outward_current_s_gate__beta_s * R(1))), R(1000.0));
+        g_inf[3] = cmx_div(-((transient_outward_current_s_gate__alpha_s * R(1.0))), ((transient_outward_current_s_gate__alpha_s * (-R(1))) - (transient_outward_current_s_gate__beta_s * R(1))));
+    } else {
+        dy[3] = (((transient_outward_current_s_gate__alpha_s * (R(1.0) - y[3])) - (transient_outward_current_s_gate__beta_s * y[3])) * (R(1) / R(1000.0)));
+    }
+    if (RL) {
+        g_k[4] = cmx_div(((fast_sodium_current_m_gate__alpha_m * (-R(1))) - (fast_sodium_current_m_gate__beta_m * R(1))), R(1000.0));
+        g_inf[4] = cmx_div(-((fast_sodium_current_m_gate__alpha_m * R(1.0))), ((fast_sodium_current_m_gate__alpha_m * (-R(1))) - (fast_sodium_current_m_gate__beta_m * R(1))));
+    } else {
+        dy[4] = (((fast_sodium_current_m_gate__alpha_m * (R(1.0) - y[4])) - (fast_sodium_current_m_gate__beta_m * y[4])) * (R(1) / R(1000.0)));
+    }
+    if (RL) {
+        g_k[5] = cmx_div(((fast_sodium_current_h_gate__alpha_h * (-R(1))) - (fast_sodium_current_h_gate__beta_h * R(1))), R(1000.0));
+        g_inf[5] = cmx_div(-((fast_sodium_current_h_gate__alpha_h * R(1.0))), ((fast_sodium_current_h_gate__alpha_h * (-R(1))) - (fast_sodium_current_h_gate__beta_h * R(1))));
+    } else {
+        dy[5] = (((fast_sodium_current_h_gate__alpha_h * (R(1.0) - y[5])) - (fast_sodium_current_h_gate__beta_h * y[5])) * (R(1) / R(1000.0)));
+    }
+    if (RL) {
+        g_k[6] = cmx_div(((second_inward_current_d_gate__alpha_d * (-R(1))) - (second_inward_current_d_gate__beta_d * R(1))), R(1000.0));
+        g_inf[6] = cmx_div(-((second_inward_current_d_gate__alpha_d * R(1.0))), ((second_inward_current_d_gate__alpha_d * (-R(1))) - (second_inward_current_d_gate__beta_d * R(1))));
+    } else {
+        dy[6] = (((second_inward_current_d_gate__alpha_d * (R(1.0) - y[6])) - (second_inward_current_d_gate__beta_d * y[6])) * (R(1) / R(1000.0)));
+    }
+    if (RL) {
+        g_k[7] = cmx_div(((second_inward_current_f_gate__alpha_f * (-R(1))) - (second_inward_current_f_gate__beta_f * R(1))), R(1000.0));
+        g_inf[7] = cmx_div(-((second_inward_current_f_gate__alpha_f * R(1.0))), ((second_inward_current_f_gate__alpha_f * (-R(1))) - (second_inward_current_f_gate__beta_f * R(1))));
+    } else {
+        dy[7] = (((second_inward_current_f_gate__alpha_f * (R(1.0) - y[7])) - (second_inward_current_f_gate__beta_f * y[7])) * (R(1) / R(1000.0)));
+    }
+    dy[8] = ((second_inward_current_f2_gate__alpha_f2 - (y[8] * (second_inward_current_f2_gate__alpha_f2 + second_inward_current_f2_gate__beta_f2))) * (R(1) / R(1000.0)));
+    dy[9] = ((((-R(1.0)) * (fast_sodium_current__i_Na + sodium_background_current__i_Na_b + hyperpolarising_activated_current__i_fNa + second_inward_current__i_siNa + (sodium_potassium_pump__i_p * R(3.0)) + ((Na_Ca_exchanger__i_NaCa * Na_Ca_exchanger__n_NaCa) * (R(1) / (Na_Ca_exchanger__n_NaCa - R(2.0)))))) * (R(1) / (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F))) * (R(1) / R(1000.0)));
+    dy[10] = ((((-R(1.0)) * ((((second_inward_current__i_siCa + calcium_background_current__i_Ca_b) - ((R(2.0) * Na_Ca_exchanger__i_NaCa) * (R(1) / (Na_Ca_exchanger__n_NaCa - R(2.0))))) - intracellular_calcium_concentration__i_rel) + intracellular_calcium_concentration__i_up)) * (R(1) / (R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F))) * (R(1) / R(1000.0)));
+    dy[11] = (((R(1.0) * (intracellular_calcium_concentration__i_up - intracellular_calcium_concentration__i_tr)) * (R(1) / (R(2.0) * R(1.0) * intracellular_calcium_concentration__V_up * membrane__F))) * (R(1) / R(1000.0)));
+    dy[12] = (((R(1.0) * (intracellular_calcium_concentration__i_tr - intracellular_calcium_concentration__i_rel)) * (R(1) / (R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F))) * (R(1) / R(1000.0)));
+    if (RL) {
+        g_k[13] = cmx_div(((intracellular_calcium_concentration__alpha_p * (-R(1))) - (intracellular_calcium_concentration__beta_p * R(1))), R(1000.0));
+        g_inf[13] = cmx_div(-((intracellular_calcium_concentration__alpha_p * R(1.0))), ((intracellular_calcium_concentration__alpha_p * (-R(1))) - (intracellular_calcium_concentration__beta_p * R(1))));
+    } else {
+        dy[13] = (((intracellular_calcium_concentration__alpha_p * (R(1.0) - y[13])) - (intracellular_calcium_concentration__beta_p * y[13])) * (R(1) / R(1000.0)));
+    }
+    dy[14] = ((((-extracellular_potassium_concentration__pf) * (y[14] - extracellular_potassium_concentration__Kb)) + ((R(1.0) * extracellular_potassium_concentration__i_mK) * (R(1) / (R(1.0) * extracellular_potassium_concentration__Ve * membrane__F)))) * (R(1) / R(1000.0)));
+    dy[15] = ((((-R(1.0)) * extracellular_potassium_concentration__i_mK) * (R(1) / (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F))) * (R(1) / R(1000.0)));
+}
+
+// dV/dt including the stimulus (single-cell integration).
+template <class R>
+CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
+{
+    CMX_MATH_USING
+    (void)p; (void)cm;
+    const R membrane__R = R(8314.472);
+    const R membrane__T = R(310.0);
+    const R membrane__F = R(96485.3415);
+    const R membrane__C = R(0.075);
+    const R membrane__i_pulse = R(0.0);
+    const R hyperpolarising_activated_current__g_f_Na = R(3.0);
+    const R hyperpolarising_activated_current__g_f_K = R(3.0);
+    const R hyperpolarising_activated_current__Km_f = R(45.0);
+    const R time_dependent_potassium_current__i_K_max = R(180.0);
+    const R time_independent_potassium_current__Km_K1 = R(210.0);
+    const R transient_outward_current__Km_to = R(10.0);
+    const R transient_outward_current__Km_Ca = R(0.0005);
+    const R sodium_background_current__g_Nab = R(0.18);
+    const R calcium_background_current__g_Cab = R(0.02);
+    const R sodium_potassium_pump__I_p = R(125.0);
+    const R sodium_potassium_pump__K_mK = R(1.0);
+    const R sodium_potassium_pump__K_mNa = R(40.0);
+    const R Na_Ca_exchanger__n_NaCa = R(3.0);
+    const R Na_Ca_exchanger__K_NaCa = R(0.02);
+    const R Na_Ca_exchanger__d_NaCa = R(0.001);
+    const R Na_Ca_exchanger__gamma = R(0.5);
+    const R extracellular_sodium_concentration__Nao = R(140.0);
+    const R extracellular_calcium_concentration__Cao = R(2.0);
+    const R i_stim = (i_stim_area / cm) * R(74.99999999999999);
+    (void)i_stim;
+    const R membrane__RTONF = ((membrane__R * membrane__T) * (R(1) / membrane__F));
+    const R Na_Ca_exchanger__i_NaCa = cmx_div((Na_Ca_exchanger__K_NaCa * ((cmx_exp_v(((Na_Ca_exchanger__gamma * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0) * pow(y[9], Na_Ca_exchanger__n_NaCa) * extracellular_calcium_concentration__Cao) - (cmx_exp_v((((Na_Ca_exchanger__gamma - R(1.0)) * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0) * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa) * y[10]))), ((R(1.0) + (Na_Ca_exchanger__d_NaCa * ((y[10] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(y[9], Na_Ca_exchanger__n_NaCa))))) * (R(1.0) + (y[10] * (R(1) / R(0.0069))))));
+    const R calcium_background_current__E_Ca = (R(0.5) * membrane__RTONF * cmx_log(cmx_div(extracellular_calcium_concentration__Cao, y[10])));
+    const R calcium_background_current__i_Ca_b = (calcium_background_current__g_Cab * (V - calcium_background_current__E_Ca));
+    const R fast_sodium_current__E_mh = (membrane__RTONF * cmx_log(cmx_div((extracellular_sodium_concentration__Nao + (R(0.12) * y[14])), (y[9] + (R(0.12) * y[15])))));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * (V - fast_sodium_current__E_mh));
+    const R hyperpolarising_activated_current__E_K = (membrane__RTONF * cmx_log(cmx_div(y[14], y[15])));
+    const R hyperpolarising_activated_current__i_fK = (cmx_div((y[1] * y[14]), (y[14] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_K * (V - hyperpolarising_activated_current__E_K));
+    const R hyperpolarising_activated_current__E_Na = (membrane__RTONF * cmx_log(cmx_div(extracellular_sodium_concentration__Nao, y[9])));
+    const R hyperpolarising_activated_current__i_fNa = (cmx_div((y[1] * y[14]), (y[14] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_Na * (V - hyperpolarising_activated_current__E_Na));
+    const R hyperpolarising_activated_current__i_f = (hyperpolarising_activated_current__i_fNa + hyperpolarising_activated_current__i_fK);
+    const R second_inward_current__i_siCa = cmx_remove_singularity(V, R(50.0000000000001), R(1.335688032984785e-06), [&](const R V) { return (cmx_div((R(4.0) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0)) * R(2.0)) * (R(1) / membrane__RTONF)))))) * ((y[10] * cmx_exp((R(100.0) * (R(1) / membrane__RTONF)))) - (extracellular_calcium_concentration__Cao * cmx_exp_v((((-R(2.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.10801137730230755, 5.400568865115377))) * y[6] * y[7] * y[8]); });
+    const R second_inward_current__i_siK = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return (cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((y[15] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (y[14] * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7] * y[8]); });
+    const R second_inward_current__i_siNa = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return (cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((y[9] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (extracellular_sodium_concentration__Nao * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7] * y[8]); });
+    const R second_inward_current__i_si = (second_inward_current__i_siCa + second_inward_current__i_siK + second_inward_current__i_siNa);
+    const R sodium_background_current__i_Na_b = (sodium_background_current__g_Nab * (V - hyperpolarising_activated_current__E_Na));
+    const R sodium_potassium_pump__i_p = cmx_div((cmx_div((sodium_potassium_pump__I_p * y[14]), (sodium_potassium_pump__K_mK + y[14])) * y[9]), (sodium_potassium_pump__K_mNa + y[9]));
+    const R time_dependent_potassium_current__I_K = ((time_dependent_potassium_current__i_K_max * (y[15] - (y[14] * cmx_exp_v(((-V) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, -0.0)))) * (R(1) / R(140.0)));
+    const R time_dependent_potassium_current__i_K = (y[2] * time_dependent_potassium_current__I_K);
+    const R time_independent_potassium_current__i_K1 = cmx_div((cmx_div((p[1] * y[14]), (y[14] + time_independent_potassium_current__Km_K1)) * (V - hyperpolarising_activated_current__E_K)), (R(1.0) + cmx_exp_c(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF)), (((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)), 0.05400568865115377)));
+    const R transient_outward_current__i_to = cmx_remove_singularity(V, R(-9.999999999999991), R(4.999999999999996e-07), [&](const R V) { return (cmx_div((cmx_div((y[3] * p[3] * (R(0.2) + cmx_div(y[14], (transient_outward_current__Km_to + y[14]))) * y[10]), (transient_outward_current__Km_Ca + y[10])) * (V + R(10.0))), (-cmx_expm1(((-R(0.2)) * (V + R(10.0)))))) * ((y[15] * cmx_exp_v(((R(0.5) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0)) - (y[14] * cmx_exp_v((((-R(0.5)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0)))); });
+    return (((-(hyperpolarising_activated_current__i_f + time_dependent_potassium_current__i_K + time_independent_potassium_current__i_K1 + transient_outward_current__i_to + sodium_background_current__i_Na_b + calcium_background_current__i_Ca_b + sodium_potassium_pump__i_p + Na_Ca_exchanger__i_NaCa + fast_sodium_current__i_Na + second_inward_current__i_si + i_stim)) * (R(1) / membrane__C)) * (R(1) / R(1000.0)));
+}
+
+// Total ionic current in uA/cm^2 (GetIIonic): minus dV/dt at zero stimulus,
+// times the membrane capacitance that converts it to a current density.
+template <class R>
+CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
+{
+    CMX_MATH_USING
+    (void)p; (void)cm;
+    const R membrane__R = R(8314.472);
+    const R membrane__T = R(310.0);
+    const R membrane__F = R(96485.3415);
+    const R membrane__C = R(0.075);
+    const R membrane__i_pulse = R(0.0);
+    const R hyperpolarising_activated_current__g_f_Na = R(3.0);
+    const R hyperpolarising_activated_current__g_f_K = R(3.0);
+    const R hyperpolarising_activated_current__Km_f = R(45.0);
+    const R time_dependent_potassium_current__i_K_max = R(180.0);
+    const R time_independent_potassium_current__Km_K1 = R(210.0);
+    const R transient_outward_current__Km_to = R(10.0);
+    const R transient_outward_current__Km_Ca = R(0.0005);
+    const R sodium_background_current__g_Nab = R(0.18);
+    const R calcium_background_current__g_Cab = R(0.02);
+    const R sodium_potassium_pump__I_p = R(125.0);
+    const R sodium_potassium_pump__K_mK = R(1.0);
+    const R sodium_potassium_pump__K_mNa = R(40.0);
+    const R Na_Ca_exchanger__n_NaCa = R(3.0);
+    const R Na_Ca_exchanger__K_NaCa = R(0.02);
+    const R Na_Ca_exchanger__d_NaCa = R(0.001);
+    const R Na_Ca_exchanger__gamma = R(0.5);
+    const R extracellular_sodium_concentration__Nao = R(140.0);
+    const R extracellular_calcium_concentration__Cao = R(2.0);
+    const R i_stim = R(0);
+    (void)i_stim;
+    const R membrane__RTONF = ((membrane__R * membrane__T) * (R(1) / membrane__F));
+    const R Na_Ca_exchanger__i_NaCa = cmx_div((Na_Ca_exchanger__K_NaCa * ((cmx_exp_v(((Na_Ca_exchanger__gamma * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0) * pow(y[9], Na_Ca_exchanger__n_NaCa) * extracellular_calcium_concentration__Cao) - (cmx_exp_v((((Na_Ca_exchanger__gamma - R(1.0)) * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0) * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa) * y[10]))), ((R(1.0) + (Na_Ca_exchanger__d_NaCa * ((y[10] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(y[9], Na_Ca_exchanger__n_NaCa))))) * (R(1.0) + (y[10] * (R(1) / R(0.0069))))));
+    const R calcium_background_current__E_Ca = (R(0.5) * membrane__RTONF * cmx_log(cmx_div(extracellular_calcium_concentration__Cao, y[10])));
+    const R calcium_background_current__i_Ca_b = (calcium_background_current__g_Cab * (V - calcium_background_current__E_Ca));
+    const R fast_sodium_current__E_mh = (membrane__RTONF * cmx_log(cmx_div((extracellular_sodium_concentration__Nao + (R(0.12) * y[14])), (y[9] + (R(0.12) * y[15])))));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * (V - fast_sodium_current__E_mh));
+    const R hyperpolarising_activated_current__E_K = (membrane__RTONF * cmx_log(cmx_div(y[14], y[15])));
+    const R hyperpolarising_activated_current__i_fK = (cmx_div((y[1] * y[14]), (y[14] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_K * (V - hyperpolarising_activated_current__E_K));
+    const R hyperpolarising_activated_current__E_Na = (membrane__RTONF * cmx_log(cmx_div(extracellular_sodium_concentration__Nao, y[9])));
+    const R hyperpolarising_activated_current__i_fNa = (cmx_div((y[1] * y[14]), (y[14] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_Na * (V - hyperpolarising_activated_current__E_Na));
+    const R hyperpolarising_activated_current__i_f = (hyperpolarising_activated_current__i_fNa + hyperpolarising_activated_current__i_fK);
+    const R second_inward_current__i_siCa = cmx_remove_singularity(V, R(50.0000000000001), R(1.335688032984785e-06), [&](const R V) { return (cmx_div((R(4.0) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0)) * R(2.0)) * (R(1) / membrane__RTONF)))))) * ((y[10] * cmx_exp((R(100.0) * (R(1) / membrane__RTONF)))) - (extracellular_calcium_concentration__Cao * cmx_exp_v((((-R(2.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.10801137730230755, 5.400568865115377))) * y[6] * y[7] * y[8]); });
+    const R second_inward_current__i_siK = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return (cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((y[15] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (y[14] * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7] * y[8]); });
+    const R second_inward_current__i_siNa = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return (cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((y[9] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (extracellular_sodium_concentration__Nao * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7] * y[8]); });
+    const R second_inward_current__i_si = (second_inward_current__i_siCa + second_inward_current__i_siK + second_inward_current__i_siNa);
+    const R sodium_background_current__i_Na_b = (sodium_background_current__g_Nab * (V - hyperpolarising_activated_current__E_Na));
+    const R sodium_potassium_pump__i_p = cmx_div((cmx_div((sodium_potassium_pump__I_p * y[14]), (sodium_potassium_pump__K_mK + y[14])) * y[9]), (sodium_potassium_pump__K_mNa + y[9]));
+    const R time_dependent_potassium_current__I_K = ((time_dependent_potassium_current__i_K_max * (y[15] - (y[14] * cmx_exp_v(((-V) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, -0.0)))) * (R(1) / R(140.0)));
+    const R time_dependent_potassium_current__i_K = (y[2] * time_dependent_potassium_current__I_K);
+    const R time_independent_potassium_current__i_K1 = cmx_div((cmx_div((p[1] * y[14]), (y[14] + time_independent_potassium_current__Km_K1)) * (V - hyperpolarising_activated_current__E_K)), (R(1.0) + cmx_exp_c(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF)), (((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)), 0.05400568865115377)));
+    const R transient_outward_current__i_to = cmx_remove_singularity(V, R(-9.999999999999991), R(4.999999999999996e-07), [&](const R V) { return (cmx_div((cmx_div((y[3] * p[3] * (R(0.2) + cmx_div(y[14], (transient_outward_current__Km_to + y[14]))) * y[10]), (transient_outward_current__Km_Ca + y[10])) * (V + R(10.0))), (-cmx_expm1(((-R(0.2)) * (V + R(10.0)))))) * ((y[15] * cmx_exp_v(((R(0.5) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0)) - (y[14] * cmx_exp_v((((-R(0.5)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0)))); });
+    const R dv = (((-(hyperpolarising_activated_current__i_f + time_dependent_potassium_current__i_K + time_independent_potassium_current__i_K1 + transient_outward_current__i_to + sodium_background_current__i_Na_b + calcium_background_current__i_Ca_b + sodium_potassium_pump__i_p + Na_Ca_exchanger__i_NaCa + fast_sodium_current__i_Na + second_inward_current__i_si + i_stim)) * (R(1) / membrane__C)) * (R(1) / R(1000.0)));
+    return -dv * cm;
+}
+
+// Backward Euler: the states solved by Newton (all but the linear gates).
+constexpr int kNumNonlinear = 7;
+static constexpr int kNonlinear[7] = {8, 9, 10, 11, 12, 14, 15};
+
+// Linear states: y <- (y + a dt) / (1 - b dt) for dy/dt = a + b y, a and b
+// functions of V (ComputeOneStepExceptVoltage, closed-form part).
+template <class R>
+CMX_HD inline void be_linear(R* y, R V, const R* p, R cm, R dt)
+{
+    CMX_MATH_USING
+    (void)p; (void)cm; (void)V;
+    const R hyperpolarising_activated_current_y_gate__delta_y = R(1e-05);
+    const R fast_sodium_current_m_gate__delta_m = R(1e-05);
+    const R second_inward_current_d_gate__delta_d = R(0.0001);
+    const R second_inward_current_f_gate__delta_f = R(0.0001);
+    const R fast_sodium_current_h_gate__alpha_h = (R(20.0) * cmx_exp_v(((-R(0.125)) * (V + R(75.0))), V, -0.18033688011112042, -13.525266008334032));
+    const R fast_sodium_current_h_gate__beta_h = cmx_div(R(2000.0), ((R(320.0) * cmx_exp_v(((-R(0.1)) * (V + R(75.0))), V, -0.14426950408889636, -10.820212806667225)) + R(1.0)));
+    const R fast_sodium_current_m_gate__E0_m = (V + R(41.0));
+    const R fast_sodium_current_m_gate__alpha_m = (((fabs(fast_sodium_current_m_gate__E0_m) < fast_sodium_current_m_gate__delta_m)) ? (R(2000.0)) : (cmx_div((R(200.0) * fast_sodium_current_m_gate__E0_m), (-cmx_expm1(((-R(0.1)) * fast_sodium_current_m_gate__E0_m))))));
+    const R fast_sodium_current_m_gate__beta_m = (R(8000.0) * cmx_exp_v(((-R(0.056)) * (V + R(66.0))), V, -0.08079092228978195, -5.332200871125609));
+    const R hyperpolarising_activated_current_y_gate__alpha_y = (R(0.05) * cmx_exp_v(((-R(0.067)) * ((V + R(52.0)) - R(10.0))), V, -0.09666056773956055, -4.059743845061543));
+    const R hyperpolarising_activated_current_y_gate__E0_y = ((V + R(52.0)) - R(10.0));
+    const R hyperpolarising_activated_current_y_gate__beta_y = (((fabs(hyperpolarising_activated_current_y_gate__E0_y) < hyperpolarising_activated_current_y_gate__delta_y)) ? (R(2.5)) : (cmx_div((R(1.0) * hyperpolarising_activated_current_y_gate__E0_y), (-cmx_expm1(((-R(0.2)) * hyperpolarising_activated_current_y_gate__E0_y))))));
+    const R intracellular_calcium_concentration__alpha_p = cmx_remove_singularity(V, R(-34.0), R(4e-07), [&](const R V) { return cmx_div((R(0.625) * (V + R(34.0))), cmx_expm1(((V + R(34.0)) * (R(1) / R(4.0))))); });
+    const R intracellular_calcium_concentration__beta_p = cmx_div(R(5.0), (R(1.0) + cmx_exp_v((((-R(1.0)) * (V + R(34.0))) * (R(1) / R(4.0))), V, -0.36067376022224085, -12.262907847556189)));
+    const R second_inward_current_d_gate__E0_d = ((V + R(24.0)) - R(5.0));
+    const R second_inward_current_d_gate__alpha_d = (((fabs(second_inward_current_d_gate__E0_d) < second_inward_current_d_gate__delta_d)) ? (R(120.0)) : (cmx_div((R(30.0) * second_inward_current_d_gate__E0_d), (-cmx_expm1((((-R(1.0)) * second_inward_current_d_gate__E0_d) * (R(1) / R(4.0))))))));
+    const R second_inward_current_d_gate__beta_d = (((fabs(second_inward_current_d_gate__E0_d) < second_inward_current_d_gate__delta_d)) ? (R(120.0)) : (cmx_div((R(12.0) * second_inward_current_d_gate__E0_d), cmx_expm1((second_inward_current_d_gate__E0_d * (R(1) / R(10.0)))))));
+    const R second_inward_current_f_gate__E0_f = (V + R(34.0));
+    const R second_inward_current_f_gate__alpha_f = (((fabs(second_inward_current_f_gate__E0_f) < second_inward_current_f_gate__delta_f)) ? (R(25.0)) : (cmx_div((R(6.25) * second_inward_current_f_gate__E0_f), cmx_expm1((second_inward_current_f_gate__E0_f * (R(1) / R(4.0)))))));
+    const R second_inward_current_f_gate__beta_f = cmx_div(R(50.0), (R(1.0) + cmx_exp_v((((-R(1.0)) * (V + R(34.0))) * (R(1) / R(4.0))), V, -0.36067376022224085, -12.262907847556189)));
+    const R time_dependent_potassium_current_x_gate__alpha_x = cmx_div((R(0.5) * cmx_exp_v((R(0.0826) * (V + R(50.0))), V, 0.11916661037742839, 5.95833051887142)), (R(1.0) + cmx_exp_v((R(0.057) * (V + R(50.0))), V, 0.08223361733067092, 4.111680866533546)));
+    const R time_dependent_potassium_current_x_gate__beta_x = cmx_div((R(1.3) * cmx_exp_v(((-R(0.06)) * (V + R(20.0))), V, -0.0865617024533378, -1.731234049066756)), (R(1.0) + cmx_exp_v(((-R(0.04)) * (V + R(20.0))), V, -0.05770780163555854, -1.1541560327111708)));
+    const R transient_outward_current_s_gate__alpha_s = (R(0.033) * cmx_exp_v(((-V) * (R(1) / R(17.0))), V, -0.08486441416993903, -0.0));
+    const R transient_outward_current_s_gate__beta_s = cmx_div(R(33.0), (R(1.0) + cmx_exp_v(((-(V + R(10.0))) * (R(1) / R(8.0))), V, -0.18033688011112042, -1.8033688011112043)));
+    y[1] = (y[1] + (((hyperpolarising_activated_current_y_gate__alpha_y * R(1.0)) / R(1000.0))) * dt) / (R(1) - ((((hyperpolarising_activated_current_y_gate__alpha_y * (-R(1))) - (hyperpolarising_activated_current_y_gate__beta_y * R(1))) / R(1000.0))) * dt);
+    y[2] = (y[2] + (((time_dependent_potassium_current_x_gate__alpha_x * R(1.0)) / R(1000.0))) * dt) / (R(1) - ((((time_dependent_potassium_current_x_gate__alpha_x * (-R(1))) - (time_dependent_potassium_current_x_gate__beta_x * R(1))) / R(1000.0))) * dt);
+    y[3] = (y[3] + (((transient_outward_current_s_gate__alpha_s * R(1.0)) / R(1000.0))) * dt) / (R(1) - ((((transient_outward_current_s_gate__alpha_s * (-R(1))) - (transient_outward_current_s_gate__beta_s * R(1))) / R(1000.0))) * dt);
+    y[4] = (y[4] + (((fast_sodium_current_m_gate__alpha_m * R(1.0)) / R(1000.0))) * dt) / (R(1) - ((((fast_sodium_current_m_gate__alpha_m * (-R(1))) - (fast_sodium_current_m_gate__beta_m * R(1))) / R(1000.0))) * dt);
+    y[5] = (y[5] + (((fast_sodium_current_h_gate__alpha_h * R(1.0)) / R(1000.0))) * dt) / (R(1) - ((((fast_sodium_current_h_gate__alpha_h * (-R(1))) - (fast_sodium_current_h_gate__beta_h * R(1))) / R(1000.0))) * dt);
+    y[6] = (y[6] + (((second_inward_current_d_gate__alpha_d * R(1.0)) / R(1000.0))) * dt) / (R(1) - ((((second_inward_current_d_gate__alpha_d * (-R(1))) - (second_inward_current_d_gate__beta_d * R(1))) / R(1000.0))) * dt);
+    y[7] = (y[7] + (((second_inward_current_f_gate__alpha_f * R(1.0)) / R(1000.0))) * dt) / (R(1) - ((((second_inward_current_f_gate__alpha_f * (-R(1))) - (second_inward_current_f_gate__beta_f * R(1))) / R(1000.0))) * dt);
+    y[13] = (y[13] + (((intracellular_calcium_concentration__alpha_p * R(1.0)) / R(1000.0))) * dt) / (R(1) - ((((intracellular_calcium_concentration__alpha_p * (-R(1))) - (intracellular_calcium_concentration__beta_p * R(1))) / R(1000.0))) * dt);
+}
+
+// Newton residual of the nonlinear states at guess g (y holds the other
+// states, linear ones already advanced): g - y_old - dt f(g).
+template <class R>
+CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const R* p, R cm, R dt, R* res)
+{
+    CMX_MATH_USING
+    (void)y; (void)p; (void)cm; (void)i_stim_area;
+    const R membrane__R = R(8314.472);
+    const R membrane__T = R(310.0);
+    const R membrane__F = R(96485.3415);
+    const R hyperpolarising_activated_current__g_f_Na = R(3.0);
+    const R hyperpolarising_activated_current__g_f_K = R(3.0);
+    const R hyperpolarising_activated_current__Km_f = R(45.0);
+    const R time_dependent_potassium_current__i_K_max = R(180.0);
+    const R time_independent_potassium_current__Km_K1 = R(210.0);
+    const R transient_outward_current__Km_to = R(10.0);
+    const R transient_outward_current__Km_Ca = R(0.0005);
+    const R sodium_background_current__g_Nab = R(0.18);
+    const R calcium_background_current__g_Cab = R(0.02);
+    const R sodium_potassium_pump__I_p = R(125.0);
+    const R sodium_potassium_pump__K_mK = R(1.0);
+    const R sodium_potassium_pump__K_mNa = R(40.0);
+    const R Na_Ca_exchanger__n_NaCa = R(3.0);
+    const R Na_Ca_exchanger__K_NaCa = R(0.02);
+    const R Na_Ca_exchanger__d_NaCa = R(0.001);
+    const R Na_Ca_exchanger__gamma = R(0.5);
+    const R second_inward_current_f2_gate__alpha_f2 = R(5.0);
+    const R second_inward_current_f2_gate__K_mf2 = R(0.001);
+    const R extracellular_sodium_concentration__Nao = R(140.0);
+    const R intracellular_sodium_concentration__radius = R(0.05);
+    const R intracellular_sodium_concentration__length = R(2.0);
+    const R intracellular_sodium_concentration__V_e_ratio = R(0.1);
+    const R extracellular_calcium_concentration__Cao = R(2.0);
+    const R intracellular_calcium_concentration__Ca_up_max = R(5.0);
+    const R intracellular_calcium_concentration__K_mCa = R(0.001);
+    const R intracellular_calcium_concentration__tau_up = R(0.025);
+    const R intracellular_calcium_concentration__tau_rep = R(2.0);
+    const R intracellular_calcium_concentration__tau_rel = R(0.05);
+    const R intracellular_calcium_concentration__rCa = R(2.0);
+    const R extracellular_potassium_concentration__Ve = R(0.00157);
+    const R extracellular_potassium_concentration__Kb = R(4.0);
+    const R extracellular_potassium_concentration__pf = R(0.7);
+    const R i_stim = (i_stim_area / cm) * R(74.99999999999999);
+    (void)i_stim;
+    const R membrane__RTONF = ((membrane__R * membrane__T) * (R(1) / membrane__F));
+    const R Na_Ca_exchanger__i_NaCa = cmx_div((Na_Ca_exchanger__K_NaCa * ((cmx_exp_v(((Na_Ca_exchanger__gamma * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0) * pow(g[1], Na_Ca_exchanger__n_NaCa) * extracellular_calcium_concentration__Cao) - (cmx_exp_v((((Na_Ca_exchanger__gamma - R(1.0)) * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0) * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa) * g[2]))), ((R(1.0) + (Na_Ca_exchanger__d_NaCa * ((g[2] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(g[1], Na_Ca_exchanger__n_NaCa))))) * (R(1.0) + (g[2] * (R(1) / R(0.0069))))));
+    const R calcium_background_current__E_Ca = (R(0.5) * membrane__RTONF * cmx_log(cmx_div(extracellular_calcium_concentration__Cao, g[2])));
+    const R calcium_background_current__i_Ca_b = (calcium_background_current__g_Cab * (V - calcium_background_current__E_Ca));
+    const R hyperpolarising_activated_current__E_K = (membrane__RTONF * cmx_log(cmx_div(g[5], g[6])));
+    const R hyperpolarising_activated_current__i_fK = (cmx_div((y[1] * g[5]), (g[5] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_K * (V - hyperpolarising_activated_current__E_K));
+    const R second_inward_current__i_siK = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return (cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((g[6] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (g[5] * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7] * g[0]); });
+    const R sodium_potassium_pump__i_p = cmx_div((cmx_div((sodium_potassium_pump__I_p * g[5]), (sodium_potassium_pump__K_mK + g[5])) * g[1]), (sodium_potassium_pump__K_mNa + g[1]));
+    const R time_dependent_potassium_current__I_K = ((time_dependent_potassium_current__i_K_max * (g[6] - (g[5] * cmx_exp_v(((-V) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, -0.0)))) * (R(1) / R(140.0)));
+    const R time_dependent_potassium_current__i_K = (y[2] * time_dependent_potassium_current__I_K);
+    const R time_independent_potassium_current__i_K1 = cmx_div((cmx_div((p[1] * g[5]), (g[5] + time_independent_potassium_current__Km_K1)) * (V - hyperpolarising_activated_current__E_K)), (R(1.0) + cmx_exp_c(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF)), (((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)), 0.05400568865115377)));
+    const R transient_outward_current__i_to = cmx_remove_singularity(V, R(-9.999999999999991), R(4.999999999999996e-07), [&](const R V) { return (cmx_div((cmx_div((y[3] * p[3] * (R(0.2) + cmx_div(g[5], (transient_outward_current__Km_to + g[5]))) * g[2]), (transient_outward_current__Km_Ca + g[2])) * (V + R(10.0))), (-cmx_expm1(((-R(0.2)) * (V + R(10.0)))))) * ((g[6] * cmx_exp_v(((R(0.5) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0)) - (g[5] * cmx_exp_v((((-R(0.5)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0)))); });
+    const R extracellular_potassium_concentration__i_mK = ((time_independent_potassium_current__i_K1 + time_dependent_potassium_current__i_K + hyperpolarising_activated_current__i_fK + second_inward_current__i_siK + transient_outward_current__i_to) - (R(2.0) * sodium_potassium_pump__i_p));
+    const R fast_sodium_current__E_mh = (membrane__RTONF * cmx_log(cmx_div((extracellular_sodium_concentration__Nao + (R(0.12) * g[5])), (g[1] + (R(0.12) * g[6])))));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * (V - fast_sodium_current__E_mh));
+    const R hyperpolarising_activated_current__E_Na = (membrane__RTONF * cmx_log(cmx_div(extracellular_sodium_concentration__Nao, g[1])));
+    const R hyperpolarising_activated_current__i_fNa = (cmx_div((y[1] * g[5]), (g[5] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_Na * (V - hyperpolarising_activated_current__E_Na));
+    const R intracellular_sodium_concentration__V_Cell = (R(3.141592654) * cmx_ipow<2>(intracellular_sodium_concentration__radius) * intracellular_sodium_concentration__length);
+    const R intracellular_sodium_concentration__Vi = (intracellular_sodium_concentration__V_Cell * (R(1.0) - intracellular_sodium_concentration__V_e_ratio));
+    const R intracellular_calcium_concentration__V_rel = (intracellular_sodium_concentration__Vi * R(0.02));
+    const R intracellular_calcium_concentration__V_up = (intracellular_sodium_concentration__Vi * R(0.05));
+    const R intracellular_calcium_concentration__i_rel = cmx_div((((R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_rel))) * g[4] * pow(g[2], intracellular_calcium_concentration__rCa)), (pow(g[2], intracellular_calcium_concentration__rCa) + pow(intracellular_calcium_concentration__K_mCa, intracellular_calcium_concentration__rCa)));
+    const R intracellular_calcium_concentration__i_tr = (((R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_rep))) * y[13] * (g[3] - g[4]));
+    const R intracellular_calcium_concentration__i_up = (((R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_up * intracellular_calcium_concentration__Ca_up_max))) * g[2] * (intracellular_calcium_concentration__Ca_up_max - g[3]));
+    const R second_inward_current__i_siCa = cmx_remove_singularity(V, R(50.0000000000001), R(1.335688032984785e-06), [&](const R V) { return (cmx_div((R(4.0) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0)) * R(2.0)) * (R(1) / membrane__RTONF)))))) * ((g[2] * cmx_exp((R(100.0) * (R(1) / membrane__RTONF)))) - (extracellular_calcium_concentration__Cao * cmx_exp_v((((-R(2.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.10801137730230755, 5.400568865115377))) * y[6] * y[7] * g[0]); });
+    const R second_inward_current__i_siNa = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return (cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((g[1] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (extracellular_sodium_concentration__Nao * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7] * g[0]); });
+    const R second_inward_current_f2_gate__beta_f2 = ((g[2] * second_inward_current_f2_gate__alpha_f2) * (R(1) / second_inward_current_f2_gate__K_mf2));
+    const R sodium_background_current__i_Na_b = (sodium_background_current__g_Nab * (V - hyperpolarising_activated_current__E_Na));
+    res[0] = g[0] - y[8] - dt * ((second_inward_current_f2_gate__alpha_f2 - (g[0] * (second_inward_current_f2_gate__alpha_f2 + second_inward_current_f2_gate__beta_f2))) * (R(1) / R(1000.0)));
+    res[1] = g[1] - y[9] - dt * ((((-R(1.0)) * (fast_sodium_current__i_Na + sodium_background_current__i_Na_b + hyperpolarising_activated_current__i_fNa + second_inward_current__i_siNa + (sodium_potassium_pump__i_p * R(3.0)) + ((Na_Ca_exchanger__i_NaCa * Na_Ca_exchanger__n_NaCa) * (R(1) / (Na_Ca_exchanger__n_NaCa - R(2.0)))))) * (R(1) / (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F))) * (R(1) / R(1000.0)));
+    res[2] = g[2] - y[10] - dt * ((((-R(1.0)) * ((((second_inward_current__i_siCa + calcium_background_current__i_Ca_b) - ((R(2.0) * Na_Ca_exchanger__i_NaCa) * (R(1) / (Na_Ca_exchanger__n_NaCa - R(2.0))))) - intracellular_calcium_concentration__i_rel) + intracellular_calcium_concentration__i_up)) * (R(1) / (R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F))) * (R(1) / R(1000.0)));
+    res[3] = g[3] - y[11] - dt * (((R(1.0) * (intracellular_calcium_concentration__i_up - intracellular_calcium_concentration__i_tr)) * (R(1) / (R(2.0) * R(1.0) * intracellular_calcium_concentration__V_up * membrane__F))) * (R(1) / R(1000.0)));
+    res[4] = g[4] - y[12] - dt * (((R(1.0) * (intracellular_calcium_concentration__i_tr - intracellular_calcium_concentration__i_rel)) * (R(1) / (R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F))) * (R(1) / R(1000.0)));
+    res[5] = g[5] - y[14] - dt * ((((-extracellular_potassium_concentration__pf) * (g[5] - extracellular_potassium_concentration__Kb)) + ((R(1.0) * extracellular_potassium_concentration__i_mK) * (R(1) / (R(1.0) * extracellular_potassium_concentration__Ve * membrane__F)))) * (R(1) / R(1000.0)));
+    res[6] = g[6] - y[15] - dt * ((((-R(1.0)) * extracellular_potassium_concentration__i_mK) * (R(1) / (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F))) * (R(1) / R(1000.0)));
+}
+
+// Jacobian of be_residual in g: J[i*N + j] = delta_ij - dt d f_i / d g_j.
+template <class R>
+CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const R* p, R cm, R dt, R* J)
+{
+    CMX_MATH_USING
+    (void)y; (void)p; (void)cm; (void)i_stim_area;
+    const R membrane__R = R(8314.472);
+    const R membrane__T = R(310.0);
+    const R membrane__F = R(96485.3415);
+    const R hyperpolarising_activated_current__g_f_Na = R(3.0);
+    const R hyperpolarising_activated_current__g_f_K = R(3.0);
+    const R hyperpolarising_activated_current__Km_f = R(45.0);
+    const R time_dependent_potassium_current__i_K_max = R(180.0);
+    const R time_independent_potassium_current__Km_K1 = R(210.0);
+    const R transient_outward_current__Km_to = R(10.0);
+    const R transient_outward_current__Km_Ca = R(0.0005);
+    const R sodium_background_current__g_Nab = R(0.18);
+    const R calcium_background_current__g_Cab = R(0.02);
+    const R sodium_potassium_pump__I_p = R(125.0);
+    const R sodium_potassium_pump__K_mK = R(1.0);
+    const R sodium_potassium_pump__K_mNa = R(40.0);
+    const R Na_Ca_exchanger__n_NaCa = R(3.0);
+    const R Na_Ca_exchanger__K_NaCa = R(0.02);
+    const R Na_Ca_exchanger__d_NaCa = R(0.001);
+    const R Na_Ca_exchanger__gamma = R(0.5);
+    const R second_inward_current_f2_gate__alpha_f2 = R(5.0);
+    const R second_inward_current_f2_gate__K_mf2 = R(0.001);
+    const R extracellular_sodium_concentration__Nao = R(140.0);
+    const R intracellular_sodium_concentration__radius = R(0.05);
+    const R intracellular_sodium_concentration__length = R(2.0);
+    const R intracellular_sodium_concentration__V_e_ratio = R(0.1);
+    const R extracellular_calcium_concentration__Cao = R(2.0);
+    const R intracellular_calcium_concentration__Ca_up_max = R(5.0);
+    const R intracellular_calcium_concentration__K_mCa = R(0.001);
+    const R intracellular_calcium_concentration__tau_up = R(0.025);
+    const R intracellular_calcium_concentration__tau_rep = R(2.0);
+    const R intracellular_calcium_concentration__tau_rel = R(0.05);
+    const R intracellular_calcium_concentration__rCa = R(2.0);
+    const R extracellular_potassium_concentration__Ve = R(0.00157);
+    const R extracellular_potassium_concentration__Kb = R(4.0);
+    const R extracellular_potassium_concentration__pf = R(0.7);
+    const R i_stim = (i_stim_area / cm) * R(74.99999999999999);
+    (void)i_stim;
+    const R membrane__RTONF = ((membrane__R * membrane__T) * (R(1) / membrane__F));
+    const R Na_Ca_exchanger__i_NaCa = cmx_div((Na_Ca_exchanger__K_NaCa * ((cmx_exp_v(((Na_Ca_exchanger__gamma * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0) * pow(g[1], Na_Ca_exchanger__n_NaCa) * extracellular_calcium_concentration__Cao) - (cmx_exp_v((((Na_Ca_exchanger__gamma - R(1.0)) * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0) * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa) * g[2]))), ((R(1.0) + (Na_Ca_exchanger__d_NaCa * ((g[2] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(g[1], Na_Ca_exchanger__n_NaCa))))) * (R(1.0) + (g[2] * (R(1) / R(0.0069))))));
+    const R calcium_background_current__E_Ca = (R(0.5) * membrane__RTONF * cmx_log(cmx_div(extracellular_calcium_concentration__Cao, g[2])));
+    const R calcium_background_current__i_Ca_b = (calcium_background_current__g_Cab * (V - calcium_background_current__E_Ca));
+    const R hyperpolarising_activated_current__E_K = (membrane__RTONF * cmx_log(cmx_div(g[5], g[6])));
+    const R hyperpolarising_activated_current__i_fK = (cmx_div((y[1] * g[5]), (g[5] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_K * (V - hyperpolarising_activated_current__E_K));
+    const R second_inward_current__i_siK = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return (cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((g[6] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (g[5] * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7] * g[0]); });
+    const R sodium_potassium_pump__i_p = cmx_div((cmx_div((sodium_potassium_pump__I_p * g[5]), (sodium_potassium_pump__K_mK + g[5])) * g[1]), (sodium_potassium_pump__K_mNa + g[1]));
+    const R time_dependent_potassium_current__I_K = ((time_dependent_potassium_current__i_K_max * (g[6] - (g[5] * cmx_exp_v(((-V) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, -0.0)))) * (R(1) / R(140.0)));
+    const R time_dependent_potassium_current__i_K = (y[2] * time_dependent_potassium_current__I_K);
+    const R time_independent_potassium_current__i_K1 = cmx_div((cmx_div((p[1] * g[5]), (g[5] + time_independent_potassium_current__Km_K1)) * (V - hyperpolarising_activated_current__E_K)), (R(1.0) + cmx_exp_c(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF)), (((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)), 0.05400568865115377)));
+    const R transient_outward_current__i_to = cmx_remove_singularity(V, R(-9.999999999999991), R(4.999999999999996e-07), [&](const R V) { return (cmx_div((cmx_div((y[3] * p[3] * (R(0.2) + cmx_div(g[5], (transient_outward_current__Km_to + g[5]))) * g[2]), (transient_outward_current__Km_Ca + g[2])) * (V + R(10.0))), (-cmx_expm1(((-R(0.2)) * (V + R(10.0)))))) * ((g[6] * cmx_exp_v(((R(0.5) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0)) - (g[5] * cmx_exp_v((((-R(0.5)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0)))); });
+    const R extracellular_potassium_concentration__i_mK = ((time_independent_potassium_current__i_K1 + time_dependent_potassium_current__i_K + hyperpolarising_activated_current__i_fK + second_inward_current__i_siK + transient_outward_current__i_to) - (R(2.0) * sodium_potassium_pump__i_p));
+    const R fast_sodium_current__E_mh = (membrane__RTONF * cmx_log(cmx_div((extracellular_sodium_concentration__Nao + (R(0.12) * g[5])), (g[1] + (R(0.12) * g[6])))));
+    const R fast_sodium_current__i_Na = (p[0] * cmx_ipow<3>(y[4]) * y[5] * (V - fast_sodium_current__E_mh));
+    const R hyperpolarising_activated_current__E_Na = (membrane__RTONF * cmx_log(cmx_div(extracellular_sodium_concentration__Nao, g[1])));
+    const R hyperpolarising_activated_current__i_fNa = (cmx_div((y[1] * g[5]), (g[5] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_Na * (V - hyperpolarising_activated_current__E_Na));
+    const R intracellular_sodium_concentration__V_Cell = (R(3.141592654) * cmx_ipow<2>(intracellular_sodium_concentration__radius) * intracellular_sodium_concentration__length);
+    const R intracellular_sodium_concentration__Vi = (intracellular_sodium_concentration__V_Cell * (R(1.0) - intracellular_sodium_concentration__V_e_ratio));
+    const R intracellular_calcium_concentration__V_rel = (intracellular_sodium_concentration__Vi * R(0.02));
+    const R intracellular_calcium_concentration__V_up = (intracellular_sodium_concentration__Vi * R(0.05));
+    const R intracellular_calcium_concentration__i_rel = cmx_div((((R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_rel))) * g[4] * pow(g[2], intracellular_calcium_concentration__rCa)), (pow(g[2], intracellular_calcium_concentration__rCa) + pow(intracellular_calcium_concentration__K_mCa, intracellular_calcium_concentration__rCa)));
+    const R intracellular_calcium_concentration__i_tr = (((R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_rep))) * y[13] * (g[3] - g[4]));
+    const R intracellular_calcium_concentration__i_up = (((R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_up * intracellular_calcium_concentration__Ca_up_max))) * g[2] * (intracellular_calcium_concentration__Ca_up_max - g[3]));
+    const R second_inward_current__i_siCa = cmx_remove_singularity(V, R(50.0000000000001), R(1.335688032984785e-06), [&](const R V) { return (cmx_div((R(4.0) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0)) * R(2.0)) * (R(1) / membrane__RTONF)))))) * ((g[2] * cmx_exp((R(100.0) * (R(1) / membrane__RTONF)))) - (extracellular_calcium_concentration__Cao * cmx_exp_v((((-R(2.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.10801137730230755, 5.400568865115377))) * y[6] * y[7] * g[0]); });
+    const R second_inward_current__i_siNa = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return (cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((g[1] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (extracellular_sodium_concentration__Nao * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7] * g[0]); });
+    const R second_inward_current_f2_gate__beta_f2 = ((g[2] * second_inward_current_f2_gate__alpha_f2) * (R(1) / second_inward_current_f2_gate__K_mf2));
+    const R sodium_background_current__i_Na_b = (sodium_background_current__g_Nab * (V - hyperpolarising_activated_current__E_Na));
+    const R d_Na_Ca_exchanger__i_NaCa__d1 = cmx_div(((((((Na_Ca_exchanger__n_NaCa * pow(g[1], Na_Ca_exchanger__n_NaCa - R(1)) * R(1)) * cmx_exp_v(((Na_Ca_exchanger__gamma * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0) * extracellular_calcium_concentration__Cao)) * Na_Ca_exchanger__K_NaCa)) * ((R(1.0) + (Na_Ca_exchanger__d_NaCa * ((g[2] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(g[1], Na_Ca_exchanger__n_NaCa))))) * (R(1.0) + (g[2] * (R(1) / R(0.0069))))) - (Na_Ca_exchanger__K_NaCa * ((cmx_exp_v(((Na_Ca_exchanger__gamma * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0) * pow(g[1], Na_Ca_exchanger__n_NaCa) * extracellular_calcium_concentration__Cao) - (cmx_exp_v((((Na_Ca_exchanger__gamma - R(1.0)) * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0) * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa) * g[2]))) * (((((((((Na_Ca_exchanger__n_NaCa * pow(g[1], Na_Ca_exchanger__n_NaCa - R(1)) * R(1)) * extracellular_calcium_concentration__Cao))) * Na_Ca_exchanger__d_NaCa))) * (R(1.0) + (g[2] * (R(1) / R(0.0069))))))), (((R(1.0) + (Na_Ca_exchanger__d_NaCa * ((g[2] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(g[1], Na_Ca_exchanger__n_NaCa))))) * (R(1.0) + (g[2] * (R(1) / R(0.0069))))) * ((R(1.0) + (Na_Ca_exchanger__d_NaCa * ((g[2] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(g[1], Na_Ca_exchanger__n_NaCa))))) * (R(1.0) + (g[2] * (R(1) / R(0.0069)))))));
+    const R d_Na_Ca_exchanger__i_NaCa__d2 = cmx_div(((((-((R(1) * cmx_exp_v((((Na_Ca_exchanger__gamma - R(1.0)) * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0) * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)))) * Na_Ca_exchanger__K_NaCa)) * ((R(1.0) + (Na_Ca_exchanger__d_NaCa * ((g[2] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(g[1], Na_Ca_exchanger__n_NaCa))))) * (R(1.0) + (g[2] * (R(1) / R(0.0069))))) - (Na_Ca_exchanger__K_NaCa * ((cmx_exp_v(((Na_Ca_exchanger__gamma * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0) * pow(g[1], Na_Ca_exchanger__n_NaCa) * extracellular_calcium_concentration__Cao) - (cmx_exp_v((((Na_Ca_exchanger__gamma - R(1.0)) * (Na_Ca_exchanger__n_NaCa - R(2.0)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0) * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa) * g[2]))) * ((((((((R(1) * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)))) * Na_Ca_exchanger__d_NaCa))) * (R(1.0) + (g[2] * (R(1) / R(0.0069))))) + ((cmx_div(R(1), R(0.0069))) * (R(1.0) + (Na_Ca_exchanger__d_NaCa * ((g[2] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(g[1], Na_Ca_exchanger__n_NaCa)))))))), (((R(1.0) + (Na_Ca_exchanger__d_NaCa * ((g[2] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(g[1], Na_Ca_exchanger__n_NaCa))))) * (R(1.0) + (g[2] * (R(1) / R(0.0069))))) * ((R(1.0) + (Na_Ca_exchanger__d_NaCa * ((g[2] * pow(extracellular_sodium_concentration__Nao, Na_Ca_exchanger__n_NaCa)) + (extracellular_calcium_concentration__Cao * pow(g[1], Na_Ca_exchanger__n_NaCa))))) * (R(1.0) + (g[2] * (R(1) / R(0.0069)))))));
+    const R d_calcium_background_current__E_Ca__d2 = ((cmx_div(cmx_div((-(extracellular_calcium_concentration__Cao * R(1))), (g[2] * g[2])), cmx_div(extracellular_calcium_concentration__Cao, g[2])) * R(0.5) * membrane__RTONF));
+    const R d_calcium_background_current__i_Ca_b__d2 = (((-d_calcium_background_current__E_Ca__d2) * calcium_background_current__g_Cab));
+    const R d_hyperpolarising_activated_current__E_K__d5 = ((cmx_div(cmx_div(R(1), g[6]), cmx_div(g[5], g[6])) * membrane__RTONF));
+    const R d_hyperpolarising_activated_current__E_K__d6 = ((cmx_div(cmx_div((-(g[5] * R(1))), (g[6] * g[6])), cmx_div(g[5], g[6])) * membrane__RTONF));
+    const R d_hyperpolarising_activated_current__i_fK__d5 = ((cmx_div((((R(1) * y[1])) * (g[5] + hyperpolarising_activated_current__Km_f) - (y[1] * g[5]) * (R(1))), ((g[5] + hyperpolarising_activated_current__Km_f) * (g[5] + hyperpolarising_activated_current__Km_f))) * hyperpolarising_activated_current__g_f_K * (V - hyperpolarising_activated_current__E_K)) + ((-d_hyperpolarising_activated_current__E_K__d5) * cmx_div((y[1] * g[5]), (g[5] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_K));
+    const R d_hyperpolarising_activated_current__i_fK__d6 = (((-d_hyperpolarising_activated_current__E_K__d6) * cmx_div((y[1] * g[5]), (g[5] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_K));
+    const R d_second_inward_current__i_siK__d0 = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return ((R(1) * cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((g[6] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (g[5] * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7])); });
+    const R d_second_inward_current__i_siK__d5 = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return (((-((R(1) * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887)))) * cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * y[6] * y[7] * g[0])); });
+    const R d_second_inward_current__i_siK__d6 = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return ((((R(1) * cmx_exp((R(50.0) * (R(1) / membrane__RTONF))))) * cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * y[6] * y[7] * g[0])); });
+    const R d_sodium_potassium_pump__i_p__d1 = cmx_div((((R(1) * cmx_div((sodium_potassium_pump__I_p * g[5]), (sodium_potassium_pump__K_mK + g[5])))) * (sodium_potassium_pump__K_mNa + g[1]) - (cmx_div((sodium_potassium_pump__I_p * g[5]), (sodium_potassium_pump__K_mK + g[5])) * g[1]) * (R(1))), ((sodium_potassium_pump__K_mNa + g[1]) * (sodium_potassium_pump__K_mNa + g[1])));
+    const R d_sodium_potassium_pump__i_p__d5 = cmx_div(((cmx_div((((R(1) * sodium_potassium_pump__I_p)) * (sodium_potassium_pump__K_mK + g[5]) - (sodium_potassium_pump__I_p * g[5]) * (R(1))), ((sodium_potassium_pump__K_mK + g[5]) * (sodium_potassium_pump__K_mK + g[5]))) * g[1])), (sodium_potassium_pump__K_mNa + g[1]));
+    const R d_time_dependent_potassium_current__I_K__d5 = cmx_div((((-((R(1) * cmx_exp_v(((-V) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, -0.0)))) * time_dependent_potassium_current__i_K_max)), R(140.0));
+    const R d_time_dependent_potassium_current__I_K__d6 = cmx_div(((R(1) * time_dependent_potassium_current__i_K_max)), R(140.0));
+    const R d_time_dependent_potassium_current__i_K__d5 = ((d_time_dependent_potassium_current__I_K__d5 * y[2]));
+    const R d_time_dependent_potassium_current__i_K__d6 = ((d_time_dependent_potassium_current__I_K__d6 * y[2]));
+    const R d_time_independent_potassium_current__i_K1__d5 = cmx_div((((cmx_div((((R(1) * p[1])) * (g[5] + time_independent_potassium_current__Km_K1) - (p[1] * g[5]) * (R(1))), ((g[5] + time_independent_potassium_current__Km_K1) * (g[5] + time_independent_potassium_current__Km_K1))) * (V - hyperpolarising_activated_current__E_K)) + ((-d_hyperpolarising_activated_current__E_K__d5) * cmx_div((p[1] * g[5]), (g[5] + time_independent_potassium_current__Km_K1)))) * (R(1.0) + cmx_exp_c(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF)), (((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)), 0.05400568865115377)) - (cmx_div((p[1] * g[5]), (g[5] + time_independent_potassium_current__Km_K1)) * (V - hyperpolarising_activated_current__E_K)) * ((cmx_exp(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF))) * cmx_div((((-d_hyperpolarising_activated_current__E_K__d5) * R(2.0))), membrane__RTONF)))), ((R(1.0) + cmx_exp_c(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF)), (((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)), 0.05400568865115377)) * (R(1.0) + cmx_exp_c(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF)), (((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)), 0.05400568865115377))));
+    const R d_time_independent_potassium_current__i_K1__d6 = cmx_div(((((-d_hyperpolarising_activated_current__E_K__d6) * cmx_div((p[1] * g[5]), (g[5] + time_independent_potassium_current__Km_K1)))) * (R(1.0) + cmx_exp_c(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF)), (((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)), 0.05400568865115377)) - (cmx_div((p[1] * g[5]), (g[5] + time_independent_potassium_current__Km_K1)) * (V - hyperpolarising_activated_current__E_K)) * ((cmx_exp(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF))) * cmx_div((((-d_hyperpolarising_activated_current__E_K__d6) * R(2.0))), membrane__RTONF)))), ((R(1.0) + cmx_exp_c(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF)), (((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)), 0.05400568865115377)) * (R(1.0) + cmx_exp_c(((((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)) * (R(1) / membrane__RTONF)), (((V + R(10.0)) - hyperpolarising_activated_current__E_K) * R(2.0)), 0.05400568865115377))));
+    const R d_transient_outward_current__i_to__d2 = cmx_remove_singularity(V, R(-9.999999999999991), R(4.999999999999996e-07), [&](const R V) { return ((cmx_div(((cmx_div((((R(1) * y[3] * p[3] * (R(0.2) + cmx_div(g[5], (transient_outward_current__Km_to + g[5]))))) * (transient_outward_current__Km_Ca + g[2]) - (y[3] * p[3] * (R(0.2) + cmx_div(g[5], (transient_outward_current__Km_to + g[5]))) * g[2]) * (R(1))), ((transient_outward_current__Km_Ca + g[2]) * (transient_outward_current__Km_Ca + g[2]))) * (V + R(10.0)))), (-cmx_expm1(((-R(0.2)) * (V + R(10.0)))))) * ((g[6] * cmx_exp_v(((R(0.5) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0)) - (g[5] * cmx_exp_v((((-R(0.5)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0))))); });
+    const R d_transient_outward_current__i_to__d5 = cmx_remove_singularity(V, R(-9.999999999999991), R(4.999999999999996e-07), [&](const R V) { return ((cmx_div(((cmx_div((((cmx_div((R(1) * (transient_outward_current__Km_to + g[5]) - g[5] * (R(1))), ((transient_outward_current__Km_to + g[5]) * (transient_outward_current__Km_to + g[5])))) * y[3] * p[3] * g[2])), (transient_outward_current__Km_Ca + g[2])) * (V + R(10.0)))), (-cmx_expm1(((-R(0.2)) * (V + R(10.0)))))) * ((g[6] * cmx_exp_v(((R(0.5) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0)) - (g[5] * cmx_exp_v((((-R(0.5)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0)))) + ((-((R(1) * cmx_exp_v((((-R(0.5)) * V) * (R(1) / membrane__RTONF)), V, -0.027002844325576886, -0.0)))) * cmx_div((cmx_div((y[3] * p[3] * (R(0.2) + cmx_div(g[5], (transient_outward_current__Km_to + g[5]))) * g[2]), (transient_outward_current__Km_Ca + g[2])) * (V + R(10.0))), (-cmx_expm1(((-R(0.2)) * (V + R(10.0)))))))); });
+    const R d_transient_outward_current__i_to__d6 = cmx_remove_singularity(V, R(-9.999999999999991), R(4.999999999999996e-07), [&](const R V) { return ((((R(1) * cmx_exp_v(((R(0.5) * V) * (R(1) / membrane__RTONF)), V, 0.027002844325576886, 0.0))) * cmx_div((cmx_div((y[3] * p[3] * (R(0.2) + cmx_div(g[5], (transient_outward_current__Km_to + g[5]))) * g[2]), (transient_outward_current__Km_Ca + g[2])) * (V + R(10.0))), (-cmx_expm1(((-R(0.2)) * (V + R(10.0)))))))); });
+    const R d_extracellular_potassium_concentration__i_mK__d0 = (d_second_inward_current__i_siK__d0);
+    const R d_extracellular_potassium_concentration__i_mK__d1 = (-((d_sodium_potassium_pump__i_p__d1 * R(2.0))));
+    const R d_extracellular_potassium_concentration__i_mK__d2 = (d_transient_outward_current__i_to__d2);
+    const R d_extracellular_potassium_concentration__i_mK__d5 = ((d_time_independent_potassium_current__i_K1__d5 + d_time_dependent_potassium_current__i_K__d5 + d_hyperpolarising_activated_current__i_fK__d5 + d_second_inward_current__i_siK__d5 + d_transient_outward_current__i_to__d5) - ((d_sodium_potassium_pump__i_p__d5 * R(2.0))));
+    const R d_extracellular_potassium_concentration__i_mK__d6 = (d_time_independent_potassium_current__i_K1__d6 + d_time_dependent_potassium_current__i_K__d6 + d_hyperpolarising_activated_current__i_fK__d6 + d_second_inward_current__i_siK__d6 + d_transient_outward_current__i_to__d6);
+    const R d_fast_sodium_current__E_mh__d1 = ((cmx_div(cmx_div((-((extracellular_sodium_concentration__Nao + (R(0.12) * g[5])) * (R(1)))), ((g[1] + (R(0.12) * g[6])) * (g[1] + (R(0.12) * g[6])))), cmx_div((extracellular_sodium_concentration__Nao + (R(0.12) * g[5])), (g[1] + (R(0.12) * g[6])))) * membrane__RTONF));
+    const R d_fast_sodium_current__E_mh__d5 = ((cmx_div(cmx_div((((R(1) * R(0.12)))), (g[1] + (R(0.12) * g[6]))), cmx_div((extracellular_sodium_concentration__Nao + (R(0.12) * g[5])), (g[1] + (R(0.12) * g[6])))) * membrane__RTONF));
+    const R d_fast_sodium_current__E_mh__d6 = ((cmx_div(cmx_div((-((extracellular_sodium_concentration__Nao + (R(0.12) * g[5])) * (((R(1) * R(0.12)))))), ((g[1] + (R(0.12) * g[6])) * (g[1] + (R(0.12) * g[6])))), cmx_div((extracellular_sodium_concentration__Nao + (R(0.12) * g[5])), (g[1] + (R(0.12) * g[6])))) * membrane__RTONF));
+    const R d_fast_sodium_current__i_Na__d1 = (((-d_fast_sodium_current__E_mh__d1) * p[0] * cmx_ipow<3>(y[4]) * y[5]));
+    const R d_fast_sodium_current__i_Na__d5 = (((-d_fast_sodium_current__E_mh__d5) * p[0] * cmx_ipow<3>(y[4]) * y[5]));
+    const R d_fast_sodium_current__i_Na__d6 = (((-d_fast_sodium_current__E_mh__d6) * p[0] * cmx_ipow<3>(y[4]) * y[5]));
+    const R d_hyperpolarising_activated_current__E_Na__d1 = ((cmx_div(cmx_div((-(extracellular_sodium_concentration__Nao * R(1))), (g[1] * g[1])), cmx_div(extracellular_sodium_concentration__Nao, g[1])) * membrane__RTONF));
+    const R d_hyperpolarising_activated_current__i_fNa__d1 = (((-d_hyperpolarising_activated_current__E_Na__d1) * cmx_div((y[1] * g[5]), (g[5] + hyperpolarising_activated_current__Km_f)) * hyperpolarising_activated_current__g_f_Na));
+    const R d_hyperpolarising_activated_current__i_fNa__d5 = ((cmx_div((((R(1) * y[1])) * (g[5] + hyperpolarising_activated_current__Km_f) - (y[1] * g[5]) * (R(1))), ((g[5] + hyperpolarising_activated_current__Km_f) * (g[5] + hyperpolarising_activated_current__Km_f))) * hyperpolarising_activated_current__g_f_Na * (V - hyperpolarising_activated_current__E_Na)));
+    const R d_intracellular_calcium_concentration__i_rel__d2 = cmx_div(((((intracellular_calcium_concentration__rCa * pow(g[2], intracellular_calcium_concentration__rCa - R(1)) * R(1)) * ((R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_rel))) * g[4])) * (pow(g[2], intracellular_calcium_concentration__rCa) + pow(intracellular_calcium_concentration__K_mCa, intracellular_calcium_concentration__rCa)) - (((R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_rel))) * g[4] * pow(g[2], intracellular_calcium_concentration__rCa)) * ((intracellular_calcium_concentration__rCa * pow(g[2], intracellular_calcium_concentration__rCa - R(1)) * R(1)))), ((pow(g[2], intracellular_calcium_concentration__rCa) + pow(intracellular_calcium_concentration__K_mCa, intracellular_calcium_concentration__rCa)) * (pow(g[2], intracellular_calcium_concentration__rCa) + pow(intracellular_calcium_concentration__K_mCa, intracellular_calcium_concentration__rCa))));
+    const R d_intracellular_calcium_concentration__i_rel__d4 = cmx_div(((R(1) * ((R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_rel))) * pow(g[2], intracellular_calcium_concentration__rCa))), (pow(g[2], intracellular_calcium_concentration__rCa) + pow(intracellular_calcium_concentration__K_mCa, intracellular_calcium_concentration__rCa)));
+    const R d_intracellular_calcium_concentration__i_tr__d3 = ((R(1) * ((R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_rep))) * y[13]));
+    const R d_intracellular_calcium_concentration__i_tr__d4 = (((-R(1)) * ((R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_rep))) * y[13]));
+    const R d_intracellular_calcium_concentration__i_up__d2 = ((R(1) * ((R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_up * intracellular_calcium_concentration__Ca_up_max))) * (intracellular_calcium_concentration__Ca_up_max - g[3])));
+    const R d_intracellular_calcium_concentration__i_up__d3 = (((-R(1)) * ((R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F) * (R(1) / (R(1.0) * intracellular_calcium_concentration__tau_up * intracellular_calcium_concentration__Ca_up_max))) * g[2]));
+    const R d_second_inward_current__i_siCa__d0 = cmx_remove_singularity(V, R(50.0000000000001), R(1.335688032984785e-06), [&](const R V) { return ((R(1) * cmx_div((R(4.0) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0)) * R(2.0)) * (R(1) / membrane__RTONF)))))) * ((g[2] * cmx_exp((R(100.0) * (R(1) / membrane__RTONF)))) - (extracellular_calcium_concentration__Cao * cmx_exp_v((((-R(2.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.10801137730230755, 5.400568865115377))) * y[6] * y[7])); });
+    const R d_second_inward_current__i_siCa__d2 = cmx_remove_singularity(V, R(50.0000000000001), R(1.335688032984785e-06), [&](const R V) { return ((((R(1) * cmx_exp((R(100.0) * (R(1) / membrane__RTONF))))) * cmx_div((R(4.0) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0)) * R(2.0)) * (R(1) / membrane__RTONF)))))) * y[6] * y[7] * g[0])); });
+    const R d_second_inward_current__i_siNa__d0 = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return ((R(1) * cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * ((g[1] * cmx_exp((R(50.0) * (R(1) / membrane__RTONF)))) - (extracellular_sodium_concentration__Nao * cmx_exp_v((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)), V, -0.05400568865115377, 2.7002844325576887))) * y[6] * y[7])); });
+    const R d_second_inward_current__i_siNa__d1 = cmx_remove_singularity(V, R(50.0000000000001), R(2.67137606596957e-06), [&](const R V) { return ((((R(1) * cmx_exp((R(50.0) * (R(1) / membrane__RTONF))))) * cmx_div((R(0.01) * p[2] * (V - R(50.0))), (membrane__RTONF * (-cmx_expm1((((-R(1.0)) * (V - R(50.0))) * (R(1) / membrane__RTONF)))))) * y[6] * y[7] * g[0])); });
+    const R d_second_inward_current_f2_gate__beta_f2__d2 = cmx_div(((R(1) * second_inward_current_f2_gate__alpha_f2)), second_inward_current_f2_gate__K_mf2);
+    const R d_sodium_background_current__i_Na_b__d1 = (((-d_hyperpolarising_activated_current__E_Na__d1) * sodium_background_current__g_Nab));
+    J[0] = R(1) - dt * cmx_div((-((R(1) * (second_inward_current_f2_gate__alpha_f2 + second_inward_current_f2_gate__beta_f2)))), R(1000.0));
+    J[1] = R(0);
+    J[2] = R(0) - dt * cmx_div((-(((d_second_inward_current_f2_gate__beta_f2__d2) * g[0]))), R(1000.0));
+    J[3] = R(0);
+    J[4] = R(0);
+    J[5] = R(0);
+    J[6] = R(0);
+    J[7] = R(0) - dt * cmx_div(cmx_div((((d_second_inward_current__i_siNa__d0) * (-R(1.0)))), (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[8] = R(1) - dt * cmx_div(cmx_div((((d_fast_sodium_current__i_Na__d1 + d_sodium_background_current__i_Na_b__d1 + d_hyperpolarising_activated_current__i_fNa__d1 + d_second_inward_current__i_siNa__d1 + ((d_sodium_potassium_pump__i_p__d1 * R(3.0))) + cmx_div(((d_Na_Ca_exchanger__i_NaCa__d1 * Na_Ca_exchanger__n_NaCa)), (Na_Ca_exchanger__n_NaCa - R(2.0)))) * (-R(1.0)))), (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[9] = R(0) - dt * cmx_div(cmx_div((((cmx_div(((d_Na_Ca_exchanger__i_NaCa__d2 * Na_Ca_exchanger__n_NaCa)), (Na_Ca_exchanger__n_NaCa - R(2.0)))) * (-R(1.0)))), (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[10] = R(0);
+    J[11] = R(0);
+    J[12] = R(0) - dt * cmx_div(cmx_div((((d_fast_sodium_current__i_Na__d5 + d_hyperpolarising_activated_current__i_fNa__d5 + ((d_sodium_potassium_pump__i_p__d5 * R(3.0)))) * (-R(1.0)))), (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[13] = R(0) - dt * cmx_div(cmx_div((((d_fast_sodium_current__i_Na__d6) * (-R(1.0)))), (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[14] = R(0) - dt * cmx_div(cmx_div(((((d_second_inward_current__i_siCa__d0)) * (-R(1.0)))), (R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[15] = R(0) - dt * cmx_div(cmx_div(((((-cmx_div(((d_Na_Ca_exchanger__i_NaCa__d1 * R(2.0))), (Na_Ca_exchanger__n_NaCa - R(2.0))))) * (-R(1.0)))), (R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[16] = R(1) - dt * cmx_div(cmx_div(((((((d_second_inward_current__i_siCa__d2 + d_calcium_background_current__i_Ca_b__d2) - cmx_div(((d_Na_Ca_exchanger__i_NaCa__d2 * R(2.0))), (Na_Ca_exchanger__n_NaCa - R(2.0)))) - d_intracellular_calcium_concentration__i_rel__d2) + d_intracellular_calcium_concentration__i_up__d2) * (-R(1.0)))), (R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[17] = R(0) - dt * cmx_div(cmx_div((((d_intracellular_calcium_concentration__i_up__d3) * (-R(1.0)))), (R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[18] = R(0) - dt * cmx_div(cmx_div(((((-d_intracellular_calcium_concentration__i_rel__d4)) * (-R(1.0)))), (R(2.0) * R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[19] = R(0);
+    J[20] = R(0);
+    J[21] = R(0);
+    J[22] = R(0);
+    J[23] = R(0) - dt * cmx_div(cmx_div(((d_intracellular_calcium_concentration__i_up__d2 * R(1.0))), (R(2.0) * R(1.0) * intracellular_calcium_concentration__V_up * membrane__F)), R(1000.0));
+    J[24] = R(1) - dt * cmx_div(cmx_div((((d_intracellular_calcium_concentration__i_up__d3 - d_intracellular_calcium_concentration__i_tr__d3) * R(1.0))), (R(2.0) * R(1.0) * intracellular_calcium_concentration__V_up * membrane__F)), R(1000.0));
+    J[25] = R(0) - dt * cmx_div(cmx_div((((-d_intracellular_calcium_concentration__i_tr__d4) * R(1.0))), (R(2.0) * R(1.0) * intracellular_calcium_concentration__V_up * membrane__F)), R(1000.0));
+    J[26] = R(0);
+    J[27] = R(0);
+    J[28] = R(0);
+    J[29] = R(0);
+    J[30] = R(0) - dt * cmx_div(cmx_div((((-d_intracellular_calcium_concentration__i_rel__d2) * R(1.0))), (R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F)), R(1000.0));
+    J[31] = R(0) - dt * cmx_div(cmx_div(((d_intracellular_calcium_concentration__i_tr__d3 * R(1.0))), (R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F)), R(1000.0));
+    J[32] = R(1) - dt * cmx_div(cmx_div((((d_intracellular_calcium_concentration__i_tr__d4 - d_intracellular_calcium_concentration__i_rel__d4) * R(1.0))), (R(2.0) * R(1.0) * intracellular_calcium_concentration__V_rel * membrane__F)), R(1000.0));
+    J[33] = R(0);
+    J[34] = R(0);
+    J[35] = R(0) - dt * cmx_div((cmx_div(((d_extracellular_potassium_concentration__i_mK__d0 * R(1.0))), (R(1.0) * extracellular_potassium_concentration__Ve * membrane__F))), R(1000.0));
+    J[36] = R(0) - dt * cmx_div((cmx_div(((d_extracellular_potassium_concentration__i_mK__d1 * R(1.0))), (R(1.0) * extracellular_potassium_concentration__Ve * membrane__F))), R(1000.0));
+    J[37] = R(0) - dt * cmx_div((cmx_div(((d_extracellular_potassium_concentration__i_mK__d2 * R(1.0))), (R(1.0) * extracellular_potassium_concentration__Ve * membrane__F))), R(1000.0));
+    J[38] = R(0);
+    J[39] = R(0);
+    J[40] = R(1) - dt * cmx_div((((R(1) * (-extracellular_potassium_concentration__pf))) + cmx_div(((d_extracellular_potassium_concentration__i_mK__d5 * R(1.0))), (R(1.0) * extracellular_potassium_concentration__Ve * membrane__F))), R(1000.0));
+    J[41] = R(0) - dt * cmx_div((cmx_div(((d_extracellular_potassium_concentration__i_mK__d6 * R(1.0))), (R(1.0) * extracellular_potassium_concentration__Ve * membrane__F))), R(1000.0));
+    J[42] = R(0) - dt * cmx_div(cmx_div(((d_extracellular_potassium_concentration__i_mK__d0 * (-R(1.0)))), (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[43] = R(0) - dt * cmx_div(cmx_div(((d_extracellular_potassium_concentration__i_mK__d1 * (-R(1.0)))), (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[44] = R(0) - dt * cmx_div(cmx_div(((d_extracellular_potassium_concentration__i_mK__d2 * (-R(1.0)))), (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[45] = R(0);
+    J[46] = R(0);
+    J[47] = R(0) - dt * cmx_div(cmx_div(((d_extracellular_potassium_concentration__i_mK__d5 * (-R(1.0)))), (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+    J[48] = R(1) - dt * cmx_div(cmx_div(((d_extracellular_potassium_concentration__i_mK__d6 * (-R(1.0)))), (R(1.0) * intracellular_sodium_concentration__Vi * membrane__F)), R(1000.0));
+}
+
+// One backward-Euler step of every state but V at the given V
+// (ComputeOneStepExceptVoltage): linear states in closed form, then the
+// nonlinear ones by Newton from their current values.  Returns the Newton
+// iteration count, -1 if it did not converge.
+template <class R>
+CMX_HD inline int be_step(R* y, R V, R i_stim_area, const R* p, R cm, R dt)
+{
+    be_linear<R>(y, V, p, cm, dt);
+    R g[kNumNonlinear];
+CMX_UNROLL
+    for (int i = 0; i < kNumNonlinear; ++i) g[i] = y[kNonlinear[i]];
+    const int its = cmx_newton<kNumNonlinear, R>(
+        g, [&](const R* x, R* r) { be_residual<R>(y, x, V, i_stim_area, p, cm, dt, r); },
+        [&](const R* x, R* J) { be_jacobian<R>(y, x, V, i_stim_area, p, cm, dt, J); });
+CMX_UNROLL
+    for (int i = 0; i < kNumNonlinear; ++i) y[kNonlinear[i]] = g[i];
+    return its;
+}
+
+// Rates of the full system: dV/dt (0 when V is clamped) and the states.
+template <class R>
+CMX_HD inline void grl_rates(const R* y, R i_stim_area, const R* p, R cm, bool update_v, R* f)
+{
+    R gi[kNumStates], gk[kNumStates];
+    evaluate<R, false>(y, y[0], i_stim_area, p, cm, f, gi, gk);
+    f[0] = update_v ? dvdt<R>(y, y[0], i_stim_area, p, cm) : R(0);
+}
+
+template <class R>
+CMX_HD inline R grl_advance(R y0, R f, R a, R dt)
+{
+    using std::fabs; using std::exp;
+    return fabs(a) < R(1e-8) ? y0 + f * dt : y0 + (f / a) * (exp(a * dt) - R(1));
+}
+
+// One generalized Rush-Larsen step of length dt (ORDER 1 or 2); V is
+// advanced too when update_v, else held.
+template <class R, int ORDER>
+CMX_HD inline void grl_step(R* y, R i_stim_area, const R* p, R cm, R dt, bool update_v)
+{
+    constexpr int S = kNumStates;
+    const R delta = R(1e-8);
+    R f[S], a[S], y0[S], t[S];
+    const int first = update_v ? 0 : 1;
+CMX_UNROLL
+    for (int i = 0; i < S; ++i) y0[i] = y[i];
+    grl_rates<R>(y, i_stim_area, p, cm, update_v, f);
+CMX_UNROLL
+    for (int i = 0; i < S; ++i) {
+        if (i < first) continue;
+        const R ys = y[i];
+        y[i] = ys + delta;
+        grl_rates<R>(y, i_stim_area, p, cm, update_v, t);
+        a[i] = (t[i] - f[i]) / delta;
+        y[i] = ys;
+    }
+    if constexpr (ORDER == 1) {
+CMX_UNROLL
+        for (int i = 0; i < S; ++i)
+            if (i >= first) y[i] = grl_advance<R>(y0[i], f[i], a[i], dt);
+        return;
+    }
+    R half = dt * R(0.5);
+CMX_UNROLL
+    for (int i = 0; i < S; ++i)
+        if (i >= first) y[i] = grl_advance<R>(y0[i], f[i], a[i], half);
+    R g[S];
+CMX_UNROLL
+    for (int i = 0; i < S; ++i) {
+        if (i < first) continue;
+        const R ym = y[i];
+        y[i] = y0[i];
+        grl_rates<R>(y, i_stim_area, p, cm, update_v, t);
+        g[i] = t[i];
+        y[i] = y0[i] + delta;
+        grl_rates<R>(y, i_stim_area, p, cm, update_v, t);
+        a[i] = (t[i] - g[i]) / delta;
+        y[i] = ym;
+    }
+CMX_UNROLL
+    for (int i = 0; i < S; ++i)
+        if (i >= first) y[i] = grl_advance<R>(y0[i], g[i], a[i], dt);
+}
+
+}}  // namespace cmx::dn85

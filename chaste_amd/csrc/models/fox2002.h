@@ -8,7 +8,7 @@ namespace cmx { namespace fox2002 {
 constexpr int kNumStates = 13;
 constexpr int kVIndex = 0;
 constexpr int kNumParams = 5;
-constexpr bool kPerCapacitanceUnits = false;
+constexpr bool kPerCapacitanceUnits = true;
 
 // State variables (CellML component__variable), V first then document order.
 static const char* const kStateNames[kNumStates] = {
@@ -95,7 +95,7 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R standard_ionic_concentrations__Na_i = R(10.0);
     const R standard_ionic_concentrations__Na_o = R(138.0);
     const R standard_ionic_concentrations__Ca_o = R(2000.0);
-    const R i_stim = i_stim_area;
+    const R i_stim = i_stim_area / cm;
     (void)i_stim;
     const R L_type_Ca_current__i_Ca_max = cmx_remove_singularity(V, R(-0.0), R(1.3354093264248706e-06), [&](const R V) { return cmx_div(((((p[4] * (R(1) / L_type_Ca_current__C_sc)) * R(4.0) * V * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((y[11] * cmx_exp_v(((R(2.0) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.10803391981328421, 0.0)) - (R(0.341) * standard_ionic_concentrations__Ca_o))), cmx_expm1(((R(2.0) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R L_type_Ca_current__i_Ca = (L_type_Ca_current__i_Ca_max * y[8] * y[9] * y[10]);
@@ -222,7 +222,7 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     const R standard_ionic_concentrations__Ca_o = R(2000.0);
     const R standard_ionic_concentrations__K_i = R(149.4);
     const R standard_ionic_concentrations__K_o = R(4.0);
-    const R i_stim = i_stim_area;
+    const R i_stim = i_stim_area / cm;
     (void)i_stim;
     const R L_type_Ca_current__i_Ca_max = cmx_remove_singularity(V, R(-0.0), R(1.3354093264248706e-06), [&](const R V) { return cmx_div(((((p[4] * (R(1) / L_type_Ca_current__C_sc)) * R(4.0) * V * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((y[11] * cmx_exp_v(((R(2.0) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.10803391981328421, 0.0)) - (R(0.341) * standard_ionic_concentrations__Ca_o))), cmx_expm1(((R(2.0) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R L_type_Ca_current__i_Ca = (L_type_Ca_current__i_Ca_max * y[8] * y[9] * y[10]);
@@ -309,7 +309,7 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     const R time_independent_potassium_current__i_K1 = (((p[3] * time_independent_potassium_current_K1_gate__K1_infinity * standard_ionic_concentrations__K_o) * (R(1) / (standard_ionic_concentrations__K_o + time_independent_potassium_current__K_mK1))) * (V - rapid_activating_delayed_rectifiyer_K_current__E_K));
     const R transient_outward_potassium_current__i_to = (transient_outward_potassium_current__g_to * y[6] * y[7] * (V - rapid_activating_delayed_rectifiyer_K_current__E_K));
     const R dv = (-(fast_sodium_current__i_Na + L_type_Ca_current__i_Ca + L_type_Ca_current__i_CaK + rapid_activating_delayed_rectifiyer_K_current__i_Kr + slow_activating_delayed_rectifiyer_K_current__i_Ks + transient_outward_potassium_current__i_to + time_independent_potassium_current__i_K1 + plateau_potassium_current__i_Kp + Na_Ca_exchanger__i_NaCa + sodium_potassium_pump__i_NaK + sarcolemmal_calcium_pump__i_p_Ca + sodium_background_current__i_Na_b + calcium_background_current__i_Ca_b + i_stim));
-    return -dv * R(1);
+    return -dv * cm;
 }
 
 // Backward Euler: the states solved by Newton (all but the linear gates).
@@ -388,7 +388,7 @@ CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const
     const R standard_ionic_concentrations__Na_i = R(10.0);
     const R standard_ionic_concentrations__Na_o = R(138.0);
     const R standard_ionic_concentrations__Ca_o = R(2000.0);
-    const R i_stim = i_stim_area;
+    const R i_stim = i_stim_area / cm;
     (void)i_stim;
     const R L_type_Ca_current__i_Ca_max = cmx_remove_singularity(V, R(-0.0), R(1.3354093264248706e-06), [&](const R V) { return cmx_div(((((p[4] * (R(1) / L_type_Ca_current__C_sc)) * R(4.0) * V * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((g[1] * cmx_exp_v(((R(2.0) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.10803391981328421, 0.0)) - (R(0.341) * standard_ionic_concentrations__Ca_o))), cmx_expm1(((R(2.0) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R L_type_Ca_current__i_Ca = (L_type_Ca_current__i_Ca_max * y[8] * y[9] * g[0]);
@@ -442,7 +442,7 @@ CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const
     const R standard_ionic_concentrations__Na_i = R(10.0);
     const R standard_ionic_concentrations__Na_o = R(138.0);
     const R standard_ionic_concentrations__Ca_o = R(2000.0);
-    const R i_stim = i_stim_area;
+    const R i_stim = i_stim_area / cm;
     (void)i_stim;
     const R L_type_Ca_current__i_Ca_max = cmx_remove_singularity(V, R(-0.0), R(1.3354093264248706e-06), [&](const R V) { return cmx_div(((((p[4] * (R(1) / L_type_Ca_current__C_sc)) * R(4.0) * V * cmx_ipow<2>(membrane__F)) * (R(1) / (membrane__R * membrane__T))) * ((g[1] * cmx_exp_v(((R(2.0) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))), V, 0.10803391981328421, 0.0)) - (R(0.341) * standard_ionic_concentrations__Ca_o))), cmx_expm1(((R(2.0) * V * membrane__F) * (R(1) / (membrane__R * membrane__T))))); });
     const R L_type_Ca_current__i_Ca = (L_type_Ca_current__i_Ca_max * y[8] * y[9] * g[0]);

@@ -8,7 +8,7 @@ namespace cmx { namespace mahajan2008 {
 constexpr int kNumStates = 26;
 constexpr int kVIndex = 0;
 constexpr int kNumParams = 5;
-constexpr bool kPerCapacitanceUnits = false;
+constexpr bool kPerCapacitanceUnits = true;
 
 // State variables (CellML component__variable), V first then document order.
 static const char* const kStateNames[kNumStates] = {
@@ -156,7 +156,7 @@ CMX_HD inline void evaluate(const R* y, R V, R i_stim_area, const R* p, R cm,
     const R Ca__btrop = R(70.0);
     const R Ca__taud = R(4.0);
     const R Ca__taups = R(0.5);
-    const R i_stim = i_stim_area;
+    const R i_stim = i_stim_area / cm;
     (void)i_stim;
     const R Ileak_Iup_Ixfer__jleak = (cmx_div((Ileak_Iup_Ixfer__gleak * y[23] * y[23]), ((y[23] * y[23]) + (Ileak_Iup_Ixfer__kj * Ileak_Iup_Ixfer__kj))) * ((y[23] * R(16.667)) - y[22]));
     const R Ileak_Iup_Ixfer__jup = cmx_div((Ileak_Iup_Ixfer__vup * y[22] * y[22]), ((y[22] * y[22]) + (Ileak_Iup_Ixfer__cup * Ileak_Iup_Ixfer__cup)));
@@ -360,7 +360,7 @@ CMX_HD inline R dvdt(const R* y, R V, R i_stim_area, const R* p, R cm)
     const R INaCa__xmcai = R(0.0036);
     const R reversal_potentials__K_i = R(140.0);
     const R reversal_potentials__prNaK = R(0.01833);
-    const R i_stim = i_stim_area;
+    const R i_stim = i_stim_area / cm;
     (void)i_stim;
     const R ICaL__po = ((((((R(1.0) - y[6]) - y[8]) - y[7]) - y[9]) - y[4]) - y[5]);
     const R Ca__csm = (y[21] * (R(1) / R(1000.0)));
@@ -475,7 +475,7 @@ CMX_HD inline R iionic(const R* y, R V, const R* p, R cm)
     const R Ito__xito = (Ito__xitos + Ito__xitof);
     const R cell__Itotal = (-(INa__xina + IK1__xik1 + IKr__xikr + IKs__xiks + Ito__xito + INaCa__xiNaCa + ICaL__xica + INaK__xiNaK + i_stim));
     const R dv = cell__Itotal;
-    return -dv * R(1);
+    return -dv * cm;
 }
 
 // Backward Euler: the states solved by Newton (all but the linear gates).
@@ -597,7 +597,7 @@ CMX_HD inline void be_residual(const R* y, const R* g, R V, R i_stim_area, const
     const R Ca__btrop = R(70.0);
     const R Ca__taud = R(4.0);
     const R Ca__taups = R(0.5);
-    const R i_stim = i_stim_area;
+    const R i_stim = i_stim_area / cm;
     (void)i_stim;
     const R Ileak_Iup_Ixfer__jleak = (cmx_div((Ileak_Iup_Ixfer__gleak * g[12] * g[12]), ((g[12] * g[12]) + (Ileak_Iup_Ixfer__kj * Ileak_Iup_Ixfer__kj))) * ((g[12] * R(16.667)) - g[11]));
     const R Ileak_Iup_Ixfer__jup = cmx_div((Ileak_Iup_Ixfer__vup * g[11] * g[11]), ((g[11] * g[11]) + (Ileak_Iup_Ixfer__cup * Ileak_Iup_Ixfer__cup)));
@@ -752,7 +752,7 @@ CMX_HD inline void be_jacobian(const R* y, const R* g, R V, R i_stim_area, const
     const R Ca__btrop = R(70.0);
     const R Ca__taud = R(4.0);
     const R Ca__taups = R(0.5);
-    const R i_stim = i_stim_area;
+    const R i_stim = i_stim_area / cm;
     (void)i_stim;
     const R Ileak_Iup_Ixfer__jleak = (cmx_div((Ileak_Iup_Ixfer__gleak * g[12] * g[12]), ((g[12] * g[12]) + (Ileak_Iup_Ixfer__kj * Ileak_Iup_Ixfer__kj))) * ((g[12] * R(16.667)) - g[11]));
     const R Ileak_Iup_Ixfer__jup = cmx_div((Ileak_Iup_Ixfer__vup * g[11] * g[11]), ((g[11] * g[11]) + (Ileak_Iup_Ixfer__cup * Ileak_Iup_Ixfer__cup)));

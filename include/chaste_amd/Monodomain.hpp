@@ -273,7 +273,8 @@ struct CellTenTusscher2006EpiFromCellMLGRL2 {
 };
 
 // Further generated models (chaste_codegen classes of
-// heart/src/odes/cellml/{HodgkinHuxley1952,FoxModel2002,Mahajan2008}.cellml).
+// heart/src/odes/cellml/{HodgkinHuxley1952,FoxModel2002,Mahajan2008,
+// DiFrancescoNoble1985,FaberRudy2000,Maleckar2008}.cellml).
 #define CHASTE_AMD_CELL_TAGS(Base, model)                                  \
     struct Base {                                                          \
         static constexpr int kModel = model;                               \
@@ -299,6 +300,9 @@ struct CellTenTusscher2006EpiFromCellMLGRL2 {
 CHASTE_AMD_CELL_TAGS(CellHodgkinHuxley1952FromCellML, CMX_MODEL_HH52)
 CHASTE_AMD_CELL_TAGS(CellFoxModel2002FromCellML, CMX_MODEL_FOX2002)
 CHASTE_AMD_CELL_TAGS(CellMahajan2008FromCellML, CMX_MODEL_MAHAJAN2008)
+CHASTE_AMD_CELL_TAGS(CellDiFrancescoNoble1985FromCellML, CMX_MODEL_DN85)
+CHASTE_AMD_CELL_TAGS(CellFaberRudy2000FromCellML, CMX_MODEL_FR2000)
+CHASTE_AMD_CELL_TAGS(CellMaleckar2008FromCellML, CMX_MODEL_MALECKAR2008)
 #undef CHASTE_AMD_CELL_TAGS
 
 // A cell model loaded at run time from a plugin built from any CellML file

@@ -13,10 +13,13 @@
 #include <omp.h>
 #endif
 
+#include "../chaste_amd/csrc/models/dn85.h"
 #include "../chaste_amd/csrc/models/fox2002.h"
+#include "../chaste_amd/csrc/models/fr2000.h"
 #include "../chaste_amd/csrc/models/hh52.h"
 #include "../chaste_amd/csrc/models/lr91.h"
 #include "../chaste_amd/csrc/models/mahajan2008.h"
+#include "../chaste_amd/csrc/models/maleckar2008.h"
 #include "../chaste_amd/csrc/models/ord_endo.h"
 #include "../chaste_amd/csrc/models/tt06_epi.h"
 
@@ -179,6 +182,9 @@ const ModelEntry* model_entry(int model)
         ORC_MODEL(cmx::hh52),                // ORC_HH52
         ORC_MODEL(cmx::fox2002),             // ORC_FOX2002
         ORC_MODEL(cmx::mahajan2008),         // ORC_MAHAJAN2008
+        ORC_MODEL(cmx::dn85),                // ORC_DN85
+        ORC_MODEL(cmx::fr2000),              // ORC_FR2000
+        ORC_MODEL(cmx::maleckar2008),        // ORC_MALECKAR2008
     };
     if (model < 0 || model >= (int)(sizeof(table) / sizeof(table[0]))) return nullptr;
     return &table[model];

@@ -1,9 +1,13 @@
-"""Further generated cell models: Hodgkin-Huxley 1952, Fox 2002 and Mahajan
-2008 (chaste_amd/codegen/models.py, from the reference's CellML files).
+"""Further generated cell models: Hodgkin-Huxley 1952, Fox 2002, Mahajan
+2008, and -- in seconds or absolute currents, converted by the generator --
+DiFrancesco-Noble 1985, Faber-Rudy 2000 and Maleckar 2008
+(chaste_amd/codegen/models.py, from the reference's CellML files).
 
 The oracle runs the generated FP64 model; it is pinned here against the
 reference's ValidData files and protocols (heart/test/ionicmodels/
-TestIonicModels.hpp:305-333 HH52, :700-748 Fox 2002, :910-983 Mahajan 2008;
+TestIonicModels.hpp:305-333 HH52, :700-748 Fox 2002, :910-983 Mahajan 2008,
+:579-697 Faber-Rudy 2000 with its endo/mid/epi parameter sets, :883-905
+DiFrancesco-Noble 1985, :987-1022 Maleckar 2008;
 voltages within 1e-3 mV at every sample, RunAndCheckIonicModels.hpp:196-221),
 the GetIIonic values those tests hard-code, and backward Euler against
 forward Euler with the reference's CompareCellModelResults rule
@@ -18,7 +22,7 @@ import pytest
 from conftest import load_dat
 from tests.test_gpu_parity import MIXED_ACT_MS, MIXED_MV
 
-HH52, FOX, MAHAJAN = 4, 5, 6        # oracle model ids
+HH52, FOX, MAHAJAN, DN85, FR2000, MALECKAR = 4, 5, 6, 7, 8, 9        # oracle model ids
 
 
 def _run(oracle, model, t_end, dt, stim, per, integ=0, y=None):
@@ -57,6 +61,47 @@ def test_mahajan2008_valid_data(oracle):
     assert np.abs(out[:, 0] - ref[:, 1]).max() < 1e-3
 
 
+def test_dn85_valid_data(oracle):
+    """Self-excitatory Purkinje cell (seconds and nA in the CellML; no
+    stimulus), 1800 ms at dt 0.01, samples every 10 ms."""
+    _, ref = load_dat('DiFrancescoNobleValidData.dat')
+    y, out = _run(oracle, DN85, 1800.0, 0.01, (0.0, 0.0, 0.0, -1.0), 1000)
+    assert out.shape[0] == ref.shape[0] == 181
+    assert np.abs(out[:, 0] - ref[:, 1]).max() < 1e-3
+    assert abs(oracle.cell_iionic(DN85, y) - (-0.0141)) < 1e-3
+
+
+def _fr2000(oracle, stim_start, params=None):
+    y = oracle.model_initial_state(FR2000)
+    return oracle.cell_compute(FR2000, y, 0.0, 1000.0, 0.007, stim=(-25.5, 2.0, stim_start, -1.0),
+                               steps_per_sample=500, params=params)
+
+
+@pytest.mark.parametrize('case', ['DelayedStim', 'Endo', 'Mid', 'Epi'])
+def test_fr2000_valid_data(oracle, case):
+    """Faber-Rudy 2000 (seconds in the CellML) at dt 0.007, samples every
+    3.5 ms; the endo/mid/epi cells set ScaleFactorGks, ScaleFactorIto and
+    ScaleFactorGkr (parameters 0-2; g_Na stays at its default)."""
+    _, ref = load_dat('FR2000%sValidData.dat' % case)
+    sets = {'Endo': (0.462, 0.0, 1.0), 'Mid': (1.154, 0.85, 1.0), 'Epi': (1.154, 1.0, 1.0)}
+    if case == 'DelayedStim':
+        out = _fr2000(oracle, 10.0)
+    else:
+        out = _fr2000(oracle, 0.0, params=np.array(list(sets[case]) + [16.0]))
+    assert out.shape[0] == ref.shape[0] == 286
+    assert np.abs(out[:, 0] - ref[:, 1]).max() < 1e-3
+
+
+def test_maleckar2008_valid_data(oracle):
+    """Human atrial cell (seconds, pA and nF in the CellML): the stimulus
+    -5.6 uA/cm^2 reaches the model as -280 pA through its 50 pF."""
+    _, ref = load_dat('Maleckar2009ValidData.dat')
+    y, out = _run(oracle, MALECKAR, 25.0, 0.001, (-5.6, 6.0, 4.0, 1000.0), 1000)
+    assert out.shape[0] == ref.shape[0] == 26
+    assert np.abs(out[:, 0] - ref[:, 1]).max() < 1e-3
+    assert abs(oracle.cell_iionic(MALECKAR, y) - 1.4426) < 1e-3
+
+
 def _compare_cell_results(v1, v2, tol, extra=()):
     """CompareCellModelResults on equal sample times: 25x the tolerance where
     the second trace moved more than 0.05 mV since its previous sample."""
@@ -85,41 +130,66 @@ def test_mahajan2008_backward_euler_against_forward(oracle):
     _compare_cell_results(fe[:, 0], be[:, 0], 0.03)
 
 
-@pytest.mark.parametrize('model', [HH52, FOX, MAHAJAN], ids=['hh52', 'fox2002', 'mahajan2008'])
+ALL = [HH52, FOX, MAHAJAN, DN85, FR2000, MALECKAR]
+ALL_IDS = ['hh52', 'fox2002', 'mahajan2008', 'dn85', 'fr2000', 'maleckar2008']
+
+
+def _smooth(oracle, model):
+    """Mask of the states whose right-hand side is smooth: Faber-Rudy 2000's
+    calcium-release trackers (APtrack*, OVRLDtrack*, Cainfluxtrack) switch
+    between branches when a state crosses a threshold, and the switching time
+    moves with dt, so they do not converge at first order."""
+    from chaste_amd import cmx
+    cid = {HH52: cmx.HH52, FOX: cmx.FOX2002, MAHAJAN: cmx.MAHAJAN2008, DN85: cmx.DN85, FR2000: cmx.FR2000,
+           MALECKAR: cmx.MALECKAR2008}[model]
+    return np.array(['track' not in n for n in cmx.state_names(cid)])
+
+
+def _clamped_reference(oracle, model, y0):
+    """Forward Euler over 1 ms with V clamped at dt 1e-6, and its error
+    estimated from the run at 2e-6 (first order: their difference)."""
+    ref, ref2 = y0.copy(), y0.copy()
+    oracle.cell_compute_except_voltage(model, ref, 0.0, 1.0, 1e-6, integrator=oracle.FE)
+    oracle.cell_compute_except_voltage(model, ref2, 0.0, 1.0, 2e-6, integrator=oracle.FE)
+    return ref, np.abs(ref - ref2)
+
+
+@pytest.mark.parametrize('model', ALL, ids=ALL_IDS)
 def test_backward_euler_first_order_clamped(oracle, model):
     """With V clamped, backward Euler over 1 ms converges at first order to
-    the dt 1e-6 forward-Euler solution (ratio 2 +- 0.2 between dt 0.01 and
-    0.005 for every state that moves)."""
+    the dt 1e-6 forward-Euler solution (ratio 2 +- 0.2 between dt 0.002 and
+    0.001 for every smooth state whose error is above the reference's)."""
     y0 = oracle.model_initial_state(model)
     y0[0] = -20.0
-    ref = y0.copy()
-    oracle.cell_compute_except_voltage(model, ref, 0.0, 1.0, 1e-6, integrator=oracle.FE)
+    ref, ref_err = _clamped_reference(oracle, model, y0)
     errs = []
-    for dt in (0.01, 0.005):
+    for dt in (0.002, 0.001):
         y = y0.copy()
         oracle.cell_compute_except_voltage(model, y, 0.0, 1.0, dt, integrator=oracle.BE)
         errs.append(y - ref)
-    moved = np.abs(ref - y0) > 1e-6 * np.maximum(np.abs(y0), 1.0)
+    # states whose error is well above the reference's own
+    moved = (np.abs(errs[1]) > 50.0 * ref_err + 1e-13 * np.maximum(np.abs(y0), 1.0)) & _smooth(oracle, model)
     moved[0] = False
     assert moved.sum() >= 3
     ratio = errs[0][moved] / errs[1][moved]
     assert np.all(np.abs(ratio - 2.0) <= 0.2), ratio
 
 
-@pytest.mark.parametrize('model', [HH52, FOX, MAHAJAN], ids=['hh52', 'fox2002', 'mahajan2008'])
+@pytest.mark.parametrize('model', ALL, ids=ALL_IDS)
 def test_rush_larsen_against_forward_euler_clamped(oracle, model):
-    """ComputeExceptVoltage over 1 ms at dt 0.001: Rush-Larsen and forward
-    Euler agree to 1e-6 relative in every state (TestRushLarsen.hpp:130-138),
+    """ComputeExceptVoltage over 1 ms at dt 1e-5, where forward Euler is
+    accurate even for the fastest gate (Faber-Rudy's Ito zdv): Rush-Larsen
+    and forward Euler agree to 1e-6 relative in every state
+    (TestRushLarsen.hpp:130-138),
     and Rush-Larsen converges at first order to the dt 1e-6 forward-Euler
     solution with V clamped at -20 mV."""
     y0 = oracle.model_initial_state(model)
     a, b = y0.copy(), y0.copy()
-    oracle.cell_compute_except_voltage(model, a, 0.0, 1.0, 0.001, integrator=oracle.RL)
-    oracle.cell_compute_except_voltage(model, b, 0.0, 1.0, 0.001, integrator=oracle.FE)
+    oracle.cell_compute_except_voltage(model, a, 0.0, 1.0, 1e-5, integrator=oracle.RL)
+    oracle.cell_compute_except_voltage(model, b, 0.0, 1.0, 1e-5, integrator=oracle.FE)
     assert np.all(np.abs(a - b) <= 1e-6 * np.maximum(np.abs(b), 1.0))
     y0[0] = -20.0
-    ref = y0.copy()
-    oracle.cell_compute_except_voltage(model, ref, 0.0, 1.0, 1e-6, integrator=oracle.FE)
+    ref, ref_err = _clamped_reference(oracle, model, y0)
     errs = []
     for dt in (0.002, 0.001):
         y = y0.copy()
@@ -131,8 +201,8 @@ def test_rush_larsen_against_forward_euler_clamped(oracle, model):
     # reference's own (forward Euler at 1e-6, ~1e-7 relative)
     exact = np.abs(errs[0] - errs[1]) <= 1e-12 * scale
     exact[0] = False
-    assert np.all(np.abs(errs[0][exact]) <= 1e-5 * scale[exact])
-    moved = (np.abs(errs[1]) > 1e-9 * scale) & ~exact
+    assert np.all(np.abs(errs[0][exact]) <= 2.0 * ref_err[exact] + 1e-12 * scale[exact])
+    moved = (np.abs(errs[1]) > 50.0 * ref_err + 1e-13 * scale) & ~exact & _smooth(oracle, model)
     moved[0] = False
     assert exact.sum() + moved.sum() >= 3
     ratio = errs[0][moved] / errs[1][moved]
@@ -144,6 +214,9 @@ CASES = {
     'hh52': ('HH52', HH52, (0.01, 0.01), (-20.0, 0.5, 20.0)),
     'fox2002': ('FOX2002', FOX, (0.002, 0.002), (-80.0, 1.0, 300.0)),
     'mahajan2008': ('MAHAJAN2008', MAHAJAN, (0.001, 0.002), (-1800.0, 0.05, 300.0)),
+    'dn85': ('DN85', DN85, (0.01, 0.01), (-50.0, 1.0, 400.0)),
+    'fr2000': ('FR2000', FR2000, (0.007, 0.007), (-25.5, 2.0, 400.0)),
+    'maleckar2008': ('MALECKAR2008', MALECKAR, (0.001, 0.002), (-5.6, 6.0, 400.0)),
 }
 
 
@@ -198,6 +271,9 @@ TISSUE = {
     'hh52': (-100000.0, 0.5, 0.01),
     'fox2002': (-400000.0, 1.0, 0.002),
     'mahajan2008': (-2600000.0, 0.1, 0.002),
+    'dn85': (-500000.0, 1.0, 0.01),
+    'fr2000': (-500000.0, 2.0, 0.005),
+    'maleckar2008': (-300000.0, 2.0, 0.002),
 }
 
 
