@@ -16,7 +16,7 @@ HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
 SOURCES = ['cmx_device.hip', 'cmx_adjoint.hip', 'cmx_cells.hip', 'cmx_host.cpp', 'cmx_io.cpp',
            'cmx_model_lr91.hip', 'cmx_model_tt06.hip', 'cmx_model_ord.hip', 'cmx_model_hh52.hip',
            'cmx_model_fox2002.hip', 'cmx_model_mahajan2008.hip', 'cmx_model_dn85.hip', 'cmx_model_fr2000.hip',
-           'cmx_model_maleckar2008.hip']
+           'cmx_model_maleckar2008.hip', 'cmx_model_n98.hip']
 
 
 CPP_TESTS = ['TestMonodomainProblem']

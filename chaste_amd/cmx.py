@@ -11,7 +11,7 @@ import numpy as np
 from .build import library_path
 
 FP64, MIXED = 0, 1
-LR91, TT06_EPI, ORD_ENDO, HH52, FOX2002, MAHAJAN2008, DN85, FR2000, MALECKAR2008 = 0, 1, 2, 3, 4, 5, 6, 7, 8
+LR91, TT06_EPI, ORD_ENDO, HH52, FOX2002, MAHAJAN2008, DN85, FR2000, MALECKAR2008, N98 = range(10)
 FORWARD_EULER, RUSH_LARSEN, BACKWARD_EULER, GRL1, GRL2 = 0, 1, 2, 3, 4
 FMT_AUTO, FMT_STENCIL, FMT_ELL, FMT_ELL_OFFSETS = 0, 1, 2, 3
 KERNEL_KINDS = ['cell', 'residual', 'spmv', 'update', 'pupdate', 'finish', 'svi']

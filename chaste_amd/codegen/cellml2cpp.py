@@ -167,6 +167,11 @@ class CellMLModel:
             else:
                 self.odes[src] = (cname, rhs)
         self.time = self.canon_map[self.time]
+        # A derivative used inside an expression (dX/dt on some right-hand
+        # side) becomes a variable of its own, defined by X's equation in X's
+        # component, and the expression refers to it by an alias.
+        for k, (c, rhs) in list(self.defs.items()) + list(self.odes.items()):
+            self._replace_derivatives(rhs, c)
         # metadata: a cmeta:id names its variable's oxmeta term, unless an RDF
         # description of that id says which term (bqbiol:is oxford-metadata#...)
         self.meta = {}
@@ -203,6 +208,27 @@ class CellMLModel:
 
     def canon(self, cname, name):
         return self.canon_map[(cname, name)]
+
+    def _replace_derivatives(self, el, cname):
+        for i, ch in enumerate(list(el)):
+            if _tag(ch) == 'apply' and _tag(list(ch)[0]) == 'diff':
+                x = [c for c in ch if _tag(c) == 'ci'][0].text.strip()
+                xk = self.canon_map[(cname, x)]
+                dkey = (xk[0], 'd_%s_dt' % xk[1])
+                if dkey not in self.defs:
+                    self.vars[dkey] = dict(units=None, init=None, cmeta=None)
+                    self.canon_map[dkey] = dkey
+                    self.defs[dkey] = self.odes[xk]
+                alias = 'd_%s__%s_dt' % xk
+                if (cname, alias) not in self.canon_map:
+                    self.vars[(cname, alias)] = dict(units=None, init=None, cmeta=None)
+                    self.canon_map[(cname, alias)] = dkey
+                ci = ET.Element(MML + 'ci')
+                ci.text = alias
+                el.remove(ch)
+                el.insert(i, ci)
+            else:
+                self._replace_derivatives(ch, cname)
 
     def unit(self, name):
         """(factor, dimensions) of a units name: the quantity in SI-prefixed

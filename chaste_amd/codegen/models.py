@@ -43,6 +43,9 @@ MODELS = {
                       'delayed_rectifier_K_currents.ScaleFactorGkr', 'delayed_rectifier_K_currents.ScaleFactorGks',
                       'Ca_independent_transient_outward_K_current.ScaleFactorIto',
                       'ultra_rapid_K_current.ScaleFactorGKur']),
+    'n98': ('NobleVargheseKohlNoble1998a.cellml',
+            ['fast_sodium_current.g_Na', 'time_independent_potassium_current.g_K1', 'L_type_Ca_channel.P_Ca_L',
+             'slow_delayed_rectifier_potassium_current.g_Ks', 'rapid_delayed_rectifier_potassium_current.g_Kr1']),
     'tt06_epi': ('TenTusscher2006Epi.cellml',
                  ['fast_sodium_current.g_Na', 'L_type_Ca_current.g_CaL', 'rapid_time_dependent_potassium_current.g_Kr',
                   'slow_time_dependent_potassium_current.g_Ks', 'inward_rectifier_potassium_current.g_K1',

@@ -18,6 +18,7 @@
 #include "models/lr91.h"
 #include "models/mahajan2008.h"
 #include "models/maleckar2008.h"
+#include "models/n98.h"
 #include "models/ord_endo.h"
 #include "models/tt06_epi.h"
 
@@ -79,6 +80,7 @@ CMX_GENERATED_MODEL(ModelMahajan2008, mahajan2008)
 CMX_GENERATED_MODEL(ModelDn85, dn85)
 CMX_GENERATED_MODEL(ModelFr2000, fr2000)
 CMX_GENERATED_MODEL(ModelMaleckar2008, maleckar2008)
+CMX_GENERATED_MODEL(ModelN98, n98)
 
 // O'Hara-Rudy 2011 endocardial cells: forward Euler and Rush-Larsen only (no
 // backward-Euler or generalized Rush-Larsen forms).

@@ -864,6 +864,7 @@ const ModelOps* model_ops_mahajan2008();
 const ModelOps* model_ops_dn85();
 const ModelOps* model_ops_fr2000();
 const ModelOps* model_ops_maleckar2008();
+const ModelOps* model_ops_n98();
 
 // Plugin models (cmx_load_model), published once registered.
 std::atomic<const ModelOps*> g_plugin_models[kMaxPluginModels];
@@ -884,6 +885,7 @@ const ModelOps* model_ops(int model)
     case CMX_MODEL_DN85: return model_ops_dn85();
     case CMX_MODEL_FR2000: return model_ops_fr2000();
     case CMX_MODEL_MALECKAR2008: return model_ops_maleckar2008();
+    case CMX_MODEL_N98: return model_ops_n98();
     default: return nullptr;
     }
 }

@@ -274,7 +274,7 @@ struct CellTenTusscher2006EpiFromCellMLGRL2 {
 
 // Further generated models (chaste_codegen classes of
 // heart/src/odes/cellml/{HodgkinHuxley1952,FoxModel2002,Mahajan2008,
-// DiFrancescoNoble1985,FaberRudy2000,Maleckar2008}.cellml).
+// DiFrancescoNoble1985,FaberRudy2000,Maleckar2008,NobleVargheseKohlNoble1998a}.cellml).
 #define CHASTE_AMD_CELL_TAGS(Base, model)                                  \
     struct Base {                                                          \
         static constexpr int kModel = model;                               \
@@ -303,6 +303,7 @@ CHASTE_AMD_CELL_TAGS(CellMahajan2008FromCellML, CMX_MODEL_MAHAJAN2008)
 CHASTE_AMD_CELL_TAGS(CellDiFrancescoNoble1985FromCellML, CMX_MODEL_DN85)
 CHASTE_AMD_CELL_TAGS(CellFaberRudy2000FromCellML, CMX_MODEL_FR2000)
 CHASTE_AMD_CELL_TAGS(CellMaleckar2008FromCellML, CMX_MODEL_MALECKAR2008)
+CHASTE_AMD_CELL_TAGS(CellNobleVargheseKohlNoble1998aFromCellML, CMX_MODEL_N98)
 #undef CHASTE_AMD_CELL_TAGS
 
 // A cell model loaded at run time from a plugin built from any CellML file

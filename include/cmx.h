@@ -36,12 +36,12 @@ enum {
 /* Cell models (CellML sources heart/src/odes/cellml/LuoRudy1991.cellml,
  * TenTusscher2006Epi.cellml, HodgkinHuxley1952.cellml, FoxModel2002.cellml,
  * Mahajan2008.cellml, DiFrancescoNoble1985.cellml, FaberRudy2000.cellml,
- * Maleckar2008.cellml; O'Hara-Rudy 2011 endocardial, which the reference tree
+ * Maleckar2008.cellml, NobleVargheseKohlNoble1998a.cellml; O'Hara-Rudy 2011 endocardial, which the reference tree
  * does not hold, from the published equations, forward-Euler and Rush-Larsen
  * integration only). */
 enum { CMX_MODEL_LR91 = 0, CMX_MODEL_TT06_EPI = 1, CMX_MODEL_ORD_ENDO = 2, CMX_MODEL_HH52 = 3,
        CMX_MODEL_FOX2002 = 4, CMX_MODEL_MAHAJAN2008 = 5, CMX_MODEL_DN85 = 6, CMX_MODEL_FR2000 = 7,
-       CMX_MODEL_MALECKAR2008 = 8 };
+       CMX_MODEL_MALECKAR2008 = 8, CMX_MODEL_N98 = 9 };
 
 /* Cell integrators: EulerIvpOdeSolver (ode/src/solver/EulerIvpOdeSolver.cpp:38-57),
  * AbstractRushLarsenCardiacCell (heart/src/odes/AbstractRushLarsenCardiacCell.cpp:57-153),

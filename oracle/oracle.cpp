@@ -20,6 +20,7 @@
 #include "../chaste_amd/csrc/models/lr91.h"
 #include "../chaste_amd/csrc/models/mahajan2008.h"
 #include "../chaste_amd/csrc/models/maleckar2008.h"
+#include "../chaste_amd/csrc/models/n98.h"
 #include "../chaste_amd/csrc/models/ord_endo.h"
 #include "../chaste_amd/csrc/models/tt06_epi.h"
 
@@ -185,6 +186,7 @@ const ModelEntry* model_entry(int model)
         ORC_MODEL(cmx::dn85),                // ORC_DN85
         ORC_MODEL(cmx::fr2000),              // ORC_FR2000
         ORC_MODEL(cmx::maleckar2008),        // ORC_MALECKAR2008
+        ORC_MODEL(cmx::n98),                 // ORC_N98
     };
     if (model < 0 || model >= (int)(sizeof(table) / sizeof(table[0]))) return nullptr;
     return &table[model];

@@ -40,7 +40,8 @@ extern "C" {
 #endif
 
 enum orc_model { ORC_LR91_HAND = 0, ORC_LR91_CELLML = 1, ORC_TT06_EPI = 2, ORC_ORD_ENDO = 3, ORC_HH52 = 4,
-                 ORC_FOX2002 = 5, ORC_MAHAJAN2008 = 6, ORC_DN85 = 7, ORC_FR2000 = 8, ORC_MALECKAR2008 = 9 };
+                 ORC_FOX2002 = 5, ORC_MAHAJAN2008 = 6, ORC_DN85 = 7, ORC_FR2000 = 8, ORC_MALECKAR2008 = 9,
+                 ORC_N98 = 10 };
 /* Cell integrators: forward Euler, Rush-Larsen, backward Euler (the
  * reference's *BackwardEuler classes), generalized Rush-Larsen of first and
  * second order (--grl1 / --grl2 cells); all but FE on generated models only. */

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'liboracle.so')
 
-LR91_HAND, LR91_CELLML, TT06_EPI, ORD_ENDO, HH52, FOX2002, MAHAJAN2008, DN85, FR2000, MALECKAR2008 = range(10)
+LR91_HAND, LR91_CELLML, TT06_EPI, ORD_ENDO, HH52, FOX2002, MAHAJAN2008, DN85, FR2000, MALECKAR2008, N98 = range(11)
 FE, RL, BE, GRL1, GRL2 = 0, 1, 2, 3, 4     # orc_integrator
 
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -1,13 +1,14 @@
 """Further generated cell models: Hodgkin-Huxley 1952, Fox 2002, Mahajan
 2008, and -- in seconds or absolute currents, converted by the generator --
-DiFrancesco-Noble 1985, Faber-Rudy 2000 and Maleckar 2008
+DiFrancesco-Noble 1985, Faber-Rudy 2000, Maleckar 2008 and Noble-Varghese-Kohl-
+Noble 1998 (whose calcium equation uses two other derivatives)
 (chaste_amd/codegen/models.py, from the reference's CellML files).
 
 The oracle runs the generated FP64 model; it is pinned here against the
 reference's ValidData files and protocols (heart/test/ionicmodels/
 TestIonicModels.hpp:305-333 HH52, :700-748 Fox 2002, :910-983 Mahajan 2008,
 :579-697 Faber-Rudy 2000 with its endo/mid/epi parameter sets, :883-905
-DiFrancesco-Noble 1985, :987-1022 Maleckar 2008;
+DiFrancesco-Noble 1985, :987-1022 Maleckar 2008, :102-142 and :752-790 N98;
 voltages within 1e-3 mV at every sample, RunAndCheckIonicModels.hpp:196-221),
 the GetIIonic values those tests hard-code, and backward Euler against
 forward Euler with the reference's CompareCellModelResults rule
@@ -22,7 +23,7 @@ import pytest
 from conftest import load_dat
 from tests.test_gpu_parity import MIXED_ACT_MS, MIXED_MV
 
-HH52, FOX, MAHAJAN, DN85, FR2000, MALECKAR = 4, 5, 6, 7, 8, 9        # oracle model ids
+HH52, FOX, MAHAJAN, DN85, FR2000, MALECKAR, N98 = 4, 5, 6, 7, 8, 9, 10        # oracle model ids
 
 
 def _run(oracle, model, t_end, dt, stim, per, integ=0, y=None):
@@ -102,6 +103,28 @@ def test_maleckar2008_valid_data(oracle):
     assert abs(oracle.cell_iionic(MALECKAR, y) - 1.4426) < 1e-3
 
 
+N98_STIM = (-3.0 / 0.095, 3.0, 10.0, -1.0)
+
+
+def test_n98_valid_data(oracle):
+    """Forward Euler at dt 0.01 for 150 ms, samples every ms; GetIIonic 0.2462."""
+    _, ref = load_dat('N98RegResultValidData.dat')
+    y, out = _run(oracle, N98, 150.0, 0.01, N98_STIM, 100)
+    assert out.shape[0] == ref.shape[0] == 151
+    assert np.abs(out[:, 0] - ref[:, 1]).max() < 1e-3
+    assert abs(oracle.cell_iionic(N98, y) - 0.2462) < 1e-3
+
+
+def test_n98_backward_euler_valid_data(oracle):
+    """The backward-Euler cell at dt 0.2, every step sampled, against the
+    reference's backward-Euler results; GetIIonic 0.2462."""
+    _, ref = load_dat('N98BackwardResultValidData.dat')
+    y, out = _run(oracle, N98, 150.0, 0.2, N98_STIM, 1, integ=oracle.BE)
+    assert out.shape[0] == ref.shape[0]
+    assert np.abs(out[:, 0] - ref[:, 1]).max() < 1e-3
+    assert abs(oracle.cell_iionic(N98, y) - 0.2462) < 1e-3
+
+
 def _compare_cell_results(v1, v2, tol, extra=()):
     """CompareCellModelResults on equal sample times: 25x the tolerance where
     the second trace moved more than 0.05 mV since its previous sample."""
@@ -130,8 +153,8 @@ def test_mahajan2008_backward_euler_against_forward(oracle):
     _compare_cell_results(fe[:, 0], be[:, 0], 0.03)
 
 
-ALL = [HH52, FOX, MAHAJAN, DN85, FR2000, MALECKAR]
-ALL_IDS = ['hh52', 'fox2002', 'mahajan2008', 'dn85', 'fr2000', 'maleckar2008']
+ALL = [HH52, FOX, MAHAJAN, DN85, FR2000, MALECKAR, N98]
+ALL_IDS = ['hh52', 'fox2002', 'mahajan2008', 'dn85', 'fr2000', 'maleckar2008', 'n98']
 
 
 def _smooth(oracle, model):
@@ -141,7 +164,7 @@ def _smooth(oracle, model):
     moves with dt, so they do not converge at first order."""
     from chaste_amd import cmx
     cid = {HH52: cmx.HH52, FOX: cmx.FOX2002, MAHAJAN: cmx.MAHAJAN2008, DN85: cmx.DN85, FR2000: cmx.FR2000,
-           MALECKAR: cmx.MALECKAR2008}[model]
+           MALECKAR: cmx.MALECKAR2008, N98: cmx.N98}[model]
     return np.array(['track' not in n for n in cmx.state_names(cid)])
 
 
@@ -217,6 +240,7 @@ CASES = {
     'dn85': ('DN85', DN85, (0.01, 0.01), (-50.0, 1.0, 400.0)),
     'fr2000': ('FR2000', FR2000, (0.007, 0.007), (-25.5, 2.0, 400.0)),
     'maleckar2008': ('MALECKAR2008', MALECKAR, (0.001, 0.002), (-5.6, 6.0, 400.0)),
+    'n98': ('N98', N98, (0.01, 0.01), (-3.0 / 0.095, 3.0, 400.0)),
 }
 
 
@@ -274,6 +298,7 @@ TISSUE = {
     'dn85': (-500000.0, 1.0, 0.01),
     'fr2000': (-500000.0, 2.0, 0.005),
     'maleckar2008': (-300000.0, 2.0, 0.002),
+    'n98': (-500000.0, 1.0, 0.01),
 }
 
 
