@@ -108,6 +108,8 @@ SIGNATURES = {
     'cmx_get_kernel_times': (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
     'cmx_model_info': (ctypes.c_int, [ctypes.c_int32, _ip, _ip]),
     'cmx_load_model': (ctypes.c_int, [ctypes.c_char_p, _ip]),
+    'cmx_model_param_alias': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p)]),
+    'cmx_model_find_param': (ctypes.c_int, [ctypes.c_int32, ctypes.c_char_p, _ip]),
     'cmx_model_state_name': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p)]),
     'cmx_model_initial_state': (ctypes.c_int, [ctypes.c_int32, _dp]),
     'cmx_model_param': (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), _dp]),
@@ -202,6 +204,24 @@ def param_info(model):
         _check(lib().cmx_model_param(model, q, ctypes.byref(c), _p(d)))
         out.append((c.value.decode(), float(d[0])))
     return out
+
+
+def param_aliases(model):
+    """The reference's names of the model's parameters ('' where none)."""
+    _, P = model_info(model)
+    out = []
+    for q in range(P):
+        c = ctypes.c_char_p()
+        _check(lib().cmx_model_param_alias(model, q, ctypes.byref(c)))
+        out.append(c.value.decode())
+    return out
+
+
+def find_param(model, name):
+    """Index of a parameter by the library's name or the reference's."""
+    i = ctypes.c_int32()
+    _check(lib().cmx_model_find_param(model, name.encode(), ctypes.byref(i)))
+    return i.value
 
 
 def cells_run(model, integrator, t_end, dt, stimuli=None, params=None, y0=None, precision=FP64,

@@ -175,11 +175,13 @@ class CellMLModel:
         # metadata: a cmeta:id names its variable's oxmeta term, unless an RDF
         # description of that id says which term (bqbiol:is oxford-metadata#...)
         self.meta = {}
+        self.annotation = {}    # variable -> its oxmeta term, else its cmeta:id
         by_id = {}
         for key, info in self.vars.items():
             if info['cmeta']:
                 self.meta[info['cmeta']] = self.canon_map[key]
                 by_id[info['cmeta']] = self.canon_map[key]
+                self.annotation.setdefault(self.canon_map[key], info['cmeta'])
         for d in root.iter(RDF + 'Description'):
             about = (d.get(RDF + 'about') or '').lstrip('#')
             if about not in by_id:
@@ -188,6 +190,7 @@ class CellMLModel:
                 res = q.get(RDF + 'resource') or ''
                 if OXMETA in res:
                     self.meta.setdefault(res.split(OXMETA)[1], by_id[about])
+                    self.annotation[by_id[about]] = res.split(OXMETA)[1]
         self.V = self.meta['membrane_voltage']
         self.stim = self.meta['membrane_stimulus_current']
         # rates per millisecond: a model in other time units has every
@@ -863,6 +866,14 @@ class Generator:
             w('    "%s",' % cident(k))
         for k in self.scales:
             w('    "scale:%s",' % cident(k))
+        if npar == 0:
+            w('    "",')
+        w('};')
+        w('// The names the reference gives the same parameters (the CellML oxmeta term,')
+        w('// else the variable\'s cmeta:id; empty when the file annotates neither).')
+        w('static const char* const kParamAliases[%d] = {' % max(npar, 1))
+        for k in self.params + self.scales:
+            w('    "%s",' % m.annotation.get(k, ''))
         if npar == 0:
             w('    "",')
         w('};')

@@ -51,7 +51,8 @@ namespace cmx {
     static const double* initial_state() { return ns::kInitialState; }                                           \
     static const double* param_defaults() { return ns::kParamDefaults; }                                         \
     static const char* const* state_names() { return ns::kStateNames; }                                          \
-    static const char* const* param_names() { return ns::kParamNames; }
+    static const char* const* param_names() { return ns::kParamNames; }                                          \
+    static const char* const* param_aliases() { return ns::kParamAliases; }
 
 #define CMX_MODEL_IMPLICIT(ns)                                                                                    \
     template <class R>                                                                                            \

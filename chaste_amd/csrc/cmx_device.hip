@@ -2649,6 +2649,32 @@ int cmx_model_param(int32_t model, int32_t index, const char** name, double* def
     return CMX_OK;
 }
 
+int cmx_model_param_alias(int32_t model, int32_t index, const char** alias)
+{
+    int S, P;
+    if (model_sizes(model, &S, &P) || index < 0 || index >= P || !alias)
+        return fail(nullptr, CMX_E_ARG, "bad model/parameter index");
+    ModelMeta m{};
+    model_meta(model, &m);
+    *alias = m.param_aliases ? m.param_aliases[index] : "";
+    return CMX_OK;
+}
+
+int cmx_model_find_param(int32_t model, const char* name, int32_t* index)
+{
+    int S, P;
+    if (model_sizes(model, &S, &P) || !name || !index) return fail(nullptr, CMX_E_ARG, "bad model or null argument");
+    ModelMeta m{};
+    model_meta(model, &m);
+    for (int q = 0; q < P; ++q)
+        if (!std::strcmp(name, m.param_names[q]) || (m.param_aliases && m.param_aliases[q][0] &&
+                                                     !std::strcmp(name, m.param_aliases[q]))) {
+            *index = q;
+            return CMX_OK;
+        }
+    return fail(nullptr, CMX_E_ARG, std::string("No parameter named '") + name + "'");
+}
+
 int cmx_set_params(cmx_ctx* c, int32_t n_params, const double* params)
 {
     if (!c) return fail(nullptr, CMX_E_ARG, "null context");

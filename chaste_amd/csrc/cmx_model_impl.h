@@ -26,7 +26,7 @@ struct ModelImpl {
     {
         ModelOps o{};
         o.meta = {Model::NS, Model::NP, Model::initial_state(), Model::param_defaults(), Model::state_names(),
-                  Model::param_names(), IMPLICIT};
+                  Model::param_names(), Model::param_aliases(), IMPLICIT};
         o.cells64 = &cells<double, double>;
         o.cells32 = &cells<float, float>;
         o.svi64[0] = &launch_svi_rows<Model, double, double, StencilRows<4>>;

@@ -19,6 +19,7 @@ struct ModelMeta {
     const double* pdef;
     const char* const* state_names;
     const char* const* param_names;
+    const char* const* param_aliases;   // the reference's names of the parameters ("" where none)
     bool implicit;       // backward-Euler and generalized Rush-Larsen forms available
 };
 

@@ -53,6 +53,14 @@ static const char* const kParamNames[4] = {
     "second_inward_current__P_si",
     "transient_outward_current__g_to",
 };
+// The names the reference gives the same parameters (the CellML oxmeta term,
+// else the variable's cmeta:id; empty when the file annotates neither).
+static const char* const kParamAliases[4] = {
+    "",
+    "",
+    "",
+    "",
+};
 static constexpr double kParamDefaults[4] = {
     750.0,
     920.0,

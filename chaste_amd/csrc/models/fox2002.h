@@ -48,6 +48,15 @@ static const char* const kParamNames[5] = {
     "time_independent_potassium_current__g_K1",
     "L_type_Ca_current__P_Ca",
 };
+// The names the reference gives the same parameters (the CellML oxmeta term,
+// else the variable's cmeta:id; empty when the file annotates neither).
+static const char* const kParamAliases[5] = {
+    "",
+    "",
+    "",
+    "",
+    "",
+};
 static constexpr double kParamDefaults[5] = {
     12.8,
     0.0136,

@@ -71,6 +71,14 @@ static const char* const kParamNames[4] = {
     "rapid_delayed_rectifier_potassium_current__ScaleFactorGkr",
     "fast_sodium_current__g_Na",
 };
+// The names the reference gives the same parameters (the CellML oxmeta term,
+// else the variable's cmeta:id; empty when the file annotates neither).
+static const char* const kParamAliases[4] = {
+    "ScaleFactorGks",
+    "ScaleFactorIto",
+    "ScaleFactorGkr",
+    "",
+};
 static constexpr double kParamDefaults[4] = {
     1.0,
     0.0,

@@ -28,6 +28,13 @@ static const char* const kParamNames[3] = {
     "potassium_channel__g_K",
     "leakage_current__g_L",
 };
+// The names the reference gives the same parameters (the CellML oxmeta term,
+// else the variable's cmeta:id; empty when the file annotates neither).
+static const char* const kParamAliases[3] = {
+    "membrane_fast_sodium_current_conductance",
+    "",
+    "",
+};
 static constexpr double kParamDefaults[3] = {
     120.0,
     36.0,

@@ -36,6 +36,13 @@ static const char* const kParamNames[3] = {
     "time_dependent_potassium_current__g_Kmax",
     "slow_inward_current__P_si",
 };
+// The names the reference gives the same parameters (the CellML oxmeta term,
+// else the variable's cmeta:id; empty when the file annotates neither).
+static const char* const kParamAliases[3] = {
+    "membrane_fast_sodium_current_conductance",
+    "membrane_rapid_delayed_rectifier_potassium_current_conductance",
+    "membrane_L_type_calcium_current_conductance",
+};
 static constexpr double kParamDefaults[3] = {
     23.0,
     0.282,

@@ -74,6 +74,15 @@ static const char* const kParamNames[5] = {
     "IKs__ScaleFactorGks",
     "Ito__ScaleFactorIto",
 };
+// The names the reference gives the same parameters (the CellML oxmeta term,
+// else the variable's cmeta:id; empty when the file annotates neither).
+static const char* const kParamAliases[5] = {
+    "membrane_fast_sodium_current_conductance",
+    "membrane_L_type_calcium_current_conductance",
+    "ScaleFactorGkr",
+    "ScaleFactorGks",
+    "ScaleFactorIto",
+};
 static constexpr double kParamDefaults[5] = {
     12.0,
     182.0,

@@ -83,6 +83,16 @@ static const char* const kParamNames[6] = {
     "Ca_independent_transient_outward_K_current__ScaleFactorIto",
     "ultra_rapid_K_current__ScaleFactorGKur",
 };
+// The names the reference gives the same parameters (the CellML oxmeta term,
+// else the variable's cmeta:id; empty when the file annotates neither).
+static const char* const kParamAliases[6] = {
+    "ScaleFactorGna",
+    "ScaleFactorGCaL",
+    "ScaleFactorGkr",
+    "ScaleFactorGks",
+    "ScaleFactorIto",
+    "ScaleFactorGKur",
+};
 static constexpr double kParamDefaults[6] = {
     1.0,
     1.0,

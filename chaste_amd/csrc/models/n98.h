@@ -66,6 +66,15 @@ static const char* const kParamNames[5] = {
     "slow_delayed_rectifier_potassium_current__g_Ks",
     "rapid_delayed_rectifier_potassium_current__g_Kr1",
 };
+// The names the reference gives the same parameters (the CellML oxmeta term,
+// else the variable's cmeta:id; empty when the file annotates neither).
+static const char* const kParamAliases[5] = {
+    "",
+    "",
+    "",
+    "",
+    "",
+};
 static constexpr double kParamDefaults[5] = {
     2.5,
     0.5,

@@ -43,6 +43,8 @@ static constexpr double kInitialState[kNumStates] = {
 static const char* const kParamNames[kNumParams] = {
     "INa__GNa", "ICaL__PCa", "IKr__GKr", "IKs__GKs", "IK1__GK1", "Ito__Gto", "INaL__GNaL",
 };
+// (no CellML file, so no annotated names)
+static const char* const kParamAliases[kNumParams] = {"", "", "", "", "", "", ""};
 static constexpr double kParamDefaults[kNumParams] = {75.0, 1.0e-4, 0.046, 0.0034, 0.1908, 0.02, 0.0075};
 static constexpr int kIsGate[kNumStates] = {
     0, 0, 0, 0, 0, 0, 0, 0, 0,

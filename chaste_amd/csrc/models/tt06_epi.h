@@ -61,6 +61,16 @@ static const char* const kParamNames[6] = {
     "inward_rectifier_potassium_current__g_K1",
     "transient_outward_current__g_to",
 };
+// The names the reference gives the same parameters (the CellML oxmeta term,
+// else the variable's cmeta:id; empty when the file annotates neither).
+static const char* const kParamAliases[6] = {
+    "",
+    "",
+    "",
+    "",
+    "",
+    "",
+};
 static constexpr double kParamDefaults[6] = {
     14.838,
     3.98e-05,

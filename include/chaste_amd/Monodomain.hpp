@@ -779,14 +779,9 @@ public:
         if (values.size() != n) throw Exception(CMX_E_ARG, "one value per node is required");
         int32_t S = 0, P = 0;
         Check(cmx_model_info(mModel, &S, &P), nullptr);
-        int index = -1;
-        for (int q = 0; q < P; ++q) {
-            const char* name = nullptr;
-            double def = 0.0;
-            Check(cmx_model_param(mModel, q, &name, &def), nullptr);
-            if (rName == name) index = q;
-        }
-        if (index < 0) throw Exception(CMX_E_ARG, "No parameter named '" + rName + "'");
+        // the library's name or the reference's (GetParameterIndex)
+        int32_t index = -1;
+        Check(cmx_model_find_param(mModel, rName.c_str(), &index), nullptr);
         if (mParams.empty()) {
             mParams.resize((size_t)P * n);
             for (int q = 0; q < P; ++q) {

@@ -263,6 +263,12 @@ int cmx_model_initial_state(int32_t model, double* y);
  * annotates with the metadata the reference's generator turns into modifiable
  * parameters: g_Na, g_Kmax, P_si (in that order). */
 int cmx_model_param(int32_t model, int32_t index, const char** name, double* default_value);
+/* The reference's name of parameter index (the CellML oxmeta term or cmeta:id
+ * that AbstractParameterisedSystem::GetParameterIndex matches,
+ * ode/src/common/AbstractParameterisedSystem.cpp:224-237; "" when the CellML
+ * annotates none), and the index of a parameter by either name. */
+int cmx_model_param_alias(int32_t model, int32_t index, const char** alias);
+int cmx_model_find_param(int32_t model, const char* name, int32_t* index);
 /* The gate lookup table that cmx_opts.lookup_tables / cmx_cells.lookup_tables
  * use for Rush-Larsen steps of length dt (the tabulated-expression idea of the
  * reference's AbstractLookupTableCollection, heart/src/odes/

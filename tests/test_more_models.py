@@ -338,3 +338,27 @@ def test_tissue_matches_oracle(oracle, name, precision):
     else:
         assert np.abs(V - ref['V']).max() <= MIXED_MV
         assert np.abs(act[both] - ref['act'][both]).max() <= MIXED_ACT_MS
+
+
+def test_parameters_by_the_reference_names():
+    """SetParameter names of the reference's tests (TestIonicModels.hpp:
+    FR2000 ScaleFactorGks/Ito/Gkr, Mahajan and Maleckar ScaleFactor*; LR91's
+    oxmeta conductance) find the same parameters as the library's names."""
+    from chaste_amd import cmx
+    assert cmx.find_param(cmx.FR2000, 'ScaleFactorGks') == 0
+    assert cmx.find_param(cmx.FR2000, 'ScaleFactorIto') == 1
+    assert cmx.find_param(cmx.FR2000, 'ScaleFactorGkr') == 2
+    assert cmx.find_param(cmx.MAHAJAN2008, 'ScaleFactorGkr') == 2
+    assert cmx.find_param(cmx.MALECKAR2008, 'ScaleFactorGna') == 0
+    assert cmx.find_param(cmx.LR91, 'membrane_fast_sodium_current_conductance') == 0
+    for model in (cmx.LR91, cmx.TT06_EPI, cmx.HH52, cmx.FOX2002, cmx.MAHAJAN2008, cmx.DN85, cmx.FR2000,
+                  cmx.MALECKAR2008, cmx.N98, cmx.ORD_ENDO):
+        names = [n for n, _ in cmx.param_info(model)]
+        aliases = cmx.param_aliases(model)
+        assert len(aliases) == len(names)
+        for q, (n, a) in enumerate(zip(names, aliases)):
+            assert cmx.find_param(model, n) == q
+            if a:
+                assert cmx.find_param(model, a) == q
+    with pytest.raises(cmx.CmxError, match='No parameter named'):
+        cmx.find_param(cmx.LR91, 'ScaleFactorGks')
