@@ -55,7 +55,8 @@ class CmxCells(ctypes.Structure):
                 ('n_cells', ctypes.c_int32), ('dt', ctypes.c_double), ('t_end', _dp),
                 ('stimuli', ctypes.POINTER(CmxStimulus)), ('n_params', ctypes.c_int32), ('params', _dp),
                 ('threshold', ctypes.c_double), ('steps_per_launch', ctypes.c_int32),
-                ('lookup_tables', ctypes.c_int32), ('lanes_per_cell', ctypes.c_int32)]
+                ('lookup_tables', ctypes.c_int32), ('lanes_per_cell', ctypes.c_int32),
+                ('trace_every', ctypes.c_int32), ('trace', _dp)]
 
 
 class CmxCellsStats(ctypes.Structure):
@@ -225,11 +226,12 @@ def find_param(model, name):
 
 
 def cells_run(model, integrator, t_end, dt, stimuli=None, params=None, y0=None, precision=FP64,
-              threshold=-30.0, steps_per_launch=0, lookup_tables=False, lanes_per_cell=0):
+              threshold=-30.0, steps_per_launch=0, lookup_tables=False, lanes_per_cell=0, trace_every=0):
     """Batched single cells (cmx_cells_run).  t_end [n]; stimuli: one
     (magnitude, start, duration[, period[, stop]]) per cell or None; params
     [n_params, n].  Returns (final states [S, n], props [4, n]: completed APs,
-    last onset, last peak, last end; stats dict)."""
+    last onset, last peak, last end; stats dict).  trace_every > 0: V at t = 0
+    and every trace_every steps, as stats['trace'] [n_samples, n]."""
     t_end = np.ascontiguousarray(t_end, dtype=np.float64)
     n = t_end.shape[0]
     S, _ = model_info(model)
@@ -251,6 +253,12 @@ def cells_run(model, integrator, t_end, dt, stimuli=None, params=None, y0=None, 
     c.threshold, c.steps_per_launch = threshold, steps_per_launch
     c.lookup_tables = 1 if lookup_tables else 0
     c.lanes_per_cell = int(lanes_per_cell)
+    trace = None
+    c.trace_every = int(trace_every)
+    if trace_every > 0:
+        longest = int(np.round(t_end / dt).max()) if n else 0
+        trace = np.zeros((longest // trace_every + 1, n))
+        c.trace = _p(trace)
     yi = None if y0 is None else np.ascontiguousarray(y0, dtype=np.float64)
     y = np.zeros((S, n))
     props = np.zeros((4, n))
@@ -259,7 +267,10 @@ def cells_run(model, integrator, t_end, dt, stimuli=None, params=None, y0=None, 
                              ctypes.byref(stats))
     if rc != 0:
         raise CmxError(rc, lib().cmx_cells_last_error().decode())
-    return y, props, {k: getattr(stats, k) for k, _ in CmxCellsStats._fields_}
+    out = {k: getattr(stats, k) for k, _ in CmxCellsStats._fields_}
+    if trace is not None:
+        out['trace'] = trace
+    return y, props, out
 
 
 def gate_table(model, dt):

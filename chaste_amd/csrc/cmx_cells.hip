@@ -184,6 +184,19 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
         a.lut_miss = d_miss;
     }
 
+    // V trace: device samples in the sorted order, NaN until written
+    double* d_trace = nullptr;
+    size_t n_samples = 0;
+    if (c.trace_every > 0) {
+        n_samples = (size_t)(n > 0 ? s_steps[0] : 0) / (size_t)c.trace_every + 1;
+        if (n_samples * (size_t)n > ((size_t)1 << 31))
+            return cfail(CMX_E_ARG, "trace too large: raise trace_every");
+        if ((rc = B.upload(&d_trace, std::vector<double>(n_samples * (size_t)n, std::nan(""))))) return rc;
+        a.trace = d_trace;
+        a.trace_every = c.trace_every;
+        a.trace_n = n;
+    }
+
     hipStream_t stream = nullptr;
     CELLS_HIP(hipStreamCreate(&stream));
     struct StreamFree {
@@ -262,6 +275,12 @@ int run_cells(const cmx_cells& c, int S, int P, const double* y0, double* y_out,
     CELLS_HIP(hipMemcpy(on.data(), a.onset, ps * sizeof(double), hipMemcpyDeviceToHost));
     CELLS_HIP(hipMemcpy(lp.data(), a.last_peak, ps * sizeof(double), hipMemcpyDeviceToHost));
     CELLS_HIP(hipMemcpy(en.data(), a.end, ps * sizeof(double), hipMemcpyDeviceToHost));
+    if (d_trace) {
+        std::vector<double> tr(n_samples * (size_t)n);
+        CELLS_HIP(hipMemcpy(tr.data(), d_trace, tr.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t s = 0; s < n_samples; ++s)
+            for (int j = 0; j < n; ++j) c.trace[s * (size_t)n + order[j]] = tr[s * (size_t)n + j];
+    }
     long long cell_steps = 0;
     for (int j = 0; j < n; ++j) {
         const int i = order[j];
@@ -510,6 +529,8 @@ int cmx_cells_run(const cmx_cells* c, const double* y0, double* y_out, double* p
         return cfail(CMX_E_ARG, "n_params must be in [0, the model's parameter count] with params given");
     if (c->lanes_per_cell < 0 || c->lanes_per_cell > 2) return cfail(CMX_E_ARG, "lanes_per_cell must be 0, 1 or 2");
     if (c->lookup_tables && c->lanes_per_cell == 2) return cfail(CMX_E_ARG, "lookup tables need lanes_per_cell 1");
+    if (c->trace_every < 0 || (c->trace_every > 0 && !c->trace))
+        return cfail(CMX_E_ARG, "trace_every must be >= 0, with a trace array when > 0");
     if (c->n_cells == 0) return CMX_OK;
     return c->precision == CMX_MIXED ? run_cells<float>(*c, S, P, y0, y_out, props, stats)
                                      : run_cells<double>(*c, S, P, y0, y_out, props, stats);
@@ -530,6 +551,7 @@ int cmx_cells_run_noalloc(const cmx_cells* c, const double* y0, double* y_out, d
     if (c->n_params < 0 || c->n_params > P || (c->n_params > 0 && !c->params))
         return cfail(CMX_E_ARG, "n_params must be in [0, the model's parameter count] with params given");
     if (c->lookup_tables) return cfail(CMX_E_ARG, "lookup tables need a table allocation: use cmx_cells_run");
+    if (c->trace_every != 0) return cfail(CMX_E_ARG, "a voltage trace needs an allocation: use cmx_cells_run");
     if (c->lanes_per_cell < 0 || c->lanes_per_cell > 2) return cfail(CMX_E_ARG, "lanes_per_cell must be 0, 1 or 2");
     if (c->n_cells == 0) return CMX_OK;
     hipStream_t s = (hipStream_t)stream;

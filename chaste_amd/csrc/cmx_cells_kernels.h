@@ -31,6 +31,10 @@ struct CellsArgs {
     float lut_vmin, lut_inv_dv;
     int lut_rows;
     int32_t* __restrict__ lut_miss;        // 1 + a cell whose voltage fell outside the table
+    // V samples, or null: trace[s * trace_n + i] = V after step s * trace_every
+    // (s = 0: the initial V)
+    double* __restrict__ trace;
+    int trace_every, trace_n;
 };
 
 template <class Model, class RS, int IG, bool LUT = false>
@@ -42,6 +46,7 @@ __global__ __launch_bounds__(kBlock) void cells_kernel(CellsArgs<RS> a, int k0, 
     static_assert(!LUT || RL, "lookup tables serve Rush-Larsen cells");
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n) return;
+    if (a.trace && k0 == 0) a.trace[i] = a.V[i];
     const int kend = min(k1, a.n_steps[i]);
     if (k0 >= kend) return;
     // Rates in RS; the states that integrate small increments over long runs
@@ -91,6 +96,7 @@ __global__ __launch_bounds__(kBlock) void cells_kernel(CellsArgs<RS> a, int k0, 
         }
         const double v0 = vd;
         vd += a.dt * (double)dv;
+        if (a.trace && (k + 1) % a.trace_every == 0) a.trace[(size_t)((k + 1) / a.trace_every) * a.trace_n + i] = vd;
         // CellProperties: onset when V rises above the threshold, the peak
         // while above, the end when V falls below it again
         const double t1 = (k + 1) * a.dt;
@@ -172,6 +178,7 @@ __global__ __launch_bounds__(2 * kSplitCells) void cells_split_kernel(CellsArgs<
     double onset = a.onset[i], peak = a.peak[i], last_peak = a.last_peak[i], end = a.end[i];
     const RS cm = (RS)a.cm, h = (RS)a.dt;
     const double thr = a.thr;
+    if (a.trace && k0 == 0 && valid && !gates_role) a.trace[i] = vd;
     for (int k = k0; k < kmax; ++k) {
         const int b = k & 1;
         if (k < kend) {
@@ -201,6 +208,8 @@ __global__ __launch_bounds__(2 * kSplitCells) void cells_split_kernel(CellsArgs<
                 const double v0 = vd;
                 vd += a.dt * (double)dv;
                 s_v[b][lane] = vd;
+                if (a.trace && (k + 1) % a.trace_every == 0)
+                    a.trace[(size_t)((k + 1) / a.trace_every) * a.trace_n + i] = vd;
                 const double t1 = (k + 1) * a.dt;
                 if (phase == 0 && vd > thr && v0 <= thr) {
                     onset = t + (t1 - t) / (vd - v0) * (thr - v0);

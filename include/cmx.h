@@ -389,6 +389,12 @@ typedef struct {
                                     0: per launch, 2 while the launch's cells would fill at most half
                                     of the device's SIMDs with one lane each, else 1 (the
                                     allocation-free variant takes 0 as 1).  Lookup tables need 1. */
+    int32_t trace_every;         /* > 0: each cell's V sampled at t = 0 and after every trace_every
+                                    steps into trace (AbstractCardiacCell::Compute's OdeSolution
+                                    with that sampling interval); 0: no trace.  Not with the
+                                    allocation-free variant. */
+    double* trace;               /* [n_samples][n_cells] host array, n_samples = (the longest
+                                    cell's steps) / trace_every + 1; NaN past a cell's own t_end */
 } cmx_cells;
 
 typedef struct {
