@@ -5,7 +5,7 @@
 
 namespace cmx {
 
-template <class Model, bool IMPLICIT>
+template <class Model, bool IMPLICIT, bool ADJOINT = IMPLICIT>
 struct ModelImpl {
     template <class RS, class RV>
     static void cells(cmx_ctx* c, int integrator, int mode, int kind, int G, const CellArgs<RS, RV>& a)
@@ -21,6 +21,11 @@ struct ModelImpl {
     static const char* table(double dt, double cm, std::vector<float>& t, int* rows)
     {
         return fill_gate_table<Model>(dt, cm, t, rows);
+    }
+    static void adjoint(cmx_ctx* c, const CellAdjArgs& a, bool rl)
+    {
+        if (rl) launch_cell_adjoint<Model, CMX_RUSH_LARSEN>(c, a);
+        else launch_cell_adjoint<Model, CMX_FORWARD_EULER>(c, a);
     }
     static ModelOps make()
     {
@@ -42,18 +47,20 @@ struct ModelImpl {
         o.batch64 = &batch<double>;
         o.batch32 = &batch<float>;
         o.gate_table = &table;
+        if constexpr (ADJOINT) o.adjoint = &adjoint;
         return o;
     }
 };
 
 }  // namespace cmx
 
-// Defines const cmx::ModelOps* cmx::model_ops_<name>() for model type Model.
-#define CMX_DEFINE_MODEL_OPS(name, Model, implicit)                                  \
+// Defines const cmx::ModelOps* cmx::model_ops_<name>() for model type Model
+// (implicit: backward-Euler and GRL forms; adjoint: the adjoint's cell step).
+#define CMX_DEFINE_MODEL_OPS(name, Model, implicit, adjoint)                        \
     namespace cmx {                                                                   \
     const ModelOps* model_ops_##name()                                                \
     {                                                                                 \
-        static const ModelOps ops = ModelImpl<Model, implicit>::make();               \
+        static const ModelOps ops = ModelImpl<Model, implicit, adjoint>::make();      \
         return &ops;                                                                  \
     }                                                                                 \
     }

@@ -2,4 +2,4 @@
 // model table.
 #include "cmx_model_impl.h"
 
-CMX_DEFINE_MODEL_OPS(n98, ModelN98, true)
+CMX_DEFINE_MODEL_OPS(n98, ModelN98, true, true)

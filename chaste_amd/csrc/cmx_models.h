@@ -7,6 +7,7 @@
 #pragma once
 #include <vector>
 
+#include "cmx_adjoint_kernels.h"
 #include "cmx_cell_kernels.h"
 #include "cmx_cells_kernels.h"
 
@@ -52,6 +53,8 @@ struct ModelOps {
     BatchLaunch<float> batch32;
     // the model's gate lookup table for ODE steps of length dt (error message or null)
     const char* (*gate_table)(double dt, double cm, std::vector<float>& table, int* rows);
+    // the adjoint's cell step (FP64, forward Euler or Rush-Larsen), or null
+    void (*adjoint)(cmx_ctx*, const CellAdjArgs&, bool rush_larsen);
 };
 
 // The entry of a model id; null for an unknown id.

@@ -126,6 +126,17 @@ struct cmx_dualn {
     {
         return chain(x, std::pow(x.v, e), e * std::pow(x.v, e - 1.0));
     }
+    // x^y with both varying: y x^(y-1) dx + x^y ln(x) dy (the second term
+    // only where x > 0, where x^y with a varying y is defined)
+    friend CMX_HD cmx_dualn pow(const cmx_dualn& x, const cmx_dualn& y)
+    {
+        const double v = std::pow(x.v, y.v);
+        const double dx = y.v * std::pow(x.v, y.v - 1.0);
+        const double dy = x.v > 0.0 ? v * std::log(x.v) : 0.0;
+        cmx_dualn r(v);
+        for (int k = 0; k < K; ++k) r.d[k] = dx * x.d[k] + dy * y.d[k];
+        return r;
+    }
 };
 
 using cmx_dual = cmx_dualn<1>;

@@ -302,7 +302,8 @@ int cmx_set_params(cmx_ctx* ctx, int32_t n_params, const double* params);
  * q ~ sqrt(checkpoint_every) steps and only q steps are kept at a time
  * (n_steps/checkpoint_every + checkpoint_every/q + q states; one forward pass
  * more).  The gradient does not depend on the schedule.  Needs CMX_FP64, forward-Euler or Rush-Larsen cells, no splitting, no SVI,
- * extrapolate = 0 and activation tracking off.  On return the context holds
+ * extrapolate = 0, activation tracking off and a cell model with an adjoint
+ * (every built-in and plugin model but O'Hara-Rudy and Maleckar 2008).  On return the context holds
  * the state at t0 + n_steps dt. */
 int cmx_adjoint(cmx_ctx* ctx, double t0, int32_t n_steps, int32_t checkpoint_every, int32_t n_obs,
                 const int32_t* obs_nodes, int32_t obs_every, const double* obs_values, double* loss,

@@ -362,3 +362,65 @@ def test_parameters_by_the_reference_names():
                 assert cmx.find_param(model, a) == q
     with pytest.raises(cmx.CmxError, match='No parameter named'):
         cmx.find_param(cmx.LR91, 'ScaleFactorGks')
+
+
+# Maleckar 2008 has no adjoint: its forward-mode derivatives with respect to
+# the dyadic calcium Ca_d grow past 1e60 through a strongly driven upstroke on
+# the CPU as well, so its gradient cannot match a central difference.
+ADJOINT_CASES = sorted(k for k in CASES if k != 'maleckar2008')
+
+
+@pytest.mark.gpu
+def test_adjoint_rejects_model_without_one():
+    from chaste_amd import cmx
+    print('case adjoint rejects maleckar2008', flush=True)
+    coords, elems = cmx.slab_mesh(2, 4, 4, 0, 0.01)
+    m = cmx.Monodomain(coords, elems, np.array([1.75, 1.75]), cmx.MALECKAR2008, cmx.RUSH_LARSEN,
+                       dt_pde=0.01, dt_ode=0.01)
+    obs = np.zeros(1, np.int32)
+    with pytest.raises(cmx.CmxError, match='has no adjoint'):
+        m.adjoint(0.0, 2, obs, 1, np.zeros((2, 1)))
+    m.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', ADJOINT_CASES)
+def test_adjoint_gradient_of_every_generated_model(name):
+    """The discrete adjoint of the FP64 ICI step (cmx_adjoint) for each
+    generated model with one, Rush-Larsen cells on a 2-D slab through a
+    stimulated upstroke: the parameter gradient along a seeded direction
+    against a central difference of the device's own forward runs (1e-4
+    relative)."""
+    from chaste_amd import cmx
+    cid, _, _, _ = CASES[name]
+    mag, dur, dt_ode = TISSUE[name]
+    model = getattr(cmx, cid)
+    print('case adjoint', name, flush=True)
+    coords, elems = cmx.slab_mesh(2, 20, 6, 0, 0.01)
+    n = coords.shape[0]
+    stim = np.where(coords[:, 0] < 0.02, 0, -1).astype(np.int32)
+    m = cmx.Monodomain(coords, elems, np.array([1.75, 1.75]), model, cmx.RUSH_LARSEN, stimuli=[(mag, 0.0, dur)],
+                       stim_index=stim, dt_pde=0.01, dt_ode=dt_ode, ksp_rtol=1e-12)
+    y0 = m.state()
+    defaults = np.array([d for _, d in cmx.param_info(model)])
+    p0 = defaults[:, None] * np.ones((1, n))
+    obs = np.arange(0, n, 7).astype(np.int32)
+    steps, every = 200, 20
+    data = np.full((steps // every, len(obs)), -20.0)
+
+    def loss_at(p):
+        m.set_state(y0)
+        m.set_params(p)
+        return m.adjoint(0.0, steps, obs, every, data, checkpoint_every=50)
+
+    loss, g, _ = loss_at(p0)
+    assert np.isfinite(loss) and np.isfinite(g).all()
+    rng = np.random.default_rng(3)
+    d = rng.standard_normal(p0.shape) * np.abs(p0) * 1e-5
+    fd = (loss_at(p0 + d)[0] - loss_at(p0 - d)[0]) / 2
+    ad = float((g * d).sum())
+    m.close()
+    print('%s adjoint: loss %.6e, directional %.6e, central difference %.6e, rel %.2e' % (
+        name, loss, ad, fd, abs(ad - fd) / abs(fd)))
+    assert abs(fd) > 0
+    assert abs(ad - fd) <= 1e-4 * abs(fd)
